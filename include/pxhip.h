@@ -1,0 +1,145 @@
+/* pxhip — MI355X (gfx950) GPU rigid-body step behind the PhysX GPU plug-in
+ * boundary. C ABI: plain pointers and sizes, no C++ or framework types.
+ *
+ * Each entry point names the reference interface it replaces (paths relative
+ * to the PhysX 5.6.1 tree, physx/). Device pointers are HIP device addresses;
+ * `stream` is a hipStream_t (NULL = the scene's own stream).
+ *
+ * Return codes: 0 on success, negative on error (pxh_last_error() describes
+ * the last error of the calling thread).
+ */
+#ifndef PXHIP_H
+#define PXHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PXH_VERSION 0x00010000
+
+enum {
+	PXH_OK = 0,
+	PXH_ERR_INVALID = -1,        /* bad argument (PxErrorCode::eINVALID_PARAMETER) */
+	PXH_ERR_DEVICE = -2,         /* HIP runtime error (PxCudaContext::getLastError) */
+	PXH_ERR_STATE = -3,          /* call not allowed now, e.g. during simulate (NpDirectGPUAPI.cpp:65-72) */
+	PXH_ERR_CAPACITY = -4,       /* buffer capacity exceeded (PxGpuDynamicsMemoryConfig, PxSceneDesc.h:457-493) */
+	PXH_ERR_UNSUPPORTED = -5     /* feature outside the implemented path */
+};
+
+/* geometry codes = PxGeometryType::Enum (physx/include/geometry/PxGeometry.h) */
+enum { PXH_GEOM_SPHERE = 0, PXH_GEOM_PLANE = 1, PXH_GEOM_CAPSULE = 2, PXH_GEOM_BOX = 3 };
+
+/* solver codes = PxSolverType::Enum (physx/include/PxSceneDesc.h) */
+enum { PXH_SOLVER_PGS = 0, PXH_SOLVER_TGS = 1 };
+
+/* Direct GPU API data kinds = PxRigidDynamicGPUAPIReadType/WriteType
+ * (physx/include/PxDirectGPUAPI.h): pose is a PxTransform (q xyzw, p xyz). */
+enum { PXH_DATA_GLOBAL_POSE = 0, PXH_DATA_LINEAR_VELOCITY = 1, PXH_DATA_ANGULAR_VELOCITY = 2 };
+
+/* Scene description: the PxSceneDesc / PxRigidDynamic / PxShape / PxMaterial
+ * parameters the step reads (PxSceneDesc.h:1035-1075 defaults,
+ * PxvDynamics.h:119-160 body defaults). */
+typedef struct pxh_scene_desc {
+	float gravity[3];
+	int32_t solver_type;                 /* PXH_SOLVER_TGS */
+	int32_t position_iterations;         /* PxRigidDynamic::setSolverIterationCounts */
+	int32_t velocity_iterations;
+	float contact_offset;                /* per shape (PxShape::setContactOffset) */
+	float rest_offset;                   /* per shape (PxShape::setRestOffset) */
+	float static_friction, dynamic_friction, restitution;  /* the scene's one material */
+	float linear_damping, angular_damping;
+	float max_linear_velocity_sq, max_angular_velocity_sq;
+	float bounce_threshold_velocity;     /* PxSceneDesc::bounceThresholdVelocity */
+	float friction_offset_threshold;     /* PxSceneDesc::frictionOffsetThreshold */
+	float friction_correlation_distance; /* PxSceneDesc::frictionCorrelationDistance */
+	float tolerance_length;              /* PxTolerancesScale::length */
+	int32_t max_pairs_per_body;          /* initial pair capacity per body; grows on demand */
+} pxh_scene_desc;
+
+typedef struct pxh_scene_t* pxh_scene;
+typedef struct pxh_broadphase_t* pxh_broadphase;
+
+int32_t pxh_get_version(void);
+const char* pxh_last_error(void);
+void pxh_default_scene_desc(pxh_scene_desc* desc);
+
+/* Scene lifecycle. Replaces the GPU pipeline objects the plug-in factory
+ * creates for a scene: PxPhysXGpu::createGpuBroadPhase / createGpuAABBManager /
+ * createGpuNphaseImplementationContext / createGpuSimulationController /
+ * createGpuDynamicsContext (physxgpu/include/PxPhysXGpu.h:101-200), constructed
+ * by Sc::Scene::Scene (simulationcontroller/src/ScScene.cpp:797-968). */
+int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out_scene);
+int32_t pxh_scene_release(pxh_scene scene);
+
+/* Adds n dynamic rigid bodies with one shape each (PxPhysics::createRigidDynamic
+ * + PxRigidActorExt::createExclusiveShape + PxScene::addActor). Host arrays:
+ * type[n] (sphere/box), dims[n*4] (radius, or half extents), pose[n*7] (p xyz,
+ * q xyzw), lin_vel[n*3], ang_vel[n*3], mass[n], inertia[n*3] (mass-space
+ * diagonal), world[n] (independent-world id, non-decreasing; NULL = world 0).
+ * Returns the index of the first added body (its PxRigidDynamicGPUIndex). */
+int32_t pxh_scene_add_bodies(pxh_scene scene, int32_t n, const int32_t* type, const float* dims, const float* pose,
+                             const float* lin_vel, const float* ang_vel, const float* mass, const float* inertia,
+                             const int32_t* world);
+
+/* Adds m static planes n.x + d = 0 (plane[m*4] = nx ny nz d; PxCreatePlane,
+ * physx/include/extensions/PxSimpleFactory.h:272), world[m] or NULL. */
+int32_t pxh_scene_add_planes(pxh_scene scene, int32_t m, const float* plane, const int32_t* world);
+
+/* One step of dt seconds: bounds, broadphase, narrowphase, partitioning,
+ * contact prep, TGS solve and integration, enqueued on the scene's stream.
+ * Replaces PxScene::simulate (physx/source/physx/src/NpScene.cpp:2907) and
+ * PxScene::fetchResults (NpSceneFetchResults.cpp:230) for the GPU pipeline. */
+int32_t pxh_scene_simulate(pxh_scene scene, float dt);
+int32_t pxh_scene_fetch_results(pxh_scene scene, int32_t block);
+
+/* PxDirectGPUAPI::getRigidDynamicData / setRigidDynamicData
+ * (physx/include/PxDirectGPUAPI.h:311,325; NpDirectGPUAPI.cpp:63-92).
+ * data: device buffer of n PxTransform (7 floats) or PxVec3 (3 floats);
+ * index: device array of n body indices (NULL = bodies 0..n-1); the copy is
+ * ordered after the scene's work and, when stream is given, before any work
+ * enqueued on that stream afterwards. */
+int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene scene, void* data, const uint32_t* index, int32_t kind, int32_t n,
+                                         void* stream);
+int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, const uint32_t* index, int32_t kind,
+                                         int32_t n, void* stream);
+
+/* PxScene::getSimulationStatistics (physx/include/PxSimulationStatistics.h):
+ * out[0] bodies, [1] broadphase pairs (body-body), [2] body-plane pairs,
+ * [3] touching pairs (constraints), [4] contact points, [5] colours,
+ * [6] steps taken. Returns the number of values written (<= n). */
+int32_t pxh_scene_get_stats(pxh_scene scene, int64_t* out, int32_t n);
+
+/* Last step's constraint table, host copy for inspection:
+ * per constraint (body a, body b or -1-plane, colour, contact count). */
+int64_t pxh_scene_get_constraints(pxh_scene scene, int32_t* out, int64_t max_constraints);
+
+/* The scene's HIP stream (hipStream_t), for ordering caller work. */
+void* pxh_scene_get_stream(pxh_scene scene);
+
+/* Standalone broadphase: PxCreateBroadPhase (physx/include/PxBroadPhase.h:629),
+ * PxBroadPhase::update (:576) and fetchResults (:587). Bounds are the
+ * non-inflated PxBounds3 (n*6 floats), inflated by distances[n]; pairs whose
+ * groups are equal are filtered (lowlevelaabb/include/BpFiltering.h:99-109).
+ * All arrays are device pointers. fetch returns device arrays of (id0,id1)
+ * uint32 pairs, id0 < id1, sorted, valid until the next update. */
+int32_t pxh_bp_create(int32_t capacity, pxh_broadphase* out_bp);
+int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* groups, const float* distances,
+                      int32_t n, void* stream);
+int32_t pxh_bp_fetch_results(pxh_broadphase bp, const uint32_t** created, int32_t* n_created,
+                             const uint32_t** deleted, int32_t* n_deleted);
+int32_t pxh_bp_release(pxh_broadphase bp);
+
+/* Batched immediate-mode contact generation on a fresh cache:
+ * PxGenerateContacts (physx/include/PxImmediateMode.h:267). Device arrays:
+ * type0/type1[n], geom0/geom1[n*4], pose0/pose1[n*7] (p xyz, q xyzw); out
+ * [n*4*8]: per pair up to 4 contacts of (point xyz, separation, normal xyz, 0);
+ * counts[n]. Normals point from shape1 to shape0. */
+int32_t pxh_generate_contacts(int32_t n, const int32_t* type0, const float* geom0, const float* pose0,
+                              const int32_t* type1, const float* geom1, const float* pose1, float contact_distance,
+                              float tolerance_length, float* out, int32_t* counts, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

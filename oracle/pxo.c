@@ -1,0 +1,1615 @@
+/* CPU oracle: plain-C restatement of the reference PhysX 5.6.1 CPU rigid-body
+ * step. TEST INFRASTRUCTURE ONLY (see pxo.h): loaded by tests/, smoke() and
+ * bench.py's CPU-baseline leg as the checker, never by the product.
+ *
+ * Every function cites the reference file:line it restates. Paths are
+ * relative to physx/source/ unless they start with physx/include.
+ * Compiled with -ffp-contract=off so float arithmetic follows the source order.
+ */
+#include "pxo.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { float x, y, z; } v3;
+typedef struct { float x, y, z, w; } qt;
+typedef struct { v3 c0, c1, c2; } m33;      /* columns */
+typedef struct { qt q; v3 p; } xf;          /* PxTransform */
+typedef struct { m33 r; v3 p; } mxf;        /* PxMatTransformV */
+
+static inline v3 V(float x, float y, float z) { v3 r = {x, y, z}; return r; }
+static inline v3 add(v3 a, v3 b) { return V(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 sub(v3 a, v3 b) { return V(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 scl(v3 a, float s) { return V(a.x * s, a.y * s, a.z * s); }
+static inline v3 neg(v3 a) { return V(-a.x, -a.y, -a.z); }
+static inline v3 mulv(v3 a, v3 b) { return V(a.x * b.x, a.y * b.y, a.z * b.z); }
+static inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline v3 cross(v3 a, v3 b) { return V(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+static inline float len2(v3 a) { return dot(a, a); }
+static inline v3 vabs(v3 a) { return V(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+static inline v3 normalize(v3 a) { return scl(a, 1.0f / sqrtf(dot(a, a))); }
+static inline float comp(v3 a, int i) { return i == 0 ? a.x : (i == 1 ? a.y : a.z); }
+
+/* PxQuat::rotate / rotateInv / operator* (physx/include/foundation/PxQuat.h) */
+static inline v3 qrot(qt q, v3 v)
+{
+	const float vx = 2.0f * v.x, vy = 2.0f * v.y, vz = 2.0f * v.z;
+	const float w2 = q.w * q.w - 0.5f;
+	const float dot2 = (q.x * vx + q.y * vy + q.z * vz);
+	return V((vx * w2 + (q.y * vz - q.z * vy) * q.w + q.x * dot2), (vy * w2 + (q.z * vx - q.x * vz) * q.w + q.y * dot2),
+	         (vz * w2 + (q.x * vy - q.y * vx) * q.w + q.z * dot2));
+}
+static inline v3 qrotinv(qt q, v3 v)
+{
+	const float vx = 2.0f * v.x, vy = 2.0f * v.y, vz = 2.0f * v.z;
+	const float w2 = q.w * q.w - 0.5f;
+	const float dot2 = (q.x * vx + q.y * vy + q.z * vz);
+	return V((vx * w2 - (q.y * vz - q.z * vy) * q.w + q.x * dot2), (vy * w2 - (q.z * vx - q.x * vz) * q.w + q.y * dot2),
+	         (vz * w2 - (q.x * vy - q.y * vx) * q.w + q.z * dot2));
+}
+static inline qt qmul(qt a, qt b)
+{
+	qt r = {a.w * b.x + b.w * a.x + a.y * b.z - b.y * a.z, a.w * b.y + b.w * a.y + a.z * b.x - b.z * a.x,
+	        a.w * b.z + b.w * a.z + a.x * b.y - b.x * a.y, a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z};
+	return r;
+}
+static inline qt qnormalize(qt q)
+{
+	const float m = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);
+	const float s = m > 0.0f ? 1.0f / m : 0.0f;
+	qt r = {q.x * s, q.y * s, q.z * s, q.w * s};
+	return r;
+}
+static inline qt qconj(qt q) { qt r = {-q.x, -q.y, -q.z, q.w}; return r; }
+
+/* PxMat33(const PxQuat&) (physx/include/foundation/PxMat33.h) */
+static m33 qmat(qt q)
+{
+	const float x2 = q.x + q.x, y2 = q.y + q.y, z2 = q.z + q.z;
+	const float xx = x2 * q.x, yy = y2 * q.y, zz = z2 * q.z;
+	const float xy = x2 * q.y, xz = x2 * q.z, xw = x2 * q.w;
+	const float yz = y2 * q.z, yw = y2 * q.w, zw = z2 * q.w;
+	m33 m;
+	m.c0 = V(1.0f - yy - zz, xy + zw, xz - yw);
+	m.c1 = V(xy - zw, 1.0f - xx - zz, yz + xw);
+	m.c2 = V(xz + yw, yz - xw, 1.0f - xx - yy);
+	return m;
+}
+static inline v3 mmul(m33 m, v3 v) { return add(add(scl(m.c0, v.x), scl(m.c1, v.y)), scl(m.c2, v.z)); }
+static inline v3 mtmul(m33 m, v3 v) { return V(dot(m.c0, v), dot(m.c1, v), dot(m.c2, v)); }
+static inline m33 mtrans(m33 m)
+{
+	m33 t;
+	t.c0 = V(m.c0.x, m.c1.x, m.c2.x);
+	t.c1 = V(m.c0.y, m.c1.y, m.c2.y);
+	t.c2 = V(m.c0.z, m.c1.z, m.c2.z);
+	return t;
+}
+static inline m33 mmm(m33 a, m33 b)
+{
+	m33 r;
+	r.c0 = mmul(a, b.c0);
+	r.c1 = mmul(a, b.c1);
+	r.c2 = mmul(a, b.c2);
+	return r;
+}
+
+static inline v3 xtrans(xf t, v3 v) { return add(qrot(t.q, v), t.p); }
+static inline v3 xtransinv(xf t, v3 v) { return qrotinv(t.q, sub(v, t.p)); }
+static inline xf xinvmul(xf a, xf b) /* a^-1 * b  (PxTransform::transformInv(PxTransform)) */
+{
+	const qt qinv = qconj(a.q);
+	xf r;
+	r.q = qmul(qinv, b.q);
+	r.p = qrot(qinv, sub(b.p, a.p));
+	return r;
+}
+static inline v3 mxtrans(mxf t, v3 v) { return add(mmul(t.r, v), t.p); }
+static inline mxf mxinvmul(mxf a, mxf b) /* PxMatTransformV::transformInv(PxMatTransformV) */
+{
+	mxf r;
+	r.r = mmm(mtrans(a.r), b.r);
+	r.p = mtmul(a.r, sub(b.p, a.p));
+	return r;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Contact generation (PCM, fresh cache)                                      */
+/* ------------------------------------------------------------------------- */
+
+typedef struct { v3 la, lb; v3 ln; float pen; } pcontact; /* PersistentContact */
+
+typedef struct {
+	v3 n;           /* from shape1 towards shape0 */
+	int count;
+	v3 p[4];
+	float sep[4];
+} cset;
+
+/* geomutils/src/pcm/GuPCMContactSphereSphere.cpp:36-67 */
+static int c_sphere_sphere(float r0, v3 p0, float r1, v3 p1, float cd, cset* o)
+{
+	const v3 d = sub(p0, p1);
+	const float dist2 = dot(d, d);
+	const float rs = r0 + r1;
+	const float inf = rs + cd;
+	if (!(inf * inf > dist2))
+		return 0;
+	const float dist = sqrtf(dist2);
+	const v3 n = (1e-5f >= dist) ? V(1.0f, 0.0f, 0.0f) : scl(d, 1.0f / dist);
+	o->n = n;
+	o->p[0] = add(scl(n, r1), p1);
+	o->sep[0] = dist - rs;
+	o->count = 1;
+	return 1;
+}
+
+/* geomutils/src/pcm/GuPCMContactSpherePlane.cpp:36-75; plane n.x + d = 0 */
+static int c_sphere_plane(float r, v3 p0, v3 pn, float pd, float cd, cset* o)
+{
+	const float sep = (dot(pn, p0) + pd) - r;
+	if (!(cd >= sep))
+		return 0;
+	o->n = pn;
+	o->p[0] = sub(p0, scl(pn, r));
+	o->sep[0] = sep;
+	o->count = 1;
+	return 1;
+}
+
+/* geomutils/src/pcm/GuPCMContactSphereBox.cpp:37-130 */
+static int c_sphere_box(float r, v3 p0, v3 h, xf t1, float cd, cset* o)
+{
+	const v3 c = xtransinv(t1, p0);
+	const v3 p = V(fminf(fmaxf(c.x, -h.x), h.x), fminf(fmaxf(c.y, -h.y), h.y), fminf(fmaxf(c.z, -h.z), h.z));
+	const v3 v = sub(c, p);
+	const float l2 = dot(v, v);
+	const float inf = r + cd;
+	if (!(inf * inf > l2))
+		return 0;
+	if (h.x >= fabsf(c.x) && h.y >= fabsf(c.y) && h.z >= fabsf(c.z))
+	{
+		const v3 ap = vabs(p);
+		const v3 ds = sub(h, ap);
+		const int con0 = ds.x >= ds.z && ds.y >= ds.z && ds.z >= ds.z;
+		const int con1 = ds.x >= ds.x && ds.y >= ds.x && ds.z >= ds.x;
+		const v3 sg = V(p.x < 0.0f ? -1.0f : 1.0f, p.y < 0.0f ? -1.0f : 1.0f, p.z < 0.0f ? -1.0f : 1.0f);
+		v3 ln;
+		float dist;
+		if (con0) { ln = V(0, 0, sg.z); dist = -ds.z; }
+		else if (con1) { ln = V(sg.x, 0, 0); dist = -ds.x; }
+		else { ln = V(0, sg.y, 0); dist = -ds.y; }
+		const v3 n = qrot(t1.q, ln);
+		o->n = n;
+		o->sep[0] = dist - r;
+		o->p[0] = sub(p0, scl(n, dist));
+	}
+	else
+	{
+		const float rl = 1.0f / sqrtf(l2);
+		const float l = 1.0f / rl;
+		const v3 n = qrot(t1.q, scl(v, rl));
+		o->n = n;
+		o->sep[0] = l - r;
+		o->p[0] = xtrans(t1, p);
+	}
+	o->count = 1;
+	return 1;
+}
+
+/* PersistentContactManifold::reduceBatchContactsCluster
+ * (geomutils/src/pcm/GuPersistentContactManifold.cpp:859-1004) */
+static void reduce_cluster(const pcontact* m, int n, pcontact* out)
+{
+	int chosen[64] = {0};
+	int idx[4];
+	float maxDist = 3.402823466e+38f;
+	int index = 0;
+	for (int i = 0; i < n; i++)
+		if (maxDist > m[i].pen) { maxDist = m[i].pen; index = i; }
+	out[0] = m[index]; chosen[index] = 1; idx[0] = index;
+	v3 v = sub(m[0].lb, out[0].lb);
+	maxDist = dot(v, v); index = 0;
+	for (int i = 1; i < n; i++)
+	{
+		v = sub(m[i].lb, out[0].lb);
+		const float d = dot(v, v);
+		if (d > maxDist) { maxDist = d; index = i; }
+	}
+	out[1] = m[index]; chosen[index] = 1; idx[1] = index;
+	maxDist = -3.402823466e+38f; index = -1;
+	v = sub(out[1].lb, out[0].lb);
+	v3 nrm = cross(v, out[0].ln);
+	const float sq = dot(nrm, nrm);
+	nrm = sq > 0.0f ? scl(nrm, 1.0f / sqrtf(sq)) : out[0].ln;
+	float minDist = 3.402823466e+38f;
+	int index1 = -1;
+	for (int i = 0; i < n; i++)
+		if (!chosen[i])
+		{
+			v = sub(m[i].lb, out[0].lb);
+			const float d = dot(v, nrm);
+			if (d > maxDist) { maxDist = d; index = i; }
+			if (minDist > d) { minDist = d; index1 = i; }
+		}
+	out[2] = m[index]; chosen[index] = 1; idx[2] = index;
+	if (minDist * maxDist > 0.0f)
+	{
+		maxDist = -3.402823466e+38f;
+		for (int i = 0; i < n; i++)
+			if (!chosen[i])
+			{
+				v = sub(m[i].lb, out[0].lb);
+				const float d = dot(v, nrm);
+				if (d > maxDist) { maxDist = d; index1 = i; }
+			}
+	}
+	out[3] = m[index1]; chosen[index1] = 1; idx[3] = index1;
+	for (int i = 0; i < n; i++)
+		if (!chosen[i])
+		{
+			float md = 3.402823466e+38f;
+			const float pen = m[i].pen;
+			int k = 0;
+			for (int j = 0; j < 4; j++)
+			{
+				const v3 w = sub(m[i].lb, out[j].lb);
+				const float d = dot(w, w);
+				if (md > d) { md = d; k = j; }
+			}
+			if (m[idx[k]].pen > pen)
+				idx[k] = i;
+		}
+	for (int j = 0; j < 4; j++)
+		out[j] = m[idx[j]];
+}
+
+/* PersistentContactManifold::reduceBatchContacts
+ * (geomutils/src/pcm/GuPersistentContactManifold.cpp:1008-1172) */
+static void reduce_batch(const pcontact* m, int n, float tol, pcontact* out)
+{
+	int ch[4];
+	int cand[64];
+	float maxPen = m[0].pen, minPen = maxPen;
+	int index = 0, ci = 0;
+	cand[0] = 0;
+	int nc = n;
+	for (int i = 1; i < n; i++)
+	{
+		cand[i] = i;
+		const float pen = m[i].pen;
+		minPen = fmaxf(minPen, pen);
+		if (maxPen > pen) { maxPen = pen; index = i; ci = i; }
+	}
+	ch[0] = index;
+	nc--; cand[ci] = cand[nc];
+	v3 v = sub(m[cand[0]].lb, m[ch[0]].lb);
+	float maxDist = dot(v, v);
+	index = cand[0]; ci = 0;
+	for (int i = 1; i < nc; i++)
+	{
+		v = sub(m[cand[i]].lb, m[ch[0]].lb);
+		const float d = dot(v, v);
+		if (d > maxDist) { maxDist = d; index = cand[i]; ci = i; }
+	}
+	ch[1] = index;
+	nc--; cand[ci] = cand[nc];
+	v = sub(m[ch[1]].lb, m[ch[0]].lb);
+	const v3 cn0 = m[ch[0]].ln;
+	v3 nrm = cross(v, cn0);
+	const float sq = dot(nrm, nrm);
+	nrm = sq > 0.0f ? scl(nrm, 1.0f / sqrtf(sq)) : cn0;
+	maxDist = -3.402823466e+38f; index = -1; ci = -1;
+	float minDist = 3.402823466e+38f;
+	int index1 = -1, ci1 = -1;
+	for (int i = 0; i < nc; i++)
+	{
+		v = sub(m[cand[i]].lb, m[ch[0]].lb);
+		const float d = dot(v, nrm);
+		if (d > maxDist) { maxDist = d; index = cand[i]; ci = i; }
+		if (minDist > d) { minDist = d; index1 = cand[i]; ci1 = i; }
+	}
+	ch[2] = index;
+	nc--; cand[ci] = cand[nc];
+	if (nc == ci1)
+		ci1 = ci;
+	if (minDist * maxDist > 0.0f)
+	{
+		maxDist = -3.402823466e+38f;
+		for (int i = 0; i < nc; i++)
+		{
+			v = sub(m[cand[i]].lb, m[ch[0]].lb);
+			const float d = dot(v, nrm);
+			if (d > maxDist) { maxDist = d; index1 = cand[i]; ci1 = i; }
+		}
+	}
+	ch[3] = index1;
+	nc--; cand[ci1] = cand[nc];
+	const float eps = tol * 0.02f;
+	if (eps > maxPen && minPen > eps)
+	{
+		for (int i = 0; i < 4; i++)
+		{
+			float pen = m[ch[i]].pen;
+			if (pen > eps)
+			{
+				int cidx = -1;
+				for (int j = 0; j < nc; j++)
+				{
+					const float pen1 = m[cand[j]].pen;
+					if (pen > pen1 && eps > pen1) { pen = pen1; cidx = j; }
+				}
+				if (cidx >= 0 && cidx < nc)
+				{
+					const int orig = ch[i];
+					ch[i] = cand[cidx];
+					cand[cidx] = orig;
+				}
+			}
+			out[i] = m[ch[i]];
+		}
+	}
+	else
+		for (int i = 0; i < 4; i++)
+			out[i] = m[ch[i]];
+}
+
+/* geomutils/src/pcm/GuPCMContactPlaneBox.cpp:36-208 (shape0 plane, shape1 box).
+ * Returns contacts with the normal from the box towards the plane. */
+static int c_plane_box(v3 pn, float pd, v3 h, xf tb, float cd, cset* o)
+{
+	/* box corner order of the reference: (x,y,z) (x,y,-z) (x,-y,z) (x,-y,-z) (-x,y,z) ... */
+	static const float sg[8][3] = {{1, 1, 1}, {1, 1, -1}, {1, -1, 1}, {1, -1, -1}, {-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+	pcontact mc[8];
+	int nc = 0;
+	const m33 R = qmat(tb.q);
+	const v3 r0 = scl(R.c0, h.x), r1 = scl(R.c1, h.y), r2 = scl(R.c2, h.z);
+	const float px = dot(pn, tb.p) + pd;  /* box centre height above the plane */
+	for (int k = 0; k < 8; k++)
+	{
+		const v3 off = add(add(scl(r0, sg[k][0]), scl(r1, sg[k][1])), scl(r2, sg[k][2]));
+		const float s = dot(pn, off);
+		if (cd - px > s)
+		{
+			const float pen = s + px;
+			const v3 corner = add(tb.p, off);
+			mc[nc].la = V(sg[k][0] * h.x, sg[k][1] * h.y, sg[k][2] * h.z);
+			mc[nc].lb = sub(corner, scl(pn, pen)); /* world point on the plane */
+			mc[nc].ln = pn;
+			mc[nc].pen = pen;
+			nc++;
+		}
+	}
+	if (nc == 0)
+		return 0;
+	pcontact red[4];
+	const pcontact* use = mc;
+	if (nc > 4)
+	{
+		reduce_cluster(mc, nc, red);
+		use = red;
+		nc = 4;
+	}
+	o->n = neg(pn);
+	o->count = 0;
+	for (int i = 0; i < nc; i++)
+		if (cd >= use[i].pen)
+		{
+			o->p[o->count] = use[i].lb;
+			o->sep[o->count] = use[i].pen;
+			o->count++;
+		}
+	return o->count;
+}
+
+/* getIncidentPolygon (geomutils/src/pcm/GuPCMContactBoxBox.cpp:42-124) */
+static void incident_polygon(v3* pts, v3* faceNormal, v3 axis, mxf t1To0, v3 e)
+{
+	const v3 u0 = t1To0.r.c0, u1 = t1To0.r.c1, u2 = t1To0.r.c2;
+	const float d0 = dot(u0, axis), d1 = dot(u1, axis), d2 = dot(u2, axis);
+	const float a0 = fabsf(d0), a1 = fabsf(d1), a2 = fabsf(d2);
+	float ex = e.x, ey = e.y, ez = e.z;
+	if (a0 >= a1 && a0 >= a2)
+	{
+		const int con = d0 > 0.0f;
+		*faceNormal = con ? neg(u0) : u0;
+		ex = con ? -ex : ex;
+		const v3 r0 = scl(u0, ex), r1 = scl(u1, ey), r2 = scl(u2, ez);
+		const v3 t0 = add(t1To0.p, r0), t1 = add(r1, r2), t2 = sub(r1, r2);
+		pts[0] = add(t0, t1); pts[1] = add(t0, t2); pts[2] = sub(t0, t1); pts[3] = sub(t0, t2);
+	}
+	else if (a1 >= a2)
+	{
+		const int con = d1 > 0.0f;
+		*faceNormal = con ? neg(u1) : u1;
+		ey = con ? -ey : ey;
+		const v3 r0 = scl(u0, ex), r1 = scl(u1, ey), r2 = scl(u2, ez);
+		const v3 t0 = add(t1To0.p, r1), t1 = add(r0, r2), t2 = sub(r0, r2);
+		pts[0] = add(t0, t1); pts[1] = add(t0, t2); pts[2] = sub(t0, t1); pts[3] = sub(t0, t2);
+	}
+	else
+	{
+		const int con = d2 > 0.0f;
+		*faceNormal = con ? neg(u2) : u2;
+		ez = con ? -ez : ez;
+		const v3 r0 = scl(u0, ex), r1 = scl(u1, ey), r2 = scl(u2, ez);
+		const v3 t0 = add(t1To0.p, r2), t1 = add(r0, r1), t2 = sub(r0, r1);
+		pts[0] = add(t0, t1); pts[1] = add(t0, t2); pts[2] = sub(t0, t1); pts[3] = sub(t0, t2);
+	}
+}
+
+/* intersectSegmentAABB (GuPCMContactBoxBox.cpp:127-167); the reference uses
+ * a fast reciprocal (V3RecipFast, ~12-bit), this restatement divides exactly. */
+static int seg_aabb(v3 p0, v3 d, v3 mx, v3 mn, float* tmin, float* tmax)
+{
+	const float eps = 1e-6f;
+	float t1[3], t2[3];
+	for (int i = 0; i < 3; i++)
+	{
+		const float pi = comp(p0, i), di = comp(d, i), mxi = comp(mx, i), mni = comp(mn, i);
+		const int par = eps > fabsf(di);
+		const int out = (pi > mxi) || (mni > pi);
+		if (par && out)
+			return 0;
+		const float od = 1.0f / di;
+		t1[i] = par ? 0.0f : (mni - pi) * od;
+		t2[i] = par ? 3.402823466e+38f : (mxi - pi) * od;
+	}
+	float ft1 = -3.402823466e+38f, ft2 = 3.402823466e+38f;
+	for (int i = 0; i < 3; i++)
+	{
+		ft1 = fmaxf(ft1, fminf(t1[i], t2[i]));
+		ft2 = fminf(ft2, fmaxf(t1[i], t2[i]));
+	}
+	const float a = fmaxf(ft1, 0.0f), b = fminf(1.0f, ft2);
+	*tmin = a;
+	*tmax = b;
+	return !((a > b) || (a > 1.0f));
+}
+
+/* Gu::contains (geomutils/src/pcm/GuPCMContactGenUtil.cpp:35-103) */
+static int poly_contains(const v3* verts, int nv, v3 p, v3 mn, v3 mx)
+{
+	if ((mn.x > p.x || p.x > mx.x) || (mn.y > p.y || p.y > mx.y))
+		return 0;
+	const float tx = p.x, ty = p.y, eps = 1.1920928955078125e-7f;
+	int inter = 0;
+	for (int i = 0, j = nv - 1; i < nv; j = i++)
+	{
+		const float jy = verts[j].y, iy = verts[i].y, jx = verts[j].x, ix = verts[i].x;
+		if ((tx == jx && ty == jy) || (tx == ix && ty == iy))
+			return 1;
+		const int f0 = jy > ty, f1 = iy > ty;
+		if (f0 != f1)
+		{
+			const float jix = ix - jx, jiy = iy - jy, jty = ty - jy;
+			const float part1 = jty * jix;
+			const float part2 = (jx + eps) * jiy;
+			const float part3 = tx * jiy;
+			const int cmp = jiy > 0.0f;
+			const float tmp = part1 + part2;
+			const float c1 = cmp ? tmp : part3, c2 = cmp ? part3 : tmp;
+			if (c1 >= c2)
+			{
+				if (inter == 1)
+					return 0;
+				inter++;
+			}
+		}
+	}
+	return inter > 0;
+}
+
+/* calculateContacts (GuPCMContactBoxBox.cpp:170-330) */
+static void calc_contacts(float exIn, float eyIn, const v3* pts, v3 incN, v3 ln, pcontact* mc, int* nc, float cd)
+{
+	const float big = 3.402823466e+38f;
+	const float ex = exIn * 1.0001f, ey = eyIn * 1.0001f;
+	int pen[4], area[4];
+	v3 bmin = V(big, big, big), bmax = V(-big, -big, -big);
+	for (int i = 0; i < 4; i++)
+	{
+		bmin = V(fminf(bmin.x, pts[i].x), fminf(bmin.y, pts[i].y), fminf(bmin.z, pts[i].z));
+		bmax = V(fmaxf(bmax.x, pts[i].x), fmaxf(bmax.y, pts[i].y), fmaxf(bmax.z, pts[i].z));
+		const float z = -pts[i].z;
+		if (cd > z)
+		{
+			pen[i] = 1;
+			if (ex >= fabsf(pts[i].x) && ey >= fabsf(pts[i].y))
+			{
+				area[i] = 1;
+				mc[*nc].la = V(pts[i].x, pts[i].y, 0.0f);
+				mc[*nc].lb = pts[i];
+				mc[*nc].ln = ln;
+				mc[*nc].pen = z;
+				(*nc)++;
+			}
+			else
+				area[i] = 0;
+		}
+		else
+		{
+			pen[i] = 0;
+			area[i] = 0;
+		}
+	}
+	if (*nc == 4)
+		return;
+	const float denom = incN.z;
+	const float qs[4][2] = {{ex, ey}, {ex, -ey}, {-ex, ey}, {-ex, -ey}};
+	for (int k = 0; k < 4; k++)
+	{
+		const v3 q0 = V(qs[k][0], qs[k][1], 0.0f);
+		if (poly_contains(pts, 4, q0, bmin, bmax))
+		{
+			const float nom = dot(incN, sub(pts[0], q0));
+			const float t = nom / denom;
+			const float pn = -t;
+			if (cd > pn)
+			{
+				mc[*nc].la = q0;
+				mc[*nc].lb = V(q0.x, q0.y, t);
+				mc[*nc].ln = ln;
+				mc[*nc].pen = pn;
+				(*nc)++;
+			}
+		}
+	}
+	const v3 ext = V(ex, ey, big);
+	const v3 negExt = V(-ex, -ey, -(cd + 1.1920928955078125e-7f));
+	for (int rs = 0, re = 3; rs < 4; re = rs++)
+	{
+		const v3 p0 = pts[rs], p1 = pts[re];
+		if (!pen[rs] && !pen[re])
+			continue;
+		const int c0 = pen[rs] && area[rs];
+		const int c1 = pen[re] && area[re];
+		if (c0 && c1)
+			continue;
+		const v3 d = sub(p1, p0);
+		float tmin, tmax;
+		if (seg_aabb(p0, d, ext, negExt, &tmin, &tmax))
+		{
+			if (!c0)
+			{
+				const v3 ip = add(scl(d, tmin), p0);
+				mc[*nc].la = V(ip.x, ip.y, 0.0f);
+				mc[*nc].lb = ip;
+				mc[*nc].ln = ln;
+				mc[*nc].pen = -ip.z;
+				(*nc)++;
+			}
+			if (!c1)
+			{
+				const v3 ip = add(scl(d, tmax), p0);
+				mc[*nc].la = V(ip.x, ip.y, 0.0f);
+				mc[*nc].lb = ip;
+				mc[*nc].ln = ln;
+				mc[*nc].pen = -ip.z;
+				(*nc)++;
+			}
+		}
+	}
+}
+
+static void seg_closest(v3 p1, v3 q1, v3 p2, v3 q2, v3* c1, v3* c2)
+{
+	/* closest points of two segments (Ericson, Real-Time Collision Detection 5.1.9) */
+	const v3 d1 = sub(q1, p1), d2 = sub(q2, p2), r = sub(p1, p2);
+	const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+	float s, t;
+	const float c = dot(d1, r), b = dot(d1, d2), den = a * e - b * b;
+	s = den > 1e-12f ? fminf(fmaxf((b * f - c * e) / den, 0.0f), 1.0f) : 0.0f;
+	t = (b * s + f) / e;
+	if (t < 0.0f) { t = 0.0f; s = fminf(fmaxf(-c / a, 0.0f), 1.0f); }
+	else if (t > 1.0f) { t = 1.0f; s = fminf(fmaxf((b - c) / a, 0.0f), 1.0f); }
+	*c1 = add(p1, scl(d1, s));
+	*c2 = add(p2, scl(d2, t));
+}
+
+/* Restatement of the GJK/EPA fallback of pcmContactBoxBox
+ * (GuPCMContactBoxBox.cpp:922-958) for the case where face clipping leaves no
+ * point: the minimum-translation axis over the 15 box SAT axes, with the
+ * witness on box1 taken from the closest points of the supporting edges. */
+static int box_box_edge(v3 e0, mxf t0, v3 e1, mxf t1, float cd, pcontact* mc)
+{
+	const v3 a[3] = {t0.r.c0, t0.r.c1, t0.r.c2}, b[3] = {t1.r.c0, t1.r.c1, t1.r.c2};
+	const float ea[3] = {e0.x, e0.y, e0.z}, eb[3] = {e1.x, e1.y, e1.z};
+	const v3 T = sub(t0.p, t1.p); /* from box1 to box0 */
+	float best = -3.402823466e+38f;
+	v3 bestN = V(0, 1, 0);
+	int bi = -1, bj = -1;
+	for (int k = 0; k < 15; k++)
+	{
+		v3 L;
+		int i = -1, j = -1;
+		if (k < 3) L = a[k];
+		else if (k < 6) L = b[k - 3];
+		else { i = (k - 6) / 3; j = (k - 6) % 3; L = cross(a[i], b[j]); }
+		const float l2 = dot(L, L);
+		if (l2 < 1e-10f)
+			continue;
+		L = scl(L, 1.0f / sqrtf(l2));
+		const float ra = ea[0] * fabsf(dot(a[0], L)) + ea[1] * fabsf(dot(a[1], L)) + ea[2] * fabsf(dot(a[2], L));
+		const float rb = eb[0] * fabsf(dot(b[0], L)) + eb[1] * fabsf(dot(b[1], L)) + eb[2] * fabsf(dot(b[2], L));
+		const float s = dot(T, L);
+		const float sep = fabsf(s) - (ra + rb);
+		if (sep > best)
+		{
+			best = sep;
+			bestN = s < 0.0f ? neg(L) : L;
+			bi = i;
+			bj = j;
+		}
+	}
+	if (best > cd)
+		return 0;
+	v3 pB;
+	if (bi >= 0)
+	{
+		/* supporting edge of box0 along -n, of box1 along +n */
+		v3 c0 = t0.p, c1 = t1.p;
+		for (int k = 0; k < 3; k++)
+		{
+			if (k != bi) c0 = sub(c0, scl(a[k], dot(a[k], bestN) > 0.0f ? ea[k] : -ea[k]));
+			if (k != bj) c1 = add(c1, scl(b[k], dot(b[k], bestN) > 0.0f ? eb[k] : -eb[k]));
+		}
+		v3 q0, q1;
+		seg_closest(sub(c0, scl(a[bi], ea[bi])), add(c0, scl(a[bi], ea[bi])), sub(c1, scl(b[bj], eb[bj])),
+		            add(c1, scl(b[bj], eb[bj])), &q0, &q1);
+		pB = q1;
+	}
+	else
+	{
+		/* face axis: deepest vertex of the incident box */
+		v3 c = bestN;
+		(void)c;
+		pB = t1.p;
+		for (int k = 0; k < 3; k++)
+			pB = add(pB, scl(b[k], dot(b[k], bestN) > 0.0f ? eb[k] : -eb[k]));
+	}
+	/* stored in box-local spaces like the reference manifold */
+	mc[0].la = mtmul(t0.r, sub(add(pB, scl(bestN, -best)), t0.p));
+	mc[0].lb = mtmul(t1.r, sub(pB, t1.p));
+	mc[0].ln = mtmul(t1.r, bestN);
+	mc[0].pen = best;
+	return 1;
+}
+
+/* doBoxBoxGenerateContacts + pcmContactBoxBox on a fresh manifold
+ * (geomutils/src/pcm/GuPCMContactBoxBox.cpp:332-818, 820-968) */
+static int c_box_box(v3 e0, xf x0, v3 e1, xf x1, float cd, float tol, cset* o)
+{
+	mxf t0, t1;
+	t0.r = qmat(x0.q); t0.p = x0.p;
+	t1.r = qmat(x1.q); t1.p = x1.p;
+	t0.r.c0 = normalize(t0.r.c0); t0.r.c1 = normalize(t0.r.c1); t0.r.c2 = normalize(t0.r.c2);
+	t1.r.c0 = normalize(t1.r.c0); t1.r.c1 = normalize(t1.r.c1); t1.r.c2 = normalize(t1.r.c2);
+
+	const mxf t1To0 = mxinvmul(t0, t1);
+	const m33 r0To1 = mtrans(t1To0.r);
+	const float uEps = 1e-6f;
+	const float ea0 = e0.x, ea1 = e0.y, ea2 = e0.z, eb0 = e1.x, eb1 = e1.y, eb2 = e1.z;
+	const float tx = t1To0.p.x, ty = t1To0.p.y, tz = t1To0.p.z;
+	const v3 col0 = t1To0.r.c0, col1 = t1To0.r.c1, col2 = t1To0.r.c2;
+	const v3 a10c0 = add(vabs(col0), V(uEps, uEps, uEps)), a10c1 = add(vabs(col1), V(uEps, uEps, uEps)),
+	         a10c2 = add(vabs(col2), V(uEps, uEps, uEps));
+	const v3 a01c0 = add(vabs(r0To1.c0), V(uEps, uEps, uEps)), a01c1 = add(vabs(r0To1.c1), V(uEps, uEps, uEps)),
+	         a01c2 = add(vabs(r0To1.c2), V(uEps, uEps, uEps));
+	float sign[6], overlap[6], ra, rb, rs;
+	v3 vt;
+
+	sign[0] = tx; vt = mulv(a01c0, e1); rb = vt.x + (vt.y + vt.z);
+	overlap[0] = ((ea0 + rb) - fabsf(sign[0])) + cd; if (0.0f > overlap[0]) return 0;
+	sign[1] = ty; vt = mulv(a01c1, e1); rb = vt.x + (vt.y + vt.z);
+	overlap[1] = ((ea1 + rb) - fabsf(sign[1])) + cd; if (0.0f > overlap[1]) return 0;
+	sign[2] = tz; vt = mulv(a01c2, e1); rb = vt.x + (vt.y + vt.z);
+	overlap[2] = ((ea2 + rb) - fabsf(sign[2])) + cd; if (0.0f > overlap[2]) return 0;
+	sign[3] = dot(t1To0.p, col0); vt = mulv(a10c0, e0); ra = vt.x + (vt.y + vt.z);
+	overlap[3] = ((ra + eb0) - fabsf(sign[3])) + cd; if (0.0f > overlap[3]) return 0;
+	sign[4] = dot(t1To0.p, col1); vt = mulv(a10c1, e0); ra = vt.x + (vt.y + vt.z);
+	overlap[4] = ((ra + eb1) - fabsf(sign[4])) + cd; if (0.0f > overlap[4]) return 0;
+	sign[5] = dot(t1To0.p, col2); vt = mulv(a10c2, e0); ra = vt.x + (vt.y + vt.z);
+	overlap[5] = ((ra + eb2) - fabsf(sign[5])) + cd; if (0.0f > overlap[5]) return 0;
+
+#define EDGE_TEST(absSign, raA, raB, rbA, rbB) \
+	{ ra = (raA) + (raB); rb = (rbA) + (rbB); rs = (ra + rb) + cd; if ((absSign) > rs) return 0; }
+	EDGE_TEST(fabsf(col0.y * tz - col0.z * ty), a10c0.z * ea1, a10c0.y * ea2, a01c0.z * eb1, a01c0.y * eb2);
+	EDGE_TEST(fabsf(col1.y * tz - col1.z * ty), a10c1.z * ea1, a10c1.y * ea2, a01c0.z * eb0, a01c0.x * eb2);
+	EDGE_TEST(fabsf(col2.y * tz - col2.z * ty), a10c2.z * ea1, a10c2.y * ea2, a01c0.y * eb0, a01c0.x * eb1);
+	EDGE_TEST(fabsf(col0.z * tx - col0.x * tz), a10c0.z * ea0, a10c0.x * ea2, a01c1.z * eb1, a01c1.y * eb2);
+	EDGE_TEST(fabsf(col1.z * tx - col1.x * tz), a10c1.z * ea0, a10c1.x * ea2, a01c1.z * eb0, a01c1.x * eb2);
+	EDGE_TEST(fabsf(col2.z * tx - col2.x * tz), a10c2.z * ea0, a10c2.x * ea2, a01c1.y * eb0, a01c1.x * eb1);
+	EDGE_TEST(fabsf(col0.x * ty - col0.y * tx), a10c0.y * ea0, a10c0.x * ea1, a01c2.z * eb1, a01c2.y * eb2);
+	EDGE_TEST(fabsf(col1.x * ty - col1.y * tx), a10c1.y * ea0, a10c1.x * ea1, a01c2.z * eb0, a01c2.x * eb2);
+	EDGE_TEST(fabsf(col2.x * ty - col2.y * tx), a10c2.y * ea0, a10c2.x * ea1, a01c2.y * eb0, a01c2.x * eb1);
+#undef EDGE_TEST
+
+	int feature = 0;
+	float minOverlap = overlap[0];
+	for (int i = 1; i < 6; i++)
+		if (minOverlap > overlap[i]) { minOverlap = overlap[i]; feature = i; }
+
+	const v3 ax00 = t0.r.c0, ax01 = t0.r.c1, ax02 = t0.r.c2, ax10 = t1.r.c0, ax11 = t1.r.c1, ax12 = t1.r.c2;
+	mxf nt;
+	v3 mtd, incN, pts[4];
+	int flip = 0;
+	pcontact mc[16];
+	int nc = 0;
+	const int le0 = 0.0f >= sign[feature];
+	switch (feature)
+	{
+	case 0:
+		if (le0) { mtd = ax00; nt.r.c0 = neg(ax02); nt.r.c1 = ax01; nt.r.c2 = ax00; nt.p = sub(t0.p, scl(ax00, ea0)); }
+		else { mtd = neg(ax00); nt.r.c0 = ax02; nt.r.c1 = ax01; nt.r.c2 = neg(ax00); nt.p = add(scl(ax00, ea0), t0.p); }
+		break;
+	case 1:
+		if (le0) { mtd = ax01; nt.r.c0 = ax00; nt.r.c1 = neg(ax02); nt.r.c2 = ax01; nt.p = sub(t0.p, scl(ax01, ea1)); }
+		else { mtd = neg(ax01); nt.r.c0 = ax00; nt.r.c1 = ax02; nt.r.c2 = neg(ax01); nt.p = add(scl(ax01, ea1), t0.p); }
+		break;
+	case 2:
+		if (le0) { mtd = ax02; nt.r.c0 = ax00; nt.r.c1 = ax01; nt.r.c2 = ax02; nt.p = sub(t0.p, scl(ax02, ea2)); }
+		else { mtd = neg(ax02); nt.r.c0 = ax00; nt.r.c1 = neg(ax01); nt.r.c2 = neg(ax02); nt.p = add(scl(ax02, ea2), t0.p); }
+		break;
+	case 3:
+		flip = 1;
+		if (le0) { mtd = ax10; nt.r.c0 = ax12; nt.r.c1 = ax11; nt.r.c2 = neg(ax10); nt.p = add(scl(ax10, eb0), t1.p); }
+		else { mtd = neg(ax10); nt.r.c0 = neg(ax12); nt.r.c1 = ax11; nt.r.c2 = ax10; nt.p = sub(t1.p, scl(ax10, eb0)); }
+		break;
+	case 4:
+		flip = 1;
+		if (le0) { mtd = ax11; nt.r.c0 = ax10; nt.r.c1 = ax12; nt.r.c2 = neg(ax11); nt.p = add(scl(ax11, eb1), t1.p); }
+		else { mtd = neg(ax11); nt.r.c0 = ax10; nt.r.c1 = neg(ax12); nt.r.c2 = ax11; nt.p = sub(t1.p, scl(ax11, eb1)); }
+		break;
+	default:
+		flip = 1;
+		if (le0) { mtd = ax12; nt.r.c0 = ax10; nt.r.c1 = neg(ax11); nt.r.c2 = neg(ax12); nt.p = add(scl(ax12, eb2), t1.p); }
+		else { mtd = neg(ax12); nt.r.c0 = ax10; nt.r.c1 = ax11; nt.r.c2 = ax12; nt.p = sub(t1.p, scl(ax12, eb2)); }
+		break;
+	}
+	const v3 ln = mtmul(nt.r, mtd);
+	if (!flip)
+	{
+		const mxf t1ToNew = mxinvmul(nt, t1);
+		incident_polygon(pts, &incN, neg(ln), t1ToNew, e1);
+		if (feature == 0) calc_contacts(ea2, ea1, pts, incN, ln, mc, &nc, cd);
+		else if (feature == 1) calc_contacts(ea0, ea2, pts, incN, ln, mc, &nc, cd);
+		else calc_contacts(ea0, ea1, pts, incN, ln, mc, &nc, cd);
+	}
+	else
+	{
+		const mxf t0ToNew = mxinvmul(nt, t0);
+		incident_polygon(pts, &incN, ln, t0ToNew, e0);
+		if (feature == 3) calc_contacts(eb2, eb1, pts, incN, ln, mc, &nc, cd);
+		else if (feature == 4) calc_contacts(eb0, eb2, pts, incN, ln, mc, &nc, cd);
+		else calc_contacts(eb0, eb1, pts, incN, ln, mc, &nc, cd);
+	}
+
+	v3 worldN;
+	if (nc != 0)
+	{
+		if (flip)
+			for (int i = 0; i < nc; i++) { const v3 t = mc[i].lb; mc[i].lb = mc[i].la; mc[i].la = t; }
+		const mxf newTo1 = mxinvmul(t1, nt);
+		const mxf newTo0 = mxinvmul(t0, nt);
+		const v3 lnB = mmul(newTo1.r, mc[0].ln);
+		for (int i = 0; i < nc; i++)
+		{
+			mc[i].la = mxtrans(newTo0, mc[i].la);
+			mc[i].lb = mxtrans(newTo1, mc[i].lb);
+			mc[i].ln = lnB;
+		}
+		pcontact red[4];
+		const pcontact* use = mc;
+		if (nc > 4)
+		{
+			reduce_batch(mc, nc, tol, red);
+			use = red;
+			nc = 4;
+		}
+		worldN = normalize(mmul(t1.r, use[0].ln));
+		o->n = worldN;
+		o->count = nc;
+		for (int i = 0; i < nc; i++)
+		{
+			o->p[i] = mxtrans(t1, use[i].lb);
+			o->sep[i] = use[i].pen;
+		}
+		return nc;
+	}
+	if (!box_box_edge(e0, t0, e1, t1, cd, mc))
+		return 0;
+	o->n = normalize(mmul(t1.r, mc[0].ln));
+	o->count = 1;
+	o->p[0] = mxtrans(t1, mc[0].lb);
+	o->sep[0] = mc[0].pen;
+	return 1;
+}
+
+static xf load_pose(const float* p)
+{
+	xf t;
+	t.p = V(p[0], p[1], p[2]);
+	t.q.x = p[3]; t.q.y = p[4]; t.q.z = p[5]; t.q.w = p[6];
+	return t;
+}
+
+/* plane shape pose -> (n, d): the plane is the local x=0 plane of its pose
+ * (PxPlaneGeometry, physx/include/geometry/PxPlaneGeometry.h) */
+static void plane_from_pose(xf t, v3* n, float* d)
+{
+	*n = qrot(t.q, V(1.0f, 0.0f, 0.0f));
+	*d = -dot(*n, t.p);
+}
+
+int pxo_generate_contacts(int type0, const float* g0, const float* pose0, int type1, const float* g1,
+                          const float* pose1, float cd, float tol, float* out)
+{
+	cset c;
+	c.count = 0;
+	const xf x0 = load_pose(pose0), x1 = load_pose(pose1);
+	int n = 0;
+	if (type0 == PXO_SPHERE && type1 == PXO_SPHERE)
+		n = c_sphere_sphere(g0[0], x0.p, g1[0], x1.p, cd, &c);
+	else if (type0 == PXO_SPHERE && type1 == PXO_PLANE)
+	{
+		v3 pn; float pd;
+		plane_from_pose(x1, &pn, &pd);
+		n = c_sphere_plane(g0[0], x0.p, pn, pd, cd, &c);
+	}
+	else if (type0 == PXO_SPHERE && type1 == PXO_BOX)
+		n = c_sphere_box(g0[0], x0.p, V(g1[0], g1[1], g1[2]), x1, cd, &c);
+	else if (type0 == PXO_PLANE && type1 == PXO_BOX)
+	{
+		v3 pn; float pd;
+		plane_from_pose(x0, &pn, &pd);
+		n = c_plane_box(pn, pd, V(g1[0], g1[1], g1[2]), x1, cd, &c);
+	}
+	else if (type0 == PXO_BOX && type1 == PXO_BOX)
+		n = c_box_box(V(g0[0], g0[1], g0[2]), x0, V(g1[0], g1[1], g1[2]), x1, cd, tol, &c);
+	else
+		return -1;
+	for (int i = 0; i < n; i++)
+	{
+		float* o = out + 8 * i;
+		o[0] = c.p[i].x; o[1] = c.p[i].y; o[2] = c.p[i].z; o[3] = c.sep[i];
+		o[4] = c.n.x; o[5] = c.n.y; o[6] = c.n.z; o[7] = 0.0f;
+	}
+	return n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Broadphase: ABP's inclusive overlap of bounds inflated by per-box distances */
+/* (lowlevelaabb/src/BpBroadPhaseABP.cpp:1187-1192 inflation, :2246-2262 and  */
+/*  :2474 inclusive Y/Z and X tests)                                          */
+/* ------------------------------------------------------------------------- */
+
+typedef struct { float minx; int idx; } sortx;
+static int cmp_sortx(const void* a, const void* b)
+{
+	const sortx* x = (const sortx*)a;
+	const sortx* y = (const sortx*)b;
+	if (x->minx < y->minx) return -1;
+	if (x->minx > y->minx) return 1;
+	return x->idx - y->idx;
+}
+static int cmp_pair(const void* a, const void* b)
+{
+	const uint32_t* x = (const uint32_t*)a;
+	const uint32_t* y = (const uint32_t*)b;
+	if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+	if (x[1] != y[1]) return x[1] < y[1] ? -1 : 1;
+	return 0;
+}
+
+typedef struct { uint32_t* p; long long n, cap; } pairvec;
+static void pv_push(pairvec* v, uint32_t a, uint32_t b)
+{
+	if (v->n == v->cap)
+	{
+		v->cap = v->cap ? v->cap * 2 : 1024;
+		v->p = (uint32_t*)realloc(v->p, sizeof(uint32_t) * 2 * (size_t)v->cap);
+	}
+	v->p[2 * v->n] = a < b ? a : b;
+	v->p[2 * v->n + 1] = a < b ? b : a;
+	v->n++;
+}
+
+static void bp_core(const float* lo, const float* hi, const int32_t* world, int n, pairvec* out)
+{
+	sortx* s = (sortx*)malloc(sizeof(sortx) * (size_t)(n ? n : 1));
+	for (int i = 0; i < n; i++) { s[i].minx = lo[3 * i]; s[i].idx = i; }
+	qsort(s, (size_t)n, sizeof(sortx), cmp_sortx);
+	for (int a = 0; a < n; a++)
+	{
+		const int i = s[a].idx;
+		const float maxx = hi[3 * i];
+		for (int b = a + 1; b < n && s[b].minx <= maxx; b++)
+		{
+			const int j = s[b].idx;
+			if (world && world[i] != world[j])
+				continue;
+			if (hi[3 * j + 1] < lo[3 * i + 1] || hi[3 * i + 1] < lo[3 * j + 1] || hi[3 * j + 2] < lo[3 * i + 2] ||
+			    hi[3 * i + 2] < lo[3 * j + 2])
+				continue;
+			pv_push(out, (uint32_t)i, (uint32_t)j);
+		}
+	}
+	free(s);
+	if (out->n)
+		qsort(out->p, (size_t)out->n, 2 * sizeof(uint32_t), cmp_pair);
+}
+
+long long pxo_bp_pairs(const float* bounds, const float* dist, const int32_t* world, int n, uint32_t* out,
+                       long long max_pairs)
+{
+	float* lo = (float*)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1));
+	float* hi = (float*)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1));
+	for (int i = 0; i < n; i++)
+		for (int k = 0; k < 3; k++)
+		{
+			lo[3 * i + k] = bounds[6 * i + k] - dist[i];
+			hi[3 * i + k] = bounds[6 * i + 3 + k] + dist[i];
+		}
+	pairvec pv = {0, 0, 0};
+	bp_core(lo, hi, world, n, &pv);
+	for (long long i = 0; i < pv.n && i < max_pairs; i++)
+	{
+		out[2 * i] = pv.p[2 * i];
+		out[2 * i + 1] = pv.p[2 * i + 1];
+	}
+	free(lo);
+	free(hi);
+	free(pv.p);
+	return pv.n;
+}
+
+/* ------------------------------------------------------------------------- */
+/* World and TGS step                                                         */
+/* ------------------------------------------------------------------------- */
+
+typedef struct {
+	int a, b;            /* body indices; b < 0: static plane -1-b */
+	int color;
+	uint64_t key;
+	cset c;              /* contacts, normal from b towards a */
+	/* prepared rows */
+	float invMassA, invMassB;
+	v3 raXnI[4], rbXnI[4];
+	float sepErr[4], biasCoef[4], velMul[4], recipResp[4], force[4];
+	int nFric;
+	v3 fn[4], fraXnI[4], frbXnI[4];
+	float ferr[4], fvelMul[4], fbiasScale, fscale, fforce[4];
+	float maxPenBias;
+} pxo_con;
+
+typedef struct {
+	pxo_desc d;
+	int n, cap;
+	int32_t *type, *scene, *local;
+	float *dims;
+	xf* pose;
+	v3 *lv, *av;
+	float* invMass;
+	v3* invI;
+	int m, mcap;
+	v3* pn;
+	float* pd;
+	int32_t *pscene, *plocal;
+	/* scratch of the last step */
+	pxo_con* con;
+	long long ncon, concap;
+	long long stats[6];
+} pxo_world;
+
+void pxo_default_desc(pxo_desc* d)
+{
+	memset(d, 0, sizeof(*d));
+	d->gravity[1] = -9.81f;
+	d->solver = 1;
+	d->pos_iters = 4;
+	d->vel_iters = 1;
+	d->contact_offset = 0.02f;
+	d->rest_offset = 0.0f;
+	d->static_friction = 0.5f;
+	d->dynamic_friction = 0.5f;
+	d->restitution = 0.0f;
+	d->linear_damping = 0.0f;
+	d->angular_damping = 0.05f;
+	d->max_lin_vel_sq = 1e32f;
+	d->max_ang_vel_sq = 100.0f * 100.0f;
+	d->bounce_threshold = 2.0f;
+	d->friction_offset_threshold = 0.04f;
+	d->correlation_distance = 0.025f;
+	d->tolerance_length = 1.0f;
+}
+
+void* pxo_world_create(const pxo_desc* d)
+{
+	pxo_world* w = (pxo_world*)calloc(1, sizeof(pxo_world));
+	w->d = *d;
+	return w;
+}
+
+void pxo_world_release(void* wp)
+{
+	pxo_world* w = (pxo_world*)wp;
+	if (!w) return;
+	free(w->type); free(w->scene); free(w->local); free(w->dims); free(w->pose); free(w->lv); free(w->av);
+	free(w->invMass); free(w->invI); free(w->pn); free(w->pd); free(w->pscene); free(w->plocal); free(w->con);
+	free(w);
+}
+
+int pxo_world_add_bodies(void* wp, int n, const int32_t* type, const float* dims, const float* pose, const float* lv,
+                         const float* av, const float* mass, const float* inertia, const int32_t* scene)
+{
+	pxo_world* w = (pxo_world*)wp;
+	const int nn = w->n + n;
+	w->type = (int32_t*)realloc(w->type, sizeof(int32_t) * nn);
+	w->scene = (int32_t*)realloc(w->scene, sizeof(int32_t) * nn);
+	w->local = (int32_t*)realloc(w->local, sizeof(int32_t) * nn);
+	w->dims = (float*)realloc(w->dims, sizeof(float) * 4 * nn);
+	w->pose = (xf*)realloc(w->pose, sizeof(xf) * nn);
+	w->lv = (v3*)realloc(w->lv, sizeof(v3) * nn);
+	w->av = (v3*)realloc(w->av, sizeof(v3) * nn);
+	w->invMass = (float*)realloc(w->invMass, sizeof(float) * nn);
+	w->invI = (v3*)realloc(w->invI, sizeof(v3) * nn);
+	for (int i = 0; i < n; i++)
+	{
+		const int k = w->n + i;
+		w->type[k] = type[i];
+		w->scene[k] = scene ? scene[i] : 0;
+		int loc = 0;
+		for (int j = 0; j < k; j++)
+			loc += w->scene[j] == w->scene[k];
+		w->local[k] = loc;
+		memcpy(w->dims + 4 * k, dims + 4 * i, 4 * sizeof(float));
+		w->pose[k] = load_pose(pose + 7 * i);
+		w->lv[k] = V(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]);
+		w->av[k] = V(av[3 * i], av[3 * i + 1], av[3 * i + 2]);
+		w->invMass[k] = mass[i] > 0.0f ? 1.0f / mass[i] : 0.0f;
+		w->invI[k] = V(inertia[3 * i] > 0.0f ? 1.0f / inertia[3 * i] : 0.0f,
+		               inertia[3 * i + 1] > 0.0f ? 1.0f / inertia[3 * i + 1] : 0.0f,
+		               inertia[3 * i + 2] > 0.0f ? 1.0f / inertia[3 * i + 2] : 0.0f);
+	}
+	w->n = nn;
+	return nn;
+}
+
+int pxo_world_add_planes(void* wp, int m, const float* planes, const int32_t* scene)
+{
+	pxo_world* w = (pxo_world*)wp;
+	const int nm = w->m + m;
+	w->pn = (v3*)realloc(w->pn, sizeof(v3) * nm);
+	w->pd = (float*)realloc(w->pd, sizeof(float) * nm);
+	w->pscene = (int32_t*)realloc(w->pscene, sizeof(int32_t) * nm);
+	w->plocal = (int32_t*)realloc(w->plocal, sizeof(int32_t) * nm);
+	for (int i = 0; i < m; i++)
+	{
+		const int k = w->m + i;
+		w->pn[k] = V(planes[4 * i], planes[4 * i + 1], planes[4 * i + 2]);
+		w->pd[k] = planes[4 * i + 3];
+		w->pscene[k] = scene ? scene[i] : 0;
+		int loc = 0;
+		for (int j = 0; j < k; j++)
+			loc += w->pscene[j] == w->pscene[k];
+		w->plocal[k] = loc;
+	}
+	w->m = nm;
+	return nm;
+}
+
+void pxo_world_get_bodies(void* wp, float* pose, float* lv, float* av)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < w->n; i++)
+	{
+		if (pose)
+		{
+			float* o = pose + 7 * i;
+			o[0] = w->pose[i].p.x; o[1] = w->pose[i].p.y; o[2] = w->pose[i].p.z;
+			o[3] = w->pose[i].q.x; o[4] = w->pose[i].q.y; o[5] = w->pose[i].q.z; o[6] = w->pose[i].q.w;
+		}
+		if (lv) { lv[3 * i] = w->lv[i].x; lv[3 * i + 1] = w->lv[i].y; lv[3 * i + 2] = w->lv[i].z; }
+		if (av) { av[3 * i] = w->av[i].x; av[3 * i + 1] = w->av[i].y; av[3 * i + 2] = w->av[i].z; }
+	}
+}
+
+void pxo_world_set_bodies(void* wp, const float* pose, const float* lv, const float* av)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < w->n; i++)
+	{
+		if (pose) w->pose[i] = load_pose(pose + 7 * i);
+		if (lv) w->lv[i] = V(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]);
+		if (av) w->av[i] = V(av[3 * i], av[3 * i + 1], av[3 * i + 2]);
+	}
+}
+
+void pxo_world_stats(void* wp, long long* out)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < 6; i++) out[i] = w->stats[i];
+}
+
+long long pxo_world_constraints(void* wp, int32_t* out, long long max)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (long long i = 0; i < w->ncon && i < max; i++)
+	{
+		out[4 * i] = w->con[i].a;
+		out[4 * i + 1] = w->con[i].b;
+		out[4 * i + 2] = w->con[i].color;
+		out[4 * i + 3] = w->con[i].c.count;
+	}
+	return w->ncon;
+}
+
+/* shape AABB (geomutils/src/GuBounds.cpp:354-395: sphere centre +- r, box
+ * basisExtent, physx/include/foundation/PxBounds3.h:301-314) */
+static void shape_bounds(int type, const float* dims, xf t, v3* mn, v3* mx)
+{
+	if (type == PXO_SPHERE)
+	{
+		const v3 e = V(dims[0], dims[0], dims[0]);
+		*mn = sub(t.p, e);
+		*mx = add(t.p, e);
+		return;
+	}
+	const m33 R = qmat(t.q);
+	const v3 c0 = scl(R.c0, dims[0]), c1 = scl(R.c1, dims[1]), c2 = scl(R.c2, dims[2]);
+	const v3 w = V(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y),
+	               fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z));
+	*mn = sub(t.p, w);
+	*mx = add(t.p, w);
+}
+
+/* Constraint ordering key: a 64-bit mix of the scene-local identities of the
+ * two bodies (or body and plane), so the colouring of a scene does not depend
+ * on other scenes batched with it. */
+static uint64_t mix64(uint64_t z)
+{
+	z += 0x9E3779B97F4A7C15ull;
+	z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+	z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+	return z ^ (z >> 31);
+}
+static uint64_t con_key(uint32_t la, uint32_t lb) { return mix64(((uint64_t)la << 32) | lb); }
+
+static void push_con(pxo_world* w, int a, int b, const cset* c)
+{
+	if (w->ncon == w->concap)
+	{
+		w->concap = w->concap ? 2 * w->concap : 1024;
+		w->con = (pxo_con*)realloc(w->con, sizeof(pxo_con) * (size_t)w->concap);
+	}
+	pxo_con* k = &w->con[w->ncon++];
+	memset(k, 0, sizeof(*k));
+	k->a = a;
+	k->b = b;
+	k->c = *c;
+	const uint32_t la = (uint32_t)w->local[a];
+	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b] : (0x80000000u | (uint32_t)w->plocal[-1 - b]);
+	k->key = con_key(la, lb);
+}
+
+/* Solver body (PxTGSSolverBodyVel + PxTGSSolverBodyTxInertia + data),
+ * lowleveldynamics/src/DyTGSDynamics.cpp:154-242 copyToSolverBodyDataStep */
+typedef struct {
+	v3 lin, ang, dlin, dang;   /* linearVelocity, angularVelocity (sqrt-inertia space), deltaLinDt, deltaAngDt */
+	v3 p;                      /* body2WorldP */
+	qt dq;                     /* deltaBody2WorldQ */
+	m33 sqrtInvI;
+	v3 origLin, origAng;
+	float invMass;
+} sbody;
+
+/* integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476) */
+static void integrate_step(sbody* b, float dt)
+{
+	const v3 delta = scl(b->lin, dt);
+	const v3 unm = b->ang;
+	const v3 amv = mmul(b->sqrtInvI, b->ang);
+	const float w2 = dot(amv, amv);
+	b->p = add(b->p, delta);
+	if (w2 != 0.0f)
+	{
+		const float wl = sqrtf(w2);
+		const float v = dt * wl * 0.5f;
+		float s = sinf(v);
+		const float q = cosf(v);
+		s /= wl;
+		const v3 pqr = scl(amv, s);
+		const qt qv = {pqr.x, pqr.y, pqr.z, 0.0f};
+		qt res = qmul(qv, b->dq);
+		res.x += b->dq.x * q; res.y += b->dq.y * q; res.z += b->dq.z * q; res.w += b->dq.w * q;
+		b->dq = qnormalize(res);
+	}
+	b->dang = add(b->dang, scl(unm, dt));
+	b->dlin = add(b->dlin, delta);
+}
+
+/* setupFinalizeSolverConstraints / constructContactConstraintStep for one
+ * rigid-rigid (or rigid-static) contact patch:
+ * lowleveldynamics/src/DyTGSContactPrep.cpp:426-812 (rows), :310-404 (normal row),
+ * friction patch creation and anchors DyFrictionCorrelation.cpp:55-292 */
+static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbody* world, const xf* frame,
+                     float invStepDt, float stepDt, float totalDt, float scale)
+{
+	const sbody* b0 = &sb[k->a];
+	const sbody* b1 = k->b >= 0 ? &sb[k->b] : world;
+	const xf f0 = frame[k->a];
+	xf f1;
+	if (k->b >= 0) f1 = frame[k->b];
+	else { f1.p = V(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1; }
+	const float invDtp8 = invStepDt * scale;
+	const v3 n = k->c.n;
+	const float nlen2 = dot(n, n);
+	k->invMassA = b0->invMass;
+	k->invMassB = b1->invMass;
+	k->maxPenBias = -1e32f; /* max(penBiasClamp0, penBiasClamp1), defaults -1e32 (PxvDynamics.h:140) */
+	const float imn0 = b0->invMass * nlen2, imn1 = -b1->invMass * nlen2;
+	const float restDist = 2.0f * w->d.rest_offset;
+	(void)totalDt;
+	for (int i = 0; i < k->c.count; i++)
+	{
+		const v3 ra = sub(k->c.p[i], f0.p), rb = sub(k->c.p[i], f1.p);
+		const v3 raXn = cross(ra, n), rbXn = cross(rb, n);
+		const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
+		const float i0 = dot(raXnI, raXnI), i1 = dot(rbXnI, rbXnI);
+		const float resp0 = imn0 + i0, resp1 = i1 - imn1;
+		const float unit = resp0 + resp1;
+		const float pen = k->c.sep[i] - restDist;
+		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+		k->raXnI[i] = raXnI;
+		k->rbXnI[i] = rbXnI;
+		k->sepErr[i] = pen;
+		k->biasCoef[i] = -(pen > 0.0f ? invStepDt : invDtp8);
+		k->velMul[i] = recip;
+		k->recipResp[i] = recip;
+		k->force[i] = 0.0f;
+	}
+	/* friction patch: anchors (growPatches) among contacts closer than the offset threshold */
+	int nAnch = 0;
+	v3 wa[2];
+	float pd2 = 0.0f;
+	const float thr = w->d.friction_offset_threshold + restDist;
+	for (int j = 0; j < k->c.count; j++)
+	{
+		const v3 wp = k->c.p[j];
+		if (k->c.sep[j] < thr)
+		{
+			if (nAnch == 0) { wa[0] = wp; nAnch = 1; }
+			else if (nAnch == 1)
+			{
+				pd2 = len2(sub(wp, wa[0]));
+				if (pd2 > 1e-8f) { wa[1] = wp; nAnch = 2; }
+			}
+			else
+			{
+				const float d0 = len2(sub(wp, wa[0])), d1 = len2(sub(wp, wa[1]));
+				if (d0 > d1) { if (d0 > pd2) { wa[1] = wp; pd2 = d0; } }
+				else if (d1 > pd2) { wa[0] = wp; pd2 = d1; }
+			}
+		}
+	}
+	k->nFric = 2 * nAnch;
+	if (nAnch == 0)
+		return;
+	const v3 lvr = sub(b0->origLin, b1->origLin);
+	const v3 fb1 = V(0.0f, -n.z, n.y), fb2 = V(-n.y, n.x, 0.0f);
+	const v3 fb = (0.70710678f > fabsf(n.x)) ? fb1 : fb2;
+	v3 t0 = sub(lvr, scl(n, dot(n, lvr)));
+	t0 = len2(t0) > 0.0001f ? t0 : fb;
+	t0 = normalize(t0);
+	const v3 t1 = normalize(cross(n, t0));
+	const v3 relTr = sub(f0.p, f1.p);
+	k->fscale = nAnch == 2 ? 0.5f : 1.0f;
+	k->fbiasScale = invStepDt * scale;
+	for (int j = 0; j < nAnch; j++)
+	{
+		const v3 a0 = xtransinv(f0, wa[j]), a1 = xtransinv(f1, wa[j]);
+		const v3 ra = qrot(f0.q, a0), rb = qrot(f1.q, a1);
+		const v3 err = add(sub(ra, rb), relTr);
+		for (int d = 0; d < 2; d++)
+		{
+			const v3 t = d == 0 ? t0 : t1;
+			const v3 raXn = cross(ra, t), rbXn = cross(rb, t);
+			const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
+			const float resp0 = imn0 + dot(raXnI, raXnI), resp1 = dot(rbXnI, rbXnI) - imn1;
+			const float unit = resp0 + resp1;
+			const int r = 2 * j + d;
+			k->fn[r] = t;
+			k->ferr[r] = dot(err, t);
+			k->fraXnI[r] = raXnI;
+			k->frbXnI[r] = rbXnI;
+			k->fvelMul[r] = unit > 0.0f ? scale / unit : 0.0f;
+			k->fforce[r] = 0.0f;
+		}
+	}
+}
+
+/* solveContact / solveDynamicContactsStep
+ * (lowleveldynamics/src/DyTGSContactPrep.cpp:1490-1843) */
+static void solve_con(pxo_con* k, sbody* b0, sbody* b1, float minPen, float elapsed, float mu_s, float mu_d)
+{
+	v3 l0 = b0->lin, l1 = b1->lin, a0 = b0->ang, a1 = b1->ang;
+	const v3 am0 = b0->dang, am1 = b1->dang;
+	const v3 rel = sub(b0->dlin, b1->dlin);
+	const v3 n = k->c.n;
+	const v3 nim0 = scl(n, k->invMassA), nim1 = scl(n, k->invMassB);
+	const float dv = dot(rel, n);
+	float acc = 0.0f;
+	for (int i = 0; i < k->c.count; i++)
+	{
+		const float ad0 = dot(am0, k->raXnI[i]), ad1 = dot(am1, k->rbXnI[i]);
+		const float dang = ad0 - ad1;
+		const float dbias = (dv + dang) - 0.0f * elapsed;
+		const float sep = fmaxf(minPen, k->sepErr[i] + dbias);
+		const float bias = fminf(-k->maxPenBias, k->biasCoef[i] * sep);
+		const v3 v0 = add(mulv(l0, n), mulv(a0, k->raXnI[i]));
+		const v3 v1 = add(mulv(l1, n), mulv(a1, k->rbXnI[i]));
+		const v3 dvv = sub(v0, v1);
+		const float nv = dvv.x + dvv.y + dvv.z;
+		const float biasNV = bias * k->recipResp[i];
+		const float lambda = biasNV - (nv - 0.0f) * k->velMul[i];
+		const float af = k->force[i];
+		const float df0 = fmaxf(lambda, -af);
+		const float nf = fminf(af + df0, 1e32f);
+		const float df = nf - af;
+		l0 = add(scl(nim0, df), l0);
+		l1 = sub(l1, scl(nim1, df));
+		a0 = add(scl(k->raXnI[i], df), a0);
+		a1 = sub(a1, scl(k->rbXnI[i], df));
+		k->force[i] = nf;
+		acc = acc + nf;
+	}
+	if (k->nFric)
+	{
+		const float maxF = mu_s * acc, maxD = mu_d * acc;
+		for (int i = 0; i < k->nFric; i += 2)
+		{
+			float ti[2], af[2];
+			for (int d = 0; d < 2; d++)
+			{
+				const int r = i + d;
+				float dvf = (dot(k->fraXnI[r], am0) - dot(k->frbXnI[r], am1)) + dot(k->fn[r], rel);
+				dvf = dvf - 0.0f * elapsed;
+				const float err = k->ferr[r] + dvf;
+				const float bias = err * k->fbiasScale;
+				const v3 v0 = add(mulv(l0, k->fn[r]), mulv(a0, k->fraXnI[r]));
+				const v3 v1 = add(mulv(l1, k->fn[r]), mulv(a1, k->frbXnI[r]));
+				const v3 dd = sub(v0, v1);
+				const float nv = dd.x + dd.y + dd.z;
+				af[d] = k->fforce[r];
+				const float tmp = af[d] - (bias - 0.0f) * k->fvelMul[r];
+				ti[d] = tmp - nv * k->fvelMul[r];
+			}
+			const float tot = sqrtf(ti[0] * ti[0] + ti[1] * ti[1]);
+			const int clamp = tot > k->fscale * maxF;
+			const float tc = clamp ? fminf(k->fscale * maxD, tot) : tot;
+			const float ratio = tot > 0.0f ? tc / tot : 0.0f;
+			const float n0 = ti[0] * ratio, n1 = ti[1] * ratio;
+			const float d0 = n0 - af[0], d1 = n1 - af[1];
+			const v3 t0 = k->fn[i], t1 = k->fn[i + 1];
+			l0 = add(scl(scl(t0, k->invMassA), d0), add(scl(scl(t1, k->invMassA), d1), l0));
+			l1 = sub(sub(l1, scl(scl(t1, k->invMassB), d1)), scl(scl(t0, k->invMassB), d0));
+			a0 = add(scl(k->fraXnI[i], d0), add(scl(k->fraXnI[i + 1], d1), a0));
+			a1 = sub(sub(a1, scl(k->frbXnI[i + 1], d1)), scl(k->frbXnI[i], d0));
+			k->fforce[i] = n0;
+			k->fforce[i + 1] = n1;
+		}
+	}
+	b0->lin = l0; b0->ang = a0;
+	if (b1->invMass > 0.0f) { b1->lin = l1; b1->ang = a1; }
+}
+
+static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
+{
+	/* shape order as the reference contact-method table: lower geometry type first */
+	int s0 = i, s1 = j;
+	if (w->type[j] < w->type[i]) { s0 = j; s1 = i; }
+	cset c;
+	c.count = 0;
+	int n = 0;
+	const float* g0 = w->dims + 4 * s0;
+	const float* g1 = w->dims + 4 * s1;
+	if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_SPHERE)
+		n = c_sphere_sphere(g0[0], w->pose[s0].p, g1[0], w->pose[s1].p, cd, &c);
+	else if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_BOX)
+		n = c_sphere_box(g0[0], w->pose[s0].p, V(g1[0], g1[1], g1[2]), w->pose[s1], cd, &c);
+	else
+		n = c_box_box(V(g0[0], g0[1], g0[2]), w->pose[s0], V(g1[0], g1[1], g1[2]), w->pose[s1], cd, tol, &c);
+	if (n > 0)
+		push_con(w, s0, s1, &c);
+}
+
+int pxo_world_step(void* wp, float dt)
+{
+	pxo_world* w = (pxo_world*)wp;
+	const pxo_desc* d = &w->d;
+	const int n = w->n;
+	const float co = d->contact_offset, cd = 2.0f * co, tol = d->tolerance_length;
+	memset(w->stats, 0, sizeof(w->stats));
+	w->ncon = 0;
+
+	/* 1. bounds + broadphase */
+	float* lo = (float*)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1));
+	float* hi = (float*)malloc(sizeof(float) * 3 * (size_t)(n ? n : 1));
+	for (int i = 0; i < n; i++)
+	{
+		v3 mn, mx;
+		shape_bounds(w->type[i], w->dims + 4 * i, w->pose[i], &mn, &mx);
+		lo[3 * i] = mn.x - co; lo[3 * i + 1] = mn.y - co; lo[3 * i + 2] = mn.z - co;
+		hi[3 * i] = mx.x + co; hi[3 * i + 1] = mx.y + co; hi[3 * i + 2] = mx.z + co;
+	}
+	pairvec pv = {0, 0, 0};
+	bp_core(lo, hi, w->scene, n, &pv);
+	free(lo);
+	free(hi);
+	w->stats[0] = pv.n;
+
+	/* 2. narrowphase: body pairs in sorted pair order, then body-plane pairs */
+	for (long long p = 0; p < pv.n; p++)
+		add_pair(w, (int)pv.p[2 * p], (int)pv.p[2 * p + 1], cd, tol);
+	free(pv.p);
+	for (int i = 0; i < n; i++)
+		for (int q = 0; q < w->m; q++)
+		{
+			if (w->pscene[q] != w->scene[i])
+				continue;
+			w->stats[1]++;
+			cset c;
+			c.count = 0;
+			int k;
+			if (w->type[i] == PXO_SPHERE)
+				k = c_sphere_plane(w->dims[4 * i], w->pose[i].p, w->pn[q], w->pd[q], cd, &c);
+			else
+			{
+				k = c_plane_box(w->pn[q], w->pd[q], V(w->dims[4 * i], w->dims[4 * i + 1], w->dims[4 * i + 2]),
+				                w->pose[i], cd, &c);
+				c.n = neg(c.n); /* box is body A: normal from the plane towards the box */
+			}
+			if (k > 0)
+				push_con(w, i, -1 - q, &c);
+		}
+	w->stats[2] = w->ncon;
+	for (long long c = 0; c < w->ncon; c++)
+		w->stats[3] += w->con[c].c.count;
+
+	/* 3. colouring: rounds of local minima of the ordering key at both dynamic bodies */
+	uint64_t* best = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
+	for (long long c = 0; c < w->ncon; c++)
+		w->con[c].color = -1;
+	long long left = w->ncon;
+	int round = 0;
+	while (left > 0)
+	{
+		for (int i = 0; i < n; i++)
+			best[i] = ~0ull;
+		for (long long c = 0; c < w->ncon; c++)
+		{
+			pxo_con* k = &w->con[c];
+			if (k->color >= 0) continue;
+			if (k->key < best[k->a]) best[k->a] = k->key;
+			if (k->b >= 0 && k->key < best[k->b]) best[k->b] = k->key;
+		}
+		for (long long c = 0; c < w->ncon; c++)
+		{
+			pxo_con* k = &w->con[c];
+			if (k->color >= 0) continue;
+			if (best[k->a] == k->key && (k->b < 0 || best[k->b] == k->key))
+			{
+				k->color = round;
+				left--;
+			}
+		}
+		round++;
+	}
+	free(best);
+	w->stats[4] = round;
+
+	/* 4. pre-integration (DyTGSDynamics.cpp:992-1021, DyBodyCoreIntegrator.h:40-75) */
+	sbody* sb = (sbody*)malloc(sizeof(sbody) * (size_t)(n ? n : 1));
+	xf* frame = (xf*)malloc(sizeof(xf) * (size_t)(n ? n : 1));
+	const v3 g = V(d->gravity[0], d->gravity[1], d->gravity[2]);
+	for (int i = 0; i < n; i++)
+	{
+		v3 lv = w->lv[i], av = w->av[i];
+		lv = add(lv, scl(scl(g, dt), 1.0f));
+		const float ld = 1.0f - d->linear_damping * dt, ad = 1.0f - d->angular_damping * dt;
+		lv = scl(lv, ld >= 0.0f ? ld : 0.0f);
+		av = scl(av, ad >= 0.0f ? ad : 0.0f);
+		const float l2 = dot(lv, lv);
+		if (l2 > d->max_lin_vel_sq) lv = scl(lv, sqrtf(d->max_lin_vel_sq / l2));
+		const float a2 = dot(av, av);
+		if (a2 > d->max_ang_vel_sq) av = scl(av, sqrtf(d->max_ang_vel_sq / a2));
+		const m33 R = qmat(w->pose[i].q);
+		const v3 si = V(sqrtf(w->invI[i].x), sqrtf(w->invI[i].y), sqrtf(w->invI[i].z));
+		const v3 sI = V(si.x == 0.0f ? 0.0f : 1.0f / si.x, si.y == 0.0f ? 0.0f : 1.0f / si.y, si.z == 0.0f ? 0.0f : 1.0f / si.z);
+		/* R diag(s) R^T (Cm::transformInertiaTensor) */
+		m33 A, B;
+		A.c0 = scl(R.c0, si.x); A.c1 = scl(R.c1, si.y); A.c2 = scl(R.c2, si.z);
+		B.c0 = scl(R.c0, sI.x); B.c1 = scl(R.c1, sI.y); B.c2 = scl(R.c2, sI.z);
+		const m33 Rt = mtrans(R);
+		sbody* b = &sb[i];
+		b->sqrtInvI = mmm(A, Rt);
+		const m33 sqrtI = mmm(B, Rt);
+		b->lin = lv;
+		b->ang = mmul(sqrtI, av);
+		b->dlin = V(0, 0, 0);
+		b->dang = V(0, 0, 0);
+		b->p = w->pose[i].p;
+		b->dq.x = 0; b->dq.y = 0; b->dq.z = 0; b->dq.w = 1;
+		b->origLin = lv;
+		b->origAng = av;
+		b->invMass = w->invMass[i];
+		frame[i] = w->pose[i];
+	}
+	sbody worldBody;
+	memset(&worldBody, 0, sizeof(worldBody));
+	worldBody.dq.w = 1.0f;
+
+	/* 5. contact prep (DyTGSDynamics.cpp:1898-1900 step dt and bias coefficient) */
+	const int P = d->pos_iters > 0 ? d->pos_iters : 1, Vi = d->vel_iters;
+	const float stepDt = dt / (float)P;
+	const float invStepDt = 1.0f / stepDt;
+	const float biasCoefficient = 2.0f * sqrtf(1.0f / (float)P);
+	const float scale = fminf(0.8f, biasCoefficient);
+	for (long long c = 0; c < w->ncon; c++)
+	{
+		prep_con(w, &w->con[c], sb, &worldBody, frame, invStepDt, stepDt, dt, scale);
+		w->stats[5] += w->con[c].nFric / 2;
+	}
+
+	/* 6. solve: position iterations by colour with substep integration, then
+	 * velocity iterations (DyTGSDynamics.cpp:2515-2790) */
+	long long* order = (long long*)malloc(sizeof(long long) * (size_t)(w->ncon ? w->ncon : 1));
+	long long* start = (long long*)calloc((size_t)round + 2, sizeof(long long));
+	for (long long c = 0; c < w->ncon; c++) start[w->con[c].color + 1]++;
+	for (int r = 0; r < round; r++) start[r + 1] += start[r];
+	{
+		long long* fill = (long long*)malloc(sizeof(long long) * ((size_t)round + 1));
+		memcpy(fill, start, sizeof(long long) * ((size_t)round + 1));
+		for (long long c = 0; c < w->ncon; c++) order[fill[w->con[c].color]++] = c;
+		free(fill);
+	}
+	float elapsed = 0.0f;
+	for (int it = 0; it < P; it++)
+	{
+		for (int r = 0; r < round; r++)
+			for (long long o = start[r]; o < start[r + 1]; o++)
+			{
+				pxo_con* k = &w->con[order[o]];
+				solve_con(k, &sb[k->a], k->b >= 0 ? &sb[k->b] : &worldBody, -3.402823466e+38f, elapsed,
+				          d->static_friction, d->dynamic_friction);
+			}
+		for (int i = 0; i < n; i++)
+			integrate_step(&sb[i], stepDt);
+		elapsed += stepDt;
+	}
+	for (int it = 0; it < Vi; it++)
+		for (int r = 0; r < round; r++)
+			for (long long o = start[r]; o < start[r + 1]; o++)
+			{
+				pxo_con* k = &w->con[order[o]];
+				solve_con(k, &sb[k->a], k->b >= 0 ? &sb[k->b] : &worldBody, 0.0f, elapsed, d->static_friction,
+				          d->dynamic_friction);
+			}
+	free(order);
+	free(start);
+
+	/* 7. copy back (DyTGSDynamics.cpp:1549-1580) */
+	for (int i = 0; i < n; i++)
+	{
+		const sbody* b = &sb[i];
+		w->pose[i].q = qnormalize(qmul(b->dq, w->pose[i].q));
+		w->pose[i].p = b->p;
+		w->lv[i] = b->lin;
+		w->av[i] = mmul(b->sqrtInvI, b->ang);
+	}
+	free(sb);
+	free(frame);
+	return 0;
+}

@@ -1,0 +1,77 @@
+/* CPU oracle for the rigid-body step: a plain-C restatement of the reference
+ * PhysX 5.6.1 CPU pipeline for the GPU rigid-body path (ABP broadphase
+ * predicate, PCM contact generation on a fresh cache, TGS contact prep/solve,
+ * body integration).
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and
+ * bench.py's CPU-baseline leg may load it, and only as the checker; the
+ * product library (physx_amd) never includes, links or calls it.
+ *
+ * Parity pinning (each function is checked in tests/ against the reference
+ * CPU SDK built from source in oracle/_ref, plus closed forms):
+ *   pxo_bp_pairs             pinned: bit-exact vs PxCreateBroadPhase(eABP)
+ *   pxo_generate_contacts    pinned: vs PxGenerateContacts (fresh PxCache)
+ *   pxo_world_* (TGS step)   pinned: one-step velocities on order-free scenes
+ *                            and long-run statistics vs the reference PxScene
+ *   box-box edge-edge case   partial: exact SAT/closest-feature restatement of
+ *                            the GJK/EPA fallback (GuPCMContactBoxBox.cpp:922-958)
+ */
+#ifndef PXO_H
+#define PXO_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { PXO_SPHERE = 0, PXO_PLANE = 1, PXO_CAPSULE = 2, PXO_BOX = 3 };
+
+typedef struct {
+	float gravity[3];
+	int   solver;             /* 1 = TGS (0 = PGS, not restated yet) */
+	int   pos_iters, vel_iters;
+	float contact_offset;     /* per shape; pair contact distance = sum */
+	float rest_offset;
+	float static_friction, dynamic_friction, restitution;
+	float linear_damping, angular_damping;
+	float max_lin_vel_sq, max_ang_vel_sq;
+	float bounce_threshold, friction_offset_threshold, correlation_distance;
+	float tolerance_length;
+} pxo_desc;
+
+void pxo_default_desc(pxo_desc* d);
+
+/* Contact generation for one pair, PhysX argument order (shape0, shape1):
+ * geometry codes as PxGeometryType, poses p xyz + q xyzw. out: 8 floats per
+ * contact (point xyz, separation, normal xyz, 0); normal points from shape1
+ * towards shape0. Returns the contact count. */
+int pxo_generate_contacts(int type0, const float* geom0, const float* pose0, int type1, const float* geom1,
+                          const float* pose1, float contact_distance, float tolerance_length, float* out);
+
+/* Inflated-AABB inclusive overlap pairs (ABP predicate) among n boxes, only
+ * between boxes of equal world id. bounds n x 6, dist n. Pairs (i<j),
+ * lexicographically sorted. Returns the count (may exceed max_pairs). */
+long long pxo_bp_pairs(const float* bounds, const float* dist, const int32_t* world, int n, uint32_t* out,
+                       long long max_pairs);
+
+/* World: bodies of possibly many independent scenes, stepped together. */
+void* pxo_world_create(const pxo_desc* d);
+void  pxo_world_release(void* w);
+int   pxo_world_add_bodies(void* w, int n, const int32_t* type, const float* dims, const float* pose,
+                           const float* lin_vel, const float* ang_vel, const float* mass, const float* inertia,
+                           const int32_t* scene);
+int   pxo_world_add_planes(void* w, int m, const float* planes, const int32_t* scene);
+int   pxo_world_step(void* w, float dt);
+void  pxo_world_get_bodies(void* w, float* pose, float* lin_vel, float* ang_vel);
+void  pxo_world_set_bodies(void* w, const float* pose, const float* lin_vel, const float* ang_vel);
+/* out: [0] BP pairs, [1] body-plane pairs tested, [2] touching pairs (constraints),
+ * [3] contact points, [4] colours, [5] friction anchors */
+void  pxo_world_stats(void* w, long long* out);
+/* last step's constraints: per constraint (bodyA, bodyB or -1-planeIndex,
+ * colour, nContacts); returns count */
+long long pxo_world_constraints(void* w, int32_t* out, long long max);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

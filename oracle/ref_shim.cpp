@@ -1,0 +1,316 @@
+// C-ABI harness over the reference PhysX 5.6.1 CPU SDK, built from the
+// reference sources by oracle/ref.mk into oracle/_ref/libpxref.so.
+//
+// Test infrastructure only: tests/ and bench.py's CPU-baseline leg load it as
+// the checker and the timed CPU baseline; the product library never links it.
+//
+// Entry points:
+//   pxref_bp_pairs          standalone ABP broadphase (PxCreateBroadPhase,
+//                           physx/include/PxBroadPhase.h:629) on given bounds
+//   pxref_generate_contacts immediate-mode PCM contact generation on a fresh
+//                           cache (PxGenerateContacts, physx/include/PxImmediateMode.h:267)
+//   pxref_scene_*           a CPU PxScene (ABP/PABP, PCM, PGS/TGS) built from
+//                           flat arrays, stepped with simulate()+fetchResults()
+#include "PxPhysicsAPI.h"
+#include "PxImmediateMode.h"
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+using namespace physx;
+using namespace physx::immediate;
+
+namespace {
+
+PxDefaultAllocator gAllocator;
+PxDefaultErrorCallback gErrorCallback;
+PxFoundation* gFoundation = nullptr;
+PxPhysics* gPhysics = nullptr;
+
+bool ensureSdk()
+{
+	if (gPhysics)
+		return true;
+	gFoundation = PxCreateFoundation(PX_PHYSICS_VERSION, gAllocator, gErrorCallback);
+	if (!gFoundation)
+		return false;
+	gPhysics = PxCreatePhysics(PX_PHYSICS_VERSION, *gFoundation, PxTolerancesScale(), false, nullptr);
+	return gPhysics != nullptr;
+}
+
+// geometry codes shared with the product C-ABI: 0 sphere, 1 plane, 2 capsule, 3 box
+PxGeometryHolder makeGeom(int type, const float* g)
+{
+	switch (type)
+	{
+	case 0: return PxGeometryHolder(PxSphereGeometry(g[0]));
+	case 1: return PxGeometryHolder(PxPlaneGeometry());
+	case 2: return PxGeometryHolder(PxCapsuleGeometry(g[0], g[1]));
+	default: return PxGeometryHolder(PxBoxGeometry(g[0], g[1], g[2]));
+	}
+}
+
+PxTransform makePose(const float* p)  // px py pz qx qy qz qw
+{
+	return PxTransform(PxVec3(p[0], p[1], p[2]), PxQuat(p[3], p[4], p[5], p[6]));
+}
+
+struct Recorder : PxContactRecorder
+{
+	float* out;
+	int maxContacts;
+	int count = 0;
+	bool recordContacts(const PxContactPoint* cp, PxU32 n, PxU32) override
+	{
+		for (PxU32 i = 0; i < n && count < maxContacts; i++, count++)
+		{
+			float* o = out + 8 * count;
+			o[0] = cp[i].point.x; o[1] = cp[i].point.y; o[2] = cp[i].point.z; o[3] = cp[i].separation;
+			o[4] = cp[i].normal.x; o[5] = cp[i].normal.y; o[6] = cp[i].normal.z; o[7] = 0.f;
+		}
+		return true;
+	}
+};
+
+struct CacheAlloc : PxCacheAllocator
+{
+	std::vector<std::vector<PxU8>> blocks;
+	PxU8* allocateCacheData(const PxU32 byteSize) override
+	{
+		blocks.emplace_back(byteSize + 16);
+		PxU8* p = blocks.back().data();
+		return reinterpret_cast<PxU8*>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+	}
+};
+
+struct RefScene
+{
+	PxDefaultCpuDispatcher* dispatcher = nullptr;
+	PxScene* scene = nullptr;
+	PxMaterial* material = nullptr;
+	std::vector<PxRigidDynamic*> bodies;
+};
+
+}  // namespace
+
+extern "C" {
+
+int pxref_version() { return PX_PHYSICS_VERSION; }
+
+// Standalone ABP over n boxes. bounds: n x (minx miny minz maxx maxy maxz);
+// dist: n inflation distances; groups: n PxBpFilterGroup values (use 0 for
+// statics, 2*i+... for dynamics, as PxGetBroadPhase*FilterGroup returns).
+// Writes up to maxPairs (id0,id1) pairs with id0<id1; returns the pair count
+// (which may exceed maxPairs) or -1 on failure.
+long long pxref_bp_pairs(const float* bounds, const float* dist, const uint32_t* groups, int n,
+                         uint32_t* outPairs, long long maxPairs, int usePabp)
+{
+	if (!ensureSdk())
+		return -1;
+	PxBroadPhaseDesc desc(usePabp ? PxBroadPhaseType::ePABP : PxBroadPhaseType::eABP);
+	PxBroadPhase* bp = PxCreateBroadPhase(desc);
+	if (!bp)
+		return -1;
+	// the API may read 4 bytes past the last bound: copy into a padded array
+	std::vector<PxBounds3> b(size_t(n) + 1);
+	for (int i = 0; i < n; i++)
+		b[i] = PxBounds3(PxVec3(bounds[6 * i], bounds[6 * i + 1], bounds[6 * i + 2]),
+		                 PxVec3(bounds[6 * i + 3], bounds[6 * i + 4], bounds[6 * i + 5]));
+	std::vector<PxBpIndex> created(n);
+	for (int i = 0; i < n; i++)
+		created[i] = PxBpIndex(i);
+	PxBroadPhaseUpdateData ud(created.data(), PxU32(n), nullptr, 0, nullptr, 0, b.data(),
+	                          reinterpret_cast<const PxBpFilterGroup*>(groups), dist, PxU32(n));
+	bp->update(ud);
+	PxBroadPhaseResults res;
+	bp->fetchResults(res);
+	long long count = res.mNbCreatedPairs;
+	for (long long i = 0; i < count && i < maxPairs; i++)
+	{
+		PxU32 a = res.mCreatedPairs[i].mID0, c = res.mCreatedPairs[i].mID1;
+		outPairs[2 * i] = a < c ? a : c;
+		outPairs[2 * i + 1] = a < c ? c : a;
+	}
+	bp->release();
+	return count;
+}
+
+uint32_t pxref_static_group() { return PxGetBroadPhaseStaticFilterGroup(); }
+uint32_t pxref_dynamic_group(uint32_t id) { return PxGetBroadPhaseDynamicFilterGroup(id); }
+
+// One pair through PxGenerateContacts with a fresh PxCache. geom: sphere (r),
+// capsule (r, halfHeight), box (hx hy hz), plane (unused). pose: p xyz + q xyzw.
+// out: per contact (point xyz, separation, normal xyz, 0). Returns the count.
+int pxref_generate_contacts(int type0, const float* geom0, const float* pose0, int type1, const float* geom1,
+                            const float* pose1, float contactDistance, float toleranceLength, float* out, int maxContacts)
+{
+	if (!ensureSdk())
+		return -1;
+	PxGeometryHolder g0 = makeGeom(type0, geom0), g1 = makeGeom(type1, geom1);
+	const PxGeometry* gp0 = &g0.any();
+	const PxGeometry* gp1 = &g1.any();
+	PxTransform p0 = makePose(pose0), p1 = makePose(pose1);
+	PxCache cache;
+	Recorder rec;
+	rec.out = out;
+	rec.maxContacts = maxContacts;
+	CacheAlloc alloc;
+	if (!PxGenerateContacts(&gp0, &gp1, &p0, &p1, &cache, 1, rec, contactDistance, 0.01f, toleranceLength, alloc))
+		return -1;
+	return rec.count;
+}
+
+// Scene: gravity xyz; solverType 0 PGS / 1 TGS; bpType 0 ABP / 1 PABP.
+void* pxref_scene_create(const float* gravity, int solverType, int bpType, int nThreads)
+{
+	if (!ensureSdk())
+		return nullptr;
+	RefScene* rs = new RefScene();
+	PxSceneDesc sd(gPhysics->getTolerancesScale());
+	sd.gravity = PxVec3(gravity[0], gravity[1], gravity[2]);
+	rs->dispatcher = PxDefaultCpuDispatcherCreate(PxU32(nThreads));
+	sd.cpuDispatcher = rs->dispatcher;
+	sd.filterShader = PxDefaultSimulationFilterShader;
+	sd.solverType = solverType ? PxSolverType::eTGS : PxSolverType::ePGS;
+	sd.broadPhaseType = bpType ? PxBroadPhaseType::ePABP : PxBroadPhaseType::eABP;
+	rs->scene = gPhysics->createScene(sd);
+	if (!rs->scene)
+	{
+		delete rs;
+		return nullptr;
+	}
+	rs->material = gPhysics->createMaterial(0.5f, 0.5f, 0.0f);
+	return rs;
+}
+
+int pxref_scene_set_material(void* h, float staticFriction, float dynamicFriction, float restitution)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	rs->material->setStaticFriction(staticFriction);
+	rs->material->setDynamicFriction(dynamicFriction);
+	rs->material->setRestitution(restitution);
+	return 0;
+}
+
+// Adds n dynamic bodies, one exclusive shape each. type: 0 sphere / 3 box;
+// dims: n x 4 (r or hx hy hz); pose: n x 7; linVel, angVel: n x 3; mass: n;
+// inertia: n x 3 (mass-space diagonal). Sleeping is disabled (threshold 0).
+int pxref_scene_add_bodies(void* h, int n, const int* type, const float* dims, const float* pose, const float* linVel,
+                           const float* angVel, const float* mass, const float* inertia, int posIters, int velIters,
+                           float contactOffset, float restOffset, float linearDamping, float angularDamping)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (int i = 0; i < n; i++)
+	{
+		PxRigidDynamic* body = gPhysics->createRigidDynamic(makePose(pose + 7 * i));
+		PxGeometryHolder g = makeGeom(type[i], dims + 4 * i);
+		PxShape* shape = PxRigidActorExt::createExclusiveShape(*body, g.any(), *rs->material);
+		shape->setContactOffset(contactOffset);
+		shape->setRestOffset(restOffset);
+		body->setMass(mass[i]);
+		body->setMassSpaceInertiaTensor(PxVec3(inertia[3 * i], inertia[3 * i + 1], inertia[3 * i + 2]));
+		body->setLinearVelocity(PxVec3(linVel[3 * i], linVel[3 * i + 1], linVel[3 * i + 2]));
+		body->setAngularVelocity(PxVec3(angVel[3 * i], angVel[3 * i + 1], angVel[3 * i + 2]));
+		body->setSolverIterationCounts(PxU32(posIters), PxU32(velIters));
+		body->setLinearDamping(linearDamping);
+		body->setAngularDamping(angularDamping);
+		body->setSleepThreshold(0.0f);
+		body->setStabilizationThreshold(0.0f);
+		rs->scene->addActor(*body);
+		rs->bodies.push_back(body);
+	}
+	return int(rs->bodies.size());
+}
+
+// Adds n static planes n.x + d = 0 (plane: n x 4 = nx ny nz d).
+int pxref_scene_add_planes(void* h, int n, const float* plane, float contactOffset)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (int i = 0; i < n; i++)
+	{
+		PxRigidStatic* s = PxCreatePlane(*gPhysics, PxPlane(plane[4 * i], plane[4 * i + 1], plane[4 * i + 2], plane[4 * i + 3]),
+		                                 *rs->material);
+		if (!s)
+			return -1;
+		PxShape* shape = nullptr;
+		s->getShapes(&shape, 1);
+		shape->setContactOffset(contactOffset);
+		shape->setRestOffset(0.0f);
+		rs->scene->addActor(*s);
+	}
+	return 0;
+}
+
+// Steps nsteps times; writes wall ms per step (simulate + fetchResults) into
+// stepMs (may be null). Returns total wall seconds.
+double pxref_scene_step(void* h, float dt, int nsteps, double* stepMs)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	auto t0 = std::chrono::steady_clock::now();
+	for (int s = 0; s < nsteps; s++)
+	{
+		auto a = std::chrono::steady_clock::now();
+		rs->scene->simulate(dt);
+		rs->scene->fetchResults(true);
+		auto b = std::chrono::steady_clock::now();
+		if (stepMs)
+			stepMs[s] = std::chrono::duration<double, std::milli>(b - a).count();
+	}
+	return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// pose: n x 7, linVel/angVel: n x 3 (any may be null)
+void pxref_scene_get_bodies(void* h, float* pose, float* linVel, float* angVel)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (size_t i = 0; i < rs->bodies.size(); i++)
+	{
+		PxRigidDynamic* b = rs->bodies[i];
+		if (pose)
+		{
+			PxTransform t = b->getGlobalPose();
+			float* o = pose + 7 * i;
+			o[0] = t.p.x; o[1] = t.p.y; o[2] = t.p.z; o[3] = t.q.x; o[4] = t.q.y; o[5] = t.q.z; o[6] = t.q.w;
+		}
+		if (linVel)
+		{
+			PxVec3 v = b->getLinearVelocity();
+			linVel[3 * i] = v.x; linVel[3 * i + 1] = v.y; linVel[3 * i + 2] = v.z;
+		}
+		if (angVel)
+		{
+			PxVec3 v = b->getAngularVelocity();
+			angVel[3 * i] = v.x; angVel[3 * i + 1] = v.y; angVel[3 * i + 2] = v.z;
+		}
+	}
+}
+
+// out[0] discrete contact pairs (all geometry pairs), out[1] BP pairs created
+// in the last step, out[2] active dynamic bodies, out[3] nb patches
+void pxref_scene_stats(void* h, long long* out)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	PxSimulationStatistics st;
+	rs->scene->getSimulationStatistics(st);
+	out[0] = st.nbDiscreteContactPairsTotal;
+	out[1] = st.getNbBroadPhaseAdds();
+	out[2] = st.nbActiveDynamicBodies;
+	out[3] = st.nbDiscreteContactPairsWithContacts;
+}
+
+void pxref_scene_release(void* h)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	if (!rs)
+		return;
+	if (rs->scene)
+		rs->scene->release();
+	if (rs->dispatcher)
+		rs->dispatcher->release();
+	if (rs->material)
+		rs->material->release();
+	delete rs;
+}
+
+}  // extern "C"

@@ -1,0 +1,281 @@
+"""physx_amd: MI355X (gfx950) GPU rigid-body step for PhysX scenes.
+
+Python mirror of the reference's scene-level interface for the GPU path
+(PxScene::simulate/fetchResults, PxDirectGPUAPI, PxCreateBroadPhase,
+PxGenerateContacts) over the C ABI of include/pxhip.h, implemented by the
+HIP library physx_amd/libpxhip.so. There is no CPU fallback: loading fails
+loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpxhip.so")
+
+PXH_GEOM_SPHERE, PXH_GEOM_PLANE, PXH_GEOM_CAPSULE, PXH_GEOM_BOX = 0, 1, 2, 3
+PXH_SOLVER_PGS, PXH_SOLVER_TGS = 0, 1
+PXH_DATA_GLOBAL_POSE, PXH_DATA_LINEAR_VELOCITY, PXH_DATA_ANGULAR_VELOCITY = 0, 1, 2
+
+_F = ctypes.POINTER(ctypes.c_float)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+
+class PxhError(RuntimeError):
+    pass
+
+
+class SceneDesc(ctypes.Structure):
+    """pxh_scene_desc (include/pxhip.h), PxSceneDesc/PxRigidDynamic defaults."""
+    _fields_ = [("gravity", ctypes.c_float * 3), ("solver_type", ctypes.c_int32),
+                ("position_iterations", ctypes.c_int32), ("velocity_iterations", ctypes.c_int32),
+                ("contact_offset", ctypes.c_float), ("rest_offset", ctypes.c_float),
+                ("static_friction", ctypes.c_float), ("dynamic_friction", ctypes.c_float),
+                ("restitution", ctypes.c_float), ("linear_damping", ctypes.c_float),
+                ("angular_damping", ctypes.c_float), ("max_linear_velocity_sq", ctypes.c_float),
+                ("max_angular_velocity_sq", ctypes.c_float), ("bounce_threshold_velocity", ctypes.c_float),
+                ("friction_offset_threshold", ctypes.c_float), ("friction_correlation_distance", ctypes.c_float),
+                ("tolerance_length", ctypes.c_float), ("max_pairs_per_body", ctypes.c_int32)]
+
+
+EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_scene_create", "pxh_scene_release",
+            "pxh_scene_add_bodies", "pxh_scene_add_planes", "pxh_scene_simulate", "pxh_scene_fetch_results",
+            "pxh_scene_get_rigid_dynamic_data", "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats",
+            "pxh_scene_get_constraints", "pxh_scene_get_stream", "pxh_bp_create", "pxh_bp_update",
+            "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts")
+
+_lib = None
+
+
+def load():
+    """Load libpxhip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    try:
+        # share the process's HIP runtime with torch (same SONAME, loaded first)
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    if not os.path.exists(LIB_PATH):
+        raise PxhError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                       "(make -C physx_amd/csrc)")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.pxh_last_error.restype = ctypes.c_char_p
+    lib.pxh_scene_get_stream.restype = _VP
+    lib.pxh_scene_get_constraints.restype = ctypes.c_int64
+    for name in ("pxh_scene_create", "pxh_scene_release", "pxh_scene_add_bodies", "pxh_scene_add_planes",
+                 "pxh_scene_simulate", "pxh_scene_fetch_results", "pxh_scene_get_rigid_dynamic_data",
+                 "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats", "pxh_bp_create", "pxh_bp_update",
+                 "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts"):
+        getattr(lib, name).restype = ctypes.c_int32
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise PxhError(f"pxhip error {rc}: {load().pxh_last_error().decode()}")
+    return rc
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def default_desc(**kw) -> SceneDesc:
+    d = SceneDesc()
+    load().pxh_default_scene_desc(ctypes.byref(d))
+    for k, v in kw.items():
+        if k == "gravity":
+            d.gravity[0], d.gravity[1], d.gravity[2] = v
+        else:
+            setattr(d, k, v)
+    return d
+
+
+def _dptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Scene:
+    """A GPU scene (one or more independent worlds) stepped on the device.
+
+    Mirrors PxScene for this path: simulate(dt) / fetch_results(), and the
+    Direct GPU API body-state getters/setters on device buffers."""
+
+    def __init__(self, scene=None, desc: SceneDesc | None = None, **kw):
+        self.lib = load()
+        self.desc = desc if desc is not None else default_desc(**kw)
+        h = ctypes.c_void_p()
+        _check(self.lib.pxh_scene_create(ctypes.byref(self.desc), ctypes.byref(h)))
+        self.h = h
+        self.n = 0
+        if scene is not None:
+            self.add(scene)
+
+    def add(self, scene):
+        t, dm, po = _i32(scene.shape_type), _f32(scene.dims), _f32(scene.pose)
+        lv, av, m, inn = _f32(scene.lin_vel), _f32(scene.ang_vel), _f32(scene.mass), _f32(scene.inertia)
+        w = _i32(scene.scene_id)
+        fp = lambda a: a.ctypes.data_as(_F)  # noqa: E731
+        ip = lambda a: a.ctypes.data_as(_I32)  # noqa: E731
+        _check(self.lib.pxh_scene_add_bodies(self.h, scene.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn),
+                                             ip(w)))
+        pl, ps = _f32(scene.planes), _i32(scene.plane_scene)
+        _check(self.lib.pxh_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps)))
+        self.n += scene.n
+
+    def simulate(self, dt: float):
+        _check(self.lib.pxh_scene_simulate(self.h, ctypes.c_float(dt)))
+
+    def fetch_results(self, block: bool = True) -> bool:
+        return _check(self.lib.pxh_scene_fetch_results(self.h, int(block))) == 0
+
+    def step(self, dt: float, n: int = 1):
+        for _ in range(n):
+            self.simulate(dt)
+            self.fetch_results(True)
+
+    @property
+    def stream(self) -> int:
+        return self.lib.pxh_scene_get_stream(self.h) or 0
+
+    def get_rigid_dynamic_data(self, out, kind: int, index=None, n: int | None = None, stream: int = 0):
+        """PxDirectGPUAPI::getRigidDynamicData into a device tensor `out`."""
+        n = self.n if n is None else n
+        _check(self.lib.pxh_scene_get_rigid_dynamic_data(self.h, _dptr(out), _dptr(index) if index is not None else None,
+                                                         kind, n, ctypes.c_void_p(stream) if stream else None))
+
+    def set_rigid_dynamic_data(self, data, kind: int, index=None, n: int | None = None, stream: int = 0):
+        n = self.n if n is None else n
+        _check(self.lib.pxh_scene_set_rigid_dynamic_data(self.h, _dptr(data), _dptr(index) if index is not None else None,
+                                                         kind, n, ctypes.c_void_p(stream) if stream else None))
+
+    def bodies(self):
+        """Host copies (pose p xyz + q xyzw, linear velocity, angular velocity)."""
+        import torch
+        dev = torch.device("cuda")
+        pose = torch.empty((self.n, 7), dtype=torch.float32, device=dev)
+        lv = torch.empty((self.n, 3), dtype=torch.float32, device=dev)
+        av = torch.empty((self.n, 3), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        self.get_rigid_dynamic_data(pose, PXH_DATA_GLOBAL_POSE)
+        self.get_rigid_dynamic_data(lv, PXH_DATA_LINEAR_VELOCITY)
+        self.get_rigid_dynamic_data(av, PXH_DATA_ANGULAR_VELOCITY)
+        p = pose.cpu().numpy()
+        # PxTransform is (q, p): reorder to p xyz + q xyzw
+        return np.concatenate([p[:, 4:7], p[:, 0:4]], 1), lv.cpu().numpy(), av.cpu().numpy()
+
+    def set_bodies(self, pose, lin_vel, ang_vel):
+        import torch
+        dev = torch.device("cuda")
+        pose = _f32(pose)
+        tr = np.concatenate([pose[:, 3:7], pose[:, 0:3]], 1)
+        self.set_rigid_dynamic_data(torch.from_numpy(_f32(tr)).to(dev), PXH_DATA_GLOBAL_POSE)
+        self.set_rigid_dynamic_data(torch.from_numpy(_f32(lin_vel)).to(dev), PXH_DATA_LINEAR_VELOCITY)
+        self.set_rigid_dynamic_data(torch.from_numpy(_f32(ang_vel)).to(dev), PXH_DATA_ANGULAR_VELOCITY)
+
+    def stats(self) -> dict:
+        out = np.zeros(7, np.int64)
+        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 7))
+        keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps")
+        return dict(zip(keys, map(int, out)))
+
+    def constraints(self) -> np.ndarray:
+        cap = 1 << 23
+        out = np.zeros((cap, 4), np.int32)
+        n = self.lib.pxh_scene_get_constraints(self.h, out.ctypes.data_as(_I32), ctypes.c_int64(cap))
+        _check(int(n))
+        return out[: min(n, cap)].copy()
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pxh_scene_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class BroadPhase:
+    """Standalone GPU broadphase (PxCreateBroadPhase / PxBroadPhase)."""
+
+    def __init__(self, capacity: int = 0):
+        self.lib = load()
+        h = ctypes.c_void_p()
+        _check(self.lib.pxh_bp_create(capacity, ctypes.byref(h)))
+        self.h = h
+
+    def update(self, bounds, distances=None, groups=None, stream: int = 0):
+        """bounds: (n,6) float32 device tensor; distances (n,), groups (n,) uint32/int32 device tensors."""
+        n = int(bounds.shape[0])
+        _check(self.lib.pxh_bp_update(self.h, _dptr(bounds), _dptr(groups) if groups is not None else None,
+                                      _dptr(distances) if distances is not None else None, n,
+                                      ctypes.c_void_p(stream) if stream else None))
+
+    def fetch_results(self):
+        """(created, deleted) as (k,2) uint32 numpy arrays, each sorted."""
+        import torch  # noqa: F401
+        cp, dp = ctypes.c_void_p(), ctypes.c_void_p()
+        nc, nd = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.pxh_bp_fetch_results(self.h, ctypes.byref(cp), ctypes.byref(nc), ctypes.byref(dp),
+                                             ctypes.byref(nd)))
+        return _copy_pairs(cp.value, nc.value), _copy_pairs(dp.value, nd.value)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pxh_bp_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _copy_pairs(ptr, n):
+    out = np.zeros((n, 2), np.uint32)
+    if n > 0:
+        import torch
+        torch.cuda.synchronize()
+        hip = ctypes.CDLL("libamdhip64.so.7")  # the runtime already in the process
+        hip.hipMemcpy.restype = ctypes.c_int
+        rc = hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(ptr), ctypes.c_size_t(out.nbytes), 2)
+        if rc != 0:
+            raise PxhError(f"hipMemcpy failed ({rc})")
+    return out
+
+
+def generate_contacts(type0, geom0, pose0, type1, geom1, pose1, contact_distance=0.04, tolerance_length=1.0):
+    """Batched PxGenerateContacts on the device. Host numpy inputs; returns
+    (counts (n,), contacts (n,4,8)) as numpy arrays."""
+    import torch
+    dev = torch.device("cuda")
+    n = len(type0)
+    t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)  # noqa: E731
+    T0, T1 = t(type0, np.int32), t(type1, np.int32)
+    G0, G1 = t(np.asarray(geom0).reshape(n, 4), np.float32), t(np.asarray(geom1).reshape(n, 4), np.float32)
+    P0, P1 = t(np.asarray(pose0).reshape(n, 7), np.float32), t(np.asarray(pose1).reshape(n, 7), np.float32)
+    out = torch.zeros((n, 4, 8), dtype=torch.float32, device=dev)
+    cnt = torch.zeros(n, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+    _check(load().pxh_generate_contacts(n, _dptr(T0), _dptr(G0), _dptr(P0), _dptr(T1), _dptr(G1), _dptr(P1),
+                                        ctypes.c_float(contact_distance), ctypes.c_float(tolerance_length),
+                                        _dptr(out), _dptr(cnt), None))
+    return cnt.cpu().numpy(), out.cpu().numpy()

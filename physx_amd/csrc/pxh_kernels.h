@@ -1,0 +1,92 @@
+// Launchers of the rigid-body step kernels (pxh_kernels.hip) and the device
+// layouts they share with the host scene (pxh_scene.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pxh {
+
+// TGS solver body state, one entry per dynamic body
+// (PxTGSSolverBodyVel + PxTGSSolverBodyTxInertia, physx/include/solver/PxSolverDefs.h)
+struct SolverBodies
+{
+	float4* lin;      // linear velocity, w = inverse mass
+	float4* ang;      // angular velocity in sqrt-inertia space
+	float4* dlin;     // accumulated linear motion this step
+	float4* dang;     // accumulated angular motion (sqrt-inertia space)
+	float4* p;        // body position during the step
+	float4* dq;       // rotation accumulated this step
+	float4* i0;       // world sqrt inverse inertia (xx yy zz xy)
+	float4* i1;       // (xz yz - -)
+	float4* origLin;  // pre-integrated linear velocity (friction directions)
+};
+
+// Contact rows in colour order; slot k of constraint o is at [k * C + o]
+struct Rows
+{
+	int4* hI;     // bodyA, bodyB (<0: static), nContacts | nFrictionRows << 8
+	float4* hF;   // normal xyz, inverse mass A
+	float4* hG;   // inverse mass B, friction scale, friction bias scale, max penetration bias
+	float4* nA;   // [4] raXnI xyz, separation
+	float4* nB;   // [4] rbXnI xyz, bias coefficient
+	float4* nC;   // [4] velocity multiplier, recip response, applied impulse
+	float4* fT;   // [4] tangent xyz, initial error
+	float4* fA;   // [4] raXnI xyz, velocity multiplier
+	float4* fB;   // [4] rbXnI xyz, applied impulse
+};
+
+int64_t scan_tmp_size(int64_t n);
+void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st);
+void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2, int n, int bitBegin,
+                       int bitEnd, uint32_t* hist, uint32_t* scanTmp, hipStream_t st, bool& resultInAlt);
+
+void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
+                   float4* bmax, hipStream_t st);
+void launch_scene_grid(int nScenes, const int* bodyStart, const float4* bmin, const float4* bmax, float4* gridF,
+                       int2* gridI, uint32_t* rows, hipStream_t st);
+void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
+                 const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_gather_sorted(int n, const uint32_t* sv, const float4* bmin, const float4* bmax, float4* smin,
+                          float4* smax, hipStream_t st);
+void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t totalRows, hipStream_t st);
+void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
+                  const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
+                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, hipStream_t st);
+void launch_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom, float cd,
+                     float tol, float4* ctN, float4* ctP, hipStream_t st);
+void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
+                      const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);
+void launch_touch_flags(int S, const float4* ctN, uint32_t* flags, hipStream_t st);
+void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs, const int* qBody,
+                      const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
+                      uint64_t* conKey, int* conColor, hipStream_t st);
+void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
+                        int* conColor, uint64_t* best, uint32_t* colored, hipStream_t st);
+void launch_color_count(int C, const int* conColor, uint32_t* colorCount, hipStream_t st);
+void launch_color_order(int C, const int* conColor, uint32_t* colorFill, int* order, hipStream_t st);
+void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
+                         const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
+                         float maxAngSq, SolverBodies sb, hipStream_t st);
+void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
+                 const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
+                 float scale, float restDist, float fot, Rows rows, hipStream_t st);
+void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
+                  hipStream_t st);
+void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
+void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
+void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
+                     const float4* ang, float* out, hipStream_t st);
+void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
+                     const float* in, hipStream_t st);
+void launch_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_set_diff(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag, hipStream_t st);
+void launch_compact_keys(int na, const uint64_t* a, const uint32_t* flag, const uint32_t* scanned, uint2* out,
+                         hipStream_t st);
+
+void launch_np_immediate(int n, const int* type0, const float* geom0, const float* pose0, const int* type1,
+                         const float* geom1, const float* pose1, float cd, float tol, float* out, int* counts,
+                         hipStream_t st);
+void launch_bp_inflate(int n, const float* bounds, const float* dist, const uint32_t* groups, float4* bmin,
+                       float4* bmax, int* scene, int* group, int* bodyStart, hipStream_t st);
+
+}  // namespace pxh

@@ -1,0 +1,1263 @@
+// HIP kernels of the rigid-body step for gfx950 (wave64). One lane per body,
+// pair or constraint; all arrays are structure-of-arrays in HBM so a wave's
+// loads are coalesced float4 rows.
+//
+// Stage map (reference stage -> kernels here):
+//   shape bounds (updateTransformCacheAndBoundArrayLaunch)  k_bounds
+//   broadphase (gpubroadphase/src/CUDA/broadphase.cu)        k_scene_grid, k_keys, radix sort, k_row_start,
+//                                                            k_gather_sorted, k_sweep<count|write>
+//   narrowphase (cudaSphere.cu, cudaBox.cu, cudaGJKEPA.cu)  k_np_pairs, k_np_planes
+//   partitioning (PxgIncrementalPartition, host in ref)     k_con_build, k_color_min, k_color_pick, k_color_order
+//   pre-integration (preIntegrationTGS.cu)                  k_preintegrate
+//   contact prep (constraintBlockPrepTGS.cu)                k_prep
+//   solve (solverMultiBlockTGS.cu solveBlockUnified)        k_solve
+//   integration (integrationTGS.cu)                         k_integrate, k_finalize
+#include "pxh_kernels.h"
+#include "pxh_np.h"
+
+namespace pxh {
+
+// ---------------------------------------------------------------------------
+// wave / block scans
+// ---------------------------------------------------------------------------
+PXH_D uint32_t waveInclScan(uint32_t v)
+{
+	const int lane = threadIdx.x & 63;
+#pragma unroll
+	for (int o = 1; o < 64; o <<= 1)
+	{
+		const uint32_t t = __shfl_up(v, o, 64);
+		if (lane >= o)
+			v += t;
+	}
+	return v;
+}
+
+// exclusive scan over the block (blockDim.x <= 1024, multiple of 64); returns
+// the exclusive prefix, writes the block total
+PXH_D uint32_t blockExclScan(uint32_t v, uint32_t* sh, uint32_t& total)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+	const uint32_t inc = waveInclScan(v);
+	if (lane == 63)
+		sh[wave] = inc;
+	__syncthreads();
+	if (threadIdx.x < 64)
+	{
+		const uint32_t w = threadIdx.x < nw ? sh[threadIdx.x] : 0u;
+		const uint32_t wi = waveInclScan(w);
+		if (threadIdx.x < nw)
+			sh[threadIdx.x] = wi - w;
+		if (threadIdx.x == nw - 1)
+			sh[32] = wi;
+	}
+	__syncthreads();
+	const uint32_t r = sh[wave] + inc - v;
+	total = sh[32];
+	__syncthreads();
+	return r;
+}
+
+constexpr int SCAN_THREADS = 256, SCAN_ITEMS = 8, SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+// in-place exclusive scan of each 2048-element tile; tile sums to blockSums
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_tiles(uint32_t* data, int64_t n, uint32_t* blockSums)
+{
+	__shared__ uint32_t sh[33];
+	__shared__ uint32_t tile[SCAN_TILE];
+	const int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
+	for (int k = 0; k < SCAN_ITEMS; k++)
+	{
+		const int64_t g = base + k * SCAN_THREADS + threadIdx.x;
+		tile[k * SCAN_THREADS + threadIdx.x] = g < n ? data[g] : 0u;
+	}
+	__syncthreads();
+	uint32_t loc[SCAN_ITEMS];
+	uint32_t sum = 0;
+	for (int k = 0; k < SCAN_ITEMS; k++)
+	{
+		loc[k] = tile[threadIdx.x * SCAN_ITEMS + k];
+		sum += loc[k];
+	}
+	uint32_t total;
+	uint32_t pre = blockExclScan(sum, sh, total);
+	for (int k = 0; k < SCAN_ITEMS; k++)
+	{
+		tile[threadIdx.x * SCAN_ITEMS + k] = pre;
+		pre += loc[k];
+	}
+	__syncthreads();
+	for (int k = 0; k < SCAN_ITEMS; k++)
+	{
+		const int64_t g = base + k * SCAN_THREADS + threadIdx.x;
+		if (g < n)
+			data[g] = tile[k * SCAN_THREADS + threadIdx.x];
+	}
+	if (threadIdx.x == 0 && blockSums)
+		blockSums[blockIdx.x] = total;
+}
+
+__global__ void k_scan_add(uint32_t* data, int64_t n, const uint32_t* blockOffsets)
+{
+	const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (g < n)
+		data[g] += blockOffsets[g / SCAN_TILE];
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort of (u64 key, u32 value), 8-bit digits, stable
+// ---------------------------------------------------------------------------
+constexpr int RS_THREADS = 256, RS_ITEMS = 8, RS_TILE = RS_THREADS * RS_ITEMS;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* keys, int n, int shift, uint32_t* hist, int nblocks)
+{
+	__shared__ uint32_t cnt[256];
+	cnt[threadIdx.x] = 0;
+	__syncthreads();
+	const int base = blockIdx.x * RS_TILE;
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		const int g = base + k * RS_THREADS + threadIdx.x;
+		if (g < n)
+			atomicAdd(&cnt[uint32_t(keys[g] >> shift) & 0xFFu], 1u);
+	}
+	__syncthreads();
+	hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* kin, const uint32_t* vin, uint64_t* kout,
+                                                            uint32_t* vout, int n, int shift, const uint32_t* hist,
+                                                            int nblocks)
+{
+	__shared__ uint64_t sk[RS_TILE];
+	__shared__ uint32_t sv[RS_TILE];
+	__shared__ uint32_t sh[33];
+	__shared__ uint32_t dstart[256];
+	const int base = blockIdx.x * RS_TILE;
+	const int nvalid = min(RS_TILE, n - base);
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		const int l = k * RS_THREADS + threadIdx.x;
+		const int g = base + l;
+		sk[l] = g < n ? kin[g] : ~0ull;
+		sv[l] = g < n ? vin[g] : 0u;
+	}
+	__syncthreads();
+	// stable split of the tile by each of the 8 bits, least significant first
+	uint64_t k8[RS_ITEMS];
+	uint32_t v8[RS_ITEMS];
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		k8[k] = sk[threadIdx.x * RS_ITEMS + k];
+		v8[k] = sv[threadIdx.x * RS_ITEMS + k];
+	}
+	for (int b = 0; b < 8; b++)
+	{
+		const int bit = shift + b;
+		uint32_t zeros = 0;
+		for (int k = 0; k < RS_ITEMS; k++)
+			zeros += ((k8[k] >> bit) & 1ull) ? 0u : 1u;
+		uint32_t totalZeros;
+		const uint32_t zb = blockExclScan(zeros, sh, totalZeros);
+		uint32_t z = zb, o = totalZeros + threadIdx.x * RS_ITEMS - zb;
+		__syncthreads();
+		for (int k = 0; k < RS_ITEMS; k++)
+		{
+			const bool one = (k8[k] >> bit) & 1ull;
+			const uint32_t dst = one ? o++ : z++;
+			sk[dst] = k8[k];
+			sv[dst] = v8[k];
+		}
+		__syncthreads();
+		for (int k = 0; k < RS_ITEMS; k++)
+		{
+			k8[k] = sk[threadIdx.x * RS_ITEMS + k];
+			v8[k] = sv[threadIdx.x * RS_ITEMS + k];
+		}
+		__syncthreads();
+	}
+	// digit starts inside the sorted tile
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		const int p = threadIdx.x * RS_ITEMS + k;
+		const uint32_t d = uint32_t(k8[k] >> shift) & 0xFFu;
+		const uint32_t dp = p == 0 ? 0xFFFFFFFFu : (uint32_t(sk[p - 1] >> shift) & 0xFFu);
+		if (p < nvalid && d != dp)
+			dstart[d] = p;
+	}
+	__syncthreads();
+	for (int k = 0; k < RS_ITEMS; k++)
+	{
+		const int p = k * RS_THREADS + threadIdx.x;
+		if (p < nvalid)
+		{
+			const uint64_t key = sk[p];
+			const uint32_t d = uint32_t(key >> shift) & 0xFFu;
+			const uint32_t dst = hist[d * nblocks + blockIdx.x] + (p - dstart[d]);
+			kout[dst] = key;
+			vout[dst] = sv[p];
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// bounds and broadphase
+// ---------------------------------------------------------------------------
+
+// Gu::computeBounds (geomutils/src/GuBounds.cpp:354-395) plus the broadphase's
+// contact-distance inflation (lowlevelaabb/src/BpBroadPhaseABP.cpp:1187-1192)
+__global__ void k_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
+                         float4* bmax)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float4 g = geom[i];
+	const f3 p = mk(pos[i]);
+	f3 w;
+	if (int(g.w) == GEOM_SPHERE)
+		w = mk(g.x, g.x, g.x);
+	else
+	{
+		const m33 R = qmat(mkq(rot[i]));
+		const f3 c0 = R.c0 * g.x, c1 = R.c1 * g.y, c2 = R.c2 * g.z;
+		w = mk(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y),
+		       fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z));
+	}
+	const f3 mn = p - w, mx = p + w;
+	bmin[i] = make_float4(mn.x - co, mn.y - co, mn.z - co, g.w);
+	bmax[i] = make_float4(mx.x + co, mx.y + co, mx.z + co, 0.0f);
+}
+
+// Per-world row grid for the sweep: rows are (y,z) cells no smaller than the
+// widest inflated box, so overlapping boxes sit in the same or adjacent rows.
+__global__ void __launch_bounds__(256) k_scene_grid(int nScenes, const int* bodyStart, const float4* bmin,
+                                                     const float4* bmax, float4* gridF, int2* gridI, uint32_t* rows)
+{
+	const int s = blockIdx.x;
+	if (s >= nScenes)
+		return;
+	__shared__ float red[6][256];
+	float lo_y = FLT_BIG, lo_z = FLT_BIG, hi_y = -FLT_BIG, hi_z = -FLT_BIG, wmax = 0.0f;
+	for (int i = bodyStart[s] + threadIdx.x; i < bodyStart[s + 1]; i += blockDim.x)
+	{
+		const float4 a = bmin[i], b = bmax[i];
+		const float cy = 0.5f * (a.y + b.y), cz = 0.5f * (a.z + b.z);
+		lo_y = fminf(lo_y, cy); hi_y = fmaxf(hi_y, cy);
+		lo_z = fminf(lo_z, cz); hi_z = fmaxf(hi_z, cz);
+		wmax = fmaxf(wmax, fmaxf(b.y - a.y, b.z - a.z));
+	}
+	red[0][threadIdx.x] = lo_y; red[1][threadIdx.x] = hi_y;
+	red[2][threadIdx.x] = lo_z; red[3][threadIdx.x] = hi_z;
+	red[4][threadIdx.x] = wmax;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1)
+	{
+		if (threadIdx.x < o)
+		{
+			red[0][threadIdx.x] = fminf(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+			red[1][threadIdx.x] = fmaxf(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+			red[2][threadIdx.x] = fminf(red[2][threadIdx.x], red[2][threadIdx.x + o]);
+			red[3][threadIdx.x] = fmaxf(red[3][threadIdx.x], red[3][threadIdx.x + o]);
+			red[4][threadIdx.x] = fmaxf(red[4][threadIdx.x], red[4][threadIdx.x + o]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		const bool empty = bodyStart[s + 1] == bodyStart[s];
+		const float ly = empty ? 0.0f : red[0][0], hy = empty ? 0.0f : red[1][0];
+		const float lz = empty ? 0.0f : red[2][0], hz = empty ? 0.0f : red[3][0];
+		float cell = red[4][0] * 1.01f + 1e-6f;
+		cell = fmaxf(cell, fmaxf((hy - ly), (hz - lz)) * (1.0f / 1000.0f));
+		int ny = int((hy - ly) / cell) + 1, nz = int((hz - lz) / cell) + 1;
+		ny = max(1, min(ny, 1024));
+		nz = max(1, min(nz, 1024));
+		gridF[s] = make_float4(ly, lz, 1.0f / cell, cell);
+		gridI[s] = make_int2(ny, nz);
+		rows[s] = uint32_t(ny * nz);
+	}
+}
+
+__global__ void k_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
+                       const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float4 a = bmin[i], b = bmax[i];
+	const int s = scene[i];
+	const float4 gf = gridF[s];
+	const int2 gi = gridI[s];
+	const float cy = 0.5f * (a.y + b.y), cz = 0.5f * (a.z + b.z);
+	const int iy = min(max(int(floorf((cy - gf.x) * gf.z)), 0), gi.x - 1);
+	const int iz = min(max(int(floorf((cz - gf.y) * gf.z)), 0), gi.y - 1);
+	const uint32_t row = rowOff[s] + uint32_t(iz * gi.x + iy);
+	keys[i] = (uint64_t(row) << 32) | encode_float(a.x);
+	vals[i] = uint32_t(i);
+}
+
+__global__ void k_gather_sorted(int n, const uint32_t* sortedVals, const float4* bmin, const float4* bmax,
+                                float4* smin, float4* smax)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+	const uint32_t i = sortedVals[s];
+	smin[s] = bmin[i];
+	float4 b = bmax[i];
+	b.w = __uint_as_float(i);
+	smax[s] = b;
+}
+
+// rowStart[r] = first sorted position of row r; rowStart[totalRows] = n
+__global__ void k_row_start(int n, const uint64_t* sortedKeys, uint32_t* rowStart, uint32_t totalRows)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+	const uint32_t r = uint32_t(sortedKeys[s] >> 32);
+	const int64_t rp = s == 0 ? -1 : int64_t(sortedKeys[s - 1] >> 32);
+	for (int64_t rr = rp + 1; rr <= int64_t(r); rr++)
+		rowStart[rr] = uint32_t(s);
+	if (s == n - 1)
+		for (uint32_t rr = r + 1; rr <= totalRows; rr++)
+			rowStart[rr] = uint32_t(n);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
+{
+	const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+// pair class from the two geometry types: 0 sphere-sphere, 1 sphere-box, 2 box-box
+PXH_D int pairClass(int t0, int t1) { return (t0 == GEOM_BOX) + (t1 == GEOM_BOX); }
+
+// Sweep of the sorted rows: lane s (body i) reports each overlapping body j in
+// its own and the 8 neighbouring rows whose (minX, index) sorts after its own,
+// so every pair is found exactly once. The overlap test is the reference ABP's
+// inclusive test on inflated bounds (BpBroadPhaseABP.cpp:2246-2262, :2474).
+// WRITE=false counts pairs per class into counts[cls*n + s]; WRITE=true writes
+// them at the scanned offsets.
+template <bool WRITE>
+__global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, const float4* smax, const int* scene,
+                        const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
+                        uint32_t* counts, const uint32_t* offsets, uint2* pairs)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= n)
+		return;
+	const uint64_t key = sortedKeys[s];
+	const float4 a0 = smin[s], a1 = smax[s];
+	const uint32_t i = __float_as_uint(a1.w);
+	const int ti = int(a0.w);
+	const int sc = scene[i];
+	const int gi_ = group ? group[i] : -1;
+	const int2 gi = gridI[sc];
+	const uint32_t r = uint32_t(key >> 32);
+	const uint32_t ro = rowOff[sc];
+	const int lr = int(r - ro);
+	const int cy = lr % gi.x, cz = lr / gi.x;
+	uint32_t cnt[3] = {0u, 0u, 0u};
+	uint32_t off[3] = {0u, 0u, 0u};
+	if (WRITE)
+	{
+		off[0] = offsets[s];
+		off[1] = offsets[n + s];
+		off[2] = offsets[2 * n + s];
+	}
+	for (int dz = -1; dz <= 1; dz++)
+	{
+		const int z = cz + dz;
+		if (z < 0 || z >= gi.y)
+			continue;
+		for (int dy = -1; dy <= 1; dy++)
+		{
+			const int y = cy + dy;
+			if (y < 0 || y >= gi.x)
+				continue;
+			const uint32_t rr = ro + uint32_t(z * gi.x + y);
+			uint32_t lo = rowStart[rr];
+			const uint32_t hi = rowStart[rr + 1];
+			if (rr == r)
+				lo = uint32_t(s) + 1u;
+			else
+			{
+				// first position whose (minX key, body index) follows ours
+				uint32_t l = lo, h = hi;
+				while (l < h)
+				{
+					const uint32_t m = (l + h) >> 1;
+					const uint64_t km = sortedKeys[m];
+					const bool after = (uint32_t(km) > uint32_t(key)) ||
+					                   (uint32_t(km) == uint32_t(key) && __float_as_uint(smax[m].w) > i);
+					if (after) h = m;
+					else l = m + 1;
+				}
+				lo = l;
+			}
+			for (uint32_t j = lo; j < hi; j++)
+			{
+				const float4 b0 = smin[j];
+				if (b0.x > a1.x)
+					break;
+				const float4 b1 = smax[j];
+				if (b1.y < a0.y || a1.y < b0.y || b1.z < a0.z || a1.z < b0.z)
+					continue;
+				const uint32_t jj = __float_as_uint(b1.w);
+				if (group && group[jj] == gi_)
+					continue;
+				const int tj = int(b0.w);
+				const int c = pairClass(ti, tj);
+				if (WRITE)
+				{
+					uint32_t pa, pb;
+					if (c == 1) { pa = ti == GEOM_SPHERE ? i : jj; pb = ti == GEOM_SPHERE ? jj : i; }
+					else { pa = min(i, jj); pb = max(i, jj); }
+					pairs[off[c]++] = make_uint2(pa, pb);
+				}
+				else
+					cnt[c]++;
+			}
+		}
+	}
+	if (!WRITE)
+	{
+		counts[s] = cnt[0];
+		counts[n + s] = cnt[1];
+		counts[2 * n + s] = cnt[2];
+	}
+}
+
+// ---------------------------------------------------------------------------
+// narrowphase
+// ---------------------------------------------------------------------------
+PXH_D void storeContacts(int slot, const Contacts& c, int cnt, float4* ctN, float4* ctP)
+{
+	ctN[slot] = make_float4(c.n.x, c.n.y, c.n.z, __int_as_float(cnt));
+	for (int k = 0; k < cnt; k++)
+		ctP[4 * int64_t(slot) + k] = make_float4(c.p[k].x, c.p[k].y, c.p[k].z, c.sep[k]);
+}
+
+// Body-body pairs; the shape order is the reference contact-method table's
+// (lower geometry type first), so body a of a sphere-box pair is the sphere.
+__global__ void k_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom,
+                           float cd, float tol, float4* ctN, float4* ctP)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p >= P)
+		return;
+	const uint2 ab = pairs[p];
+	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
+	const int t0 = int(g0.w), t1 = int(g1.w);
+	Contacts c;
+	c.count = 0;
+	int cnt = 0;
+	const float4 p0 = pos[ab.x], p1 = pos[ab.y];
+	if (t0 == GEOM_SPHERE && t1 == GEOM_SPHERE)
+		cnt = sphereSphere(g0.x, mk(p0), g1.x, mk(p1), cd, c);
+	else if (t0 == GEOM_SPHERE)
+	{
+		xf x1; x1.p = mk(p1); x1.q = mkq(rot[ab.y]);
+		cnt = sphereBox(g0.x, mk(p0), mk(g1.x, g1.y, g1.z), x1, cd, c);
+	}
+	else
+	{
+		xf x0, x1;
+		x0.p = mk(p0); x0.q = mkq(rot[ab.x]);
+		x1.p = mk(p1); x1.q = mkq(rot[ab.y]);
+		cnt = boxBox(mk(g0.x, g0.y, g0.z), x0, mk(g1.x, g1.y, g1.z), x1, cd, tol, c);
+	}
+	storeContacts(p, c, cnt, ctN, ctP);
+}
+
+// Body-plane pairs (every body against every plane of its world; a plane's
+// bounds are unbounded so the broadphase always reports these pairs). The body
+// is constraint body a; normals point from the plane to the body.
+__global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
+                            const float4* rot, const float4* geom, const float4* planes, float cd, float4* ctN,
+                            float4* ctP)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= Q)
+		return;
+	const int i = qBody[q];
+	const float4 pl = planes[qPlane[q]];
+	const float4 g = geom[i];
+	Contacts c;
+	c.count = 0;
+	int cnt;
+	if (int(g.w) == GEOM_SPHERE)
+		cnt = spherePlane(g.x, mk(pos[i]), mk(pl), pl.w, cd, c);
+	else
+	{
+		xf x;
+		x.p = mk(pos[i]); x.q = mkq(rot[i]);
+		cnt = planeBox(mk(pl), pl.w, mk(g.x, g.y, g.z), x, cd, c);
+		c.n = -c.n;
+	}
+	storeContacts(slotBase + q, c, cnt, ctN, ctP);
+}
+
+__global__ void k_touch_flags(int S, const float4* ctN, uint32_t* flags)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < S)
+		flags[s] = __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
+}
+
+// constraint list: touching pair slots in slot order, with the scene-local
+// ordering key used by the colouring
+__global__ void k_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
+                            const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
+                            int* conB, int* conSlot, uint64_t* conKey, int* conColor)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= S || __float_as_int(ctN[s].w) <= 0)
+		return;
+	const int c = int(scanned[s]);
+	int a, b;
+	uint32_t la, lb;
+	if (s < P)
+	{
+		const uint2 ab = pairs[s];
+		a = int(ab.x); b = int(ab.y);
+		la = uint32_t(local[a]); lb = uint32_t(local[b]);
+	}
+	else
+	{
+		a = qBody[s - P];
+		const int pl = qPlane[s - P];
+		b = -1 - pl;
+		la = uint32_t(local[a]);
+		lb = 0x80000000u | uint32_t(planeLocal[pl]);
+	}
+	conA[c] = a;
+	conB[c] = b;
+	conSlot[c] = s;
+	conKey[c] = mix64((uint64_t(la) << 32) | lb);
+	conColor[c] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// colouring: a constraint joins colour r when its key is the minimum over the
+// uncoloured constraints of both its dynamic bodies (deterministic, and a
+// function of each world's own content)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_u64(uint64_t* p, int n, uint64_t v)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		p[i] = v;
+}
+
+__global__ void k_color_min(int C, const int* conA, const int* conB, const uint64_t* conKey, const int* conColor,
+                            unsigned long long* best)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C || conColor[c] >= 0)
+		return;
+	const unsigned long long k = conKey[c];
+	atomicMin(&best[conA[c]], k);
+	const int b = conB[c];
+	if (b >= 0)
+		atomicMin(&best[b], k);
+}
+
+__global__ void k_color_pick(int C, int round, const int* conA, const int* conB, const uint64_t* conKey, int* conColor,
+                             const unsigned long long* best, unsigned long long* bestReset, uint32_t* colored)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C || conColor[c] >= 0)
+		return;
+	const unsigned long long k = conKey[c];
+	const int a = conA[c], b = conB[c];
+	if (best[a] == k && (b < 0 || best[b] == k))
+	{
+		conColor[c] = round;
+		atomicAdd(colored, 1u);
+	}
+	(void)bestReset;
+}
+
+__global__ void k_color_count(int C, const int* conColor, uint32_t* colorCount)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < C)
+		atomicAdd(&colorCount[conColor[c]], 1u);
+}
+
+__global__ void k_color_order(int C, const int* conColor, uint32_t* colorFill, int* order)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < C)
+		order[atomicAdd(&colorFill[conColor[c]], 1u)] = c;
+}
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+
+// bodyCoreComputeUnconstrainedVelocity (lowleveldynamics/src/DyBodyCoreIntegrator.h:40-75)
+// + copyToSolverBodyDataStep (lowleveldynamics/src/DyTGSDynamics.cpp:154-242)
+__global__ void k_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
+                               const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
+                               float maxAngSq, float4* svLin, float4* svAng, float4* svDLin, float4* svDAng,
+                               float4* svP, float4* svDQ, float4* svI0, float4* svI1, float4* svOrigLin)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float4 l = lin[i];
+	f3 lv = mk(l) + mk(gdt.x, gdt.y, gdt.z);
+	f3 av = mk(ang[i]);
+	const float ld = 1.0f - linDampDt, ad = 1.0f - angDampDt;
+	lv = lv * (ld >= 0.0f ? ld : 0.0f);
+	av = av * (ad >= 0.0f ? ad : 0.0f);
+	const float l2 = dot(lv, lv);
+	if (l2 > maxLinSq) lv = lv * sqrtf(maxLinSq / l2);
+	const float a2 = dot(av, av);
+	if (a2 > maxAngSq) av = av * sqrtf(maxAngSq / a2);
+	const m33 R = qmat(mkq(rot[i]));
+	const float4 ii = invI[i];
+	const f3 si = mk(sqrtf(ii.x), sqrtf(ii.y), sqrtf(ii.z));
+	const f3 sI = mk(si.x == 0.0f ? 0.0f : 1.0f / si.x, si.y == 0.0f ? 0.0f : 1.0f / si.y,
+	                 si.z == 0.0f ? 0.0f : 1.0f / si.z);
+	m33 A, B;
+	A.c0 = R.c0 * si.x; A.c1 = R.c1 * si.y; A.c2 = R.c2 * si.z;
+	B.c0 = R.c0 * sI.x; B.c1 = R.c1 * sI.y; B.c2 = R.c2 * sI.z;
+	const m33 Rt = mtrans(R);
+	const m33 S = mmm(A, Rt);
+	const m33 SI = mmm(B, Rt);
+	svLin[i] = f4(lv, l.w);
+	svAng[i] = f4(mmul(SI, av), 0.0f);
+	svDLin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	svDAng[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	svP[i] = pos[i];
+	svDQ[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+	// symmetric sqrt inverse inertia: (xx yy zz xy) (xz yz)
+	svI0[i] = make_float4(S.c0.x, S.c1.y, S.c2.z, S.c1.x);
+	svI1[i] = make_float4(S.c2.x, S.c2.y, 0.0f, 0.0f);
+	svOrigLin[i] = f4(lv, 0.0f);
+}
+
+PXH_D m33 loadSym(float4 a, float4 b)
+{
+	m33 m;
+	m.c0 = mk(a.x, a.w, b.x);
+	m.c1 = mk(a.w, a.y, b.y);
+	m.c2 = mk(b.x, b.y, a.z);
+	return m;
+}
+
+// Contact rows of one constraint at its colour-ordered position o
+// (setupFinalizeSolverConstraints / constructContactConstraintStep,
+//  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction anchors
+//  growPatches, DyFrictionCorrelation.cpp:187-292)
+__global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                       const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
+                       const float4* svLin, const float4* svOrigLin, const float4* svI0, const float4* svI1,
+                       float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows)
+{
+	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= C)
+		return;
+	const int c = order[o];
+	const int a = conA[c], b = conB[c], slot = conSlot[c];
+	const float4 nn = ctN[slot];
+	const int cnt = __float_as_int(nn.w);
+	const f3 n = mk(nn);
+	const bool dynB = b >= 0;
+	const m33 Sa = loadSym(svI0[a], svI1[a]);
+	m33 Sb;
+	Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
+	float invMassB = 0.0f;
+	xf f0, f1;
+	f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
+	f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
+	f3 linB = mk(0, 0, 0);
+	if (dynB)
+	{
+		Sb = loadSym(svI0[b], svI1[b]);
+		invMassB = svLin[b].w;
+		f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
+		linB = mk(svOrigLin[b]);
+	}
+	const float invMassA = svLin[a].w;
+	const float nlen2 = dot(n, n);
+	const float imn0 = invMassA * nlen2, imn1 = -invMassB * nlen2;
+	const float invDtp8 = invStepDt * scale;
+	f3 cp[4];
+	float cs[4];
+	for (int k = 0; k < cnt; k++)
+	{
+		const float4 q = ctP[4 * int64_t(slot) + k];
+		cp[k] = mk(q);
+		cs[k] = q.w;
+		const f3 ra = cp[k] - f0.p, rb = cp[k] - f1.p;
+		const f3 raXnI = mmul(Sa, cross(ra, n)), rbXnI = mmul(Sb, cross(rb, n));
+		const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
+		const float pen = q.w - restDist;
+		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+		rows.nA[k * int64_t(C) + o] = f4(raXnI, pen);
+		rows.nB[k * int64_t(C) + o] = f4(rbXnI, -(pen > 0.0f ? invStepDt : invDtp8));
+		rows.nC[k * int64_t(C) + o] = make_float4(recip, recip, 0.0f, 0.0f);
+	}
+	// friction anchors among points closer than the offset threshold
+	int nAnch = 0;
+	f3 wa0 = mk(0, 0, 0), wa1 = mk(0, 0, 0);
+	float pd2 = 0.0f;
+	const float thr = frictionOffsetThreshold + restDist;
+	for (int j = 0; j < cnt; j++)
+	{
+		if (!(cs[j] < thr))
+			continue;
+		const f3 wp = cp[j];
+		if (nAnch == 0) { wa0 = wp; nAnch = 1; }
+		else if (nAnch == 1)
+		{
+			const f3 d = wp - wa0;
+			pd2 = dot(d, d);
+			if (pd2 > 1e-8f) { wa1 = wp; nAnch = 2; }
+		}
+		else
+		{
+			const f3 e0 = wp - wa0, e1 = wp - wa1;
+			const float d0 = dot(e0, e0), d1 = dot(e1, e1);
+			if (d0 > d1) { if (d0 > pd2) { wa1 = wp; pd2 = d0; } }
+			else if (d1 > pd2) { wa0 = wp; pd2 = d1; }
+		}
+	}
+	const float fscale = nAnch == 2 ? 0.5f : 1.0f;
+	rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), 0);
+	rows.hF[o] = make_float4(n.x, n.y, n.z, invMassA);
+	rows.hG[o] = make_float4(invMassB, fscale, invStepDt * scale, -1e32f);
+	if (nAnch == 0)
+		return;
+	const f3 lvr = mk(svOrigLin[a]) - linB;
+	const f3 fb = (0.70710678f > fabsf(n.x)) ? mk(0.0f, -n.z, n.y) : mk(-n.y, n.x, 0.0f);
+	f3 t0 = lvr - n * dot(n, lvr);
+	t0 = dot(t0, t0) > 0.0001f ? t0 : fb;
+	t0 = normalize(t0);
+	const f3 t1 = normalize(cross(n, t0));
+	const f3 relTr = f0.p - f1.p;
+	for (int j = 0; j < nAnch; j++)
+	{
+		const f3 w = j == 0 ? wa0 : wa1;
+		const f3 ra = qrot(f0.q, xtransinv(f0, w)), rb = qrot(f1.q, xtransinv(f1, w));
+		const f3 err = (ra - rb) + relTr;
+		for (int d = 0; d < 2; d++)
+		{
+			const f3 t = d == 0 ? t0 : t1;
+			const f3 raXnI = mmul(Sa, cross(ra, t)), rbXnI = mmul(Sb, cross(rb, t));
+			const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
+			const int r = 2 * j + d;
+			rows.fT[r * int64_t(C) + o] = f4(t, dot(err, t));
+			rows.fA[r * int64_t(C) + o] = f4(raXnI, unit > 0.0f ? scale / unit : 0.0f);
+			rows.fB[r * int64_t(C) + o] = f4(rbXnI, 0.0f);
+		}
+	}
+}
+
+// one colour: each lane solves one constraint (solveContact /
+// solveDynamicContactsStep, lowleveldynamics/src/DyTGSContactPrep.cpp:1490-1843)
+__global__ void __launch_bounds__(256) k_solve(int begin, int end, int C, Rows rows, float4* svLin, float4* svAng,
+                                               const float4* svDLin, const float4* svDAng, float minPen, float muS,
+                                               float muD)
+{
+	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= end)
+		return;
+	const int4 hi = rows.hI[o];
+	const float4 hf = rows.hF[o];
+	const float4 hg = rows.hG[o];
+	const int a = hi.x, b = hi.y;
+	const int cnt = hi.z & 0xFF, nF = hi.z >> 8;
+	const bool dynB = b >= 0;
+	const f3 n = mk(hf);
+	const float invMassA = hf.w, invMassB = hg.x;
+	f3 l0 = mk(svLin[a]), a0 = mk(svAng[a]);
+	const f3 am0 = mk(svDAng[a]);
+	f3 rel = mk(svDLin[a]);
+	f3 l1 = mk(0, 0, 0), a1 = mk(0, 0, 0), am1 = mk(0, 0, 0);
+	if (dynB)
+	{
+		l1 = mk(svLin[b]);
+		a1 = mk(svAng[b]);
+		am1 = mk(svDAng[b]);
+		rel = rel - mk(svDLin[b]);
+	}
+	const f3 nim0 = n * invMassA, nim1 = n * invMassB;
+	const float dv = dot(rel, n);
+	const float maxPenBias = hg.w;
+	float acc = 0.0f;
+	for (int k = 0; k < cnt; k++)
+	{
+		const int64_t ri = k * int64_t(C) + o;
+		const float4 ra = rows.nA[ri], rb = rows.nB[ri];
+		float4 rc = rows.nC[ri];
+		const f3 raXnI = mk(ra), rbXnI = mk(rb);
+		const float dang = dot(am0, raXnI) - dot(am1, rbXnI);
+		const float sep = fmaxf(minPen, ra.w + (dv + dang));
+		const float bias = fminf(-maxPenBias, rb.w * sep);
+		const f3 v0 = mulv(l0, n) + mulv(a0, raXnI);
+		const f3 v1 = mulv(l1, n) + mulv(a1, rbXnI);
+		const f3 dvv = v0 - v1;
+		const float nv = dvv.x + dvv.y + dvv.z;
+		const float lambda = bias * rc.y - nv * rc.x;
+		const float af = rc.z;
+		const float nf = fminf(af + fmaxf(lambda, -af), 1e32f);
+		const float df = nf - af;
+		l0 = nim0 * df + l0;
+		l1 = l1 - nim1 * df;
+		a0 = raXnI * df + a0;
+		a1 = a1 - rbXnI * df;
+		rc.z = nf;
+		rows.nC[ri] = rc;
+		acc = acc + nf;
+	}
+	if (nF)
+	{
+		const float maxF = muS * acc, maxD = muD * acc;
+		const float fscale = hg.y, biasScale = hg.z;
+		for (int i = 0; i < nF; i += 2)
+		{
+			float ti[2], af[2];
+			f3 t[2], raI[2], rbI[2];
+			for (int d = 0; d < 2; d++)
+			{
+				const int64_t ri = (i + d) * int64_t(C) + o;
+				const float4 ft = rows.fT[ri], fa = rows.fA[ri], fb = rows.fB[ri];
+				t[d] = mk(ft); raI[d] = mk(fa); rbI[d] = mk(fb);
+				const float dvf = (dot(raI[d], am0) - dot(rbI[d], am1)) + dot(t[d], rel);
+				const float bias = (ft.w + dvf) * biasScale;
+				const f3 v0 = mulv(l0, t[d]) + mulv(a0, raI[d]);
+				const f3 v1 = mulv(l1, t[d]) + mulv(a1, rbI[d]);
+				const f3 dd = v0 - v1;
+				const float nv = dd.x + dd.y + dd.z;
+				af[d] = fb.w;
+				ti[d] = (af[d] - bias * fa.w) - nv * fa.w;
+			}
+			const float tot = sqrtf(ti[0] * ti[0] + ti[1] * ti[1]);
+			const bool clamp = tot > fscale * maxF;
+			const float tc = clamp ? fminf(fscale * maxD, tot) : tot;
+			const float ratio = tot > 0.0f ? tc / tot : 0.0f;
+			const float n0 = ti[0] * ratio, n1 = ti[1] * ratio;
+			const float d0 = n0 - af[0], d1 = n1 - af[1];
+			l0 = (t[0] * invMassA) * d0 + ((t[1] * invMassA) * d1 + l0);
+			l1 = (l1 - (t[1] * invMassB) * d1) - (t[0] * invMassB) * d0;
+			a0 = raI[0] * d0 + (raI[1] * d1 + a0);
+			a1 = (a1 - rbI[1] * d1) - rbI[0] * d0;
+			rows.fB[i * int64_t(C) + o].w = n0;
+			rows.fB[(i + 1) * int64_t(C) + o].w = n1;
+		}
+	}
+	svLin[a] = f4(l0, invMassA);
+	svAng[a] = f4(a0, 0.0f);
+	if (dynB)
+	{
+		svLin[b] = f4(l1, invMassB);
+		svAng[b] = f4(a1, 0.0f);
+	}
+}
+
+// integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476)
+__global__ void k_integrate(int n, float dt, const float4* svLin, const float4* svAng, float4* svDLin, float4* svDAng,
+                            float4* svP, float4* svDQ, const float4* svI0, const float4* svI1)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const f3 lv = mk(svLin[i]);
+	const f3 av = mk(svAng[i]);
+	const f3 delta = lv * dt;
+	const f3 amv = mmul(loadSym(svI0[i], svI1[i]), av);
+	const float w2 = dot(amv, amv);
+	svP[i] = f4(mk(svP[i]) + delta, 0.0f);
+	if (w2 != 0.0f)
+	{
+		const float w = sqrtf(w2);
+		const float v = dt * w * 0.5f;
+		float s, q;
+		sincosf(v, &s, &q);
+		s /= w;
+		const f3 pqr = amv * s;
+		qt qv; qv.x = pqr.x; qv.y = pqr.y; qv.z = pqr.z; qv.w = 0.0f;
+		const qt dq = mkq(svDQ[i]);
+		qt res = qmul(qv, dq);
+		res.x += dq.x * q; res.y += dq.y * q; res.z += dq.z * q; res.w += dq.w * q;
+		svDQ[i] = f4(qnormalize(res));
+	}
+	svDAng[i] = f4(mk(svDAng[i]) + av * dt, 0.0f);
+	svDLin[i] = f4(mk(svDLin[i]) + delta, 0.0f);
+}
+
+// copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580)
+__global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, const float4* svLin,
+                           const float4* svAng, const float4* svP, const float4* svDQ, const float4* svI0,
+                           const float4* svI1)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	rot[i] = f4(qnormalize(qmul(mkq(svDQ[i]), mkq(rot[i]))));
+	pos[i] = svP[i];
+	const float4 l = svLin[i];
+	lin[i] = l;
+	ang[i] = f4(mmul(loadSym(svI0[i], svI1[i]), mk(svAng[i])), 0.0f);
+}
+
+// ---------------------------------------------------------------------------
+// Direct GPU API gather/scatter (updateBodiesAndShapes.cu:999-1253)
+// ---------------------------------------------------------------------------
+__global__ void k_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot,
+                           const float4* lin, const float4* ang, float* out)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const uint32_t i = idx ? idx[k] : uint32_t(k);
+	if (what == 0)
+	{
+		const float4 q = rot[i], p = pos[i];
+		float* o = out + 7 * int64_t(k);
+		o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w; o[4] = p.x; o[5] = p.y; o[6] = p.z;
+	}
+	else
+	{
+		const float4 v = what == 1 ? lin[i] : ang[i];
+		float* o = out + 3 * int64_t(k);
+		o[0] = v.x; o[1] = v.y; o[2] = v.z;
+	}
+}
+
+__global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
+                           const float* in)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const uint32_t i = idx ? idx[k] : uint32_t(k);
+	if (what == 0)
+	{
+		const float* s = in + 7 * int64_t(k);
+		rot[i] = make_float4(s[0], s[1], s[2], s[3]);
+		pos[i] = make_float4(s[4], s[5], s[6], 0.0f);
+	}
+	else if (what == 1)
+	{
+		const float* s = in + 3 * int64_t(k);
+		lin[i] = make_float4(s[0], s[1], s[2], lin[i].w);
+	}
+	else
+	{
+		const float* s = in + 3 * int64_t(k);
+		ang[i] = make_float4(s[0], s[1], s[2], 0.0f);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// standalone broadphase: found/lost pairs between consecutive pair sets, both
+// sorted by (id0, id1) (BpBroadPhase.h:159-185 created/deleted semantics)
+// ---------------------------------------------------------------------------
+PXH_D uint64_t pkey(uint2 p) { return (uint64_t(p.x) << 32) | p.y; }
+
+__global__ void k_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals)
+{
+	const int p = blockIdx.x * blockDim.x + threadIdx.x;
+	if (p < P)
+	{
+		keys[p] = pkey(pairs[p]);
+		vals[p] = uint32_t(p);
+	}
+}
+
+// flag[i] = 1 if key a[i] is absent from sorted b
+__global__ void k_set_diff_flags(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= na)
+		return;
+	const uint64_t k = a[i];
+	int l = 0, h = nb;
+	while (l < h)
+	{
+		const int m = (l + h) >> 1;
+		if (b[m] < k) l = m + 1;
+		else h = m;
+	}
+	flag[i] = (l < nb && b[l] == k) ? 0u : 1u;
+}
+
+__global__ void k_compact_keys(int na, const uint64_t* a, const uint32_t* flag, const uint32_t* scanned, uint2* out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < na && flag[i])
+		out[scanned[i]] = make_uint2(uint32_t(a[i] >> 32), uint32_t(a[i]));
+}
+
+
+// batched immediate-mode contacts (PxGenerateContacts on a fresh cache)
+__global__ void k_np_immediate(int n, const int* type0, const float* geom0, const float* pose0, const int* type1,
+                               const float* geom1, const float* pose1, float cd, float tol, float* out, int* counts)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= n)
+		return;
+	const int t0 = type0[k], t1 = type1[k];
+	const float* g0 = geom0 + 4 * int64_t(k);
+	const float* g1 = geom1 + 4 * int64_t(k);
+	const float* q0 = pose0 + 7 * int64_t(k);
+	const float* q1 = pose1 + 7 * int64_t(k);
+	xf x0, x1;
+	x0.p = mk(q0[0], q0[1], q0[2]); x0.q.x = q0[3]; x0.q.y = q0[4]; x0.q.z = q0[5]; x0.q.w = q0[6];
+	x1.p = mk(q1[0], q1[1], q1[2]); x1.q.x = q1[3]; x1.q.y = q1[4]; x1.q.z = q1[5]; x1.q.w = q1[6];
+	Contacts c;
+	c.count = 0;
+	int cnt = -1;
+	if (t0 == GEOM_SPHERE && t1 == GEOM_SPHERE)
+		cnt = sphereSphere(g0[0], x0.p, g1[0], x1.p, cd, c);
+	else if (t0 == GEOM_SPHERE && t1 == GEOM_PLANE)
+	{
+		const f3 pn = qrot(x1.q, mk(1.0f, 0.0f, 0.0f));
+		cnt = spherePlane(g0[0], x0.p, pn, -dot(pn, x1.p), cd, c);
+	}
+	else if (t0 == GEOM_SPHERE && t1 == GEOM_BOX)
+		cnt = sphereBox(g0[0], x0.p, mk(g1[0], g1[1], g1[2]), x1, cd, c);
+	else if (t0 == GEOM_PLANE && t1 == GEOM_BOX)
+	{
+		const f3 pn = qrot(x0.q, mk(1.0f, 0.0f, 0.0f));
+		cnt = planeBox(pn, -dot(pn, x0.p), mk(g1[0], g1[1], g1[2]), x1, cd, c);
+	}
+	else if (t0 == GEOM_BOX && t1 == GEOM_BOX)
+		cnt = boxBox(mk(g0[0], g0[1], g0[2]), x0, mk(g1[0], g1[1], g1[2]), x1, cd, tol, c);
+	counts[k] = cnt;
+	for (int i = 0; i < (cnt > 0 ? cnt : 0); i++)
+	{
+		float* o = out + (4 * int64_t(k) + i) * 8;
+		o[0] = c.p[i].x; o[1] = c.p[i].y; o[2] = c.p[i].z; o[3] = c.sep[i];
+		o[4] = c.n.x; o[5] = c.n.y; o[6] = c.n.z; o[7] = 0.0f;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int nblk(int64_t n, int t) { return int((n + t - 1) / t); }
+
+void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st)
+{
+	// tmp must hold ceil(n/2048) + ceil(n/2048^2) + ... entries
+	if (n <= 0)
+		return;
+	const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+	k_scan_tiles<<<int(nb), SCAN_THREADS, 0, st>>>(data, n, nb > 1 ? tmp : nullptr);
+	if (nb > 1)
+	{
+		launch_scan(tmp, nb, tmp + nb, st);
+		k_scan_add<<<nblk(n, 256), 256, 0, st>>>(data, n, tmp);
+	}
+}
+
+int64_t scan_tmp_size(int64_t n)
+{
+	int64_t total = 0;
+	while (n > 1)
+	{
+		n = (n + SCAN_TILE - 1) / SCAN_TILE;
+		total += n;
+	}
+	return total + 1;
+}
+
+void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2, int n, int bitBegin,
+                       int bitEnd, uint32_t* hist, uint32_t* scanTmp, hipStream_t st, bool& resultInAlt)
+{
+	resultInAlt = false;
+	if (n <= 0)
+		return;
+	const int nb = nblk(n, RS_TILE);
+	uint64_t* ka = keys;
+	uint32_t* va = vals;
+	uint64_t* kb = keys2;
+	uint32_t* vb = vals2;
+	for (int shift = bitBegin; shift < bitEnd; shift += 8)
+	{
+		k_rs_hist<<<nb, RS_THREADS, 0, st>>>(ka, n, shift, hist, nb);
+		launch_scan(hist, int64_t(256) * nb, scanTmp, st);
+		k_rs_scatter<<<nb, RS_THREADS, 0, st>>>(ka, va, kb, vb, n, shift, hist, nb);
+		uint64_t* tk = ka; ka = kb; kb = tk;
+		uint32_t* tv = va; va = vb; vb = tv;
+		resultInAlt = !resultInAlt;
+	}
+}
+
+#define PXH_LAUNCH(kernel, n, ...) \
+	do { if ((n) > 0) kernel<<<nblk((n), 256), 256, 0, st>>>(__VA_ARGS__); } while (0)
+
+void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin, float4* bmax,
+                   hipStream_t st)
+{
+	PXH_LAUNCH(k_bounds, n, n, pos, rot, geom, co, bmin, bmax);
+}
+void launch_scene_grid(int nScenes, const int* bodyStart, const float4* bmin, const float4* bmax, float4* gridF,
+                       int2* gridI, uint32_t* rows, hipStream_t st)
+{
+	if (nScenes > 0)
+		k_scene_grid<<<nScenes, 256, 0, st>>>(nScenes, bodyStart, bmin, bmax, gridF, gridI, rows);
+}
+void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
+                 const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st)
+{
+	PXH_LAUNCH(k_keys, n, n, bmin, bmax, scene, gridF, gridI, rowOff, keys, vals);
+}
+void launch_gather_sorted(int n, const uint32_t* sv, const float4* bmin, const float4* bmax, float4* smin,
+                          float4* smax, hipStream_t st)
+{
+	PXH_LAUNCH(k_gather_sorted, n, n, sv, bmin, bmax, smin, smax);
+}
+void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t totalRows, hipStream_t st)
+{
+	if (n == 0)
+	{
+		PXH_LAUNCH(k_fill_u32, int64_t(totalRows) + 1, rowStart, int64_t(totalRows) + 1, 0u);
+		return;
+	}
+	PXH_LAUNCH(k_row_start, n, n, sk, rowStart, totalRows);
+}
+void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
+                  const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
+                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, hipStream_t st)
+{
+	if (write)
+		PXH_LAUNCH(k_sweep<true>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs);
+	else
+		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs);
+}
+void launch_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom, float cd,
+                     float tol, float4* ctN, float4* ctP, hipStream_t st)
+{
+	PXH_LAUNCH(k_np_pairs, P, P, pairs, pos, rot, geom, cd, tol, ctN, ctP);
+}
+void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
+                      const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st)
+{
+	PXH_LAUNCH(k_np_planes, Q, Q, slotBase, qBody, qPlane, pos, rot, geom, planes, cd, ctN, ctP);
+}
+void launch_touch_flags(int S, const float4* ctN, uint32_t* flags, hipStream_t st)
+{
+	PXH_LAUNCH(k_touch_flags, S, S, ctN, flags);
+}
+void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs, const int* qBody,
+                      const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
+                      uint64_t* conKey, int* conColor, hipStream_t st)
+{
+	PXH_LAUNCH(k_con_build, S, S, P, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, conA, conB, conSlot, conKey,
+	           conColor);
+}
+void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
+                        int* conColor, uint64_t* best, uint32_t* colored, hipStream_t st)
+{
+	PXH_LAUNCH(k_fill_u64, n, best, n, ~0ull);
+	PXH_LAUNCH(k_color_min, C, C, conA, conB, conKey, conColor, reinterpret_cast<unsigned long long*>(best));
+	PXH_LAUNCH(k_color_pick, C, C, round, conA, conB, conKey, conColor,
+	           reinterpret_cast<const unsigned long long*>(best), nullptr, colored);
+}
+void launch_color_count(int C, const int* conColor, uint32_t* colorCount, hipStream_t st)
+{
+	PXH_LAUNCH(k_color_count, C, C, conColor, colorCount);
+}
+void launch_color_order(int C, const int* conColor, uint32_t* colorFill, int* order, hipStream_t st)
+{
+	PXH_LAUNCH(k_color_order, C, C, conColor, colorFill, order);
+}
+void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
+                         const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
+                         float maxAngSq, SolverBodies sb, hipStream_t st)
+{
+	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, sb.lin,
+	           sb.ang, sb.dlin, sb.dang, sb.p, sb.dq, sb.i0, sb.i1, sb.origLin);
+}
+void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
+                 const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
+                 float scale, float restDist, float fot, Rows rows, hipStream_t st)
+{
+	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.lin, sb.origLin, sb.i0, sb.i1,
+	           invStepDt, scale, restDist, fot, rows);
+}
+void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
+                  hipStream_t st)
+{
+	if (end > begin)
+		k_solve<<<nblk(end - begin, 256), 256, 0, st>>>(begin, end, C, rows, sb.lin, sb.ang, sb.dlin, sb.dang, minPen,
+		                                                 muS, muD);
+}
+void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
+{
+	PXH_LAUNCH(k_integrate, n, n, dt, sb.lin, sb.ang, sb.dlin, sb.dang, sb.p, sb.dq, sb.i0, sb.i1);
+}
+void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st)
+{
+	PXH_LAUNCH(k_finalize, n, n, pos, rot, lin, ang, sb.lin, sb.ang, sb.p, sb.dq, sb.i0, sb.i1);
+}
+void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
+                     const float4* ang, float* out, hipStream_t st)
+{
+	PXH_LAUNCH(k_get_data, n, n, idx, what, pos, rot, lin, ang, out);
+}
+void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
+                     const float* in, hipStream_t st)
+{
+	PXH_LAUNCH(k_set_data, n, n, idx, what, pos, rot, lin, ang, in);
+}
+void launch_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals, hipStream_t st)
+{
+	PXH_LAUNCH(k_pair_keys, P, P, pairs, keys, vals);
+}
+void launch_set_diff(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag, hipStream_t st)
+{
+	PXH_LAUNCH(k_set_diff_flags, na, na, a, nb, b, flag);
+}
+void launch_compact_keys(int na, const uint64_t* a, const uint32_t* flag, const uint32_t* scanned, uint2* out,
+                         hipStream_t st)
+{
+	PXH_LAUNCH(k_compact_keys, na, na, a, flag, scanned, out);
+}
+
+void launch_np_immediate(int n, const int* type0, const float* geom0, const float* pose0, const int* type1,
+                         const float* geom1, const float* pose1, float cd, float tol, float* out, int* counts,
+                         hipStream_t st)
+{
+	PXH_LAUNCH(k_np_immediate, n, n, type0, geom0, pose0, type1, geom1, pose1, cd, tol, out, counts);
+}
+
+__global__ void k_bp_inflate(int n, const float* bounds, const float* dist, const uint32_t* groups, float4* bmin,
+                             float4* bmax, int* scene, int* group, int* bodyStart)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i == 0)
+	{
+		bodyStart[0] = 0;
+		bodyStart[1] = n;
+	}
+	if (i >= n)
+		return;
+	const float* b = bounds + 6 * int64_t(i);
+	const float d = dist ? dist[i] : 0.0f;
+	// geometry type slot unused by the standalone broadphase (all pairs class 2)
+	bmin[i] = make_float4(b[0] - d, b[1] - d, b[2] - d, float(GEOM_BOX));
+	bmax[i] = make_float4(b[3] + d, b[4] + d, b[5] + d, 0.0f);
+	scene[i] = 0;
+	group[i] = groups ? int(groups[i]) : i;
+}
+
+void launch_bp_inflate(int n, const float* bounds, const float* dist, const uint32_t* groups, float4* bmin,
+                       float4* bmax, int* scene, int* group, int* bodyStart, hipStream_t st)
+{
+	k_bp_inflate<<<nblk(n > 0 ? n : 1, 256), 256, 0, st>>>(n, bounds, dist, groups, bmin, bmax, scene, group,
+	                                                       bodyStart);
+}
+
+}  // namespace pxh

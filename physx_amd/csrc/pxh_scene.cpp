@@ -1,0 +1,905 @@
+// Host side of the GPU rigid-body step: the C ABI of include/pxhip.h.
+//
+// A pxh_scene owns one device-resident structure-of-arrays world (bodies of
+// one or more independent worlds, their static planes) and the per-step
+// buffers; pxh_scene_simulate enqueues the whole step on the scene's stream.
+// The reference splits this work across the GPU pipeline objects created by
+// PxPhysXGpu (physxgpu/src/PxgPhysXGpu.cpp:153-248) and CPU-side islands and
+// partitioning (PxgConstraintPartition.cpp:1940); here every stage runs on
+// the device and the host only reads back counts that size the next launches.
+#include "pxh_kernels.h"
+#include "../../include/pxhip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string gLastError;
+
+int fail(int code, const char* what)
+{
+	gLastError = what;
+	return code;
+}
+
+#define PXH_CHECK(call)                                                                         \
+	do {                                                                                        \
+		hipError_t e_ = (call);                                                                 \
+		if (e_ != hipSuccess)                                                                   \
+		{                                                                                       \
+			gLastError = std::string(#call) + ": " + hipGetErrorString(e_);                     \
+			return PXH_ERR_DEVICE;                                                              \
+		}                                                                                       \
+	} while (0)
+
+// growable device buffer
+template <typename T>
+struct DBuf
+{
+	T* p = nullptr;
+	size_t cap = 0;
+	hipError_t ensure(size_t n)
+	{
+		if (n <= cap && p)
+			return hipSuccess;
+		size_t nc = std::max<size_t>(n + n / 4, 64);
+		T* q = nullptr;
+		hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), nc * sizeof(T));
+		if (e != hipSuccess)
+			return e;
+		if (p)
+		{
+			// buffers may still be read by enqueued work: drain the device first
+			(void)hipDeviceSynchronize();
+			(void)hipFree(p);
+		}
+		p = q;
+		cap = nc;
+		return hipSuccess;
+	}
+	// grow keeping the first n_keep elements
+	hipError_t grow_keep(size_t n, size_t n_keep, hipStream_t st)
+	{
+		if (n <= cap && p)
+			return hipSuccess;
+		size_t nc = std::max<size_t>(n + n / 4, 64);
+		T* q = nullptr;
+		hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), nc * sizeof(T));
+		if (e != hipSuccess)
+			return e;
+		if (p && n_keep)
+		{
+			e = hipMemcpyAsync(q, p, n_keep * sizeof(T), hipMemcpyDeviceToDevice, st);
+			if (e != hipSuccess)
+				return e;
+			e = hipStreamSynchronize(st);
+			if (e != hipSuccess)
+				return e;
+		}
+		if (p)
+			(void)hipFree(p);
+		p = q;
+		cap = nc;
+		return hipSuccess;
+	}
+	void release()
+	{
+		if (p)
+			(void)hipFree(p);
+		p = nullptr;
+		cap = 0;
+	}
+};
+
+}  // namespace
+
+struct pxh_scene_t
+{
+	pxh_scene_desc desc;
+	hipStream_t stream = nullptr;
+	bool simulating = false;
+
+	// bodies
+	int n = 0;
+	DBuf<float4> pos, rot, lin, ang, geom, invI;
+	DBuf<int> scene, local;
+	std::vector<int> hScene;              // world of each body
+	std::vector<int> hBodyStart{0};       // body range of each world (size nWorlds + 1)
+	DBuf<int> bodyStart;
+	// planes
+	int m = 0;
+	DBuf<float4> planes;
+	DBuf<int> planeLocal;
+	std::vector<int> hPlaneScene;
+	std::vector<std::vector<int>> worldPlanes;
+	// body-plane pair table
+	int Q = 0;
+	DBuf<int> qBody, qPlane;
+	bool topologyDirty = true;
+
+	// per-step buffers
+	DBuf<float4> bmin, bmax, smin, smax, gridF;
+	DBuf<int2> gridI;
+	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags;
+	DBuf<uint64_t> keys, keys2;
+	DBuf<uint2> pairs;
+	DBuf<float4> ctN, ctP;
+	DBuf<int> conA, conB, conSlot, conColor, order;
+	DBuf<uint64_t> conKey, best;
+	DBuf<uint32_t> colorCount, scratch;
+	pxh::SolverBodies sb{};
+	DBuf<float4> sbLin, sbAng, sbDLin, sbDAng, sbP, sbDQ, sbI0, sbI1, sbOrig;
+	DBuf<int4> rhI;
+	DBuf<float4> rhF, rhG, rnA, rnB, rnC, rfT, rfA, rfB;
+	uint32_t* hostCounts = nullptr;       // pinned
+
+	// statistics of the last step
+	int64_t statPairs = 0, statPlanePairs = 0, statConstraints = 0, statContacts = 0, statColors = 0, statSteps = 0;
+};
+
+struct pxh_broadphase_t
+{
+	hipStream_t stream = nullptr;
+	int n = 0;
+	DBuf<float4> bmin, bmax, smin, smax, gridF;
+	DBuf<int2> gridI;
+	DBuf<int> scene, group, bodyStart;
+	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags, flags2;
+	DBuf<uint64_t> keys, keys2, pkPrev, pkCur, pkTmp;
+	DBuf<uint2> pairs, created, deleted;
+	int nPrev = 0, nCur = 0, nCreated = 0, nDeleted = 0;
+	uint32_t* hostCounts = nullptr;
+};
+
+namespace {
+
+int rebuild_topology(pxh_scene_t* s)
+{
+	// body-plane pairs: every body against the planes of its world
+	std::vector<int> qb, qp;
+	for (int i = 0; i < s->n; i++)
+	{
+		const int w = s->hScene[i];
+		if (w < int(s->worldPlanes.size()))
+			for (int p : s->worldPlanes[w])
+			{
+				qb.push_back(i);
+				qp.push_back(p);
+			}
+	}
+	s->Q = int(qb.size());
+	PXH_CHECK(s->qBody.ensure(std::max<size_t>(qb.size(), 1)));
+	PXH_CHECK(s->qPlane.ensure(std::max<size_t>(qp.size(), 1)));
+	if (s->Q)
+	{
+		PXH_CHECK(hipMemcpy(s->qBody.p, qb.data(), qb.size() * sizeof(int), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->qPlane.p, qp.data(), qp.size() * sizeof(int), hipMemcpyHostToDevice));
+	}
+	PXH_CHECK(s->bodyStart.ensure(s->hBodyStart.size()));
+	PXH_CHECK(hipMemcpy(s->bodyStart.p, s->hBodyStart.data(), s->hBodyStart.size() * sizeof(int),
+	                    hipMemcpyHostToDevice));
+	s->topologyDirty = false;
+	return PXH_OK;
+}
+
+int readCounts(hipStream_t st, uint32_t* host, const uint32_t* const* src, int k)
+{
+	for (int i = 0; i < k; i++)
+		PXH_CHECK(hipMemcpyAsync(host + i, src[i], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+	PXH_CHECK(hipStreamSynchronize(st));
+	return PXH_OK;
+}
+
+int bitsFor(uint64_t v)
+{
+	int b = 0;
+	while (b < 64 && (v >> b))
+		b++;
+	return b;
+}
+
+// Broadphase core shared by the scene and the standalone API: from inflated
+// bounds (bmin/bmax, bmin.w = geometry type) of n boxes in nWorlds worlds,
+// produce the class-ordered pair list. Returns total pairs in *outP and the
+// class boundaries in cls[0..3].
+template <typename Ctx>
+int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_group, const int* d_bodyStart,
+               hipStream_t st, uint32_t* cls, uint32_t* hostCounts)
+{
+	PXH_CHECK(s->gridF.ensure(nWorlds));
+	PXH_CHECK(s->gridI.ensure(nWorlds));
+	PXH_CHECK(s->rows.ensure(nWorlds + 1));
+	PXH_CHECK(s->rowOff.ensure(nWorlds + 1));
+	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(3) * n + 1, 256 * 1024)) + 1024));
+	pxh::launch_scene_grid(nWorlds, d_bodyStart, s->bmin.p, s->bmax.p, s->gridF.p, s->gridI.p, s->rows.p, st);
+	PXH_CHECK(hipMemsetAsync(s->rows.p + nWorlds, 0, sizeof(uint32_t), st));
+	PXH_CHECK(hipMemcpyAsync(s->rowOff.p, s->rows.p, sizeof(uint32_t) * (nWorlds + 1), hipMemcpyDeviceToDevice, st));
+	pxh::launch_scan(s->rowOff.p, nWorlds + 1, s->scanTmp.p, st);
+	{
+		const uint32_t* src[1] = {s->rowOff.p + nWorlds};
+		int r = readCounts(st, hostCounts, src, 1);
+		if (r)
+			return r;
+	}
+	const uint32_t totalRows = hostCounts[0];
+
+	PXH_CHECK(s->keys.ensure(n));
+	PXH_CHECK(s->keys2.ensure(n));
+	PXH_CHECK(s->vals.ensure(n));
+	PXH_CHECK(s->vals2.ensure(n));
+	pxh::launch_keys(n, s->bmin.p, s->bmax.p, d_scene, s->gridF.p, s->gridI.p, s->rowOff.p, s->keys.p, s->vals.p, st);
+	const int nblocks = (n + 2047) / 2048;
+	PXH_CHECK(s->hist.ensure(size_t(256) * std::max(nblocks, 1)));
+	bool alt = false;
+	const int bitEnd = 32 + ((bitsFor(totalRows) + 7) / 8) * 8;
+	pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, n, 0, bitEnd, s->hist.p, s->scanTmp.p, st, alt);
+	uint64_t* sk = alt ? s->keys2.p : s->keys.p;
+	uint32_t* sv = alt ? s->vals2.p : s->vals.p;
+
+	PXH_CHECK(s->smin.ensure(n));
+	PXH_CHECK(s->smax.ensure(n));
+	PXH_CHECK(s->rowStart.ensure(size_t(totalRows) + 2));
+	pxh::launch_gather_sorted(n, sv, s->bmin.p, s->bmax.p, s->smin.p, s->smax.p, st);
+	pxh::launch_row_start(n, sk, s->rowStart.p, totalRows, st);
+
+	const int64_t nc = int64_t(3) * n + 1;
+	PXH_CHECK(s->counts.ensure(nc));
+	PXH_CHECK(hipMemsetAsync(s->counts.p + 3 * int64_t(n), 0, sizeof(uint32_t), st));
+	pxh::launch_sweep(false, n, sk, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, s->rowOff.p, s->rowStart.p,
+	                  s->counts.p, nullptr, nullptr, st);
+	pxh::launch_scan(s->counts.p, nc, s->scanTmp.p, st);
+	{
+		const uint32_t* src[3] = {s->counts.p + n, s->counts.p + 2 * int64_t(n), s->counts.p + 3 * int64_t(n)};
+		int r = readCounts(st, hostCounts, src, 3);
+		if (r)
+			return r;
+	}
+	cls[0] = 0;
+	cls[1] = hostCounts[0];
+	cls[2] = hostCounts[1];
+	cls[3] = hostCounts[2];
+	PXH_CHECK(s->pairs.ensure(std::max<size_t>(cls[3], 1)));
+	pxh::launch_sweep(true, n, sk, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, s->rowOff.p, s->rowStart.p,
+	                  nullptr, s->counts.p, s->pairs.p, st);
+	return PXH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t pxh_get_version(void) { return PXH_VERSION; }
+
+const char* pxh_last_error(void) { return gLastError.c_str(); }
+
+void pxh_default_scene_desc(pxh_scene_desc* d)
+{
+	if (!d)
+		return;
+	std::memset(d, 0, sizeof(*d));
+	d->gravity[1] = -9.81f;
+	d->solver_type = PXH_SOLVER_TGS;
+	d->position_iterations = 4;
+	d->velocity_iterations = 1;
+	d->contact_offset = 0.02f;
+	d->rest_offset = 0.0f;
+	d->static_friction = 0.5f;
+	d->dynamic_friction = 0.5f;
+	d->restitution = 0.0f;
+	d->linear_damping = 0.0f;
+	d->angular_damping = 0.05f;
+	d->max_linear_velocity_sq = 1e32f;
+	d->max_angular_velocity_sq = 100.0f * 100.0f;
+	d->bounce_threshold_velocity = 2.0f;
+	d->friction_offset_threshold = 0.04f;
+	d->friction_correlation_distance = 0.025f;
+	d->tolerance_length = 1.0f;
+	d->max_pairs_per_body = 12;
+}
+
+int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
+{
+	if (!desc || !out)
+		return fail(PXH_ERR_INVALID, "pxh_scene_create: null argument");
+	if (desc->solver_type != PXH_SOLVER_TGS)
+		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: only the TGS solver is implemented");
+	if (desc->position_iterations < 1 || desc->velocity_iterations < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_create: iteration counts out of range");
+	if (desc->restitution != 0.0f)
+		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: restitution other than 0 is not implemented");
+	int dev = 0;
+	PXH_CHECK(hipGetDevice(&dev));
+	pxh_scene_t* s = new pxh_scene_t();
+	s->desc = *desc;
+	if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+	    hipHostMalloc(reinterpret_cast<void**>(&s->hostCounts), 16384, hipHostMallocDefault) != hipSuccess)
+	{
+		delete s;
+		return fail(PXH_ERR_DEVICE, "pxh_scene_create: stream or pinned buffer creation failed");
+	}
+	*out = s;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_release(pxh_scene s)
+{
+	if (!s)
+		return PXH_OK;
+	(void)hipStreamSynchronize(s->stream);
+	DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->planes, &s->bmin, &s->bmax,
+	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbLin, &s->sbAng, &s->sbDLin,
+	                       &s->sbDAng, &s->sbP, &s->sbDQ, &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF, &s->rhG,
+	                       &s->rnA, &s->rnB, &s->rnC, &s->rfT, &s->rfA, &s->rfB};
+	for (auto* b : f4s)
+		b->release();
+	DBuf<int>* is[] = {&s->scene, &s->local, &s->bodyStart, &s->planeLocal, &s->qBody, &s->qPlane, &s->conA,
+	                   &s->conB, &s->conSlot, &s->conColor, &s->order};
+	for (auto* b : is)
+		b->release();
+	DBuf<uint32_t>* us[] = {&s->rows, &s->rowOff, &s->rowStart, &s->vals, &s->vals2, &s->hist, &s->scanTmp,
+	                        &s->counts, &s->flags, &s->colorCount, &s->scratch};
+	for (auto* b : us)
+		b->release();
+	s->keys.release();
+	s->keys2.release();
+	s->conKey.release();
+	s->best.release();
+	s->pairs.release();
+	s->gridI.release();
+	s->rhI.release();
+	if (s->hostCounts)
+		(void)hipHostFree(s->hostCounts);
+	if (s->stream)
+		(void)hipStreamDestroy(s->stream);
+	delete s;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const float* dims, const float* pose,
+                             const float* lin_vel, const float* ang_vel, const float* mass, const float* inertia,
+                             const int32_t* world)
+{
+	if (!s || n < 0 || (n > 0 && (!type || !dims || !pose || !mass || !inertia)))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_bodies: not allowed while simulating");
+	const int base = s->n;
+	const int lastWorld = s->hScene.empty() ? 0 : s->hScene.back();
+	std::vector<float4> hp(n), hq(n), hl(n), ha(n), hg(n), hi(n);
+	std::vector<int> hw(n), hloc(n);
+	for (int i = 0; i < n; i++)
+	{
+		const int w = world ? world[i] : 0;
+		if (w < lastWorld || (i > 0 && w < hw[i - 1]) || w < 0)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: world ids must be non-decreasing");
+		if (type[i] != PXH_GEOM_SPHERE && type[i] != PXH_GEOM_BOX)
+			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_bodies: only sphere and box shapes are implemented");
+		if (!(mass[i] > 0.0f))
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: mass must be positive");
+		hw[i] = w;
+		const float* p = pose + 7 * i;
+		hp[i] = make_float4(p[0], p[1], p[2], 0.0f);
+		hq[i] = make_float4(p[3], p[4], p[5], p[6]);
+		const float im = 1.0f / mass[i];
+		hl[i] = make_float4(lin_vel ? lin_vel[3 * i] : 0.0f, lin_vel ? lin_vel[3 * i + 1] : 0.0f,
+		                    lin_vel ? lin_vel[3 * i + 2] : 0.0f, im);
+		ha[i] = make_float4(ang_vel ? ang_vel[3 * i] : 0.0f, ang_vel ? ang_vel[3 * i + 1] : 0.0f,
+		                    ang_vel ? ang_vel[3 * i + 2] : 0.0f, 0.0f);
+		hg[i] = make_float4(dims[4 * i], dims[4 * i + 1], dims[4 * i + 2], float(type[i]));
+		const float* I = inertia + 3 * i;
+		hi[i] = make_float4(I[0] > 0.0f ? 1.0f / I[0] : 0.0f, I[1] > 0.0f ? 1.0f / I[1] : 0.0f,
+		                    I[2] > 0.0f ? 1.0f / I[2] : 0.0f, 0.0f);
+	}
+	// world body ranges and scene-local indices
+	for (int i = 0; i < n; i++)
+	{
+		const int w = hw[i];
+		while (int(s->hBodyStart.size()) - 1 <= w)
+			s->hBodyStart.push_back(s->hBodyStart.back());
+		hloc[i] = s->hBodyStart[w + 1] - s->hBodyStart[w];
+		s->hBodyStart[w + 1]++;
+		s->hScene.push_back(w);
+	}
+	const size_t nn = size_t(base) + n;
+	hipStream_t st = s->stream;
+	PXH_CHECK(s->pos.grow_keep(nn, base, st));
+	PXH_CHECK(s->rot.grow_keep(nn, base, st));
+	PXH_CHECK(s->lin.grow_keep(nn, base, st));
+	PXH_CHECK(s->ang.grow_keep(nn, base, st));
+	PXH_CHECK(s->geom.grow_keep(nn, base, st));
+	PXH_CHECK(s->invI.grow_keep(nn, base, st));
+	PXH_CHECK(s->scene.grow_keep(nn, base, st));
+	PXH_CHECK(s->local.grow_keep(nn, base, st));
+	if (n > 0)
+	{
+		PXH_CHECK(hipMemcpy(s->pos.p + base, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->rot.p + base, hq.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->lin.p + base, hl.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->ang.p + base, ha.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->geom.p + base, hg.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->invI.p + base, hi.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->scene.p + base, hw.data(), n * sizeof(int), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->local.p + base, hloc.data(), n * sizeof(int), hipMemcpyHostToDevice));
+	}
+	s->n = int(nn);
+	s->topologyDirty = true;
+	return base;
+}
+
+int32_t pxh_scene_add_planes(pxh_scene s, int32_t m, const float* plane, const int32_t* world)
+{
+	if (!s || m < 0 || (m > 0 && !plane))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_planes: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_planes: not allowed while simulating");
+	std::vector<float4> hp(m);
+	std::vector<int> hloc(m);
+	for (int i = 0; i < m; i++)
+	{
+		const int w = world ? world[i] : 0;
+		if (w < 0)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_planes: negative world id");
+		const float* p = plane + 4 * i;
+		const float l = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+		if (!(l > 0.0f))
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_planes: zero normal");
+		hp[i] = make_float4(p[0] / l, p[1] / l, p[2] / l, p[3] / l);
+		if (int(s->worldPlanes.size()) <= w)
+			s->worldPlanes.resize(w + 1);
+		hloc[i] = int(s->worldPlanes[w].size());
+		s->worldPlanes[w].push_back(s->m + i);
+		s->hPlaneScene.push_back(w);
+	}
+	const size_t nm = size_t(s->m) + m;
+	PXH_CHECK(s->planes.grow_keep(nm, s->m, s->stream));
+	PXH_CHECK(s->planeLocal.grow_keep(nm, s->m, s->stream));
+	if (m > 0)
+	{
+		PXH_CHECK(hipMemcpy(s->planes.p + s->m, hp.data(), m * sizeof(float4), hipMemcpyHostToDevice));
+		PXH_CHECK(hipMemcpy(s->planeLocal.p + s->m, hloc.data(), m * sizeof(int), hipMemcpyHostToDevice));
+	}
+	s->m = int(nm);
+	s->topologyDirty = true;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_simulate(pxh_scene s, float dt)
+{
+	if (!s || !(dt > 0.0f))
+		return fail(PXH_ERR_INVALID, "pxh_scene_simulate: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_simulate: previous step not fetched");
+	if (s->topologyDirty)
+	{
+		int r = rebuild_topology(s);
+		if (r)
+			return r;
+	}
+	const pxh_scene_desc& d = s->desc;
+	hipStream_t st = s->stream;
+	const int n = s->n;
+	const int nWorlds = int(s->hBodyStart.size()) - 1;
+	s->statSteps++;
+	s->statPairs = s->statPlanePairs = s->statConstraints = s->statContacts = s->statColors = 0;
+	if (n == 0)
+		return PXH_OK;
+	const float co = d.contact_offset, cd = 2.0f * co;
+
+	// 1. bounds + broadphase
+	PXH_CHECK(s->bmin.ensure(n));
+	PXH_CHECK(s->bmax.ensure(n));
+	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, co, s->bmin.p, s->bmax.p, st);
+	uint32_t cls[4];
+	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->bodyStart.p, st, cls, s->hostCounts);
+	if (r)
+		return r;
+	const int P = int(cls[3]);
+	const int Q = s->Q;
+	const int S = P + Q;
+	s->statPairs = P;
+	s->statPlanePairs = Q;
+
+	// 2. narrowphase into per-slot contact sets (slots: body pairs, then body-plane pairs)
+	PXH_CHECK(s->ctN.ensure(std::max(S, 1)));
+	PXH_CHECK(s->ctP.ensure(size_t(4) * std::max(S, 1)));
+	pxh::launch_np_pairs(P, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length, s->ctN.p, s->ctP.p, st);
+	pxh::launch_np_planes(Q, P, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p, cd, s->ctN.p,
+	                      s->ctP.p, st);
+
+	// 3. constraints = touching slots
+	PXH_CHECK(s->flags.ensure(size_t(S) + 1));
+	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
+	pxh::launch_touch_flags(S, s->ctN.p, s->flags.p, st);
+	PXH_CHECK(hipMemsetAsync(s->flags.p + S, 0, sizeof(uint32_t), st));
+	pxh::launch_scan(s->flags.p, int64_t(S) + 1, s->scanTmp.p, st);
+	{
+		const uint32_t* src[1] = {s->flags.p + S};
+		r = readCounts(st, s->hostCounts, src, 1);
+		if (r)
+			return r;
+	}
+	const int C = int(s->hostCounts[0]);
+	s->statConstraints = C;
+	PXH_CHECK(s->conA.ensure(std::max(C, 1)));
+	PXH_CHECK(s->conB.ensure(std::max(C, 1)));
+	PXH_CHECK(s->conSlot.ensure(std::max(C, 1)));
+	PXH_CHECK(s->conColor.ensure(std::max(C, 1)));
+	PXH_CHECK(s->conKey.ensure(std::max(C, 1)));
+	PXH_CHECK(s->order.ensure(std::max(C, 1)));
+	pxh::launch_con_build(S, P, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
+	                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
+
+	// 4. colouring
+	PXH_CHECK(s->best.ensure(n));
+	PXH_CHECK(s->scratch.ensure(4));
+	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, sizeof(uint32_t), st));
+	int rounds = 0;
+	const int kMaxColors = 1024;
+	uint32_t colored = 0;
+	while (int(colored) < C)
+	{
+		for (int k = 0; k < 4 && rounds < kMaxColors; k++, rounds++)
+			pxh::launch_color_round(C, n, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p, s->best.p,
+			                        s->scratch.p, st);
+		const uint32_t* src[1] = {s->scratch.p};
+		r = readCounts(st, s->hostCounts, src, 1);
+		if (r)
+			return r;
+		colored = s->hostCounts[0];
+		if (rounds >= kMaxColors && int(colored) < C)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring exceeded 1024 colours");
+	}
+	PXH_CHECK(s->colorCount.ensure(size_t(rounds) + 1));
+	PXH_CHECK(hipMemsetAsync(s->colorCount.p, 0, sizeof(uint32_t) * (rounds + 1), st));
+	pxh::launch_color_count(C, s->conColor.p, s->colorCount.p, st);
+	std::vector<uint32_t> colorStart(rounds + 1, 0);
+	if (rounds > 0)
+	{
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts, s->colorCount.p, sizeof(uint32_t) * rounds, hipMemcpyDeviceToHost, st));
+		PXH_CHECK(hipStreamSynchronize(st));
+		// the colour count array is small; trailing empty colours (rounds after
+		// the last colouring) are dropped
+		for (int c = 0; c < rounds; c++)
+			colorStart[c + 1] = colorStart[c] + s->hostCounts[c];
+		while (rounds > 0 && s->hostCounts[rounds - 1] == 0)
+			rounds--;
+		// stage the colour offsets in pinned memory (the copy is asynchronous)
+		uint32_t* pinnedStart = s->hostCounts + 2048;
+		std::memcpy(pinnedStart, colorStart.data(), sizeof(uint32_t) * (rounds + 1));
+		PXH_CHECK(hipMemcpyAsync(s->colorCount.p, pinnedStart, sizeof(uint32_t) * (rounds + 1),
+		                         hipMemcpyHostToDevice, st));
+		pxh::launch_color_order(C, s->conColor.p, s->colorCount.p, s->order.p, st);
+	}
+	s->statColors = rounds;
+
+	// 5. solver bodies + contact rows
+	PXH_CHECK(s->sbLin.ensure(n)); PXH_CHECK(s->sbAng.ensure(n)); PXH_CHECK(s->sbDLin.ensure(n));
+	PXH_CHECK(s->sbDAng.ensure(n)); PXH_CHECK(s->sbP.ensure(n)); PXH_CHECK(s->sbDQ.ensure(n));
+	PXH_CHECK(s->sbI0.ensure(n)); PXH_CHECK(s->sbI1.ensure(n)); PXH_CHECK(s->sbOrig.ensure(n));
+	pxh::SolverBodies sb{s->sbLin.p, s->sbAng.p, s->sbDLin.p, s->sbDAng.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p,
+	                     s->sbOrig.p};
+	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
+	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
+	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, sb, st);
+	const size_t Cc = std::max(C, 1);
+	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc)); PXH_CHECK(s->rhG.ensure(Cc));
+	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnC.ensure(4 * Cc));
+	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
+	pxh::Rows rows{s->rhI.p, s->rhF.p, s->rhG.p, s->rnA.p, s->rnB.p, s->rnC.p, s->rfT.p, s->rfA.p, s->rfB.p};
+	const int P_ = d.position_iterations, V_ = d.velocity_iterations;
+	const float stepDt = dt / float(P_);
+	const float invStepDt = 1.0f / stepDt;
+	const float biasCoefficient = 2.0f * std::sqrt(1.0f / float(P_));
+	const float scale = std::min(0.8f, biasCoefficient);
+	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
+	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, st);
+
+	// 6. solve: position iterations by colour with substep integration, then
+	//    velocity iterations (DyTGSDynamics.cpp:2515-2790)
+	for (int it = 0; it < P_; it++)
+	{
+		for (int c = 0; c < rounds; c++)
+			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb, -3.402823466e+38f,
+			                  d.static_friction, d.dynamic_friction, st);
+		pxh::launch_integrate(n, stepDt, sb, st);
+	}
+	for (int it = 0; it < V_; it++)
+		for (int c = 0; c < rounds; c++)
+			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb, 0.0f, d.static_friction,
+			                  d.dynamic_friction, st);
+
+	// 7. write back poses and velocities
+	pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
+	PXH_CHECK(hipGetLastError());
+	s->simulating = true;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
+{
+	if (!s)
+		return fail(PXH_ERR_INVALID, "pxh_scene_fetch_results: null scene");
+	if (!s->simulating)
+		return PXH_OK;
+	if (!block)
+	{
+		hipError_t q = hipStreamQuery(s->stream);
+		if (q == hipErrorNotReady)
+			return 1;
+		if (q != hipSuccess)
+			return fail(PXH_ERR_DEVICE, hipGetErrorString(q));
+	}
+	else
+		PXH_CHECK(hipStreamSynchronize(s->stream));
+	s->simulating = false;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene s, void* data, const uint32_t* index, int32_t kind, int32_t n,
+                                         void* stream)
+{
+	if (!s || !data || n < 0 || kind < 0 || kind > 2)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_rigid_dynamic_data: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_rigid_dynamic_data: not allowed while simulating");
+	if (!index && n > s->n)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_rigid_dynamic_data: n exceeds body count");
+	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+	if (stream)
+	{
+		// order after the scene's stream
+		hipEvent_t ev;
+		PXH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		PXH_CHECK(hipEventRecord(ev, s->stream));
+		PXH_CHECK(hipStreamWaitEvent(st, ev, 0));
+		PXH_CHECK(hipEventDestroy(ev));
+	}
+	pxh::launch_get_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, static_cast<float*>(data), st);
+	PXH_CHECK(hipGetLastError());
+	if (!stream)
+		PXH_CHECK(hipStreamSynchronize(st));
+	return PXH_OK;
+}
+
+int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene s, const void* data, const uint32_t* index, int32_t kind, int32_t n,
+                                         void* stream)
+{
+	if (!s || !data || n < 0 || kind < 0 || kind > 2)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_rigid_dynamic_data: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_set_rigid_dynamic_data: not allowed while simulating");
+	if (!index && n > s->n)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_rigid_dynamic_data: n exceeds body count");
+	if (stream)
+	{
+		// the scene's stream waits for the caller's data
+		hipEvent_t ev;
+		PXH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		PXH_CHECK(hipEventRecord(ev, static_cast<hipStream_t>(stream)));
+		PXH_CHECK(hipStreamWaitEvent(s->stream, ev, 0));
+		PXH_CHECK(hipEventDestroy(ev));
+	}
+	pxh::launch_set_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, static_cast<const float*>(data),
+	                     s->stream);
+	PXH_CHECK(hipGetLastError());
+	PXH_CHECK(hipStreamSynchronize(s->stream));
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
+{
+	if (!s || !out || n < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
+	// contact points: sum of per-slot counts of the last step
+	if (s->statContacts == 0 && s->statConstraints > 0)
+	{
+		const int S = int(s->statPairs + s->statPlanePairs);
+		std::vector<float4> hn(S);
+		if (hipMemcpy(hn.data(), s->ctN.p, S * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess)
+		{
+			int64_t k = 0;
+			for (int i = 0; i < S; i++)
+			{
+				int c;
+				std::memcpy(&c, &hn[i].w, 4);
+				k += c > 0 ? c : 0;
+			}
+			s->statContacts = k;
+		}
+	}
+	const int64_t v[7] = {s->n, s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts, s->statColors,
+	                      s->statSteps};
+	const int k = std::min(n, 7);
+	for (int i = 0; i < k; i++)
+		out[i] = v[i];
+	return k;
+}
+
+int64_t pxh_scene_get_constraints(pxh_scene s, int32_t* out, int64_t maxc)
+{
+	if (!s || !out)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_constraints: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_constraints: not allowed while simulating");
+	const int64_t C = s->statConstraints;
+	const int64_t k = std::min(C, maxc);
+	if (k <= 0)
+		return C;
+	std::vector<int> a(k), b(k), col(k), slot(k);
+	PXH_CHECK(hipMemcpy(a.data(), s->conA.p, k * sizeof(int), hipMemcpyDeviceToHost));
+	PXH_CHECK(hipMemcpy(b.data(), s->conB.p, k * sizeof(int), hipMemcpyDeviceToHost));
+	PXH_CHECK(hipMemcpy(col.data(), s->conColor.p, k * sizeof(int), hipMemcpyDeviceToHost));
+	PXH_CHECK(hipMemcpy(slot.data(), s->conSlot.p, k * sizeof(int), hipMemcpyDeviceToHost));
+	std::vector<float4> hn(size_t(s->statPairs + s->statPlanePairs) + 1);
+	PXH_CHECK(hipMemcpy(hn.data(), s->ctN.p, (hn.size() - 1) * sizeof(float4), hipMemcpyDeviceToHost));
+	for (int64_t i = 0; i < k; i++)
+	{
+		int cnt;
+		std::memcpy(&cnt, &hn[slot[i]].w, 4);
+		out[4 * i] = a[i];
+		out[4 * i + 1] = b[i];
+		out[4 * i + 2] = col[i];
+		out[4 * i + 3] = cnt;
+	}
+	return C;
+}
+
+void* pxh_scene_get_stream(pxh_scene s) { return s ? static_cast<void*>(s->stream) : nullptr; }
+
+// ---------------------------------------------------------------------------
+// standalone broadphase
+// ---------------------------------------------------------------------------
+int32_t pxh_bp_create(int32_t capacity, pxh_broadphase* out)
+{
+	if (!out || capacity < 0)
+		return fail(PXH_ERR_INVALID, "pxh_bp_create: bad arguments");
+	pxh_broadphase_t* bp = new pxh_broadphase_t();
+	if (hipStreamCreateWithFlags(&bp->stream, hipStreamNonBlocking) != hipSuccess ||
+	    hipHostMalloc(reinterpret_cast<void**>(&bp->hostCounts), 4096, hipHostMallocDefault) != hipSuccess)
+	{
+		delete bp;
+		return fail(PXH_ERR_DEVICE, "pxh_bp_create: stream or pinned buffer creation failed");
+	}
+	if (capacity > 0)
+	{
+		(void)bp->pkPrev.ensure(capacity);
+		(void)bp->pkCur.ensure(capacity);
+	}
+	*out = bp;
+	return PXH_OK;
+}
+
+int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* groups, const float* distances,
+                      int32_t n, void* stream)
+{
+	if (!bp || n < 0 || (n > 0 && !bounds))
+		return fail(PXH_ERR_INVALID, "pxh_bp_update: bad arguments");
+	hipStream_t st = bp->stream;
+	if (stream)
+	{
+		hipEvent_t ev;
+		PXH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		PXH_CHECK(hipEventRecord(ev, static_cast<hipStream_t>(stream)));
+		PXH_CHECK(hipStreamWaitEvent(st, ev, 0));
+		PXH_CHECK(hipEventDestroy(ev));
+	}
+	const size_t nn = std::max(n, 1);
+	PXH_CHECK(bp->bmin.ensure(nn));
+	PXH_CHECK(bp->bmax.ensure(nn));
+	PXH_CHECK(bp->scene.ensure(nn));
+	PXH_CHECK(bp->group.ensure(nn));
+	PXH_CHECK(bp->bodyStart.ensure(2));
+	pxh::launch_bp_inflate(n, bounds, distances, groups, bp->bmin.p, bp->bmax.p, bp->scene.p, bp->group.p,
+	                       bp->bodyStart.p, st);
+	int P = 0;
+	if (n > 0)
+	{
+		uint32_t cls[4];
+		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->bodyStart.p, st, cls, bp->hostCounts);
+		if (r)
+			return r;
+		P = int(cls[3]);
+	}
+	// sorted pair keys of this update
+	std::swap(bp->pkPrev, bp->pkCur);
+	bp->nPrev = bp->nCur;
+	PXH_CHECK(bp->pkCur.ensure(std::max(P, 1)));
+	PXH_CHECK(bp->pkTmp.ensure(std::max(P, 1)));
+	PXH_CHECK(bp->vals.ensure(std::max(P, 1)));
+	PXH_CHECK(bp->vals2.ensure(std::max(P, 1)));
+	pxh::launch_pair_keys(P, bp->pairs.p, bp->pkCur.p, bp->vals.p, st);
+	const int nblocks = (P + 2047) / 2048;
+	PXH_CHECK(bp->hist.ensure(size_t(256) * std::max(nblocks, 1)));
+	PXH_CHECK(bp->scanTmp.ensure(
+	    pxh::scan_tmp_size(std::max<int64_t>(std::max<int64_t>(int64_t(256) * std::max(nblocks, 1), P + 1),
+	                                         int64_t(bp->nCur) + 1)) +
+	    1024));
+	bool alt = false;
+	const int bits = 32 + ((bitsFor(uint64_t(std::max(n, 1))) + 7) / 8) * 8;
+	pxh::launch_radix_sort(bp->pkCur.p, bp->vals.p, bp->pkTmp.p, bp->vals2.p, P, 0, bits, bp->hist.p, bp->scanTmp.p,
+	                       st, alt);
+	if (alt)
+		std::swap(bp->pkCur, bp->pkTmp);
+	bp->nCur = P;
+	// created = cur \ prev, deleted = prev \ cur
+	const int np = bp->nPrev;
+	PXH_CHECK(bp->flags.ensure(size_t(P) + 1));
+	PXH_CHECK(bp->flags2.ensure(size_t(np) + 1));
+	PXH_CHECK(bp->created.ensure(std::max(P, 1)));
+	PXH_CHECK(bp->deleted.ensure(std::max(np, 1)));
+	pxh::launch_set_diff(P, bp->pkCur.p, np, bp->pkPrev.p, bp->flags.p, st);
+	PXH_CHECK(hipMemsetAsync(bp->flags.p + P, 0, sizeof(uint32_t), st));
+	PXH_CHECK(bp->counts.ensure(size_t(P) + 1));
+	PXH_CHECK(hipMemcpyAsync(bp->counts.p, bp->flags.p, sizeof(uint32_t) * (P + 1), hipMemcpyDeviceToDevice, st));
+	pxh::launch_scan(bp->counts.p, int64_t(P) + 1, bp->scanTmp.p, st);
+	pxh::launch_compact_keys(P, bp->pkCur.p, bp->flags.p, bp->counts.p, bp->created.p, st);
+	PXH_CHECK(hipMemcpyAsync(bp->hostCounts, bp->counts.p + P, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+	pxh::launch_set_diff(np, bp->pkPrev.p, P, bp->pkCur.p, bp->flags2.p, st);
+	PXH_CHECK(hipMemsetAsync(bp->flags2.p + np, 0, sizeof(uint32_t), st));
+	PXH_CHECK(bp->rows.ensure(size_t(np) + 1));
+	PXH_CHECK(hipMemcpyAsync(bp->rows.p, bp->flags2.p, sizeof(uint32_t) * (np + 1), hipMemcpyDeviceToDevice, st));
+	pxh::launch_scan(bp->rows.p, int64_t(np) + 1, bp->scanTmp.p, st);
+	pxh::launch_compact_keys(np, bp->pkPrev.p, bp->flags2.p, bp->rows.p, bp->deleted.p, st);
+	PXH_CHECK(hipMemcpyAsync(bp->hostCounts + 1, bp->rows.p + np, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+	PXH_CHECK(hipStreamSynchronize(st));
+	PXH_CHECK(hipGetLastError());
+	bp->nCreated = int(bp->hostCounts[0]);
+	bp->nDeleted = int(bp->hostCounts[1]);
+	bp->n = n;
+	return PXH_OK;
+}
+
+int32_t pxh_bp_fetch_results(pxh_broadphase bp, const uint32_t** created, int32_t* n_created, const uint32_t** deleted,
+                             int32_t* n_deleted)
+{
+	if (!bp)
+		return fail(PXH_ERR_INVALID, "pxh_bp_fetch_results: null broadphase");
+	if (created) *created = reinterpret_cast<const uint32_t*>(bp->created.p);
+	if (n_created) *n_created = bp->nCreated;
+	if (deleted) *deleted = reinterpret_cast<const uint32_t*>(bp->deleted.p);
+	if (n_deleted) *n_deleted = bp->nDeleted;
+	return PXH_OK;
+}
+
+int32_t pxh_bp_release(pxh_broadphase bp)
+{
+	if (!bp)
+		return PXH_OK;
+	(void)hipStreamSynchronize(bp->stream);
+	bp->bmin.release(); bp->bmax.release(); bp->smin.release(); bp->smax.release(); bp->gridF.release();
+	bp->gridI.release(); bp->scene.release(); bp->group.release(); bp->bodyStart.release();
+	bp->rows.release(); bp->rowOff.release(); bp->rowStart.release(); bp->vals.release(); bp->vals2.release();
+	bp->hist.release(); bp->scanTmp.release(); bp->counts.release(); bp->flags.release(); bp->flags2.release();
+	bp->keys.release(); bp->keys2.release(); bp->pkPrev.release(); bp->pkCur.release(); bp->pkTmp.release();
+	bp->pairs.release(); bp->created.release(); bp->deleted.release();
+	if (bp->hostCounts)
+		(void)hipHostFree(bp->hostCounts);
+	if (bp->stream)
+		(void)hipStreamDestroy(bp->stream);
+	delete bp;
+	return PXH_OK;
+}
+
+int32_t pxh_generate_contacts(int32_t n, const int32_t* type0, const float* geom0, const float* pose0,
+                              const int32_t* type1, const float* geom1, const float* pose1, float contact_distance,
+                              float tolerance_length, float* out, int32_t* counts, void* stream)
+{
+	if (n < 0 || (n > 0 && (!type0 || !geom0 || !pose0 || !type1 || !geom1 || !pose1 || !out || !counts)))
+		return fail(PXH_ERR_INVALID, "pxh_generate_contacts: bad arguments");
+	hipStream_t st = static_cast<hipStream_t>(stream);
+	pxh::launch_np_immediate(n, type0, geom0, pose0, type1, geom1, pose1, contact_distance, tolerance_length, out,
+	                         counts, st);
+	PXH_CHECK(hipGetLastError());
+	if (!stream)
+		PXH_CHECK(hipStreamSynchronize(st));
+	return PXH_OK;
+}
+
+}  // extern "C"

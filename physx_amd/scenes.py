@@ -1,0 +1,192 @@
+"""Seeded synthetic scene generators for the rigid-body step workloads.
+
+Every random draw comes from a counter-based SplitMix64 stream keyed by
+(master seed, object id, draw index), so a scene is a pure function of its
+parameters and does not depend on library-specific random distributions.
+The same arrays feed the GPU library and the reference CPU pipeline.
+
+Geometry type codes follow PxGeometryType (physx/include/geometry/PxGeometry.h):
+0 sphere, 1 plane, 2 capsule, 3 box.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+SPHERE, PLANE, CAPSULE, BOX = 0, 1, 2, 3
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def _splitmix(x: np.ndarray) -> np.ndarray:
+    with np.errstate(over="ignore"):
+        z = x.astype(np.uint64)
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform(seed: int, ids: np.ndarray, draw: int, lo=0.0, hi=1.0) -> np.ndarray:
+    """U[lo,hi) float64 for each object id, draw index `draw` of its stream."""
+    with np.errstate(over="ignore"):
+        ctr = (np.asarray(ids, np.uint64) * np.uint64(64) + np.uint64(draw + 1)) * _GOLDEN
+        x = _splitmix(np.uint64(seed) ^ ctr)
+    u = (x >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+@dataclass
+class Scene:
+    """Flat body/plane arrays of one or more independent scenes (worlds)."""
+    shape_type: np.ndarray          # (n,) int32, SPHERE or BOX
+    dims: np.ndarray                # (n,4) f32: radius or half extents
+    pose: np.ndarray                # (n,7) f32: p xyz, q xyzw
+    lin_vel: np.ndarray             # (n,3) f32
+    ang_vel: np.ndarray             # (n,3) f32
+    mass: np.ndarray                # (n,) f32
+    inertia: np.ndarray             # (n,3) f32, mass-space diagonal
+    scene_id: np.ndarray            # (n,) int32 world index of each body
+    planes: np.ndarray              # (m,4) f32: n xyz, d with n.x + d = 0
+    plane_scene: np.ndarray         # (m,) int32 world index of each plane
+    n_scenes: int = 1
+    name: str = ""
+    params: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.shape_type.shape[0])
+
+    def subset(self, world: int) -> "Scene":
+        """The bodies and planes of one world, renumbered as world 0."""
+        b = self.scene_id == world
+        p = self.plane_scene == world
+        return Scene(self.shape_type[b].copy(), self.dims[b].copy(), self.pose[b].copy(),
+                     self.lin_vel[b].copy(), self.ang_vel[b].copy(), self.mass[b].copy(),
+                     self.inertia[b].copy(), np.zeros(int(b.sum()), np.int32),
+                     self.planes[p].copy(), np.zeros(int(p.sum()), np.int32), 1,
+                     f"{self.name}[{world}]", dict(self.params))
+
+
+def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
+    """Mass and mass-space inertia of solid spheres and boxes (closed forms)."""
+    dims = dims.astype(np.float64)
+    n = shape_type.shape[0]
+    mass = np.zeros(n)
+    inertia = np.zeros((n, 3))
+    s = shape_type == SPHERE
+    r = dims[s, 0]
+    mass[s] = density * (4.0 / 3.0) * math.pi * r ** 3
+    inertia[s] = (0.4 * mass[s] * r * r)[:, None]
+    b = shape_type == BOX
+    h = dims[b, :3]
+    mass[b] = density * 8.0 * h[:, 0] * h[:, 1] * h[:, 2]
+    m3 = mass[b] / 3.0
+    inertia[b, 0] = m3 * (h[:, 1] ** 2 + h[:, 2] ** 2)
+    inertia[b, 1] = m3 * (h[:, 0] ** 2 + h[:, 2] ** 2)
+    inertia[b, 2] = m3 * (h[:, 0] ** 2 + h[:, 1] ** 2)
+    return mass.astype(np.float32), inertia.astype(np.float32)
+
+
+def _yaw_quat(yaw: np.ndarray) -> np.ndarray:
+    q = np.zeros((yaw.shape[0], 4))
+    q[:, 1] = np.sin(0.5 * yaw)
+    q[:, 3] = np.cos(0.5 * yaw)
+    return q
+
+
+def _finish(shape_type, dims, pos, quat, scene_id, planes, plane_scene, n_scenes, name, params, density=1000.0):
+    n = shape_type.shape[0]
+    mass, inertia = mass_props(shape_type, dims, density)
+    pose = np.concatenate([pos, quat], 1).astype(np.float32)
+    return Scene(shape_type.astype(np.int32), dims.astype(np.float32), pose,
+                 np.zeros((n, 3), np.float32), np.zeros((n, 3), np.float32), mass, inertia,
+                 scene_id.astype(np.int32), np.asarray(planes, np.float32).reshape(-1, 4),
+                 np.asarray(plane_scene, np.int32), n_scenes, name, params)
+
+
+def box_grid_drop(nx=16, ny=16, nz=16, pitch=1.2, jitter=0.05, hmin=0.25, hmax=0.5, seed=1) -> Scene:
+    """4,096 boxes (16^3) with half extents U[0.25,0.5], random yaw, dropped
+    from a jittered 1.2 m grid onto the plane y=0."""
+    n = nx * ny * nz
+    ids = np.arange(n)
+    ix, iy, iz = ids % nx, (ids // nx) % ny, ids // (nx * ny)
+    dims = np.zeros((n, 4))
+    for k in range(3):
+        dims[:, k] = uniform(seed, ids, k, hmin, hmax)
+    pos = np.stack([(ix - (nx - 1) / 2) * pitch, hmax + 0.1 + iy * pitch, (iz - (nz - 1) / 2) * pitch], 1)
+    for k in range(3):
+        pos[:, k] += uniform(seed, ids, 3 + k, -jitter, jitter)
+    quat = _yaw_quat(uniform(seed, ids, 6, 0.0, 2 * math.pi))
+    st = np.full(n, BOX)
+    return _finish(st, dims, pos, quat, np.zeros(n), [[0, 1, 0, 0]], [0], 1, "box_grid_drop",
+                   dict(nx=nx, ny=ny, nz=nz, pitch=pitch, jitter=jitter, seed=seed))
+
+
+def sphere_box_bin(n_bodies=262144, bin_half=20.0, rmin=0.1, rmax=0.3, pitch_xz=0.9, pitch_y=0.65,
+                   jitter=0.02, seed=2) -> Scene:
+    """Spheres (r U[0.1,0.3]) and boxes (half extents U[0.1,0.3], random yaw)
+    alternating on a jittered grid, poured into a walled bin: floor y=0 and
+    four walls at |x|,|z| = bin_half."""
+    per_row = int((2 * bin_half - 0.5) // pitch_xz)
+    n = n_bodies
+    ids = np.arange(n)
+    ix, iz, iy = ids % per_row, (ids // per_row) % per_row, ids // (per_row * per_row)
+    st = np.where((ix + iy + iz) % 2 == 0, SPHERE, BOX)
+    dims = np.zeros((n, 4))
+    r = uniform(seed, ids, 0, rmin, rmax)
+    dims[:, 0] = r
+    for k in range(3):
+        hk = uniform(seed, ids, 1 + k, rmin, rmax)
+        dims[:, k] = np.where(st == BOX, hk, dims[:, k])
+    pos = np.stack([(ix - (per_row - 1) / 2) * pitch_xz, rmax + 0.05 + iy * pitch_y,
+                    (iz - (per_row - 1) / 2) * pitch_xz], 1)
+    for k in range(3):
+        pos[:, k] += uniform(seed, ids, 4 + k, -jitter, jitter)
+    quat = _yaw_quat(np.where(st == BOX, uniform(seed, ids, 7, 0.0, 2 * math.pi), 0.0))
+    planes = [[0, 1, 0, 0], [1, 0, 0, bin_half], [-1, 0, 0, bin_half], [0, 0, 1, bin_half], [0, 0, -1, bin_half]]
+    return _finish(st, dims, pos, quat, np.zeros(n), planes, [0] * 5, 1, "sphere_box_bin",
+                   dict(n_bodies=n_bodies, bin_half=bin_half, pitch_xz=pitch_xz, pitch_y=pitch_y, seed=seed))
+
+
+def batched_piles(n_scenes=8192, nmin=8, nmax=512, seed=4) -> Scene:
+    """Independent worlds with body counts log-uniform in [nmin, nmax]; even
+    worlds are box stacks (columns of boxes), odd worlds sphere/box piles on a
+    jittered grid. Each world has its own floor plane y=0."""
+    sid = np.arange(n_scenes)
+    counts = np.floor(np.exp(uniform(seed, sid, 0, math.log(nmin), math.log(nmax + 1)))).astype(np.int64)
+    counts = np.clip(counts, nmin, nmax)
+    total = int(counts.sum())
+    world = np.repeat(sid, counts)
+    first = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    local = np.arange(total) - np.repeat(first, counts)
+    gid = np.arange(total) + (seed << 32)
+    stack = (world % 2) == 0
+    # box stacks: columns of 8 boxes, columns on a 1.2 m grid
+    side = np.ceil(np.sqrt(np.ceil(counts / 8.0))).astype(np.int64)[world]
+    col = local // 8
+    lvl = local % 8
+    h = np.stack([uniform(seed, gid, 1 + k, 0.2, 0.3) for k in range(3)], 1)
+    st = np.where(stack, BOX, np.where(local % 2 == 0, SPHERE, BOX))
+    dims = np.zeros((total, 4))
+    dims[:, :3] = h
+    r = uniform(seed, gid, 4, 0.1, 0.3)
+    dims[:, 0] = np.where(st == SPHERE, r, dims[:, 0])
+    pos = np.zeros((total, 3))
+    # stacks: boxes centred on the column, each level resting 1 cm above the one below
+    pos[:, 0] = np.where(stack, (col % side - (side - 1) / 2.0) * 1.2, 0.0)
+    pos[:, 2] = np.where(stack, (col // side - (side - 1) / 2.0) * 1.2, 0.0)
+    pos[:, 1] = np.where(stack, 0.31 + lvl * 0.62, 0.0)
+    # piles: jittered grid at 0.9 m
+    pside = np.ceil(np.cbrt(counts)).astype(np.int64)[world]
+    px, pz, py = local % pside, (local // pside) % pside, local // (pside * pside)
+    for k, (a, c) in enumerate(((px, 0), (pz, 2))):
+        pos[:, c] = np.where(stack, pos[:, c], (a - (pside - 1) / 2.0) * 0.9 + uniform(seed, gid, 5 + k, -0.02, 0.02))
+    pos[:, 1] = np.where(stack, pos[:, 1], 0.35 + py * 0.9)
+    quat = _yaw_quat(np.where(st == BOX, uniform(seed, gid, 8, 0.0, 2 * math.pi), 0.0) * np.where(stack, 0.05, 1.0))
+    planes = np.tile(np.array([[0, 1, 0, 0]], np.float32), (n_scenes, 1))
+    return _finish(st, dims, pos, quat, world, planes, sid, n_scenes, "batched_piles",
+                   dict(n_scenes=n_scenes, nmin=nmin, nmax=nmax, seed=seed))

@@ -1,0 +1,235 @@
+"""ctypes bindings of the test-side checkers: the plain-C oracle
+(oracle/libpxo.so) and the reference PhysX CPU SDK harness
+(oracle/_ref/libpxref.so, built from the reference sources when present).
+Only tests, smoke() and bench.py's CPU-baseline leg use these."""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PXO_PATH = os.environ.get("PXO_LIB", os.path.join(ROOT, "oracle", "libpxo.so"))
+REF_PATH = os.path.join(ROOT, "oracle", "_ref", "libpxref.so")
+
+F = ctypes.POINTER(ctypes.c_float)
+I32 = ctypes.POINTER(ctypes.c_int32)
+U32 = ctypes.POINTER(ctypes.c_uint32)
+I64 = ctypes.POINTER(ctypes.c_longlong)
+D = ctypes.POINTER(ctypes.c_double)
+VP = ctypes.c_void_p
+
+
+def fp(a):
+    return a.ctypes.data_as(F)
+
+
+def ip(a):
+    return a.ctypes.data_as(I32)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class PxoDesc(ctypes.Structure):
+    _fields_ = [("gravity", ctypes.c_float * 3), ("solver", ctypes.c_int), ("pos_iters", ctypes.c_int),
+                ("vel_iters", ctypes.c_int), ("contact_offset", ctypes.c_float), ("rest_offset", ctypes.c_float),
+                ("static_friction", ctypes.c_float), ("dynamic_friction", ctypes.c_float),
+                ("restitution", ctypes.c_float), ("linear_damping", ctypes.c_float),
+                ("angular_damping", ctypes.c_float), ("max_lin_vel_sq", ctypes.c_float),
+                ("max_ang_vel_sq", ctypes.c_float), ("bounce_threshold", ctypes.c_float),
+                ("friction_offset_threshold", ctypes.c_float), ("correlation_distance", ctypes.c_float),
+                ("tolerance_length", ctypes.c_float)]
+
+
+_pxo = None
+_ref = None
+
+
+def pxo():
+    global _pxo
+    if _pxo is None:
+        lib = ctypes.CDLL(PXO_PATH)
+        lib.pxo_generate_contacts.restype = ctypes.c_int
+        lib.pxo_bp_pairs.restype = ctypes.c_longlong
+        lib.pxo_world_create.restype = VP
+        lib.pxo_world_step.restype = ctypes.c_int
+        lib.pxo_world_constraints.restype = ctypes.c_longlong
+        for name in ("pxo_world_release", "pxo_world_add_bodies", "pxo_world_add_planes", "pxo_world_step",
+                     "pxo_world_get_bodies", "pxo_world_set_bodies", "pxo_world_stats", "pxo_world_constraints"):
+            getattr(lib, name).argtypes = None
+        _pxo = lib
+    return _pxo
+
+
+def have_ref():
+    return os.path.exists(REF_PATH)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = ctypes.CDLL(REF_PATH)
+        lib.pxref_generate_contacts.restype = ctypes.c_int
+        lib.pxref_bp_pairs.restype = ctypes.c_longlong
+        lib.pxref_scene_create.restype = VP
+        lib.pxref_scene_step.restype = ctypes.c_double
+        lib.pxref_dynamic_group.restype = ctypes.c_uint32
+        lib.pxref_static_group.restype = ctypes.c_uint32
+        _ref = lib
+    return _ref
+
+
+def contacts(lib_name: str, t0, g0, p0, t1, g1, p1, cd=0.04, tol=1.0):
+    """(n,8) contacts: point xyz, separation, normal xyz, 0."""
+    out = np.zeros(64 * 8, np.float32)
+    g0 = f32(np.resize(np.asarray(g0, np.float32), 4))
+    g1 = f32(np.resize(np.asarray(g1, np.float32), 4))
+    p0, p1 = f32(p0), f32(p1)
+    if lib_name == "pxo":
+        n = pxo().pxo_generate_contacts(t0, fp(g0), fp(p0), t1, fp(g1), fp(p1), ctypes.c_float(cd),
+                                        ctypes.c_float(tol), fp(out))
+    else:
+        n = ref().pxref_generate_contacts(t0, fp(g0), fp(p0), t1, fp(g1), fp(p1), ctypes.c_float(cd),
+                                          ctypes.c_float(tol), fp(out), 64)
+    assert n >= 0
+    return out[: 8 * n].reshape(n, 8)
+
+
+def bp_pairs_pxo(bounds, dist, world=None):
+    b, d = f32(bounds), f32(dist)
+    n = b.shape[0]
+    w = i32(world) if world is not None else None
+    cap = max(1024, 64 * n)
+    out = np.zeros(2 * cap, np.uint32)
+    cnt = pxo().pxo_bp_pairs(fp(b), fp(d), ip(w) if w is not None else None, n, out.ctypes.data_as(U32),
+                             ctypes.c_longlong(cap))
+    assert cnt <= cap
+    return out[: 2 * cnt].reshape(-1, 2)
+
+
+def bp_pairs_ref(bounds, dist, groups=None, pabp=False):
+    lib = ref()
+    b, d = f32(bounds), f32(dist)
+    n = b.shape[0]
+    if groups is None:
+        groups = np.array([lib.pxref_dynamic_group(i) for i in range(n)], np.uint32)
+    g = np.ascontiguousarray(groups, np.uint32)
+    cap = max(1024, 64 * n)
+    out = np.zeros(2 * cap, np.uint32)
+    cnt = lib.pxref_bp_pairs(fp(b), fp(d), g.ctypes.data_as(U32), n, out.ctypes.data_as(U32),
+                             ctypes.c_longlong(cap), int(pabp))
+    assert 0 <= cnt <= cap
+    p = out[: 2 * cnt].reshape(-1, 2)
+    return p[np.lexsort((p[:, 1], p[:, 0]))]
+
+
+def default_desc(pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0)):
+    d = PxoDesc()
+    pxo().pxo_default_desc(ctypes.byref(d))
+    d.pos_iters, d.vel_iters = pos_iters, vel_iters
+    d.gravity[0], d.gravity[1], d.gravity[2] = gravity
+    return d
+
+
+class OracleWorld:
+    """The plain-C oracle's TGS world."""
+
+    def __init__(self, scene, pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0), friction=0.5):
+        self.lib = pxo()
+        self.desc = default_desc(pos_iters, vel_iters, gravity)
+        self.desc.static_friction = self.desc.dynamic_friction = friction
+        self.h = VP(self.lib.pxo_world_create(ctypes.byref(self.desc)))
+        self.n = scene.n
+        self._keep = [i32(scene.shape_type), f32(scene.dims), f32(scene.pose), f32(scene.lin_vel),
+                      f32(scene.ang_vel), f32(scene.mass), f32(scene.inertia), i32(scene.scene_id)]
+        t, dm, po, lv, av, m, inn, sc = self._keep
+        self.lib.pxo_world_add_bodies(self.h, self.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn), ip(sc))
+        pl, ps = f32(scene.planes), i32(scene.plane_scene)
+        self._keep += [pl, ps]
+        self.lib.pxo_world_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps))
+
+    def step(self, dt, n=1):
+        for _ in range(n):
+            self.lib.pxo_world_step(self.h, ctypes.c_float(dt))
+
+    def bodies(self):
+        pose = np.zeros((self.n, 7), np.float32)
+        lv = np.zeros((self.n, 3), np.float32)
+        av = np.zeros((self.n, 3), np.float32)
+        self.lib.pxo_world_get_bodies(self.h, fp(pose), fp(lv), fp(av))
+        return pose, lv, av
+
+    def set_bodies(self, pose, lv, av):
+        pose, lv, av = f32(pose), f32(lv), f32(av)
+        self.lib.pxo_world_set_bodies(self.h, fp(pose), fp(lv), fp(av))
+
+    def stats(self):
+        out = np.zeros(6, np.int64)
+        self.lib.pxo_world_stats(self.h, out.ctypes.data_as(I64))
+        return dict(bp_pairs=int(out[0]), plane_tests=int(out[1]), constraints=int(out[2]), contacts=int(out[3]),
+                    colors=int(out[4]), anchors=int(out[5]))
+
+    def constraints(self):
+        cap = 1 << 22
+        out = np.zeros((cap, 4), np.int32)
+        n = self.lib.pxo_world_constraints(self.h, ip(out), ctypes.c_longlong(cap))
+        return out[:n]
+
+    def __del__(self):
+        try:
+            self.lib.pxo_world_release(self.h)
+        except Exception:
+            pass
+
+
+class RefScene:
+    """The reference PhysX CPU PxScene (TGS or PGS) on the same arrays."""
+
+    def __init__(self, scene, solver="tgs", pos_iters=4, vel_iters=1, threads=8, gravity=(0.0, -9.81, 0.0),
+                 pabp=False, friction=0.5):
+        self.lib = ref()
+        g = f32(gravity)
+        self.h = VP(self.lib.pxref_scene_create(fp(g), 1 if solver == "tgs" else 0, int(pabp), int(threads)))
+        assert self.h.value
+        self.lib.pxref_scene_set_material(self.h, ctypes.c_float(friction), ctypes.c_float(friction),
+                                          ctypes.c_float(0.0))
+        self.n = scene.n
+        self._keep = [i32(scene.shape_type), f32(scene.dims), f32(scene.pose), f32(scene.lin_vel),
+                      f32(scene.ang_vel), f32(scene.mass), f32(scene.inertia)]
+        t, dm, po, lv, av, m, inn = self._keep
+        self.lib.pxref_scene_add_bodies(self.h, self.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn),
+                                        pos_iters, vel_iters, ctypes.c_float(0.02), ctypes.c_float(0.0),
+                                        ctypes.c_float(0.0), ctypes.c_float(0.05))
+        pl = f32(scene.planes)
+        self._keep.append(pl)
+        assert self.lib.pxref_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ctypes.c_float(0.02)) == 0
+
+    def step(self, dt, n=1):
+        ms = np.zeros(n, np.float64)
+        total = self.lib.pxref_scene_step(self.h, ctypes.c_float(dt), n, ms.ctypes.data_as(D))
+        return total, ms
+
+    def bodies(self):
+        pose = np.zeros((self.n, 7), np.float32)
+        lv = np.zeros((self.n, 3), np.float32)
+        av = np.zeros((self.n, 3), np.float32)
+        self.lib.pxref_scene_get_bodies(self.h, fp(pose), fp(lv), fp(av))
+        return pose, lv, av
+
+    def stats(self):
+        out = np.zeros(4, np.int64)
+        self.lib.pxref_scene_stats(self.h, out.ctypes.data_as(I64))
+        return dict(pairs=int(out[0]), bp_adds=int(out[1]), active=int(out[2]), touching=int(out[3]))
+
+    def __del__(self):
+        try:
+            self.lib.pxref_scene_release(self.h)
+        except Exception:
+            pass
