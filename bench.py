@@ -1,0 +1,164 @@
+"""Benchmark of the GPU rigid-body step (plain per-scene path).
+
+Workload (one GPU): 131,072 spheres (r ~ U[0.1,0.3] m) + 131,072 boxes (half
+extents ~ U[0.1,0.3] m) poured from a seeded jittered grid into a 40 x 40 m
+walled bin (floor + 4 wall planes), TGS with 8 position / 1 velocity
+iterations, dt = 1/60 s, float32. W untimed warm-up steps let the pile form,
+then K steps are timed (wall clock around simulate + fetchResults, bracketed by
+device synchronisation).
+
+Prints one JSON line with the body-steps/s, the roofline of the dominant
+kernel (the colour-partition solve, HBM-bound, timed with HIP events on the
+scene's stream over the timed steps) and the reference CPU pipeline timed on a
+bounded sample of the same state on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def solve_bytes_per_step(st: dict, iters: int) -> float:
+    """Algorithmic HBM bytes of all solve launches of one step: per constraint
+    and iteration a 48 B header, per contact point 48 B of rows read + 4 B
+    impulse written, per friction row 48 B read + 4 B written, per dynamic body
+    endpoint 64 B of velocity/motion state read + 32 B written."""
+    T = st["constraints"]
+    K = st["contacts"]
+    F = st["friction_rows"]
+    endpoints = T + st["body_body_constraints"]
+    return iters * (48.0 * T + 52.0 * K + 52.0 * F + 96.0 * endpoints)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--bodies", type=int, default=262144)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: every box of this pool has one MI355X, and the step of one scene "
+              "does not shard across GPUs in this build (see DESIGN.md for how it would). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import physx_amd as P
+    from physx_amd import scenes
+
+    assert torch.cuda.is_available()
+    torch.cuda.set_device(0)
+    dt = 1.0 / 60.0
+    pos_iters, vel_iters = 8, 1
+    sc = scenes.sphere_box_bin(n_bodies=args.bodies)
+    log(f"scene: {sc.n} bodies, {sc.planes.shape[0]} planes")
+    gpu = P.Scene(sc, position_iterations=pos_iters, velocity_iterations=vel_iters)
+
+    t0 = time.perf_counter()
+    for s in range(args.warmup):
+        gpu.step(dt)
+        if (s + 1) % 50 == 0:
+            log(f"warmup step {s + 1}/{args.warmup} ({time.perf_counter() - t0:.1f}s) {gpu.stats()}")
+    torch.cuda.synchronize()
+
+    gpu.set_profiling(True)
+    step_ms = []
+    sums = dict(constraints=0, contacts=0, friction_rows=0, body_body_constraints=0, bp_pairs=0, plane_pairs=0,
+                colors=0)
+    solve_bytes = 0.0
+    torch.cuda.synchronize()
+    t_begin = time.perf_counter()
+    for s in range(args.steps):
+        a = time.perf_counter()
+        gpu.simulate(dt)
+        gpu.fetch_results(True)
+        step_ms.append((time.perf_counter() - a) * 1e3)
+        st = gpu.stats()
+        for k in sums:
+            sums[k] += st[k]
+        solve_bytes += solve_bytes_per_step(st, pos_iters + vel_iters)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_begin
+    prof = gpu.profile()
+    gpu.set_profiling(False)
+    log(f"timed {args.steps} steps in {elapsed:.3f}s; stage GPU ms (total, launches): {prof}")
+
+    value = sc.n * args.steps / elapsed
+    solve_ms, solve_launches = prof.get("solve", (0.0, 0))
+    achieved = (solve_bytes / (solve_ms * 1e-3)) / 1e9 if solve_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": "k_solve", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "bytes_per_launch": round(solve_bytes / max(solve_launches, 1), 1),
+                "avg_launch_us": round(1e3 * solve_ms / max(solve_launches, 1), 3),
+                "launches": solve_launches}
+
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_libs as L
+        if L.have_ref():
+            pose, lv, av = gpu.bodies()
+            st = scenes.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
+                              sc.plane_scene, 1, sc.name)
+            log(f"cpu baseline: reference PxScene, {args.cpu_threads} threads, {args.cpu_steps} steps")
+            ref = L.RefScene(st, "tgs", pos_iters, vel_iters, threads=args.cpu_threads)
+            ref.step(dt, 1)  # untimed: first step creates every broadphase pair
+            total, ms = ref.step(dt, args.cpu_steps)
+            cpu = {"value": round(sc.n * args.cpu_steps / total, 1), "unit": "body-steps/s",
+                   "cores": args.cpu_threads, "kind": "reference",
+                   "sample": f"{args.cpu_steps} steps of the same {sc.n}-body pile state after {args.warmup} GPU "
+                             f"warm-up steps (+1 untimed step), reference PhysX 5.6.1 CPU SDK (ABP, PCM, TGS "
+                             f"{pos_iters}/{vel_iters}), {args.cpu_threads} dispatcher threads",
+                   "ms_per_step": round(1e3 * total / args.cpu_steps, 2)}
+            log(f"cpu baseline: {cpu}")
+
+    sms = np.array(step_ms)
+    out = {
+        "metric": "body-steps/sec at 262k bodies",
+        "value": round(value, 1),
+        "unit": "body-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "sphere_box_bin: 131072 spheres + 131072 boxes, 40x40 m walled bin, TGS 8/1, "
+                               "dt 1/60", "bodies": sc.n, "solver": "TGS", "position_iterations": pos_iters,
+                   "velocity_iterations": vel_iters, "dt": dt, "parallelism": "single GPU, single scene"},
+        "p50_step_ms": round(float(np.percentile(sms, 50)), 3),
+        "p99_step_ms": round(float(np.percentile(sms, 99)), 3),
+        "contact_pairs_per_s": round(sums["constraints"] / elapsed, 1),
+        "avg_counts": {k: round(v / args.steps, 1) for k, v in sums.items()},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -107,12 +107,23 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, cons
 /* PxScene::getSimulationStatistics (physx/include/PxSimulationStatistics.h):
  * out[0] bodies, [1] broadphase pairs (body-body), [2] body-plane pairs,
  * [3] touching pairs (constraints), [4] contact points, [5] colours,
- * [6] steps taken. Returns the number of values written (<= n). */
+ * [6] steps taken, [7] friction rows, [8] body-body constraints.
+ * Returns the number of values written (<= n). */
 int32_t pxh_scene_get_stats(pxh_scene scene, int64_t* out, int32_t n);
 
 /* Last step's constraint table, host copy for inspection:
  * per constraint (body a, body b or -1-plane, colour, contact count). */
 int64_t pxh_scene_get_constraints(pxh_scene scene, int32_t* out, int64_t max_constraints);
+
+/* Per-stage GPU time from HIP events recorded on the scene's stream around
+ * each launch, the device-side counterpart of the reference's profiler zones
+ * (PX_PROFILE_ZONE, physx/include/common/PxProfileZone.h:34-50). Stages:
+ * 0 bounds, 1 broadphase (untimed), 2 body-pair narrowphase, 3 body-plane
+ * narrowphase, 4 contact prep, 5 solve (one colour per launch), 6 integration.
+ * ms[i] accumulates milliseconds and launches[i] the timed launches since
+ * profiling was (re)enabled. */
+int32_t pxh_scene_set_profiling(pxh_scene scene, int32_t enable);
+int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, int32_t n);
 
 /* The scene's HIP stream (hipStream_t), for ordering caller work. */
 void* pxh_scene_get_stream(pxh_scene scene);

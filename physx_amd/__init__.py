@@ -49,7 +49,8 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_add_bodies", "pxh_scene_add_planes", "pxh_scene_simulate", "pxh_scene_fetch_results",
             "pxh_scene_get_rigid_dynamic_data", "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats",
             "pxh_scene_get_constraints", "pxh_scene_get_stream", "pxh_bp_create", "pxh_bp_update",
-            "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts")
+            "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts", "pxh_scene_set_profiling",
+            "pxh_scene_get_profile")
 
 _lib = None
 
@@ -74,7 +75,8 @@ def load():
     for name in ("pxh_scene_create", "pxh_scene_release", "pxh_scene_add_bodies", "pxh_scene_add_planes",
                  "pxh_scene_simulate", "pxh_scene_fetch_results", "pxh_scene_get_rigid_dynamic_data",
                  "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats", "pxh_bp_create", "pxh_bp_update",
-                 "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts"):
+                 "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts", "pxh_scene_set_profiling",
+            "pxh_scene_get_profile"):
         getattr(lib, name).restype = ctypes.c_int32
     _lib = lib
     return lib
@@ -188,10 +190,23 @@ class Scene:
         self.set_rigid_dynamic_data(torch.from_numpy(_f32(ang_vel)).to(dev), PXH_DATA_ANGULAR_VELOCITY)
 
     def stats(self) -> dict:
-        out = np.zeros(7, np.int64)
-        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 7))
-        keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps")
+        out = np.zeros(9, np.int64)
+        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 9))
+        keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps", "friction_rows",
+                "body_body_constraints")
         return dict(zip(keys, map(int, out)))
+
+    STAGES = ("bounds", "broadphase", "np_pairs", "np_planes", "prep", "solve", "integrate", "unused")
+
+    def set_profiling(self, enable: bool):
+        _check(self.lib.pxh_scene_set_profiling(self.h, int(enable)))
+
+    def profile(self) -> dict:
+        ms = np.zeros(8, np.float64)
+        cnt = np.zeros(8, np.int64)
+        _check(self.lib.pxh_scene_get_profile(self.h, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                              cnt.ctypes.data_as(_I64), 8))
+        return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(self.STAGES) if cnt[i] > 0}
 
     def constraints(self) -> np.ndarray:
         cap = 1 << 23

@@ -69,7 +69,7 @@ void launch_preintegrate(int n, const float4* pos, const float4* rot, const floa
                          float maxAngSq, SolverBodies sb, hipStream_t st);
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
-                 float scale, float restDist, float fot, Rows rows, hipStream_t st);
+                 float scale, float restDist, float fot, Rows rows, unsigned long long* counters, hipStream_t st);
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   hipStream_t st);
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);

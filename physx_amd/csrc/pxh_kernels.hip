@@ -652,6 +652,11 @@ PXH_D m33 loadSym(float4 a, float4 b)
 	return m;
 }
 
+PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
+                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
+                    float restDist, float frictionOffsetThreshold, Rows& rows, uint32_t& nFricOut);
+
 // Contact rows of one constraint at its colour-ordered position o
 // (setupFinalizeSolverConstraints / constructContactConstraintStep,
 //  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction anchors
@@ -659,9 +664,34 @@ PXH_D m33 loadSym(float4 a, float4 b)
 __global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                        const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
                        const float4* svLin, const float4* svOrigLin, const float4* svI0, const float4* svI1,
-                       float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows)
+                       float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
+                       unsigned long long* counters)
 {
 	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	// per-step counters: contact points, friction rows, dynamic-dynamic constraints
+	uint32_t myCnt = 0, myFric = 0, myDyn = 0;
+	if (o < C)
+	{
+		const int c = order[o];
+		myCnt = uint32_t(__float_as_int(ctN[conSlot[c]].w));
+		myDyn = conB[c] >= 0 ? 1u : 0u;
+	}
+	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, svLin, svOrigLin, svI0, svI1, invStepDt, scale,
+	         restDist, frictionOffsetThreshold, rows, myFric);
+	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
+	if ((threadIdx.x & 63) == 63)
+	{
+		atomicAdd(&counters[0], (unsigned long long)sc);
+		atomicAdd(&counters[1], (unsigned long long)sf);
+		atomicAdd(&counters[2], (unsigned long long)sd);
+	}
+}
+
+PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
+                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
+                    float restDist, float frictionOffsetThreshold, Rows& rows, uint32_t& nFricOut)
+{
 	if (o >= C)
 		return;
 	const int c = order[o];
@@ -731,6 +761,7 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 		}
 	}
 	const float fscale = nAnch == 2 ? 0.5f : 1.0f;
+	nFricOut = uint32_t(2 * nAnch);
 	rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), 0);
 	rows.hF[o] = make_float4(n.x, n.y, n.z, invMassA);
 	rows.hG[o] = make_float4(invMassB, fscale, invStepDt * scale, -1e32f);
@@ -1182,10 +1213,10 @@ void launch_preintegrate(int n, const float4* pos, const float4* rot, const floa
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
-                 float scale, float restDist, float fot, Rows rows, hipStream_t st)
+                 float scale, float restDist, float fot, Rows rows, unsigned long long* counters, hipStream_t st)
 {
 	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.lin, sb.origLin, sb.i0, sb.i1,
-	           invStepDt, scale, restDist, fot, rows);
+	           invStepDt, scale, restDist, fot, rows, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   hipStream_t st)

@@ -132,11 +132,54 @@ struct pxh_scene_t
 	DBuf<int> conA, conB, conSlot, conColor, order;
 	DBuf<uint64_t> conKey, best;
 	DBuf<uint32_t> colorCount, scratch;
+	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
+	int64_t statFriction = 0, statDynCon = 0;
 	pxh::SolverBodies sb{};
 	DBuf<float4> sbLin, sbAng, sbDLin, sbDAng, sbP, sbDQ, sbI0, sbI1, sbOrig;
 	DBuf<int4> rhI;
 	DBuf<float4> rhF, rhG, rnA, rnB, rnC, rfT, rfA, rfB;
 	uint32_t* hostCounts = nullptr;       // pinned
+
+	// optional per-stage GPU timing with HIP events on the scene stream
+	struct ProfPair { hipEvent_t a, b; int stage; };
+	bool profiling = false;
+	std::vector<ProfPair> prof;
+	size_t profUsed = 0;
+	double profMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	int64_t profLaunches[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	int profBegin(int stage)
+	{
+		if (!profiling)
+			return -1;
+		if (profUsed == prof.size())
+		{
+			ProfPair p{};
+			if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess)
+				return -1;
+			prof.push_back(p);
+		}
+		prof[profUsed].stage = stage;
+		(void)hipEventRecord(prof[profUsed].a, stream);
+		return int(profUsed++);
+	}
+	void profEnd(int k)
+	{
+		if (k >= 0)
+			(void)hipEventRecord(prof[k].b, stream);
+	}
+	void profCollect()
+	{
+		for (size_t k = 0; k < profUsed; k++)
+		{
+			float ms = 0.0f;
+			if (hipEventElapsedTime(&ms, prof[k].a, prof[k].b) == hipSuccess)
+			{
+				profMs[prof[k].stage] += ms;
+				profLaunches[prof[k].stage]++;
+			}
+		}
+		profUsed = 0;
+	}
 
 	// statistics of the last step
 	int64_t statPairs = 0, statPlanePairs = 0, statConstraints = 0, statContacts = 0, statColors = 0, statSteps = 0;
@@ -486,6 +529,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const int nWorlds = int(s->hBodyStart.size()) - 1;
 	s->statSteps++;
 	s->statPairs = s->statPlanePairs = s->statConstraints = s->statContacts = s->statColors = 0;
+	s->statFriction = s->statDynCon = 0;
 	if (n == 0)
 		return PXH_OK;
 	const float co = d.contact_offset, cd = 2.0f * co;
@@ -493,7 +537,9 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// 1. bounds + broadphase
 	PXH_CHECK(s->bmin.ensure(n));
 	PXH_CHECK(s->bmax.ensure(n));
+	int pk = s->profBegin(0);
 	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, co, s->bmin.p, s->bmax.p, st);
+	s->profEnd(pk);
 	uint32_t cls[4];
 	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->bodyStart.p, st, cls, s->hostCounts);
 	if (r)
@@ -507,9 +553,13 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// 2. narrowphase into per-slot contact sets (slots: body pairs, then body-plane pairs)
 	PXH_CHECK(s->ctN.ensure(std::max(S, 1)));
 	PXH_CHECK(s->ctP.ensure(size_t(4) * std::max(S, 1)));
+	pk = s->profBegin(2);
 	pxh::launch_np_pairs(P, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length, s->ctN.p, s->ctP.p, st);
+	s->profEnd(pk);
+	pk = s->profBegin(3);
 	pxh::launch_np_planes(Q, P, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p, cd, s->ctN.p,
 	                      s->ctP.p, st);
+	s->profEnd(pk);
 
 	// 3. constraints = touching slots
 	PXH_CHECK(s->flags.ensure(size_t(S) + 1));
@@ -596,22 +646,34 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float invStepDt = 1.0f / stepDt;
 	const float biasCoefficient = 2.0f * std::sqrt(1.0f / float(P_));
 	const float scale = std::min(0.8f, biasCoefficient);
+	PXH_CHECK(s->counters.ensure(4));
+	PXH_CHECK(hipMemsetAsync(s->counters.p, 0, 4 * sizeof(unsigned long long), st));
+	pk = s->profBegin(4);
 	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
-	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, st);
+	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, s->counters.p, st);
+	PXH_CHECK(hipMemcpyAsync(s->hostCounts + 4000, s->counters.p, 3 * sizeof(unsigned long long),
+	                         hipMemcpyDeviceToHost, st));
+	s->profEnd(pk);
 
 	// 6. solve: position iterations by colour with substep integration, then
 	//    velocity iterations (DyTGSDynamics.cpp:2515-2790)
-	for (int it = 0; it < P_; it++)
+	for (int it = 0; it < P_ + V_; it++)
 	{
+		const bool posIter = it < P_;
 		for (int c = 0; c < rounds; c++)
-			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb, -3.402823466e+38f,
-			                  d.static_friction, d.dynamic_friction, st);
-		pxh::launch_integrate(n, stepDt, sb, st);
+		{
+			pk = s->profBegin(5);
+			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
+			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, st);
+			s->profEnd(pk);
+		}
+		if (posIter)
+		{
+			pk = s->profBegin(6);
+			pxh::launch_integrate(n, stepDt, sb, st);
+			s->profEnd(pk);
+		}
 	}
-	for (int it = 0; it < V_; it++)
-		for (int c = 0; c < rounds; c++)
-			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb, 0.0f, d.static_friction,
-			                  d.dynamic_friction, st);
 
 	// 7. write back poses and velocities
 	pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
@@ -637,7 +699,44 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 	else
 		PXH_CHECK(hipStreamSynchronize(s->stream));
 	s->simulating = false;
+	s->profCollect();
+	if (s->statConstraints > 0)
+	{
+		unsigned long long c3[3];
+		std::memcpy(c3, s->hostCounts + 4000, sizeof(c3));
+		s->statContacts = int64_t(c3[0]);
+		s->statFriction = int64_t(c3[1]);
+		s->statDynCon = int64_t(c3[2]);
+	}
 	return PXH_OK;
+}
+
+int32_t pxh_scene_set_profiling(pxh_scene s, int32_t enable)
+{
+	if (!s)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_profiling: null scene");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_set_profiling: not allowed while simulating");
+	s->profiling = enable != 0;
+	for (int i = 0; i < 8; i++)
+	{
+		s->profMs[i] = 0.0;
+		s->profLaunches[i] = 0;
+	}
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_profile(pxh_scene s, double* ms, int64_t* launches, int32_t n)
+{
+	if (!s || n < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_profile: bad arguments");
+	const int k = std::min(n, 8);
+	for (int i = 0; i < k; i++)
+	{
+		if (ms) ms[i] = s->profMs[i];
+		if (launches) launches[i] = s->profLaunches[i];
+	}
+	return k;
 }
 
 int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene s, void* data, const uint32_t* index, int32_t kind, int32_t n,
@@ -697,26 +796,9 @@ int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
-	// contact points: sum of per-slot counts of the last step
-	if (s->statContacts == 0 && s->statConstraints > 0)
-	{
-		const int S = int(s->statPairs + s->statPlanePairs);
-		std::vector<float4> hn(S);
-		if (hipMemcpy(hn.data(), s->ctN.p, S * sizeof(float4), hipMemcpyDeviceToHost) == hipSuccess)
-		{
-			int64_t k = 0;
-			for (int i = 0; i < S; i++)
-			{
-				int c;
-				std::memcpy(&c, &hn[i].w, 4);
-				k += c > 0 ? c : 0;
-			}
-			s->statContacts = k;
-		}
-	}
-	const int64_t v[7] = {s->n, s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts, s->statColors,
-	                      s->statSteps};
-	const int k = std::min(n, 7);
+	const int64_t v[9] = {s->n,           s->statPairs,  s->statPlanePairs, s->statConstraints, s->statContacts,
+	                      s->statColors,  s->statSteps,  s->statFriction,   s->statDynCon};
+	const int k = std::min(n, 9);
 	for (int i = 0; i < k; i++)
 		out[i] = v[i];
 	return k;
