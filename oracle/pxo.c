@@ -1478,12 +1478,17 @@ int pxo_world_step(void* wp, float dt)
 	for (long long c = 0; c < w->ncon; c++)
 		w->stats[3] += w->con[c].c.count;
 
-	/* 3. colouring: rounds of local minima of the ordering key at both dynamic bodies */
+	/* 3. colouring: rounds of local minima of the ordering key at both dynamic
+	 * bodies; each minimum takes the first colour free at both bodies (64-bit
+	 * masks; past 63 one colour per round, 64 + round). This replaces the
+	 * reference's greedy partitioning (lowleveldynamics/src/DyConstraintPartition.cpp:858)
+	 * with an order-free rule the GPU evaluates in parallel. */
 	uint64_t* best = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
+	uint64_t* mask = (uint64_t*)calloc((size_t)(n ? n : 1), sizeof(uint64_t));
 	for (long long c = 0; c < w->ncon; c++)
 		w->con[c].color = -1;
 	long long left = w->ncon;
-	int round = 0;
+	int round = 0, ncolors = 0;
 	while (left > 0)
 	{
 		for (int i = 0; i < n; i++)
@@ -1501,14 +1506,26 @@ int pxo_world_step(void* wp, float dt)
 			if (k->color >= 0) continue;
 			if (best[k->a] == k->key && (k->b < 0 || best[k->b] == k->key))
 			{
-				k->color = round;
+				const uint64_t used = mask[k->a] | (k->b >= 0 ? mask[k->b] : 0ull);
+				int col = 64 + round;
+				if (used != ~0ull)
+					for (col = 0; (used >> col) & 1ull; col++)
+						;
+				k->color = col;
+				if (col < 64)
+				{
+					mask[k->a] |= 1ull << col;
+					if (k->b >= 0) mask[k->b] |= 1ull << col;
+				}
+				if (col + 1 > ncolors) ncolors = col + 1;
 				left--;
 			}
 		}
 		round++;
 	}
 	free(best);
-	w->stats[4] = round;
+	free(mask);
+	w->stats[4] = ncolors;
 
 	/* 4. pre-integration (DyTGSDynamics.cpp:992-1021, DyBodyCoreIntegrator.h:40-75) */
 	sbody* sb = (sbody*)malloc(sizeof(sbody) * (size_t)(n ? n : 1));
@@ -1566,19 +1583,19 @@ int pxo_world_step(void* wp, float dt)
 	/* 6. solve: position iterations by colour with substep integration, then
 	 * velocity iterations (DyTGSDynamics.cpp:2515-2790) */
 	long long* order = (long long*)malloc(sizeof(long long) * (size_t)(w->ncon ? w->ncon : 1));
-	long long* start = (long long*)calloc((size_t)round + 2, sizeof(long long));
+	long long* start = (long long*)calloc((size_t)ncolors + 2, sizeof(long long));
 	for (long long c = 0; c < w->ncon; c++) start[w->con[c].color + 1]++;
-	for (int r = 0; r < round; r++) start[r + 1] += start[r];
+	for (int r = 0; r < ncolors; r++) start[r + 1] += start[r];
 	{
-		long long* fill = (long long*)malloc(sizeof(long long) * ((size_t)round + 1));
-		memcpy(fill, start, sizeof(long long) * ((size_t)round + 1));
+		long long* fill = (long long*)malloc(sizeof(long long) * ((size_t)ncolors + 1));
+		memcpy(fill, start, sizeof(long long) * ((size_t)ncolors + 1));
 		for (long long c = 0; c < w->ncon; c++) order[fill[w->con[c].color]++] = c;
 		free(fill);
 	}
 	float elapsed = 0.0f;
 	for (int it = 0; it < P; it++)
 	{
-		for (int r = 0; r < round; r++)
+		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
 			{
 				pxo_con* k = &w->con[order[o]];
@@ -1590,7 +1607,7 @@ int pxo_world_step(void* wp, float dt)
 		elapsed += stepDt;
 	}
 	for (int it = 0; it < Vi; it++)
-		for (int r = 0; r < round; r++)
+		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
 			{
 				pxo_con* k = &w->con[order[o]];

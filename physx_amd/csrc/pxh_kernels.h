@@ -42,8 +42,8 @@ void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t
 
 void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
                    float4* bmax, hipStream_t st);
-void launch_scene_grid(int nScenes, const int* bodyStart, const float4* bmin, const float4* bmax, float4* gridF,
-                       int2* gridI, uint32_t* rows, hipStream_t st);
+void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
+                       float4* gridF, int2* gridI, uint32_t* rows, hipStream_t st);
 void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
                  const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_gather_sorted(int n, const uint32_t* sv, const float4* bmin, const float4* bmax, float4* smin,
@@ -61,9 +61,10 @@ void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, 
                       const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
                       uint64_t* conKey, int* conColor, hipStream_t st);
 void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint32_t* colored, hipStream_t st);
-void launch_color_count(int C, const int* conColor, uint32_t* colorCount, hipStream_t st);
-void launch_color_order(int C, const int* conColor, uint32_t* colorFill, int* order, hipStream_t st);
+                        int* conColor, uint64_t* best, uint64_t* mask, uint32_t* colored, hipStream_t st);
+void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st);
+void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                          float maxAngSq, SolverBodies sb, hipStream_t st);

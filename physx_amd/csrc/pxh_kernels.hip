@@ -7,7 +7,7 @@
 //   broadphase (gpubroadphase/src/CUDA/broadphase.cu)        k_scene_grid, k_keys, radix sort, k_row_start,
 //                                                            k_gather_sorted, k_sweep<count|write>
 //   narrowphase (cudaSphere.cu, cudaBox.cu, cudaGJKEPA.cu)  k_np_pairs, k_np_planes
-//   partitioning (PxgIncrementalPartition, host in ref)     k_con_build, k_color_min, k_color_pick, k_color_order
+//   partitioning (PxgIncrementalPartition, host in ref)     k_con_build, k_color_min, k_color_pick, colour sort
 //   pre-integration (preIntegrationTGS.cu)                  k_preintegrate
 //   contact prep (constraintBlockPrepTGS.cu)                k_prep
 //   solve (solverMultiBlockTGS.cu solveBlockUnified)        k_solve
@@ -231,52 +231,78 @@ __global__ void k_bounds(int n, const float4* pos, const float4* rot, const floa
 
 // Per-world row grid for the sweep: rows are (y,z) cells no smaller than the
 // widest inflated box, so overlapping boxes sit in the same or adjacent rows.
-__global__ void __launch_bounds__(256) k_scene_grid(int nScenes, const int* bodyStart, const float4* bmin,
-                                                     const float4* bmax, float4* gridF, int2* gridI, uint32_t* rows)
+// Pass 1 reduces each world's centre range and widest box with order-
+// preserving integer atomics (one per wave when the wave holds one world).
+PXH_D float decode_float(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u ^ 0x80000000u) : ~u); }
+
+__global__ void k_grid_reduce(int n, const float4* bmin, const float4* bmax, const int* scene, uint32_t* acc)
 {
-	const int s = blockIdx.x;
-	if (s >= nScenes)
-		return;
-	__shared__ float red[6][256];
-	float lo_y = FLT_BIG, lo_z = FLT_BIG, hi_y = -FLT_BIG, hi_z = -FLT_BIG, wmax = 0.0f;
-	for (int i = bodyStart[s] + threadIdx.x; i < bodyStart[s + 1]; i += blockDim.x)
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const bool valid = i < n;
+	int s = valid ? scene[i] : -1;
+	uint32_t v[5] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0u};
+	if (valid)
 	{
 		const float4 a = bmin[i], b = bmax[i];
-		const float cy = 0.5f * (a.y + b.y), cz = 0.5f * (a.z + b.z);
-		lo_y = fminf(lo_y, cy); hi_y = fmaxf(hi_y, cy);
-		lo_z = fminf(lo_z, cz); hi_z = fmaxf(hi_z, cz);
-		wmax = fmaxf(wmax, fmaxf(b.y - a.y, b.z - a.z));
+		const uint32_t cy = encode_float(0.5f * (a.y + b.y)), cz = encode_float(0.5f * (a.z + b.z));
+		v[0] = cy; v[1] = cy; v[2] = cz; v[3] = cz;
+		v[4] = encode_float(fmaxf(b.y - a.y, b.z - a.z));
 	}
-	red[0][threadIdx.x] = lo_y; red[1][threadIdx.x] = hi_y;
-	red[2][threadIdx.x] = lo_z; red[3][threadIdx.x] = hi_z;
-	red[4][threadIdx.x] = wmax;
-	__syncthreads();
-	for (int o = 128; o > 0; o >>= 1)
+	const int s0 = __shfl(s, 0, 64);
+	const bool uniform = __all(s == s0 || s < 0);
+	if (uniform)
 	{
-		if (threadIdx.x < o)
+		for (int o = 32; o > 0; o >>= 1)
 		{
-			red[0][threadIdx.x] = fminf(red[0][threadIdx.x], red[0][threadIdx.x + o]);
-			red[1][threadIdx.x] = fmaxf(red[1][threadIdx.x], red[1][threadIdx.x + o]);
-			red[2][threadIdx.x] = fminf(red[2][threadIdx.x], red[2][threadIdx.x + o]);
-			red[3][threadIdx.x] = fmaxf(red[3][threadIdx.x], red[3][threadIdx.x + o]);
-			red[4][threadIdx.x] = fmaxf(red[4][threadIdx.x], red[4][threadIdx.x + o]);
+			v[0] = min(v[0], __shfl_xor(v[0], o, 64));
+			v[1] = max(v[1], __shfl_xor(v[1], o, 64));
+			v[2] = min(v[2], __shfl_xor(v[2], o, 64));
+			v[3] = max(v[3], __shfl_xor(v[3], o, 64));
+			v[4] = max(v[4], __shfl_xor(v[4], o, 64));
 		}
-		__syncthreads();
+		if ((threadIdx.x & 63) != 0 || s0 < 0)
+			return;
+		s = s0;
 	}
-	if (threadIdx.x == 0)
+	else if (!valid)
+		return;
+	uint32_t* w = acc + 5 * int64_t(s);
+	atomicMin(&w[0], v[0]);
+	atomicMax(&w[1], v[1]);
+	atomicMin(&w[2], v[2]);
+	atomicMax(&w[3], v[3]);
+	atomicMax(&w[4], v[4]);
+}
+
+__global__ void k_grid_init(int nScenes, uint32_t* acc)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < nScenes)
 	{
-		const bool empty = bodyStart[s + 1] == bodyStart[s];
-		const float ly = empty ? 0.0f : red[0][0], hy = empty ? 0.0f : red[1][0];
-		const float lz = empty ? 0.0f : red[2][0], hz = empty ? 0.0f : red[3][0];
-		float cell = red[4][0] * 1.01f + 1e-6f;
-		cell = fmaxf(cell, fmaxf((hy - ly), (hz - lz)) * (1.0f / 1000.0f));
-		int ny = int((hy - ly) / cell) + 1, nz = int((hz - lz) / cell) + 1;
-		ny = max(1, min(ny, 1024));
-		nz = max(1, min(nz, 1024));
-		gridF[s] = make_float4(ly, lz, 1.0f / cell, cell);
-		gridI[s] = make_int2(ny, nz);
-		rows[s] = uint32_t(ny * nz);
+		acc[5 * s + 0] = 0xFFFFFFFFu; acc[5 * s + 1] = 0u;
+		acc[5 * s + 2] = 0xFFFFFFFFu; acc[5 * s + 3] = 0u;
+		acc[5 * s + 4] = 0u;
 	}
+}
+
+__global__ void k_grid_finish(int nScenes, const uint32_t* acc, float4* gridF, int2* gridI, uint32_t* rows)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nScenes)
+		return;
+	const uint32_t* w = acc + 5 * int64_t(s);
+	const bool empty = w[0] == 0xFFFFFFFFu && w[1] == 0u;
+	const float ly = empty ? 0.0f : decode_float(w[0]), hy = empty ? 0.0f : decode_float(w[1]);
+	const float lz = empty ? 0.0f : decode_float(w[2]), hz = empty ? 0.0f : decode_float(w[3]);
+	const float wm = empty ? 0.0f : decode_float(w[4]);
+	float cell = wm * 1.01f + 1e-6f;
+	cell = fmaxf(cell, fmaxf(hy - ly, hz - lz) * (1.0f / 1000.0f));
+	int ny = int((hy - ly) / cell) + 1, nz = int((hz - lz) / cell) + 1;
+	ny = max(1, min(ny, 1024));
+	nz = max(1, min(nz, 1024));
+	gridF[s] = make_float4(ly, lz, 1.0f / cell, cell);
+	gridI[s] = make_int2(ny, nz);
+	rows[s] = uint32_t(ny * nz);
 }
 
 __global__ void k_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
@@ -566,34 +592,58 @@ __global__ void k_color_min(int C, const int* conA, const int* conB, const uint6
 		atomicMin(&best[b], k);
 }
 
+// A local minimum takes the smallest colour used by neither of its bodies
+// (first fit on 64-bit per-body colour masks; colours past 63 fall back to
+// one colour per round, 64 + round). Minima of one round share no body, so
+// their mask updates never race with each other's reads.
 __global__ void k_color_pick(int C, int round, const int* conA, const int* conB, const uint64_t* conKey, int* conColor,
-                             const unsigned long long* best, unsigned long long* bestReset, uint32_t* colored)
+                             const unsigned long long* best, unsigned long long* mask, uint32_t* colored)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= C || conColor[c] >= 0)
-		return;
-	const unsigned long long k = conKey[c];
-	const int a = conA[c], b = conB[c];
-	if (best[a] == k && (b < 0 || best[b] == k))
+	bool picked = false;
+	if (c < C && conColor[c] < 0)
 	{
-		conColor[c] = round;
-		atomicAdd(colored, 1u);
+		const unsigned long long k = conKey[c];
+		const int a = conA[c], b = conB[c];
+		if (best[a] == k && (b < 0 || best[b] == k))
+		{
+			const unsigned long long used = mask[a] | (b >= 0 ? mask[b] : 0ull);
+			int col = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
+			conColor[c] = col;
+			if (col < 64)
+			{
+				atomicOr(&mask[a], 1ull << col);
+				if (b >= 0)
+					atomicOr(&mask[b], 1ull << col);
+			}
+			picked = true;
+		}
 	}
-	(void)bestReset;
+	// one counter update per wave
+	const unsigned long long bal = __ballot(picked);
+	if ((threadIdx.x & 63) == 0 && bal)
+		atomicAdd(colored, uint32_t(__popcll(bal)));
 }
 
-__global__ void k_color_count(int C, const int* conColor, uint32_t* colorCount)
+// colour-ordered constraint list by a stable radix sort of (colour, index)
+__global__ void k_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c < C)
-		atomicAdd(&colorCount[conColor[c]], 1u);
+	{
+		keys[c] = (uint64_t(uint32_t(conColor[c])) << 32) | uint32_t(c);
+		vals[c] = uint32_t(c);
+	}
 }
 
-__global__ void k_color_order(int C, const int* conColor, uint32_t* colorFill, int* order)
+__global__ void k_max_color(int C, const int* conColor, uint32_t* maxColor)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c < C)
-		order[atomicAdd(&colorFill[conColor[c]], 1u)] = c;
+	int v = c < C ? conColor[c] : 0;
+	for (int o = 32; o > 0; o >>= 1)
+		v = max(v, __shfl_xor(v, o, 64));
+	if ((threadIdx.x & 63) == 0)
+		atomicMax(maxColor, uint32_t(v));
 }
 
 // ---------------------------------------------------------------------------
@@ -1133,11 +1183,12 @@ void launch_bounds(int n, const float4* pos, const float4* rot, const float4* ge
 {
 	PXH_LAUNCH(k_bounds, n, n, pos, rot, geom, co, bmin, bmax);
 }
-void launch_scene_grid(int nScenes, const int* bodyStart, const float4* bmin, const float4* bmax, float4* gridF,
-                       int2* gridI, uint32_t* rows, hipStream_t st)
+void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
+                       float4* gridF, int2* gridI, uint32_t* rows, hipStream_t st)
 {
-	if (nScenes > 0)
-		k_scene_grid<<<nScenes, 256, 0, st>>>(nScenes, bodyStart, bmin, bmax, gridF, gridI, rows);
+	PXH_LAUNCH(k_grid_init, nScenes, nScenes, acc);
+	PXH_LAUNCH(k_grid_reduce, n, n, bmin, bmax, scene, acc);
+	PXH_LAUNCH(k_grid_finish, nScenes, nScenes, acc, gridF, gridI, rows);
 }
 void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
                  const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st)
@@ -1189,20 +1240,21 @@ void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, 
 	           conColor);
 }
 void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint32_t* colored, hipStream_t st)
+                        int* conColor, uint64_t* best, uint64_t* mask, uint32_t* colored, hipStream_t st)
 {
 	PXH_LAUNCH(k_fill_u64, n, best, n, ~0ull);
 	PXH_LAUNCH(k_color_min, C, C, conA, conB, conKey, conColor, reinterpret_cast<unsigned long long*>(best));
 	PXH_LAUNCH(k_color_pick, C, C, round, conA, conB, conKey, conColor,
-	           reinterpret_cast<const unsigned long long*>(best), nullptr, colored);
+	           reinterpret_cast<const unsigned long long*>(best), reinterpret_cast<unsigned long long*>(mask), colored);
 }
-void launch_color_count(int C, const int* conColor, uint32_t* colorCount, hipStream_t st)
+void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st) { PXH_LAUNCH(k_fill_u64, n, p, n, v); }
+void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st)
 {
-	PXH_LAUNCH(k_color_count, C, C, conColor, colorCount);
+	PXH_LAUNCH(k_color_keys, C, C, conColor, keys, vals);
 }
-void launch_color_order(int C, const int* conColor, uint32_t* colorFill, int* order, hipStream_t st)
+void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st)
 {
-	PXH_LAUNCH(k_color_order, C, C, conColor, colorFill, order);
+	PXH_LAUNCH(k_max_color, C, C, conColor, maxColor);
 }
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,

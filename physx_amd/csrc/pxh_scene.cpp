@@ -125,15 +125,15 @@ struct pxh_scene_t
 	// per-step buffers
 	DBuf<float4> bmin, bmax, smin, smax, gridF;
 	DBuf<int2> gridI;
-	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags;
+	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags, gridAcc;
 	DBuf<uint64_t> keys, keys2;
 	DBuf<uint2> pairs;
 	DBuf<float4> ctN, ctP;
 	DBuf<int> conA, conB, conSlot, conColor, order;
-	DBuf<uint64_t> conKey, best;
+	DBuf<uint64_t> conKey, best, mask;
 	DBuf<uint32_t> colorCount, scratch;
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
-	int64_t statFriction = 0, statDynCon = 0;
+	int64_t statFriction = 0, statDynCon = 0, statRounds = 0;
 	pxh::SolverBodies sb{};
 	DBuf<float4> sbLin, sbAng, sbDLin, sbDAng, sbP, sbDQ, sbI0, sbI1, sbOrig;
 	DBuf<int4> rhI;
@@ -192,7 +192,7 @@ struct pxh_broadphase_t
 	DBuf<float4> bmin, bmax, smin, smax, gridF;
 	DBuf<int2> gridI;
 	DBuf<int> scene, group, bodyStart;
-	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags, flags2;
+	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags, flags2, gridAcc;
 	DBuf<uint64_t> keys, keys2, pkPrev, pkCur, pkTmp;
 	DBuf<uint2> pairs, created, deleted;
 	int nPrev = 0, nCur = 0, nCreated = 0, nDeleted = 0;
@@ -259,7 +259,10 @@ int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_grou
 	PXH_CHECK(s->rows.ensure(nWorlds + 1));
 	PXH_CHECK(s->rowOff.ensure(nWorlds + 1));
 	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(3) * n + 1, 256 * 1024)) + 1024));
-	pxh::launch_scene_grid(nWorlds, d_bodyStart, s->bmin.p, s->bmax.p, s->gridF.p, s->gridI.p, s->rows.p, st);
+	PXH_CHECK(s->gridAcc.ensure(size_t(5) * nWorlds));
+	(void)d_bodyStart;
+	pxh::launch_scene_grid(n, nWorlds, d_scene, s->bmin.p, s->bmax.p, s->gridAcc.p, s->gridF.p, s->gridI.p, s->rows.p,
+	                       st);
 	PXH_CHECK(hipMemsetAsync(s->rows.p + nWorlds, 0, sizeof(uint32_t), st));
 	PXH_CHECK(hipMemcpyAsync(s->rowOff.p, s->rows.p, sizeof(uint32_t) * (nWorlds + 1), hipMemcpyDeviceToDevice, st));
 	pxh::launch_scan(s->rowOff.p, nWorlds + 1, s->scanTmp.p, st);
@@ -385,13 +388,14 @@ int32_t pxh_scene_release(pxh_scene s)
 	for (auto* b : is)
 		b->release();
 	DBuf<uint32_t>* us[] = {&s->rows, &s->rowOff, &s->rowStart, &s->vals, &s->vals2, &s->hist, &s->scanTmp,
-	                        &s->counts, &s->flags, &s->colorCount, &s->scratch};
+	                        &s->counts, &s->flags, &s->colorCount, &s->scratch, &s->gridAcc};
 	for (auto* b : us)
 		b->release();
 	s->keys.release();
 	s->keys2.release();
 	s->conKey.release();
 	s->best.release();
+	s->mask.release();
 	s->pairs.release();
 	s->gridI.release();
 	s->rhI.release();
@@ -584,48 +588,63 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_con_build(S, P, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
 	                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
 
-	// 4. colouring
+	// 4. colouring: rounds of local key minima, each taking its first free colour
 	PXH_CHECK(s->best.ensure(n));
+	PXH_CHECK(s->mask.ensure(n));
 	PXH_CHECK(s->scratch.ensure(4));
-	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, sizeof(uint32_t), st));
+	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, 2 * sizeof(uint32_t), st));
+	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
 	int rounds = 0;
-	const int kMaxColors = 1024;
+	const int kMaxRounds = 960;
 	uint32_t colored = 0;
+	int batch = 4;
 	while (int(colored) < C)
 	{
-		for (int k = 0; k < 4 && rounds < kMaxColors; k++, rounds++)
+		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
 			pxh::launch_color_round(C, n, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p, s->best.p,
-			                        s->scratch.p, st);
+			                        s->mask.p, s->scratch.p, st);
+		batch = 2;
 		const uint32_t* src[1] = {s->scratch.p};
 		r = readCounts(st, s->hostCounts, src, 1);
 		if (r)
 			return r;
 		colored = s->hostCounts[0];
-		if (rounds >= kMaxColors && int(colored) < C)
-			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring exceeded 1024 colours");
+		if (rounds >= kMaxRounds && int(colored) < C)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring did not finish in 960 rounds");
 	}
-	PXH_CHECK(s->colorCount.ensure(size_t(rounds) + 1));
-	PXH_CHECK(hipMemsetAsync(s->colorCount.p, 0, sizeof(uint32_t) * (rounds + 1), st));
-	pxh::launch_color_count(C, s->conColor.p, s->colorCount.p, st);
-	std::vector<uint32_t> colorStart(rounds + 1, 0);
-	if (rounds > 0)
+	int nColors = 0;
+	std::vector<uint32_t> colorStart(1, 0);
+	if (C > 0)
 	{
-		PXH_CHECK(hipMemcpyAsync(s->hostCounts, s->colorCount.p, sizeof(uint32_t) * rounds, hipMemcpyDeviceToHost, st));
+		pxh::launch_max_color(C, s->conColor.p, s->scratch.p + 1, st);
+		const uint32_t* src[1] = {s->scratch.p + 1};
+		r = readCounts(st, s->hostCounts, src, 1);
+		if (r)
+			return r;
+		nColors = int(s->hostCounts[0]) + 1;
+		PXH_CHECK(s->keys.ensure(C));
+		PXH_CHECK(s->keys2.ensure(C));
+		PXH_CHECK(s->vals.ensure(C));
+		PXH_CHECK(s->vals2.ensure(C));
+		const int nb = (C + 2047) / 2048;
+		PXH_CHECK(s->hist.ensure(size_t(256) * nb));
+		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(256) * nb) + 1024));
+		PXH_CHECK(s->colorCount.ensure(size_t(nColors) + 2));
+		pxh::launch_color_keys(C, s->conColor.p, s->keys.p, s->vals.p, st);
+		bool alt = false;
+		pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, C, 32, 32 + ((bitsFor(nColors) + 7) / 8) * 8,
+		                       s->hist.p, s->scanTmp.p, st, alt);
+		const uint64_t* ck = alt ? s->keys2.p : s->keys.p;
+		const uint32_t* cv = alt ? s->vals2.p : s->vals.p;
+		pxh::launch_row_start(C, ck, s->colorCount.p, uint32_t(nColors), st);
+		PXH_CHECK(hipMemcpyAsync(s->order.p, cv, sizeof(int) * C, hipMemcpyDeviceToDevice, st));
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts, s->colorCount.p, sizeof(uint32_t) * (nColors + 1),
+		                         hipMemcpyDeviceToHost, st));
 		PXH_CHECK(hipStreamSynchronize(st));
-		// the colour count array is small; trailing empty colours (rounds after
-		// the last colouring) are dropped
-		for (int c = 0; c < rounds; c++)
-			colorStart[c + 1] = colorStart[c] + s->hostCounts[c];
-		while (rounds > 0 && s->hostCounts[rounds - 1] == 0)
-			rounds--;
-		// stage the colour offsets in pinned memory (the copy is asynchronous)
-		uint32_t* pinnedStart = s->hostCounts + 2048;
-		std::memcpy(pinnedStart, colorStart.data(), sizeof(uint32_t) * (rounds + 1));
-		PXH_CHECK(hipMemcpyAsync(s->colorCount.p, pinnedStart, sizeof(uint32_t) * (rounds + 1),
-		                         hipMemcpyHostToDevice, st));
-		pxh::launch_color_order(C, s->conColor.p, s->colorCount.p, s->order.p, st);
+		colorStart.assign(s->hostCounts, s->hostCounts + nColors + 1);
 	}
-	s->statColors = rounds;
+	s->statColors = nColors;
+	s->statRounds = rounds;
 
 	// 5. solver bodies + contact rows
 	PXH_CHECK(s->sbLin.ensure(n)); PXH_CHECK(s->sbAng.ensure(n)); PXH_CHECK(s->sbDLin.ensure(n));
@@ -660,8 +679,10 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	for (int it = 0; it < P_ + V_; it++)
 	{
 		const bool posIter = it < P_;
-		for (int c = 0; c < rounds; c++)
+		for (int c = 0; c < nColors; c++)
 		{
+			if (colorStart[c + 1] == colorStart[c])
+				continue;
 			pk = s->profBegin(5);
 			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
 			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, st);
@@ -796,9 +817,9 @@ int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
-	const int64_t v[9] = {s->n,           s->statPairs,  s->statPlanePairs, s->statConstraints, s->statContacts,
-	                      s->statColors,  s->statSteps,  s->statFriction,   s->statDynCon};
-	const int k = std::min(n, 9);
+	const int64_t v[10] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
+	                       s->statColors, s->statSteps, s->statFriction,   s->statDynCon,      s->statRounds};
+	const int k = std::min(n, 10);
 	for (int i = 0; i < k; i++)
 		out[i] = v[i];
 	return k;
@@ -958,7 +979,7 @@ int32_t pxh_bp_release(pxh_broadphase bp)
 	bp->bmin.release(); bp->bmax.release(); bp->smin.release(); bp->smax.release(); bp->gridF.release();
 	bp->gridI.release(); bp->scene.release(); bp->group.release(); bp->bodyStart.release();
 	bp->rows.release(); bp->rowOff.release(); bp->rowStart.release(); bp->vals.release(); bp->vals2.release();
-	bp->hist.release(); bp->scanTmp.release(); bp->counts.release(); bp->flags.release(); bp->flags2.release();
+	bp->hist.release(); bp->scanTmp.release(); bp->gridAcc.release(); bp->counts.release(); bp->flags.release(); bp->flags2.release();
 	bp->keys.release(); bp->keys2.release(); bp->pkPrev.release(); bp->pkCur.release(); bp->pkTmp.release();
 	bp->pairs.release(); bp->created.release(); bp->deleted.release();
 	if (bp->hostCounts)
