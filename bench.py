@@ -34,15 +34,17 @@ def log(msg):
 
 
 def solve_bytes_per_step(st: dict, iters: int) -> float:
-    """Algorithmic HBM bytes of all solve launches of one step: per constraint
-    and iteration a 48 B header, per contact point 48 B of rows read + 4 B
-    impulse written, per friction row 48 B read + 4 B written, per dynamic body
-    endpoint 64 B of velocity/motion state read + 32 B written."""
+    """Algorithmic HBM bytes of all solve launches of one step. Per constraint
+    and iteration: a 32 B header; per contact point 32 B of row read and the
+    4 B impulse read and written; per friction row 32 B of row (+16 B of body-B
+    row when B is dynamic) and the 4 B impulse read and written; per dynamic
+    body endpoint 64 B of velocity/motion state read and 32 B written."""
     T = st["constraints"]
     K = st["contacts"]
     F = st["friction_rows"]
+    Fd = st["body_body_friction_rows"]
     endpoints = T + st["body_body_constraints"]
-    return iters * (48.0 * T + 52.0 * K + 52.0 * F + 96.0 * endpoints)
+    return iters * (32.0 * T + 40.0 * K + 40.0 * F + 16.0 * Fd + 96.0 * endpoints)
 
 
 def main():
@@ -84,7 +86,7 @@ def main():
     gpu.set_profiling(True)
     step_ms = []
     sums = dict(constraints=0, contacts=0, friction_rows=0, body_body_constraints=0, bp_pairs=0, plane_pairs=0,
-                colors=0)
+                colors=0, color_rounds=0, body_body_friction_rows=0)
     solve_bytes = 0.0
     torch.cuda.synchronize()
     t_begin = time.perf_counter()

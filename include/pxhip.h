@@ -107,7 +107,8 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, cons
 /* PxScene::getSimulationStatistics (physx/include/PxSimulationStatistics.h):
  * out[0] bodies, [1] broadphase pairs (body-body), [2] body-plane pairs,
  * [3] touching pairs (constraints), [4] contact points, [5] colours,
- * [6] steps taken, [7] friction rows, [8] body-body constraints, [9] colouring rounds.
+ * [6] steps taken, [7] friction rows, [8] body-body constraints, [9] colouring rounds,
+ * [10] friction rows of body-body constraints.
  * Returns the number of values written (<= n). */
 int32_t pxh_scene_get_stats(pxh_scene scene, int64_t* out, int32_t n);
 

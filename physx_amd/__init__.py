@@ -190,10 +190,10 @@ class Scene:
         self.set_rigid_dynamic_data(torch.from_numpy(_f32(ang_vel)).to(dev), PXH_DATA_ANGULAR_VELOCITY)
 
     def stats(self) -> dict:
-        out = np.zeros(10, np.int64)
-        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 10))
+        out = np.zeros(11, np.int64)
+        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 11))
         keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps", "friction_rows",
-                "body_body_constraints", "color_rounds")
+                "body_body_constraints", "color_rounds", "body_body_friction_rows")
         return dict(zip(keys, map(int, out)))
 
     STAGES = ("bounds", "broadphase", "np_pairs", "np_planes", "prep", "solve", "integrate", "unused")

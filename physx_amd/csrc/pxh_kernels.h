@@ -25,14 +25,14 @@ struct SolverBodies
 struct Rows
 {
 	int4* hI;     // bodyA, bodyB (<0: static), nContacts | nFrictionRows << 8
-	float4* hF;   // normal xyz, inverse mass A
-	float4* hG;   // inverse mass B, friction scale, friction bias scale, max penetration bias
+	float4* hF;   // normal xyz, -
 	float4* nA;   // [4] raXnI xyz, separation
-	float4* nB;   // [4] rbXnI xyz, bias coefficient
-	float4* nC;   // [4] velocity multiplier, recip response, applied impulse
+	float4* nB;   // [4] rbXnI xyz, reciprocal response (negative: separated point)
+	float* nF;    // [4] applied normal impulse
 	float4* fT;   // [4] tangent xyz, initial error
 	float4* fA;   // [4] raXnI xyz, velocity multiplier
-	float4* fB;   // [4] rbXnI xyz, applied impulse
+	float4* fB;   // [4] rbXnI xyz (dynamic body B only)
+	float* fF;    // [4] applied friction impulse
 };
 
 int64_t scan_tmp_size(int64_t n);
@@ -72,7 +72,7 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, unsigned long long* counters, hipStream_t st);
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  hipStream_t st);
+                  float invStepDt, float biasScale, float scale, hipStream_t st);
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
 void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,

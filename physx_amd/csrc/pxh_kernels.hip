@@ -729,11 +729,13 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, svLin, svOrigLin, svI0, svI1, invStepDt, scale,
 	         restDist, frictionOffsetThreshold, rows, myFric);
 	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
+	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
 	if ((threadIdx.x & 63) == 63)
 	{
 		atomicAdd(&counters[0], (unsigned long long)sc);
 		atomicAdd(&counters[1], (unsigned long long)sf);
 		atomicAdd(&counters[2], (unsigned long long)sd);
+		atomicAdd(&counters[3], (unsigned long long)sfd);
 	}
 }
 
@@ -782,8 +784,10 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 		const float pen = q.w - restDist;
 		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
 		rows.nA[k * int64_t(C) + o] = f4(raXnI, pen);
-		rows.nB[k * int64_t(C) + o] = f4(rbXnI, -(pen > 0.0f ? invStepDt : invDtp8));
-		rows.nC[k * int64_t(C) + o] = make_float4(recip, recip, 0.0f, 0.0f);
+		// the bias coefficient is -invStepDt for a separated point, else -invStepDt*scale:
+		// the sign of the stored reciprocal response carries that choice
+		rows.nB[k * int64_t(C) + o] = f4(rbXnI, pen > 0.0f ? -recip : recip);
+		rows.nF[k * int64_t(C) + o] = 0.0f;
 	}
 	// friction anchors among points closer than the offset threshold
 	int nAnch = 0;
@@ -812,9 +816,10 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	}
 	const float fscale = nAnch == 2 ? 0.5f : 1.0f;
 	nFricOut = uint32_t(2 * nAnch);
+	(void)fscale;
+	(void)invDtp8;
 	rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), 0);
-	rows.hF[o] = make_float4(n.x, n.y, n.z, invMassA);
-	rows.hG[o] = make_float4(invMassB, fscale, invStepDt * scale, -1e32f);
+	rows.hF[o] = make_float4(n.x, n.y, n.z, 0.0f);
 	if (nAnch == 0)
 		return;
 	const f3 lvr = mk(svOrigLin[a]) - linB;
@@ -838,102 +843,135 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 			rows.fT[r * int64_t(C) + o] = f4(t, dot(err, t));
 			rows.fA[r * int64_t(C) + o] = f4(raXnI, unit > 0.0f ? scale / unit : 0.0f);
 			rows.fB[r * int64_t(C) + o] = f4(rbXnI, 0.0f);
+			rows.fF[r * int64_t(C) + o] = 0.0f;
 		}
 	}
 }
 
 // one colour: each lane solves one constraint (solveContact /
-// solveDynamicContactsStep, lowleveldynamics/src/DyTGSContactPrep.cpp:1490-1843)
-__global__ void __launch_bounds__(256) k_solve(int begin, int end, int C, Rows rows, float4* svLin, float4* svAng,
-                                               const float4* svDLin, const float4* svDAng, float minPen, float muS,
-                                               float muD)
+// solveDynamicContactsStep, lowleveldynamics/src/DyTGSContactPrep.cpp:1490-1843).
+// All rows of the lane are loaded before any arithmetic so a wave keeps every
+// gather in flight at once; 64-lane blocks spread a colour over all CUs.
+constexpr int SOLVE_THREADS = 64;
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* svLin,
+                                                          float4* svAng, const float4* svDLin, const float4* svDAng,
+                                                          float minPen, float muS, float muD, float invStepDt,
+                                                          float biasScale, float scale)
 {
 	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= end)
 		return;
 	const int4 hi = rows.hI[o];
 	const float4 hf = rows.hF[o];
-	const float4 hg = rows.hG[o];
 	const int a = hi.x, b = hi.y;
 	const int cnt = hi.z & 0xFF, nF = hi.z >> 8;
 	const bool dynB = b >= 0;
-	const f3 n = mk(hf);
-	const float invMassA = hf.w, invMassB = hg.x;
-	f3 l0 = mk(svLin[a]), a0 = mk(svAng[a]);
-	const f3 am0 = mk(svDAng[a]);
-	f3 rel = mk(svDLin[a]);
-	f3 l1 = mk(0, 0, 0), a1 = mk(0, 0, 0), am1 = mk(0, 0, 0);
+	// gathers: both bodies
+	const float4 la4 = svLin[a], aa4 = svAng[a], am04 = svDAng[a], dla4 = svDLin[a];
+	float4 lb4 = make_float4(0, 0, 0, 0), ab4 = lb4, am14 = lb4, dlb4 = lb4;
 	if (dynB)
 	{
-		l1 = mk(svLin[b]);
-		a1 = mk(svAng[b]);
-		am1 = mk(svDAng[b]);
-		rel = rel - mk(svDLin[b]);
+		lb4 = svLin[b]; ab4 = svAng[b]; am14 = svDAng[b]; dlb4 = svDLin[b];
 	}
-	const f3 nim0 = n * invMassA, nim1 = n * invMassB;
-	const float dv = dot(rel, n);
-	const float maxPenBias = hg.w;
-	float acc = 0.0f;
-	for (int k = 0; k < cnt; k++)
+	// rows
+	float4 nA[4], nB[4], fT[4], fA[4], fB[4];
+	float nFc[4], fFc[4];
+#pragma unroll
+	for (int k = 0; k < 4; k++)
 	{
 		const int64_t ri = k * int64_t(C) + o;
-		const float4 ra = rows.nA[ri], rb = rows.nB[ri];
-		float4 rc = rows.nC[ri];
-		const f3 raXnI = mk(ra), rbXnI = mk(rb);
+		if (k < cnt)
+		{
+			nA[k] = rows.nA[ri]; nB[k] = rows.nB[ri]; nFc[k] = rows.nF[ri];
+		}
+		if (k < nF)
+		{
+			fT[k] = rows.fT[ri]; fA[k] = rows.fA[ri]; fFc[k] = rows.fF[ri];
+			fB[k] = dynB ? rows.fB[ri] : make_float4(0, 0, 0, 0);
+		}
+	}
+	const f3 n = mk(hf);
+	const float invMassA = la4.w, invMassB = lb4.w;
+	f3 l0 = mk(la4), a0 = mk(aa4), l1 = mk(lb4), a1 = mk(ab4);
+	const f3 am0 = mk(am04), am1 = mk(am14);
+	const f3 rel = mk(dla4) - mk(dlb4);
+	const f3 nim0 = n * invMassA, nim1 = n * invMassB;
+	const float dv = dot(rel, n);
+	const float biasSep = -invStepDt, biasPen = -invStepDt * scale;
+	float acc = 0.0f;
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		if (k >= cnt)
+			break;
+		const f3 raXnI = mk(nA[k]), rbXnI = mk(nB[k]);
+		const float rs = nB[k].w;
+		const float recip = fabsf(rs);
+		const float biasCoef = rs < 0.0f ? biasSep : biasPen;
 		const float dang = dot(am0, raXnI) - dot(am1, rbXnI);
-		const float sep = fmaxf(minPen, ra.w + (dv + dang));
-		const float bias = fminf(-maxPenBias, rb.w * sep);
+		const float sep = fmaxf(minPen, nA[k].w + (dv + dang));
+		const float bias = fminf(1e32f, biasCoef * sep);
 		const f3 v0 = mulv(l0, n) + mulv(a0, raXnI);
 		const f3 v1 = mulv(l1, n) + mulv(a1, rbXnI);
 		const f3 dvv = v0 - v1;
 		const float nv = dvv.x + dvv.y + dvv.z;
-		const float lambda = bias * rc.y - nv * rc.x;
-		const float af = rc.z;
+		const float lambda = bias * recip - nv * recip;
+		const float af = nFc[k];
 		const float nf = fminf(af + fmaxf(lambda, -af), 1e32f);
 		const float df = nf - af;
 		l0 = nim0 * df + l0;
 		l1 = l1 - nim1 * df;
 		a0 = raXnI * df + a0;
 		a1 = a1 - rbXnI * df;
-		rc.z = nf;
-		rows.nC[ri] = rc;
+		nFc[k] = nf;
 		acc = acc + nf;
 	}
 	if (nF)
 	{
 		const float maxF = muS * acc, maxD = muD * acc;
-		const float fscale = hg.y, biasScale = hg.z;
-		for (int i = 0; i < nF; i += 2)
+		const float fscale = nF == 4 ? 0.5f : 1.0f;
+#pragma unroll
+		for (int i = 0; i < 4; i += 2)
 		{
-			float ti[2], af[2];
-			f3 t[2], raI[2], rbI[2];
+			if (i >= nF)
+				break;
+			float ti[2];
+#pragma unroll
 			for (int d = 0; d < 2; d++)
 			{
-				const int64_t ri = (i + d) * int64_t(C) + o;
-				const float4 ft = rows.fT[ri], fa = rows.fA[ri], fb = rows.fB[ri];
-				t[d] = mk(ft); raI[d] = mk(fa); rbI[d] = mk(fb);
-				const float dvf = (dot(raI[d], am0) - dot(rbI[d], am1)) + dot(t[d], rel);
-				const float bias = (ft.w + dvf) * biasScale;
-				const f3 v0 = mulv(l0, t[d]) + mulv(a0, raI[d]);
-				const f3 v1 = mulv(l1, t[d]) + mulv(a1, rbI[d]);
+				const f3 t = mk(fT[i + d]), raI = mk(fA[i + d]), rbI = mk(fB[i + d]);
+				const float dvf = (dot(raI, am0) - dot(rbI, am1)) + dot(t, rel);
+				const float bias = (fT[i + d].w + dvf) * biasScale;
+				const f3 v0 = mulv(l0, t) + mulv(a0, raI);
+				const f3 v1 = mulv(l1, t) + mulv(a1, rbI);
 				const f3 dd = v0 - v1;
 				const float nv = dd.x + dd.y + dd.z;
-				af[d] = fb.w;
-				ti[d] = (af[d] - bias * fa.w) - nv * fa.w;
+				const float vm = fA[i + d].w;
+				ti[d] = (fFc[i + d] - bias * vm) - nv * vm;
 			}
 			const float tot = sqrtf(ti[0] * ti[0] + ti[1] * ti[1]);
 			const bool clamp = tot > fscale * maxF;
 			const float tc = clamp ? fminf(fscale * maxD, tot) : tot;
 			const float ratio = tot > 0.0f ? tc / tot : 0.0f;
 			const float n0 = ti[0] * ratio, n1 = ti[1] * ratio;
-			const float d0 = n0 - af[0], d1 = n1 - af[1];
-			l0 = (t[0] * invMassA) * d0 + ((t[1] * invMassA) * d1 + l0);
-			l1 = (l1 - (t[1] * invMassB) * d1) - (t[0] * invMassB) * d0;
-			a0 = raI[0] * d0 + (raI[1] * d1 + a0);
-			a1 = (a1 - rbI[1] * d1) - rbI[0] * d0;
-			rows.fB[i * int64_t(C) + o].w = n0;
-			rows.fB[(i + 1) * int64_t(C) + o].w = n1;
+			const float d0 = n0 - fFc[i], d1 = n1 - fFc[i + 1];
+			const f3 t0 = mk(fT[i]), t1 = mk(fT[i + 1]);
+			l0 = (t0 * invMassA) * d0 + ((t1 * invMassA) * d1 + l0);
+			l1 = (l1 - (t1 * invMassB) * d1) - (t0 * invMassB) * d0;
+			a0 = mk(fA[i]) * d0 + (mk(fA[i + 1]) * d1 + a0);
+			a1 = (a1 - mk(fB[i + 1]) * d1) - mk(fB[i]) * d0;
+			fFc[i] = n0;
+			fFc[i + 1] = n1;
 		}
+	}
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		const int64_t ri = k * int64_t(C) + o;
+		if (k < cnt)
+			rows.nF[ri] = nFc[k];
+		if (k < nF)
+			rows.fF[ri] = fFc[k];
 	}
 	svLin[a] = f4(l0, invMassA);
 	svAng[a] = f4(a0, 0.0f);
@@ -1271,11 +1309,12 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
 	           invStepDt, scale, restDist, fot, rows, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  hipStream_t st)
+                  float invStepDt, float biasScale, float scale, hipStream_t st)
 {
 	if (end > begin)
-		k_solve<<<nblk(end - begin, 256), 256, 0, st>>>(begin, end, C, rows, sb.lin, sb.ang, sb.dlin, sb.dang, minPen,
-		                                                 muS, muD);
+		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.lin, sb.ang, sb.dlin,
+		                                                                     sb.dang, minPen, muS, muD, invStepDt,
+		                                                                     biasScale, scale);
 }
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
 {

@@ -133,11 +133,12 @@ struct pxh_scene_t
 	DBuf<uint64_t> conKey, best, mask;
 	DBuf<uint32_t> colorCount, scratch;
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
-	int64_t statFriction = 0, statDynCon = 0, statRounds = 0;
+	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0;
 	pxh::SolverBodies sb{};
 	DBuf<float4> sbLin, sbAng, sbDLin, sbDAng, sbP, sbDQ, sbI0, sbI1, sbOrig;
 	DBuf<int4> rhI;
-	DBuf<float4> rhF, rhG, rnA, rnB, rnC, rfT, rfA, rfB;
+	DBuf<float4> rhF, rnA, rnB, rfT, rfA, rfB;
+	DBuf<float> rnF, rfF;
 	uint32_t* hostCounts = nullptr;       // pinned
 
 	// optional per-stage GPU timing with HIP events on the scene stream
@@ -379,8 +380,8 @@ int32_t pxh_scene_release(pxh_scene s)
 	(void)hipStreamSynchronize(s->stream);
 	DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->planes, &s->bmin, &s->bmax,
 	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbLin, &s->sbAng, &s->sbDLin,
-	                       &s->sbDAng, &s->sbP, &s->sbDQ, &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF, &s->rhG,
-	                       &s->rnA, &s->rnB, &s->rnC, &s->rfT, &s->rfA, &s->rfB};
+	                       &s->sbDAng, &s->sbP, &s->sbDQ, &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF,
+	                       &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
 	for (auto* b : f4s)
 		b->release();
 	DBuf<int>* is[] = {&s->scene, &s->local, &s->bodyStart, &s->planeLocal, &s->qBody, &s->qPlane, &s->conA,
@@ -399,6 +400,8 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->pairs.release();
 	s->gridI.release();
 	s->rhI.release();
+	s->rnF.release();
+	s->rfF.release();
 	if (s->hostCounts)
 		(void)hipHostFree(s->hostCounts);
 	if (s->stream)
@@ -533,7 +536,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const int nWorlds = int(s->hBodyStart.size()) - 1;
 	s->statSteps++;
 	s->statPairs = s->statPlanePairs = s->statConstraints = s->statContacts = s->statColors = 0;
-	s->statFriction = s->statDynCon = 0;
+	s->statFriction = s->statDynCon = s->statDynFriction = 0;
 	if (n == 0)
 		return PXH_OK;
 	const float co = d.contact_offset, cd = 2.0f * co;
@@ -656,10 +659,11 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
 	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, sb, st);
 	const size_t Cc = std::max(C, 1);
-	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc)); PXH_CHECK(s->rhG.ensure(Cc));
-	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnC.ensure(4 * Cc));
+	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc));
+	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnF.ensure(4 * Cc));
 	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
-	pxh::Rows rows{s->rhI.p, s->rhF.p, s->rhG.p, s->rnA.p, s->rnB.p, s->rnC.p, s->rfT.p, s->rfA.p, s->rfB.p};
+	PXH_CHECK(s->rfF.ensure(4 * Cc));
+	pxh::Rows rows{s->rhI.p, s->rhF.p, s->rnA.p, s->rnB.p, s->rnF.p, s->rfT.p, s->rfA.p, s->rfB.p, s->rfF.p};
 	const int P_ = d.position_iterations, V_ = d.velocity_iterations;
 	const float stepDt = dt / float(P_);
 	const float invStepDt = 1.0f / stepDt;
@@ -670,7 +674,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pk = s->profBegin(4);
 	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
 	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, s->counters.p, st);
-	PXH_CHECK(hipMemcpyAsync(s->hostCounts + 4000, s->counters.p, 3 * sizeof(unsigned long long),
+	PXH_CHECK(hipMemcpyAsync(s->hostCounts + 4000, s->counters.p, 4 * sizeof(unsigned long long),
 	                         hipMemcpyDeviceToHost, st));
 	s->profEnd(pk);
 
@@ -685,7 +689,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 				continue;
 			pk = s->profBegin(5);
 			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
-			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, st);
+			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, invStepDt,
+			                  invStepDt * scale, scale, st);
 			s->profEnd(pk);
 		}
 		if (posIter)
@@ -723,11 +728,12 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 	s->profCollect();
 	if (s->statConstraints > 0)
 	{
-		unsigned long long c3[3];
+		unsigned long long c3[4];
 		std::memcpy(c3, s->hostCounts + 4000, sizeof(c3));
 		s->statContacts = int64_t(c3[0]);
 		s->statFriction = int64_t(c3[1]);
 		s->statDynCon = int64_t(c3[2]);
+		s->statDynFriction = int64_t(c3[3]);
 	}
 	return PXH_OK;
 }
@@ -817,9 +823,10 @@ int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
-	const int64_t v[10] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
-	                       s->statColors, s->statSteps, s->statFriction,   s->statDynCon,      s->statRounds};
-	const int k = std::min(n, 10);
+	const int64_t v[11] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
+	                       s->statColors, s->statSteps, s->statFriction,   s->statDynCon,      s->statRounds,
+	                       s->statDynFriction};
+	const int k = std::min(n, 11);
 	for (int i = 0; i < k; i++)
 		out[i] = v[i];
 	return k;
