@@ -60,8 +60,11 @@ void launch_touch_flags(int S, const float4* ctN, uint32_t* flags, hipStream_t s
 void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs, const int* qBody,
                       const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
                       uint64_t* conKey, int* conColor, hipStream_t st);
-void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint64_t* mask, uint32_t* colored, hipStream_t st);
+void launch_color_round(int m, const int* list, int round, const int* conA, const int* conB, const uint64_t* conKey,
+                        int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st);
+void launch_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags, hipStream_t st);
+void launch_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out,
+                          hipStream_t st);
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st);
 void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);

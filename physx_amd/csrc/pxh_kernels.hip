@@ -15,6 +15,8 @@
 #include "pxh_kernels.h"
 #include "pxh_np.h"
 
+#include <algorithm>
+
 namespace pxh {
 
 // ---------------------------------------------------------------------------
@@ -235,43 +237,65 @@ __global__ void k_bounds(int n, const float4* pos, const float4* rot, const floa
 // preserving integer atomics (one per wave when the wave holds one world).
 PXH_D float decode_float(uint32_t u) { return __uint_as_float((u & 0x80000000u) ? (u ^ 0x80000000u) : ~u); }
 
-__global__ void k_grid_reduce(int n, const float4* bmin, const float4* bmax, const int* scene, uint32_t* acc)
+constexpr int GRID_ITEMS = 16;
+__global__ void __launch_bounds__(256) k_grid_reduce(int n, const float4* bmin, const float4* bmax, const int* scene,
+                                                      uint32_t* acc)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const bool valid = i < n;
-	int s = valid ? scene[i] : -1;
+	__shared__ uint32_t red[5][256];
+	const int base = blockIdx.x * 256 * GRID_ITEMS;
+	const int first = base + threadIdx.x * GRID_ITEMS;
+	const int blockLast = min(n, base + 256 * GRID_ITEMS) - 1;
+	// one world in the whole block: reduce in LDS, one atomic set per block
+	const bool blockUniform = scene[base] == scene[blockLast];
 	uint32_t v[5] = {0xFFFFFFFFu, 0u, 0xFFFFFFFFu, 0u, 0u};
-	if (valid)
+	int cur = -1;
+	for (int i = first; i < min(n, first + GRID_ITEMS); i++)
 	{
+		const int s = scene[i];
+		if (s != cur && cur >= 0 && !blockUniform)
+		{
+			uint32_t* w = acc + 5 * int64_t(cur);
+			atomicMin(&w[0], v[0]); atomicMax(&w[1], v[1]); atomicMin(&w[2], v[2]); atomicMax(&w[3], v[3]);
+			atomicMax(&w[4], v[4]);
+			v[0] = 0xFFFFFFFFu; v[1] = 0u; v[2] = 0xFFFFFFFFu; v[3] = 0u; v[4] = 0u;
+		}
+		cur = s;
 		const float4 a = bmin[i], b = bmax[i];
 		const uint32_t cy = encode_float(0.5f * (a.y + b.y)), cz = encode_float(0.5f * (a.z + b.z));
-		v[0] = cy; v[1] = cy; v[2] = cz; v[3] = cz;
-		v[4] = encode_float(fmaxf(b.y - a.y, b.z - a.z));
+		v[0] = min(v[0], cy); v[1] = max(v[1], cy); v[2] = min(v[2], cz); v[3] = max(v[3], cz);
+		v[4] = max(v[4], encode_float(fmaxf(b.y - a.y, b.z - a.z)));
 	}
-	const int s0 = __shfl(s, 0, 64);
-	const bool uniform = __all(s == s0 || s < 0);
-	if (uniform)
+	if (!blockUniform)
 	{
-		for (int o = 32; o > 0; o >>= 1)
+		if (cur >= 0)
 		{
-			v[0] = min(v[0], __shfl_xor(v[0], o, 64));
-			v[1] = max(v[1], __shfl_xor(v[1], o, 64));
-			v[2] = min(v[2], __shfl_xor(v[2], o, 64));
-			v[3] = max(v[3], __shfl_xor(v[3], o, 64));
-			v[4] = max(v[4], __shfl_xor(v[4], o, 64));
+			uint32_t* w = acc + 5 * int64_t(cur);
+			atomicMin(&w[0], v[0]); atomicMax(&w[1], v[1]); atomicMin(&w[2], v[2]); atomicMax(&w[3], v[3]);
+			atomicMax(&w[4], v[4]);
 		}
-		if ((threadIdx.x & 63) != 0 || s0 < 0)
-			return;
-		s = s0;
-	}
-	else if (!valid)
 		return;
-	uint32_t* w = acc + 5 * int64_t(s);
-	atomicMin(&w[0], v[0]);
-	atomicMax(&w[1], v[1]);
-	atomicMin(&w[2], v[2]);
-	atomicMax(&w[3], v[3]);
-	atomicMax(&w[4], v[4]);
+	}
+	for (int k = 0; k < 5; k++)
+		red[k][threadIdx.x] = v[k];
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1)
+	{
+		if (threadIdx.x < o)
+		{
+			red[0][threadIdx.x] = min(red[0][threadIdx.x], red[0][threadIdx.x + o]);
+			red[1][threadIdx.x] = max(red[1][threadIdx.x], red[1][threadIdx.x + o]);
+			red[2][threadIdx.x] = min(red[2][threadIdx.x], red[2][threadIdx.x + o]);
+			red[3][threadIdx.x] = max(red[3][threadIdx.x], red[3][threadIdx.x + o]);
+			red[4][threadIdx.x] = max(red[4][threadIdx.x], red[4][threadIdx.x + o]);
+		}
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+	{
+		uint32_t* w = acc + 5 * int64_t(scene[base]);
+		atomicMin(&w[0], red[0][0]); atomicMax(&w[1], red[1][0]); atomicMin(&w[2], red[2][0]);
+		atomicMax(&w[3], red[3][0]); atomicMax(&w[4], red[4][0]);
+	}
 }
 
 __global__ void k_grid_init(int nScenes, uint32_t* acc)
@@ -579,11 +603,30 @@ __global__ void k_fill_u64(uint64_t* p, int n, uint64_t v)
 		p[i] = v;
 }
 
-__global__ void k_color_min(int C, const int* conA, const int* conB, const uint64_t* conKey, const int* conColor,
-                            unsigned long long* best)
+// Colouring kernels run over the still-uncoloured constraints: list[i] when a
+// compacted list is given, else i.
+PXH_D int listed(const int* list, int i) { return list ? list[i] : i; }
+
+__global__ void k_color_reset(int m, const int* list, const int* conA, const int* conB, unsigned long long* best)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	if (c >= C || conColor[c] >= 0)
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m)
+		return;
+	const int c = listed(list, i);
+	best[conA[c]] = ~0ull;
+	const int b = conB[c];
+	if (b >= 0)
+		best[b] = ~0ull;
+}
+
+__global__ void k_color_min(int m, const int* list, const int* conA, const int* conB, const uint64_t* conKey,
+                            const int* conColor, unsigned long long* best)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m)
+		return;
+	const int c = listed(list, i);
+	if (conColor[c] >= 0)
 		return;
 	const unsigned long long k = conKey[c];
 	atomicMin(&best[conA[c]], k);
@@ -596,33 +639,63 @@ __global__ void k_color_min(int C, const int* conA, const int* conB, const uint6
 // (first fit on 64-bit per-body colour masks; colours past 63 fall back to
 // one colour per round, 64 + round). Minima of one round share no body, so
 // their mask updates never race with each other's reads.
-__global__ void k_color_pick(int C, int round, const int* conA, const int* conB, const uint64_t* conKey, int* conColor,
-                             const unsigned long long* best, unsigned long long* mask, uint32_t* colored)
+__global__ void k_color_pick(int m, const int* list, int round, const int* conA, const int* conB,
+                             const uint64_t* conKey, int* conColor, const unsigned long long* best,
+                             unsigned long long* mask)
 {
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	bool picked = false;
-	if (c < C && conColor[c] < 0)
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= m)
+		return;
+	const int c = listed(list, i);
+	if (conColor[c] >= 0)
+		return;
+	const unsigned long long k = conKey[c];
+	const int a = conA[c], b = conB[c];
+	if (best[a] == k && (b < 0 || best[b] == k))
 	{
-		const unsigned long long k = conKey[c];
-		const int a = conA[c], b = conB[c];
-		if (best[a] == k && (b < 0 || best[b] == k))
+		const unsigned long long used = mask[a] | (b >= 0 ? mask[b] : 0ull);
+		const int col = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
+		conColor[c] = col;
+		if (col < 64)
 		{
-			const unsigned long long used = mask[a] | (b >= 0 ? mask[b] : 0ull);
-			int col = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
-			conColor[c] = col;
-			if (col < 64)
-			{
-				atomicOr(&mask[a], 1ull << col);
-				if (b >= 0)
-					atomicOr(&mask[b], 1ull << col);
-			}
-			picked = true;
+			atomicOr(&mask[a], 1ull << col);
+			if (b >= 0)
+				atomicOr(&mask[b], 1ull << col);
 		}
 	}
-	// one counter update per wave
-	const unsigned long long bal = __ballot(picked);
-	if ((threadIdx.x & 63) == 0 && bal)
-		atomicAdd(colored, uint32_t(__popcll(bal)));
+}
+
+__global__ void k_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m)
+		flags[i] = conColor[listed(list, i)] < 0 ? 1u : 0u;
+}
+
+__global__ void k_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m && flags[i])
+		out[scanned[i]] = listed(list, i);
+}
+
+// block-level maximum then one atomic per block (grid-stride)
+__global__ void __launch_bounds__(256) k_max_color(int C, const int* conColor, uint32_t* maxColor)
+{
+	__shared__ int red[256];
+	int v = 0;
+	for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < C; c += gridDim.x * blockDim.x)
+		v = max(v, conColor[c]);
+	red[threadIdx.x] = v;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1)
+	{
+		if (threadIdx.x < o)
+			red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + o]);
+		__syncthreads();
+	}
+	if (threadIdx.x == 0)
+		atomicMax(maxColor, uint32_t(red[0]));
 }
 
 // colour-ordered constraint list by a stable radix sort of (colour, index)
@@ -634,16 +707,6 @@ __global__ void k_color_keys(int C, const int* conColor, uint64_t* keys, uint32_
 		keys[c] = (uint64_t(uint32_t(conColor[c])) << 32) | uint32_t(c);
 		vals[c] = uint32_t(c);
 	}
-}
-
-__global__ void k_max_color(int C, const int* conColor, uint32_t* maxColor)
-{
-	const int c = blockIdx.x * blockDim.x + threadIdx.x;
-	int v = c < C ? conColor[c] : 0;
-	for (int o = 32; o > 0; o >>= 1)
-		v = max(v, __shfl_xor(v, o, 64));
-	if ((threadIdx.x & 63) == 0)
-		atomicMax(maxColor, uint32_t(v));
 }
 
 // ---------------------------------------------------------------------------
@@ -1225,7 +1288,8 @@ void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin,
                        float4* gridF, int2* gridI, uint32_t* rows, hipStream_t st)
 {
 	PXH_LAUNCH(k_grid_init, nScenes, nScenes, acc);
-	PXH_LAUNCH(k_grid_reduce, n, n, bmin, bmax, scene, acc);
+	if (n > 0)
+		k_grid_reduce<<<nblk(n, 256 * GRID_ITEMS), 256, 0, st>>>(n, bmin, bmax, scene, acc);
 	PXH_LAUNCH(k_grid_finish, nScenes, nScenes, acc, gridF, gridI, rows);
 }
 void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
@@ -1277,13 +1341,23 @@ void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, 
 	PXH_LAUNCH(k_con_build, S, S, P, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, conA, conB, conSlot, conKey,
 	           conColor);
 }
-void launch_color_round(int C, int n, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint64_t* mask, uint32_t* colored, hipStream_t st)
+void launch_color_round(int m, const int* list, int round, const int* conA, const int* conB, const uint64_t* conKey,
+                        int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st)
 {
-	PXH_LAUNCH(k_fill_u64, n, best, n, ~0ull);
-	PXH_LAUNCH(k_color_min, C, C, conA, conB, conKey, conColor, reinterpret_cast<unsigned long long*>(best));
-	PXH_LAUNCH(k_color_pick, C, C, round, conA, conB, conKey, conColor,
-	           reinterpret_cast<const unsigned long long*>(best), reinterpret_cast<unsigned long long*>(mask), colored);
+	unsigned long long* bst = reinterpret_cast<unsigned long long*>(best);
+	PXH_LAUNCH(k_color_reset, m, m, list, conA, conB, bst);
+	PXH_LAUNCH(k_color_min, m, m, list, conA, conB, conKey, conColor, bst);
+	PXH_LAUNCH(k_color_pick, m, m, list, round, conA, conB, conKey, conColor, bst,
+	           reinterpret_cast<unsigned long long*>(mask));
+}
+void launch_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags, hipStream_t st)
+{
+	PXH_LAUNCH(k_color_uncolored, m, m, list, conColor, flags);
+}
+void launch_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out,
+                          hipStream_t st)
+{
+	PXH_LAUNCH(k_color_compact, m, m, list, flags, scanned, out);
 }
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st) { PXH_LAUNCH(k_fill_u64, n, p, n, v); }
 void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st)
@@ -1292,7 +1366,8 @@ void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* val
 }
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st)
 {
-	PXH_LAUNCH(k_max_color, C, C, conColor, maxColor);
+	if (C > 0)
+		k_max_color<<<std::min(nblk(C, 256), 256), 256, 0, st>>>(C, conColor, maxColor);
 }
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,

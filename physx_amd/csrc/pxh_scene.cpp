@@ -131,6 +131,8 @@ struct pxh_scene_t
 	DBuf<float4> ctN, ctP;
 	DBuf<int> conA, conB, conSlot, conColor, order;
 	DBuf<uint64_t> conKey, best, mask;
+	DBuf<int> colList, colList2;
+	DBuf<uint32_t> colFlags;
 	DBuf<uint32_t> colorCount, scratch;
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
 	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0;
@@ -397,6 +399,9 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->conKey.release();
 	s->best.release();
 	s->mask.release();
+	s->colList.release();
+	s->colList2.release();
+	s->colFlags.release();
 	s->pairs.release();
 	s->gridI.release();
 	s->rhI.release();
@@ -591,28 +596,43 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_con_build(S, P, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
 	                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
 
-	// 4. colouring: rounds of local key minima, each taking its first free colour
+	// 4. colouring: rounds of local key minima, each taking its first free
+	//    colour; the uncoloured constraints are compacted between batches
 	PXH_CHECK(s->best.ensure(n));
 	PXH_CHECK(s->mask.ensure(n));
 	PXH_CHECK(s->scratch.ensure(4));
+	PXH_CHECK(s->colList.ensure(std::max(C, 1)));
+	PXH_CHECK(s->colList2.ensure(std::max(C, 1)));
+	PXH_CHECK(s->colFlags.ensure(size_t(C) + 1));
+	PXH_CHECK(s->counts.ensure(size_t(C) + 1));
 	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, 2 * sizeof(uint32_t), st));
 	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
 	int rounds = 0;
 	const int kMaxRounds = 960;
-	uint32_t colored = 0;
-	int batch = 4;
-	while (int(colored) < C)
+	int remaining = C;
+	const int* list = nullptr;
+	int batch = 8;
+	while (remaining > 0)
 	{
 		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
-			pxh::launch_color_round(C, n, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p, s->best.p,
-			                        s->mask.p, s->scratch.p, st);
-		batch = 2;
-		const uint32_t* src[1] = {s->scratch.p};
+			pxh::launch_color_round(remaining, list, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p,
+			                        s->best.p, s->mask.p, st);
+		batch = 4;
+		// compact the uncoloured constraints
+		pxh::launch_color_uncolored(remaining, list, s->conColor.p, s->colFlags.p, st);
+		PXH_CHECK(hipMemsetAsync(s->colFlags.p + remaining, 0, sizeof(uint32_t), st));
+		PXH_CHECK(hipMemcpyAsync(s->counts.p, s->colFlags.p, sizeof(uint32_t) * (remaining + 1),
+		                         hipMemcpyDeviceToDevice, st));
+		pxh::launch_scan(s->counts.p, int64_t(remaining) + 1, s->scanTmp.p, st);
+		int* out = list == s->colList.p ? s->colList2.p : s->colList.p;
+		pxh::launch_color_compact(remaining, list, s->colFlags.p, s->counts.p, out, st);
+		const uint32_t* src[1] = {s->counts.p + remaining};
 		r = readCounts(st, s->hostCounts, src, 1);
 		if (r)
 			return r;
-		colored = s->hostCounts[0];
-		if (rounds >= kMaxRounds && int(colored) < C)
+		remaining = int(s->hostCounts[0]);
+		list = out;
+		if (rounds >= kMaxRounds && remaining > 0)
 			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring did not finish in 960 rounds");
 	}
 	int nColors = 0;
