@@ -645,6 +645,63 @@ static int box_box_edge(v3 e0, mxf t0, v3 e1, mxf t1, float cd, pcontact* mc)
 	}
 	if (best > cd)
 		return 0;
+	if (best > 0.0f)
+	{
+		/* separated: GJK's closest points, here by exhaustive feature pairs
+		 * (vertex against box, edge against edge) */
+		float dmin = 3.402823466e+38f;
+		v3 qa = t0.p, qb = t1.p;
+		for (int side = 0; side < 2; side++)
+		{
+			const mxf tv = side ? t1 : t0, tb = side ? t0 : t1;
+			const v3 ev = side ? e1 : e0, eb = side ? e0 : e1;
+			for (int k = 0; k < 8; k++)
+			{
+				const v3 lv = V((k & 4) ? -ev.x : ev.x, (k & 2) ? -ev.y : ev.y, (k & 1) ? -ev.z : ev.z);
+				const v3 wv = mxtrans(tv, lv);
+				const v3 lb = mtmul(tb.r, sub(wv, tb.p));
+				const v3 cl = V(fminf(fmaxf(lb.x, -eb.x), eb.x), fminf(fmaxf(lb.y, -eb.y), eb.y), fminf(fmaxf(lb.z, -eb.z), eb.z));
+				const v3 wq = mxtrans(tb, cl);
+				const float dd = len2(sub(wv, wq));
+				if (dd < dmin) { dmin = dd; qa = side ? wq : wv; qb = side ? wv : wq; }
+			}
+		}
+		for (int ia = 0; ia < 12; ia++)
+			for (int ib = 0; ib < 12; ib++)
+			{
+				v3 s0[2], s1[2];
+				for (int side = 0; side < 2; side++)
+				{
+					const int e = side ? ib : ia;
+					const mxf t = side ? t1 : t0;
+					const v3 ex = side ? e1 : e0;
+					const int ax = e / 4, u = (e & 1) ? -1 : 1, w = (e & 2) ? -1 : 1;
+					v3 c = V(0, 0, 0), dlt = V(0, 0, 0);
+					const float ev[3] = {ex.x, ex.y, ex.z};
+					float cc[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
+					cc[(ax + 1) % 3] = u * ev[(ax + 1) % 3];
+					cc[(ax + 2) % 3] = w * ev[(ax + 2) % 3];
+					dd[ax] = ev[ax];
+					c = V(cc[0], cc[1], cc[2]);
+					dlt = V(dd[0], dd[1], dd[2]);
+					(side ? s1 : s0)[0] = mxtrans(t, sub(c, dlt));
+					(side ? s1 : s0)[1] = mxtrans(t, add(c, dlt));
+				}
+				v3 c0, c1;
+				seg_closest(s0[0], s0[1], s1[0], s1[1], &c0, &c1);
+				const float dd = len2(sub(c0, c1));
+				if (dd < dmin) { dmin = dd; qa = c0; qb = c1; }
+			}
+		const float dist = sqrtf(dmin);
+		if (dist > cd)
+			return 0;
+		const v3 nrm = dist > 0.0f ? scl(sub(qa, qb), 1.0f / dist) : bestN;
+		mc[0].la = mtmul(t0.r, sub(qa, t0.p));
+		mc[0].lb = mtmul(t1.r, sub(qb, t1.p));
+		mc[0].ln = mtmul(t1.r, nrm);
+		mc[0].pen = dist;
+		return 1;
+	}
 	v3 pB;
 	if (bi >= 0)
 	{

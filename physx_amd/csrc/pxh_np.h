@@ -493,6 +493,72 @@ PXH_D int boxBoxEdge(f3 e0, const mxf& t0, f3 e1, const mxf& t1, float cd, Conta
 	}
 	if (best > cd)
 		return 0;
+	if (best > 0.0f)
+	{
+		// separated: GJK's closest points, here by exhaustive feature pairs
+		// (vertex against box, edge against edge)
+		float dmin = FLT_BIG;
+		f3 qa = t0.p, qb = t1.p;
+		for (int side = 0; side < 2; side++)
+		{
+			const mxf& tv = side ? t1 : t0;
+			const mxf& tb = side ? t0 : t1;
+			const f3 ev = side ? e1 : e0, eb = side ? e0 : e1;
+			for (int k = 0; k < 8; k++)
+			{
+				const f3 lv = mk((k & 4) ? -ev.x : ev.x, (k & 2) ? -ev.y : ev.y, (k & 1) ? -ev.z : ev.z);
+				const f3 wv = mxtrans(tv, lv);
+				const f3 lb = mtmul(tb.r, wv - tb.p);
+				const f3 cl = mk(fminf(fmaxf(lb.x, -eb.x), eb.x), fminf(fmaxf(lb.y, -eb.y), eb.y),
+				                 fminf(fmaxf(lb.z, -eb.z), eb.z));
+				const f3 wq = mxtrans(tb, cl);
+				const f3 dv = wv - wq;
+				const float dd = dot(dv, dv);
+				if (dd < dmin) { dmin = dd; qa = side ? wq : wv; qb = side ? wv : wq; }
+			}
+		}
+		for (int ia = 0; ia < 12; ia++)
+		{
+			f3 s0a, s0b;
+			{
+				const int ax = ia / 4;
+				const float ev[3] = {e0.x, e0.y, e0.z};
+				float cc[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
+				cc[(ax + 1) % 3] = ((ia & 1) ? -1.0f : 1.0f) * ev[(ax + 1) % 3];
+				cc[(ax + 2) % 3] = ((ia & 2) ? -1.0f : 1.0f) * ev[(ax + 2) % 3];
+				dd[ax] = ev[ax];
+				s0a = mxtrans(t0, mk(cc[0] - dd[0], cc[1] - dd[1], cc[2] - dd[2]));
+				s0b = mxtrans(t0, mk(cc[0] + dd[0], cc[1] + dd[1], cc[2] + dd[2]));
+			}
+			for (int ib = 0; ib < 12; ib++)
+			{
+				const int ax = ib / 4;
+				const float ev[3] = {e1.x, e1.y, e1.z};
+				float cc[3] = {0, 0, 0}, dd[3] = {0, 0, 0};
+				cc[(ax + 1) % 3] = ((ib & 1) ? -1.0f : 1.0f) * ev[(ax + 1) % 3];
+				cc[(ax + 2) % 3] = ((ib & 2) ? -1.0f : 1.0f) * ev[(ax + 2) % 3];
+				dd[ax] = ev[ax];
+				const f3 s1a = mxtrans(t1, mk(cc[0] - dd[0], cc[1] - dd[1], cc[2] - dd[2]));
+				const f3 s1b = mxtrans(t1, mk(cc[0] + dd[0], cc[1] + dd[1], cc[2] + dd[2]));
+				f3 c0, c1;
+				segClosest(s0a, s0b, s1a, s1b, c0, c1);
+				const f3 dv = c0 - c1;
+				const float dd2 = dot(dv, dv);
+				if (dd2 < dmin) { dmin = dd2; qa = c0; qb = c1; }
+			}
+		}
+		const float dist = sqrtf(dmin);
+		if (dist > cd)
+			return 0;
+		const f3 nrm = dist > 0.0f ? (qa - qb) * (1.0f / dist) : bestN;
+		const f3 lb = mtmul(t1.r, qb - t1.p);
+		const f3 ln = mtmul(t1.r, nrm);
+		o.n = normalize(mmul(t1.r, ln));
+		o.count = 1;
+		o.p[0] = mxtrans(t1, lb);
+		o.sep[0] = dist;
+		return 1;
+	}
 	const f3 a[3] = {t0.r.c0, t0.r.c1, t0.r.c2}, b[3] = {t1.r.c0, t1.r.c1, t1.r.c2};
 	const float ea[3] = {e0.x, e0.y, e0.z}, eb[3] = {e1.x, e1.y, e1.z};
 	f3 pB;

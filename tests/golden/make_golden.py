@@ -1,0 +1,169 @@
+"""Writes the golden vectors under tests/golden/ from the reference PhysX
+5.6.1 CPU SDK, compiled from the reference sources into
+oracle/_ref/libpxref.so by oracle/ref.mk (run `make -f oracle/ref.mk -j8`).
+
+The reference ships no test vectors for this path, so these fixtures are the
+reference's own outputs on seeded inputs:
+  contacts.npz  PxGenerateContacts (fresh PxCache) for sphere-sphere,
+                sphere-plane, sphere-box, plane-box and box-box pairs
+  broadphase.npz  PxCreateBroadPhase(eABP) created pairs for seeded boxes
+  step.npz      one PxScene TGS step (4/1 iterations, dt 1/60) of single bodies
+                on a plane: positions, orientations, velocities afterwards
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+import oracle_libs as L  # noqa: E402
+from physx_amd import scenes as S  # noqa: E402
+
+KINDS = ("ss", "sp", "sb", "pb", "bb")
+PLANE_POSE = np.array([0, 0, 0, 0, 0, np.sqrt(0.5), np.sqrt(0.5)], np.float32)
+
+
+def rq(rng):
+    q = rng.normal(size=4)
+    return q / np.linalg.norm(q)
+
+
+def qm(q):
+    x, y, z, w = q
+    return np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                     [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                     [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
+
+
+def box_sat_sep(e0, R0, c0, e1, R1, c1):
+    A = [R0[:, i] for i in range(3)]
+    B = [R1[:, i] for i in range(3)]
+    best = -1e9
+    for L_ in A + B + [np.cross(a, b) for a in A for b in B]:
+        nl = np.linalg.norm(L_)
+        if nl < 1e-6:
+            continue
+        L_ = L_ / nl
+        ra = sum(e0[i] * abs(A[i] @ L_) for i in range(3))
+        rb = sum(e1[i] * abs(B[i] @ L_) for i in range(3))
+        best = max(best, abs((c0 - c1) @ L_) - ra - rb)
+    return best
+
+
+def pair(rng, kind):
+    """Seeded pair near contact: (type0, geom0, pose0, type1, geom1, pose1)."""
+    g0 = np.zeros(4, np.float32)
+    g1 = np.zeros(4, np.float32)
+    if kind == "ss":
+        g0[0], g1[0] = rng.uniform(.1, .3, 2)
+        d = rng.normal(size=3); d /= np.linalg.norm(d)
+        p0 = np.r_[d * (g0[0] + g1[0] + rng.uniform(-.05, .035)), 0, 0, 0, 1]
+        p1 = np.r_[0, 0, 0, 0, 0, 0, 1]
+        return 0, g0, p0, 0, g1, p1
+    if kind == "sp":
+        g0[0] = rng.uniform(.1, .3)
+        p0 = np.r_[rng.uniform(-1, 1), g0[0] + rng.uniform(-.05, .035), rng.uniform(-1, 1), 0, 0, 0, 1]
+        return 0, g0, p0, 1, g1, PLANE_POSE
+    if kind == "sb":
+        g0[0] = rng.uniform(.1, .3)
+        g1[:3] = rng.uniform(.1, .3, 3)
+        p0 = np.r_[rng.uniform(-.5, .5, 3), 0, 0, 0, 1]
+        p1 = np.r_[rng.uniform(-.05, .05, 3), rq(rng)]
+        return 0, g0, p0, 3, g1, p1
+    if kind == "pb":
+        # at most 4 corners within the contact distance (no reduction needed)
+        g1[:3] = rng.uniform(.1, .5, 3)
+        while True:
+            q = rq(rng)
+            R = qm(q)
+            s = np.array([[a, b, c] for a in (1, -1) for b in (1, -1) for c in (1, -1)], np.float64)
+            low = -((R @ (s * g1[:3]).T).T[:, 1]).max()
+            y = -low + rng.uniform(-.03, .035)
+            h = (R @ (s * g1[:3]).T).T[:, 1] + y
+            if (h < 0.04).sum() <= 4:
+                break
+        return 1, g1 * 0, PLANE_POSE, 3, g1, np.r_[rng.uniform(-1, 1), y, rng.uniform(-1, 1), q]
+    g0[:3] = rng.uniform(.1, .3, 3)
+    g1[:3] = rng.uniform(.1, .3, 3)
+    q0, q1 = rq(rng), rq(rng)
+    d = rng.normal(size=3); d /= np.linalg.norm(d)
+    target = rng.uniform(-0.03, 0.035)
+    lo, hi = 0.0, 2.0
+    for _ in range(40):
+        mid = 0.5 * (lo + hi)
+        if box_sat_sep(g0, qm(q0), d * mid, g1, qm(q1), np.zeros(3)) < target:
+            lo = mid
+        else:
+            hi = mid
+    return 3, g0, np.r_[d * lo, q0], 3, g1, np.r_[0, 0, 0, q1]
+
+
+def make_contacts(n_per_kind=400, seed=1234):
+    rng = np.random.default_rng(seed)
+    rows = {k: [] for k in ("kind", "t0", "g0", "p0", "t1", "g1", "p1", "count", "contacts")}
+    for ki, kind in enumerate(KINDS):
+        for _ in range(n_per_kind):
+            t0, g0, p0, t1, g1, p1 = pair(rng, kind)
+            c = L.contacts("ref", t0, g0, p0, t1, g1, p1, 0.04, 1.0)
+            out = np.zeros((4, 8), np.float32)
+            out[: c.shape[0]] = c[:4]
+            rows["kind"].append(ki); rows["t0"].append(t0); rows["g0"].append(g0); rows["p0"].append(p0)
+            rows["t1"].append(t1); rows["g1"].append(g1); rows["p1"].append(p1)
+            rows["count"].append(c.shape[0]); rows["contacts"].append(out)
+    np.savez_compressed(os.path.join(HERE, "contacts.npz"), **{k: np.array(v) for k, v in rows.items()},
+                        contact_distance=np.float32(0.04), tolerance_length=np.float32(1.0))
+
+
+def make_broadphase(n=3000, seed=99):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, 15, (n, 3)).astype(np.float32)
+    h = rng.uniform(0.1, 0.4, (n, 3)).astype(np.float32)
+    b = np.concatenate([c - h, c + h], 1).astype(np.float32)
+    d = rng.uniform(0.0, 0.05, n).astype(np.float32)
+    pairs = L.bp_pairs_ref(b, d)
+    np.savez_compressed(os.path.join(HERE, "broadphase.npz"), bounds=b, distances=d, pairs=pairs.astype(np.uint32))
+
+
+STEP_CASES = [
+    # name, type, dims, pose, lin vel, ang vel
+    ("free_fall", 0, [0.3, 0, 0, 0], [0, 5, 0, 0, 0, 0, 1], [0, 0, 0], [0, 0, 0]),
+    ("sphere_rest", 0, [0.3, 0, 0, 0], [0, 0.29, 0, 0, 0, 0, 1], [0, 0, 0], [0, 0, 0]),
+    ("sphere_roll", 0, [0.3, 0, 0, 0], [0, 0.299, 0, 0, 0, 0, 1], [2, 0, 0], [0, 0, 0]),
+    ("box_rest", 3, [0.3, 0.2, 0.25, 0], [0, 0.19, 0, 0, 0, 0, 1], [0, 0, 0], [0, 0, 0]),
+    ("box_slide", 3, [0.3, 0.2, 0.25, 0], [0, 0.199, 0, 0, 0, 0, 1], [2, 0, 0.5], [0, 0, 0]),
+    ("box_spin", 3, [0.3, 0.2, 0.25, 0], [0, 0.199, 0, 0, 0, 0, 1], [0.5, 0, 0], [0, 3, 1]),
+    ("box_tilted", 3, [0.3, 0.2, 0.25, 0], [0, 0.3, 0, 0, 0, np.sin(0.3), np.cos(0.3)], [0, 0, 0], [0, 0, 0]),
+]
+
+
+def make_step(dt=1.0 / 60.0):
+    out = {"name": [], "type": [], "dims": [], "pose0": [], "lv0": [], "av0": [], "pose1": [], "lv1": [], "av1": []}
+    for name, t, dims, pose, lv, av in STEP_CASES:
+        dims = np.array([dims], np.float32)
+        m, inertia = S.mass_props(np.array([t]), dims, 1000.0)
+        sc = S.Scene(np.array([t], np.int32), dims, np.array([pose], np.float32), np.array([lv], np.float32),
+                     np.array([av], np.float32), m, inertia, np.zeros(1, np.int32),
+                     np.array([[0, 1, 0, 0]], np.float32), np.zeros(1, np.int32))
+        r = L.RefScene(sc, "tgs", 4, 1, threads=1)
+        r.step(dt)
+        p1, l1, a1 = r.bodies()
+        for k, v in (("name", name), ("type", t), ("dims", dims[0]), ("pose0", np.float32(pose)),
+                     ("lv0", np.float32(lv)), ("av0", np.float32(av)), ("pose1", p1[0]), ("lv1", l1[0]),
+                     ("av1", a1[0])):
+            out[k].append(v)
+    np.savez_compressed(os.path.join(HERE, "step.npz"), **{k: np.array(v) for k, v in out.items()},
+                        dt=np.float32(dt), pos_iters=np.int32(4), vel_iters=np.int32(1))
+
+
+if __name__ == "__main__":
+    assert L.have_ref(), "build oracle/_ref/libpxref.so first (make -f oracle/ref.mk -j8)"
+    make_contacts()
+    make_broadphase()
+    make_step()
+    for f in ("contacts.npz", "broadphase.npz", "step.npz"):
+        print(f, os.path.getsize(os.path.join(HERE, f)))
