@@ -59,9 +59,9 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: every box of this pool has one MI355X, and the step of one scene "
-              "does not shard across GPUs in this build (see DESIGN.md for how it would). Run with --gpus 1.",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the step of one scene runs on a single MI355X in this build "
+              "(independent worlds would be placed on separate GPUs; one large scene would need slab halos). "
+              "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
