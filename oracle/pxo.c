@@ -1026,10 +1026,21 @@ long long pxo_bp_pairs(const float* bounds, const float* dist, const int32_t* wo
 /* World and TGS step                                                         */
 /* ------------------------------------------------------------------------- */
 
+/* Friction patch kept between steps (lowleveldynamics/src/DyFrictionPatch.h):
+ * body-space normals and anchors of both bodies, the anchor count and the
+ * broken flag the last solve pass wrote. */
+typedef struct {
+	uint64_t key;        /* pair key: a << 32 | b, planes 0x80000000 | plane */
+	v3 n0, n1, a0[2], a1[2];
+	int count, broken;
+} fpatch;
+
 typedef struct {
 	int a, b;            /* body indices; b < 0: static plane -1-b */
 	int color;
 	uint64_t key;
+	uint64_t pkey;       /* pair key of the friction patch */
+	fpatch fp;
 	cset c;              /* contacts, normal from b towards a */
 	/* prepared rows */
 	float invMassA, invMassB;
@@ -1058,6 +1069,9 @@ typedef struct {
 	pxo_con* con;
 	long long ncon, concap;
 	long long stats[6];
+	/* friction patches of the previous step, sorted by pair key */
+	fpatch* prev;
+	long long nprev;
 } pxo_world;
 
 void pxo_default_desc(pxo_desc* d)
@@ -1095,6 +1109,7 @@ void pxo_world_release(void* wp)
 	if (!w) return;
 	free(w->type); free(w->scene); free(w->local); free(w->dims); free(w->pose); free(w->lv); free(w->av);
 	free(w->invMass); free(w->invI); free(w->pn); free(w->pd); free(w->pscene); free(w->plocal); free(w->con);
+	free(w->prev);
 	free(w);
 }
 
@@ -1249,6 +1264,25 @@ static void push_con(pxo_world* w, int a, int b, const cset* c)
 	const uint32_t la = (uint32_t)w->local[a];
 	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b] : (0x80000000u | (uint32_t)w->plocal[-1 - b]);
 	k->key = con_key(la, lb);
+	k->pkey = ((uint64_t)(uint32_t)a << 32) | (b >= 0 ? (uint32_t)b : (0x80000000u | (uint32_t)(-1 - b)));
+}
+
+static int cmp_patch(const void* x, const void* y)
+{
+	const uint64_t a = ((const fpatch*)x)->key, b = ((const fpatch*)y)->key;
+	return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+static const fpatch* find_patch(const pxo_world* w, uint64_t key)
+{
+	long long lo = 0, hi = w->nprev;
+	while (lo < hi)
+	{
+		const long long mid = (lo + hi) / 2;
+		if (w->prev[mid].key < key) lo = mid + 1;
+		else hi = mid;
+	}
+	return lo < w->nprev && w->prev[lo].key == key ? &w->prev[lo] : NULL;
 }
 
 /* Solver body (PxTGSSolverBodyVel + PxTGSSolverBodyTxInertia + data),
@@ -1327,30 +1361,93 @@ static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbod
 		k->recipResp[i] = recip;
 		k->force[i] = 0.0f;
 	}
-	/* friction patch: anchors (growPatches) among contacts closer than the offset threshold */
-	int nAnch = 0;
-	v3 wa[2];
-	float pd2 = 0.0f;
-	const float thr = w->d.friction_offset_threshold + restDist;
-	for (int j = 0; j < k->c.count; j++)
+	/* friction patch. The previous step's patch of this pair is kept when it
+	 * is unbroken, has anchors, its body normals still agree, no anchor has
+	 * drifted apart along the normal (getFrictionPatches, isSeparated:
+	 * DyContactPrepShared.h:53-127) and its world normal matches this step's
+	 * contact normal (correlatePatches, DyFrictionCorrelation.cpp:136-184);
+	 * otherwise a new patch starts from the contacts. */
+	fpatch* fp = &k->fp;
+	const fpatch* pp = find_patch(w, k->pkey);
+	int kept = 0;
+	if (pp && !pp->broken && pp->count != 0)
 	{
-		const v3 wp = k->c.p[j];
-		if (k->c.sep[j] < thr)
+		const xf b10 = xinvmul(f0, f1);
+		if (dot(pp->n0, qrot(b10.q, pp->n1)) > 0.999f)
 		{
-			if (nAnch == 0) { wa[0] = wp; nAnch = 1; }
-			else if (nAnch == 1)
-			{
-				pd2 = len2(sub(wp, wa[0]));
-				if (pd2 > 1e-8f) { wa[1] = wp; nAnch = 2; }
-			}
-			else
-			{
-				const float d0 = len2(sub(wp, wa[0])), d1 = len2(sub(wp, wa[1]));
-				if (d0 > d1) { if (d0 > pd2) { wa[1] = wp; pd2 = d0; } }
-				else if (d1 > pd2) { wa[0] = wp; pd2 = d1; }
-			}
+			int separated = 0;
+			for (int a = 0; a < pp->count; a++)
+				if (!(fabsf(dot(sub(pp->a0[a], xtrans(b10, pp->a1[a])), pp->n0)) < w->d.correlation_distance))
+					separated = 1;
+			if (!separated && !(dot(n, qrot(f0.q, pp->n0)) < 0.999f))
+				kept = 1;
 		}
 	}
+	if (kept)
+		*fp = *pp;
+	else
+	{
+		fp->n0 = qrotinv(f0.q, n);
+		fp->n1 = qrotinv(f1.q, n);
+		fp->count = 0;
+	}
+	fp->key = k->pkey;
+	fp->broken = 0;
+	/* anchors (growPatches, DyFrictionCorrelation.cpp:189-316): two kept
+	 * anchors stay unless they are closer than half the diagonal of this
+	 * step's contact bounds; one kept anchor is completed from the contacts;
+	 * otherwise up to two contacts closer than the offset threshold */
+	if (fp->count == 2)
+	{
+		v3 mn = k->c.p[0], mx = k->c.p[0];
+		for (int j = 1; j < k->c.count; j++)
+		{
+			const v3 q = k->c.p[j];
+			mn = V(fminf(mn.x, q.x), fminf(mn.y, q.y), fminf(mn.z, q.z));
+			mx = V(fmaxf(mx.x, q.x), fmaxf(mx.y, q.y), fmaxf(mx.z, q.z));
+		}
+		const float diag2 = len2(sub(mx, mn));
+		const float anch2 = len2(sub(fp->a0[0], fp->a0[1]));
+		if (!(anch2 * 4.0f >= diag2))
+			fp->count = 0;
+	}
+	if (fp->count != 2)
+	{
+		int nAnch = 0;
+		v3 wa[2];
+		float pd2 = 0.0f;
+		if (fp->count == 1)
+			wa[nAnch++] = xtrans(f0, fp->a0[0]);
+		const float thr = w->d.friction_offset_threshold + restDist;
+		for (int j = 0; j < k->c.count; j++)
+		{
+			const v3 wp = k->c.p[j];
+			if (k->c.sep[j] < thr)
+			{
+				if (nAnch == 0) { wa[0] = wp; nAnch = 1; }
+				else if (nAnch == 1)
+				{
+					pd2 = len2(sub(wp, wa[0]));
+					if (pd2 > 1e-8f) { wa[1] = wp; nAnch = 2; }
+				}
+				else
+				{
+					const float d0 = len2(sub(wp, wa[0])), d1 = len2(sub(wp, wa[1]));
+					if (d0 > d1) { if (d0 > pd2) { wa[1] = wp; pd2 = d0; } }
+					else if (d1 > pd2) { wa[0] = wp; pd2 = d1; }
+				}
+			}
+		}
+		for (int j = fp->count; j < nAnch; j++)
+		{
+			fp->a0[j] = xtransinv(f0, wa[j]);
+			fp->a1[j] = xtransinv(f1, wa[j]);
+		}
+		if (nAnch == 0)
+			fp->a0[0] = fp->a1[0] = V(0.0f, 0.0f, 0.0f);
+		fp->count = nAnch;
+	}
+	const int nAnch = fp->count;
 	k->nFric = 2 * nAnch;
 	if (nAnch == 0)
 		return;
@@ -1366,8 +1463,7 @@ static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbod
 	k->fbiasScale = invStepDt * scale;
 	for (int j = 0; j < nAnch; j++)
 	{
-		const v3 a0 = xtransinv(f0, wa[j]), a1 = xtransinv(f1, wa[j]);
-		const v3 ra = qrot(f0.q, a0), rb = qrot(f1.q, a1);
+		const v3 ra = qrot(f0.q, fp->a0[j]), rb = qrot(f1.q, fp->a1[j]);
 		const v3 err = add(sub(ra, rb), relTr);
 		for (int d = 0; d < 2; d++)
 		{
@@ -1425,6 +1521,7 @@ static void solve_con(pxo_con* k, sbody* b0, sbody* b1, float minPen, float elap
 	if (k->nFric)
 	{
 		const float maxF = mu_s * acc, maxD = mu_d * acc;
+		int broken = 0;
 		for (int i = 0; i < k->nFric; i += 2)
 		{
 			float ti[2], af[2];
@@ -1456,7 +1553,9 @@ static void solve_con(pxo_con* k, sbody* b0, sbody* b1, float minPen, float elap
 			a1 = sub(sub(a1, scl(k->frbXnI[i + 1], d1)), scl(k->frbXnI[i], d0));
 			k->fforce[i] = n0;
 			k->fforce[i + 1] = n1;
+			broken |= clamp;
 		}
+		k->fp.broken = broken; /* the last pass's value is kept (DyTGSContactPrep.cpp:1837, 1907) */
 	}
 	b0->lin = l0; b0->ang = a0;
 	if (b1->invMass > 0.0f) { b1->lin = l1; b1->ang = a1; }
@@ -1542,9 +1641,12 @@ int pxo_world_step(void* wp, float dt)
 	 * with an order-free rule the GPU evaluates in parallel. */
 	uint64_t* best = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
 	uint64_t* mask = (uint64_t*)calloc((size_t)(n ? n : 1), sizeof(uint64_t));
+	long long left = 0;
 	for (long long c = 0; c < w->ncon; c++)
-		w->con[c].color = -1;
-	long long left = w->ncon;
+	{
+		w->con[c].color = w->con[c].b >= 0 ? -1 : -2; /* -2: body-plane, placed after the rounds */
+		left += w->con[c].b >= 0;
+	}
 	int round = 0, ncolors = 0;
 	while (left > 0)
 	{
@@ -1553,14 +1655,14 @@ int pxo_world_step(void* wp, float dt)
 		for (long long c = 0; c < w->ncon; c++)
 		{
 			pxo_con* k = &w->con[c];
-			if (k->color >= 0) continue;
+			if (k->color != -1) continue;
 			if (k->key < best[k->a]) best[k->a] = k->key;
 			if (k->b >= 0 && k->key < best[k->b]) best[k->b] = k->key;
 		}
 		for (long long c = 0; c < w->ncon; c++)
 		{
 			pxo_con* k = &w->con[c];
-			if (k->color >= 0) continue;
+			if (k->color != -1) continue;
 			if (best[k->a] == k->key && (k->b < 0 || best[k->b] == k->key))
 			{
 				const uint64_t used = mask[k->a] | (k->b >= 0 ? mask[k->b] : 0ull);
@@ -1580,6 +1682,25 @@ int pxo_world_step(void* wp, float dt)
 		}
 		round++;
 	}
+	/* body-plane constraints go after every partition of their body's
+	 * body-body constraints, one partition each, in constraint order
+	 * (lowleveldynamics/src/DyConstraintPartition.cpp:168-179, 604-616) */
+	int* next = (int*)calloc((size_t)(n ? n : 1), sizeof(int));
+	for (long long c = 0; c < w->ncon; c++)
+	{
+		const pxo_con* k = &w->con[c];
+		if (k->b < 0) continue;
+		if (k->color + 1 > next[k->a]) next[k->a] = k->color + 1;
+		if (k->color + 1 > next[k->b]) next[k->b] = k->color + 1;
+	}
+	for (long long c = 0; c < w->ncon; c++)
+	{
+		pxo_con* k = &w->con[c];
+		if (k->b >= 0) continue;
+		k->color = next[k->a]++;
+		if (k->color + 1 > ncolors) ncolors = k->color + 1;
+	}
+	free(next);
 	free(best);
 	free(mask);
 	w->stats[4] = ncolors;
@@ -1673,6 +1794,13 @@ int pxo_world_step(void* wp, float dt)
 			}
 	free(order);
 	free(start);
+
+	/* friction patches for the next step */
+	w->prev = (fpatch*)realloc(w->prev, sizeof(fpatch) * (size_t)(w->ncon ? w->ncon : 1));
+	for (long long c = 0; c < w->ncon; c++)
+		w->prev[c] = w->con[c].fp;
+	w->nprev = w->ncon;
+	qsort(w->prev, (size_t)w->nprev, sizeof(fpatch), cmp_patch);
 
 	/* 7. copy back (DyTGSDynamics.cpp:1549-1580) */
 	for (int i = 0; i < n; i++)

@@ -35,6 +35,23 @@ struct Rows
 	float* fF;    // [4] applied friction impulse
 };
 
+// Friction patches kept from one step to the next, one per constraint row
+// (FrictionPatch, lowleveldynamics/src/DyFrictionPatch.h). A patch is six
+// float4 at [6 * o]: body-0 normal | anchor count, body-1 normal, the two
+// body-0 anchors, the two body-1 anchors. Rows of the previous step are found
+// by pair key through an open-addressing table (keys ~0 = empty).
+struct PatchIO
+{
+	const uint64_t* htKey;     // previous step's table (null: no previous step)
+	const int* htVal;          // row of the previous step holding the key
+	uint32_t htMask;           // table size - 1 (power of two)
+	const float4* prev;        // previous step's patches
+	const uint32_t* prevBroken;
+	float4* cur;               // this step's patches
+	uint64_t* curKey;          // this step's pair keys
+	float corrDist;            // friction correlation distance
+};
+
 int64_t scan_tmp_size(int64_t n);
 void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st);
 void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2, int n, int bitBegin,
@@ -67,15 +84,20 @@ void launch_color_compact(int m, const int* list, const uint32_t* flags, const u
                           hipStream_t st);
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st);
 void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
+                          hipStream_t st);
+void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal, uint32_t htMask, hipStream_t st);
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                          float maxAngSq, SolverBodies sb, hipStream_t st);
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
-                 float scale, float restDist, float fot, Rows rows, unsigned long long* counters, hipStream_t st);
+                 float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
+                 hipStream_t st);
+// broken: when not null, each row's friction-clamp flag of this pass is stored there
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  float invStepDt, float biasScale, float scale, hipStream_t st);
+                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st);
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
 void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,

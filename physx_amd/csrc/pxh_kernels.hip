@@ -588,13 +588,14 @@ __global__ void k_con_build(int S, int P, const float4* ctN, const uint32_t* sca
 	conB[c] = b;
 	conSlot[c] = s;
 	conKey[c] = mix64((uint64_t(la) << 32) | lb);
-	conColor[c] = -1;
+	conColor[c] = b >= 0 ? -1 : -2;  // -2: body-plane, placed after the body-body rounds
 }
 
 // ---------------------------------------------------------------------------
-// colouring: a constraint joins colour r when its key is the minimum over the
-// uncoloured constraints of both its dynamic bodies (deterministic, and a
-// function of each world's own content)
+// colouring: a body-body constraint joins colour r when its key is the minimum
+// over the uncoloured constraints of both its bodies (deterministic, and a
+// function of each world's own content); body-plane constraints then follow
+// every colour of their body's body-body constraints
 // ---------------------------------------------------------------------------
 __global__ void k_fill_u64(uint64_t* p, int n, uint64_t v)
 {
@@ -626,7 +627,7 @@ __global__ void k_color_min(int m, const int* list, const int* conA, const int* 
 	if (i >= m)
 		return;
 	const int c = listed(list, i);
-	if (conColor[c] >= 0)
+	if (conColor[c] != -1)
 		return;
 	const unsigned long long k = conKey[c];
 	atomicMin(&best[conA[c]], k);
@@ -647,7 +648,7 @@ __global__ void k_color_pick(int m, const int* list, int round, const int* conA,
 	if (i >= m)
 		return;
 	const int c = listed(list, i);
-	if (conColor[c] >= 0)
+	if (conColor[c] != -1)
 		return;
 	const unsigned long long k = conKey[c];
 	const int a = conA[c], b = conB[c];
@@ -669,7 +670,7 @@ __global__ void k_color_uncolored(int m, const int* list, const int* conColor, u
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < m)
-		flags[i] = conColor[listed(list, i)] < 0 ? 1u : 0u;
+		flags[i] = conColor[listed(list, i)] == -1 ? 1u : 0u;
 }
 
 __global__ void k_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out)
@@ -677,6 +678,73 @@ __global__ void k_color_compact(int m, const int* list, const uint32_t* flags, c
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < m && flags[i])
 		out[scanned[i]] = listed(list, i);
+}
+
+// Body-plane constraints go after every colour of their body's body-body
+// constraints, one colour each in constraint order (the reference places
+// static constraints at maxSolverNormalProgress + k:
+// lowleveldynamics/src/DyConstraintPartition.cpp:168-179, 604-616).
+__global__ void k_body_next(int C, const int* conA, const int* conB, const int* conColor, uint32_t* bodyNext)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C || conB[c] < 0)
+		return;
+	const uint32_t nx = uint32_t(conColor[c] + 1);
+	atomicMax(&bodyNext[conA[c]], nx);
+	atomicMax(&bodyNext[conB[c]], nx);
+}
+
+// A body's body-plane constraints are consecutive in the constraint list
+// (slots are body-major); k counts the ones before c.
+__global__ void k_static_color(int C, const int* conA, const int* conB, const uint32_t* bodyNext, int* conColor)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C || conB[c] >= 0)
+		return;
+	const int a = conA[c];
+	int k = 0;
+	for (int e = c - 1; e >= 0 && conA[e] == a && conB[e] < 0; e--)
+		k++;
+	conColor[c] = int(bodyNext[a]) + k;
+}
+
+// open-addressing table of the previous step's pair keys -> row (keys are
+// unique; the table has at least twice as many slots as keys)
+PXH_D uint32_t patchSlot(uint64_t key, uint32_t mask) { return uint32_t(mix64(key)) & mask; }
+
+__global__ void k_patch_insert(int m, const uint64_t* keys, unsigned long long* htKey, int* htVal, uint32_t mask)
+{
+	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= m)
+		return;
+	const unsigned long long key = keys[o];
+	uint32_t slot = patchSlot(key, mask);
+	for (uint32_t probe = 0; probe <= mask; probe++)
+	{
+		if (atomicCAS(&htKey[slot], ~0ull, key) == ~0ull)
+		{
+			htVal[slot] = o;
+			return;
+		}
+		slot = (slot + 1) & mask;
+	}
+}
+
+PXH_D int patchFind(const PatchIO& io, uint64_t key)
+{
+	if (!io.htKey)
+		return -1;
+	uint32_t slot = patchSlot(key, io.htMask);
+	for (uint32_t probe = 0; probe <= io.htMask; probe++)
+	{
+		const uint64_t k = io.htKey[slot];
+		if (k == key)
+			return io.htVal[slot];
+		if (k == ~0ull)
+			return -1;
+		slot = (slot + 1) & io.htMask;
+	}
+	return -1;
 }
 
 // block-level maximum then one atomic per block (grid-stride)
@@ -768,17 +836,19 @@ PXH_D m33 loadSym(float4 a, float4 b)
 PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                     const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
                     const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, uint32_t& nFricOut);
+                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
+                    uint32_t& nFricOut);
 
 // Contact rows of one constraint at its colour-ordered position o
 // (setupFinalizeSolverConstraints / constructContactConstraintStep,
-//  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction anchors
-//  growPatches, DyFrictionCorrelation.cpp:187-292)
+//  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction patches
+//  getFrictionPatches / correlatePatches / growPatches, DyContactPrepShared.h:53-127,
+//  DyFrictionCorrelation.cpp:136-316)
 __global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                        const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
                        const float4* svLin, const float4* svOrigLin, const float4* svI0, const float4* svI1,
                        float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
-                       unsigned long long* counters)
+                       PatchIO io, unsigned long long* counters)
 {
 	const int o = blockIdx.x * blockDim.x + threadIdx.x;
 	// per-step counters: contact points, friction rows, dynamic-dynamic constraints
@@ -790,7 +860,7 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 		myDyn = conB[c] >= 0 ? 1u : 0u;
 	}
 	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, svLin, svOrigLin, svI0, svI1, invStepDt, scale,
-	         restDist, frictionOffsetThreshold, rows, myFric);
+	         restDist, frictionOffsetThreshold, rows, io, myFric);
 	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
 	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
 	if ((threadIdx.x & 63) == 63)
@@ -805,7 +875,8 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                     const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
                     const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, uint32_t& nFricOut)
+                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
+                    uint32_t& nFricOut)
 {
 	if (o >= C)
 		return;
@@ -852,31 +923,121 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 		rows.nB[k * int64_t(C) + o] = f4(rbXnI, pen > 0.0f ? -recip : recip);
 		rows.nF[k * int64_t(C) + o] = 0.0f;
 	}
-	// friction anchors among points closer than the offset threshold
-	int nAnch = 0;
-	f3 wa0 = mk(0, 0, 0), wa1 = mk(0, 0, 0);
-	float pd2 = 0.0f;
-	const float thr = frictionOffsetThreshold + restDist;
-	for (int j = 0; j < cnt; j++)
+	// friction patch: the previous step's patch of this pair is kept when it is
+	// unbroken, has anchors, its body normals still agree, no anchor drifted
+	// apart along the normal, and its world normal matches this step's normal
+	const uint64_t pkey = (uint64_t(uint32_t(a)) << 32) | (dynB ? uint32_t(b) : (0x80000000u | uint32_t(-1 - b)));
+	f3 pn0, pn1, pa0[2], pa1[2];
+	int nKept = 0;
 	{
-		if (!(cs[j] < thr))
-			continue;
-		const f3 wp = cp[j];
-		if (nAnch == 0) { wa0 = wp; nAnch = 1; }
-		else if (nAnch == 1)
+		const int po = patchFind(io, pkey);
+		bool kept = false;
+		if (po >= 0 && !io.prevBroken[po])
 		{
-			const f3 d = wp - wa0;
-			pd2 = dot(d, d);
-			if (pd2 > 1e-8f) { wa1 = wp; nAnch = 2; }
+			const float4 q0 = io.prev[6 * int64_t(po) + 0];
+			const int pc = __float_as_int(q0.w);
+			if (pc != 0)
+			{
+				const float4 q1 = io.prev[6 * int64_t(po) + 1];
+				pn0 = mk(q0); pn1 = mk(q1);
+				pa0[0] = mk(io.prev[6 * int64_t(po) + 2]); pa0[1] = mk(io.prev[6 * int64_t(po) + 3]);
+				pa1[0] = mk(io.prev[6 * int64_t(po) + 4]); pa1[1] = mk(io.prev[6 * int64_t(po) + 5]);
+				const xf b10 = xinvmul(f0, f1);
+				if (dot(pn0, qrot(b10.q, pn1)) > 0.999f)
+				{
+					bool separated = false;
+					for (int j = 0; j < pc; j++)
+						if (!(fabsf(dot(pa0[j] - xtrans(b10, pa1[j]), pn0)) < io.corrDist))
+							separated = true;
+					if (!separated && !(dot(n, qrot(f0.q, pn0)) < 0.999f))
+					{
+						kept = true;
+						nKept = pc;
+					}
+				}
+			}
 		}
-		else
+		if (!kept)
 		{
-			const f3 e0 = wp - wa0, e1 = wp - wa1;
-			const float d0 = dot(e0, e0), d1 = dot(e1, e1);
-			if (d0 > d1) { if (d0 > pd2) { wa1 = wp; pd2 = d0; } }
-			else if (d1 > pd2) { wa0 = wp; pd2 = d1; }
+			pn0 = qrotinv(f0.q, n);
+			pn1 = qrotinv(f1.q, n);
 		}
 	}
+	// anchors: two kept anchors stay unless closer than half the diagonal of
+	// this step's contact bounds; one kept anchor is completed from the
+	// contacts; otherwise up to two points closer than the offset threshold
+	if (nKept == 2)
+	{
+		f3 mn = cp[0], mx = cp[0];
+		for (int j = 1; j < cnt; j++)
+		{
+			mn = mk(fminf(mn.x, cp[j].x), fminf(mn.y, cp[j].y), fminf(mn.z, cp[j].z));
+			mx = mk(fmaxf(mx.x, cp[j].x), fmaxf(mx.y, cp[j].y), fmaxf(mx.z, cp[j].z));
+		}
+		const f3 dg = mx - mn, da = pa0[0] - pa0[1];
+		if (!(dot(da, da) * 4.0f >= dot(dg, dg)))
+			nKept = 0;
+	}
+	int nAnch = nKept;
+	if (nKept != 2)
+	{
+		f3 wa0 = mk(0, 0, 0), wa1 = mk(0, 0, 0);
+		nAnch = 0;
+		if (nKept == 1)
+		{
+			wa0 = xtrans(f0, pa0[0]);
+			nAnch = 1;
+		}
+		float pd2 = 0.0f;
+		const float thr = frictionOffsetThreshold + restDist;
+		for (int j = 0; j < cnt; j++)
+		{
+			if (!(cs[j] < thr))
+				continue;
+			const f3 wp = cp[j];
+			if (nAnch == 0) { wa0 = wp; nAnch = 1; }
+			else if (nAnch == 1)
+			{
+				const f3 d = wp - wa0;
+				pd2 = dot(d, d);
+				if (pd2 > 1e-8f) { wa1 = wp; nAnch = 2; }
+			}
+			else
+			{
+				const f3 e0 = wp - wa0, e1 = wp - wa1;
+				const float d0 = dot(e0, e0), d1 = dot(e1, e1);
+				if (d0 > d1) { if (d0 > pd2) { wa1 = wp; pd2 = d0; } }
+				else if (d1 > pd2) { wa0 = wp; pd2 = d1; }
+			}
+		}
+		if (nKept == 0 && nAnch > 0)
+		{
+			pa0[0] = xtransinv(f0, wa0);
+			pa1[0] = xtransinv(f1, wa0);
+		}
+		if (nAnch == 2)
+		{
+			pa0[1] = xtransinv(f0, wa1);
+			pa1[1] = xtransinv(f1, wa1);
+		}
+		if (nAnch == 0)
+		{
+			pa0[0] = mk(0, 0, 0);
+			pa1[0] = pa0[0];
+		}
+		if (nAnch < 2)
+		{
+			pa0[1] = mk(0, 0, 0);
+			pa1[1] = pa0[1];
+		}
+	}
+	io.curKey[o] = pkey;
+	io.cur[6 * int64_t(o) + 0] = f4(pn0, __int_as_float(nAnch));
+	io.cur[6 * int64_t(o) + 1] = f4(pn1, 0.0f);
+	io.cur[6 * int64_t(o) + 2] = f4(pa0[0], 0.0f);
+	io.cur[6 * int64_t(o) + 3] = f4(pa0[1], 0.0f);
+	io.cur[6 * int64_t(o) + 4] = f4(pa1[0], 0.0f);
+	io.cur[6 * int64_t(o) + 5] = f4(pa1[1], 0.0f);
 	const float fscale = nAnch == 2 ? 0.5f : 1.0f;
 	nFricOut = uint32_t(2 * nAnch);
 	(void)fscale;
@@ -894,8 +1055,7 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	const f3 relTr = f0.p - f1.p;
 	for (int j = 0; j < nAnch; j++)
 	{
-		const f3 w = j == 0 ? wa0 : wa1;
-		const f3 ra = qrot(f0.q, xtransinv(f0, w)), rb = qrot(f1.q, xtransinv(f1, w));
+		const f3 ra = qrot(f0.q, pa0[j]), rb = qrot(f1.q, pa1[j]);
 		const f3 err = (ra - rb) + relTr;
 		for (int d = 0; d < 2; d++)
 		{
@@ -919,7 +1079,7 @@ constexpr int SOLVE_THREADS = 64;
 __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* svLin,
                                                           float4* svAng, const float4* svDLin, const float4* svDAng,
                                                           float minPen, float muS, float muD, float invStepDt,
-                                                          float biasScale, float scale)
+                                                          float biasScale, float scale, uint32_t* broken)
 {
 	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
 	if (o >= end)
@@ -989,6 +1149,7 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 		nFc[k] = nf;
 		acc = acc + nf;
 	}
+	bool anyClamp = false;
 	if (nF)
 	{
 		const float maxF = muS * acc, maxD = muD * acc;
@@ -1025,8 +1186,13 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 			a1 = (a1 - mk(fB[i + 1]) * d1) - mk(fB[i]) * d0;
 			fFc[i] = n0;
 			fFc[i + 1] = n1;
+			anyClamp |= clamp;
 		}
 	}
+	// the broken flag of the friction patch is the last pass's clamp state
+	// (DyTGSContactPrep.cpp:1837, 1907)
+	if (broken)
+		broken[o] = anyClamp ? 1u : 0u;
 #pragma unroll
 	for (int k = 0; k < 4; k++)
 	{
@@ -1360,6 +1526,18 @@ void launch_color_compact(int m, const int* list, const uint32_t* flags, const u
 	PXH_LAUNCH(k_color_compact, m, m, list, flags, scanned, out);
 }
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st) { PXH_LAUNCH(k_fill_u64, n, p, n, v); }
+void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
+                          hipStream_t st)
+{
+	PXH_LAUNCH(k_fill_u32, n, bodyNext, int64_t(n), 0u);
+	PXH_LAUNCH(k_body_next, C, C, conA, conB, conColor, bodyNext);
+	PXH_LAUNCH(k_static_color, C, C, conA, conB, bodyNext, conColor);
+}
+void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal, uint32_t htMask, hipStream_t st)
+{
+	PXH_LAUNCH(k_fill_u64, int(htMask) + 1, htKey, int(htMask) + 1, ~0ull);
+	PXH_LAUNCH(k_patch_insert, m, m, keys, reinterpret_cast<unsigned long long*>(htKey), htVal, htMask);
+}
 void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st)
 {
 	PXH_LAUNCH(k_color_keys, C, C, conColor, keys, vals);
@@ -1378,18 +1556,19 @@ void launch_preintegrate(int n, const float4* pos, const float4* rot, const floa
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
-                 float scale, float restDist, float fot, Rows rows, unsigned long long* counters, hipStream_t st)
+                 float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
+                 hipStream_t st)
 {
 	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.lin, sb.origLin, sb.i0, sb.i1,
-	           invStepDt, scale, restDist, fot, rows, counters);
+	           invStepDt, scale, restDist, fot, rows, patches, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  float invStepDt, float biasScale, float scale, hipStream_t st)
+                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st)
 {
 	if (end > begin)
 		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.lin, sb.ang, sb.dlin,
 		                                                                     sb.dang, minPen, muS, muD, invStepDt,
-		                                                                     biasScale, scale);
+		                                                                     biasScale, scale, broken);
 }
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
 {

@@ -95,6 +95,15 @@ PXH_D m33 mmm(const m33& a, const m33& b)
 }
 PXH_D f3 xtrans(const xf& t, f3 v) { return qrot(t.q, v) + t.p; }
 PXH_D f3 xtransinv(const xf& t, f3 v) { return qrotinv(t.q, v - t.p); }
+// a^-1 * b (PxTransform::transformInv(const PxTransform&))
+PXH_D xf xinvmul(const xf& a, const xf& b)
+{
+	qt qi; qi.x = -a.q.x; qi.y = -a.q.y; qi.z = -a.q.z; qi.w = a.q.w;
+	xf r;
+	r.q = qmul(qi, b.q);
+	r.p = qrot(qi, b.p - a.p);
+	return r;
+}
 
 struct mxf { m33 r; f3 p; };
 PXH_D f3 mxtrans(const mxf& t, f3 v) { return mmul(t.r, v) + t.p; }

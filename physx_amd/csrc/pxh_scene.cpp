@@ -141,6 +141,16 @@ struct pxh_scene_t
 	DBuf<int4> rhI;
 	DBuf<float4> rhF, rnA, rnB, rfT, rfA, rfB;
 	DBuf<float> rnF, rfF;
+	// friction patches of the previous and the current step (double-buffered by
+	// patchCur), the previous step's row count and the pair-key table
+	DBuf<float4> patch[2];
+	DBuf<uint64_t> patchKey[2];
+	DBuf<uint32_t> patchBroken[2];
+	DBuf<uint64_t> htKey;
+	DBuf<int> htVal;
+	DBuf<uint32_t> bodyNext;
+	int patchCur = 0;
+	int nPrevRows = 0;
 	uint32_t* hostCounts = nullptr;       // pinned
 
 	// optional per-stage GPU timing with HIP events on the scene stream
@@ -407,6 +417,15 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->rhI.release();
 	s->rnF.release();
 	s->rfF.release();
+	for (int k = 0; k < 2; k++)
+	{
+		s->patch[k].release();
+		s->patchKey[k].release();
+		s->patchBroken[k].release();
+	}
+	s->htKey.release();
+	s->htVal.release();
+	s->bodyNext.release();
 	if (s->hostCounts)
 		(void)hipHostFree(s->hostCounts);
 	if (s->stream)
@@ -639,6 +658,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	std::vector<uint32_t> colorStart(1, 0);
 	if (C > 0)
 	{
+		PXH_CHECK(s->bodyNext.ensure(n));
+		pxh::launch_static_colors(n, C, s->conA.p, s->conB.p, s->conColor.p, s->bodyNext.p, st);
 		pxh::launch_max_color(C, s->conColor.p, s->scratch.p + 1, st);
 		const uint32_t* src[1] = {s->scratch.p + 1};
 		r = readCounts(st, s->hostCounts, src, 1);
@@ -691,9 +712,34 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float scale = std::min(0.8f, biasCoefficient);
 	PXH_CHECK(s->counters.ensure(4));
 	PXH_CHECK(hipMemsetAsync(s->counters.p, 0, 4 * sizeof(unsigned long long), st));
+	// friction patches: this step's rows go to patch[cur]; the previous step's
+	// rows (patch[prev]) are found through a table of their pair keys
+	const int cur = s->patchCur, prv = cur ^ 1;
+	PXH_CHECK(s->patch[cur].ensure(6 * Cc));
+	PXH_CHECK(s->patchKey[cur].ensure(Cc));
+	PXH_CHECK(s->patchBroken[cur].ensure(Cc));
+	pxh::PatchIO pio{};
+	pio.cur = s->patch[cur].p;
+	pio.curKey = s->patchKey[cur].p;
+	pio.corrDist = d.friction_correlation_distance;
+	if (s->nPrevRows > 0)
+	{
+		uint32_t size = 64;
+		while (size < 2u * uint32_t(s->nPrevRows))
+			size <<= 1;
+		PXH_CHECK(s->htKey.ensure(size));
+		PXH_CHECK(s->htVal.ensure(size));
+		pxh::launch_patch_table(s->nPrevRows, s->patchKey[prv].p, s->htKey.p, s->htVal.p, size - 1, st);
+		pio.htKey = s->htKey.p;
+		pio.htVal = s->htVal.p;
+		pio.htMask = size - 1;
+		pio.prev = s->patch[prv].p;
+		pio.prevBroken = s->patchBroken[prv].p;
+	}
 	pk = s->profBegin(4);
 	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
-	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, s->counters.p, st);
+	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, pio, s->counters.p,
+	                 st);
 	PXH_CHECK(hipMemcpyAsync(s->hostCounts + 4000, s->counters.p, 4 * sizeof(unsigned long long),
 	                         hipMemcpyDeviceToHost, st));
 	s->profEnd(pk);
@@ -710,7 +756,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			pk = s->profBegin(5);
 			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
 			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, invStepDt,
-			                  invStepDt * scale, scale, st);
+			                  invStepDt * scale, scale, it == P_ + V_ - 1 ? s->patchBroken[cur].p : nullptr, st);
 			s->profEnd(pk);
 		}
 		if (posIter)
@@ -720,6 +766,9 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			s->profEnd(pk);
 		}
 	}
+
+	s->nPrevRows = C;
+	s->patchCur = cur ^ 1;
 
 	// 7. write back poses and velocities
 	pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);

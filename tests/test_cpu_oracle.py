@@ -179,3 +179,63 @@ def test_broadphase_vs_live_reference_abp_and_pabp():
     got = L.bp_pairs_pxo(b, d)
     assert np.array_equal(got, L.bp_pairs_ref(b, d))
     assert np.array_equal(got, L.bp_pairs_ref(b, d, pabp=True))
+
+
+def _single_column(height):
+    n = height
+    st = np.full(n, S.BOX)
+    dims = np.zeros((n, 4))
+    dims[:, :3] = 0.5
+    pos = np.zeros((n, 3))
+    pos[:, 1] = 0.5 + np.arange(n)
+    quat = np.tile([0, 0, 0, 1.0], (n, 1))
+    return S._finish(st, dims, pos, quat, np.zeros(n), [[0, 1, 0, 0]], [0], 1, "column", {})
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+def test_plane_constraints_solved_after_body_constraints_vs_live_reference():
+    """A 10-box column: the reference places each body's static constraints in
+    the partitions after its body-body ones, which keeps the bottom box on the
+    floor (solved in any other order it sinks by ~2 mm under the column)."""
+    sc = _single_column(10)
+    o = L.OracleWorld(sc, 4, 1)
+    r = L.RefScene(sc, "tgs", 4, 1, threads=1)
+    for _ in range(100):
+        o.step(1 / 60)
+        r.step(1 / 60)
+    po, _, _ = o.bodies()
+    pr, _, _ = r.bodies()
+    assert abs(po[0, 1] - 0.5) < 2e-4 and abs(pr[0, 1] - 0.5) < 2e-4, (po[0, 1], pr[0, 1])
+    assert np.abs(po[:, 1] - pr[:, 1]).max() < 2e-3
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+@pytest.mark.parametrize("vx", [0.05, 0.3])
+def test_friction_anchors_persist_vs_live_reference(vx):
+    """A box on the floor pushed sideways: below the static limit the kept
+    friction anchor pulls it back towards where it first touched (a fresh
+    anchor every step would leave it ~1e-4 m away); above it the box slides
+    and stops. Both follow the reference within 1e-5 m."""
+    sc = _single_column(1)
+    sc.lin_vel[:, 0] = vx
+    o = L.OracleWorld(sc, 4, 1)
+    r = L.RefScene(sc, "tgs", 4, 1, threads=1)
+    for _ in range(40):
+        o.step(1 / 60)
+        r.step(1 / 60)
+    po, _, _ = o.bodies()
+    pr, _, _ = r.bodies()
+    assert abs(po[0, 0] - pr[0, 0]) < 1e-5, (po[0, 0], pr[0, 0])
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+@pytest.mark.parametrize("grid,seed", [((6, 3, 6), 1), ((8, 4, 8), 2)])
+def test_long_run_statistics_vs_live_reference(grid, seed):
+    """300 steps of a settling box pile: trajectories diverge chaotically, the
+    pile statistics stay within 5% (height, potential energy), 1 mm (worst
+    floor penetration) and 0.02 m/s (mean speed over the last second)."""
+    import pile_stats as PS
+    sc = S.box_grid_drop(nx=grid[0], ny=grid[1], nz=grid[2], seed=seed)
+    a = PS.run_stats(L.OracleWorld(sc, 4, 1), sc, 300)
+    b = PS.run_stats(L.RefScene(sc, "tgs", 4, 1, threads=8), sc, 300)
+    PS.assert_stats_close(a, b)

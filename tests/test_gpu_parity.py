@@ -1,7 +1,9 @@
 """GPU parity: the HIP path (through the C ABI) against the C oracle on the
 same seeded inputs. Tolerances: contacts 1e-4 m (points, separations) and
 1e-3 (normals); broadphase pair sets bit-exact; one-step velocities within
-1e-4 of max(|v|, g*dt)."""
+1e-5 of max(|v|, g*dt); multi-step runs within bounds that allow for chaotic
+growth of last-bit differences; long runs against the reference CPU SDK by
+pile statistics."""
 import numpy as np
 import pytest
 
@@ -154,9 +156,12 @@ def test_one_step_matches_oracle(cuda):
     gp, gl, ga = g.bodies()
     op, ol, oa = o.bodies()
     scale = 9.81 * dt
-    tol_v = 1e-4 * np.maximum(np.linalg.norm(ol, axis=1), scale)
-    assert np.all(np.abs(gl - ol).max(1) <= tol_v * 10)
-    assert np.abs(gp[:, :3] - op[:, :3]).max() < 1e-5
+    # the device code is built without FMA contraction and with correctly
+    # rounded division and square root, so one step agrees with the oracle to
+    # the last bits except where sin/cos of the substep rotation round apart
+    tol_v = 1e-5 * np.maximum(np.linalg.norm(ol, axis=1), scale)
+    assert np.all(np.abs(gl - ol).max(1) <= tol_v)
+    assert np.abs(gp[:, :3] - op[:, :3]).max() < 1e-6
 
 
 def test_multi_step_tracks_oracle(cuda):
@@ -183,3 +188,52 @@ def test_direct_gpu_api_roundtrip(cuda):
     out = torch.zeros((2, 3), dtype=torch.float32, device=cuda)
     g.get_rigid_dynamic_data(out, P.PXH_DATA_LINEAR_VELOCITY, index=idx, n=2)
     assert torch.equal(out.cpu(), newv.cpu())
+
+
+def _column(height):
+    n = height
+    st = np.full(n, S.BOX)
+    dims = np.zeros((n, 4))
+    dims[:, :3] = 0.5
+    pos = np.zeros((n, 3))
+    pos[:, 1] = 0.5 + np.arange(n)
+    quat = np.tile([0, 0, 0, 1.0], (n, 1))
+    return S._finish(st, dims, pos, quat, np.zeros(n), [[0, 1, 0, 0]], [0], 1, "column", {})
+
+
+@pytest.mark.parametrize("vx", [0.05, 0.3])
+def test_friction_anchors_persist_like_oracle(cuda, vx):
+    """Kept friction anchors (previous step's patch found by pair key) pull a
+    lightly pushed box back; a harder push breaks the patch and the box
+    slides. The GPU follows the oracle within 1e-6 m over 40 steps."""
+    sc = _column(1)
+    sc.lin_vel[:, 0] = vx
+    g, o = _run_pair(sc, 40, 1.0 / 60.0)
+    gp, _, _ = g.bodies()
+    op, _, _ = o.bodies()
+    assert np.abs(gp[:, :3] - op[:, :3]).max() < 1e-6, (gp[0], op[0])
+
+
+def test_column_plane_constraints_last(cuda):
+    """10-box column: body-plane constraints after the body-body colours keep
+    the bottom box on the floor; the GPU heights follow the oracle's. (The
+    column's sideways sway is chaotic in the reference too - two friction
+    anchors per face give it a small twist - so x and z are not compared.)"""
+    sc = _column(10)
+    g, o = _run_pair(sc, 100, 1.0 / 60.0)
+    gp, _, _ = g.bodies()
+    op, _, _ = o.bodies()
+    assert abs(gp[0, 1] - 0.5) < 2e-4
+    assert np.abs(gp[:, 1] - op[:, 1]).max() < 3e-3
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+def test_long_run_statistics_match_reference(cuda):
+    """300 steps of a 256-box settling pile on the GPU against the reference
+    CPU SDK: height and potential energy within 5%, worst floor penetration
+    within 1 mm, mean speed over the last second within 0.02 m/s."""
+    import pile_stats as PS
+    sc = S.box_grid_drop(nx=8, ny=4, nz=8, seed=2)
+    a = PS.run_stats(P.Scene(sc, position_iterations=4, velocity_iterations=1), sc, 300)
+    b = PS.run_stats(L.RefScene(sc, "tgs", 4, 1, threads=8), sc, 300)
+    PS.assert_stats_close(a, b)
