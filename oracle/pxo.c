@@ -1033,6 +1033,7 @@ typedef struct {
 	uint64_t key;        /* pair key: a << 32 | b, planes 0x80000000 | plane */
 	v3 n0, n1, a0[2], a1[2];
 	int count, broken;
+	int color;           /* the constraint's colour in that step */
 } fpatch;
 
 typedef struct {
@@ -1634,20 +1635,39 @@ int pxo_world_step(void* wp, float dt)
 	for (long long c = 0; c < w->ncon; c++)
 		w->stats[3] += w->con[c].c.count;
 
-	/* 3. colouring: rounds of local minima of the ordering key at both dynamic
-	 * bodies; each minimum takes the first colour free at both bodies (64-bit
-	 * masks; past 63 one colour per round, 64 + round). This replaces the
-	 * reference's greedy partitioning (lowleveldynamics/src/DyConstraintPartition.cpp:858)
-	 * with an order-free rule the GPU evaluates in parallel. */
+	/* 3. colouring. A body-body constraint that existed in the previous step
+	 * keeps its colour (below 64): those constraints were coloured together
+	 * then, so they still never share a colour at a body. The others go
+	 * through rounds of local minima of the ordering key at both bodies; each
+	 * minimum takes the first colour free at both bodies (64-bit masks; past 63
+	 * one colour per round, 64 + round). This replaces the reference's
+	 * greedy partitioning (lowleveldynamics/src/DyConstraintPartition.cpp:858)
+	 * with an order-free rule the GPU evaluates in parallel, kept from step to
+	 * step like the reference GPU solver's incremental partitions
+	 * (gpusolver/src/PxgConstraintPartition.cpp:1940). */
 	uint64_t* best = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)(n ? n : 1));
 	uint64_t* mask = (uint64_t*)calloc((size_t)(n ? n : 1), sizeof(uint64_t));
 	long long left = 0;
 	for (long long c = 0; c < w->ncon; c++)
 	{
-		w->con[c].color = w->con[c].b >= 0 ? -1 : -2; /* -2: body-plane, placed after the rounds */
-		left += w->con[c].b >= 0;
+		pxo_con* k = &w->con[c];
+		k->color = k->b >= 0 ? -1 : -2; /* -2: body-plane, placed after the rounds */
+		if (k->b >= 0)
+		{
+			const fpatch* pp = find_patch(w, k->pkey);
+			if (pp && pp->color >= 0 && pp->color < 64)
+			{
+				k->color = pp->color;
+				mask[k->a] |= 1ull << pp->color;
+				mask[k->b] |= 1ull << pp->color;
+			}
+			else
+				left++;
+		}
 	}
 	int round = 0, ncolors = 0;
+	for (long long c = 0; c < w->ncon; c++)
+		if (w->con[c].color + 1 > ncolors) ncolors = w->con[c].color + 1;
 	while (left > 0)
 	{
 		for (int i = 0; i < n; i++)
@@ -1798,7 +1818,10 @@ int pxo_world_step(void* wp, float dt)
 	/* friction patches for the next step */
 	w->prev = (fpatch*)realloc(w->prev, sizeof(fpatch) * (size_t)(w->ncon ? w->ncon : 1));
 	for (long long c = 0; c < w->ncon; c++)
+	{
 		w->prev[c] = w->con[c].fp;
+		w->prev[c].color = w->con[c].color;
+	}
 	w->nprev = w->ncon;
 	qsort(w->prev, (size_t)w->nprev, sizeof(fpatch), cmp_patch);
 

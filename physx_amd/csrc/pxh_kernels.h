@@ -47,8 +47,12 @@ struct PatchIO
 	uint32_t htMask;           // table size - 1 (power of two)
 	const float4* prev;        // previous step's patches
 	const uint32_t* prevBroken;
+	const int* prevColor;      // previous step's colour of each row
+	int* prevRow;              // per constraint: its row in the previous step, or -1
+	const int* conColor;       // per constraint: this step's colour
 	float4* cur;               // this step's patches
 	uint64_t* curKey;          // this step's pair keys
+	int* curColor;             // this step's colour of each row
 	float corrDist;            // friction correlation distance
 };
 
@@ -87,6 +91,10 @@ void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* val
 void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
                           hipStream_t st);
 void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal, uint32_t htMask, hipStream_t st);
+// finds each constraint's row of the previous step; body-body constraints keep
+// their previous colour (marked in the per-body colour masks)
+void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor, uint64_t* mask,
+                     hipStream_t st);
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,

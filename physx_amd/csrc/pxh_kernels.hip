@@ -747,6 +747,36 @@ PXH_D int patchFind(const PatchIO& io, uint64_t key)
 	return -1;
 }
 
+PXH_D uint64_t pairKey(int a, int b)
+{
+	return (uint64_t(uint32_t(a)) << 32) | (b >= 0 ? uint32_t(b) : (0x80000000u | uint32_t(-1 - b)));
+}
+
+// A body-body constraint that existed in the previous step keeps its colour
+// (below 64): those constraints were coloured together then, so they never
+// share a colour at a body now. The reference GPU solver keeps its partitions
+// from step to step the same way (gpusolver/src/PxgConstraintPartition.cpp:1940).
+__global__ void k_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor,
+                           unsigned long long* mask)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C)
+		return;
+	const int a = conA[c], b = conB[c];
+	const int po = patchFind(io, pairKey(a, b));
+	io.prevRow[c] = po;
+	if (b >= 0 && po >= 0)
+	{
+		const int pc = io.prevColor[po];
+		if (pc >= 0 && pc < 64)
+		{
+			conColor[c] = pc;
+			atomicOr(&mask[a], 1ull << pc);
+			atomicOr(&mask[b], 1ull << pc);
+		}
+	}
+}
+
 // block-level maximum then one atomic per block (grid-stride)
 __global__ void __launch_bounds__(256) k_max_color(int C, const int* conColor, uint32_t* maxColor)
 {
@@ -926,11 +956,11 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	// friction patch: the previous step's patch of this pair is kept when it is
 	// unbroken, has anchors, its body normals still agree, no anchor drifted
 	// apart along the normal, and its world normal matches this step's normal
-	const uint64_t pkey = (uint64_t(uint32_t(a)) << 32) | (dynB ? uint32_t(b) : (0x80000000u | uint32_t(-1 - b)));
+	const uint64_t pkey = pairKey(a, b);
 	f3 pn0, pn1, pa0[2], pa1[2];
 	int nKept = 0;
 	{
-		const int po = patchFind(io, pkey);
+		const int po = io.prevRow ? io.prevRow[c] : -1;
 		bool kept = false;
 		if (po >= 0 && !io.prevBroken[po])
 		{
@@ -1032,6 +1062,7 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 		}
 	}
 	io.curKey[o] = pkey;
+	io.curColor[o] = io.conColor[c];
 	io.cur[6 * int64_t(o) + 0] = f4(pn0, __int_as_float(nAnch));
 	io.cur[6 * int64_t(o) + 1] = f4(pn1, 0.0f);
 	io.cur[6 * int64_t(o) + 2] = f4(pa0[0], 0.0f);
@@ -1537,6 +1568,11 @@ void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal
 {
 	PXH_LAUNCH(k_fill_u64, int(htMask) + 1, htKey, int(htMask) + 1, ~0ull);
 	PXH_LAUNCH(k_patch_insert, m, m, keys, reinterpret_cast<unsigned long long*>(htKey), htVal, htMask);
+}
+void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor, uint64_t* mask,
+                     hipStream_t st)
+{
+	PXH_LAUNCH(k_con_prev, C, C, conA, conB, io, conColor, reinterpret_cast<unsigned long long*>(mask));
 }
 void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st)
 {

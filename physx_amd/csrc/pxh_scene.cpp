@@ -146,6 +146,8 @@ struct pxh_scene_t
 	DBuf<float4> patch[2];
 	DBuf<uint64_t> patchKey[2];
 	DBuf<uint32_t> patchBroken[2];
+	DBuf<int> patchColor[2];
+	DBuf<int> prevRow;
 	DBuf<uint64_t> htKey;
 	DBuf<int> htVal;
 	DBuf<uint32_t> bodyNext;
@@ -422,7 +424,9 @@ int32_t pxh_scene_release(pxh_scene s)
 		s->patch[k].release();
 		s->patchKey[k].release();
 		s->patchBroken[k].release();
+		s->patchColor[k].release();
 	}
+	s->prevRow.release();
 	s->htKey.release();
 	s->htVal.release();
 	s->bodyNext.release();
@@ -615,28 +619,55 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_con_build(S, P, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
 	                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
 
-	// 4. colouring: rounds of local key minima, each taking its first free
-	//    colour; the uncoloured constraints are compacted between batches
+	// 4. previous step's rows: a pair-key table over them gives each
+	//    constraint its previous row (friction patch) and, for body-body
+	//    constraints, its previous colour
+	const size_t Cc = std::max(C, 1);
+	const int cur = s->patchCur, prv = cur ^ 1;
 	PXH_CHECK(s->best.ensure(n));
 	PXH_CHECK(s->mask.ensure(n));
+	PXH_CHECK(s->prevRow.ensure(Cc));
+	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
+	pxh::PatchIO pio{};
+	pio.corrDist = d.friction_correlation_distance;
+	if (s->nPrevRows > 0)
+	{
+		uint32_t size = 64;
+		while (size < 2u * uint32_t(s->nPrevRows))
+			size <<= 1;
+		PXH_CHECK(s->htKey.ensure(size));
+		PXH_CHECK(s->htVal.ensure(size));
+		pxh::launch_patch_table(s->nPrevRows, s->patchKey[prv].p, s->htKey.p, s->htVal.p, size - 1, st);
+		pio.htKey = s->htKey.p;
+		pio.htVal = s->htVal.p;
+		pio.htMask = size - 1;
+		pio.prev = s->patch[prv].p;
+		pio.prevBroken = s->patchBroken[prv].p;
+		pio.prevColor = s->patchColor[prv].p;
+		pio.prevRow = s->prevRow.p;
+		pxh::launch_con_prev(C, s->conA.p, s->conB.p, pio, s->conColor.p, s->mask.p, st);
+	}
+
+	// 5. colouring of the remaining body-body constraints: rounds of local key
+	//    minima, each taking its first free colour; the uncoloured constraints
+	//    are compacted before the rounds and between batches of rounds
 	PXH_CHECK(s->scratch.ensure(4));
 	PXH_CHECK(s->colList.ensure(std::max(C, 1)));
 	PXH_CHECK(s->colList2.ensure(std::max(C, 1)));
 	PXH_CHECK(s->colFlags.ensure(size_t(C) + 1));
 	PXH_CHECK(s->counts.ensure(size_t(C) + 1));
 	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, 2 * sizeof(uint32_t), st));
-	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
 	int rounds = 0;
 	const int kMaxRounds = 960;
 	int remaining = C;
 	const int* list = nullptr;
-	int batch = 8;
+	int batch = 0;
 	while (remaining > 0)
 	{
 		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
 			pxh::launch_color_round(remaining, list, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p,
 			                        s->best.p, s->mask.p, st);
-		batch = 4;
+		batch = list ? 4 : 8;
 		// compact the uncoloured constraints
 		pxh::launch_color_uncolored(remaining, list, s->conColor.p, s->colFlags.p, st);
 		PXH_CHECK(hipMemsetAsync(s->colFlags.p + remaining, 0, sizeof(uint32_t), st));
@@ -690,7 +721,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	s->statColors = nColors;
 	s->statRounds = rounds;
 
-	// 5. solver bodies + contact rows
+	// 6. solver bodies + contact rows
 	PXH_CHECK(s->sbLin.ensure(n)); PXH_CHECK(s->sbAng.ensure(n)); PXH_CHECK(s->sbDLin.ensure(n));
 	PXH_CHECK(s->sbDAng.ensure(n)); PXH_CHECK(s->sbP.ensure(n)); PXH_CHECK(s->sbDQ.ensure(n));
 	PXH_CHECK(s->sbI0.ensure(n)); PXH_CHECK(s->sbI1.ensure(n)); PXH_CHECK(s->sbOrig.ensure(n));
@@ -699,7 +730,6 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
 	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, sb, st);
-	const size_t Cc = std::max(C, 1);
 	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc));
 	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnF.ensure(4 * Cc));
 	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
@@ -712,30 +742,15 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float scale = std::min(0.8f, biasCoefficient);
 	PXH_CHECK(s->counters.ensure(4));
 	PXH_CHECK(hipMemsetAsync(s->counters.p, 0, 4 * sizeof(unsigned long long), st));
-	// friction patches: this step's rows go to patch[cur]; the previous step's
-	// rows (patch[prev]) are found through a table of their pair keys
-	const int cur = s->patchCur, prv = cur ^ 1;
+	// friction patches and colours of this step's rows go to the [cur] buffers
 	PXH_CHECK(s->patch[cur].ensure(6 * Cc));
 	PXH_CHECK(s->patchKey[cur].ensure(Cc));
 	PXH_CHECK(s->patchBroken[cur].ensure(Cc));
-	pxh::PatchIO pio{};
+	PXH_CHECK(s->patchColor[cur].ensure(Cc));
 	pio.cur = s->patch[cur].p;
 	pio.curKey = s->patchKey[cur].p;
-	pio.corrDist = d.friction_correlation_distance;
-	if (s->nPrevRows > 0)
-	{
-		uint32_t size = 64;
-		while (size < 2u * uint32_t(s->nPrevRows))
-			size <<= 1;
-		PXH_CHECK(s->htKey.ensure(size));
-		PXH_CHECK(s->htVal.ensure(size));
-		pxh::launch_patch_table(s->nPrevRows, s->patchKey[prv].p, s->htKey.p, s->htVal.p, size - 1, st);
-		pio.htKey = s->htKey.p;
-		pio.htVal = s->htVal.p;
-		pio.htMask = size - 1;
-		pio.prev = s->patch[prv].p;
-		pio.prevBroken = s->patchBroken[prv].p;
-	}
+	pio.curColor = s->patchColor[cur].p;
+	pio.conColor = s->conColor.p;
 	pk = s->profBegin(4);
 	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
 	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, pio, s->counters.p,
@@ -744,7 +759,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	                         hipMemcpyDeviceToHost, st));
 	s->profEnd(pk);
 
-	// 6. solve: position iterations by colour with substep integration, then
+	// 7. solve: position iterations by colour with substep integration, then
 	//    velocity iterations (DyTGSDynamics.cpp:2515-2790)
 	for (int it = 0; it < P_ + V_; it++)
 	{
@@ -770,7 +785,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	s->nPrevRows = C;
 	s->patchCur = cur ^ 1;
 
-	// 7. write back poses and velocities
+	// 8. write back poses and velocities
 	pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
 	PXH_CHECK(hipGetLastError());
 	s->simulating = true;

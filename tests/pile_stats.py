@@ -13,37 +13,45 @@ def lowest_points(shape_type, dims, pose):
     return pose[:, 1] - reach
 
 
-def pile_stats(scene, pose, lin_vel):
+def pile_stats(scene, pose, lin_vel, rest_speed=0.1):
+    """height: mean body height; potential: total potential energy;
+    penetration: worst floor penetration among resting bodies (speed below
+    `rest_speed`); jitter: mean speed of the resting bodies."""
     low = lowest_points(scene.shape_type, scene.dims, pose)
+    speed = np.linalg.norm(lin_vel, axis=1)
+    rest = speed < rest_speed
     return {
         "height": float(pose[:, 1].mean()),
         "potential": float((scene.mass * 9.81 * pose[:, 1]).sum()),
-        "penetration": float(max(0.0, -low.min())),
-        "speed": float(np.linalg.norm(lin_vel, axis=1).mean()),
+        "penetration": float(max(0.0, -low[rest].min())) if rest.any() else 0.0,
+        "jitter": float(speed[rest].mean()) if rest.any() else 0.0,
     }
 
 
 def run_stats(world, scene, steps, dt=1.0 / 60.0, tail=60):
     """Steps `world` (anything with step(dt) and bodies()) and returns the final
-    height and potential energy with the worst penetration and the mean speed
-    over the last `tail` steps."""
-    pen, speed = 0.0, []
+    height and potential energy, with the resting penetration and jitter
+    averaged over the last `tail` steps (transient impacts of bodies still
+    tumbling off the pile are left out: they are single chaotic events)."""
+    pen, jit = [], []
     for s in range(steps):
         world.step(dt)
         if s >= steps - tail:
             pose, lv, _ = world.bodies()
             st = pile_stats(scene, pose, lv)
-            pen = max(pen, st["penetration"])
-            speed.append(st["speed"])
-    st["penetration"] = pen
-    st["speed"] = float(np.mean(speed))
+            pen.append(st["penetration"])
+            jit.append(st["jitter"])
+    st["penetration"] = float(np.mean(pen))
+    st["jitter"] = float(np.mean(jit))
     return st
 
 
-def assert_stats_close(a, b, rel=0.05, pen_abs=1e-3, speed_abs=0.02):
-    """Height and potential energy within `rel`; worst penetration within
-    `pen_abs` metres; mean speed within `speed_abs` m/s."""
+def assert_stats_close(a, b, rel=0.05, pen_abs=3e-4, jitter_rel=0.1):
+    """Height and potential energy within `rel`; resting penetration within
+    `pen_abs` metres (about a third of its typical 0.8 mm; per-seed spread in
+    the reference itself is of that order); resting jitter within
+    `jitter_rel`."""
     assert abs(a["height"] - b["height"]) <= rel * abs(b["height"]), (a, b)
     assert abs(a["potential"] - b["potential"]) <= rel * abs(b["potential"]), (a, b)
     assert abs(a["penetration"] - b["penetration"]) <= pen_abs, (a, b)
-    assert abs(a["speed"] - b["speed"]) <= speed_abs, (a, b)
+    assert abs(a["jitter"] - b["jitter"]) <= jitter_rel * abs(b["jitter"]), (a, b)

@@ -232,8 +232,8 @@ def test_friction_anchors_persist_vs_live_reference(vx):
 @pytest.mark.parametrize("grid,seed", [((6, 3, 6), 1), ((8, 4, 8), 2)])
 def test_long_run_statistics_vs_live_reference(grid, seed):
     """300 steps of a settling box pile: trajectories diverge chaotically, the
-    pile statistics stay within 5% (height, potential energy), 1 mm (worst
-    floor penetration) and 0.02 m/s (mean speed over the last second)."""
+    pile statistics stay close: height and potential energy within 5%, resting
+    floor penetration within 0.3 mm, resting jitter within 10%."""
     import pile_stats as PS
     sc = S.box_grid_drop(nx=grid[0], ny=grid[1], nz=grid[2], seed=seed)
     a = PS.run_stats(L.OracleWorld(sc, 4, 1), sc, 300)

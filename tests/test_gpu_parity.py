@@ -230,8 +230,8 @@ def test_column_plane_constraints_last(cuda):
 @pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
 def test_long_run_statistics_match_reference(cuda):
     """300 steps of a 256-box settling pile on the GPU against the reference
-    CPU SDK: height and potential energy within 5%, worst floor penetration
-    within 1 mm, mean speed over the last second within 0.02 m/s."""
+    CPU SDK: height and potential energy within 5%, resting floor penetration
+    within 0.3 mm, resting jitter within 10%."""
     import pile_stats as PS
     sc = S.box_grid_drop(nx=8, ny=4, nz=8, seed=2)
     a = PS.run_stats(P.Scene(sc, position_iterations=4, velocity_iterations=1), sc, 300)
