@@ -47,6 +47,31 @@ def solve_bytes_per_step(st: dict, iters: int) -> float:
     return iters * (32.0 * T + 40.0 * K + 40.0 * F + 16.0 * Fd + 96.0 * endpoints)
 
 
+def trace_summary(path: str, skip_steps: int) -> None:
+    """Per-step kernel table from a rocprofv3 kernel-trace CSV
+    (`--kernel-trace --output-format csv`), over the dispatches after the
+    `skip_steps`-th k_finalize (one k_finalize ends every step)."""
+    import collections
+    import csv
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    fin = [i for i, r in enumerate(rows) if "k_finalize" in r["Kernel_Name"]]
+    sel = rows[fin[skip_steps - 1] + 1:] if skip_steps > 0 else rows
+    steps = max(1, sum(1 for r in sel if "k_finalize" in r["Kernel_Name"]))
+    agg = collections.defaultdict(lambda: [0, 0])
+    for r in sel:
+        name = r["Kernel_Name"].split("(")[0].replace("void ", "")
+        agg[name][0] += 1
+        agg[name][1] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+    busy = sum(v[1] for v in agg.values())
+    span = int(sel[-1]["End_Timestamp"]) - int(sel[0]["Start_Timestamp"])
+    print(f"steps {steps}; dispatches {len(sel)}; per step: kernel busy {busy / 1e6 / steps:.3f} ms, "
+          f"first start to last end {span / 1e6 / steps:.3f} ms\n")
+    print("| kernel | calls/step | ms/step | avg us | % busy |")
+    print("|---|---:|---:|---:|---:|")
+    for k, (c, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        print(f"| {k} | {c / steps:.1f} | {d / 1e6 / steps:.3f} | {d / c / 1e3:.2f} | {100 * d / busy:.1f} |")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -56,7 +81,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="print the per-step kernel table of a rocprofv3 kernel trace and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(args.trace_summary, args.warmup)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the step of one scene runs on a single MI355X in this build "

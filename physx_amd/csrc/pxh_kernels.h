@@ -10,10 +10,11 @@ namespace pxh {
 // (PxTGSSolverBodyVel + PxTGSSolverBodyTxInertia, physx/include/solver/PxSolverDefs.h)
 struct SolverBodies
 {
-	float4* lin;      // linear velocity, w = inverse mass
-	float4* ang;      // angular velocity in sqrt-inertia space
-	float4* dlin;     // accumulated linear motion this step
-	float4* dang;     // accumulated angular motion (sqrt-inertia space)
+	// velocity record, 4 float4 per body at [4 * i] (one 64-byte line for the
+	// solver's gathers): linear velocity | inverse mass, angular velocity in
+	// sqrt-inertia space, accumulated linear motion this step, accumulated
+	// angular motion (sqrt-inertia space)
+	float4* vel;
 	float4* p;        // body position during the step
 	float4* dq;       // rotation accumulated this step
 	float4* i0;       // world sqrt inverse inertia (xx yy zz xy)

@@ -815,8 +815,8 @@ __global__ void k_color_keys(int C, const int* conColor, uint64_t* keys, uint32_
 // + copyToSolverBodyDataStep (lowleveldynamics/src/DyTGSDynamics.cpp:154-242)
 __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                                const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                               float maxAngSq, float4* svLin, float4* svAng, float4* svDLin, float4* svDAng,
-                               float4* svP, float4* svDQ, float4* svI0, float4* svI1, float4* svOrigLin)
+                               float maxAngSq, float4* vel, float4* svP, float4* svDQ, float4* svI0, float4* svI1,
+                               float4* svOrigLin)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
@@ -842,10 +842,11 @@ __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, cons
 	const m33 Rt = mtrans(R);
 	const m33 S = mmm(A, Rt);
 	const m33 SI = mmm(B, Rt);
-	svLin[i] = f4(lv, l.w);
-	svAng[i] = f4(mmul(SI, av), 0.0f);
-	svDLin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	svDAng[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	float4* v = vel + 4 * int64_t(i);
+	v[0] = f4(lv, l.w);
+	v[1] = f4(mmul(SI, av), 0.0f);
+	v[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	v[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	svP[i] = pos[i];
 	svDQ[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 	// symmetric sqrt inverse inertia: (xx yy zz xy) (xz yz)
@@ -864,7 +865,7 @@ PXH_D m33 loadSym(float4 a, float4 b)
 }
 
 PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
                     const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
                     float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
                     uint32_t& nFricOut);
@@ -876,7 +877,7 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 //  DyFrictionCorrelation.cpp:136-316)
 __global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                        const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
-                       const float4* svLin, const float4* svOrigLin, const float4* svI0, const float4* svI1,
+                       const float4* vel, const float4* svOrigLin, const float4* svI0, const float4* svI1,
                        float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
                        PatchIO io, unsigned long long* counters)
 {
@@ -889,7 +890,7 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 		myCnt = uint32_t(__float_as_int(ctN[conSlot[c]].w));
 		myDyn = conB[c] >= 0 ? 1u : 0u;
 	}
-	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, svLin, svOrigLin, svI0, svI1, invStepDt, scale,
+	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, vel, svOrigLin, svI0, svI1, invStepDt, scale,
 	         restDist, frictionOffsetThreshold, rows, io, myFric);
 	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
 	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
@@ -903,7 +904,7 @@ __global__ void k_prep(int C, const int* order, const int* conA, const int* conB
 }
 
 PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* svLin,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
                     const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
                     float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
                     uint32_t& nFricOut)
@@ -927,18 +928,25 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	if (dynB)
 	{
 		Sb = loadSym(svI0[b], svI1[b]);
-		invMassB = svLin[b].w;
+		invMassB = vel[4 * int64_t(b)].w;
 		f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
 		linB = mk(svOrigLin[b]);
 	}
-	const float invMassA = svLin[a].w;
+	const float invMassA = vel[4 * int64_t(a)].w;
 	const float nlen2 = dot(n, n);
 	const float imn0 = invMassA * nlen2, imn1 = -invMassB * nlen2;
 	const float invDtp8 = invStepDt * scale;
+	// loops run to the fixed bound 4 with guards so the point arrays stay in
+	// registers (a loop to cnt would index them dynamically, in scratch)
 	f3 cp[4];
 	float cs[4];
-	for (int k = 0; k < cnt; k++)
+#pragma unroll
+	for (int k = 0; k < 4; k++)
 	{
+		cp[k] = mk(0, 0, 0);
+		cs[k] = 0.0f;
+		if (k >= cnt)
+			continue;
 		const float4 q = ctP[4 * int64_t(slot) + k];
 		cp[k] = mk(q);
 		cs[k] = q.w;
@@ -976,8 +984,9 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 				if (dot(pn0, qrot(b10.q, pn1)) > 0.999f)
 				{
 					bool separated = false;
-					for (int j = 0; j < pc; j++)
-						if (!(fabsf(dot(pa0[j] - xtrans(b10, pa1[j]), pn0)) < io.corrDist))
+#pragma unroll
+					for (int j = 0; j < 2; j++)
+						if (j < pc && !(fabsf(dot(pa0[j] - xtrans(b10, pa1[j]), pn0)) < io.corrDist))
 							separated = true;
 					if (!separated && !(dot(n, qrot(f0.q, pn0)) < 0.999f))
 					{
@@ -999,8 +1008,11 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	if (nKept == 2)
 	{
 		f3 mn = cp[0], mx = cp[0];
-		for (int j = 1; j < cnt; j++)
+#pragma unroll
+		for (int j = 1; j < 4; j++)
 		{
+			if (j >= cnt)
+				break;
 			mn = mk(fminf(mn.x, cp[j].x), fminf(mn.y, cp[j].y), fminf(mn.z, cp[j].z));
 			mx = mk(fmaxf(mx.x, cp[j].x), fmaxf(mx.y, cp[j].y), fmaxf(mx.z, cp[j].z));
 		}
@@ -1020,9 +1032,10 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 		}
 		float pd2 = 0.0f;
 		const float thr = frictionOffsetThreshold + restDist;
-		for (int j = 0; j < cnt; j++)
+#pragma unroll
+		for (int j = 0; j < 4; j++)
 		{
-			if (!(cs[j] < thr))
+			if (j >= cnt || !(cs[j] < thr))
 				continue;
 			const f3 wp = cp[j];
 			if (nAnch == 0) { wa0 = wp; nAnch = 1; }
@@ -1084,8 +1097,11 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	t0 = normalize(t0);
 	const f3 t1 = normalize(cross(n, t0));
 	const f3 relTr = f0.p - f1.p;
-	for (int j = 0; j < nAnch; j++)
+#pragma unroll
+	for (int j = 0; j < 2; j++)
 	{
+		if (j >= nAnch)
+			break;
 		const f3 ra = qrot(f0.q, pa0[j]), rb = qrot(f1.q, pa1[j]);
 		const f3 err = (ra - rb) + relTr;
 		for (int d = 0; d < 2; d++)
@@ -1096,7 +1112,8 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 			const int r = 2 * j + d;
 			rows.fT[r * int64_t(C) + o] = f4(t, dot(err, t));
 			rows.fA[r * int64_t(C) + o] = f4(raXnI, unit > 0.0f ? scale / unit : 0.0f);
-			rows.fB[r * int64_t(C) + o] = f4(rbXnI, 0.0f);
+			if (dynB)
+				rows.fB[r * int64_t(C) + o] = f4(rbXnI, 0.0f);
 			rows.fF[r * int64_t(C) + o] = 0.0f;
 		}
 	}
@@ -1107,8 +1124,7 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 // All rows of the lane are loaded before any arithmetic so a wave keeps every
 // gather in flight at once; 64-lane blocks spread a colour over all CUs.
 constexpr int SOLVE_THREADS = 64;
-__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* svLin,
-                                                          float4* svAng, const float4* svDLin, const float4* svDAng,
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* vel,
                                                           float minPen, float muS, float muD, float invStepDt,
                                                           float biasScale, float scale, uint32_t* broken)
 {
@@ -1121,11 +1137,13 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 	const int cnt = hi.z & 0xFF, nF = hi.z >> 8;
 	const bool dynB = b >= 0;
 	// gathers: both bodies
-	const float4 la4 = svLin[a], aa4 = svAng[a], am04 = svDAng[a], dla4 = svDLin[a];
+	const float4* va = vel + 4 * int64_t(a);
+	const float4 la4 = va[0], aa4 = va[1], dla4 = va[2], am04 = va[3];
 	float4 lb4 = make_float4(0, 0, 0, 0), ab4 = lb4, am14 = lb4, dlb4 = lb4;
 	if (dynB)
 	{
-		lb4 = svLin[b]; ab4 = svAng[b]; am14 = svDAng[b]; dlb4 = svDLin[b];
+		const float4* vb = vel + 4 * int64_t(b);
+		lb4 = vb[0]; ab4 = vb[1]; dlb4 = vb[2]; am14 = vb[3];
 	}
 	// rows
 	float4 nA[4], nB[4], fT[4], fA[4], fB[4];
@@ -1233,24 +1251,25 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 		if (k < nF)
 			rows.fF[ri] = fFc[k];
 	}
-	svLin[a] = f4(l0, invMassA);
-	svAng[a] = f4(a0, 0.0f);
+	vel[4 * int64_t(a)] = f4(l0, invMassA);
+	vel[4 * int64_t(a) + 1] = f4(a0, 0.0f);
 	if (dynB)
 	{
-		svLin[b] = f4(l1, invMassB);
-		svAng[b] = f4(a1, 0.0f);
+		vel[4 * int64_t(b)] = f4(l1, invMassB);
+		vel[4 * int64_t(b) + 1] = f4(a1, 0.0f);
 	}
 }
 
 // integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476)
-__global__ void k_integrate(int n, float dt, const float4* svLin, const float4* svAng, float4* svDLin, float4* svDAng,
-                            float4* svP, float4* svDQ, const float4* svI0, const float4* svI1)
+__global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* svDQ, const float4* svI0,
+                            const float4* svI1)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	const f3 lv = mk(svLin[i]);
-	const f3 av = mk(svAng[i]);
+	float4* v = vel + 4 * int64_t(i);
+	const f3 lv = mk(v[0]);
+	const f3 av = mk(v[1]);
 	const f3 delta = lv * dt;
 	const f3 amv = mmul(loadSym(svI0[i], svI1[i]), av);
 	const float w2 = dot(amv, amv);
@@ -1269,23 +1288,21 @@ __global__ void k_integrate(int n, float dt, const float4* svLin, const float4* 
 		res.x += dq.x * q; res.y += dq.y * q; res.z += dq.z * q; res.w += dq.w * q;
 		svDQ[i] = f4(qnormalize(res));
 	}
-	svDAng[i] = f4(mk(svDAng[i]) + av * dt, 0.0f);
-	svDLin[i] = f4(mk(svDLin[i]) + delta, 0.0f);
+	v[3] = f4(mk(v[3]) + av * dt, 0.0f);
+	v[2] = f4(mk(v[2]) + delta, 0.0f);
 }
 
 // copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580)
-__global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, const float4* svLin,
-                           const float4* svAng, const float4* svP, const float4* svDQ, const float4* svI0,
-                           const float4* svI1)
+__global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, const float4* vel,
+                           const float4* svP, const float4* svDQ, const float4* svI0, const float4* svI1)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	rot[i] = f4(qnormalize(qmul(mkq(svDQ[i]), mkq(rot[i]))));
 	pos[i] = svP[i];
-	const float4 l = svLin[i];
-	lin[i] = l;
-	ang[i] = f4(mmul(loadSym(svI0[i], svI1[i]), mk(svAng[i])), 0.0f);
+	lin[i] = vel[4 * int64_t(i)];
+	ang[i] = f4(mmul(loadSym(svI0[i], svI1[i]), mk(vel[4 * int64_t(i) + 1])), 0.0f);
 }
 
 // ---------------------------------------------------------------------------
@@ -1587,32 +1604,31 @@ void launch_preintegrate(int n, const float4* pos, const float4* rot, const floa
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                          float maxAngSq, SolverBodies sb, hipStream_t st)
 {
-	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, sb.lin,
-	           sb.ang, sb.dlin, sb.dang, sb.p, sb.dq, sb.i0, sb.i1, sb.origLin);
+	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, sb.vel,
+	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin);
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
                  hipStream_t st)
 {
-	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.lin, sb.origLin, sb.i0, sb.i1,
+	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.vel, sb.origLin, sb.i0, sb.i1,
 	           invStepDt, scale, restDist, fot, rows, patches, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st)
 {
 	if (end > begin)
-		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.lin, sb.ang, sb.dlin,
-		                                                                     sb.dang, minPen, muS, muD, invStepDt,
+		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.vel, minPen, muS, muD, invStepDt,
 		                                                                     biasScale, scale, broken);
 }
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
 {
-	PXH_LAUNCH(k_integrate, n, n, dt, sb.lin, sb.ang, sb.dlin, sb.dang, sb.p, sb.dq, sb.i0, sb.i1);
+	PXH_LAUNCH(k_integrate, n, n, dt, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
 }
 void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st)
 {
-	PXH_LAUNCH(k_finalize, n, n, pos, rot, lin, ang, sb.lin, sb.ang, sb.p, sb.dq, sb.i0, sb.i1);
+	PXH_LAUNCH(k_finalize, n, n, pos, rot, lin, ang, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
 }
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
                      const float4* ang, float* out, hipStream_t st)

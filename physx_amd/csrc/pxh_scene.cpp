@@ -137,7 +137,7 @@ struct pxh_scene_t
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
 	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0;
 	pxh::SolverBodies sb{};
-	DBuf<float4> sbLin, sbAng, sbDLin, sbDAng, sbP, sbDQ, sbI0, sbI1, sbOrig;
+	DBuf<float4> sbVel, sbP, sbDQ, sbI0, sbI1, sbOrig;
 	DBuf<int4> rhI;
 	DBuf<float4> rhF, rnA, rnB, rfT, rfA, rfB;
 	DBuf<float> rnF, rfF;
@@ -393,9 +393,8 @@ int32_t pxh_scene_release(pxh_scene s)
 		return PXH_OK;
 	(void)hipStreamSynchronize(s->stream);
 	DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->planes, &s->bmin, &s->bmax,
-	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbLin, &s->sbAng, &s->sbDLin,
-	                       &s->sbDAng, &s->sbP, &s->sbDQ, &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF,
-	                       &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
+	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbVel, &s->sbP, &s->sbDQ,
+	                       &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF, &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
 	for (auto* b : f4s)
 		b->release();
 	DBuf<int>* is[] = {&s->scene, &s->local, &s->bodyStart, &s->planeLocal, &s->qBody, &s->qPlane, &s->conA,
@@ -722,11 +721,9 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	s->statRounds = rounds;
 
 	// 6. solver bodies + contact rows
-	PXH_CHECK(s->sbLin.ensure(n)); PXH_CHECK(s->sbAng.ensure(n)); PXH_CHECK(s->sbDLin.ensure(n));
-	PXH_CHECK(s->sbDAng.ensure(n)); PXH_CHECK(s->sbP.ensure(n)); PXH_CHECK(s->sbDQ.ensure(n));
+	PXH_CHECK(s->sbVel.ensure(size_t(4) * n)); PXH_CHECK(s->sbP.ensure(n)); PXH_CHECK(s->sbDQ.ensure(n));
 	PXH_CHECK(s->sbI0.ensure(n)); PXH_CHECK(s->sbI1.ensure(n)); PXH_CHECK(s->sbOrig.ensure(n));
-	pxh::SolverBodies sb{s->sbLin.p, s->sbAng.p, s->sbDLin.p, s->sbDAng.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p,
-	                     s->sbOrig.p};
+	pxh::SolverBodies sb{s->sbVel.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p, s->sbOrig.p};
 	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
 	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, sb, st);
