@@ -11,6 +11,13 @@ Prints one JSON line with the body-steps/s, the roofline of the dominant
 kernel (the colour-partition solve, HBM-bound, timed with HIP events on the
 scene's stream over the timed steps) and the reference CPU pipeline timed on a
 bounded sample of the same state on the host cores.
+
+The same line carries the batched form ("batched"): 8,192 independent worlds
+of log-uniform [8, 512] bodies (box stacks and sphere/box piles, about 1.0M
+bodies), TGS 4/1, dt = 1/120 s, stepped as one scene (one launch sequence per
+step for all worlds), with its body-steps/s and p50/p99 step latency, beside
+the plain per-scene path (each world its own scene, stepped one after the
+other with the same kernels) measured on an evenly spaced sample of the worlds.
 """
 from __future__ import annotations
 
@@ -72,6 +79,63 @@ def trace_summary(path: str, skip_steps: int) -> None:
         print(f"| {k} | {c / steps:.1f} | {d / 1e6 / steps:.3f} | {d / c / 1e3:.2f} | {100 * d / busy:.1f} |")
 
 
+def run_batched(args, P, scenes):
+    """Batched step of many small worlds against the plain per-scene path."""
+    import torch
+    dt = 1.0 / 120.0
+    sc = scenes.batched_piles(n_scenes=args.scenes)
+    log(f"batched: {sc.n_scenes} worlds, {sc.n} bodies")
+    g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
+    for s in range(args.batched_warmup):
+        g.step(dt)
+    torch.cuda.synchronize()
+    times = []
+    t0 = time.perf_counter()
+    for s in range(args.batched_steps):
+        a = time.perf_counter()
+        g.simulate(dt)
+        g.fetch_results(True)
+        times.append((time.perf_counter() - a) * 1e3)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = g.stats()
+    g.close()
+    batched = sc.n * args.batched_steps / elapsed
+    # plain path: every k-th world as its own scene, same warm-up, stepped in turn
+    stride = max(1, sc.n_scenes // args.plain_sample)
+    worlds = list(range(0, sc.n_scenes, stride))[: args.plain_sample]
+    subs = [sc.subset(w) for w in worlds]
+    plain = [P.Scene(x, position_iterations=4, velocity_iterations=1) for x in subs]
+    for s in range(args.batched_warmup):
+        for x in plain:
+            x.step(dt)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.plain_steps):
+        for x in plain:
+            x.simulate(dt)
+            x.fetch_results(True)
+    torch.cuda.synchronize()
+    pe = time.perf_counter() - t0
+    nb = sum(x.n for x in subs)
+    for x in plain:
+        x.close()
+    plain_rate = nb * args.plain_steps / pe
+    ts = np.array(times)
+    out = {"workload": f"batched_piles: {sc.n_scenes} worlds, {sc.n} bodies, log-uniform [8,512] per world, "
+                       f"TGS 4/1, dt 1/120", "worlds": sc.n_scenes, "bodies": sc.n, "steps": args.batched_steps,
+           "warmup": args.batched_warmup, "body_steps_per_s": round(batched, 1),
+           "ms_per_step": round(1e3 * elapsed / args.batched_steps, 3),
+           "p50_step_ms": round(float(np.percentile(ts, 50)), 3), "p99_step_ms": round(float(np.percentile(ts, 99)), 3),
+           "contact_pairs_per_s": round(st["constraints"] * args.batched_steps / elapsed, 1),
+           "plain_body_steps_per_s": round(plain_rate, 1),
+           "plain_sample": f"{len(worlds)} worlds (every {stride}th, {nb} bodies), {args.plain_steps} steps each "
+                           f"after the same warm-up, one scene per world stepped in turn",
+           "speedup_vs_plain": round(batched / plain_rate, 2)}
+    log(f"batched: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,6 +145,12 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--scenes", type=int, default=8192)
+    ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--batched-warmup", type=int, default=120)
+    ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="print the per-step kernel table of a rocprofv3 kernel trace and exit")
     args = ap.parse_args()
@@ -164,6 +234,9 @@ def main():
                    "ms_per_step": round(1e3 * total / args.cpu_steps, 2)}
             log(f"cpu baseline: {cpu}")
 
+    gpu.close()
+    batched = None if args.no_batched else run_batched(args, P, scenes)
+
     sms = np.array(step_ms)
     out = {
         "metric": "body-steps/sec at 262k bodies",
@@ -188,6 +261,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

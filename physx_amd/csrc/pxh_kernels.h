@@ -64,31 +64,40 @@ void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t
 
 void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
                    float4* bmax, hipStream_t st);
+// rowOff: per-world row table (nScenes + 1), each world's slice capping its rows
 void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
-                       float4* gridF, int2* gridI, uint32_t* rows, hipStream_t st);
+                       float4* gridF, int2* gridI, const uint32_t* rowOff, hipStream_t st);
+void launch_row_off_single(uint32_t* rowOff, uint32_t cap, hipStream_t st);
 void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
                  const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_gather_sorted(int n, const uint32_t* sv, const float4* bmin, const float4* bmax, float4* smin,
                           float4* smax, hipStream_t st);
 void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t totalRows, hipStream_t st);
+// write pass: pairs beyond pairCap are counted but not written
 void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
                   const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, hipStream_t st);
-void launch_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom, float cd,
-                     float tol, float4* ctN, float4* ctP, hipStream_t st);
+                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap, hipStream_t st);
+// pair p < min(cap, *dP) fills slot slotBase + p
+void launch_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
+                     const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP,
+                     hipStream_t st);
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);
-void launch_touch_flags(int S, const float4* ctN, uint32_t* flags, hipStream_t st);
-void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs, const int* qBody,
-                      const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
-                      uint64_t* conKey, int* conColor, hipStream_t st);
-void launch_color_round(int m, const int* list, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st);
-void launch_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags, hipStream_t st);
+// slots [0, Q) are body-plane pairs, body pairs follow up to the device count *dP
+void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st);
+void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
+                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
+                      int* conB, int* conSlot, uint64_t* conKey, int* conColor, hipStream_t st);
+// colouring over list[i] (identity when null) for i below *mDev (or m when
+// mDev is null); m bounds the count
+void launch_color_round(int m, const uint32_t* mDev, const int* list, int round, const int* conA, const int* conB,
+                        const uint64_t* conKey, int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st);
+void launch_color_uncolored(int m, const uint32_t* mDev, const int* list, const int* conColor, uint32_t* flags,
+                            hipStream_t st);
 void launch_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out,
                           hipStream_t st);
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st);
-void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st);
+void launch_color_keys(int C, const int* conColor, uint32_t limit, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
                           hipStream_t st);
 void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal, uint32_t htMask, hipStream_t st);

@@ -309,7 +309,11 @@ __global__ void k_grid_init(int nScenes, uint32_t* acc)
 	}
 }
 
-__global__ void k_grid_finish(int nScenes, const uint32_t* acc, float4* gridF, int2* gridI, uint32_t* rows)
+// Each world's rows are capped by its slice of rowOff (the host sizes the row
+// table up front, so no count comes back before the sort): the cell grows
+// until ny * nz fits. Cells only need to be at least as wide as the widest
+// box; wider cells give the same pairs with longer sweeps.
+__global__ void k_grid_finish(int nScenes, const uint32_t* acc, float4* gridF, int2* gridI, const uint32_t* rowOff)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= nScenes)
@@ -321,12 +325,34 @@ __global__ void k_grid_finish(int nScenes, const uint32_t* acc, float4* gridF, i
 	const float wm = empty ? 0.0f : decode_float(w[4]);
 	float cell = wm * 1.01f + 1e-6f;
 	cell = fmaxf(cell, fmaxf(hy - ly, hz - lz) * (1.0f / 1000.0f));
-	int ny = int((hy - ly) / cell) + 1, nz = int((hz - lz) / cell) + 1;
-	ny = max(1, min(ny, 1024));
-	nz = max(1, min(nz, 1024));
+	const int64_t cap = int64_t(rowOff[s + 1] - rowOff[s]);
+	int ny = 1, nz = 1;
+	for (int it = 0; it < 96; it++)
+	{
+		ny = max(1, min(int((hy - ly) / cell) + 1, 1024));
+		nz = max(1, min(int((hz - lz) / cell) + 1, 1024));
+		if (int64_t(ny) * nz <= cap)
+			break;
+		cell *= 1.25f;
+	}
+	if (int64_t(ny) * nz > cap)
+	{
+		ny = 1;
+		nz = 1;
+		cell = fmaxf(cell, fmaxf(hy - ly, hz - lz) + 1.0f);
+	}
 	gridF[s] = make_float4(ly, lz, 1.0f / cell, cell);
 	gridI[s] = make_int2(ny, nz);
-	rows[s] = uint32_t(ny * nz);
+}
+
+// row table of a single world with cap rows
+__global__ void k_row_off_single(uint32_t* rowOff, uint32_t cap)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0)
+	{
+		rowOff[0] = 0u;
+		rowOff[1] = cap;
+	}
 }
 
 __global__ void k_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
@@ -360,19 +386,29 @@ __global__ void k_gather_sorted(int n, const uint32_t* sortedVals, const float4*
 	smax[s] = b;
 }
 
-// rowStart[r] = first sorted position of row r; rowStart[totalRows] = n
+// rowStart[r] = first sorted position of row r; rowStart[totalRows] = n.
+// Keys past totalRows count as totalRows (rowStart has totalRows + 2 entries).
+// Lane s fills the rows from the previous key's row (exclusive) to its own;
+// the rows after the last key are filled by k_row_tail, one lane per row.
 __global__ void k_row_start(int n, const uint64_t* sortedKeys, uint32_t* rowStart, uint32_t totalRows)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= n)
 		return;
-	const uint32_t r = uint32_t(sortedKeys[s] >> 32);
-	const int64_t rp = s == 0 ? -1 : int64_t(sortedKeys[s - 1] >> 32);
+	const uint32_t r = min(uint32_t(sortedKeys[s] >> 32), totalRows);
+	const int64_t rp = s == 0 ? -1 : int64_t(min(uint32_t(sortedKeys[s - 1] >> 32), totalRows));
 	for (int64_t rr = rp + 1; rr <= int64_t(r); rr++)
 		rowStart[rr] = uint32_t(s);
-	if (s == n - 1)
-		for (uint32_t rr = r + 1; rr <= totalRows; rr++)
-			rowStart[rr] = uint32_t(n);
+}
+
+__global__ void k_row_tail(int n, const uint64_t* sortedKeys, uint32_t* rowStart, uint32_t totalRows)
+{
+	const int64_t rr = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+	if (rr > int64_t(totalRows) || n <= 0)
+		return;
+	const uint32_t last = min(uint32_t(sortedKeys[n - 1] >> 32), totalRows);
+	if (rr > int64_t(last))
+		rowStart[rr] = uint32_t(n);
 }
 
 __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
@@ -394,7 +430,7 @@ PXH_D int pairClass(int t0, int t1) { return (t0 == GEOM_BOX) + (t1 == GEOM_BOX)
 template <bool WRITE>
 __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, const float4* smax, const int* scene,
                         const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                        uint32_t* counts, const uint32_t* offsets, uint2* pairs)
+                        uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= n)
@@ -466,7 +502,10 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 					uint32_t pa, pb;
 					if (c == 1) { pa = ti == GEOM_SPHERE ? i : jj; pb = ti == GEOM_SPHERE ? jj : i; }
 					else { pa = min(i, jj); pb = max(i, jj); }
-					pairs[off[c]++] = make_uint2(pa, pb);
+					// pairs past the buffer's capacity are counted, not written
+					if (off[c] < pairCap)
+						pairs[off[c]] = make_uint2(pa, pb);
+					off[c]++;
 				}
 				else
 					cnt[c]++;
@@ -493,11 +532,13 @@ PXH_D void storeContacts(int slot, const Contacts& c, int cnt, float4* ctN, floa
 
 // Body-body pairs; the shape order is the reference contact-method table's
 // (lower geometry type first), so body a of a sphere-box pair is the sphere.
-__global__ void k_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom,
-                           float cd, float tol, float4* ctN, float4* ctP)
+// Lanes past the pair count on the device (*dP) exit; pair p fills slot
+// slotBase + p.
+__global__ void k_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
+                           const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= P)
+	if (p >= cap || uint32_t(p) >= *dP)
 		return;
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
@@ -520,7 +561,7 @@ __global__ void k_np_pairs(int P, const uint2* pairs, const float4* pos, const f
 		x1.p = mk(p1); x1.q = mkq(rot[ab.y]);
 		cnt = boxBox(mk(g0.x, g0.y, g0.z), x0, mk(g1.x, g1.y, g1.z), x1, cd, tol, c);
 	}
-	storeContacts(p, c, cnt, ctN, ctP);
+	storeContacts(slotBase + p, c, cnt, ctN, ctP);
 }
 
 // Body-plane pairs (every body against every plane of its world; a plane's
@@ -551,35 +592,36 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 	storeContacts(slotBase + q, c, cnt, ctN, ctP);
 }
 
-__global__ void k_touch_flags(int S, const float4* ctN, uint32_t* flags)
+// slots: [0, Q) body-plane pairs, then body pairs up to the device count *dP
+__global__ void k_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s < S)
-		flags[s] = __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
+		flags[s] = (s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
 }
 
 // constraint list: touching pair slots in slot order, with the scene-local
 // ordering key used by the colouring
-__global__ void k_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
-                            const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
-                            int* conB, int* conSlot, uint64_t* conKey, int* conColor)
+__global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
+                            const uint2* pairs, const int* qBody, const int* qPlane, const int* local,
+                            const int* planeLocal, int* conA, int* conB, int* conSlot, uint64_t* conKey, int* conColor)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= S || __float_as_int(ctN[s].w) <= 0)
+	if (s >= S || !((s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0))
 		return;
 	const int c = int(scanned[s]);
 	int a, b;
 	uint32_t la, lb;
-	if (s < P)
+	if (s >= Q)
 	{
-		const uint2 ab = pairs[s];
+		const uint2 ab = pairs[s - Q];
 		a = int(ab.x); b = int(ab.y);
 		la = uint32_t(local[a]); lb = uint32_t(local[b]);
 	}
 	else
 	{
-		a = qBody[s - P];
-		const int pl = qPlane[s - P];
+		a = qBody[s];
+		const int pl = qPlane[s];
 		b = -1 - pl;
 		la = uint32_t(local[a]);
 		lb = 0x80000000u | uint32_t(planeLocal[pl]);
@@ -605,79 +647,89 @@ __global__ void k_fill_u64(uint64_t* p, int n, uint64_t v)
 }
 
 // Colouring kernels run over the still-uncoloured constraints: list[i] when a
-// compacted list is given, else i.
+// compacted list is given, else i, for i below the count *mDev on the device
+// (or the host bound m when mDev is null). They loop grid-stride so the grid
+// can be sized for the bound while few constraints remain.
 PXH_D int listed(const int* list, int i) { return list ? list[i] : i; }
+PXH_D int listCount(int m, const uint32_t* mDev) { return mDev ? min(m, int(*mDev)) : m; }
+#define PXH_GRID_STRIDE(i, count) \
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < (count); i += gridDim.x * blockDim.x)
 
-__global__ void k_color_reset(int m, const int* list, const int* conA, const int* conB, unsigned long long* best)
+__global__ void k_color_reset(int m, const uint32_t* mDev, const int* list, const int* conA, const int* conB,
+                              unsigned long long* best)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m)
-		return;
-	const int c = listed(list, i);
-	best[conA[c]] = ~0ull;
-	const int b = conB[c];
-	if (b >= 0)
-		best[b] = ~0ull;
+	const int mm = listCount(m, mDev);
+	PXH_GRID_STRIDE(i, mm)
+	{
+		const int c = listed(list, i);
+		best[conA[c]] = ~0ull;
+		const int b = conB[c];
+		if (b >= 0)
+			best[b] = ~0ull;
+	}
 }
 
-__global__ void k_color_min(int m, const int* list, const int* conA, const int* conB, const uint64_t* conKey,
-                            const int* conColor, unsigned long long* best)
+__global__ void k_color_min(int m, const uint32_t* mDev, const int* list, const int* conA, const int* conB,
+                            const uint64_t* conKey, const int* conColor, unsigned long long* best)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m)
-		return;
-	const int c = listed(list, i);
-	if (conColor[c] != -1)
-		return;
-	const unsigned long long k = conKey[c];
-	atomicMin(&best[conA[c]], k);
-	const int b = conB[c];
-	if (b >= 0)
-		atomicMin(&best[b], k);
+	const int mm = listCount(m, mDev);
+	PXH_GRID_STRIDE(i, mm)
+	{
+		const int c = listed(list, i);
+		if (conColor[c] != -1)
+			continue;
+		const unsigned long long k = conKey[c];
+		atomicMin(&best[conA[c]], k);
+		const int b = conB[c];
+		if (b >= 0)
+			atomicMin(&best[b], k);
+	}
 }
 
 // A local minimum takes the smallest colour used by neither of its bodies
 // (first fit on 64-bit per-body colour masks; colours past 63 fall back to
 // one colour per round, 64 + round). Minima of one round share no body, so
 // their mask updates never race with each other's reads.
-__global__ void k_color_pick(int m, const int* list, int round, const int* conA, const int* conB,
-                             const uint64_t* conKey, int* conColor, const unsigned long long* best,
+__global__ void k_color_pick(int m, const uint32_t* mDev, const int* list, int round, const int* conA,
+                             const int* conB, const uint64_t* conKey, int* conColor, const unsigned long long* best,
                              unsigned long long* mask)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= m)
-		return;
-	const int c = listed(list, i);
-	if (conColor[c] != -1)
-		return;
-	const unsigned long long k = conKey[c];
-	const int a = conA[c], b = conB[c];
-	if (best[a] == k && (b < 0 || best[b] == k))
+	const int mm = listCount(m, mDev);
+	PXH_GRID_STRIDE(i, mm)
 	{
-		const unsigned long long used = mask[a] | (b >= 0 ? mask[b] : 0ull);
-		const int col = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
-		conColor[c] = col;
-		if (col < 64)
+		const int c = listed(list, i);
+		if (conColor[c] != -1)
+			continue;
+		const unsigned long long k = conKey[c];
+		const int a = conA[c], b = conB[c];
+		if (best[a] == k && (b < 0 || best[b] == k))
 		{
-			atomicOr(&mask[a], 1ull << col);
-			if (b >= 0)
-				atomicOr(&mask[b], 1ull << col);
+			const unsigned long long used = mask[a] | (b >= 0 ? mask[b] : 0ull);
+			const int col = used == ~0ull ? 64 + round : __ffsll((long long)~used) - 1;
+			conColor[c] = col;
+			if (col < 64)
+			{
+				atomicOr(&mask[a], 1ull << col);
+				if (b >= 0)
+					atomicOr(&mask[b], 1ull << col);
+			}
 		}
 	}
 }
 
-__global__ void k_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags)
+// flags[i] for i <= m (flags[m] = 0 ends the scan)
+__global__ void k_color_uncolored(int m, const uint32_t* mDev, const int* list, const int* conColor, uint32_t* flags)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < m)
-		flags[i] = conColor[listed(list, i)] == -1 ? 1u : 0u;
+	const int mm = listCount(m, mDev);
+	PXH_GRID_STRIDE(i, m + 1)
+		flags[i] = i < mm && conColor[listed(list, i)] == -1 ? 1u : 0u;
 }
 
 __global__ void k_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < m && flags[i])
-		out[scanned[i]] = listed(list, i);
+	PXH_GRID_STRIDE(i, m)
+		if (flags[i])
+			out[scanned[i]] = listed(list, i);
 }
 
 // Body-plane constraints go after every colour of their body's body-body
@@ -797,12 +849,16 @@ __global__ void __launch_bounds__(256) k_max_color(int C, const int* conColor, u
 }
 
 // colour-ordered constraint list by a stable radix sort of (colour, index)
-__global__ void k_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals)
+// A constraint not coloured yet (-1, when the rounds have not finished) sorts
+// to colour `limit`, past every real colour.
+__global__ void k_color_keys(int C, const int* conColor, uint32_t limit, uint64_t* keys, uint32_t* vals)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c < C)
 	{
-		keys[c] = (uint64_t(uint32_t(conColor[c])) << 32) | uint32_t(c);
+		const int col = conColor[c];
+		const uint32_t k = col < 0 ? limit : min(uint32_t(col), limit);
+		keys[c] = (uint64_t(k) << 32) | uint32_t(c);
 		vals[c] = uint32_t(c);
 	}
 }
@@ -1499,12 +1555,16 @@ void launch_bounds(int n, const float4* pos, const float4* rot, const float4* ge
 	PXH_LAUNCH(k_bounds, n, n, pos, rot, geom, co, bmin, bmax);
 }
 void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
-                       float4* gridF, int2* gridI, uint32_t* rows, hipStream_t st)
+                       float4* gridF, int2* gridI, const uint32_t* rowOff, hipStream_t st)
 {
 	PXH_LAUNCH(k_grid_init, nScenes, nScenes, acc);
 	if (n > 0)
 		k_grid_reduce<<<nblk(n, 256 * GRID_ITEMS), 256, 0, st>>>(n, bmin, bmax, scene, acc);
-	PXH_LAUNCH(k_grid_finish, nScenes, nScenes, acc, gridF, gridI, rows);
+	PXH_LAUNCH(k_grid_finish, nScenes, nScenes, acc, gridF, gridI, rowOff);
+}
+void launch_row_off_single(uint32_t* rowOff, uint32_t cap, hipStream_t st)
+{
+	k_row_off_single<<<1, 64, 0, st>>>(rowOff, cap);
 }
 void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene, const float4* gridF,
                  const int2* gridI, const uint32_t* rowOff, uint64_t* keys, uint32_t* vals, hipStream_t st)
@@ -1524,54 +1584,65 @@ void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t to
 		return;
 	}
 	PXH_LAUNCH(k_row_start, n, n, sk, rowStart, totalRows);
+	PXH_LAUNCH(k_row_tail, int64_t(totalRows) + 1, n, sk, rowStart, totalRows);
 }
 void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
                   const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, hipStream_t st)
+                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap, hipStream_t st)
 {
 	if (write)
-		PXH_LAUNCH(k_sweep<true>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs);
+		PXH_LAUNCH(k_sweep<true>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs,
+		           pairCap);
 	else
-		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs);
+		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs,
+		           pairCap);
 }
-void launch_np_pairs(int P, const uint2* pairs, const float4* pos, const float4* rot, const float4* geom, float cd,
-                     float tol, float4* ctN, float4* ctP, hipStream_t st)
+void launch_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
+                     const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP,
+                     hipStream_t st)
 {
-	PXH_LAUNCH(k_np_pairs, P, P, pairs, pos, rot, geom, cd, tol, ctN, ctP);
+	PXH_LAUNCH(k_np_pairs, cap, cap, dP, slotBase, pairs, pos, rot, geom, cd, tol, ctN, ctP);
 }
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st)
 {
 	PXH_LAUNCH(k_np_planes, Q, Q, slotBase, qBody, qPlane, pos, rot, geom, planes, cd, ctN, ctP);
 }
-void launch_touch_flags(int S, const float4* ctN, uint32_t* flags, hipStream_t st)
+void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st)
 {
-	PXH_LAUNCH(k_touch_flags, S, S, ctN, flags);
+	PXH_LAUNCH(k_touch_flags, S, S, Q, dP, ctN, flags);
 }
-void launch_con_build(int S, int P, const float4* ctN, const uint32_t* scanned, const uint2* pairs, const int* qBody,
-                      const int* qPlane, const int* local, const int* planeLocal, int* conA, int* conB, int* conSlot,
-                      uint64_t* conKey, int* conColor, hipStream_t st)
+void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
+                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
+                      int* conB, int* conSlot, uint64_t* conKey, int* conColor, hipStream_t st)
 {
-	PXH_LAUNCH(k_con_build, S, S, P, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, conA, conB, conSlot, conKey,
-	           conColor);
+	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, conA, conB, conSlot,
+	           conKey, conColor);
 }
-void launch_color_round(int m, const int* list, int round, const int* conA, const int* conB, const uint64_t* conKey,
-                        int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st)
+// colouring launches are grid-stride over at most 1024 blocks
+static int colorBlocks(int m) { return std::max(1, std::min(nblk(m, 256), 1024)); }
+void launch_color_round(int m, const uint32_t* mDev, const int* list, int round, const int* conA, const int* conB,
+                        const uint64_t* conKey, int* conColor, uint64_t* best, uint64_t* mask, hipStream_t st)
 {
+	if (m <= 0)
+		return;
 	unsigned long long* bst = reinterpret_cast<unsigned long long*>(best);
-	PXH_LAUNCH(k_color_reset, m, m, list, conA, conB, bst);
-	PXH_LAUNCH(k_color_min, m, m, list, conA, conB, conKey, conColor, bst);
-	PXH_LAUNCH(k_color_pick, m, m, list, round, conA, conB, conKey, conColor, bst,
-	           reinterpret_cast<unsigned long long*>(mask));
+	const int g = colorBlocks(m);
+	k_color_reset<<<g, 256, 0, st>>>(m, mDev, list, conA, conB, bst);
+	k_color_min<<<g, 256, 0, st>>>(m, mDev, list, conA, conB, conKey, conColor, bst);
+	k_color_pick<<<g, 256, 0, st>>>(m, mDev, list, round, conA, conB, conKey, conColor, bst,
+	                                reinterpret_cast<unsigned long long*>(mask));
 }
-void launch_color_uncolored(int m, const int* list, const int* conColor, uint32_t* flags, hipStream_t st)
+void launch_color_uncolored(int m, const uint32_t* mDev, const int* list, const int* conColor, uint32_t* flags,
+                            hipStream_t st)
 {
-	PXH_LAUNCH(k_color_uncolored, m, m, list, conColor, flags);
+	k_color_uncolored<<<colorBlocks(m + 1), 256, 0, st>>>(m, mDev, list, conColor, flags);
 }
 void launch_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out,
                           hipStream_t st)
 {
-	PXH_LAUNCH(k_color_compact, m, m, list, flags, scanned, out);
+	if (m > 0)
+		k_color_compact<<<colorBlocks(m), 256, 0, st>>>(m, list, flags, scanned, out);
 }
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st) { PXH_LAUNCH(k_fill_u64, n, p, n, v); }
 void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
@@ -1591,9 +1662,9 @@ void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* c
 {
 	PXH_LAUNCH(k_con_prev, C, C, conA, conB, io, conColor, reinterpret_cast<unsigned long long*>(mask));
 }
-void launch_color_keys(int C, const int* conColor, uint64_t* keys, uint32_t* vals, hipStream_t st)
+void launch_color_keys(int C, const int* conColor, uint32_t limit, uint64_t* keys, uint32_t* vals, hipStream_t st)
 {
-	PXH_LAUNCH(k_color_keys, C, C, conColor, keys, vals);
+	PXH_LAUNCH(k_color_keys, C, C, conColor, limit, keys, vals);
 }
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st)
 {

@@ -153,6 +153,12 @@ struct pxh_scene_t
 	DBuf<uint32_t> bodyNext;
 	int patchCur = 0;
 	int nPrevRows = 0;
+	// capacities and tables sized on the host so a step needs few host round trips
+	uint64_t* sortedKeys = nullptr;       // broadphase keys after the sort
+	size_t pairCap = 0;                   // pair buffer capacity (grown when exceeded)
+	uint32_t totalRows = 0;               // broadphase row table size (rowOff on the device)
+	int maxPlanes = 0;                    // most planes in one world
+	DBuf<uint32_t> colCounts;             // colouring compaction offsets
 	uint32_t* hostCounts = nullptr;       // pinned
 
 	// optional per-stage GPU timing with HIP events on the scene stream
@@ -210,6 +216,8 @@ struct pxh_broadphase_t
 	DBuf<uint32_t> rows, rowOff, rowStart, vals, vals2, hist, scanTmp, counts, flags, flags2, gridAcc;
 	DBuf<uint64_t> keys, keys2, pkPrev, pkCur, pkTmp;
 	DBuf<uint2> pairs, created, deleted;
+	uint64_t* sortedKeys = nullptr;
+	size_t pairCap = 0;
 	int nPrev = 0, nCur = 0, nCreated = 0, nDeleted = 0;
 	uint32_t* hostCounts = nullptr;
 };
@@ -238,12 +246,32 @@ int rebuild_topology(pxh_scene_t* s)
 		PXH_CHECK(hipMemcpy(s->qBody.p, qb.data(), qb.size() * sizeof(int), hipMemcpyHostToDevice));
 		PXH_CHECK(hipMemcpy(s->qPlane.p, qp.data(), qp.size() * sizeof(int), hipMemcpyHostToDevice));
 	}
+	// broadphase rows: world w gets at most 2 * (bodies in w) rows
+	const int nW = int(s->hBodyStart.size()) - 1;
+	std::vector<uint32_t> ro(size_t(nW) + 1, 0u);
+	for (int w = 0; w < nW; w++)
+	{
+		const uint32_t nb = uint32_t(s->hBodyStart[w + 1] - s->hBodyStart[w]);
+		ro[w + 1] = ro[w] + std::min<uint32_t>(std::max<uint32_t>(2u * nb, 1u), 1u << 20);
+	}
+	s->totalRows = ro[nW];
+	PXH_CHECK(s->rowOff.ensure(ro.size()));
+	PXH_CHECK(hipMemcpy(s->rowOff.p, ro.data(), ro.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+	s->maxPlanes = 0;
+	for (const auto& wp : s->worldPlanes)
+		s->maxPlanes = std::max(s->maxPlanes, int(wp.size()));
 	PXH_CHECK(s->bodyStart.ensure(s->hBodyStart.size()));
 	PXH_CHECK(hipMemcpy(s->bodyStart.p, s->hBodyStart.data(), s->hBodyStart.size() * sizeof(int),
 	                    hipMemcpyHostToDevice));
 	s->topologyDirty = false;
 	return PXH_OK;
 }
+
+// pinned host staging of a scene: colour ranges in [0, kHostColorSlots), the
+// step's device counters (u64) at kHostCounterSlot
+constexpr size_t kHostCountsBytes = 65536;
+constexpr int kHostColorSlots = 15000;
+constexpr int kHostCounterSlot = 15000;
 
 int readCounts(hipStream_t st, uint32_t* host, const uint32_t* const* src, int k)
 {
@@ -261,73 +289,59 @@ int bitsFor(uint64_t v)
 	return b;
 }
 
-// Broadphase core shared by the scene and the standalone API: from inflated
-// bounds (bmin/bmax, bmin.w = geometry type) of n boxes in nWorlds worlds,
-// produce the class-ordered pair list. Returns total pairs in *outP and the
-// class boundaries in cls[0..3].
+// the write pass of the sweep (again after the pair buffer has grown)
 template <typename Ctx>
-int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_group, const int* d_bodyStart,
-               hipStream_t st, uint32_t* cls, uint32_t* hostCounts)
+int bp_write_pairs(Ctx* s, int n, const int* d_scene, const int* d_group, const uint32_t* d_rowOff, hipStream_t st,
+                   size_t pairCap)
+{
+	PXH_CHECK(s->pairs.ensure(std::max<size_t>(pairCap, 1)));
+	pxh::launch_sweep(true, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, d_rowOff,
+	                  s->rowStart.p, nullptr, s->counts.p, s->pairs.p, uint32_t(pairCap), st);
+	return PXH_OK;
+}
+
+// Broadphase core shared by the scene and the standalone API: from inflated
+// bounds (bmin/bmax, bmin.w = geometry type) of n boxes in nWorlds worlds
+// whose row table d_rowOff (nWorlds + 1 entries, totalRows in all) is already
+// on the device, produce the class-ordered pair list without a host round
+// trip: the pair count stays on the device at counts[3n] (class ends at
+// counts[n], counts[2n], counts[3n]) and at most pairCap pairs are written.
+template <typename Ctx>
+int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_group, const uint32_t* d_rowOff,
+               uint32_t totalRows, hipStream_t st, size_t pairCap)
 {
 	PXH_CHECK(s->gridF.ensure(nWorlds));
 	PXH_CHECK(s->gridI.ensure(nWorlds));
-	PXH_CHECK(s->rows.ensure(nWorlds + 1));
-	PXH_CHECK(s->rowOff.ensure(nWorlds + 1));
 	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(3) * n + 1, 256 * 1024)) + 1024));
 	PXH_CHECK(s->gridAcc.ensure(size_t(5) * nWorlds));
-	(void)d_bodyStart;
-	pxh::launch_scene_grid(n, nWorlds, d_scene, s->bmin.p, s->bmax.p, s->gridAcc.p, s->gridF.p, s->gridI.p, s->rows.p,
+	pxh::launch_scene_grid(n, nWorlds, d_scene, s->bmin.p, s->bmax.p, s->gridAcc.p, s->gridF.p, s->gridI.p, d_rowOff,
 	                       st);
-	PXH_CHECK(hipMemsetAsync(s->rows.p + nWorlds, 0, sizeof(uint32_t), st));
-	PXH_CHECK(hipMemcpyAsync(s->rowOff.p, s->rows.p, sizeof(uint32_t) * (nWorlds + 1), hipMemcpyDeviceToDevice, st));
-	pxh::launch_scan(s->rowOff.p, nWorlds + 1, s->scanTmp.p, st);
-	{
-		const uint32_t* src[1] = {s->rowOff.p + nWorlds};
-		int r = readCounts(st, hostCounts, src, 1);
-		if (r)
-			return r;
-	}
-	const uint32_t totalRows = hostCounts[0];
-
 	PXH_CHECK(s->keys.ensure(n));
 	PXH_CHECK(s->keys2.ensure(n));
 	PXH_CHECK(s->vals.ensure(n));
 	PXH_CHECK(s->vals2.ensure(n));
-	pxh::launch_keys(n, s->bmin.p, s->bmax.p, d_scene, s->gridF.p, s->gridI.p, s->rowOff.p, s->keys.p, s->vals.p, st);
+	pxh::launch_keys(n, s->bmin.p, s->bmax.p, d_scene, s->gridF.p, s->gridI.p, d_rowOff, s->keys.p, s->vals.p, st);
 	const int nblocks = (n + 2047) / 2048;
 	PXH_CHECK(s->hist.ensure(size_t(256) * std::max(nblocks, 1)));
 	bool alt = false;
 	const int bitEnd = 32 + ((bitsFor(totalRows) + 7) / 8) * 8;
 	pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, n, 0, bitEnd, s->hist.p, s->scanTmp.p, st, alt);
-	uint64_t* sk = alt ? s->keys2.p : s->keys.p;
-	uint32_t* sv = alt ? s->vals2.p : s->vals.p;
+	s->sortedKeys = alt ? s->keys2.p : s->keys.p;
+	const uint32_t* sv = alt ? s->vals2.p : s->vals.p;
 
 	PXH_CHECK(s->smin.ensure(n));
 	PXH_CHECK(s->smax.ensure(n));
 	PXH_CHECK(s->rowStart.ensure(size_t(totalRows) + 2));
 	pxh::launch_gather_sorted(n, sv, s->bmin.p, s->bmax.p, s->smin.p, s->smax.p, st);
-	pxh::launch_row_start(n, sk, s->rowStart.p, totalRows, st);
+	pxh::launch_row_start(n, s->sortedKeys, s->rowStart.p, totalRows, st);
 
 	const int64_t nc = int64_t(3) * n + 1;
 	PXH_CHECK(s->counts.ensure(nc));
 	PXH_CHECK(hipMemsetAsync(s->counts.p + 3 * int64_t(n), 0, sizeof(uint32_t), st));
-	pxh::launch_sweep(false, n, sk, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, s->rowOff.p, s->rowStart.p,
-	                  s->counts.p, nullptr, nullptr, st);
+	pxh::launch_sweep(false, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, d_rowOff,
+	                  s->rowStart.p, s->counts.p, nullptr, nullptr, 0u, st);
 	pxh::launch_scan(s->counts.p, nc, s->scanTmp.p, st);
-	{
-		const uint32_t* src[3] = {s->counts.p + n, s->counts.p + 2 * int64_t(n), s->counts.p + 3 * int64_t(n)};
-		int r = readCounts(st, hostCounts, src, 3);
-		if (r)
-			return r;
-	}
-	cls[0] = 0;
-	cls[1] = hostCounts[0];
-	cls[2] = hostCounts[1];
-	cls[3] = hostCounts[2];
-	PXH_CHECK(s->pairs.ensure(std::max<size_t>(cls[3], 1)));
-	pxh::launch_sweep(true, n, sk, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, s->rowOff.p, s->rowStart.p,
-	                  nullptr, s->counts.p, s->pairs.p, st);
-	return PXH_OK;
+	return bp_write_pairs(s, n, d_scene, d_group, d_rowOff, st, pairCap);
 }
 
 }  // namespace
@@ -378,7 +392,7 @@ int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
 	pxh_scene_t* s = new pxh_scene_t();
 	s->desc = *desc;
 	if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-	    hipHostMalloc(reinterpret_cast<void**>(&s->hostCounts), 16384, hipHostMallocDefault) != hipSuccess)
+	    hipHostMalloc(reinterpret_cast<void**>(&s->hostCounts), kHostCountsBytes, hipHostMallocDefault) != hipSuccess)
 	{
 		delete s;
 		return fail(PXH_ERR_DEVICE, "pxh_scene_create: stream or pinned buffer creation failed");
@@ -568,55 +582,77 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		return PXH_OK;
 	const float co = d.contact_offset, cd = 2.0f * co;
 
-	// 1. bounds + broadphase
+	// 1. bounds + broadphase; the pair count stays on the device (dP)
 	PXH_CHECK(s->bmin.ensure(n));
 	PXH_CHECK(s->bmax.ensure(n));
 	int pk = s->profBegin(0);
 	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, co, s->bmin.p, s->bmax.p, st);
 	s->profEnd(pk);
-	uint32_t cls[4];
-	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->bodyStart.p, st, cls, s->hostCounts);
+	s->pairCap = std::max(s->pairCap, size_t(4) * n);
+	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->rowOff.p, s->totalRows, st, s->pairCap);
 	if (r)
 		return r;
-	const int P = int(cls[3]);
+	const uint32_t* dP = s->counts.p + 3 * int64_t(n);
 	const int Q = s->Q;
-	const int S = P + Q;
-	s->statPairs = P;
 	s->statPlanePairs = Q;
 
-	// 2. narrowphase into per-slot contact sets (slots: body pairs, then body-plane pairs)
-	PXH_CHECK(s->ctN.ensure(std::max(S, 1)));
-	PXH_CHECK(s->ctP.ensure(size_t(4) * std::max(S, 1)));
-	pk = s->profBegin(2);
-	pxh::launch_np_pairs(P, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length, s->ctN.p, s->ctP.p, st);
-	s->profEnd(pk);
-	pk = s->profBegin(3);
-	pxh::launch_np_planes(Q, P, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p, cd, s->ctN.p,
-	                      s->ctP.p, st);
-	s->profEnd(pk);
-
-	// 3. constraints = touching slots
-	PXH_CHECK(s->flags.ensure(size_t(S) + 1));
-	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
-	pxh::launch_touch_flags(S, s->ctN.p, s->flags.p, st);
-	PXH_CHECK(hipMemsetAsync(s->flags.p + S, 0, sizeof(uint32_t), st));
-	pxh::launch_scan(s->flags.p, int64_t(S) + 1, s->scanTmp.p, st);
+	// 2. narrowphase into per-slot contact sets (slots: body-plane pairs, then
+	//    body pairs up to the count on the device) and 3. the constraint list
+	//    (touching slots in slot order). One host round trip reads the pair
+	//    and constraint counts; when the pairs outgrew their buffer it is
+	//    enlarged and the stage repeated.
+	int P = 0, C = 0;
+	for (int attempt = 0;; attempt++)
 	{
-		const uint32_t* src[1] = {s->flags.p + S};
-		r = readCounts(st, s->hostCounts, src, 1);
+		const int cap = int(s->pairCap);
+		const int S = Q + cap;
+		if (attempt > 0)
+		{
+			r = bp_write_pairs(s, n, s->scene.p, nullptr, s->rowOff.p, st, s->pairCap);
+			if (r)
+				return r;
+		}
+		PXH_CHECK(s->ctN.ensure(size_t(S)));
+		PXH_CHECK(s->ctP.ensure(size_t(4) * S));
+		pk = s->profBegin(3);
+		pxh::launch_np_planes(Q, 0, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p, cd, s->ctN.p,
+		                      s->ctP.p, st);
+		s->profEnd(pk);
+		pk = s->profBegin(2);
+		pxh::launch_np_pairs(cap, dP, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length, s->ctN.p,
+		                     s->ctP.p, st);
+		s->profEnd(pk);
+		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
+		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
+		pxh::launch_touch_flags(S, Q, dP, s->ctN.p, s->flags.p, st);
+		PXH_CHECK(hipMemsetAsync(s->flags.p + S, 0, sizeof(uint32_t), st));
+		pxh::launch_scan(s->flags.p, int64_t(S) + 1, s->scanTmp.p, st);
+		const size_t Sc = std::max(S, 1);
+		PXH_CHECK(s->conA.ensure(Sc));
+		PXH_CHECK(s->conB.ensure(Sc));
+		PXH_CHECK(s->conSlot.ensure(Sc));
+		PXH_CHECK(s->conColor.ensure(Sc));
+		PXH_CHECK(s->conKey.ensure(Sc));
+		pxh::launch_con_build(S, Q, dP, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
+		                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
+		const uint32_t* src[2] = {dP, s->flags.p + S};
+		r = readCounts(st, s->hostCounts, src, 2);
 		if (r)
 			return r;
+		P = int(s->hostCounts[0]);
+		C = int(s->hostCounts[1]);
+		if (P <= cap)
+			break;
+		if (attempt > 0)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: pair buffer overflow after growth");
+		s->pairCap = size_t(P) + size_t(P) / 4;
 	}
-	const int C = int(s->hostCounts[0]);
+	// grow ahead of need so the next steps keep to the single round trip
+	if (size_t(P) + size_t(P) / 8 > s->pairCap)
+		s->pairCap = size_t(P) + size_t(P) / 4;
+	s->statPairs = P;
 	s->statConstraints = C;
-	PXH_CHECK(s->conA.ensure(std::max(C, 1)));
-	PXH_CHECK(s->conB.ensure(std::max(C, 1)));
-	PXH_CHECK(s->conSlot.ensure(std::max(C, 1)));
-	PXH_CHECK(s->conColor.ensure(std::max(C, 1)));
-	PXH_CHECK(s->conKey.ensure(std::max(C, 1)));
 	PXH_CHECK(s->order.ensure(std::max(C, 1)));
-	pxh::launch_con_build(S, P, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
-	                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
 
 	// 4. previous step's rows: a pair-key table over them gives each
 	//    constraint its previous row (friction patch) and, for body-body
@@ -648,74 +684,84 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	}
 
 	// 5. colouring of the remaining body-body constraints: rounds of local key
-	//    minima, each taking its first free colour; the uncoloured constraints
-	//    are compacted before the rounds and between batches of rounds
-	PXH_CHECK(s->scratch.ensure(4));
-	PXH_CHECK(s->colList.ensure(std::max(C, 1)));
-	PXH_CHECK(s->colList2.ensure(std::max(C, 1)));
-	PXH_CHECK(s->colFlags.ensure(size_t(C) + 1));
-	PXH_CHECK(s->counts.ensure(size_t(C) + 1));
-	PXH_CHECK(hipMemsetAsync(s->scratch.p, 0, 2 * sizeof(uint32_t), st));
+	//    minima, each taking its first free colour, over a compacted list of the
+	//    uncoloured constraints whose count stays on the device; then the
+	//    body-plane colours and the colour order. One host round trip reads the
+	//    colour ranges and what is left uncoloured (a first step, with nothing
+	//    carried over, may need further rounds).
+	PXH_CHECK(s->colList.ensure(Cc));
+	PXH_CHECK(s->colList2.ensure(Cc));
+	PXH_CHECK(s->colFlags.ensure(Cc + 1));
+	PXH_CHECK(s->colCounts.ensure(Cc + 1));
+	PXH_CHECK(s->bodyNext.ensure(n));
+	PXH_CHECK(s->keys.ensure(Cc));
+	PXH_CHECK(s->keys2.ensure(Cc));
+	PXH_CHECK(s->vals.ensure(Cc));
+	PXH_CHECK(s->vals2.ensure(Cc));
+	{
+		const int nb = (C + 2047) / 2048;
+		PXH_CHECK(s->hist.ensure(size_t(256) * std::max(nb, 1)));
+		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(256) * std::max(nb, 1), C + 1)) +
+		                            1024));
+	}
 	int rounds = 0;
 	const int kMaxRounds = 960;
-	int remaining = C;
-	const int* list = nullptr;
-	int batch = 0;
-	while (remaining > 0)
-	{
-		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
-			pxh::launch_color_round(remaining, list, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p,
-			                        s->best.p, s->mask.p, st);
-		batch = list ? 4 : 8;
-		// compact the uncoloured constraints
-		pxh::launch_color_uncolored(remaining, list, s->conColor.p, s->colFlags.p, st);
-		PXH_CHECK(hipMemsetAsync(s->colFlags.p + remaining, 0, sizeof(uint32_t), st));
-		PXH_CHECK(hipMemcpyAsync(s->counts.p, s->colFlags.p, sizeof(uint32_t) * (remaining + 1),
-		                         hipMemcpyDeviceToDevice, st));
-		pxh::launch_scan(s->counts.p, int64_t(remaining) + 1, s->scanTmp.p, st);
+	const int* list = nullptr;         // null: every constraint
+	const uint32_t* mDev = nullptr;    // null: count C
+	int bound = C;                     // host bound on the list length
+	if (64 + kMaxRounds + s->maxPlanes + 2 > kHostColorSlots)
+		return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: too many planes in one world");
+	auto compact = [&]() {
 		int* out = list == s->colList.p ? s->colList2.p : s->colList.p;
-		pxh::launch_color_compact(remaining, list, s->colFlags.p, s->counts.p, out, st);
-		const uint32_t* src[1] = {s->counts.p + remaining};
-		r = readCounts(st, s->hostCounts, src, 1);
-		if (r)
-			return r;
-		remaining = int(s->hostCounts[0]);
+		pxh::launch_color_uncolored(bound, mDev, list, s->conColor.p, s->colFlags.p, st);
+		(void)hipMemcpyAsync(s->colCounts.p, s->colFlags.p, sizeof(uint32_t) * (size_t(bound) + 1),
+		                     hipMemcpyDeviceToDevice, st);
+		pxh::launch_scan(s->colCounts.p, int64_t(bound) + 1, s->scanTmp.p, st);
+		pxh::launch_color_compact(bound, list, s->colFlags.p, s->colCounts.p, out, st);
 		list = out;
-		if (rounds >= kMaxRounds && remaining > 0)
-			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring did not finish in 960 rounds");
-	}
+		mDev = s->colCounts.p + bound;
+	};
 	int nColors = 0;
 	std::vector<uint32_t> colorStart(1, 0);
-	if (C > 0)
+	int remaining = C;
+	while (C > 0)
 	{
-		PXH_CHECK(s->bodyNext.ensure(n));
+		compact();
+		const int batch = s->nPrevRows > 0 && rounds == 0 ? 8 : 48;
+		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
+			pxh::launch_color_round(bound, mDev, list, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p,
+			                        s->best.p, s->mask.p, st);
+		compact();
 		pxh::launch_static_colors(n, C, s->conA.p, s->conB.p, s->conColor.p, s->bodyNext.p, st);
-		pxh::launch_max_color(C, s->conColor.p, s->scratch.p + 1, st);
-		const uint32_t* src[1] = {s->scratch.p + 1};
-		r = readCounts(st, s->hostCounts, src, 1);
-		if (r)
-			return r;
-		nColors = int(s->hostCounts[0]) + 1;
-		PXH_CHECK(s->keys.ensure(C));
-		PXH_CHECK(s->keys2.ensure(C));
-		PXH_CHECK(s->vals.ensure(C));
-		PXH_CHECK(s->vals2.ensure(C));
-		const int nb = (C + 2047) / 2048;
-		PXH_CHECK(s->hist.ensure(size_t(256) * nb));
-		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(256) * nb) + 1024));
-		PXH_CHECK(s->colorCount.ensure(size_t(nColors) + 2));
-		pxh::launch_color_keys(C, s->conColor.p, s->keys.p, s->vals.p, st);
+		// colour order: stable radix sort of (colour, index); colours stay below
+		// 64 + rounds + planes per world
+		const int colorBound = 64 + rounds + s->maxPlanes + 1;
+		PXH_CHECK(s->colorCount.ensure(size_t(colorBound) + 2));
+		pxh::launch_color_keys(C, s->conColor.p, uint32_t(colorBound), s->keys.p, s->vals.p, st);
 		bool alt = false;
-		pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, C, 32, 32 + ((bitsFor(nColors) + 7) / 8) * 8,
-		                       s->hist.p, s->scanTmp.p, st, alt);
+		pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, C, 32,
+		                       32 + ((bitsFor(uint64_t(colorBound)) + 7) / 8) * 8, s->hist.p, s->scanTmp.p, st, alt);
 		const uint64_t* ck = alt ? s->keys2.p : s->keys.p;
 		const uint32_t* cv = alt ? s->vals2.p : s->vals.p;
-		pxh::launch_row_start(C, ck, s->colorCount.p, uint32_t(nColors), st);
+		pxh::launch_row_start(C, ck, s->colorCount.p, uint32_t(colorBound), st);
 		PXH_CHECK(hipMemcpyAsync(s->order.p, cv, sizeof(int) * C, hipMemcpyDeviceToDevice, st));
-		PXH_CHECK(hipMemcpyAsync(s->hostCounts, s->colorCount.p, sizeof(uint32_t) * (nColors + 1),
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts, s->colorCount.p, sizeof(uint32_t) * (size_t(colorBound) + 1),
 		                         hipMemcpyDeviceToHost, st));
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts + colorBound + 1, mDev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
 		PXH_CHECK(hipStreamSynchronize(st));
-		colorStart.assign(s->hostCounts, s->hostCounts + nColors + 1);
+		remaining = int(s->hostCounts[colorBound + 1]);
+		if (remaining == 0)
+		{
+			colorStart.assign(s->hostCounts, s->hostCounts + colorBound + 1);
+			nColors = colorBound;
+			while (nColors > 0 && colorStart[nColors] == colorStart[nColors - 1])
+				nColors--;
+			colorStart.resize(size_t(nColors) + 1);
+			break;
+		}
+		if (rounds >= kMaxRounds)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring did not finish in 960 rounds");
+		bound = remaining;
 	}
 	s->statColors = nColors;
 	s->statRounds = rounds;
@@ -752,7 +798,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
 	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, pio, s->counters.p,
 	                 st);
-	PXH_CHECK(hipMemcpyAsync(s->hostCounts + 4000, s->counters.p, 4 * sizeof(unsigned long long),
+	PXH_CHECK(hipMemcpyAsync(s->hostCounts + kHostCounterSlot, s->counters.p, 4 * sizeof(unsigned long long),
 	                         hipMemcpyDeviceToHost, st));
 	s->profEnd(pk);
 
@@ -810,7 +856,7 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 	if (s->statConstraints > 0)
 	{
 		unsigned long long c3[4];
-		std::memcpy(c3, s->hostCounts + 4000, sizeof(c3));
+		std::memcpy(c3, s->hostCounts + kHostCounterSlot, sizeof(c3));
 		s->statContacts = int64_t(c3[0]);
 		s->statFriction = int64_t(c3[1]);
 		s->statDynCon = int64_t(c3[2]);
@@ -992,11 +1038,26 @@ int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* gr
 	int P = 0;
 	if (n > 0)
 	{
-		uint32_t cls[4];
-		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->bodyStart.p, st, cls, bp->hostCounts);
+		// one world of at most 2n rows
+		const uint32_t rows = std::min<uint32_t>(2u * uint32_t(n), 1u << 20);
+		PXH_CHECK(bp->rowOff.ensure(2));
+		pxh::launch_row_off_single(bp->rowOff.p, rows, st);
+		bp->pairCap = std::max(bp->pairCap, size_t(4) * n);
+		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->rowOff.p, rows, st, bp->pairCap);
 		if (r)
 			return r;
-		P = int(cls[3]);
+		const uint32_t* src[1] = {bp->counts.p + 3 * int64_t(n)};
+		r = readCounts(st, bp->hostCounts, src, 1);
+		if (r)
+			return r;
+		P = int(bp->hostCounts[0]);
+		if (size_t(P) > bp->pairCap)
+		{
+			bp->pairCap = size_t(P) + size_t(P) / 4;
+			r = bp_write_pairs(bp, n, bp->scene.p, bp->group.p, bp->rowOff.p, st, bp->pairCap);
+			if (r)
+				return r;
+		}
 	}
 	// sorted pair keys of this update
 	std::swap(bp->pkPrev, bp->pkCur);

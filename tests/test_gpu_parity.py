@@ -237,3 +237,45 @@ def test_long_run_statistics_match_reference(cuda):
     a = PS.run_stats(P.Scene(sc, position_iterations=4, velocity_iterations=1), sc, 300)
     b = PS.run_stats(L.RefScene(sc, "tgs", 4, 1, threads=8), sc, 300)
     PS.assert_stats_close(a, b)
+
+
+def test_colouring_beyond_first_batch_of_rounds(cuda):
+    """Bodies first rest apart on their floors (only body-plane constraints),
+    then are moved into their piles: every body-body constraint of the second
+    step is new, more than the first batch of colouring rounds can place, so
+    the step takes the further-rounds path. Colours and velocities match the
+    oracle."""
+    sc = S.batched_piles(n_scenes=4, nmin=64, nmax=256, seed=24)
+    local = np.arange(sc.n) - np.searchsorted(sc.scene_id, sc.scene_id, side="left")
+    apart = sc.pose.copy()
+    apart[:, 0] = (local % 16) * 1.5
+    apart[:, 2] = (local // 16) * 1.5
+    apart[:, 1] = np.where(sc.shape_type == S.SPHERE, sc.dims[:, 0], sc.dims[:, 1]) + 0.005
+    apart[:, 3:] = [0, 0, 0, 1]
+    # then a tight cluster where every body touches many others
+    cluster = sc.pose.copy()
+    cluster[:, 0] *= 0.4
+    cluster[:, 2] *= 0.4
+    cluster[:, 1] = cluster[:, 1] * 0.4 + 0.35
+    zero = np.zeros((sc.n, 3), np.float32)
+    g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
+    o = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
+    g.set_bodies(apart, zero, zero)
+    o.set_bodies(apart, zero, zero)
+    g.step(1.0 / 60.0)
+    o.step(1.0 / 60.0)
+    assert g.stats()["body_body_constraints"] == 0 < g.stats()["constraints"]
+    g.set_bodies(cluster, zero, zero)
+    o.set_bodies(cluster, zero, zero)
+    g.step(1.0 / 60.0)
+    o.step(1.0 / 60.0)
+    st = g.stats()
+    assert st["color_rounds"] > 8, st
+    key = lambda c: set(map(tuple, c.tolist()))  # noqa: E731
+    assert key(g.constraints()) == key(o.constraints())
+    _, gl, _ = g.bodies()
+    _, ol, _ = o.bodies()
+    # deep overlaps push bodies apart at tens of m/s; last-bit differences of
+    # sin/cos spread through the cluster at that scale: within 1e-5 of the
+    # largest speed
+    assert np.abs(gl - ol).max() <= 1e-5 * np.linalg.norm(ol, axis=1).max()
