@@ -1051,6 +1051,9 @@ typedef struct {
 	v3 fn[4], fraXnI[4], frbXnI[4];
 	float ferr[4], fvelMul[4], fbiasScale, fscale, fforce[4];
 	float maxPenBias;
+	/* PGS rows: normal biased/unbiased error, friction bias and target
+	 * velocity, friction coefficients scaled for the anchor count */
+	float biasedErr[4], unbiasedErr[4], fbias[4], ftarget[4], muS, muD;
 } pxo_con;
 
 typedef struct {
@@ -1264,7 +1267,7 @@ static void push_con(pxo_world* w, int a, int b, const cset* c)
 	k->c = *c;
 	const uint32_t la = (uint32_t)w->local[a];
 	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b] : (0x80000000u | (uint32_t)w->plocal[-1 - b]);
-	k->key = con_key(la, lb);
+	k->key = con_key(la, lb) ^ ((uint64_t)w->d.order_salt * 0x9E3779B97F4A7C15ull);
 	k->pkey = ((uint64_t)(uint32_t)a << 32) | (b >= 0 ? (uint32_t)b : (0x80000000u | (uint32_t)(-1 - b)));
 }
 
@@ -1322,52 +1325,15 @@ static void integrate_step(sbody* b, float dt)
 	b->dlin = add(b->dlin, delta);
 }
 
-/* setupFinalizeSolverConstraints / constructContactConstraintStep for one
- * rigid-rigid (or rigid-static) contact patch:
- * lowleveldynamics/src/DyTGSContactPrep.cpp:426-812 (rows), :310-404 (normal row),
- * friction patch creation and anchors DyFrictionCorrelation.cpp:55-292 */
-static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbody* world, const xf* frame,
-                     float invStepDt, float stepDt, float totalDt, float scale)
+/* Friction patch of a constraint; returns its anchor count. The previous
+ * step's patch of this pair is kept when it is unbroken, has anchors, its body
+ * normals still agree, no anchor has drifted apart along the normal
+ * (getFrictionPatches, isSeparated: DyContactPrepShared.h:53-127) and its world
+ * normal matches this step's contact normal (correlatePatches,
+ * DyFrictionCorrelation.cpp:136-184); otherwise a new patch starts from the
+ * contacts. Shared by the TGS and PGS preparation. */
+static int friction_patch(const pxo_world* w, pxo_con* k, xf f0, xf f1, v3 n, float restDist)
 {
-	const sbody* b0 = &sb[k->a];
-	const sbody* b1 = k->b >= 0 ? &sb[k->b] : world;
-	const xf f0 = frame[k->a];
-	xf f1;
-	if (k->b >= 0) f1 = frame[k->b];
-	else { f1.p = V(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1; }
-	const float invDtp8 = invStepDt * scale;
-	const v3 n = k->c.n;
-	const float nlen2 = dot(n, n);
-	k->invMassA = b0->invMass;
-	k->invMassB = b1->invMass;
-	k->maxPenBias = -1e32f; /* max(penBiasClamp0, penBiasClamp1), defaults -1e32 (PxvDynamics.h:140) */
-	const float imn0 = b0->invMass * nlen2, imn1 = -b1->invMass * nlen2;
-	const float restDist = 2.0f * w->d.rest_offset;
-	(void)totalDt;
-	for (int i = 0; i < k->c.count; i++)
-	{
-		const v3 ra = sub(k->c.p[i], f0.p), rb = sub(k->c.p[i], f1.p);
-		const v3 raXn = cross(ra, n), rbXn = cross(rb, n);
-		const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
-		const float i0 = dot(raXnI, raXnI), i1 = dot(rbXnI, rbXnI);
-		const float resp0 = imn0 + i0, resp1 = i1 - imn1;
-		const float unit = resp0 + resp1;
-		const float pen = k->c.sep[i] - restDist;
-		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
-		k->raXnI[i] = raXnI;
-		k->rbXnI[i] = rbXnI;
-		k->sepErr[i] = pen;
-		k->biasCoef[i] = -(pen > 0.0f ? invStepDt : invDtp8);
-		k->velMul[i] = recip;
-		k->recipResp[i] = recip;
-		k->force[i] = 0.0f;
-	}
-	/* friction patch. The previous step's patch of this pair is kept when it
-	 * is unbroken, has anchors, its body normals still agree, no anchor has
-	 * drifted apart along the normal (getFrictionPatches, isSeparated:
-	 * DyContactPrepShared.h:53-127) and its world normal matches this step's
-	 * contact normal (correlatePatches, DyFrictionCorrelation.cpp:136-184);
-	 * otherwise a new patch starts from the contacts. */
 	fpatch* fp = &k->fp;
 	const fpatch* pp = find_patch(w, k->pkey);
 	int kept = 0;
@@ -1448,7 +1414,51 @@ static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbod
 			fp->a0[0] = fp->a1[0] = V(0.0f, 0.0f, 0.0f);
 		fp->count = nAnch;
 	}
-	const int nAnch = fp->count;
+	return fp->count;
+}
+
+/* setupFinalizeSolverConstraints / constructContactConstraintStep for one
+ * rigid-rigid (or rigid-static) contact patch:
+ * lowleveldynamics/src/DyTGSContactPrep.cpp:426-812 (rows), :310-404 (normal row),
+ * friction patch creation and anchors DyFrictionCorrelation.cpp:55-292 */
+static void prep_con(const pxo_world* w, pxo_con* k, const sbody* sb, const sbody* world, const xf* frame,
+                     float invStepDt, float stepDt, float totalDt, float scale)
+{
+	const sbody* b0 = &sb[k->a];
+	const sbody* b1 = k->b >= 0 ? &sb[k->b] : world;
+	const xf f0 = frame[k->a];
+	xf f1;
+	if (k->b >= 0) f1 = frame[k->b];
+	else { f1.p = V(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1; }
+	const float invDtp8 = invStepDt * scale;
+	const v3 n = k->c.n;
+	const float nlen2 = dot(n, n);
+	k->invMassA = b0->invMass;
+	k->invMassB = b1->invMass;
+	k->maxPenBias = -1e32f; /* max(penBiasClamp0, penBiasClamp1), defaults -1e32 (PxvDynamics.h:140) */
+	const float imn0 = b0->invMass * nlen2, imn1 = -b1->invMass * nlen2;
+	const float restDist = 2.0f * w->d.rest_offset;
+	(void)totalDt;
+	for (int i = 0; i < k->c.count; i++)
+	{
+		const v3 ra = sub(k->c.p[i], f0.p), rb = sub(k->c.p[i], f1.p);
+		const v3 raXn = cross(ra, n), rbXn = cross(rb, n);
+		const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
+		const float i0 = dot(raXnI, raXnI), i1 = dot(rbXnI, rbXnI);
+		const float resp0 = imn0 + i0, resp1 = i1 - imn1;
+		const float unit = resp0 + resp1;
+		const float pen = k->c.sep[i] - restDist;
+		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+		k->raXnI[i] = raXnI;
+		k->rbXnI[i] = rbXnI;
+		k->sepErr[i] = pen;
+		k->biasCoef[i] = -(pen > 0.0f ? invStepDt : invDtp8);
+		k->velMul[i] = recip;
+		k->recipResp[i] = recip;
+		k->force[i] = 0.0f;
+	}
+	const int nAnch = friction_patch(w, k, f0, f1, n, restDist);
+	const fpatch* fp = &k->fp;
 	k->nFric = 2 * nAnch;
 	if (nAnch == 0)
 		return;
@@ -1562,11 +1572,164 @@ static void solve_con(pxo_con* k, sbody* b0, sbody* b1, float minPen, float elap
 	if (b1->invMass > 0.0f) { b1->lin = l1; b1->ang = a1; }
 }
 
+/* PGS contact rows (setupFinalizeSolverConstraints, lowleveldynamics/src/
+ * DyContactPrep.cpp:49-357; normal rows constructContactConstraint,
+ * DyContactPrepShared.h:298-393). The solver works on velocity changes: the
+ * bodies' pre-integrated velocities (origLin, origAng world) enter as the
+ * target velocity -vrel of every row. */
+static void prep_con_pgs(const pxo_world* w, pxo_con* k, const sbody* sb, const sbody* world, const xf* frame,
+                         float invDt, float mu_s, float mu_d)
+{
+	const sbody* b0 = &sb[k->a];
+	const sbody* b1 = k->b >= 0 ? &sb[k->b] : world;
+	const xf f0 = frame[k->a];
+	xf f1;
+	if (k->b >= 0) f1 = frame[k->b];
+	else { f1.p = V(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1; }
+	const v3 n = k->c.n;
+	const float nl2 = dot(n, n);
+	const float invMass0 = b0->invMass, invMass1 = b1->invMass;
+	const float imn0 = invMass0 * nl2, imn1 = -invMass1 * nl2;
+	const v3 v0 = b0->origLin, v1 = b1->origLin, w0 = b0->origAng, w1 = b1->origAng;
+	const float restDist = 2.0f * w->d.rest_offset;
+	const float invDtp8 = invDt * 0.8f;
+	const float maxPenBias = -1e32f; /* max(penBiasClamp0, penBiasClamp1), DyDynamics.cpp:144 */
+	k->invMassA = invMass0;
+	k->invMassB = invMass1;
+	const float norVel = (n.x * v0.x - n.x * v1.x) + (n.y * v0.y - n.y * v1.y) + (n.z * v0.z - n.z * v1.z);
+	for (int i = 0; i < k->c.count; i++)
+	{
+		const v3 ra = sub(k->c.p[i], f0.p), rb = sub(k->c.p[i], f1.p);
+		const v3 raXn = cross(ra, n), rbXn = cross(rb, n);
+		const float vrel = norVel + (dot(raXn, w0) - dot(rbXn, w1));
+		const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
+		const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
+		const float pen = k->c.sep[i] - restDist;
+		const float target = 0.0f - vrel;
+		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+		const float penScaled = pen >= 0.0f ? pen * invDt : pen * invDtp8;
+		const float scaledBias = recip * fmaxf(maxPenBias, penScaled);
+		k->raXnI[i] = raXnI;
+		k->rbXnI[i] = rbXnI;
+		k->velMul[i] = recip;
+		k->biasedErr[i] = target * recip + -scaledBias;
+		k->unbiasedErr[i] = target * recip + -fmaxf(scaledBias, 0.0f);
+		k->force[i] = 0.0f;
+	}
+	const int nAnch = friction_patch(w, k, f0, f1, n, restDist);
+	const fpatch* fp = &k->fp;
+	k->nFric = 2 * nAnch;
+	const float fc = nAnch == 2 ? 0.5f : 1.0f;
+	k->muS = mu_s * fc;
+	k->muD = mu_d * fc;
+	if (nAnch == 0)
+		return;
+	const v3 lvr = sub(v0, v1);
+	const v3 fb1 = V(0.0f, -n.z, n.y), fb2 = V(-n.y, n.x, 0.0f);
+	const v3 fb = (0.70710678f > fabsf(n.x)) ? fb1 : fb2;
+	v3 t0 = sub(lvr, scl(n, dot(n, lvr)));
+	t0 = len2(t0) > 0.0001f ? t0 : fb;
+	t0 = normalize(t0);
+	const v3 t1 = cross(n, t0);
+	for (int j = 0; j < nAnch; j++)
+	{
+		const v3 ra = qrot(f0.q, fp->a0[j]), rb = qrot(f1.q, fp->a1[j]);
+		const v3 err = sub(add(ra, f0.p), add(rb, f1.p));
+		for (int d = 0; d < 2; d++)
+		{
+			const v3 t = d == 0 ? t0 : t1;
+			const v3 raXn = cross(ra, t), rbXn = cross(rb, t);
+			const v3 raXnI = mmul(b0->sqrtInvI, raXn), rbXnI = mmul(b1->sqrtInvI, rbXn);
+			const float resp = (invMass0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - -invMass1);
+			const float vrel = (dot(t, v0) + dot(raXn, w0)) - (dot(t, v1) + dot(rbXn, w1));
+			const int r = 2 * j + d;
+			k->fn[r] = t;
+			k->fraXnI[r] = raXnI;
+			k->frbXnI[r] = rbXnI;
+			k->fvelMul[r] = resp > 0.0f ? 0.8f / resp : 0.0f;
+			k->ftarget[r] = 0.0f - vrel;
+			k->fbias[r] = dot(t, err) * invDt;
+			k->fforce[r] = 0.0f;
+		}
+	}
+}
+
+/* PGS solveContact (lowleveldynamics/src/DySolverConstraints.cpp:221-371,
+ * normal rows solveDynamicContacts, DySolverConstraintsShared.h:49-105) on the
+ * velocity changes l (linear) and a (sqrt-inertia angular) of both bodies;
+ * friction rows are clamped one at a time. conclude: after the last position
+ * pass the rows switch to their unbiased error and friction loses its bias
+ * (concludeContact, DySolverConstraints.cpp:508-549). */
+static void solve_con_pgs(pxo_con* k, sbody* b0, sbody* b1, int doFriction, int conclude)
+{
+	v3 l0 = b0->lin, l1 = b1->lin, a0 = b0->ang, a1 = b1->ang;
+	const v3 n = k->c.n;
+	const v3 dl0 = scl(n, k->invMassA), dl1 = scl(n, k->invMassB);
+	float acc = 0.0f;
+	for (int i = 0; i < k->c.count; i++)
+	{
+		const v3 v0 = add(mulv(l0, n), mulv(a0, k->raXnI[i]));
+		const v3 v1 = add(mulv(l1, n), mulv(a1, k->rbXnI[i]));
+		const v3 dv = sub(v0, v1);
+		const float nv = dv.x + dv.y + dv.z;
+		const float af = k->force[i];
+		const float d0 = fmaxf(k->biasedErr[i] - nv * k->velMul[i], -af);
+		const float nf = fminf(af + d0, 1e32f);
+		const float df = nf - af;
+		l0 = add(scl(dl0, df), l0);
+		l1 = sub(l1, scl(dl1, df));
+		a0 = add(scl(k->raXnI[i], df), a0);
+		a1 = sub(a1, scl(k->rbXnI[i], df));
+		k->force[i] = nf;
+		acc = acc + nf;
+	}
+	if (doFriction && k->nFric)
+	{
+		const float maxF = k->muS * acc, maxD = k->muD * acc;
+		int broken = 0;
+		for (int r = 0; r < k->nFric; r++)
+		{
+			const v3 t = k->fn[r];
+			const v3 v0 = add(mulv(l0, t), mulv(a0, k->fraXnI[r]));
+			const v3 v1 = add(mulv(l1, t), mulv(a1, k->frbXnI[r]));
+			const v3 dv = sub(v0, v1);
+			const float nv = dv.x + dv.y + dv.z;
+			const float af = k->fforce[r];
+			const float vm = k->fvelMul[r];
+			const float tmp = af - (k->fbias[r] - k->ftarget[r]) * vm;
+			const float total = tmp - nv * vm;
+			const int clamp = fabsf(total) > maxF;
+			const float tc = fminf(maxD, fmaxf(-maxD, total));
+			const float nf = clamp ? tc : total;
+			broken |= clamp;
+			const float df = nf - af;
+			l0 = add(scl(scl(t, k->invMassA), df), l0);
+			l1 = sub(l1, scl(scl(t, k->invMassB), df));
+			a0 = add(scl(k->fraXnI[r], df), a0);
+			a1 = sub(a1, scl(k->frbXnI[r], df));
+			k->fforce[r] = nf;
+		}
+		k->fp.broken = broken;
+	}
+	if (conclude)
+	{
+		for (int i = 0; i < k->c.count; i++)
+			k->biasedErr[i] = k->unbiasedErr[i];
+		for (int r = 0; r < k->nFric; r++)
+			k->fbias[r] = 0.0f;
+	}
+	b0->lin = l0; b0->ang = a0;
+	if (b1->invMass > 0.0f) { b1->lin = l1; b1->ang = a1; }
+}
+
 static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 {
-	/* shape order as the reference contact-method table: lower geometry type first */
-	int s0 = i, s1 = j;
-	if (w->type[j] < w->type[i]) { s0 = j; s1 = i; }
+	/* shape order: lower geometry type first (the contact-method table order);
+	 * for two shapes of one type the later-added body first, as the
+	 * reference's contact managers order them (shouldSwapBodies,
+	 * simulationcontroller/src/ScNPhaseCore.cpp:182-249; i < j here) */
+	int s0 = j, s1 = i;
+	if (w->type[i] < w->type[j]) { s0 = i; s1 = j; }
 	cset c;
 	c.count = 0;
 	int n = 0;
@@ -1772,9 +1935,19 @@ int pxo_world_step(void* wp, float dt)
 	const float invStepDt = 1.0f / stepDt;
 	const float biasCoefficient = 2.0f * sqrtf(1.0f / (float)P);
 	const float scale = fminf(0.8f, biasCoefficient);
+	const int pgs = d->solver == 0;
+	if (pgs)
+		for (int i = 0; i < n; i++)
+		{
+			sb[i].lin = V(0, 0, 0); /* PGS solves for velocity changes */
+			sb[i].ang = V(0, 0, 0);
+		}
 	for (long long c = 0; c < w->ncon; c++)
 	{
-		prep_con(w, &w->con[c], sb, &worldBody, frame, invStepDt, stepDt, dt, scale);
+		if (pgs)
+			prep_con_pgs(w, &w->con[c], sb, &worldBody, frame, 1.0f / dt, d->static_friction, d->dynamic_friction);
+		else
+			prep_con(w, &w->con[c], sb, &worldBody, frame, invStepDt, stepDt, dt, scale);
 		w->stats[5] += w->con[c].nFric / 2;
 	}
 
@@ -1790,8 +1963,40 @@ int pxo_world_step(void* wp, float dt)
 		for (long long c = 0; c < w->ncon; c++) order[fill[w->con[c].color]++] = c;
 		free(fill);
 	}
+	v3* motionLin = NULL;
+	v3* motionAng = NULL;
 	float elapsed = 0.0f;
-	for (int it = 0; it < P; it++)
+	if (pgs)
+	{
+		/* PGS (solveV_Blocks, lowleveldynamics/src/DySolverControl.cpp:163-330):
+		 * position passes, friction only in the last three (no
+		 * friction-every-iteration flag), the last one concluding; the
+		 * velocity changes at that point move the bodies; then velocity
+		 * passes */
+		for (int it = P; it > 0; it--)
+			for (int r = 0; r < ncolors; r++)
+				for (long long o = start[r]; o < start[r + 1]; o++)
+				{
+					pxo_con* k = &w->con[order[o]];
+					solve_con_pgs(k, &sb[k->a], k->b >= 0 ? &sb[k->b] : &worldBody, it <= 3, it == 1);
+				}
+		motionLin = (v3*)malloc(sizeof(v3) * (size_t)(n ? n : 1));
+		motionAng = (v3*)malloc(sizeof(v3) * (size_t)(n ? n : 1));
+		for (int i = 0; i < n; i++)
+		{
+			motionLin[i] = sb[i].lin;
+			motionAng[i] = sb[i].ang;
+		}
+		/* at least one velocity pass (DySolverControl.cpp:343) */
+		for (int it = 0; it < (Vi > 1 ? Vi : 1); it++)
+			for (int r = 0; r < ncolors; r++)
+				for (long long o = start[r]; o < start[r + 1]; o++)
+				{
+					pxo_con* k = &w->con[order[o]];
+					solve_con_pgs(k, &sb[k->a], k->b >= 0 ? &sb[k->b] : &worldBody, 1, 0);
+				}
+	}
+	for (int it = 0; it < P && !pgs; it++)
 	{
 		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
@@ -1804,7 +2009,7 @@ int pxo_world_step(void* wp, float dt)
 			integrate_step(&sb[i], stepDt);
 		elapsed += stepDt;
 	}
-	for (int it = 0; it < Vi; it++)
+	for (int it = 0; it < Vi && !pgs; it++)
 		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
 			{
@@ -1824,6 +2029,46 @@ int pxo_world_step(void* wp, float dt)
 	}
 	w->nprev = w->ncon;
 	qsort(w->prev, (size_t)w->nprev, sizeof(fpatch), cmp_patch);
+
+	/* 7. PGS: integrate with the motion velocities, keep the final velocities
+	 * (integrateCore, DyBodyCoreIntegrator.h:83-175) */
+	if (pgs)
+	{
+		for (int i = 0; i < n; i++)
+		{
+			const sbody* b = &sb[i];
+			const v3 lm = add(b->origLin, motionLin[i]);
+			w->pose[i].p = add(w->pose[i].p, scl(lm, dt));
+			v3 am = add(b->origAng, mmul(b->sqrtInvI, motionAng[i]));
+			float ww = dot(am, am);
+			if (ww != 0.0f)
+			{
+				ww = sqrtf(ww);
+				if (ww > 1e7f)
+				{
+					am = scl(normalize(am), 1e7f);
+					ww = 1e7f;
+				}
+				const float v = dt * ww * 0.5f;
+				float sn = sinf(v);
+				const float cs = cosf(v);
+				sn /= ww;
+				const v3 pqr = scl(am, sn);
+				const qt qv = {pqr.x, pqr.y, pqr.z, 0.0f};
+				const qt q0 = w->pose[i].q;
+				qt res = qmul(qv, q0);
+				res.x += q0.x * cs; res.y += q0.y * cs; res.z += q0.z * cs; res.w += q0.w * cs;
+				w->pose[i].q = qnormalize(res);
+			}
+			w->lv[i] = add(b->origLin, b->lin);
+			w->av[i] = add(b->origAng, mmul(b->sqrtInvI, b->ang));
+		}
+		free(motionLin);
+		free(motionAng);
+		free(sb);
+		free(frame);
+		return 0;
+	}
 
 	/* 7. copy back (DyTGSDynamics.cpp:1549-1580) */
 	for (int i = 0; i < n; i++)

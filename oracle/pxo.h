@@ -28,7 +28,7 @@ enum { PXO_SPHERE = 0, PXO_PLANE = 1, PXO_CAPSULE = 2, PXO_BOX = 3 };
 
 typedef struct {
 	float gravity[3];
-	int   solver;             /* 1 = TGS (0 = PGS, not restated yet) */
+	int   solver;             /* 1 = TGS, 0 = PGS */
 	int   pos_iters, vel_iters;
 	float contact_offset;     /* per shape; pair contact distance = sum */
 	float rest_offset;
@@ -37,6 +37,9 @@ typedef struct {
 	float max_lin_vel_sq, max_ang_vel_sq;
 	float bounce_threshold, friction_offset_threshold, correlation_distance;
 	float tolerance_length;
+	/* test knob: xored into the colouring order keys (0 = the order the GPU
+	 * path uses); a different value only reorders the Gauss-Seidel passes */
+	unsigned int order_salt;
 } pxo_desc;
 
 void pxo_default_desc(pxo_desc* d);

@@ -84,12 +84,51 @@ struct CacheAlloc : PxCacheAllocator
 	}
 };
 
+// Contact points the scene reports for every touching pair of a step
+// (actor ids: dynamic body i -> i + 1, static plane j -> -(j + 1)).
+struct ContactCollector : public PxSimulationEventCallback
+{
+	std::vector<float> rows;  // a, b, point xyz, separation, normal xyz, impulse
+	void onContact(const PxContactPairHeader& h, const PxContactPair* pairs, PxU32 nb) override
+	{
+		const float a = float(reinterpret_cast<intptr_t>(h.actors[0]->userData));
+		const float b = float(reinterpret_cast<intptr_t>(h.actors[1]->userData));
+		PxContactPairPoint pts[64];
+		for (PxU32 i = 0; i < nb; i++)
+		{
+			const PxU32 k = pairs[i].extractContacts(pts, 64);
+			for (PxU32 j = 0; j < k; j++)
+			{
+				const float r[10] = {a, b, pts[j].position.x, pts[j].position.y, pts[j].position.z, pts[j].separation,
+				                     pts[j].normal.x, pts[j].normal.y, pts[j].normal.z, pts[j].impulse.magnitude()};
+				rows.insert(rows.end(), r, r + 10);
+			}
+		}
+	}
+	void onConstraintBreak(PxConstraintInfo*, PxU32) override {}
+	void onWake(PxActor**, PxU32) override {}
+	void onSleep(PxActor**, PxU32) override {}
+	void onTrigger(PxTriggerPair*, PxU32) override {}
+	void onAdvance(const PxRigidBody* const*, const PxTransform*, const PxU32) override {}
+};
+
+PxFilterFlags reportingFilterShader(PxFilterObjectAttributes a0, PxFilterData d0, PxFilterObjectAttributes a1,
+                                    PxFilterData d1, PxPairFlags& pairFlags, const void* block, PxU32 size)
+{
+	const PxFilterFlags f = PxDefaultSimulationFilterShader(a0, d0, a1, d1, pairFlags, block, size);
+	pairFlags |= PxPairFlag::eNOTIFY_TOUCH_FOUND | PxPairFlag::eNOTIFY_TOUCH_PERSISTS |
+	             PxPairFlag::eNOTIFY_CONTACT_POINTS;
+	return f;
+}
+
 struct RefScene
 {
 	PxDefaultCpuDispatcher* dispatcher = nullptr;
 	PxScene* scene = nullptr;
 	PxMaterial* material = nullptr;
 	std::vector<PxRigidDynamic*> bodies;
+	int nPlanes = 0;
+	ContactCollector* contacts = nullptr;
 };
 
 }  // namespace
@@ -162,7 +201,9 @@ int pxref_generate_contacts(int type0, const float* geom0, const float* pose0, i
 }
 
 // Scene: gravity xyz; solverType 0 PGS / 1 TGS; bpType 0 ABP / 1 PABP.
-void* pxref_scene_create(const float* gravity, int solverType, int bpType, int nThreads)
+// reportContacts != 0: the scene reports every touching pair's contact points
+// each step (pxref_scene_contacts); the simulation itself is unchanged
+void* pxref_scene_create2(const float* gravity, int solverType, int bpType, int nThreads, int reportContacts)
 {
 	if (!ensureSdk())
 		return nullptr;
@@ -172,6 +213,12 @@ void* pxref_scene_create(const float* gravity, int solverType, int bpType, int n
 	rs->dispatcher = PxDefaultCpuDispatcherCreate(PxU32(nThreads));
 	sd.cpuDispatcher = rs->dispatcher;
 	sd.filterShader = PxDefaultSimulationFilterShader;
+	if (reportContacts)
+	{
+		rs->contacts = new ContactCollector();
+		sd.filterShader = reportingFilterShader;
+		sd.simulationEventCallback = rs->contacts;
+	}
 	sd.solverType = solverType ? PxSolverType::eTGS : PxSolverType::ePGS;
 	sd.broadPhaseType = bpType ? PxBroadPhaseType::ePABP : PxBroadPhaseType::eABP;
 	rs->scene = gPhysics->createScene(sd);
@@ -182,6 +229,23 @@ void* pxref_scene_create(const float* gravity, int solverType, int bpType, int n
 	}
 	rs->material = gPhysics->createMaterial(0.5f, 0.5f, 0.0f);
 	return rs;
+}
+
+void* pxref_scene_create(const float* gravity, int solverType, int bpType, int nThreads)
+{
+	return pxref_scene_create2(gravity, solverType, bpType, nThreads, 0);
+}
+
+// Contact rows of the last step (10 floats each, see ContactCollector);
+// returns the row count (rows beyond max are not copied).
+int pxref_scene_contacts(void* h, float* out, int max)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	if (!rs->contacts)
+		return -1;
+	const int k = int(rs->contacts->rows.size() / 10);
+	std::memcpy(out, rs->contacts->rows.data(), sizeof(float) * 10 * size_t(k < max ? k : max));
+	return k;
 }
 
 int pxref_scene_set_material(void* h, float staticFriction, float dynamicFriction, float restitution)
@@ -217,6 +281,7 @@ int pxref_scene_add_bodies(void* h, int n, const int* type, const float* dims, c
 		body->setAngularDamping(angularDamping);
 		body->setSleepThreshold(0.0f);
 		body->setStabilizationThreshold(0.0f);
+		body->userData = reinterpret_cast<void*>(intptr_t(rs->bodies.size() + 1));
 		rs->scene->addActor(*body);
 		rs->bodies.push_back(body);
 	}
@@ -237,6 +302,7 @@ int pxref_scene_add_planes(void* h, int n, const float* plane, float contactOffs
 		s->getShapes(&shape, 1);
 		shape->setContactOffset(contactOffset);
 		shape->setRestOffset(0.0f);
+		s->userData = reinterpret_cast<void*>(intptr_t(-(++rs->nPlanes)));
 		rs->scene->addActor(*s);
 	}
 	return 0;
@@ -250,6 +316,8 @@ double pxref_scene_step(void* h, float dt, int nsteps, double* stepMs)
 	auto t0 = std::chrono::steady_clock::now();
 	for (int s = 0; s < nsteps; s++)
 	{
+		if (rs->contacts)
+			rs->contacts->rows.clear();
 		auto a = std::chrono::steady_clock::now();
 		rs->scene->simulate(dt);
 		rs->scene->fetchResults(true);
@@ -310,6 +378,7 @@ void pxref_scene_release(void* h)
 		rs->dispatcher->release();
 	if (rs->material)
 		rs->material->release();
+	delete rs->contacts;
 	delete rs;
 }
 

@@ -108,7 +108,7 @@ void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* c
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                         float maxAngSq, SolverBodies sb, hipStream_t st);
+                         float maxAngSq, bool pgs, SolverBodies sb, hipStream_t st);
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
@@ -116,6 +116,18 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
 // broken: when not null, each row's friction-clamp flag of this pass is stored there
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st);
+// PGS: rows with the unbiased normal errors in rowsU; position passes
+// (velocityPass false) then velocity passes; SolverBodies p/dq hold the motion
+// velocities between the two
+void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                     const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb,
+                     float invDt, float muS, float muD, float restDist, float fot, Rows rows, float* rowsU,
+                     PatchIO patches, unsigned long long* counters, hipStream_t st);
+void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, SolverBodies sb, bool doFriction,
+                      bool velocityPass, uint32_t* broken, hipStream_t st);
+void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st);
+void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb,
+                         hipStream_t st);
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
 void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,

@@ -499,9 +499,13 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 				const int c = pairClass(ti, tj);
 				if (WRITE)
 				{
+					// shape order of the reference's contact managers: a sphere-box
+					// pair runs sphere first (contact-method table order); two
+					// shapes of one type put the later-added body first
+					// (shouldSwapBodies, simulationcontroller/src/ScNPhaseCore.cpp:182-249)
 					uint32_t pa, pb;
 					if (c == 1) { pa = ti == GEOM_SPHERE ? i : jj; pb = ti == GEOM_SPHERE ? jj : i; }
-					else { pa = min(i, jj); pb = max(i, jj); }
+					else { pa = max(i, jj); pb = min(i, jj); }
 					// pairs past the buffer's capacity are counted, not written
 					if (off[c] < pairCap)
 						pairs[off[c]] = make_uint2(pa, pb);
@@ -871,8 +875,8 @@ __global__ void k_color_keys(int C, const int* conColor, uint32_t limit, uint64_
 // + copyToSolverBodyDataStep (lowleveldynamics/src/DyTGSDynamics.cpp:154-242)
 __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                                const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                               float maxAngSq, float4* vel, float4* svP, float4* svDQ, float4* svI0, float4* svI1,
-                               float4* svOrigLin)
+                               float maxAngSq, int pgs, float4* vel, float4* svP, float4* svDQ, float4* svI0,
+                               float4* svI1, float4* svOrigLin)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
@@ -899,10 +903,23 @@ __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, cons
 	const m33 S = mmm(A, Rt);
 	const m33 SI = mmm(B, Rt);
 	float4* v = vel + 4 * int64_t(i);
-	v[0] = f4(lv, l.w);
-	v[1] = f4(mmul(SI, av), 0.0f);
-	v[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-	v[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	if (pgs)
+	{
+		// PGS solves for velocity changes from the pre-integrated velocities,
+		// kept in the record's second half (copyToSolverBodyData,
+		// lowleveldynamics/src/DyRigidBodyToSolverBody.cpp:37-113)
+		v[0] = make_float4(0.0f, 0.0f, 0.0f, l.w);
+		v[1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		v[2] = f4(lv, 0.0f);
+		v[3] = f4(av, 0.0f);
+	}
+	else
+	{
+		v[0] = f4(lv, l.w);
+		v[1] = f4(mmul(SI, av), 0.0f);
+		v[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		v[3] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
 	svP[i] = pos[i];
 	svDQ[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 	// symmetric sqrt inverse inertia: (xx yy zz xy) (xz yz)
@@ -920,108 +937,24 @@ PXH_D m33 loadSym(float4 a, float4 b)
 	return m;
 }
 
-PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
-                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
-                    uint32_t& nFricOut);
-
-// Contact rows of one constraint at its colour-ordered position o
-// (setupFinalizeSolverConstraints / constructContactConstraintStep,
-//  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction patches
-//  getFrictionPatches / correlatePatches / growPatches, DyContactPrepShared.h:53-127,
-//  DyFrictionCorrelation.cpp:136-316)
-__global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                       const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
-                       const float4* vel, const float4* svOrigLin, const float4* svI0, const float4* svI1,
-                       float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
-                       PatchIO io, unsigned long long* counters)
+// Friction patch of the constraint at row o (constraint c, bodies a, b,
+// normal n, frames f0/f1, contact points cp/cs): the previous step's patch of
+// this pair is kept when it is unbroken, has anchors, its body normals still
+// agree, no anchor drifted apart along the normal, and its world normal
+// matches this step's normal (getFrictionPatches, correlatePatches,
+// growPatches: lowleveldynamics/src/DyContactPrepShared.h:53-127,
+// DyFrictionCorrelation.cpp:136-316). Writes the patch for the next step and
+// returns the anchor count, the anchors in pa0/pa1 (body space). Shared by the
+// TGS and PGS preparation.
+PXH_D int frictionPatch(int o, int c, int a, int b, f3 n, const xf& f0, const xf& f1, const f3 (&cp)[4],
+                        const float (&cs)[4], int cnt, float restDist, float frictionOffsetThreshold,
+                        const PatchIO& io, f3 (&pa0)[2], f3 (&pa1)[2])
 {
-	const int o = blockIdx.x * blockDim.x + threadIdx.x;
-	// per-step counters: contact points, friction rows, dynamic-dynamic constraints
-	uint32_t myCnt = 0, myFric = 0, myDyn = 0;
-	if (o < C)
-	{
-		const int c = order[o];
-		myCnt = uint32_t(__float_as_int(ctN[conSlot[c]].w));
-		myDyn = conB[c] >= 0 ? 1u : 0u;
-	}
-	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, vel, svOrigLin, svI0, svI1, invStepDt, scale,
-	         restDist, frictionOffsetThreshold, rows, io, myFric);
-	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
-	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
-	if ((threadIdx.x & 63) == 63)
-	{
-		atomicAdd(&counters[0], (unsigned long long)sc);
-		atomicAdd(&counters[1], (unsigned long long)sf);
-		atomicAdd(&counters[2], (unsigned long long)sd);
-		atomicAdd(&counters[3], (unsigned long long)sfd);
-	}
-}
-
-PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
-                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
-                    uint32_t& nFricOut)
-{
-	if (o >= C)
-		return;
-	const int c = order[o];
-	const int a = conA[c], b = conB[c], slot = conSlot[c];
-	const float4 nn = ctN[slot];
-	const int cnt = __float_as_int(nn.w);
-	const f3 n = mk(nn);
-	const bool dynB = b >= 0;
-	const m33 Sa = loadSym(svI0[a], svI1[a]);
-	m33 Sb;
-	Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
-	float invMassB = 0.0f;
-	xf f0, f1;
-	f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
-	f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
-	f3 linB = mk(0, 0, 0);
-	if (dynB)
-	{
-		Sb = loadSym(svI0[b], svI1[b]);
-		invMassB = vel[4 * int64_t(b)].w;
-		f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
-		linB = mk(svOrigLin[b]);
-	}
-	const float invMassA = vel[4 * int64_t(a)].w;
-	const float nlen2 = dot(n, n);
-	const float imn0 = invMassA * nlen2, imn1 = -invMassB * nlen2;
-	const float invDtp8 = invStepDt * scale;
-	// loops run to the fixed bound 4 with guards so the point arrays stay in
-	// registers (a loop to cnt would index them dynamically, in scratch)
-	f3 cp[4];
-	float cs[4];
-#pragma unroll
-	for (int k = 0; k < 4; k++)
-	{
-		cp[k] = mk(0, 0, 0);
-		cs[k] = 0.0f;
-		if (k >= cnt)
-			continue;
-		const float4 q = ctP[4 * int64_t(slot) + k];
-		cp[k] = mk(q);
-		cs[k] = q.w;
-		const f3 ra = cp[k] - f0.p, rb = cp[k] - f1.p;
-		const f3 raXnI = mmul(Sa, cross(ra, n)), rbXnI = mmul(Sb, cross(rb, n));
-		const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
-		const float pen = q.w - restDist;
-		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
-		rows.nA[k * int64_t(C) + o] = f4(raXnI, pen);
-		// the bias coefficient is -invStepDt for a separated point, else -invStepDt*scale:
-		// the sign of the stored reciprocal response carries that choice
-		rows.nB[k * int64_t(C) + o] = f4(rbXnI, pen > 0.0f ? -recip : recip);
-		rows.nF[k * int64_t(C) + o] = 0.0f;
-	}
 	// friction patch: the previous step's patch of this pair is kept when it is
 	// unbroken, has anchors, its body normals still agree, no anchor drifted
 	// apart along the normal, and its world normal matches this step's normal
 	const uint64_t pkey = pairKey(a, b);
-	f3 pn0, pn1, pa0[2], pa1[2];
+	f3 pn0, pn1;
 	int nKept = 0;
 	{
 		const int po = io.prevRow ? io.prevRow[c] : -1;
@@ -1138,9 +1071,110 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	io.cur[6 * int64_t(o) + 3] = f4(pa0[1], 0.0f);
 	io.cur[6 * int64_t(o) + 4] = f4(pa1[0], 0.0f);
 	io.cur[6 * int64_t(o) + 5] = f4(pa1[1], 0.0f);
-	const float fscale = nAnch == 2 ? 0.5f : 1.0f;
+	return nAnch;
+}
+
+PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
+                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
+                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
+                    uint32_t& nFricOut);
+
+// Contact rows of one constraint at its colour-ordered position o
+// (setupFinalizeSolverConstraints / constructContactConstraintStep,
+//  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction patches
+//  getFrictionPatches / correlatePatches / growPatches, DyContactPrepShared.h:53-127,
+//  DyFrictionCorrelation.cpp:136-316)
+__global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                       const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
+                       const float4* vel, const float4* svOrigLin, const float4* svI0, const float4* svI1,
+                       float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
+                       PatchIO io, unsigned long long* counters)
+{
+	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	// per-step counters: contact points, friction rows, dynamic-dynamic constraints
+	uint32_t myCnt = 0, myFric = 0, myDyn = 0;
+	if (o < C)
+	{
+		const int c = order[o];
+		myCnt = uint32_t(__float_as_int(ctN[conSlot[c]].w));
+		myDyn = conB[c] >= 0 ? 1u : 0u;
+	}
+	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, vel, svOrigLin, svI0, svI1, invStepDt, scale,
+	         restDist, frictionOffsetThreshold, rows, io, myFric);
+	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
+	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
+	if ((threadIdx.x & 63) == 63)
+	{
+		atomicAdd(&counters[0], (unsigned long long)sc);
+		atomicAdd(&counters[1], (unsigned long long)sf);
+		atomicAdd(&counters[2], (unsigned long long)sd);
+		atomicAdd(&counters[3], (unsigned long long)sfd);
+	}
+}
+
+PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
+                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
+                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
+                    uint32_t& nFricOut)
+{
+	if (o >= C)
+		return;
+	const int c = order[o];
+	const int a = conA[c], b = conB[c], slot = conSlot[c];
+	const float4 nn = ctN[slot];
+	const int cnt = __float_as_int(nn.w);
+	const f3 n = mk(nn);
+	const bool dynB = b >= 0;
+	const m33 Sa = loadSym(svI0[a], svI1[a]);
+	m33 Sb;
+	Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
+	float invMassB = 0.0f;
+	xf f0, f1;
+	f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
+	f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
+	f3 linB = mk(0, 0, 0);
+	if (dynB)
+	{
+		Sb = loadSym(svI0[b], svI1[b]);
+		invMassB = vel[4 * int64_t(b)].w;
+		f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
+		linB = mk(svOrigLin[b]);
+	}
+	const float invMassA = vel[4 * int64_t(a)].w;
+	const float nlen2 = dot(n, n);
+	const float imn0 = invMassA * nlen2, imn1 = -invMassB * nlen2;
+	const float invDtp8 = invStepDt * scale;
+	// loops run to the fixed bound 4 with guards so the point arrays stay in
+	// registers (a loop to cnt would index them dynamically, in scratch)
+	f3 cp[4];
+	float cs[4];
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		cp[k] = mk(0, 0, 0);
+		cs[k] = 0.0f;
+		if (k >= cnt)
+			continue;
+		const float4 q = ctP[4 * int64_t(slot) + k];
+		cp[k] = mk(q);
+		cs[k] = q.w;
+		const f3 ra = cp[k] - f0.p, rb = cp[k] - f1.p;
+		const f3 raXnI = mmul(Sa, cross(ra, n)), rbXnI = mmul(Sb, cross(rb, n));
+		const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
+		const float pen = q.w - restDist;
+		const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+		rows.nA[k * int64_t(C) + o] = f4(raXnI, pen);
+		// the bias coefficient is -invStepDt for a separated point, else -invStepDt*scale:
+		// the sign of the stored reciprocal response carries that choice
+		rows.nB[k * int64_t(C) + o] = f4(rbXnI, pen > 0.0f ? -recip : recip);
+		rows.nF[k * int64_t(C) + o] = 0.0f;
+	}
+	f3 pa0[2], pa1[2];
+	const int nAnch =
+	    frictionPatch(o, c, a, b, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
 	nFricOut = uint32_t(2 * nAnch);
-	(void)fscale;
 	(void)invDtp8;
 	rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), 0);
 	rows.hF[o] = make_float4(n.x, n.y, n.z, 0.0f);
@@ -1316,6 +1350,307 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 	}
 }
 
+// ---------------------------------------------------------------------------
+// PGS (DynamicsContext, lowleveldynamics/src/DyDynamics.cpp; DySolverControl.cpp)
+// Rows (PGS use): hI.w = dynamic friction (scaled for the anchor count, float
+// bits), hF.w = static friction (scaled); nA = raXnI | biased error; nB =
+// rbXnI | velocity multiplier; unbiased error in its own array; fT = t |
+// bias - target velocity; fA = raXnI | velocity multiplier; fB = rbXnI |
+// -target velocity.
+// ---------------------------------------------------------------------------
+
+// setupFinalizeSolverConstraints (lowleveldynamics/src/DyContactPrep.cpp:49-357)
+// with constructContactConstraint (DyContactPrepShared.h:298-393)
+__global__ void k_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                           const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
+                           const float4* vel, const float4* svI0, const float4* svI1, float invDt, float muS,
+                           float muD, float restDist, float frictionOffsetThreshold, Rows rows, float* rowsU,
+                           PatchIO io, unsigned long long* counters)
+{
+	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	uint32_t myCnt = 0, myFric = 0, myDyn = 0;
+	if (o < C)
+	{
+		const int c = order[o];
+		const int a = conA[c], b = conB[c], slot = conSlot[c];
+		const float4 nn = ctN[slot];
+		const int cnt = __float_as_int(nn.w);
+		const f3 n = mk(nn);
+		const bool dynB = b >= 0;
+		myCnt = uint32_t(cnt);
+		myDyn = dynB ? 1u : 0u;
+		const m33 Sa = loadSym(svI0[a], svI1[a]);
+		m33 Sb;
+		Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
+		xf f0, f1;
+		f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
+		f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
+		const float4* va = vel + 4 * int64_t(a);
+		const float invMassA = va[0].w;
+		const f3 v0 = mk(va[2]), w0 = mk(va[3]);
+		float invMassB = 0.0f;
+		f3 v1 = mk(0, 0, 0), w1 = mk(0, 0, 0);
+		if (dynB)
+		{
+			const float4* vb = vel + 4 * int64_t(b);
+			Sb = loadSym(svI0[b], svI1[b]);
+			invMassB = vb[0].w;
+			v1 = mk(vb[2]);
+			w1 = mk(vb[3]);
+			f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
+		}
+		const float nl2 = dot(n, n);
+		const float imn0 = invMassA * nl2, imn1 = -invMassB * nl2;
+		const float invDtp8 = invDt * 0.8f;
+		const float norVel = (n.x * v0.x - n.x * v1.x) + (n.y * v0.y - n.y * v1.y) + (n.z * v0.z - n.z * v1.z);
+		f3 cp[4];
+		float cs[4];
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+		{
+			cp[k] = mk(0, 0, 0);
+			cs[k] = 0.0f;
+			if (k >= cnt)
+				continue;
+			const float4 q = ctP[4 * int64_t(slot) + k];
+			cp[k] = mk(q);
+			cs[k] = q.w;
+			const f3 ra = cp[k] - f0.p, rb = cp[k] - f1.p;
+			const f3 raXn = cross(ra, n), rbXn = cross(rb, n);
+			const float vrel = norVel + (dot(raXn, w0) - dot(rbXn, w1));
+			const f3 raXnI = mmul(Sa, raXn), rbXnI = mmul(Sb, rbXn);
+			const float unit = (imn0 + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - imn1);
+			const float pen = q.w - restDist;
+			const float target = 0.0f - vrel;
+			const float recip = unit > 0.0f ? 1.0f / unit : 0.0f;
+			const float penScaled = pen >= 0.0f ? pen * invDt : pen * invDtp8;
+			const float scaledBias = recip * fmaxf(-1e32f, penScaled);
+			const int64_t ri = k * int64_t(C) + o;
+			rows.nA[ri] = f4(raXnI, target * recip + -scaledBias);
+			rows.nB[ri] = f4(rbXnI, recip);
+			rowsU[ri] = target * recip + -fmaxf(scaledBias, 0.0f);
+			rows.nF[ri] = 0.0f;
+		}
+		f3 pa0[2], pa1[2];
+		const int nAnch =
+		    frictionPatch(o, c, a, b, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
+		myFric = uint32_t(2 * nAnch);
+		const float fc = nAnch == 2 ? 0.5f : 1.0f;
+		rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), __float_as_int(muD * fc));
+		rows.hF[o] = make_float4(n.x, n.y, n.z, muS * fc);
+		if (nAnch > 0)
+		{
+			const f3 lvr = v0 - v1;
+			const f3 fb = (0.70710678f > fabsf(n.x)) ? mk(0.0f, -n.z, n.y) : mk(-n.y, n.x, 0.0f);
+			f3 t0 = lvr - n * dot(n, lvr);
+			t0 = dot(t0, t0) > 0.0001f ? t0 : fb;
+			t0 = normalize(t0);
+			const f3 t1 = cross(n, t0);
+#pragma unroll
+			for (int j = 0; j < 2; j++)
+			{
+				if (j >= nAnch)
+					break;
+				const f3 ra = qrot(f0.q, pa0[j]), rb = qrot(f1.q, pa1[j]);
+				const f3 err = (ra + f0.p) - (rb + f1.p);
+#pragma unroll
+				for (int d = 0; d < 2; d++)
+				{
+					const f3 t = d == 0 ? t0 : t1;
+					const f3 raXn = cross(ra, t), rbXn = cross(rb, t);
+					const f3 raXnI = mmul(Sa, raXn), rbXnI = mmul(Sb, rbXn);
+					const float resp = (invMassA + dot(raXnI, raXnI)) + (dot(rbXnI, rbXnI) - -invMassB);
+					const float vrel = (dot(t, v0) + dot(raXn, w0)) - (dot(t, v1) + dot(rbXn, w1));
+					const float target = 0.0f - vrel;
+					const float bias = dot(t, err) * invDt;
+					const int64_t ri = (2 * j + d) * int64_t(C) + o;
+					rows.fT[ri] = f4(t, bias - target);
+					rows.fA[ri] = f4(raXnI, resp > 0.0f ? 0.8f / resp : 0.0f);
+					rows.fB[ri] = f4(rbXnI, 0.0f - target);
+					rows.fF[ri] = 0.0f;
+				}
+			}
+		}
+	}
+	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
+	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
+	if ((threadIdx.x & 63) == 63)
+	{
+		atomicAdd(&counters[0], (unsigned long long)sc);
+		atomicAdd(&counters[1], (unsigned long long)sf);
+		atomicAdd(&counters[2], (unsigned long long)sd);
+		atomicAdd(&counters[3], (unsigned long long)sfd);
+	}
+}
+
+// one colour of PGS (solveContact, lowleveldynamics/src/DySolverConstraints.cpp:221-371;
+// normal rows solveDynamicContacts, DySolverConstraintsShared.h:49-105) on the
+// velocity changes in vel[4i], vel[4i+1]. velocityPass: rows concluded after the
+// last position pass use their unbiased error and friction without bias
+// (concludeContact, DySolverConstraints.cpp:508-549).
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve_pgs(int begin, int end, int C, Rows rows,
+                                                              const float* rowsU, float4* vel, int doFriction,
+                                                              int velocityPass, uint32_t* broken)
+{
+	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= end)
+		return;
+	const int4 hi = rows.hI[o];
+	const float4 hf = rows.hF[o];
+	const int a = hi.x, b = hi.y;
+	const int cnt = hi.z & 0xFF, nF = doFriction ? hi.z >> 8 : 0;
+	const bool dynB = b >= 0;
+	const float4* va = vel + 4 * int64_t(a);
+	const float4 la4 = va[0], aa4 = va[1];
+	float4 lb4 = make_float4(0, 0, 0, 0), ab4 = lb4;
+	if (dynB)
+	{
+		const float4* vb = vel + 4 * int64_t(b);
+		lb4 = vb[0]; ab4 = vb[1];
+	}
+	float4 nA[4], nB[4], fT[4], fA[4], fB[4];
+	float nFc[4], fFc[4], err[4];
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		const int64_t ri = k * int64_t(C) + o;
+		if (k < cnt)
+		{
+			nA[k] = rows.nA[ri]; nB[k] = rows.nB[ri]; nFc[k] = rows.nF[ri];
+			err[k] = velocityPass ? rowsU[ri] : nA[k].w;
+		}
+		if (k < nF)
+		{
+			fT[k] = rows.fT[ri]; fA[k] = rows.fA[ri]; fB[k] = rows.fB[ri]; fFc[k] = rows.fF[ri];
+		}
+	}
+	const f3 n = mk(hf);
+	const float invMassA = la4.w, invMassB = lb4.w;
+	f3 l0 = mk(la4), a0 = mk(aa4), l1 = mk(lb4), a1 = mk(ab4);
+	const f3 dl0 = n * invMassA, dl1 = n * invMassB;
+	float acc = 0.0f;
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		if (k >= cnt)
+			break;
+		const f3 raXnI = mk(nA[k]), rbXnI = mk(nB[k]);
+		const f3 v0 = mulv(l0, n) + mulv(a0, raXnI);
+		const f3 v1 = mulv(l1, n) + mulv(a1, rbXnI);
+		const f3 dv = v0 - v1;
+		const float nv = dv.x + dv.y + dv.z;
+		const float af = nFc[k];
+		const float d0 = fmaxf(err[k] - nv * nB[k].w, -af);
+		const float nf = fminf(af + d0, 1e32f);
+		const float df = nf - af;
+		l0 = dl0 * df + l0;
+		l1 = l1 - dl1 * df;
+		a0 = raXnI * df + a0;
+		a1 = a1 - rbXnI * df;
+		nFc[k] = nf;
+		acc = acc + nf;
+	}
+	bool anyClamp = false;
+	if (nF)
+	{
+		const float maxF = hf.w * acc, maxD = __int_as_float(hi.w) * acc;
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+		{
+			if (r >= nF)
+				break;
+			const f3 t = mk(fT[r]), raI = mk(fA[r]), rbI = mk(fB[r]);
+			const f3 v0 = mulv(l0, t) + mulv(a0, raI);
+			const f3 v1 = mulv(l1, t) + mulv(a1, rbI);
+			const f3 dv = v0 - v1;
+			const float nv = dv.x + dv.y + dv.z;
+			const float vm = fA[r].w;
+			const float af = fFc[r];
+			const float tmp = af - (velocityPass ? fB[r].w : fT[r].w) * vm;
+			const float total = tmp - nv * vm;
+			const bool clamp = fabsf(total) > maxF;
+			const float tc = fminf(maxD, fmaxf(-maxD, total));
+			const float nf = clamp ? tc : total;
+			anyClamp |= clamp;
+			const float df = nf - af;
+			l0 = (t * invMassA) * df + l0;
+			l1 = l1 - (t * invMassB) * df;
+			a0 = raI * df + a0;
+			a1 = a1 - rbI * df;
+			fFc[r] = nf;
+		}
+	}
+	if (broken)
+		broken[o] = anyClamp ? 1u : 0u;
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+	{
+		const int64_t ri = k * int64_t(C) + o;
+		if (k < cnt)
+			rows.nF[ri] = nFc[k];
+		if (k < nF)
+			rows.fF[ri] = fFc[k];
+	}
+	vel[4 * int64_t(a)] = f4(l0, invMassA);
+	vel[4 * int64_t(a) + 1] = f4(a0, 0.0f);
+	if (dynB)
+	{
+		vel[4 * int64_t(b)] = f4(l1, invMassB);
+		vel[4 * int64_t(b) + 1] = f4(a1, 0.0f);
+	}
+}
+
+// velocity changes after the position passes move the bodies (saveMotionVelocities,
+// lowleveldynamics/src/DySolverCore.cpp:40-55)
+__global__ void k_pgs_save_motion(int n, const float4* vel, float4* motionLin, float4* motionAng)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	motionLin[i] = vel[4 * int64_t(i)];
+	motionAng[i] = vel[4 * int64_t(i) + 1];
+}
+
+// integrateCore (lowleveldynamics/src/DyBodyCoreIntegrator.h:83-175): poses
+// from the motion velocities, body velocities from the final changes
+__global__ void k_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang,
+                               const float4* vel, const float4* motionLin, const float4* motionAng,
+                               const float4* svI0, const float4* svI1)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	const float4* v = vel + 4 * int64_t(i);
+	const f3 vD = mk(v[2]), wD = mk(v[3]);
+	const m33 S = loadSym(svI0[i], svI1[i]);
+	const f3 lm = vD + mk(motionLin[i]);
+	pos[i] = f4(mk(pos[i]) + lm * dt, pos[i].w);
+	f3 am = wD + mmul(S, mk(motionAng[i]));
+	float w = dot(am, am);
+	if (w != 0.0f)
+	{
+		w = sqrtf(w);
+		if (w > 1e7f)
+		{
+			am = normalize(am) * 1e7f;
+			w = 1e7f;
+		}
+		const float hv = dt * w * 0.5f;
+		float s, c;
+		sincosf(hv, &s, &c);
+		s /= w;
+		const f3 pqr = am * s;
+		qt qv; qv.x = pqr.x; qv.y = pqr.y; qv.z = pqr.z; qv.w = 0.0f;
+		const qt q0 = mkq(rot[i]);
+		qt res = qmul(qv, q0);
+		res.x += q0.x * c; res.y += q0.y * c; res.z += q0.z * c; res.w += q0.w * c;
+		rot[i] = f4(qnormalize(res));
+	}
+	const float4 l = v[0];
+	lin[i] = f4(vD + mk(l), l.w);
+	ang[i] = f4(wD + mmul(S, mk(v[1])), 0.0f);
+}
+
 // integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476)
 __global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* svDQ, const float4* svI0,
                             const float4* svI1)
@@ -1414,7 +1749,8 @@ __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, fl
 // standalone broadphase: found/lost pairs between consecutive pair sets, both
 // sorted by (id0, id1) (BpBroadPhase.h:159-185 created/deleted semantics)
 // ---------------------------------------------------------------------------
-PXH_D uint64_t pkey(uint2 p) { return (uint64_t(p.x) << 32) | p.y; }
+// canonical pair key (lower index first)
+PXH_D uint64_t pkey(uint2 p) { return (uint64_t(min(p.x, p.y)) << 32) | max(p.x, p.y); }
 
 __global__ void k_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals)
 {
@@ -1673,9 +2009,9 @@ void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_
 }
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                         float maxAngSq, SolverBodies sb, hipStream_t st)
+                         float maxAngSq, bool pgs, SolverBodies sb, hipStream_t st)
 {
-	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, sb.vel,
+	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, pgs ? 1 : 0, sb.vel,
 	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin);
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
@@ -1692,6 +2028,30 @@ void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float m
 	if (end > begin)
 		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.vel, minPen, muS, muD, invStepDt,
 		                                                                     biasScale, scale, broken);
+}
+void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
+                     const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb,
+                     float invDt, float muS, float muD, float restDist, float fot, Rows rows, float* rowsU,
+                     PatchIO patches, unsigned long long* counters, hipStream_t st)
+{
+	PXH_LAUNCH(k_prep_pgs, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.vel, sb.i0, sb.i1, invDt, muS,
+	           muD, restDist, fot, rows, rowsU, patches, counters);
+}
+void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, SolverBodies sb, bool doFriction,
+                      bool velocityPass, uint32_t* broken, hipStream_t st)
+{
+	if (end > begin)
+		k_solve_pgs<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(
+		    begin, end, C, rows, rowsU, sb.vel, doFriction ? 1 : 0, velocityPass ? 1 : 0, broken);
+}
+void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st)
+{
+	PXH_LAUNCH(k_pgs_save_motion, n, n, sb.vel, sb.p, sb.dq);
+}
+void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb,
+                         hipStream_t st)
+{
+	PXH_LAUNCH(k_pgs_finalize, n, n, dt, pos, rot, lin, ang, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
 }
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
 {

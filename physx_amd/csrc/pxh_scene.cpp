@@ -140,7 +140,7 @@ struct pxh_scene_t
 	DBuf<float4> sbVel, sbP, sbDQ, sbI0, sbI1, sbOrig;
 	DBuf<int4> rhI;
 	DBuf<float4> rhF, rnA, rnB, rfT, rfA, rfB;
-	DBuf<float> rnF, rfF;
+	DBuf<float> rnF, rfF, rnU;
 	// friction patches of the previous and the current step (double-buffered by
 	// patchCur), the previous step's row count and the pair-key table
 	DBuf<float4> patch[2];
@@ -381,8 +381,8 @@ int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
 {
 	if (!desc || !out)
 		return fail(PXH_ERR_INVALID, "pxh_scene_create: null argument");
-	if (desc->solver_type != PXH_SOLVER_TGS)
-		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: only the TGS solver is implemented");
+	if (desc->solver_type != PXH_SOLVER_TGS && desc->solver_type != PXH_SOLVER_PGS)
+		return fail(PXH_ERR_INVALID, "pxh_scene_create: unknown solver type");
 	if (desc->position_iterations < 1 || desc->velocity_iterations < 0)
 		return fail(PXH_ERR_INVALID, "pxh_scene_create: iteration counts out of range");
 	if (desc->restitution != 0.0f)
@@ -432,6 +432,7 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->rhI.release();
 	s->rnF.release();
 	s->rfF.release();
+	s->rnU.release();
 	for (int k = 0; k < 2; k++)
 	{
 		s->patch[k].release();
@@ -771,8 +772,9 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	PXH_CHECK(s->sbI0.ensure(n)); PXH_CHECK(s->sbI1.ensure(n)); PXH_CHECK(s->sbOrig.ensure(n));
 	pxh::SolverBodies sb{s->sbVel.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p, s->sbOrig.p};
 	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
+	const bool pgs = d.solver_type == PXH_SOLVER_PGS;
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
-	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, sb, st);
+	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, pgs, sb, st);
 	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc));
 	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnF.ensure(4 * Cc));
 	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
@@ -795,16 +797,46 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pio.curColor = s->patchColor[cur].p;
 	pio.conColor = s->conColor.p;
 	pk = s->profBegin(4);
-	pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p, sb,
-	                 invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, pio, s->counters.p,
-	                 st);
+	if (pgs)
+	{
+		PXH_CHECK(s->rnU.ensure(4 * Cc));
+		pxh::launch_prep_pgs(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p,
+		                     s->rot.p, sb, 1.0f / dt, d.static_friction, d.dynamic_friction, 2.0f * d.rest_offset,
+		                     d.friction_offset_threshold, rows, s->rnU.p, pio, s->counters.p, st);
+	}
+	else
+		pxh::launch_prep(C, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p, s->pos.p, s->rot.p,
+		                 sb, invStepDt, scale, 2.0f * d.rest_offset, d.friction_offset_threshold, rows, pio,
+		                 s->counters.p, st);
 	PXH_CHECK(hipMemcpyAsync(s->hostCounts + kHostCounterSlot, s->counters.p, 4 * sizeof(unsigned long long),
 	                         hipMemcpyDeviceToHost, st));
 	s->profEnd(pk);
 
-	// 7. solve: position iterations by colour with substep integration, then
-	//    velocity iterations (DyTGSDynamics.cpp:2515-2790)
-	for (int it = 0; it < P_ + V_; it++)
+	// 7. solve. PGS (solveV_Blocks, lowleveldynamics/src/DySolverControl.cpp:163-380):
+	//    position passes by colour, friction in the last three only (no
+	//    friction-every-iteration flag), then the velocity changes are kept
+	//    for the pose update and the velocity passes run on the concluded rows.
+	//    TGS: position iterations with substep integration, then velocity
+	//    iterations (DyTGSDynamics.cpp:2515-2790).
+	//    PGS always runs at least one velocity pass (DySolverControl.cpp:343).
+	const int pgsPasses = P_ + std::max(V_, 1);
+	for (int it = 0; pgs && it < pgsPasses; it++)
+	{
+		const bool posIter = it < P_;
+		const bool doFriction = !posIter || P_ - it <= 3;
+		for (int c = 0; c < nColors; c++)
+		{
+			if (colorStart[c + 1] == colorStart[c])
+				continue;
+			pk = s->profBegin(5);
+			pxh::launch_solve_pgs(int(colorStart[c]), int(colorStart[c + 1]), C, rows, s->rnU.p, sb, doFriction,
+			                      !posIter, it == pgsPasses - 1 ? s->patchBroken[cur].p : nullptr, st);
+			s->profEnd(pk);
+		}
+		if (it == P_ - 1)
+			pxh::launch_pgs_save_motion(n, sb, st);
+	}
+	for (int it = 0; !pgs && it < P_ + V_; it++)
 	{
 		const bool posIter = it < P_;
 		for (int c = 0; c < nColors; c++)
@@ -829,7 +861,10 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	s->patchCur = cur ^ 1;
 
 	// 8. write back poses and velocities
-	pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
+	if (pgs)
+		pxh::launch_pgs_finalize(n, dt, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
+	else
+		pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
 	PXH_CHECK(hipGetLastError());
 	s->simulating = true;
 	return PXH_OK;

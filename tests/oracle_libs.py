@@ -45,7 +45,7 @@ class PxoDesc(ctypes.Structure):
                 ("angular_damping", ctypes.c_float), ("max_lin_vel_sq", ctypes.c_float),
                 ("max_ang_vel_sq", ctypes.c_float), ("bounce_threshold", ctypes.c_float),
                 ("friction_offset_threshold", ctypes.c_float), ("correlation_distance", ctypes.c_float),
-                ("tolerance_length", ctypes.c_float)]
+                ("tolerance_length", ctypes.c_float), ("order_salt", ctypes.c_uint)]
 
 
 _pxo = None
@@ -79,6 +79,7 @@ def ref():
         lib.pxref_generate_contacts.restype = ctypes.c_int
         lib.pxref_bp_pairs.restype = ctypes.c_longlong
         lib.pxref_scene_create.restype = VP
+        lib.pxref_scene_create2.restype = VP
         lib.pxref_scene_step.restype = ctypes.c_double
         lib.pxref_dynamic_group.restype = ctypes.c_uint32
         lib.pxref_static_group.restype = ctypes.c_uint32
@@ -139,12 +140,15 @@ def default_desc(pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0)):
 
 
 class OracleWorld:
-    """The plain-C oracle's TGS world."""
+    """The plain-C oracle's world (TGS or PGS)."""
 
-    def __init__(self, scene, pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0), friction=0.5):
+    def __init__(self, scene, pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0), friction=0.5, solver="tgs",
+                 order_salt=0):
         self.lib = pxo()
         self.desc = default_desc(pos_iters, vel_iters, gravity)
         self.desc.static_friction = self.desc.dynamic_friction = friction
+        self.desc.solver = 1 if solver == "tgs" else 0
+        self.desc.order_salt = order_salt
         self.h = VP(self.lib.pxo_world_create(ctypes.byref(self.desc)))
         self.n = scene.n
         self._keep = [i32(scene.shape_type), f32(scene.dims), f32(scene.pose), f32(scene.lin_vel),
@@ -193,10 +197,11 @@ class RefScene:
     """The reference PhysX CPU PxScene (TGS or PGS) on the same arrays."""
 
     def __init__(self, scene, solver="tgs", pos_iters=4, vel_iters=1, threads=8, gravity=(0.0, -9.81, 0.0),
-                 pabp=False, friction=0.5):
+                 pabp=False, friction=0.5, report_contacts=False):
         self.lib = ref()
         g = f32(gravity)
-        self.h = VP(self.lib.pxref_scene_create(fp(g), 1 if solver == "tgs" else 0, int(pabp), int(threads)))
+        self.h = VP(self.lib.pxref_scene_create2(fp(g), 1 if solver == "tgs" else 0, int(pabp), int(threads),
+                                                 int(report_contacts)))
         assert self.h.value
         self.lib.pxref_scene_set_material(self.h, ctypes.c_float(friction), ctypes.c_float(friction),
                                           ctypes.c_float(0.0))
@@ -222,6 +227,16 @@ class RefScene:
         av = np.zeros((self.n, 3), np.float32)
         self.lib.pxref_scene_get_bodies(self.h, fp(pose), fp(lv), fp(av))
         return pose, lv, av
+
+    def contacts(self):
+        """(k, 10) contact rows of the last step: actor a, actor b (body i -> i + 1,
+        plane j -> -(j + 1)), point xyz, separation, normal xyz (from b towards
+        a), impulse magnitude."""
+        cap = 1 << 20
+        out = np.zeros((cap, 10), np.float32)
+        k = self.lib.pxref_scene_contacts(self.h, fp(out), cap)
+        assert k >= 0, "scene created without contact reports"
+        return out[: min(k, cap)].copy()
 
     def stats(self):
         out = np.zeros(4, np.int64)

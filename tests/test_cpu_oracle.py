@@ -239,3 +239,74 @@ def test_long_run_statistics_vs_live_reference(grid, seed):
     a = PS.run_stats(L.OracleWorld(sc, 4, 1), sc, 300)
     b = PS.run_stats(L.RefScene(sc, "tgs", 4, 1, threads=8), sc, 300)
     PS.assert_stats_close(a, b)
+
+
+def _settled(sc, solver, steps):
+    """State of `sc` after `steps` reference steps (bodies touching the floor
+    and each other), as a new scene."""
+    r = L.RefScene(sc, solver, 4, 1, threads=8)
+    r.step(1 / 60, steps)
+    pose, lv, av = r.bodies()
+    return S.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
+                   sc.plane_scene, 1, sc.name + "_settled")
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+@pytest.mark.parametrize("solver", ["pgs", "tgs"])
+def test_one_step_from_contact_rich_state_vs_live_reference(solver):
+    """From a contact-rich state (a box pile after 40 reference steps, handed
+    to fresh worlds on both sides so neither has friction history) one step of
+    coupled contacts depends on the Gauss-Seidel order, which the reference
+    takes from its island graph and the colouring here from constraint keys.
+    The oracle's deviation from the reference stays within the spread that
+    reordering alone produces in the oracle (same constraints, four other
+    colour orders): largest and mean deviation within 1.5 times the largest of
+    those. The constraint set is the reference's."""
+    sc = _settled(S.box_grid_drop(nx=6, ny=3, nz=6, seed=3), solver, 40)
+    worlds = [L.OracleWorld(sc, 4, 1, solver=solver, order_salt=k) for k in range(5)]
+    r = L.RefScene(sc, solver, 4, 1, threads=1)
+    for w in worlds + [r]:
+        w.step(1 / 60)
+    assert worlds[0].stats()["constraints"] == r.stats()["touching"] > sc.n
+    v = [w.bodies()[1] for w in worlds]
+    rl = r.bodies()[1]
+    scale = np.maximum(np.linalg.norm(rl, axis=1), 9.81 / 60)
+    dev_ref = np.abs(v[0] - rl).max(1) / scale
+    dev_order = [np.abs(v[0] - x).max(1) / scale for x in v[1:]]
+    assert max(d.max() for d in dev_order) > 1e-2  # the order matters for this state
+    assert dev_ref.max() <= 1.5 * max(d.max() for d in dev_order)
+    assert dev_ref.mean() <= 1.5 * max(d.mean() for d in dev_order)
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+@pytest.mark.parametrize("solver", ["pgs", "tgs"])
+def test_steps_of_separate_piles_vs_live_reference(solver):
+    """Where the order cannot matter - bodies of a sphere/box pile still
+    falling apart, each touching at most the floor - 5 steps agree with the
+    reference within 1e-4 of max(|v|, g dt)."""
+    sc = S.batched_piles(n_scenes=4, nmin=64, nmax=256, seed=21).subset(1)
+    o = L.OracleWorld(sc, 4, 1, solver=solver)
+    r = L.RefScene(sc, solver, 4, 1, threads=1)
+    o.step(1 / 60, 5)
+    r.step(1 / 60, 5)
+    _, ol, _ = o.bodies()
+    _, rl, _ = r.bodies()
+    scale = np.maximum(np.linalg.norm(rl, axis=1), 9.81 / 60)
+    assert (np.abs(ol - rl).max(1) / scale).max() < 1e-4
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+def test_pgs_long_run_statistics_vs_live_reference():
+    """The box-drop configuration under PGS 4/1 (8x4x8 grids, four seeds, 300
+    steps each): PGS leaves more penetration and single runs scatter more, so
+    the statistics are compared as means over the seeds - height and potential
+    energy within 5%, resting penetration and jitter within 15%."""
+    import pile_stats as PS
+    a, b = [], []
+    for seed in (1, 2, 3, 4):
+        sc = S.box_grid_drop(nx=8, ny=4, nz=8, seed=seed)
+        a.append(PS.run_stats(L.OracleWorld(sc, 4, 1, solver="pgs"), sc, 300))
+        b.append(PS.run_stats(L.RefScene(sc, "pgs", 4, 1, threads=8), sc, 300))
+    mean = lambda runs, k: float(np.mean([x[k] for x in runs]))  # noqa: E731
+    for k, rel in (("height", 0.05), ("potential", 0.05), ("penetration", 0.15), ("jitter", 0.15)):
+        assert abs(mean(a, k) - mean(b, k)) <= rel * abs(mean(b, k)), (k, mean(a, k), mean(b, k))

@@ -133,9 +133,10 @@ def test_broadphase_pairs_bit_exact(cuda):
     assert set(map(tuple, deleted2.tolist())) == s1 - s2
 
 
-def _run_pair(sc, steps, dt, pos=4, vel=1):
-    g = P.Scene(sc, position_iterations=pos, velocity_iterations=vel)
-    o = L.OracleWorld(sc, pos_iters=pos, vel_iters=vel)
+def _run_pair(sc, steps, dt, pos=4, vel=1, solver="tgs"):
+    g = P.Scene(sc, position_iterations=pos, velocity_iterations=vel,
+                solver_type=P.PXH_SOLVER_TGS if solver == "tgs" else P.PXH_SOLVER_PGS)
+    o = L.OracleWorld(sc, pos_iters=pos, vel_iters=vel, solver=solver)
     for _ in range(steps):
         g.step(dt)
         o.step(dt)
@@ -279,3 +280,43 @@ def test_colouring_beyond_first_batch_of_rounds(cuda):
     # sin/cos spread through the cluster at that scale: within 1e-5 of the
     # largest speed
     assert np.abs(gl - ol).max() <= 1e-5 * np.linalg.norm(ol, axis=1).max()
+
+
+def test_pgs_one_step_matches_oracle(cuda):
+    """PGS 4/1: one step of 16 worlds of piles and stacks, constraint tables
+    equal, velocities within 1e-5 of max(|v|, g dt) (the solve itself is the
+    oracle's arithmetic; sin/cos of the pose update may round apart)."""
+    sc = S.batched_piles(n_scenes=16, nmin=16, nmax=128, seed=25)
+    g, o = _run_pair(sc, 1, 1.0 / 60.0, solver="pgs")
+    key = lambda c: set(map(tuple, c.tolist()))  # noqa: E731
+    assert key(g.constraints()) == key(o.constraints())
+    _, gl, ga = g.bodies()
+    _, ol, oa = o.bodies()
+    tol = 1e-5 * np.maximum(np.linalg.norm(ol, axis=1), 9.81 / 60.0)
+    assert np.all(np.abs(gl - ol).max(1) <= tol)
+
+
+def test_pgs_multi_step_tracks_oracle(cuda):
+    sc = S.batched_piles(n_scenes=8, nmin=16, nmax=64, seed=26)
+    g, o = _run_pair(sc, 30, 1.0 / 60.0, solver="pgs")
+    gp, _, _ = g.bodies()
+    op, _, _ = o.bodies()
+    assert np.abs(gp[:, :3] - op[:, :3]).max() < 5e-3
+    assert g.stats()["constraints"] > 0
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+def test_pgs_long_run_statistics_match_reference(cuda):
+    """PGS 4/1 box drop on the GPU against the reference CPU SDK, means over
+    four seeds of 300 steps: height and potential energy within 5%, resting
+    penetration and jitter within 15%."""
+    import pile_stats as PS
+    a, b = [], []
+    for seed in (1, 2, 3, 4):
+        sc = S.box_grid_drop(nx=8, ny=4, nz=8, seed=seed)
+        a.append(PS.run_stats(P.Scene(sc, position_iterations=4, velocity_iterations=1,
+                                      solver_type=P.PXH_SOLVER_PGS), sc, 300))
+        b.append(PS.run_stats(L.RefScene(sc, "pgs", 4, 1, threads=8), sc, 300))
+    mean = lambda runs, k: float(np.mean([x[k] for x in runs]))  # noqa: E731
+    for k, rel in (("height", 0.05), ("potential", 0.05), ("penetration", 0.15), ("jitter", 0.15)):
+        assert abs(mean(a, k) - mean(b, k)) <= rel * abs(mean(b, k)), (k, mean(a, k), mean(b, k))
