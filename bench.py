@@ -34,6 +34,19 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solve_traffic_sphere_box_bin_262k.md")
+
+
+def solve_traffic():
+    """HBM bytes per k_solve launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (calibrated against a device copy),
+    or None when the summary is absent."""
+    import re
+    try:
+        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(TRAFFIC_PROFILE).read(), re.M)
+    except OSError:
+        return None
+    return int(m.group(1)) if m else None
 
 
 def log(msg):
@@ -77,6 +90,36 @@ def trace_summary(path: str, skip_steps: int) -> None:
     print("|---|---:|---:|---:|---:|")
     for k, (c, d) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
         print(f"| {k} | {c / steps:.1f} | {d / 1e6 / steps:.3f} | {d / c / 1e3:.2f} | {100 * d / busy:.1f} |")
+
+
+def pmc_summary(path: str, kernel: str, last: int) -> None:
+    """Mean counter value per dispatch of `kernel` over its last `last`
+    dispatches in a rocprofv3 counter-collection CSV (`--pmc ... --output-format
+    csv`), with the mean dispatch duration."""
+    import csv
+    rows = [r for r in csv.DictReader(open(path)) if kernel in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    rows = rows[-last:]
+    by = {}
+    for r in rows:
+        by.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
+    dur = np.mean([int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows]) / 1e3
+    for k, v in by.items():
+        print(f"{kernel}: {k} mean {np.mean(v):.1f} KiB per dispatch over {len(v)} dispatches "
+              f"(mean duration under counters {dur:.2f} us)")
+
+
+def calibrate_copy(mb: int) -> None:
+    """Device-to-device copies of a known size (for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes): 5 copies of `mb` MiB."""
+    import torch
+    n = mb * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mb} MiB 5 times")
 
 
 def run_batched(args, P, scenes):
@@ -153,9 +196,20 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="print the per-step kernel table of a rocprofv3 kernel trace and exit")
+    ap.add_argument("--calibrate-copy", type=int, metavar="MIB",
+                    help="run 5 device copies of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--pmc-summary", metavar="CSV",
+                    help="print the mean counter value per k_solve dispatch of a rocprofv3 counter CSV and exit")
+    ap.add_argument("--pmc-last", type=int, default=600, help="dispatches averaged by --pmc-summary")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(args.pmc_summary, "k_solve", args.pmc_last)
+        return
     if args.trace_summary:
         trace_summary(args.trace_summary, args.warmup)
+        return
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -209,7 +263,7 @@ def main():
     solve_ms, solve_launches = prof.get("solve", (0.0, 0))
     achieved = (solve_bytes / (solve_ms * 1e-3)) / 1e9 if solve_ms > 0 else 0.0
     roofline = {"bound": "hbm", "kernel": "k_solve", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": solve_traffic(),
                 "bytes_per_launch": round(solve_bytes / max(solve_launches, 1), 1),
                 "avg_launch_us": round(1e3 * solve_ms / max(solve_launches, 1), 3),
                 "launches": solve_launches}

@@ -1268,7 +1268,9 @@ static void push_con(pxo_world* w, int a, int b, const cset* c)
 	const uint32_t la = (uint32_t)w->local[a];
 	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b] : (0x80000000u | (uint32_t)w->plocal[-1 - b]);
 	k->key = con_key(la, lb) ^ ((uint64_t)w->d.order_salt * 0x9E3779B97F4A7C15ull);
-	k->pkey = ((uint64_t)(uint32_t)a << 32) | (b >= 0 ? (uint32_t)b : (0x80000000u | (uint32_t)(-1 - b)));
+	/* world-relative pair identity: world 16 | local a 24 | local b or 0x800000 | plane 24 */
+	k->pkey = ((uint64_t)((uint32_t)w->scene[a] & 0xFFFFu) << 48) | ((uint64_t)(la & 0xFFFFFFu) << 24) |
+	          ((lb & 0x80000000u) ? (0x800000u | (lb & 0x7FFFFFu)) : (lb & 0x7FFFFFu));
 }
 
 static int cmp_patch(const void* x, const void* y)

@@ -50,6 +50,7 @@ struct PatchIO
 	const uint32_t* prevBroken;
 	const int* prevColor;      // previous step's colour of each row
 	int* prevRow;              // per constraint: its row in the previous step, or -1
+	const uint64_t* conPKey;   // per constraint: world-relative pair key
 	const int* conColor;       // per constraint: this step's colour
 	float4* cur;               // this step's patches
 	uint64_t* curKey;          // this step's pair keys
@@ -85,9 +86,12 @@ void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, 
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);
 // slots [0, Q) are body-plane pairs, body pairs follow up to the device count *dP
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st);
+// constraint list; conPKey: world-relative pair identity (world 16 | local a 24 |
+// local b or 0x800000 | plane 24)
 void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
-                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
-                      int* conB, int* conSlot, uint64_t* conKey, int* conColor, hipStream_t st);
+                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
+                      int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor,
+                      hipStream_t st);
 // colouring over list[i] (identity when null) for i below *mDev (or m when
 // mDev is null); m bounds the count
 void launch_color_round(int m, const uint32_t* mDev, const int* list, int round, const int* conA, const int* conB,
@@ -114,8 +118,10 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
                  hipStream_t st);
 // broken: when not null, each row's friction-clamp flag of this pass is stored there
+// evStart/evStop (optional): recorded by the dispatch itself at the kernel's start and end
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st);
+                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st,
+                  hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
 // PGS: rows with the unbiased normal errors in rowsU; position passes
 // (velocityPass false) then velocity passes; SolverBodies p/dq hold the motion
 // velocities between the two
@@ -124,7 +130,8 @@ void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, 
                      float invDt, float muS, float muD, float restDist, float fot, Rows rows, float* rowsU,
                      PatchIO patches, unsigned long long* counters, hipStream_t st);
 void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, SolverBodies sb, bool doFriction,
-                      bool velocityPass, uint32_t* broken, hipStream_t st);
+                      bool velocityPass, uint32_t* broken, hipStream_t st, hipEvent_t evStart = nullptr,
+                      hipEvent_t evStop = nullptr);
 void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st);
 void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb,
                          hipStream_t st);

@@ -13,6 +13,7 @@
 //   solve (solverMultiBlockTGS.cu solveBlockUnified)        k_solve
 //   integration (integrationTGS.cu)                         k_integrate, k_finalize
 #include "pxh_kernels.h"
+#include <hip/hip_ext.h>
 #include "pxh_np.h"
 
 #include <algorithm>
@@ -604,11 +605,21 @@ __global__ void k_touch_flags(int S, int Q, const uint32_t* dP, const float4* ct
 		flags[s] = (s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
 }
 
+// Pair identity across steps, world-relative so it survives changes to other
+// worlds: world (16 bits) | local index of a (24 bits) | local index of b or
+// 0x800000 | plane index (24 bits).
+PXH_D uint64_t worldPairKey(int w, uint32_t la, uint32_t lb)
+{
+	const uint32_t b24 = (lb & 0x80000000u) ? (0x800000u | (lb & 0x7FFFFFu)) : (lb & 0x7FFFFFu);
+	return (uint64_t(uint32_t(w) & 0xFFFFu) << 48) | (uint64_t(la & 0xFFFFFFu) << 24) | b24;
+}
+
 // constraint list: touching pair slots in slot order, with the scene-local
 // ordering key used by the colouring
 __global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
                             const uint2* pairs, const int* qBody, const int* qPlane, const int* local,
-                            const int* planeLocal, int* conA, int* conB, int* conSlot, uint64_t* conKey, int* conColor)
+                            const int* planeLocal, const int* world, int* conA, int* conB, int* conSlot,
+                            uint64_t* conKey, uint64_t* conPKey, int* conColor)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= S || !((s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0))
@@ -634,6 +645,7 @@ __global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN,
 	conB[c] = b;
 	conSlot[c] = s;
 	conKey[c] = mix64((uint64_t(la) << 32) | lb);
+	conPKey[c] = worldPairKey(world[a], la, lb);
 	conColor[c] = b >= 0 ? -1 : -2;  // -2: body-plane, placed after the body-body rounds
 }
 
@@ -803,10 +815,6 @@ PXH_D int patchFind(const PatchIO& io, uint64_t key)
 	return -1;
 }
 
-PXH_D uint64_t pairKey(int a, int b)
-{
-	return (uint64_t(uint32_t(a)) << 32) | (b >= 0 ? uint32_t(b) : (0x80000000u | uint32_t(-1 - b)));
-}
 
 // A body-body constraint that existed in the previous step keeps its colour
 // (below 64): those constraints were coloured together then, so they never
@@ -819,7 +827,7 @@ __global__ void k_con_prev(int C, const int* conA, const int* conB, PatchIO io, 
 	if (c >= C)
 		return;
 	const int a = conA[c], b = conB[c];
-	const int po = patchFind(io, pairKey(a, b));
+	const int po = patchFind(io, io.conPKey[c]);
 	io.prevRow[c] = po;
 	if (b >= 0 && po >= 0)
 	{
@@ -953,7 +961,7 @@ PXH_D int frictionPatch(int o, int c, int a, int b, f3 n, const xf& f0, const xf
 	// friction patch: the previous step's patch of this pair is kept when it is
 	// unbroken, has anchors, its body normals still agree, no anchor drifted
 	// apart along the normal, and its world normal matches this step's normal
-	const uint64_t pkey = pairKey(a, b);
+	const uint64_t pkey = io.conPKey[c];
 	f3 pn0, pn1;
 	int nKept = 0;
 	{
@@ -1949,11 +1957,12 @@ void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uin
 	PXH_LAUNCH(k_touch_flags, S, S, Q, dP, ctN, flags);
 }
 void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
-                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, int* conA,
-                      int* conB, int* conSlot, uint64_t* conKey, int* conColor, hipStream_t st)
+                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
+                      int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor,
+                      hipStream_t st)
 {
-	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, conA, conB, conSlot,
-	           conKey, conColor);
+	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, world, conA, conB,
+	           conSlot, conKey, conPKey, conColor);
 }
 // colouring launches are grid-stride over at most 1024 blocks
 static int colorBlocks(int m) { return std::max(1, std::min(nblk(m, 256), 1024)); }
@@ -2023,11 +2032,18 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
 	           invStepDt, scale, restDist, fot, rows, patches, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
-                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st)
+                  float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st, hipEvent_t evStart,
+                  hipEvent_t evStop)
 {
-	if (end > begin)
-		k_solve<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(begin, end, C, rows, sb.vel, minPen, muS, muD, invStepDt,
-		                                                                     biasScale, scale, broken);
+	if (end <= begin)
+		return;
+	const dim3 grid(nblk(end - begin, SOLVE_THREADS)), block(SOLVE_THREADS);
+	if (evStart)
+		hipExtLaunchKernelGGL(k_solve, grid, block, 0, st, evStart, evStop, 0, begin, end, C, rows, sb.vel, minPen,
+		                      muS, muD, invStepDt, biasScale, scale, broken);
+	else
+		k_solve<<<grid, block, 0, st>>>(begin, end, C, rows, sb.vel, minPen, muS, muD, invStepDt, biasScale, scale,
+		                                broken);
 }
 void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                      const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb,
@@ -2038,11 +2054,17 @@ void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, 
 	           muD, restDist, fot, rows, rowsU, patches, counters);
 }
 void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, SolverBodies sb, bool doFriction,
-                      bool velocityPass, uint32_t* broken, hipStream_t st)
+                      bool velocityPass, uint32_t* broken, hipStream_t st, hipEvent_t evStart, hipEvent_t evStop)
 {
-	if (end > begin)
-		k_solve_pgs<<<nblk(end - begin, SOLVE_THREADS), SOLVE_THREADS, 0, st>>>(
-		    begin, end, C, rows, rowsU, sb.vel, doFriction ? 1 : 0, velocityPass ? 1 : 0, broken);
+	if (end <= begin)
+		return;
+	const dim3 grid(nblk(end - begin, SOLVE_THREADS)), block(SOLVE_THREADS);
+	if (evStart)
+		hipExtLaunchKernelGGL(k_solve_pgs, grid, block, 0, st, evStart, evStop, 0, begin, end, C, rows, rowsU, sb.vel,
+		                      doFriction ? 1 : 0, velocityPass ? 1 : 0, broken);
+	else
+		k_solve_pgs<<<grid, block, 0, st>>>(begin, end, C, rows, rowsU, sb.vel, doFriction ? 1 : 0,
+		                                    velocityPass ? 1 : 0, broken);
 }
 void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st)
 {

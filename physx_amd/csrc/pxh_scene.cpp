@@ -130,7 +130,7 @@ struct pxh_scene_t
 	DBuf<uint2> pairs;
 	DBuf<float4> ctN, ctP;
 	DBuf<int> conA, conB, conSlot, conColor, order;
-	DBuf<uint64_t> conKey, best, mask;
+	DBuf<uint64_t> conKey, conPKey, best, mask;
 	DBuf<int> colList, colList2;
 	DBuf<uint32_t> colFlags;
 	DBuf<uint32_t> colorCount, scratch;
@@ -187,6 +187,25 @@ struct pxh_scene_t
 	{
 		if (k >= 0)
 			(void)hipEventRecord(prof[k].b, stream);
+	}
+	// a pair whose events the launch itself records at the kernel's start and
+	// end (hipExtLaunchKernelGGL), so the time is the kernel's own
+	int profKernel(int stage, hipEvent_t* a, hipEvent_t* b)
+	{
+		*a = *b = nullptr;
+		if (!profiling)
+			return -1;
+		if (profUsed == prof.size())
+		{
+			ProfPair p{};
+			if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess)
+				return -1;
+			prof.push_back(p);
+		}
+		prof[profUsed].stage = stage;
+		*a = prof[profUsed].a;
+		*b = prof[profUsed].b;
+		return int(profUsed++);
 	}
 	void profCollect()
 	{
@@ -422,6 +441,7 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->keys.release();
 	s->keys2.release();
 	s->conKey.release();
+	s->conPKey.release();
 	s->best.release();
 	s->mask.release();
 	s->colList.release();
@@ -634,8 +654,10 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		PXH_CHECK(s->conSlot.ensure(Sc));
 		PXH_CHECK(s->conColor.ensure(Sc));
 		PXH_CHECK(s->conKey.ensure(Sc));
+		PXH_CHECK(s->conPKey.ensure(Sc));
 		pxh::launch_con_build(S, Q, dP, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
-		                      s->planeLocal.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p, s->conColor.p, st);
+		                      s->planeLocal.p, s->scene.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p,
+		                      s->conPKey.p, s->conColor.p, st);
 		const uint32_t* src[2] = {dP, s->flags.p + S};
 		r = readCounts(st, s->hostCounts, src, 2);
 		if (r)
@@ -666,6 +688,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
 	pxh::PatchIO pio{};
 	pio.corrDist = d.friction_correlation_distance;
+	pio.conPKey = s->conPKey.p;
 	if (s->nPrevRows > 0)
 	{
 		uint32_t size = 64;
@@ -828,10 +851,10 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		{
 			if (colorStart[c + 1] == colorStart[c])
 				continue;
-			pk = s->profBegin(5);
+			hipEvent_t e0, e1;
+			s->profKernel(5, &e0, &e1);
 			pxh::launch_solve_pgs(int(colorStart[c]), int(colorStart[c + 1]), C, rows, s->rnU.p, sb, doFriction,
-			                      !posIter, it == pgsPasses - 1 ? s->patchBroken[cur].p : nullptr, st);
-			s->profEnd(pk);
+			                      !posIter, it == pgsPasses - 1 ? s->patchBroken[cur].p : nullptr, st, e0, e1);
 		}
 		if (it == P_ - 1)
 			pxh::launch_pgs_save_motion(n, sb, st);
@@ -843,11 +866,12 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		{
 			if (colorStart[c + 1] == colorStart[c])
 				continue;
-			pk = s->profBegin(5);
+			hipEvent_t e0, e1;
+			s->profKernel(5, &e0, &e1);
 			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
 			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, invStepDt,
-			                  invStepDt * scale, scale, it == P_ + V_ - 1 ? s->patchBroken[cur].p : nullptr, st);
-			s->profEnd(pk);
+			                  invStepDt * scale, scale, it == P_ + V_ - 1 ? s->patchBroken[cur].p : nullptr, st, e0,
+			                  e1);
 		}
 		if (posIter)
 		{
@@ -887,7 +911,10 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 	else
 		PXH_CHECK(hipStreamSynchronize(s->stream));
 	s->simulating = false;
-	s->profCollect();
+	// event pairs are read when the profile is asked for (reading each pair
+	// costs host time between steps); the pool is drained past 64k pairs
+	if (s->profUsed > 65536)
+		s->profCollect();
 	if (s->statConstraints > 0)
 	{
 		unsigned long long c3[4];
@@ -906,6 +933,7 @@ int32_t pxh_scene_set_profiling(pxh_scene s, int32_t enable)
 		return fail(PXH_ERR_INVALID, "pxh_scene_set_profiling: null scene");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_set_profiling: not allowed while simulating");
+	s->profCollect();
 	s->profiling = enable != 0;
 	for (int i = 0; i < 8; i++)
 	{
@@ -919,6 +947,9 @@ int32_t pxh_scene_get_profile(pxh_scene s, double* ms, int64_t* launches, int32_
 {
 	if (!s || n < 0)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_profile: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_profile: not allowed while simulating");
+	s->profCollect();
 	const int k = std::min(n, 8);
 	for (int i = 0; i < k; i++)
 	{
