@@ -18,6 +18,12 @@ bodies), TGS 4/1, dt = 1/120 s, stepped as one scene (one launch sequence per
 step for all worlds), with its body-steps/s and p50/p99 step latency, beside
 the plain per-scene path (each world its own scene, stepped one after the
 other with the same kernels) measured on an evenly spaced sample of the worlds.
+
+"served" is the request stream: Poisson arrivals at several rates of step
+requests for scenes of log-uniform [16, 4096] bodies, each for k ~ U{1..32}
+steps at dt = 1/60, admitted against a 200 GB HBM budget into one batched
+step per serving iteration, with body-steps/s, serving-step and request
+latency percentiles.
 """
 from __future__ import annotations
 
@@ -179,6 +185,77 @@ def run_batched(args, P, scenes):
     return out
 
 
+def run_served(args, P, scenes):
+    """Served request stream: Poisson arrivals (rate lam per second, wall
+    clock) of step requests for scenes of log-uniform [16, 4096] bodies, each
+    asking for k ~ U{1..32} steps at dt = 1/60, admitted against an HBM budget
+    and merged into one batched step per serving iteration."""
+    import torch
+    dt = 1.0 / 60.0
+    pool_sc = scenes.batched_piles(n_scenes=args.serve_pool, nmin=16, nmax=4096, seed=11)
+    pool = [pool_sc.subset(w) for w in range(pool_sc.n_scenes)]
+    log(f"served: pool of {len(pool)} scenes, {pool_sc.n} bodies")
+    results = []
+    for lam in args.serve_rates:
+        rng = np.random.default_rng(5)
+        sv = P.Server(hbm_budget_bytes=args.serve_budget_gb << 30, max_worlds=65536, bytes_per_body=8192,
+                      position_iterations=4, velocity_iterations=1)
+        t0 = time.perf_counter()
+        t_warm, t_end = t0 + args.serve_warmup_s, t0 + args.serve_warmup_s + args.serve_seconds
+        next_arrival = t0 + rng.exponential(1.0 / lam)
+        submitted = {}
+        step_ms, lat_ms, wait_ms, bodies_active = [], [], [], []
+        bs0 = None
+        n_req = 0
+        while True:
+            now = time.perf_counter()
+            if now >= t_end:
+                break
+            while next_arrival <= now:
+                x = pool[int(rng.integers(len(pool)))]
+                k = int(rng.integers(1, 33))
+                rid = sv.submit(x, k)
+                submitted[rid] = next_arrival
+                n_req += 1
+                next_arrival += rng.exponential(1.0 / lam)
+            st = sv.stats()
+            if st["active"] == 0 and st["queued"] == 0:
+                time.sleep(max(0.0, min(next_arrival, t_end) - time.perf_counter()))
+                continue
+            if bs0 is None and now >= t_warm:
+                bs0, tb0 = st["body_steps"], now
+            a = time.perf_counter()
+            sv.step(dt)
+            b = time.perf_counter()
+            if bs0 is not None:
+                step_ms.append((b - a) * 1e3)
+                bodies_active.append(st["active_bodies"])
+            for rid in sv.finished():
+                _, _, _, tm = sv.result(rid)
+                t_arr = submitted.pop(rid)
+                if t_arr >= t_warm:
+                    lat_ms.append((tm[2] - t_arr) * 1e3)
+                    wait_ms.append((tm[1] - t_arr) * 1e3)
+        torch.cuda.synchronize()
+        st = sv.stats()
+        elapsed = time.perf_counter() - tb0 if bs0 is not None else 1.0
+        rate = (st["body_steps"] - (bs0 or 0)) / elapsed
+        sv.close()
+        sm, lm, wm = np.array(step_ms or [0.0]), np.array(lat_ms or [0.0]), np.array(wait_ms or [0.0])
+        r = {"lambda_per_s": lam, "requests": n_req, "timed_s": round(elapsed, 2), "body_steps_per_s": round(rate, 1),
+             "serving_steps": len(step_ms), "mean_active_bodies": round(float(np.mean(bodies_active or [0])), 1),
+             "p50_step_ms": round(float(np.percentile(sm, 50)), 3), "p99_step_ms": round(float(np.percentile(sm, 99)), 3),
+             "p50_request_ms": round(float(np.percentile(lm, 50)), 3),
+             "p99_request_ms": round(float(np.percentile(lm, 99)), 3),
+             "p99_admission_wait_ms": round(float(np.percentile(wm, 99)), 3), "completed_timed": len(lat_ms)}
+        log(f"served: {r}")
+        results.append(r)
+    return {"workload": f"Poisson step requests, scenes of log-uniform [16,4096] bodies (pool of {len(pool)} seeded "
+                        f"box stacks / sphere-box piles), k ~ U{{1..32}} steps at dt 1/60, TGS 4/1, HBM budget "
+                        f"{args.serve_budget_gb} GB at 8 KiB per body, one batched step per serving iteration",
+            "warmup_s": args.serve_warmup_s, "rates": results}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -194,6 +271,12 @@ def main():
     ap.add_argument("--batched-warmup", type=int, default=120)
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[250.0, 1000.0, 4000.0])
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-warmup-s", type=float, default=1.0)
+    ap.add_argument("--serve-pool", type=int, default=256)
+    ap.add_argument("--serve-budget-gb", type=int, default=200)
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="print the per-step kernel table of a rocprofv3 kernel trace and exit")
     ap.add_argument("--calibrate-copy", type=int, metavar="MIB",
@@ -290,6 +373,7 @@ def main():
 
     gpu.close()
     batched = None if args.no_batched else run_batched(args, P, scenes)
+    served = None if args.no_served else run_served(args, P, scenes)
 
     sms = np.array(step_ms)
     out = {
@@ -316,6 +400,7 @@ def main():
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "batched": batched,
+        "served": served,
     }
     print(json.dumps(out), flush=True)
 

@@ -129,6 +129,47 @@ int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, in
 /* The scene's HIP stream (hipStream_t), for ordering caller work. */
 void* pxh_scene_get_stream(pxh_scene scene);
 
+/* Removes whole worlds (every body and plane whose world id is listed in
+ * worlds[k], host array) between steps: PxScene::removeActors
+ * (physx/include/PxScene.h:437, NpScene.cpp:752) applied to all actors of
+ * those worlds. The remaining worlds keep their order and are renumbered
+ * 0..W-k-1; their bodies are compacted, so body indices change. Friction
+ * patches and colours of the remaining worlds carry over unchanged. */
+int32_t pxh_scene_remove_worlds(pxh_scene scene, const int32_t* worlds, int32_t k);
+
+/* Number of worlds, and the body range [first, first + count) of one world
+ * (PxScene::getNbActors / getActors, PxScene.h:497). */
+int32_t pxh_scene_world_count(pxh_scene scene);
+int32_t pxh_scene_world_range(pxh_scene scene, int32_t world, int32_t* first, int32_t* count);
+
+/* Serving loop over one batched scene (no counterpart in the reference, which
+ * steps one scene per simulate call): independent step requests are queued,
+ * admitted as worlds while max_worlds and the HBM budget (bytes_per_body per
+ * body) allow, stepped together, and retired when their steps are done. */
+typedef struct pxh_server_t* pxh_server;
+int32_t pxh_server_create(const pxh_scene_desc* desc, int64_t hbm_budget_bytes, int32_t max_worlds,
+                          int64_t bytes_per_body, pxh_server* out_server);
+int32_t pxh_server_release(pxh_server server);
+const char* pxh_server_last_error(void);
+/* Queues one scene (arrays as pxh_scene_add_bodies / add_planes; lin_vel and
+ * ang_vel may be NULL) to be stepped `steps` times; returns its id. */
+int32_t pxh_server_submit(pxh_server server, int32_t n, const int32_t* type, const float* dims, const float* pose,
+                          const float* lin_vel, const float* ang_vel, const float* mass, const float* inertia,
+                          int32_t m, const float* planes, int32_t steps, int64_t* out_id);
+/* Admits queued scenes in arrival order, steps all admitted scenes once by
+ * dt, retires the finished ones. */
+int32_t pxh_server_step(pxh_server server, float dt, int32_t* admitted, int32_t* completed);
+/* A finished scene's bodies (host arrays pose[n*7] p xyz q xyzw, lin_vel[n*3],
+ * ang_vel[n*3]) and times[3] = submit, admit, finish (seconds, steady clock).
+ * Each result is read once. */
+int32_t pxh_server_result(pxh_server server, int64_t id, float* pose, float* lin_vel, float* ang_vel,
+                          double* times);
+/* Ids of finished scenes not yet read (ascending, at most max_ids); returns the count. */
+int32_t pxh_server_finished(pxh_server server, int64_t* ids, int32_t max_ids);
+/* out: queued scenes, admitted scenes, admitted bodies, admitted HBM bytes,
+ * finished scenes, serving steps, body-steps. Returns the count written. */
+int32_t pxh_server_stats(pxh_server server, int64_t* out, int32_t n);
+
 /* Standalone broadphase: PxCreateBroadPhase (physx/include/PxBroadPhase.h:629),
  * PxBroadPhase::update (:576) and fetchResults (:587). Bounds are the
  * non-inflated PxBounds3 (n*6 floats), inflated by distances[n]; pairs whose

@@ -50,7 +50,9 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_get_rigid_dynamic_data", "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats",
             "pxh_scene_get_constraints", "pxh_scene_get_stream", "pxh_bp_create", "pxh_bp_update",
             "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts", "pxh_scene_set_profiling",
-            "pxh_scene_get_profile")
+            "pxh_scene_get_profile", "pxh_scene_remove_worlds", "pxh_scene_world_count", "pxh_scene_world_range",
+            "pxh_server_create", "pxh_server_release", "pxh_server_last_error", "pxh_server_submit",
+            "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats")
 
 _lib = None
 
@@ -70,6 +72,7 @@ def load():
                        "(make -C physx_amd/csrc)")
     lib = ctypes.CDLL(LIB_PATH)
     lib.pxh_last_error.restype = ctypes.c_char_p
+    lib.pxh_server_last_error.restype = ctypes.c_char_p
     lib.pxh_scene_get_stream.restype = _VP
     lib.pxh_scene_get_constraints.restype = ctypes.c_int64
     for name in ("pxh_scene_create", "pxh_scene_release", "pxh_scene_add_bodies", "pxh_scene_add_planes",
@@ -208,6 +211,21 @@ class Scene:
                                               cnt.ctypes.data_as(_I64), 8))
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(self.STAGES) if cnt[i] > 0}
 
+    def remove_worlds(self, worlds):
+        """PxScene::removeActors for every actor of the listed worlds; the
+        remaining worlds are renumbered in order and their bodies compacted."""
+        w = _i32(worlds)
+        _check(self.lib.pxh_scene_remove_worlds(self.h, w.ctypes.data_as(_I32), int(w.size)))
+        self.n = sum(self.world_range(i)[1] for i in range(self.world_count()))
+
+    def world_count(self) -> int:
+        return _check(self.lib.pxh_scene_world_count(self.h))
+
+    def world_range(self, world: int):
+        first, count = ctypes.c_int32(), ctypes.c_int32()
+        _check(self.lib.pxh_scene_world_range(self.h, world, ctypes.byref(first), ctypes.byref(count)))
+        return first.value, count.value
+
     def constraints(self) -> np.ndarray:
         cap = 1 << 23
         out = np.zeros((cap, 4), np.int32)
@@ -218,6 +236,82 @@ class Scene:
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
             self.lib.pxh_scene_release(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Serving loop over one batched scene (pxh_server_*): submitted scenes are
+    admitted as worlds while max_worlds and the HBM budget allow, stepped
+    together, and retired when their requested steps are done."""
+
+    STATS = ("queued", "active", "active_bodies", "active_bytes", "completed", "steps", "body_steps")
+
+    def __init__(self, hbm_budget_bytes: int, max_worlds: int, bytes_per_body: int = 8192,
+                 desc: SceneDesc | None = None, **kw):
+        self.lib = load()
+        self.desc = desc if desc is not None else default_desc(**kw)
+        h = ctypes.c_void_p()
+        self._check(self.lib.pxh_server_create(ctypes.byref(self.desc), ctypes.c_int64(hbm_budget_bytes), max_worlds,
+                                               ctypes.c_int64(bytes_per_body), ctypes.byref(h)))
+        self.h = h
+        self.sizes = {}
+
+    def _check(self, rc):
+        if rc < 0:
+            raise PxhError(f"pxhip server error {rc}: {self.lib.pxh_server_last_error().decode()}")
+        return rc
+
+    def submit(self, scene, steps: int) -> int:
+        """Queue one scene (a scenes.SceneArrays of a single world) for `steps` steps."""
+        t, dm, po = _i32(scene.shape_type), _f32(scene.dims), _f32(scene.pose)
+        lv, av, m, inn = _f32(scene.lin_vel), _f32(scene.ang_vel), _f32(scene.mass), _f32(scene.inertia)
+        pl = _f32(scene.planes).reshape(-1, 4)
+        fp = lambda a: a.ctypes.data_as(_F)  # noqa: E731
+        rid = ctypes.c_int64()
+        self._check(self.lib.pxh_server_submit(self.h, scene.n, t.ctypes.data_as(_I32), fp(dm), fp(po), fp(lv), fp(av),
+                                               fp(m), fp(inn), int(pl.shape[0]), fp(pl), steps, ctypes.byref(rid)))
+        self.sizes[rid.value] = scene.n
+        return rid.value
+
+    def step(self, dt: float):
+        """One serving step; returns (admitted, completed)."""
+        a, c = ctypes.c_int32(), ctypes.c_int32()
+        self._check(self.lib.pxh_server_step(self.h, ctypes.c_float(dt), ctypes.byref(a), ctypes.byref(c)))
+        return a.value, c.value
+
+    def result(self, rid: int):
+        """(pose (n,7) p xyz + q xyzw, lin (n,3), ang (n,3), times (submit, admit, finish))."""
+        n = self.sizes[rid]
+        pose = np.zeros((n, 7), np.float32)
+        lv = np.zeros((n, 3), np.float32)
+        av = np.zeros((n, 3), np.float32)
+        tm = np.zeros(3, np.float64)
+        self._check(self.lib.pxh_server_result(self.h, ctypes.c_int64(rid), pose.ctypes.data_as(_F),
+                                               lv.ctypes.data_as(_F), av.ctypes.data_as(_F),
+                                               tm.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        del self.sizes[rid]
+        return pose, lv, av, tm
+
+    def finished(self):
+        """Ids of finished scenes whose results are not read yet."""
+        ids = np.zeros(max(1, len(self.sizes)), np.int64)
+        k = self._check(self.lib.pxh_server_finished(self.h, ids.ctypes.data_as(_I64), int(ids.size)))
+        return [int(i) for i in ids[:k]]
+
+    def stats(self) -> dict:
+        out = np.zeros(len(self.STATS), np.int64)
+        self._check(self.lib.pxh_server_stats(self.h, out.ctypes.data_as(_I64), len(self.STATS)))
+        return dict(zip(self.STATS, map(int, out)))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pxh_server_release(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

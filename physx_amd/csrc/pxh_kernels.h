@@ -86,12 +86,14 @@ void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, 
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);
 // slots [0, Q) are body-plane pairs, body pairs follow up to the device count *dP
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st);
-// constraint list; conPKey: world-relative pair identity (world 16 | local a 24 |
-// local b or 0x800000 | plane 24)
+// constraint list; conPKey: world-relative pair identity (world tag 16 | local a
+// 24 | local b or 0x800000 | plane 24)
 void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
                       const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
-                      int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor,
-                      hipStream_t st);
+                      const int* worldTag, int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey,
+                      int* conColor, hipStream_t st);
+// out[i] = in[idx[i]], elements of 4 or 16 bytes
+void launch_gather(int n, const int* idx, const void* in, void* out, int bytes, hipStream_t st);
 // colouring over list[i] (identity when null) for i below *mDev (or m when
 // mDev is null); m bounds the count
 void launch_color_round(int m, const uint32_t* mDev, const int* list, int round, const int* conA, const int* conB,

@@ -618,8 +618,8 @@ PXH_D uint64_t worldPairKey(int w, uint32_t la, uint32_t lb)
 // ordering key used by the colouring
 __global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
                             const uint2* pairs, const int* qBody, const int* qPlane, const int* local,
-                            const int* planeLocal, const int* world, int* conA, int* conB, int* conSlot,
-                            uint64_t* conKey, uint64_t* conPKey, int* conColor)
+                            const int* planeLocal, const int* world, const int* worldTag, int* conA, int* conB,
+                            int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= S || !((s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0))
@@ -645,7 +645,7 @@ __global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN,
 	conB[c] = b;
 	conSlot[c] = s;
 	conKey[c] = mix64((uint64_t(la) << 32) | lb);
-	conPKey[c] = worldPairKey(world[a], la, lb);
+	conPKey[c] = worldPairKey(worldTag[world[a]], la, lb);
 	conColor[c] = b >= 0 ? -1 : -2;  // -2: body-plane, placed after the body-body rounds
 }
 
@@ -1958,11 +1958,31 @@ void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uin
 }
 void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
                       const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
-                      int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor,
-                      hipStream_t st)
+                      const int* worldTag, int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey,
+                      int* conColor, hipStream_t st)
 {
-	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, world, conA, conB,
-	           conSlot, conKey, conPKey, conColor);
+	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, world, worldTag, conA,
+	           conB, conSlot, conKey, conPKey, conColor);
+}
+// out[i] = in[idx[i]] for elements of `bytes` (4 or 16) bytes
+__global__ void k_gather4(int n, const int* idx, const uint32_t* in, uint32_t* out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = in[idx[i]];
+}
+__global__ void k_gather16(int n, const int* idx, const float4* in, float4* out)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		out[i] = in[idx[i]];
+}
+void launch_gather(int n, const int* idx, const void* in, void* out, int bytes, hipStream_t st)
+{
+	if (bytes == 16)
+		PXH_LAUNCH(k_gather16, n, n, idx, static_cast<const float4*>(in), static_cast<float4*>(out));
+	else
+		PXH_LAUNCH(k_gather4, n, n, idx, static_cast<const uint32_t*>(in), static_cast<uint32_t*>(out));
 }
 // colouring launches are grid-stride over at most 1024 blocks
 static int colorBlocks(int m) { return std::max(1, std::min(nblk(m, 256), 1024)); }

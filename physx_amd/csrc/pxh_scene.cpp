@@ -43,6 +43,21 @@ struct DBuf
 {
 	T* p = nullptr;
 	size_t cap = 0;
+	DBuf() = default;
+	DBuf(const DBuf&) = delete;
+	DBuf& operator=(const DBuf&) = delete;
+	DBuf(DBuf&& o) noexcept : p(o.p), cap(o.cap)
+	{
+		o.p = nullptr;
+		o.cap = 0;
+	}
+	DBuf& operator=(DBuf&& o) noexcept
+	{
+		std::swap(p, o.p);
+		std::swap(cap, o.cap);
+		return *this;
+	}
+	~DBuf() { release(); }
 	hipError_t ensure(size_t n)
 	{
 		if (n <= cap && p)
@@ -117,6 +132,11 @@ struct pxh_scene_t
 	DBuf<int> planeLocal;
 	std::vector<int> hPlaneScene;
 	std::vector<std::vector<int>> worldPlanes;
+	// each world's tag: its identity in the pair keys that carry friction
+	// patches and colours between steps; unchanged when other worlds are removed
+	std::vector<int> hWorldTag;
+	int nextTag = 0;
+	DBuf<int> worldTag;
 	// body-plane pair table
 	int Q = 0;
 	DBuf<int> qBody, qPlane;
@@ -159,6 +179,8 @@ struct pxh_scene_t
 	uint32_t totalRows = 0;               // broadphase row table size (rowOff on the device)
 	int maxPlanes = 0;                    // most planes in one world
 	DBuf<uint32_t> colCounts;             // colouring compaction offsets
+	DBuf<unsigned char> gatherTmp;        // world removal: gather scratch and indices
+	DBuf<int> gatherIdx;
 	uint32_t* hostCounts = nullptr;       // pinned
 
 	// optional per-stage GPU timing with HIP events on the scene stream
@@ -243,8 +265,20 @@ struct pxh_broadphase_t
 
 namespace {
 
+int worldCount(const pxh_scene_t* s)
+{
+	return std::max(int(s->hBodyStart.size()) - 1, int(s->worldPlanes.size()));
+}
+
 int rebuild_topology(pxh_scene_t* s)
 {
+	// tags of new worlds, in creation order
+	while (int(s->hWorldTag.size()) < worldCount(s))
+		s->hWorldTag.push_back(s->nextTag++);
+	PXH_CHECK(s->worldTag.ensure(std::max<size_t>(s->hWorldTag.size(), 1)));
+	if (!s->hWorldTag.empty())
+		PXH_CHECK(hipMemcpy(s->worldTag.p, s->hWorldTag.data(), s->hWorldTag.size() * sizeof(int),
+		                    hipMemcpyHostToDevice));
 	// body-plane pairs: every body against the planes of its world
 	std::vector<int> qb, qp;
 	for (int i = 0; i < s->n; i++)
@@ -363,6 +397,16 @@ int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_grou
 	return bp_write_pairs(s, n, d_scene, d_group, d_rowOff, st, pairCap);
 }
 
+// b[i] = b[idx[i]] for i < n, through a scratch buffer that stays with the scene
+template <typename T>
+int gatherBuf(DBuf<T>& b, DBuf<unsigned char>& tmp, const int* idx, int n, hipStream_t st)
+{
+	PXH_CHECK(tmp.ensure(sizeof(T) * size_t(std::max(n, 1))));
+	pxh::launch_gather(n, idx, b.p, tmp.p, int(sizeof(T)), st);
+	PXH_CHECK(hipMemcpyAsync(b.p, tmp.p, sizeof(T) * size_t(n), hipMemcpyDeviceToDevice, st));
+	return PXH_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -464,6 +508,7 @@ int32_t pxh_scene_release(pxh_scene s)
 	s->htKey.release();
 	s->htVal.release();
 	s->bodyNext.release();
+	s->worldTag.release();
 	if (s->hostCounts)
 		(void)hipHostFree(s->hostCounts);
 	if (s->stream)
@@ -580,6 +625,114 @@ int32_t pxh_scene_add_planes(pxh_scene s, int32_t m, const float* plane, const i
 	return PXH_OK;
 }
 
+int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
+{
+	if (!s || k < 0 || (k > 0 && !worlds))
+		return fail(PXH_ERR_INVALID, "pxh_scene_remove_worlds: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_remove_worlds: not allowed while simulating");
+	const int nW = worldCount(s);
+	while (int(s->hWorldTag.size()) < nW)
+		s->hWorldTag.push_back(s->nextTag++);
+	std::vector<char> drop(size_t(nW), 0);
+	for (int i = 0; i < k; i++)
+	{
+		if (worlds[i] < 0 || worlds[i] >= nW)
+			return fail(PXH_ERR_INVALID, "pxh_scene_remove_worlds: world id out of range");
+		drop[worlds[i]] = 1;
+	}
+	std::vector<int> newIdx(size_t(nW), -1), tags;
+	int nw = 0;
+	for (int w = 0; w < nW; w++)
+		if (!drop[w])
+		{
+			newIdx[w] = nw++;
+			tags.push_back(s->hWorldTag[w]);
+		}
+	// bodies of the kept worlds, in order
+	std::vector<int> keep, hScene, hStart(1, 0), newWorldOfBody;
+	const int nBW = int(s->hBodyStart.size()) - 1;
+	for (int w = 0; w < nW; w++)
+	{
+		if (drop[w])
+			continue;
+		if (w < nBW)
+			for (int i = s->hBodyStart[w]; i < s->hBodyStart[w + 1]; i++)
+			{
+				keep.push_back(i);
+				hScene.push_back(newIdx[w]);
+			}
+		hStart.push_back(int(keep.size()));
+	}
+	// planes of the kept worlds
+	std::vector<int> keepPl, hPlaneScene;
+	std::vector<std::vector<int>> worldPlanes(static_cast<size_t>(nw));
+	for (int p = 0; p < s->m; p++)
+	{
+		const int w = s->hPlaneScene[p];
+		if (drop[w])
+			continue;
+		worldPlanes[newIdx[w]].push_back(int(keepPl.size()));
+		hPlaneScene.push_back(newIdx[w]);
+		keepPl.push_back(p);
+	}
+	hipStream_t st = s->stream;
+	PXH_CHECK(hipStreamSynchronize(st));
+	const int nn = int(keep.size()), mm = int(keepPl.size());
+	DBuf<int>& idx = s->gatherIdx;
+	PXH_CHECK(idx.ensure(std::max<size_t>(keep.size() + keepPl.size(), 1)));
+	if (nn > 0)
+	{
+		PXH_CHECK(hipMemcpy(idx.p, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice));
+		DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI};
+		for (auto* b : f4s)
+			if (int r = gatherBuf(*b, s->gatherTmp, idx.p, nn, st))
+				return r;
+		if (int r = gatherBuf(s->local, s->gatherTmp, idx.p, nn, st))
+			return r;
+		PXH_CHECK(s->scene.ensure(size_t(nn)));
+		PXH_CHECK(hipMemcpyAsync(s->scene.p, hScene.data(), hScene.size() * sizeof(int), hipMemcpyHostToDevice, st));
+	}
+	if (mm > 0)
+	{
+		int* pidx = idx.p + keep.size();
+		PXH_CHECK(hipMemcpy(pidx, keepPl.data(), keepPl.size() * sizeof(int), hipMemcpyHostToDevice));
+		if (int r = gatherBuf(s->planes, s->gatherTmp, pidx, mm, st))
+			return r;
+		if (int r = gatherBuf(s->planeLocal, s->gatherTmp, pidx, mm, st))
+			return r;
+	}
+	PXH_CHECK(hipStreamSynchronize(st));
+	s->n = nn;
+	s->m = mm;
+	s->hScene.swap(hScene);
+	s->hBodyStart.swap(hStart);
+	s->hPlaneScene.swap(hPlaneScene);
+	s->worldPlanes.swap(worldPlanes);
+	s->hWorldTag.swap(tags);
+	s->topologyDirty = true;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_world_range(pxh_scene s, int32_t world, int32_t* first, int32_t* count)
+{
+	if (!s || world < 0 || world >= worldCount(s))
+		return fail(PXH_ERR_INVALID, "pxh_scene_world_range: bad arguments");
+	const int nBW = int(s->hBodyStart.size()) - 1;
+	const int f = world < nBW ? s->hBodyStart[world] : s->n;
+	const int c = world < nBW ? s->hBodyStart[world + 1] - f : 0;
+	if (first) *first = f;
+	if (count) *count = c;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_world_count(pxh_scene s)
+{
+	if (!s)
+		return fail(PXH_ERR_INVALID, "pxh_scene_world_count: null scene");
+	return worldCount(s);
+}
+
 int32_t pxh_scene_simulate(pxh_scene s, float dt)
 {
 	if (!s || !(dt > 0.0f))
@@ -656,8 +809,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		PXH_CHECK(s->conKey.ensure(Sc));
 		PXH_CHECK(s->conPKey.ensure(Sc));
 		pxh::launch_con_build(S, Q, dP, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
-		                      s->planeLocal.p, s->scene.p, s->conA.p, s->conB.p, s->conSlot.p, s->conKey.p,
-		                      s->conPKey.p, s->conColor.p, st);
+		                      s->planeLocal.p, s->scene.p, s->worldTag.p, s->conA.p, s->conB.p, s->conSlot.p,
+		                      s->conKey.p, s->conPKey.p, s->conColor.p, st);
 		const uint32_t* src[2] = {dP, s->flags.p + S};
 		r = readCounts(st, s->hostCounts, src, 2);
 		if (r)
