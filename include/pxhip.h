@@ -28,7 +28,7 @@ enum {
 };
 
 /* geometry codes = PxGeometryType::Enum (physx/include/geometry/PxGeometry.h) */
-enum { PXH_GEOM_SPHERE = 0, PXH_GEOM_PLANE = 1, PXH_GEOM_CAPSULE = 2, PXH_GEOM_BOX = 3 };
+enum { PXH_GEOM_SPHERE = 0, PXH_GEOM_PLANE = 1, PXH_GEOM_CAPSULE = 2, PXH_GEOM_BOX = 3, PXH_GEOM_CONVEXMESH = 5 };
 
 /* solver codes = PxSolverType::Enum (physx/include/PxSceneDesc.h) */
 enum { PXH_SOLVER_PGS = 0, PXH_SOLVER_TGS = 1 };
@@ -126,8 +126,39 @@ int64_t pxh_scene_get_constraints(pxh_scene scene, int32_t* out, int64_t max_con
 int32_t pxh_scene_set_profiling(pxh_scene scene, int32_t enable);
 int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, int32_t n);
 
+/* PxDirectGPUAPI::copyContactData (physx/include/PxDirectGPUAPI.h:401): the
+ * last step's touching pairs into a device buffer, one 128-byte record per
+ * pair (in place of PxGpuContactPair, physx/include/PxContact.h:818, whose
+ * pointers lead into internal buffers): int32 body0, body1 (-1 - plane index
+ * for a plane), contact count, 0 | normal xyz (from shape1 towards shape0),
+ * 0 | 4 x (point xyz, separation) | 4 applied normal impulses | 0.
+ * nb_pairs (device, may be NULL) receives the number of records written. */
+int32_t pxh_scene_copy_contact_data(pxh_scene scene, void* data, uint32_t* nb_pairs, int32_t max_pairs, void* stream);
+
 /* The scene's HIP stream (hipStream_t), for ordering caller work. */
 void* pxh_scene_get_stream(pxh_scene scene);
+
+/* Convex hull cooking on the host: PxCreateConvexMesh with
+ * PxConvexFlag::eCOMPUTE_CONVEX (physx/include/cooking/PxCooking.h:528) read
+ * back through PxConvexMesh::getVertices / getPolygonData / getIndexBuffer /
+ * getMassInformation (physx/include/geometry/PxConvexMesh.h:97-145).
+ * pts[n*3] in; out (host arrays, capacities PXH_HULL_MAX_*): verts[nv*3]
+ * (bit-identical input points), planes[np*4] (n xyz outward, d), polys[np*2]
+ * (first, count into vrefs), vrefs[] (counter-clockwise loops), edges[ne*4]
+ * (v0, v1, face left, face right), mass[13] (volume, centre of mass xyz,
+ * inertia about it at unit density, row-major 3x3), shape[10] (local bounds
+ * centre xyz, extents xyz, internal radius, internal extents xyz), counts[4]
+ * = nv, np, number of vrefs, ne. Returns 0 or a negative code. */
+enum { PXH_HULL_MAX_VERTS = 255, PXH_HULL_MAX_POLYS = 255, PXH_HULL_MAX_VREFS = 1530, PXH_HULL_MAX_EDGES = 765 };
+int32_t pxh_convex_hull(const float* pts, int32_t n, float* verts, float* planes, int32_t* polys, int32_t* vrefs,
+                        int32_t* edges, double* mass, float* shape, int32_t* counts);
+
+/* Registers k convex meshes with a scene (PxCreateConvexMesh for each; host
+ * arrays): counts[k] points each, points[sum*3]. Bodies of type
+ * PXH_GEOM_CONVEXMESH name one by dims[0] = its mesh index (the index of the
+ * first registered mesh comes back in *first). */
+int32_t pxh_scene_add_convex_meshes(pxh_scene scene, int32_t k, const int32_t* counts, const float* points,
+                                    int32_t* first);
 
 /* Removes whole worlds (every body and plane whose world id is listed in
  * worlds[k], host array) between steps: PxScene::removeActors

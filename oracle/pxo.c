@@ -222,7 +222,7 @@ static void reduce_cluster(const pcontact* m, int n, pcontact* out)
 	v = sub(out[1].lb, out[0].lb);
 	v3 nrm = cross(v, out[0].ln);
 	const float sq = dot(nrm, nrm);
-	nrm = sq > 0.0f ? scl(nrm, 1.0f / sqrtf(sq)) : out[0].ln;
+	if (sq > 0.0f) { const float L = sqrtf(sq); nrm = V(nrm.x / L, nrm.y / L, nrm.z / L); } else nrm = out[0].ln; /* V3ScaleInv divides */
 	float minDist = 3.402823466e+38f;
 	int index1 = -1;
 	for (int i = 0; i < n; i++)
@@ -299,7 +299,7 @@ static void reduce_batch(const pcontact* m, int n, float tol, pcontact* out)
 	const v3 cn0 = m[ch[0]].ln;
 	v3 nrm = cross(v, cn0);
 	const float sq = dot(nrm, nrm);
-	nrm = sq > 0.0f ? scl(nrm, 1.0f / sqrtf(sq)) : cn0;
+	if (sq > 0.0f) { const float L = sqrtf(sq); nrm = V(nrm.x / L, nrm.y / L, nrm.z / L); } else nrm = cn0; /* V3ScaleInv divides */
 	maxDist = -3.402823466e+38f; index = -1; ci = -1;
 	float minDist = 3.402823466e+38f;
 	int index1 = -1, ci1 = -1;
@@ -884,6 +884,784 @@ static int c_box_box(v3 e0, xf x0, v3 e1, xf x1, float cd, float tol, cset* o)
 	return 1;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Convex meshes (PxConvexMeshGeometry, identity scale) and the box seen as a */
+/* polytope, PCM contact generation on a fresh manifold                       */
+/* ------------------------------------------------------------------------- */
+
+/* Gu::PolygonalData (geomutils/src/pcm/GuPCMShapeConvex.h) plus the margins
+ * the contact generation reads */
+typedef struct {
+	int nv, np, ne;
+	const v3* v;
+	const v3* pn;
+	const float* pd;
+	const int* pf;      /* polygon first index into vr */
+	const int* pc;      /* polygon vertex count */
+	const int* vr;
+	const int* ev;      /* 2 per edge: vertices */
+	const int* ef;      /* 2 per edge: faces (v0 -> v1 runs counter-clockwise in the first) */
+	v3 bc, be;          /* local bounds centre and extents */
+	float margin;       /* ConvexHullV / BoxV getMarginF */
+} polyd;
+
+typedef struct {
+	polyd d;
+	v3* v; v3* pn; float* pd; int* pf; int* pc; int* vr; int* ev; int* ef;
+} ohull;
+
+static ohull* g_hull = NULL;
+static int g_nhull = 0, g_hcap = 0;
+
+/* edges of a polygon soup, in order of first appearance */
+static int build_edges(int np, const int* pf, const int* pc, const int* vr, int* ev, int* ef)
+{
+	int ne = 0;
+	for (int p = 0; p < np; p++)
+		for (int k = 0; k < pc[p]; k++)
+		{
+			const int a = vr[pf[p] + k], b = vr[pf[p] + (k + 1) % pc[p]];
+			int found = -1;
+			for (int e = 0; e < ne; e++)
+				if (ev[2 * e] == b && ev[2 * e + 1] == a)
+				{
+					found = e;
+					break;
+				}
+			if (found >= 0)
+				ef[2 * found + 1] = p;
+			else
+			{
+				ev[2 * ne] = a;
+				ev[2 * ne + 1] = b;
+				ef[2 * ne] = p;
+				ef[2 * ne + 1] = p;
+				ne++;
+			}
+		}
+	return ne;
+}
+
+int pxo_hull_add(int nv, const float* verts, int np, const float* planes, const int32_t* polys, const int32_t* vrefs,
+                 const float* internal_extents)
+{
+	if (g_nhull == g_hcap)
+	{
+		g_hcap = g_hcap ? 2 * g_hcap : 64;
+		g_hull = (ohull*)realloc(g_hull, sizeof(ohull) * (size_t)g_hcap);
+	}
+	ohull* h = &g_hull[g_nhull];
+	int nr = 0;
+	for (int p = 0; p < np; p++)
+		if (polys[2 * p] + polys[2 * p + 1] > nr)
+			nr = polys[2 * p] + polys[2 * p + 1];
+	h->v = (v3*)malloc(sizeof(v3) * (size_t)nv);
+	h->pn = (v3*)malloc(sizeof(v3) * (size_t)np);
+	h->pd = (float*)malloc(sizeof(float) * (size_t)np);
+	h->pf = (int*)malloc(sizeof(int) * (size_t)np);
+	h->pc = (int*)malloc(sizeof(int) * (size_t)np);
+	h->vr = (int*)malloc(sizeof(int) * (size_t)(nr ? nr : 1));
+	h->ev = (int*)malloc(sizeof(int) * 2 * (size_t)nr);
+	h->ef = (int*)malloc(sizeof(int) * 2 * (size_t)nr);
+	v3 mn = V(3.402823466e+38f, 3.402823466e+38f, 3.402823466e+38f), mx = neg(mn);
+	for (int i = 0; i < nv; i++)
+	{
+		h->v[i] = V(verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]);
+		mn = V(fminf(mn.x, h->v[i].x), fminf(mn.y, h->v[i].y), fminf(mn.z, h->v[i].z));
+		mx = V(fmaxf(mx.x, h->v[i].x), fmaxf(mx.y, h->v[i].y), fmaxf(mx.z, h->v[i].z));
+	}
+	for (int p = 0; p < np; p++)
+	{
+		h->pn[p] = V(planes[4 * p], planes[4 * p + 1], planes[4 * p + 2]);
+		h->pd[p] = planes[4 * p + 3];
+		h->pf[p] = polys[2 * p];
+		h->pc[p] = polys[2 * p + 1];
+	}
+	for (int k = 0; k < nr; k++)
+		h->vr[k] = vrefs[k];
+	polyd* d = &h->d;
+	d->nv = nv; d->np = np;
+	d->ne = build_edges(np, h->pf, h->pc, h->vr, h->ev, h->ef);
+	d->v = h->v; d->pn = h->pn; d->pd = h->pd; d->pf = h->pf; d->pc = h->pc; d->vr = h->vr;
+	d->ev = h->ev; d->ef = h->ef;
+	/* PxBounds3 centre / extents (physx/include/foundation/PxBounds3.h) */
+	d->bc = scl(add(mx, mn), 0.5f);
+	d->be = scl(sub(mx, mn), 0.5f);
+	/* ConvexHullV margin: 10% of the smallest internal extent
+	 * (CalculateConvexMargin, geomutils/src/gjk/GuVecConvexHull.h:76-93) */
+	const float me = fminf(internal_extents[0], fminf(internal_extents[1], internal_extents[2]));
+	d->margin = me * 0.1f;
+	return g_nhull++;
+}
+
+void pxo_hull_clear(void)
+{
+	for (int i = 0; i < g_nhull; i++)
+	{
+		ohull* h = &g_hull[i];
+		free(h->v); free(h->pn); free(h->pd); free(h->pf); free(h->pc); free(h->vr); free(h->ev); free(h->ef);
+	}
+	g_nhull = 0;
+}
+
+static const polyd* hull_of(float id) { return &g_hull[(int)id].d; }
+
+/* PCMPolygonalBox (geomutils/src/pcm/GuPCMShapeConvex.cpp:40-136): corners,
+ * the six quads of gPCMBoxPolygonData and their planes; BoxV margin is 15%
+ * of the smallest half extent (geomutils/src/gjk/GuVecBox.h:46, :56-75) */
+typedef struct { polyd d; v3 v[8]; v3 pn[6]; float pd[6]; int pf[6], pc[6], vr[24], ev[24], ef[24]; } boxpoly;
+static void box_poly(v3 h, boxpoly* b)
+{
+	static const int quads[24] = {0, 3, 2, 1, 1, 2, 6, 5, 5, 6, 7, 4, 4, 7, 3, 0, 3, 7, 6, 2, 4, 0, 1, 5};
+	b->v[0] = V(-h.x, -h.y, -h.z); b->v[1] = V(h.x, -h.y, -h.z); b->v[2] = V(h.x, h.y, -h.z);
+	b->v[3] = V(-h.x, h.y, -h.z); b->v[4] = V(-h.x, -h.y, h.z); b->v[5] = V(h.x, -h.y, h.z);
+	b->v[6] = V(h.x, h.y, h.z); b->v[7] = V(-h.x, h.y, h.z);
+	for (int i = 0; i < 24; i++)
+		b->vr[i] = quads[i];
+	for (int i = 0; i < 6; i++) { b->pf[i] = 4 * i; b->pc[i] = 4; }
+	b->pn[0] = V(0, 0, -1); b->pd[0] = -h.z;
+	b->pn[1] = V(1, 0, 0); b->pd[1] = -h.x;
+	b->pn[2] = V(0, 0, 1); b->pd[2] = -h.z;
+	b->pn[3] = V(-1, 0, 0); b->pd[3] = -h.x;
+	b->pn[4] = V(0, 1, 0); b->pd[4] = -h.y;
+	b->pn[5] = V(0, -1, 0); b->pd[5] = -h.y;
+	polyd* d = &b->d;
+	d->nv = 8; d->np = 6;
+	d->ne = build_edges(6, b->pf, b->pc, b->vr, b->ev, b->ef);
+	d->v = b->v; d->pn = b->pn; d->pd = b->pd; d->pf = b->pf; d->pc = b->pc; d->vr = b->vr; d->ev = b->ev;
+	d->ef = b->ef;
+	d->bc = V(0, 0, 0);
+	d->be = h;
+	d->margin = fminf(h.x, fminf(h.y, h.z)) * 0.15f;
+}
+
+static inline v3 mxtransinv(mxf t, v3 v) { return mtmul(t.r, sub(v, t.p)); }
+static inline mxf mxf_of(xf t) { mxf m; m.r = qmat(t.q); m.p = t.p; return m; }
+
+/* point of a polygon's plane inside its loop (counter-clockwise about n) */
+static int in_polygon(const polyd* P, int f, v3 q)
+{
+	const v3 n = P->pn[f];
+	for (int k = 0; k < P->pc[f]; k++)
+	{
+		const v3 a = P->v[P->vr[P->pf[f] + k]];
+		const v3 b = P->v[P->vr[P->pf[f] + (k + 1) % P->pc[f]]];
+		if (dot(sub(q, a), cross(sub(b, a), n)) > 0.0f)
+			return 0;
+	}
+	return 1;
+}
+
+/* closest points of segments p1q1, p2q2 (Ericson, Real-Time Collision
+ * Detection, 5.1.9) */
+static float seg_closest2(v3 p1, v3 q1, v3 p2, v3 q2, v3* c1, v3* c2)
+{
+	const v3 d1 = sub(q1, p1), d2 = sub(q2, p2), r = sub(p1, p2);
+	const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+	float s, t;
+	if (a <= 1e-20f && e <= 1e-20f) { s = t = 0.0f; }
+	else if (a <= 1e-20f) { s = 0.0f; t = fminf(fmaxf(f / e, 0.0f), 1.0f); }
+	else
+	{
+		const float c = dot(d1, r);
+		if (e <= 1e-20f) { t = 0.0f; s = fminf(fmaxf(-c / a, 0.0f), 1.0f); }
+		else
+		{
+			const float b = dot(d1, d2), den = a * e - b * b;
+			s = den != 0.0f ? fminf(fmaxf((b * f - c * e) / den, 0.0f), 1.0f) : 0.0f;
+			t = (b * s + f) / e;
+			if (t < 0.0f) { t = 0.0f; s = fminf(fmaxf(-c / a, 0.0f), 1.0f); }
+			else if (t > 1.0f) { t = 1.0f; s = fminf(fmaxf((b - c) / a, 0.0f), 1.0f); }
+		}
+	}
+	*c1 = add(p1, scl(d1, s));
+	*c2 = add(p2, scl(d2, t));
+	const v3 w = sub(*c1, *c2);
+	return dot(w, w);
+}
+
+/* Separation of polytope A (vertices va, in B's frame; planes of A mapped by
+ * aToB) and polytope B. A face axis separating them by more than cd ends the
+ * search (no contact is possible; dist is that separation). Separated: the exact closest points (what GJK
+ * converges to on polytopes), returns 1 with dist > 0 and n = (cA - cB)/dist.
+ * Overlapping: the minimum translation over the face normals of both and all
+ * edge-pair directions (what EPA converges to), returns 0 with dist = -depth
+ * and n the direction that separates A from B. */
+static int poly_separation(const polyd* A, mxf aToB, const v3* va, const polyd* B, float cd, float* dist, v3* n,
+                           v3* cA, v3* cB)
+{
+	/* overlap along every candidate axis (normal from B towards A) */
+	float best = 3.402823466e+38f;
+	int sep = 0;
+	for (int f = 0; f < B->np && !sep; f++)
+	{
+		float mn = 3.402823466e+38f;
+		int vi = 0;
+		for (int i = 0; i < A->nv; i++)
+		{
+			const float h = dot(B->pn[f], va[i]) + B->pd[f];
+			if (h < mn) { mn = h; vi = i; }
+		}
+		if (mn > cd)
+		{
+			*dist = mn; /* farther apart than the contact distance along this axis */
+			return 1;
+		}
+		if (mn > 0.0f) sep = 1;
+		else if (-mn < best)
+		{
+			best = -mn;
+			*n = B->pn[f];
+			*cA = va[vi];
+			*cB = sub(va[vi], scl(B->pn[f], mn));
+		}
+	}
+	for (int f = 0; f < A->np && !sep; f++)
+	{
+		const v3 an = mmul(aToB.r, A->pn[f]);
+		const float ad = A->pd[f] - dot(an, aToB.p);
+		float mn = 3.402823466e+38f;
+		int vi = 0;
+		for (int i = 0; i < B->nv; i++)
+		{
+			const float h = dot(an, B->v[i]) + ad;
+			if (h < mn) { mn = h; vi = i; }
+		}
+		if (mn > cd)
+		{
+			*dist = mn;
+			return 1;
+		}
+		if (mn > 0.0f) sep = 1;
+		else if (-mn < best)
+		{
+			best = -mn;
+			*n = neg(an);
+			*cB = B->v[vi];
+			*cA = sub(B->v[vi], scl(an, mn));
+		}
+	}
+	/* edge pairs that form a face of the Minkowski difference: their arcs on
+	 * the Gauss maps of A and -B cross (Gregorius, "The Separating Axis Test
+	 * between Convex Polyhedra", GDC 2013) */
+	v3 bmean = V(0, 0, 0);
+	for (int i = 0; i < B->nv; i++)
+		bmean = add(bmean, B->v[i]);
+	bmean = scl(bmean, 1.0f / (float)B->nv);
+	for (int ea = 0; ea < A->ne && !sep; ea++)
+	{
+		const v3 a0 = va[A->ev[2 * ea]], a1 = va[A->ev[2 * ea + 1]];
+		const v3 na = mmul(aToB.r, A->pn[A->ef[2 * ea]]), nb = mmul(aToB.r, A->pn[A->ef[2 * ea + 1]]);
+		const v3 bxa = cross(nb, na);
+		for (int eb = 0; eb < B->ne && !sep; eb++)
+		{
+			const v3 nc_ = neg(B->pn[B->ef[2 * eb]]), nd = neg(B->pn[B->ef[2 * eb + 1]]);
+			const v3 dxc = cross(nd, nc_);
+			const float cba = dot(nc_, bxa), dba = dot(nd, bxa), adc = dot(na, dxc), bdc = dot(nb, dxc);
+			if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
+				continue;
+			const v3 b0 = B->v[B->ev[2 * eb]], b1 = B->v[B->ev[2 * eb + 1]];
+			v3 ax = cross(sub(a1, a0), sub(b1, b0));
+			const float l2 = dot(ax, ax);
+			if (l2 < 1e-12f)
+				continue;
+			ax = scl(ax, 1.0f / sqrtf(l2));
+			/* outward from B (its vertex mean lies inside it); on a Minkowski
+			 * face the supports along the axis are the two edges */
+			if (dot(ax, sub(b0, bmean)) < 0.0f)
+				ax = neg(ax);
+			const float bmax = dot(ax, b0), amin = dot(ax, a0);
+			const float ov = bmax - amin;
+			if (ov < 0.0f) sep = 1;
+			else if (ov < best)
+			{
+				v3 c1, c2;
+				seg_closest(a0, a1, b0, b1, &c1, &c2);
+				best = ov;
+				*n = ax;
+				*cA = c1;
+				*cB = c2;
+			}
+		}
+	}
+	if (!sep)
+	{
+		*dist = -best;
+		return 0;
+	}
+	/* separated: the closest features (vertex-face both ways, edge-edge) */
+	float d2 = 3.402823466e+38f;
+	for (int f = 0; f < B->np; f++)
+		for (int i = 0; i < A->nv; i++)
+		{
+			const float h = dot(B->pn[f], va[i]) + B->pd[f];
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const v3 q = sub(va[i], scl(B->pn[f], h));
+			if (!in_polygon(B, f, q))
+				continue;
+			d2 = h * h; *cA = va[i]; *cB = q;
+		}
+	for (int f = 0; f < A->np; f++)
+	{
+		const v3 an = mmul(aToB.r, A->pn[f]);
+		const float ad = A->pd[f] - dot(an, aToB.p);
+		for (int i = 0; i < B->nv; i++)
+		{
+			const float h = dot(an, B->v[i]) + ad;
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const v3 q = sub(B->v[i], scl(an, h));
+			if (!in_polygon(A, f, mxtransinv(aToB, q)))
+				continue;
+			d2 = h * h; *cB = B->v[i]; *cA = q;
+		}
+	}
+	for (int ea = 0; ea < A->ne; ea++)
+		for (int eb = 0; eb < B->ne; eb++)
+		{
+			v3 c1, c2;
+			const float s = seg_closest2(va[A->ev[2 * ea]], va[A->ev[2 * ea + 1]], B->v[B->ev[2 * eb]],
+			                            B->v[B->ev[2 * eb + 1]], &c1, &c2);
+			if (s < d2) { d2 = s; *cA = c1; *cB = c2; }
+		}
+	*dist = sqrtf(d2);
+	*n = scl(sub(*cA, *cB), 1.0f / *dist);
+	return 1;
+}
+
+/* Gu::getWitnessPolygonIndex (geomutils/src/pcm/GuPCMContactGenUtil.cpp:201-271) */
+static int witness_polygon(const polyd* P, v3 normal, v3 closest, float tolerance)
+{
+	float pd[256];
+	const float eps = -tolerance;
+	float dist = dot(P->pn[0], closest) + P->pd[0];
+	float minDist = dist >= eps ? fabsf(dist) : 3.402823466e+38f;
+	pd[0] = minDist;
+	float maxDist = dist;
+	int maxIdx = 0, idx = 0;
+	for (int i = 1; i < P->np; i++)
+	{
+		dist = dot(P->pn[i], closest) + P->pd[i];
+		pd[i] = dist >= eps ? fabsf(dist) : 3.402823466e+38f;
+		if (minDist > pd[i]) { minDist = pd[i]; idx = i; }
+		if (dist > maxDist) { maxDist = dist; maxIdx = i; }
+	}
+	if (minDist == 3.402823466e+38f)
+		return maxIdx;
+	float bestProj = dot(normalize(P->pn[idx]), normal);
+	const int first = idx;
+	for (int i = 0; i < P->np; i++)
+		if ((tolerance > (pd[i] - minDist)) && (first != i))
+		{
+			const float proj = dot(normalize(P->pn[i]), normal);
+			if (bestProj > proj) { idx = i; bestProj = proj; }
+		}
+	return idx;
+}
+
+/* Gu::findRotationMatrixFromZAxis (geomutils/src/pcm/GuPersistentContactManifold.cpp:289-354):
+ * a rotation whose third row is `to` */
+static m33 rot_from_z(v3 to)
+{
+	m33 m;
+	const float e = to.z, f = fabsf(e);
+	if (0.9999f > f)
+	{
+		const float vx = -to.y, vy = to.x;
+		const float h = 1.0f / (1.0f + e);
+		const float hvx = h * vx, hvxy = hvx * vy;
+		m.c0 = V(hvx * vx + e, hvxy, vy);
+		m.c1 = V(hvxy, h * (vy * vy) + e, -vx);
+		m.c2 = V(-vy, vx, e);
+		return m;
+	}
+	const v3 from = V(0, 0, 1), absFrom = V(0, 1, 0);
+	const v3 u = sub(absFrom, from), v = sub(absFrom, to);
+	const float du = dot(u, u), dv = dot(v, v), duv = dot(u, v);
+	const float c1 = -(2.0f / du), c2 = -(2.0f / dv), c3 = c1 * (c2 * duv);
+	const v3 c1u = scl(u, c1), c2v = scl(v, c2), c3v = scl(v, c3);
+	m.c0 = add(scl(u, c1u.x), add(scl(v, c2v.x), scl(u, c3v.x)));
+	m.c0.x += 1.0f;
+	m.c1 = add(scl(u, c1u.y), add(scl(v, c2v.y), scl(u, c3v.y)));
+	m.c1.y += 1.0f;
+	m.c2 = add(scl(u, c1u.z), add(scl(v, c2v.z), scl(u, c3v.z)));
+	m.c2.z += 1.0f;
+	return m;
+}
+
+static inline float tri_area2(v3 a, v3 b, v3 c)
+{
+	const v3 ca = sub(a, c), cb = sub(b, c);
+	return ca.x * cb.y - ca.y * cb.x;
+}
+
+/* generatedContacts (geomutils/src/pcm/GuPCMContactGenBoxConvex.cpp:331-530):
+ * reference polygon `ref` of P0, incident polygon `inc` of P1, t0To1 maps
+ * P0's frame to P1's. Contacts keep la on P1 (P1's frame), lb on P0's
+ * reference plane (P0's frame), ln the reference normal (P0's frame). */
+static void generated_contacts(const polyd* P0, const polyd* P1, int ref, int inc, mxf t0To1, pcontact* mc, int* nc,
+                               float cd)
+{
+	const float feps = 1.1920928955078125e-7f, fmax_ = 3.402823466e+38f;
+	const v3 n = normalize(P0->pn[ref]);
+	const m33 rot = rot_from_z(n);
+	const int nr = P0->pc[ref], ni = P1->pc[inc];
+	v3 p0[64], p1[64];
+	float t1[64];
+	int pen1[64];
+	const v3 sPoint = P1->v[P1->vr[P1->pf[inc]]];
+	v3 rMin = V(fmax_, fmax_, fmax_), rMax = neg(rMin);
+	for (int i = 0; i < nr; i++)
+	{
+		p0[i] = mmul(rot, P0->v[P0->vr[P0->pf[ref] + i]]);
+		rMin = V(fminf(rMin.x, p0[i].x), fminf(rMin.y, p0[i].y), fminf(rMin.z, p0[i].z));
+		rMax = V(fmaxf(rMax.x, p0[i].x), fmaxf(rMax.y, p0[i].y), fmaxf(rMax.z, p0[i].z));
+	}
+	rMin = sub(rMin, V(feps, feps, feps));
+	rMax = add(rMax, V(feps, feps, feps));
+	const float d = p0[0].z, rd = d + cd;
+	v3 iMin = V(fmax_, fmax_, fmax_), iMax = neg(iMin);
+	int inside = 0;
+	for (int i = 0; i < ni; i++)
+	{
+		const v3 vert1 = P1->v[P1->vr[P1->pf[inc] + i]];
+		const v3 a = mxtransinv(t0To1, vert1);
+		v3 q = mmul(rot, a);
+		const float z = q.z;
+		t1[i] = z - d;
+		q.z = d;
+		p1[i] = q;
+		iMin = V(fminf(iMin.x, q.x), fminf(iMin.y, q.y), fminf(iMin.z, q.z));
+		iMax = V(fmaxf(iMax.x, q.x), fmaxf(iMax.y, q.y), fmaxf(iMax.z, q.z));
+		if (rd > z)
+		{
+			pen1[i] = 1;
+			if (poly_contains(p0, nr, q, rMin, rMax))
+			{
+				inside++;
+				if (*nc == 64)
+					return;
+				pcontact* c = &mc[(*nc)++];
+				c->la = vert1;
+				c->lb = mtmul(rot, q);
+				c->ln = n;
+				c->pen = t1[i];
+			}
+		}
+		else
+			pen1[i] = 0;
+	}
+	if (inside == ni)
+		return;
+	inside = 0;
+	iMin = sub(iMin, V(feps, feps, feps));
+	iMax = add(iMax, V(feps, feps, feps));
+	const v3 incN = normalize(P1->pn[inc]);
+	const v3 nIn1 = mmul(t0To1.r, n);
+	for (int i = 0; i < nr; i++)
+	{
+		if (!poly_contains(p1, ni, p0[i], iMin, iMax))
+			continue;
+		const v3 vert0 = mtmul(rot, p0[i]);
+		const v3 a = mxtrans(t0To1, vert0);
+		const float nom = dot(incN, sub(sPoint, a));
+		const float den = dot(incN, nIn1);
+		const float t = nom / den;
+		if (t > cd)
+			continue;
+		inside++;
+		if (*nc == 64)
+			return;
+		pcontact* c = &mc[(*nc)++];
+		c->la = add(scl(nIn1, t), a);
+		c->lb = vert0;
+		c->ln = n;
+		c->pen = t;
+	}
+	if (inside == nr)
+		return;
+	for (int iS = 0, iE = ni - 1; iS < ni; iE = iS++)
+	{
+		if (!pen1[iS] && !pen1[iE])
+			continue;
+		const v3 ipA = p1[iS], ipB = p1[iE];
+		const v3 ipAO = V(ipA.x, ipA.y, t1[iS] + d), ipBO = V(ipB.x, ipB.y, t1[iE] + d);
+		const v3 iMn = V(fminf(ipA.x, ipB.x), fminf(ipA.y, ipB.y), 0), iMx = V(fmaxf(ipA.x, ipB.x), fmaxf(ipA.y, ipB.y), 0);
+		for (int rS = 0, rE = nr - 1; rS < nr; rE = rS++)
+		{
+			const v3 rpA = p0[rS], rpB = p0[rE];
+			const v3 rMn = V(fminf(rpA.x, rpB.x), fminf(rpA.y, rpB.y), 0),
+			         rMx = V(fmaxf(rpA.x, rpB.x), fmaxf(rpA.y, rpB.y), 0);
+			if (iMn.x > rMx.x || iMn.y > rMx.y || rMn.x > iMx.x || rMn.y > iMx.y)
+				continue;
+			const float a1 = tri_area2(rpA, rpB, ipA), a2 = tri_area2(rpA, rpB, ipB);
+			if (!(0.0f > a1 * a2))
+				continue;
+			const float a3 = tri_area2(ipA, ipB, rpA), a4 = tri_area2(ipA, ipB, rpB);
+			if (!(0.0f > a3 * a4))
+				continue;
+			const float t = a1 / (a2 - a1);
+			const v3 pBB = sub(ipAO, scl(sub(ipBO, ipAO), t));
+			const v3 pAA = V(pBB.x, pBB.y, d);
+			const v3 pA = mtmul(rot, pAA);
+			const v3 pB = mxtrans(t0To1, mtmul(rot, pBB));
+			const float pen = pBB.z - pAA.z;
+			if (pen > cd)
+				continue;
+			if (*nc == 64)
+				return;
+			pcontact* c = &mc[(*nc)++];
+			c->la = pB;
+			c->lb = pA;
+			c->ln = n;
+			c->pen = pen;
+		}
+	}
+}
+
+/* manifold -> contact buffer: points on B (shape1) in the world, the summed
+ * normal, only points within the contact distance
+ * (PersistentContactManifold::addManifoldContactsToContactBuffer and
+ * getWorldNormal, geomutils/src/pcm/GuPersistentContactManifold.cpp:739-762,
+ * GuPersistentContactManifold.h:405-416) */
+static int manifold_out(const pcontact* m, int n, xf x1, float cd, cset* o)
+{
+	if (n == 0)
+		return 0;
+	v3 s = m[0].ln;
+	for (int i = 1; i < n; i++)
+		s = add(s, m[i].ln);
+	o->n = normalize(qrot(x1.q, s));
+	int k = 0;
+	for (int i = 0; i < n && k < 4; i++)
+		if (cd >= m[i].pen)
+		{
+			o->p[k] = xtrans(x1, m[i].lb);
+			o->sep[k] = m[i].pen;
+			k++;
+		}
+	o->count = k;
+	return k;
+}
+
+/* reduction of a batch of manifold points to at most four
+ * (PersistentContactManifold::addBatchManifoldContacts, :805-830) */
+static int batch_reduce(pcontact* mc, int nc, float tol)
+{
+	if (nc <= 4)
+		return nc;
+	pcontact out[4];
+	reduce_batch(mc, nc, tol, out);
+	for (int i = 0; i < 4; i++)
+		mc[i] = out[i];
+	return 4;
+}
+
+/* Gu::pcmContactConvexConvex / pcmContactBoxConvex on a fresh manifold
+ * (geomutils/src/pcm/GuPCMContactConvexConvex.cpp:194-276, :111-157;
+ * GuPCMContactBoxConvex.cpp; the non-degenerate branch of
+ * Gu::generateFullContactManifold and Gu::addGJKEPAContacts,
+ * GuPCMContactGenBoxConvex.cpp:532-720). A is shape0, B is shape1. */
+static int c_convex_convex(const polyd* A, xf x0, const polyd* B, xf x1, float cd, float tol, cset* o)
+{
+	const xf rel = xinvmul(x1, x0);
+	const mxf aToB = mxf_of(rel);
+	v3 va[256];
+	for (int i = 0; i < A->nv; i++)
+		va[i] = mxtrans(aToB, A->v[i]);
+	float dist = 0.0f;
+	v3 n = V(1, 0, 0), cA = V(0, 0, 0), cB = V(0, 0, 0);
+	poly_separation(A, aToB, va, B, cd, &dist, &n, &cA, &cB);
+	if (dist > cd)
+		return 0;
+	const float lo = tol * 1e-2f, hi = tol * 5e-2f;
+	const float tA = fminf(fmaxf(A->margin, lo), hi), tB = fminf(fmaxf(B->margin, lo), hi);
+	const v3 negN = neg(n);
+	const v3 nIn0 = mtmul(aToB.r, n);
+	const int f1 = witness_polygon(B, negN, cB, tB);
+	const int f0 = witness_polygon(A, nIn0, mxtransinv(aToB, cA), tA);
+	const v3 refN = normalize(B->pn[f1]), incN = normalize(A->pn[f0]);
+	pcontact mc[64];
+	int nc = 0;
+	if (fabsf(dot(refN, negN)) >= fabsf(dot(incN, nIn0)))
+	{
+		const mxf bToA = mxf_of(xinvmul(x0, x1));
+		generated_contacts(B, A, f1, f0, bToA, mc, &nc, cd);
+	}
+	else
+	{
+		generated_contacts(A, B, f0, f1, aToB, mc, &nc, cd);
+		const v3 nn = neg(mmul(aToB.r, incN));
+		for (int i = 0; i < nc; i++)
+		{
+			const v3 t = mc[i].lb;
+			mc[i].lb = mc[i].la;
+			mc[i].la = t;
+			mc[i].ln = nn;
+		}
+	}
+	if (nc == 0)
+	{
+		/* the GJK/EPA point stays the manifold's only point */
+		mc[0].la = mxtransinv(aToB, cA);
+		mc[0].lb = cB;
+		mc[0].ln = n;
+		mc[0].pen = dist;
+		nc = 1;
+	}
+	else
+		nc = batch_reduce(mc, nc, tol);
+	return manifold_out(mc, nc, x1, cd, o);
+}
+
+/* Gu::pcmContactSphereConvex on a fresh manifold
+ * (geomutils/src/pcm/GuPCMContactSphereConvex.cpp:110-246): sphere centre
+ * against the hull, the closest hull point when outside, the least deep face
+ * when inside; one contact on the sphere's surface. */
+static int c_sphere_convex(float r, v3 p0, const polyd* B, xf x1, float cd, cset* o)
+{
+	const v3 c = xtransinv(x1, p0);
+	float hmax = -3.402823466e+38f;
+	int fmax_ = 0;
+	for (int f = 0; f < B->np; f++)
+	{
+		const float h = dot(B->pn[f], c) + B->pd[f];
+		if (h > hmax) { hmax = h; fmax_ = f; }
+	}
+	v3 n;
+	float pen;
+	if (hmax <= 0.0f)
+	{
+		n = B->pn[fmax_];
+		pen = hmax - r;
+	}
+	else
+	{
+		float d2 = 3.402823466e+38f;
+		v3 q = c;
+		for (int f = 0; f < B->np; f++)
+		{
+			const float h = dot(B->pn[f], c) + B->pd[f];
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const v3 pq = sub(c, scl(B->pn[f], h));
+			if (in_polygon(B, f, pq)) { d2 = h * h; q = pq; }
+		}
+		for (int e = 0; e < B->ne; e++)
+		{
+			v3 c1, c2;
+			const float s = seg_closest2(c, c, B->v[B->ev[2 * e]], B->v[B->ev[2 * e + 1]], &c1, &c2);
+			if (s < d2) { d2 = s; q = c2; }
+		}
+		const float dist = sqrtf(d2);
+		if (dist > r + cd)
+			return 0;
+		n = scl(sub(c, q), 1.0f / dist);
+		pen = dist - r;
+	}
+	const v3 wn = qrot(x1.q, n);
+	o->n = wn;
+	o->p[0] = sub(p0, scl(wn, r));
+	o->sep[0] = pen;
+	o->count = 1;
+	return 1;
+}
+
+/* Gu::pcmContactPlaneConvex on a fresh manifold
+ * (geomutils/src/pcm/GuPCMContactPlaneConvex.cpp:36-227): the hull face most
+ * opposed to the plane normal, or the better face of the edge whose summed
+ * face normals are most opposed to it (with a 5e-4 bias towards edges); the
+ * vertices of the chosen face below the contact distance, taken once for the
+ * face and once more when an edge was chosen (the reference loops twice over
+ * the same face); reduced to four. Normal from the hull towards the plane. */
+static int c_plane_convex(v3 pn, float pd, const polyd* B, xf tb, float cd, float tol, cset* o)
+{
+	const v3 n = normalize(qrotinv(tb.q, pn));
+	const v3 nn = neg(n);
+	float minProj = 3.402823466e+38f;
+	int face = 0;
+	for (int i = 0; i < B->np; i++)
+	{
+		const float proj = dot(n, B->pn[i]);
+		if (minProj > proj) { minProj = proj; face = i; }
+	}
+	int edge = -1;
+	minProj = minProj - 5e-4f;
+	float maxDpSq = minProj * minProj;
+	for (int e = 0; e < B->ne; e++)
+	{
+		const v3 en = add(B->pn[B->ef[2 * e]], B->pn[B->ef[2 * e + 1]]);
+		const float enMagSq = dot(en, en);
+		const float dp = dot(en, nn);
+		const float sqDp = dp * dp;
+		if (dp >= 0.0f && sqDp > maxDpSq * enMagSq)
+		{
+			maxDpSq = sqDp / enMagSq;
+			edge = e;
+		}
+	}
+	int passes = 1;
+	if (edge >= 0)
+	{
+		const int f0 = B->ef[2 * edge], f1 = B->ef[2 * edge + 1];
+		face = dot(B->pn[f0], nn) > dot(B->pn[f1], nn) ? f0 : f1;
+		passes = 2;
+	}
+	pcontact mc[64];
+	int nc = 0;
+	int deepestTaken = 0;
+	float deepest = 3.402823466e+38f;
+	int dv = 0;
+	for (int i = 0; i < B->nv; i++)
+	{
+		const float s = dot(pn, xtrans(tb, B->v[i])) + pd;
+		if (s < deepest) { deepest = s; dv = i; }
+	}
+	for (int pass = 0; pass < passes; pass++)
+		for (int k = 0; k < B->pc[face]; k++)
+		{
+			const int vi = B->vr[B->pf[face] + k];
+			const v3 pw = xtrans(tb, B->v[vi]);
+			const float s = dot(pn, pw) + pd;
+			if (cd > s && nc < 64)
+			{
+				mc[nc].la = B->v[vi];
+				mc[nc].lb = sub(pw, scl(pn, s));
+				mc[nc].ln = pn;
+				mc[nc].pen = s;
+				nc++;
+				deepestTaken |= vi == dv;
+			}
+		}
+	/* Deliberate deviation: when the chosen face does not hold the hull's
+	 * deepest vertex (an irregular hull can rest on a vertex that the face
+	 * most opposed to the plane does not contain), that vertex joins the
+	 * batch. The reference scene's persistent manifold keeps such a vertex
+	 * from earlier steps; a step that generates contacts afresh has to add it. */
+	if (!deepestTaken && cd > deepest && nc < 64)
+	{
+		const v3 pw = xtrans(tb, B->v[dv]);
+		mc[nc].la = B->v[dv];
+		mc[nc].lb = sub(pw, scl(pn, deepest));
+		mc[nc].ln = pn;
+		mc[nc].pen = deepest;
+		nc++;
+	}
+	nc = batch_reduce(mc, nc, tol);
+	o->n = neg(pn);
+	int k = 0;
+	for (int i = 0; i < nc; i++)
+		if (cd >= mc[i].pen)
+		{
+			o->p[k] = mc[i].lb;
+			o->sep[k] = mc[i].pen;
+			k++;
+		}
+	o->count = k;
+	return k;
+}
+
 static xf load_pose(const float* p)
 {
 	xf t;
@@ -925,6 +1703,22 @@ int pxo_generate_contacts(int type0, const float* g0, const float* pose0, int ty
 	}
 	else if (type0 == PXO_BOX && type1 == PXO_BOX)
 		n = c_box_box(V(g0[0], g0[1], g0[2]), x0, V(g1[0], g1[1], g1[2]), x1, cd, tol, &c);
+	else if (type0 == PXO_PLANE && type1 == PXO_CONVEX)
+	{
+		v3 pn; float pd;
+		plane_from_pose(x0, &pn, &pd);
+		n = c_plane_convex(pn, pd, hull_of(g1[0]), x1, cd, tol, &c);
+	}
+	else if (type0 == PXO_SPHERE && type1 == PXO_CONVEX)
+		n = c_sphere_convex(g0[0], x0.p, hull_of(g1[0]), x1, cd, &c);
+	else if (type0 == PXO_BOX && type1 == PXO_CONVEX)
+	{
+		boxpoly bp;
+		box_poly(V(g0[0], g0[1], g0[2]), &bp);
+		n = c_convex_convex(&bp.d, x0, hull_of(g1[0]), x1, cd, tol, &c);
+	}
+	else if (type0 == PXO_CONVEX && type1 == PXO_CONVEX)
+		n = c_convex_convex(hull_of(g0[0]), x0, hull_of(g1[0]), x1, cd, tol, &c);
 	else
 		return -1;
 	for (int i = 0; i < n; i++)
@@ -1209,6 +2003,27 @@ void pxo_world_stats(void* wp, long long* out)
 	for (int i = 0; i < 6; i++) out[i] = w->stats[i];
 }
 
+long long pxo_world_contacts(void* wp, float* out, long long max)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (long long i = 0; i < w->ncon && i < max; i++)
+	{
+		const pxo_con* k = &w->con[i];
+		float* r = out + 32 * i;
+		memset(r, 0, 32 * sizeof(float));
+		const int32_t hdr[3] = {k->a, k->b, k->c.count};
+		memcpy(r, hdr, sizeof(hdr));
+		r[4] = k->c.n.x; r[5] = k->c.n.y; r[6] = k->c.n.z;
+		for (int j = 0; j < k->c.count; j++)
+		{
+			r[8 + 4 * j] = k->c.p[j].x; r[9 + 4 * j] = k->c.p[j].y; r[10 + 4 * j] = k->c.p[j].z;
+			r[11 + 4 * j] = k->c.sep[j];
+			r[24 + j] = k->force[j];
+		}
+	}
+	return w->ncon;
+}
+
 long long pxo_world_constraints(void* wp, int32_t* out, long long max)
 {
 	pxo_world* w = (pxo_world*)wp;
@@ -1234,11 +2049,19 @@ static void shape_bounds(int type, const float* dims, xf t, v3* mn, v3* mx)
 		return;
 	}
 	const m33 R = qmat(t.q);
-	const v3 c0 = scl(R.c0, dims[0]), c1 = scl(R.c1, dims[1]), c2 = scl(R.c2, dims[2]);
+	v3 e = V(dims[0], dims[1], dims[2]), c = t.p;
+	if (type == PXO_CONVEX)
+	{
+		/* hull bounds: the local box rotated (GuBounds.cpp:136-140 transformNoEmptyTest) */
+		const polyd* h = hull_of(dims[0]);
+		e = h->be;
+		c = add(mmul(R, h->bc), t.p);
+	}
+	const v3 c0 = scl(R.c0, e.x), c1 = scl(R.c1, e.y), c2 = scl(R.c2, e.z);
 	const v3 w = V(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y),
 	               fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z));
-	*mn = sub(t.p, w);
-	*mx = add(t.p, w);
+	*mn = sub(c, w);
+	*mx = add(c, w);
 }
 
 /* Constraint ordering key: a 64-bit mix of the scene-local identities of the
@@ -1741,8 +2564,18 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 		n = c_sphere_sphere(g0[0], w->pose[s0].p, g1[0], w->pose[s1].p, cd, &c);
 	else if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_BOX)
 		n = c_sphere_box(g0[0], w->pose[s0].p, V(g1[0], g1[1], g1[2]), w->pose[s1], cd, &c);
-	else
+	else if (w->type[s0] == PXO_BOX && w->type[s1] == PXO_BOX)
 		n = c_box_box(V(g0[0], g0[1], g0[2]), w->pose[s0], V(g1[0], g1[1], g1[2]), w->pose[s1], cd, tol, &c);
+	else if (w->type[s0] == PXO_SPHERE)
+		n = c_sphere_convex(g0[0], w->pose[s0].p, hull_of(g1[0]), w->pose[s1], cd, &c);
+	else if (w->type[s0] == PXO_BOX)
+	{
+		boxpoly bp;
+		box_poly(V(g0[0], g0[1], g0[2]), &bp);
+		n = c_convex_convex(&bp.d, w->pose[s0], hull_of(g1[0]), w->pose[s1], cd, tol, &c);
+	}
+	else
+		n = c_convex_convex(hull_of(g0[0]), w->pose[s0], hull_of(g1[0]), w->pose[s1], cd, tol, &c);
 	if (n > 0)
 		push_con(w, s0, s1, &c);
 }
@@ -1787,6 +2620,11 @@ int pxo_world_step(void* wp, float dt)
 			int k;
 			if (w->type[i] == PXO_SPHERE)
 				k = c_sphere_plane(w->dims[4 * i], w->pose[i].p, w->pn[q], w->pd[q], cd, &c);
+			else if (w->type[i] == PXO_CONVEX)
+			{
+				k = c_plane_convex(w->pn[q], w->pd[q], hull_of(w->dims[4 * i]), w->pose[i], cd, tol, &c);
+				c.n = neg(c.n); /* the hull is body A: normal from the plane towards it */
+			}
 			else
 			{
 				k = c_plane_box(w->pn[q], w->pd[q], V(w->dims[4 * i], w->dims[4 * i + 1], w->dims[4 * i + 2]),

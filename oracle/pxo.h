@@ -15,6 +15,9 @@
  *                            and long-run statistics vs the reference PxScene
  *   box-box edge-edge case   partial: exact SAT/closest-feature restatement of
  *                            the GJK/EPA fallback (GuPCMContactBoxBox.cpp:922-958)
+ *   convex hull contacts     pinned: vs PxGenerateContacts on hulls cooked by the
+ *                            reference (exact closest features / minimum
+ *                            translation stand in for GJK / EPA)
  */
 #ifndef PXO_H
 #define PXO_H
@@ -24,7 +27,15 @@
 extern "C" {
 #endif
 
-enum { PXO_SPHERE = 0, PXO_PLANE = 1, PXO_CAPSULE = 2, PXO_BOX = 3 };
+enum { PXO_SPHERE = 0, PXO_PLANE = 1, PXO_CAPSULE = 2, PXO_BOX = 3, PXO_CONVEX = 5 };
+
+/* Convex meshes: registers a cooked hull (verts nv x 3, planes np x 4 with
+ * outward unit normals, polys np x 2 = first/count into vrefs, counter-
+ * clockwise loops, internal extents xyz) and returns its index; a convex
+ * shape's geometry (geom / dims) is that index in element 0. */
+int  pxo_hull_add(int nv, const float* verts, int np, const float* planes, const int32_t* polys,
+                  const int32_t* vrefs, const float* internal_extents);
+void pxo_hull_clear(void);
 
 typedef struct {
 	float gravity[3];
@@ -73,6 +84,10 @@ void  pxo_world_stats(void* w, long long* out);
 /* last step's constraints: per constraint (bodyA, bodyB or -1-planeIndex,
  * colour, nContacts); returns count */
 long long pxo_world_constraints(void* w, int32_t* out, long long max);
+/* last step's contacts: 32 floats per constraint in the layout of the
+ * library's contact-data records (int32 a, b, count; normal; 4 x (point,
+ * separation); 4 applied normal impulses); returns count */
+long long pxo_world_contacts(void* w, float* out, long long max);
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,8 @@
 //                           flat arrays, stepped with simulate()+fetchResults()
 #include "PxPhysicsAPI.h"
 #include "PxImmediateMode.h"
+#include "cooking/PxCooking.h"
+#include "GuConvexMesh.h"
 #include <chrono>
 #include <cstdint>
 #include <cstring>
@@ -39,7 +41,11 @@ bool ensureSdk()
 	return gPhysics != nullptr;
 }
 
-// geometry codes shared with the product C-ABI: 0 sphere, 1 plane, 2 capsule, 3 box
+// convex meshes cooked by pxref_convex_create, named by index
+std::vector<PxConvexMesh*> gConvex;
+
+// geometry codes shared with the product C-ABI: 0 sphere, 1 plane, 2 capsule,
+// 3 box, 5 convex mesh (g[0] = index of a mesh from pxref_convex_create)
 PxGeometryHolder makeGeom(int type, const float* g)
 {
 	switch (type)
@@ -47,6 +53,7 @@ PxGeometryHolder makeGeom(int type, const float* g)
 	case 0: return PxGeometryHolder(PxSphereGeometry(g[0]));
 	case 1: return PxGeometryHolder(PxPlaneGeometry());
 	case 2: return PxGeometryHolder(PxCapsuleGeometry(g[0], g[1]));
+	case 5: return PxGeometryHolder(PxConvexMeshGeometry(gConvex[size_t(g[0])]));
 	default: return PxGeometryHolder(PxBoxGeometry(g[0], g[1], g[2]));
 	}
 }
@@ -173,6 +180,78 @@ long long pxref_bp_pairs(const float* bounds, const float* dist, const uint32_t*
 	}
 	bp->release();
 	return count;
+}
+
+// Cooks the hull of n points (PxCreateConvexMesh, eCOMPUTE_CONVEX, default
+// cooking parameters) and returns its mesh index, or -1.
+int pxref_convex_create(const float* pts, int n)
+{
+	if (!ensureSdk())
+		return -1;
+	PxConvexMeshDesc desc;
+	desc.points.count = PxU32(n);
+	desc.points.stride = sizeof(PxVec3);
+	desc.points.data = pts;
+	desc.flags = PxConvexFlag::eCOMPUTE_CONVEX;
+	PxCookingParams params{PxTolerancesScale()};
+	PxConvexMesh* m = PxCreateConvexMesh(params, desc, gPhysics->getPhysicsInsertionCallback());
+	if (!m)
+		return -1;
+	gConvex.push_back(m);
+	return int(gConvex.size()) - 1;
+}
+
+// A cooked hull read back: verts[nv*3], planes[np*4], polys[np*2] (first,
+// count into vrefs), vrefs[], mass[13] (mass at unit density, centre of mass,
+// inertia 3x3 row-major), shape[10] (bounds centre, extents, internal radius,
+// internal extents), counts[3] = nv, np, vrefs.
+int pxref_convex_data(int id, float* verts, float* planes, int* polys, int* vrefs, double* mass, float* shape,
+                      int* counts)
+{
+	if (id < 0 || size_t(id) >= gConvex.size())
+		return -1;
+	PxConvexMesh* m = gConvex[size_t(id)];
+	const PxU32 nv = m->getNbVertices(), np = m->getNbPolygons();
+	const PxVec3* v = m->getVertices();
+	for (PxU32 i = 0; i < nv; i++)
+	{
+		verts[3 * i] = v[i].x; verts[3 * i + 1] = v[i].y; verts[3 * i + 2] = v[i].z;
+	}
+	const PxU8* ib = m->getIndexBuffer();
+	int nr = 0;
+	for (PxU32 p = 0; p < np; p++)
+	{
+		PxHullPolygon hp;
+		m->getPolygonData(p, hp);
+		for (int c = 0; c < 4; c++)
+			planes[4 * p + c] = hp.mPlane[c];
+		polys[2 * p] = hp.mIndexBase;
+		polys[2 * p + 1] = hp.mNbVerts;
+		for (int k = 0; k < hp.mNbVerts; k++)
+			vrefs[hp.mIndexBase + k] = ib[hp.mIndexBase + k];
+		nr = PxMax(nr, int(hp.mIndexBase + hp.mNbVerts));
+	}
+	PxReal ms;
+	PxMat33 I;
+	PxVec3 com;
+	m->getMassInformation(ms, I, com);
+	mass[0] = ms;
+	mass[1] = com.x; mass[2] = com.y; mass[3] = com.z;
+	for (int r = 0; r < 3; r++)
+		for (int c = 0; c < 3; c++)
+			mass[4 + 3 * r + c] = I[c][r];
+	const Gu::ConvexHullData& h = static_cast<const Gu::ConvexMesh*>(m)->getHull();
+	const PxVec3 bc = h.mAABB.mCenter, be = h.mAABB.mExtents;
+	shape[0] = bc.x; shape[1] = bc.y; shape[2] = bc.z;
+	shape[3] = be.x; shape[4] = be.y; shape[5] = be.z;
+	shape[6] = h.mInternal.mInternalRadius;
+	shape[7] = h.mInternal.mInternalExtents.x;
+	shape[8] = h.mInternal.mInternalExtents.y;
+	shape[9] = h.mInternal.mInternalExtents.z;
+	counts[0] = int(nv);
+	counts[1] = int(np);
+	counts[2] = nr;
+	return 0;
 }
 
 uint32_t pxref_static_group() { return PxGetBroadPhaseStaticFilterGroup(); }

@@ -18,7 +18,7 @@ from . import scenes  # noqa: F401
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpxhip.so")
 
-PXH_GEOM_SPHERE, PXH_GEOM_PLANE, PXH_GEOM_CAPSULE, PXH_GEOM_BOX = 0, 1, 2, 3
+PXH_GEOM_SPHERE, PXH_GEOM_PLANE, PXH_GEOM_CAPSULE, PXH_GEOM_BOX, PXH_GEOM_CONVEXMESH = 0, 1, 2, 3, 5
 PXH_SOLVER_PGS, PXH_SOLVER_TGS = 0, 1
 PXH_DATA_GLOBAL_POSE, PXH_DATA_LINEAR_VELOCITY, PXH_DATA_ANGULAR_VELOCITY = 0, 1, 2
 
@@ -52,7 +52,8 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts", "pxh_scene_set_profiling",
             "pxh_scene_get_profile", "pxh_scene_remove_worlds", "pxh_scene_world_count", "pxh_scene_world_range",
             "pxh_server_create", "pxh_server_release", "pxh_server_last_error", "pxh_server_submit",
-            "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats")
+            "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats", "pxh_convex_hull",
+            "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data")
 
 _lib = None
 
@@ -132,6 +133,15 @@ class Scene:
 
     def add(self, scene):
         t, dm, po = _i32(scene.shape_type), _f32(scene.dims), _f32(scene.pose)
+        if scene.hull_counts is not None and len(scene.hull_counts):
+            # the scene's convex meshes, renumbered after those already registered
+            hc, hp = _i32(scene.hull_counts), _f32(scene.hull_points)
+            first = ctypes.c_int32()
+            _check(self.lib.pxh_scene_add_convex_meshes(self.h, int(hc.size), hc.ctypes.data_as(_I32),
+                                                        hp.ctypes.data_as(_F), ctypes.byref(first)))
+            dm = dm.copy()
+            cvx = t == PXH_GEOM_CONVEXMESH
+            dm[cvx, 0] += first.value
         lv, av, m, inn = _f32(scene.lin_vel), _f32(scene.ang_vel), _f32(scene.mass), _f32(scene.inertia)
         w = _i32(scene.scene_id)
         fp = lambda a: a.ctypes.data_as(_F)  # noqa: E731
@@ -211,6 +221,19 @@ class Scene:
                                               cnt.ctypes.data_as(_I64), 8))
         return {name: (float(ms[i]), int(cnt[i])) for i, name in enumerate(self.STAGES) if cnt[i] > 0}
 
+    def contact_data(self):
+        """PxDirectGPUAPI::copyContactData: the last step's touching pairs as a
+        (k, 32) float32 array; columns 0-2 hold int32 body0, body1 (-1 - plane),
+        count; 4-6 the normal; 8-23 four (point, separation); 24-27 normal impulses."""
+        import torch
+        k = self.stats()["constraints"]
+        buf = torch.zeros((max(k, 1), 32), dtype=torch.float32, device="cuda")
+        nb = torch.zeros(1, dtype=torch.int32, device="cuda")
+        torch.cuda.synchronize()
+        _check(self.lib.pxh_scene_copy_contact_data(self.h, _dptr(buf), _dptr(nb), k, None))
+        out = buf.cpu().numpy()[: int(nb.cpu().item())]
+        return out
+
     def remove_worlds(self, worlds):
         """PxScene::removeActors for every actor of the listed worlds; the
         remaining worlds are renumbered in order and their bodies compacted."""
@@ -243,6 +266,31 @@ class Scene:
             self.close()
         except Exception:
             pass
+
+
+def cook_convex(points):
+    """Convex hull cooking on the host (pxh_convex_hull): a dict of the hull's
+    vertices, planes, polygon loops, edges, mass properties at unit density and
+    internal extents, or None for a degenerate point cloud."""
+    lib = load()
+    pts = _f32(points).reshape(-1, 3)
+    V = np.zeros((255, 3), np.float32)
+    Pl = np.zeros((255, 4), np.float32)
+    Po = np.zeros((255, 2), np.int32)
+    R = np.zeros(1530, np.int32)
+    E = np.zeros((765, 4), np.int32)
+    M = np.zeros(13, np.float64)
+    S = np.zeros(10, np.float32)
+    C = np.zeros(4, np.int32)
+    vp = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+    rc = lib.pxh_convex_hull(vp(pts), int(pts.shape[0]), vp(V), vp(Pl), vp(Po), vp(R), vp(E), vp(M), vp(S), vp(C))
+    if rc != 0:
+        return None
+    nv, npol, nr, ne = (int(x) for x in C)
+    return dict(verts=V[:nv].copy(), planes=Pl[:npol].copy(), polys=Po[:npol].copy(), vrefs=R[:nr].copy(),
+                edges=E[:ne].copy(), poly_counts=Po[:npol, 1].copy(), volume=float(M[0]), com=M[1:4].copy(),
+                inertia=M[4:13].reshape(3, 3).copy(), bounds_center=S[0:3].copy(), bounds_extents=S[3:6].copy(),
+                internal_radius=float(S[6]), internal_extents=S[7:10].copy())
 
 
 class Server:

@@ -1,0 +1,756 @@
+// Convex contact generation on the device, one pair per lane: convex mesh
+// (and the box seen as a polytope) against convex mesh, sphere and plane.
+// Each generator follows the reference's PCM contact generation on a fresh
+// manifold, so a pair gives the contacts of PxGenerateContacts with an empty
+// PxCache: the separation is the exact closest-feature distance or minimum
+// translation (what the reference's GJK / EPA converge to on polytopes), the
+// witness polygons, polygon clipping and reduction to four points follow
+// geomutils/src/pcm/GuPCMContactGenBoxConvex.cpp and GuPCMContactGenUtil.cpp.
+#pragma once
+#include "pxh_kernels.h"
+#include "pxh_np.h"
+
+namespace pxh {
+
+constexpr int GEOM_CONVEXMESH = 5;
+constexpr int HULL_MAX_POLY_VERTS = 16;   // polygon size the device path accepts
+constexpr int HULL_MAX_CONTACTS = 32;     // clipped points before reduction
+
+// PCMPolygonalBox (geomutils/src/pcm/GuPCMShapeConvex.cpp:40-136): the quads
+// of gPCMBoxPolygonData, their planes, and the box's edges in polygon order
+__constant__ const uint8_t kBoxQuads[24] = {0, 3, 2, 1, 1, 2, 6, 5, 5, 6, 7, 4, 4, 7, 3, 0, 3, 7, 6, 2, 4, 0, 1, 5};
+__constant__ const uint32_t kBoxEdges[12] = {0x03000300u, 0x04000203u, 0x01000102u, 0x05000001u,
+                                             0x04010602u, 0x02010506u, 0x05010105u, 0x04020706u,
+                                             0x03020407u, 0x05020504u, 0x04030307u, 0x05030400u};
+
+// One polytope: a hull of the set, or a box by its half extents.
+struct Poly
+{
+	const HullSet* hs;
+	int vb, pb, rb, eb;     // bases of vertices, polygons, vrefs, edges
+	int nv, np, ne;
+	bool box;
+	f3 h;
+	float margin;
+
+	PXH_D f3 vert(int i) const
+	{
+		if (box)
+			return mk((i == 1 || i == 2 || i == 5 || i == 6) ? h.x : -h.x,
+			          (i == 2 || i == 3 || i == 6 || i == 7) ? h.y : -h.y, i >= 4 ? h.z : -h.z);
+		return mk(hs->v[vb + i]);
+	}
+	PXH_D f3 pn(int f) const
+	{
+		if (box)
+		{
+			const float s = (f == 1 || f == 2 || f == 4) ? 1.0f : -1.0f;
+			return (f == 1 || f == 3) ? mk(s, 0.0f, 0.0f) : ((f == 4 || f == 5) ? mk(0.0f, s, 0.0f) : mk(0.0f, 0.0f, s));
+		}
+		return mk(hs->pl[pb + f]);
+	}
+	PXH_D float pd(int f) const
+	{
+		if (box)
+			return (f == 1 || f == 3) ? -h.x : ((f == 4 || f == 5) ? -h.y : -h.z);
+		return hs->pl[pb + f].w;
+	}
+	PXH_D int pcount(int f) const { return box ? 4 : int(hs->poly[pb + f] >> 16); }
+	PXH_D int pvert(int f, int k) const
+	{
+		return box ? int(kBoxQuads[4 * f + k]) : int(hs->vr[rb + int(hs->poly[pb + f] & 0xFFFFu) + k]);
+	}
+	PXH_D uint32_t edge(int e) const { return box ? kBoxEdges[e] : hs->edge[eb + e]; }
+};
+
+PXH_D Poly hullPoly(const HullSet& hs, int id)
+{
+	Poly p;
+	p.hs = &hs;
+	const int4 a = hs.info[id], b = hs.info2[id];
+	p.vb = a.x; p.nv = a.y; p.pb = a.z; p.np = a.w;
+	p.rb = b.x; p.eb = b.y; p.ne = b.z;
+	p.box = false;
+	p.h = mk(0.0f, 0.0f, 0.0f);
+	p.margin = hs.bc[id].w;
+	return p;
+}
+
+// BoxV margin: 15% of the smallest half extent (geomutils/src/gjk/GuVecBox.h:46, :56-75)
+PXH_D Poly boxPoly(f3 h)
+{
+	Poly p;
+	p.hs = nullptr;
+	p.vb = p.pb = p.rb = p.eb = 0;
+	p.nv = 8; p.np = 6; p.ne = 12;
+	p.box = true;
+	p.h = h;
+	p.margin = fminf(h.x, fminf(h.y, h.z)) * 0.15f;
+	return p;
+}
+
+PXH_D f3 mxtransinv(const mxf& t, f3 v) { return mtmul(t.r, v - t.p); }
+PXH_D mxf mxfOf(const xf& t) { mxf m; m.r = qmat(t.q); m.p = t.p; return m; }
+
+// a point of polygon f's plane inside its counter-clockwise loop
+PXH_D bool inPolygon(const Poly& P, int f, f3 q)
+{
+	const f3 n = P.pn(f);
+	const int c = P.pcount(f);
+	for (int k = 0; k < c; k++)
+	{
+		const f3 a = P.vert(P.pvert(f, k)), b = P.vert(P.pvert(f, (k + 1) % c));
+		if (dot(q - a, cross(b - a, n)) > 0.0f)
+			return false;
+	}
+	return true;
+}
+
+// closest points of segments p1q1, p2q2 (Ericson, Real-Time Collision
+// Detection, 5.1.9); returns the squared distance
+PXH_D float segClosestSq(f3 p1, f3 q1, f3 p2, f3 q2, f3& c1, f3& c2)
+{
+	const f3 d1 = q1 - p1, d2 = q2 - p2, r = p1 - p2;
+	const float a = dot(d1, d1), e = dot(d2, d2), f = dot(d2, r);
+	float s, t;
+	if (a <= 1e-20f && e <= 1e-20f) { s = t = 0.0f; }
+	else if (a <= 1e-20f) { s = 0.0f; t = fminf(fmaxf(f / e, 0.0f), 1.0f); }
+	else
+	{
+		const float c = dot(d1, r);
+		if (e <= 1e-20f) { t = 0.0f; s = fminf(fmaxf(-c / a, 0.0f), 1.0f); }
+		else
+		{
+			const float b = dot(d1, d2), den = a * e - b * b;
+			s = den != 0.0f ? fminf(fmaxf((b * f - c * e) / den, 0.0f), 1.0f) : 0.0f;
+			t = (b * s + f) / e;
+			if (t < 0.0f) { t = 0.0f; s = fminf(fmaxf(-c / a, 0.0f), 1.0f); }
+			else if (t > 1.0f) { t = 1.0f; s = fminf(fmaxf((b - c) / a, 0.0f), 1.0f); }
+		}
+	}
+	c1 = p1 + d1 * s;
+	c2 = p2 + d2 * t;
+	const f3 w = c1 - c2;
+	return dot(w, w);
+}
+
+// Separation of A (placed in B's frame by aToB) and B. A face axis that
+// separates them by more than cd ends the search (no contact possible; dist
+// is that separation). Separated: exact
+// closest points, dist > 0, n = (cA - cB) / dist. Overlapping: the minimum
+// translation over both polytopes' face normals and the edge pairs that form
+// a face of the Minkowski difference (their arcs cross on the Gauss maps;
+// Gregorius, "The Separating Axis Test between Convex Polyhedra", GDC 2013);
+// dist = -depth, n from B towards A.
+PXH_D void polySeparation(const Poly& A, const mxf& aToB, const Poly& B, float cd, float& dist, f3& n, f3& cA,
+                           f3& cB)
+{
+	float best = FLT_BIG;
+	bool sep = false;
+	for (int f = 0; f < B.np && !sep; f++)
+	{
+		const f3 bn = B.pn(f);
+		const float bd = B.pd(f);
+		float mn = FLT_BIG;
+		f3 vmin = mk(0.0f, 0.0f, 0.0f);
+		for (int i = 0; i < A.nv; i++)
+		{
+			const f3 va = mxtrans(aToB, A.vert(i));
+			const float h = dot(bn, va) + bd;
+			if (h < mn) { mn = h; vmin = va; }
+		}
+		if (mn > cd)
+		{
+			dist = mn;   // farther apart than the contact distance along this axis
+			return;
+		}
+		if (mn > 0.0f) sep = true;
+		else if (-mn < best)
+		{
+			best = -mn;
+			n = bn;
+			cA = vmin;
+			cB = vmin - bn * mn;
+		}
+	}
+	for (int f = 0; f < A.np && !sep; f++)
+	{
+		const f3 an = mmul(aToB.r, A.pn(f));
+		const float ad = A.pd(f) - dot(an, aToB.p);
+		float mn = FLT_BIG;
+		f3 vmin = mk(0.0f, 0.0f, 0.0f);
+		for (int i = 0; i < B.nv; i++)
+		{
+			const f3 vb = B.vert(i);
+			const float h = dot(an, vb) + ad;
+			if (h < mn) { mn = h; vmin = vb; }
+		}
+		if (mn > cd)
+		{
+			dist = mn;
+			return;
+		}
+		if (mn > 0.0f) sep = true;
+		else if (-mn < best)
+		{
+			best = -mn;
+			n = -an;
+			cB = vmin;
+			cA = vmin - an * mn;
+		}
+	}
+	if (!sep)
+	{
+		f3 bmean = mk(0.0f, 0.0f, 0.0f);
+		for (int i = 0; i < B.nv; i++)
+			bmean = bmean + B.vert(i);
+		bmean = bmean * (1.0f / float(B.nv));
+		for (int ea = 0; ea < A.ne && !sep; ea++)
+		{
+			const uint32_t e = A.edge(ea);
+			const f3 a0 = mxtrans(aToB, A.vert(int(e & 0xFF))), a1 = mxtrans(aToB, A.vert(int((e >> 8) & 0xFF)));
+			const f3 na = mmul(aToB.r, A.pn(int((e >> 16) & 0xFF))), nb = mmul(aToB.r, A.pn(int(e >> 24)));
+			const f3 bxa = cross(nb, na);
+			for (int ebi = 0; ebi < B.ne && !sep; ebi++)
+			{
+				const uint32_t g = B.edge(ebi);
+				const f3 nc = -B.pn(int((g >> 16) & 0xFF)), nd = -B.pn(int(g >> 24));
+				const f3 dxc = cross(nd, nc);
+				const float cba = dot(nc, bxa), dba = dot(nd, bxa), adc = dot(na, dxc), bdc = dot(nb, dxc);
+				if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
+					continue;
+				const f3 b0 = B.vert(int(g & 0xFF)), b1 = B.vert(int((g >> 8) & 0xFF));
+				f3 ax = cross(a1 - a0, b1 - b0);
+				const float l2 = dot(ax, ax);
+				if (l2 < 1e-12f)
+					continue;
+				ax = ax * (1.0f / sqrtf(l2));
+				if (dot(ax, b0 - bmean) < 0.0f)
+					ax = -ax;
+				const float ov = dot(ax, b0) - dot(ax, a0);
+				if (ov < 0.0f) sep = true;
+				else if (ov < best)
+				{
+					f3 c1, c2;
+					segClosestSq(a0, a1, b0, b1, c1, c2);
+					best = ov;
+					n = ax;
+					cA = c1;
+					cB = c2;
+				}
+			}
+		}
+	}
+	if (!sep)
+	{
+		dist = -best;
+		return;
+	}
+	float d2 = FLT_BIG;
+	for (int f = 0; f < B.np; f++)
+	{
+		const f3 bn = B.pn(f);
+		const float bd = B.pd(f);
+		for (int i = 0; i < A.nv; i++)
+		{
+			const f3 va = mxtrans(aToB, A.vert(i));
+			const float h = dot(bn, va) + bd;
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const f3 q = va - bn * h;
+			if (!inPolygon(B, f, q))
+				continue;
+			d2 = h * h; cA = va; cB = q;
+		}
+	}
+	for (int f = 0; f < A.np; f++)
+	{
+		const f3 an = mmul(aToB.r, A.pn(f));
+		const float ad = A.pd(f) - dot(an, aToB.p);
+		for (int i = 0; i < B.nv; i++)
+		{
+			const f3 vb = B.vert(i);
+			const float h = dot(an, vb) + ad;
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const f3 q = vb - an * h;
+			if (!inPolygon(A, f, mxtransinv(aToB, q)))
+				continue;
+			d2 = h * h; cB = vb; cA = q;
+		}
+	}
+	for (int ea = 0; ea < A.ne; ea++)
+	{
+		const uint32_t e = A.edge(ea);
+		const f3 a0 = mxtrans(aToB, A.vert(int(e & 0xFF))), a1 = mxtrans(aToB, A.vert(int((e >> 8) & 0xFF)));
+		for (int ebi = 0; ebi < B.ne; ebi++)
+		{
+			const uint32_t g = B.edge(ebi);
+			f3 c1, c2;
+			const float s = segClosestSq(a0, a1, B.vert(int(g & 0xFF)), B.vert(int((g >> 8) & 0xFF)), c1, c2);
+			if (s < d2) { d2 = s; cA = c1; cB = c2; }
+		}
+	}
+	dist = sqrtf(d2);
+	n = (cA - cB) * (1.0f / dist);
+}
+
+// Gu::getWitnessPolygonIndex (geomutils/src/pcm/GuPCMContactGenUtil.cpp:201-271);
+// the per-polygon distances are evaluated again in the second pass
+PXH_D int witnessPolygon(const Poly& P, f3 normal, f3 closest, float tolerance)
+{
+	const float eps = -tolerance;
+	float dist = dot(P.pn(0), closest) + P.pd(0);
+	float minDist = dist >= eps ? fabsf(dist) : FLT_BIG;
+	float maxDist = dist;
+	int maxIdx = 0, idx = 0;
+	for (int i = 1; i < P.np; i++)
+	{
+		dist = dot(P.pn(i), closest) + P.pd(i);
+		const float pdi = dist >= eps ? fabsf(dist) : FLT_BIG;
+		if (minDist > pdi) { minDist = pdi; idx = i; }
+		if (dist > maxDist) { maxDist = dist; maxIdx = i; }
+	}
+	if (minDist == FLT_BIG)
+		return maxIdx;
+	float bestProj = dot(normalize(P.pn(idx)), normal);
+	const int first = idx;
+	for (int i = 0; i < P.np; i++)
+	{
+		const float di = dot(P.pn(i), closest) + P.pd(i);
+		const float pdi = di >= eps ? fabsf(di) : FLT_BIG;
+		if ((tolerance > (pdi - minDist)) && (first != i))
+		{
+			const float proj = dot(normalize(P.pn(i)), normal);
+			if (bestProj > proj) { idx = i; bestProj = proj; }
+		}
+	}
+	return idx;
+}
+
+// Gu::findRotationMatrixFromZAxis (geomutils/src/pcm/GuPersistentContactManifold.cpp:289-354):
+// a rotation whose third row is `to`
+PXH_D m33 rotFromZ(f3 to)
+{
+	m33 m;
+	const float e = to.z, f = fabsf(e);
+	if (0.9999f > f)
+	{
+		const float vx = -to.y, vy = to.x;
+		const float h = 1.0f / (1.0f + e);
+		const float hvx = h * vx, hvxy = hvx * vy;
+		m.c0 = mk(hvx * vx + e, hvxy, vy);
+		m.c1 = mk(hvxy, h * (vy * vy) + e, -vx);
+		m.c2 = mk(-vy, vx, e);
+		return m;
+	}
+	const f3 from = mk(0.0f, 0.0f, 1.0f), absFrom = mk(0.0f, 1.0f, 0.0f);
+	const f3 u = absFrom - from, v = absFrom - to;
+	const float du = dot(u, u), dv = dot(v, v), duv = dot(u, v);
+	const float c1 = -(2.0f / du), c2 = -(2.0f / dv), c3 = c1 * (c2 * duv);
+	const f3 c1u = u * c1, c2v = v * c2, c3v = v * c3;
+	m.c0 = u * c1u.x + (v * c2v.x + u * c3v.x);
+	m.c0.x += 1.0f;
+	m.c1 = u * c1u.y + (v * c2v.y + u * c3v.y);
+	m.c1.y += 1.0f;
+	m.c2 = u * c1u.z + (v * c2v.z + u * c3v.z);
+	m.c2.z += 1.0f;
+	return m;
+}
+
+PXH_D float triArea2(f3 a, f3 b, f3 c)
+{
+	const f3 ca = a - c, cb = b - c;
+	return ca.x * cb.y - ca.y * cb.x;
+}
+
+// Gu::contains (geomutils/src/pcm/GuPCMContactGenUtil.cpp:35-103) for an n-gon
+PXH_D bool polyContainsN(const f3* verts, int nv, f3 p, f3 mn, f3 mx)
+{
+	if ((mn.x > p.x || p.x > mx.x) || (mn.y > p.y || p.y > mx.y))
+		return false;
+	const float tx = p.x, ty = p.y, eps = 1.1920928955078125e-7f;
+	int inter = 0;
+	for (int i = 0, j = nv - 1; i < nv; j = i++)
+	{
+		const float jy = verts[j].y, iy = verts[i].y, jx = verts[j].x, ix = verts[i].x;
+		if ((tx == jx && ty == jy) || (tx == ix && ty == iy))
+			return true;
+		if ((jy > ty) != (iy > ty))
+		{
+			const float jix = ix - jx, jiy = iy - jy, jty = ty - jy;
+			const float part1 = jty * jix, part2 = (jx + eps) * jiy, part3 = tx * jiy;
+			const bool cmp = jiy > 0.0f;
+			const float tmp = part1 + part2;
+			if ((cmp ? tmp : part3) >= (cmp ? part3 : tmp))
+			{
+				if (inter == 1)
+					return false;
+				inter++;
+			}
+		}
+	}
+	return inter > 0;
+}
+
+// generatedContacts (geomutils/src/pcm/GuPCMContactGenBoxConvex.cpp:331-530):
+// reference polygon `ref` of P0, incident polygon `inc` of P1, t0To1 maps
+// P0's frame to P1's. Points keep la on P1 (P1's frame), lb on P0's
+// reference plane (P0's frame), ln the reference normal (P0's frame).
+PXH_D void generatedContacts(const Poly& P0, const Poly& P1, int ref, int inc, const mxf& t0To1, PC* mc, int& nc,
+                             float cd)
+{
+	const float feps = 1.1920928955078125e-7f;
+	const f3 n = normalize(P0.pn(ref));
+	const m33 rot = rotFromZ(n);
+	const int nr = P0.pcount(ref), ni = P1.pcount(inc);
+	f3 p0[HULL_MAX_POLY_VERTS], p1[HULL_MAX_POLY_VERTS];
+	float t1[HULL_MAX_POLY_VERTS];
+	bool pen1[HULL_MAX_POLY_VERTS];
+	const f3 sPoint = P1.vert(P1.pvert(inc, 0));
+	f3 rMin = mk(FLT_BIG, FLT_BIG, FLT_BIG), rMax = -rMin;
+	for (int i = 0; i < nr; i++)
+	{
+		p0[i] = mmul(rot, P0.vert(P0.pvert(ref, i)));
+		rMin = mk(fminf(rMin.x, p0[i].x), fminf(rMin.y, p0[i].y), fminf(rMin.z, p0[i].z));
+		rMax = mk(fmaxf(rMax.x, p0[i].x), fmaxf(rMax.y, p0[i].y), fmaxf(rMax.z, p0[i].z));
+	}
+	rMin = rMin - mk(feps, feps, feps);
+	rMax = rMax + mk(feps, feps, feps);
+	const float d = p0[0].z, rd = d + cd;
+	f3 iMin = mk(FLT_BIG, FLT_BIG, FLT_BIG), iMax = -iMin;
+	int inside = 0;
+	for (int i = 0; i < ni; i++)
+	{
+		const f3 vert1 = P1.vert(P1.pvert(inc, i));
+		const f3 a = mxtransinv(t0To1, vert1);
+		f3 q = mmul(rot, a);
+		const float z = q.z;
+		t1[i] = z - d;
+		q.z = d;
+		p1[i] = q;
+		iMin = mk(fminf(iMin.x, q.x), fminf(iMin.y, q.y), fminf(iMin.z, q.z));
+		iMax = mk(fmaxf(iMax.x, q.x), fmaxf(iMax.y, q.y), fmaxf(iMax.z, q.z));
+		pen1[i] = rd > z;
+		if (pen1[i] && polyContainsN(p0, nr, q, rMin, rMax))
+		{
+			inside++;
+			if (nc == HULL_MAX_CONTACTS)
+				return;
+			PC& c = mc[nc++];
+			c.la = vert1;
+			c.lb = mtmul(rot, q);
+			c.ln = n;
+			c.pen = t1[i];
+		}
+	}
+	if (inside == ni)
+		return;
+	inside = 0;
+	iMin = iMin - mk(feps, feps, feps);
+	iMax = iMax + mk(feps, feps, feps);
+	const f3 incN = normalize(P1.pn(inc));
+	const f3 nIn1 = mmul(t0To1.r, n);
+	for (int i = 0; i < nr; i++)
+	{
+		if (!polyContainsN(p1, ni, p0[i], iMin, iMax))
+			continue;
+		const f3 vert0 = mtmul(rot, p0[i]);
+		const f3 a = mxtrans(t0To1, vert0);
+		const float t = dot(incN, sPoint - a) / dot(incN, nIn1);
+		if (t > cd)
+			continue;
+		inside++;
+		if (nc == HULL_MAX_CONTACTS)
+			return;
+		PC& c = mc[nc++];
+		c.la = nIn1 * t + a;
+		c.lb = vert0;
+		c.ln = n;
+		c.pen = t;
+	}
+	if (inside == nr)
+		return;
+	for (int iS = 0, iE = ni - 1; iS < ni; iE = iS++)
+	{
+		if (!pen1[iS] && !pen1[iE])
+			continue;
+		const f3 ipA = p1[iS], ipB = p1[iE];
+		const f3 ipAO = mk(ipA.x, ipA.y, t1[iS] + d), ipBO = mk(ipB.x, ipB.y, t1[iE] + d);
+		const float iMnx = fminf(ipA.x, ipB.x), iMny = fminf(ipA.y, ipB.y);
+		const float iMxx = fmaxf(ipA.x, ipB.x), iMxy = fmaxf(ipA.y, ipB.y);
+		for (int rS = 0, rE = nr - 1; rS < nr; rE = rS++)
+		{
+			const f3 rpA = p0[rS], rpB = p0[rE];
+			const float rMnx = fminf(rpA.x, rpB.x), rMny = fminf(rpA.y, rpB.y);
+			const float rMxx = fmaxf(rpA.x, rpB.x), rMxy = fmaxf(rpA.y, rpB.y);
+			if (iMnx > rMxx || iMny > rMxy || rMnx > iMxx || rMny > iMxy)
+				continue;
+			const float a1 = triArea2(rpA, rpB, ipA), a2 = triArea2(rpA, rpB, ipB);
+			if (!(0.0f > a1 * a2))
+				continue;
+			const float a3 = triArea2(ipA, ipB, rpA), a4 = triArea2(ipA, ipB, rpB);
+			if (!(0.0f > a3 * a4))
+				continue;
+			const float t = a1 / (a2 - a1);
+			const f3 pBB = ipAO - (ipBO - ipAO) * t;
+			const f3 pAA = mk(pBB.x, pBB.y, d);
+			const float pen = pBB.z - pAA.z;
+			if (pen > cd)
+				continue;
+			if (nc == HULL_MAX_CONTACTS)
+				return;
+			PC& c = mc[nc++];
+			c.la = mxtrans(t0To1, mtmul(rot, pBB));
+			c.lb = mtmul(rot, pAA);
+			c.ln = n;
+			c.pen = pen;
+		}
+	}
+}
+
+// reduction to four (PersistentContactManifold::addBatchManifoldContacts,
+// geomutils/src/pcm/GuPersistentContactManifold.cpp:805-830)
+PXH_D int batchReduce(PC* mc, int nc, float tol)
+{
+	if (nc <= 4)
+		return nc;
+	int ch[4];
+	reduceBatchT<HULL_MAX_CONTACTS>(mc, nc, tol, ch);
+	PC out[4];
+	for (int i = 0; i < 4; i++)
+		out[i] = mc[ch[i]];
+	for (int i = 0; i < 4; i++)
+		mc[i] = out[i];
+	return 4;
+}
+
+// manifold -> contacts: points on shape1 in the world, the summed normal,
+// points within the contact distance (PersistentContactManifold::
+// addManifoldContactsToContactBuffer / getWorldNormal,
+// geomutils/src/pcm/GuPersistentContactManifold.cpp:739-762, GuPersistentContactManifold.h:405-416)
+PXH_D int manifoldOut(const PC* m, int n, const xf& x1, float cd, Contacts& o)
+{
+	if (n == 0)
+		return 0;
+	f3 s = m[0].ln;
+	for (int i = 1; i < n; i++)
+		s = s + m[i].ln;
+	o.n = normalize(qrot(x1.q, s));
+	int k = 0;
+	for (int i = 0; i < n && k < 4; i++)
+		if (cd >= m[i].pen)
+		{
+			o.p[k] = xtrans(x1, m[i].lb);
+			o.sep[k] = m[i].pen;
+			k++;
+		}
+	o.count = k;
+	return k;
+}
+
+// Gu::pcmContactConvexConvex / pcmContactBoxConvex on a fresh manifold
+// (geomutils/src/pcm/GuPCMContactConvexConvex.cpp:111-276, GuPCMContactBoxConvex.cpp;
+// Gu::generateFullContactManifold and Gu::addGJKEPAContacts,
+// GuPCMContactGenBoxConvex.cpp:532-720). A is shape0, B is shape1.
+PXH_D int convexConvex(const Poly& A, const xf& x0, const Poly& B, const xf& x1, float cd, float tol, Contacts& o)
+{
+	const mxf aToB = mxfOf(xinvmul(x1, x0));
+	float dist = 0.0f;
+	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
+	polySeparation(A, aToB, B, cd, dist, n, cA, cB);
+	if (dist > cd)
+		return 0;
+	const float lo = tol * 1e-2f, hi = tol * 5e-2f;
+	const float tA = fminf(fmaxf(A.margin, lo), hi), tB = fminf(fmaxf(B.margin, lo), hi);
+	const f3 negN = -n;
+	const f3 nIn0 = mtmul(aToB.r, n);
+	const int f1 = witnessPolygon(B, negN, cB, tB);
+	const int f0 = witnessPolygon(A, nIn0, mxtransinv(aToB, cA), tA);
+	const f3 refN = normalize(B.pn(f1)), incN = normalize(A.pn(f0));
+	PC mc[HULL_MAX_CONTACTS];
+	int nc = 0;
+	if (fabsf(dot(refN, negN)) >= fabsf(dot(incN, nIn0)))
+	{
+		const mxf bToA = mxfOf(xinvmul(x0, x1));
+		generatedContacts(B, A, f1, f0, bToA, mc, nc, cd);
+	}
+	else
+	{
+		generatedContacts(A, B, f0, f1, aToB, mc, nc, cd);
+		const f3 nn = -mmul(aToB.r, incN);
+		for (int i = 0; i < nc; i++)
+		{
+			const f3 t = mc[i].lb;
+			mc[i].lb = mc[i].la;
+			mc[i].la = t;
+			mc[i].ln = nn;
+		}
+	}
+	if (nc == 0)
+	{
+		// the closest-feature / minimum-translation point stays the only point
+		mc[0].la = mxtransinv(aToB, cA);
+		mc[0].lb = cB;
+		mc[0].ln = n;
+		mc[0].pen = dist;
+		nc = 1;
+	}
+	else
+		nc = batchReduce(mc, nc, tol);
+	return manifoldOut(mc, nc, x1, cd, o);
+}
+
+// Gu::pcmContactSphereConvex on a fresh manifold
+// (geomutils/src/pcm/GuPCMContactSphereConvex.cpp:110-246)
+PXH_D int sphereConvex(float r, f3 p0, const Poly& B, const xf& x1, float cd, Contacts& o)
+{
+	const f3 c = xtransinv(x1, p0);
+	float hmax = -FLT_BIG;
+	int fm = 0;
+	for (int f = 0; f < B.np; f++)
+	{
+		const float h = dot(B.pn(f), c) + B.pd(f);
+		if (h > hmax) { hmax = h; fm = f; }
+	}
+	f3 n;
+	float pen;
+	if (hmax <= 0.0f)
+	{
+		n = B.pn(fm);
+		pen = hmax - r;
+	}
+	else
+	{
+		float d2 = FLT_BIG;
+		f3 q = c;
+		for (int f = 0; f < B.np; f++)
+		{
+			const f3 bn = B.pn(f);
+			const float h = dot(bn, c) + B.pd(f);
+			if (h < 0.0f || h * h >= d2)
+				continue;
+			const f3 pq = c - bn * h;
+			if (inPolygon(B, f, pq)) { d2 = h * h; q = pq; }
+		}
+		for (int e = 0; e < B.ne; e++)
+		{
+			const uint32_t g = B.edge(e);
+			f3 c1, c2;
+			const float s = segClosestSq(c, c, B.vert(int(g & 0xFF)), B.vert(int((g >> 8) & 0xFF)), c1, c2);
+			if (s < d2) { d2 = s; q = c2; }
+		}
+		const float dist = sqrtf(d2);
+		if (dist > r + cd)
+			return 0;
+		n = (c - q) * (1.0f / dist);
+		pen = dist - r;
+	}
+	const f3 wn = qrot(x1.q, n);
+	o.n = wn;
+	o.p[0] = p0 - wn * r;
+	o.sep[0] = pen;
+	o.count = 1;
+	return 1;
+}
+
+// Gu::pcmContactPlaneConvex on a fresh manifold
+// (geomutils/src/pcm/GuPCMContactPlaneConvex.cpp:36-227): the face most
+// opposed to the plane normal, or the better face of the edge whose summed
+// face normals are most opposed to it (5e-4 bias towards edges); the chosen
+// face's vertices below the contact distance, taken once more when an edge
+// was chosen (the reference passes over the same face twice); reduced to
+// four. Normal from the hull towards the plane.
+PXH_D int planeConvex(f3 pn, float pd, const Poly& B, const xf& tb, float cd, float tol, Contacts& o)
+{
+	const f3 n = normalize(qrotinv(tb.q, pn));
+	const f3 nn = -n;
+	float minProj = FLT_BIG;
+	int face = 0;
+	for (int i = 0; i < B.np; i++)
+	{
+		const float proj = dot(n, B.pn(i));
+		if (minProj > proj) { minProj = proj; face = i; }
+	}
+	int edgeSel = -1;
+	minProj = minProj - 5e-4f;
+	float maxDpSq = minProj * minProj;
+	for (int e = 0; e < B.ne; e++)
+	{
+		const uint32_t g = B.edge(e);
+		const f3 en = B.pn(int((g >> 16) & 0xFF)) + B.pn(int(g >> 24));
+		const float enMagSq = dot(en, en);
+		const float dp = dot(en, nn);
+		const float sqDp = dp * dp;
+		if (dp >= 0.0f && sqDp > maxDpSq * enMagSq)
+		{
+			maxDpSq = sqDp / enMagSq;
+			edgeSel = e;
+		}
+	}
+	int passes = 1;
+	if (edgeSel >= 0)
+	{
+		const uint32_t g = B.edge(edgeSel);
+		const int f0 = int((g >> 16) & 0xFF), f1 = int(g >> 24);
+		face = dot(B.pn(f0), nn) > dot(B.pn(f1), nn) ? f0 : f1;
+		passes = 2;
+	}
+	PC mc[HULL_MAX_CONTACTS];
+	int nc = 0;
+	float deepest = FLT_BIG;
+	int dv = 0;
+	for (int i = 0; i < B.nv; i++)
+	{
+		const float s = dot(pn, xtrans(tb, B.vert(i))) + pd;
+		if (s < deepest) { deepest = s; dv = i; }
+	}
+	bool deepestTaken = false;
+	const int cnt = B.pcount(face);
+	for (int pass = 0; pass < passes; pass++)
+		for (int k = 0; k < cnt; k++)
+		{
+			const int vi = B.pvert(face, k);
+			const f3 lv = B.vert(vi);
+			const f3 pw = xtrans(tb, lv);
+			const float s = dot(pn, pw) + pd;
+			if (cd > s && nc < HULL_MAX_CONTACTS)
+			{
+				mc[nc].la = lv;
+				mc[nc].lb = pw - pn * s;
+				mc[nc].ln = pn;
+				mc[nc].pen = s;
+				nc++;
+				deepestTaken |= vi == dv;
+			}
+		}
+	// Deliberate deviation: when the chosen face does not hold the hull's
+	// deepest vertex (an irregular hull can rest on a vertex that the face most
+	// opposed to the plane does not contain), that vertex joins the batch. The
+	// reference scene's persistent manifold keeps such a vertex from earlier
+	// steps; a step that generates contacts afresh has to add it.
+	if (!deepestTaken && cd > deepest && nc < HULL_MAX_CONTACTS)
+	{
+		const f3 lv = B.vert(dv);
+		const f3 pw = xtrans(tb, lv);
+		mc[nc].la = lv;
+		mc[nc].lb = pw - pn * deepest;
+		mc[nc].ln = pn;
+		mc[nc].pen = deepest;
+		nc++;
+	}
+	nc = batchReduce(mc, nc, tol);
+	o.n = -pn;
+	int k = 0;
+	for (int i = 0; i < nc; i++)
+		if (cd >= mc[i].pen)
+		{
+			o.p[k] = mc[i].lb;
+			o.sep[k] = mc[i].pen;
+			k++;
+		}
+	o.count = k;
+	return k;
+}
+
+}  // namespace pxh

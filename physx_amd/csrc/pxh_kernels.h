@@ -6,6 +6,25 @@
 
 namespace pxh {
 
+// broadphase pair classes: sphere-sphere, sphere-box, box-box, sphere-convex,
+// box-convex, convex-convex (pairs are written class by class)
+constexpr int PAIR_CLASSES = 6;
+constexpr int PAIR_CLASS_FIRST_CONVEX = 3;
+
+// Scene-wide convex hull storage (structure of arrays; per hull ranges in info)
+struct HullSet
+{
+	const float4* v;         // vertices (xyz)
+	const float4* pl;        // polygon planes (n xyz outward, d)
+	const uint32_t* poly;    // first vref (relative to the hull's vref base) | count << 16
+	const uint8_t* vr;       // polygon vertex loops (hull-local vertex indices)
+	const uint32_t* edge;    // v0 | v1 << 8 | face left << 16 | face right << 24
+	const int4* info;        // vertex first, vertex count, polygon first, polygon count
+	const int4* info2;       // vref first, edge first, edge count, 0
+	const float4* bc;        // local bounds centre, GJK margin
+	const float4* be;        // local bounds extents
+};
+
 // TGS solver body state, one entry per dynamic body
 // (PxTGSSolverBodyVel + PxTGSSolverBodyTxInertia, physx/include/solver/PxSolverDefs.h)
 struct SolverBodies
@@ -63,8 +82,8 @@ void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st);
 void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2, int n, int bitBegin,
                        int bitEnd, uint32_t* hist, uint32_t* scanTmp, hipStream_t st, bool& resultInAlt);
 
-void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
-                   float4* bmax, hipStream_t st);
+void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float co,
+                   float4* bmin, float4* bmax, hipStream_t st);
 // rowOff: per-world row table (nScenes + 1), each world's slice capping its rows
 void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
                        float4* gridF, int2* gridI, const uint32_t* rowOff, hipStream_t st);
@@ -78,12 +97,24 @@ void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t to
 void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
                   const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
                   uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap, hipStream_t st);
-// pair p < min(cap, *dP) fills slot slotBase + p
-void launch_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
-                     const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP,
-                     hipStream_t st);
+// pair p < min(cap, *dP) fills slot slotBase + p; sphere/box pairs are those
+// below *dConvex (the first convex pair), convex pairs run in launch_np_convex
+void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                     const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
+                     float4* ctP, hipStream_t st);
+void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                      const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
+                      float4* ctN, float4* ctP, hipStream_t st);
+// body-plane pairs: launch_np_planes skips convex bodies, launch_np_planes_convex takes only them
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);
+void launch_np_planes_convex(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
+                             const float4* rot, const float4* geom, const float4* planes, const HullSet& hs, float cd,
+                             float tol, float4* ctN, float4* ctP, hipStream_t st);
+// contact records of the last step (128 bytes per constraint, constraint order)
+void launch_copy_contacts(int C, int maxPairs, const int* order, const int* conA, const int* conB, const int* conSlot,
+                          const float4* ctN, const float4* ctP, const float* nF, float4* out, uint32_t* nbPairs,
+                          hipStream_t st);
 // slots [0, Q) are body-plane pairs, body pairs follow up to the device count *dP
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st);
 // constraint list; conPKey: world-relative pair identity (world tag 16 | local a

@@ -15,6 +15,7 @@
 #include "pxh_kernels.h"
 #include <hip/hip_ext.h>
 #include "pxh_np.h"
+#include "pxh_convex.h"
 
 #include <algorithm>
 
@@ -209,21 +210,30 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* kin, 
 
 // Gu::computeBounds (geomutils/src/GuBounds.cpp:354-395) plus the broadphase's
 // contact-distance inflation (lowlevelaabb/src/BpBroadPhaseABP.cpp:1187-1192)
-__global__ void k_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin,
-                         float4* bmax)
+__global__ void k_bounds(int n, const float4* pos, const float4* rot, const float4* geom, HullSet hs, float co,
+                         float4* bmin, float4* bmax)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	const float4 g = geom[i];
-	const f3 p = mk(pos[i]);
+	f3 p = mk(pos[i]);
 	f3 w;
 	if (int(g.w) == GEOM_SPHERE)
 		w = mk(g.x, g.x, g.x);
 	else
 	{
 		const m33 R = qmat(mkq(rot[i]));
-		const f3 c0 = R.c0 * g.x, c1 = R.c1 * g.y, c2 = R.c2 * g.z;
+		f3 e = mk(g.x, g.y, g.z);
+		if (int(g.w) == GEOM_CONVEXMESH)
+		{
+			// hull bounds: the local box rotated about the pose
+			// (geomutils/src/GuBounds.cpp:136-140 transformNoEmptyTest)
+			const int h = int(g.x);
+			e = mk(hs.be[h]);
+			p = mmul(R, mk(hs.bc[h])) + p;
+		}
+		const f3 c0 = R.c0 * e.x, c1 = R.c1 * e.y, c2 = R.c2 * e.z;
 		w = mk(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y),
 		       fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z));
 	}
@@ -419,8 +429,14 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
 		p[i] = v;
 }
 
-// pair class from the two geometry types: 0 sphere-sphere, 1 sphere-box, 2 box-box
-PXH_D int pairClass(int t0, int t1) { return (t0 == GEOM_BOX) + (t1 == GEOM_BOX); }
+// pair class from the two geometry types: 0 sphere-sphere, 1 sphere-box,
+// 2 box-box, 3 sphere-convex, 4 box-convex, 5 convex-convex
+PXH_D int pairClass(int t0, int t1)
+{
+	const int nc = (t0 == GEOM_CONVEXMESH) + (t1 == GEOM_CONVEXMESH);
+	const int nb = (t0 == GEOM_BOX) + (t1 == GEOM_BOX);
+	return nc == 0 ? nb : (nc == 2 ? 5 : 3 + nb);
+}
 
 // Sweep of the sorted rows: lane s (body i) reports each overlapping body j in
 // its own and the 8 neighbouring rows whose (minX, index) sorts after its own,
@@ -447,13 +463,12 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 	const uint32_t ro = rowOff[sc];
 	const int lr = int(r - ro);
 	const int cy = lr % gi.x, cz = lr / gi.x;
-	uint32_t cnt[3] = {0u, 0u, 0u};
-	uint32_t off[3] = {0u, 0u, 0u};
-	if (WRITE)
+	uint32_t cnt[PAIR_CLASSES];
+	uint32_t off[PAIR_CLASSES];
+	for (int c = 0; c < PAIR_CLASSES; c++)
 	{
-		off[0] = offsets[s];
-		off[1] = offsets[n + s];
-		off[2] = offsets[2 * n + s];
+		cnt[c] = 0u;
+		off[c] = WRITE ? offsets[int64_t(c) * n + s] : 0u;
 	}
 	for (int dz = -1; dz <= 1; dz++)
 	{
@@ -500,12 +515,12 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 				const int c = pairClass(ti, tj);
 				if (WRITE)
 				{
-					// shape order of the reference's contact managers: a sphere-box
-					// pair runs sphere first (contact-method table order); two
+					// shape order of the reference's contact managers: the lower
+					// geometry type first (contact-method table order); two
 					// shapes of one type put the later-added body first
 					// (shouldSwapBodies, simulationcontroller/src/ScNPhaseCore.cpp:182-249)
 					uint32_t pa, pb;
-					if (c == 1) { pa = ti == GEOM_SPHERE ? i : jj; pb = ti == GEOM_SPHERE ? jj : i; }
+					if (ti != tj) { pa = ti < tj ? i : jj; pb = ti < tj ? jj : i; }
 					else { pa = max(i, jj); pb = min(i, jj); }
 					// pairs past the buffer's capacity are counted, not written
 					if (off[c] < pairCap)
@@ -518,11 +533,8 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 		}
 	}
 	if (!WRITE)
-	{
-		counts[s] = cnt[0];
-		counts[n + s] = cnt[1];
-		counts[2 * n + s] = cnt[2];
-	}
+		for (int c = 0; c < PAIR_CLASSES; c++)
+			counts[int64_t(c) * n + s] = cnt[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -539,11 +551,12 @@ PXH_D void storeContacts(int slot, const Contacts& c, int cnt, float4* ctN, floa
 // (lower geometry type first), so body a of a sphere-box pair is the sphere.
 // Lanes past the pair count on the device (*dP) exit; pair p fills slot
 // slotBase + p.
-__global__ void k_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
-                           const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP)
+__global__ void k_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                           const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
+                           float4* ctP)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
-	if (p >= cap || uint32_t(p) >= *dP)
+	if (p >= cap || uint32_t(p) >= *dConvex || uint32_t(p) >= *dP)
 		return;
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
@@ -580,8 +593,10 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 	if (q >= Q)
 		return;
 	const int i = qBody[q];
-	const float4 pl = planes[qPlane[q]];
 	const float4 g = geom[i];
+	if (int(g.w) == GEOM_CONVEXMESH)
+		return;
+	const float4 pl = planes[qPlane[q]];
 	Contacts c;
 	c.count = 0;
 	int cnt;
@@ -595,6 +610,92 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 		c.n = -c.n;
 	}
 	storeContacts(slotBase + q, c, cnt, ctN, ctP);
+}
+
+// Convex pairs: the class segments from *dConvex to *dP; lane q takes pair
+// *dConvex + q. Shape order as for the other pairs (sphere, then box, then
+// convex; two hulls: the later-added body first).
+__global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                            const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float tol,
+                            float4* ctN, float4* ctP)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t p = *dConvex + uint32_t(q);
+	if (q >= cap || p >= *dP || p >= uint32_t(cap))
+		return;
+	const uint2 ab = pairs[p];
+	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
+	const int t0 = int(g0.w);
+	xf x0, x1;
+	x0.p = mk(pos[ab.x]); x0.q = mkq(rot[ab.x]);
+	x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
+	const Poly B = hullPoly(hs, int(g1.x));
+	Contacts c;
+	c.count = 0;
+	int cnt;
+	if (t0 == GEOM_SPHERE)
+		cnt = sphereConvex(g0.x, x0.p, B, x1, cd, c);
+	else if (t0 == GEOM_BOX)
+		cnt = convexConvex(boxPoly(mk(g0.x, g0.y, g0.z)), x0, B, x1, cd, tol, c);
+	else
+		cnt = convexConvex(hullPoly(hs, int(g0.x)), x0, B, x1, cd, tol, c);
+	storeContacts(slotBase + int(p), c, cnt, ctN, ctP);
+}
+
+// Body-plane pairs of convex bodies (the others are left to k_np_planes); the
+// hull is constraint body a, so normals point from the plane to the hull.
+__global__ void k_np_planes_convex(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
+                                   const float4* rot, const float4* geom, const float4* planes, HullSet hs, float cd,
+                                   float tol, float4* ctN, float4* ctP)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q >= Q)
+		return;
+	const int i = qBody[q];
+	const float4 g = geom[i];
+	if (int(g.w) != GEOM_CONVEXMESH)
+		return;
+	const float4 pl = planes[qPlane[q]];
+	xf x;
+	x.p = mk(pos[i]); x.q = mkq(rot[i]);
+	Contacts c;
+	c.count = 0;
+	const int cnt = planeConvex(mk(pl), pl.w, hullPoly(hs, int(g.x)), x, cd, tol, c);
+	c.n = -c.n;
+	storeContacts(slotBase + q, c, cnt, ctN, ctP);
+}
+
+// Contact data of the last step, one 128-byte record per constraint in
+// constraint order (row o of the colour order holds constraint order[o]):
+// body0, body1 (or -1 - plane), contact count | normal (shape1 -> shape0) |
+// 4 x (point, separation) | 4 applied normal impulses | 0. Lane 0 stores the
+// number of records written.
+__global__ void k_copy_contacts(int C, int maxPairs, const int* order, const int* conA, const int* conB,
+                                const int* conSlot, const float4* ctN, const float4* ctP, const float* nF,
+                                float4* out, uint32_t* nbPairs)
+{
+	const int o = blockIdx.x * blockDim.x + threadIdx.x;
+	if (o == 0 && nbPairs)
+		*nbPairs = uint32_t(min(C, maxPairs));
+	if (o >= C)
+		return;
+	const int c = order[o];
+	if (c >= maxPairs)
+		return;
+	const int slot = conSlot[c];
+	const float4 n = ctN[slot];
+	const int cnt = __float_as_int(n.w);
+	float4* r = out + 8 * int64_t(c);
+	r[0] = make_float4(__int_as_float(conA[c]), __int_as_float(conB[c]), __int_as_float(cnt), 0.0f);
+	r[1] = make_float4(n.x, n.y, n.z, 0.0f);
+	float f[4];
+	for (int k = 0; k < 4; k++)
+	{
+		r[2 + k] = k < cnt ? ctP[4 * int64_t(slot) + k] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		f[k] = k < cnt ? nF[int64_t(k) * C + o] : 0.0f;
+	}
+	r[6] = make_float4(f[0], f[1], f[2], f[3]);
+	r[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
 // slots: [0, Q) body-plane pairs, then body pairs up to the device count *dP
@@ -1893,10 +1994,10 @@ void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t
 #define PXH_LAUNCH(kernel, n, ...) \
 	do { if ((n) > 0) kernel<<<nblk((n), 256), 256, 0, st>>>(__VA_ARGS__); } while (0)
 
-void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, float co, float4* bmin, float4* bmax,
-                   hipStream_t st)
+void launch_bounds(int n, const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float co,
+                   float4* bmin, float4* bmax, hipStream_t st)
 {
-	PXH_LAUNCH(k_bounds, n, n, pos, rot, geom, co, bmin, bmax);
+	PXH_LAUNCH(k_bounds, n, n, pos, rot, geom, hs, co, bmin, bmax);
 }
 void launch_scene_grid(int n, int nScenes, const int* scene, const float4* bmin, const float4* bmax, uint32_t* acc,
                        float4* gridF, int2* gridI, const uint32_t* rowOff, hipStream_t st)
@@ -1941,16 +2042,35 @@ void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, con
 		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs,
 		           pairCap);
 }
-void launch_np_pairs(int cap, const uint32_t* dP, int slotBase, const uint2* pairs, const float4* pos,
-                     const float4* rot, const float4* geom, float cd, float tol, float4* ctN, float4* ctP,
-                     hipStream_t st)
+void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                     const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
+                     float4* ctP, hipStream_t st)
 {
-	PXH_LAUNCH(k_np_pairs, cap, cap, dP, slotBase, pairs, pos, rot, geom, cd, tol, ctN, ctP);
+	PXH_LAUNCH(k_np_pairs, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, cd, tol, ctN, ctP);
+}
+void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                      const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
+                      float4* ctN, float4* ctP, hipStream_t st)
+{
+	PXH_LAUNCH(k_np_convex, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, ctN, ctP);
+}
+void launch_np_planes_convex(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
+                             const float4* rot, const float4* geom, const float4* planes, const HullSet& hs, float cd,
+                             float tol, float4* ctN, float4* ctP, hipStream_t st)
+{
+	PXH_LAUNCH(k_np_planes_convex, Q, Q, slotBase, qBody, qPlane, pos, rot, geom, planes, hs, cd, tol, ctN, ctP);
 }
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st)
 {
 	PXH_LAUNCH(k_np_planes, Q, Q, slotBase, qBody, qPlane, pos, rot, geom, planes, cd, ctN, ctP);
+}
+void launch_copy_contacts(int C, int maxPairs, const int* order, const int* conA, const int* conB, const int* conSlot,
+                          const float4* ctN, const float4* ctP, const float* nF, float4* out, uint32_t* nbPairs,
+                          hipStream_t st)
+{
+	const int n = std::max(C, 1);
+	k_copy_contacts<<<nblk(n, 256), 256, 0, st>>>(C, maxPairs, order, conA, conB, conSlot, ctN, ctP, nF, out, nbPairs);
 }
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st)
 {

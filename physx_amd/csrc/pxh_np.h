@@ -7,7 +7,7 @@
 
 namespace pxh {
 
-enum { GEOM_SPHERE = 0, GEOM_PLANE = 1, GEOM_CAPSULE = 2, GEOM_BOX = 3 };
+enum { GEOM_SPHERE = 0, GEOM_PLANE = 1, GEOM_CAPSULE = 2, GEOM_BOX = 3 };  // GEOM_CONVEXMESH = 5 in pxh_convex.h
 constexpr float FLT_BIG = 3.402823466e+38f;
 
 struct Contacts
@@ -119,7 +119,7 @@ PXH_D void reduceCluster(const PC* m, int n, int* idx)
 	v = m[i1].lb - m[i0].lb;
 	f3 nrm = cross(v, m[i0].ln);
 	const float sq = dot(nrm, nrm);
-	nrm = sq > 0.0f ? nrm * (1.0f / sqrtf(sq)) : m[i0].ln;
+	if (sq > 0.0f) { const float L = sqrtf(sq); nrm = mk(nrm.x / L, nrm.y / L, nrm.z / L); } else nrm = m[i0].ln;
 	float minDist = FLT_BIG;
 	int index1 = 0;
 	for (int i = 0; i < n; i++)
@@ -355,9 +355,10 @@ PXH_D void calcContacts(float exIn, float eyIn, const f3* pts, f3 incN, f3 ln, P
 
 // PersistentContactManifold::reduceBatchContacts
 // (geomutils/src/pcm/GuPersistentContactManifold.cpp:1008-1172)
-PXH_D void reduceBatch(const PC* m, int n, float tol, int* ch)
+template <int MAXN>
+PXH_D void reduceBatchT(const PC* m, int n, float tol, int* ch)
 {
-	int cand[16];
+	int cand[MAXN];
 	float maxPen = m[0].pen, minPen = maxPen;
 	int index = 0, ci = 0;
 	cand[0] = 0;
@@ -386,7 +387,7 @@ PXH_D void reduceBatch(const PC* m, int n, float tol, int* ch)
 	const f3 cn0 = m[ch[0]].ln;
 	f3 nrm = cross(v, cn0);
 	const float sq = dot(nrm, nrm);
-	nrm = sq > 0.0f ? nrm * (1.0f / sqrtf(sq)) : cn0;
+	if (sq > 0.0f) { const float L = sqrtf(sq); nrm = mk(nrm.x / L, nrm.y / L, nrm.z / L); } else nrm = cn0;
 	maxDist = -FLT_BIG; index = cand[0]; ci = 0;
 	float minDist = FLT_BIG;
 	int index1 = cand[0], ci1 = 0;
@@ -437,6 +438,8 @@ PXH_D void reduceBatch(const PC* m, int n, float tol, int* ch)
 		}
 	}
 }
+
+PXH_D void reduceBatch(const PC* m, int n, float tol, int* ch) { reduceBatchT<16>(m, n, tol, ch); }
 
 PXH_D void segClosest(f3 p1, f3 q1, f3 p2, f3 q2, f3& c1, f3& c2)
 {

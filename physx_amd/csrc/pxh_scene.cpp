@@ -7,6 +7,7 @@
 // PxPhysXGpu (physxgpu/src/PxgPhysXGpu.cpp:153-248) and CPU-side islands and
 // partitioning (PxgConstraintPartition.cpp:1940); here every stage runs on
 // the device and the host only reads back counts that size the next launches.
+#include "pxh_hull.h"
 #include "pxh_kernels.h"
 #include "../../include/pxhip.h"
 
@@ -118,6 +119,24 @@ struct pxh_scene_t
 	pxh_scene_desc desc;
 	hipStream_t stream = nullptr;
 	bool simulating = false;
+
+	// convex meshes (scene-wide, shared by any number of bodies)
+	std::vector<float4> hHullV, hHullP, hHullBC, hHullBE;
+	std::vector<uint32_t> hHullPoly, hHullEdge;
+	std::vector<uint8_t> hHullVR;
+	std::vector<int4> hHullInfo, hHullInfo2;
+	DBuf<float4> hullV, hullP, hullBC, hullBE;
+	DBuf<uint32_t> hullPoly, hullEdge;
+	DBuf<uint8_t> hullVR;
+	DBuf<int4> hullInfo, hullInfo2;
+	int nConvexBodies = 0;
+	pxh::HullSet hullSet() const
+	{
+		pxh::HullSet h;
+		h.v = hullV.p; h.pl = hullP.p; h.poly = hullPoly.p; h.vr = hullVR.p; h.edge = hullEdge.p;
+		h.info = hullInfo.p; h.info2 = hullInfo2.p; h.bc = hullBC.p; h.be = hullBE.p;
+		return h;
+	}
 
 	// bodies
 	int n = 0;
@@ -357,15 +376,15 @@ int bp_write_pairs(Ctx* s, int n, const int* d_scene, const int* d_group, const 
 // bounds (bmin/bmax, bmin.w = geometry type) of n boxes in nWorlds worlds
 // whose row table d_rowOff (nWorlds + 1 entries, totalRows in all) is already
 // on the device, produce the class-ordered pair list without a host round
-// trip: the pair count stays on the device at counts[3n] (class ends at
-// counts[n], counts[2n], counts[3n]) and at most pairCap pairs are written.
+// trip: the pair count stays on the device at counts[PAIR_CLASSES * n] (class
+// c ends at counts[(c + 1) * n]) and at most pairCap pairs are written.
 template <typename Ctx>
 int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_group, const uint32_t* d_rowOff,
                uint32_t totalRows, hipStream_t st, size_t pairCap)
 {
 	PXH_CHECK(s->gridF.ensure(nWorlds));
 	PXH_CHECK(s->gridI.ensure(nWorlds));
-	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(3) * n + 1, 256 * 1024)) + 1024));
+	PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(pxh::PAIR_CLASSES) * n + 1, 256 * 1024)) + 1024));
 	PXH_CHECK(s->gridAcc.ensure(size_t(5) * nWorlds));
 	pxh::launch_scene_grid(n, nWorlds, d_scene, s->bmin.p, s->bmax.p, s->gridAcc.p, s->gridF.p, s->gridI.p, d_rowOff,
 	                       st);
@@ -388,9 +407,9 @@ int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_grou
 	pxh::launch_gather_sorted(n, sv, s->bmin.p, s->bmax.p, s->smin.p, s->smax.p, st);
 	pxh::launch_row_start(n, s->sortedKeys, s->rowStart.p, totalRows, st);
 
-	const int64_t nc = int64_t(3) * n + 1;
+	const int64_t nc = int64_t(pxh::PAIR_CLASSES) * n + 1;
 	PXH_CHECK(s->counts.ensure(nc));
-	PXH_CHECK(hipMemsetAsync(s->counts.p + 3 * int64_t(n), 0, sizeof(uint32_t), st));
+	PXH_CHECK(hipMemsetAsync(s->counts.p + pxh::PAIR_CLASSES * int64_t(n), 0, sizeof(uint32_t), st));
 	pxh::launch_sweep(false, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, d_rowOff,
 	                  s->rowStart.p, s->counts.p, nullptr, nullptr, 0u, st);
 	pxh::launch_scan(s->counts.p, nc, s->scanTmp.p, st);
@@ -404,6 +423,16 @@ int gatherBuf(DBuf<T>& b, DBuf<unsigned char>& tmp, const int* idx, int n, hipSt
 	PXH_CHECK(tmp.ensure(sizeof(T) * size_t(std::max(n, 1))));
 	pxh::launch_gather(n, idx, b.p, tmp.p, int(sizeof(T)), st);
 	PXH_CHECK(hipMemcpyAsync(b.p, tmp.p, sizeof(T) * size_t(n), hipMemcpyDeviceToDevice, st));
+	return PXH_OK;
+}
+
+// whole host vector to a device buffer (grown as needed)
+template <typename T>
+int uploadVec(DBuf<T>& b, const std::vector<T>& v)
+{
+	PXH_CHECK(b.ensure(std::max<size_t>(v.size(), 1)));
+	if (!v.empty())
+		PXH_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
 	return PXH_OK;
 }
 
@@ -534,8 +563,12 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 		const int w = world ? world[i] : 0;
 		if (w < lastWorld || (i > 0 && w < hw[i - 1]) || w < 0)
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: world ids must be non-decreasing");
-		if (type[i] != PXH_GEOM_SPHERE && type[i] != PXH_GEOM_BOX)
-			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_bodies: only sphere and box shapes are implemented");
+		if (type[i] != PXH_GEOM_SPHERE && type[i] != PXH_GEOM_BOX && type[i] != PXH_GEOM_CONVEXMESH)
+			return fail(PXH_ERR_UNSUPPORTED,
+			            "pxh_scene_add_bodies: only sphere, box and convex mesh shapes are implemented");
+		if (type[i] == PXH_GEOM_CONVEXMESH &&
+		    !(dims[4 * i] >= 0.0f && int(dims[4 * i]) < int(s->hHullInfo.size()) && float(int(dims[4 * i])) == dims[4 * i]))
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: convex mesh index out of range");
 		if (!(mass[i] > 0.0f))
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: mass must be positive");
 		hw[i] = w;
@@ -583,6 +616,8 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 		PXH_CHECK(hipMemcpy(s->scene.p + base, hw.data(), n * sizeof(int), hipMemcpyHostToDevice));
 		PXH_CHECK(hipMemcpy(s->local.p + base, hloc.data(), n * sizeof(int), hipMemcpyHostToDevice));
 	}
+	for (int i = 0; i < n; i++)
+		s->nConvexBodies += type[i] == PXH_GEOM_CONVEXMESH;
 	s->n = int(nn);
 	s->topologyDirty = true;
 	return base;
@@ -622,6 +657,67 @@ int32_t pxh_scene_add_planes(pxh_scene s, int32_t m, const float* plane, const i
 	}
 	s->m = int(nm);
 	s->topologyDirty = true;
+	return PXH_OK;
+}
+
+int32_t pxh_scene_add_convex_meshes(pxh_scene s, int32_t k, const int32_t* counts, const float* points,
+                                    int32_t* first)
+{
+	if (!s || k < 0 || (k > 0 && (!counts || !points)))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_convex_meshes: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_convex_meshes: not allowed while simulating");
+	if (first)
+		*first = int32_t(s->hHullInfo.size());
+	const float* pts = points;
+	for (int m = 0; m < k; m++)
+	{
+		pxh::Hull h;
+		if (counts[m] < 4 || pxh::build_hull(pts, counts[m], h))
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_convex_meshes: degenerate or oversized point cloud");
+		pts += 3 * size_t(counts[m]);
+		for (const pxh::HullPolygon& hp : h.polys)
+			if (hp.count > 16)
+				return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_convex_meshes: polygon with more than 16 vertices");
+		const int nv = int(h.verts.size() / 3), np = int(h.polys.size()), ne = int(h.edgeVerts.size() / 2);
+		s->hHullInfo.push_back(make_int4(int(s->hHullV.size()), nv, int(s->hHullP.size()), np));
+		s->hHullInfo2.push_back(make_int4(int(s->hHullVR.size()), int(s->hHullEdge.size()), ne, 0));
+		for (int i = 0; i < nv; i++)
+			s->hHullV.push_back(make_float4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0f));
+		for (const pxh::HullPolygon& hp : h.polys)
+		{
+			s->hHullP.push_back(make_float4(hp.plane[0], hp.plane[1], hp.plane[2], hp.plane[3]));
+			s->hHullPoly.push_back(uint32_t(hp.first) | (uint32_t(hp.count) << 16));
+		}
+		s->hHullVR.insert(s->hHullVR.end(), h.vref.begin(), h.vref.end());
+		for (int e = 0; e < ne; e++)
+			s->hHullEdge.push_back(uint32_t(h.edgeVerts[2 * e]) | (uint32_t(h.edgeVerts[2 * e + 1]) << 8) |
+			                       (uint32_t(h.edgeFaces[2 * e]) << 16) | (uint32_t(h.edgeFaces[2 * e + 1]) << 24));
+		// GJK margin of ConvexHullV: 10% of the smallest internal extent
+		// (geomutils/src/gjk/GuVecConvexHull.h:76-93)
+		const float me = std::min(h.internalExtents[0], std::min(h.internalExtents[1], h.internalExtents[2]));
+		s->hHullBC.push_back(make_float4(h.aabbCenter[0], h.aabbCenter[1], h.aabbCenter[2], me * 0.1f));
+		s->hHullBE.push_back(make_float4(h.aabbExtents[0], h.aabbExtents[1], h.aabbExtents[2], 0.0f));
+	}
+	PXH_CHECK(hipStreamSynchronize(s->stream));
+	if (int r = uploadVec(s->hullV, s->hHullV))
+		return r;
+	if (int r = uploadVec(s->hullP, s->hHullP))
+		return r;
+	if (int r = uploadVec(s->hullBC, s->hHullBC))
+		return r;
+	if (int r = uploadVec(s->hullBE, s->hHullBE))
+		return r;
+	if (int r = uploadVec(s->hullPoly, s->hHullPoly))
+		return r;
+	if (int r = uploadVec(s->hullEdge, s->hHullEdge))
+		return r;
+	if (int r = uploadVec(s->hullVR, s->hHullVR))
+		return r;
+	if (int r = uploadVec(s->hullInfo, s->hHullInfo))
+		return r;
+	if (int r = uploadVec(s->hullInfo2, s->hHullInfo2))
+		return r;
 	return PXH_OK;
 }
 
@@ -760,13 +856,13 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	PXH_CHECK(s->bmin.ensure(n));
 	PXH_CHECK(s->bmax.ensure(n));
 	int pk = s->profBegin(0);
-	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, co, s->bmin.p, s->bmax.p, st);
+	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), co, s->bmin.p, s->bmax.p, st);
 	s->profEnd(pk);
 	s->pairCap = std::max(s->pairCap, size_t(4) * n);
 	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->rowOff.p, s->totalRows, st, s->pairCap);
 	if (r)
 		return r;
-	const uint32_t* dP = s->counts.p + 3 * int64_t(n);
+	const uint32_t* dP = s->counts.p + pxh::PAIR_CLASSES * int64_t(n);
 	const int Q = s->Q;
 	s->statPlanePairs = Q;
 
@@ -788,13 +884,20 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		}
 		PXH_CHECK(s->ctN.ensure(size_t(S)));
 		PXH_CHECK(s->ctP.ensure(size_t(4) * S));
+		const uint32_t* dConvex = s->counts.p + pxh::PAIR_CLASS_FIRST_CONVEX * int64_t(n);
 		pk = s->profBegin(3);
 		pxh::launch_np_planes(Q, 0, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p, cd, s->ctN.p,
 		                      s->ctP.p, st);
+		if (s->nConvexBodies > 0)
+			pxh::launch_np_planes_convex(Q, 0, s->qBody.p, s->qPlane.p, s->pos.p, s->rot.p, s->geom.p, s->planes.p,
+			                             s->hullSet(), cd, d.tolerance_length, s->ctN.p, s->ctP.p, st);
 		s->profEnd(pk);
 		pk = s->profBegin(2);
-		pxh::launch_np_pairs(cap, dP, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length, s->ctN.p,
-		                     s->ctP.p, st);
+		pxh::launch_np_pairs(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length,
+		                     s->ctN.p, s->ctP.p, st);
+		if (s->nConvexBodies > 0)
+			pxh::launch_np_convex(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), cd,
+			                      d.tolerance_length, s->ctN.p, s->ctP.p, st);
 		s->profEnd(pk);
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
@@ -1207,6 +1310,32 @@ int64_t pxh_scene_get_constraints(pxh_scene s, int32_t* out, int64_t maxc)
 	return C;
 }
 
+int32_t pxh_scene_copy_contact_data(pxh_scene s, void* data, uint32_t* nb_pairs, int32_t max_pairs, void* stream)
+{
+	if (!s || !data || max_pairs < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_copy_contact_data: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_copy_contact_data: not allowed while simulating");
+	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
+	if (stream)
+	{
+		hipEvent_t ev;
+		PXH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		PXH_CHECK(hipEventRecord(ev, s->stream));
+		PXH_CHECK(hipStreamWaitEvent(st, ev, 0));
+		PXH_CHECK(hipEventDestroy(ev));
+	}
+	const int C = int(s->statConstraints);
+	if (C > 0 && s->n > 0)
+		pxh::launch_copy_contacts(C, max_pairs, s->order.p, s->conA.p, s->conB.p, s->conSlot.p, s->ctN.p, s->ctP.p,
+		                          s->rnF.p, static_cast<float4*>(data), nb_pairs, st);
+	else if (nb_pairs)
+		PXH_CHECK(hipMemsetAsync(nb_pairs, 0, sizeof(uint32_t), st));
+	if (!stream)
+		PXH_CHECK(hipStreamSynchronize(st));
+	return PXH_OK;
+}
+
 void* pxh_scene_get_stream(pxh_scene s) { return s ? static_cast<void*>(s->stream) : nullptr; }
 
 // ---------------------------------------------------------------------------
@@ -1265,7 +1394,7 @@ int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* gr
 		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->rowOff.p, rows, st, bp->pairCap);
 		if (r)
 			return r;
-		const uint32_t* src[1] = {bp->counts.p + 3 * int64_t(n)};
+		const uint32_t* src[1] = {bp->counts.p + pxh::PAIR_CLASSES * int64_t(n)};
 		r = readCounts(st, bp->hostCounts, src, 1);
 		if (r)
 			return r;

@@ -6,7 +6,8 @@ parameters and does not depend on library-specific random distributions.
 The same arrays feed the GPU library and the reference CPU pipeline.
 
 Geometry type codes follow PxGeometryType (physx/include/geometry/PxGeometry.h):
-0 sphere, 1 plane, 2 capsule, 3 box.
+0 sphere, 1 plane, 2 capsule, 3 box, 5 convex mesh (dims[0] = the scene's
+mesh index; the meshes are point clouds in hull_points / hull_counts).
 """
 from __future__ import annotations
 
@@ -15,7 +16,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-SPHERE, PLANE, CAPSULE, BOX = 0, 1, 2, 3
+SPHERE, PLANE, CAPSULE, BOX, CONVEX = 0, 1, 2, 3, 5
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -55,6 +56,8 @@ class Scene:
     n_scenes: int = 1
     name: str = ""
     params: dict = field(default_factory=dict)
+    hull_points: np.ndarray | None = None   # (P,3) f32: point clouds of the convex meshes, concatenated
+    hull_counts: np.ndarray | None = None   # (H,) int32: points per mesh
 
     @property
     def n(self) -> int:
@@ -68,7 +71,7 @@ class Scene:
                      self.lin_vel[b].copy(), self.ang_vel[b].copy(), self.mass[b].copy(),
                      self.inertia[b].copy(), np.zeros(int(b.sum()), np.int32),
                      self.planes[p].copy(), np.zeros(int(p.sum()), np.int32), 1,
-                     f"{self.name}[{world}]", dict(self.params))
+                     f"{self.name}[{world}]", dict(self.params), self.hull_points, self.hull_counts)
 
 
 def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
@@ -190,3 +193,55 @@ def batched_piles(n_scenes=8192, nmin=8, nmax=512, seed=4) -> Scene:
     planes = np.tile(np.array([[0, 1, 0, 0]], np.float32), (n_scenes, 1))
     return _finish(st, dims, pos, quat, world, planes, sid, n_scenes, "batched_piles",
                    dict(n_scenes=n_scenes, nmin=nmin, nmax=nmax, seed=seed))
+
+
+def _rand_quat(seed: int, ids: np.ndarray, draw: int) -> np.ndarray:
+    """Uniform random rotations (Shoemake's subgroup algorithm), xyzw."""
+    u1, u2, u3 = (uniform(seed, ids, draw + k) for k in range(3))
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    t1, t2 = 2 * math.pi * u2, 2 * math.pi * u3
+    return np.stack([a * np.sin(t1), a * np.cos(t1), b * np.sin(t2), b * np.cos(t2)], 1)
+
+
+def convex_layers(n_bodies=65536, per_side=64, pitch=1.3, n_points=24, amin=0.2, amax=0.6, density=1000.0,
+                  seed=5) -> Scene:
+    """Convex hulls of n_points points on seeded ellipsoids (semi-axes
+    U[amin, amax] m, directions uniform on the sphere), dropped in layers of
+    per_side x per_side on a 1.3 m grid onto the plane y=0 with random
+    orientations. Each hull is expressed in its principal frame (points
+    shifted by the hull's centre of mass and rotated onto its principal axes),
+    so the body frame is the shape frame and the inertia is diagonal."""
+    from . import cook_convex
+    ids = np.arange(n_bodies)
+    ax = np.stack([uniform(seed, ids, d, amin, amax) for d in range(3)], 1)
+    gid = ((ids[:, None] << 8) + np.arange(n_points)[None, :]).ravel()
+    z = uniform(seed + 1, gid, 0, -1.0, 1.0).reshape(n_bodies, n_points)
+    phi = uniform(seed + 1, gid, 1, 0.0, 2 * math.pi).reshape(n_bodies, n_points)
+    rxy = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    pts = (np.stack([rxy * np.cos(phi), rxy * np.sin(phi), z], 2) * ax[:, None, :]).astype(np.float32)
+    com = np.zeros((n_bodies, 3))
+    tensor = np.zeros((n_bodies, 3, 3))
+    vol = np.zeros(n_bodies)
+    for i in range(n_bodies):
+        h = cook_convex(pts[i])
+        if h is None:
+            raise RuntimeError(f"convex_layers: degenerate point set for hull {i}")
+        com[i], tensor[i], vol[i] = h["com"], h["inertia"], h["volume"]
+    w, vecs = np.linalg.eigh(tensor)
+    flip = np.linalg.det(vecs) < 0
+    vecs[flip, :, 2] *= -1.0
+    local = np.einsum("nkc,ncd->nkd", pts.astype(np.float64) - com[:, None, :], vecs).astype(np.float32)
+    mass = (density * vol).astype(np.float32)
+    inertia = (density * w).astype(np.float32)
+    per_layer = per_side * per_side
+    lx, lz, ly = ids % per_side, (ids // per_side) % per_side, ids // per_layer
+    pos = np.stack([(lx - (per_side - 1) / 2) * pitch, 0.7 + ly * pitch, (lz - (per_side - 1) / 2) * pitch], 1)
+    quat = _rand_quat(seed + 2, ids, 0)
+    dims = np.zeros((n_bodies, 4), np.float32)
+    dims[:, 0] = ids
+    pose = np.concatenate([pos, quat], 1).astype(np.float32)
+    return Scene(np.full(n_bodies, CONVEX, np.int32), dims, pose, np.zeros((n_bodies, 3), np.float32),
+                 np.zeros((n_bodies, 3), np.float32), mass, inertia, np.zeros(n_bodies, np.int32),
+                 np.array([[0, 1, 0, 0]], np.float32), np.zeros(1, np.int32), 1, "convex_layers",
+                 dict(n_bodies=n_bodies, per_side=per_side, pitch=pitch, n_points=n_points, seed=seed),
+                 local.reshape(-1, 3), np.full(n_bodies, n_points, np.int32))

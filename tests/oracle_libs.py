@@ -61,8 +61,11 @@ def pxo():
         lib.pxo_world_create.restype = VP
         lib.pxo_world_step.restype = ctypes.c_int
         lib.pxo_world_constraints.restype = ctypes.c_longlong
+        lib.pxo_world_contacts.restype = ctypes.c_longlong
+        lib.pxo_hull_add.restype = ctypes.c_int
         for name in ("pxo_world_release", "pxo_world_add_bodies", "pxo_world_add_planes", "pxo_world_step",
-                     "pxo_world_get_bodies", "pxo_world_set_bodies", "pxo_world_stats", "pxo_world_constraints"):
+                     "pxo_world_get_bodies", "pxo_world_set_bodies", "pxo_world_stats", "pxo_world_constraints",
+                     "pxo_world_contacts"):
             getattr(lib, name).argtypes = None
         _pxo = lib
     return _pxo
@@ -83,6 +86,7 @@ def ref():
         lib.pxref_scene_step.restype = ctypes.c_double
         lib.pxref_dynamic_group.restype = ctypes.c_uint32
         lib.pxref_static_group.restype = ctypes.c_uint32
+        lib.pxref_convex_create.restype = ctypes.c_int
         _ref = lib
     return _ref
 
@@ -131,6 +135,47 @@ def bp_pairs_ref(bounds, dist, groups=None, pabp=False):
     return p[np.lexsort((p[:, 1], p[:, 0]))]
 
 
+def _vp(a):
+    return a.ctypes.data_as(VP)
+
+
+def pxo_add_hull(h):
+    """Register a cooked hull (dict of verts, planes, polys, vrefs, internal
+    extents) with the oracle; returns its index."""
+    v, pl = f32(h["verts"]), f32(h["planes"])
+    po, vr = i32(h["polys"]), i32(h["vrefs"])
+    ie = f32(h["internal_extents"])
+    return pxo().pxo_hull_add(int(v.shape[0]), _vp(v), int(pl.shape[0]), _vp(pl), _vp(po), _vp(vr), _vp(ie))
+
+
+def ref_cook(points):
+    """Cook a hull with the reference cooker; returns (mesh index, hull dict)."""
+    lib = ref()
+    pts = f32(points).reshape(-1, 3)
+    mid = lib.pxref_convex_create(_vp(pts), int(pts.shape[0]))
+    assert mid >= 0
+    V = np.zeros((255, 3), np.float32)
+    Pl = np.zeros((255, 4), np.float32)
+    Po = np.zeros((255, 2), np.int32)
+    R = np.zeros(1530, np.int32)
+    M = np.zeros(13, np.float64)
+    S = np.zeros(10, np.float32)
+    C = np.zeros(3, np.int32)
+    lib.pxref_convex_data(mid, _vp(V), _vp(Pl), _vp(Po), _vp(R), _vp(M), _vp(S), _vp(C))
+    nv, npol, nr = (int(x) for x in C)
+    return mid, dict(verts=V[:nv].copy(), planes=Pl[:npol].copy(), polys=Po[:npol].copy(), vrefs=R[:nr].copy(),
+                     poly_counts=Po[:npol, 1].copy(), mass=M[0], com=M[1:4].copy(),
+                     inertia_origin=M[4:13].reshape(3, 3).copy(), bounds_center=S[0:3].copy(),
+                     bounds_extents=S[3:6].copy(), internal_radius=float(S[6]), internal_extents=S[7:10].copy())
+
+
+def _hull_clouds(scene):
+    if scene.hull_counts is None or not len(scene.hull_counts):
+        return []
+    off = np.concatenate([[0], np.cumsum(scene.hull_counts)])
+    return [scene.hull_points[off[i]:off[i + 1]] for i in range(len(scene.hull_counts))]
+
+
 def default_desc(pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0)):
     d = PxoDesc()
     pxo().pxo_default_desc(ctypes.byref(d))
@@ -151,7 +196,15 @@ class OracleWorld:
         self.desc.order_salt = order_salt
         self.h = VP(self.lib.pxo_world_create(ctypes.byref(self.desc)))
         self.n = scene.n
-        self._keep = [i32(scene.shape_type), f32(scene.dims), f32(scene.pose), f32(scene.lin_vel),
+        dims = f32(scene.dims).copy()
+        clouds = _hull_clouds(scene)
+        if clouds:
+            # the hulls the GPU library cooks from the same point clouds
+            import physx_amd as P
+            ids = np.array([pxo_add_hull(P.cook_convex(c)) for c in clouds], np.float32)
+            cvx = scene.shape_type == 5
+            dims[cvx, 0] = ids[dims[cvx, 0].astype(np.int64)]
+        self._keep = [i32(scene.shape_type), dims, f32(scene.pose), f32(scene.lin_vel),
                       f32(scene.ang_vel), f32(scene.mass), f32(scene.inertia), i32(scene.scene_id)]
         t, dm, po, lv, av, m, inn, sc = self._keep
         self.lib.pxo_world_add_bodies(self.h, self.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn), ip(sc))
@@ -186,6 +239,13 @@ class OracleWorld:
         n = self.lib.pxo_world_constraints(self.h, ip(out), ctypes.c_longlong(cap))
         return out[:n]
 
+    def contact_data(self):
+        """(k, 32) records like the library's contact data (oracle constraint order)."""
+        cap = 1 << 21
+        out = np.zeros((cap, 32), np.float32)
+        n = self.lib.pxo_world_contacts(self.h, fp(out), ctypes.c_longlong(cap))
+        return out[:n].copy()
+
     def __del__(self):
         try:
             self.lib.pxo_world_release(self.h)
@@ -206,7 +266,13 @@ class RefScene:
         self.lib.pxref_scene_set_material(self.h, ctypes.c_float(friction), ctypes.c_float(friction),
                                           ctypes.c_float(0.0))
         self.n = scene.n
-        self._keep = [i32(scene.shape_type), f32(scene.dims), f32(scene.pose), f32(scene.lin_vel),
+        dims = f32(scene.dims).copy()
+        clouds = _hull_clouds(scene)
+        if clouds:
+            ids = np.array([ref_cook(c)[0] for c in clouds], np.float32)
+            cvx = scene.shape_type == 5
+            dims[cvx, 0] = ids[dims[cvx, 0].astype(np.int64)]
+        self._keep = [i32(scene.shape_type), dims, f32(scene.pose), f32(scene.lin_vel),
                       f32(scene.ang_vel), f32(scene.mass), f32(scene.inertia)]
         t, dm, po, lv, av, m, inn = self._keep
         self.lib.pxref_scene_add_bodies(self.h, self.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn),

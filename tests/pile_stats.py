@@ -3,13 +3,21 @@ long runs whose trajectories diverge chaotically but whose statistics agree."""
 import numpy as np
 
 
-def lowest_points(shape_type, dims, pose):
-    """Height of the lowest point of each sphere (type 0) or box (type 3)."""
+def lowest_points(shape_type, dims, pose, scene=None):
+    """Height of the lowest point of each sphere (type 0), box (type 3) or
+    convex mesh (type 5, vertices from scene.hull_points)."""
     x, y, z, w = pose[:, 3], pose[:, 4], pose[:, 5], pose[:, 6]
     # second row of the rotation matrix: world-y component of each body axis
     ry = np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1)
     reach = (np.abs(ry) * dims[:, :3]).sum(1)
     reach = np.where(shape_type == 0, dims[:, 0], reach)
+    cvx = np.nonzero(shape_type == 5)[0]
+    if len(cvx):
+        off = np.concatenate([[0], np.cumsum(scene.hull_counts)])
+        for i in cvx:
+            h = int(dims[i, 0])
+            v = scene.hull_points[off[h]:off[h + 1]]
+            reach[i] = -(v @ ry[i]).min()
     return pose[:, 1] - reach
 
 
@@ -17,7 +25,7 @@ def pile_stats(scene, pose, lin_vel, rest_speed=0.1):
     """height: mean body height; potential: total potential energy;
     penetration: worst floor penetration among resting bodies (speed below
     `rest_speed`); jitter: mean speed of the resting bodies."""
-    low = lowest_points(scene.shape_type, scene.dims, pose)
+    low = lowest_points(scene.shape_type, scene.dims, pose, scene)
     speed = np.linalg.norm(lin_vel, axis=1)
     rest = speed < rest_speed
     return {
