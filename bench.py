@@ -24,6 +24,10 @@ requests for scenes of log-uniform [16, 4096] bodies, each for k ~ U{1..32}
 steps at dt = 1/60, admitted against a 200 GB HBM budget into one batched
 step per serving iteration, with body-steps/s, serving-step and request
 latency percentiles.
+
+"convex" is the convex-heavy scene: 65,536 hulls of 24 points on seeded
+ellipsoids dropped in layers onto a plane, TGS 4/1, timed after 200 steps of
+settling, beside the reference CPU SDK stepping the same state.
 """
 from __future__ import annotations
 
@@ -185,6 +189,64 @@ def run_batched(args, P, scenes):
     return out
 
 
+def run_convex(args, P, scenes):
+    """Convex-heavy scene: 65,536 hulls of 24 points on seeded ellipsoids
+    (semi-axes U[0.2,0.6] m) dropped in layers onto a plane, TGS 4/1,
+    dt = 1/60; timed after the pile has formed, with the reference CPU SDK
+    stepping the same state on the host cores."""
+    import torch
+    dt = 1.0 / 60.0
+    t0 = time.perf_counter()
+    sc = scenes.convex_layers(n_bodies=args.convex_bodies)
+    log(f"convex: {sc.n} hulls generated in {time.perf_counter() - t0:.1f}s")
+    g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
+    for s in range(args.convex_warmup):
+        g.step(dt)
+        if (s + 1) % 100 == 0:
+            log(f"convex warmup {s + 1}/{args.convex_warmup} {g.stats()}")
+    torch.cuda.synchronize()
+    g.set_profiling(True)
+    times, pairs = [], 0
+    t0 = time.perf_counter()
+    for s in range(args.convex_steps):
+        a = time.perf_counter()
+        g.simulate(dt)
+        g.fetch_results(True)
+        times.append((time.perf_counter() - a) * 1e3)
+        pairs += g.stats()["constraints"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = g.profile()
+    st = g.stats()
+    ts = np.array(times)
+    out = {"workload": f"convex_layers: {sc.n} hulls of 24 points on ellipsoids U[0.2,0.6] m, layers onto a plane, "
+                       f"TGS 4/1, dt 1/60", "bodies": sc.n, "steps": args.convex_steps,
+           "warmup": args.convex_warmup, "body_steps_per_s": round(sc.n * args.convex_steps / elapsed, 1),
+           "ms_per_step": round(1e3 * elapsed / args.convex_steps, 3),
+           "p50_step_ms": round(float(np.percentile(ts, 50)), 3), "p99_step_ms": round(float(np.percentile(ts, 99)), 3),
+           "contact_pairs_per_s": round(pairs / elapsed, 1), "last_step_counts": st,
+           "stage_ms_per_step": {k: round(v[0] / args.convex_steps, 3) for k, v in prof.items()}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_libs as L
+        if L.have_ref():
+            pose, lv, av = g.bodies()
+            stt = scenes.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
+                               sc.plane_scene, 1, sc.name, {}, sc.hull_points, sc.hull_counts)
+            ref = L.RefScene(stt, "tgs", 4, 1, threads=args.cpu_threads)
+            ref.step(dt, 1)
+            total, _ = ref.step(dt, args.cpu_steps)
+            cpu = sc.n * args.cpu_steps / total
+            out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "body-steps/s", "cores": args.cpu_threads,
+                                   "kind": "reference", "ms_per_step": round(1e3 * total / args.cpu_steps, 2),
+                                   "sample": f"{args.cpu_steps} steps of the same pile state (+1 untimed), "
+                                             f"reference PhysX 5.6.1 CPU SDK cooking the same point clouds"}
+            out["speedup_vs_cpu"] = round(out["body_steps_per_s"] / cpu, 2)
+    g.close()
+    log(f"convex: {out}")
+    return out
+
+
 def run_served(args, P, scenes):
     """Served request stream: Poisson arrivals (rate lam per second, wall
     clock) of step requests for scenes of log-uniform [16, 4096] bodies, each
@@ -272,6 +334,10 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-convex", action="store_true")
+    ap.add_argument("--convex-bodies", type=int, default=65536)
+    ap.add_argument("--convex-warmup", type=int, default=200)
+    ap.add_argument("--convex-steps", type=int, default=40)
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[250.0, 1000.0, 4000.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-warmup-s", type=float, default=1.0)
@@ -374,6 +440,7 @@ def main():
     gpu.close()
     batched = None if args.no_batched else run_batched(args, P, scenes)
     served = None if args.no_served else run_served(args, P, scenes)
+    convex = None if args.no_convex else run_convex(args, P, scenes)
 
     sms = np.array(step_ms)
     out = {
@@ -401,6 +468,7 @@ def main():
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,
         "batched": batched,
         "served": served,
+        "convex": convex,
     }
     print(json.dumps(out), flush=True)
 

@@ -23,11 +23,15 @@ __constant__ const uint32_t kBoxEdges[12] = {0x03000300u, 0x04000203u, 0x0100010
                                              0x04010602u, 0x02010506u, 0x05010105u, 0x04020706u,
                                              0x03020407u, 0x05020504u, 0x04030307u, 0x05030400u};
 
-// One polytope: a hull of the set, or a box by its half extents.
+// One polytope: a hull (its arrays, in global memory or staged in LDS), or a
+// box by its half extents.
 struct Poly
 {
-	const HullSet* hs;
-	int vb, pb, rb, eb;     // bases of vertices, polygons, vrefs, edges
+	const float4* v;         // vertices
+	const float4* pl;        // planes
+	const uint32_t* poly;    // first vref | count << 16
+	const uint8_t* vr;       // vertex loops
+	const uint32_t* ed;      // edges
 	int nv, np, ne;
 	bool box;
 	f3 h;
@@ -38,7 +42,7 @@ struct Poly
 		if (box)
 			return mk((i == 1 || i == 2 || i == 5 || i == 6) ? h.x : -h.x,
 			          (i == 2 || i == 3 || i == 6 || i == 7) ? h.y : -h.y, i >= 4 ? h.z : -h.z);
-		return mk(hs->v[vb + i]);
+		return mk(v[i]);
 	}
 	PXH_D f3 pn(int f) const
 	{
@@ -47,29 +51,32 @@ struct Poly
 			const float s = (f == 1 || f == 2 || f == 4) ? 1.0f : -1.0f;
 			return (f == 1 || f == 3) ? mk(s, 0.0f, 0.0f) : ((f == 4 || f == 5) ? mk(0.0f, s, 0.0f) : mk(0.0f, 0.0f, s));
 		}
-		return mk(hs->pl[pb + f]);
+		return mk(pl[f]);
 	}
 	PXH_D float pd(int f) const
 	{
 		if (box)
 			return (f == 1 || f == 3) ? -h.x : ((f == 4 || f == 5) ? -h.y : -h.z);
-		return hs->pl[pb + f].w;
+		return pl[f].w;
 	}
-	PXH_D int pcount(int f) const { return box ? 4 : int(hs->poly[pb + f] >> 16); }
+	PXH_D int pcount(int f) const { return box ? 4 : int(poly[f] >> 16); }
 	PXH_D int pvert(int f, int k) const
 	{
-		return box ? int(kBoxQuads[4 * f + k]) : int(hs->vr[rb + int(hs->poly[pb + f] & 0xFFFFu) + k]);
+		return box ? int(kBoxQuads[4 * f + k]) : int(vr[int(poly[f] & 0xFFFFu) + k]);
 	}
-	PXH_D uint32_t edge(int e) const { return box ? kBoxEdges[e] : hs->edge[eb + e]; }
+	PXH_D uint32_t edge(int e) const { return box ? kBoxEdges[e] : ed[e]; }
 };
 
 PXH_D Poly hullPoly(const HullSet& hs, int id)
 {
 	Poly p;
-	p.hs = &hs;
 	const int4 a = hs.info[id], b = hs.info2[id];
-	p.vb = a.x; p.nv = a.y; p.pb = a.z; p.np = a.w;
-	p.rb = b.x; p.eb = b.y; p.ne = b.z;
+	p.v = hs.v + a.x;
+	p.pl = hs.pl + a.z;
+	p.poly = hs.poly + a.z;
+	p.vr = hs.vr + b.x;
+	p.ed = hs.edge + b.y;
+	p.nv = a.y; p.np = a.w; p.ne = b.z;
 	p.box = false;
 	p.h = mk(0.0f, 0.0f, 0.0f);
 	p.margin = hs.bc[id].w;
@@ -80,8 +87,9 @@ PXH_D Poly hullPoly(const HullSet& hs, int id)
 PXH_D Poly boxPoly(f3 h)
 {
 	Poly p;
-	p.hs = nullptr;
-	p.vb = p.pb = p.rb = p.eb = 0;
+	p.v = p.pl = nullptr;
+	p.poly = p.ed = nullptr;
+	p.vr = nullptr;
 	p.nv = 8; p.np = 6; p.ne = 12;
 	p.box = true;
 	p.h = h;
@@ -549,16 +557,12 @@ PXH_D int manifoldOut(const PC* m, int n, const xf& x1, float cd, Contacts& o)
 	return k;
 }
 
-// Gu::pcmContactConvexConvex / pcmContactBoxConvex on a fresh manifold
-// (geomutils/src/pcm/GuPCMContactConvexConvex.cpp:111-276, GuPCMContactBoxConvex.cpp;
-// Gu::generateFullContactManifold and Gu::addGJKEPAContacts,
-// GuPCMContactGenBoxConvex.cpp:532-720). A is shape0, B is shape1.
-PXH_D int convexConvex(const Poly& A, const xf& x0, const Poly& B, const xf& x1, float cd, float tol, Contacts& o)
+// Contacts of a box- or convex-convex pair from its separation (dist, n from
+// B towards A, closest points or minimum-translation witnesses cA, cB in B's
+// frame): witness polygons, clipping and reduction to four.
+PXH_D int convexContactsFrom(const Poly& A, const xf& x0, const Poly& B, const xf& x1, const mxf& aToB, float dist,
+                             f3 n, f3 cA, f3 cB, float cd, float tol, Contacts& o)
 {
-	const mxf aToB = mxfOf(xinvmul(x1, x0));
-	float dist = 0.0f;
-	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
-	polySeparation(A, aToB, B, cd, dist, n, cA, cB);
 	if (dist > cd)
 		return 0;
 	const float lo = tol * 1e-2f, hi = tol * 5e-2f;
@@ -599,6 +603,19 @@ PXH_D int convexConvex(const Poly& A, const xf& x0, const Poly& B, const xf& x1,
 	else
 		nc = batchReduce(mc, nc, tol);
 	return manifoldOut(mc, nc, x1, cd, o);
+}
+
+// Gu::pcmContactConvexConvex / pcmContactBoxConvex on a fresh manifold
+// (geomutils/src/pcm/GuPCMContactConvexConvex.cpp:111-276, GuPCMContactBoxConvex.cpp;
+// Gu::generateFullContactManifold and Gu::addGJKEPAContacts,
+// GuPCMContactGenBoxConvex.cpp:532-720). A is shape0, B is shape1.
+PXH_D int convexConvex(const Poly& A, const xf& x0, const Poly& B, const xf& x1, float cd, float tol, Contacts& o)
+{
+	const mxf aToB = mxfOf(xinvmul(x1, x0));
+	float dist = 0.0f;
+	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
+	polySeparation(A, aToB, B, cd, dist, n, cA, cB);
+	return convexContactsFrom(A, x0, B, x1, aToB, dist, n, cA, cB, cd, tol, o);
 }
 
 // Gu::pcmContactSphereConvex on a fresh manifold
@@ -751,6 +768,341 @@ PXH_D int planeConvex(f3 pn, float pd, const Poly& B, const xf& tb, float cd, fl
 		}
 	o.count = k;
 	return k;
+}
+
+}  // namespace pxh
+
+namespace pxh {
+
+// ---------------------------------------------------------------------------
+// One wavefront per convex pair: the 64 lanes split the face-axis, edge-pair
+// and closest-feature searches of polySeparation over the pair's vertices,
+// planes and edges staged in LDS; lane 0 then picks the witness polygons,
+// clips and reduces. Every search keeps polySeparation's order (the first
+// minimum in loop order wins), so the result is the same as the per-lane
+// path's.
+// ---------------------------------------------------------------------------
+constexpr int CVX_WAVE_V = 64, CVX_WAVE_P = 128, CVX_WAVE_E = 192;
+
+struct CvxLds
+{
+	float4 va[CVX_WAVE_V];   // A's vertices in B's frame
+	float4 vb[CVX_WAVE_V];   // B's vertices
+	float4 pa[CVX_WAVE_P];   // A's planes in B's frame
+	float4 pb[CVX_WAVE_P];   // B's planes
+	float4 vaL[CVX_WAVE_V];  // A's vertices and planes in its own frame
+	float4 paL[CVX_WAVE_P];
+	uint32_t polyA[CVX_WAVE_P], polyB[CVX_WAVE_P];
+	uint32_t ea[CVX_WAVE_E];
+	uint32_t eb[CVX_WAVE_E];
+	uint8_t vrA[2 * CVX_WAVE_E], vrB[2 * CVX_WAVE_E];   // a closed polytope has 2 vrefs per edge
+};
+
+PXH_D bool fitsWave(const Poly& A, const Poly& B)
+{
+	return A.nv <= CVX_WAVE_V && B.nv <= CVX_WAVE_V && A.np <= CVX_WAVE_P && B.np <= CVX_WAVE_P &&
+	       A.ne <= CVX_WAVE_E && B.ne <= CVX_WAVE_E;
+}
+
+// LDS written by some lanes of the wave becomes visible to the others
+PXH_D void waveSync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+}
+
+// smallest value over the wave, the lowest order index among equal values
+PXH_D void waveArgMin(float& v, uint32_t& idx)
+{
+	for (int off = 32; off >= 1; off >>= 1)
+	{
+		const float ov = __shfl_xor(v, off);
+		const uint32_t oi = __shfl_xor(idx, off);
+		if (ov < v || (ov == v && oi < idx))
+		{
+			v = ov;
+			idx = oi;
+		}
+	}
+}
+
+PXH_D bool waveAny(bool b) { return __ballot(b) != 0ull; }
+
+PXH_D f3 ldsV(const float4* a, int i) { return mk(a[i].x, a[i].y, a[i].z); }
+
+// polySeparation over the wave (same results); A, B in LDS as staged
+PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, float cd, const CvxLds& L, int lane,
+                              float& dist, f3& n, f3& cA, f3& cB)
+{
+	const float BIG = FLT_BIG;
+	float best = BIG;
+	uint32_t bi = 0xFFFFFFFFu;
+	bool far = false, sep = false;
+	// face axes of B (order 0 | f << 8 | vertex), then of A (order 1 | ...)
+	for (int f = lane; f < B.np; f += 64)
+	{
+		const float4 pl = L.pb[f];
+		float mn = BIG;
+		int vi = 0;
+		for (int i = 0; i < A.nv; i++)
+		{
+			const float4 v = L.va[i];
+			const float h = pl.x * v.x + pl.y * v.y + pl.z * v.z + pl.w;
+			if (h < mn) { mn = h; vi = i; }
+		}
+		far |= mn > cd;
+		sep |= mn > 0.0f;
+		if (!(mn > 0.0f) && -mn < best)
+		{
+			best = -mn;
+			bi = (uint32_t(f) << 8) | uint32_t(vi);
+		}
+	}
+	for (int f = lane; f < A.np; f += 64)
+	{
+		const float4 pl = L.pa[f];
+		float mn = BIG;
+		int vi = 0;
+		for (int i = 0; i < B.nv; i++)
+		{
+			const float4 v = L.vb[i];
+			const float h = pl.x * v.x + pl.y * v.y + pl.z * v.z + pl.w;
+			if (h < mn) { mn = h; vi = i; }
+		}
+		far |= mn > cd;
+		sep |= mn > 0.0f;
+		if (!(mn > 0.0f) && -mn < best)
+		{
+			best = -mn;
+			bi = (1u << 24) | (uint32_t(f) << 8) | uint32_t(vi);
+		}
+	}
+	if (waveAny(far))
+	{
+		dist = 2.0f * cd + 1.0f;   // farther apart than the contact distance
+		return;
+	}
+	sep = waveAny(sep);
+	if (!sep)
+	{
+		f3 bmean = mk(0.0f, 0.0f, 0.0f);
+		for (int i = 0; i < B.nv; i++)
+			bmean = bmean + ldsV(L.vb, i);
+		bmean = bmean * (1.0f / float(B.nv));
+		const int pairsE = A.ne * B.ne;
+		for (int k = lane; k < pairsE; k += 64)
+		{
+			const int ea = k / B.ne, ebi = k - ea * B.ne;
+			const uint32_t e = L.ea[ea], g = L.eb[ebi];
+			const f3 na = ldsV(L.pa, int((e >> 16) & 0xFF)), nb = ldsV(L.pa, int(e >> 24));
+			const f3 bxa = cross(nb, na);
+			const f3 nc = -ldsV(L.pb, int((g >> 16) & 0xFF)), nd = -ldsV(L.pb, int(g >> 24));
+			const f3 dxc = cross(nd, nc);
+			const float cba = dot(nc, bxa), dba = dot(nd, bxa), adc = dot(na, dxc), bdc = dot(nb, dxc);
+			if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
+				continue;
+			const f3 a0 = ldsV(L.va, int(e & 0xFF)), a1 = ldsV(L.va, int((e >> 8) & 0xFF));
+			const f3 b0 = ldsV(L.vb, int(g & 0xFF)), b1 = ldsV(L.vb, int((g >> 8) & 0xFF));
+			f3 ax = cross(a1 - a0, b1 - b0);
+			const float l2 = dot(ax, ax);
+			if (l2 < 1e-12f)
+				continue;
+			ax = ax * (1.0f / sqrtf(l2));
+			if (dot(ax, b0 - bmean) < 0.0f)
+				ax = -ax;
+			const float ov = dot(ax, b0) - dot(ax, a0);
+			if (ov < 0.0f)
+				sep = true;
+			else if (ov < best)
+			{
+				best = ov;
+				bi = (2u << 24) | uint32_t(ea * CVX_WAVE_E + ebi);
+			}
+		}
+		sep = waveAny(sep);
+	}
+	if (!sep)
+	{
+		waveArgMin(best, bi);
+		if (bi == 0xFFFFFFFFu)
+		{
+			dist = FLT_BIG;
+			return;
+		}
+		const uint32_t kind = bi >> 24;
+		if (kind == 2u)
+		{
+			const int k = int(bi & 0xFFFFFFu), ea = k / CVX_WAVE_E, ebi = k - ea * CVX_WAVE_E;
+			const uint32_t e = L.ea[ea], g = L.eb[ebi];
+			const f3 a0 = ldsV(L.va, int(e & 0xFF)), a1 = ldsV(L.va, int((e >> 8) & 0xFF));
+			const f3 b0 = ldsV(L.vb, int(g & 0xFF)), b1 = ldsV(L.vb, int((g >> 8) & 0xFF));
+			f3 bmean = mk(0.0f, 0.0f, 0.0f);
+			for (int i = 0; i < B.nv; i++)
+				bmean = bmean + ldsV(L.vb, i);
+			bmean = bmean * (1.0f / float(B.nv));
+			f3 ax = cross(a1 - a0, b1 - b0);
+			ax = ax * (1.0f / sqrtf(dot(ax, ax)));
+			if (dot(ax, b0 - bmean) < 0.0f)
+				ax = -ax;
+			f3 c1, c2;
+			segClosestSq(a0, a1, b0, b1, c1, c2);
+			n = ax;
+			cA = c1;
+			cB = c2;
+		}
+		else
+		{
+			const int f = int((bi >> 8) & 0xFFFFu), vi = int(bi & 0xFFu);
+			if (kind == 0u)
+			{
+				const float4 pl = L.pb[f];
+				const f3 bn = mk(pl.x, pl.y, pl.z), va = ldsV(L.va, vi);
+				const float mn = dot(bn, va) + pl.w;
+				n = bn;
+				cA = va;
+				cB = va - bn * mn;
+			}
+			else
+			{
+				const float4 pl = L.pa[f];
+				const f3 an = mk(pl.x, pl.y, pl.z), vb = ldsV(L.vb, vi);
+				const float mn = dot(an, vb) + pl.w;
+				n = -an;
+				cB = vb;
+				cA = vb - an * mn;
+			}
+		}
+		dist = -best;
+		return;
+	}
+	// separated: closest features (B faces x A vertices, A faces x B vertices,
+	// edge pairs), the first minimum in that order
+	float d2 = BIG;
+	uint32_t di = 0xFFFFFFFFu;
+	for (int k = lane; k < B.np * A.nv; k += 64)
+	{
+		const int f = k / A.nv, i = k - f * A.nv;
+		const float4 pl = L.pb[f];
+		const f3 bn = mk(pl.x, pl.y, pl.z), va = ldsV(L.va, i);
+		const float h = dot(bn, va) + pl.w;
+		if (h < 0.0f || h * h >= d2)
+			continue;
+		if (!inPolygon(B, f, va - bn * h))
+			continue;
+		d2 = h * h;
+		di = uint32_t(k);
+	}
+	for (int k = lane; k < A.np * B.nv; k += 64)
+	{
+		const int f = k / B.nv, i = k - f * B.nv;
+		const float4 pl = L.pa[f];
+		const f3 an = mk(pl.x, pl.y, pl.z), vb = ldsV(L.vb, i);
+		const float h = dot(an, vb) + pl.w;
+		if (h < 0.0f || h * h >= d2)
+			continue;
+		if (!inPolygon(A, f, mxtransinv(aToB, vb - an * h)))
+			continue;
+		d2 = h * h;
+		di = (1u << 24) | uint32_t(k);
+	}
+	for (int k = lane; k < A.ne * B.ne; k += 64)
+	{
+		const int ea = k / B.ne, ebi = k - ea * B.ne;
+		const uint32_t e = L.ea[ea], g = L.eb[ebi];
+		f3 c1, c2;
+		const float s = segClosestSq(ldsV(L.va, int(e & 0xFF)), ldsV(L.va, int((e >> 8) & 0xFF)),
+		                             ldsV(L.vb, int(g & 0xFF)), ldsV(L.vb, int((g >> 8) & 0xFF)), c1, c2);
+		if (s < d2)
+		{
+			d2 = s;
+			di = (2u << 24) | uint32_t(k);
+		}
+	}
+	waveArgMin(d2, di);
+	if (di == 0xFFFFFFFFu)
+	{
+		dist = FLT_BIG;   // no feature pair (an empty polytope)
+		return;
+	}
+	const uint32_t kind = di >> 24, k = di & 0xFFFFFFu;
+	if (kind == 0u)
+	{
+		const int f = int(k) / A.nv, i = int(k) - f * A.nv;
+		const float4 pl = L.pb[f];
+		const f3 bn = mk(pl.x, pl.y, pl.z), va = ldsV(L.va, i);
+		cA = va;
+		cB = va - bn * (dot(bn, va) + pl.w);
+	}
+	else if (kind == 1u)
+	{
+		const int f = int(k) / B.nv, i = int(k) - f * B.nv;
+		const float4 pl = L.pa[f];
+		const f3 an = mk(pl.x, pl.y, pl.z), vb = ldsV(L.vb, i);
+		cB = vb;
+		cA = vb - an * (dot(an, vb) + pl.w);
+	}
+	else
+	{
+		const int ea = int(k) / B.ne, ebi = int(k) - ea * B.ne;
+		const uint32_t e = L.ea[ea], g = L.eb[ebi];
+		segClosestSq(ldsV(L.va, int(e & 0xFF)), ldsV(L.va, int((e >> 8) & 0xFF)), ldsV(L.vb, int(g & 0xFF)),
+		             ldsV(L.vb, int((g >> 8) & 0xFF)), cA, cB);
+	}
+	dist = sqrtf(d2);
+	n = (cA - cB) * (1.0f / dist);
+}
+
+// polySeparation of a box- or convex-convex pair over the wave: dist and n
+// (sep = dist, n xyz) and the witnesses cA, cB in B's frame, the same on every lane
+PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, const xf& x1, float cd, CvxLds& L,
+                                int lane, float4& sep, f3& wA, f3& wB)
+{
+	const mxf aToB = mxfOf(xinvmul(x1, x0));
+	for (int i = lane; i < A.nv; i += 64)
+		L.va[i] = f4(mxtrans(aToB, A.vert(i)), 0.0f);
+	for (int i = lane; i < B.nv; i += 64)
+		L.vb[i] = f4(B.vert(i), 0.0f);
+	for (int f = lane; f < A.np; f += 64)
+	{
+		const f3 an = mmul(aToB.r, A.pn(f));
+		L.pa[f] = f4(an, A.pd(f) - dot(an, aToB.p));
+	}
+	for (int f = lane; f < B.np; f += 64)
+		L.pb[f] = f4(B.pn(f), B.pd(f));
+	for (int e = lane; e < A.ne; e += 64)
+		L.ea[e] = A.edge(e);
+	for (int e = lane; e < B.ne; e += 64)
+		L.eb[e] = B.edge(e);
+	// the polytopes' own arrays in LDS too (polygon tests, witness polygons,
+	// clipping), read through the same accessors
+	Poly Al = A, Bl = B;
+	if (!A.box)
+	{
+		for (int i = lane; i < A.nv; i += 64)
+			L.vaL[i] = A.v[i];
+		for (int f = lane; f < A.np; f += 64)
+		{
+			L.paL[f] = A.pl[f];
+			L.polyA[f] = A.poly[f];
+		}
+		for (int k = lane; k < 2 * A.ne; k += 64)
+			L.vrA[k] = A.vr[k];
+		Al.v = L.vaL; Al.pl = L.paL; Al.poly = L.polyA; Al.vr = L.vrA; Al.ed = L.ea;
+	}
+	for (int f = lane; f < B.np; f += 64)
+		L.polyB[f] = B.poly[f];
+	for (int k = lane; k < 2 * B.ne; k += 64)
+		L.vrB[k] = B.vr[k];
+	Bl.v = L.vb; Bl.pl = L.pb; Bl.poly = L.polyB; Bl.vr = L.vrB; Bl.ed = L.eb;
+	waveSync();
+	float dist = 0.0f;
+	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
+	polySeparationWave(Al, aToB, Bl, cd, L, lane, dist, n, cA, cB);
+	waveSync();   // the staged data stay in use until every lane is done
+	sep.x = dist;
+	sep.y = n.x; sep.z = n.y; sep.w = n.z;
+	wA = cA;
+	wB = cB;
 }
 
 }  // namespace pxh

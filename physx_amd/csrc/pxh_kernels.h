@@ -22,7 +22,7 @@ struct HullSet
 	const int4* info;        // vertex first, vertex count, polygon first, polygon count
 	const int4* info2;       // vref first, edge first, edge count, 0
 	const float4* bc;        // local bounds centre, GJK margin
-	const float4* be;        // local bounds extents
+	const float4* be;        // local bounds extents, radius about the shape origin
 };
 
 // TGS solver body state, one entry per dynamic body
@@ -102,9 +102,10 @@ void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, con
 void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                      const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
                      float4* ctP, hipStream_t st);
+// convex pairs; wit: 3 float4 per pair of scratch (separation records)
 void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                       const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
-                      float4* ctN, float4* ctP, hipStream_t st);
+                      float4* wit, float4* ctN, float4* ctP, hipStream_t st);
 // body-plane pairs: launch_np_planes skips convex bodies, launch_np_planes_convex takes only them
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);

@@ -612,9 +612,9 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 	storeContacts(slotBase + q, c, cnt, ctN, ctP);
 }
 
-// Convex pairs: the class segments from *dConvex to *dP; lane q takes pair
-// *dConvex + q. Shape order as for the other pairs (sphere, then box, then
-// convex; two hulls: the later-added body first).
+// Sphere-convex pairs: the class segments from *dConvex to *dP hold the
+// convex pairs; lane q takes pair *dConvex + q when its first shape is a
+// sphere (box- and convex-convex pairs run in k_np_convex_wave).
 __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                             const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float tol,
                             float4* ctN, float4* ctP)
@@ -625,20 +625,101 @@ __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex
 		return;
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
-	const int t0 = int(g0.w);
-	xf x0, x1;
-	x0.p = mk(pos[ab.x]); x0.q = mkq(rot[ab.x]);
+	if (int(g0.w) != GEOM_SPHERE)
+		return;
+	xf x1;
 	x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
-	const Poly B = hullPoly(hs, int(g1.x));
 	Contacts c;
 	c.count = 0;
-	int cnt;
+	const int cnt = sphereConvex(g0.x, mk(pos[ab.x]), hullPoly(hs, int(g1.x)), x1, cd, c);
+	storeContacts(slotBase + int(p), c, cnt, ctN, ctP);
+}
+
+// Box- and convex-convex pairs, separation phase: one wavefront per pair
+// (grid-stride over the convex segments) finds the closest features or the
+// minimum translation and stores (dist, n), cA, cB at wit[3p .. 3p+2];
+// polytopes past the LDS limits are searched by lane 0 alone. Pairs whose
+// bounding spheres are farther apart than the contact distance are marked
+// without a search.
+constexpr int CVX_WAVES = 4;
+__global__ void __launch_bounds__(64 * CVX_WAVES)
+k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, const uint2* pairs, const float4* pos,
+                 const float4* rot, const float4* geom, HullSet hs, float cd, float4* wit)
+{
+	__shared__ CvxLds lds[CVX_WAVES];
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const uint32_t end = min(*dP, uint32_t(cap));
+	for (uint32_t p = *dConvex + blockIdx.x * CVX_WAVES + w; p < end; p += gridDim.x * CVX_WAVES)
+	{
+		const uint2 ab = pairs[p];
+		const float4 g0 = geom[ab.x], g1 = geom[ab.y];
+		const int t0 = int(g0.w);
+		if (t0 == GEOM_SPHERE)
+			continue;
+		const f3 p0 = mk(pos[ab.x]), p1 = mk(pos[ab.y]);
+		const float r0 = t0 == GEOM_BOX ? sqrtf(g0.x * g0.x + g0.y * g0.y + g0.z * g0.z) : hs.be[int(g0.x)].w;
+		const float r1 = hs.be[int(g1.x)].w;
+		const f3 d = p1 - p0;
+		const float reach = r0 + r1 + cd;
+		float4 sep = make_float4(2.0f * cd + 1.0f, 1.0f, 0.0f, 0.0f);
+		f3 cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
+		if (dot(d, d) <= reach * reach)
+		{
+			xf x0, x1;
+			x0.p = p0; x0.q = mkq(rot[ab.x]);
+			x1.p = p1; x1.q = mkq(rot[ab.y]);
+			const Poly B = hullPoly(hs, int(g1.x));
+			const Poly A = t0 == GEOM_BOX ? boxPoly(mk(g0.x, g0.y, g0.z)) : hullPoly(hs, int(g0.x));
+			if (fitsWave(A, B))
+				convexSeparationWave(A, x0, B, x1, cd, lds[w], lane, sep, cA, cB);
+			else if (lane == 0)
+			{
+				const mxf aToB = mxfOf(xinvmul(x1, x0));
+				f3 n = mk(1.0f, 0.0f, 0.0f);
+				polySeparation(A, aToB, B, cd, sep.x, n, cA, cB);
+				sep.y = n.x; sep.z = n.y; sep.w = n.z;
+			}
+		}
+		if (lane == 0)
+		{
+			wit[3 * int64_t(p)] = sep;
+			wit[3 * int64_t(p) + 1] = f4(cA, 0.0f);
+			wit[3 * int64_t(p) + 2] = f4(cB, 0.0f);
+		}
+	}
+}
+
+// Box- and convex-convex pairs, contact phase: one lane per pair of the convex
+// segments builds the contacts from the separation record (witness polygons,
+// clipping, reduction).
+__global__ void k_np_convex_finish(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase,
+                                   const uint2* pairs, const float4* pos, const float4* rot, const float4* geom,
+                                   HullSet hs, float cd, float tol, const float4* wit, float4* ctN, float4* ctP)
+{
+	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t p = *dConvex + uint32_t(q);
+	if (q >= cap || p >= *dP || p >= uint32_t(cap))
+		return;
+	const uint2 ab = pairs[p];
+	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
+	const int t0 = int(g0.w);
 	if (t0 == GEOM_SPHERE)
-		cnt = sphereConvex(g0.x, x0.p, B, x1, cd, c);
-	else if (t0 == GEOM_BOX)
-		cnt = convexConvex(boxPoly(mk(g0.x, g0.y, g0.z)), x0, B, x1, cd, tol, c);
-	else
-		cnt = convexConvex(hullPoly(hs, int(g0.x)), x0, B, x1, cd, tol, c);
+		return;
+	const float4 sep = wit[3 * int64_t(p)];
+	Contacts c;
+	c.count = 0;
+	int cnt = 0;
+	if (sep.x <= cd)
+	{
+		xf x0, x1;
+		x0.p = mk(pos[ab.x]); x0.q = mkq(rot[ab.x]);
+		x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
+		const Poly B = hullPoly(hs, int(g1.x));
+		const Poly A = t0 == GEOM_BOX ? boxPoly(mk(g0.x, g0.y, g0.z)) : hullPoly(hs, int(g0.x));
+		const mxf aToB = mxfOf(xinvmul(x1, x0));
+		cnt = convexContactsFrom(A, x0, B, x1, aToB, sep.x, mk(sep.y, sep.z, sep.w), mk(wit[3 * int64_t(p) + 1]),
+		                         mk(wit[3 * int64_t(p) + 2]), cd, tol, c);
+	}
 	storeContacts(slotBase + int(p), c, cnt, ctN, ctP);
 }
 
@@ -2050,9 +2131,16 @@ void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int s
 }
 void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                       const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
-                      float4* ctN, float4* ctP, hipStream_t st)
+                      float4* wit, float4* ctN, float4* ctP, hipStream_t st)
 {
 	PXH_LAUNCH(k_np_convex, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, ctN, ctP);
+	if (cap > 0)
+	{
+		// grid-stride: enough waves to fill the chip several times over
+		const int blocks = std::min((cap + CVX_WAVES - 1) / CVX_WAVES, 8192);
+		k_np_convex_wave<<<blocks, 64 * CVX_WAVES, 0, st>>>(cap, dP, dConvex, pairs, pos, rot, geom, hs, cd, wit);
+	}
+	PXH_LAUNCH(k_np_convex_finish, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, wit, ctN, ctP);
 }
 void launch_np_planes_convex(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
                              const float4* rot, const float4* geom, const float4* planes, const HullSet& hs, float cd,

@@ -129,7 +129,8 @@ struct pxh_scene_t
 	DBuf<uint32_t> hullPoly, hullEdge;
 	DBuf<uint8_t> hullVR;
 	DBuf<int4> hullInfo, hullInfo2;
-	int nConvexBodies = 0;
+	DBuf<float4> cvxWit;     // separation records of the convex pairs (3 per pair)
+	int nConvexBodies = 0;   // convex bodies ever added (the convex kernels run when nonzero)
 	pxh::HullSet hullSet() const
 	{
 		pxh::HullSet h;
@@ -697,7 +698,13 @@ int32_t pxh_scene_add_convex_meshes(pxh_scene s, int32_t k, const int32_t* count
 		// (geomutils/src/gjk/GuVecConvexHull.h:76-93)
 		const float me = std::min(h.internalExtents[0], std::min(h.internalExtents[1], h.internalExtents[2]));
 		s->hHullBC.push_back(make_float4(h.aabbCenter[0], h.aabbCenter[1], h.aabbCenter[2], me * 0.1f));
-		s->hHullBE.push_back(make_float4(h.aabbExtents[0], h.aabbExtents[1], h.aabbExtents[2], 0.0f));
+		float rad = 0.0f;
+		for (int i = 0; i < nv; i++)
+			rad = std::max(rad, std::sqrt(h.verts[3 * i] * h.verts[3 * i] + h.verts[3 * i + 1] * h.verts[3 * i + 1] +
+			                              h.verts[3 * i + 2] * h.verts[3 * i + 2]));
+		// the bounding radius is padded by a relative 1e-5 so float rounding
+		// in the pair test never discards a touching pair
+		s->hHullBE.push_back(make_float4(h.aabbExtents[0], h.aabbExtents[1], h.aabbExtents[2], rad * 1.00001f + 1e-6f));
 	}
 	PXH_CHECK(hipStreamSynchronize(s->stream));
 	if (int r = uploadVec(s->hullV, s->hHullV))
@@ -896,8 +903,11 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		pxh::launch_np_pairs(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, cd, d.tolerance_length,
 		                     s->ctN.p, s->ctP.p, st);
 		if (s->nConvexBodies > 0)
+		{
+			PXH_CHECK(s->cvxWit.ensure(size_t(3) * std::max(cap, 1)));
 			pxh::launch_np_convex(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), cd,
-			                      d.tolerance_length, s->ctN.p, s->ctP.p, st);
+			                      d.tolerance_length, s->cvxWit.p, s->ctN.p, s->ctP.p, st);
+		}
 		s->profEnd(pk);
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
