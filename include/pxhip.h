@@ -74,7 +74,9 @@ int32_t pxh_scene_release(pxh_scene scene);
 
 /* Adds n dynamic rigid bodies with one shape each (PxPhysics::createRigidDynamic
  * + PxRigidActorExt::createExclusiveShape + PxScene::addActor). Host arrays:
- * type[n] (sphere/box), dims[n*4] (radius, or half extents), pose[n*7] (p xyz,
+ * type[n] (sphere, capsule, box or convex mesh), dims[n*4] (sphere: radius;
+ * capsule: radius, half height along the local x axis (PxCapsuleGeometry);
+ * box: half extents; convex mesh: index from pxh_scene_add_convex_meshes), pose[n*7] (p xyz,
  * q xyzw), lin_vel[n*3], ang_vel[n*3], mass[n], inertia[n*3] (mass-space
  * diagonal), world[n] (independent-world id, non-decreasing; NULL = world 0).
  * Returns the index of the first added body (its PxRigidDynamicGPUIndex). */

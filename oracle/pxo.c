@@ -198,6 +198,183 @@ static int c_sphere_box(float r, v3 p0, v3 h, xf t1, float cd, cset* o)
 	return 1;
 }
 
+/* ------------------------------------------------------------------------- */
+/* Capsules: the segment along the local x axis, endpoints p +- x*halfHeight  */
+/* (geomutils/src/GuInternal.h:60-70), inflated by the radius                 */
+/* ------------------------------------------------------------------------- */
+
+/* the vector library's min/max/clamp (SSE minps/maxps: the second operand
+ * when the comparison is unordered) */
+static inline float vmin(float a, float b) { return a < b ? a : b; }
+static inline float vmax(float a, float b) { return a > b ? a : b; }
+static inline float clamp01(float x) { return vmax(vmin(x, 1.0f), 0.0f); }
+
+static inline void capsule_segment(xf t, float hh, v3* s, v3* e)
+{
+	const v3 tmp = scl(qrot(t.q, V(1.0f, 0.0f, 0.0f)), hh);
+	*s = add(t.p, tmp);
+	*e = sub(t.p, tmp);
+}
+
+/* Gu::distanceSegmentSegmentSquared, vector form
+ * (geomutils/src/distance/GuDistanceSegmentSegment.cpp:411-470): closest
+ * parameters (s on p1 + s d1, t on p2 + t d2); parallel segments (a e - b^2
+ * below FLT_EPSILON) take s = 1/2 before the clamped refinement */
+static float seg_seg_sq(v3 p1, v3 d1, v3 p2, v3 d2, float* s, float* t)
+{
+	const float eps = 1.1920929e-07f;
+	const v3 r = sub(p1, p2);
+	const float a = dot(d1, d1), e = dot(d2, d2), b = dot(d1, d2), c = dot(d1, r);
+	const float aR = a > eps ? 1.0f / a : 0.0f;
+	const float eR = e > eps ? 1.0f / e : 0.0f;
+	const float f = dot(d2, r);
+	const float denom = a * e - b * b;
+	const float temp = b * f - c * e;
+	const float s0 = clamp01(temp / denom);
+	const float sT = eps > denom ? 0.5f : s0;
+	const float t2 = clamp01((b * sT + f) * eR);
+	const float s2 = clamp01((b * t2 - c) * aR);
+	*s = s2;
+	*t = t2;
+	const v3 vv = sub(add(scl(d1, s2), p1), add(scl(d2, t2), p2));
+	return dot(vv, vv);
+}
+
+/* geomutils/src/pcm/GuPCMContactSphereCapsule.cpp:38-96 */
+static int c_sphere_capsule(float r, v3 c, float cr, float hh, xf t1, float cd, cset* o)
+{
+	v3 s, e;
+	capsule_segment(t1, hh, &s, &e);
+	const float rs = r + cr, inf = rs + cd;
+	/* distancePointSegmentSquared (:38-53) */
+	const v3 ap = sub(c, s), ab = sub(e, s);
+	const float nom = dot(ap, ab), den = dot(ab, ab);
+	const float t = den == 0.0f ? 0.0f : clamp01(nom / den);
+	const v3 v = sub(ap, scl(ab, t));
+	const float d2 = dot(v, v);
+	if (!(inf * inf > d2))
+		return 0;
+	const v3 p = add(scl(sub(e, s), t), s);
+	const v3 dir = sub(c, p);
+	const float l = sqrtf(dot(dir, dir));
+	const v3 n = l > 1.1920929e-07f ? scl(dir, 1.0f / l) : V(1.0f, 0.0f, 0.0f); /* V3NormalizeSafe */
+	o->n = n;
+	o->p[0] = sub(c, scl(n, r));
+	o->sep[0] = sqrtf(d2) - rs;
+	o->count = 1;
+	return 1;
+}
+
+/* geomutils/src/pcm/GuPCMContactCapsuleCapsule.cpp:81-283: the closest points
+ * of the two segments, or, for segments within 0.9998 of parallel, the
+ * endpoints projecting inside the other segment (up to four points), all
+ * computed about the midpoint of the two centres */
+static int c_capsule_capsule(float r0, float hh0, xf x0, float r1, float hh1, xf x1, float cd, cset* o)
+{
+	const v3 off = scl(add(x0.p, x1.p), 0.5f);
+	xf a = x0, b = x1;
+	a.p = sub(x0.p, off);
+	b.p = sub(x1.p, off);
+	v3 s0, e0, s1, e1;
+	capsule_segment(a, hh0, &s0, &e0);
+	capsule_segment(b, hh1, &s1, &e1);
+	const v3 d0 = sub(e0, s0), d1 = sub(e1, s1);
+	const float sumR = r0 + r1, inf = sumR + cd, inf2 = inf * inf;
+	const float la = dot(d0, d0), le = dot(d1, d1);
+	const float eps = 1e-6f;
+	float t0, t1;
+	const float sq0 = seg_seg_sq(s0, d0, s1, d1, &t0, &t1);
+	if (!(inf2 >= sq0))
+		return 0;
+	o->count = 0;
+	const v3 dir0 = eps > la ? V(0, 0, 0) : scl(d0, 1.0f / sqrtf(la));
+	const v3 dir1 = eps > le ? V(0, 0, 0) : scl(d1, 1.0f / sqrtf(le));
+	if (fabsf(dot(dir0, dir1)) > 0.9998f)
+	{
+		/* pcmDistancePointSegmentTValue22 (:40-74): s1, e1 onto s0e0; s0, e0 onto s1e1 */
+		const float tv[4] = {
+			la == 0.0f ? 0.0f : dot(sub(s1, s0), d0) / la, la == 0.0f ? 0.0f : dot(sub(e1, s0), d0) / la,
+			le == 0.0f ? 0.0f : dot(sub(s0, s1), d1) / le, le == 0.0f ? 0.0f : dot(sub(e0, s1), d1) / le};
+		for (int k = 0; k < 4; k++)
+		{
+			if (!(tv[k] >= 0.0f && 1.0f >= tv[k]))
+				continue;
+			/* k < 2: a point of segment 1 against its projection on segment 0;
+			 * k >= 2: an endpoint of segment 0 against its projection on segment 1 */
+			const v3 on0 = k < 2 ? add(scl(d0, tv[k]), s0) : (k == 2 ? s0 : e0);
+			const v3 on1 = k < 2 ? (k == 0 ? s1 : e1) : add(scl(d1, tv[k]), s1);
+			const v3 v = sub(on0, on1);
+			const float sq = dot(v, v);
+			if (!(sq > eps && inf2 > sq))
+				continue;
+			const float dist = sqrtf(sq);
+			const v3 n = scl(v, 1.0f / dist);
+			o->n = n;
+			o->p[o->count] = add(sub(on0, scl(n, r0)), off);
+			o->sep[o->count] = dist - sumR;
+			o->count++;
+		}
+		if (o->count)
+			return o->count;
+	}
+	const v3 cA = add(scl(d0, t0), s0), cB = add(scl(d1, t1), s1);
+	const int con = eps > sq0;
+	const v3 nn = con ? (la > eps ? d0 : V(1.0f, 0.0f, 0.0f)) : sub(cA, cB);
+	const v3 n = normalize(nn);
+	o->n = n;
+	o->p[0] = add(sub(cA, scl(n, r0)), off);
+	o->sep[0] = (con ? 0.0f : sqrtf(sq0)) - sumR;
+	o->count = 1;
+	return 1;
+}
+
+/* geomutils/src/pcm/GuPCMContactPlaneCapsule.cpp:36-114 with a fresh manifold:
+ * each endpoint closer to the plane than radius + contact distance, the second
+ * replacing the first when they coincide within radius/1000
+ * (addManifoldPoint2, GuPersistentContactManifold.cpp:1269); output with the
+ * radius variant of addManifoldContactsToContactBuffer (:783-806): point on
+ * the capsule surface, separation = endpoint height - radius, normal from the
+ * capsule towards the plane */
+static int c_plane_capsule(v3 pn, float pd, float r, float hh, xf tb, float cd, cset* o)
+{
+	v3 ends[2];
+	capsule_segment(tb, hh, &ends[0], &ends[1]);
+	const float repl = r * 0.001f;
+	v3 pt[2];
+	float sd[2];
+	int n = 0;
+	for (int k = 0; k < 2; k++)
+	{
+		const float sdist = dot(pn, ends[k]) + pd;
+		if (!(r + cd > sdist))
+			continue;
+		if (n == 1)
+		{
+			const v3 da = sub(pt[0], ends[k]);
+			const v3 db = sub(sub(pt[0], scl(pn, sd[0])), sub(ends[k], scl(pn, sdist)));
+			if (repl * repl > fminf(dot(db, db), dot(da, da)))
+			{
+				pt[0] = ends[k];
+				sd[0] = sdist;
+				continue;
+			}
+		}
+		pt[n] = ends[k];
+		sd[n] = sdist;
+		n++;
+	}
+	o->n = neg(pn);
+	o->count = 0;
+	for (int k = 0; k < n; k++)
+		if (cd >= sd[k] - r)
+		{
+			o->p[o->count] = sub(pt[k], scl(pn, r));
+			o->sep[o->count] = sd[k] - r;
+			o->count++;
+		}
+	return o->count;
+}
+
 /* PersistentContactManifold::reduceBatchContactsCluster
  * (geomutils/src/pcm/GuPersistentContactManifold.cpp:859-1004) */
 static void reduce_cluster(const pcontact* m, int n, pcontact* out)
@@ -1568,6 +1745,343 @@ static int c_sphere_convex(float r, v3 p0, const polyd* B, xf x1, float cd, cset
 	return 1;
 }
 
+/* Capsule core segment pq against polytope B (B's frame), standing in for
+ * GJK / EPA on the segment (geomutils/src/gjk/GuGJKPenetration.h:95-306,
+ * takeCoreShape: points on the segment, distance without the radius).
+ * Apart: the exact closest points, returns dist > 0 with n = (cA - cB)/dist.
+ * Entering B: the minimum translation over B's face normals and the
+ * directions normal to the segment and each edge of B (the face normals of
+ * their Minkowski difference), returns -depth with n the direction that moves
+ * the segment out, cB = cA + n depth on B's surface. */
+static float seg_poly(const polyd* B, v3 p, v3 q, v3* n, v3* cA, v3* cB)
+{
+	const float fmax_ = 3.402823466e+38f;
+	const v3 d = sub(q, p);
+	float t0 = 0.0f, t1 = 1.0f;
+	int enters = 1;
+	*cA = p;
+	*cB = p;
+	for (int f = 0; f < B->np && enters; f++)
+	{
+		const float hp = dot(B->pn[f], p) + B->pd[f], dn = dot(B->pn[f], d);
+		if (dn == 0.0f)
+		{
+			if (hp > 0.0f)
+				enters = 0;
+			continue;
+		}
+		const float t = -hp / dn;
+		if (dn < 0.0f)
+			t0 = fmaxf(t0, t);
+		else
+			t1 = fminf(t1, t);
+		if (t0 > t1)
+			enters = 0;
+	}
+	if (!enters)
+	{
+		float best = fmax_;
+		for (int k = 0; k < 2; k++)
+		{
+			const v3 c = k ? q : p;
+			for (int f = 0; f < B->np; f++)
+			{
+				const float h = dot(B->pn[f], c) + B->pd[f];
+				if (h < 0.0f || h * h >= best)
+					continue;
+				const v3 pc = sub(c, scl(B->pn[f], h));
+				if (in_polygon(B, f, pc)) { best = h * h; *cA = c; *cB = pc; }
+			}
+		}
+		for (int e = 0; e < B->ne; e++)
+		{
+			v3 c1, c2;
+			const float s = seg_closest2(p, q, B->v[B->ev[2 * e]], B->v[B->ev[2 * e + 1]], &c1, &c2);
+			if (s < best) { best = s; *cA = c1; *cB = c2; }
+		}
+		const float dist = sqrtf(best);
+		*n = scl(sub(*cA, *cB), 1.0f / dist);
+		return dist;
+	}
+	float best = fmax_;
+	int bestEdge = -1;
+	v3 axis = V(1.0f, 0.0f, 0.0f);
+	for (int f = 0; f < B->np; f++)
+	{
+		const float depth = -B->pd[f] - fminf(dot(B->pn[f], p), dot(B->pn[f], q));
+		if (depth < best) { best = depth; axis = B->pn[f]; }
+	}
+	for (int e = 0; e < B->ne; e++)
+	{
+		const v3 a = B->v[B->ev[2 * e]], b = B->v[B->ev[2 * e + 1]];
+		v3 c = cross(d, sub(b, a));
+		const float l2 = dot(c, c);
+		if (!(l2 > 1e-12f))
+			continue;
+		c = scl(c, 1.0f / sqrtf(l2));
+		float mnB = fmax_, mxB = -fmax_;
+		for (int i = 0; i < B->nv; i++)
+		{
+			const float s = dot(c, B->v[i]);
+			mnB = fminf(mnB, s);
+			mxB = fmaxf(mxB, s);
+		}
+		const float sp = dot(c, p), sq = dot(c, q);
+		const float dp = mxB - fminf(sp, sq), dm = fmaxf(sp, sq) - mnB;
+		if (dp < best) { best = dp; axis = c; bestEdge = e; }
+		if (dm < best) { best = dm; axis = neg(c); bestEdge = e; }
+	}
+	*n = axis;
+	if (bestEdge >= 0)
+	{
+		v3 c1, c2;
+		seg_closest2(p, q, B->v[B->ev[2 * bestEdge]], B->v[B->ev[2 * bestEdge + 1]], &c1, &c2);
+		*cA = c1;
+	}
+	else
+		*cA = dot(axis, q) < dot(axis, p) ? q : p;
+	*cB = add(*cA, scl(axis, best));
+	return -best;
+}
+
+/* generatedCapsuleBoxFaceContacts (geomutils/src/pcm/GuPCMContactGenSphereCapsule.cpp:221-284):
+ * each endpoint moved along -n onto the reference polygon's plane, kept when
+ * that distance is within radius + contact distance and the foot lies in the
+ * polygon */
+static void capsule_face_contacts(const polyd* B, int f, v3 p0, v3 p1, v3 n, float infl, mxf aToB, pcontact* mc,
+                                  int* nc)
+{
+	const float fmax_ = 3.402823466e+38f;
+	const v3 pn = normalize(B->pn[f]);
+	const int nv = B->pc[f];
+	const v3 a = B->v[B->vr[B->pf[f]]];
+	const float den0 = dot(pn, sub(p0, a)), den1 = dot(pn, sub(p1, a));
+	const float proj = dot(pn, n);
+	const float numer = proj > 0.0f ? 1.0f / proj : 0.0f;
+	const float t[2] = {den0 * numer, den1 * numer};
+	if (!(infl >= t[0] || infl >= t[1]))
+		return;
+	const m33 rot = rot_from_z(pn);
+	v3 pts[256];
+	v3 mn = V(fmax_, fmax_, fmax_), mx = neg(mn);
+	for (int i = 0; i < nv; i++)
+	{
+		pts[i] = mmul(rot, B->v[B->vr[B->pf[f] + i]]);
+		mn = V(fminf(mn.x, pts[i].x), fminf(mn.y, pts[i].y), fminf(mn.z, pts[i].z));
+		mx = V(fmaxf(mx.x, pts[i].x), fmaxf(mx.y, pts[i].y), fmaxf(mx.z, pts[i].z));
+	}
+	for (int k = 0; k < 2; k++)
+	{
+		if (!(infl >= t[k]))
+			continue;
+		const v3 e = k ? p1 : p0;
+		const v3 foot = sub(e, scl(n, t[k]));
+		if (poly_contains(pts, nv, mmul(rot, foot), mn, mx))
+		{
+			mc[*nc].la = mxtransinv(aToB, e);
+			mc[*nc].lb = foot;
+			mc[*nc].ln = n;
+			mc[*nc].pen = t[k];
+			(*nc)++;
+		}
+	}
+}
+
+/* intersectRayPolyhedron (GuPCMContactGenSphereCapsule.cpp:98-141): the entry
+ * parameter of the ray a + t dir, t >= 0, into B */
+static int ray_polyhedron(const polyd* B, v3 a, v3 dir, float* tEnter)
+{
+	float tFirst = 0.0f, tLast = 3.402823466e+38f;
+	for (int k = 0; k < B->np; k++)
+	{
+		const float den = dot(B->pn[k], dir);
+		const float dist = dot(B->pn[k], a) + B->pd[k];
+		if (1e-7f > fabsf(den))
+		{
+			if (dist > 0.0f)
+				return 0;
+		}
+		else
+		{
+			const float t = -(dist / den);
+			if (0.0f > den && t > tFirst)
+				tFirst = t;
+			if (!(0.0f > den) && tLast > t)
+				tLast = t;
+		}
+		if (tFirst > tLast)
+			return 0;
+	}
+	*tEnter = tFirst;
+	return 1;
+}
+
+/* generatedFaceContacts (GuPCMContactGenSphereCapsule.cpp:286-310): each
+ * endpoint's ray along -n into the hull, entered within radius + contact
+ * distance */
+static void capsule_ray_contacts(const polyd* B, v3 p0, v3 p1, v3 n, float infl, mxf aToB, pcontact* mc, int* nc)
+{
+	const v3 dir = neg(n);
+	for (int k = 0; k < 2; k++)
+	{
+		const v3 e = k ? p1 : p0;
+		float tE;
+		if (ray_polyhedron(B, e, dir, &tE) && infl >= tE)
+		{
+			mc[*nc].la = mxtransinv(aToB, e);
+			mc[*nc].lb = add(scl(dir, tE), e);
+			mc[*nc].ln = n;
+			mc[*nc].pen = tE;
+			(*nc)++;
+		}
+	}
+}
+
+/* generateEE / generatedContactsEEContacts (GuPCMContactGenSphereCapsule.cpp:312-377):
+ * where the segment crosses the plane through each reference polygon edge
+ * spanned with n, a contact against that edge (its parameter within
+ * [-0.005, 1.005]) when the signed distance along n is within the radius */
+static void capsule_ee_contacts(const polyd* B, int f, v3 p, v3 q, v3 normal, float infl, mxf aToB, pcontact* mc,
+                                int* nc)
+{
+	const int nv = B->pc[f];
+	for (int rs = 0, re = nv - 1; rs < nv; re = rs++)
+	{
+		const v3 a = B->v[B->vr[B->pf[f] + rs]], b = B->v[B->vr[B->pf[f] + re]];
+		const v3 ab = sub(b, a);
+		const v3 n = cross(ab, normal);
+		const float d = dot(n, a);
+		const float sp = dot(n, p) - d, sq = dot(n, q) - d;
+		if (sp * sq > 0.0f)
+			continue;
+		const v3 pq = sub(q, p);
+		const float npq = dot(n, pq);
+		if (npq == 0.0f)
+			continue;
+		const float segT = (d - dot(n, p)) / npq;
+		const v3 lpa = add(scl(pq, segT), p);
+		const v3 perN = cross(normal, pq);
+		const v3 ap = sub(lpa, a);
+		const float tv = dot(perN, ap) / dot(perN, ab);
+		if (tv > 1.005f || -0.005f > tv)
+			continue;
+		const v3 v = sub(ap, scl(ab, tv));
+		const float sd = dot(v, normal);
+		if (infl >= sd)
+		{
+			mc[*nc].la = mxtransinv(aToB, lpa);
+			mc[*nc].lb = sub(lpa, v);
+			mc[*nc].ln = normal;
+			mc[*nc].pen = sd;
+			(*nc)++;
+		}
+	}
+}
+
+/* PersistentContactManifold::reduceBatchContacts2
+ * (geomutils/src/pcm/GuPersistentContactManifold.cpp:1177-1243): the deepest
+ * point, the point farthest from it, then the last point (in order) of those
+ * nearer the second than the first that is shallower than the second */
+static void reduce_batch2(const pcontact* m, int n, pcontact* out)
+{
+	int idx = 0;
+	float maxD = m[0].pen;
+	for (int i = 1; i < n; i++)
+		if (maxD > m[i].pen) { maxD = m[i].pen; idx = i; }
+	out[0] = m[idx];
+	int chosen[300] = {0};
+	chosen[idx] = 1;
+	v3 v = sub(m[0].lb, out[0].lb);
+	float far = dot(v, v);
+	idx = 0;
+	for (int i = 1; i < n; i++)
+	{
+		v = sub(m[i].lb, out[0].lb);
+		const float d = dot(v, v);
+		if (d > far) { far = d; idx = i; }
+	}
+	out[1] = m[idx];
+	chosen[idx] = 1;
+	int second = idx;
+	const float maxDepth = m[idx].pen;
+	for (int i = 0; i < n; i++)
+	{
+		if (chosen[i])
+			continue;
+		const v3 d0 = sub(out[0].lb, m[i].lb), d1 = sub(out[1].lb, m[i].lb);
+		if (dot(d0, d0) > dot(d1, d1) && maxDepth > m[i].pen)
+			second = i;
+	}
+	if (second != idx)
+		out[1] = m[second];
+}
+
+/* Gu::pcmContactCapsuleBox / pcmContactCapsuleConvex on a fresh manifold
+ * (geomutils/src/pcm/GuPCMContactCapsuleBox.cpp:42-200,
+ * GuPCMContactCapsuleConvex.cpp:43-236, full contact generation
+ * GuPCMContactGenSphereCapsule.cpp:379-468): the core segment against the
+ * polytope; box: endpoint contacts on the witness polygon, convex: endpoint
+ * rays into the hull; fewer than two: edge contacts on the witness polygon;
+ * none: the closest-point contact; reduced to two. Points on the capsule's
+ * surface, separation less the radius, normal from the polytope. */
+static int c_capsule_poly(float r, float hh, xf x0, const polyd* B, int isBox, xf x1, float cd, float tol, cset* o)
+{
+	const mxf aToB = mxf_of(xinvmul(x1, x0));
+	const v3 ext = mmul(aToB.r, V(hh, 0.0f, 0.0f));
+	const v3 p0 = add(aToB.p, ext), p1 = sub(aToB.p, ext);
+	v3 n, cA, cB;
+	const float dist = seg_poly(B, p0, p1, &n, &cA, &cB);
+	if (dist > r + cd)
+		return 0;
+	const float infl = r + cd;
+	const float wtol = fminf(fmaxf(B->margin, tol * 1e-2f), tol * 5e-2f);
+	pcontact mc[300];
+	int nc = 0;
+	if (isBox)
+	{
+		const int f = witness_polygon(B, neg(n), cB, wtol);
+		capsule_face_contacts(B, f, p0, p1, n, infl, aToB, mc, &nc);
+		if (nc < 2)
+			capsule_ee_contacts(B, f, p0, p1, n, infl, aToB, mc, &nc);
+	}
+	else
+	{
+		capsule_ray_contacts(B, p0, p1, n, infl, aToB, mc, &nc);
+		if (nc < 2)
+			capsule_ee_contacts(B, witness_polygon(B, neg(n), cB, wtol), p0, p1, n, infl, aToB, mc, &nc);
+	}
+	pcontact red[2];
+	const pcontact* use = mc;
+	if (nc == 0)
+	{
+		mc[0].la = mxtransinv(aToB, cA);
+		mc[0].lb = cB;
+		mc[0].ln = n;
+		mc[0].pen = dist;
+		nc = 1;
+	}
+	else if (nc > 2)
+	{
+		reduce_batch2(mc, nc, red);
+		use = red;
+		nc = 2;
+	}
+	const v3 wn = qrot(x1.q, n);
+	o->n = wn;
+	o->count = 0;
+	for (int i = 0; i < nc; i++)
+	{
+		const float sep = use[i].pen - r;
+		if (cd >= sep)
+		{
+			o->p[o->count] = sub(xtrans(x0, use[i].la), scl(wn, r));
+			o->sep[o->count] = sep;
+			o->count++;
+		}
+	}
+	return o->count;
+}
+
 /* Gu::pcmContactPlaneConvex on a fresh manifold
  * (geomutils/src/pcm/GuPCMContactPlaneConvex.cpp:36-227): the hull face most
  * opposed to the plane normal, or the better face of the edge whose summed
@@ -1703,6 +2217,24 @@ int pxo_generate_contacts(int type0, const float* g0, const float* pose0, int ty
 	}
 	else if (type0 == PXO_BOX && type1 == PXO_BOX)
 		n = c_box_box(V(g0[0], g0[1], g0[2]), x0, V(g1[0], g1[1], g1[2]), x1, cd, tol, &c);
+	else if (type0 == PXO_SPHERE && type1 == PXO_CAPSULE)
+		n = c_sphere_capsule(g0[0], x0.p, g1[0], g1[1], x1, cd, &c);
+	else if (type0 == PXO_CAPSULE && type1 == PXO_CAPSULE)
+		n = c_capsule_capsule(g0[0], g0[1], x0, g1[0], g1[1], x1, cd, &c);
+	else if (type0 == PXO_CAPSULE && type1 == PXO_BOX)
+	{
+		boxpoly bp;
+		box_poly(V(g1[0], g1[1], g1[2]), &bp);
+		n = c_capsule_poly(g0[0], g0[1], x0, &bp.d, 1, x1, cd, tol, &c);
+	}
+	else if (type0 == PXO_CAPSULE && type1 == PXO_CONVEX)
+		n = c_capsule_poly(g0[0], g0[1], x0, hull_of(g1[0]), 0, x1, cd, tol, &c);
+	else if (type0 == PXO_PLANE && type1 == PXO_CAPSULE)
+	{
+		v3 pn; float pd;
+		plane_from_pose(x0, &pn, &pd);
+		n = c_plane_capsule(pn, pd, g1[0], g1[1], x1, cd, &c);
+	}
 	else if (type0 == PXO_PLANE && type1 == PXO_CONVEX)
 	{
 		v3 pn; float pd;
@@ -2049,6 +2581,15 @@ static void shape_bounds(int type, const float* dims, xf t, v3* mn, v3* mx)
 		return;
 	}
 	const m33 R = qmat(t.q);
+	if (type == PXO_CAPSULE)
+	{
+		/* computeCapsuleBounds (geomutils/include/GuBounds.h:54-62) */
+		const v3 w = V(fabsf(R.c0.x) * dims[1] + dims[0], fabsf(R.c0.y) * dims[1] + dims[0],
+		               fabsf(R.c0.z) * dims[1] + dims[0]);
+		*mn = sub(t.p, w);
+		*mx = add(t.p, w);
+		return;
+	}
 	v3 e = V(dims[0], dims[1], dims[2]), c = t.p;
 	if (type == PXO_CONVEX)
 	{
@@ -2562,6 +3103,18 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 	const float* g1 = w->dims + 4 * s1;
 	if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_SPHERE)
 		n = c_sphere_sphere(g0[0], w->pose[s0].p, g1[0], w->pose[s1].p, cd, &c);
+	else if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_CAPSULE)
+		n = c_sphere_capsule(g0[0], w->pose[s0].p, g1[0], g1[1], w->pose[s1], cd, &c);
+	else if (w->type[s0] == PXO_CAPSULE && w->type[s1] == PXO_CAPSULE)
+		n = c_capsule_capsule(g0[0], g0[1], w->pose[s0], g1[0], g1[1], w->pose[s1], cd, &c);
+	else if (w->type[s0] == PXO_CAPSULE && w->type[s1] == PXO_BOX)
+	{
+		boxpoly bp;
+		box_poly(V(g1[0], g1[1], g1[2]), &bp);
+		n = c_capsule_poly(g0[0], g0[1], w->pose[s0], &bp.d, 1, w->pose[s1], cd, tol, &c);
+	}
+	else if (w->type[s0] == PXO_CAPSULE)
+		n = c_capsule_poly(g0[0], g0[1], w->pose[s0], hull_of(g1[0]), 0, w->pose[s1], cd, tol, &c);
 	else if (w->type[s0] == PXO_SPHERE && w->type[s1] == PXO_BOX)
 		n = c_sphere_box(g0[0], w->pose[s0].p, V(g1[0], g1[1], g1[2]), w->pose[s1], cd, &c);
 	else if (w->type[s0] == PXO_BOX && w->type[s1] == PXO_BOX)
@@ -2620,6 +3173,11 @@ int pxo_world_step(void* wp, float dt)
 			int k;
 			if (w->type[i] == PXO_SPHERE)
 				k = c_sphere_plane(w->dims[4 * i], w->pose[i].p, w->pn[q], w->pd[q], cd, &c);
+			else if (w->type[i] == PXO_CAPSULE)
+			{
+				k = c_plane_capsule(w->pn[q], w->pd[q], w->dims[4 * i], w->dims[4 * i + 1], w->pose[i], cd, &c);
+				c.n = neg(c.n); /* the capsule is body A: normal from the plane towards it */
+			}
 			else if (w->type[i] == PXO_CONVEX)
 			{
 				k = c_plane_convex(w->pn[q], w->pd[q], hull_of(w->dims[4 * i]), w->pose[i], cd, tol, &c);

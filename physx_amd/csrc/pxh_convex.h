@@ -671,6 +671,338 @@ PXH_D int sphereConvex(float r, f3 p0, const Poly& B, const xf& x1, float cd, Co
 	return 1;
 }
 
+// Capsule core segment pq against polytope B (B's frame), standing in for
+// GJK / EPA on the segment (geomutils/src/gjk/GuGJKPenetration.h:95-306 with
+// takeCoreShape: points on the segment, distance without the radius). Apart:
+// the exact closest points, returns dist > 0 with n = (cA - cB)/dist.
+// Entering B: the minimum translation over B's face normals and the
+// directions normal to the segment and each edge of B (the face normals of
+// their Minkowski difference), returns -depth, cB = cA + n depth.
+PXH_D float segPoly(const Poly& B, f3 p, f3 q, f3& n, f3& cA, f3& cB)
+{
+	const f3 d = q - p;
+	float t0 = 0.0f, t1 = 1.0f;
+	bool enters = true;
+	cA = p;
+	cB = p;
+	for (int f = 0; f < B.np && enters; f++)
+	{
+		const f3 fn = B.pn(f);
+		const float hp = dot(fn, p) + B.pd(f), dn = dot(fn, d);
+		if (dn == 0.0f)
+		{
+			enters = !(hp > 0.0f);
+			continue;
+		}
+		const float t = -hp / dn;
+		if (dn < 0.0f)
+			t0 = fmaxf(t0, t);
+		else
+			t1 = fminf(t1, t);
+		enters = !(t0 > t1);
+	}
+	if (!enters)
+	{
+		float best = FLT_BIG;
+		for (int k = 0; k < 2; k++)
+		{
+			const f3 c = k ? q : p;
+			for (int f = 0; f < B.np; f++)
+			{
+				const f3 fn = B.pn(f);
+				const float h = dot(fn, c) + B.pd(f);
+				if (h < 0.0f || h * h >= best)
+					continue;
+				const f3 pc = c - fn * h;
+				if (inPolygon(B, f, pc)) { best = h * h; cA = c; cB = pc; }
+			}
+		}
+		for (int e = 0; e < B.ne; e++)
+		{
+			const uint32_t ed = B.edge(e);
+			f3 c1, c2;
+			const float s = segClosestSq(p, q, B.vert(ed & 0xFF), B.vert((ed >> 8) & 0xFF), c1, c2);
+			if (s < best) { best = s; cA = c1; cB = c2; }
+		}
+		const float dist = sqrtf(best);
+		n = (cA - cB) * (1.0f / dist);
+		return dist;
+	}
+	float best = FLT_BIG;
+	int bestEdge = -1;
+	f3 axis = mk(1.0f, 0.0f, 0.0f);
+	for (int f = 0; f < B.np; f++)
+	{
+		const f3 fn = B.pn(f);
+		const float depth = -B.pd(f) - fminf(dot(fn, p), dot(fn, q));
+		if (depth < best) { best = depth; axis = fn; }
+	}
+	for (int e = 0; e < B.ne; e++)
+	{
+		const uint32_t ed = B.edge(e);
+		const f3 a = B.vert(ed & 0xFF), b = B.vert((ed >> 8) & 0xFF);
+		f3 c = cross(d, b - a);
+		const float l2 = dot(c, c);
+		if (!(l2 > 1e-12f))
+			continue;
+		c = c * (1.0f / sqrtf(l2));
+		float mnB = FLT_BIG, mxB = -FLT_BIG;
+		for (int i = 0; i < B.nv; i++)
+		{
+			const float s = dot(c, B.vert(i));
+			mnB = fminf(mnB, s);
+			mxB = fmaxf(mxB, s);
+		}
+		const float sp = dot(c, p), sq = dot(c, q);
+		const float dp = mxB - fminf(sp, sq), dm = fmaxf(sp, sq) - mnB;
+		if (dp < best) { best = dp; axis = c; bestEdge = e; }
+		if (dm < best) { best = dm; axis = -c; bestEdge = e; }
+	}
+	n = axis;
+	if (bestEdge >= 0)
+	{
+		const uint32_t ed = B.edge(bestEdge);
+		f3 c2;
+		segClosestSq(p, q, B.vert(ed & 0xFF), B.vert((ed >> 8) & 0xFF), cA, c2);
+	}
+	else
+		cA = dot(axis, q) < dot(axis, p) ? q : p;
+	cB = cA + axis * best;
+	return -best;
+}
+
+constexpr int CAPSULE_MAX_CONTACTS = HULL_MAX_POLY_VERTS + 2;
+
+// generatedCapsuleBoxFaceContacts (geomutils/src/pcm/GuPCMContactGenSphereCapsule.cpp:221-284):
+// each endpoint moved along -n onto the reference polygon's plane, kept when
+// that distance is within radius + contact distance and the foot lies in the
+// polygon
+PXH_D void capsuleFaceContacts(const Poly& B, int f, f3 p0, f3 p1, f3 n, float infl, const mxf& aToB, PC* mc, int& nc)
+{
+	const f3 pn = normalize(B.pn(f));
+	const int nv = B.pcount(f);
+	const f3 a = B.vert(B.pvert(f, 0));
+	const float proj = dot(pn, n);
+	const float numer = proj > 0.0f ? 1.0f / proj : 0.0f;
+	const float t[2] = {dot(pn, p0 - a) * numer, dot(pn, p1 - a) * numer};
+	if (!(infl >= t[0] || infl >= t[1]))
+		return;
+	const m33 rot = rotFromZ(pn);
+	f3 pts[HULL_MAX_POLY_VERTS];
+	f3 mn = mk(FLT_BIG, FLT_BIG, FLT_BIG), mx = -mn;
+	for (int i = 0; i < nv; i++)
+	{
+		pts[i] = mmul(rot, B.vert(B.pvert(f, i)));
+		mn = mk(fminf(mn.x, pts[i].x), fminf(mn.y, pts[i].y), fminf(mn.z, pts[i].z));
+		mx = mk(fmaxf(mx.x, pts[i].x), fmaxf(mx.y, pts[i].y), fmaxf(mx.z, pts[i].z));
+	}
+	for (int k = 0; k < 2; k++)
+	{
+		if (!(infl >= t[k]))
+			continue;
+		const f3 e = k ? p1 : p0;
+		const f3 foot = e - n * t[k];
+		if (polyContainsN(pts, nv, mmul(rot, foot), mn, mx))
+		{
+			mc[nc].la = mxtransinv(aToB, e);
+			mc[nc].lb = foot;
+			mc[nc].ln = n;
+			mc[nc].pen = t[k];
+			nc++;
+		}
+	}
+}
+
+// intersectRayPolyhedron (GuPCMContactGenSphereCapsule.cpp:98-141): the entry
+// parameter of the ray a + t dir, t >= 0, into B
+PXH_D bool rayPolyhedron(const Poly& B, f3 a, f3 dir, float& tEnter)
+{
+	float tFirst = 0.0f, tLast = FLT_BIG;
+	for (int k = 0; k < B.np; k++)
+	{
+		const f3 fn = B.pn(k);
+		const float den = dot(fn, dir);
+		const float dist = dot(fn, a) + B.pd(k);
+		if (1e-7f > fabsf(den))
+		{
+			if (dist > 0.0f)
+				return false;
+		}
+		else
+		{
+			const float t = -(dist / den);
+			if (0.0f > den && t > tFirst)
+				tFirst = t;
+			if (!(0.0f > den) && tLast > t)
+				tLast = t;
+		}
+		if (tFirst > tLast)
+			return false;
+	}
+	tEnter = tFirst;
+	return true;
+}
+
+// generatedFaceContacts (GuPCMContactGenSphereCapsule.cpp:286-310): each
+// endpoint's ray along -n into the hull, entered within radius + contact
+// distance
+PXH_D void capsuleRayContacts(const Poly& B, f3 p0, f3 p1, f3 n, float infl, const mxf& aToB, PC* mc, int& nc)
+{
+	for (int k = 0; k < 2; k++)
+	{
+		const f3 e = k ? p1 : p0;
+		float tE;
+		if (rayPolyhedron(B, e, -n, tE) && infl >= tE)
+		{
+			mc[nc].la = mxtransinv(aToB, e);
+			mc[nc].lb = (-n) * tE + e;
+			mc[nc].ln = n;
+			mc[nc].pen = tE;
+			nc++;
+		}
+	}
+}
+
+// generateEE / generatedContactsEEContacts (GuPCMContactGenSphereCapsule.cpp:312-377):
+// where the segment crosses the plane through a reference polygon edge
+// spanned with n, a contact against that edge (its parameter within
+// [-0.005, 1.005]) when the signed distance along n is within the radius
+PXH_D void capsuleEEContacts(const Poly& B, int f, f3 p, f3 q, f3 normal, float infl, const mxf& aToB, PC* mc, int& nc)
+{
+	const int nv = B.pcount(f);
+	for (int rs = 0, re = nv - 1; rs < nv; re = rs++)
+	{
+		const f3 a = B.vert(B.pvert(f, rs)), b = B.vert(B.pvert(f, re));
+		const f3 ab = b - a;
+		const f3 n = cross(ab, normal);
+		const float d = dot(n, a);
+		const float sp = dot(n, p) - d, sq = dot(n, q) - d;
+		if (sp * sq > 0.0f)
+			continue;
+		const f3 pq = q - p;
+		const float npq = dot(n, pq);
+		if (npq == 0.0f)
+			continue;
+		const f3 lpa = pq * ((d - dot(n, p)) / npq) + p;
+		const f3 perN = cross(normal, pq);
+		const f3 ap = lpa - a;
+		const float tv = dot(perN, ap) / dot(perN, ab);
+		if (tv > 1.005f || -0.005f > tv)
+			continue;
+		const f3 v = ap - ab * tv;
+		const float sd = dot(v, normal);
+		if (infl >= sd)
+		{
+			mc[nc].la = mxtransinv(aToB, lpa);
+			mc[nc].lb = lpa - v;
+			mc[nc].ln = normal;
+			mc[nc].pen = sd;
+			nc++;
+		}
+	}
+}
+
+// PersistentContactManifold::reduceBatchContacts2
+// (geomutils/src/pcm/GuPersistentContactManifold.cpp:1177-1243): the deepest
+// point, the point farthest from it, then the last point (in order) of those
+// nearer the second than the first that is shallower than the second
+PXH_D void reduceBatch2(const PC* m, int n, int& i0, int& i1)
+{
+	int idx = 0;
+	float maxD = m[0].pen;
+	for (int i = 1; i < n; i++)
+		if (maxD > m[i].pen) { maxD = m[i].pen; idx = i; }
+	i0 = idx;
+	const f3 b0 = m[i0].lb;
+	f3 v = m[0].lb - b0;
+	float far = dot(v, v);
+	idx = 0;
+	for (int i = 1; i < n; i++)
+	{
+		v = m[i].lb - b0;
+		const float d = dot(v, v);
+		if (d > far) { far = d; idx = i; }
+	}
+	const int first = idx;
+	const float maxDepth = m[first].pen;
+	const f3 b1 = m[first].lb;
+	int second = first;
+	for (int i = 0; i < n; i++)
+	{
+		if (i == i0 || i == first)
+			continue;
+		const f3 d0 = b0 - m[i].lb, d1 = b1 - m[i].lb;
+		if (dot(d0, d0) > dot(d1, d1) && maxDepth > m[i].pen)
+			second = i;
+	}
+	i1 = second;
+}
+
+// Gu::pcmContactCapsuleBox / pcmContactCapsuleConvex on a fresh manifold
+// (geomutils/src/pcm/GuPCMContactCapsuleBox.cpp:42-200,
+// GuPCMContactCapsuleConvex.cpp:43-236, full contact generation
+// GuPCMContactGenSphereCapsule.cpp:379-468): the core segment against the
+// polytope; box: endpoint contacts on the witness polygon, convex: endpoint
+// rays into the hull; fewer than two: edge contacts on the witness polygon;
+// none: the closest-point contact; reduced to two. Points on the capsule's
+// surface, separation less the radius, normal from the polytope.
+PXH_D int capsulePoly(float r, float hh, const xf& x0, const Poly& B, const xf& x1, float cd, float tol, Contacts& o)
+{
+	const mxf aToB = mxfOf(xinvmul(x1, x0));
+	const f3 ext = mmul(aToB.r, mk(hh, 0.0f, 0.0f));
+	const f3 p0 = aToB.p + ext, p1 = aToB.p - ext;
+	f3 n, cA, cB;
+	const float dist = segPoly(B, p0, p1, n, cA, cB);
+	if (dist > r + cd)
+		return 0;
+	const float infl = r + cd;
+	const float wtol = fminf(fmaxf(B.margin, tol * 1e-2f), tol * 5e-2f);
+	PC mc[CAPSULE_MAX_CONTACTS];
+	int nc = 0;
+	if (B.box)
+	{
+		const int f = witnessPolygon(B, -n, cB, wtol);
+		capsuleFaceContacts(B, f, p0, p1, n, infl, aToB, mc, nc);
+		if (nc < 2)
+			capsuleEEContacts(B, f, p0, p1, n, infl, aToB, mc, nc);
+	}
+	else
+	{
+		capsuleRayContacts(B, p0, p1, n, infl, aToB, mc, nc);
+		if (nc < 2)
+			capsuleEEContacts(B, witnessPolygon(B, -n, cB, wtol), p0, p1, n, infl, aToB, mc, nc);
+	}
+	int i0 = 0, i1 = 1;
+	if (nc == 0)
+	{
+		mc[0].la = mxtransinv(aToB, cA);
+		mc[0].lb = cB;
+		mc[0].ln = n;
+		mc[0].pen = dist;
+		nc = 1;
+	}
+	else if (nc > 2)
+	{
+		reduceBatch2(mc, nc, i0, i1);
+		nc = 2;
+	}
+	const f3 wn = qrot(x1.q, n);
+	o.n = wn;
+	o.count = 0;
+	for (int i = 0; i < nc; i++)
+	{
+		const PC& c = mc[i == 0 ? i0 : i1];
+		const float sep = c.pen - r;
+		if (cd >= sep)
+		{
+			o.p[o.count] = xtrans(x0, c.la) - wn * r;
+			o.sep[o.count] = sep;
+			o.count++;
+		}
+	}
+	return o.count;
+}
+
 // Gu::pcmContactPlaneConvex on a fresh manifold
 // (geomutils/src/pcm/GuPCMContactPlaneConvex.cpp:36-227): the face most
 // opposed to the plane normal, or the better face of the edge whose summed

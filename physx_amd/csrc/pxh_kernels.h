@@ -6,8 +6,9 @@
 
 namespace pxh {
 
-// broadphase pair classes: sphere-sphere, sphere-box, box-box, sphere-convex,
-// box-convex, convex-convex (pairs are written class by class)
+// broadphase pair classes, spheres and capsules counting as round shapes:
+// round-round, round-box, box-box, round-convex, box-convex, convex-convex
+// (pairs are written class by class)
 constexpr int PAIR_CLASSES = 6;
 constexpr int PAIR_CLASS_FIRST_CONVEX = 3;
 

@@ -221,6 +221,12 @@ __global__ void k_bounds(int n, const float4* pos, const float4* rot, const floa
 	f3 w;
 	if (int(g.w) == GEOM_SPHERE)
 		w = mk(g.x, g.x, g.x);
+	else if (int(g.w) == GEOM_CAPSULE)
+	{
+		// computeCapsuleBounds (geomutils/include/GuBounds.h:54-62)
+		const m33 R = qmat(mkq(rot[i]));
+		w = mk(fabsf(R.c0.x) * g.y + g.x, fabsf(R.c0.y) * g.y + g.x, fabsf(R.c0.z) * g.y + g.x);
+	}
 	else
 	{
 		const m33 R = qmat(mkq(rot[i]));
@@ -429,8 +435,9 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
 		p[i] = v;
 }
 
-// pair class from the two geometry types: 0 sphere-sphere, 1 sphere-box,
-// 2 box-box, 3 sphere-convex, 4 box-convex, 5 convex-convex
+// pair class from the two geometry types, spheres and capsules counting as
+// round shapes: 0 round-round, 1 round-box, 2 box-box, 3 round-convex,
+// 4 box-convex, 5 convex-convex
 PXH_D int pairClass(int t0, int t1)
 {
 	const int nc = (t0 == GEOM_CONVEXMESH) + (t1 == GEOM_CONVEXMESH);
@@ -567,17 +574,21 @@ __global__ void k_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex,
 	const float4 p0 = pos[ab.x], p1 = pos[ab.y];
 	if (t0 == GEOM_SPHERE && t1 == GEOM_SPHERE)
 		cnt = sphereSphere(g0.x, mk(p0), g1.x, mk(p1), cd, c);
-	else if (t0 == GEOM_SPHERE)
-	{
-		xf x1; x1.p = mk(p1); x1.q = mkq(rot[ab.y]);
-		cnt = sphereBox(g0.x, mk(p0), mk(g1.x, g1.y, g1.z), x1, cd, c);
-	}
 	else
 	{
 		xf x0, x1;
-		x0.p = mk(p0); x0.q = mkq(rot[ab.x]);
+		x0.p = mk(p0); x0.q = t0 == GEOM_SPHERE ? mkq(make_float4(0.0f, 0.0f, 0.0f, 1.0f)) : mkq(rot[ab.x]);
 		x1.p = mk(p1); x1.q = mkq(rot[ab.y]);
-		cnt = boxBox(mk(g0.x, g0.y, g0.z), x0, mk(g1.x, g1.y, g1.z), x1, cd, tol, c);
+		if (t0 == GEOM_SPHERE && t1 == GEOM_CAPSULE)
+			cnt = sphereCapsule(g0.x, x0.p, g1.x, g1.y, x1, cd, c);
+		else if (t0 == GEOM_SPHERE)
+			cnt = sphereBox(g0.x, x0.p, mk(g1.x, g1.y, g1.z), x1, cd, c);
+		else if (t1 == GEOM_CAPSULE)
+			cnt = capsuleCapsule(g0.x, g0.y, x0, g1.x, g1.y, x1, cd, c);
+		else if (t0 == GEOM_CAPSULE)
+			cnt = capsulePoly(g0.x, g0.y, x0, boxPoly(mk(g1.x, g1.y, g1.z)), x1, cd, tol, c);
+		else
+			cnt = boxBox(mk(g0.x, g0.y, g0.z), x0, mk(g1.x, g1.y, g1.z), x1, cd, tol, c);
 	}
 	storeContacts(slotBase + p, c, cnt, ctN, ctP);
 }
@@ -606,15 +617,18 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 	{
 		xf x;
 		x.p = mk(pos[i]); x.q = mkq(rot[i]);
-		cnt = planeBox(mk(pl), pl.w, mk(g.x, g.y, g.z), x, cd, c);
+		if (int(g.w) == GEOM_CAPSULE)
+			cnt = planeCapsule(mk(pl), pl.w, g.x, g.y, x, cd, c);
+		else
+			cnt = planeBox(mk(pl), pl.w, mk(g.x, g.y, g.z), x, cd, c);
 		c.n = -c.n;
 	}
 	storeContacts(slotBase + q, c, cnt, ctN, ctP);
 }
 
-// Sphere-convex pairs: the class segments from *dConvex to *dP hold the
-// convex pairs; lane q takes pair *dConvex + q when its first shape is a
-// sphere (box- and convex-convex pairs run in k_np_convex_wave).
+// Sphere- and capsule-convex pairs: the class segments from *dConvex to *dP
+// hold the convex pairs; lane q takes pair *dConvex + q when its first shape
+// is round (box- and convex-convex pairs run in k_np_convex_wave).
 __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                             const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float tol,
                             float4* ctN, float4* ctP)
@@ -625,13 +639,22 @@ __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex
 		return;
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
-	if (int(g0.w) != GEOM_SPHERE)
+	const int t0 = int(g0.w);
+	if (t0 != GEOM_SPHERE && t0 != GEOM_CAPSULE)
 		return;
 	xf x1;
 	x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
 	Contacts c;
 	c.count = 0;
-	const int cnt = sphereConvex(g0.x, mk(pos[ab.x]), hullPoly(hs, int(g1.x)), x1, cd, c);
+	int cnt;
+	if (t0 == GEOM_SPHERE)
+		cnt = sphereConvex(g0.x, mk(pos[ab.x]), hullPoly(hs, int(g1.x)), x1, cd, c);
+	else
+	{
+		xf x0;
+		x0.p = mk(pos[ab.x]); x0.q = mkq(rot[ab.x]);
+		cnt = capsulePoly(g0.x, g0.y, x0, hullPoly(hs, int(g1.x)), x1, cd, tol, c);
+	}
 	storeContacts(slotBase + int(p), c, cnt, ctN, ctP);
 }
 
@@ -654,7 +677,7 @@ k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, const uin
 		const uint2 ab = pairs[p];
 		const float4 g0 = geom[ab.x], g1 = geom[ab.y];
 		const int t0 = int(g0.w);
-		if (t0 == GEOM_SPHERE)
+		if (t0 == GEOM_SPHERE || t0 == GEOM_CAPSULE)
 			continue;
 		const f3 p0 = mk(pos[ab.x]), p1 = mk(pos[ab.y]);
 		const float r0 = t0 == GEOM_BOX ? sqrtf(g0.x * g0.x + g0.y * g0.y + g0.z * g0.z) : hs.be[int(g0.x)].w;
@@ -703,7 +726,7 @@ __global__ void k_np_convex_finish(int cap, const uint32_t* dP, const uint32_t* 
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
 	const int t0 = int(g0.w);
-	if (t0 == GEOM_SPHERE)
+	if (t0 == GEOM_SPHERE || t0 == GEOM_CAPSULE)
 		return;
 	const float4 sep = wit[3 * int64_t(p)];
 	Contacts c;

@@ -92,6 +92,170 @@ PXH_D int sphereBox(float r, f3 p0, f3 h, const xf& t1, float cd, Contacts& o)
 	return 1;
 }
 
+// Capsules: the segment along the local x axis, endpoints p +- x*halfHeight
+// (geomutils/src/GuInternal.h:60-70), inflated by the radius. The vector
+// library's min/max take the second operand when the comparison is unordered.
+PXH_D float vmin(float a, float b) { return a < b ? a : b; }
+PXH_D float vmax(float a, float b) { return a > b ? a : b; }
+PXH_D float clamp01(float x) { return vmax(vmin(x, 1.0f), 0.0f); }
+
+PXH_D void capsuleSegment(const xf& t, float hh, f3& s, f3& e)
+{
+	const f3 tmp = qrot(t.q, mk(1.0f, 0.0f, 0.0f)) * hh;
+	s = t.p + tmp;
+	e = t.p - tmp;
+}
+
+// Gu::distanceSegmentSegmentSquared, vector form
+// (geomutils/src/distance/GuDistanceSegmentSegment.cpp:411-470): parallel
+// segments (a e - b^2 below FLT_EPSILON) take s = 1/2 before the clamped
+// refinement
+PXH_D float segSegSq(f3 p1, f3 d1, f3 p2, f3 d2, float& s, float& t)
+{
+	const float eps = 1.1920929e-07f;
+	const f3 r = p1 - p2;
+	const float a = dot(d1, d1), e = dot(d2, d2), b = dot(d1, d2), c = dot(d1, r);
+	const float aR = a > eps ? 1.0f / a : 0.0f;
+	const float eR = e > eps ? 1.0f / e : 0.0f;
+	const float f = dot(d2, r);
+	const float denom = a * e - b * b;
+	const float temp = b * f - c * e;
+	const float sT = eps > denom ? 0.5f : clamp01(temp / denom);
+	t = clamp01((b * sT + f) * eR);
+	s = clamp01((b * t - c) * aR);
+	const f3 vv = (d1 * s + p1) - (d2 * t + p2);
+	return dot(vv, vv);
+}
+
+// geomutils/src/pcm/GuPCMContactSphereCapsule.cpp:38-96
+PXH_D int sphereCapsule(float r, f3 c, float cr, float hh, const xf& t1, float cd, Contacts& o)
+{
+	f3 s, e;
+	capsuleSegment(t1, hh, s, e);
+	const float rs = r + cr, inf = rs + cd;
+	const f3 ap = c - s, ab = e - s;
+	const float nom = dot(ap, ab), den = dot(ab, ab);
+	const float t = den == 0.0f ? 0.0f : clamp01(nom / den);
+	const f3 v = ap - ab * t;
+	const float d2 = dot(v, v);
+	if (!(inf * inf > d2))
+		return 0;
+	const f3 dir = c - ((e - s) * t + s);
+	const float l = sqrtf(dot(dir, dir));
+	const f3 n = l > 1.1920929e-07f ? dir * (1.0f / l) : mk(1.0f, 0.0f, 0.0f);
+	o.n = n;
+	o.p[0] = c - n * r;
+	o.sep[0] = sqrtf(d2) - rs;
+	o.count = 1;
+	return 1;
+}
+
+// geomutils/src/pcm/GuPCMContactCapsuleCapsule.cpp:81-283: the closest points
+// of the two segments, or, for segments within 0.9998 of parallel, the
+// endpoints projecting inside the other segment (up to four points), about the
+// midpoint of the two centres
+PXH_D int capsuleCapsule(float r0, float hh0, const xf& x0, float r1, float hh1, const xf& x1, float cd, Contacts& o)
+{
+	const f3 off = (x0.p + x1.p) * 0.5f;
+	xf a = x0, b = x1;
+	a.p = x0.p - off;
+	b.p = x1.p - off;
+	f3 s0, e0, s1, e1;
+	capsuleSegment(a, hh0, s0, e0);
+	capsuleSegment(b, hh1, s1, e1);
+	const f3 d0 = e0 - s0, d1 = e1 - s1;
+	const float sumR = r0 + r1, inf = sumR + cd, inf2 = inf * inf;
+	const float la = dot(d0, d0), le = dot(d1, d1);
+	const float eps = 1e-6f;
+	float t0, t1;
+	const float sq0 = segSegSq(s0, d0, s1, d1, t0, t1);
+	if (!(inf2 >= sq0))
+		return 0;
+	o.count = 0;
+	const f3 dir0 = eps > la ? mk(0.0f, 0.0f, 0.0f) : d0 * (1.0f / sqrtf(la));
+	const f3 dir1 = eps > le ? mk(0.0f, 0.0f, 0.0f) : d1 * (1.0f / sqrtf(le));
+	if (fabsf(dot(dir0, dir1)) > 0.9998f)
+	{
+		// s1, e1 projected on s0e0; s0, e0 projected on s1e1 (:40-74)
+		for (int k = 0; k < 4; k++)
+		{
+			const f3 pt = k == 0 ? s1 : (k == 1 ? e1 : (k == 2 ? s0 : e0));
+			const float tv = k < 2 ? (la == 0.0f ? 0.0f : dot(pt - s0, d0) / la)
+			                       : (le == 0.0f ? 0.0f : dot(pt - s1, d1) / le);
+			if (!(tv >= 0.0f && 1.0f >= tv))
+				continue;
+			const f3 on0 = k < 2 ? d0 * tv + s0 : pt;
+			const f3 on1 = k < 2 ? pt : d1 * tv + s1;
+			const f3 v = on0 - on1;
+			const float sq = dot(v, v);
+			if (!(sq > eps && inf2 > sq))
+				continue;
+			const float dist = sqrtf(sq);
+			const f3 n = v * (1.0f / dist);
+			o.n = n;
+			o.p[o.count] = (on0 - n * r0) + off;
+			o.sep[o.count] = dist - sumR;
+			o.count++;
+		}
+		if (o.count)
+			return o.count;
+	}
+	const f3 cA = d0 * t0 + s0, cB = d1 * t1 + s1;
+	const bool con = eps > sq0;
+	const f3 n = normalize(con ? (la > eps ? d0 : mk(1.0f, 0.0f, 0.0f)) : cA - cB);
+	o.n = n;
+	o.p[0] = (cA - n * r0) + off;
+	o.sep[0] = (con ? 0.0f : sqrtf(sq0)) - sumR;
+	o.count = 1;
+	return 1;
+}
+
+// geomutils/src/pcm/GuPCMContactPlaneCapsule.cpp:36-114 with a fresh
+// manifold: each endpoint closer to the plane than radius + contact distance,
+// the second replacing the first when they coincide within radius/1000
+// (addManifoldPoint2, GuPersistentContactManifold.cpp:1269); points on the
+// capsule surface, separation = endpoint height - radius, normal from the
+// capsule towards the plane
+PXH_D int planeCapsule(f3 pn, float pd, float r, float hh, const xf& tb, float cd, Contacts& o)
+{
+	f3 ends[2];
+	capsuleSegment(tb, hh, ends[0], ends[1]);
+	const float repl = r * 0.001f;
+	f3 pt[2];
+	float sd[2];
+	int n = 0;
+	for (int k = 0; k < 2; k++)
+	{
+		const float h = dot(pn, ends[k]) + pd;
+		if (!(r + cd > h))
+			continue;
+		if (n == 1)
+		{
+			const f3 da = pt[0] - ends[k];
+			const f3 db = (pt[0] - pn * sd[0]) - (ends[k] - pn * h);
+			if (repl * repl > fminf(dot(db, db), dot(da, da)))
+			{
+				pt[0] = ends[k];
+				sd[0] = h;
+				continue;
+			}
+		}
+		pt[n] = ends[k];
+		sd[n] = h;
+		n++;
+	}
+	o.n = -pn;
+	o.count = 0;
+	for (int k = 0; k < n; k++)
+		if (cd >= sd[k] - r)
+		{
+			o.p[o.count] = pt[k] - pn * r;
+			o.sep[o.count] = sd[k] - r;
+			o.count++;
+		}
+	return o.count;
+}
+
 // PersistentContactManifold::reduceBatchContactsCluster
 // (geomutils/src/pcm/GuPersistentContactManifold.cpp:859-1004)
 PXH_D void reduceCluster(const PC* m, int n, int* idx)

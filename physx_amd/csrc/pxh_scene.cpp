@@ -564,9 +564,12 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 		const int w = world ? world[i] : 0;
 		if (w < lastWorld || (i > 0 && w < hw[i - 1]) || w < 0)
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: world ids must be non-decreasing");
-		if (type[i] != PXH_GEOM_SPHERE && type[i] != PXH_GEOM_BOX && type[i] != PXH_GEOM_CONVEXMESH)
+		if (type[i] != PXH_GEOM_SPHERE && type[i] != PXH_GEOM_CAPSULE && type[i] != PXH_GEOM_BOX &&
+		    type[i] != PXH_GEOM_CONVEXMESH)
 			return fail(PXH_ERR_UNSUPPORTED,
-			            "pxh_scene_add_bodies: only sphere, box and convex mesh shapes are implemented");
+			            "pxh_scene_add_bodies: only sphere, capsule, box and convex mesh shapes are implemented");
+		if (type[i] == PXH_GEOM_CAPSULE && !(dims[4 * i] > 0.0f && dims[4 * i + 1] >= 0.0f))
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: capsule needs radius > 0 and half height >= 0");
 		if (type[i] == PXH_GEOM_CONVEXMESH &&
 		    !(dims[4 * i] >= 0.0f && int(dims[4 * i]) < int(s->hHullInfo.size()) && float(int(dims[4 * i])) == dims[4 * i]))
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_bodies: convex mesh index out of range");

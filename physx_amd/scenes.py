@@ -75,7 +75,11 @@ class Scene:
 
 
 def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
-    """Mass and mass-space inertia of solid spheres and boxes (closed forms)."""
+    """Mass and mass-space inertia of solid spheres, capsules and boxes (closed
+    forms). A capsule (radius r, half height h along x) is a cylinder of length
+    2h and two hemispherical caps; each cap's inertia about the capsule centre
+    perpendicular to the axis is m_cap (2/5 r^2 + h^2 + 3 h r / 4) (its own
+    centroid sits 3r/8 from the flat face)."""
     dims = dims.astype(np.float64)
     n = shape_type.shape[0]
     mass = np.zeros(n)
@@ -84,6 +88,15 @@ def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
     r = dims[s, 0]
     mass[s] = density * (4.0 / 3.0) * math.pi * r ** 3
     inertia[s] = (0.4 * mass[s] * r * r)[:, None]
+    c = shape_type == CAPSULE
+    r, hh = dims[c, 0], dims[c, 1]
+    m_cyl = density * math.pi * r * r * 2.0 * hh
+    m_sph = density * (4.0 / 3.0) * math.pi * r ** 3
+    mass[c] = m_cyl + m_sph
+    inertia[c, 0] = 0.5 * m_cyl * r * r + 0.4 * m_sph * r * r
+    perp = m_cyl * (0.25 * r * r + (2.0 * hh) ** 2 / 12.0) + m_sph * (0.4 * r * r + hh * hh + 0.75 * hh * r)
+    inertia[c, 1] = perp
+    inertia[c, 2] = perp
     b = shape_type == BOX
     h = dims[b, :3]
     mass[b] = density * 8.0 * h[:, 0] * h[:, 1] * h[:, 2]
@@ -245,3 +258,59 @@ def convex_layers(n_bodies=65536, per_side=64, pitch=1.3, n_points=24, amin=0.2,
                  np.array([[0, 1, 0, 0]], np.float32), np.zeros(1, np.int32), 1, "convex_layers",
                  dict(n_bodies=n_bodies, per_side=per_side, pitch=pitch, n_points=n_points, seed=seed),
                  local.reshape(-1, 3), np.full(n_bodies, n_points, np.int32))
+
+
+def mixed_pile(n_bodies=512, per_side=8, pitch=1.0, jitter=0.1, n_points=16, seed=9) -> Scene:
+    """Spheres, capsules, boxes and convex hulls in equal shares, dropped in
+    layers of per_side x per_side onto the plane y=0 (grid cell (x, z, layer)
+    holds kind (x + 2 z + layer) mod 4, so neighbours along x, along z and
+    above differ in kind) with seeded
+    horizontal jitter U[-jitter, jitter] and random orientations: sphere
+    radius U[0.1,0.3], capsule radius U[0.08,0.15] and half height
+    U[0.05,0.25], box half extents U[0.1,0.3], hulls of n_points points on
+    ellipsoids with semi-axes U[0.15,0.35] in their principal frames."""
+    from . import cook_convex
+    ids = np.arange(n_bodies)
+    per_layer = per_side * per_side
+    lx, lz, ly = ids % per_side, (ids // per_side) % per_side, ids // per_layer
+    kind = (lx + 2 * lz + ly) % 4
+    shape_type = np.array([SPHERE, CAPSULE, BOX, CONVEX], np.int32)[kind]
+    dims = np.zeros((n_bodies, 4), np.float32)
+    sp, ca, bx, cv = (kind == k for k in range(4))
+    dims[sp, 0] = uniform(seed, ids[sp], 0, 0.1, 0.3)
+    dims[ca, 0] = uniform(seed, ids[ca], 0, 0.08, 0.15)
+    dims[ca, 1] = uniform(seed, ids[ca], 1, 0.05, 0.25)
+    for d in range(3):
+        dims[bx, d] = uniform(seed, ids[bx], d, 0.1, 0.3)
+    cvx_ids = ids[cv]
+    nc = cvx_ids.shape[0]
+    ax = np.stack([uniform(seed, cvx_ids, 3 + d, 0.15, 0.35) for d in range(3)], 1)
+    gid = ((cvx_ids[:, None] << 8) + np.arange(n_points)[None, :]).ravel()
+    z = uniform(seed + 1, gid, 0, -1.0, 1.0).reshape(nc, n_points)
+    phi = uniform(seed + 1, gid, 1, 0.0, 2 * math.pi).reshape(nc, n_points)
+    rxy = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    pts = (np.stack([rxy * np.cos(phi), rxy * np.sin(phi), z], 2) * ax[:, None, :]).astype(np.float32)
+    mass, inertia = mass_props(shape_type, dims, 1000.0)
+    local = np.zeros_like(pts)
+    for k in range(nc):
+        h = cook_convex(pts[k])
+        if h is None:
+            raise RuntimeError(f"mixed_pile: degenerate point set for hull {k}")
+        w, vecs = np.linalg.eigh(h["inertia"])
+        if np.linalg.det(vecs) < 0:
+            vecs[:, 2] *= -1.0
+        local[k] = ((pts[k].astype(np.float64) - h["com"]) @ vecs).astype(np.float32)
+        mass[cvx_ids[k]] = 1000.0 * h["volume"]
+        inertia[cvx_ids[k]] = 1000.0 * w
+    dims[cv, 0] = np.arange(nc)
+    jx = uniform(seed + 3, ids, 0, -jitter, jitter)
+    jz = uniform(seed + 3, ids, 1, -jitter, jitter)
+    pos = np.stack([(lx - (per_side - 1) / 2) * pitch + jx, 0.45 + ly * pitch,
+                    (lz - (per_side - 1) / 2) * pitch + jz], 1)
+    quat = _rand_quat(seed + 2, ids, 0)
+    pose = np.concatenate([pos, quat], 1).astype(np.float32)
+    return Scene(shape_type, dims, pose, np.zeros((n_bodies, 3), np.float32), np.zeros((n_bodies, 3), np.float32),
+                 mass.astype(np.float32), inertia.astype(np.float32), np.zeros(n_bodies, np.int32),
+                 np.array([[0, 1, 0, 0]], np.float32), np.zeros(1, np.int32), 1, "mixed_pile",
+                 dict(n_bodies=n_bodies, per_side=per_side, pitch=pitch, jitter=jitter, n_points=n_points, seed=seed),
+                 local.reshape(-1, 3), np.full(nc, n_points, np.int32))

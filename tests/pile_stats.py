@@ -4,13 +4,15 @@ import numpy as np
 
 
 def lowest_points(shape_type, dims, pose, scene=None):
-    """Height of the lowest point of each sphere (type 0), box (type 3) or
-    convex mesh (type 5, vertices from scene.hull_points)."""
+    """Height of the lowest point of each sphere (type 0), capsule (type 2,
+    radius and half height along the body x axis), box (type 3) or convex
+    mesh (type 5, vertices from scene.hull_points)."""
     x, y, z, w = pose[:, 3], pose[:, 4], pose[:, 5], pose[:, 6]
     # second row of the rotation matrix: world-y component of each body axis
     ry = np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], 1)
     reach = (np.abs(ry) * dims[:, :3]).sum(1)
     reach = np.where(shape_type == 0, dims[:, 0], reach)
+    reach = np.where(shape_type == 2, np.abs(ry[:, 0]) * dims[:, 1] + dims[:, 0], reach)
     cvx = np.nonzero(shape_type == 5)[0]
     if len(cvx):
         off = np.concatenate([[0], np.cumsum(scene.hull_counts)])
