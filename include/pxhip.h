@@ -220,7 +220,10 @@ int32_t pxh_bp_release(pxh_broadphase bp);
  * PxGenerateContacts (physx/include/PxImmediateMode.h:267). Device arrays:
  * type0/type1[n], geom0/geom1[n*4], pose0/pose1[n*7] (p xyz, q xyzw); out
  * [n*4*8]: per pair up to 4 contacts of (point xyz, separation, normal xyz, 0);
- * counts[n]. Normals point from shape1 to shape0. */
+ * counts[n] (-1 for a pair kind without a generator here). Pairs: sphere-
+ * sphere, sphere-plane, sphere-capsule, sphere-box, plane-capsule, plane-box,
+ * capsule-capsule, capsule-box, box-box, in that shape order (lower geometry
+ * type first). Normals point from shape1 to shape0. */
 int32_t pxh_generate_contacts(int32_t n, const int32_t* type0, const float* geom0, const float* pose0,
                               const int32_t* type1, const float* geom1, const float* pose1, float contact_distance,
                               float tolerance_length, float* out, int32_t* counts, void* stream);

@@ -2034,6 +2034,17 @@ __global__ void k_np_immediate(int n, const int* type0, const float* geom0, cons
 	}
 	else if (t0 == GEOM_BOX && t1 == GEOM_BOX)
 		cnt = boxBox(mk(g0[0], g0[1], g0[2]), x0, mk(g1[0], g1[1], g1[2]), x1, cd, tol, c);
+	else if (t0 == GEOM_SPHERE && t1 == GEOM_CAPSULE)
+		cnt = sphereCapsule(g0[0], x0.p, g1[0], g1[1], x1, cd, c);
+	else if (t0 == GEOM_CAPSULE && t1 == GEOM_CAPSULE)
+		cnt = capsuleCapsule(g0[0], g0[1], x0, g1[0], g1[1], x1, cd, c);
+	else if (t0 == GEOM_PLANE && t1 == GEOM_CAPSULE)
+	{
+		const f3 pn = qrot(x0.q, mk(1.0f, 0.0f, 0.0f));
+		cnt = planeCapsule(pn, -dot(pn, x0.p), g1[0], g1[1], x1, cd, c);
+	}
+	else if (t0 == GEOM_CAPSULE && t1 == GEOM_BOX)
+		cnt = capsulePoly(g0[0], g0[1], x0, boxPoly(mk(g1[0], g1[1], g1[2])), x1, cd, tol, c);
 	counts[k] = cnt;
 	for (int i = 0; i < (cnt > 0 ? cnt : 0); i++)
 	{
