@@ -40,7 +40,13 @@ struct SolverBodies
 	float4* i0;       // world sqrt inverse inertia (xx yy zz xy)
 	float4* i1;       // (xz yz - -)
 	float4* origLin;  // pre-integrated linear velocity (friction directions)
+	// what the contact preparation reads of a body, packed into one 128-byte
+	// line at [8 * i]: position | inverse mass, rotation, sqrt inverse inertia
+	// (xx yy zz xy), (xz yz - -), pre-integrated linear velocity, and for PGS
+	// the pre-integrated angular velocity
+	float4* rec;
 };
+constexpr int PREP_REC = 8;
 
 // Contact rows in colour order; slot k of constraint o is at [k * C + o]
 struct Rows
@@ -76,6 +82,9 @@ struct PatchIO
 	uint64_t* curKey;          // this step's pair keys
 	int* curColor;             // this step's colour of each row
 	float corrDist;            // friction correlation distance
+	// per constraint c, what the preparation gathers, packed into 32 bytes at
+	// [2 * c]: body a, body b, contact slot, previous row | pair key (lo, hi)
+	int4* conRec;
 };
 
 int64_t scan_tmp_size(int64_t n);
@@ -142,8 +151,8 @@ void launch_static_colors(int n, int C, const int* conA, const int* conB, int* c
 void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal, uint32_t htMask, hipStream_t st);
 // finds each constraint's row of the previous step; body-body constraints keep
 // their previous colour (marked in the per-body colour masks)
-void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor, uint64_t* mask,
-                     hipStream_t st);
+void launch_con_prev(int C, const int* conA, const int* conB, const int* conSlot, PatchIO io, int* conColor,
+                     uint64_t* mask, hipStream_t st);
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,

@@ -1025,15 +1025,19 @@ PXH_D int patchFind(const PatchIO& io, uint64_t key)
 // (below 64): those constraints were coloured together then, so they never
 // share a colour at a body now. The reference GPU solver keeps its partitions
 // from step to step the same way (gpusolver/src/PxgConstraintPartition.cpp:1940).
-__global__ void k_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor,
+__global__ void k_con_prev(int C, const int* conA, const int* conB, const int* conSlot, PatchIO io, int* conColor,
                            unsigned long long* mask)
 {
 	const int c = blockIdx.x * blockDim.x + threadIdx.x;
 	if (c >= C)
 		return;
 	const int a = conA[c], b = conB[c];
-	const int po = patchFind(io, io.conPKey[c]);
-	io.prevRow[c] = po;
+	const uint64_t pkey = io.conPKey[c];
+	const int po = patchFind(io, pkey);
+	if (io.prevRow)
+		io.prevRow[c] = po;
+	io.conRec[2 * int64_t(c)] = make_int4(a, b, conSlot[c], po);
+	io.conRec[2 * int64_t(c) + 1] = make_int4(int(uint32_t(pkey)), int(uint32_t(pkey >> 32)), 0, 0);
 	if (b >= 0 && po >= 0)
 	{
 		const int pc = io.prevColor[po];
@@ -1089,7 +1093,7 @@ __global__ void k_color_keys(int C, const int* conColor, uint32_t limit, uint64_
 __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                                const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                                float maxAngSq, int pgs, float4* vel, float4* svP, float4* svDQ, float4* svI0,
-                               float4* svI1, float4* svOrigLin)
+                               float4* svI1, float4* svOrigLin, float4* rec)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
@@ -1136,9 +1140,19 @@ __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, cons
 	svP[i] = pos[i];
 	svDQ[i] = make_float4(0.0f, 0.0f, 0.0f, 1.0f);
 	// symmetric sqrt inverse inertia: (xx yy zz xy) (xz yz)
-	svI0[i] = make_float4(S.c0.x, S.c1.y, S.c2.z, S.c1.x);
-	svI1[i] = make_float4(S.c2.x, S.c2.y, 0.0f, 0.0f);
+	const float4 i0 = make_float4(S.c0.x, S.c1.y, S.c2.z, S.c1.x);
+	const float4 i1 = make_float4(S.c2.x, S.c2.y, 0.0f, 0.0f);
+	svI0[i] = i0;
+	svI1[i] = i1;
 	svOrigLin[i] = f4(lv, 0.0f);
+	float4* r = rec + PREP_REC * int64_t(i);
+	const float4 p = pos[i];
+	r[0] = make_float4(p.x, p.y, p.z, l.w);
+	r[1] = rot[i];
+	r[2] = i0;
+	r[3] = i1;
+	r[4] = f4(lv, 0.0f);
+	r[5] = f4(av, 0.0f);
 }
 
 PXH_D m33 loadSym(float4 a, float4 b)
@@ -1159,18 +1173,16 @@ PXH_D m33 loadSym(float4 a, float4 b)
 // DyFrictionCorrelation.cpp:136-316). Writes the patch for the next step and
 // returns the anchor count, the anchors in pa0/pa1 (body space). Shared by the
 // TGS and PGS preparation.
-PXH_D int frictionPatch(int o, int c, int a, int b, f3 n, const xf& f0, const xf& f1, const f3 (&cp)[4],
+PXH_D int frictionPatch(int o, int c, int po, uint64_t pkey, f3 n, const xf& f0, const xf& f1, const f3 (&cp)[4],
                         const float (&cs)[4], int cnt, float restDist, float frictionOffsetThreshold,
                         const PatchIO& io, f3 (&pa0)[2], f3 (&pa1)[2])
 {
 	// friction patch: the previous step's patch of this pair is kept when it is
 	// unbroken, has anchors, its body normals still agree, no anchor drifted
 	// apart along the normal, and its world normal matches this step's normal
-	const uint64_t pkey = io.conPKey[c];
 	f3 pn0, pn1;
 	int nKept = 0;
 	{
-		const int po = io.prevRow ? io.prevRow[c] : -1;
 		bool kept = false;
 		if (po >= 0 && !io.prevBroken[po])
 		{
@@ -1287,75 +1299,109 @@ PXH_D int frictionPatch(int o, int c, int a, int b, f3 n, const xf& f0, const xf
 	return nAnch;
 }
 
-PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
-                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
-                    uint32_t& nFricOut);
+PXH_D void prepBody(int o, int C, const int* order, const float4* ctN, const float4* ctP, const float4* rec,
+                    float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows& rows,
+                    const PatchIO& io, uint32_t& nCntOut, uint32_t& nDynOut, uint32_t& nFricOut);
+
+// Per-block sums of four per-thread counts (256-thread blocks) stored at
+// partial[4 * block]: no atomics on shared counters from every wave (with
+// thousands of waves those serialise); k_sum_counters adds the blocks up.
+PXH_D void blockCounters4(uint32_t v0, uint32_t v1, uint32_t v2, uint32_t v3, unsigned long long* partial)
+{
+	__shared__ uint32_t red[4][4];
+	const uint32_t s0 = waveInclScan(v0), s1 = waveInclScan(v1), s2 = waveInclScan(v2), s3 = waveInclScan(v3);
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 63)
+	{
+		red[w][0] = s0; red[w][1] = s1; red[w][2] = s2; red[w][3] = s3;
+	}
+	__syncthreads();
+	if (threadIdx.x < 4)
+	{
+		unsigned long long t = 0;
+		for (int k = 0; k < int(blockDim.x >> 6); k++)
+			t += red[k][threadIdx.x];
+		partial[4 * int64_t(blockIdx.x) + threadIdx.x] = t;
+	}
+}
+
+// counters[k] = sum over blocks of partial[4 * block + k] (one block)
+__global__ void __launch_bounds__(256) k_sum_counters(int nBlocks, const unsigned long long* partial,
+                                                      unsigned long long* counters)
+{
+	__shared__ unsigned long long red[4][256];
+	unsigned long long t[4] = {0, 0, 0, 0};
+	for (int b = threadIdx.x; b < nBlocks; b += blockDim.x)
+		for (int k = 0; k < 4; k++)
+			t[k] += partial[4 * int64_t(b) + k];
+	for (int k = 0; k < 4; k++)
+		red[k][threadIdx.x] = t[k];
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1)
+	{
+		if (threadIdx.x < o)
+			for (int k = 0; k < 4; k++)
+				red[k][threadIdx.x] += red[k][threadIdx.x + o];
+		__syncthreads();
+	}
+	if (threadIdx.x < 4)
+		counters[threadIdx.x] = red[threadIdx.x][0];
+}
 
 // Contact rows of one constraint at its colour-ordered position o
 // (setupFinalizeSolverConstraints / constructContactConstraintStep,
 //  lowleveldynamics/src/DyTGSContactPrep.cpp:310-404, 426-812; friction patches
 //  getFrictionPatches / correlatePatches / growPatches, DyContactPrepShared.h:53-127,
 //  DyFrictionCorrelation.cpp:136-316)
-__global__ void k_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                       const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
-                       const float4* vel, const float4* svOrigLin, const float4* svI0, const float4* svI1,
+__global__ void k_prep(int C, const int* order, const float4* ctN, const float4* ctP, const float4* rec,
                        float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows rows,
                        PatchIO io, unsigned long long* counters)
 {
 	const int o = blockIdx.x * blockDim.x + threadIdx.x;
 	// per-step counters: contact points, friction rows, dynamic-dynamic constraints
 	uint32_t myCnt = 0, myFric = 0, myDyn = 0;
-	if (o < C)
-	{
-		const int c = order[o];
-		myCnt = uint32_t(__float_as_int(ctN[conSlot[c]].w));
-		myDyn = conB[c] >= 0 ? 1u : 0u;
-	}
-	prepBody(o, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, vel, svOrigLin, svI0, svI1, invStepDt, scale,
-	         restDist, frictionOffsetThreshold, rows, io, myFric);
-	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
-	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
-	if ((threadIdx.x & 63) == 63)
-	{
-		atomicAdd(&counters[0], (unsigned long long)sc);
-		atomicAdd(&counters[1], (unsigned long long)sf);
-		atomicAdd(&counters[2], (unsigned long long)sd);
-		atomicAdd(&counters[3], (unsigned long long)sfd);
-	}
+	prepBody(o, C, order, ctN, ctP, rec, invStepDt, scale, restDist, frictionOffsetThreshold, rows, io, myCnt, myDyn,
+	         myFric);
+	blockCounters4(myCnt, myFric, myDyn, myDyn ? myFric : 0u, counters);
 }
 
-PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                    const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, const float4* vel,
-                    const float4* svOrigLin, const float4* svI0, const float4* svI1, float invStepDt, float scale,
-                    float restDist, float frictionOffsetThreshold, Rows& rows, const PatchIO& io,
-                    uint32_t& nFricOut)
+PXH_D void prepBody(int o, int C, const int* order, const float4* ctN, const float4* ctP, const float4* rec,
+                    float invStepDt, float scale, float restDist, float frictionOffsetThreshold, Rows& rows,
+                    const PatchIO& io, uint32_t& nCntOut, uint32_t& nDynOut, uint32_t& nFricOut)
 {
 	if (o >= C)
 		return;
 	const int c = order[o];
-	const int a = conA[c], b = conB[c], slot = conSlot[c];
+	const int4 cr = io.conRec[2 * int64_t(c)], ck = io.conRec[2 * int64_t(c) + 1];
+	const int a = cr.x, b = cr.y, slot = cr.z, po = cr.w;
+	const uint64_t pkey = (uint64_t(uint32_t(ck.y)) << 32) | uint32_t(ck.x);
 	const float4 nn = ctN[slot];
 	const int cnt = __float_as_int(nn.w);
 	const f3 n = mk(nn);
 	const bool dynB = b >= 0;
-	const m33 Sa = loadSym(svI0[a], svI1[a]);
+	nCntOut = uint32_t(cnt);
+	nDynOut = dynB ? 1u : 0u;
+	const float4* ra4 = rec + PREP_REC * int64_t(a);
+	const float4 pa4 = ra4[0];
+	const m33 Sa = loadSym(ra4[2], ra4[3]);
+	const f3 linA = mk(ra4[4]);
 	m33 Sb;
 	Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
 	float invMassB = 0.0f;
 	xf f0, f1;
-	f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
+	f0.p = mk(pa4); f0.q = mkq(ra4[1]);
 	f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
 	f3 linB = mk(0, 0, 0);
 	if (dynB)
 	{
-		Sb = loadSym(svI0[b], svI1[b]);
-		invMassB = vel[4 * int64_t(b)].w;
-		f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
-		linB = mk(svOrigLin[b]);
+		const float4* rb4 = rec + PREP_REC * int64_t(b);
+		const float4 pb4 = rb4[0];
+		Sb = loadSym(rb4[2], rb4[3]);
+		invMassB = pb4.w;
+		f1.p = mk(pb4); f1.q = mkq(rb4[1]);
+		linB = mk(rb4[4]);
 	}
-	const float invMassA = vel[4 * int64_t(a)].w;
+	const float invMassA = pa4.w;
 	const float nlen2 = dot(n, n);
 	const float imn0 = invMassA * nlen2, imn1 = -invMassB * nlen2;
 	const float invDtp8 = invStepDt * scale;
@@ -1386,14 +1432,14 @@ PXH_D void prepBody(int o, int C, const int* order, const int* conA, const int* 
 	}
 	f3 pa0[2], pa1[2];
 	const int nAnch =
-	    frictionPatch(o, c, a, b, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
+	    frictionPatch(o, c, po, pkey, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
 	nFricOut = uint32_t(2 * nAnch);
 	(void)invDtp8;
 	rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), 0);
 	rows.hF[o] = make_float4(n.x, n.y, n.z, 0.0f);
 	if (nAnch == 0)
 		return;
-	const f3 lvr = mk(svOrigLin[a]) - linB;
+	const f3 lvr = linA - linB;
 	const f3 fb = (0.70710678f > fabsf(n.x)) ? mk(0.0f, -n.z, n.y) : mk(-n.y, n.x, 0.0f);
 	f3 t0 = lvr - n * dot(n, lvr);
 	t0 = dot(t0, t0) > 0.0001f ? t0 : fb;
@@ -1574,9 +1620,8 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 
 // setupFinalizeSolverConstraints (lowleveldynamics/src/DyContactPrep.cpp:49-357)
 // with constructContactConstraint (DyContactPrepShared.h:298-393)
-__global__ void k_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
-                           const float4* ctN, const float4* ctP, const float4* pos, const float4* rot,
-                           const float4* vel, const float4* svI0, const float4* svI1, float invDt, float muS,
+__global__ void k_prep_pgs(int C, const int* order, const float4* ctN, const float4* ctP, const float4* rec,
+                           float invDt, float muS,
                            float muD, float restDist, float frictionOffsetThreshold, Rows rows, float* rowsU,
                            PatchIO io, unsigned long long* counters)
 {
@@ -1585,32 +1630,36 @@ __global__ void k_prep_pgs(int C, const int* order, const int* conA, const int* 
 	if (o < C)
 	{
 		const int c = order[o];
-		const int a = conA[c], b = conB[c], slot = conSlot[c];
+		const int4 cr = io.conRec[2 * int64_t(c)], ck = io.conRec[2 * int64_t(c) + 1];
+		const int a = cr.x, b = cr.y, slot = cr.z, po = cr.w;
+		const uint64_t pkey = (uint64_t(uint32_t(ck.y)) << 32) | uint32_t(ck.x);
 		const float4 nn = ctN[slot];
 		const int cnt = __float_as_int(nn.w);
 		const f3 n = mk(nn);
 		const bool dynB = b >= 0;
 		myCnt = uint32_t(cnt);
 		myDyn = dynB ? 1u : 0u;
-		const m33 Sa = loadSym(svI0[a], svI1[a]);
+		const float4* ra4 = rec + PREP_REC * int64_t(a);
+		const float4 pa4 = ra4[0];
+		const m33 Sa = loadSym(ra4[2], ra4[3]);
 		m33 Sb;
 		Sb.c0 = mk(0, 0, 0); Sb.c1 = Sb.c0; Sb.c2 = Sb.c0;
 		xf f0, f1;
-		f0.p = mk(pos[a]); f0.q = mkq(rot[a]);
+		f0.p = mk(pa4); f0.q = mkq(ra4[1]);
 		f1.p = mk(0, 0, 0); f1.q.x = 0; f1.q.y = 0; f1.q.z = 0; f1.q.w = 1;
-		const float4* va = vel + 4 * int64_t(a);
-		const float invMassA = va[0].w;
-		const f3 v0 = mk(va[2]), w0 = mk(va[3]);
+		const float invMassA = pa4.w;
+		const f3 v0 = mk(ra4[4]), w0 = mk(ra4[5]);
 		float invMassB = 0.0f;
 		f3 v1 = mk(0, 0, 0), w1 = mk(0, 0, 0);
 		if (dynB)
 		{
-			const float4* vb = vel + 4 * int64_t(b);
-			Sb = loadSym(svI0[b], svI1[b]);
-			invMassB = vb[0].w;
-			v1 = mk(vb[2]);
-			w1 = mk(vb[3]);
-			f1.p = mk(pos[b]); f1.q = mkq(rot[b]);
+			const float4* rb4 = rec + PREP_REC * int64_t(b);
+			const float4 pb4 = rb4[0];
+			Sb = loadSym(rb4[2], rb4[3]);
+			invMassB = pb4.w;
+			v1 = mk(rb4[4]);
+			w1 = mk(rb4[5]);
+			f1.p = mk(pb4); f1.q = mkq(rb4[1]);
 		}
 		const float nl2 = dot(n, n);
 		const float imn0 = invMassA * nl2, imn1 = -invMassB * nl2;
@@ -1646,7 +1695,7 @@ __global__ void k_prep_pgs(int C, const int* order, const int* conA, const int* 
 		}
 		f3 pa0[2], pa1[2];
 		const int nAnch =
-		    frictionPatch(o, c, a, b, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
+		    frictionPatch(o, c, po, pkey, n, f0, f1, cp, cs, cnt, restDist, frictionOffsetThreshold, io, pa0, pa1);
 		myFric = uint32_t(2 * nAnch);
 		const float fc = nAnch == 2 ? 0.5f : 1.0f;
 		rows.hI[o] = make_int4(a, b, cnt | ((2 * nAnch) << 8), __float_as_int(muD * fc));
@@ -1685,15 +1734,7 @@ __global__ void k_prep_pgs(int C, const int* order, const int* conA, const int* 
 			}
 		}
 	}
-	const uint32_t sc = waveInclScan(myCnt), sf = waveInclScan(myFric), sd = waveInclScan(myDyn);
-	const uint32_t sfd = waveInclScan(myDyn ? myFric : 0u);
-	if ((threadIdx.x & 63) == 63)
-	{
-		atomicAdd(&counters[0], (unsigned long long)sc);
-		atomicAdd(&counters[1], (unsigned long long)sf);
-		atomicAdd(&counters[2], (unsigned long long)sd);
-		atomicAdd(&counters[3], (unsigned long long)sfd);
-	}
+	blockCounters4(myCnt, myFric, myDyn, myDyn ? myFric : 0u, counters);
 }
 
 // one colour of PGS (solveContact, lowleveldynamics/src/DySolverConstraints.cpp:221-371;
@@ -2264,10 +2305,10 @@ void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal
 	PXH_LAUNCH(k_fill_u64, int(htMask) + 1, htKey, int(htMask) + 1, ~0ull);
 	PXH_LAUNCH(k_patch_insert, m, m, keys, reinterpret_cast<unsigned long long*>(htKey), htVal, htMask);
 }
-void launch_con_prev(int C, const int* conA, const int* conB, PatchIO io, int* conColor, uint64_t* mask,
-                     hipStream_t st)
+void launch_con_prev(int C, const int* conA, const int* conB, const int* conSlot, PatchIO io, int* conColor,
+                     uint64_t* mask, hipStream_t st)
 {
-	PXH_LAUNCH(k_con_prev, C, C, conA, conB, io, conColor, reinterpret_cast<unsigned long long*>(mask));
+	PXH_LAUNCH(k_con_prev, C, C, conA, conB, conSlot, io, conColor, reinterpret_cast<unsigned long long*>(mask));
 }
 void launch_color_keys(int C, const int* conColor, uint32_t limit, uint64_t* keys, uint32_t* vals, hipStream_t st)
 {
@@ -2283,15 +2324,20 @@ void launch_preintegrate(int n, const float4* pos, const float4* rot, const floa
                          float maxAngSq, bool pgs, SolverBodies sb, hipStream_t st)
 {
 	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, pgs ? 1 : 0, sb.vel,
-	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin);
+	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin, sb.rec);
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
                  hipStream_t st)
 {
-	PXH_LAUNCH(k_prep, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.vel, sb.origLin, sb.i0, sb.i1,
-	           invStepDt, scale, restDist, fot, rows, patches, counters);
+	(void)pos;
+	(void)rot;
+	(void)conA;
+	(void)conB;
+	(void)conSlot;
+	PXH_LAUNCH(k_prep, C, C, order, ctN, ctP, sb.rec, invStepDt, scale, restDist, fot, rows, patches, counters + 4);
+	k_sum_counters<<<1, 256, 0, st>>>(nblk(C, 256), counters + 4, counters);
 }
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st, hipEvent_t evStart,
@@ -2312,8 +2358,14 @@ void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, 
                      float invDt, float muS, float muD, float restDist, float fot, Rows rows, float* rowsU,
                      PatchIO patches, unsigned long long* counters, hipStream_t st)
 {
-	PXH_LAUNCH(k_prep_pgs, C, C, order, conA, conB, conSlot, ctN, ctP, pos, rot, sb.vel, sb.i0, sb.i1, invDt, muS,
-	           muD, restDist, fot, rows, rowsU, patches, counters);
+	(void)pos;
+	(void)rot;
+	(void)conA;
+	(void)conB;
+	(void)conSlot;
+	PXH_LAUNCH(k_prep_pgs, C, C, order, ctN, ctP, sb.rec, invDt, muS, muD, restDist, fot, rows, rowsU, patches,
+	           counters + 4);
+	k_sum_counters<<<1, 256, 0, st>>>(nblk(C, 256), counters + 4, counters);
 }
 void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, SolverBodies sb, bool doFriction,
                       bool velocityPass, uint32_t* broken, hipStream_t st, hipEvent_t evStart, hipEvent_t evStop)

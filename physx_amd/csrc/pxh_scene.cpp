@@ -177,7 +177,7 @@ struct pxh_scene_t
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
 	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0;
 	pxh::SolverBodies sb{};
-	DBuf<float4> sbVel, sbP, sbDQ, sbI0, sbI1, sbOrig;
+	DBuf<float4> sbVel, sbP, sbDQ, sbI0, sbI1, sbOrig, sbRec;
 	DBuf<int4> rhI;
 	DBuf<float4> rhF, rnA, rnB, rfT, rfA, rfB;
 	DBuf<float> rnF, rfF, rnU;
@@ -188,6 +188,7 @@ struct pxh_scene_t
 	DBuf<uint32_t> patchBroken[2];
 	DBuf<int> patchColor[2];
 	DBuf<int> prevRow;
+	DBuf<int4> conRec;   // packed per-constraint record the preparation gathers
 	DBuf<uint64_t> htKey;
 	DBuf<int> htVal;
 	DBuf<uint32_t> bodyNext;
@@ -501,7 +502,7 @@ int32_t pxh_scene_release(pxh_scene s)
 	(void)hipStreamSynchronize(s->stream);
 	DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->planes, &s->bmin, &s->bmax,
 	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbVel, &s->sbP, &s->sbDQ,
-	                       &s->sbI0, &s->sbI1, &s->sbOrig, &s->rhF, &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
+	                       &s->sbI0, &s->sbI1, &s->sbOrig, &s->sbRec, &s->rhF, &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
 	for (auto* b : f4s)
 		b->release();
 	DBuf<int>* is[] = {&s->scene, &s->local, &s->bodyStart, &s->planeLocal, &s->qBody, &s->qPlane, &s->conA,
@@ -535,6 +536,7 @@ int32_t pxh_scene_release(pxh_scene s)
 		s->patchColor[k].release();
 	}
 	s->prevRow.release();
+	s->conRec.release();
 	s->htKey.release();
 	s->htVal.release();
 	s->bodyNext.release();
@@ -954,10 +956,12 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	PXH_CHECK(s->best.ensure(n));
 	PXH_CHECK(s->mask.ensure(n));
 	PXH_CHECK(s->prevRow.ensure(Cc));
+	PXH_CHECK(s->conRec.ensure(2 * Cc));
 	pxh::launch_fill_u64(s->mask.p, n, 0ull, st);
 	pxh::PatchIO pio{};
 	pio.corrDist = d.friction_correlation_distance;
 	pio.conPKey = s->conPKey.p;
+	pio.conRec = s->conRec.p;
 	if (s->nPrevRows > 0)
 	{
 		uint32_t size = 64;
@@ -973,8 +977,10 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		pio.prevBroken = s->patchBroken[prv].p;
 		pio.prevColor = s->patchColor[prv].p;
 		pio.prevRow = s->prevRow.p;
-		pxh::launch_con_prev(C, s->conA.p, s->conB.p, pio, s->conColor.p, s->mask.p, st);
 	}
+	// every step: the packed constraint records (and, after a previous step,
+	// each constraint's previous row and colour)
+	pxh::launch_con_prev(C, s->conA.p, s->conB.p, s->conSlot.p, pio, s->conColor.p, s->mask.p, st);
 
 	// 5. colouring of the remaining body-body constraints: rounds of local key
 	//    minima, each taking its first free colour, over a compacted list of the
@@ -1062,7 +1068,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// 6. solver bodies + contact rows
 	PXH_CHECK(s->sbVel.ensure(size_t(4) * n)); PXH_CHECK(s->sbP.ensure(n)); PXH_CHECK(s->sbDQ.ensure(n));
 	PXH_CHECK(s->sbI0.ensure(n)); PXH_CHECK(s->sbI1.ensure(n)); PXH_CHECK(s->sbOrig.ensure(n));
-	pxh::SolverBodies sb{s->sbVel.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p, s->sbOrig.p};
+	PXH_CHECK(s->sbRec.ensure(size_t(pxh::PREP_REC) * n));
+	pxh::SolverBodies sb{s->sbVel.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p, s->sbOrig.p, s->sbRec.p};
 	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
 	const bool pgs = d.solver_type == PXH_SOLVER_PGS;
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
@@ -1077,8 +1084,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float invStepDt = 1.0f / stepDt;
 	const float biasCoefficient = 2.0f * std::sqrt(1.0f / float(P_));
 	const float scale = std::min(0.8f, biasCoefficient);
-	PXH_CHECK(s->counters.ensure(4));
-	PXH_CHECK(hipMemsetAsync(s->counters.p, 0, 4 * sizeof(unsigned long long), st));
+	// four step counters, then four per-block partial sums of the preparation
+	PXH_CHECK(s->counters.ensure(4 + 4 * ((Cc + 255) / 256)));
 	// friction patches and colours of this step's rows go to the [cur] buffers
 	PXH_CHECK(s->patch[cur].ensure(6 * Cc));
 	PXH_CHECK(s->patchKey[cur].ensure(Cc));
