@@ -1128,6 +1128,7 @@ struct CvxLds
 	uint32_t ea[CVX_WAVE_E];
 	uint32_t eb[CVX_WAVE_E];
 	uint8_t vrA[2 * CVX_WAVE_E], vrB[2 * CVX_WAVE_E];   // a closed polytope has 2 vrefs per edge
+	uint8_t elA[CVX_WAVE_E], elB[CVX_WAVE_E];           // edges kept for the closest-feature search
 };
 
 PXH_D bool fitsWave(const Poly& A, const Poly& B)
@@ -1337,9 +1338,74 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 		d2 = h * h;
 		di = (1u << 24) | uint32_t(k);
 	}
-	for (int k = lane; k < A.ne * B.ne; k += 64)
+	// Edge pairs. The face-vertex pairs bound the squared distance by U; an
+	// edge whose bounding sphere lies farther than sqrt(U) (plus a margin well
+	// above rounding) from the other polytope's bounding box cannot be part of
+	// a pair that reaches U, so only the remaining edges are paired. Pair
+	// indices keep their unpruned order, so the first minimum is unchanged.
+	float U = d2;
+	for (int off = 32; off >= 1; off >>= 1)
+		U = fminf(U, __shfl_xor(U, off));
+	const float lim = U < BIG ? sqrtf(U) + 1e-4f : BIG;
+	f3 aMin = mk(BIG, BIG, BIG), aMax = -aMin, bMin = aMin, bMax = aMax;
+	if (lane < A.nv)
 	{
-		const int ea = k / B.ne, ebi = k - ea * B.ne;
+		aMin = ldsV(L.va, lane);
+		aMax = aMin;
+	}
+	if (lane < B.nv)
+	{
+		bMin = ldsV(L.vb, lane);
+		bMax = bMin;
+	}
+	for (int off = 32; off >= 1; off >>= 1)
+	{
+		aMin = mk(fminf(aMin.x, __shfl_xor(aMin.x, off)), fminf(aMin.y, __shfl_xor(aMin.y, off)),
+		          fminf(aMin.z, __shfl_xor(aMin.z, off)));
+		aMax = mk(fmaxf(aMax.x, __shfl_xor(aMax.x, off)), fmaxf(aMax.y, __shfl_xor(aMax.y, off)),
+		          fmaxf(aMax.z, __shfl_xor(aMax.z, off)));
+		bMin = mk(fminf(bMin.x, __shfl_xor(bMin.x, off)), fminf(bMin.y, __shfl_xor(bMin.y, off)),
+		          fminf(bMin.z, __shfl_xor(bMin.z, off)));
+		bMax = mk(fmaxf(bMax.x, __shfl_xor(bMax.x, off)), fmaxf(bMax.y, __shfl_xor(bMax.y, off)),
+		          fmaxf(bMax.z, __shfl_xor(bMax.z, off)));
+	}
+	CvxLds& W = const_cast<CvxLds&>(L);
+	int nA = 0, nB = 0;
+	for (int side = 0; side < 2; side++)
+	{
+		const int ne = side ? B.ne : A.ne;
+		const f3 lo = side ? aMin : bMin, hi = side ? aMax : bMax;
+		int cnt = 0;
+		for (int base = 0; base < ne; base += 64)
+		{
+			const int e = base + lane;
+			bool keep = false;
+			if (e < ne)
+			{
+				const uint32_t g = side ? L.eb[e] : L.ea[e];
+				const float4* vv = side ? L.vb : L.va;
+				const f3 p0 = ldsV(vv, int(g & 0xFF)), p1 = ldsV(vv, int((g >> 8) & 0xFF));
+				const f3 mid = (p0 + p1) * 0.5f, h = (p1 - p0) * 0.5f;
+				const f3 q = mk(fminf(fmaxf(mid.x, lo.x), hi.x), fminf(fmaxf(mid.y, lo.y), hi.y),
+				                fminf(fmaxf(mid.z, lo.z), hi.z));
+				const f3 dq = mid - q;
+				keep = !(sqrtf(dot(dq, dq)) - sqrtf(dot(h, h)) > lim);
+			}
+			const uint64_t m = __ballot(keep);
+			if (keep)
+			{
+				const int at = cnt + int(__popcll(m & ((1ull << lane) - 1ull)));
+				(side ? W.elB : W.elA)[at] = uint8_t(e);
+			}
+			cnt += int(__popcll(m));
+		}
+		(side ? nB : nA) = cnt;
+	}
+	waveSync();
+	for (int k = lane; k < nA * nB; k += 64)
+	{
+		const int ia = k / nB, ib = k - ia * nB;
+		const int ea = L.elA[ia], ebi = L.elB[ib];
 		const uint32_t e = L.ea[ea], g = L.eb[ebi];
 		f3 c1, c2;
 		const float s = segClosestSq(ldsV(L.va, int(e & 0xFF)), ldsV(L.va, int((e >> 8) & 0xFF)),
@@ -1347,7 +1413,7 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 		if (s < d2)
 		{
 			d2 = s;
-			di = (2u << 24) | uint32_t(k);
+			di = (2u << 24) | uint32_t(ea * B.ne + ebi);
 		}
 	}
 	waveArgMin(d2, di);
