@@ -89,6 +89,8 @@ struct PatchIO
 
 int64_t scan_tmp_size(int64_t n);
 void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st);
+// keys per radix-sort workgroup (its histogram holds 256 counts per workgroup)
+int radix_tile();
 void launch_radix_sort(uint64_t* keys, uint32_t* vals, uint64_t* keys2, uint32_t* vals2, int n, int bitBegin,
                        int bitEnd, uint32_t* hist, uint32_t* scanTmp, hipStream_t st, bool& resultInAlt);
 

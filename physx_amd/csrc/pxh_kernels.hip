@@ -111,7 +111,9 @@ __global__ void k_scan_add(uint32_t* data, int64_t n, const uint32_t* blockOffse
 // ---------------------------------------------------------------------------
 // LSD radix sort of (u64 key, u32 value), 8-bit digits, stable
 // ---------------------------------------------------------------------------
-constexpr int RS_THREADS = 256, RS_ITEMS = 8, RS_TILE = RS_THREADS * RS_ITEMS;
+// 512-key tiles: a 262k-key sort spreads over 512 workgroups, two per CU
+constexpr int RS_THREADS = 256, RS_ITEMS = 2, RS_TILE = RS_THREADS * RS_ITEMS;
+int radix_tile() { return RS_TILE; }
 
 __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t* keys, int n, int shift, uint32_t* hist, int nblocks)
 {

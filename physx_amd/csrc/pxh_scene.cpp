@@ -395,7 +395,7 @@ int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_grou
 	PXH_CHECK(s->vals.ensure(n));
 	PXH_CHECK(s->vals2.ensure(n));
 	pxh::launch_keys(n, s->bmin.p, s->bmax.p, d_scene, s->gridF.p, s->gridI.p, d_rowOff, s->keys.p, s->vals.p, st);
-	const int nblocks = (n + 2047) / 2048;
+	const int nblocks = (n + pxh::radix_tile() - 1) / pxh::radix_tile();
 	PXH_CHECK(s->hist.ensure(size_t(256) * std::max(nblocks, 1)));
 	bool alt = false;
 	const int bitEnd = 32 + ((bitsFor(totalRows) + 7) / 8) * 8;
@@ -998,7 +998,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	PXH_CHECK(s->vals.ensure(Cc));
 	PXH_CHECK(s->vals2.ensure(Cc));
 	{
-		const int nb = (C + 2047) / 2048;
+		const int nb = (C + pxh::radix_tile() - 1) / pxh::radix_tile();
 		PXH_CHECK(s->hist.ensure(size_t(256) * std::max(nb, 1)));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(std::max<int64_t>(int64_t(256) * std::max(nb, 1), C + 1)) +
 		                            1024));
@@ -1435,7 +1435,7 @@ int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* gr
 	PXH_CHECK(bp->vals.ensure(std::max(P, 1)));
 	PXH_CHECK(bp->vals2.ensure(std::max(P, 1)));
 	pxh::launch_pair_keys(P, bp->pairs.p, bp->pkCur.p, bp->vals.p, st);
-	const int nblocks = (P + 2047) / 2048;
+	const int nblocks = (P + pxh::radix_tile() - 1) / pxh::radix_tile();
 	PXH_CHECK(bp->hist.ensure(size_t(256) * std::max(nblocks, 1)));
 	PXH_CHECK(bp->scanTmp.ensure(
 	    pxh::scan_tmp_size(std::max<int64_t>(std::max<int64_t>(int64_t(256) * std::max(nblocks, 1), P + 1),
