@@ -43,8 +43,8 @@ def uniform(seed: int, ids: np.ndarray, draw: int, lo=0.0, hi=1.0) -> np.ndarray
 @dataclass
 class Scene:
     """Flat body/plane arrays of one or more independent scenes (worlds)."""
-    shape_type: np.ndarray          # (n,) int32, SPHERE or BOX
-    dims: np.ndarray                # (n,4) f32: radius or half extents
+    shape_type: np.ndarray          # (n,) int32: SPHERE, CAPSULE, BOX or CONVEX
+    dims: np.ndarray                # (n,4) f32: radius; radius, half height; half extents; mesh index
     pose: np.ndarray                # (n,7) f32: p xyz, q xyzw
     lin_vel: np.ndarray             # (n,3) f32
     ang_vel: np.ndarray             # (n,3) f32
