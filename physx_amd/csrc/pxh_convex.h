@@ -1163,6 +1163,70 @@ PXH_D bool waveAny(bool b) { return __ballot(b) != 0ull; }
 
 PXH_D f3 ldsV(const float4* a, int i) { return mk(a[i].x, a[i].y, a[i].z); }
 
+// The edges of each polytope whose bounding sphere comes within lim of the
+// other polytope's bounding box (both in B's frame, from the staged vertices),
+// listed in index order in L.elA / L.elB; every edge when lim is FLT_BIG.
+PXH_D void nearEdges(const Poly& A, const Poly& B, const CvxLds& L, int lane, float lim, int& nA, int& nB)
+{
+	const float BIG = FLT_BIG;
+	f3 aMin = mk(BIG, BIG, BIG), aMax = -aMin, bMin = aMin, bMax = aMax;
+	if (lane < A.nv)
+	{
+		aMin = ldsV(L.va, lane);
+		aMax = aMin;
+	}
+	if (lane < B.nv)
+	{
+		bMin = ldsV(L.vb, lane);
+		bMax = bMin;
+	}
+	for (int off = 32; off >= 1; off >>= 1)
+	{
+		aMin = mk(fminf(aMin.x, __shfl_xor(aMin.x, off)), fminf(aMin.y, __shfl_xor(aMin.y, off)),
+		          fminf(aMin.z, __shfl_xor(aMin.z, off)));
+		aMax = mk(fmaxf(aMax.x, __shfl_xor(aMax.x, off)), fmaxf(aMax.y, __shfl_xor(aMax.y, off)),
+		          fmaxf(aMax.z, __shfl_xor(aMax.z, off)));
+		bMin = mk(fminf(bMin.x, __shfl_xor(bMin.x, off)), fminf(bMin.y, __shfl_xor(bMin.y, off)),
+		          fminf(bMin.z, __shfl_xor(bMin.z, off)));
+		bMax = mk(fmaxf(bMax.x, __shfl_xor(bMax.x, off)), fmaxf(bMax.y, __shfl_xor(bMax.y, off)),
+		          fmaxf(bMax.z, __shfl_xor(bMax.z, off)));
+	}
+	CvxLds& W = const_cast<CvxLds&>(L);
+	nA = 0;
+	nB = 0;
+	for (int side = 0; side < 2; side++)
+	{
+		const int ne = side ? B.ne : A.ne;
+		const f3 lo = side ? aMin : bMin, hi = side ? aMax : bMax;
+		int cnt = 0;
+		for (int base = 0; base < ne; base += 64)
+		{
+			const int e = base + lane;
+			bool keep = false;
+			if (e < ne)
+			{
+				const uint32_t g = side ? L.eb[e] : L.ea[e];
+				const float4* vv = side ? L.vb : L.va;
+				const f3 p0 = ldsV(vv, int(g & 0xFF)), p1 = ldsV(vv, int((g >> 8) & 0xFF));
+				const f3 mid = (p0 + p1) * 0.5f, h = (p1 - p0) * 0.5f;
+				const f3 q = mk(fminf(fmaxf(mid.x, lo.x), hi.x), fminf(fmaxf(mid.y, lo.y), hi.y),
+				                fminf(fmaxf(mid.z, lo.z), hi.z));
+				const f3 dq = mid - q;
+				keep = !(sqrtf(dot(dq, dq)) - sqrtf(dot(h, h)) > lim);
+			}
+			const uint64_t m = __ballot(keep);
+			if (keep)
+			{
+				const int at = cnt + int(__popcll(m & ((1ull << lane) - 1ull)));
+				(side ? W.elB : W.elA)[at] = uint8_t(e);
+			}
+			cnt += int(__popcll(m));
+		}
+		(side ? nB : nA) = cnt;
+	}
+	waveSync();
+}
+
 // polySeparation over the wave (same results); A, B in LDS as staged
 PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, float cd, const CvxLds& L, int lane,
                               float& dist, f3& n, f3& cA, f3& cB)
@@ -1347,60 +1411,8 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 	for (int off = 32; off >= 1; off >>= 1)
 		U = fminf(U, __shfl_xor(U, off));
 	const float lim = U < BIG ? sqrtf(U) + 1e-4f : BIG;
-	f3 aMin = mk(BIG, BIG, BIG), aMax = -aMin, bMin = aMin, bMax = aMax;
-	if (lane < A.nv)
-	{
-		aMin = ldsV(L.va, lane);
-		aMax = aMin;
-	}
-	if (lane < B.nv)
-	{
-		bMin = ldsV(L.vb, lane);
-		bMax = bMin;
-	}
-	for (int off = 32; off >= 1; off >>= 1)
-	{
-		aMin = mk(fminf(aMin.x, __shfl_xor(aMin.x, off)), fminf(aMin.y, __shfl_xor(aMin.y, off)),
-		          fminf(aMin.z, __shfl_xor(aMin.z, off)));
-		aMax = mk(fmaxf(aMax.x, __shfl_xor(aMax.x, off)), fmaxf(aMax.y, __shfl_xor(aMax.y, off)),
-		          fmaxf(aMax.z, __shfl_xor(aMax.z, off)));
-		bMin = mk(fminf(bMin.x, __shfl_xor(bMin.x, off)), fminf(bMin.y, __shfl_xor(bMin.y, off)),
-		          fminf(bMin.z, __shfl_xor(bMin.z, off)));
-		bMax = mk(fmaxf(bMax.x, __shfl_xor(bMax.x, off)), fmaxf(bMax.y, __shfl_xor(bMax.y, off)),
-		          fmaxf(bMax.z, __shfl_xor(bMax.z, off)));
-	}
-	CvxLds& W = const_cast<CvxLds&>(L);
 	int nA = 0, nB = 0;
-	for (int side = 0; side < 2; side++)
-	{
-		const int ne = side ? B.ne : A.ne;
-		const f3 lo = side ? aMin : bMin, hi = side ? aMax : bMax;
-		int cnt = 0;
-		for (int base = 0; base < ne; base += 64)
-		{
-			const int e = base + lane;
-			bool keep = false;
-			if (e < ne)
-			{
-				const uint32_t g = side ? L.eb[e] : L.ea[e];
-				const float4* vv = side ? L.vb : L.va;
-				const f3 p0 = ldsV(vv, int(g & 0xFF)), p1 = ldsV(vv, int((g >> 8) & 0xFF));
-				const f3 mid = (p0 + p1) * 0.5f, h = (p1 - p0) * 0.5f;
-				const f3 q = mk(fminf(fmaxf(mid.x, lo.x), hi.x), fminf(fmaxf(mid.y, lo.y), hi.y),
-				                fminf(fmaxf(mid.z, lo.z), hi.z));
-				const f3 dq = mid - q;
-				keep = !(sqrtf(dot(dq, dq)) - sqrtf(dot(h, h)) > lim);
-			}
-			const uint64_t m = __ballot(keep);
-			if (keep)
-			{
-				const int at = cnt + int(__popcll(m & ((1ull << lane) - 1ull)));
-				(side ? W.elB : W.elA)[at] = uint8_t(e);
-			}
-			cnt += int(__popcll(m));
-		}
-		(side ? nB : nA) = cnt;
-	}
+	nearEdges(A, B, L, lane, lim, nA, nB);
 	waveSync();
 	for (int k = lane; k < nA * nB; k += 64)
 	{
