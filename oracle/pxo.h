@@ -18,6 +18,11 @@
  *   convex hull contacts     pinned: vs PxGenerateContacts on hulls cooked by the
  *                            reference (exact closest features / minimum
  *                            translation stand in for GJK / EPA)
+ *   capsule contacts         pinned: sphere-, capsule-, plane-capsule pair for
+ *                            pair vs PxGenerateContacts and closed forms;
+ *                            capsule-box / -convex vs PxGenerateContacts as
+ *                            point sets (exact segment-polytope separation
+ *                            stands in for GJK / EPA on the core segment)
  */
 #ifndef PXO_H
 #define PXO_H
