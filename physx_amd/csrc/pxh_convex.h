@@ -1163,6 +1163,31 @@ PXH_D bool waveAny(bool b) { return __ballot(b) != 0ull; }
 
 PXH_D f3 ldsV(const float4* a, int i) { return mk(a[i].x, a[i].y, a[i].z); }
 
+// (i, j) of the flat index k = i * n + j as a lane walks k = lane, lane + 64,
+// ...: one division up front, then a step of 64 costs an add and one compare
+// (a runtime integer division is a long instruction sequence on this target).
+struct FlatPair
+{
+	int n, q, r, i, j;
+	PXH_D FlatPair(int lane, int n_) : n(n_ > 0 ? n_ : 1)
+	{
+		q = 64 / n;
+		r = 64 - q * n;
+		i = lane / n;
+		j = lane - i * n;
+	}
+	PXH_D void next()
+	{
+		i += q;
+		j += r;
+		if (j >= n)
+		{
+			j -= n;
+			i++;
+		}
+	}
+};
+
 // The edges of each polytope whose bounding sphere comes within lim of the
 // other polytope's bounding box (both in B's frame, from the staged vertices),
 // listed in index order in L.elA / L.elB; every edge when lim is FLT_BIG.
@@ -1287,9 +1312,10 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 			bmean = bmean + ldsV(L.vb, i);
 		bmean = bmean * (1.0f / float(B.nv));
 		const int pairsE = A.ne * B.ne;
-		for (int k = lane; k < pairsE; k += 64)
+		FlatPair fp(lane, B.ne);
+		for (int k = lane; k < pairsE; k += 64, fp.next())
 		{
-			const int ea = k / B.ne, ebi = k - ea * B.ne;
+			const int ea = fp.i, ebi = fp.j;
 			const uint32_t e = L.ea[ea], g = L.eb[ebi];
 			const f3 na = ldsV(L.pa, int((e >> 16) & 0xFF)), nb = ldsV(L.pa, int(e >> 24));
 			const f3 bxa = cross(nb, na);
@@ -1376,9 +1402,10 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 	// edge pairs), the first minimum in that order
 	float d2 = BIG;
 	uint32_t di = 0xFFFFFFFFu;
-	for (int k = lane; k < B.np * A.nv; k += 64)
+	FlatPair fb(lane, A.nv);
+	for (int k = lane; k < B.np * A.nv; k += 64, fb.next())
 	{
-		const int f = k / A.nv, i = k - f * A.nv;
+		const int f = fb.i, i = fb.j;
 		const float4 pl = L.pb[f];
 		const f3 bn = mk(pl.x, pl.y, pl.z), va = ldsV(L.va, i);
 		const float h = dot(bn, va) + pl.w;
@@ -1389,9 +1416,10 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 		d2 = h * h;
 		di = uint32_t(k);
 	}
-	for (int k = lane; k < A.np * B.nv; k += 64)
+	FlatPair fa(lane, B.nv);
+	for (int k = lane; k < A.np * B.nv; k += 64, fa.next())
 	{
-		const int f = k / B.nv, i = k - f * B.nv;
+		const int f = fa.i, i = fa.j;
 		const float4 pl = L.pa[f];
 		const f3 an = mk(pl.x, pl.y, pl.z), vb = ldsV(L.vb, i);
 		const float h = dot(an, vb) + pl.w;
@@ -1414,10 +1442,10 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 	int nA = 0, nB = 0;
 	nearEdges(A, B, L, lane, lim, nA, nB);
 	waveSync();
-	for (int k = lane; k < nA * nB; k += 64)
+	FlatPair fe(lane, nB);
+	for (int k = lane; k < nA * nB; k += 64, fe.next())
 	{
-		const int ia = k / nB, ib = k - ia * nB;
-		const int ea = L.elA[ia], ebi = L.elB[ib];
+		const int ea = L.elA[fe.i], ebi = L.elB[fe.j];
 		const uint32_t e = L.ea[ea], g = L.eb[ebi];
 		f3 c1, c2;
 		const float s = segClosestSq(ldsV(L.va, int(e & 0xFF)), ldsV(L.va, int((e >> 8) & 0xFF)),
