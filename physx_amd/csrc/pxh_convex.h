@@ -1311,35 +1311,46 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 		for (int i = 0; i < B.nv; i++)
 			bmean = bmean + ldsV(L.vb, i);
 		bmean = bmean * (1.0f / float(B.nv));
-		const int pairsE = A.ne * B.ne;
-		FlatPair fp(lane, B.ne);
-		for (int k = lane; k < pairsE; k += 64, fp.next())
+		// Edge pairs: each lane holds one of A's edges (its Gauss-map arc in
+		// registers) and walks B's edges, so B's edge data are read as LDS
+		// broadcasts. A block of R < 64 of A's edges gives each edge S = 64 / R
+		// lanes, which split B's edges between them. A lane visits its pairs in
+		// ascending index order, so with the lowest-index rule of waveArgMin
+		// the pair chosen is the first minimum of the flat order.
+		for (int base = 0; base < A.ne; base += 64)
 		{
-			const int ea = fp.i, ebi = fp.j;
-			const uint32_t e = L.ea[ea], g = L.eb[ebi];
+			const int R = min(64, A.ne - base), S = 64 / R, part = lane / R;
+			if (part >= S)
+				continue;
+			const int ea = base + lane - part * R;
+			const uint32_t e = L.ea[ea];
 			const f3 na = ldsV(L.pa, int((e >> 16) & 0xFF)), nb = ldsV(L.pa, int(e >> 24));
 			const f3 bxa = cross(nb, na);
-			const f3 nc = -ldsV(L.pb, int((g >> 16) & 0xFF)), nd = -ldsV(L.pb, int(g >> 24));
-			const f3 dxc = cross(nd, nc);
-			const float cba = dot(nc, bxa), dba = dot(nd, bxa), adc = dot(na, dxc), bdc = dot(nb, dxc);
-			if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
-				continue;
-			const f3 a0 = ldsV(L.va, int(e & 0xFF)), a1 = ldsV(L.va, int((e >> 8) & 0xFF));
-			const f3 b0 = ldsV(L.vb, int(g & 0xFF)), b1 = ldsV(L.vb, int((g >> 8) & 0xFF));
-			f3 ax = cross(a1 - a0, b1 - b0);
-			const float l2 = dot(ax, ax);
-			if (l2 < 1e-12f)
-				continue;
-			ax = ax * (1.0f / sqrtf(l2));
-			if (dot(ax, b0 - bmean) < 0.0f)
-				ax = -ax;
-			const float ov = dot(ax, b0) - dot(ax, a0);
-			if (ov < 0.0f)
-				sep = true;
-			else if (ov < best)
+			for (int ebi = part; ebi < B.ne; ebi += S)
 			{
-				best = ov;
-				bi = (2u << 24) | uint32_t(ea * CVX_WAVE_E + ebi);
+				const uint32_t g = L.eb[ebi];
+				const f3 nc = -ldsV(L.pb, int((g >> 16) & 0xFF)), nd = -ldsV(L.pb, int(g >> 24));
+				const f3 dxc = cross(nd, nc);
+				const float cba = dot(nc, bxa), dba = dot(nd, bxa), adc = dot(na, dxc), bdc = dot(nb, dxc);
+				if (!(cba * dba < 0.0f && adc * bdc < 0.0f && cba * bdc > 0.0f))
+					continue;
+				const f3 a0 = ldsV(L.va, int(e & 0xFF)), a1 = ldsV(L.va, int((e >> 8) & 0xFF));
+				const f3 b0 = ldsV(L.vb, int(g & 0xFF)), b1 = ldsV(L.vb, int((g >> 8) & 0xFF));
+				f3 ax = cross(a1 - a0, b1 - b0);
+				const float l2 = dot(ax, ax);
+				if (l2 < 1e-12f)
+					continue;
+				ax = ax * (1.0f / sqrtf(l2));
+				if (dot(ax, b0 - bmean) < 0.0f)
+					ax = -ax;
+				const float ov = dot(ax, b0) - dot(ax, a0);
+				if (ov < 0.0f)
+					sep = true;
+				else if (ov < best)
+				{
+					best = ov;
+					bi = (2u << 24) | uint32_t(ea * CVX_WAVE_E + ebi);
+				}
 			}
 		}
 		sep = waveAny(sep);
