@@ -1252,15 +1252,19 @@ PXH_D void nearEdges(const Poly& A, const Poly& B, const CvxLds& L, int lane, fl
 	waveSync();
 }
 
-// polySeparation over the wave (same results); A, B in LDS as staged
-PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, float cd, const CvxLds& L, int lane,
-                              float& dist, f3& n, f3& cA, f3& cB)
+// The face axes of polySeparation over the wave, from the staged vertices and
+// planes only: B's (order 0 | f << 8 | vertex), then A's (order 1 | ...). The
+// lane's least penetrating axis goes to best, bi; sep is set on every lane
+// when some axis shows no penetration. True when some axis separates the
+// polytopes by more than cd.
+PXH_D bool faceAxesWave(const Poly& A, const Poly& B, float cd, const CvxLds& L, int lane, float& best, uint32_t& bi,
+                        bool& sep)
 {
 	const float BIG = FLT_BIG;
-	float best = BIG;
-	uint32_t bi = 0xFFFFFFFFu;
-	bool far = false, sep = false;
-	// face axes of B (order 0 | f << 8 | vertex), then of A (order 1 | ...)
+	best = BIG;
+	bi = 0xFFFFFFFFu;
+	bool far = false;
+	sep = false;
 	for (int f = lane; f < B.np; f += 64)
 	{
 		const float4 pl = L.pb[f];
@@ -1299,12 +1303,16 @@ PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, flo
 			bi = (1u << 24) | (uint32_t(f) << 8) | uint32_t(vi);
 		}
 	}
-	if (waveAny(far))
-	{
-		dist = 2.0f * cd + 1.0f;   // farther apart than the contact distance
-		return;
-	}
 	sep = waveAny(sep);
+	return waveAny(far);
+}
+
+// The rest of polySeparation over the wave (same results), after
+// faceAxesWave found no axis farther than cd; A, B in LDS as staged
+PXH_D void polySeparationWave(const Poly& A, const mxf& aToB, const Poly& B, const CvxLds& L, int lane, float best,
+                              uint32_t bi, bool sep, float& dist, f3& n, f3& cA, f3& cB)
+{
+	const float BIG = FLT_BIG;
 	if (!sep)
 	{
 		f3 bmean = mk(0.0f, 0.0f, 0.0f);
@@ -1522,8 +1530,21 @@ PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, cons
 		L.ea[e] = A.edge(e);
 	for (int e = lane; e < B.ne; e += 64)
 		L.eb[e] = B.edge(e);
+	waveSync();
+	float best;
+	uint32_t bi;
+	bool sepd;
+	if (faceAxesWave(A, B, cd, L, lane, best, bi, sepd))
+	{
+		// farther apart than the contact distance
+		sep = make_float4(2.0f * cd + 1.0f, 1.0f, 0.0f, 0.0f);
+		wA = wB = mk(0.0f, 0.0f, 0.0f);
+		waveSync();
+		return;
+	}
 	// the polytopes' own arrays in LDS too (polygon tests, witness polygons,
-	// clipping), read through the same accessors
+	// clipping), read through the same accessors; pairs rejected by a face
+	// axis above never need them
 	Poly Al = A, Bl = B;
 	if (!A.box)
 	{
@@ -1546,7 +1567,7 @@ PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, cons
 	waveSync();
 	float dist = 0.0f;
 	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
-	polySeparationWave(Al, aToB, Bl, cd, L, lane, dist, n, cA, cB);
+	polySeparationWave(Al, aToB, Bl, L, lane, best, bi, sepd, dist, n, cA, cB);
 	waveSync();   // the staged data stay in use until every lane is done
 	sep.x = dist;
 	sep.y = n.x; sep.z = n.y; sep.w = n.z;
