@@ -1122,8 +1122,6 @@ struct CvxLds
 	float4 vb[CVX_WAVE_V];   // B's vertices
 	float4 pa[CVX_WAVE_P];   // A's planes in B's frame
 	float4 pb[CVX_WAVE_P];   // B's planes
-	float4 vaL[CVX_WAVE_V];  // A's vertices and planes in its own frame
-	float4 paL[CVX_WAVE_P];
 	uint32_t polyA[CVX_WAVE_P], polyB[CVX_WAVE_P];
 	uint32_t ea[CVX_WAVE_E];
 	uint32_t eb[CVX_WAVE_E];
@@ -1542,22 +1540,19 @@ PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, cons
 		waveSync();
 		return;
 	}
-	// the polytopes' own arrays in LDS too (polygon tests, witness polygons,
-	// clipping), read through the same accessors; pairs rejected by a face
-	// axis above never need them
+	// the polytopes' polygon and vertex-ref arrays in LDS too (polygon tests,
+	// witness polygons, clipping), read through the same accessors; pairs
+	// rejected by a face axis above never need them. A's vertices and planes
+	// in its own frame stay in global memory (cached): keeping them out of
+	// LDS lets four waves share a SIMD.
 	Poly Al = A, Bl = B;
 	if (!A.box)
 	{
-		for (int i = lane; i < A.nv; i += 64)
-			L.vaL[i] = A.v[i];
 		for (int f = lane; f < A.np; f += 64)
-		{
-			L.paL[f] = A.pl[f];
 			L.polyA[f] = A.poly[f];
-		}
 		for (int k = lane; k < 2 * A.ne; k += 64)
 			L.vrA[k] = A.vr[k];
-		Al.v = L.vaL; Al.pl = L.paL; Al.poly = L.polyA; Al.vr = L.vrA; Al.ed = L.ea;
+		Al.poly = L.polyA; Al.vr = L.vrA; Al.ed = L.ea;
 	}
 	for (int f = lane; f < B.np; f += 64)
 		L.polyB[f] = B.poly[f];

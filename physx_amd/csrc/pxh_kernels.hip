@@ -667,7 +667,7 @@ __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex
 // bounding spheres are farther apart than the contact distance are marked
 // without a search.
 constexpr int CVX_WAVES = 4;
-__global__ void __launch_bounds__(64 * CVX_WAVES)
+__global__ void __launch_bounds__(64 * CVX_WAVES, 4)   // at least 4 waves per SIMD
 k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, const uint2* pairs, const float4* pos,
                  const float4* rot, const float4* geom, HullSet hs, float cd, float4* wit)
 {
