@@ -114,10 +114,11 @@ void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, con
 void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                      const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
                      float4* ctP, hipStream_t st);
-// convex pairs; wit: 3 float4 per pair of scratch (separation records)
+// convex pairs; scratch: wit, 3 float4 per pair (separation records), and
+// list, cap + 1 entries (the touching pairs)
 void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                       const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
-                      float4* wit, float4* ctN, float4* ctP, hipStream_t st);
+                      float4* wit, uint32_t* list, float4* ctN, float4* ctP, hipStream_t st);
 // body-plane pairs: launch_np_planes skips convex bodies, launch_np_planes_convex takes only them
 void launch_np_planes(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos, const float4* rot,
                       const float4* geom, const float4* planes, float cd, float4* ctN, float4* ctP, hipStream_t st);

@@ -633,9 +633,11 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 // is round (box- and convex-convex pairs run in k_np_convex_wave).
 __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                             const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float tol,
-                            float4* ctN, float4* ctP)
+                            float4* ctN, float4* ctP, uint32_t* list)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
+	if (q == 0)
+		list[0] = 0u;   // the touching-pair list k_np_convex_wave fills next
 	const uint32_t p = *dConvex + uint32_t(q);
 	if (q >= cap || p >= *dP || p >= uint32_t(cap))
 		return;
@@ -662,14 +664,16 @@ __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex
 
 // Box- and convex-convex pairs, separation phase: one wavefront per pair
 // (grid-stride over the convex segments) finds the closest features or the
-// minimum translation and stores (dist, n), cA, cB at wit[3p .. 3p+2];
-// polytopes past the LDS limits are searched by lane 0 alone. Pairs whose
-// bounding spheres are farther apart than the contact distance are marked
-// without a search.
+// minimum translation. A pair within the contact distance stores (dist, n),
+// cA, cB at wit[3p .. 3p+2] and is appended to list (list[0] counts, entries
+// from list[1]); any other pair gets its empty contact slot here. Polytopes
+// past the LDS limits are searched by lane 0 alone. Pairs whose bounding
+// spheres are farther apart than the contact distance skip the search.
 constexpr int CVX_WAVES = 4;
 __global__ void __launch_bounds__(64 * CVX_WAVES, 4)   // at least 4 waves per SIMD
-k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, const uint2* pairs, const float4* pos,
-                 const float4* rot, const float4* geom, HullSet hs, float cd, float4* wit)
+k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
+                 const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float4* wit,
+                 uint32_t* list, float4* ctN)
 {
 	__shared__ CvxLds lds[CVX_WAVES];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -707,24 +711,32 @@ k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, const uin
 		}
 		if (lane == 0)
 		{
-			wit[3 * int64_t(p)] = sep;
-			wit[3 * int64_t(p) + 1] = f4(cA, 0.0f);
-			wit[3 * int64_t(p) + 2] = f4(cB, 0.0f);
+			if (sep.x <= cd)
+			{
+				wit[3 * int64_t(p)] = sep;
+				wit[3 * int64_t(p) + 1] = f4(cA, 0.0f);
+				wit[3 * int64_t(p) + 2] = f4(cB, 0.0f);
+				list[1 + atomicAdd(list, 1u)] = p;
+			}
+			else
+				ctN[slotBase + int(p)] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
 		}
 	}
 }
 
-// Box- and convex-convex pairs, contact phase: one lane per pair of the convex
-// segments builds the contacts from the separation record (witness polygons,
-// clipping, reduction).
+// Box- and convex-convex pairs, contact phase: one lane per listed (touching)
+// pair builds the contacts from the separation record (witness polygons,
+// clipping, reduction); the list order does not matter, pair p fills slot
+// slotBase + p.
 __global__ void k_np_convex_finish(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase,
                                    const uint2* pairs, const float4* pos, const float4* rot, const float4* geom,
-                                   HullSet hs, float cd, float tol, const float4* wit, float4* ctN, float4* ctP)
+                                   HullSet hs, float cd, float tol, const float4* wit, const uint32_t* list,
+                                   float4* ctN, float4* ctP)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	const uint32_t p = *dConvex + uint32_t(q);
-	if (q >= cap || p >= *dP || p >= uint32_t(cap))
+	if (q >= cap || uint32_t(q) >= list[0])
 		return;
+	const uint32_t p = list[1 + q];
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
 	const int t0 = int(g0.w);
@@ -2208,16 +2220,18 @@ void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int s
 }
 void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                       const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
-                      float4* wit, float4* ctN, float4* ctP, hipStream_t st)
+                      float4* wit, uint32_t* list, float4* ctN, float4* ctP, hipStream_t st)
 {
-	PXH_LAUNCH(k_np_convex, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, ctN, ctP);
+	PXH_LAUNCH(k_np_convex, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, ctN, ctP, list);
 	if (cap > 0)
 	{
 		// grid-stride: enough waves to fill the chip several times over
 		const int blocks = std::min((cap + CVX_WAVES - 1) / CVX_WAVES, 8192);
-		k_np_convex_wave<<<blocks, 64 * CVX_WAVES, 0, st>>>(cap, dP, dConvex, pairs, pos, rot, geom, hs, cd, wit);
+		k_np_convex_wave<<<blocks, 64 * CVX_WAVES, 0, st>>>(cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd,
+		                                                    wit, list, ctN);
 	}
-	PXH_LAUNCH(k_np_convex_finish, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, wit, ctN, ctP);
+	PXH_LAUNCH(k_np_convex_finish, cap, cap, dP, dConvex, slotBase, pairs, pos, rot, geom, hs, cd, tol, wit, list, ctN,
+	           ctP);
 }
 void launch_np_planes_convex(int Q, int slotBase, const int* qBody, const int* qPlane, const float4* pos,
                              const float4* rot, const float4* geom, const float4* planes, const HullSet& hs, float cd,

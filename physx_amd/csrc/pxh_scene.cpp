@@ -130,6 +130,7 @@ struct pxh_scene_t
 	DBuf<uint8_t> hullVR;
 	DBuf<int4> hullInfo, hullInfo2;
 	DBuf<float4> cvxWit;     // separation records of the convex pairs (3 per pair)
+	DBuf<uint32_t> cvxList;  // touching convex pairs: count, then pair indices
 	int nConvexBodies = 0;   // convex bodies ever added (the convex kernels run when nonzero)
 	pxh::HullSet hullSet() const
 	{
@@ -910,8 +911,9 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		if (s->nConvexBodies > 0)
 		{
 			PXH_CHECK(s->cvxWit.ensure(size_t(3) * std::max(cap, 1)));
+			PXH_CHECK(s->cvxList.ensure(size_t(cap) + 1));
 			pxh::launch_np_convex(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), cd,
-			                      d.tolerance_length, s->cvxWit.p, s->ctN.p, s->ctP.p, st);
+			                      d.tolerance_length, s->cvxWit.p, s->cvxList.p, s->ctN.p, s->ctP.p, st);
 		}
 		s->profEnd(pk);
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
