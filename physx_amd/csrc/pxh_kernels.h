@@ -115,7 +115,8 @@ void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int s
                      const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
                      float4* ctP, hipStream_t st);
 // convex pairs; scratch: wit, 3 float4 per pair (separation records), and
-// list, cap + 1 entries (the touching pairs)
+// list, 2 + 2 cap entries (counts of touching and searched pairs, then the
+// two lists), whose first two entries the caller zeroes
 void launch_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                       const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd, float tol,
                       float4* wit, uint32_t* list, float4* ctN, float4* ctP, hipStream_t st);

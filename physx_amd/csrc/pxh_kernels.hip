@@ -630,21 +630,54 @@ __global__ void k_np_planes(int Q, int slotBase, const int* qBody, const int* qP
 
 // Sphere- and capsule-convex pairs: the class segments from *dConvex to *dP
 // hold the convex pairs; lane q takes pair *dConvex + q when its first shape
-// is round (box- and convex-convex pairs run in k_np_convex_wave).
+// is round. For a box- or convex-convex pair it makes the bounding-sphere
+// test and lists the pair for k_np_convex_wave (list[1] counts, entries from
+// list[2 + cap]) or writes its empty slot.
 __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                             const float4* pos, const float4* rot, const float4* geom, HullSet hs, float cd, float tol,
                             float4* ctN, float4* ctP, uint32_t* list)
 {
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
-	if (q == 0)
-		list[0] = 0u;   // the touching-pair list k_np_convex_wave fills next
 	const uint32_t p = *dConvex + uint32_t(q);
-	if (q >= cap || p >= *dP || p >= uint32_t(cap))
-		return;
-	const uint2 ab = pairs[p];
-	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
-	const int t0 = int(g0.w);
-	if (t0 != GEOM_SPHERE && t0 != GEOM_CAPSULE)
+	const bool live = q < cap && p < *dP && p < uint32_t(cap);
+	uint2 ab = make_uint2(0u, 0u);
+	float4 g0 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), g1 = g0;
+	int t0 = -1;
+	if (live)
+	{
+		ab = pairs[p];
+		g0 = geom[ab.x];
+		g1 = geom[ab.y];
+		t0 = int(g0.w);
+	}
+	const bool round = t0 == GEOM_SPHERE || t0 == GEOM_CAPSULE;
+	// Polytope pairs: those whose bounding spheres come within the contact
+	// distance go to the search list (one atomic per wave), the others get
+	// their empty contact slot here.
+	bool search = false;
+	if (live && !round)
+	{
+		const f3 p0 = mk(pos[ab.x]), p1 = mk(pos[ab.y]);
+		const float r0 = t0 == GEOM_BOX ? sqrtf(g0.x * g0.x + g0.y * g0.y + g0.z * g0.z) : hs.be[int(g0.x)].w;
+		const float r1 = hs.be[int(g1.x)].w;
+		const f3 d = p1 - p0;
+		const float reach = r0 + r1 + cd;
+		search = dot(d, d) <= reach * reach;
+		if (!search)
+			ctN[slotBase + int(p)] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
+	}
+	const uint64_t m = __ballot(search);
+	if (m)
+	{
+		const int lane = threadIdx.x & 63, lead = __ffsll((unsigned long long)m) - 1;
+		uint32_t at = 0u;
+		if (lane == lead)
+			at = atomicAdd(list + 1, uint32_t(__popcll(m)));
+		at = __shfl(at, lead);
+		if (search)
+			list[2 + cap + at + uint32_t(__popcll(m & ((1ull << lane) - 1ull)))] = p;
+	}
+	if (!live || !round)
 		return;
 	xf x1;
 	x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
@@ -664,11 +697,12 @@ __global__ void k_np_convex(int cap, const uint32_t* dP, const uint32_t* dConvex
 
 // Box- and convex-convex pairs, separation phase: one wavefront per pair
 // (grid-stride over the convex segments) finds the closest features or the
-// minimum translation. A pair within the contact distance stores (dist, n),
-// cA, cB at wit[3p .. 3p+2] and is appended to list (list[0] counts, entries
-// from list[1]); any other pair gets its empty contact slot here. Polytopes
-// past the LDS limits are searched by lane 0 alone. Pairs whose bounding
-// spheres are farther apart than the contact distance skip the search.
+// minimum translation for each pair of the search list k_np_convex wrote
+// (list[1] entries from list[2 + cap]). A pair within the contact distance
+// stores (dist, n), cA, cB at wit[3p .. 3p+2] and is appended to the
+// touching list (list[0] entries from list[2]); any other pair gets its empty
+// contact slot here. Polytopes past the LDS limits are searched by lane 0
+// alone.
 constexpr int CVX_WAVES = 4;
 __global__ void __launch_bounds__(64 * CVX_WAVES, 4)   // at least 4 waves per SIMD
 k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
@@ -677,37 +711,28 @@ k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotB
 {
 	__shared__ CvxLds lds[CVX_WAVES];
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-	const uint32_t end = min(*dP, uint32_t(cap));
-	for (uint32_t p = *dConvex + blockIdx.x * CVX_WAVES + w; p < end; p += gridDim.x * CVX_WAVES)
+	const uint32_t n = min(list[1], uint32_t(cap));
+	for (uint32_t i = blockIdx.x * CVX_WAVES + w; i < n; i += gridDim.x * CVX_WAVES)
 	{
+		const uint32_t p = list[2 + cap + i];
 		const uint2 ab = pairs[p];
 		const float4 g0 = geom[ab.x], g1 = geom[ab.y];
 		const int t0 = int(g0.w);
-		if (t0 == GEOM_SPHERE || t0 == GEOM_CAPSULE)
-			continue;
-		const f3 p0 = mk(pos[ab.x]), p1 = mk(pos[ab.y]);
-		const float r0 = t0 == GEOM_BOX ? sqrtf(g0.x * g0.x + g0.y * g0.y + g0.z * g0.z) : hs.be[int(g0.x)].w;
-		const float r1 = hs.be[int(g1.x)].w;
-		const f3 d = p1 - p0;
-		const float reach = r0 + r1 + cd;
 		float4 sep = make_float4(2.0f * cd + 1.0f, 1.0f, 0.0f, 0.0f);
 		f3 cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
-		if (dot(d, d) <= reach * reach)
+		xf x0, x1;
+		x0.p = mk(pos[ab.x]); x0.q = mkq(rot[ab.x]);
+		x1.p = mk(pos[ab.y]); x1.q = mkq(rot[ab.y]);
+		const Poly B = hullPoly(hs, int(g1.x));
+		const Poly A = t0 == GEOM_BOX ? boxPoly(mk(g0.x, g0.y, g0.z)) : hullPoly(hs, int(g0.x));
+		if (fitsWave(A, B))
+			convexSeparationWave(A, x0, B, x1, cd, lds[w], lane, sep, cA, cB);
+		else if (lane == 0)
 		{
-			xf x0, x1;
-			x0.p = p0; x0.q = mkq(rot[ab.x]);
-			x1.p = p1; x1.q = mkq(rot[ab.y]);
-			const Poly B = hullPoly(hs, int(g1.x));
-			const Poly A = t0 == GEOM_BOX ? boxPoly(mk(g0.x, g0.y, g0.z)) : hullPoly(hs, int(g0.x));
-			if (fitsWave(A, B))
-				convexSeparationWave(A, x0, B, x1, cd, lds[w], lane, sep, cA, cB);
-			else if (lane == 0)
-			{
-				const mxf aToB = mxfOf(xinvmul(x1, x0));
-				f3 n = mk(1.0f, 0.0f, 0.0f);
-				polySeparation(A, aToB, B, cd, sep.x, n, cA, cB);
-				sep.y = n.x; sep.z = n.y; sep.w = n.z;
-			}
+			const mxf aToB = mxfOf(xinvmul(x1, x0));
+			f3 n = mk(1.0f, 0.0f, 0.0f);
+			polySeparation(A, aToB, B, cd, sep.x, n, cA, cB);
+			sep.y = n.x; sep.z = n.y; sep.w = n.z;
 		}
 		if (lane == 0)
 		{
@@ -716,7 +741,7 @@ k_np_convex_wave(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotB
 				wit[3 * int64_t(p)] = sep;
 				wit[3 * int64_t(p) + 1] = f4(cA, 0.0f);
 				wit[3 * int64_t(p) + 2] = f4(cB, 0.0f);
-				list[1 + atomicAdd(list, 1u)] = p;
+				list[2 + atomicAdd(list, 1u)] = p;
 			}
 			else
 				ctN[slotBase + int(p)] = make_float4(0.0f, 0.0f, 0.0f, __int_as_float(0));
@@ -736,7 +761,7 @@ __global__ void k_np_convex_finish(int cap, const uint32_t* dP, const uint32_t* 
 	const int q = blockIdx.x * blockDim.x + threadIdx.x;
 	if (q >= cap || uint32_t(q) >= list[0])
 		return;
-	const uint32_t p = list[1 + q];
+	const uint32_t p = list[2 + q];
 	const uint2 ab = pairs[p];
 	const float4 g0 = geom[ab.x], g1 = geom[ab.y];
 	const int t0 = int(g0.w);

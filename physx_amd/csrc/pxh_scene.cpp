@@ -130,7 +130,7 @@ struct pxh_scene_t
 	DBuf<uint8_t> hullVR;
 	DBuf<int4> hullInfo, hullInfo2;
 	DBuf<float4> cvxWit;     // separation records of the convex pairs (3 per pair)
-	DBuf<uint32_t> cvxList;  // touching convex pairs: count, then pair indices
+	DBuf<uint32_t> cvxList;  // convex pairs to search and touching ones: 2 counts, 2 lists
 	int nConvexBodies = 0;   // convex bodies ever added (the convex kernels run when nonzero)
 	pxh::HullSet hullSet() const
 	{
@@ -911,7 +911,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		if (s->nConvexBodies > 0)
 		{
 			PXH_CHECK(s->cvxWit.ensure(size_t(3) * std::max(cap, 1)));
-			PXH_CHECK(s->cvxList.ensure(size_t(cap) + 1));
+			PXH_CHECK(s->cvxList.ensure(2 + 2 * size_t(cap)));
+			PXH_CHECK(hipMemsetAsync(s->cvxList.p, 0, 2 * sizeof(uint32_t), st));
 			pxh::launch_np_convex(cap, dP, dConvex, Q, s->pairs.p, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), cd,
 			                      d.tolerance_length, s->cvxWit.p, s->cvxList.p, s->ctN.p, s->ctP.p, st);
 		}
