@@ -73,6 +73,20 @@ def _pair(sc, steps):
 @pytest.mark.parametrize("kind", ["convex", "mixed"])
 def test_convex_one_step_matches_oracle(cuda, kind):
     sc = S.convex_layers(n_bodies=256, per_side=8) if kind == "convex" else _mixed()
+    _one_step_from_pile(sc, kind, min_constraints=100)
+
+
+def test_large_hulls_one_step_matches_oracle(cuda):
+    """Hulls past what one wavefront stages in LDS (more than 64 vertices or
+    128 faces) take the single-lane separation search; same one-step parity."""
+    sc = S.convex_layers(n_bodies=96, per_side=4, n_points=100, seed=8)
+    n0 = int(sc.hull_counts[0])
+    h = P.cook_convex(sc.hull_points[:n0])
+    assert len(h["verts"]) > 64 or len(h["planes"]) > 128, (len(h["verts"]), len(h["planes"]))
+    _one_step_from_pile(sc, "large", min_constraints=30)
+
+
+def _one_step_from_pile(sc, kind, min_constraints):
     g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
     g.step(DT, 45)   # the first layers have landed and pile up
     pose, lv, av = g.bodies()
@@ -83,7 +97,7 @@ def test_convex_one_step_matches_oracle(cuda, kind):
     g, o = _pair(st, 1)
     gs, os_ = g.stats(), o.stats()
     print(kind, gs, flush=True)
-    assert gs["constraints"] > 100
+    assert gs["constraints"] > min_constraints
     for k in ("bp_pairs", "constraints", "contacts"):
         assert gs[k] == os_[k], (k, gs, os_)
     key = lambda c: set(map(tuple, c[:, [0, 1, 3]].tolist()))  # noqa: E731
