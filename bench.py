@@ -3,14 +3,16 @@
 Workload (one GPU): 131,072 spheres (r ~ U[0.1,0.3] m) + 131,072 boxes (half
 extents ~ U[0.1,0.3] m) poured from a seeded jittered grid into a 40 x 40 m
 walled bin (floor + 4 wall planes), TGS with 8 position / 1 velocity
-iterations, dt = 1/60 s, float32. W untimed warm-up steps let the pile form,
-then K steps are timed (wall clock around simulate + fetchResults, bracketed by
-device synchronisation).
+iterations, dt = 1/60 s, float32. A fixed, untimed pour of --settle steps
+(default 300, 5 s of simulated time) lets the pile form; then W warm-up steps
+and K timed steps (wall clock around simulate + fetchResults, bracketed by
+device synchronisation) run on the settled pile.
 
 Prints one JSON line with the body-steps/s, the roofline of the dominant
-kernel (the colour-partition solve, HBM-bound, timed with HIP events on the
-scene's stream over the timed steps) and the reference CPU pipeline timed on a
-bounded sample of the same state on the host cores.
+kernel (the colour-partition solve, HBM-bound; its launches carry HIP events
+recorded by the dispatches themselves on the scene's stream during the timed
+steps) and the reference CPU pipeline timed on a bounded sample of the same
+state on the host cores at 16, 8 and 1 threads.
 
 The same line carries the batched form ("batched"): 8,192 independent worlds
 of log-uniform [8, 512] bodies (box stacks and sphere/box piles, about 1.0M
@@ -26,8 +28,9 @@ step per serving iteration, with body-steps/s, serving-step and request
 latency percentiles.
 
 "convex" is the convex-heavy scene: 65,536 hulls of 24 points on seeded
-ellipsoids dropped in layers onto a plane, TGS 4/1, timed after 200 steps of
-settling, beside the reference CPU SDK stepping the same state.
+ellipsoids dropped in layers onto a plane, TGS 4/1, all 400 steps timed (the
+settled last 40 also reported alone), beside the reference CPU SDK stepping
+the final state.
 """
 from __future__ import annotations
 
@@ -47,16 +50,29 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_solve_traffic_sphere_box_bin_262k.md")
 
 
-def solve_traffic():
+def solve_traffic(settle, warmup, steps, constraints_avg):
     """HBM bytes per k_solve launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (calibrated against a device copy),
-    or None when the summary is absent."""
+    WRITE_SIZE summary (calibrated against a device copy), used only when that
+    summary was taken on this very state: the same settle / warm-up / timed
+    step counts and the same mean constraint count over the timed steps
+    (within 0.5%). Returns (bytes or None, note)."""
     import re
     try:
-        m = re.search(r"^traffic_bytes_per_launch:\s*(\d+)", open(TRAFFIC_PROFILE).read(), re.M)
+        text = open(TRAFFIC_PROFILE).read()
     except OSError:
-        return None
-    return int(m.group(1)) if m else None
+        return None, "no counter summary committed"
+    got = {k: re.search(rf"^{k}:\s*([0-9.]+)", text, re.M) for k in
+           ("traffic_bytes_per_launch", "settle", "warmup", "steps", "constraints_avg")}
+    if not all(got.values()):
+        return None, "counter summary lacks the state it was taken on"
+    v = {k: float(m.group(1)) for k, m in got.items()}
+    if (int(v["settle"]), int(v["warmup"]), int(v["steps"])) != (settle, warmup, steps):
+        return None, (f"counter summary taken at settle {int(v['settle'])} warmup {int(v['warmup'])} steps "
+                      f"{int(v['steps'])}, not this run's state")
+    if abs(v["constraints_avg"] - constraints_avg) > 0.005 * max(constraints_avg, 1.0):
+        return None, (f"counter summary state has {v['constraints_avg']:.0f} constraints per step, this run "
+                      f"{constraints_avg:.0f}")
+    return int(v["traffic_bytes_per_launch"]), os.path.relpath(TRAFFIC_PROFILE, ROOT)
 
 
 def log(msg):
@@ -64,11 +80,21 @@ def log(msg):
 
 
 def solve_bytes_per_step(st: dict, iters: int) -> float:
-    """Algorithmic HBM bytes of all solve launches of one step. Per constraint
-    and iteration: a 32 B header; per contact point 32 B of row read and the
-    4 B impulse read and written; per friction row 32 B of row (+16 B of body-B
-    row when B is dynamic) and the 4 B impulse read and written; per dynamic
-    body endpoint 64 B of velocity/motion state read and 32 B written."""
+    """Minimal algorithmic HBM bytes of all solve launches of one step, the
+    roofline's byte count: per iteration 152 B per touching pair (a 64 B row
+    header and two friction rows of 44 B), 60 B per contact point (56 B of row
+    read, the 4 B impulse written) and 128 B per body (velocity and motion
+    state read and written once)."""
+    return iters * (152.0 * st["constraints"] + 60.0 * st["contacts"] + 128.0 * st["bodies"])
+
+
+def solve_bytes_per_step_endpoints(st: dict, iters: int) -> float:
+    """The same launches counted as this solver's row layout moves them. Per
+    constraint and iteration: a 32 B header; per contact point 32 B of row
+    read and the 4 B impulse read and written; per friction row 32 B of row
+    (+16 B of body-B row when B is dynamic) and the 4 B impulse read and
+    written; per dynamic body endpoint 64 B of velocity/motion state read and
+    32 B written (a body is gathered once for every constraint it is in)."""
     T = st["constraints"]
     K = st["contacts"]
     F = st["friction_rows"]
@@ -77,16 +103,26 @@ def solve_bytes_per_step(st: dict, iters: int) -> float:
     return iters * (32.0 * T + 40.0 * K + 40.0 * F + 16.0 * Fd + 96.0 * endpoints)
 
 
-def trace_summary(path: str, skip_steps: int) -> None:
+def _window(rows, window):
+    """The dispatches of steps SKIP+1 .. SKIP+STEPS (one k_finalize or
+    k_pgs_finalize dispatch ends every step), rows sorted by start time."""
+    fin = [i for i, r in enumerate(rows) if "finalize" in r["Kernel_Name"]]
+    if window is None:
+        return rows, len(fin)
+    skip, steps = window
+    lo = fin[skip - 1] + 1 if skip > 0 else 0
+    hi = fin[skip + steps - 1] + 1
+    return rows[lo:hi], steps
+
+
+def trace_summary(path: str, window) -> None:
     """Per-step kernel table from a rocprofv3 kernel-trace CSV
-    (`--kernel-trace --output-format csv`), over the dispatches after the
-    `skip_steps`-th k_finalize (one k_finalize ends every step)."""
+    (`--kernel-trace --output-format csv`) over a window of steps."""
     import collections
     import csv
     rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
-    fin = [i for i, r in enumerate(rows) if "k_finalize" in r["Kernel_Name"]]
-    sel = rows[fin[skip_steps - 1] + 1:] if skip_steps > 0 else rows
-    steps = max(1, sum(1 for r in sel if "k_finalize" in r["Kernel_Name"]))
+    sel, steps = _window(rows, window)
+    steps = max(1, steps)
     agg = collections.defaultdict(lambda: [0, 0])
     for r in sel:
         name = r["Kernel_Name"].split("(")[0].replace("void ", "")
@@ -102,21 +138,23 @@ def trace_summary(path: str, skip_steps: int) -> None:
         print(f"| {k} | {c / steps:.1f} | {d / 1e6 / steps:.3f} | {d / c / 1e3:.2f} | {100 * d / busy:.1f} |")
 
 
-def pmc_summary(path: str, kernel: str, last: int) -> None:
-    """Mean counter value per dispatch of `kernel` over its last `last`
-    dispatches in a rocprofv3 counter-collection CSV (`--pmc ... --output-format
-    csv`), with the mean dispatch duration."""
+def pmc_summary(path: str, kernel: str, window) -> None:
+    """Mean counter value per dispatch of `kernel` over a window of steps in a
+    rocprofv3 counter-collection CSV (`--pmc ... --output-format csv`), with
+    the mean dispatch duration."""
     import csv
-    rows = [r for r in csv.DictReader(open(path)) if kernel in r["Kernel_Name"]]
-    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
-    rows = rows[-last:]
+    rows = sorted(csv.DictReader(open(path)), key=lambda r: int(r["Start_Timestamp"]))
+    # one row per counter and dispatch: the step window is found on one counter's rows
+    names = sorted({r["Counter_Name"] for r in rows})
     by = {}
-    for r in rows:
-        by.setdefault(r["Counter_Name"], []).append(float(r["Counter_Value"]))
-    dur = np.mean([int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in rows]) / 1e3
-    for k, v in by.items():
+    for cn in names:
+        sub, _ = _window([r for r in rows if r["Counter_Name"] == cn], window)
+        sub = [r for r in sub if kernel in r["Kernel_Name"] and "pgs" not in r["Kernel_Name"]]
+        by[cn] = ([float(r["Counter_Value"]) for r in sub],
+                  [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sub])
+    for k, (v, d) in by.items():
         print(f"{kernel}: {k} mean {np.mean(v):.1f} KiB per dispatch over {len(v)} dispatches "
-              f"(mean duration under counters {dur:.2f} us)")
+              f"(mean duration under counters {np.mean(d) / 1e3:.2f} us)")
 
 
 def calibrate_copy(mb: int) -> None:
@@ -133,7 +171,8 @@ def calibrate_copy(mb: int) -> None:
 
 
 def run_batched(args, P, scenes):
-    """Batched step of many small worlds against the plain per-scene path."""
+    """Batched step of many small worlds (every step of the run timed, from
+    the initial drop) against the plain per-scene path."""
     import torch
     dt = 1.0 / 120.0
     sc = scenes.batched_piles(n_scenes=args.scenes)
@@ -143,15 +182,16 @@ def run_batched(args, P, scenes):
         g.step(dt)
     torch.cuda.synchronize()
     times = []
+    pairs = 0
     t0 = time.perf_counter()
     for s in range(args.batched_steps):
         a = time.perf_counter()
         g.simulate(dt)
         g.fetch_results(True)
         times.append((time.perf_counter() - a) * 1e3)
+        pairs += g.stats()["constraints"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    st = g.stats()
     g.close()
     batched = sc.n * args.batched_steps / elapsed
     # plain path: every k-th world as its own scene, same warm-up, stepped in turn
@@ -159,7 +199,7 @@ def run_batched(args, P, scenes):
     worlds = list(range(0, sc.n_scenes, stride))[: args.plain_sample]
     subs = [sc.subset(w) for w in worlds]
     plain = [P.Scene(x, position_iterations=4, velocity_iterations=1) for x in subs]
-    for s in range(args.batched_warmup):
+    for s in range(args.plain_warmup):
         for x in plain:
             x.step(dt)
     torch.cuda.synchronize()
@@ -180,10 +220,10 @@ def run_batched(args, P, scenes):
            "warmup": args.batched_warmup, "body_steps_per_s": round(batched, 1),
            "ms_per_step": round(1e3 * elapsed / args.batched_steps, 3),
            "p50_step_ms": round(float(np.percentile(ts, 50)), 3), "p99_step_ms": round(float(np.percentile(ts, 99)), 3),
-           "contact_pairs_per_s": round(st["constraints"] * args.batched_steps / elapsed, 1),
+           "contact_pairs_per_s": round(pairs / elapsed, 1),
            "plain_body_steps_per_s": round(plain_rate, 1),
            "plain_sample": f"{len(worlds)} worlds (every {stride}th, {nb} bodies), {args.plain_steps} steps each "
-                           f"after the same warm-up, one scene per world stepped in turn",
+                           f"after {args.plain_warmup} warm-up steps, one scene per world stepped in turn",
            "speedup_vs_plain": round(batched / plain_rate, 2)}
     log(f"batched: {out}")
     return out
@@ -192,46 +232,52 @@ def run_batched(args, P, scenes):
 def run_convex(args, P, scenes):
     """Convex-heavy scene: 65,536 hulls of 24 points on seeded ellipsoids
     (semi-axes U[0.2,0.6] m) dropped in layers onto a plane, TGS 4/1,
-    dt = 1/60; timed after the pile has formed, with the reference CPU SDK
-    stepping the same state on the host cores."""
+    dt = 1/60, every step of the run timed, the last --convex-window steps
+    (the settled pile) reported separately, with the reference CPU SDK
+    stepping the final state on the host cores."""
     import torch
     dt = 1.0 / 60.0
     t0 = time.perf_counter()
     sc = scenes.convex_layers(n_bodies=args.convex_bodies)
     log(f"convex: {sc.n} hulls generated in {time.perf_counter() - t0:.1f}s")
     g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
-    for s in range(args.convex_warmup):
-        g.step(dt)
-        if (s + 1) % 100 == 0:
-            log(f"convex warmup {s + 1}/{args.convex_warmup} {g.stats()}")
     torch.cuda.synchronize()
-    times, pairs = [], 0
+    times, pairs = [], []
     t0 = time.perf_counter()
     for s in range(args.convex_steps):
         a = time.perf_counter()
         g.simulate(dt)
         g.fetch_results(True)
         times.append((time.perf_counter() - a) * 1e3)
-        pairs += g.stats()["constraints"]
+        pairs.append(g.stats()["constraints"])
+        if (s + 1) % 100 == 0:
+            log(f"convex step {s + 1}/{args.convex_steps} {g.stats()}")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     st = g.stats()
-    # stage breakdown from a profiled pass of as many further steps
+    # stage breakdown from a profiled pass of a window's length
+    w = min(args.convex_window, args.convex_steps)
     g.set_profiling(True)
-    for s in range(args.convex_steps):
+    for s in range(w):
         g.simulate(dt)
         g.fetch_results(True)
     torch.cuda.synchronize()
     prof = g.profile()
     g.set_profiling(False)
     ts = np.array(times)
+    tw = ts[-w:]
     out = {"workload": f"convex_layers: {sc.n} hulls of 24 points on ellipsoids U[0.2,0.6] m, layers onto a plane, "
-                       f"TGS 4/1, dt 1/60", "bodies": sc.n, "steps": args.convex_steps,
-           "warmup": args.convex_warmup, "body_steps_per_s": round(sc.n * args.convex_steps / elapsed, 1),
+                       f"TGS 4/1, dt 1/60, all {args.convex_steps} steps timed", "bodies": sc.n,
+           "steps": args.convex_steps, "body_steps_per_s": round(sc.n * args.convex_steps / elapsed, 1),
            "ms_per_step": round(1e3 * elapsed / args.convex_steps, 3),
            "p50_step_ms": round(float(np.percentile(ts, 50)), 3), "p99_step_ms": round(float(np.percentile(ts, 99)), 3),
-           "contact_pairs_per_s": round(pairs / elapsed, 1), "last_step_counts": st,
-           "stage_ms_per_step": {k: round(v[0] / args.convex_steps, 3) for k, v in prof.items()}}
+           "contact_pairs_per_s": round(sum(pairs) / elapsed, 1), "last_step_counts": st,
+           "settled_window": {"steps": f"{args.convex_steps - w + 1}-{args.convex_steps}",
+                              "ms_per_step": round(float(tw.mean()), 3),
+                              "body_steps_per_s": round(sc.n * 1e3 / float(tw.mean()), 1),
+                              "p99_step_ms": round(float(np.percentile(tw, 99)), 3),
+                              "contact_pairs_per_step": round(float(np.mean(pairs[-w:])), 1)},
+           "stage_ms_per_step": {k: round(v[0] / w, 3) for k, v in prof.items()}}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_libs as L
@@ -239,15 +285,18 @@ def run_convex(args, P, scenes):
             pose, lv, av = g.bodies()
             stt = scenes.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
                                sc.plane_scene, 1, sc.name, {}, sc.hull_points, sc.hull_counts)
-            ref = L.RefScene(stt, "tgs", 4, 1, threads=args.cpu_threads)
+            threads = args.cpu_threads[0]
+            ref = L.RefScene(stt, "tgs", 4, 1, threads=threads)
             ref.step(dt, 1)
-            total, _ = ref.step(dt, args.cpu_steps)
-            cpu = sc.n * args.cpu_steps / total
-            out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "body-steps/s", "cores": args.cpu_threads,
-                                   "kind": "reference", "ms_per_step": round(1e3 * total / args.cpu_steps, 2),
-                                   "sample": f"{args.cpu_steps} steps of the same pile state (+1 untimed), "
-                                             f"reference PhysX 5.6.1 CPU SDK cooking the same point clouds"}
-            out["speedup_vs_cpu"] = round(out["body_steps_per_s"] / cpu, 2)
+            n = 2 * args.cpu_steps[0]
+            total, _ = ref.step(dt, n)
+            cpu = sc.n * n / total
+            out["cpu_baseline"] = {"value": round(cpu, 1), "unit": "body-steps/s", "cores": threads,
+                                   "kind": "reference", "ms_per_step": round(1e3 * total / n, 2),
+                                   "sample": f"{n} steps of the pile state after step "
+                                             f"{args.convex_steps + w} (+1 untimed), reference PhysX 5.6.1 CPU SDK "
+                                             f"cooking the same point clouds"}
+            out["speedup_vs_cpu_settled"] = round(out["settled_window"]["body_steps_per_s"] / cpu, 2)
     g.close()
     log(f"convex: {out}")
     return out
@@ -324,26 +373,82 @@ def run_served(args, P, scenes):
             "warmup_s": args.serve_warmup_s, "rates": results}
 
 
+def host_info() -> dict:
+    """The host the CPU baseline ran on: logical CPUs, the CPUs this process
+    may use, and the CPU model."""
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cpus": usable, "cpu_model": model}
+
+
+def cpu_baseline(sc, pose, lv, av, dt, pos_iters, vel_iters, args):
+    """The reference PhysX CPU SDK stepping the GPU's state at each thread
+    count of --cpu-threads (one PxScene; the dispatcher is swapped between
+    legs), a bounded number of steps per leg after one untimed step."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_libs as L
+    from physx_amd import scenes
+    if not L.have_ref():
+        return None
+    st = scenes.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
+                      sc.plane_scene, 1, sc.name)
+    threads = args.cpu_threads
+    t0 = time.perf_counter()
+    ref = L.RefScene(st, "tgs", pos_iters, vel_iters, threads=threads[0])
+    ref.step(dt, 1)  # untimed: the first step creates every broadphase pair
+    log(f"cpu baseline: reference scene built and first step in {time.perf_counter() - t0:.1f}s")
+    legs = []
+    for k, (t, n) in enumerate(zip(threads, args.cpu_steps)):
+        if k > 0:
+            ref.set_threads(t)
+            ref.step(dt, 1)  # untimed: the new dispatcher's workers start
+        total, ms = ref.step(dt, n)
+        legs.append({"threads": t, "steps": n, "ms_per_step": round(1e3 * total / n, 2),
+                     "body_steps_per_s": round(sc.n * n / total, 1)})
+        log(f"cpu baseline: {legs[-1]}")
+    first = legs[0]
+    return {"value": first["body_steps_per_s"], "unit": "body-steps/s", "cores": first["threads"],
+            "kind": "reference", "ms_per_step": first["ms_per_step"], "legs": legs, "host": host_info(),
+            "sample": f"{first['steps']} steps of the settled {sc.n}-body pile state that the GPU reached at step "
+                      f"{args.settle + args.warmup + 2 * args.steps} (+1 untimed step per leg), reference PhysX "
+                      f"5.6.1 CPU SDK built from source (ABP, PCM, TGS {pos_iters}/{vel_iters}); value is the "
+                      f"{first['threads']}-thread leg (the CPU share of one GPU on the host)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
-    ap.add_argument("--warmup", type=int, default=300)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle", type=int, default=300,
+                    help="untimed pour steps before --warmup: the timed steps run on the settled pile")
     ap.add_argument("--bodies", type=int, default=262144)
-    ap.add_argument("--cpu-steps", type=int, default=4)
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, nargs="+", default=[16, 8, 1])
+    ap.add_argument("--cpu-steps", type=int, nargs="+", default=[3, 2, 1])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--scenes", type=int, default=8192)
-    ap.add_argument("--batched-steps", type=int, default=100)
-    ap.add_argument("--batched-warmup", type=int, default=120)
+    ap.add_argument("--batched-steps", type=int, default=1000)
+    ap.add_argument("--batched-warmup", type=int, default=0)
     ap.add_argument("--plain-sample", type=int, default=64)
+    ap.add_argument("--plain-warmup", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=20)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-convex", action="store_true")
     ap.add_argument("--convex-bodies", type=int, default=65536)
-    ap.add_argument("--convex-warmup", type=int, default=200)
-    ap.add_argument("--convex-steps", type=int, default=40)
+    ap.add_argument("--convex-steps", type=int, default=400)
+    ap.add_argument("--convex-window", type=int, default=40,
+                    help="the last steps of the convex run reported separately (the settled pile)")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[250.0, 1000.0, 4000.0])
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-warmup-s", type=float, default=1.0)
@@ -355,13 +460,16 @@ def main():
                     help="run 5 device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--pmc-summary", metavar="CSV",
                     help="print the mean counter value per k_solve dispatch of a rocprofv3 counter CSV and exit")
-    ap.add_argument("--pmc-last", type=int, default=600, help="dispatches averaged by --pmc-summary")
+    ap.add_argument("--window", type=int, nargs=2, metavar=("SKIP", "STEPS"),
+                    help="--trace-summary / --pmc-summary: the STEPS steps after the first SKIP")
     args = ap.parse_args()
+    if len(args.cpu_steps) != len(args.cpu_threads):
+        ap.error("--cpu-steps needs one count per --cpu-threads entry")
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, "k_solve", args.pmc_last)
+        pmc_summary(args.pmc_summary, "k_solve", args.window)
         return
     if args.trace_summary:
-        trace_summary(args.trace_summary, args.warmup)
+        trace_summary(args.trace_summary, args.window)
         return
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -385,17 +493,26 @@ def main():
     log(f"scene: {sc.n} bodies, {sc.planes.shape[0]} planes")
     gpu = P.Scene(sc, position_iterations=pos_iters, velocity_iterations=vel_iters)
 
+    # the pour: untimed for the headline, its step times kept for the full-run figure
+    pour_ms = []
     t0 = time.perf_counter()
-    for s in range(args.warmup):
-        gpu.step(dt)
+    for s in range(args.settle + args.warmup):
+        a = time.perf_counter()
+        gpu.simulate(dt)
+        gpu.fetch_results(True)
+        pour_ms.append((time.perf_counter() - a) * 1e3)
         if (s + 1) % 50 == 0:
-            log(f"warmup step {s + 1}/{args.warmup} ({time.perf_counter() - t0:.1f}s) {gpu.stats()}")
+            log(f"pour step {s + 1}/{args.settle + args.warmup} ({time.perf_counter() - t0:.1f}s) {gpu.stats()}")
     torch.cuda.synchronize()
 
-    # timed steps: the library as a caller runs it (no per-launch events)
+    # timed steps: the library as a caller runs it; the solve launches carry
+    # their own start/end events (recorded by the dispatches), nothing else
+    SOLVE = P.Scene.STAGES.index("solve")
+    gpu.set_profiling(1 << SOLVE)
     step_ms = []
-    sums = dict(constraints=0, contacts=0, friction_rows=0, body_body_constraints=0, bp_pairs=0, plane_pairs=0,
-                colors=0, color_rounds=0, body_body_friction_rows=0)
+    sums = dict(bodies=0, constraints=0, contacts=0, friction_rows=0, body_body_constraints=0, bp_pairs=0,
+                plane_pairs=0, colors=0, color_rounds=0, body_body_friction_rows=0)
+    solve_bytes = solve_bytes_ep = 0.0
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
     for s in range(args.steps):
@@ -406,59 +523,53 @@ def main():
         st = gpu.stats()
         for k in sums:
             sums[k] += st[k]
+        solve_bytes += solve_bytes_per_step(st, pos_iters + vel_iters)
+        solve_bytes_ep += solve_bytes_per_step_endpoints(st, pos_iters + vel_iters)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_begin
-    log(f"timed {args.steps} steps in {elapsed:.3f}s")
+    solve_ms, solve_launches = gpu.profile().get("solve", (0.0, 0))
+    log(f"timed {args.steps} steps in {elapsed:.3f}s; solve launches {solve_launches}, {solve_ms:.3f} ms")
 
-    # the same number of steps again with HIP events around every launch: the
-    # per-stage GPU time and k_solve's average launch duration (the events add
-    # host work per launch, so this pass is not the timed one)
+    # the same number of steps again with every stage timed (events around
+    # each launch add marker packets, so this pass is not the timed one)
     gpu.set_profiling(True)
-    solve_bytes = 0.0
     for s in range(args.steps):
         gpu.simulate(dt)
         gpu.fetch_results(True)
-        solve_bytes += solve_bytes_per_step(gpu.stats(), pos_iters + vel_iters)
     torch.cuda.synchronize()
     prof = gpu.profile()
     gpu.set_profiling(False)
     log(f"profiled {args.steps} steps; stage GPU ms (total, launches): {prof}")
 
     value = sc.n * args.steps / elapsed
-    solve_ms, solve_launches = prof.get("solve", (0.0, 0))
-    achieved = (solve_bytes / (solve_ms * 1e-3)) / 1e9 if solve_ms > 0 else 0.0
+    avg = {k: v / args.steps for k, v in sums.items()}
+    avg_launch_s = solve_ms * 1e-3 / max(solve_launches, 1)
+    bytes_launch = solve_bytes / max(solve_launches, 1)
+    achieved = bytes_launch / avg_launch_s / 1e9 if solve_ms > 0 else 0.0
+    achieved_ep = solve_bytes_ep / max(solve_launches, 1) / avg_launch_s / 1e9 if solve_ms > 0 else 0.0
+    traffic, traffic_src = solve_traffic(args.settle, args.warmup, args.steps, avg["constraints"])
     roofline = {"bound": "hbm", "kernel": "k_solve", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": solve_traffic(),
-                "bytes_per_launch": round(solve_bytes / max(solve_launches, 1), 1),
-                "avg_launch_us": round(1e3 * solve_ms / max(solve_launches, 1), 3),
-                "launches": solve_launches}
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "traffic_source": traffic_src,
+                "bytes_per_launch": round(bytes_launch, 1),
+                "byte_model": "per iteration 152 B per touching pair + 60 B per contact point + 128 B per body",
+                "avg_launch_us": round(1e6 * avg_launch_s, 3), "launches": solve_launches,
+                "solve_ms_per_step": round(solve_ms / args.steps, 4),
+                "achieved_row_layout_model": round(achieved_ep, 2),
+                "bytes_per_launch_row_layout_model": round(solve_bytes_ep / max(solve_launches, 1), 1)}
 
     cpu = None
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import oracle_libs as L
-        if L.have_ref():
-            pose, lv, av = gpu.bodies()
-            st = scenes.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
-                              sc.plane_scene, 1, sc.name)
-            log(f"cpu baseline: reference PxScene, {args.cpu_threads} threads, {args.cpu_steps} steps")
-            ref = L.RefScene(st, "tgs", pos_iters, vel_iters, threads=args.cpu_threads)
-            ref.step(dt, 1)  # untimed: first step creates every broadphase pair
-            total, ms = ref.step(dt, args.cpu_steps)
-            cpu = {"value": round(sc.n * args.cpu_steps / total, 1), "unit": "body-steps/s",
-                   "cores": args.cpu_threads, "kind": "reference",
-                   "sample": f"{args.cpu_steps} steps of the same {sc.n}-body pile state after {args.warmup} GPU "
-                             f"warm-up steps (+1 untimed step), reference PhysX 5.6.1 CPU SDK (ABP, PCM, TGS "
-                             f"{pos_iters}/{vel_iters}), {args.cpu_threads} dispatcher threads",
-                   "ms_per_step": round(1e3 * total / args.cpu_steps, 2)}
-            log(f"cpu baseline: {cpu}")
-
+        pose, lv, av = gpu.bodies()
+        cpu = cpu_baseline(sc, pose, lv, av, dt, pos_iters, vel_iters, args)
     gpu.close()
     batched = None if args.no_batched else run_batched(args, P, scenes)
     served = None if args.no_served else run_served(args, P, scenes)
     convex = None if args.no_convex else run_convex(args, P, scenes)
 
     sms = np.array(step_ms)
+    pour = np.array(pour_ms)
+    first = args.settle + args.warmup + 1
     out = {
         "metric": "body-steps/sec at 262k bodies",
         "value": round(value, 1),
@@ -472,14 +583,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "sphere_box_bin: 131072 spheres + 131072 boxes, 40x40 m walled bin, TGS 8/1, "
-                               "dt 1/60", "bodies": sc.n, "solver": "TGS", "position_iterations": pos_iters,
-                   "velocity_iterations": vel_iters, "dt": dt, "parallelism": "single GPU, single scene"},
+        "config": {"workload": f"sphere_box_bin settled pile: 131072 spheres + 131072 boxes poured into a 40x40 m "
+                               f"walled bin, TGS 8/1, dt 1/60; timed steps {first}-{first + args.steps - 1} after "
+                               f"an untimed {args.settle}-step pour and {args.warmup} warm-up steps",
+                   "bodies": sc.n, "solver": "TGS", "position_iterations": pos_iters,
+                   "velocity_iterations": vel_iters, "dt": dt, "settle_steps": args.settle,
+                   "parallelism": "single GPU, single scene"},
         "p50_step_ms": round(float(np.percentile(sms, 50)), 3),
         "p99_step_ms": round(float(np.percentile(sms, 99)), 3),
         "contact_pairs_per_s": round(sums["constraints"] / elapsed, 1),
-        "avg_counts": {k: round(v / args.steps, 1) for k, v in sums.items()},
+        "avg_counts": {k: round(v, 1) for k, v in avg.items()},
         "stage_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in prof.items()},
+        "pour": {"steps": len(pour_ms), "ms_per_step": round(float(pour.mean()), 4) if len(pour) else None,
+                 "body_steps_per_s": round(sc.n * 1e3 / float(pour.mean()), 1) if len(pour) else None,
+                 "note": "steps 1..settle+warmup (free fall into the forming pile), not the headline"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "speedup_vs_cpu": round(value / cpu["value"], 2) if cpu else None,

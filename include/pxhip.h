@@ -123,9 +123,13 @@ int64_t pxh_scene_get_constraints(pxh_scene scene, int32_t* out, int64_t max_con
  * (PX_PROFILE_ZONE, physx/include/common/PxProfileZone.h:34-50). Stages:
  * 0 bounds, 1 broadphase (untimed), 2 body-pair narrowphase, 3 body-plane
  * narrowphase, 4 contact prep, 5 solve (one colour per launch), 6 integration.
+ * stage_mask: bit i times stage i (0 = off, PXH_PROFILE_ALL = every stage);
+ * solve launches carry their events in the dispatch itself, so timing the
+ * solve alone adds no marker packets to the stream.
  * ms[i] accumulates milliseconds and launches[i] the timed launches since
- * profiling was (re)enabled. */
-int32_t pxh_scene_set_profiling(pxh_scene scene, int32_t enable);
+ * profiling was (re)set. */
+#define PXH_PROFILE_ALL 0xff
+int32_t pxh_scene_set_profiling(pxh_scene scene, int32_t stage_mask);
 int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, int32_t n);
 
 /* PxDirectGPUAPI::copyContactData (physx/include/PxDirectGPUAPI.h:401): the

@@ -131,6 +131,7 @@ PxFilterFlags reportingFilterShader(PxFilterObjectAttributes a0, PxFilterData d0
 struct RefScene
 {
 	PxDefaultCpuDispatcher* dispatcher = nullptr;
+	std::vector<PxDefaultCpuDispatcher*> retired;   // dispatchers replaced by pxref_scene_set_threads
 	PxScene* scene = nullptr;
 	PxMaterial* material = nullptr;
 	std::vector<PxRigidDynamic*> bodies;
@@ -446,6 +447,23 @@ void pxref_scene_stats(void* h, long long* out)
 	out[3] = st.nbDiscreteContactPairsWithContacts;
 }
 
+// Replaces the scene's CPU dispatcher by one of nThreads workers between
+// steps (PxTaskManager::setCpuDispatcher, physx/include/task/PxTaskManager.h:88);
+// every stage reads the worker count from the task manager when it runs
+// (e.g. ScPipeline.cpp:350, DyTGSDynamics.cpp:2409), so the next step uses
+// nThreads. Returns 0, or -1 for a bad argument.
+int pxref_scene_set_threads(void* h, int nThreads)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	if (!rs || !rs->scene || nThreads < 0)
+		return -1;
+	PxDefaultCpuDispatcher* d = PxDefaultCpuDispatcherCreate(PxU32(nThreads));
+	rs->scene->getTaskManager()->setCpuDispatcher(*d);
+	rs->retired.push_back(rs->dispatcher);
+	rs->dispatcher = d;
+	return 0;
+}
+
 void pxref_scene_release(void* h)
 {
 	RefScene* rs = static_cast<RefScene*>(h);
@@ -455,6 +473,8 @@ void pxref_scene_release(void* h)
 		rs->scene->release();
 	if (rs->dispatcher)
 		rs->dispatcher->release();
+	for (PxDefaultCpuDispatcher* d : rs->retired)
+		d->release();
 	if (rs->material)
 		rs->material->release();
 	delete rs->contacts;

@@ -80,7 +80,7 @@ def load():
                  "pxh_scene_simulate", "pxh_scene_fetch_results", "pxh_scene_get_rigid_dynamic_data",
                  "pxh_scene_set_rigid_dynamic_data", "pxh_scene_get_stats", "pxh_bp_create", "pxh_bp_update",
                  "pxh_bp_fetch_results", "pxh_bp_release", "pxh_generate_contacts", "pxh_scene_set_profiling",
-            "pxh_scene_get_profile"):
+                 "pxh_scene_get_profile"):
         getattr(lib, name).restype = ctypes.c_int32
     _lib = lib
     return lib
@@ -211,8 +211,11 @@ class Scene:
 
     STAGES = ("bounds", "broadphase", "np_pairs", "np_planes", "prep", "solve", "integrate", "unused")
 
-    def set_profiling(self, enable: bool):
-        _check(self.lib.pxh_scene_set_profiling(self.h, int(enable)))
+    def set_profiling(self, enable):
+        """True: time every stage; False: off; an int: bit mask of the stages
+        (index in STAGES) to time."""
+        mask = 0xFF if enable is True else (0 if enable is False else int(enable))
+        _check(self.lib.pxh_scene_set_profiling(self.h, mask))
 
     def profile(self) -> dict:
         ms = np.zeros(8, np.float64)

@@ -282,6 +282,11 @@ class RefScene:
         self._keep.append(pl)
         assert self.lib.pxref_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ctypes.c_float(0.02)) == 0
 
+    def set_threads(self, threads):
+        """Dispatcher of `threads` workers for the following steps
+        (PxTaskManager::setCpuDispatcher between steps)."""
+        assert self.lib.pxref_scene_set_threads(self.h, int(threads)) == 0
+
     def step(self, dt, n=1):
         ms = np.zeros(n, np.float64)
         total = self.lib.pxref_scene_step(self.h, ctypes.c_float(dt), n, ms.ctypes.data_as(D))
