@@ -178,6 +178,9 @@ def run_batched(args, P, scenes):
     sc = scenes.batched_piles(n_scenes=args.scenes)
     log(f"batched: {sc.n_scenes} worlds, {sc.n} bodies")
     g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
+    # the solve's launch shape is autotuned on the first steps of the run
+    # (every candidate for 2 real steps; no bit of the result depends on it)
+    g.set_tuning(autotune_steps=args.autotune_steps)
     for s in range(args.batched_warmup):
         g.step(dt)
     torch.cuda.synchronize()
@@ -192,6 +195,7 @@ def run_batched(args, P, scenes):
         pairs += g.stats()["constraints"]
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    tuning, tuning_log = g.tuning(), g.tuning_log()
     g.close()
     batched = sc.n * args.batched_steps / elapsed
     # plain path: every k-th world as its own scene, same warm-up, stepped in turn
@@ -224,7 +228,10 @@ def run_batched(args, P, scenes):
            "plain_body_steps_per_s": round(plain_rate, 1),
            "plain_sample": f"{len(worlds)} worlds (every {stride}th, {nb} bodies), {args.plain_steps} steps each "
                            f"after {args.plain_warmup} warm-up steps, one scene per world stepped in turn",
-           "speedup_vs_plain": round(batched / plain_rate, 2)}
+           "speedup_vs_plain": round(batched / plain_rate, 2),
+           "solve_shape": {"layout": ["auto", "colour launches", "per-world persistent"][tuning[0]],
+                           "workgroups": tuning[1], "lds_bodies": tuning[2],
+                           "autotune_log_ms_per_step": [[a, b, c, round(m, 4)] for a, b, c, m in tuning_log]}}
     log(f"batched: {out}")
     return out
 
@@ -440,6 +447,8 @@ def main():
     ap.add_argument("--scenes", type=int, default=8192)
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--batched-warmup", type=int, default=0)
+    ap.add_argument("--autotune-steps", type=int, default=2,
+                    help="batched run: steps per candidate solve shape in the autotuning (0: none)")
     ap.add_argument("--plain-sample", type=int, default=64)
     ap.add_argument("--plain-warmup", type=int, default=120)
     ap.add_argument("--plain-steps", type=int, default=20)

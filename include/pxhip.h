@@ -34,8 +34,14 @@ enum { PXH_GEOM_SPHERE = 0, PXH_GEOM_PLANE = 1, PXH_GEOM_CAPSULE = 2, PXH_GEOM_B
 enum { PXH_SOLVER_PGS = 0, PXH_SOLVER_TGS = 1 };
 
 /* Direct GPU API data kinds = PxRigidDynamicGPUAPIReadType/WriteType
- * (physx/include/PxDirectGPUAPI.h): pose is a PxTransform (q xyzw, p xyz). */
+ * (physx/include/PxDirectGPUAPI.h:45-71): pose is a PxTransform (q xyzw,
+ * p xyz), the others one PxVec3 per body. Reads 3/4 are the accelerations of
+ * the last step (velocity change over its dt; need enable_body_accelerations),
+ * writes 3/4 a force / torque at the centre of mass for the next step
+ * (PxForceMode::eFORCE, cleared after that step). */
 enum { PXH_DATA_GLOBAL_POSE = 0, PXH_DATA_LINEAR_VELOCITY = 1, PXH_DATA_ANGULAR_VELOCITY = 2 };
+enum { PXH_READ_LINEAR_ACCELERATION = 3, PXH_READ_ANGULAR_ACCELERATION = 4 };
+enum { PXH_WRITE_FORCE = 3, PXH_WRITE_TORQUE = 4 };
 
 /* Scene description: the PxSceneDesc / PxRigidDynamic / PxShape / PxMaterial
  * parameters the step reads (PxSceneDesc.h:1035-1075 defaults,
@@ -54,7 +60,11 @@ typedef struct pxh_scene_desc {
 	float friction_offset_threshold;     /* PxSceneDesc::frictionOffsetThreshold */
 	float friction_correlation_distance; /* PxSceneDesc::frictionCorrelationDistance */
 	float tolerance_length;              /* PxTolerancesScale::length */
-	int32_t max_pairs_per_body;          /* initial pair capacity per body; grows on demand */
+	int32_t max_pairs_per_body;          /* initial pair buffer per body (default 4, >= 1); grows on demand */
+	int32_t enable_body_accelerations;   /* PxSceneFlag::eENABLE_BODY_ACCELERATIONS: keep the velocities
+	                                        before each step for the acceleration reads */
+	int32_t report_touch_events;         /* PxPairFlag::eNOTIFY_TOUCH_FOUND | eNOTIFY_TOUCH_LOST on every
+	                                        pair: compute the touch events of each step */
 } pxh_scene_desc;
 
 typedef struct pxh_scene_t* pxh_scene;
@@ -110,9 +120,25 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, cons
  * out[0] bodies, [1] broadphase pairs (body-body), [2] body-plane pairs,
  * [3] touching pairs (constraints), [4] contact points, [5] colours,
  * [6] steps taken, [7] friction rows, [8] body-body constraints, [9] colouring rounds,
- * [10] friction rows of body-body constraints.
+ * [10] friction rows of body-body constraints, [11] device bytes the scene
+ * holds (PxSimulationStatistics::gpuMemHeap, PxSimulationStatistics.h:333).
  * Returns the number of values written (<= n). */
 int32_t pxh_scene_get_stats(pxh_scene scene, int64_t* out, int32_t n);
+
+/* Touch events of the last step: pairs that touch (have contacts) now and did
+ * not in the step before (found), and pairs that touched then and do not now
+ * (lost). What PxSimulationEventCallback::onContact reports for
+ * PxPairFlag::eNOTIFY_TOUCH_FOUND / eNOTIFY_TOUCH_LOST (physx/include/
+ * PxSimulationEventCallback.h), found on the device like the GPU reference's
+ * prepareLostFoundPairs (gpunarrowphase/src/CUDA/cudaGJKEPA.cu:1468,1532; CPU
+ * PxsNphaseImplementationContext::appendNewLostPairs,
+ * lowlevel/software/src/PxsNphaseImplementationContext.cpp:928). Needs
+ * report_touch_events. Host arrays of (body a, body b or -1 - plane index)
+ * pairs in the current body numbering, at most max_found / max_lost of each;
+ * counts[0..1] receive the full counts. Lost pairs of worlds removed since are
+ * not reported. The first step reports every touching pair as found. */
+int32_t pxh_scene_get_touch_events(pxh_scene scene, int32_t* found, int64_t max_found, int32_t* lost,
+                                   int64_t max_lost, int64_t* counts);
 
 /* Last step's constraint table, host copy for inspection:
  * per constraint (body a, body b or -1-plane, colour, contact count). */
@@ -141,6 +167,30 @@ int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, in
  * nb_pairs (device, may be NULL) receives the number of records written. */
 int32_t pxh_scene_copy_contact_data(pxh_scene scene, void* data, uint32_t* nb_pairs, int32_t max_pairs, void* stream);
 
+/* Launch shapes of the step. No counterpart in the reference, whose GPU
+ * kernels have fixed shapes; results are the same bit for bit under every
+ * setting. tuning[0] the TGS solve layout: PXH_SOLVE_LAYOUT_AUTO (the
+ * per-world solve when every world has at most PXH_WORLD_SOLVE_MAX_BODIES
+ * bodies, else one launch per colour and iteration), _COLORS or _WORLDS (one
+ * persistent launch of every iteration of every world, each world in one
+ * workgroup); tuning[1] workgroups of the per-world solve (0 = 4 per compute
+ * unit); tuning[2] bodies per world whose velocity records the per-world
+ * solve keeps in LDS (at most 1000, 64 B each; larger worlds work in HBM);
+ * tuning[3] autotuning: when positive, the next steps try every candidate
+ * shape of the solve (the colour launches; where every world fits one
+ * workgroup, the per-world solve at 1, 2, 4 and 8 workgroups per compute unit
+ * with records in LDS or HBM) for tuning[3] steps each, timing the solve
+ * with events, then keep the fastest (tuning[3] reads 0 again once done).
+ * The getter returns the count written; the log of the last autotuning
+ * (each candidate's shape tuning[0..2] and its mean solve milliseconds per
+ * step) comes from pxh_scene_get_tuning_log, which returns the candidates'
+ * count. */
+enum { PXH_SOLVE_LAYOUT_AUTO = 0, PXH_SOLVE_LAYOUT_COLORS = 1, PXH_SOLVE_LAYOUT_WORLDS = 2 };
+#define PXH_WORLD_SOLVE_MAX_BODIES 1024
+int32_t pxh_scene_set_tuning(pxh_scene scene, const int32_t* tuning, int32_t n);
+int32_t pxh_scene_get_tuning(pxh_scene scene, int32_t* tuning, int32_t n);
+int32_t pxh_scene_get_tuning_log(pxh_scene scene, int32_t* shapes, double* ms, int32_t max);
+
 /* The scene's HIP stream (hipStream_t), for ordering caller work. */
 void* pxh_scene_get_stream(pxh_scene scene);
 
@@ -166,6 +216,19 @@ int32_t pxh_convex_hull(const float* pts, int32_t n, float* verts, float* planes
 int32_t pxh_scene_add_convex_meshes(pxh_scene scene, int32_t k, const int32_t* counts, const float* points,
                                     int32_t* first);
 
+/* Registers one convex mesh already cooked, from the arrays a PxConvexMesh
+ * gives (physx/include/geometry/PxConvexMesh.h:97-145): verts[nv*3]
+ * (getVertices), planes[np*4] and polys[np*2] = (mIndexBase, mNbVerts) of each
+ * PxHullPolygon (getPolygonData), indices (getIndexBuffer: counter-clockwise
+ * vertex loops), com[3] (getMassInformation's local centre of mass), so the
+ * step runs on the very hulls the reference cooked. The edges and the
+ * internal extents (the contact-generation margin) follow from them as in
+ * the reference cooker (GuCookingConvexMeshBuilder.cpp:420-531). *index
+ * receives the mesh index bodies name in dims[0]. At most 255 vertices and
+ * polygons, polygons of at most 16 vertices. */
+int32_t pxh_scene_add_cooked_convex(pxh_scene scene, int32_t nv, const float* verts, int32_t np, const float* planes,
+                                    const int32_t* polys, const uint8_t* indices, const float* com, int32_t* index);
+
 /* Removes whole worlds (every body and plane whose world id is listed in
  * worlds[k], host array) between steps: PxScene::removeActors
  * (physx/include/PxScene.h:437, NpScene.cpp:752) applied to all actors of
@@ -181,8 +244,10 @@ int32_t pxh_scene_world_range(pxh_scene scene, int32_t world, int32_t* first, in
 
 /* Serving loop over one batched scene (no counterpart in the reference, which
  * steps one scene per simulate call): independent step requests are queued,
- * admitted as worlds while max_worlds and the HBM budget (bytes_per_body per
- * body) allow, stepped together, and retired when their steps are done. */
+ * admitted as worlds while max_worlds and the HBM budget allow (each body
+ * charged bytes_per_body; no admission while the batched scene's measured
+ * device bytes exceed the budget, unless nothing is admitted), stepped
+ * together, and retired when their steps are done. */
 typedef struct pxh_server_t* pxh_server;
 int32_t pxh_server_create(const pxh_scene_desc* desc, int64_t hbm_budget_bytes, int32_t max_worlds,
                           int64_t bytes_per_body, pxh_server* out_server);
@@ -203,8 +268,9 @@ int32_t pxh_server_result(pxh_server server, int64_t id, float* pose, float* lin
                           double* times);
 /* Ids of finished scenes not yet read (ascending, at most max_ids); returns the count. */
 int32_t pxh_server_finished(pxh_server server, int64_t* ids, int32_t max_ids);
-/* out: queued scenes, admitted scenes, admitted bodies, admitted HBM bytes,
- * finished scenes, serving steps, body-steps. Returns the count written. */
+/* out: queued scenes, admitted scenes, admitted bodies, admitted HBM bytes
+ * (at the admission price), finished scenes, serving steps, body-steps, device
+ * bytes the batched scene held after the last step. Returns the count written. */
 int32_t pxh_server_stats(pxh_server server, int64_t* out, int32_t n);
 
 /* Standalone broadphase: PxCreateBroadPhase (physx/include/PxBroadPhase.h:629),
@@ -216,6 +282,20 @@ int32_t pxh_server_stats(pxh_server server, int64_t* out, int32_t n);
 int32_t pxh_bp_create(int32_t capacity, pxh_broadphase* out_bp);
 int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* groups, const float* distances,
                       int32_t n, void* stream);
+/* The same update with the caller's object handles and the reference's
+ * group filter: what a Bp::BroadPhase over persistent handles needs
+ * (lowlevelaabb/include/BpBroadPhase.h:98-206, PxBroadPhaseUpdateData,
+ * PxBroadPhase.h:351-420). ids[n] (device): the handle of each box, every
+ * one below id_bound; pairs are reported as handle pairs (id0 < id1), and
+ * created / deleted are diffed against the previous update in handle space.
+ * filter_lut: boxes of equal groups never pair; otherwise bit
+ * (type0 * 8 + type1) of filter_lut, type = group & 7, says whether the two
+ * may (Bp::groupFiltering, BpFiltering.h:99-109, with BpFilter's 7 x 7 table,
+ * BpFiltering.cpp); PXH_BP_FILTER_ALL lets any two different groups pair, as
+ * pxh_bp_update does. */
+#define PXH_BP_FILTER_ALL 0xffffffffffffffffull
+int32_t pxh_bp_update_handles(pxh_broadphase bp, const float* bounds, const uint32_t* groups, const float* distances,
+                              const uint32_t* ids, uint32_t id_bound, uint64_t filter_lut, int32_t n, void* stream);
 int32_t pxh_bp_fetch_results(pxh_broadphase bp, const uint32_t** created, int32_t* n_created,
                              const uint32_t** deleted, int32_t* n_deleted);
 int32_t pxh_bp_release(pxh_broadphase bp);

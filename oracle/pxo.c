@@ -2391,6 +2391,8 @@ typedef struct {
 	v3 *lv, *av;
 	float* invMass;
 	v3* invI;
+	/* accelerations of forces and torques set for the next step */
+	v3 *extLin, *extAng;
 	int m, mcap;
 	v3* pn;
 	float* pd;
@@ -2438,7 +2440,8 @@ void pxo_world_release(void* wp)
 	pxo_world* w = (pxo_world*)wp;
 	if (!w) return;
 	free(w->type); free(w->scene); free(w->local); free(w->dims); free(w->pose); free(w->lv); free(w->av);
-	free(w->invMass); free(w->invI); free(w->pn); free(w->pd); free(w->pscene); free(w->plocal); free(w->con);
+	free(w->invMass); free(w->invI); free(w->extLin); free(w->extAng); free(w->pn); free(w->pd); free(w->pscene);
+	free(w->plocal); free(w->con);
 	free(w->prev);
 	free(w);
 }
@@ -2457,15 +2460,22 @@ int pxo_world_add_bodies(void* wp, int n, const int32_t* type, const float* dims
 	w->av = (v3*)realloc(w->av, sizeof(v3) * nn);
 	w->invMass = (float*)realloc(w->invMass, sizeof(float) * nn);
 	w->invI = (v3*)realloc(w->invI, sizeof(v3) * nn);
+	w->extLin = (v3*)realloc(w->extLin, sizeof(v3) * nn);
+	w->extAng = (v3*)realloc(w->extAng, sizeof(v3) * nn);
+	/* index of each body within its world: bodies of that world added before it */
+	int maxScene = 0;
+	for (int j = 0; j < w->n; j++) if (w->scene[j] > maxScene) maxScene = w->scene[j];
+	for (int i = 0; i < n; i++) if (scene && scene[i] > maxScene) maxScene = scene[i];
+	int* count = (int*)calloc((size_t)maxScene + 1, sizeof(int));
+	for (int j = 0; j < w->n; j++) count[w->scene[j]]++;
 	for (int i = 0; i < n; i++)
 	{
 		const int k = w->n + i;
 		w->type[k] = type[i];
 		w->scene[k] = scene ? scene[i] : 0;
-		int loc = 0;
-		for (int j = 0; j < k; j++)
-			loc += w->scene[j] == w->scene[k];
-		w->local[k] = loc;
+		w->local[k] = count[w->scene[k]]++;
+		w->extLin[k] = V(0, 0, 0);
+		w->extAng[k] = V(0, 0, 0);
 		memcpy(w->dims + 4 * k, dims + 4 * i, 4 * sizeof(float));
 		w->pose[k] = load_pose(pose + 7 * i);
 		w->lv[k] = V(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]);
@@ -2475,6 +2485,7 @@ int pxo_world_add_bodies(void* wp, int n, const int32_t* type, const float* dims
 		               inertia[3 * i + 1] > 0.0f ? 1.0f / inertia[3 * i + 1] : 0.0f,
 		               inertia[3 * i + 2] > 0.0f ? 1.0f / inertia[3 * i + 2] : 0.0f);
 	}
+	free(count);
 	w->n = nn;
 	return nn;
 }
@@ -2526,6 +2537,28 @@ void pxo_world_set_bodies(void* wp, const float* pose, const float* lv, const fl
 		if (pose) w->pose[i] = load_pose(pose + 7 * i);
 		if (lv) w->lv[i] = V(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]);
 		if (av) w->av[i] = V(av[3 * i], av[3 * i + 1], av[3 * i + 2]);
+	}
+}
+
+/* PxRigidDynamic::addForce / addTorque with PxForceMode::eFORCE at the centre
+ * of mass for the next step (NpRigidBodyTemplate.h:507-528): force[n*3] and
+ * torque[n*3] (either may be NULL) become the accelerations F / m and
+ * I_world^-1 tau (R diag(invI) R^T, Cm::transformInertiaTensor, with the
+ * current orientation), replacing those set before. */
+void pxo_world_set_forces(void* wp, const float* force, const float* torque)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < w->n; i++)
+	{
+		if (force)
+			w->extLin[i] = scl(V(force[3 * i], force[3 * i + 1], force[3 * i + 2]), w->invMass[i]);
+		if (torque)
+		{
+			const m33 R = qmat(w->pose[i].q);
+			m33 A;
+			A.c0 = scl(R.c0, w->invI[i].x); A.c1 = scl(R.c1, w->invI[i].y); A.c2 = scl(R.c2, w->invI[i].z);
+			w->extAng[i] = mmul(mmm(A, mtrans(R)), V(torque[3 * i], torque[3 * i + 1], torque[3 * i + 2]));
+		}
 	}
 }
 
@@ -3293,6 +3326,13 @@ int pxo_world_step(void* wp, float dt)
 	for (int i = 0; i < n; i++)
 	{
 		v3 lv = w->lv[i], av = w->av[i];
+		/* forces and torques change the velocities before gravity
+		 * (BodySim::updateForces, ScBodySim.cpp:656-713) and last one step */
+		const v3 el = w->extLin[i], ea = w->extAng[i];
+		if (el.x != 0.0f || el.y != 0.0f || el.z != 0.0f) lv = add(lv, scl(el, dt));
+		if (ea.x != 0.0f || ea.y != 0.0f || ea.z != 0.0f) av = add(av, scl(ea, dt));
+		w->extLin[i] = V(0, 0, 0);
+		w->extAng[i] = V(0, 0, 0);
 		lv = add(lv, scl(scl(g, dt), 1.0f));
 		const float ld = 1.0f - d->linear_damping * dt, ad = 1.0f - d->angular_damping * dt;
 		lv = scl(lv, ld >= 0.0f ? ld : 0.0f);

@@ -83,6 +83,7 @@ int   pxo_world_add_planes(void* w, int m, const float* planes, const int32_t* s
 int   pxo_world_step(void* w, float dt);
 void  pxo_world_get_bodies(void* w, float* pose, float* lin_vel, float* ang_vel);
 void  pxo_world_set_bodies(void* w, const float* pose, const float* lin_vel, const float* ang_vel);
+void  pxo_world_set_forces(void* w, const float* force, const float* torque);
 /* out: [0] BP pairs, [1] body-plane pairs tested, [2] touching pairs (constraints),
  * [3] contact points, [4] colours, [5] friction anchors */
 void  pxo_world_stats(void* w, long long* out);

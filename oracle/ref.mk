@@ -69,7 +69,52 @@ $(OUT)/libpxref.so: oracle/ref_shim.cpp $(OUT)/libphysx_ref.a
 	  oracle/ref_shim.cpp -Wl,--whole-archive $(OUT)/libphysx_ref.a -Wl,--no-whole-archive \
 	  -lpthread -ldl -o $@
 
+# --- the SDK with its GPU support on, and the HIP GPU plug-in ---------------
+# The same sources compiled without DISABLE_CUDA_PHYSX (PX_SUPPORT_GPU_PHYSX)
+# and with PX_PHYSX_GPU_STATIC, the static plug-in mode in which the SDK calls
+# PxCreatePhysXGpu directly (physx/source/compiler/cmake/linux/PhysX.cmake:86-88,
+# LowLevel.cmake:41-45). The plug-in (physx_amd/plugin/pxh_physxgpu.cpp, over
+# libpxhip) becomes its own archive, and the test harness
+# oracle/ref_gpu_shim.cpp links both into oracle/_ref/gpu/libpxrefgpu.so.
+#
+#   make -f oracle/ref.mk gpu -j8      (after make -C physx_amd/csrc)
+GOUT     := $(OUT)/gpu
+GOBJ     := $(GOUT)/obj
+GCXXFLAGS := $(filter-out -DDISABLE_CUDA_PHYSX,$(CXXFLAGS)) -DPX_PHYSX_GPU_STATIC
+GOBJS    := $(addprefix $(GOBJ)/,$(ALLSRC:.cpp=.o))
+HIPINC   := -D__HIP_PLATFORM_AMD__ -I/opt/rocm/include -Iinclude
+
+$(GOBJ)/%.o: $(SRC)/%.cpp
+	@mkdir -p $(dir $@)
+	@$(CXX) $(GCXXFLAGS) $(INC) -c $< -o $@
+
+$(GOUT)/libphysx_refgpu.a: $(GOBJS)
+	@rm -f $@
+	@ar rcs $@ $(GOBJS)
+
+$(GOUT)/pxh_physxgpu.o: physx_amd/plugin/pxh_physxgpu.cpp physx_amd/plugin/pxh_hip_rt.h include/pxhip.h
+	@mkdir -p $(GOUT)
+	$(CXX) $(GCXXFLAGS) $(INC) -Iinclude -c $< -o $@
+
+# the plug-in's HIP runtime calls, compiled without any PhysX header
+$(GOUT)/pxh_hip_rt.o: physx_amd/plugin/pxh_hip_rt.cpp physx_amd/plugin/pxh_hip_rt.h
+	@mkdir -p $(GOUT)
+	$(CXX) -std=c++14 -O2 -fPIC -fno-rtti -Wall $(HIPINC) -c $< -o $@
+
+$(GOUT)/libPhysXGpuHip_static.a: $(GOUT)/pxh_physxgpu.o $(GOUT)/pxh_hip_rt.o
+	@rm -f $@
+	ar rcs $@ $^
+
+$(GOUT)/libpxrefgpu.so: oracle/ref_gpu_shim.cpp $(GOUT)/libphysx_refgpu.a $(GOUT)/libPhysXGpuHip_static.a \
+                        physx_amd/libpxhip.so
+	$(CXX) -std=c++14 -O2 -fPIC -shared -fno-rtti -DNDEBUG -DPX_PHYSX_STATIC_LIB -DPX_PHYSX_GPU_STATIC -w $(INC) \
+	  oracle/ref_gpu_shim.cpp -Wl,--whole-archive $(GOUT)/libphysx_refgpu.a $(GOUT)/libPhysXGpuHip_static.a \
+	  -Wl,--no-whole-archive -Lphysx_amd -lpxhip -Wl,-rpath,'$$ORIGIN/../../../physx_amd' \
+	  -L/opt/rocm/lib -lamdhip64 -lpthread -ldl -Wl,-Bsymbolic -Wl,--no-undefined -o $@
+
+gpu: $(GOUT)/libpxrefgpu.so
+
 clean:
 	rm -rf $(OUT)
 
-.PHONY: all clean
+.PHONY: all gpu clean

@@ -408,6 +408,20 @@ double pxref_scene_step(void* h, float dt, int nsteps, double* stepMs)
 	return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// PxRigidDynamic::addForce / addTorque (PxForceMode::eFORCE) on every body
+// for the next step; force / torque: n x 3 (either may be null)
+void pxref_scene_add_forces(void* h, const float* force, const float* torque)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (size_t i = 0; i < rs->bodies.size(); i++)
+	{
+		if (force)
+			rs->bodies[i]->addForce(PxVec3(force[3 * i], force[3 * i + 1], force[3 * i + 2]), PxForceMode::eFORCE);
+		if (torque)
+			rs->bodies[i]->addTorque(PxVec3(torque[3 * i], torque[3 * i + 1], torque[3 * i + 2]), PxForceMode::eFORCE);
+	}
+}
+
 // pose: n x 7, linVel/angVel: n x 3 (any may be null)
 void pxref_scene_get_bodies(void* h, float* pose, float* linVel, float* angVel)
 {

@@ -20,7 +20,10 @@ LIB_PATH = os.path.join(_HERE, "libpxhip.so")
 
 PXH_GEOM_SPHERE, PXH_GEOM_PLANE, PXH_GEOM_CAPSULE, PXH_GEOM_BOX, PXH_GEOM_CONVEXMESH = 0, 1, 2, 3, 5
 PXH_SOLVER_PGS, PXH_SOLVER_TGS = 0, 1
+PXH_SOLVE_LAYOUT_AUTO, PXH_SOLVE_LAYOUT_COLORS, PXH_SOLVE_LAYOUT_WORLDS = 0, 1, 2
 PXH_DATA_GLOBAL_POSE, PXH_DATA_LINEAR_VELOCITY, PXH_DATA_ANGULAR_VELOCITY = 0, 1, 2
+PXH_READ_LINEAR_ACCELERATION, PXH_READ_ANGULAR_ACCELERATION = 3, 4
+PXH_WRITE_FORCE, PXH_WRITE_TORQUE = 3, 4
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -42,7 +45,8 @@ class SceneDesc(ctypes.Structure):
                 ("angular_damping", ctypes.c_float), ("max_linear_velocity_sq", ctypes.c_float),
                 ("max_angular_velocity_sq", ctypes.c_float), ("bounce_threshold_velocity", ctypes.c_float),
                 ("friction_offset_threshold", ctypes.c_float), ("friction_correlation_distance", ctypes.c_float),
-                ("tolerance_length", ctypes.c_float), ("max_pairs_per_body", ctypes.c_int32)]
+                ("tolerance_length", ctypes.c_float), ("max_pairs_per_body", ctypes.c_int32),
+                ("enable_body_accelerations", ctypes.c_int32), ("report_touch_events", ctypes.c_int32)]
 
 
 EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_scene_create", "pxh_scene_release",
@@ -53,7 +57,9 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_get_profile", "pxh_scene_remove_worlds", "pxh_scene_world_count", "pxh_scene_world_range",
             "pxh_server_create", "pxh_server_release", "pxh_server_last_error", "pxh_server_submit",
             "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats", "pxh_convex_hull",
-            "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data")
+            "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
+            "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
+            "pxh_scene_add_cooked_convex")
 
 _lib = None
 
@@ -193,6 +199,30 @@ class Scene:
         # PxTransform is (q, p): reorder to p xyz + q xyzw
         return np.concatenate([p[:, 4:7], p[:, 0:4]], 1), lv.cpu().numpy(), av.cpu().numpy()
 
+    def set_forces(self, force=None, torque=None, index=None):
+        """Direct GPU API eFORCE / eTORQUE writes from host (k,3) arrays, for the
+        bodies in `index` (all bodies when None)."""
+        import torch
+        dev = torch.device("cuda")
+        idx = torch.from_numpy(_i32(index)).to(dev) if index is not None else None
+        for kind, arr in ((PXH_WRITE_FORCE, force), (PXH_WRITE_TORQUE, torque)):
+            if arr is not None:
+                t = torch.from_numpy(_f32(arr)).to(dev)
+                torch.cuda.synchronize()
+                self.set_rigid_dynamic_data(t, kind, index=idx, n=int(t.shape[0]))
+
+    def accelerations(self):
+        """Linear and angular accelerations of the last step (needs
+        enable_body_accelerations), host (n,3) arrays."""
+        import torch
+        dev = torch.device("cuda")
+        la = torch.empty((self.n, 3), dtype=torch.float32, device=dev)
+        aa = torch.empty((self.n, 3), dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        self.get_rigid_dynamic_data(la, PXH_READ_LINEAR_ACCELERATION)
+        self.get_rigid_dynamic_data(aa, PXH_READ_ANGULAR_ACCELERATION)
+        return la.cpu().numpy(), aa.cpu().numpy()
+
     def set_bodies(self, pose, lin_vel, ang_vel):
         import torch
         dev = torch.device("cuda")
@@ -203,11 +233,36 @@ class Scene:
         self.set_rigid_dynamic_data(torch.from_numpy(_f32(ang_vel)).to(dev), PXH_DATA_ANGULAR_VELOCITY)
 
     def stats(self) -> dict:
-        out = np.zeros(11, np.int64)
-        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 11))
+        out = np.zeros(12, np.int64)
+        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 12))
         keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps", "friction_rows",
-                "body_body_constraints", "color_rounds", "body_body_friction_rows")
+                "body_body_constraints", "color_rounds", "body_body_friction_rows", "device_bytes")
         return dict(zip(keys, map(int, out)))
+
+    def set_tuning(self, layout=None, world_workgroups=None, lds_bodies=None, autotune_steps=None):
+        """Launch shapes (pxh_scene_set_tuning); None keeps a setting. Results
+        do not depend on them. autotune_steps > 0 tries every candidate shape
+        for that many steps each and keeps the fastest."""
+        t = self.tuning()
+        for i, v in enumerate((layout, world_workgroups, lds_bodies, autotune_steps)):
+            if v is not None:
+                t[i] = int(v)
+        arr = np.array(t, np.int32)
+        _check(self.lib.pxh_scene_set_tuning(self.h, arr.ctypes.data_as(_I32), 4))
+
+    def tuning(self) -> list:
+        arr = np.zeros(4, np.int32)
+        _check(self.lib.pxh_scene_get_tuning(self.h, arr.ctypes.data_as(_I32), 4))
+        return [int(x) for x in arr]
+
+    def tuning_log(self) -> list:
+        """The last autotuning: [(layout, workgroups, lds bodies, solve ms per step)]."""
+        k = _check(self.lib.pxh_scene_get_tuning_log(self.h, None, None, 0))
+        shapes = np.zeros((max(k, 1), 3), np.int32)
+        ms = np.zeros(max(k, 1), np.float64)
+        self.lib.pxh_scene_get_tuning_log(self.h, shapes.ctypes.data_as(_I32),
+                                          ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), k)
+        return [(int(a), int(b), int(c), float(m)) for (a, b, c), m in zip(shapes[:k], ms[:k])]
 
     STAGES = ("bounds", "broadphase", "np_pairs", "np_planes", "prep", "solve", "integrate", "unused")
 
@@ -236,6 +291,18 @@ class Scene:
         _check(self.lib.pxh_scene_copy_contact_data(self.h, _dptr(buf), _dptr(nb), k, None))
         out = buf.cpu().numpy()[: int(nb.cpu().item())]
         return out
+
+    def touch_events(self):
+        """Touch found / lost pairs of the last step (needs report_touch_events):
+        two (k,2) int32 arrays of (body a, body b or -1 - plane)."""
+        cnt = np.zeros(2, np.int64)
+        _check(self.lib.pxh_scene_get_touch_events(self.h, None, 0, None, 0, cnt.ctypes.data_as(_I64)))
+        f = np.zeros((max(int(cnt[0]), 1), 2), np.int32)
+        lo = np.zeros((max(int(cnt[1]), 1), 2), np.int32)
+        _check(self.lib.pxh_scene_get_touch_events(self.h, f.ctypes.data_as(_I32), ctypes.c_int64(int(cnt[0])),
+                                                   lo.ctypes.data_as(_I32), ctypes.c_int64(int(cnt[1])),
+                                                   cnt.ctypes.data_as(_I64)))
+        return f[: cnt[0]], lo[: cnt[1]]
 
     def remove_worlds(self, worlds):
         """PxScene::removeActors for every actor of the listed worlds; the
@@ -301,7 +368,7 @@ class Server:
     admitted as worlds while max_worlds and the HBM budget allow, stepped
     together, and retired when their requested steps are done."""
 
-    STATS = ("queued", "active", "active_bodies", "active_bytes", "completed", "steps", "body_steps")
+    STATS = ("queued", "active", "active_bodies", "active_bytes", "completed", "steps", "body_steps", "device_bytes")
 
     def __init__(self, hbm_budget_bytes: int, max_worlds: int, bytes_per_body: int = 8192,
                  desc: SceneDesc | None = None, **kw):

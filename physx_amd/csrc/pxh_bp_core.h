@@ -51,11 +51,11 @@ inline int readCounts(hipStream_t st, uint32_t* host, const uint32_t* const* src
 // the write pass of the sweep (again after the pair buffer has grown)
 template <typename Ctx>
 int bp_write_pairs(Ctx* s, int n, const int* d_scene, const int* d_group, const uint32_t* d_rowOff, hipStream_t st,
-                   size_t pairCap)
+                   size_t pairCap, uint64_t groupLut = GROUP_LUT_ALL)
 {
 	PXH_CHECK(s->pairs.ensure(std::max<size_t>(pairCap, 1)));
-	launch_sweep(true, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, d_rowOff, s->rowStart.p,
-	             nullptr, s->counts.p, s->pairs.p, uint32_t(pairCap), st);
+	launch_sweep(true, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, groupLut, s->gridI.p, d_rowOff,
+	             s->rowStart.p, nullptr, s->counts.p, s->pairs.p, uint32_t(pairCap), st);
 	return PXH_OK;
 }
 
@@ -67,7 +67,7 @@ int bp_write_pairs(Ctx* s, int n, const int* d_scene, const int* d_group, const 
 // pairCap pairs are written.
 template <typename Ctx>
 int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_group, const uint32_t* d_rowOff,
-               uint32_t totalRows, hipStream_t st, size_t pairCap)
+               uint32_t totalRows, hipStream_t st, size_t pairCap, uint64_t groupLut = GROUP_LUT_ALL)
 {
 	PXH_CHECK(s->gridF.ensure(nWorlds));
 	PXH_CHECK(s->gridI.ensure(nWorlds));
@@ -96,23 +96,26 @@ int broadphase(Ctx* s, int n, int nWorlds, const int* d_scene, const int* d_grou
 	const int64_t nc = int64_t(PAIR_CLASSES) * n + 1;
 	PXH_CHECK(s->counts.ensure(nc));
 	PXH_CHECK(hipMemsetAsync(s->counts.p + PAIR_CLASSES * int64_t(n), 0, sizeof(uint32_t), st));
-	launch_sweep(false, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, s->gridI.p, d_rowOff, s->rowStart.p,
-	             s->counts.p, nullptr, nullptr, 0u, st);
+	launch_sweep(false, n, s->sortedKeys, s->smin.p, s->smax.p, d_scene, d_group, groupLut, s->gridI.p, d_rowOff,
+	             s->rowStart.p, s->counts.p, nullptr, nullptr, 0u, st);
 	launch_scan(s->counts.p, nc, s->scanTmp.p, st);
-	return bp_write_pairs(s, n, d_scene, d_group, d_rowOff, st, pairCap);
+	return bp_write_pairs(s, n, d_scene, d_group, d_rowOff, st, pairCap, groupLut);
 }
 
 // Standalone broadphase update (PxBroadPhase::update, PxBroadPhase.h:576) on a
 // context holding the previous update's sorted pair keys (pkPrev, nPrev):
-// inflates the n bounds, finds the overlapping pairs, sorts their keys into
-// pkCur (nCur) and writes created = cur \ prev and deleted = prev \ cur as
-// sorted (id0, id1) pairs (nCreated, nDeleted). hostCounts: at least two
-// host-visible uint32. growPairs: whether the pair buffer may grow past
-// pairCap when the pairs outnumber it; when it may not, the update stops with
-// PXH_ERR_CAPACITY and *required set to the pair count.
+// inflates the n bounds, finds the overlapping pairs (groups filtered by
+// groupLut), sorts their keys into pkCur (nCur) and writes created = cur \ prev
+// and deleted = prev \ cur as sorted (id0, id1) pairs (nCreated, nDeleted),
+// ids being the boxes' handles (ids, device, every id below idBound) or their
+// indices (ids null). hostCounts: at least two host-visible uint32.
+// growPairs: whether the pair buffer may grow past pairCap when the pairs
+// outnumber it; when it may not, the update stops with PXH_ERR_CAPACITY and
+// *required set to the pair count.
 template <typename Ctx>
 int bp_update(Ctx* bp, const float* bounds, const uint32_t* groups, const float* distances, int n, hipStream_t st,
-              uint32_t* hostCounts, bool growPairs, int* required)
+              uint32_t* hostCounts, bool growPairs, int* required, const uint32_t* ids = nullptr,
+              uint32_t idBound = 0, uint64_t groupLut = GROUP_LUT_ALL)
 {
 	const size_t nn = std::max(n, 1);
 	PXH_CHECK(bp->bmin.ensure(nn));
@@ -131,7 +134,7 @@ int bp_update(Ctx* bp, const float* bounds, const uint32_t* groups, const float*
 		launch_row_off_single(bp->rowOff.p, rows, st);
 		if (growPairs)
 			bp->pairCap = std::max(bp->pairCap, size_t(4) * n);
-		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->rowOff.p, rows, st, bp->pairCap);
+		int r = broadphase(bp, n, 1, bp->scene.p, bp->group.p, bp->rowOff.p, rows, st, bp->pairCap, groupLut);
 		if (r)
 			return r;
 		const uint32_t* src[1] = {bp->counts.p + PAIR_CLASSES * int64_t(n)};
@@ -150,7 +153,7 @@ int bp_update(Ctx* bp, const float* bounds, const uint32_t* groups, const float*
 				return PXH_ERR_CAPACITY;
 			}
 			bp->pairCap = size_t(P) + size_t(P) / 4;
-			r = bp_write_pairs(bp, n, bp->scene.p, bp->group.p, bp->rowOff.p, st, bp->pairCap);
+			r = bp_write_pairs(bp, n, bp->scene.p, bp->group.p, bp->rowOff.p, st, bp->pairCap, groupLut);
 			if (r)
 				return r;
 		}
@@ -162,7 +165,7 @@ int bp_update(Ctx* bp, const float* bounds, const uint32_t* groups, const float*
 	PXH_CHECK(bp->pkTmp.ensure(std::max(P, 1)));
 	PXH_CHECK(bp->vals.ensure(std::max(P, 1)));
 	PXH_CHECK(bp->vals2.ensure(std::max(P, 1)));
-	launch_pair_keys(P, bp->pairs.p, bp->pkCur.p, bp->vals.p, st);
+	launch_pair_keys(P, bp->pairs.p, ids, bp->pkCur.p, bp->vals.p, st);
 	const int nblocks = (P + radix_tile() - 1) / radix_tile();
 	PXH_CHECK(bp->hist.ensure(size_t(256) * std::max(nblocks, 1)));
 	PXH_CHECK(bp->scanTmp.ensure(
@@ -170,7 +173,7 @@ int bp_update(Ctx* bp, const float* bounds, const uint32_t* groups, const float*
 	                                    int64_t(bp->nCur) + 1)) +
 	    1024));
 	bool alt = false;
-	const int bits = 32 + ((bitsFor(uint64_t(std::max(n, 1))) + 7) / 8) * 8;
+	const int bits = 32 + ((bitsFor(uint64_t(ids ? std::max<uint32_t>(idBound, 1u) : uint32_t(std::max(n, 1)))) + 7) / 8) * 8;
 	launch_radix_sort(bp->pkCur.p, bp->vals.p, bp->pkTmp.p, bp->vals2.p, P, 0, bits, bp->hist.p, bp->scanTmp.p, st,
 	                  alt);
 	if (alt)

@@ -106,6 +106,99 @@ void faceNormal(const float* pts, FaceLoop& f)
 
 }  // namespace
 
+void hull_edges_bounds(Hull& out)
+{
+	// edges in order of first appearance around the polygons: v0 -> v1 runs
+	// counter-clockwise in the face on the left, the reverse in the other
+	out.edgeVerts.clear();
+	out.edgeFaces.clear();
+	std::map<std::pair<int, int>, int> edgeIndex;
+	for (size_t p = 0; p < out.polys.size(); p++)
+	{
+		const HullPolygon& hp = out.polys[p];
+		for (int k = 0; k < hp.count; k++)
+		{
+			const int a = out.vref[hp.first + k], b = out.vref[hp.first + (k + 1) % hp.count];
+			auto it = edgeIndex.find({b, a});
+			if (it != edgeIndex.end())
+				out.edgeFaces[2 * it->second + 1] = uint8_t(p);
+			else
+			{
+				edgeIndex[{a, b}] = int(out.edgeVerts.size() / 2);
+				out.edgeVerts.push_back(uint8_t(a));
+				out.edgeVerts.push_back(uint8_t(b));
+				out.edgeFaces.push_back(uint8_t(p));
+				out.edgeFaces.push_back(uint8_t(p));
+			}
+		}
+	}
+	// local bounds from the float vertices
+	float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+	for (size_t k = 0; k < out.verts.size() / 3; k++)
+		for (int c = 0; c < 3; c++)
+		{
+			mn[c] = std::min(mn[c], out.verts[3 * k + c]);
+			mx[c] = std::max(mx[c], out.verts[3 * k + c]);
+		}
+	for (int c = 0; c < 3; c++)
+	{
+		out.aabbCenter[c] = (mx[c] + mn[c]) * 0.5f;
+		out.aabbExtents[c] = (mx[c] - mn[c]) * 0.5f;
+	}
+}
+
+void hull_internal_objects(Hull& out, const float com[3])
+{
+	// internal objects: the largest sphere about the centre of mass inside the
+	// hull, and a box of that radius grown along the largest bounds axis, then
+	// along the other two (ConvexMeshBuilder::computeInternalObjects,
+	// geomutils/src/cooking/GuCookingConvexMeshBuilder.cpp:420-531)
+	float ir = FLT_MAX;
+	for (const HullPolygon& hp : out.polys)
+	{
+		const float d = std::fabs(hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2] + hp.plane[3]);
+		ir = std::min(ir, d);
+	}
+	out.internalRadius = ir;
+	const float e[3] = {2.0f * out.aabbExtents[0], 2.0f * out.aabbExtents[1], 2.0f * out.aabbExtents[2]};
+	const int L = (e[0] >= e[1] && e[0] >= e[2]) ? 0 : (e[1] >= e[2] ? 1 : 2);
+	int e0 = (L + 1) % 3, e1 = (e0 + 1) % 3;
+	if (e[e0] < e[e1])
+		std::swap(e0, e1);
+	const float r = ir / std::sqrt(3.0f);
+	const float tiny = 1e-7f;
+	float ext[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+	for (const HullPolygon& hp : out.polys)
+	{
+		const float dl = hp.plane[L];
+		if (-tiny < dl && dl < tiny)
+			continue;
+		const float base = -hp.plane[3] - (hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2]);
+		const float inv = 1.0f / dl;
+		const float a0 = r * hp.plane[e0], a1 = r * hp.plane[e1];
+		const float nums[4] = {base - a0 - a1, base - a0 + a1, base + a0 + a1, base + a0 - a1};
+		for (float num : nums)
+			ext[L] = std::min(ext[L], std::max(std::fabs(num * inv), r));
+	}
+	for (const HullPolygon& hp : out.polys)
+	{
+		const float add = hp.plane[e0] + hp.plane[e1];
+		const float sub = hp.plane[e0] - hp.plane[e1];
+		const float base = -hp.plane[3] - (hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2]);
+		const float b0 = ext[L] * hp.plane[L];
+		for (float den : {add, sub})
+		{
+			if (-tiny < den && den < tiny)
+				continue;
+			ext[e0] = std::min(ext[e0], std::max(std::fabs((base - b0) / den), r));
+			ext[e0] = std::min(ext[e0], std::max(std::fabs((base + b0) / den), r));
+		}
+	}
+	ext[e1] = ext[e0];
+	for (int c = 0; c < 3; c++)
+		out.internalExtents[c] = ext[c];
+}
+
 int build_hull(const float* pts, int n, Hull& out)
 {
 	out = Hull();
@@ -363,7 +456,6 @@ int build_hull(const float* pts, int n, Hull& out)
 	for (size_t k = 0; k < order.size(); k++)
 		for (int c = 0; c < 3; c++)
 			out.verts[3 * k + c] = pts[3 * order[k] + c];
-	std::map<std::pair<int, int>, int> edgeIndex;
 	for (size_t p = 0; p < loops.size(); p++)
 	{
 		const auto& loop = loops[p];
@@ -377,36 +469,8 @@ int build_hull(const float* pts, int n, Hull& out)
 		for (int v : loop)
 			out.vref.push_back(uint8_t(local[v]));
 		out.polys.push_back(hp);
-		for (size_t k = 0; k < loop.size(); k++)
-		{
-			const int a = local[loop[k]], b = local[loop[(k + 1) % loop.size()]];
-			auto it = edgeIndex.find({b, a});
-			if (it != edgeIndex.end())
-				out.edgeFaces[2 * it->second + 1] = uint8_t(p);
-			else
-			{
-				edgeIndex[{a, b}] = int(out.edgeVerts.size() / 2);
-				out.edgeVerts.push_back(uint8_t(a));
-				out.edgeVerts.push_back(uint8_t(b));
-				out.edgeFaces.push_back(uint8_t(p));
-				out.edgeFaces.push_back(uint8_t(p));
-			}
-		}
 	}
-
-	// local bounds from the float vertices
-	float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-	for (size_t k = 0; k < order.size(); k++)
-		for (int c = 0; c < 3; c++)
-		{
-			mn[c] = std::min(mn[c], out.verts[3 * k + c]);
-			mx[c] = std::max(mx[c], out.verts[3 * k + c]);
-		}
-	for (int c = 0; c < 3; c++)
-	{
-		out.aabbCenter[c] = (mx[c] + mn[c]) * 0.5f;
-		out.aabbExtents[c] = (mx[c] - mn[c]) * 0.5f;
-	}
+	hull_edges_bounds(out);
 
 	// mass properties at unit density of the hull of the points: tetrahedra
 	// (origin, a, b, c) over the hull's triangles; second moments through the
@@ -444,55 +508,8 @@ int build_hull(const float* pts, int n, Hull& out)
 		for (int q = 0; q < 3; q++)
 			out.inertia[3 * r + q] = (r == q ? tr : 0.0) - Cc[3 * r + q];
 
-	// internal objects: the largest sphere about the centre of mass inside the
-	// hull, and a box of that radius grown along the largest bounds axis, then
-	// along the other two (ConvexMeshBuilder::computeInternalObjects,
-	// geomutils/src/cooking/GuCookingConvexMeshBuilder.cpp:420-531)
 	const float com[3] = {float(out.com[0]), float(out.com[1]), float(out.com[2])};
-	float ir = FLT_MAX;
-	for (const HullPolygon& hp : out.polys)
-	{
-		const float d = std::fabs(hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2] + hp.plane[3]);
-		ir = std::min(ir, d);
-	}
-	out.internalRadius = ir;
-	const float e[3] = {mx[0] - mn[0], mx[1] - mn[1], mx[2] - mn[2]};
-	const int L = (e[0] >= e[1] && e[0] >= e[2]) ? 0 : (e[1] >= e[2] ? 1 : 2);
-	int e0 = (L + 1) % 3, e1 = (e0 + 1) % 3;
-	if (e[e0] < e[e1])
-		std::swap(e0, e1);
-	const float r = ir / std::sqrt(3.0f);
-	const float tiny = 1e-7f;
-	float ext[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
-	for (const HullPolygon& hp : out.polys)
-	{
-		const float dl = hp.plane[L];
-		if (-tiny < dl && dl < tiny)
-			continue;
-		const float base = -hp.plane[3] - (hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2]);
-		const float inv = 1.0f / dl;
-		const float a0 = r * hp.plane[e0], a1 = r * hp.plane[e1];
-		const float nums[4] = {base - a0 - a1, base - a0 + a1, base + a0 + a1, base + a0 - a1};
-		for (float num : nums)
-			ext[L] = std::min(ext[L], std::max(std::fabs(num * inv), r));
-	}
-	for (const HullPolygon& hp : out.polys)
-	{
-		const float add = hp.plane[e0] + hp.plane[e1];
-		const float sub = hp.plane[e0] - hp.plane[e1];
-		const float base = -hp.plane[3] - (hp.plane[0] * com[0] + hp.plane[1] * com[1] + hp.plane[2] * com[2]);
-		const float b0 = ext[L] * hp.plane[L];
-		for (float den : {add, sub})
-		{
-			if (-tiny < den && den < tiny)
-				continue;
-			ext[e0] = std::min(ext[e0], std::max(std::fabs((base - b0) / den), r));
-			ext[e0] = std::min(ext[e0], std::max(std::fabs((base + b0) / den), r));
-		}
-	}
-	ext[e1] = ext[e0];
-	for (int c = 0; c < 3; c++)
-		out.internalExtents[c] = ext[c];
+	hull_internal_objects(out, com);
 	return 0;
 }
 

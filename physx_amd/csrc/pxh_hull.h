@@ -35,6 +35,15 @@ struct Hull
 // vertices or polygons.
 int build_hull(const float* pts, int n, Hull& out);
 
+// Edges (with their two faces) and local bounds of a hull whose verts, polys
+// and vref are set.
+void hull_edges_bounds(Hull& out);
+
+// Internal radius and extents about the centre of mass com of a hull whose
+// polygons and bounds are set (ConvexMeshBuilder::computeInternalObjects,
+// geomutils/src/cooking/GuCookingConvexMeshBuilder.cpp:420-531).
+void hull_internal_objects(Hull& out, const float com[3]);
+
 // PCM margin of a hull: a quarter of the smallest internal extent, capped at
 // 5% of the tolerance length.
 float hull_margin(const Hull& h, float toleranceLength);

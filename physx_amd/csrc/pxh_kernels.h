@@ -105,10 +105,14 @@ void launch_keys(int n, const float4* bmin, const float4* bmax, const int* scene
 void launch_gather_sorted(int n, const uint32_t* sv, const float4* bmin, const float4* bmax, float4* smin,
                           float4* smax, hipStream_t st);
 void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t totalRows, hipStream_t st);
-// write pass: pairs beyond pairCap are counted but not written
+// write pass: pairs beyond pairCap are counted but not written; group (may be
+// null: no filter) with groupLut, the filter table of BpFilter as a bit mask
+// (bit type0 * 8 + type1, type = group & 7), equal groups never pairing
+constexpr uint64_t GROUP_LUT_ALL = ~0ull;
 void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
-                  const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap, hipStream_t st);
+                  const int* group, uint64_t groupLut, const int2* gridI, const uint32_t* rowOff,
+                  const uint32_t* rowStart, uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap,
+                  hipStream_t st);
 // pair p < min(cap, *dP) fills slot slotBase + p; sphere/box pairs are those
 // below *dConvex (the first convex pair), convex pairs run in launch_np_convex
 void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
@@ -149,6 +153,7 @@ void launch_color_uncolored(int m, const uint32_t* mDev, const int* list, const 
 void launch_color_compact(int m, const int* list, const uint32_t* flags, const uint32_t* scanned, int* out,
                           hipStream_t st);
 void launch_fill_u64(uint64_t* p, int n, uint64_t v, hipStream_t st);
+void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st);
 void launch_color_keys(int C, const int* conColor, uint32_t limit, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_static_colors(int n, int C, const int* conA, const int* conB, int* conColor, uint32_t* bodyNext,
                           hipStream_t st);
@@ -158,9 +163,13 @@ void launch_patch_table(int m, const uint64_t* keys, uint64_t* htKey, int* htVal
 void launch_con_prev(int C, const int* conA, const int* conB, const int* conSlot, PatchIO io, int* conColor,
                      uint64_t* mask, hipStream_t st);
 void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_t st);
+// extLin / extAng (may be null: none set): accelerations set through the
+// Direct GPU API, applied for this step and cleared; prevLin / prevAng (may be
+// null) receive the velocities before the step
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                         float maxAngSq, bool pgs, SolverBodies sb, hipStream_t st);
+                         float maxAngSq, bool pgs, SolverBodies sb, float dt, float4* extLin, float4* extAng,
+                         float4* prevLin, float4* prevAng, hipStream_t st);
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
@@ -170,6 +179,19 @@ void launch_prep(int C, const int* order, const int* conA, const int* conB, cons
 void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float minPen, float muS, float muD,
                   float invStepDt, float biasScale, float scale, uint32_t* broken, hipStream_t st,
                   hipEvent_t evStart = nullptr, hipEvent_t evStop = nullptr);
+// rows in (world, colour) order for the per-world solve
+void launch_color_keys_world(int C, const int* conColor, const int* conA, const int* bodyWorld, uint32_t limit,
+                             uint64_t* keys, uint32_t* vals, hipStream_t st);
+int solve_world_threads();
+// every TGS iteration of every world in one launch of `grid` persistent
+// workgroups; queueHead: one device counter (zeroed by the launch);
+// segStart: nWorlds * CB + 1 row starts of (world, colour); worlds of at most
+// ldsBodies bodies keep their velocity records in LDS
+void launch_solve_worlds(int grid, int nWorlds, const int* worldOrder, uint32_t* queueHead, const int* bodyStart,
+                         const uint32_t* segStart, int CB, int C, Rows rows, SolverBodies sb, int posIters,
+                         int velIters, float stepDt, float muS, float muD, float invStepDt, float biasScale,
+                         float scale, uint32_t* broken, int ldsBodies, hipStream_t st, hipEvent_t evStart = nullptr,
+                         hipEvent_t evStop = nullptr);
 // PGS: rows with the unbiased normal errors in rowsU; position passes
 // (velocityPass false) then velocity passes; SolverBodies p/dq hold the motion
 // velocities between the two
@@ -186,11 +208,18 @@ void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin,
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
 void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
-                     const float4* ang, float* out, hipStream_t st);
+                     const float4* ang, const float4* prevLin, const float4* prevAng, float invDt, float* out,
+                     hipStream_t st);
 void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                     const float* in, hipStream_t st);
-void launch_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals, hipStream_t st);
+                     const float4* invI, float4* extLin, float4* extAng, const float* in, hipStream_t st);
+// keys of pairs mapped through ids (device, may be null: box indices)
+void launch_pair_keys(int P, const uint2* pairs, const uint32_t* ids, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_set_diff(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag, hipStream_t st);
+// touch events: found[c] for the C constraints without a previous row
+// (prevRow null: all), lost[r] for the R previous rows none of them found;
+// matched: R + 1 scratch flags
+void launch_touch_events(int C, const int* prevRow, int R, uint32_t* found, uint32_t* matched, uint32_t* lost,
+                         hipStream_t st);
 void launch_compact_keys(int na, const uint64_t* a, const uint32_t* flag, const uint32_t* scanned, uint2* out,
                          hipStream_t st);
 

@@ -453,10 +453,20 @@ PXH_D int pairClass(int t0, int t1)
 // inclusive test on inflated bounds (BpBroadPhaseABP.cpp:2246-2262, :2474).
 // WRITE=false counts pairs per class into counts[cls*n + s]; WRITE=true writes
 // them at the scanned offsets.
+// Broadphase group filter (Bp::groupFiltering, lowlevelaabb/include/BpFiltering.h:99-109):
+// boxes of equal groups never pair; otherwise bit (type0 * 8 + type1) of lut,
+// type = group & 7 (BpFilter's 7 x 7 table, lowlevelaabb/src/BpFiltering.cpp,
+// in an 8 x 8 bit mask; all bits set: only equal groups are filtered)
+PXH_D bool groupsPair(int g0, int g1, uint64_t lut)
+{
+	return g0 != g1 && ((lut >> ((g0 & 7) * 8 + (g1 & 7))) & 1ull);
+}
+
 template <bool WRITE>
 __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, const float4* smax, const int* scene,
-                        const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                        uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap)
+                        const int* group, uint64_t groupLut, const int2* gridI, const uint32_t* rowOff,
+                        const uint32_t* rowStart, uint32_t* counts, const uint32_t* offsets, uint2* pairs,
+                        uint32_t pairCap)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s >= n)
@@ -518,7 +528,7 @@ __global__ void k_sweep(int n, const uint64_t* sortedKeys, const float4* smin, c
 				if (b1.y < a0.y || a1.y < b0.y || b1.z < a0.z || a1.z < b0.z)
 					continue;
 				const uint32_t jj = __float_as_uint(b1.w);
-				if (group && group[jj] == gi_)
+				if (group && !groupsPair(gi_, group[jj], groupLut))
 					continue;
 				const int tj = int(b0.w);
 				const int c = pairClass(ti, tj);
@@ -1122,6 +1132,20 @@ __global__ void k_color_keys(int C, const int* conColor, uint32_t limit, uint64_
 		vals[c] = uint32_t(c);
 	}
 }
+// rows ordered by (world, colour): key world * CB + colour, CB = limit + 1
+__global__ void k_color_keys_world(int C, const int* conColor, const int* conA, const int* bodyWorld,
+                                   uint32_t limit, uint64_t* keys, uint32_t* vals)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c < C)
+	{
+		const int col = conColor[c];
+		const uint32_t k = col < 0 ? limit : min(uint32_t(col), limit);
+		const uint64_t w = uint64_t(bodyWorld[conA[c]]);
+		keys[c] = ((w * (limit + 1) + k) << 32) | uint32_t(c);
+		vals[c] = uint32_t(c);
+	}
+}
 
 // ---------------------------------------------------------------------------
 // solver
@@ -1132,14 +1156,36 @@ __global__ void k_color_keys(int C, const int* conColor, uint32_t limit, uint64_
 __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                                const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                                float maxAngSq, int pgs, float4* vel, float4* svP, float4* svDQ, float4* svI0,
-                               float4* svI1, float4* svOrigLin, float4* rec)
+                               float4* svI1, float4* svOrigLin, float4* rec, float dt, float4* extLin, float4* extAng,
+                               float4* prevLin, float4* prevAng)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
 	const float4 l = lin[i];
-	f3 lv = mk(l) + mk(gdt.x, gdt.y, gdt.z);
+	f3 lv = mk(l);
 	f3 av = mk(ang[i]);
+	// velocities before the step, for the acceleration reads
+	if (prevLin)
+	{
+		prevLin[i] = f4(lv, 0.0f);
+		prevAng[i] = f4(av, 0.0f);
+	}
+	// external accelerations (forces and torques set through the Direct GPU
+	// API) change the velocities first, then gravity (BodySim::updateForces,
+	// simulationcontroller/src/ScBodySim.cpp:656-713, before
+	// bodyCoreComputeUnconstrainedVelocity); they last one step
+	if (extLin)
+	{
+		const float4 el = extLin[i], ea = extAng[i];
+		if (el.x != 0.0f || el.y != 0.0f || el.z != 0.0f)
+			lv = lv + mk(el) * dt;
+		if (ea.x != 0.0f || ea.y != 0.0f || ea.z != 0.0f)
+			av = av + mk(ea) * dt;
+		extLin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		extAng[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
+	lv = lv + mk(gdt.x, gdt.y, gdt.z);
 	const float ld = 1.0f - linDampDt, ad = 1.0f - angDampDt;
 	lv = lv * (ld >= 0.0f ? ld : 0.0f);
 	av = av * (ad >= 0.0f ? ad : 0.0f);
@@ -1512,18 +1558,20 @@ PXH_D void prepBody(int o, int C, const int* order, const float4* ctN, const flo
 // All rows of the lane are loaded before any arithmetic so a wave keeps every
 // gather in flight at once; 64-lane blocks spread a colour over all CUs.
 constexpr int SOLVE_THREADS = 64;
-__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* vel,
-                                                          float minPen, float muS, float muD, float invStepDt,
-                                                          float biasScale, float scale, uint32_t* broken)
+constexpr int SOLVE_WORLD_THREADS = 64;
+
+// One TGS contact row o. Body velocity records are at vel[4 * (index -
+// bodyBase)] (bodyBase 0: the scene-wide array; a world's first body: that
+// world's records staged in LDS); the arithmetic is the same either way.
+__device__ __forceinline__ void solveContactRowTGS(int o, int C, const Rows& rows, float4* vel, int bodyBase,
+                                                   float minPen, float muS, float muD, float invStepDt,
+                                                   float biasScale, float scale, uint32_t* broken)
 {
-	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
-	if (o >= end)
-		return;
 	const int4 hi = rows.hI[o];
 	const float4 hf = rows.hF[o];
-	const int a = hi.x, b = hi.y;
+	const int a = hi.x - bodyBase, b = hi.y < 0 ? hi.y : hi.y - bodyBase;
 	const int cnt = hi.z & 0xFF, nF = hi.z >> 8;
-	const bool dynB = b >= 0;
+	const bool dynB = hi.y >= 0;
 	// gathers: both bodies
 	const float4* va = vel + 4 * int64_t(a);
 	const float4 la4 = va[0], aa4 = va[1], dla4 = va[2], am04 = va[3];
@@ -1646,6 +1694,16 @@ __global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int
 		vel[4 * int64_t(b)] = f4(l1, invMassB);
 		vel[4 * int64_t(b) + 1] = f4(a1, 0.0f);
 	}
+}
+
+__global__ void __launch_bounds__(SOLVE_THREADS) k_solve(int begin, int end, int C, Rows rows, float4* vel,
+                                                          float minPen, float muS, float muD, float invStepDt,
+                                                          float biasScale, float scale, uint32_t* broken)
+{
+	const int o = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (o >= end)
+		return;
+	solveContactRowTGS(o, C, rows, vel, 0, minPen, muS, muD, invStepDt, biasScale, scale, broken);
 }
 
 // ---------------------------------------------------------------------------
@@ -1944,14 +2002,11 @@ __global__ void k_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4
 	ang[i] = f4(wD + mmul(S, mk(v[1])), 0.0f);
 }
 
-// integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476)
-__global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* svDQ, const float4* svI0,
-                            const float4* svI1)
+// integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476) of
+// body i whose velocity record is at v (scene-wide array or LDS)
+__device__ __forceinline__ void integrateBodyTGS(int i, float dt, float4* v, float4* svP, float4* svDQ,
+                                                 const float4* svI0, const float4* svI1)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
-	float4* v = vel + 4 * int64_t(i);
 	const f3 lv = mk(v[0]);
 	const f3 av = mk(v[1]);
 	const f3 delta = lv * dt;
@@ -1961,9 +2016,9 @@ __global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* s
 	if (w2 != 0.0f)
 	{
 		const float w = sqrtf(w2);
-		const float v = dt * w * 0.5f;
+		const float half = dt * w * 0.5f;
 		float s, q;
-		sincosf(v, &s, &q);
+		sincosf(half, &s, &q);
 		s /= w;
 		const f3 pqr = amv * s;
 		qt qv; qv.x = pqr.x; qv.y = pqr.y; qv.z = pqr.z; qv.w = 0.0f;
@@ -1974,6 +2029,88 @@ __global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* s
 	}
 	v[3] = f4(mk(v[3]) + av * dt, 0.0f);
 	v[2] = f4(mk(v[2]) + delta, 0.0f);
+}
+
+__global__ void k_integrate(int n, float dt, float4* vel, float4* svP, float4* svDQ, const float4* svI0,
+                            const float4* svI1)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	integrateBodyTGS(i, dt, vel + 4 * int64_t(i), svP, svDQ, svI0, svI1);
+}
+
+// The whole TGS iteration loop of independent worlds (the batched form).
+// Persistent workgroups take worlds from a queue ordered by size (largest
+// first); a workgroup stages its world's velocity records in LDS (worlds of
+// at most ldsBodies bodies; larger ones work on the scene-wide array), then
+// for every position and velocity iteration solves the world's colours in
+// order, one barrier between colours, and after each position iteration
+// integrates the world's bodies, as the per-colour launches and k_integrate
+// do for all worlds at once. Rows are ordered by (world, colour):
+// segStart[w * CB + c] is the first row of colour c of world w. Every wave
+// leaves once the queue is empty. Same arithmetic in the same order as the
+// per-colour path (cf. solveWholeIslandTGS, gpusolver/src/CUDA/
+// solverMultiBlockTGS.cu:1686, which runs a small island in one block).
+__global__ void __launch_bounds__(SOLVE_WORLD_THREADS) k_solve_worlds(
+    int nWorlds, const int* worldOrder, uint32_t* queueHead, const int* bodyStart, const uint32_t* segStart, int CB,
+    int C, Rows rows, float4* vel, float4* svP, float4* svDQ, const float4* svI0, const float4* svI1, int posIters,
+    int velIters, float stepDt, float muS, float muD, float invStepDt, float biasScale, float scale, uint32_t* broken,
+    int ldsBodies)
+{
+	extern __shared__ float4 lds[];
+	__shared__ int sWorld;
+	const int tid = threadIdx.x;
+	for (;;)
+	{
+		if (tid == 0)
+		{
+			const uint32_t q = atomicAdd(queueHead, 1u);
+			sWorld = q < uint32_t(nWorlds) ? worldOrder[q] : -1;
+		}
+		__syncthreads();
+		const int w = sWorld;
+		__syncthreads();
+		if (w < 0)
+			return;
+		const int b0 = bodyStart[w], nb = bodyStart[w + 1] - b0;
+		const bool inLds = nb <= ldsBodies;
+		float4* V = inLds ? lds : vel + 4 * int64_t(b0);
+		if (inLds)
+			for (int i = tid; i < 4 * nb; i += SOLVE_WORLD_THREADS)
+				lds[i] = vel[4 * int64_t(b0) + i];
+		__syncthreads();
+		const uint32_t* seg = segStart + int64_t(w) * CB;
+		const uint32_t wEnd = seg[CB];
+		for (int it = 0; it < posIters + velIters; it++)
+		{
+			const bool posIter = it < posIters;
+			const float minPen = posIter ? -3.402823466e+38f : 0.0f;
+			uint32_t* brk = it == posIters + velIters - 1 ? broken : nullptr;
+			for (int c = 0; c < CB; c++)
+			{
+				const uint32_t s = seg[c];
+				if (s >= wEnd)
+					break;
+				const uint32_t e = seg[c + 1];
+				if (e == s)
+					continue;
+				for (uint32_t o = s + tid; o < e; o += SOLVE_WORLD_THREADS)
+					solveContactRowTGS(int(o), C, rows, V, b0, minPen, muS, muD, invStepDt, biasScale, scale, brk);
+				__syncthreads();
+			}
+			if (posIter)
+			{
+				for (int i = tid; i < nb; i += SOLVE_WORLD_THREADS)
+					integrateBodyTGS(b0 + i, stepDt, V + 4 * int64_t(i), svP, svDQ, svI0, svI1);
+				__syncthreads();
+			}
+		}
+		if (inLds)
+			for (int i = tid; i < 4 * nb; i += SOLVE_WORLD_THREADS)
+				vel[4 * int64_t(b0) + i] = lds[i];
+		__syncthreads();
+	}
 }
 
 // copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580)
@@ -1992,8 +2129,13 @@ __global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4*
 // ---------------------------------------------------------------------------
 // Direct GPU API gather/scatter (updateBodiesAndShapes.cu:999-1253)
 // ---------------------------------------------------------------------------
+// what: PxRigidDynamicGPUAPIReadType (0 pose, 1 linear velocity, 2 angular
+// velocity, 3 linear acceleration, 4 angular acceleration: the velocity change
+// of the last step over its dt, getRigidDynamicLinearAcceleration,
+// gpusimulationcontroller/src/CUDA/updateBodiesAndShapes.cu:1063-1106)
 __global__ void k_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot,
-                           const float4* lin, const float4* ang, float* out)
+                           const float4* lin, const float4* ang, const float4* prevLin, const float4* prevAng,
+                           float invDt, float* out)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
@@ -2005,16 +2147,27 @@ __global__ void k_get_data(int n, const uint32_t* idx, int what, const float4* p
 		float* o = out + 7 * int64_t(k);
 		o[0] = q.x; o[1] = q.y; o[2] = q.z; o[3] = q.w; o[4] = p.x; o[5] = p.y; o[6] = p.z;
 	}
-	else
+	else if (what <= 2)
 	{
 		const float4 v = what == 1 ? lin[i] : ang[i];
 		float* o = out + 3 * int64_t(k);
 		o[0] = v.x; o[1] = v.y; o[2] = v.z;
 	}
+	else
+	{
+		const f3 a = (what == 3 ? mk(lin[i]) - mk(prevLin[i]) : mk(ang[i]) - mk(prevAng[i])) * invDt;
+		float* o = out + 3 * int64_t(k);
+		o[0] = a.x; o[1] = a.y; o[2] = a.z;
+	}
 }
 
+// what: PxRigidDynamicGPUAPIWriteType (0 pose, 1 linear velocity, 2 angular
+// velocity, 3 force, 4 torque). A force becomes the linear acceleration
+// F / m, a torque the angular acceleration I_world^-1 tau with the body's
+// current orientation (PxForceMode::eFORCE, NpRigidBodyTemplate.h:507-528;
+// setRigidDynamicForce / setRigidDynamicTorque, updateBodiesAndShapes.cu:1229-1278)
 __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                           const float* in)
+                           const float4* invI, float4* extLin, float4* extAng, const float* in)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
@@ -2025,16 +2178,24 @@ __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, fl
 		const float* s = in + 7 * int64_t(k);
 		rot[i] = make_float4(s[0], s[1], s[2], s[3]);
 		pos[i] = make_float4(s[4], s[5], s[6], 0.0f);
+		return;
 	}
-	else if (what == 1)
-	{
-		const float* s = in + 3 * int64_t(k);
-		lin[i] = make_float4(s[0], s[1], s[2], lin[i].w);
-	}
+	const float* s = in + 3 * int64_t(k);
+	const f3 v = mk(s[0], s[1], s[2]);
+	if (what == 1)
+		lin[i] = make_float4(v.x, v.y, v.z, lin[i].w);
+	else if (what == 2)
+		ang[i] = make_float4(v.x, v.y, v.z, 0.0f);
+	else if (what == 3)
+		extLin[i] = f4(v * lin[i].w, 0.0f);
 	else
 	{
-		const float* s = in + 3 * int64_t(k);
-		ang[i] = make_float4(s[0], s[1], s[2], 0.0f);
+		// R diag(invI) R^T (Cm::transformInertiaTensor)
+		const m33 R = qmat(mkq(rot[i]));
+		const float4 ii = invI[i];
+		m33 A;
+		A.c0 = R.c0 * ii.x; A.c1 = R.c1 * ii.y; A.c2 = R.c2 * ii.z;
+		extAng[i] = f4(mmul(mmm(A, mtrans(R)), v), 0.0f);
 	}
 }
 
@@ -2045,12 +2206,16 @@ __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, fl
 // canonical pair key (lower index first)
 PXH_D uint64_t pkey(uint2 p) { return (uint64_t(min(p.x, p.y)) << 32) | max(p.x, p.y); }
 
-__global__ void k_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals)
+// pair keys in the caller's handle space (ids[box], or the box index)
+__global__ void k_pair_keys(int P, const uint2* pairs, const uint32_t* ids, uint64_t* keys, uint32_t* vals)
 {
 	const int p = blockIdx.x * blockDim.x + threadIdx.x;
 	if (p < P)
 	{
-		keys[p] = pkey(pairs[p]);
+		uint2 q = pairs[p];
+		if (ids)
+			q = make_uint2(ids[q.x], ids[q.y]);
+		keys[p] = pkey(q);
 		vals[p] = uint32_t(p);
 	}
 }
@@ -2070,6 +2235,26 @@ __global__ void k_set_diff_flags(int na, const uint64_t* a, int nb, const uint64
 		else h = m;
 	}
 	flag[i] = (l < nb && b[l] == k) ? 0u : 1u;
+}
+
+// touch events (prepareLostFoundPairs_Stage1, gpunarrowphase/src/CUDA/cudaGJKEPA.cu:1468):
+// a constraint without a row in the previous step starts touching; the
+// previous rows it finds are marked, the unmarked ones stopped touching
+__global__ void k_found_flags(int C, const int* prevRow, uint32_t* found, uint32_t* matched)
+{
+	const int c = blockIdx.x * blockDim.x + threadIdx.x;
+	if (c >= C)
+		return;
+	const int po = prevRow ? prevRow[c] : -1;
+	found[c] = po < 0 ? 1u : 0u;
+	if (po >= 0)
+		matched[po] = 1u;
+}
+__global__ void k_lost_flags(int R, const uint32_t* matched, uint32_t* lost)
+{
+	const int r = blockIdx.x * blockDim.x + threadIdx.x;
+	if (r < R)
+		lost[r] = matched[r] ? 0u : 1u;
 }
 
 __global__ void k_compact_keys(int na, const uint64_t* a, const uint32_t* flag, const uint32_t* scanned, uint2* out)
@@ -2227,15 +2412,16 @@ void launch_row_start(int n, const uint64_t* sk, uint32_t* rowStart, uint32_t to
 	PXH_LAUNCH(k_row_tail, int64_t(totalRows) + 1, n, sk, rowStart, totalRows);
 }
 void launch_sweep(bool write, int n, const uint64_t* sk, const float4* smin, const float4* smax, const int* scene,
-                  const int* group, const int2* gridI, const uint32_t* rowOff, const uint32_t* rowStart,
-                  uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap, hipStream_t st)
+                  const int* group, uint64_t groupLut, const int2* gridI, const uint32_t* rowOff,
+                  const uint32_t* rowStart, uint32_t* counts, const uint32_t* offsets, uint2* pairs, uint32_t pairCap,
+                  hipStream_t st)
 {
 	if (write)
-		PXH_LAUNCH(k_sweep<true>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs,
-		           pairCap);
+		PXH_LAUNCH(k_sweep<true>, n, n, sk, smin, smax, scene, group, groupLut, gridI, rowOff, rowStart, counts,
+		           offsets, pairs, pairCap);
 	else
-		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, gridI, rowOff, rowStart, counts, offsets, pairs,
-		           pairCap);
+		PXH_LAUNCH(k_sweep<false>, n, n, sk, smin, smax, scene, group, groupLut, gridI, rowOff, rowStart, counts,
+		           offsets, pairs, pairCap);
 }
 void launch_np_pairs(int cap, const uint32_t* dP, const uint32_t* dConvex, int slotBase, const uint2* pairs,
                      const float4* pos, const float4* rot, const float4* geom, float cd, float tol, float4* ctN,
@@ -2362,10 +2548,11 @@ void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_
 }
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
-                         float maxAngSq, bool pgs, SolverBodies sb, hipStream_t st)
+                         float maxAngSq, bool pgs, SolverBodies sb, float dt, float4* extLin, float4* extAng,
+                         float4* prevLin, float4* prevAng, hipStream_t st)
 {
 	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, pgs ? 1 : 0, sb.vel,
-	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin, sb.rec);
+	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin, sb.rec, dt, extLin, extAng, prevLin, prevAng);
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
@@ -2393,6 +2580,38 @@ void launch_solve(int begin, int end, int C, Rows rows, SolverBodies sb, float m
 	else
 		k_solve<<<grid, block, 0, st>>>(begin, end, C, rows, sb.vel, minPen, muS, muD, invStepDt, biasScale, scale,
 		                                broken);
+}
+void launch_color_keys_world(int C, const int* conColor, const int* conA, const int* bodyWorld, uint32_t limit,
+                             uint64_t* keys, uint32_t* vals, hipStream_t st)
+{
+	PXH_LAUNCH(k_color_keys_world, C, C, conColor, conA, bodyWorld, limit, keys, vals);
+}
+int solve_world_threads() { return SOLVE_WORLD_THREADS; }
+void launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t st)
+{
+	PXH_LAUNCH(k_fill_u32, n, p, n, v);
+}
+void launch_solve_worlds(int grid, int nWorlds, const int* worldOrder, uint32_t* queueHead, const int* bodyStart,
+                         const uint32_t* segStart, int CB, int C, Rows rows, SolverBodies sb, int posIters,
+                         int velIters, float stepDt, float muS, float muD, float invStepDt, float biasScale,
+                         float scale, uint32_t* broken, int ldsBodies, hipStream_t st, hipEvent_t evStart,
+                         hipEvent_t evStop)
+{
+	if (nWorlds <= 0 || grid <= 0)
+		return;
+	// the queue starts at zero (a fill kernel, no memset packet)
+	k_fill_u32<<<1, 64, 0, st>>>(queueHead, 1, 0u);
+	const size_t shm = sizeof(float4) * 4 * size_t(ldsBodies);
+	if (evStart)
+		hipExtLaunchKernelGGL(k_solve_worlds, dim3(grid), dim3(SOLVE_WORLD_THREADS), shm, st, evStart, evStop, 0,
+		                      nWorlds, worldOrder, queueHead, bodyStart, segStart, CB, C, rows, sb.vel, sb.p, sb.dq,
+		                      sb.i0, sb.i1, posIters, velIters, stepDt, muS, muD, invStepDt, biasScale, scale, broken,
+		                      ldsBodies);
+	else
+		k_solve_worlds<<<grid, SOLVE_WORLD_THREADS, shm, st>>>(nWorlds, worldOrder, queueHead, bodyStart, segStart, CB,
+		                                                       C, rows, sb.vel, sb.p, sb.dq, sb.i0, sb.i1, posIters,
+		                                                       velIters, stepDt, muS, muD, invStepDt, biasScale,
+		                                                       scale, broken, ldsBodies);
 }
 void launch_prep_pgs(int C, const int* order, const int* conA, const int* conB, const int* conSlot,
                      const float4* ctN, const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb,
@@ -2439,18 +2658,26 @@ void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, 
 	PXH_LAUNCH(k_finalize, n, n, pos, rot, lin, ang, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
 }
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
-                     const float4* ang, float* out, hipStream_t st)
+                     const float4* ang, const float4* prevLin, const float4* prevAng, float invDt, float* out,
+                     hipStream_t st)
 {
-	PXH_LAUNCH(k_get_data, n, n, idx, what, pos, rot, lin, ang, out);
+	PXH_LAUNCH(k_get_data, n, n, idx, what, pos, rot, lin, ang, prevLin, prevAng, invDt, out);
 }
 void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                     const float* in, hipStream_t st)
+                     const float4* invI, float4* extLin, float4* extAng, const float* in, hipStream_t st)
 {
-	PXH_LAUNCH(k_set_data, n, n, idx, what, pos, rot, lin, ang, in);
+	PXH_LAUNCH(k_set_data, n, n, idx, what, pos, rot, lin, ang, invI, extLin, extAng, in);
 }
-void launch_pair_keys(int P, const uint2* pairs, uint64_t* keys, uint32_t* vals, hipStream_t st)
+void launch_touch_events(int C, const int* prevRow, int R, uint32_t* found, uint32_t* matched, uint32_t* lost,
+                         hipStream_t st)
 {
-	PXH_LAUNCH(k_pair_keys, P, P, pairs, keys, vals);
+	PXH_LAUNCH(k_fill_u32, int64_t(R) + 1, matched, int64_t(R) + 1, 0u);
+	PXH_LAUNCH(k_found_flags, C, C, prevRow, found, matched);
+	PXH_LAUNCH(k_lost_flags, R, R, matched, lost);
+}
+void launch_pair_keys(int P, const uint2* pairs, const uint32_t* ids, uint64_t* keys, uint32_t* vals, hipStream_t st)
+{
+	PXH_LAUNCH(k_pair_keys, P, P, pairs, ids, keys, vals);
 }
 void launch_set_diff(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag, hipStream_t st)
 {

@@ -111,6 +111,7 @@ struct DBuf
 		p = nullptr;
 		cap = 0;
 	}
+	size_t bytes() const { return p ? cap * sizeof(T) : 0; }
 };
 
 }  // namespace
@@ -202,6 +203,40 @@ struct pxh_scene_t
 	uint32_t totalRows = 0;               // broadphase row table size (rowOff on the device)
 	int maxPlanes = 0;                    // most planes in one world
 	DBuf<uint32_t> colCounts;             // colouring compaction offsets
+	// Direct GPU API forces and torques as accelerations for the next step
+	// (allocated, zeroed, on the first one) and the velocities before the
+	// last step (with enable_body_accelerations)
+	DBuf<float4> extLin, extAng, prevLin, prevAng;
+	bool extPending = false;
+	float lastDt = 0.0f;
+	// touch events of the last step (report_touch_events): flags, their
+	// scans and the compacted world-relative pair keys of found / lost pairs
+	DBuf<uint32_t> evFound, evFoundScan, evMatched, evLost, evLostScan;
+	DBuf<uint2> evFoundKeys, evLostKeys;
+	int evC = -1, evR = 0;   // constraints and previous rows of the last step (-1: no events yet)
+	// per-world solve (the batched form): worlds by descending size, the
+	// (world, colour) row starts, the work-queue head and the largest colour
+	int maxWorldBodies = 0;
+	DBuf<int> worldOrder;
+	DBuf<uint32_t> segStart, solveQueue;
+	bool lastWorldSolve = false;          // the last step used the per-world solve
+	// launch shapes (pxh_scene_set_tuning): solve layout, per-world solve
+	// workgroups (0: 4 per CU), bodies staged in LDS per world
+	int32_t tuning[4] = {PXH_SOLVE_LAYOUT_AUTO, 0, 512, 0};
+	// autotuning (tuning[3] steps per candidate): every candidate launch shape
+	// of the solve runs for that many real steps, its solve time taken from
+	// events on the stream; the fastest is kept. Results do not depend on the
+	// shape, so tuning during real steps changes no bit of the simulation.
+	struct TuneCand
+	{
+		int32_t t[3];
+		double ms = 0.0;
+		int steps = 0;
+	};
+	std::vector<TuneCand> tuneCands, tuneLog;
+	size_t tuneIdx = 0;
+	hipEvent_t tuneEv[2] = {nullptr, nullptr};
+	bool tuneTimed = false;
 	DBuf<unsigned char> gatherTmp;        // world removal: gather scratch and indices
 	DBuf<int> gatherIdx;
 	uint32_t* hostCounts = nullptr;       // pinned
@@ -268,6 +303,37 @@ struct pxh_scene_t
 
 	// statistics of the last step
 	int64_t statPairs = 0, statPlanePairs = 0, statConstraints = 0, statContacts = 0, statColors = 0, statSteps = 0;
+
+	// every device buffer of the scene
+	template <typename F>
+	void forEachBuf(F f)
+	{
+		f(hullV); f(hullP); f(hullBC); f(hullBE); f(hullPoly); f(hullEdge); f(hullVR); f(hullInfo); f(hullInfo2);
+		f(cvxWit); f(cvxList);
+		f(pos); f(rot); f(lin); f(ang); f(geom); f(invI); f(scene); f(local); f(bodyStart);
+		f(planes); f(planeLocal); f(worldTag); f(qBody); f(qPlane);
+		f(bmin); f(bmax); f(smin); f(smax); f(gridF); f(gridI);
+		f(rows); f(rowOff); f(rowStart); f(vals); f(vals2); f(hist); f(scanTmp); f(counts); f(flags); f(gridAcc);
+		f(keys); f(keys2); f(pairs); f(ctN); f(ctP);
+		f(conA); f(conB); f(conSlot); f(conColor); f(order); f(conKey); f(conPKey); f(best); f(mask);
+		f(colList); f(colList2); f(colFlags); f(colorCount); f(scratch); f(counters);
+		f(sbVel); f(sbP); f(sbDQ); f(sbI0); f(sbI1); f(sbOrig); f(sbRec);
+		f(rhI); f(rhF); f(rnA); f(rnB); f(rfT); f(rfA); f(rfB); f(rnF); f(rfF); f(rnU);
+		for (int k = 0; k < 2; k++)
+		{
+			f(patch[k]); f(patchKey[k]); f(patchBroken[k]); f(patchColor[k]);
+		}
+		f(prevRow); f(conRec); f(htKey); f(htVal); f(bodyNext); f(colCounts); f(gatherTmp); f(gatherIdx);
+		f(worldOrder); f(segStart); f(solveQueue); f(extLin); f(extAng); f(prevLin); f(prevAng);
+		f(evFound); f(evFoundScan); f(evMatched); f(evLost); f(evLostScan); f(evFoundKeys); f(evLostKeys);
+	}
+	// device bytes the scene holds (PxSimulationStatistics::gpuMemHeap)
+	int64_t deviceBytes()
+	{
+		int64_t total = 0;
+		forEachBuf([&](const auto& b) { total += int64_t(b.bytes()); });
+		return total;
+	}
 };
 
 struct pxh_broadphase_t
@@ -291,6 +357,16 @@ namespace {
 int worldCount(const pxh_scene_t* s)
 {
 	return std::max(int(s->hBodyStart.size()) - 1, int(s->worldPlanes.size()));
+}
+
+// whole host vector to a device buffer (grown as needed)
+template <typename T>
+int uploadVec(DBuf<T>& b, const std::vector<T>& v)
+{
+	PXH_CHECK(b.ensure(std::max<size_t>(v.size(), 1)));
+	if (!v.empty())
+		PXH_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	return PXH_OK;
 }
 
 int rebuild_topology(pxh_scene_t* s)
@@ -336,6 +412,21 @@ int rebuild_topology(pxh_scene_t* s)
 	s->maxPlanes = 0;
 	for (const auto& wp : s->worldPlanes)
 		s->maxPlanes = std::max(s->maxPlanes, int(wp.size()));
+	// worlds by descending body count (stable): the per-world solve's queue order
+	{
+		std::vector<int> order(size_t(std::max(nW, 0)));
+		s->maxWorldBodies = 0;
+		for (int w = 0; w < nW; w++)
+		{
+			order[w] = w;
+			s->maxWorldBodies = std::max(s->maxWorldBodies, s->hBodyStart[w + 1] - s->hBodyStart[w]);
+		}
+		std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+			return s->hBodyStart[a + 1] - s->hBodyStart[a] > s->hBodyStart[b + 1] - s->hBodyStart[b];
+		});
+		if (int r = uploadVec(s->worldOrder, order))
+			return r;
+	}
 	PXH_CHECK(s->bodyStart.ensure(s->hBodyStart.size()));
 	PXH_CHECK(hipMemcpy(s->bodyStart.p, s->hBodyStart.data(), s->hBodyStart.size() * sizeof(int),
 	                    hipMemcpyHostToDevice));
@@ -359,14 +450,62 @@ int gatherBuf(DBuf<T>& b, DBuf<unsigned char>& tmp, const int* idx, int n, hipSt
 	return PXH_OK;
 }
 
-// whole host vector to a device buffer (grown as needed)
-template <typename T>
-int uploadVec(DBuf<T>& b, const std::vector<T>& v)
+
+// appends one hull to the scene's host-side hull arrays
+int appendHull(pxh_scene_t* s, const pxh::Hull& h)
 {
-	PXH_CHECK(b.ensure(std::max<size_t>(v.size(), 1)));
-	if (!v.empty())
-		PXH_CHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+	for (const pxh::HullPolygon& hp : h.polys)
+		if (hp.count > 16)
+			return fail(PXH_ERR_UNSUPPORTED, "convex mesh: polygon with more than 16 vertices");
+	const int nv = int(h.verts.size() / 3), np = int(h.polys.size()), ne = int(h.edgeVerts.size() / 2);
+	s->hHullInfo.push_back(make_int4(int(s->hHullV.size()), nv, int(s->hHullP.size()), np));
+	s->hHullInfo2.push_back(make_int4(int(s->hHullVR.size()), int(s->hHullEdge.size()), ne, 0));
+	for (int i = 0; i < nv; i++)
+		s->hHullV.push_back(make_float4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0f));
+	for (const pxh::HullPolygon& hp : h.polys)
+	{
+		s->hHullP.push_back(make_float4(hp.plane[0], hp.plane[1], hp.plane[2], hp.plane[3]));
+		s->hHullPoly.push_back(uint32_t(hp.first) | (uint32_t(hp.count) << 16));
+	}
+	s->hHullVR.insert(s->hHullVR.end(), h.vref.begin(), h.vref.end());
+	for (int e = 0; e < ne; e++)
+		s->hHullEdge.push_back(uint32_t(h.edgeVerts[2 * e]) | (uint32_t(h.edgeVerts[2 * e + 1]) << 8) |
+		                       (uint32_t(h.edgeFaces[2 * e]) << 16) | (uint32_t(h.edgeFaces[2 * e + 1]) << 24));
+	// GJK margin of ConvexHullV: 10% of the smallest internal extent
+	// (geomutils/src/gjk/GuVecConvexHull.h:76-93)
+	const float me = std::min(h.internalExtents[0], std::min(h.internalExtents[1], h.internalExtents[2]));
+	s->hHullBC.push_back(make_float4(h.aabbCenter[0], h.aabbCenter[1], h.aabbCenter[2], me * 0.1f));
+	float rad = 0.0f;
+	for (int i = 0; i < nv; i++)
+		rad = std::max(rad, std::sqrt(h.verts[3 * i] * h.verts[3 * i] + h.verts[3 * i + 1] * h.verts[3 * i + 1] +
+		                              h.verts[3 * i + 2] * h.verts[3 * i + 2]));
+	// the bounding radius is padded by a relative 1e-5 so float rounding
+	// in the pair test never discards a touching pair
+	s->hHullBE.push_back(make_float4(h.aabbExtents[0], h.aabbExtents[1], h.aabbExtents[2], rad * 1.00001f + 1e-6f));
 	return PXH_OK;
+}
+
+// the host-side hull arrays to the device
+int uploadHulls(pxh_scene_t* s)
+{
+	PXH_CHECK(hipStreamSynchronize(s->stream));
+	if (int r = uploadVec(s->hullV, s->hHullV))
+		return r;
+	if (int r = uploadVec(s->hullP, s->hHullP))
+		return r;
+	if (int r = uploadVec(s->hullBC, s->hHullBC))
+		return r;
+	if (int r = uploadVec(s->hullBE, s->hHullBE))
+		return r;
+	if (int r = uploadVec(s->hullPoly, s->hHullPoly))
+		return r;
+	if (int r = uploadVec(s->hullEdge, s->hHullEdge))
+		return r;
+	if (int r = uploadVec(s->hullVR, s->hHullVR))
+		return r;
+	if (int r = uploadVec(s->hullInfo, s->hHullInfo))
+		return r;
+	return uploadVec(s->hullInfo2, s->hHullInfo2);
 }
 
 }  // namespace
@@ -399,7 +538,9 @@ void pxh_default_scene_desc(pxh_scene_desc* d)
 	d->friction_offset_threshold = 0.04f;
 	d->friction_correlation_distance = 0.025f;
 	d->tolerance_length = 1.0f;
-	d->max_pairs_per_body = 12;
+	d->max_pairs_per_body = 4;
+	d->enable_body_accelerations = 0;
+	d->report_touch_events = 0;
 }
 
 int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
@@ -410,6 +551,8 @@ int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
 		return fail(PXH_ERR_INVALID, "pxh_scene_create: unknown solver type");
 	if (desc->position_iterations < 1 || desc->velocity_iterations < 0)
 		return fail(PXH_ERR_INVALID, "pxh_scene_create: iteration counts out of range");
+	if (desc->max_pairs_per_body < 1)
+		return fail(PXH_ERR_INVALID, "pxh_scene_create: max_pairs_per_body must be at least 1");
 	if (desc->restitution != 0.0f)
 		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: restitution other than 0 is not implemented");
 	int dev = 0;
@@ -431,47 +574,10 @@ int32_t pxh_scene_release(pxh_scene s)
 	if (!s)
 		return PXH_OK;
 	(void)hipStreamSynchronize(s->stream);
-	DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->planes, &s->bmin, &s->bmax,
-	                       &s->smin, &s->smax, &s->gridF, &s->ctN, &s->ctP, &s->sbVel, &s->sbP, &s->sbDQ,
-	                       &s->sbI0, &s->sbI1, &s->sbOrig, &s->sbRec, &s->rhF, &s->rnA, &s->rnB, &s->rfT, &s->rfA, &s->rfB};
-	for (auto* b : f4s)
-		b->release();
-	DBuf<int>* is[] = {&s->scene, &s->local, &s->bodyStart, &s->planeLocal, &s->qBody, &s->qPlane, &s->conA,
-	                   &s->conB, &s->conSlot, &s->conColor, &s->order};
-	for (auto* b : is)
-		b->release();
-	DBuf<uint32_t>* us[] = {&s->rows, &s->rowOff, &s->rowStart, &s->vals, &s->vals2, &s->hist, &s->scanTmp,
-	                        &s->counts, &s->flags, &s->colorCount, &s->scratch, &s->gridAcc};
-	for (auto* b : us)
-		b->release();
-	s->keys.release();
-	s->keys2.release();
-	s->conKey.release();
-	s->conPKey.release();
-	s->best.release();
-	s->mask.release();
-	s->colList.release();
-	s->colList2.release();
-	s->colFlags.release();
-	s->pairs.release();
-	s->gridI.release();
-	s->rhI.release();
-	s->rnF.release();
-	s->rfF.release();
-	s->rnU.release();
-	for (int k = 0; k < 2; k++)
-	{
-		s->patch[k].release();
-		s->patchKey[k].release();
-		s->patchBroken[k].release();
-		s->patchColor[k].release();
-	}
-	s->prevRow.release();
-	s->conRec.release();
-	s->htKey.release();
-	s->htVal.release();
-	s->bodyNext.release();
-	s->worldTag.release();
+	s->forEachBuf([](auto& b) { b.release(); });
+	for (hipEvent_t e : s->tuneEv)
+		if (e)
+			(void)hipEventDestroy(e);
 	if (s->hostCounts)
 		(void)hipHostFree(s->hostCounts);
 	if (s->stream)
@@ -542,6 +648,18 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 	PXH_CHECK(s->invI.grow_keep(nn, base, st));
 	PXH_CHECK(s->scene.grow_keep(nn, base, st));
 	PXH_CHECK(s->local.grow_keep(nn, base, st));
+	if (s->extLin.p)
+	{
+		// the new bodies start without forces
+		PXH_CHECK(s->extLin.grow_keep(nn, base, st));
+		PXH_CHECK(s->extAng.grow_keep(nn, base, st));
+		if (n > 0)
+		{
+			PXH_CHECK(hipMemsetAsync(s->extLin.p + base, 0, sizeof(float4) * size_t(n), st));
+			PXH_CHECK(hipMemsetAsync(s->extAng.p + base, 0, sizeof(float4) * size_t(n), st));
+		}
+	}
+	s->lastDt = 0.0f;   // no acceleration to read until the next step
 	if (n > 0)
 	{
 		PXH_CHECK(hipMemcpy(s->pos.p + base, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice));
@@ -613,55 +731,55 @@ int32_t pxh_scene_add_convex_meshes(pxh_scene s, int32_t k, const int32_t* count
 		if (counts[m] < 4 || pxh::build_hull(pts, counts[m], h))
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_convex_meshes: degenerate or oversized point cloud");
 		pts += 3 * size_t(counts[m]);
-		for (const pxh::HullPolygon& hp : h.polys)
-			if (hp.count > 16)
-				return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_convex_meshes: polygon with more than 16 vertices");
-		const int nv = int(h.verts.size() / 3), np = int(h.polys.size()), ne = int(h.edgeVerts.size() / 2);
-		s->hHullInfo.push_back(make_int4(int(s->hHullV.size()), nv, int(s->hHullP.size()), np));
-		s->hHullInfo2.push_back(make_int4(int(s->hHullVR.size()), int(s->hHullEdge.size()), ne, 0));
-		for (int i = 0; i < nv; i++)
-			s->hHullV.push_back(make_float4(h.verts[3 * i], h.verts[3 * i + 1], h.verts[3 * i + 2], 0.0f));
-		for (const pxh::HullPolygon& hp : h.polys)
-		{
-			s->hHullP.push_back(make_float4(hp.plane[0], hp.plane[1], hp.plane[2], hp.plane[3]));
-			s->hHullPoly.push_back(uint32_t(hp.first) | (uint32_t(hp.count) << 16));
-		}
-		s->hHullVR.insert(s->hHullVR.end(), h.vref.begin(), h.vref.end());
-		for (int e = 0; e < ne; e++)
-			s->hHullEdge.push_back(uint32_t(h.edgeVerts[2 * e]) | (uint32_t(h.edgeVerts[2 * e + 1]) << 8) |
-			                       (uint32_t(h.edgeFaces[2 * e]) << 16) | (uint32_t(h.edgeFaces[2 * e + 1]) << 24));
-		// GJK margin of ConvexHullV: 10% of the smallest internal extent
-		// (geomutils/src/gjk/GuVecConvexHull.h:76-93)
-		const float me = std::min(h.internalExtents[0], std::min(h.internalExtents[1], h.internalExtents[2]));
-		s->hHullBC.push_back(make_float4(h.aabbCenter[0], h.aabbCenter[1], h.aabbCenter[2], me * 0.1f));
-		float rad = 0.0f;
-		for (int i = 0; i < nv; i++)
-			rad = std::max(rad, std::sqrt(h.verts[3 * i] * h.verts[3 * i] + h.verts[3 * i + 1] * h.verts[3 * i + 1] +
-			                              h.verts[3 * i + 2] * h.verts[3 * i + 2]));
-		// the bounding radius is padded by a relative 1e-5 so float rounding
-		// in the pair test never discards a touching pair
-		s->hHullBE.push_back(make_float4(h.aabbExtents[0], h.aabbExtents[1], h.aabbExtents[2], rad * 1.00001f + 1e-6f));
+		if (int r = appendHull(s, h))
+			return r;
 	}
-	PXH_CHECK(hipStreamSynchronize(s->stream));
-	if (int r = uploadVec(s->hullV, s->hHullV))
+	return uploadHulls(s);
+}
+
+int32_t pxh_scene_add_cooked_convex(pxh_scene s, int32_t nv, const float* verts, int32_t np, const float* planes,
+                                    const int32_t* polys, const uint8_t* indices, const float* com, int32_t* index)
+{
+	if (!s || nv < 4 || np < 4 || !verts || !planes || !polys || !indices || !com)
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_cooked_convex: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_cooked_convex: not allowed while simulating");
+	if (nv > 255 || np > 255)
+		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_cooked_convex: more than 255 vertices or polygons");
+	pxh::Hull h;
+	h.verts.assign(verts, verts + 3 * size_t(nv));
+	int refs = 0;
+	for (int p = 0; p < np; p++)
+	{
+		const int base = polys[2 * p], cnt = polys[2 * p + 1];
+		if (cnt < 3 || base < 0)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_cooked_convex: polygon with fewer than 3 vertices");
+		pxh::HullPolygon hp;
+		for (int c = 0; c < 4; c++)
+			hp.plane[c] = planes[4 * p + c];
+		hp.first = int(h.vref.size());
+		hp.count = cnt;
+		for (int v = 0; v < cnt; v++)
+		{
+			const int vi = indices[base + v];
+			if (vi >= nv)
+				return fail(PXH_ERR_INVALID, "pxh_scene_add_cooked_convex: vertex index out of range");
+			h.vref.push_back(uint8_t(vi));
+		}
+		refs += cnt;
+		h.polys.push_back(hp);
+	}
+	pxh::hull_edges_bounds(h);
+	// every edge must join two polygons (a closed hull)
+	for (size_t e = 0; e < h.edgeFaces.size() / 2; e++)
+		if (h.edgeFaces[2 * e] == h.edgeFaces[2 * e + 1])
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_cooked_convex: polygons do not close the hull");
+	pxh::hull_internal_objects(h, com);
+	if (index)
+		*index = int32_t(s->hHullInfo.size());
+	if (int r = appendHull(s, h))
 		return r;
-	if (int r = uploadVec(s->hullP, s->hHullP))
-		return r;
-	if (int r = uploadVec(s->hullBC, s->hHullBC))
-		return r;
-	if (int r = uploadVec(s->hullBE, s->hHullBE))
-		return r;
-	if (int r = uploadVec(s->hullPoly, s->hHullPoly))
-		return r;
-	if (int r = uploadVec(s->hullEdge, s->hHullEdge))
-		return r;
-	if (int r = uploadVec(s->hullVR, s->hHullVR))
-		return r;
-	if (int r = uploadVec(s->hullInfo, s->hHullInfo))
-		return r;
-	if (int r = uploadVec(s->hullInfo2, s->hHullInfo2))
-		return r;
-	return PXH_OK;
+	return uploadHulls(s);
 }
 
 int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
@@ -723,10 +841,12 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 	if (nn > 0)
 	{
 		PXH_CHECK(hipMemcpy(idx.p, keep.data(), keep.size() * sizeof(int), hipMemcpyHostToDevice));
-		DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI};
+		DBuf<float4>* f4s[] = {&s->pos, &s->rot, &s->lin, &s->ang, &s->geom, &s->invI, &s->extLin, &s->extAng};
 		for (auto* b : f4s)
-			if (int r = gatherBuf(*b, s->gatherTmp, idx.p, nn, st))
-				return r;
+			if (b->p)
+				if (int r = gatherBuf(*b, s->gatherTmp, idx.p, nn, st))
+					return r;
+		s->lastDt = 0.0f;
 		if (int r = gatherBuf(s->local, s->gatherTmp, idx.p, nn, st))
 			return r;
 		PXH_CHECK(s->scene.ensure(size_t(nn)));
@@ -788,6 +908,31 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	hipStream_t st = s->stream;
 	const int n = s->n;
 	const int nWorlds = int(s->hBodyStart.size()) - 1;
+	s->tuneTimed = false;
+	if (s->tuning[3] > 0 && n > 0)
+	{
+		if (s->tuneCands.empty())
+		{
+			// the colour launches, and where every world fits one workgroup the
+			// per-world solve at 1, 2, 4 and 8 workgroups per compute unit with
+			// velocity records in LDS or in HBM
+			int dev = 0, cus = 256;
+			(void)hipGetDevice(&dev);
+			(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+			s->tuneCands.push_back(pxh_scene_t::TuneCand{{PXH_SOLVE_LAYOUT_COLORS, 0, s->tuning[2]}});
+			if (d.solver_type == PXH_SOLVER_TGS && s->maxWorldBodies <= PXH_WORLD_SOLVE_MAX_BODIES)
+				for (int per : {1, 2, 4, 8})
+					for (int lds : {std::min(1000, std::max(s->maxWorldBodies, 1)), 0})
+						s->tuneCands.push_back(pxh_scene_t::TuneCand{{PXH_SOLVE_LAYOUT_WORLDS, per * cus, lds}});
+			s->tuneIdx = 0;
+			for (hipEvent_t& e : s->tuneEv)
+				if (!e)
+					PXH_CHECK(hipEventCreate(&e));
+		}
+		for (int i = 0; i < 3; i++)
+			s->tuning[i] = s->tuneCands[s->tuneIdx].t[i];
+		s->tuneTimed = true;
+	}
 	s->statSteps++;
 	s->statPairs = s->statPlanePairs = s->statConstraints = s->statContacts = s->statColors = 0;
 	s->statFriction = s->statDynCon = s->statDynFriction = 0;
@@ -801,7 +946,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	int pk = s->profBegin(0);
 	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), co, s->bmin.p, s->bmax.p, st);
 	s->profEnd(pk);
-	s->pairCap = std::max(s->pairCap, size_t(4) * n);
+	s->pairCap = std::max(s->pairCap, size_t(s->desc.max_pairs_per_body) * n);
 	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->rowOff.p, s->totalRows, st, s->pairCap);
 	if (r)
 		return r;
@@ -915,6 +1060,35 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// each constraint's previous row and colour)
 	pxh::launch_con_prev(C, s->conA.p, s->conB.p, s->conSlot.p, pio, s->conColor.p, s->mask.p, st);
 
+	// touch found / lost events: constraints without a previous row, and
+	// previous rows that no constraint found, as pair keys
+	if (d.report_touch_events)
+	{
+		const int R = s->nPrevRows;
+		PXH_CHECK(s->evFound.ensure(Cc + 1));
+		PXH_CHECK(s->evFoundScan.ensure(Cc + 1));
+		PXH_CHECK(s->evFoundKeys.ensure(Cc));
+		PXH_CHECK(s->evMatched.ensure(size_t(R) + 1));
+		PXH_CHECK(s->evLost.ensure(size_t(R) + 1));
+		PXH_CHECK(s->evLostScan.ensure(size_t(R) + 1));
+		PXH_CHECK(s->evLostKeys.ensure(std::max<size_t>(R, 1)));
+		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(std::max(C, R)) + 1) + 1024));
+		pxh::launch_touch_events(C, R > 0 ? s->prevRow.p : nullptr, R, s->evFound.p, s->evMatched.p, s->evLost.p, st);
+		pxh::launch_fill_u32(s->evFound.p + C, 1, 0u, st);
+		pxh::launch_fill_u32(s->evLost.p + R, 1, 0u, st);
+		PXH_CHECK(hipMemcpyAsync(s->evFoundScan.p, s->evFound.p, sizeof(uint32_t) * (size_t(C) + 1),
+		                         hipMemcpyDeviceToDevice, st));
+		pxh::launch_scan(s->evFoundScan.p, int64_t(C) + 1, s->scanTmp.p, st);
+		pxh::launch_compact_keys(C, s->conPKey.p, s->evFound.p, s->evFoundScan.p, s->evFoundKeys.p, st);
+		PXH_CHECK(hipMemcpyAsync(s->evLostScan.p, s->evLost.p, sizeof(uint32_t) * (size_t(R) + 1),
+		                         hipMemcpyDeviceToDevice, st));
+		pxh::launch_scan(s->evLostScan.p, int64_t(R) + 1, s->scanTmp.p, st);
+		if (R > 0)
+			pxh::launch_compact_keys(R, s->patchKey[prv].p, s->evLost.p, s->evLostScan.p, s->evLostKeys.p, st);
+		s->evC = C;
+		s->evR = R;
+	}
+
 	// 5. colouring of the remaining body-body constraints: rounds of local key
 	//    minima, each taking its first free colour, over a compacted list of the
 	//    uncoloured constraints whose count stays on the device; then the
@@ -956,7 +1130,65 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	int nColors = 0;
 	std::vector<uint32_t> colorStart(1, 0);
 	int remaining = C;
-	while (C > 0)
+	// the per-world solve: TGS, and every world small enough for one workgroup
+	// (layout auto), or as the tuning asks
+	const int layout = s->tuning[0];
+	const bool worldSolve = d.solver_type == PXH_SOLVER_TGS && nWorlds > 0 &&
+	                        (layout == PXH_SOLVE_LAYOUT_WORLDS ||
+	                         (layout == PXH_SOLVE_LAYOUT_AUTO && s->maxWorldBodies <= PXH_WORLD_SOLVE_MAX_BODIES));
+	s->lastWorldSolve = worldSolve;
+	int CB = 0;   // per-world solve: colour slots per world
+	if (worldSolve)
+		PXH_CHECK(s->solveQueue.ensure(2));
+	while (C > 0 && worldSolve)
+	{
+		compact();
+		const int batch = s->nPrevRows > 0 && rounds == 0 ? 8 : 48;
+		for (int k = 0; k < batch && rounds < kMaxRounds; k++, rounds++)
+			pxh::launch_color_round(bound, mDev, list, rounds, s->conA.p, s->conB.p, s->conKey.p, s->conColor.p,
+			                        s->best.p, s->mask.p, st);
+		compact();
+		pxh::launch_static_colors(n, C, s->conA.p, s->conB.p, s->conColor.p, s->bodyNext.p, st);
+		// row order: stable radix sort of (world, colour, index); the row
+		// starts of every (world, colour) come from the sorted keys
+		const int colorBound = 64 + rounds + s->maxPlanes + 1;
+		CB = colorBound + 1;
+		const uint64_t slots = uint64_t(nWorlds) * uint64_t(CB);
+		if (slots >= (1ull << 31))
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: too many (world, colour) slots for the per-world solve");
+		PXH_CHECK(s->segStart.ensure(size_t(slots) + 2));
+		pxh::launch_color_keys_world(C, s->conColor.p, s->conA.p, s->scene.p, uint32_t(colorBound), s->keys.p,
+		                             s->vals.p, st);
+		bool alt = false;
+		pxh::launch_radix_sort(s->keys.p, s->vals.p, s->keys2.p, s->vals2.p, C, 32,
+		                       32 + ((bitsFor(slots) + 7) / 8) * 8, s->hist.p, s->scanTmp.p, st, alt);
+		const uint64_t* ck = alt ? s->keys2.p : s->keys.p;
+		const uint32_t* cv = alt ? s->vals2.p : s->vals.p;
+		pxh::launch_row_start(C, ck, s->segStart.p, uint32_t(slots), st);
+		PXH_CHECK(hipMemcpyAsync(s->order.p, cv, sizeof(int) * C, hipMemcpyDeviceToDevice, st));
+		pxh::launch_fill_u32(s->solveQueue.p + 1, 1, 0u, st);
+		pxh::launch_max_color(C, s->conColor.p, s->solveQueue.p + 1, st);
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts, mDev, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+		PXH_CHECK(hipMemcpyAsync(s->hostCounts + 1, s->solveQueue.p + 1, sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+		PXH_CHECK(hipStreamSynchronize(st));
+		remaining = int(s->hostCounts[0]);
+		if (remaining == 0)
+		{
+			nColors = int(s->hostCounts[1]) + 1;
+			break;
+		}
+		if (rounds >= kMaxRounds)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: colouring did not finish in 960 rounds");
+		bound = remaining;
+	}
+	if (worldSolve && C == 0)
+	{
+		// nothing to solve; the per-world solve still integrates every world
+		CB = 1;
+		PXH_CHECK(s->segStart.ensure(size_t(nWorlds) + 2));
+		pxh::launch_fill_u32(s->segStart.p, int64_t(nWorlds) + 1, 0u, st);
+	}
+	while (C > 0 && !worldSolve)
 	{
 		compact();
 		const int batch = s->nPrevRows > 0 && rounds == 0 ? 8 : 48;
@@ -1005,8 +1237,18 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::SolverBodies sb{s->sbVel.p, s->sbP.p, s->sbDQ.p, s->sbI0.p, s->sbI1.p, s->sbOrig.p, s->sbRec.p};
 	const float3 gdt = make_float3(d.gravity[0] * dt, d.gravity[1] * dt, d.gravity[2] * dt);
 	const bool pgs = d.solver_type == PXH_SOLVER_PGS;
+	if (d.enable_body_accelerations)
+	{
+		PXH_CHECK(s->prevLin.ensure(n));
+		PXH_CHECK(s->prevAng.ensure(n));
+	}
 	pxh::launch_preintegrate(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, gdt, d.linear_damping * dt,
-	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, pgs, sb, st);
+	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, pgs, sb, dt,
+	                         s->extPending ? s->extLin.p : nullptr, s->extPending ? s->extAng.p : nullptr,
+	                         d.enable_body_accelerations ? s->prevLin.p : nullptr,
+	                         d.enable_body_accelerations ? s->prevAng.p : nullptr, st);
+	s->extPending = false;
+	s->lastDt = dt;
 	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc));
 	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnF.ensure(4 * Cc));
 	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
@@ -1051,6 +1293,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	//    TGS: position iterations with substep integration, then velocity
 	//    iterations (DyTGSDynamics.cpp:2515-2790).
 	//    PGS always runs at least one velocity pass (DySolverControl.cpp:343).
+	if (s->tuneTimed)
+		PXH_CHECK(hipEventRecord(s->tuneEv[0], st));
 	const int pgsPasses = P_ + std::max(V_, 1);
 	for (int it = 0; pgs && it < pgsPasses; it++)
 	{
@@ -1068,7 +1312,24 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		if (it == P_ - 1)
 			pxh::launch_pgs_save_motion(n, sb, st);
 	}
-	for (int it = 0; !pgs && it < P_ + V_; it++)
+	if (!pgs && worldSolve)
+	{
+		int grid = s->tuning[1];
+		if (grid <= 0)
+		{
+			int dev = 0, cus = 256;
+			(void)hipGetDevice(&dev);
+			(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+			grid = 4 * cus;
+		}
+		grid = std::min(grid, nWorlds);
+		hipEvent_t e0, e1;
+		s->profKernel(5, &e0, &e1);
+		pxh::launch_solve_worlds(grid, nWorlds, s->worldOrder.p, s->solveQueue.p, s->bodyStart.p, s->segStart.p, CB,
+		                         C, rows, sb, P_, V_, stepDt, d.static_friction, d.dynamic_friction, invStepDt,
+		                         invStepDt * scale, scale, s->patchBroken[cur].p, s->tuning[2], st, e0, e1);
+	}
+	for (int it = 0; !pgs && !worldSolve && it < P_ + V_; it++)
 	{
 		const bool posIter = it < P_;
 		for (int c = 0; c < nColors; c++)
@@ -1090,6 +1351,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		}
 	}
 
+	if (s->tuneTimed)
+		PXH_CHECK(hipEventRecord(s->tuneEv[1], st));
 	s->nPrevRows = C;
 	s->patchCur = cur ^ 1;
 
@@ -1120,6 +1383,26 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 	else
 		PXH_CHECK(hipStreamSynchronize(s->stream));
 	s->simulating = false;
+	if (s->tuneTimed)
+	{
+		s->tuneTimed = false;
+		float ms = 0.0f;
+		PXH_CHECK(hipEventElapsedTime(&ms, s->tuneEv[0], s->tuneEv[1]));
+		pxh_scene_t::TuneCand& c = s->tuneCands[s->tuneIdx];
+		c.ms += ms;
+		if (++c.steps >= s->tuning[3] && ++s->tuneIdx == s->tuneCands.size())
+		{
+			size_t best = 0;
+			for (size_t k = 1; k < s->tuneCands.size(); k++)
+				if (s->tuneCands[k].ms < s->tuneCands[best].ms)
+					best = k;
+			for (int i = 0; i < 3; i++)
+				s->tuning[i] = s->tuneCands[best].t[i];
+			s->tuning[3] = 0;
+			s->tuneLog.swap(s->tuneCands);
+			s->tuneCands.clear();
+		}
+	}
 	// event pairs are read when the profile is asked for (reading each pair
 	// costs host time between steps); the pool is drained past 64k pairs
 	if (s->profUsed > 65536)
@@ -1134,6 +1417,51 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 		s->statDynFriction = int64_t(c3[3]);
 	}
 	return PXH_OK;
+}
+
+int32_t pxh_scene_set_tuning(pxh_scene s, const int32_t* t, int32_t n)
+{
+	if (!s || n < 0 || (n > 0 && !t))
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_tuning: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_set_tuning: not allowed while simulating");
+	if (n > 0 && (t[0] < PXH_SOLVE_LAYOUT_AUTO || t[0] > PXH_SOLVE_LAYOUT_WORLDS))
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_tuning: unknown solve layout");
+	if (n > 1 && t[1] < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_tuning: negative workgroup count");
+	if (n > 2 && (t[2] < 0 || t[2] > 1000))
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_tuning: LDS bodies out of [0, 1000]");
+	if (n > 3 && t[3] < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_tuning: negative autotuning steps");
+	for (int i = 0; i < std::min(n, 4); i++)
+		s->tuning[i] = t[i];
+	s->tuneCands.clear();
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_tuning_log(pxh_scene s, int32_t* shapes, double* ms, int32_t max)
+{
+	if (!s || max < 0 || (max > 0 && (!shapes || !ms)))
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_tuning_log: bad arguments");
+	const int k = std::min<int>(max, int(s->tuneLog.size()));
+	for (int i = 0; i < k; i++)
+	{
+		const pxh_scene_t::TuneCand& c = s->tuneLog[i];
+		for (int j = 0; j < 3; j++)
+			shapes[3 * i + j] = c.t[j];
+		ms[i] = c.steps ? c.ms / c.steps : 0.0;
+	}
+	return int32_t(s->tuneLog.size());
+}
+
+int32_t pxh_scene_get_tuning(pxh_scene s, int32_t* t, int32_t n)
+{
+	if (!s || n < 0 || (n > 0 && !t))
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_tuning: bad arguments");
+	const int k = std::min(n, 4);
+	for (int i = 0; i < k; i++)
+		t[i] = s->tuning[i];
+	return k;
 }
 
 int32_t pxh_scene_set_profiling(pxh_scene s, int32_t enable)
@@ -1171,10 +1499,14 @@ int32_t pxh_scene_get_profile(pxh_scene s, double* ms, int64_t* launches, int32_
 int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene s, void* data, const uint32_t* index, int32_t kind, int32_t n,
                                          void* stream)
 {
-	if (!s || !data || n < 0 || kind < 0 || kind > 2)
+	if (!s || !data || n < 0 || kind < 0 || kind > 4)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_rigid_dynamic_data: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_rigid_dynamic_data: not allowed while simulating");
+	if (kind >= PXH_READ_LINEAR_ACCELERATION && !s->desc.enable_body_accelerations)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_rigid_dynamic_data: accelerations need enable_body_accelerations");
+	if (kind >= PXH_READ_LINEAR_ACCELERATION && !(s->lastDt > 0.0f && s->prevLin.p))
+		return fail(PXH_ERR_STATE, "pxh_scene_get_rigid_dynamic_data: no step since the bodies changed");
 	if (!index && n > s->n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_rigid_dynamic_data: n exceeds body count");
 	hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->stream;
@@ -1187,7 +1519,8 @@ int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene s, void* data, const uint32_t
 		PXH_CHECK(hipStreamWaitEvent(st, ev, 0));
 		PXH_CHECK(hipEventDestroy(ev));
 	}
-	pxh::launch_get_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, static_cast<float*>(data), st);
+	pxh::launch_get_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->prevLin.p, s->prevAng.p,
+	                     s->lastDt > 0.0f ? 1.0f / s->lastDt : 0.0f, static_cast<float*>(data), st);
 	PXH_CHECK(hipGetLastError());
 	if (!stream)
 		PXH_CHECK(hipStreamSynchronize(st));
@@ -1197,7 +1530,7 @@ int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene s, void* data, const uint32_t
 int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene s, const void* data, const uint32_t* index, int32_t kind, int32_t n,
                                          void* stream)
 {
-	if (!s || !data || n < 0 || kind < 0 || kind > 2)
+	if (!s || !data || n < 0 || kind < 0 || kind > 4)
 		return fail(PXH_ERR_INVALID, "pxh_scene_set_rigid_dynamic_data: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_set_rigid_dynamic_data: not allowed while simulating");
@@ -1212,10 +1545,93 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene s, const void* data, const ui
 		PXH_CHECK(hipStreamWaitEvent(s->stream, ev, 0));
 		PXH_CHECK(hipEventDestroy(ev));
 	}
-	pxh::launch_set_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, static_cast<const float*>(data),
-	                     s->stream);
+	if (kind >= PXH_WRITE_FORCE && !s->extLin.p && s->n > 0)
+	{
+		PXH_CHECK(s->extLin.ensure(size_t(s->n)));
+		PXH_CHECK(s->extAng.ensure(size_t(s->n)));
+		PXH_CHECK(hipMemsetAsync(s->extLin.p, 0, sizeof(float4) * s->extLin.cap, s->stream));
+		PXH_CHECK(hipMemsetAsync(s->extAng.p, 0, sizeof(float4) * s->extAng.cap, s->stream));
+	}
+	pxh::launch_set_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, s->extLin.p, s->extAng.p,
+	                     static_cast<const float*>(data), s->stream);
+	if (kind >= PXH_WRITE_FORCE)
+		s->extPending = true;
 	PXH_CHECK(hipGetLastError());
 	PXH_CHECK(hipStreamSynchronize(s->stream));
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_touch_events(pxh_scene s, int32_t* found, int64_t max_found, int32_t* lost, int64_t max_lost,
+                                   int64_t* counts)
+{
+	if (!s || !counts || max_found < 0 || max_lost < 0 || (max_found > 0 && !found) || (max_lost > 0 && !lost))
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_touch_events: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_touch_events: not allowed while simulating");
+	if (!s->desc.report_touch_events)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_touch_events: the scene does not report touch events");
+	counts[0] = counts[1] = 0;
+	if (s->evC < 0)
+		return PXH_OK;
+	uint32_t nf = 0, nl = 0;
+	PXH_CHECK(hipMemcpy(&nf, s->evFoundScan.p + s->evC, sizeof(uint32_t), hipMemcpyDeviceToHost));
+	PXH_CHECK(hipMemcpy(&nl, s->evLostScan.p + s->evR, sizeof(uint32_t), hipMemcpyDeviceToHost));
+	std::vector<uint2> fk(nf), lk(nl);
+	if (nf)
+		PXH_CHECK(hipMemcpy(fk.data(), s->evFoundKeys.p, sizeof(uint2) * nf, hipMemcpyDeviceToHost));
+	if (nl)
+		PXH_CHECK(hipMemcpy(lk.data(), s->evLostKeys.p, sizeof(uint2) * nl, hipMemcpyDeviceToHost));
+	// world-relative keys (world tag 16 | local a 24 | local b or plane 24)
+	// to body indices of the current worlds; pairs of removed worlds are dropped
+	std::vector<int> worldOfTag(65536, -1);
+	for (size_t w = 0; w < s->hWorldTag.size(); w++)
+		worldOfTag[size_t(s->hWorldTag[w]) & 0xFFFF] = int(w);
+	const int nBW = int(s->hBodyStart.size()) - 1;
+	auto decode = [&](const uint2& k, int32_t* ab) {
+		const uint64_t key = (uint64_t(k.x) << 32) | k.y;
+		const int w = worldOfTag[size_t(key >> 48)];
+		if (w < 0 || w >= nBW)
+			return false;
+		const uint32_t la = uint32_t(key >> 24) & 0xFFFFFFu, lb = uint32_t(key) & 0xFFFFFFu;
+		ab[0] = s->hBodyStart[w] + int(la);
+		if (lb & 0x800000u)
+		{
+			const uint32_t pl = lb & 0x7FFFFFu;
+			if (size_t(w) >= s->worldPlanes.size() || pl >= s->worldPlanes[w].size())
+				return false;
+			ab[1] = -1 - s->worldPlanes[w][pl];
+		}
+		else
+			ab[1] = s->hBodyStart[w] + int(lb);
+		return true;
+	};
+	int64_t kf = 0, kl = 0;
+	for (const uint2& k : fk)
+	{
+		int32_t ab[2];
+		if (!decode(k, ab))
+			continue;
+		if (kf < max_found)
+		{
+			found[2 * kf] = ab[0];
+			found[2 * kf + 1] = ab[1];
+		}
+		kf++;
+	}
+	for (const uint2& k : lk)
+	{
+		int32_t ab[2];
+		if (!decode(k, ab))
+			continue;
+		if (kl < max_lost)
+		{
+			lost[2 * kl] = ab[0];
+			lost[2 * kl + 1] = ab[1];
+		}
+		kl++;
+	}
+	counts[0] = kf;
+	counts[1] = kl;
 	return PXH_OK;
 }
 
@@ -1225,10 +1641,10 @@ int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
-	const int64_t v[11] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
+	const int64_t v[12] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
 	                       s->statColors, s->statSteps, s->statFriction,   s->statDynCon,      s->statRounds,
-	                       s->statDynFriction};
-	const int k = std::min(n, 11);
+	                       s->statDynFriction, s->deviceBytes()};
+	const int k = std::min(n, 12);
 	for (int i = 0; i < k; i++)
 		out[i] = v[i];
 	return k;
@@ -1329,6 +1745,24 @@ int32_t pxh_bp_update(pxh_broadphase bp, const float* bounds, const uint32_t* gr
 		PXH_CHECK(hipEventDestroy(ev));
 	}
 	return pxh::bp_update(bp, bounds, groups, distances, n, st, bp->hostCounts, true, nullptr);
+}
+
+int32_t pxh_bp_update_handles(pxh_broadphase bp, const float* bounds, const uint32_t* groups, const float* distances,
+                              const uint32_t* ids, uint32_t id_bound, uint64_t filter_lut, int32_t n, void* stream)
+{
+	if (!bp || n < 0 || (n > 0 && (!bounds || !ids)) || (n > 0 && id_bound == 0))
+		return fail(PXH_ERR_INVALID, "pxh_bp_update_handles: bad arguments");
+	hipStream_t st = bp->stream;
+	if (stream)
+	{
+		hipEvent_t ev;
+		PXH_CHECK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+		PXH_CHECK(hipEventRecord(ev, static_cast<hipStream_t>(stream)));
+		PXH_CHECK(hipStreamWaitEvent(st, ev, 0));
+		PXH_CHECK(hipEventDestroy(ev));
+	}
+	return pxh::bp_update(bp, bounds, groups, distances, n, st, bp->hostCounts, true, nullptr, ids, id_bound,
+	                      filter_lut);
 }
 
 int32_t pxh_bp_fetch_results(pxh_broadphase bp, const uint32_t** created, int32_t* n_created, const uint32_t** deleted,

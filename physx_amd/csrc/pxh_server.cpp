@@ -73,6 +73,7 @@ struct pxh_server_t
 	std::vector<ActiveScene> active;    // in world order
 	std::unordered_map<int64_t, Result> done;
 	int64_t steps = 0, completed = 0, bodySteps = 0;
+	int64_t deviceBytes = 0;   // the batched scene's device bytes after the last step
 	// device staging for finished scenes' bodies (13 floats per body) and their indices
 	float* devOut = nullptr;
 	uint32_t* devIdx = nullptr;
@@ -170,19 +171,29 @@ int32_t pxh_server_step(pxh_server sv, float dt, int32_t* out_admitted, int32_t*
 	if (!sv || !(dt > 0.0f))
 		return serverFail(PXH_ERR_INVALID, "pxh_server_step: bad arguments");
 	// admission: queued scenes in arrival order while slots and budget allow,
-	// appended as new worlds with one add call for all of them
+	// appended as new worlds with one add call for all of them. Each body is
+	// charged bytes_per_body; besides, while the batched scene's measured
+	// device bytes exceed the budget, nothing is admitted until the admitted
+	// scenes have finished (an idle server always admits the next scene, so
+	// the queue keeps moving). The batch is staged first; server state
+	// changes only once the scene holds it.
 	int admitted = 0;
+	if (sv->active.empty() || sv->deviceBytes <= sv->budget)
 	{
-		int world = pxh_scene_world_count(sv->scene);
+		const int world0 = pxh_scene_world_count(sv->scene);
+		if (world0 < 0)
+			return serverFail(world0, pxh_last_error());
+		int64_t used = sv->used;
 		std::vector<int32_t> type, wb, wp;
 		std::vector<float> dims, pose, lin, ang, mass, inertia, planes;
+		std::vector<ActiveScene> staged;
 		const double tAdmit = nowSec();
-		while (!sv->queue.empty() && int(sv->active.size()) < sv->maxWorlds)
+		for (const PendingScene& p : sv->queue)
 		{
-			PendingScene& p = sv->queue.front();
 			const int64_t bytes = int64_t(p.n) * sv->bytesPerBody;
-			if (sv->used + bytes > sv->budget)
+			if (int(sv->active.size() + staged.size()) >= sv->maxWorlds || used + bytes > sv->budget)
 				break;
+			const int world = world0 + int(staged.size());
 			type.insert(type.end(), p.type.begin(), p.type.end());
 			dims.insert(dims.end(), p.dims.begin(), p.dims.end());
 			pose.insert(pose.end(), p.pose.begin(), p.pose.end());
@@ -193,11 +204,8 @@ int32_t pxh_server_step(pxh_server sv, float dt, int32_t* out_admitted, int32_t*
 			planes.insert(planes.end(), p.planes.begin(), p.planes.end());
 			wb.insert(wb.end(), size_t(p.n), world);
 			wp.insert(wp.end(), size_t(p.m), world);
-			sv->active.push_back(ActiveScene{p.id, world, p.n, p.steps, 0, bytes, p.tSubmit, tAdmit});
-			sv->used += bytes;
-			sv->queue.pop_front();
-			world++;
-			admitted++;
+			staged.push_back(ActiveScene{p.id, world, p.n, p.steps, 0, bytes, p.tSubmit, tAdmit});
+			used += bytes;
 		}
 		if (!type.empty())
 		{
@@ -211,7 +219,22 @@ int32_t pxh_server_step(pxh_server sv, float dt, int32_t* out_admitted, int32_t*
 		{
 			const int32_t r = pxh_scene_add_planes(sv->scene, int32_t(wp.size()), planes.data(), wp.data());
 			if (r)
-				return serverFail(r, pxh_last_error());
+			{
+				// take the staged worlds' bodies out again: the scene is as before
+				std::string why = pxh_last_error();
+				std::vector<int32_t> ws;
+				for (int w = world0; w < pxh_scene_world_count(sv->scene); w++)
+					ws.push_back(w);
+				(void)pxh_scene_remove_worlds(sv->scene, ws.data(), int32_t(ws.size()));
+				return serverFail(r, why.c_str());
+			}
+		}
+		for (ActiveScene& a : staged)
+		{
+			sv->used += a.bytes;
+			sv->active.push_back(a);
+			sv->queue.pop_front();
+			admitted++;
 		}
 	}
 	int completed = 0;
@@ -224,6 +247,11 @@ int32_t pxh_server_step(pxh_server sv, float dt, int32_t* out_admitted, int32_t*
 		if (r)
 			return serverFail(r, pxh_last_error());
 		sv->steps++;
+		{
+			int64_t st[12] = {};
+			if (pxh_scene_get_stats(sv->scene, st, 12) == 12)
+				sv->deviceBytes = st[11];
+		}
 		// finished scenes: their bodies in one gather per quantity, one copy out
 		std::vector<int32_t> finished;
 		std::vector<uint32_t> idx;
@@ -362,8 +390,9 @@ int32_t pxh_server_finished(pxh_server sv, int64_t* ids, int32_t max_ids)
 	return k;
 }
 
-// out[0..6]: queued scenes, admitted scenes, admitted bodies, HBM bytes
-// admitted, finished scenes, serving steps, body-steps
+// out[0..7]: queued scenes, admitted scenes, admitted bodies, HBM bytes
+// admitted (at the admission price), finished scenes, serving steps,
+// body-steps, device bytes the batched scene held after the last step
 int32_t pxh_server_stats(pxh_server sv, int64_t* out, int32_t n)
 {
 	if (!sv || !out || n < 0)
@@ -371,11 +400,11 @@ int32_t pxh_server_stats(pxh_server sv, int64_t* out, int32_t n)
 	int64_t bodies = 0;
 	for (const ActiveScene& a : sv->active)
 		bodies += a.n;
-	const int64_t v[7] = {int64_t(sv->queue.size()), int64_t(sv->active.size()), bodies, sv->used, sv->completed,
-	                      sv->steps, sv->bodySteps};
-	for (int i = 0; i < n && i < 7; i++)
+	const int64_t v[8] = {int64_t(sv->queue.size()), int64_t(sv->active.size()), bodies, sv->used, sv->completed,
+	                      sv->steps, sv->bodySteps, sv->deviceBytes};
+	for (int i = 0; i < n && i < 8; i++)
 		out[i] = v[i];
-	return std::min(n, 7);
+	return std::min(n, 8);
 }
 
 }  // extern "C"

@@ -65,13 +65,18 @@ class Scene:
 
     def subset(self, world: int) -> "Scene":
         """The bodies and planes of one world, renumbered as world 0."""
-        b = self.scene_id == world
-        p = self.plane_scene == world
+        return self.subset_range(world, world + 1)
+
+    def subset_range(self, w0: int, w1: int) -> "Scene":
+        """The bodies and planes of worlds [w0, w1), renumbered from 0 (in order)."""
+        b = (self.scene_id >= w0) & (self.scene_id < w1)
+        p = (self.plane_scene >= w0) & (self.plane_scene < w1)
+        name = f"{self.name}[{w0}]" if w1 == w0 + 1 else f"{self.name}[{w0}:{w1}]"
         return Scene(self.shape_type[b].copy(), self.dims[b].copy(), self.pose[b].copy(),
                      self.lin_vel[b].copy(), self.ang_vel[b].copy(), self.mass[b].copy(),
-                     self.inertia[b].copy(), np.zeros(int(b.sum()), np.int32),
-                     self.planes[p].copy(), np.zeros(int(p.sum()), np.int32), 1,
-                     f"{self.name}[{world}]", dict(self.params), self.hull_points, self.hull_counts)
+                     self.inertia[b].copy(), (self.scene_id[b] - w0).astype(np.int32),
+                     self.planes[p].copy(), (self.plane_scene[p] - w0).astype(np.int32), w1 - w0,
+                     name, dict(self.params), self.hull_points, self.hull_counts)
 
 
 def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):

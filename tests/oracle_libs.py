@@ -91,6 +91,70 @@ def ref():
     return _ref
 
 
+REFGPU_PATH = os.path.join(ROOT, "oracle", "_ref", "gpu", "libpxrefgpu.so")
+_refgpu = None
+
+
+def have_refgpu():
+    return os.path.exists(REFGPU_PATH)
+
+
+def refgpu():
+    """The reference SDK built with its GPU support on and the HIP GPU plug-in
+    linked in (static plug-in mode), with the broadphase harness."""
+    global _refgpu
+    if _refgpu is None:
+        import physx_amd
+        physx_amd.load()   # libpxhip.so first: the plug-in's dependency
+        lib = ctypes.CDLL(REFGPU_PATH)
+        lib.pxrefgpu_bp_create.restype = VP
+        for name in ("pxrefgpu_static_group", "pxrefgpu_dynamic_group", "pxrefgpu_kinematic_group"):
+            getattr(lib, name).restype = ctypes.c_uint32
+        _refgpu = lib
+    return _refgpu
+
+
+class SdkBroadPhase:
+    """PxCreateBroadPhase of the GPU-enabled reference SDK: bp_type 2 eABP,
+    3 ePABP, 0 eSAP (CPU), or 4 eGPU (through PxCreatePhysXGpu to the HIP
+    plug-in). update() takes the SDK's incremental form: handle lists plus
+    persistent bounds / groups / distances arrays of the capacity."""
+
+    SAP, MBP, ABP, PABP, GPU = 0, 1, 2, 3, 4
+
+    def __init__(self, bp_type):
+        self.lib = refgpu()
+        self.h = VP(self.lib.pxrefgpu_bp_create(int(bp_type), 0, 0))
+        if not self.h.value:
+            raise RuntimeError(f"PxCreateBroadPhase failed for type {bp_type}")
+
+    def update(self, created, updated, removed, bounds, groups, distances):
+        u32 = lambda a: np.ascontiguousarray(a, dtype=np.uint32)  # noqa: E731
+        c, u, r, g = u32(created), u32(updated), u32(removed), u32(groups)
+        b, d = f32(bounds), f32(distances)
+        up = lambda a: a.ctypes.data_as(ctypes.c_void_p)  # noqa: E731
+        assert self.lib.pxrefgpu_bp_update(self.h, up(c), len(c), up(u), len(u), up(r), len(r), up(b), up(g), up(d),
+                                           int(b.shape[0])) == 0
+        cnt = np.zeros(2, np.int32)
+        self.lib.pxrefgpu_bp_results(self.h, None, 0, None, 0, cnt.ctypes.data_as(ctypes.c_void_p))
+        cr = np.zeros((max(cnt[0], 1), 2), np.uint32)
+        dl = np.zeros((max(cnt[1], 1), 2), np.uint32)
+        self.lib.pxrefgpu_bp_results(self.h, up(cr), int(cnt[0]), up(dl), int(cnt[1]),
+                                     cnt.ctypes.data_as(ctypes.c_void_p))
+        return cr[: cnt[0]], dl[: cnt[1]]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.pxrefgpu_bp_release(self.h)
+            self.h = VP()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def contacts(lib_name: str, t0, g0, p0, t1, g1, p1, cd=0.04, tol=1.0):
     """(n,8) contacts: point xyz, separation, normal xyz, 0."""
     out = np.zeros(64 * 8, np.float32)
@@ -227,6 +291,12 @@ class OracleWorld:
         pose, lv, av = f32(pose), f32(lv), f32(av)
         self.lib.pxo_world_set_bodies(self.h, fp(pose), fp(lv), fp(av))
 
+    def set_forces(self, force=None, torque=None):
+        """addForce / addTorque (eFORCE) for the next step, (n,3) arrays."""
+        f = f32(force) if force is not None else None
+        t = f32(torque) if torque is not None else None
+        self.lib.pxo_world_set_forces(self.h, fp(f) if f is not None else None, fp(t) if t is not None else None)
+
     def stats(self):
         out = np.zeros(6, np.int64)
         self.lib.pxo_world_stats(self.h, out.ctypes.data_as(I64))
@@ -251,6 +321,64 @@ class OracleWorld:
             self.lib.pxo_world_release(self.h)
         except Exception:
             pass
+
+
+class ChunkedOracle:
+    """The oracle over a scene of independent worlds split into chunks of
+    whole worlds, each chunk an OracleWorld stepped on its own thread (the
+    library call releases the GIL). Every stage of the oracle keys its work by
+    world (world-local colour keys and pair keys, per-world broadphase), so
+    the chunks together give exactly what one OracleWorld over all worlds
+    gives. Scenes with convex meshes are not split (the hull registry is
+    shared)."""
+
+    def __init__(self, scene, chunks=16, **kw):
+        from concurrent.futures import ThreadPoolExecutor
+        assert scene.hull_counts is None or len(scene.hull_counts) == 0
+        nw = scene.n_scenes
+        bounds = np.linspace(0, nw, min(chunks, nw) + 1).astype(int)
+        self.parts = [scene.subset_range(int(a), int(b)) for a, b in zip(bounds[:-1], bounds[1:])]
+        self.body0 = np.concatenate([[0], np.cumsum([p.n for p in self.parts])])
+        self.plane0 = np.concatenate([[0], np.cumsum([p.planes.shape[0] for p in self.parts])])
+        self.pool = ThreadPoolExecutor(max_workers=len(self.parts))
+        self.worlds = list(self.pool.map(lambda p: OracleWorld(p, **kw), self.parts))
+        self.n = scene.n
+
+    def step(self, dt, n=1):
+        list(self.pool.map(lambda w: w.step(dt, n), self.worlds))
+
+    def bodies(self):
+        parts = [w.bodies() for w in self.worlds]
+        return tuple(np.concatenate([p[k] for p in parts]) for k in range(3))
+
+    def stats(self):
+        out = {}
+        for w in self.worlds:
+            for k, v in w.stats().items():
+                out[k] = max(out.get(k, 0), v) if k == "colors" else out.get(k, 0) + v
+        return out
+
+    def _shift(self, ids, c):
+        """chunk-local body (>= 0) and plane (-1 - p) ids to scene ids"""
+        return np.where(ids >= 0, ids + self.body0[c], ids - self.plane0[c]).astype(np.int32)
+
+    def constraints(self):
+        out = []
+        for c, w in enumerate(self.worlds):
+            t = w.constraints().copy()
+            t[:, 0] = self._shift(t[:, 0], c)
+            t[:, 1] = self._shift(t[:, 1], c)
+            out.append(t)
+        return np.concatenate(out)
+
+    def contact_data(self):
+        out = []
+        for c, w in enumerate(self.worlds):
+            r = w.contact_data()
+            ids = r[:, :2].copy().view(np.int32)
+            r[:, :2] = self._shift(ids, c).view(np.float32)
+            out.append(r)
+        return np.concatenate(out)
 
 
 class RefScene:
@@ -281,6 +409,12 @@ class RefScene:
         pl = f32(scene.planes)
         self._keep.append(pl)
         assert self.lib.pxref_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ctypes.c_float(0.02)) == 0
+
+    def add_forces(self, force=None, torque=None):
+        """PxRigidDynamic::addForce / addTorque (eFORCE) on every body, (n,3) arrays."""
+        f = f32(force) if force is not None else None
+        t = f32(torque) if torque is not None else None
+        self.lib.pxref_scene_add_forces(self.h, fp(f) if f is not None else None, fp(t) if t is not None else None)
 
     def set_threads(self, threads):
         """Dispatcher of `threads` workers for the following steps

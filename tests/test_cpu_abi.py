@@ -31,3 +31,32 @@ def test_default_desc_matches_reference_defaults():
     assert abs(d.friction_offset_threshold - 0.04) < 1e-9
     assert abs(d.angular_damping - 0.05) < 1e-9           # ScBodyCore.cpp:54
     assert abs(d.max_angular_velocity_sq - 1e4) < 1e-3
+
+
+def test_gpu_plugin_exports_and_sdk_harness():
+    """The reference SDK built with its GPU support on links the HIP GPU
+    plug-in: the plug-in's exports (physxgpu/include/PxPhysXGpu.h:203-241) and
+    the public GPU entry points of the static plug-in mode (include/gpu/
+    PxGpu.h:70-144) are there, and the same SDK's CPU broadphase runs the
+    incremental scenario of the GPU test (no device calls)."""
+    import pytest
+    import oracle_libs as L
+    import plugin_scenario
+    if not L.have_refgpu():
+        pytest.skip("GPU-enabled reference SDK not built (oracle/_ref/gpu)")
+    lib = L.refgpu()
+    for name in ("PxCreatePhysXGpu", "PxCreateCudaContextManager", "PxSetPhysXGpuProfilerCallback",
+                 "PxSetPhysXGpuFoundationInstance", "PxGetSuggestedCudaDeviceOrdinal", "PxGpuCudaRegisterFunction",
+                 "PxGpuCudaRegisterFatBinary", "PxGpuGetCudaFunctionTable", "PxGpuGetCudaFunctionTableSize",
+                 "PxGpuGetCudaModuleTable", "PxGpuGetCudaModuleTableSize", "PxGpuCreatePhysicsGpu",
+                 "PxCudaRegisterFunction", "PxCudaRegisterFatBinary", "PxGetCudaModuleTable",
+                 "PxGetCudaModuleTableSize", "PxGetCudaFunctionTable", "PxGetCudaFunctionTableSize",
+                 "PxGetPhysicsGpu"):
+        assert hasattr(lib, name), name
+    abp = L.SdkBroadPhase(L.SdkBroadPhase.ABP)
+    n = 0
+    for c, u, r, b, g, d in plugin_scenario.updates(lib, cap=2000, n0=1500, rounds=3):
+        cr, dl = abp.update(c, u, r, b, g, d)
+        n += len(cr)
+    assert n > 100
+    abp.close()

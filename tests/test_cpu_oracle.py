@@ -310,3 +310,77 @@ def test_pgs_long_run_statistics_vs_live_reference():
     mean = lambda runs, k: float(np.mean([x[k] for x in runs]))  # noqa: E731
     for k, rel in (("height", 0.05), ("potential", 0.05), ("penetration", 0.15), ("jitter", 0.15)):
         assert abs(mean(a, k) - mean(b, k)) <= rel * abs(mean(b, k)), (k, mean(a, k), mean(b, k))
+
+
+def test_chunked_oracle_equals_whole_scene_oracle():
+    """The full-size batched GPU check splits the oracle into chunks of whole
+    worlds stepped on threads: after three steps the chunks give the same
+    bodies bit for bit, the same counts and the same constraint and contact
+    tables as one oracle over every world."""
+    from physx_amd import scenes as S
+    sc = S.batched_piles(n_scenes=24, nmin=8, nmax=64, seed=41)
+    whole = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
+    parts = L.ChunkedOracle(sc, chunks=5, pos_iters=4, vel_iters=1)
+    whole.step(1.0 / 120.0, 3)
+    parts.step(1.0 / 120.0, 3)
+    for a, b in zip(whole.bodies(), parts.bodies()):
+        assert np.array_equal(a, b)
+    ws, ps = whole.stats(), parts.stats()
+    for k in ("bp_pairs", "constraints", "contacts"):
+        assert ws[k] == ps[k], (k, ws, ps)
+    key = lambda c: set(map(tuple, c.tolist()))  # noqa: E731
+    assert key(whole.constraints()) == key(parts.constraints())
+    rec = lambda r: sorted(map(tuple, r.view(np.int32)[:, :3].tolist()))  # noqa: E731
+    wc, pc = whole.contact_data(), parts.contact_data()
+    assert rec(wc) == rec(pc)
+    order = lambda r: np.lexsort((r.view(np.int32)[:, 1], r.view(np.int32)[:, 0]))  # noqa: E731
+    assert np.array_equal(wc[order(wc)].view(np.int32), pc[order(pc)].view(np.int32))   # bit for bit
+
+
+def _free_boxes(n=24, seed=51):
+    """Boxes far apart in the air, random orientations and velocities."""
+    rng = np.random.default_rng(seed)
+    st = np.full(n, S.BOX)
+    dims = np.zeros((n, 4))
+    dims[:, :3] = rng.uniform(0.2, 0.6, (n, 3))
+    pos = np.stack([np.arange(n) * 3.0, np.full(n, 20.0), np.zeros(n)], 1)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    sc = S._finish(st, dims, pos, q, np.zeros(n), [[0, 1, 0, 0]], [0], 1, "free_boxes", {})
+    sc.lin_vel[:] = rng.uniform(-2, 2, (n, 3))
+    sc.ang_vel[:] = rng.uniform(-2, 2, (n, 3))
+    force = (rng.uniform(-30, 30, (n, 3)) * sc.mass[:, None]).astype(np.float32)
+    torque = (rng.uniform(-5, 5, (n, 3)) * sc.inertia).astype(np.float32)
+    return sc, force, torque
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+@pytest.mark.parametrize("solver", ["tgs", "pgs"])
+def test_forces_and_torques_match_reference(solver):
+    """PxRigidDynamic::addForce / addTorque (eFORCE) on free bodies: one step
+    of the oracle against the reference CPU SDK within 1e-5 of max(|v|, g dt),
+    the linear velocity against the closed form v0 + (F/m + g) dt (before
+    damping, which is 0 here), and the forces lasting one step only."""
+    sc, force, torque = _free_boxes()
+    dt = 1.0 / 60.0
+    o = L.OracleWorld(sc, pos_iters=4, vel_iters=1, solver=solver)
+    r = L.RefScene(sc, solver, 4, 1, threads=2)
+    o.set_forces(force, torque)
+    r.add_forces(force, torque)
+    o.step(dt)
+    r.step(dt)
+    _, ol, oa = o.bodies()
+    _, rl, ra = r.bodies()
+    scale = np.maximum(np.linalg.norm(rl, axis=1), 9.81 * dt)[:, None]
+    assert np.all(np.abs(ol - rl) <= 1e-5 * scale), np.abs(ol - rl).max()
+    assert np.all(np.abs(oa - ra) <= 1e-5 * np.maximum(np.linalg.norm(ra, axis=1), 1.0)[:, None])
+    want = sc.lin_vel + (force / sc.mass[:, None] + [0.0, -9.81, 0.0]) * dt
+    np.testing.assert_allclose(ol, want, rtol=0, atol=2e-5)
+    # the angular change is the world-space I^-1 tau dt (then angular damping 0.05)
+    assert np.abs(oa - sc.ang_vel).max() > 0.01
+    o.step(dt)
+    r.step(dt)
+    _, ol2, _ = o.bodies()
+    _, rl2, _ = r.bodies()
+    np.testing.assert_allclose(ol2, ol + [0.0, -9.81 * dt, 0.0], rtol=0, atol=2e-5)
+    assert np.all(np.abs(ol2 - rl2) <= 1e-5 * np.maximum(np.linalg.norm(rl2, axis=1), 9.81 * dt)[:, None])

@@ -332,38 +332,34 @@ def _contact_table(rec):
     return ids[:, :2], ids[:, 2], rec[:, 4:7], rec[:, 8:24].reshape(-1, 4, 4)
 
 
-@pytest.mark.parametrize("workload", ["sphere_box_bin", "batched_piles"])
-def test_full_size_pile_step_matches_oracle(cuda, workload):
-    """Large workloads after their warm-up: the full 262,144-body sphere/box
-    bin at step 300 (TGS 8/1, dt 1/60; about a million touching pairs) and
-    2,048 of the batched worlds (a quarter of the 8,192, about 250k bodies, so
-    the oracle's step stays near 15 s) at step 120 (TGS 4/1, dt 1/120). The
-    full 8,192 worlds pass the same test (the oracle then takes about a
-    minute). One step from that state on the GPU and in the oracle gives the
+def _full_size_step(sc, pos, dt, warm, solver="tgs", chunks=0, min_constraints=200000):
+    """From the GPU's state after `warm` steps, one step on the GPU and in the
+    oracle (split over `chunks` threads of whole worlds when nonzero): the
     same broadphase pair count, the same touching pairs with the same contact
     counts, every contact within 1e-4 m / 1e-3 (points of a pair matched as
     sets when a reduction to four chose among near-ties), and velocities within
     1e-4 of max(|v|, g*dt) for all but 0.1% of the bodies."""
     import time
-    if workload == "sphere_box_bin":
-        sc, pos, dt, warm = S.sphere_box_bin(), 8, 1.0 / 60.0, 300
-    else:
-        sc, pos, dt, warm = S.batched_piles(n_scenes=2048), 4, 1.0 / 120.0, 120
-    g = P.Scene(sc, position_iterations=pos, velocity_iterations=1)
+    kw = dict(position_iterations=pos, velocity_iterations=1,
+              solver_type=P.PXH_SOLVER_TGS if solver == "tgs" else P.PXH_SOLVER_PGS)
+    g = P.Scene(sc, **kw)
     g.step(dt, warm)
     pose, lv, av = g.bodies()
     g.close()
     st = S.Scene(sc.shape_type, sc.dims, pose, lv, av, sc.mass, sc.inertia, sc.scene_id, sc.planes,
                  sc.plane_scene, sc.n_scenes, sc.name, {})
-    g = P.Scene(st, position_iterations=pos, velocity_iterations=1)
-    o = L.OracleWorld(st, pos_iters=pos, vel_iters=1)
+    g = P.Scene(st, **kw)
+    if chunks:
+        o = L.ChunkedOracle(st, chunks=chunks, pos_iters=pos, vel_iters=1, solver=solver)
+    else:
+        o = L.OracleWorld(st, pos_iters=pos, vel_iters=1, solver=solver)
     g.step(dt)
     t0 = time.time()
     o.step(dt)
-    print("oracle step", round(time.time() - t0, 1), "s", flush=True)
+    print(f"{sc.name}: {sc.n} bodies, oracle step {time.time() - t0:.1f} s", flush=True)
     gs, os_ = g.stats(), o.stats()
     print(gs, flush=True)
-    assert gs["constraints"] > 200000
+    assert gs["constraints"] > min_constraints
     for k in ("bp_pairs", "constraints", "contacts"):
         assert gs[k] == os_[k], (k, gs, os_)
     gi, gc, gn, gp = _contact_table(g.contact_data())
@@ -385,3 +381,158 @@ def test_full_size_pile_step_matches_oracle(cuda, workload):
     off = np.abs(gl - ol).max(1) > tol
     print("bodies off:", int(off.sum()), "max", float(np.abs(gl - ol).max()), flush=True)
     assert off.sum() <= len(gl) // 1000
+    return gs
+
+
+@pytest.mark.parametrize("workload", ["sphere_box_bin", "batched_piles_8192", "box_grid_drop_pgs"])
+def test_full_size_pile_step_matches_oracle(cuda, workload):
+    """The workloads at full size after their warm-up: the 262,144-body
+    sphere/box bin at step 300 (TGS 8/1, dt 1/60; about a million touching
+    pairs); all 8,192 batched worlds (about a million bodies, log-uniform
+    [8, 512] per world) at step 120 (TGS 4/1, dt 1/120), the oracle split
+    over 16 threads of whole worlds; the 4,096-box drop (PGS 4/1, dt 1/60) at
+    step 120. One step from that state on the GPU and in the oracle; see
+    _full_size_step for what must agree."""
+    if workload == "sphere_box_bin":
+        _full_size_step(S.sphere_box_bin(), 8, 1.0 / 60.0, 300)
+    elif workload == "batched_piles_8192":
+        _full_size_step(S.batched_piles(n_scenes=8192), 4, 1.0 / 120.0, 120, chunks=16)
+    else:
+        _full_size_step(S.box_grid_drop(), 4, 1.0 / 60.0, 120, solver="pgs", min_constraints=3000)
+
+
+@pytest.mark.skipif(not L.have_ref(), reason="reference CPU SDK not built (oracle/_ref)")
+def test_box_drop_4096_pgs_long_run_matches_reference(cuda):
+    """The 4,096-box drop at full size (16x16x16 grid, PGS 4/1, dt 1/60, 300
+    steps) on the GPU and in the reference CPU SDK: height and potential
+    energy within 5%, resting floor penetration within 0.3 mm, resting jitter
+    within 15%."""
+    import pile_stats as PS
+    sc = S.box_grid_drop()
+    a = PS.run_stats(P.Scene(sc, position_iterations=4, velocity_iterations=1, solver_type=P.PXH_SOLVER_PGS), sc,
+                     300)
+    b = PS.run_stats(L.RefScene(sc, "pgs", 4, 1, threads=8), sc, 300)
+    print("gpu", a, "ref", b, flush=True)
+    PS.assert_stats_close(a, b, jitter_rel=0.15)
+
+
+def test_direct_gpu_api_forces_torques_and_accelerations(cuda):
+    """Direct GPU API eFORCE / eTORQUE writes (PxDirectGPUAPI.h:57-71) on a
+    subset of bodies, through index arrays: one step of piles in contact
+    matches the oracle given the same forces (within 1e-5 of max(|v|, g dt));
+    free bodies follow v0 + (F/m + g) dt; the acceleration reads return the
+    velocity change over dt; the forces last one step (a second step without
+    them still matches the oracle); and pose and velocity writes read back
+    bit for bit."""
+    import torch
+    dt = 1.0 / 60.0
+    sc = S.batched_piles(n_scenes=4, nmin=32, nmax=64, seed=27)
+    n = sc.n
+    rng = np.random.default_rng(28)
+    idx = np.sort(rng.choice(n, size=n // 3, replace=False)).astype(np.int32)
+    force = np.zeros((n, 3), np.float32)
+    torque = np.zeros((n, 3), np.float32)
+    force[idx] = rng.uniform(-20, 20, (len(idx), 3)) * sc.mass[idx, None]
+    torque[idx] = rng.uniform(-4, 4, (len(idx), 3)) * sc.inertia[idx]
+    g = P.Scene(sc, position_iterations=4, velocity_iterations=1, enable_body_accelerations=1)
+    o = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
+    _, v0, w0 = g.bodies()
+    g.set_forces(force[idx], torque[idx], index=idx)
+    o.set_forces(force, torque)
+    g.step(dt)
+    o.step(dt)
+    _, gl, ga = g.bodies()
+    _, ol, oa = o.bodies()
+    tol = 1e-5 * np.maximum(np.linalg.norm(ol, axis=1), 9.81 * dt)
+    assert np.all(np.abs(gl - ol).max(1) <= tol), np.abs(gl - ol).max()
+    assert np.abs(ga - oa).max() <= 1e-5 * max(1.0, float(np.abs(oa).max()))
+    la, aa = g.accelerations()
+    inv = np.float32(1.0) / np.float32(dt)
+    np.testing.assert_allclose(la, (gl - v0) * inv, rtol=1e-6, atol=1e-4)
+    np.testing.assert_allclose(aa, (ga - w0) * inv, rtol=1e-6, atol=1e-4)
+    g.step(dt)
+    o.step(dt)
+    _, gl2, _ = g.bodies()
+    _, ol2, _ = o.bodies()
+    tol = 1e-4 * np.maximum(np.linalg.norm(ol2, axis=1), 9.81 * dt)
+    assert np.all(np.abs(gl2 - ol2).max(1) <= tol), np.abs(gl2 - ol2).max()
+    # free bodies: the closed form
+    st = np.full(6, S.BOX)
+    dims = np.zeros((6, 4))
+    dims[:, :3] = 0.3
+    free = S._finish(st, dims, np.stack([np.arange(6) * 3.0, np.full(6, 30.0), np.zeros(6)], 1),
+                     np.tile([0, 0, 0, 1.0], (6, 1)), np.zeros(6), [[0, 1, 0, 0]], [0], 1, "free", {})
+    f = (rng.uniform(-10, 10, (6, 3)) * free.mass[:, None]).astype(np.float32)
+    h = P.Scene(free, position_iterations=4, velocity_iterations=1)
+    h.set_forces(f)
+    h.step(dt)
+    _, hl, _ = h.bodies()
+    np.testing.assert_allclose(hl, (f / free.mass[:, None] + [0.0, -9.81, 0.0]) * dt, rtol=0, atol=2e-5)
+    with pytest.raises(P.PxhError):
+        h.accelerations()   # not enabled on this scene
+    # pose and velocity writes through an index array read back bit for bit
+    sel = torch.tensor([5, 2, 0], dtype=torch.int32, device=cuda)
+    pose = torch.tensor([[0, 0, 0.6, 0.8, 1, 2, 3], [0.6, 0, 0, 0.8, 4, 5, 6], [0, 0, 0, 1, 7, 8, 9]],
+                        dtype=torch.float32, device=cuda)
+    torch.cuda.synchronize()
+    h.set_rigid_dynamic_data(pose, P.PXH_DATA_GLOBAL_POSE, index=sel, n=3)
+    back = torch.zeros_like(pose)
+    h.get_rigid_dynamic_data(back, P.PXH_DATA_GLOBAL_POSE, index=sel, n=3)
+    assert torch.equal(back.cpu(), pose.cpu())
+    for kind in (P.PXH_DATA_LINEAR_VELOCITY, P.PXH_DATA_ANGULAR_VELOCITY):
+        v = torch.tensor([[1, 2, 3], [4, 5, 6], [7, 8, 9]], dtype=torch.float32, device=cuda) * (kind + 1)
+        torch.cuda.synchronize()
+        h.set_rigid_dynamic_data(v, kind, index=sel, n=3)
+        out = torch.zeros_like(v)
+        h.get_rigid_dynamic_data(out, kind, index=sel, n=3)
+        assert torch.equal(out.cpu(), v.cpu())
+
+
+def _pairs(table):
+    """(body a, body b or -1 - plane) pairs of a constraint table"""
+    return set(map(tuple, table[:, :2].tolist()))
+
+
+def test_touch_found_lost_events(cuda):
+    """Touch found / lost events (PxPairFlag::eNOTIFY_TOUCH_FOUND / LOST):
+    every step of 12 steps of settling piles, found = touching pairs now and
+    not in the previous step, lost = the reverse, against the constraint
+    tables of consecutive steps; the second step's events against the
+    oracle's tables; and after a world is removed, its pairs are not
+    reported and the others keep their events in the new numbering."""
+    dt = 1.0 / 60.0
+    sc = S.batched_piles(n_scenes=6, nmin=24, nmax=96, seed=29)
+    g = P.Scene(sc, position_iterations=4, velocity_iterations=1, report_touch_events=1)
+    o = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
+    prev, oprev = set(), set()
+    nf = nl = 0
+    for s in range(12):
+        g.step(dt)
+        cur = _pairs(g.constraints())
+        found, lost = g.touch_events()
+        assert set(map(tuple, found.tolist())) == cur - prev, s
+        assert set(map(tuple, lost.tolist())) == prev - cur, s
+        assert len(found) == len(cur - prev) and len(lost) == len(prev - cur)
+        nf += len(found)
+        nl += len(lost)
+        if s < 2:
+            o.step(dt)
+            ocur = _pairs(o.constraints())
+            assert set(map(tuple, found.tolist())) == ocur - oprev, ("oracle", s)
+            assert set(map(tuple, lost.tolist())) == oprev - ocur, ("oracle", s)
+            oprev = ocur
+        prev = cur
+    assert nf > 100 and nl > 5, (nf, nl)
+    # remove world 2: its pairs vanish without lost events; the rest renumbered
+    f2, c2 = g.world_range(2)
+    old = g.constraints()
+    g.remove_worlds([2])
+    g.step(dt)
+    shift = lambda i: i if i < f2 or i < 0 else i - c2  # noqa: E731
+    kept = {(shift(a), b if b < 0 else shift(b)) for a, b in _pairs(old) if not f2 <= a < f2 + c2}
+    # planes: one per world, renumbered likewise (plane of world w is plane w)
+    kept = {(a, b if b >= 0 else (b if -1 - b < 2 else b + 1)) for a, b in kept}
+    cur = _pairs(g.constraints())
+    found, lost = g.touch_events()
+    assert set(map(tuple, found.tolist())) == cur - kept
+    assert set(map(tuple, lost.tolist())) == kept - cur
