@@ -507,6 +507,16 @@ def test_touch_found_lost_events(cuda):
     prev, oprev = set(), set()
     nf = nl = 0
     for s in range(12):
+        if s == 6:
+            # every third body launched upwards: its touching pairs end (lost)
+            import torch
+            idx = np.arange(0, sc.n, 3).astype(np.int32)
+            v = np.zeros((idx.size, 3), np.float32)
+            v[:, 1] = 6.0
+            dev = torch.device("cuda")
+            torch.cuda.synchronize()
+            g.set_rigid_dynamic_data(torch.from_numpy(v).to(dev), P.PXH_DATA_LINEAR_VELOCITY,
+                                     index=torch.from_numpy(idx).to(dev), n=int(idx.size))
         g.step(dt)
         cur = _pairs(g.constraints())
         found, lost = g.touch_events()
