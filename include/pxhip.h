@@ -300,6 +300,29 @@ int32_t pxh_bp_fetch_results(pxh_broadphase bp, const uint32_t** created, int32_
                              const uint32_t** deleted, int32_t* n_deleted);
 int32_t pxh_bp_release(pxh_broadphase bp);
 
+/* Allocation-free broadphase update. No counterpart in the reference: for
+ * callers that cannot allocate device memory (serving loops, harnesses that
+ * forbid hipMalloc). The pxh_bp_update computation (same kernels, same pair
+ * sets bit for bit) with every intermediate in one fixed device array of
+ * PXH_BP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; no
+ * broadphase object. Inputs as pxh_bp_update (device arrays, n at most
+ * PXH_BP_NOALLOC_MAX_BOXES, at most PXH_BP_NOALLOC_MAX_PAIRS overlapping
+ * pairs); outputs: created[max_created*2] and deleted[max_deleted*2]
+ * (device, sorted (id0, id1) pairs) and their counts (host). created /
+ * deleted are diffed against the last successful call on the same GPU: the
+ * device array keeps that call's pairs, and calls on one GPU are serialised
+ * within the process, so interleaved callers see each other's updates. Sizes
+ * past the maxima fail with PXH_ERR_CAPACITY (the state is unchanged); so do
+ * outputs too small for the result (*n_created / *n_deleted then hold the
+ * sizes needed, and the update is dropped). Returns when the outputs are
+ * written. */
+#define PXH_BP_NOALLOC_MAX_BOXES 65536
+#define PXH_BP_NOALLOC_MAX_PAIRS 262144
+#define PXH_BP_NOALLOC_SCRATCH_BYTES (32u << 20)
+int32_t pxh_bp_update_noalloc(const float* bounds, const uint32_t* groups, const float* distances, int32_t n,
+                              uint32_t* created, int32_t max_created, int32_t* n_created, uint32_t* deleted,
+                              int32_t max_deleted, int32_t* n_deleted, void* stream);
+
 /* Batched immediate-mode contact generation on a fresh cache:
  * PxGenerateContacts (physx/include/PxImmediateMode.h:267). Device arrays:
  * type0/type1[n], geom0/geom1[n*4], pose0/pose1[n*7] (p xyz, q xyzw); out

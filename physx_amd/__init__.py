@@ -24,6 +24,7 @@ PXH_SOLVE_LAYOUT_AUTO, PXH_SOLVE_LAYOUT_COLORS, PXH_SOLVE_LAYOUT_WORLDS = 0, 1, 
 PXH_DATA_GLOBAL_POSE, PXH_DATA_LINEAR_VELOCITY, PXH_DATA_ANGULAR_VELOCITY = 0, 1, 2
 PXH_READ_LINEAR_ACCELERATION, PXH_READ_ANGULAR_ACCELERATION = 3, 4
 PXH_WRITE_FORCE, PXH_WRITE_TORQUE = 3, 4
+PXH_BP_NOALLOC_MAX_BOXES, PXH_BP_NOALLOC_MAX_PAIRS = 65536, 262144
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -59,7 +60,7 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats", "pxh_convex_hull",
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
-            "pxh_scene_add_cooked_convex")
+            "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc")
 
 _lib = None
 
@@ -474,6 +475,28 @@ class BroadPhase:
             self.close()
         except Exception:
             pass
+
+
+def bp_update_noalloc(bounds, distances=None, groups=None, max_created=None, max_deleted=None, stream: int = 0):
+    """pxh_bp_update_noalloc: the standalone broadphase update without any
+    device allocation in the library, diffed against the previous call on
+    this GPU. bounds (n,6) float32 device tensor, distances (n,) and groups
+    (n,) device tensors. Returns (created, deleted) as sorted (k,2) uint32
+    numpy arrays."""
+    import torch
+    lib = load()
+    n = int(bounds.shape[0])
+    mc = max(1, max_created if max_created is not None else 4 * max(n, 1))
+    md = max(1, max_deleted if max_deleted is not None else PXH_BP_NOALLOC_MAX_PAIRS)
+    cr = torch.empty((mc, 2), dtype=torch.int32, device=bounds.device)
+    dl = torch.empty((md, 2), dtype=torch.int32, device=bounds.device)
+    nc, nd = ctypes.c_int32(), ctypes.c_int32()
+    _check(lib.pxh_bp_update_noalloc(_dptr(bounds), _dptr(groups) if groups is not None else None,
+                                     _dptr(distances) if distances is not None else None, n, _dptr(cr), mc,
+                                     ctypes.byref(nc), _dptr(dl), md, ctypes.byref(nd),
+                                     ctypes.c_void_p(stream) if stream else None))
+    return (cr[: nc.value].cpu().numpy().view(np.uint32).reshape(-1, 2),
+            dl[: nd.value].cpu().numpy().view(np.uint32).reshape(-1, 2))
 
 
 def _copy_pairs(ptr, n):
