@@ -149,7 +149,8 @@ def pmc_summary(path: str, kernel: str, window) -> None:
     by = {}
     for cn in names:
         sub, _ = _window([r for r in rows if r["Counter_Name"] == cn], window)
-        sub = [r for r in sub if kernel in r["Kernel_Name"] and "pgs" not in r["Kernel_Name"]]
+        sub = [r for r in sub if kernel in r["Kernel_Name"] and "pgs" not in r["Kernel_Name"]
+               and "worlds" not in r["Kernel_Name"]]
         by[cn] = ([float(r["Counter_Value"]) for r in sub],
                   [int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sub])
     for k, (v, d) in by.items():
@@ -440,6 +441,9 @@ def main():
     ap.add_argument("--settle", type=int, default=300,
                     help="untimed pour steps before --warmup: the timed steps run on the settled pile")
     ap.add_argument("--bodies", type=int, default=262144)
+    ap.add_argument("--solve-event-stride", type=int, default=0,
+                    help="timed steps: launch j of step k carries HIP events when (j + k) %% stride == 0 "
+                         "(default: the number of timed steps; 1: every launch)")
     ap.add_argument("--cpu-threads", type=int, nargs="+", default=[16, 8, 1])
     ap.add_argument("--cpu-steps", type=int, nargs="+", default=[3, 2, 1])
     ap.add_argument("--no-cpu", action="store_true")
@@ -469,13 +473,15 @@ def main():
                     help="run 5 device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--pmc-summary", metavar="CSV",
                     help="print the mean counter value per k_solve dispatch of a rocprofv3 counter CSV and exit")
+    ap.add_argument("--kernel", default="k_solve",
+                    help="--pmc-summary: the kernel (name substring) whose dispatches are summarised")
     ap.add_argument("--window", type=int, nargs=2, metavar=("SKIP", "STEPS"),
                     help="--trace-summary / --pmc-summary: the STEPS steps after the first SKIP")
     args = ap.parse_args()
     if len(args.cpu_steps) != len(args.cpu_threads):
         ap.error("--cpu-steps needs one count per --cpu-threads entry")
     if args.pmc_summary:
-        pmc_summary(args.pmc_summary, "k_solve", args.window)
+        pmc_summary(args.pmc_summary, args.kernel, args.window)
         return
     if args.trace_summary:
         trace_summary(args.trace_summary, args.window)
@@ -514,13 +520,17 @@ def main():
             log(f"pour step {s + 1}/{args.settle + args.warmup} ({time.perf_counter() - t0:.1f}s) {gpu.stats()}")
     torch.cuda.synchronize()
 
-    # timed steps: the library as a caller runs it; the solve launches carry
-    # their own start/end events (recorded by the dispatches), nothing else
+    # timed steps: the library as a caller runs it; launch j of timed step k
+    # carries its own start/end events (recorded by the dispatch) when
+    # (j + k) % stride == 0, stride = the number of timed steps: every solve
+    # launch position of a step is timed once over the window
     SOLVE = P.Scene.STAGES.index("solve")
+    stride = args.solve_event_stride or args.steps
     gpu.set_profiling(1 << SOLVE)
+    gpu.set_profiling_stride(stride)
     step_ms = []
     sums = dict(bodies=0, constraints=0, contacts=0, friction_rows=0, body_body_constraints=0, bp_pairs=0,
-                plane_pairs=0, colors=0, color_rounds=0, body_body_friction_rows=0)
+                plane_pairs=0, colors=0, color_rounds=0, body_body_friction_rows=0, solve_launches=0)
     solve_bytes = solve_bytes_ep = 0.0
     torch.cuda.synchronize()
     t_begin = time.perf_counter()
@@ -537,11 +547,12 @@ def main():
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_begin
     solve_ms, solve_launches = gpu.profile().get("solve", (0.0, 0))
-    log(f"timed {args.steps} steps in {elapsed:.3f}s; solve launches {solve_launches}, {solve_ms:.3f} ms")
+    log(f"timed {args.steps} steps in {elapsed:.3f}s; timed solve launches {solve_launches}, {solve_ms:.3f} ms")
 
     # the same number of steps again with every stage timed (events around
     # each launch add marker packets, so this pass is not the timed one)
     gpu.set_profiling(True)
+    gpu.set_profiling_stride(1)
     for s in range(args.steps):
         gpu.simulate(dt)
         gpu.fetch_results(True)
@@ -552,20 +563,24 @@ def main():
 
     value = sc.n * args.steps / elapsed
     avg = {k: v / args.steps for k, v in sums.items()}
+    # average duration of the sampled launches; bytes per launch over every
+    # solve launch of the window
+    all_launches = max(sums["solve_launches"], 1)
     avg_launch_s = solve_ms * 1e-3 / max(solve_launches, 1)
-    bytes_launch = solve_bytes / max(solve_launches, 1)
+    bytes_launch = solve_bytes / all_launches
     achieved = bytes_launch / avg_launch_s / 1e9 if solve_ms > 0 else 0.0
-    achieved_ep = solve_bytes_ep / max(solve_launches, 1) / avg_launch_s / 1e9 if solve_ms > 0 else 0.0
+    achieved_ep = solve_bytes_ep / all_launches / avg_launch_s / 1e9 if solve_ms > 0 else 0.0
     traffic, traffic_src = solve_traffic(args.settle, args.warmup, args.steps, avg["constraints"])
     roofline = {"bound": "hbm", "kernel": "k_solve", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "traffic_source": traffic_src,
                 "bytes_per_launch": round(bytes_launch, 1),
                 "byte_model": "per iteration 152 B per touching pair + 60 B per contact point + 128 B per body",
-                "avg_launch_us": round(1e6 * avg_launch_s, 3), "launches": solve_launches,
-                "solve_ms_per_step": round(solve_ms / args.steps, 4),
+                "avg_launch_us": round(1e6 * avg_launch_s, 3), "launches": sums["solve_launches"],
+                "timed_launches": solve_launches, "event_stride": stride,
+                "solve_ms_per_step": round(avg_launch_s * 1e3 * sums["solve_launches"] / args.steps, 4),
                 "achieved_row_layout_model": round(achieved_ep, 2),
-                "bytes_per_launch_row_layout_model": round(solve_bytes_ep / max(solve_launches, 1), 1)}
+                "bytes_per_launch_row_layout_model": round(solve_bytes_ep / all_launches, 1)}
 
     cpu = None
     if not args.no_cpu:

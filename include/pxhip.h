@@ -121,7 +121,8 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, cons
  * [3] touching pairs (constraints), [4] contact points, [5] colours,
  * [6] steps taken, [7] friction rows, [8] body-body constraints, [9] colouring rounds,
  * [10] friction rows of body-body constraints, [11] device bytes the scene
- * holds (PxSimulationStatistics::gpuMemHeap, PxSimulationStatistics.h:333).
+ * holds (PxSimulationStatistics::gpuMemHeap, PxSimulationStatistics.h:333),
+ * [12] solve launches of the last step.
  * Returns the number of values written (<= n). */
 int32_t pxh_scene_get_stats(pxh_scene scene, int64_t* out, int32_t n);
 
@@ -156,6 +157,15 @@ int64_t pxh_scene_get_constraints(pxh_scene scene, int32_t* out, int64_t max_con
  * profiling was (re)set. */
 #define PXH_PROFILE_ALL 0xff
 int32_t pxh_scene_set_profiling(pxh_scene scene, int32_t stage_mask);
+/* Sampling of the stages whose launches carry their own events (the solve):
+ * launch j of the k-th step since profiling was set is timed when
+ * (j + k) % stride == 0 (1 = every launch), so over `stride` consecutive steps
+ * each launch position of a step is timed once, and a caller can keep
+ * per-launch timing on while it measures throughput (the events of each timed
+ * launch cost host work at submission).
+ * Profiler-side statistics, no reference counterpart (PxProfilerCallback
+ * zones are the reference's instrumentation). */
+int32_t pxh_scene_set_profiling_stride(pxh_scene scene, int32_t stride);
 int32_t pxh_scene_get_profile(pxh_scene scene, double* ms, int64_t* launches, int32_t n);
 
 /* PxDirectGPUAPI::copyContactData (physx/include/PxDirectGPUAPI.h:401): the

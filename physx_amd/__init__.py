@@ -60,7 +60,8 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_server_step", "pxh_server_result", "pxh_server_finished", "pxh_server_stats", "pxh_convex_hull",
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
-            "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc")
+            "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
+            "pxh_scene_set_profiling_stride")
 
 _lib = None
 
@@ -234,10 +235,10 @@ class Scene:
         self.set_rigid_dynamic_data(torch.from_numpy(_f32(ang_vel)).to(dev), PXH_DATA_ANGULAR_VELOCITY)
 
     def stats(self) -> dict:
-        out = np.zeros(12, np.int64)
-        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 12))
+        out = np.zeros(13, np.int64)
+        _check(self.lib.pxh_scene_get_stats(self.h, out.ctypes.data_as(_I64), 13))
         keys = ("bodies", "bp_pairs", "plane_pairs", "constraints", "contacts", "colors", "steps", "friction_rows",
-                "body_body_constraints", "color_rounds", "body_body_friction_rows", "device_bytes")
+                "body_body_constraints", "color_rounds", "body_body_friction_rows", "device_bytes", "solve_launches")
         return dict(zip(keys, map(int, out)))
 
     def set_tuning(self, layout=None, world_workgroups=None, lds_bodies=None, autotune_steps=None):
@@ -272,6 +273,10 @@ class Scene:
         (index in STAGES) to time."""
         mask = 0xFF if enable is True else (0 if enable is False else int(enable))
         _check(self.lib.pxh_scene_set_profiling(self.h, mask))
+
+    def set_profiling_stride(self, stride: int):
+        """Time one in `stride` of the solve launches (counted across steps)."""
+        _check(self.lib.pxh_scene_set_profiling_stride(self.h, int(stride)))
 
     def profile(self) -> dict:
         ms = np.zeros(8, np.float64)

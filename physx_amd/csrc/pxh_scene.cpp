@@ -178,7 +178,7 @@ struct pxh_scene_t
 	DBuf<uint32_t> colFlags;
 	DBuf<uint32_t> colorCount, scratch;
 	DBuf<unsigned long long> counters;    // contact points, friction rows, dynamic-dynamic constraints
-	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0;
+	int64_t statFriction = 0, statDynCon = 0, statRounds = 0, statDynFriction = 0, statSolveLaunches = 0;
 	pxh::SolverBodies sb{};
 	DBuf<float4> sbVel, sbP, sbDQ, sbI0, sbI1, sbOrig, sbRec;
 	DBuf<int4> rhI;
@@ -244,6 +244,9 @@ struct pxh_scene_t
 	// optional per-stage GPU timing with HIP events on the scene stream
 	struct ProfPair { hipEvent_t a, b; int stage; };
 	uint32_t profiling = 0;   // bit i: stage i is timed
+	uint32_t profStride = 1;  // dispatch-timed launches: one in profStride carries events
+	uint64_t profSeq = 0;     // dispatch-timed launches of the current step so far
+	uint64_t profStep = 0;    // steps since profiling was set
 	std::vector<ProfPair> prof;
 	size_t profUsed = 0;
 	double profMs[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -274,6 +277,10 @@ struct pxh_scene_t
 	{
 		*a = *b = nullptr;
 		if (!(profiling >> stage & 1u))
+			return -1;
+		// launch j of step k is timed when (j + k) % profStride == 0: over
+		// profStride consecutive steps every launch index is timed once
+		if (profStride > 1 && ((profSeq++ + profStep) % profStride) != 0)
 			return -1;
 		if (profUsed == prof.size())
 		{
@@ -934,6 +941,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		s->tuneTimed = true;
 	}
 	s->statSteps++;
+	s->profStep += s->profSeq > 0 ? 1 : 0;
+	s->profSeq = 0;
 	s->statPairs = s->statPlanePairs = s->statConstraints = s->statContacts = s->statColors = 0;
 	s->statFriction = s->statDynCon = s->statDynFriction = 0;
 	if (n == 0)
@@ -1295,6 +1304,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	//    PGS always runs at least one velocity pass (DySolverControl.cpp:343).
 	if (s->tuneTimed)
 		PXH_CHECK(hipEventRecord(s->tuneEv[0], st));
+	s->statSolveLaunches = 0;
 	const int pgsPasses = P_ + std::max(V_, 1);
 	for (int it = 0; pgs && it < pgsPasses; it++)
 	{
@@ -1306,6 +1316,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 				continue;
 			hipEvent_t e0, e1;
 			s->profKernel(5, &e0, &e1);
+			s->statSolveLaunches++;
 			pxh::launch_solve_pgs(int(colorStart[c]), int(colorStart[c + 1]), C, rows, s->rnU.p, sb, doFriction,
 			                      !posIter, it == pgsPasses - 1 ? s->patchBroken[cur].p : nullptr, st, e0, e1);
 		}
@@ -1325,6 +1336,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		grid = std::min(grid, nWorlds);
 		hipEvent_t e0, e1;
 		s->profKernel(5, &e0, &e1);
+		s->statSolveLaunches++;
 		pxh::launch_solve_worlds(grid, nWorlds, s->worldOrder.p, s->solveQueue.p, s->bodyStart.p, s->segStart.p, CB,
 		                         C, rows, sb, P_, V_, stepDt, d.static_friction, d.dynamic_friction, invStepDt,
 		                         invStepDt * scale, scale, s->patchBroken[cur].p, s->tuning[2], st, e0, e1);
@@ -1338,6 +1350,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 				continue;
 			hipEvent_t e0, e1;
 			s->profKernel(5, &e0, &e1);
+			s->statSolveLaunches++;
 			pxh::launch_solve(int(colorStart[c]), int(colorStart[c + 1]), C, rows, sb,
 			                  posIter ? -3.402823466e+38f : 0.0f, d.static_friction, d.dynamic_friction, invStepDt,
 			                  invStepDt * scale, scale, it == P_ + V_ - 1 ? s->patchBroken[cur].p : nullptr, st, e0,
@@ -1472,11 +1485,25 @@ int32_t pxh_scene_set_profiling(pxh_scene s, int32_t enable)
 		return fail(PXH_ERR_STATE, "pxh_scene_set_profiling: not allowed while simulating");
 	s->profCollect();
 	s->profiling = uint32_t(enable) & 0xffu;
+	s->profSeq = 0;
+	s->profStep = 0;
 	for (int i = 0; i < 8; i++)
 	{
 		s->profMs[i] = 0.0;
 		s->profLaunches[i] = 0;
 	}
+	return PXH_OK;
+}
+
+int32_t pxh_scene_set_profiling_stride(pxh_scene s, int32_t stride)
+{
+	if (!s || stride < 1)
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_profiling_stride: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_set_profiling_stride: not allowed while simulating");
+	s->profStride = uint32_t(stride);
+	s->profSeq = 0;
+	s->profStep = 0;
 	return PXH_OK;
 }
 
@@ -1641,10 +1668,10 @@ int32_t pxh_scene_get_stats(pxh_scene s, int64_t* out, int32_t n)
 		return fail(PXH_ERR_INVALID, "pxh_scene_get_stats: bad arguments");
 	if (s->simulating)
 		return fail(PXH_ERR_STATE, "pxh_scene_get_stats: not allowed while simulating");
-	const int64_t v[12] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
+	const int64_t v[13] = {s->n,          s->statPairs, s->statPlanePairs, s->statConstraints, s->statContacts,
 	                       s->statColors, s->statSteps, s->statFriction,   s->statDynCon,      s->statRounds,
-	                       s->statDynFriction, s->deviceBytes()};
-	const int k = std::min(n, 12);
+	                       s->statDynFriction, s->deviceBytes(), s->statSolveLaunches};
+	const int k = std::min(n, 13);
 	for (int i = 0; i < k; i++)
 		out[i] = v[i];
 	return k;
