@@ -179,9 +179,9 @@ def run_batched(args, P, scenes):
     sc = scenes.batched_piles(n_scenes=args.scenes)
     log(f"batched: {sc.n_scenes} worlds, {sc.n} bodies")
     g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
-    # the solve's launch shape is autotuned on the first steps of the run
-    # (every candidate for 2 real steps; no bit of the result depends on it)
-    g.set_tuning(autotune_steps=args.autotune_steps)
+    # the solve's launch shape is autotuned once the piles have formed, at
+    # step --autotune-at of the timed run (every candidate for
+    # --autotune-steps real steps; no bit of the result depends on it)
     for s in range(args.batched_warmup):
         g.step(dt)
     torch.cuda.synchronize()
@@ -189,6 +189,8 @@ def run_batched(args, P, scenes):
     pairs = 0
     t0 = time.perf_counter()
     for s in range(args.batched_steps):
+        if s == args.autotune_at and args.autotune_steps > 0:
+            g.set_tuning(autotune_steps=args.autotune_steps)
         a = time.perf_counter()
         g.simulate(dt)
         g.fetch_results(True)
@@ -231,7 +233,7 @@ def run_batched(args, P, scenes):
                            f"after {args.plain_warmup} warm-up steps, one scene per world stepped in turn",
            "speedup_vs_plain": round(batched / plain_rate, 2),
            "solve_shape": {"layout": ["auto", "colour launches", "per-world persistent"][tuning[0]],
-                           "workgroups": tuning[1], "lds_bodies": tuning[2],
+                           "workgroups": tuning[1], "lds_bodies": tuning[2], "tuned_at_step": args.autotune_at,
                            "autotune_log_ms_per_step": [[a, b, c, round(m, 4)] for a, b, c, m in tuning_log]}}
     log(f"batched: {out}")
     return out
@@ -451,6 +453,8 @@ def main():
     ap.add_argument("--scenes", type=int, default=8192)
     ap.add_argument("--batched-steps", type=int, default=1000)
     ap.add_argument("--batched-warmup", type=int, default=0)
+    ap.add_argument("--autotune-at", type=int, default=200,
+                    help="batched run: the step at which the solve's launch shape is autotuned")
     ap.add_argument("--autotune-steps", type=int, default=2,
                     help="batched run: steps per candidate solve shape in the autotuning (0: none)")
     ap.add_argument("--plain-sample", type=int, default=64)
