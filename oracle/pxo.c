@@ -2382,6 +2382,30 @@ typedef struct {
 	float biasedErr[4], unbiasedErr[4], fbias[4], ftarget[4], muS, muD;
 } pxo_con;
 
+/* A joint: kind, bodies (-1: the world frame), local frames, colour, and
+ * the rows its prep produced this step (SolverConstraint1DHeaderStep and
+ * SolverConstraint1DStep, lowleveldynamics/src/DySolverConstraint1DStep.h:
+ * 139-255). */
+typedef struct {
+	v3 lin0, lin1, ang0, ang1;
+	float error, biasScale, velMul, velTarget, minImp, maxImp, applied, maxBias;
+	int flags;           /* JF_* */
+} jrow;
+
+enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
+
+typedef struct {
+	int type, a, b, color;
+	xf f0, f1;
+	/* prepared each step */
+	int count;
+	float invMass0, invMass1;
+	v3 rA, rB;
+	v3 o0[3], o1[3];     /* angOrthoAxis0 / angOrthoAxis1 */
+	float orecip[3], oerr[3];
+	jrow r[6];
+} pxo_joint;
+
 typedef struct {
 	pxo_desc d;
 	int n, cap;
@@ -2404,6 +2428,11 @@ typedef struct {
 	/* friction patches of the previous step, sorted by pair key */
 	fpatch* prev;
 	long long nprev;
+	/* joints, their colours, and the sorted body-pair keys they keep from
+	 * colliding */
+	pxo_joint* joint;
+	int nj, jcap, njcolors;
+	uint64_t* jkeys;
 } pxo_world;
 
 void pxo_default_desc(pxo_desc* d)
@@ -2443,6 +2472,7 @@ void pxo_world_release(void* wp)
 	free(w->invMass); free(w->invI); free(w->extLin); free(w->extAng); free(w->pn); free(w->pd); free(w->pscene);
 	free(w->plocal); free(w->con);
 	free(w->prev);
+	free(w->joint); free(w->jkeys);
 	free(w);
 }
 
@@ -3121,8 +3151,27 @@ static void solve_con_pgs(pxo_con* k, sbody* b0, sbody* b1, int doFriction, int 
 	if (b1->invMass > 0.0f) { b1->lin = l1; b1->ang = a1; }
 }
 
+static int cmp_u64(const void* x, const void* y)
+{
+	const uint64_t a = *(const uint64_t*)x, b = *(const uint64_t*)y;
+	return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+/* whether bodies i < j are joined by a joint: jointed actors do not collide
+ * (PxConstraintFlag::eCOLLISION_ENABLED is off by default, physx/include/
+ * PxConstraint.h; the pair is filtered out before the narrowphase) */
+static int jointed(const pxo_world* w, int i, int j)
+{
+	if (!w->nj)
+		return 0;
+	const uint64_t k = ((uint64_t)(uint32_t)i << 32) | (uint32_t)j;
+	return bsearch(&k, w->jkeys, (size_t)w->nj, sizeof(uint64_t), cmp_u64) != NULL;
+}
+
 static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 {
+	if (jointed(w, i, j))
+		return;
 	/* shape order: lower geometry type first (the contact-method table order);
 	 * for two shapes of one type the later-added body first, as the
 	 * reference's contact managers order them (shouldSwapBodies,
@@ -3164,6 +3213,438 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 		n = c_convex_convex(hull_of(g0[0]), w->pose[s0], hull_of(g1[0]), w->pose[s1], cd, tol, &c);
 	if (n > 0)
 		push_con(w, s0, s1, &c);
+}
+
+/* ---------------------------------------------------------------------------
+ * Joints: 1D constraint rows of fixed, spherical, revolute and prismatic
+ * joints without limits or drives, TGS.
+ * ------------------------------------------------------------------------- */
+
+/* Px1DConstraint (physx/include/PxConstraintDesc.h:143-230), the fields the
+ * locked-axis rows set; rows start from setupConstraintRows
+ * (lowleveldynamics/src/DyConstraintPrep.h:115-133): zeros, impulses
+ * unbounded. */
+typedef struct {
+	v3 lin0, ang0, lin1, ang1;
+	float geomErr, velTarget, minImp, maxImp;
+	int hint, angular;
+} row1d;
+
+enum { HINT_ROT_EQ = 1024, HINT_EQ = 2048 };   /* PxConstraintSolveHint */
+
+/* Jacobian of the relative rotation qa* qb against (omega_b - omega_a)
+ * (computeJacobianAxes, physxextensions/src/ExtConstraintHelper.h:77-102) */
+static void jacobian_axes(v3 row[3], qt qa, qt qb)
+{
+	const float wa = qa.w, wb = qb.w;
+	const v3 va = V(qa.x, qa.y, qa.z), vb = V(qb.x, qb.y, qb.z);
+	const v3 c = add(scl(vb, wa), scl(va, wb));
+	const float d0 = wa * wb;
+	const float d1 = dot(va, vb);
+	const float d = d0 - d1;
+	row[0] = scl(add(add(scl(va, vb.x), scl(vb, va.x)), V(d, c.z, -c.y)), 0.5f);
+	row[1] = scl(add(add(scl(va, vb.y), scl(vb, va.y)), V(-c.z, d, c.x)), 0.5f);
+	row[2] = scl(add(add(scl(va, vb.z), scl(vb, va.z)), V(c.y, -c.x, d)), 0.5f);
+	if ((d0 + d1) == 0.0f)
+	{
+		row[0].x += 1.192092896e-07f;
+		row[1].y += 1.192092896e-07f;
+		row[2].z += 1.192092896e-07f;
+	}
+}
+
+/* locked axes of each joint kind (linear, angular bit masks x y z):
+ * FixedJointSolverPrep (ExtFixedJoint.cpp:70), SphericalJointSolverPrep
+ * (ExtSphericalJoint.cpp:157), RevoluteJointSolverPrep
+ * (ExtRevoluteJoint.cpp:221, twist about x free), PrismaticJointSolverPrep
+ * (ExtPrismaticJoint.cpp:135, translation along x free) */
+static void joint_axes(int type, int* lin, int* ang)
+{
+	switch (type)
+	{
+	case PXO_JOINT_FIXED: *lin = 7; *ang = 7; break;
+	case PXO_JOINT_SPHERICAL: *lin = 7; *ang = 0; break;
+	case PXO_JOINT_REVOLUTE: *lin = 7; *ang = 6; break;
+	default: *lin = 6; *ang = 7; break;
+	}
+}
+
+/* The joint's solverPrep with ConstraintHelper::prepareLockedAxes
+ * (ExtConstraintHelper.h:150-208, 243-297): joint frames in the world
+ * (computeJointFrames, :61-73), the neighbourhood operator (:44-48), rows
+ * for the locked linear axes of frame A through the anchor, rows for the
+ * locked rotation axes. Returns the row count; *ra / *rb the anchor offsets
+ * (cA2wOut - bA2w.p, cB2wOut - bB2w.p). */
+static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, v3* rbOut)
+{
+	xf cA, cB;
+	cA.q = qmul(bA.q, j->f0.q);
+	cA.p = add(qrot(bA.q, j->f0.p), bA.p);
+	cB.q = qmul(bB.q, j->f1.q);
+	cB.p = add(qrot(bB.q, j->f1.p), bB.p);
+	if (cA.q.x * cB.q.x + cA.q.y * cB.q.y + cA.q.z * cB.q.z + cA.q.w * cB.q.w < 0.0f)
+	{
+		cB.q.x = -cB.q.x; cB.q.y = -cB.q.y; cB.q.z = -cB.q.z; cB.q.w = -cB.q.w;
+	}
+	v3 ra = sub(cB.p, bA.p);
+	const v3 rb = sub(cB.p, bB.p);
+	int lin, ang;
+	joint_axes(j->type, &lin, &ang);
+	const v3 cB2cAp = qrotinv(cA.q, sub(cB.p, cA.p));
+	int k = 0;
+	for (int i = 0; i < 6; i++)
+	{
+		memset(&rows[i], 0, sizeof(row1d));
+		rows[i].minImp = -3.402823466e+38f;
+		rows[i].maxImp = 3.402823466e+38f;
+	}
+	if (lin)
+	{
+		const m33 axes = qmat(cA.q);
+		v3 ev = V(0, 0, 0);
+		if (lin & 1) ev = sub(ev, scl(axes.c0, cB2cAp.x));
+		if (lin & 2) ev = sub(ev, scl(axes.c1, cB2cAp.y));
+		if (lin & 4) ev = sub(ev, scl(axes.c2, cB2cAp.z));
+		ra = add(ra, ev);
+		const v3 ax[3] = {axes.c0, axes.c1, axes.c2};
+		for (int i = 0; i < 3; i++)
+			if (lin & (1 << i))
+			{
+				row1d* c = &rows[k++];
+				c->hint = HINT_EQ;
+				c->lin0 = ax[i];
+				c->ang0 = cross(ra, ax[i]);
+				c->lin1 = ax[i];
+				c->ang1 = cross(rb, ax[i]);
+				c->geomErr = -comp(cB2cAp, i);
+			}
+	}
+	if (ang)
+	{
+		const qt qB2qA = qmul(qconj(cA.q), cB.q);
+		v3 row[3];
+		jacobian_axes(row, cA.q, cB.q);
+		const float qe[3] = {qB2qA.x, qB2qA.y, qB2qA.z};
+		for (int i = 0; i < 3; i++)
+			if (ang & (1 << i))
+			{
+				row1d* c = &rows[k++];
+				c->hint = HINT_EQ;
+				c->ang0 = row[i];
+				c->ang1 = row[i];
+				c->geomErr = -qe[i];
+				c->angular = 1;
+			}
+	}
+	*raOut = ra;
+	*rbOut = rb;
+	return k;
+}
+
+/* orthogonalize (lowleveldynamics/src/DyConstraintSetup.cpp:313-470):
+ * Gram-Schmidt in the inverse-mass metric over one group of rows (all
+ * equality rows here), geometric error and target velocity carried along */
+static void orthogonalize_rows(row1d** row, v3* as0, v3* as1, int count, float im0, float im1)
+{
+	v3 lin0m[6], ang0m[6], lin1m[6], ang1m[6], lin0[6], ang0[6], lin1[6], ang1[6];
+	float g[6], t[6];
+	for (int i = 0; i < count; i++)
+	{
+		v3 l0 = row[i]->lin0, a0 = row[i]->ang0, l1 = row[i]->lin1, a1 = row[i]->ang1;
+		float gi = row[i]->geomErr, ti = row[i]->velTarget;
+		v3 s0v = as0[i], s1v = as1[i];
+		for (int k = 0; k < i; k++)
+		{
+			const v3 sa = add(mulv(l1, lin1m[k]), mulv(l0, lin0m[k]));
+			const v3 sb = add(mulv(s1v, ang1m[k]), mulv(s0v, ang0m[k]));
+			const v3 ss = add(sa, sb);
+			const float tt = ss.x + ss.y + ss.z;
+			l0 = sub(l0, scl(lin0[k], tt));
+			gi = gi - g[k] * tt;
+			a0 = sub(a0, scl(ang0[k], tt));
+			ti = ti - t[k] * tt;
+			l1 = sub(l1, scl(lin1[k], tt));
+			a1 = sub(a1, scl(ang1[k], tt));
+			s0v = sub(s0v, scl(as0[k], tt));
+			s1v = sub(s1v, scl(as1[k], tt));
+		}
+		row[i]->lin0 = l0; row[i]->geomErr = gi;
+		row[i]->ang0 = a0; row[i]->velTarget = ti;
+		row[i]->lin1 = l1;
+		row[i]->ang1 = a1;
+		as0[i] = s0v;
+		as1[i] = s1v;
+		lin0[i] = l0; g[i] = gi; ang0[i] = a0; t[i] = ti; lin1[i] = l1; ang1[i] = a1;
+		const v3 l0m = scl(l0, im0), l1m = scl(l1, im1);
+		const v3 a0m = s0v, a1m = s1v;   /* angular inverse-mass scales are 1 */
+		const v3 q0 = add(mulv(l0, l0m), mulv(l1, l1m));
+		const v3 q1 = add(mulv(a0m, s0v), mulv(a1m, s1v));
+		const v3 qq = add(q0, q1);
+		const float sum = qq.x + qq.y + qq.z;
+		const float a = sum > 0.0f ? 1.0f / sum : 0.0f;
+		lin0m[i] = scl(l0m, a);
+		ang0m[i] = scl(a0m, a);
+		lin1m[i] = scl(l1m, a);
+		ang1m[i] = scl(a1m, a);
+	}
+}
+
+/* One joint's rows for this step: the solverPrep rows, then
+ * setupSolverConstraintStep (lowleveldynamics/src/DyTGSContactPrep.cpp:
+ * 1939-2164) with preprocessRows (DyConstraintSetup.cpp:492-597): angular
+ * equality rows take the rotational hint, rows sorted by hint, each group
+ * orthogonalized; per row the unit response, the hard-constraint solver
+ * constants (compute1dConstraintSolverConstantsTGS,
+ * lowleveldynamics/shared/DyCpuGpu1dConstraint.h:418-486, erp = half the
+ * bias coefficient), the maximum bias (computeMaxBiasVelocityTGS, :332-363)
+ * and the orthogonal-basis terms of the rotational rows. */
+static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody* sb_, float invStepDt, float invDt,
+                       float erp, float lengthScale)
+{
+	row1d rows[6];
+	v3 ra, rb;
+	const int n = joint_rows(j, bA, bB, rows, &ra, &rb);
+	j->count = n;
+	j->invMass0 = sa->invMass;
+	j->invMass1 = sb_->invMass;
+	j->rA = ra;
+	j->rB = rb;
+	for (int k = 0; k < 3; k++)
+	{
+		j->o0[k] = V(0, 0, 0); j->o1[k] = V(0, 0, 0);
+		j->orecip[k] = 0.0f; j->oerr[k] = 0.0f;
+	}
+	for (int i = 0; i < n; i++)
+		if (rows[i].angular)
+			rows[i].hint = HINT_ROT_EQ;
+	/* insertion sort by hint (stable) */
+	row1d* sorted[6];
+	for (int i = 0; i < n; i++)
+	{
+		int k = i;
+		for (; k > 0 && rows[i].hint < sorted[k - 1]->hint; k--)
+			sorted[k] = sorted[k - 1];
+		sorted[k] = &rows[i];
+	}
+	v3 as0[6], as1[6];
+	for (int i = 0; i < n; i++)
+	{
+		as0[i] = mmul(sa->sqrtInvI, sorted[i]->ang0);
+		as1[i] = mmul(sb_->sqrtInvI, sorted[i]->ang1);
+	}
+	for (int i = 0; i < n;)
+	{
+		const int start = i++;
+		while (i < n && sorted[i]->hint == sorted[start]->hint)
+			i++;
+		orthogonalize_rows(sorted + start, as0 + start, as1 + start, i - start, sa->invMass, sb_->invMass);
+	}
+	int ortho = 0;
+	for (int i = 0; i < n; i++)
+	{
+		const row1d* c = sorted[i];
+		jrow* s = &j->r[i];
+		s->lin0 = c->lin0; s->lin1 = c->lin1; s->ang0 = c->ang0; s->ang1 = c->ang1;
+		s->minImp = c->minImp; s->maxImp = c->maxImp;
+		s->applied = 0.0f;
+		s->flags = JF_ANGULAR_ERROR;
+		const float linSum = len2(s->lin0) * sa->invMass + len2(s->lin1) * sb_->invMass;
+		const float resp0 = len2(as0[i]), resp1 = len2(as1[i]);
+		const float unit = resp0 + resp1 + linSum;
+		if (!c->angular)
+		{
+			s->ang0 = V(0, 0, 0);
+			s->ang1 = V(0, 0, 0);
+			s->flags &= ~JF_ANGULAR_ERROR;
+		}
+		const float recip = unit <= 0.0f ? 0.0f : 1.0f / unit;
+		s->maxBias = c->angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+		s->velMul = -1.0f;
+		s->biasScale = -invStepDt * erp;
+		s->error = c->geomErr * s->biasScale;
+		s->velTarget = c->velTarget - s->velMul * 0.0f;
+		if (c->hint == HINT_ROT_EQ)
+		{
+			s->flags |= JF_ROT_EQ;
+			j->o0[ortho] = as0[i];
+			j->orecip[ortho] = recip;
+			j->o1[ortho] = as1[i];
+			j->oerr[ortho] = c->geomErr;
+			ortho++;
+		}
+		else
+			s->flags |= JF_ORTHO_TARGET;
+	}
+}
+
+/* solve1DStep (lowleveldynamics/src/DyTGSContactPrep.cpp:2418-2747): the
+ * anchor offsets rotated by the bodies' rotation since the step began,
+ * linear rows' angular terms recomputed from them, the rotational basis
+ * projected out of the linear rows, the error resolved so far from the
+ * bodies' accumulated motion, the response recomputed, and the clamped
+ * impulse applied. conclude: conclude1DStep (:2750-2780) after the row is
+ * solved in the last position iteration. */
+static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int conclude)
+{
+	(void)elapsed;   /* target velocities are zero */
+	const v3 ra = qrot(b0->dq, j->rA);
+	const v3 rb = qrot(b1->dq, j->rB);
+	const v3 raMotion = sub(add(ra, b0->dlin), j->rA);
+	const v3 rbMotion = sub(add(rb, b1->dlin), j->rB);
+	float oerr[3];
+	for (int k = 0; k < 3; k++)
+		oerr[k] = j->oerr[k] + (dot(j->o0[k], b0->dang) - dot(j->o1[k], b1->dang));
+	v3 l0 = b0->lin, l1 = b1->lin, a0 = b0->ang, a1 = b1->ang;
+	const float im0 = j->invMass0, im1 = j->invMass1;
+	for (int i = 0; i < j->count; i++)
+	{
+		jrow* c = &j->r[i];
+		const v3 cang0 = add(c->ang0, cross(ra, c->lin0));
+		const v3 cang1 = add(c->ang1, cross(rb, c->lin1));
+		float error = c->error;
+		const float minBias = (c->flags & JF_INEQUALITY) ? -3.402823466e+38f : -c->maxBias;
+		v3 raXnI = mmul(b0->sqrtInvI, cang0);
+		v3 rbXnI = mmul(b1->sqrtInvI, cang1);
+		if (c->flags & JF_ORTHO_TARGET)
+		{
+			float proj[3];
+			for (int k = 0; k < 3; k++)
+			{
+				const v3 pp = add(mulv(raXnI, j->o0[k]), mulv(rbXnI, j->o1[k]));
+				proj[k] = (pp.x + pp.y + pp.z) * j->orecip[k];
+			}
+			const v3 d0 = add(scl(j->o0[0], proj[0]), add(scl(j->o0[1], proj[1]), scl(j->o0[2], proj[2])));
+			const v3 d1 = add(scl(j->o1[0], proj[0]), add(scl(j->o1[1], proj[1]), scl(j->o1[2], proj[2])));
+			raXnI = sub(raXnI, d0);
+			rbXnI = sub(rbXnI, d1);
+			const float ob = c->biasScale * (oerr[0] * proj[0] + (oerr[1] * proj[1] + oerr[2] * proj[2]));
+			error = error - ob;
+		}
+		const float angScale = (c->flags & JF_ANGULAR_ERROR) ? 1.0f : 0.0f;
+		const float dAng = angScale * (dot(raXnI, b0->dang) - dot(rbXnI, b1->dang));
+		const float dLin = dot(c->lin0, raMotion) - dot(c->lin1, rbMotion);
+		const float errorChange = (dLin + dAng) - c->velTarget * elapsed;
+		const v3 r0 = mulv(raXnI, raXnI), r1 = mulv(rbXnI, rbXnI);
+		const float resp0 = im0 * dot(c->lin0, c->lin0) + (r0.x + r0.y + r0.z);
+		const float resp1 = im1 * dot(c->lin1, c->lin1) - -(r1.x + r1.y + r1.z);
+		const float response = resp0 + resp1;
+		const float recip = response > 0.0f ? 1.0f / response : 0.0f;
+		const float vMul = recip * c->velMul;
+		const float ub = errorChange * c->biasScale + error;
+		const float bias = fminf(fmaxf(ub, minBias), c->maxBias);
+		const float constant = recip * (bias + c->velTarget);
+		const v3 v0 = add(mulv(l0, c->lin0), mulv(a0, raXnI));
+		const v3 v1 = add(mulv(l1, c->lin1), mulv(a1, rbXnI));
+		const v3 dv = sub(v0, v1);
+		const float normalVel = dv.x + dv.y + dv.z;
+		const float unclamped = c->applied + (vMul * normalVel + constant);
+		const float clamped = fminf(fmaxf(unclamped, c->minImp), c->maxImp);
+		const float df = clamped - c->applied;
+		c->applied = clamped;
+		l0 = add(scl(c->lin0, df * im0), l0);
+		l1 = sub(l1, scl(c->lin1, df * im1));
+		a0 = add(scl(raXnI, df), a0);
+		a1 = add(scl(rbXnI, df * -1.0f), a1);
+		if (conclude && !(c->flags & JF_KEEP_BIAS))
+		{
+			c->biasScale = 0.0f;
+			c->error = 0.0f;
+		}
+	}
+	b0->lin = l0; b0->ang = a0;
+	b1->lin = l1; b1->ang = a1;
+}
+
+int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
+                         const float* frame0, const float* frame1)
+{
+	pxo_world* w = (pxo_world*)wp;
+	if (k <= 0)
+		return w->nj;
+	for (int i = 0; i < k; i++)
+	{
+		if (type[i] < 0 || type[i] > PXO_JOINT_PRISMATIC || body0[i] >= w->n || body1[i] >= w->n ||
+		    (body0[i] < 0 && body1[i] < 0) || body0[i] == body1[i] || body0[i] < -1 || body1[i] < -1)
+			return -1;
+	}
+	if (w->nj + k > w->jcap)
+	{
+		w->jcap = (w->nj + k) * 2;
+		w->joint = (pxo_joint*)realloc(w->joint, sizeof(pxo_joint) * (size_t)w->jcap);
+		w->jkeys = (uint64_t*)realloc(w->jkeys, sizeof(uint64_t) * (size_t)w->jcap);
+	}
+	const int first = w->nj;
+	for (int i = 0; i < k; i++)
+	{
+		pxo_joint* j = &w->joint[w->nj++];
+		memset(j, 0, sizeof(*j));
+		j->type = type[i];
+		j->a = body0[i];
+		j->b = body1[i];
+		j->f0.p = V(frame0[7 * i], frame0[7 * i + 1], frame0[7 * i + 2]);
+		j->f0.q.x = frame0[7 * i + 3]; j->f0.q.y = frame0[7 * i + 4]; j->f0.q.z = frame0[7 * i + 5]; j->f0.q.w = frame0[7 * i + 6];
+		j->f1.p = V(frame1[7 * i], frame1[7 * i + 1], frame1[7 * i + 2]);
+		j->f1.q.x = frame1[7 * i + 3]; j->f1.q.y = frame1[7 * i + 4]; j->f1.q.z = frame1[7 * i + 5]; j->f1.q.w = frame1[7 * i + 6];
+	}
+	/* colours (the GPU host computes the same): joints between two bodies
+	 * first fit, newest joint first (the order in which the reference's
+	 * island lists them: its partitions of a jointed chain come out so),
+	 * over the colours each body's joints hold; then each joint to the world
+	 * takes the colour after every colour of its body's other joints, one
+	 * colour per such joint in joint order, as the reference partitions
+	 * constraints against static bodies after their body's dynamic ones
+	 * (lowleveldynamics/src/DyConstraintPartition.cpp:168-179, 604-616) */
+	uint64_t* used = (uint64_t*)calloc((size_t)(w->n ? w->n : 1), sizeof(uint64_t));
+	int* next = (int*)calloc((size_t)(w->n ? w->n : 1), sizeof(int));
+	w->njcolors = 0;
+	for (int i = w->nj - 1; i >= 0; i--)
+	{
+		pxo_joint* j = &w->joint[i];
+		if (j->a < 0 || j->b < 0)
+			continue;
+		const uint64_t u = used[j->a] | used[j->b];
+		if (u == ~0ull)
+		{
+			free(used);
+			free(next);
+			w->nj = first;
+			return -1;
+		}
+		int c = 0;
+		while ((u >> c) & 1ull)
+			c++;
+		j->color = c;
+		used[j->a] |= 1ull << c;
+		used[j->b] |= 1ull << c;
+		if (c + 1 > next[j->a]) next[j->a] = c + 1;
+		if (c + 1 > next[j->b]) next[j->b] = c + 1;
+		if (c + 1 > w->njcolors) w->njcolors = c + 1;
+	}
+	for (int i = 0; i < w->nj; i++)
+	{
+		pxo_joint* j = &w->joint[i];
+		if (j->a >= 0 && j->b >= 0)
+			continue;
+		const int body = j->a >= 0 ? j->a : j->b;
+		j->color = next[body]++;
+		if (j->color + 1 > w->njcolors) w->njcolors = j->color + 1;
+	}
+	free(used);
+	free(next);
+	int nk = 0;
+	for (int i = 0; i < w->nj; i++)
+	{
+		const pxo_joint* j = &w->joint[i];
+		if (j->a < 0 || j->b < 0)
+			continue;
+		const uint32_t lo = (uint32_t)(j->a < j->b ? j->a : j->b), hi = (uint32_t)(j->a < j->b ? j->b : j->a);
+		w->jkeys[nk++] = ((uint64_t)lo << 32) | hi;
+	}
+	qsort(w->jkeys, (size_t)nk, sizeof(uint64_t), cmp_u64);
+	/* jointed() searches nj keys: pad with keys no pair has */
+	for (int i = nk; i < w->nj; i++)
+		w->jkeys[i] = ~0ull;
+	return first;
 }
 
 int pxo_world_step(void* wp, float dt)
@@ -3374,6 +3855,25 @@ int pxo_world_step(void* wp, float dt)
 	const float biasCoefficient = 2.0f * sqrtf(1.0f / (float)P);
 	const float scale = fminf(0.8f, biasCoefficient);
 	const int pgs = d->solver == 0;
+	if (pgs && w->nj > 0)
+	{
+		free(sb);
+		free(frame);
+		return -5;   /* joints: TGS only */
+	}
+	/* joint rows (DyTGSDynamics.cpp:1180-1256): poses at the start of the
+	 * step, the world frame for a missing body; erp = half the bias
+	 * coefficient (DyTGSContactPrep.cpp:2018) */
+	{
+		const xf id = {{0.0f, 0.0f, 0.0f, 1.0f}, {0.0f, 0.0f, 0.0f}};
+		for (int i = 0; i < w->nj && !pgs; i++)
+		{
+			pxo_joint* j = &w->joint[i];
+			prep_joint(j, j->a >= 0 ? w->pose[j->a] : id, j->b >= 0 ? w->pose[j->b] : id,
+			           j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, invStepDt, 1.0f / dt,
+			           0.5f * biasCoefficient, d->tolerance_length);
+		}
+	}
 	if (pgs)
 		for (int i = 0; i < n; i++)
 		{
@@ -3436,6 +3936,16 @@ int pxo_world_step(void* wp, float dt)
 	}
 	for (int it = 0; it < P && !pgs; it++)
 	{
+		/* joints by colour, then contacts by colour; the last position
+		 * iteration concludes the joint rows */
+		for (int r = 0; r < w->njcolors; r++)
+			for (int i = 0; i < w->nj; i++)
+			{
+				pxo_joint* j = &w->joint[i];
+				if (j->color != r) continue;
+				solve_joint(j, j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, elapsed,
+				            it == P - 1);
+			}
 		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
 			{
@@ -3448,6 +3958,14 @@ int pxo_world_step(void* wp, float dt)
 		elapsed += stepDt;
 	}
 	for (int it = 0; it < Vi && !pgs; it++)
+	{
+		for (int r = 0; r < w->njcolors; r++)
+			for (int i = 0; i < w->nj; i++)
+			{
+				pxo_joint* j = &w->joint[i];
+				if (j->color != r) continue;
+				solve_joint(j, j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, elapsed, 0);
+			}
 		for (int r = 0; r < ncolors; r++)
 			for (long long o = start[r]; o < start[r + 1]; o++)
 			{
@@ -3455,6 +3973,7 @@ int pxo_world_step(void* wp, float dt)
 				solve_con(k, &sb[k->a], k->b >= 0 ? &sb[k->b] : &worldBody, 0.0f, elapsed, d->static_friction,
 				          d->dynamic_friction);
 			}
+	}
 	free(order);
 	free(start);
 

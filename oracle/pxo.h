@@ -84,6 +84,14 @@ int   pxo_world_step(void* w, float dt);
 void  pxo_world_get_bodies(void* w, float* pose, float* lin_vel, float* ang_vel);
 void  pxo_world_set_bodies(void* w, const float* pose, const float* lin_vel, const float* ang_vel);
 void  pxo_world_set_forces(void* w, const float* force, const float* torque);
+/* Joints without limits or drives (PxFixedJointCreate, PxSphericalJointCreate,
+ * PxRevoluteJointCreate, PxPrismaticJointCreate): body0/body1 body indices
+ * or -1 for the world frame; frame0/frame1 k x 7 local frames (p xyz, q xyzw)
+ * in each body's frame (the world's: world coordinates). Jointed bodies do
+ * not collide. TGS only. Returns the first joint's index or -1. */
+enum { PXO_JOINT_FIXED = 0, PXO_JOINT_SPHERICAL = 1, PXO_JOINT_REVOLUTE = 2, PXO_JOINT_PRISMATIC = 3 };
+int   pxo_world_add_joints(void* w, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
+                           const float* frame0, const float* frame1);
 /* out: [0] BP pairs, [1] body-plane pairs tested, [2] touching pairs (constraints),
  * [3] contact points, [4] colours, [5] friction anchors */
 void  pxo_world_stats(void* w, long long* out);

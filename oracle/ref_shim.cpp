@@ -136,6 +136,7 @@ struct RefScene
 	PxMaterial* material = nullptr;
 	std::vector<PxRigidDynamic*> bodies;
 	int nPlanes = 0;
+	int joints = 0;
 	ContactCollector* contacts = nullptr;
 };
 
@@ -420,6 +421,37 @@ void pxref_scene_add_forces(void* h, const float* force, const float* torque)
 		if (torque)
 			rs->bodies[i]->addTorque(PxVec3(torque[3 * i], torque[3 * i + 1], torque[3 * i + 2]), PxForceMode::eFORCE);
 	}
+}
+
+// Adds k joints without limits or drives: type 0 fixed, 1 spherical,
+// 2 revolute, 3 prismatic (PxFixedJointCreate, PxSphericalJointCreate,
+// PxRevoluteJointCreate, PxPrismaticJointCreate); body0/body1: indices of
+// bodies added by pxref_scene_add_bodies or -1 for the world; frame0/frame1:
+// k x 7 local frames (px py pz qx qy qz qw). Collision between the jointed
+// bodies stays off (the joints' default). Returns the joint count or -1.
+int pxref_scene_add_joints(void* h, int k, const int* type, const int* body0, const int* body1, const float* frame0,
+                           const float* frame1)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (int i = 0; i < k; i++)
+	{
+		PxRigidActor* a0 = body0[i] >= 0 ? rs->bodies[size_t(body0[i])] : nullptr;
+		PxRigidActor* a1 = body1[i] >= 0 ? rs->bodies[size_t(body1[i])] : nullptr;
+		const PxTransform f0 = makePose(frame0 + 7 * i), f1 = makePose(frame1 + 7 * i);
+		PxJoint* j = nullptr;
+		switch (type[i])
+		{
+		case 0: j = PxFixedJointCreate(*gPhysics, a0, f0, a1, f1); break;
+		case 1: j = PxSphericalJointCreate(*gPhysics, a0, f0, a1, f1); break;
+		case 2: j = PxRevoluteJointCreate(*gPhysics, a0, f0, a1, f1); break;
+		case 3: j = PxPrismaticJointCreate(*gPhysics, a0, f0, a1, f1); break;
+		default: return -1;
+		}
+		if (!j)
+			return -1;
+		rs->joints++;
+	}
+	return rs->joints;
 }
 
 // pose: n x 7, linVel/angVel: n x 3 (any may be null)

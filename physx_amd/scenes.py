@@ -17,6 +17,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SPHERE, PLANE, CAPSULE, BOX, CONVEX = 0, 1, 2, 3, 5
+JOINT_FIXED, JOINT_SPHERICAL, JOINT_REVOLUTE, JOINT_PRISMATIC = 0, 1, 2, 3
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -58,6 +59,9 @@ class Scene:
     params: dict = field(default_factory=dict)
     hull_points: np.ndarray | None = None   # (P,3) f32: point clouds of the convex meshes, concatenated
     hull_counts: np.ndarray | None = None   # (H,) int32: points per mesh
+    # joints: type (k,) int32 (JOINT_*), body0 / body1 (k,) int32 (-1: the
+    # world frame), frame0 / frame1 (k,7) f32 local frames (p xyz, q xyzw)
+    joints: dict | None = None
 
     @property
     def n(self) -> int:
@@ -72,11 +76,25 @@ class Scene:
         b = (self.scene_id >= w0) & (self.scene_id < w1)
         p = (self.plane_scene >= w0) & (self.plane_scene < w1)
         name = f"{self.name}[{w0}]" if w1 == w0 + 1 else f"{self.name}[{w0}:{w1}]"
+        joints = None
+        if self.joints is not None:
+            # joints of bodies in the range (a joint to the world belongs to
+            # its body's world), body indices renumbered
+            new = np.full(self.n, -1, np.int64)
+            new[b] = np.arange(int(b.sum()))
+            j0, j1 = self.joints["body0"], self.joints["body1"]
+            keep = np.where(j0 >= 0, b[np.maximum(j0, 0)], True) & np.where(j1 >= 0, b[np.maximum(j1, 0)], True)
+            keep &= (j0 >= 0) | (j1 >= 0)
+            keep &= np.where(j0 >= 0, b[np.maximum(j0, 0)], b[np.maximum(j1, 0)])
+            joints = {"type": self.joints["type"][keep].copy(),
+                      "body0": np.where(j0[keep] >= 0, new[np.maximum(j0[keep], 0)], -1).astype(np.int32),
+                      "body1": np.where(j1[keep] >= 0, new[np.maximum(j1[keep], 0)], -1).astype(np.int32),
+                      "frame0": self.joints["frame0"][keep].copy(), "frame1": self.joints["frame1"][keep].copy()}
         return Scene(self.shape_type[b].copy(), self.dims[b].copy(), self.pose[b].copy(),
                      self.lin_vel[b].copy(), self.ang_vel[b].copy(), self.mass[b].copy(),
                      self.inertia[b].copy(), (self.scene_id[b] - w0).astype(np.int32),
                      self.planes[p].copy(), (self.plane_scene[p] - w0).astype(np.int32), w1 - w0,
-                     name, dict(self.params), self.hull_points, self.hull_counts)
+                     name, dict(self.params), self.hull_points, self.hull_counts, joints)
 
 
 def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
@@ -319,3 +337,145 @@ def mixed_pile(n_bodies=512, per_side=8, pitch=1.0, jitter=0.1, n_points=16, see
                  np.array([[0, 1, 0, 0]], np.float32), np.zeros(1, np.int32), 1, "mixed_pile",
                  dict(n_bodies=n_bodies, per_side=per_side, pitch=pitch, jitter=jitter, n_points=n_points, seed=seed),
                  local.reshape(-1, 3), np.full(nc, n_points, np.int32))
+
+
+def _quat_axis_angle(axis, angle):
+    axis = np.asarray(axis, np.float64)
+    axis = axis / np.linalg.norm(axis, axis=-1, keepdims=True)
+    h = 0.5 * np.asarray(angle, np.float64)[..., None]
+    return np.concatenate([axis * np.sin(h), np.cos(h)], -1)
+
+
+def joint_singles(n_groups=64, seed=5) -> Scene:
+    """Independent jointed groups 3 m apart (no contacts): even groups
+    one body joined to the world, odd groups two bodies joined to each other.
+    Kinds cycle fixed, spherical, revolute, prismatic; bodies are boxes,
+    spheres and capsules with random orientation and velocity, and the second
+    frame is placed off the first by up to 5 cm / 0.1 rad so each joint starts
+    with an error to correct. No gravity-free assumption: the step runs with
+    gravity. Each body is in exactly one joint, so one step does not depend on
+    the solver's constraint order."""
+    types, dims, pos, quat, lv, av = [], [], [], [], [], []
+    jt, jb0, jb1, jf0, jf1 = [], [], [], [], []
+    for g in range(n_groups):
+        ids = np.array([g])
+        u = lambda d, lo=0.0, hi=1.0: float(uniform(seed, ids, d, lo, hi)[0])  # noqa: E731
+        kind = g // 2 % 4
+        nb = 1 if g % 2 == 0 else 2
+        # near the origin: a joint's error over the step dt amplifies the
+        # rounding of positions, which grows with their magnitude
+        base = np.array([3.0 * (g % 8 - 3.5), 1.0, 3.0 * (g // 8 - 3.5)])
+        first = len(types)
+        for k in range(nb):
+            st = [BOX, SPHERE, CAPSULE][(g + k) % 3]
+            types.append(st)
+            if st == BOX:
+                dims.append([u(10 + k, 0.1, 0.4), u(12 + k, 0.1, 0.4), u(14 + k, 0.1, 0.4), 0.0])
+            elif st == SPHERE:
+                dims.append([u(10 + k, 0.15, 0.4), 0.0, 0.0, 0.0])
+            else:
+                dims.append([u(10 + k, 0.1, 0.25), u(12 + k, 0.15, 0.4), 0.0, 0.0])
+            pos.append(base + np.array([1.2 * k, 0.0, 0.0]))
+            quat.append(_quat_axis_angle([u(20 + k, -1, 1), u(22 + k, -1, 1), u(24 + k, -1, 1)], u(26 + k, 0, 3.0)))
+            lv.append([u(30 + k, -2, 2), u(32 + k, -2, 2), u(34 + k, -2, 2)])
+            av.append([u(36 + k, -3, 3), u(38 + k, -3, 3), u(40 + k, -3, 3)])
+        # joint frame in the world: an anchor between / beside the bodies
+        anchor = base + np.array([0.6 if nb == 2 else -0.5, 0.1, 0.0])
+        jq = _quat_axis_angle([u(50, -1, 1), u(51, -1, 1), u(52, -1, 1)], u(53, 0, 3.0))
+        err_p = np.array([u(54, -0.05, 0.05), u(55, -0.05, 0.05), u(56, -0.05, 0.05)])
+        err_q = _quat_axis_angle([u(57, -1, 1), u(58, -1, 1), u(59, -1, 1)], u(60, 0, 0.1))
+
+        def local(body, p_w, q_w):
+            """the world frame (p_w, q_w) in body `body`'s frame"""
+            qb = np.asarray(quat[body], np.float64)
+            inv = np.array([-qb[0], -qb[1], -qb[2], qb[3]])
+            return np.concatenate([_qrot(inv, p_w - pos[body]), _qmul(inv, q_w)])
+
+        world_f = np.concatenate([anchor, jq])
+        q_b = _qmul(err_q, jq)
+        if nb == 1:
+            if g % 4 == 0:      # the world as body 0
+                jb0.append(-1); jb1.append(first)
+                jf0.append(world_f); jf1.append(local(first, anchor + err_p, q_b))
+            else:               # the world as body 1
+                jb0.append(first); jb1.append(-1)
+                jf0.append(local(first, anchor + err_p, q_b)); jf1.append(world_f)
+        else:
+            jb0.append(first); jb1.append(first + 1)
+            jf0.append(local(first, anchor, jq)); jf1.append(local(first + 1, anchor + err_p, q_b))
+        jt.append(kind)
+    st = np.array(types, np.int32)
+    sc = _finish(st, np.array(dims), np.array(pos), np.array(quat), np.zeros(len(types), np.int32),
+                 np.zeros((0, 4)), np.zeros(0, np.int32), 1, "joint_singles", {"n_groups": n_groups, "seed": seed})
+    sc.lin_vel = np.array(lv, np.float32)
+    sc.ang_vel = np.array(av, np.float32)
+    sc.joints = {"type": np.array(jt, np.int32), "body0": np.array(jb0, np.int32), "body1": np.array(jb1, np.int32),
+                 "frame0": np.array(jf0, np.float32), "frame1": np.array(jf1, np.float32)}
+    return sc
+
+
+def _qmul(a, b):
+    ax, ay, az, aw = a
+    bx, by, bz, bw = b
+    return np.array([aw * bx + bw * ax + ay * bz - by * az, aw * by + bw * ay + az * bx - bz * ax,
+                     aw * bz + bw * az + ax * by - bx * ay, aw * bw - ax * bx - ay * by - az * bz])
+
+
+def _qrot(q, v):
+    u = np.asarray(q[:3], np.float64)
+    w = q[3]
+    v = np.asarray(v, np.float64)
+    return v + 2.0 * w * np.cross(u, v) + 2.0 * np.cross(u, np.cross(u, v))
+
+
+def jointed_chains(n_scenes=8, links=12, drops=6, seed=3) -> Scene:
+    """Chains of boxes (half extents 0.25 x 0.08 x 0.08 m) laid along x at
+    2.5 m above a floor, the first link joined to the world at its free end,
+    each link to the next at their shared end (seeded jitter of 2 mm / 0.02
+    rad per link, so joints start with small errors). World w's joints are
+    all of kind w % 4 (fixed: a sagging cantilever; spherical; revolute about
+    z; prismatic along x, which lets the chain slide). Each world also drops
+    `drops` boxes onto its chain, so contacts and joints act on the same
+    bodies. Worlds are independent."""
+    types, dims, pos, quat, wid = [], [], [], [], []
+    jt, jb0, jb1, jf0, jf1 = [], [], [], [], []
+    qz = np.array([0.0, -math.sin(math.pi / 4), 0.0, math.cos(math.pi / 4)])   # frame x axis -> world z
+    planes, pscene = [], []
+    for w in range(n_scenes):
+        kind = w % 4
+        # revolute: twist (frame x) about world z; the others: identity frames
+        jq = qz if kind == JOINT_REVOLUTE else np.array([0.0, 0.0, 0.0, 1.0])
+        first = len(types)
+        for i in range(links):
+            ids = np.array([w * 1000 + i])
+            jit = np.array([uniform(seed, ids, d, -0.002, 0.002)[0] for d in range(3)])
+            types.append(BOX)
+            dims.append([0.25, 0.08, 0.08, 0.0])
+            pos.append(np.array([0.5 * i, 2.5, 0.0]) + jit)
+            quat.append(_quat_axis_angle([0.0, 0.0, 1.0], uniform(seed, ids, 3, -0.02, 0.02)[0]))
+            wid.append(w)
+        for i in range(links):
+            if i == 0:
+                jb0.append(-1); jb1.append(first)
+                jf0.append(np.concatenate([[-0.25, 2.5, 0.0], jq])); jf1.append(np.concatenate([[-0.25, 0.0, 0.0], jq]))
+            else:
+                jb0.append(first + i - 1); jb1.append(first + i)
+                jf0.append(np.concatenate([[0.25, 0.0, 0.0], jq])); jf1.append(np.concatenate([[-0.25, 0.0, 0.0], jq]))
+            jt.append(kind)
+        for k in range(drops):
+            ids = np.array([w * 1000 + 500 + k])
+            types.append(BOX)
+            h = [uniform(seed, ids, d, 0.1, 0.2)[0] for d in range(3)]
+            dims.append(h + [0.0])
+            pos.append(np.array([0.5 + 0.9 * k, 3.2 + 0.5 * (k % 2), uniform(seed, ids, 4, -0.05, 0.05)[0]]))
+            quat.append(_quat_axis_angle([0.0, 1.0, 0.0], uniform(seed, ids, 5, 0.0, 3.0)[0]))
+            wid.append(w)
+        planes.append([0.0, 1.0, 0.0, 0.0])
+        pscene.append(w)
+    st = np.array(types, np.int32)
+    sc = _finish(st, np.array(dims), np.array(pos), np.array(quat), np.array(wid), np.array(planes),
+                 np.array(pscene), n_scenes, "jointed_chains",
+                 {"n_scenes": n_scenes, "links": links, "drops": drops, "seed": seed})
+    sc.joints = {"type": np.array(jt, np.int32), "body0": np.array(jb0, np.int32), "body1": np.array(jb1, np.int32),
+                 "frame0": np.array(jf0, np.float32), "frame1": np.array(jf1, np.float32)}
+    return sc

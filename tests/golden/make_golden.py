@@ -9,6 +9,10 @@ reference's own outputs on seeded inputs:
   broadphase.npz  PxCreateBroadPhase(eABP) created pairs for seeded boxes
   step.npz      one PxScene TGS step (4/1 iterations, dt 1/60) of single bodies
                 on a plane: positions, orientations, velocities afterwards
+  joints.npz    one PxScene TGS step of independent joints (fixed,
+                spherical, revolute, prismatic; one body to the world or two
+                bodies to each other, each starting with an error) at 4/1,
+                8/1 and 1/0 iterations: the inputs and every body afterwards
 
     python tests/golden/make_golden.py
 """
@@ -160,10 +164,28 @@ def make_step(dt=1.0 / 60.0):
                         dt=np.float32(dt), pos_iters=np.int32(4), vel_iters=np.int32(1))
 
 
+JOINT_ITERS = ((4, 1), (8, 1), (1, 0))
+
+
+def make_joints(dt=1.0 / 60.0, n_groups=64, seed=5):
+    sc = S.joint_singles(n_groups, seed)
+    out = {"type": sc.shape_type, "dims": sc.dims, "pose": sc.pose, "lv": sc.lin_vel, "av": sc.ang_vel,
+           "mass": sc.mass, "inertia": sc.inertia, "jtype": sc.joints["type"], "jbody0": sc.joints["body0"],
+           "jbody1": sc.joints["body1"], "jframe0": sc.joints["frame0"], "jframe1": sc.joints["frame1"],
+           "iters": np.array(JOINT_ITERS, np.int32), "dt": np.float32(dt)}
+    for k, (p_it, v_it) in enumerate(JOINT_ITERS):
+        r = L.RefScene(sc, "tgs", p_it, v_it, threads=1)
+        r.step(dt)
+        p1, l1, a1 = r.bodies()
+        out[f"pose1_{k}"], out[f"lv1_{k}"], out[f"av1_{k}"] = p1, l1, a1
+    np.savez_compressed(os.path.join(HERE, "joints.npz"), **out)
+
+
 if __name__ == "__main__":
     assert L.have_ref(), "build oracle/_ref/libpxref.so first (make -f oracle/ref.mk -j8)"
     make_contacts()
     make_broadphase()
     make_step()
-    for f in ("contacts.npz", "broadphase.npz", "step.npz"):
+    make_joints()
+    for f in ("contacts.npz", "broadphase.npz", "step.npz", "joints.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))

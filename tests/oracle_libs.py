@@ -275,10 +275,18 @@ class OracleWorld:
         pl, ps = f32(scene.planes), i32(scene.plane_scene)
         self._keep += [pl, ps]
         self.lib.pxo_world_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps))
+        j = getattr(scene, "joints", None)
+        if j is not None and len(j["type"]):
+            arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
+            self._keep += arrs
+            r = self.lib.pxo_world_add_joints(self.h, int(arrs[0].shape[0]), ip(arrs[0]), ip(arrs[1]), ip(arrs[2]),
+                                              fp(arrs[3]), fp(arrs[4]))
+            assert r >= 0, "oracle rejected the joints"
 
     def step(self, dt, n=1):
         for _ in range(n):
-            self.lib.pxo_world_step(self.h, ctypes.c_float(dt))
+            r = self.lib.pxo_world_step(self.h, ctypes.c_float(dt))
+            assert r == 0, f"oracle step failed ({r})"
 
     def bodies(self):
         pose = np.zeros((self.n, 7), np.float32)
@@ -409,6 +417,12 @@ class RefScene:
         pl = f32(scene.planes)
         self._keep.append(pl)
         assert self.lib.pxref_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ctypes.c_float(0.02)) == 0
+        j = getattr(scene, "joints", None)
+        if j is not None and len(j["type"]):
+            arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
+            self._keep += arrs
+            assert self.lib.pxref_scene_add_joints(self.h, int(arrs[0].shape[0]), ip(arrs[0]), ip(arrs[1]),
+                                                   ip(arrs[2]), fp(arrs[3]), fp(arrs[4])) == arrs[0].shape[0]
 
     def add_forces(self, force=None, torque=None):
         """PxRigidDynamic::addForce / addTorque (eFORCE) on every body, (n,3) arrays."""
