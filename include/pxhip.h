@@ -105,6 +105,23 @@ int32_t pxh_scene_add_planes(pxh_scene scene, int32_t m, const float* plane, con
 int32_t pxh_scene_simulate(pxh_scene scene, float dt);
 int32_t pxh_scene_fetch_results(pxh_scene scene, int32_t block);
 
+/* Joints without limits, drives or break forces: PxFixedJointCreate,
+ * PxSphericalJointCreate, PxRevoluteJointCreate (free twist about the joint
+ * frame's x axis), PxPrismaticJointCreate (free translation along x)
+ * (physx/include/extensions/PxFixedJoint.h, PxSphericalJoint.h,
+ * PxRevoluteJoint.h, PxPrismaticJoint.h), solved as 1D constraint rows
+ * (the GPU reference's constraintBlockPrePrep.cu:1168,1219 and
+ * solve1DBlockTGS, gpusolver/src/CUDA/solverBlockTGS.cuh:744). Host arrays:
+ * type[k], body0[k] / body1[k] (body indices, -1 = the world frame, never
+ * both; bodies of one world), frame0[k*7] / frame1[k*7] local joint frames
+ * (p xyz, q xyzw) relative to each body (for the world: world coordinates).
+ * Jointed bodies do not collide (PxConstraintFlag::eCOLLISION_ENABLED off,
+ * the joints' default). TGS only (a PGS step with joints returns
+ * PXH_ERR_UNSUPPORTED). Returns the index of the first joint. */
+enum { PXH_JOINT_FIXED = 0, PXH_JOINT_SPHERICAL = 1, PXH_JOINT_REVOLUTE = 2, PXH_JOINT_PRISMATIC = 3 };
+int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, const int32_t* body0,
+                             const int32_t* body1, const float* frame0, const float* frame1);
+
 /* PxDirectGPUAPI::getRigidDynamicData / setRigidDynamicData
  * (physx/include/PxDirectGPUAPI.h:311,325; NpDirectGPUAPI.cpp:63-92).
  * data: device buffer of n PxTransform (7 floats) or PxVec3 (3 floats);

@@ -25,6 +25,7 @@ PXH_DATA_GLOBAL_POSE, PXH_DATA_LINEAR_VELOCITY, PXH_DATA_ANGULAR_VELOCITY = 0, 1
 PXH_READ_LINEAR_ACCELERATION, PXH_READ_ANGULAR_ACCELERATION = 3, 4
 PXH_WRITE_FORCE, PXH_WRITE_TORQUE = 3, 4
 PXH_BP_NOALLOC_MAX_BOXES, PXH_BP_NOALLOC_MAX_PAIRS = 65536, 262144
+PXH_JOINT_FIXED, PXH_JOINT_SPHERICAL, PXH_JOINT_REVOLUTE, PXH_JOINT_PRISMATIC = 0, 1, 2, 3
 
 _F = ctypes.POINTER(ctypes.c_float)
 _I32 = ctypes.POINTER(ctypes.c_int32)
@@ -61,7 +62,7 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
-            "pxh_scene_set_profiling_stride")
+            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints")
 
 _lib = None
 
@@ -154,11 +155,26 @@ class Scene:
         w = _i32(scene.scene_id)
         fp = lambda a: a.ctypes.data_as(_F)  # noqa: E731
         ip = lambda a: a.ctypes.data_as(_I32)  # noqa: E731
-        _check(self.lib.pxh_scene_add_bodies(self.h, scene.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m), fp(inn),
-                                             ip(w)))
+        base = _check(self.lib.pxh_scene_add_bodies(self.h, scene.n, ip(t), fp(dm), fp(po), fp(lv), fp(av), fp(m),
+                                                    fp(inn), ip(w)))
         pl, ps = _f32(scene.planes), _i32(scene.plane_scene)
         _check(self.lib.pxh_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps)))
         self.n += scene.n
+        j = getattr(scene, "joints", None)
+        if j is not None and len(j["type"]):
+            self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
+                            np.where(j["body1"] >= 0, j["body1"] + base, -1), j["frame0"], j["frame1"])
+
+    def add_joints(self, kind, body0, body1, frame0, frame1) -> int:
+        """pxh_scene_add_joints (PxFixedJointCreate / PxSphericalJointCreate /
+        PxRevoluteJointCreate / PxPrismaticJointCreate): host arrays; body -1
+        is the world frame; frames (k,7) p xyz, q xyzw. Returns the first
+        joint's index."""
+        t, b0, b1 = _i32(kind), _i32(body0), _i32(body1)
+        f0, f1 = _f32(frame0).reshape(-1, 7), _f32(frame1).reshape(-1, 7)
+        return _check(self.lib.pxh_scene_add_joints(self.h, int(t.size), t.ctypes.data_as(_I32),
+                                                    b0.ctypes.data_as(_I32), b1.ctypes.data_as(_I32),
+                                                    f0.ctypes.data_as(_F), f1.ctypes.data_as(_F)))
 
     def simulate(self, dt: float):
         _check(self.lib.pxh_scene_simulate(self.h, ctypes.c_float(dt)))

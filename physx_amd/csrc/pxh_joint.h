@@ -1,0 +1,364 @@
+// Joints on the device: the 1D constraint rows of fixed, spherical, revolute
+// and prismatic joints without limits or drives, for the TGS solver. One lane
+// per joint: the joint's solverPrep with ConstraintHelper::prepareLockedAxes
+// (physxextensions/src/ExtConstraintHelper.h:150-208, 243-297), then
+// setupSolverConstraintStep with preprocessRows / orthogonalize
+// (lowleveldynamics/src/DyTGSContactPrep.cpp:1939-2164,
+// DyConstraintSetup.cpp:313-597); each iteration solve1DStep
+// (DyTGSContactPrep.cpp:2418-2747) and, after the last position iteration,
+// conclude1DStep (:2750-2780). The GPU reference's counterparts are
+// constraintBlockPrePrep.cu:1168,1219 and solve1DBlockTGS
+// (gpusolver/src/CUDA/solverBlockTGS.cuh:744).
+//
+// Layout per joint j (in colour order):
+//   info[j]        int4  kind, body a, body b (-1: the world), colour
+//   frame[4j..+3]  float4 frame0 p, frame0 q, frame1 p, frame1 q
+//   head[8j..+7]   float4 (rA, invMass0), (rB, invMass1),
+//                         (angOrthoAxis0[k], recipResponse[k]) k = 0..2,
+//                         (angOrthoAxis1[k], error[k]) k = 0..2
+//   rows[36j + 6k ..+5] float4 per row k: (lin0, error), (lin1, biasScale),
+//                         (ang0, velMultiplier), (ang1, velTarget),
+//                         (minImpulse, maxImpulse, appliedForce, maxBias),
+//                         (flags as bits, 0, 0, 0)
+#pragma once
+#include "pxh_math.h"
+
+namespace pxh {
+
+enum { JOINT_FIXED = 0, JOINT_SPHERICAL = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
+enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
+enum { JHINT_ROT_EQ = 1024, JHINT_EQ = 2048 };   // PxConstraintSolveHint
+
+// locked linear / angular axes (bit masks x y z) of each kind
+PXH_D void jointAxes(int kind, int& lin, int& ang)
+{
+	lin = kind == JOINT_PRISMATIC ? 6 : 7;
+	ang = kind == JOINT_SPHERICAL ? 0 : (kind == JOINT_REVOLUTE ? 6 : 7);
+}
+
+PXH_D int jointRowCount(int kind)
+{
+	int lin, ang;
+	jointAxes(kind, lin, ang);
+	return __popc(lin) + __popc(ang);
+}
+
+struct Row1D
+{
+	f3 lin0, ang0, lin1, ang1;
+	float geomErr, velTarget, minImp, maxImp;
+	int hint, angular;
+};
+
+// computeJacobianAxes (ExtConstraintHelper.h:77-102)
+PXH_D void jacobianAxes(f3 row[3], qt qa, qt qb)
+{
+	const float wa = qa.w, wb = qb.w;
+	const f3 va = mk(qa.x, qa.y, qa.z), vb = mk(qb.x, qb.y, qb.z);
+	const f3 c = vb * wa + va * wb;
+	const float d0 = wa * wb;
+	const float d1 = dot(va, vb);
+	const float d = d0 - d1;
+	row[0] = (va * vb.x + vb * va.x + mk(d, c.z, -c.y)) * 0.5f;
+	row[1] = (va * vb.y + vb * va.y + mk(-c.z, d, c.x)) * 0.5f;
+	row[2] = (va * vb.z + vb * va.z + mk(c.y, -c.x, d)) * 0.5f;
+	if ((d0 + d1) == 0.0f)
+	{
+		row[0].x += 1.192092896e-07f;
+		row[1].y += 1.192092896e-07f;
+		row[2].z += 1.192092896e-07f;
+	}
+}
+
+// the joint's rows (solverPrep); returns the count, ra / rb the anchor offsets
+PXH_D int jointRows(int kind, const xf& bA, const xf& bB, const xf& f0, const xf& f1, Row1D* rows, f3& raOut,
+                    f3& rbOut)
+{
+	xf cA, cB;
+	cA.q = qmul(bA.q, f0.q);
+	cA.p = qrot(bA.q, f0.p) + bA.p;
+	cB.q = qmul(bB.q, f1.q);
+	cB.p = qrot(bB.q, f1.p) + bB.p;
+	if (cA.q.x * cB.q.x + cA.q.y * cB.q.y + cA.q.z * cB.q.z + cA.q.w * cB.q.w < 0.0f)
+	{
+		cB.q.x = -cB.q.x; cB.q.y = -cB.q.y; cB.q.z = -cB.q.z; cB.q.w = -cB.q.w;
+	}
+	f3 ra = cB.p - bA.p;
+	const f3 rb = cB.p - bB.p;
+	int lin, ang;
+	jointAxes(kind, lin, ang);
+	const f3 cB2cAp = qrotinv(cA.q, cB.p - cA.p);
+	for (int i = 0; i < 6; i++)
+	{
+		rows[i].lin0 = rows[i].ang0 = rows[i].lin1 = rows[i].ang1 = mk(0.0f, 0.0f, 0.0f);
+		rows[i].geomErr = rows[i].velTarget = 0.0f;
+		rows[i].minImp = -3.402823466e+38f;
+		rows[i].maxImp = 3.402823466e+38f;
+		rows[i].hint = 0;
+		rows[i].angular = 0;
+	}
+	int k = 0;
+	if (lin)
+	{
+		const m33 axes = qmat(cA.q);
+		f3 ev = mk(0.0f, 0.0f, 0.0f);
+		if (lin & 1) ev = ev - axes.c0 * cB2cAp.x;
+		if (lin & 2) ev = ev - axes.c1 * cB2cAp.y;
+		if (lin & 4) ev = ev - axes.c2 * cB2cAp.z;
+		ra = ra + ev;
+		for (int i = 0; i < 3; i++)
+			if (lin & (1 << i))
+			{
+				const f3 ax = i == 0 ? axes.c0 : (i == 1 ? axes.c1 : axes.c2);
+				Row1D& c = rows[k++];
+				c.hint = JHINT_EQ;
+				c.lin0 = ax;
+				c.ang0 = cross(ra, ax);
+				c.lin1 = ax;
+				c.ang1 = cross(rb, ax);
+				c.geomErr = -comp(cB2cAp, i);
+			}
+	}
+	if (ang)
+	{
+		qt ca; ca.x = -cA.q.x; ca.y = -cA.q.y; ca.z = -cA.q.z; ca.w = cA.q.w;
+		const qt qB2qA = qmul(ca, cB.q);
+		f3 row[3];
+		jacobianAxes(row, cA.q, cB.q);
+		for (int i = 0; i < 3; i++)
+			if (ang & (1 << i))
+			{
+				Row1D& c = rows[k++];
+				c.hint = JHINT_EQ;
+				c.ang0 = row[i];
+				c.ang1 = row[i];
+				c.geomErr = -(i == 0 ? qB2qA.x : (i == 1 ? qB2qA.y : qB2qA.z));
+				c.angular = 1;
+			}
+	}
+	raOut = ra;
+	rbOut = rb;
+	return k;
+}
+
+// orthogonalize over one group of equality rows (Gram-Schmidt in the
+// inverse-mass metric; angular inverse-mass scales are 1)
+PXH_D void orthogonalizeRows(Row1D* const* row, f3* as0, f3* as1, int count, float im0, float im1)
+{
+	f3 lin0m[6], ang0m[6], lin1m[6], ang1m[6], lin0[6], ang0[6], lin1[6], ang1[6];
+	float g[6], t[6];
+	for (int i = 0; i < count; i++)
+	{
+		f3 l0 = row[i]->lin0, a0 = row[i]->ang0, l1 = row[i]->lin1, a1 = row[i]->ang1;
+		float gi = row[i]->geomErr, ti = row[i]->velTarget;
+		f3 s0v = as0[i], s1v = as1[i];
+		for (int k = 0; k < i; k++)
+		{
+			const f3 sa = mulv(l1, lin1m[k]) + mulv(l0, lin0m[k]);
+			const f3 sb = mulv(s1v, ang1m[k]) + mulv(s0v, ang0m[k]);
+			const f3 ss = sa + sb;
+			const float tt = ss.x + ss.y + ss.z;
+			l0 = l0 - lin0[k] * tt;
+			gi = gi - g[k] * tt;
+			a0 = a0 - ang0[k] * tt;
+			ti = ti - t[k] * tt;
+			l1 = l1 - lin1[k] * tt;
+			a1 = a1 - ang1[k] * tt;
+			s0v = s0v - as0[k] * tt;
+			s1v = s1v - as1[k] * tt;
+		}
+		row[i]->lin0 = l0; row[i]->geomErr = gi;
+		row[i]->ang0 = a0; row[i]->velTarget = ti;
+		row[i]->lin1 = l1;
+		row[i]->ang1 = a1;
+		as0[i] = s0v;
+		as1[i] = s1v;
+		lin0[i] = l0; g[i] = gi; ang0[i] = a0; t[i] = ti; lin1[i] = l1; ang1[i] = a1;
+		const f3 l0m = l0 * im0, l1m = l1 * im1;
+		const f3 q0 = mulv(l0, l0m) + mulv(l1, l1m);
+		const f3 q1 = mulv(s0v, s0v) + mulv(s1v, s1v);
+		const f3 qq = q0 + q1;
+		const float sum = qq.x + qq.y + qq.z;
+		const float a = sum > 0.0f ? 1.0f / sum : 0.0f;
+		lin0m[i] = l0m * a;
+		ang0m[i] = s0v * a;
+		lin1m[i] = l1m * a;
+		ang1m[i] = s1v * a;
+	}
+}
+
+// prepare joint j's header and rows for this step. sqrtInvI0 / sqrtInvI1:
+// world-space square-root inverse inertia of the two bodies (zero for the
+// world); invMass0 / invMass1 likewise.
+PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const xf& f1, const m33& sI0, const m33& sI1,
+                     float invMass0, float invMass1, float invStepDt, float invDt, float erp, float lengthScale,
+                     float4* head, float4* rowsOut)
+{
+	Row1D rows[6];
+	f3 ra, rb;
+	const int n = jointRows(kind, bA, bB, f0, f1, rows, ra, rb);
+	for (int i = 0; i < n; i++)
+		if (rows[i].angular)
+			rows[i].hint = JHINT_ROT_EQ;
+	Row1D* sorted[6];
+	for (int i = 0; i < n; i++)
+	{
+		int k = i;
+		for (; k > 0 && rows[i].hint < sorted[k - 1]->hint; k--)
+			sorted[k] = sorted[k - 1];
+		sorted[k] = &rows[i];
+	}
+	f3 as0[6], as1[6];
+	for (int i = 0; i < n; i++)
+	{
+		as0[i] = mmul(sI0, sorted[i]->ang0);
+		as1[i] = mmul(sI1, sorted[i]->ang1);
+	}
+	for (int i = 0; i < n;)
+	{
+		const int start = i++;
+		while (i < n && sorted[i]->hint == sorted[start]->hint)
+			i++;
+		orthogonalizeRows(sorted + start, as0 + start, as1 + start, i - start, invMass0, invMass1);
+	}
+	f3 o0[3], o1[3];
+	float orecip[3], oerr[3];
+	for (int k = 0; k < 3; k++)
+	{
+		o0[k] = o1[k] = mk(0.0f, 0.0f, 0.0f);
+		orecip[k] = oerr[k] = 0.0f;
+	}
+	int ortho = 0;
+	for (int i = 0; i < n; i++)
+	{
+		const Row1D& c = *sorted[i];
+		int flags = JF_ANGULAR_ERROR;
+		const float linSum = dot(c.lin0, c.lin0) * invMass0 + dot(c.lin1, c.lin1) * invMass1;
+		const float resp0 = dot(as0[i], as0[i]), resp1 = dot(as1[i], as1[i]);
+		const float unit = resp0 + resp1 + linSum;
+		f3 ang0 = c.ang0, ang1 = c.ang1;
+		if (!c.angular)
+		{
+			ang0 = ang1 = mk(0.0f, 0.0f, 0.0f);
+			flags &= ~JF_ANGULAR_ERROR;
+		}
+		const float recip = unit <= 0.0f ? 0.0f : 1.0f / unit;
+		const float maxBias = c.angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+		const float velMul = -1.0f;
+		const float biasScale = -invStepDt * erp;
+		const float error = c.geomErr * biasScale;
+		const float velTarget = c.velTarget - velMul * 0.0f;
+		if (c.hint == JHINT_ROT_EQ)
+		{
+			flags |= JF_ROT_EQ;
+			o0[ortho] = as0[i];
+			orecip[ortho] = recip;
+			o1[ortho] = as1[i];
+			oerr[ortho] = c.geomErr;
+			ortho++;
+		}
+		else
+			flags |= JF_ORTHO_TARGET;
+		float4* r = rowsOut + 6 * i;
+		r[0] = f4(c.lin0, error);
+		r[1] = f4(c.lin1, biasScale);
+		r[2] = f4(ang0, velMul);
+		r[3] = f4(ang1, velTarget);
+		r[4] = make_float4(c.minImp, c.maxImp, 0.0f, maxBias);
+		r[5] = make_float4(__int_as_float(flags), 0.0f, 0.0f, 0.0f);
+	}
+	head[0] = f4(ra, invMass0);
+	head[1] = f4(rb, invMass1);
+	for (int k = 0; k < 3; k++)
+	{
+		head[2 + k] = f4(o0[k], orecip[k]);
+		head[5 + k] = f4(o1[k], oerr[k]);
+	}
+}
+
+// One pass over joint rows. v0 / v1: the bodies' velocity records (linear
+// velocity, angular velocity in sqrt-inertia space, accumulated linear and
+// angular motion); dq0 / dq1 their rotation since the step began.
+PXH_D void solveJoint(int n, const float4* head, float4* rows, f3& l0, f3& a0, f3 dlin0, f3 dang0, qt dq0,
+                      const m33& sI0, f3& l1, f3& a1, f3 dlin1, f3 dang1, qt dq1, const m33& sI1, float elapsed,
+                      bool conclude)
+{
+	const f3 rA = mk(head[0]), rB = mk(head[1]);
+	const float im0 = head[0].w, im1 = head[1].w;
+	const f3 ra = qrot(dq0, rA);
+	const f3 rb = qrot(dq1, rB);
+	const f3 raMotion = (ra + dlin0) - rA;
+	const f3 rbMotion = (rb + dlin1) - rB;
+	f3 o0[3], o1[3];
+	float orecip[3], oerr[3];
+	for (int k = 0; k < 3; k++)
+	{
+		const float4 h0 = head[2 + k], h1 = head[5 + k];
+		o0[k] = mk(h0);
+		orecip[k] = h0.w;
+		o1[k] = mk(h1);
+		oerr[k] = h1.w + (dot(o0[k], dang0) - dot(o1[k], dang1));
+	}
+	for (int i = 0; i < n; i++)
+	{
+		float4* r = rows + 6 * i;
+		const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
+		const int flags = __float_as_int(r[5].x);
+		const f3 lin0 = mk(r0), lin1 = mk(r1);
+		const float biasScale = r1.w, velMul = r2.w, velTarget = r3.w;
+		const float minImp = r4.x, maxImp = r4.y, applied = r4.z, maxBias = r4.w;
+		const f3 cang0 = mk(r2) + cross(ra, lin0);
+		const f3 cang1 = mk(r3) + cross(rb, lin1);
+		float error = r0.w;
+		const float minBias = (flags & JF_INEQUALITY) ? -3.402823466e+38f : -maxBias;
+		f3 raXnI = mmul(sI0, cang0);
+		f3 rbXnI = mmul(sI1, cang1);
+		if (flags & JF_ORTHO_TARGET)
+		{
+			float proj[3];
+			for (int k = 0; k < 3; k++)
+			{
+				const f3 pp = mulv(raXnI, o0[k]) + mulv(rbXnI, o1[k]);
+				proj[k] = (pp.x + pp.y + pp.z) * orecip[k];
+			}
+			const f3 d0 = o0[0] * proj[0] + (o0[1] * proj[1] + o0[2] * proj[2]);
+			const f3 d1 = o1[0] * proj[0] + (o1[1] * proj[1] + o1[2] * proj[2]);
+			raXnI = raXnI - d0;
+			rbXnI = rbXnI - d1;
+			const float ob = biasScale * (oerr[0] * proj[0] + (oerr[1] * proj[1] + oerr[2] * proj[2]));
+			error = error - ob;
+		}
+		const float angScale = (flags & JF_ANGULAR_ERROR) ? 1.0f : 0.0f;
+		const float dAng = angScale * (dot(raXnI, dang0) - dot(rbXnI, dang1));
+		const float dLin = dot(lin0, raMotion) - dot(lin1, rbMotion);
+		const float errorChange = (dLin + dAng) - velTarget * elapsed;
+		const f3 q0 = mulv(raXnI, raXnI), q1 = mulv(rbXnI, rbXnI);
+		const float resp0 = im0 * dot(lin0, lin0) + (q0.x + q0.y + q0.z);
+		const float resp1 = im1 * dot(lin1, lin1) - -(q1.x + q1.y + q1.z);
+		const float response = resp0 + resp1;
+		const float recip = response > 0.0f ? 1.0f / response : 0.0f;
+		const float vMul = recip * velMul;
+		const float ub = errorChange * biasScale + error;
+		const float bias = fminf(fmaxf(ub, minBias), maxBias);
+		const float constant = recip * (bias + velTarget);
+		const f3 v0 = mulv(l0, lin0) + mulv(a0, raXnI);
+		const f3 v1 = mulv(l1, lin1) + mulv(a1, rbXnI);
+		const f3 dv = v0 - v1;
+		const float normalVel = dv.x + dv.y + dv.z;
+		const float unclamped = applied + (vMul * normalVel + constant);
+		const float clamped = fminf(fmaxf(unclamped, minImp), maxImp);
+		const float df = clamped - applied;
+		r[4].z = clamped;
+		l0 = lin0 * (df * im0) + l0;
+		l1 = l1 - lin1 * (df * im1);
+		a0 = raXnI * df + a0;
+		a1 = rbXnI * (df * -1.0f) + a1;
+		if (conclude && !(flags & JF_KEEP_BIAS))
+		{
+			r[0].w = 0.0f;
+			r[1].w = 0.0f;
+		}
+	}
+}
+
+}  // namespace pxh

@@ -16,6 +16,7 @@
 #include <hip/hip_ext.h>
 #include "pxh_np.h"
 #include "pxh_convex.h"
+#include "pxh_joint.h"
 
 #include <algorithm>
 
@@ -2113,6 +2114,126 @@ __global__ void __launch_bounds__(SOLVE_WORLD_THREADS) k_solve_worlds(
 	}
 }
 
+// ---------------------------------------------------------------------------
+// joints (pxh_joint.h)
+// ---------------------------------------------------------------------------
+// jointed bodies do not collide (PxConstraintFlag::eCOLLISION_ENABLED is off
+// by default, physx/include/PxConstraint.h): their pair slots lose their
+// contacts before the constraint list is built
+__global__ void k_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs, const uint64_t* keys, int nKeys,
+                               float4* ctN)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= cap || uint32_t(i) >= *dP)
+		return;
+	const uint2 ab = pairs[i];
+	const uint32_t lo = ab.x < ab.y ? ab.x : ab.y, hi = ab.x < ab.y ? ab.y : ab.x;
+	const uint64_t k = (uint64_t(lo) << 32) | hi;
+	int a = 0, b = nKeys;
+	while (a < b)
+	{
+		const int m = (a + b) >> 1;
+		if (keys[m] < k)
+			a = m + 1;
+		else
+			b = m;
+	}
+	if (a < nKeys && keys[a] == k)
+		ctN[Q + i].w = __int_as_float(0);
+}
+
+PXH_D xf bodyPose(int i, const float4* pos, const float4* rot)
+{
+	xf t;
+	if (i < 0)
+	{
+		t.q.x = t.q.y = t.q.z = 0.0f;
+		t.q.w = 1.0f;
+		t.p = mk(0.0f, 0.0f, 0.0f);
+		return t;
+	}
+	t.q = mkq(rot[i]);
+	t.p = mk(pos[i]);
+	return t;
+}
+
+PXH_D m33 bodySqrtInvI(int i, const float4* svI0, const float4* svI1)
+{
+	if (i < 0)
+	{
+		m33 z;
+		z.c0 = z.c1 = z.c2 = mk(0.0f, 0.0f, 0.0f);
+		return z;
+	}
+	return loadSym(svI0[i], svI1[i]);
+}
+
+PXH_D xf jointFrame(const float4* f)
+{
+	xf t;
+	t.p = mk(f[0]);
+	t.q = mkq(f[1]);
+	return t;
+}
+
+__global__ void k_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
+                             const float4* vel, const float4* svI0, const float4* svI1, float invStepDt, float invDt,
+                             float erp, float lengthScale, float4* head, float4* rows)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= J)
+		return;
+	const int4 ji = info[j];
+	const xf bA = bodyPose(ji.y, pos, rot), bB = bodyPose(ji.z, pos, rot);
+	const m33 sI0 = bodySqrtInvI(ji.y, svI0, svI1), sI1 = bodySqrtInvI(ji.z, svI0, svI1);
+	const float im0 = ji.y >= 0 ? vel[4 * int64_t(ji.y)].w : 0.0f;
+	const float im1 = ji.z >= 0 ? vel[4 * int64_t(ji.z)].w : 0.0f;
+	prepJoint(ji.x, bA, bB, jointFrame(frame + 4 * int64_t(j)), jointFrame(frame + 4 * int64_t(j) + 2), sI0, sI1, im0,
+	          im1, invStepDt, invDt, erp, lengthScale, head + JOINT_HEAD * int64_t(j), rows + JOINT_ROWS * int64_t(j));
+}
+
+__global__ void k_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, float4* vel,
+                              const float4* svDQ, const float4* svI0, const float4* svI1, float elapsed, int conclude)
+{
+	const int j = begin + blockIdx.x * blockDim.x + threadIdx.x;
+	if (j >= end)
+		return;
+	const int4 ji = info[j];
+	const int a = ji.y, b = ji.z;
+	f3 l0 = mk(0.0f, 0.0f, 0.0f), a0 = l0, dl0 = l0, da0 = l0, l1 = l0, a1 = l0, dl1 = l0, da1 = l0;
+	qt q0, q1;
+	q0.x = q0.y = q0.z = 0.0f; q0.w = 1.0f;
+	q1 = q0;
+	float w0 = 0.0f, w1 = 0.0f;
+	if (a >= 0)
+	{
+		const float4* v = vel + 4 * int64_t(a);
+		const float4 v0 = v[0];
+		l0 = mk(v0); w0 = v0.w; a0 = mk(v[1]); dl0 = mk(v[2]); da0 = mk(v[3]);
+		q0 = mkq(svDQ[a]);
+	}
+	if (b >= 0)
+	{
+		const float4* v = vel + 4 * int64_t(b);
+		const float4 v0 = v[0];
+		l1 = mk(v0); w1 = v0.w; a1 = mk(v[1]); dl1 = mk(v[2]); da1 = mk(v[3]);
+		q1 = mkq(svDQ[b]);
+	}
+	const m33 sI0 = bodySqrtInvI(a, svI0, svI1), sI1 = bodySqrtInvI(b, svI0, svI1);
+	solveJoint(jointRowCount(ji.x), head + JOINT_HEAD * int64_t(j), rows + JOINT_ROWS * int64_t(j), l0, a0, dl0, da0,
+	           q0, sI0, l1, a1, dl1, da1, q1, sI1, elapsed, conclude != 0);
+	if (a >= 0)
+	{
+		vel[4 * int64_t(a)] = f4(l0, w0);
+		vel[4 * int64_t(a) + 1] = f4(a0, 0.0f);
+	}
+	if (b >= 0)
+	{
+		vel[4 * int64_t(b)] = f4(l1, w1);
+		vel[4 * int64_t(b) + 1] = f4(a1, 0.0f);
+	}
+}
+
 // copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580)
 __global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, const float4* vel,
                            const float4* svP, const float4* svDQ, const float4* svI0, const float4* svI1)
@@ -2461,6 +2582,27 @@ void launch_copy_contacts(int C, int maxPairs, const int* order, const int* conA
 {
 	const int n = std::max(C, 1);
 	k_copy_contacts<<<nblk(n, 256), 256, 0, st>>>(C, maxPairs, order, conA, conB, conSlot, ctN, ctP, nF, out, nbPairs);
+}
+void launch_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs, const uint64_t* jointKeys, int nKeys,
+                         float4* ctN, hipStream_t st)
+{
+	if (nKeys > 0)
+		PXH_LAUNCH(k_joint_filter, cap, cap, Q, dP, pairs, jointKeys, nKeys, ctN);
+}
+void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
+                       SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale, float4* head,
+                       float4* rows, hipStream_t st)
+{
+	PXH_LAUNCH(k_joint_prep, J, J, info, frame, pos, rot, sb.vel, sb.i0, sb.i1, invStepDt, invDt, erp, lengthScale, head,
+	           rows);
+}
+void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
+                        float elapsed, bool conclude, hipStream_t st)
+{
+	if (end <= begin)
+		return;
+	k_joint_solve<<<nblk(end - begin, 64), 64, 0, st>>>(begin, end, info, head, rows, sb.vel, sb.dq, sb.i0, sb.i1,
+	                                                     elapsed, conclude ? 1 : 0);
 }
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st)
 {

@@ -237,6 +237,16 @@ struct pxh_scene_t
 	size_t tuneIdx = 0;
 	hipEvent_t tuneEv[2] = {nullptr, nullptr};
 	bool tuneTimed = false;
+	// joints (TGS): host copies in insertion order (kind, bodies, frames as
+	// p xyz, q xyzw); on the device in colour order with the colour ranges and
+	// the sorted body-pair keys that keep jointed bodies from colliding
+	std::vector<int> hJType, hJA, hJB;
+	std::vector<float> hJF0, hJF1;
+	std::vector<int> jColorStart{0};
+	int nJKeys = 0;
+	DBuf<int4> jInfo;
+	DBuf<float4> jFrame, jHead, jRows;
+	DBuf<uint64_t> jKeys;
 	DBuf<unsigned char> gatherTmp;        // world removal: gather scratch and indices
 	DBuf<int> gatherIdx;
 	uint32_t* hostCounts = nullptr;       // pinned
@@ -333,6 +343,7 @@ struct pxh_scene_t
 		f(prevRow); f(conRec); f(htKey); f(htVal); f(bodyNext); f(colCounts); f(gatherTmp); f(gatherIdx);
 		f(worldOrder); f(segStart); f(solveQueue); f(extLin); f(extAng); f(prevLin); f(prevAng);
 		f(evFound); f(evFoundScan); f(evMatched); f(evLost); f(evLostScan); f(evFoundKeys); f(evLostKeys);
+		f(jInfo); f(jFrame); f(jHead); f(jRows); f(jKeys);
 	}
 	// device bytes the scene holds (PxSimulationStatistics::gpuMemHeap)
 	int64_t deviceBytes()
@@ -722,6 +733,136 @@ int32_t pxh_scene_add_planes(pxh_scene s, int32_t m, const float* plane, const i
 	return PXH_OK;
 }
 
+namespace {
+
+// Joint colours and device arrays from the host copies. Joints between two
+// bodies take colours first fit, newest joint first, over the colours each
+// body's joints hold; each joint to the world then takes the colour after all
+// of its body's other joint colours (one per such joint, in joint order): the
+// order in which the reference partitions a jointed chain, constraints on
+// static bodies after their body's dynamic ones
+// (lowleveldynamics/src/DyConstraintPartition.cpp:168-179, 604-616).
+int uploadJoints(pxh_scene_t* s)
+{
+	const int J = int(s->hJType.size());
+	std::vector<int> color(size_t(J), 0);
+	std::vector<uint64_t> used(size_t(std::max(s->n, 1)), 0ull);
+	std::vector<int> next(size_t(std::max(s->n, 1)), 0);
+	int nColors = 0;
+	for (int i = J - 1; i >= 0; i--)
+	{
+		const int a = s->hJA[i], b = s->hJB[i];
+		if (a < 0 || b < 0)
+			continue;
+		const uint64_t u = used[a] | used[b];
+		if (u == ~0ull)
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_add_joints: a body is in more than 64 joints");
+		int c = 0;
+		while ((u >> c) & 1ull)
+			c++;
+		color[i] = c;
+		used[a] |= 1ull << c;
+		used[b] |= 1ull << c;
+		next[a] = std::max(next[a], c + 1);
+		next[b] = std::max(next[b], c + 1);
+		nColors = std::max(nColors, c + 1);
+	}
+	for (int i = 0; i < J; i++)
+	{
+		const int a = s->hJA[i], b = s->hJB[i];
+		if (a >= 0 && b >= 0)
+			continue;
+		const int body = a >= 0 ? a : b;
+		color[i] = next[body]++;
+		nColors = std::max(nColors, color[i] + 1);
+	}
+	// colour order, joint order within a colour
+	std::vector<int> order;
+	for (int i = 0; i < J; i++)
+		order.push_back(i);
+	std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return color[x] < color[y]; });
+	s->jColorStart.assign(size_t(nColors) + 1, 0);
+	for (int i = 0; i < J; i++)
+		s->jColorStart[color[i] + 1]++;
+	for (int c = 0; c < nColors; c++)
+		s->jColorStart[c + 1] += s->jColorStart[c];
+	std::vector<int4> info(size_t(std::max(J, 1)));
+	std::vector<float4> frame(size_t(4) * std::max(J, 1));
+	std::vector<uint64_t> keys;
+	for (int k = 0; k < J; k++)
+	{
+		const int i = order[k];
+		info[k] = make_int4(s->hJType[i], s->hJA[i], s->hJB[i], color[i]);
+		const float* f0 = &s->hJF0[7 * size_t(i)];
+		const float* f1 = &s->hJF1[7 * size_t(i)];
+		frame[4 * k] = make_float4(f0[0], f0[1], f0[2], 0.0f);
+		frame[4 * k + 1] = make_float4(f0[3], f0[4], f0[5], f0[6]);
+		frame[4 * k + 2] = make_float4(f1[0], f1[1], f1[2], 0.0f);
+		frame[4 * k + 3] = make_float4(f1[3], f1[4], f1[5], f1[6]);
+		if (s->hJA[i] >= 0 && s->hJB[i] >= 0)
+		{
+			const uint32_t lo = uint32_t(std::min(s->hJA[i], s->hJB[i])), hi = uint32_t(std::max(s->hJA[i], s->hJB[i]));
+			keys.push_back((uint64_t(lo) << 32) | hi);
+		}
+	}
+	std::sort(keys.begin(), keys.end());
+	keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
+	s->nJKeys = int(keys.size());
+	PXH_CHECK(hipStreamSynchronize(s->stream));
+	PXH_CHECK(s->jInfo.ensure(info.size()));
+	PXH_CHECK(s->jFrame.ensure(frame.size()));
+	PXH_CHECK(s->jHead.ensure(size_t(pxh::JOINT_HEAD) * std::max(J, 1)));
+	PXH_CHECK(s->jRows.ensure(size_t(pxh::JOINT_ROWS) * std::max(J, 1)));
+	PXH_CHECK(s->jKeys.ensure(std::max<size_t>(keys.size(), 1)));
+	PXH_CHECK(hipMemcpy(s->jInfo.p, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
+	PXH_CHECK(hipMemcpy(s->jFrame.p, frame.data(), frame.size() * sizeof(float4), hipMemcpyHostToDevice));
+	if (!keys.empty())
+		PXH_CHECK(hipMemcpy(s->jKeys.p, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+	return PXH_OK;
+}
+
+}  // namespace
+
+int32_t pxh_scene_add_joints(pxh_scene s, int32_t k, const int32_t* type, const int32_t* body0, const int32_t* body1,
+                             const float* frame0, const float* frame1)
+{
+	if (!s || k < 0 || (k > 0 && (!type || !body0 || !body1 || !frame0 || !frame1)))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_joints: not allowed while simulating");
+	for (int i = 0; i < k; i++)
+	{
+		if (type[i] < PXH_JOINT_FIXED || type[i] > PXH_JOINT_PRISMATIC)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: unknown joint type");
+		if (body0[i] < -1 || body1[i] < -1 || body0[i] >= s->n || body1[i] >= s->n)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: body index out of range");
+		if (body0[i] == body1[i])
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: a joint needs two different actors, one dynamic");
+		if (body0[i] >= 0 && body1[i] >= 0 && s->hScene[body0[i]] != s->hScene[body1[i]])
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: the bodies are in different worlds");
+		for (int c = 0; c < 7; c++)
+			if (!std::isfinite(frame0[7 * i + c]) || !std::isfinite(frame1[7 * i + c]))
+				return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: frame is not finite");
+	}
+	const int first = int(s->hJType.size());
+	for (int i = 0; i < k; i++)
+	{
+		s->hJType.push_back(type[i]);
+		s->hJA.push_back(body0[i]);
+		s->hJB.push_back(body1[i]);
+		s->hJF0.insert(s->hJF0.end(), frame0 + 7 * i, frame0 + 7 * i + 7);
+		s->hJF1.insert(s->hJF1.end(), frame1 + 7 * i, frame1 + 7 * i + 7);
+	}
+	if (int r = uploadJoints(s))
+	{
+		s->hJType.resize(first); s->hJA.resize(first); s->hJB.resize(first);
+		s->hJF0.resize(7 * size_t(first)); s->hJF1.resize(7 * size_t(first));
+		(void)uploadJoints(s);
+		return r;
+	}
+	return first;
+}
+
 int32_t pxh_scene_add_convex_meshes(pxh_scene s, int32_t k, const int32_t* counts, const float* points,
                                     int32_t* first)
 {
@@ -877,6 +1018,31 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 	s->worldPlanes.swap(worldPlanes);
 	s->hWorldTag.swap(tags);
 	s->topologyDirty = true;
+	// joints of the kept worlds, bodies renumbered (a joint to the world
+	// belongs to its body's world)
+	if (!s->hJType.empty())
+	{
+		std::vector<int> newBody(size_t(keep.empty() ? 0 : keep.back() + 1), -1);
+		for (int i = 0; i < nn; i++)
+			newBody[keep[i]] = i;
+		auto remap = [&](int b) { return b < 0 ? -1 : (b < int(newBody.size()) ? newBody[b] : -1); };
+		std::vector<int> t, a, b;
+		std::vector<float> f0, f1;
+		for (size_t i = 0; i < s->hJType.size(); i++)
+		{
+			const int na = remap(s->hJA[i]), nb = remap(s->hJB[i]);
+			if ((s->hJA[i] >= 0 && na < 0) || (s->hJB[i] >= 0 && nb < 0))
+				continue;
+			t.push_back(s->hJType[i]);
+			a.push_back(na);
+			b.push_back(nb);
+			f0.insert(f0.end(), s->hJF0.begin() + 7 * i, s->hJF0.begin() + 7 * i + 7);
+			f1.insert(f1.end(), s->hJF1.begin() + 7 * i, s->hJF1.begin() + 7 * i + 7);
+		}
+		s->hJType.swap(t); s->hJA.swap(a); s->hJB.swap(b); s->hJF0.swap(f0); s->hJF1.swap(f1);
+		if (int r = uploadJoints(s))
+			return r;
+	}
 	return PXH_OK;
 }
 
@@ -912,6 +1078,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			return r;
 	}
 	const pxh_scene_desc& d = s->desc;
+	if (!s->hJType.empty() && d.solver_type != PXH_SOLVER_TGS)
+		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_simulate: joints run with the TGS solver only");
 	hipStream_t st = s->stream;
 	const int n = s->n;
 	const int nWorlds = int(s->hBodyStart.size()) - 1;
@@ -1001,6 +1169,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			                      d.tolerance_length, s->cvxWit.p, s->cvxList.p, s->ctN.p, s->ctP.p, st);
 		}
 		s->profEnd(pk);
+		pxh::launch_joint_filter(cap, Q, dP, s->pairs.p, s->jKeys.p, s->nJKeys, s->ctN.p, st);
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
 		pxh::launch_touch_flags(S, Q, dP, s->ctN.p, s->flags.p, st);
@@ -1142,7 +1311,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// the per-world solve: TGS, and every world small enough for one workgroup
 	// (layout auto), or as the tuning asks
 	const int layout = s->tuning[0];
-	const bool worldSolve = d.solver_type == PXH_SOLVER_TGS && nWorlds > 0 &&
+	const bool worldSolve = d.solver_type == PXH_SOLVER_TGS && nWorlds > 0 && s->hJType.empty() &&
 	                        (layout == PXH_SOLVE_LAYOUT_WORLDS ||
 	                         (layout == PXH_SOLVE_LAYOUT_AUTO && s->maxWorldBodies <= PXH_WORLD_SOLVE_MAX_BODIES));
 	s->lastWorldSolve = worldSolve;
@@ -1258,6 +1427,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	                         d.enable_body_accelerations ? s->prevAng.p : nullptr, st);
 	s->extPending = false;
 	s->lastDt = dt;
+
 	PXH_CHECK(s->rhI.ensure(Cc)); PXH_CHECK(s->rhF.ensure(Cc));
 	PXH_CHECK(s->rnA.ensure(4 * Cc)); PXH_CHECK(s->rnB.ensure(4 * Cc)); PXH_CHECK(s->rnF.ensure(4 * Cc));
 	PXH_CHECK(s->rfT.ensure(4 * Cc)); PXH_CHECK(s->rfA.ensure(4 * Cc)); PXH_CHECK(s->rfB.ensure(4 * Cc));
@@ -1268,6 +1438,12 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const float invStepDt = 1.0f / stepDt;
 	const float biasCoefficient = 2.0f * std::sqrt(1.0f / float(P_));
 	const float scale = std::min(0.8f, biasCoefficient);
+	// joint rows from the poses at the step's start (DyTGSDynamics.cpp:
+	// 1180-1256; erp = half the bias coefficient, DyTGSContactPrep.cpp:2018)
+	const int nJ = int(s->hJType.size());
+	if (nJ > 0)
+		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->pos.p, s->rot.p, sb, invStepDt, 1.0f / dt,
+		                       0.5f * biasCoefficient, d.tolerance_length, s->jHead.p, s->jRows.p, st);
 	// four step counters, then four per-block partial sums of the preparation
 	PXH_CHECK(s->counters.ensure(4 + 4 * ((Cc + 255) / 256)));
 	// friction patches and colours of this step's rows go to the [cur] buffers
@@ -1344,6 +1520,12 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	for (int it = 0; !pgs && !worldSolve && it < P_ + V_; it++)
 	{
 		const bool posIter = it < P_;
+		// joints by colour, then contacts by colour; the last position
+		// iteration concludes the joint rows (conclude1DStep)
+		const float elapsed = stepDt * float(std::min(it, P_));
+		for (int c = 0; c + 1 < int(s->jColorStart.size()); c++)
+			pxh::launch_joint_solve(s->jColorStart[c], s->jColorStart[c + 1], s->jInfo.p, s->jHead.p, s->jRows.p, sb,
+			                        elapsed, it == P_ - 1, st);
 		for (int c = 0; c < nColors; c++)
 		{
 			if (colorStart[c + 1] == colorStart[c])
