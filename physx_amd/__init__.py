@@ -176,6 +176,21 @@ class Scene:
                                                     b0.ctypes.data_as(_I32), b1.ctypes.data_as(_I32),
                                                     f0.ctypes.data_as(_F), f1.ctypes.data_as(_F)))
 
+    def add_cooked_convex(self, verts, planes, polys, indices, com) -> int:
+        """pxh_scene_add_cooked_convex: a hull as the reference cooker wrote it
+        (PxConvexMesh getVertices / getPolygonData / getIndexBuffer and the
+        local centre of mass); returns the mesh index bodies name in dims[0]."""
+        v, pl = _f32(verts).reshape(-1, 3), _f32(planes).reshape(-1, 4)
+        po = _i32(polys).reshape(-1, 2)
+        ix = np.ascontiguousarray(indices, dtype=np.uint8)
+        c = _f32(com).reshape(3)
+        out = ctypes.c_int32()
+        _check(self.lib.pxh_scene_add_cooked_convex(self.h, int(v.shape[0]), v.ctypes.data_as(_F), int(pl.shape[0]),
+                                                    pl.ctypes.data_as(_F), po.ctypes.data_as(_I32),
+                                                    ix.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                                    c.ctypes.data_as(_F), ctypes.byref(out)))
+        return out.value
+
     def simulate(self, dt: float):
         _check(self.lib.pxh_scene_simulate(self.h, ctypes.c_float(dt)))
 

@@ -9,6 +9,12 @@ reference's own outputs on seeded inputs:
   broadphase.npz  PxCreateBroadPhase(eABP) created pairs for seeded boxes
   step.npz      one PxScene TGS step (4/1 iterations, dt 1/60) of single bodies
                 on a plane: positions, orientations, velocities afterwards
+  convex_pairs.npz  PxGenerateContacts (fresh PxCache) on the touching pairs
+                of two piles formed in the reference SDK (4,096 24-point
+                hulls, and a 12,288-body mix of spheres, capsules, boxes and
+                16-point hulls, 100 steps each), 2,000 pairs per class
+                (convex-convex, box-convex, sphere-convex, plane-convex,
+                capsule-convex, capsule-box) with the reference-cooked hulls
   joints.npz    one PxScene TGS step of independent joints (fixed,
                 spherical, revolute, prismatic; one body to the world or two
                 bodies to each other, each starting with an error) at 4/1,
@@ -164,6 +170,106 @@ def make_step(dt=1.0 / 60.0):
                         dt=np.float32(dt), pos_iters=np.int32(4), vel_iters=np.int32(1))
 
 
+CONVEX_CLASSES = ("convex-convex", "box-convex", "sphere-convex", "plane-convex", "capsule-convex", "capsule-box")
+
+
+def _pair_class(ta, tb):
+    k = {0: "sphere", 1: "plane", 2: "capsule", 3: "box", 5: "convex"}
+    lo, hi = sorted((ta, tb))
+    return f"{k[lo]}-{k[hi]}"
+
+
+def make_convex_pairs(dt=1.0 / 60.0, steps=100, per_class=2000):
+    """Touching pairs of piles formed in the reference SDK, regenerated with
+    PxGenerateContacts on a fresh cache in the reference's shape order (lower
+    geometry type first; two shapes of one type: the later-added body
+    first)."""
+    hulls, hull_of_ref = [], {}
+    rows = {c: [] for c in CONVEX_CLASSES}
+    for sc in (S.convex_layers(n_bodies=4096, per_side=32), S.mixed_pile(n_bodies=12288, per_side=32)):
+        # reference-cooked meshes of the scene's point clouds
+        counts = sc.hull_counts
+        offs = np.concatenate([[0], np.cumsum(counts)])
+        mesh = []
+        for k in range(len(counts)):
+            mid, h = L.ref_cook(sc.hull_points[offs[k]:offs[k + 1]])
+            hull_of_ref[mid] = len(hulls)
+            hulls.append(h)
+            mesh.append(mid)
+        r = L.RefScene(sc, "tgs", 4, 1, threads=8, report_contacts=True)
+        r.step(dt, steps)
+        pose, _, _ = r.bodies()
+        pairs = {(int(a), int(b)) for a, b in r.contacts()[:, :2]}
+        for a, b in sorted(pairs):
+            i = a - 1
+            if b < 0:
+                ti, tj = int(sc.shape_type[i]), 1
+            else:
+                j = b - 1
+                ti, tj = int(sc.shape_type[i]), int(sc.shape_type[j])
+            cls = _pair_class(ti, tj)
+            if cls not in rows or len(rows[cls]) >= per_class:
+                continue
+
+            def shape(body):
+                t = int(sc.shape_type[body])
+                g = sc.dims[body].copy()
+                if t == 5:
+                    g = np.array([mesh[int(g[0])], 0, 0, 0], np.float32)
+                return t, g, pose[body]
+            if b < 0:
+                s0 = (1, np.zeros(4, np.float32), PLANE_POSE)
+                s1 = shape(i)
+            else:
+                j = b - 1
+                if ti == tj:
+                    s0, s1 = shape(max(i, j)), shape(min(i, j))
+                elif ti < tj:
+                    s0, s1 = shape(i), shape(j)
+                else:
+                    s0, s1 = shape(j), shape(i)
+            c = L.contacts("ref", s0[0], s0[1], s0[2], s1[0], s1[1], s1[2])
+            if len(c) == 0:
+                continue
+            out = np.zeros((4, 8), np.float32)
+            out[:len(c)] = c[:4]
+            g0, g1 = s0[1].copy(), s1[1].copy()
+            if s0[0] == 5:
+                g0[0] = hull_of_ref[int(g0[0])]
+            if s1[0] == 5:
+                g1[0] = hull_of_ref[int(g1[0])]
+            rows[cls].append((s0[0], g0, s0[2], s1[0], g1, s1[2], len(c), out))
+    # keep the hulls the stored pairs use, renumbered in order of use
+    used = {}
+    for rr in rows.values():
+        for x in rr:
+            for t, g in ((x[0], x[1]), (x[3], x[4])):
+                if t == 5:
+                    g[0] = used.setdefault(int(g[0]), len(used))
+    hulls = [h for _, h in sorted(((v, hulls[k]) for k, v in used.items()), key=lambda kv: kv[0])]
+    out = {}
+    for cls, rr in rows.items():
+        key = cls.replace("-", "_")
+        out[f"{key}_t"] = np.array([[x[0], x[3]] for x in rr], np.int32).reshape(-1, 2)
+        out[f"{key}_g"] = np.array([np.concatenate([x[1], x[4]]) for x in rr], np.float32).reshape(-1, 8)
+        out[f"{key}_p"] = np.array([np.concatenate([x[2], x[5]]) for x in rr], np.float32).reshape(-1, 14)
+        out[f"{key}_n"] = np.array([x[6] for x in rr], np.int32)
+        out[f"{key}_c"] = np.array([x[7] for x in rr], np.float32).reshape(-1, 4, 8)
+    # the hulls, concatenated
+    nv = np.array([h["verts"].shape[0] for h in hulls], np.int32)
+    npl = np.array([h["planes"].shape[0] for h in hulls], np.int32)
+    nr = np.array([h["vrefs"].shape[0] for h in hulls], np.int32)
+    out.update(hull_nv=nv, hull_np=npl, hull_nr=nr,
+               hull_verts=np.concatenate([h["verts"] for h in hulls]).astype(np.float32),
+               hull_planes=np.concatenate([h["planes"] for h in hulls]).astype(np.float32),
+               hull_polys=np.concatenate([h["polys"] for h in hulls]).astype(np.uint16),
+               hull_vrefs=np.concatenate([h["vrefs"] for h in hulls]).astype(np.uint8),
+               hull_ie=np.array([h["internal_extents"] for h in hulls], np.float32),
+               hull_com=np.array([h["com"] for h in hulls], np.float32))
+    np.savez_compressed(os.path.join(HERE, "convex_pairs.npz"), **out)
+    return {c: len(r) for c, r in rows.items()}
+
+
 JOINT_ITERS = ((4, 1), (8, 1), (1, 0))
 
 
@@ -187,5 +293,6 @@ if __name__ == "__main__":
     make_broadphase()
     make_step()
     make_joints()
-    for f in ("contacts.npz", "broadphase.npz", "step.npz", "joints.npz"):
+    print(make_convex_pairs())
+    for f in ("contacts.npz", "broadphase.npz", "step.npz", "joints.npz", "convex_pairs.npz"):
         print(f, os.path.getsize(os.path.join(HERE, f)))

@@ -467,3 +467,72 @@ class RefScene:
             self.lib.pxref_scene_release(self.h)
         except Exception:
             pass
+
+
+CONVEX_CLASSES = ("convex-convex", "box-convex", "sphere-convex", "plane-convex", "capsule-convex", "capsule-box")
+
+
+class ConvexGolden:
+    """tests/golden/convex_pairs.npz: touching pairs of piles formed in the
+    reference SDK with PxGenerateContacts' fresh-cache contacts, and the
+    reference-cooked hulls they use (tests/golden/make_golden.py)."""
+
+    def __init__(self):
+        # every array read once (an npz decompresses an array at each access)
+        with np.load(os.path.join(ROOT, "tests", "golden", "convex_pairs.npz")) as z:
+            self.g = {k: z[k] for k in z.files}
+        g = self.g
+        self.nv, self.np_, self.nr = g["hull_nv"], g["hull_np"], g["hull_nr"]
+        self._ov = np.concatenate([[0], np.cumsum(self.nv)])
+        self._op = np.concatenate([[0], np.cumsum(self.np_)])
+        self._or = np.concatenate([[0], np.cumsum(self.nr)])
+
+    @property
+    def n_hulls(self):
+        return int(self.nv.shape[0])
+
+    def hull(self, h):
+        g = self.g
+        return dict(verts=g["hull_verts"][self._ov[h]:self._ov[h + 1]],
+                    planes=g["hull_planes"][self._op[h]:self._op[h + 1]],
+                    polys=g["hull_polys"][self._op[h]:self._op[h + 1]].astype(np.int32),
+                    vrefs=g["hull_vrefs"][self._or[h]:self._or[h + 1]].astype(np.int32),
+                    internal_extents=g["hull_ie"][h], com=g["hull_com"][h])
+
+    def pairs(self, cls):
+        """(types (k,2), geoms (k,8), poses (k,14), counts (k,), contacts (k,4,8))"""
+        k = cls.replace("-", "_")
+        g = self.g
+        return g[k + "_t"], g[k + "_g"], g[k + "_p"], g[k + "_n"], g[k + "_c"]
+
+    def register_with_oracle(self):
+        """every hull with the oracle, hull h as oracle hull h"""
+        pxo().pxo_hull_clear()
+        for h in range(self.n_hulls):
+            assert pxo_add_hull(self.hull(h)) == h
+
+
+def dedupe(c):
+    """contacts as a set: coincident points (to 1e-5) counted once"""
+    return np.unique(np.round(c, 5), axis=0) if len(c) else c
+
+
+def same_contacts(a, b):
+    """None when the contact sets agree (1e-4 m points and separations,
+    1e-3 on the normal), else what differs: 'count', 'normal' or 'point'"""
+    if len(a) != len(b):
+        return "count"
+    if len(a) == 0:
+        return None
+    if np.abs(a[0, 4:7] - b[0, 4:7]).max() > 1e-3:
+        return "normal"
+    used = set()
+    for r in a:
+        d = np.abs(b[:, :4] - r[:4]).max(1)
+        for j in used:
+            d[j] = 9.0
+        j = int(np.argmin(d))
+        if d[j] > 1e-4:
+            return "point"
+        used.add(j)
+    return None

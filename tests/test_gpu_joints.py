@@ -72,27 +72,30 @@ def test_jointed_chains_one_step_from_mid_run_state(cuda):
     _close(g.bodies(), o.bodies(), 1e-5, "chains")
 
 
-def test_jointed_chains_track_oracle(cuda):
-    """The same chains stepped side by side on the GPU and in the oracle: the
-    constraint count of each of the first 30 steps equal, and the link
-    positions within 5 mm after 60 steps. The GPU rounds a few operations
-    apart from the C oracle (sincosf; the square-root inverse inertia kept as
-    a symmetric matrix), and dropped boxes landing on swinging chains amplify
-    that, so by step 50 a pair at the contact distance can touch on one side
-    only."""
-    sc = S.jointed_chains(n_scenes=8, links=12, drops=6, seed=3)
+@pytest.mark.parametrize("drops,steps", [(6, 40), (0, 60)])
+def test_jointed_chains_track_oracle(cuda, drops, steps):
+    """The chains stepped side by side on the GPU and in the oracle: every
+    step's constraint count equal and the link positions within 1 mm at the
+    end. The GPU rounds a few operations apart from the C oracle (sincosf;
+    the square-root inverse inertia kept as a symmetric matrix); with boxes
+    landing on swinging chains the run turns chaotic after step 40 (in the
+    oracle alone, moving every body by 1e-6 m grows to a 1.6 cm difference by
+    step 50, when one pair touches on one side only), so that run is
+    compared over its first 40 steps; without the boxes over 60 steps (the
+    same perturbation stays below 4e-6 m)."""
+    sc = S.jointed_chains(n_scenes=8, links=12, drops=drops, seed=3)
     g = P.Scene(sc, position_iterations=4, velocity_iterations=1)
     o = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
-    for s in range(60):
+    for s in range(steps):
         g.step(DT)
         o.step(DT)
-        if s < 30:
-            assert g.stats()["constraints"] == o.stats()["constraints"], s
+        assert g.stats()["constraints"] == o.stats()["constraints"], s
     gp, _, _ = g.bodies()
     op, _, _ = o.bodies()
     links = np.concatenate([np.where(sc.scene_id == w)[0][:12] for w in range(8)])
-    print("largest link position difference after 60 steps:", float(np.abs(gp[links, :3] - op[links, :3]).max()))
-    assert np.abs(gp[links, :3] - op[links, :3]).max() < 5e-3
+    d = float(np.abs(gp[links, :3] - op[links, :3]).max())
+    print(f"largest link position difference after {steps} steps: {d:.2e}")
+    assert d < 1e-3
     assert np.isfinite(gp).all()
 
 
