@@ -105,6 +105,16 @@ int32_t pxh_scene_add_planes(pxh_scene scene, int32_t m, const float* plane, con
 int32_t pxh_scene_simulate(pxh_scene scene, float dt);
 int32_t pxh_scene_fetch_results(pxh_scene scene, int32_t block);
 
+/* Aggregates: PxPhysics::createAggregate + PxAggregate::addActor +
+ * PxScene::addAggregate (physx/include/PxAggregate.h; filter hint of
+ * PxAggregateType::eGENERIC with or without self-collisions). The k bodies
+ * (host array of indices, none already in an aggregate) form one aggregate;
+ * without self-collisions its members never pair with each other in the
+ * broadphase, while pairs with every other body are found as before (the
+ * reference's aggregate bounds are a broadphase acceleration: the pair set
+ * is the same). Returns the aggregate's id. */
+int32_t pxh_scene_add_aggregate(pxh_scene scene, int32_t k, const int32_t* bodies, int32_t self_collisions);
+
 /* Joints without limits, drives or break forces: PxFixedJointCreate,
  * PxSphericalJointCreate, PxRevoluteJointCreate (free twist about the joint
  * frame's x axis), PxPrismaticJointCreate (free translation along x)

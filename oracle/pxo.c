@@ -2433,6 +2433,11 @@ typedef struct {
 	pxo_joint* joint;
 	int nj, jcap, njcolors;
 	uint64_t* jkeys;
+	/* aggregates: each body's aggregate (-1: none, array of n or NULL) and
+	 * whether each aggregate's members collide with each other */
+	int* agg;
+	char* aggSelf;
+	int nagg;
 } pxo_world;
 
 void pxo_default_desc(pxo_desc* d)
@@ -2473,6 +2478,7 @@ void pxo_world_release(void* wp)
 	free(w->plocal); free(w->con);
 	free(w->prev);
 	free(w->joint); free(w->jkeys);
+	free(w->agg); free(w->aggSelf);
 	free(w);
 }
 
@@ -2481,6 +2487,12 @@ int pxo_world_add_bodies(void* wp, int n, const int32_t* type, const float* dims
 {
 	pxo_world* w = (pxo_world*)wp;
 	const int nn = w->n + n;
+	if (w->agg)
+	{
+		w->agg = (int*)realloc(w->agg, sizeof(int) * nn);
+		for (size_t i = (size_t)w->n; i < nn; i++)
+			w->agg[i] = -1;
+	}
 	w->type = (int32_t*)realloc(w->type, sizeof(int32_t) * nn);
 	w->scene = (int32_t*)realloc(w->scene, sizeof(int32_t) * nn);
 	w->local = (int32_t*)realloc(w->local, sizeof(int32_t) * nn);
@@ -3555,6 +3567,37 @@ static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int c
 	b1->lin = l1; b1->ang = a1;
 }
 
+/* PxAggregate (physx/include/PxAggregate.h): the members of an aggregate
+ * without self-collisions never pair with each other (the reference's
+ * broadphase sees the aggregate's bounds and its own self-collision pass is
+ * off); every other pair is found as before */
+int pxo_world_add_aggregate(void* wp, int k, const int32_t* bodies, int self_collisions)
+{
+	pxo_world* w = (pxo_world*)wp;
+	if (!w->agg)
+	{
+		w->agg = (int*)malloc(sizeof(int) * (size_t)(w->n ? w->n : 1));
+		for (int i = 0; i < w->n; i++)
+			w->agg[i] = -1;
+	}
+	for (int i = 0; i < k; i++)
+		if (bodies[i] < 0 || bodies[i] >= w->n || w->agg[bodies[i]] >= 0)
+			return -1;
+	w->aggSelf = (char*)realloc(w->aggSelf, (size_t)(w->nagg + 1));
+	w->aggSelf[w->nagg] = (char)(self_collisions != 0);
+	for (int i = 0; i < k; i++)
+		w->agg[bodies[i]] = w->nagg;
+	return w->nagg++;
+}
+
+static int aggregate_filtered(const pxo_world* w, int i, int j)
+{
+	if (!w->agg)
+		return 0;
+	const int a = w->agg[i];
+	return a >= 0 && a == w->agg[j] && !w->aggSelf[a];
+}
+
 int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
                          const float* frame0, const float* frame1)
 {
@@ -3670,6 +3713,18 @@ int pxo_world_step(void* wp, float dt)
 	bp_core(lo, hi, w->scene, n, &pv);
 	free(lo);
 	free(hi);
+	if (w->agg)
+	{
+		long long k = 0;
+		for (long long p = 0; p < pv.n; p++)
+			if (!aggregate_filtered(w, (int)pv.p[2 * p], (int)pv.p[2 * p + 1]))
+			{
+				pv.p[2 * k] = pv.p[2 * p];
+				pv.p[2 * k + 1] = pv.p[2 * p + 1];
+				k++;
+			}
+		pv.n = k;
+	}
 	w->stats[0] = pv.n;
 
 	/* 2. narrowphase: body pairs in sorted pair order, then body-plane pairs */

@@ -92,6 +92,9 @@ void  pxo_world_set_forces(void* w, const float* force, const float* torque);
 enum { PXO_JOINT_FIXED = 0, PXO_JOINT_SPHERICAL = 1, PXO_JOINT_REVOLUTE = 2, PXO_JOINT_PRISMATIC = 3 };
 int   pxo_world_add_joints(void* w, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
                            const float* frame0, const float* frame1);
+/* Aggregate of k bodies (PxAggregate): without self-collisions its members
+ * never pair with each other. Returns the aggregate id or -1. */
+int   pxo_world_add_aggregate(void* w, int k, const int32_t* bodies, int self_collisions);
 /* out: [0] BP pairs, [1] body-plane pairs tested, [2] touching pairs (constraints),
  * [3] contact points, [4] colours, [5] friction anchors */
 void  pxo_world_stats(void* w, long long* out);

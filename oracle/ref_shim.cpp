@@ -423,6 +423,26 @@ void pxref_scene_add_forces(void* h, const float* force, const float* torque)
 	}
 }
 
+// An aggregate of k bodies already added (PxPhysics::createAggregate with the
+// eGENERIC filter hint, with or without self-collisions; the bodies leave the
+// scene, join the aggregate, and the aggregate is added). Returns 0 or -1.
+int pxref_scene_add_aggregate(void* h, int k, const int* bodies, int selfCollisions)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	PxAggregate* agg = gPhysics->createAggregate(PxU32(k), PxU32(k),
+	                                             PxGetAggregateFilterHint(PxAggregateType::eGENERIC, selfCollisions != 0));
+	if (!agg)
+		return -1;
+	for (int i = 0; i < k; i++)
+	{
+		PxRigidDynamic* b = rs->bodies[size_t(bodies[i])];
+		rs->scene->removeActor(*b);
+		if (!agg->addActor(*b))
+			return -1;
+	}
+	return rs->scene->addAggregate(*agg) ? 0 : -1;
+}
+
 // Adds k joints without limits or drives: type 0 fixed, 1 spherical,
 // 2 revolute, 3 prismatic (PxFixedJointCreate, PxSphericalJointCreate,
 // PxRevoluteJointCreate, PxPrismaticJointCreate); body0/body1: indices of

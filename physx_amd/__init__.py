@@ -62,7 +62,7 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
-            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints")
+            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate")
 
 _lib = None
 
@@ -160,10 +160,18 @@ class Scene:
         pl, ps = _f32(scene.planes), _i32(scene.plane_scene)
         _check(self.lib.pxh_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps)))
         self.n += scene.n
+        for members, self_coll in (getattr(scene, "aggregates", None) or []):
+            self.add_aggregate(np.asarray(members) + base, self_coll)
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
                             np.where(j["body1"] >= 0, j["body1"] + base, -1), j["frame0"], j["frame1"])
+
+    def add_aggregate(self, bodies, self_collisions: bool = False) -> int:
+        """pxh_scene_add_aggregate (PxAggregate with the bodies): returns its id."""
+        b = _i32(bodies)
+        return _check(self.lib.pxh_scene_add_aggregate(self.h, int(b.size), b.ctypes.data_as(_I32),
+                                                       int(bool(self_collisions))))
 
     def add_joints(self, kind, body0, body1, frame0, frame1) -> int:
         """pxh_scene_add_joints (PxFixedJointCreate / PxSphericalJointCreate /

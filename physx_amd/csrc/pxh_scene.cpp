@@ -237,6 +237,14 @@ struct pxh_scene_t
 	size_t tuneIdx = 0;
 	hipEvent_t tuneEv[2] = {nullptr, nullptr};
 	bool tuneTimed = false;
+	// aggregates (PxAggregate): each body's aggregate (-1: none) and whether
+	// each aggregate's members collide with each other; the broadphase group
+	// of every body (members of an aggregate without self-collisions share
+	// one group, so they never pair) rebuilt when these change
+	std::vector<int> hAggregate;
+	std::vector<char> hAggSelf;
+	bool aggDirty = false;
+	DBuf<int> bpGroup;
 	// joints (TGS): host copies in insertion order (kind, bodies, frames as
 	// p xyz, q xyzw); on the device in colour order with the colour ranges and
 	// the sorted body-pair keys that keep jointed bodies from colliding
@@ -343,7 +351,7 @@ struct pxh_scene_t
 		f(prevRow); f(conRec); f(htKey); f(htVal); f(bodyNext); f(colCounts); f(gatherTmp); f(gatherIdx);
 		f(worldOrder); f(segStart); f(solveQueue); f(extLin); f(extAng); f(prevLin); f(prevAng);
 		f(evFound); f(evFoundScan); f(evMatched); f(evLost); f(evLostScan); f(evFoundKeys); f(evLostKeys);
-		f(jInfo); f(jFrame); f(jHead); f(jRows); f(jKeys);
+		f(jInfo); f(jFrame); f(jHead); f(jRows); f(jKeys); f(bpGroup);
 	}
 	// device bytes the scene holds (PxSimulationStatistics::gpuMemHeap)
 	int64_t deviceBytes()
@@ -693,6 +701,11 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 		s->nConvexBodies += type[i] == PXH_GEOM_CONVEXMESH;
 	s->n = int(nn);
 	s->topologyDirty = true;
+	if (!s->hAggSelf.empty())
+	{
+		s->hAggregate.resize(size_t(s->n), -1);
+		s->aggDirty = true;
+	}
 	return base;
 }
 
@@ -731,6 +744,31 @@ int32_t pxh_scene_add_planes(pxh_scene s, int32_t m, const float* plane, const i
 	s->m = int(nm);
 	s->topologyDirty = true;
 	return PXH_OK;
+}
+
+int32_t pxh_scene_add_aggregate(pxh_scene s, int32_t k, const int32_t* bodies, int32_t self_collisions)
+{
+	if (!s || k < 0 || (k > 0 && !bodies))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_aggregate: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_aggregate: not allowed while simulating");
+	s->hAggregate.resize(size_t(s->n), -1);
+	for (int i = 0; i < k; i++)
+	{
+		if (bodies[i] < 0 || bodies[i] >= s->n)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_aggregate: body index out of range");
+		if (s->hAggregate[bodies[i]] >= 0)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_aggregate: a body is already in an aggregate");
+		for (int j = 0; j < i; j++)
+			if (bodies[j] == bodies[i])
+				return fail(PXH_ERR_INVALID, "pxh_scene_add_aggregate: a body is listed twice");
+	}
+	const int id = int(s->hAggSelf.size());
+	s->hAggSelf.push_back(self_collisions ? 1 : 0);
+	for (int i = 0; i < k; i++)
+		s->hAggregate[bodies[i]] = id;
+	s->aggDirty = true;
+	return id;
 }
 
 namespace {
@@ -1018,6 +1056,15 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 	s->worldPlanes.swap(worldPlanes);
 	s->hWorldTag.swap(tags);
 	s->topologyDirty = true;
+	// aggregate membership of the kept bodies
+	if (!s->hAggSelf.empty())
+	{
+		std::vector<int> agg(size_t(nn), -1);
+		for (int i = 0; i < nn; i++)
+			agg[i] = keep[i] < int(s->hAggregate.size()) ? s->hAggregate[keep[i]] : -1;
+		s->hAggregate.swap(agg);
+		s->aggDirty = true;
+	}
 	// joints of the kept worlds, bodies renumbered (a joint to the world
 	// belongs to its body's world)
 	if (!s->hJType.empty())
@@ -1124,7 +1171,22 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	pxh::launch_bounds(n, s->pos.p, s->rot.p, s->geom.p, s->hullSet(), co, s->bmin.p, s->bmax.p, st);
 	s->profEnd(pk);
 	s->pairCap = std::max(s->pairCap, size_t(s->desc.max_pairs_per_body) * n);
-	int r = broadphase(s, n, nWorlds, s->scene.p, nullptr, s->rowOff.p, s->totalRows, st, s->pairCap);
+	if (s->aggDirty)
+	{
+		// broadphase groups: odd = one body's own, even = an aggregate's
+		std::vector<int> grp(size_t(std::max(n, 1)));
+		for (int i = 0; i < n; i++)
+		{
+			const int a = s->hAggregate[i];
+			grp[i] = (a >= 0 && !s->hAggSelf[a]) ? 2 * a : 2 * i + 1;
+		}
+		PXH_CHECK(s->bpGroup.ensure(grp.size()));
+		PXH_CHECK(hipMemcpyAsync(s->bpGroup.p, grp.data(), grp.size() * sizeof(int), hipMemcpyHostToDevice, st));
+		PXH_CHECK(hipStreamSynchronize(st));
+		s->aggDirty = false;
+	}
+	const int* bpGroup = s->hAggSelf.empty() ? nullptr : s->bpGroup.p;
+	int r = broadphase(s, n, nWorlds, s->scene.p, bpGroup, s->rowOff.p, s->totalRows, st, s->pairCap);
 	if (r)
 		return r;
 	const uint32_t* dP = s->counts.p + pxh::PAIR_CLASSES * int64_t(n);
@@ -1143,7 +1205,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		const int S = Q + cap;
 		if (attempt > 0)
 		{
-			r = bp_write_pairs(s, n, s->scene.p, nullptr, s->rowOff.p, st, s->pairCap);
+			r = bp_write_pairs(s, n, s->scene.p, bpGroup, s->rowOff.p, st, s->pairCap);
 			if (r)
 				return r;
 		}

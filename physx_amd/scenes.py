@@ -62,6 +62,8 @@ class Scene:
     # joints: type (k,) int32 (JOINT_*), body0 / body1 (k,) int32 (-1: the
     # world frame), frame0 / frame1 (k,7) f32 local frames (p xyz, q xyzw)
     joints: dict | None = None
+    # aggregates: list of (member body indices, self_collisions)
+    aggregates: list | None = None
 
     @property
     def n(self) -> int:
@@ -478,4 +480,29 @@ def jointed_chains(n_scenes=8, links=12, drops=6, seed=3) -> Scene:
                  {"n_scenes": n_scenes, "links": links, "drops": drops, "seed": seed})
     sc.joints = {"type": np.array(jt, np.int32), "body0": np.array(jb0, np.int32), "body1": np.array(jb1, np.int32),
                  "frame0": np.array(jf0, np.float32), "frame1": np.array(jf1, np.float32)}
+    return sc
+
+
+def aggregate_pile(n_groups=64, per_group=4, seed=7) -> Scene:
+    """Groups of per_group boxes whose members overlap (half extents 0.15 to
+    0.25 m, 0.3 m apart along x), laid out 1.4 m apart in a grid above a
+    plane, each group an aggregate; every other group with self-collisions
+    on. Without self-collisions a group's members pass through each other;
+    groups collide with each other and the floor."""
+    n = n_groups * per_group
+    ids = np.arange(n)
+    grp = ids // per_group
+    k = ids % per_group
+    side = int(math.ceil(math.sqrt(n_groups)))
+    gx, gz = grp % side, grp // side
+    h = np.stack([uniform(seed, ids, d, 0.15, 0.25) for d in range(3)], 1)
+    pos = np.stack([(gx - side / 2) * 1.4 + 0.3 * k - 0.45, 0.6 + 0.2 * (k % 2) + uniform(seed, ids, 3, 0, 0.1),
+                    (gz - side / 2) * 1.4 + uniform(seed, ids, 4, -0.05, 0.05)], 1)
+    quat = _quat_axis_angle(np.stack([uniform(seed, ids, 5, -1, 1), uniform(seed, ids, 6, -1, 1),
+                                      uniform(seed, ids, 7, -1, 1)], 1), uniform(seed, ids, 8, 0, 0.6))
+    dims = np.concatenate([h, np.zeros((n, 1))], 1)
+    sc = _finish(np.full(n, BOX, np.int32), dims, pos, quat, np.zeros(n, np.int32), [[0, 1, 0, 0]], [0], 1,
+                 "aggregate_pile", {"n_groups": n_groups, "per_group": per_group, "seed": seed})
+    sc.aggregates = [(np.arange(g * per_group, (g + 1) * per_group, dtype=np.int32), g % 2 == 1)
+                     for g in range(n_groups)]
     return sc

@@ -275,6 +275,9 @@ class OracleWorld:
         pl, ps = f32(scene.planes), i32(scene.plane_scene)
         self._keep += [pl, ps]
         self.lib.pxo_world_add_planes(self.h, int(pl.shape[0]), fp(pl), ip(ps))
+        for members, self_coll in (getattr(scene, "aggregates", None) or []):
+            m = i32(members)
+            assert self.lib.pxo_world_add_aggregate(self.h, int(m.size), ip(m), int(bool(self_coll))) >= 0
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
@@ -417,6 +420,9 @@ class RefScene:
         pl = f32(scene.planes)
         self._keep.append(pl)
         assert self.lib.pxref_scene_add_planes(self.h, int(pl.shape[0]), fp(pl), ctypes.c_float(0.02)) == 0
+        for members, self_coll in (getattr(scene, "aggregates", None) or []):
+            m = i32(members)
+            assert self.lib.pxref_scene_add_aggregate(self.h, int(m.size), ip(m), int(bool(self_coll))) == 0
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
