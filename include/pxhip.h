@@ -65,6 +65,10 @@ typedef struct pxh_scene_desc {
 	                                        before each step for the acceleration reads */
 	int32_t report_touch_events;         /* PxPairFlag::eNOTIFY_TOUCH_FOUND | eNOTIFY_TOUCH_LOST on every
 	                                        pair: compute the touch events of each step */
+	int32_t enable_sleeping;             /* bodies sleep (PxRigidDynamic::setSleepThreshold with a positive
+	                                        threshold; 0 = never, as with threshold 0) */
+	float sleep_threshold;               /* PxRigidDynamic::setSleepThreshold of every body (default
+	                                        5e-5 * speed^2 = 0.005 for PxTolerancesScale speed 10) */
 } pxh_scene_desc;
 
 typedef struct pxh_scene_t* pxh_scene;
@@ -142,6 +146,30 @@ int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene scene, void* data, const uint
                                          void* stream);
 int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, const uint32_t* index, int32_t kind,
                                          int32_t n, void* stream);
+
+/* Sleeping (enable_sleeping): bodies and islands fall asleep and wake as in
+ * the reference's CPU pipeline. Each body keeps a wake counter (0.4 s when
+ * added or woken) that counts down while the body's mean motion over the
+ * step, accumulated in the last 0.2 s, stays below (1 + interactions) times
+ * the sleep threshold, and restarts when it rises above
+ * (Dy::sleepCheck, lowleveldynamics/src/DySleep.cpp:173-235); an island of
+ * bodies joined by touching contacts and joints whose wake counters all
+ * reached zero by the end of a step sleeps from the next step on: zero
+ * velocities, no integration, no contacts among its bodies (the island
+ * manager's deactivation, lowlevel/software/src/PxsIslandSim.cpp:1630-1720,
+ * ScBodySim.cpp:469-490). An awake body that touches a sleeping one wakes the
+ * sleeping body's island (ScPipeline.cpp:1726-1776).
+ * pxh_scene_get_sleep_state: PxRigidDynamic::getWakeCounter / isSleeping
+ * (physx/include/PxRigidDynamic.h) of every body into host arrays
+ * wake_counter[n] / asleep[n] (either may be NULL); n at most the body count.
+ * pxh_scene_wake_bodies: PxRigidDynamic::wakeUp of k bodies (host array of
+ * indices): wake counter 0.4 s; the rest of a sleeping body's island wakes
+ * at the next step. Writing poses or velocities through
+ * pxh_scene_set_rigid_dynamic_data wakes the written bodies the same way,
+ * forces and torques when nonzero (autowake). Sleeping does not combine with
+ * report_touch_events (PXH_ERR_UNSUPPORTED). */
+int32_t pxh_scene_get_sleep_state(pxh_scene scene, float* wake_counter, int32_t* asleep, int32_t n);
+int32_t pxh_scene_wake_bodies(pxh_scene scene, const int32_t* bodies, int32_t k);
 
 /* PxScene::getSimulationStatistics (physx/include/PxSimulationStatistics.h):
  * out[0] bodies, [1] broadphase pairs (body-body), [2] body-plane pairs,

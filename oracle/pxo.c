@@ -2438,6 +2438,14 @@ typedef struct {
 	int* agg;
 	char* aggSelf;
 	int nagg;
+	/* sleeping: wake counter, accumulated motion velocities (linear; angular
+	 * in the body frame), asleep and ready-for-sleep flags, island label (the
+	 * smallest body index of the island the body last belonged to), islands
+	 * to wake at the next step (by label), joints per body */
+	float* wc;
+	v3 *accL, *accA;
+	unsigned char *asleep, *ready, *wakeIsl;
+	int *isl, *jcount;
 } pxo_world;
 
 void pxo_default_desc(pxo_desc* d)
@@ -2479,6 +2487,8 @@ void pxo_world_release(void* wp)
 	free(w->prev);
 	free(w->joint); free(w->jkeys);
 	free(w->agg); free(w->aggSelf);
+	free(w->wc); free(w->accL); free(w->accA); free(w->asleep); free(w->ready); free(w->wakeIsl); free(w->isl);
+	free(w->jcount);
 	free(w);
 }
 
@@ -2504,6 +2514,27 @@ int pxo_world_add_bodies(void* wp, int n, const int32_t* type, const float* dims
 	w->invI = (v3*)realloc(w->invI, sizeof(v3) * nn);
 	w->extLin = (v3*)realloc(w->extLin, sizeof(v3) * nn);
 	w->extAng = (v3*)realloc(w->extAng, sizeof(v3) * nn);
+	w->wc = (float*)realloc(w->wc, sizeof(float) * nn);
+	w->accL = (v3*)realloc(w->accL, sizeof(v3) * nn);
+	w->accA = (v3*)realloc(w->accA, sizeof(v3) * nn);
+	w->asleep = (unsigned char*)realloc(w->asleep, (size_t)nn);
+	w->ready = (unsigned char*)realloc(w->ready, (size_t)nn);
+	w->wakeIsl = (unsigned char*)realloc(w->wakeIsl, (size_t)nn);
+	w->isl = (int*)realloc(w->isl, sizeof(int) * nn);
+	w->jcount = (int*)realloc(w->jcount, sizeof(int) * nn);
+	for (int k = w->n; k < nn; k++)
+	{
+		/* a new body is awake with the scene's wake counter reset value
+		 * (PxSceneDesc::wakeCounterResetValue, 20 * 0.02 s) */
+		w->wc[k] = 20.0f * 0.02f;
+		w->accL[k] = V(0, 0, 0);
+		w->accA[k] = V(0, 0, 0);
+		w->asleep[k] = 0;
+		w->ready[k] = 0;
+		w->wakeIsl[k] = 0;
+		w->isl[k] = k;
+		w->jcount[k] = 0;
+	}
 	/* index of each body within its world: bodies of that world added before it */
 	int maxScene = 0;
 	for (int j = 0; j < w->n; j++) if (w->scene[j] > maxScene) maxScene = w->scene[j];
@@ -2571,6 +2602,42 @@ void pxo_world_get_bodies(void* wp, float* pose, float* lv, float* av)
 	}
 }
 
+/* PxRigidDynamic::wakeUp (NpRigidDynamic.cpp wakeUp -> BodyCore::wakeUp with
+ * the scene's wake counter reset value): the body is awake now with a wake
+ * counter of 0.4 s and not ready for sleep; the rest of its island wakes at
+ * the next step (IslandSim::activateNode, then wakeIslands) */
+static void wake_body(pxo_world* w, int i)
+{
+	w->wc[i] = 20.0f * 0.02f;
+	w->ready[i] = 0;
+	if (w->asleep[i])
+	{
+		w->asleep[i] = 0;
+		w->wakeIsl[w->isl[i]] = 1;
+	}
+}
+
+void pxo_world_wake(void* wp, int k, const int32_t* bodies)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int j = 0; j < k; j++)
+		if (bodies[j] >= 0 && bodies[j] < w->n)
+			wake_body(w, bodies[j]);
+}
+
+void pxo_world_get_sleep(void* wp, float* wc, int32_t* asleep)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < w->n; i++)
+	{
+		if (wc) wc[i] = w->wc[i];
+		if (asleep) asleep[i] = w->asleep[i];
+	}
+}
+
+/* Writing poses or velocities wakes the bodies when sleeping is on (the
+ * autowake of PxRigidActor::setGlobalPose, PxRigidDynamic::setLinearVelocity
+ * and setAngularVelocity). */
 void pxo_world_set_bodies(void* wp, const float* pose, const float* lv, const float* av)
 {
 	pxo_world* w = (pxo_world*)wp;
@@ -2579,6 +2646,8 @@ void pxo_world_set_bodies(void* wp, const float* pose, const float* lv, const fl
 		if (pose) w->pose[i] = load_pose(pose + 7 * i);
 		if (lv) w->lv[i] = V(lv[3 * i], lv[3 * i + 1], lv[3 * i + 2]);
 		if (av) w->av[i] = V(av[3 * i], av[3 * i + 1], av[3 * i + 2]);
+		if (w->d.enable_sleeping && (pose || lv || av))
+			wake_body(w, i);
 	}
 }
 
@@ -2592,6 +2661,10 @@ void pxo_world_set_forces(void* wp, const float* force, const float* torque)
 	pxo_world* w = (pxo_world*)wp;
 	for (int i = 0; i < w->n; i++)
 	{
+		if (w->d.enable_sleeping &&
+		    ((force && (force[3 * i] != 0.0f || force[3 * i + 1] != 0.0f || force[3 * i + 2] != 0.0f)) ||
+		     (torque && (torque[3 * i] != 0.0f || torque[3 * i + 1] != 0.0f || torque[3 * i + 2] != 0.0f))))
+			wake_body(w, i);   /* addForce / addTorque wake the body (autowake) */
 		if (force)
 			w->extLin[i] = scl(V(force[3 * i], force[3 * i + 1], force[3 * i + 2]), w->invMass[i]);
 		if (torque)
@@ -3628,6 +3701,10 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 		j->f0.q.x = frame0[7 * i + 3]; j->f0.q.y = frame0[7 * i + 4]; j->f0.q.z = frame0[7 * i + 5]; j->f0.q.w = frame0[7 * i + 6];
 		j->f1.p = V(frame1[7 * i], frame1[7 * i + 1], frame1[7 * i + 2]);
 		j->f1.q.x = frame1[7 * i + 3]; j->f1.q.y = frame1[7 * i + 4]; j->f1.q.z = frame1[7 * i + 5]; j->f1.q.w = frame1[7 * i + 6];
+		/* each jointed body counts the joint among its interactions
+		 * (BodySim::onConstraintAttach, ScBodySim.h:159) */
+		if (j->a >= 0) w->jcount[j->a]++;
+		if (j->b >= 0) w->jcount[j->b]++;
 	}
 	/* colours (the GPU host computes the same): joints between two bodies
 	 * first fit, newest joint first (the order in which the reference's
@@ -3690,6 +3767,179 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 	return first;
 }
 
+/* ---- sleeping ----
+ * Islands, wake-up and deactivation follow the reference CPU pipeline's
+ * observable order (simulationcontroller/src/ScPipeline.cpp:1472-1776,
+ * lowlevel/software/src/PxsIslandSim.cpp:1630-1720): the narrowphase runs on
+ * every pair with an awake body; a touching pair of an awake and a sleeping
+ * body wakes the sleeping body's whole island; then an island whose bodies
+ * were all ready for sleep after the last step goes to sleep before the solve
+ * (velocities zeroed, BodySim::deactivate, ScBodySim.cpp:469-490), and the
+ * solve and integration skip sleeping bodies. */
+
+/* A body counts a plane among its interactions when their broadphase bounds
+ * overlap: the plane's bounds are the half-space behind it when its normal is
+ * a coordinate axis, else the whole space (computePlaneBounds,
+ * geomutils/src/GuBounds.cpp:210-250), each inflated by its contact offset. */
+static int plane_counted(const pxo_world* w, int i, int q, float co)
+{
+	const v3 nr = w->pn[q];
+	const float ax = fabsf(nr.x), ay = fabsf(nr.y), az = fabsf(nr.z);
+	const float e = 1e-6f, one = 1.0f - e;
+	int axis = -1;
+	if (ax > one && ay < e && az < e) axis = 0;
+	else if (ax < e && ay > one && az < e) axis = 1;
+	else if (ax < e && ay < e && az > one) axis = 2;
+	if (axis < 0)
+		return 1;
+	v3 mn, mx;
+	shape_bounds(w->type[i], w->dims + 4 * i, w->pose[i], &mn, &mx);
+	const float lo = (axis == 0 ? mn.x : axis == 1 ? mn.y : mn.z) - co;
+	const float hi = (axis == 0 ? mx.x : axis == 1 ? mx.y : mx.z) + co;
+	const float c = axis == 0 ? nr.x : axis == 1 ? nr.y : nr.z;
+	return c > 0.0f ? lo <= -w->pd[q] + co : hi >= w->pd[q] - co;
+}
+
+/* drops the constraints of sleeping bodies (their contact managers are
+ * inactive); returns the number kept */
+static void drop_sleeping(pxo_world* w)
+{
+	long long k = 0;
+	for (long long c = 0; c < w->ncon; c++)
+	{
+		const pxo_con* x = &w->con[c];
+		const int off = x->b >= 0 ? (w->asleep[x->a] && w->asleep[x->b]) : w->asleep[x->a];
+		if (!off)
+			w->con[k++] = *x;
+	}
+	w->ncon = k;
+}
+
+static int isl_find(int* par, int x)
+{
+	while (par[x] != x)
+	{
+		par[x] = par[par[x]];
+		x = par[x];
+	}
+	return x;
+}
+
+static void isl_union(int* par, int a, int b)
+{
+	a = isl_find(par, a);
+	b = isl_find(par, b);
+	if (a < b) par[b] = a;
+	else if (b < a) par[a] = b;
+}
+
+static int joint_asleep(const pxo_world* w, const pxo_joint* j)
+{
+	return (j->a < 0 || w->asleep[j->a]) && (j->b < 0 || w->asleep[j->b]);
+}
+
+/* wake-up, islands and deactivation between the narrowphase and the solve */
+static void sleep_islands(pxo_world* w)
+{
+	const int n = w->n;
+	/* a touching pair of an awake and a sleeping body wakes the sleeping
+	 * body's island (setEdgesConnected, secondPassIslandGen, wakeObjectsUp:
+	 * ScPipeline.cpp:1726-1776); so do islands named by a wake-up. The woken
+	 * bodies keep their wake counter and are not ready for sleep
+	 * (IslandSim::activateNode clears the flag, PxsIslandSim.cpp:400-432) */
+	for (long long c = 0; c < w->ncon; c++)
+	{
+		const pxo_con* x = &w->con[c];
+		if (x->b >= 0 && w->asleep[x->a] != w->asleep[x->b])
+			w->wakeIsl[w->isl[w->asleep[x->a] ? x->a : x->b]] = 1;
+	}
+	for (int i = 0; i < n; i++)
+		if (w->asleep[i] && w->wakeIsl[w->isl[i]])
+		{
+			w->asleep[i] = 0;
+			w->ready[i] = 0;
+		}
+	memset(w->wakeIsl, 0, (size_t)n);
+	drop_sleeping(w);
+	/* islands of the awake bodies over touching body pairs and joints,
+	 * labelled by their smallest body index; sleeping bodies keep theirs */
+	int* par = (int*)malloc(sizeof(int) * (size_t)(n ? n : 1));
+	for (int i = 0; i < n; i++)
+		par[i] = i;
+	for (long long c = 0; c < w->ncon; c++)
+		if (w->con[c].b >= 0)
+			isl_union(par, w->con[c].a, w->con[c].b);
+	for (int k = 0; k < w->nj; k++)
+	{
+		const pxo_joint* j = &w->joint[k];
+		if (j->a >= 0 && j->b >= 0 && !w->asleep[j->a] && !w->asleep[j->b])
+			isl_union(par, j->a, j->b);
+	}
+	unsigned char* notReady = (unsigned char*)calloc((size_t)(n ? n : 1), 1);
+	for (int i = 0; i < n; i++)
+		if (!w->asleep[i])
+		{
+			w->isl[i] = isl_find(par, i);
+			if (!w->ready[i])
+				notReady[w->isl[i]] = 1;
+		}
+	for (int i = 0; i < n; i++)
+		if (!w->asleep[i] && !notReady[w->isl[i]])
+		{
+			w->asleep[i] = 1;
+			w->lv[i] = V(0, 0, 0);
+			w->av[i] = V(0, 0, 0);
+		}
+	free(notReady);
+	free(par);
+	drop_sleeping(w);
+}
+
+/* updateWakeCounter without stabilization and the readiness it reports
+ * (lowleveldynamics/src/DySleep.cpp:173-235): motion velocities ml, ma
+ * (world frame) of the step just taken, `counted` interactions */
+static void sleep_check(pxo_world* w, int i, float dt, v3 ml, v3 ma, int counted)
+{
+	const float resetTime = 20.0f * 0.02f;
+	float wc = w->wc[i];
+	if (wc < resetTime * 0.5f || wc < dt)
+	{
+		const v3 t = w->invI[i];
+		const v3 I = V(t.x > 0.0f ? 1.0f / t.x : 1.0f, t.y > 0.0f ? 1.0f / t.y : 1.0f, t.z > 0.0f ? 1.0f / t.z : 1.0f);
+		w->accL[i] = add(w->accL[i], ml);
+		w->accA[i] = add(w->accA[i], qrotinv(w->pose[i].q, ma));
+		float invMass = w->invMass[i];
+		if (invMass == 0.0f)
+			invMass = 1.0f;
+		const v3 a = w->accA[i], l = w->accL[i];
+		const float angular = (a.x * a.x * I.x + a.y * a.y * I.y + a.z * a.z * I.z) * invMass;
+		const float linear = l.x * l.x + l.y * l.y + l.z * l.z;
+		const float energy = 0.5f * (angular + linear);
+		const float cluster = (float)(1 + counted);
+		const float threshold = cluster * w->d.sleep_threshold;
+		if (energy >= threshold)
+		{
+			w->accL[i] = V(0, 0, 0);
+			w->accA[i] = V(0, 0, 0);
+			const float factor = threshold == 0.0f ? 2.0f : fminf(energy / threshold, 2.0f);
+			const float old = wc;
+			w->wc[i] = factor * 0.5f * resetTime + dt * (cluster - 1.0f);
+			if (old == 0.0f)
+				w->ready[i] = 0;   /* eACTIVATE_THIS_FRAME: not ready for sleep */
+			return;
+		}
+	}
+	wc = fmaxf(wc - dt, 0.0f);
+	w->wc[i] = wc;
+	if (wc == 0.0f)
+	{
+		/* eDEACTIVATE_THIS_FRAME: ready for sleep, filter reset */
+		w->ready[i] = 1;
+		w->accL[i] = V(0, 0, 0);
+		w->accA[i] = V(0, 0, 0);
+	}
+}
+
 int pxo_world_step(void* wp, float dt)
 {
 	pxo_world* w = (pxo_world*)wp;
@@ -3727,7 +3977,35 @@ int pxo_world_step(void* wp, float dt)
 	}
 	w->stats[0] = pv.n;
 
-	/* 2. narrowphase: body pairs in sorted pair order, then body-plane pairs */
+	/* interactions each body counts for its sleep threshold: broadphase
+	 * pairs that are not filtered out (ShapeInteraction, ScShapeInteraction.cpp:84-93,
+	 * ScPipeline.cpp:1222-1228), planes whose bounds it overlaps, and its
+	 * joints (ScBodySim.h:159) */
+	int* counted = NULL;
+	if (d->enable_sleeping)
+	{
+		counted = (int*)calloc((size_t)(n ? n : 1), sizeof(int));
+		for (long long p = 0; p < pv.n; p++)
+		{
+			const int a = (int)pv.p[2 * p], b = (int)pv.p[2 * p + 1];
+			if (!jointed(w, a, b))
+			{
+				counted[a]++;
+				counted[b]++;
+			}
+		}
+		for (int i = 0; i < n; i++)
+		{
+			counted[i] += w->jcount[i];
+			for (int q = 0; q < w->m; q++)
+				if (w->pscene[q] == w->scene[i] && plane_counted(w, i, q, co))
+					counted[i]++;
+		}
+	}
+
+	/* 2. narrowphase: body pairs in sorted pair order, then body-plane pairs
+	 * (pairs of two sleeping bodies and the planes of sleeping bodies are
+	 * dropped after the wake-up below; they never touch an awake body) */
 	for (long long p = 0; p < pv.n; p++)
 		add_pair(w, (int)pv.p[2 * p], (int)pv.p[2 * p + 1], cd, tol);
 	free(pv.p);
@@ -3761,6 +4039,8 @@ int pxo_world_step(void* wp, float dt)
 			if (k > 0)
 				push_con(w, i, -1 - q, &c);
 		}
+	if (d->enable_sleeping)
+		sleep_islands(w);
 	w->stats[2] = w->ncon;
 	for (long long c = 0; c < w->ncon; c++)
 		w->stats[3] += w->con[c].c.count;
@@ -3869,7 +4149,8 @@ int pxo_world_step(void* wp, float dt)
 		if (ea.x != 0.0f || ea.y != 0.0f || ea.z != 0.0f) av = add(av, scl(ea, dt));
 		w->extLin[i] = V(0, 0, 0);
 		w->extAng[i] = V(0, 0, 0);
-		lv = add(lv, scl(scl(g, dt), 1.0f));
+		if (!w->asleep[i])   /* sleeping bodies are not in the solver: no gravity */
+			lv = add(lv, scl(scl(g, dt), 1.0f));
 		const float ld = 1.0f - d->linear_damping * dt, ad = 1.0f - d->angular_damping * dt;
 		lv = scl(lv, ld >= 0.0f ? ld : 0.0f);
 		av = scl(av, ad >= 0.0f ? ad : 0.0f);
@@ -3914,6 +4195,7 @@ int pxo_world_step(void* wp, float dt)
 	{
 		free(sb);
 		free(frame);
+		free(counted);
 		return -5;   /* joints: TGS only */
 	}
 	/* joint rows (DyTGSDynamics.cpp:1180-1256): poses at the start of the
@@ -3924,6 +4206,8 @@ int pxo_world_step(void* wp, float dt)
 		for (int i = 0; i < w->nj && !pgs; i++)
 		{
 			pxo_joint* j = &w->joint[i];
+			if (joint_asleep(w, j))
+				continue;
 			prep_joint(j, j->a >= 0 ? w->pose[j->a] : id, j->b >= 0 ? w->pose[j->b] : id,
 			           j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, invStepDt, 1.0f / dt,
 			           0.5f * biasCoefficient, d->tolerance_length);
@@ -3997,7 +4281,7 @@ int pxo_world_step(void* wp, float dt)
 			for (int i = 0; i < w->nj; i++)
 			{
 				pxo_joint* j = &w->joint[i];
-				if (j->color != r) continue;
+				if (j->color != r || joint_asleep(w, j)) continue;
 				solve_joint(j, j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, elapsed,
 				            it == P - 1);
 			}
@@ -4018,7 +4302,7 @@ int pxo_world_step(void* wp, float dt)
 			for (int i = 0; i < w->nj; i++)
 			{
 				pxo_joint* j = &w->joint[i];
-				if (j->color != r) continue;
+				if (j->color != r || joint_asleep(w, j)) continue;
 				solve_joint(j, j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, elapsed, 0);
 			}
 		for (int r = 0; r < ncolors; r++)
@@ -4048,6 +4332,8 @@ int pxo_world_step(void* wp, float dt)
 	{
 		for (int i = 0; i < n; i++)
 		{
+			if (w->asleep[i])
+				continue;   /* sleeping: pose and zero velocities kept */
 			const sbody* b = &sb[i];
 			const v3 lm = add(b->origLin, motionLin[i]);
 			w->pose[i].p = add(w->pose[i].p, scl(lm, dt));
@@ -4074,24 +4360,35 @@ int pxo_world_step(void* wp, float dt)
 			}
 			w->lv[i] = add(b->origLin, b->lin);
 			w->av[i] = add(b->origAng, mmul(b->sqrtInvI, b->ang));
+			/* sleep check on the motion velocities (DyDynamics.cpp:1403-1420) */
+			if (counted)
+				sleep_check(w, i, dt, lm, am, counted[i]);
 		}
 		free(motionLin);
 		free(motionAng);
 		free(sb);
 		free(frame);
+		free(counted);
 		return 0;
 	}
 
 	/* 7. copy back (DyTGSDynamics.cpp:1549-1580) */
+	const float invDt = 1.0f / dt;
 	for (int i = 0; i < n; i++)
 	{
+		if (w->asleep[i])
+			continue;   /* sleeping: pose and zero velocities kept */
 		const sbody* b = &sb[i];
 		w->pose[i].q = qnormalize(qmul(b->dq, w->pose[i].q));
 		w->pose[i].p = b->p;
 		w->lv[i] = b->lin;
 		w->av[i] = mmul(b->sqrtInvI, b->ang);
+		/* sleep check on the step's mean velocities (DyTGSDynamics.cpp:1561, 1577) */
+		if (counted)
+			sleep_check(w, i, dt, scl(b->dlin, invDt), mmul(b->sqrtInvI, scl(b->dang, invDt)), counted[i]);
 	}
 	free(sb);
 	free(frame);
+	free(counted);
 	return 0;
 }

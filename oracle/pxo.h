@@ -56,6 +56,10 @@ typedef struct {
 	/* test knob: xored into the colouring order keys (0 = the order the GPU
 	 * path uses); a different value only reorders the Gauss-Seidel passes */
 	unsigned int order_salt;
+	/* sleeping (PxRigidDynamic::setSleepThreshold; 0 = bodies never sleep,
+	 * as with a zero threshold) and the threshold of every body */
+	int   enable_sleeping;
+	float sleep_threshold;
 } pxo_desc;
 
 void pxo_default_desc(pxo_desc* d);
@@ -84,6 +88,11 @@ int   pxo_world_step(void* w, float dt);
 void  pxo_world_get_bodies(void* w, float* pose, float* lin_vel, float* ang_vel);
 void  pxo_world_set_bodies(void* w, const float* pose, const float* lin_vel, const float* ang_vel);
 void  pxo_world_set_forces(void* w, const float* force, const float* torque);
+/* Sleep state per body (PxRigidDynamic::getWakeCounter, isSleeping). */
+void  pxo_world_get_sleep(void* w, float* wake_counter, int32_t* asleep);
+/* PxRigidDynamic::wakeUp on k bodies: wake counter 0.4 s, the rest of each
+ * body's island wakes at the next step. */
+void  pxo_world_wake(void* w, int k, const int32_t* bodies);
 /* Joints without limits or drives (PxFixedJointCreate, PxSphericalJointCreate,
  * PxRevoluteJointCreate, PxPrismaticJointCreate): body0/body1 body indices
  * or -1 for the world frame; frame0/frame1 k x 7 local frames (p xyz, q xyzw)

@@ -513,6 +513,31 @@ void pxref_scene_stats(void* h, long long* out)
 	out[3] = st.nbDiscreteContactPairsWithContacts;
 }
 
+// Turns sleeping on for every body added so far: PxRigidDynamic::
+// setSleepThreshold(threshold) (threshold 0 keeps them awake, the default of
+// pxref_scene_add_bodies). Returns 0.
+int pxref_scene_set_sleep_threshold(void* h, float threshold)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (PxRigidDynamic* b : rs->bodies)
+		b->setSleepThreshold(threshold);
+	return 0;
+}
+
+// Per body: PxRigidDynamic::getWakeCounter() into wc and isSleeping() into
+// asleep (either may be null).
+void pxref_scene_sleep_state(void* h, float* wc, int* asleep)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (size_t i = 0; i < rs->bodies.size(); i++)
+	{
+		if (wc)
+			wc[i] = rs->bodies[i]->getWakeCounter();
+		if (asleep)
+			asleep[i] = rs->bodies[i]->isSleeping() ? 1 : 0;
+	}
+}
+
 // Replaces the scene's CPU dispatcher by one of nThreads workers between
 // steps (PxTaskManager::setCpuDispatcher, physx/include/task/PxTaskManager.h:88);
 // every stage reads the worker count from the task manager when it runs

@@ -48,7 +48,8 @@ class SceneDesc(ctypes.Structure):
                 ("max_angular_velocity_sq", ctypes.c_float), ("bounce_threshold_velocity", ctypes.c_float),
                 ("friction_offset_threshold", ctypes.c_float), ("friction_correlation_distance", ctypes.c_float),
                 ("tolerance_length", ctypes.c_float), ("max_pairs_per_body", ctypes.c_int32),
-                ("enable_body_accelerations", ctypes.c_int32), ("report_touch_events", ctypes.c_int32)]
+                ("enable_body_accelerations", ctypes.c_int32), ("report_touch_events", ctypes.c_int32),
+                ("enable_sleeping", ctypes.c_int32), ("sleep_threshold", ctypes.c_float)]
 
 
 EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_scene_create", "pxh_scene_release",
@@ -62,7 +63,8 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
-            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate")
+            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate",
+            "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies")
 
 _lib = None
 
@@ -348,6 +350,20 @@ class Scene:
                                                    lo.ctypes.data_as(_I32), ctypes.c_int64(int(cnt[1])),
                                                    cnt.ctypes.data_as(_I64)))
         return f[: cnt[0]], lo[: cnt[1]]
+
+    def sleep_state(self):
+        """(wake counter, asleep) of every body (PxRigidDynamic::getWakeCounter,
+        isSleeping); needs enable_sleeping."""
+        wc = np.zeros(self.n, np.float32)
+        asleep = np.zeros(self.n, np.int32)
+        _check(self.lib.pxh_scene_get_sleep_state(self.h, wc.ctypes.data_as(_F), asleep.ctypes.data_as(_I32),
+                                                  self.n))
+        return wc, asleep.astype(bool)
+
+    def wake_bodies(self, bodies):
+        """PxRigidDynamic::wakeUp on each listed body."""
+        b = _i32(bodies)
+        _check(self.lib.pxh_scene_wake_bodies(self.h, b.ctypes.data_as(_I32), int(b.size)))
 
     def remove_worlds(self, worlds):
         """PxScene::removeActors for every actor of the listed worlds; the

@@ -87,6 +87,25 @@ struct PatchIO
 	int4* conRec;
 };
 
+// Sleeping, one entry per body: the sleep filter of PxsRigidBody
+// (lowlevel/software/include/PxsRigidBody.h) and the island manager's node
+// state. flags == nullptr: sleeping is off.
+constexpr uint32_t SLEEP_ASLEEP = 1u, SLEEP_READY = 2u;
+struct SleepState
+{
+	uint32_t* flags;     // SLEEP_ASLEEP | SLEEP_READY (ready for sleep)
+	float* wc;           // wake counter (s)
+	float4* accL;        // accumulated motion velocity: linear (world frame)
+	float4* accA;        // angular (body frame)
+	int* isl;            // island label: smallest body index of the island
+	int* counted;        // interactions counted this step (numCountedInteractions)
+	const int* jcount;   // joints per body
+	uint32_t* wakeIsl;   // per label: wake the island at the next step
+	uint32_t* notReady;  // per label: the island has a body not ready for sleep
+	int* parent;         // island union-find
+	float threshold;     // PxRigidDynamic::setSleepThreshold
+};
+
 int64_t scan_tmp_size(int64_t n);
 void launch_scan(uint32_t* data, int64_t n, uint32_t* tmp, hipStream_t st);
 // keys per radix-sort workgroup (its histogram holds 256 counts per workgroup)
@@ -169,7 +188,7 @@ void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                          float maxAngSq, bool pgs, SolverBodies sb, float dt, float4* extLin, float4* extAng,
-                         float4* prevLin, float4* prevAng, hipStream_t st);
+                         float4* prevLin, float4* prevAng, const uint32_t* sleepFlags, hipStream_t st);
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
                  float scale, float restDist, float fot, Rows rows, PatchIO patches, unsigned long long* counters,
@@ -203,15 +222,21 @@ void launch_solve_pgs(int begin, int end, int C, Rows rows, const float* rowsU, 
                       bool velocityPass, uint32_t* broken, hipStream_t st, hipEvent_t evStart = nullptr,
                       hipEvent_t evStop = nullptr);
 void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st);
-void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb,
-                         hipStream_t st);
+// write-back; with sleeping on, sleeping bodies keep their state and the
+// others take the sleep check (ss.flags null: off)
+void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, const float4* invI,
+                         SolverBodies sb, SleepState ss, hipStream_t st);
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st);
-void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st);
+void launch_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, const float4* invI,
+                     SolverBodies sb, SleepState ss, hipStream_t st);
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
                      const float4* ang, const float4* prevLin, const float4* prevAng, float invDt, float* out,
                      hipStream_t st);
+// writes wake the written bodies when sleeping is on (forces and torques when
+// nonzero)
 void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                     const float4* invI, float4* extLin, float4* extAng, const float* in, hipStream_t st);
+                     const float4* invI, float4* extLin, float4* extAng, const float* in, SleepState ss,
+                     hipStream_t st);
 // keys of pairs mapped through ids (device, may be null: box indices)
 void launch_pair_keys(int P, const uint2* pairs, const uint32_t* ids, uint64_t* keys, uint32_t* vals, hipStream_t st);
 void launch_set_diff(int na, const uint64_t* a, int nb, const uint64_t* b, uint32_t* flag, hipStream_t st);
@@ -237,11 +262,34 @@ constexpr int JOINT_HEAD = 8, JOINT_ROWS = 36;
 void launch_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs, const uint64_t* jointKeys, int nKeys,
                          float4* ctN, hipStream_t st);
 // rows of J joints from the poses at the step's start and the solver bodies
+// (joints whose bodies all sleep are skipped; sleepFlags may be null)
 void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
                        SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale, float4* head,
-                       float4* rows, hipStream_t st);
+                       float4* rows, const uint32_t* sleepFlags, hipStream_t st);
 // one pass over joints [begin, end) (one colour)
 void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
-                        float elapsed, bool conclude, hipStream_t st);
+                        float elapsed, bool conclude, const uint32_t* sleepFlags, hipStream_t st);
+
+// sleeping stages between the narrowphase and the constraint list (slots:
+// Q body-plane pairs, then the body pairs [Q, Q + min(*dP, cap))):
+// interactions each body counts (broadphase pairs that are not jointed,
+// planes whose bounds it overlaps, joints)
+void launch_sleep_count(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
+                        const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
+                        const uint64_t* jointKeys, int nKeys, SleepState ss, hipStream_t st);
+// touching pairs of an awake and a sleeping body wake the sleeping body's
+// island, as do islands a wake-up named; then the pairs of sleeping bodies
+// lose their contacts
+void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
+                       SleepState ss, hipStream_t st);
+// islands of the awake bodies over touching pairs and joints; islands whose
+// bodies were all ready for sleep go to sleep (velocities zeroed) and their
+// pairs lose their contacts
+void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
+                          int J, const int4* jInfo, float4* lin, float4* ang, SleepState ss, hipStream_t st);
+// PxRigidDynamic::wakeUp of k bodies (device indices)
+void launch_sleep_wake_bodies(int k, const int* bodies, SleepState ss, hipStream_t st);
+// sleep state readback: wake counter and asleep flag per body
+void launch_sleep_read(int n, SleepState ss, float* wc, int* asleep, hipStream_t st);
 
 }  // namespace pxh

@@ -1158,7 +1158,7 @@ __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, cons
                                const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                                float maxAngSq, int pgs, float4* vel, float4* svP, float4* svDQ, float4* svI0,
                                float4* svI1, float4* svOrigLin, float4* rec, float dt, float4* extLin, float4* extAng,
-                               float4* prevLin, float4* prevAng)
+                               float4* prevLin, float4* prevAng, const uint32_t* sleepFlags)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
@@ -1186,7 +1186,9 @@ __global__ void k_preintegrate(int n, const float4* pos, const float4* rot, cons
 		extLin[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 		extAng[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 	}
-	lv = lv + mk(gdt.x, gdt.y, gdt.z);
+	// sleeping bodies are not in the solver: no gravity, velocities stay zero
+	if (!sleepFlags || !(sleepFlags[i] & SLEEP_ASLEEP))
+		lv = lv + mk(gdt.x, gdt.y, gdt.z);
 	const float ld = 1.0f - linDampDt, ad = 1.0f - angDampDt;
 	lv = lv * (ld >= 0.0f ? ld : 0.0f);
 	av = av * (ad >= 0.0f ? ad : 0.0f);
@@ -1963,15 +1965,64 @@ __global__ void k_pgs_save_motion(int n, const float4* vel, float4* motionLin, f
 	motionAng[i] = vel[4 * int64_t(i) + 1];
 }
 
+// updateWakeCounter without stabilization and the readiness it reports
+// (lowleveldynamics/src/DySleep.cpp:173-235) for body i after its step: ml,
+// ma its motion velocities (world frame), q its new orientation, invMass and
+// invI (mass-space inverse inertia) its mass properties
+PXH_D void sleepCheck(const SleepState& ss, int i, float dt, f3 ml, f3 ma, qt q, float invMass, float4 invI)
+{
+	const float resetTime = 20.0f * 0.02f;
+	float wc = ss.wc[i];
+	if (wc < resetTime * 0.5f || wc < dt)
+	{
+		const f3 I = mk(invI.x > 0.0f ? 1.0f / invI.x : 1.0f, invI.y > 0.0f ? 1.0f / invI.y : 1.0f,
+		                invI.z > 0.0f ? 1.0f / invI.z : 1.0f);
+		const f3 l = mk(ss.accL[i]) + ml;
+		const f3 a = mk(ss.accA[i]) + qrotinv(q, ma);
+		const float im = invMass == 0.0f ? 1.0f : invMass;
+		const float angular = (a.x * a.x * I.x + a.y * a.y * I.y + a.z * a.z * I.z) * im;
+		const float linear = l.x * l.x + l.y * l.y + l.z * l.z;
+		const float energy = 0.5f * (angular + linear);
+		const float cluster = float(1 + ss.counted[i]);
+		const float threshold = cluster * ss.threshold;
+		if (energy >= threshold)
+		{
+			// woken by its own motion: filter reset, new wake counter; from a
+			// zero counter (eACTIVATE_THIS_FRAME) no longer ready for sleep
+			ss.accL[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			ss.accA[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+			const float factor = threshold == 0.0f ? 2.0f : fminf(energy / threshold, 2.0f);
+			ss.wc[i] = factor * 0.5f * resetTime + dt * (cluster - 1.0f);
+			if (wc == 0.0f)
+				ss.flags[i] &= ~SLEEP_READY;
+			return;
+		}
+		ss.accL[i] = f4(l, 0.0f);
+		ss.accA[i] = f4(a, 0.0f);
+	}
+	wc = fmaxf(wc - dt, 0.0f);
+	ss.wc[i] = wc;
+	if (wc == 0.0f)
+	{
+		// eDEACTIVATE_THIS_FRAME: ready for sleep, filter reset
+		ss.flags[i] |= SLEEP_READY;
+		ss.accL[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+		ss.accA[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+	}
+}
+
 // integrateCore (lowleveldynamics/src/DyBodyCoreIntegrator.h:83-175): poses
-// from the motion velocities, body velocities from the final changes
+// from the motion velocities, body velocities from the final changes; the
+// sleep check on the motion velocities (DyDynamics.cpp:1403-1420)
 __global__ void k_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang,
-                               const float4* vel, const float4* motionLin, const float4* motionAng,
-                               const float4* svI0, const float4* svI1)
+                               const float4* invI, const float4* vel, const float4* motionLin,
+                               const float4* motionAng, const float4* svI0, const float4* svI1, SleepState ss)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
+	if (ss.flags && (ss.flags[i] & SLEEP_ASLEEP))
+		return;   // sleeping: pose and zero velocities kept
 	const float4* v = vel + 4 * int64_t(i);
 	const f3 vD = mk(v[2]), wD = mk(v[3]);
 	const m33 S = loadSym(svI0[i], svI1[i]);
@@ -2001,6 +2052,8 @@ __global__ void k_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4
 	const float4 l = v[0];
 	lin[i] = f4(vD + mk(l), l.w);
 	ang[i] = f4(wD + mmul(S, mk(v[1])), 0.0f);
+	if (ss.flags)
+		sleepCheck(ss, i, dt, lm, am, mkq(rot[i]), l.w, invI[i]);
 }
 
 // integrateCoreStep (lowleveldynamics/src/DyTGSDynamics.cpp:1403-1476) of
@@ -2120,13 +2173,10 @@ __global__ void __launch_bounds__(SOLVE_WORLD_THREADS) k_solve_worlds(
 // jointed bodies do not collide (PxConstraintFlag::eCOLLISION_ENABLED is off
 // by default, physx/include/PxConstraint.h): their pair slots lose their
 // contacts before the constraint list is built
-__global__ void k_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs, const uint64_t* keys, int nKeys,
-                               float4* ctN)
+// whether the bodies of a pair are joined: its (lower, higher) key is among
+// the sorted joint keys
+PXH_D bool pairJointed(uint2 ab, const uint64_t* keys, int nKeys)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= cap || uint32_t(i) >= *dP)
-		return;
-	const uint2 ab = pairs[i];
 	const uint32_t lo = ab.x < ab.y ? ab.x : ab.y, hi = ab.x < ab.y ? ab.y : ab.x;
 	const uint64_t k = (uint64_t(lo) << 32) | hi;
 	int a = 0, b = nKeys;
@@ -2138,8 +2188,237 @@ __global__ void k_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* 
 		else
 			b = m;
 	}
-	if (a < nKeys && keys[a] == k)
+	return a < nKeys && keys[a] == k;
+}
+
+__global__ void k_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs, const uint64_t* keys, int nKeys,
+                               float4* ctN)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= cap || uint32_t(i) >= *dP)
+		return;
+	if (pairJointed(pairs[i], keys, nKeys))
 		ctN[Q + i].w = __int_as_float(0);
+}
+
+// ---------------------------------------------------------------------------
+// sleeping. The observable order of the reference CPU pipeline
+// (simulationcontroller/src/ScPipeline.cpp:1472-1776, lowlevel/software/src/
+// PxsIslandSim.cpp:1630-1720): the narrowphase runs on every pair; a touching
+// pair of an awake and a sleeping body wakes the sleeping body's island; an
+// island whose bodies were all ready for sleep after the last step sleeps
+// before the solve; the solve and integration skip sleeping bodies.
+// ---------------------------------------------------------------------------
+PXH_D bool asleepBody(const uint32_t* flags, uint32_t i) { return (flags[i] & SLEEP_ASLEEP) != 0u; }
+
+__global__ void k_sleep_count_init(int n, SleepState ss)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+		ss.counted[i] = ss.jcount[i];   // each joint counts (BodySim::onConstraintAttach, ScBodySim.h:159)
+}
+
+// Interactions a body counts: broadphase pairs that are not filtered out
+// (ScShapeInteraction.cpp:84-93, ScPipeline.cpp:1222-1228) and planes whose
+// bounds its inflated bounds overlap: the half-space behind the plane when its
+// normal is a coordinate axis, else the whole space (computePlaneBounds,
+// geomutils/src/GuBounds.cpp:210-250), inflated by the plane's contact offset.
+__global__ void k_sleep_count(int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
+                              const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
+                              const uint64_t* keys, int nKeys, SleepState ss)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < Q)
+	{
+		const int i = qBody[s];
+		const float4 pl = planes[qPlane[s]];
+		const float ax = fabsf(pl.x), ay = fabsf(pl.y), az = fabsf(pl.z);
+		const float e = 1e-6f, one = 1.0f - e;
+		int axis = -1;
+		if (ax > one && ay < e && az < e) axis = 0;
+		else if (ax < e && ay > one && az < e) axis = 1;
+		else if (ax < e && ay < e && az > one) axis = 2;
+		bool counted = true;
+		if (axis >= 0)
+		{
+			const float4 lo4 = bmin[i], hi4 = bmax[i];
+			const float lo = axis == 0 ? lo4.x : axis == 1 ? lo4.y : lo4.z;
+			const float hi = axis == 0 ? hi4.x : axis == 1 ? hi4.y : hi4.z;
+			const float c = axis == 0 ? pl.x : axis == 1 ? pl.y : pl.z;
+			counted = c > 0.0f ? lo <= -pl.w + co : hi >= pl.w - co;
+		}
+		if (counted)
+			atomicAdd(&ss.counted[i], 1);
+		return;
+	}
+	const int k = s - Q;
+	if (k >= cap || uint32_t(k) >= *dP)
+		return;
+	const uint2 ab = pairs[k];
+	if (nKeys > 0 && pairJointed(ab, keys, nKeys))
+		return;   // jointed bodies: the pair is filtered out, no interaction
+	atomicAdd(&ss.counted[ab.x], 1);
+	atomicAdd(&ss.counted[ab.y], 1);
+}
+
+// a touching pair of an awake and a sleeping body names the sleeping body's
+// island (setEdgesConnected, secondPassIslandGen: ScPipeline.cpp:1726-1776)
+__global__ void k_sleep_wake_mark(int Q, int cap, const uint32_t* dP, const uint2* pairs, const float4* ctN,
+                                  SleepState ss)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= cap || uint32_t(k) >= *dP || __float_as_int(ctN[Q + k].w) <= 0)
+		return;
+	const uint2 ab = pairs[k];
+	const bool sa = asleepBody(ss.flags, ab.x), sb = asleepBody(ss.flags, ab.y);
+	if (sa != sb)
+		ss.wakeIsl[ss.isl[sa ? ab.x : ab.y]] = 1u;
+}
+
+// bodies of named islands wake (wakeObjectsUp), keeping their wake counter,
+// not ready for sleep (IslandSim::activateNode, PxsIslandSim.cpp:400-432)
+__global__ void k_sleep_wake_apply(int n, SleepState ss)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && asleepBody(ss.flags, i) && ss.wakeIsl[ss.isl[i]])
+		ss.flags[i] = 0u;
+}
+
+// pairs of sleeping bodies lose their contacts (inactive contact managers)
+__global__ void k_sleep_drop(int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
+                             const uint32_t* flags)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < Q)
+	{
+		if (asleepBody(flags, qBody[s]))
+			ctN[s].w = __int_as_float(0);
+		return;
+	}
+	const int k = s - Q;
+	if (k >= cap || uint32_t(k) >= *dP)
+		return;
+	const uint2 ab = pairs[k];
+	if (asleepBody(flags, ab.x) && asleepBody(flags, ab.y))
+		ctN[s].w = __int_as_float(0);
+}
+
+__global__ void k_sleep_isl_init(int n, SleepState ss)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+	{
+		ss.parent[i] = i;
+		ss.notReady[i] = 0u;
+		ss.wakeIsl[i] = 0u;
+	}
+}
+
+// Island union-find over the body indices, roots the smallest index of their
+// tree. Parents only ever move to smaller indices, so a stale read is still an
+// ancestor; roots are linked only by compare-and-swap, and path halving writes
+// only to non-roots.
+PXH_D int islFind(int* par, int x)
+{
+	int cur = par[x];
+	if (cur == x)
+		return x;
+	int prev = x, next;
+	while (cur > (next = par[cur]))
+	{
+		par[prev] = next;
+		prev = cur;
+		cur = next;
+	}
+	return cur;
+}
+
+PXH_D void islUnion(int* par, int a, int b)
+{
+	int ra = islFind(par, a), rb = islFind(par, b);
+	while (ra != rb)
+	{
+		if (ra < rb)
+		{
+			const int t = ra;
+			ra = rb;
+			rb = t;
+		}
+		// link the larger root under the smaller one
+		const int old = atomicCAS(&par[ra], ra, rb);
+		if (old == ra)
+			return;
+		ra = islFind(par, old);
+		rb = islFind(par, rb);
+	}
+}
+
+// touching body pairs (both awake after the wake-up) and joints of two awake
+// bodies join islands
+__global__ void k_sleep_union(int Q, int cap, const uint32_t* dP, const uint2* pairs, const float4* ctN, int J,
+                              const int4* jInfo, SleepState ss)
+{
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s < cap)
+	{
+		if (uint32_t(s) < *dP && __float_as_int(ctN[Q + s].w) > 0)
+		{
+			const uint2 ab = pairs[s];
+			islUnion(ss.parent, int(ab.x), int(ab.y));
+		}
+		return;
+	}
+	const int j = s - cap;
+	if (j >= J)
+		return;
+	const int4 ji = jInfo[j];
+	if (ji.y >= 0 && ji.z >= 0 && !asleepBody(ss.flags, ji.y) && !asleepBody(ss.flags, ji.z))
+		islUnion(ss.parent, ji.y, ji.z);
+}
+
+// awake bodies take their island's label; a body not ready for sleep keeps
+// its island awake
+__global__ void k_sleep_label(int n, SleepState ss)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n || asleepBody(ss.flags, i))
+		return;
+	const int r = islFind(ss.parent, i);
+	ss.isl[i] = r;
+	if (!(ss.flags[i] & SLEEP_READY))
+		ss.notReady[r] = 1u;
+}
+
+// islands whose bodies are all ready go to sleep, velocities zeroed
+// (IslandSim::processLostEdges deactivation, PxsIslandSim.cpp:1690-1720;
+// BodySim::deactivate, ScBodySim.cpp:469-490)
+__global__ void k_sleep_deactivate(int n, SleepState ss, float4* lin, float4* ang)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n || asleepBody(ss.flags, i) || ss.notReady[ss.isl[i]])
+		return;
+	ss.flags[i] |= SLEEP_ASLEEP;
+	lin[i] = make_float4(0.0f, 0.0f, 0.0f, lin[i].w);
+	ang[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+PXH_D void wakeBody(const SleepState& ss, uint32_t i);
+
+__global__ void k_sleep_wake_bodies(int k, const int* bodies, SleepState ss)
+{
+	const int j = blockIdx.x * blockDim.x + threadIdx.x;
+	if (j < k)
+		wakeBody(ss, uint32_t(bodies[j]));
+}
+
+__global__ void k_sleep_read(int n, SleepState ss, float* wc, int* asleep)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n)
+	{
+		wc[i] = ss.wc[i];
+		asleep[i] = asleepBody(ss.flags, i) ? 1 : 0;
+	}
 }
 
 PXH_D xf bodyPose(int i, const float4* pos, const float4* rot)
@@ -2176,14 +2455,23 @@ PXH_D xf jointFrame(const float4* f)
 	return t;
 }
 
+// a joint whose bodies all sleep is out of the solve
+PXH_D bool jointAsleep(int4 ji, const uint32_t* sleepFlags)
+{
+	return sleepFlags && (ji.y < 0 || (sleepFlags[ji.y] & SLEEP_ASLEEP)) &&
+	       (ji.z < 0 || (sleepFlags[ji.z] & SLEEP_ASLEEP));
+}
+
 __global__ void k_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
                              const float4* vel, const float4* svI0, const float4* svI1, float invStepDt, float invDt,
-                             float erp, float lengthScale, float4* head, float4* rows)
+                             float erp, float lengthScale, float4* head, float4* rows, const uint32_t* sleepFlags)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= J)
 		return;
 	const int4 ji = info[j];
+	if (jointAsleep(ji, sleepFlags))
+		return;
 	const xf bA = bodyPose(ji.y, pos, rot), bB = bodyPose(ji.z, pos, rot);
 	const m33 sI0 = bodySqrtInvI(ji.y, svI0, svI1), sI1 = bodySqrtInvI(ji.z, svI0, svI1);
 	const float im0 = ji.y >= 0 ? vel[4 * int64_t(ji.y)].w : 0.0f;
@@ -2193,12 +2481,15 @@ __global__ void k_joint_prep(int J, const int4* info, const float4* frame, const
 }
 
 __global__ void k_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, float4* vel,
-                              const float4* svDQ, const float4* svI0, const float4* svI1, float elapsed, int conclude)
+                              const float4* svDQ, const float4* svI0, const float4* svI1, float elapsed, int conclude,
+                              const uint32_t* sleepFlags)
 {
 	const int j = begin + blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= end)
 		return;
 	const int4 ji = info[j];
+	if (jointAsleep(ji, sleepFlags))
+		return;
 	const int a = ji.y, b = ji.z;
 	f3 l0 = mk(0.0f, 0.0f, 0.0f), a0 = l0, dl0 = l0, da0 = l0, l1 = l0, a1 = l0, dl1 = l0, da1 = l0;
 	qt q0, q1;
@@ -2234,17 +2525,30 @@ __global__ void k_joint_solve(int begin, int end, const int4* info, const float4
 	}
 }
 
-// copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580)
-__global__ void k_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, const float4* vel,
-                           const float4* svP, const float4* svDQ, const float4* svI0, const float4* svI1)
+// copyBackBodies (lowleveldynamics/src/DyTGSDynamics.cpp:1549-1580), with
+// the sleep check on the step's mean velocities (DyTGSDynamics.cpp:1561, 1577)
+__global__ void k_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, const float4* invI,
+                           const float4* vel, const float4* svP, const float4* svDQ, const float4* svI0,
+                           const float4* svI1, SleepState ss)
 {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n)
 		return;
-	rot[i] = f4(qnormalize(qmul(mkq(svDQ[i]), mkq(rot[i]))));
+	if (ss.flags && (ss.flags[i] & SLEEP_ASLEEP))
+		return;   // sleeping: pose and zero velocities kept
+	const qt q = qnormalize(qmul(mkq(svDQ[i]), mkq(rot[i])));
+	rot[i] = f4(q);
 	pos[i] = svP[i];
-	lin[i] = vel[4 * int64_t(i)];
-	ang[i] = f4(mmul(loadSym(svI0[i], svI1[i]), mk(vel[4 * int64_t(i) + 1])), 0.0f);
+	const float4* v = vel + 4 * int64_t(i);
+	const float4 l = v[0];
+	lin[i] = l;
+	const m33 S = loadSym(svI0[i], svI1[i]);
+	ang[i] = f4(mmul(S, mk(v[1])), 0.0f);
+	if (ss.flags)
+	{
+		const float invDt = 1.0f / dt;
+		sleepCheck(ss, i, dt, mk(v[2]) * invDt, mmul(S, mk(v[3]) * invDt), q, l.w, invI[i]);
+	}
 }
 
 // ---------------------------------------------------------------------------
@@ -2287,8 +2591,20 @@ __global__ void k_get_data(int n, const uint32_t* idx, int what, const float4* p
 // F / m, a torque the angular acceleration I_world^-1 tau with the body's
 // current orientation (PxForceMode::eFORCE, NpRigidBodyTemplate.h:507-528;
 // setRigidDynamicForce / setRigidDynamicTorque, updateBodiesAndShapes.cu:1229-1278)
+// PxRigidDynamic::wakeUp (BodyCore::wakeUp with the scene's wake counter
+// reset value, 20 * 0.02 s): awake now, not ready for sleep; the rest of a
+// sleeping body's island wakes at the next step (IslandSim::activateNode)
+PXH_D void wakeBody(const SleepState& ss, uint32_t i)
+{
+	ss.wc[i] = 20.0f * 0.02f;
+	const uint32_t f = ss.flags[i];
+	ss.flags[i] = 0u;
+	if (f & SLEEP_ASLEEP)
+		ss.wakeIsl[ss.isl[i]] = 1u;
+}
+
 __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                           const float4* invI, float4* extLin, float4* extAng, const float* in)
+                           const float4* invI, float4* extLin, float4* extAng, const float* in, SleepState ss)
 {
 	const int k = blockIdx.x * blockDim.x + threadIdx.x;
 	if (k >= n)
@@ -2299,10 +2615,16 @@ __global__ void k_set_data(int n, const uint32_t* idx, int what, float4* pos, fl
 		const float* s = in + 7 * int64_t(k);
 		rot[i] = make_float4(s[0], s[1], s[2], s[3]);
 		pos[i] = make_float4(s[4], s[5], s[6], 0.0f);
+		if (ss.flags)
+			wakeBody(ss, i);   // autowake of setGlobalPose
 		return;
 	}
 	const float* s = in + 3 * int64_t(k);
 	const f3 v = mk(s[0], s[1], s[2]);
+	// velocities always wake the body, forces and torques when nonzero
+	// (autowake of setLinearVelocity / addForce)
+	if (ss.flags && (what <= 2 || v.x != 0.0f || v.y != 0.0f || v.z != 0.0f))
+		wakeBody(ss, i);
 	if (what == 1)
 		lin[i] = make_float4(v.x, v.y, v.z, lin[i].w);
 	else if (what == 2)
@@ -2589,20 +2911,51 @@ void launch_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs,
 	if (nKeys > 0)
 		PXH_LAUNCH(k_joint_filter, cap, cap, Q, dP, pairs, jointKeys, nKeys, ctN);
 }
+void launch_sleep_count(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
+                        const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
+                        const uint64_t* jointKeys, int nKeys, SleepState ss, hipStream_t st)
+{
+	PXH_LAUNCH(k_sleep_count_init, n, n, ss);
+	PXH_LAUNCH(k_sleep_count, Q + cap, Q, cap, dP, pairs, qBody, qPlane, planes, bmin, bmax, co, jointKeys, nKeys, ss);
+}
+void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
+                       SleepState ss, hipStream_t st)
+{
+	PXH_LAUNCH(k_sleep_wake_mark, cap, Q, cap, dP, pairs, ctN, ss);
+	PXH_LAUNCH(k_sleep_wake_apply, n, n, ss);
+	PXH_LAUNCH(k_sleep_drop, Q + cap, Q, cap, dP, pairs, qBody, ctN, ss.flags);
+}
+void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
+                          int J, const int4* jInfo, float4* lin, float4* ang, SleepState ss, hipStream_t st)
+{
+	PXH_LAUNCH(k_sleep_isl_init, n, n, ss);
+	PXH_LAUNCH(k_sleep_union, cap + J, Q, cap, dP, pairs, ctN, J, jInfo, ss);
+	PXH_LAUNCH(k_sleep_label, n, n, ss);
+	PXH_LAUNCH(k_sleep_deactivate, n, n, ss, lin, ang);
+	PXH_LAUNCH(k_sleep_drop, Q + cap, Q, cap, dP, pairs, qBody, ctN, ss.flags);
+}
+void launch_sleep_wake_bodies(int k, const int* bodies, SleepState ss, hipStream_t st)
+{
+	PXH_LAUNCH(k_sleep_wake_bodies, k, k, bodies, ss);
+}
+void launch_sleep_read(int n, SleepState ss, float* wc, int* asleep, hipStream_t st)
+{
+	PXH_LAUNCH(k_sleep_read, n, n, ss, wc, asleep);
+}
 void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
                        SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale, float4* head,
-                       float4* rows, hipStream_t st)
+                       float4* rows, const uint32_t* sleepFlags, hipStream_t st)
 {
 	PXH_LAUNCH(k_joint_prep, J, J, info, frame, pos, rot, sb.vel, sb.i0, sb.i1, invStepDt, invDt, erp, lengthScale, head,
-	           rows);
+	           rows, sleepFlags);
 }
 void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
-                        float elapsed, bool conclude, hipStream_t st)
+                        float elapsed, bool conclude, const uint32_t* sleepFlags, hipStream_t st)
 {
 	if (end <= begin)
 		return;
 	k_joint_solve<<<nblk(end - begin, 64), 64, 0, st>>>(begin, end, info, head, rows, sb.vel, sb.dq, sb.i0, sb.i1,
-	                                                     elapsed, conclude ? 1 : 0);
+	                                                     elapsed, conclude ? 1 : 0, sleepFlags);
 }
 void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st)
 {
@@ -2691,10 +3044,10 @@ void launch_max_color(int C, const int* conColor, uint32_t* maxColor, hipStream_
 void launch_preintegrate(int n, const float4* pos, const float4* rot, const float4* lin, const float4* ang,
                          const float4* invI, float3 gdt, float linDampDt, float angDampDt, float maxLinSq,
                          float maxAngSq, bool pgs, SolverBodies sb, float dt, float4* extLin, float4* extAng,
-                         float4* prevLin, float4* prevAng, hipStream_t st)
+                         float4* prevLin, float4* prevAng, const uint32_t* sleepFlags, hipStream_t st)
 {
 	PXH_LAUNCH(k_preintegrate, n, n, pos, rot, lin, ang, invI, gdt, linDampDt, angDampDt, maxLinSq, maxAngSq, pgs ? 1 : 0, sb.vel,
-	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin, sb.rec, dt, extLin, extAng, prevLin, prevAng);
+	           sb.p, sb.dq, sb.i0, sb.i1, sb.origLin, sb.rec, dt, extLin, extAng, prevLin, prevAng, sleepFlags);
 }
 void launch_prep(int C, const int* order, const int* conA, const int* conB, const int* conSlot, const float4* ctN,
                  const float4* ctP, const float4* pos, const float4* rot, SolverBodies sb, float invStepDt,
@@ -2786,18 +3139,19 @@ void launch_pgs_save_motion(int n, SolverBodies sb, hipStream_t st)
 {
 	PXH_LAUNCH(k_pgs_save_motion, n, n, sb.vel, sb.p, sb.dq);
 }
-void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb,
-                         hipStream_t st)
+void launch_pgs_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, const float4* invI,
+                         SolverBodies sb, SleepState ss, hipStream_t st)
 {
-	PXH_LAUNCH(k_pgs_finalize, n, n, dt, pos, rot, lin, ang, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
+	PXH_LAUNCH(k_pgs_finalize, n, n, dt, pos, rot, lin, ang, invI, sb.vel, sb.p, sb.dq, sb.i0, sb.i1, ss);
 }
 void launch_integrate(int n, float dt, SolverBodies sb, hipStream_t st)
 {
 	PXH_LAUNCH(k_integrate, n, n, dt, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
 }
-void launch_finalize(int n, float4* pos, float4* rot, float4* lin, float4* ang, SolverBodies sb, hipStream_t st)
+void launch_finalize(int n, float dt, float4* pos, float4* rot, float4* lin, float4* ang, const float4* invI,
+                     SolverBodies sb, SleepState ss, hipStream_t st)
 {
-	PXH_LAUNCH(k_finalize, n, n, pos, rot, lin, ang, sb.vel, sb.p, sb.dq, sb.i0, sb.i1);
+	PXH_LAUNCH(k_finalize, n, n, dt, pos, rot, lin, ang, invI, sb.vel, sb.p, sb.dq, sb.i0, sb.i1, ss);
 }
 void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, const float4* rot, const float4* lin,
                      const float4* ang, const float4* prevLin, const float4* prevAng, float invDt, float* out,
@@ -2806,9 +3160,10 @@ void launch_get_data(int n, const uint32_t* idx, int what, const float4* pos, co
 	PXH_LAUNCH(k_get_data, n, n, idx, what, pos, rot, lin, ang, prevLin, prevAng, invDt, out);
 }
 void launch_set_data(int n, const uint32_t* idx, int what, float4* pos, float4* rot, float4* lin, float4* ang,
-                     const float4* invI, float4* extLin, float4* extAng, const float* in, hipStream_t st)
+                     const float4* invI, float4* extLin, float4* extAng, const float* in, SleepState ss,
+                     hipStream_t st)
 {
-	PXH_LAUNCH(k_set_data, n, n, idx, what, pos, rot, lin, ang, invI, extLin, extAng, in);
+	PXH_LAUNCH(k_set_data, n, n, idx, what, pos, rot, lin, ang, invI, extLin, extAng, in, ss);
 }
 void launch_touch_events(int C, const int* prevRow, int R, uint32_t* found, uint32_t* matched, uint32_t* lost,
                          hipStream_t st)

@@ -255,6 +255,35 @@ struct pxh_scene_t
 	DBuf<int4> jInfo;
 	DBuf<float4> jFrame, jHead, jRows;
 	DBuf<uint64_t> jKeys;
+	// sleeping (desc.enable_sleeping): per body the sleep flags, wake counter,
+	// accumulated motion velocities and island label; per island label the
+	// pending wake-up (set between steps by wake-ups); per step the counted
+	// interactions, joints per body, not-ready flags and union-find parents
+	DBuf<uint32_t> slFlags, slWakeIsl, slNotReady;
+	DBuf<float> slWc;
+	DBuf<float4> slAccL, slAccA;
+	DBuf<int> slIsl, slCounted, slJCount, slParent, slList;
+	DBuf<float> slWcOut;
+	DBuf<int> slAsleepOut;
+	bool slJCountDirty = true;
+	pxh::SleepState sleepState()
+	{
+		pxh::SleepState ss{};
+		if (!desc.enable_sleeping)
+			return ss;
+		ss.flags = slFlags.p;
+		ss.wc = slWc.p;
+		ss.accL = slAccL.p;
+		ss.accA = slAccA.p;
+		ss.isl = slIsl.p;
+		ss.counted = slCounted.p;
+		ss.jcount = slJCount.p;
+		ss.wakeIsl = slWakeIsl.p;
+		ss.notReady = slNotReady.p;
+		ss.parent = slParent.p;
+		ss.threshold = desc.sleep_threshold;
+		return ss;
+	}
 	DBuf<unsigned char> gatherTmp;        // world removal: gather scratch and indices
 	DBuf<int> gatherIdx;
 	uint32_t* hostCounts = nullptr;       // pinned
@@ -567,6 +596,8 @@ void pxh_default_scene_desc(pxh_scene_desc* d)
 	d->max_pairs_per_body = 4;
 	d->enable_body_accelerations = 0;
 	d->report_touch_events = 0;
+	d->enable_sleeping = 0;
+	d->sleep_threshold = 5e-5f * 10.0f * 10.0f;
 }
 
 int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
@@ -581,6 +612,10 @@ int32_t pxh_scene_create(const pxh_scene_desc* desc, pxh_scene* out)
 		return fail(PXH_ERR_INVALID, "pxh_scene_create: max_pairs_per_body must be at least 1");
 	if (desc->restitution != 0.0f)
 		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: restitution other than 0 is not implemented");
+	if (desc->enable_sleeping && !(desc->sleep_threshold > 0.0f))
+		return fail(PXH_ERR_INVALID, "pxh_scene_create: sleeping needs a positive sleep threshold");
+	if (desc->enable_sleeping && desc->report_touch_events)
+		return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_create: touch events do not combine with sleeping");
 	int dev = 0;
 	PXH_CHECK(hipGetDevice(&dev));
 	pxh_scene_t* s = new pxh_scene_t();
@@ -686,6 +721,34 @@ int32_t pxh_scene_add_bodies(pxh_scene s, int32_t n, const int32_t* type, const 
 		}
 	}
 	s->lastDt = 0.0f;   // no acceleration to read until the next step
+	if (s->desc.enable_sleeping)
+	{
+		// new bodies are awake with the wake counter reset value (20 * 0.02 s,
+		// PxSceneDesc::wakeCounterResetValue), each its own island
+		PXH_CHECK(s->slFlags.grow_keep(nn, base, st));
+		PXH_CHECK(s->slWakeIsl.grow_keep(nn, base, st));
+		PXH_CHECK(s->slWc.grow_keep(nn, base, st));
+		PXH_CHECK(s->slAccL.grow_keep(nn, base, st));
+		PXH_CHECK(s->slAccA.grow_keep(nn, base, st));
+		PXH_CHECK(s->slIsl.grow_keep(nn, base, st));
+		if (n > 0)
+		{
+			std::vector<uint32_t> zu(size_t(n), 0u);
+			std::vector<float> wc(size_t(n), 20.0f * 0.02f);
+			std::vector<float4> z4(size_t(n), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+			std::vector<int> isl;
+			for (int i = 0; i < n; i++)
+				isl.push_back(base + i);
+			PXH_CHECK(hipStreamSynchronize(st));
+			PXH_CHECK(hipMemcpy(s->slFlags.p + base, zu.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+			PXH_CHECK(hipMemcpy(s->slWakeIsl.p + base, zu.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+			PXH_CHECK(hipMemcpy(s->slWc.p + base, wc.data(), n * sizeof(float), hipMemcpyHostToDevice));
+			PXH_CHECK(hipMemcpy(s->slAccL.p + base, z4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+			PXH_CHECK(hipMemcpy(s->slAccA.p + base, z4.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+			PXH_CHECK(hipMemcpy(s->slIsl.p + base, isl.data(), n * sizeof(int), hipMemcpyHostToDevice));
+		}
+		s->slJCountDirty = true;
+	}
 	if (n > 0)
 	{
 		PXH_CHECK(hipMemcpy(s->pos.p + base, hp.data(), n * sizeof(float4), hipMemcpyHostToDevice));
@@ -846,6 +909,7 @@ int uploadJoints(pxh_scene_t* s)
 	std::sort(keys.begin(), keys.end());
 	keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
 	s->nJKeys = int(keys.size());
+	s->slJCountDirty = true;
 	PXH_CHECK(hipStreamSynchronize(s->stream));
 	PXH_CHECK(s->jInfo.ensure(info.size()));
 	PXH_CHECK(s->jFrame.ensure(frame.size()));
@@ -1035,6 +1099,32 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 		s->lastDt = 0.0f;
 		if (int r = gatherBuf(s->local, s->gatherTmp, idx.p, nn, st))
 			return r;
+		if (s->desc.enable_sleeping)
+		{
+			// sleep state of the kept bodies; island labels (body indices of
+			// the same world) renumbered like the bodies
+			if (int r = gatherBuf(s->slFlags, s->gatherTmp, idx.p, nn, st))
+				return r;
+			if (int r = gatherBuf(s->slWakeIsl, s->gatherTmp, idx.p, nn, st))
+				return r;
+			if (int r = gatherBuf(s->slWc, s->gatherTmp, idx.p, nn, st))
+				return r;
+			if (int r = gatherBuf(s->slAccL, s->gatherTmp, idx.p, nn, st))
+				return r;
+			if (int r = gatherBuf(s->slAccA, s->gatherTmp, idx.p, nn, st))
+				return r;
+			if (int r = gatherBuf(s->slIsl, s->gatherTmp, idx.p, nn, st))
+				return r;
+			std::vector<int> isl(static_cast<size_t>(nn), 0), newBody(static_cast<size_t>(keep.back() + 1), -1);
+			for (int i = 0; i < nn; i++)
+				newBody[keep[i]] = i;
+			PXH_CHECK(hipMemcpyAsync(isl.data(), s->slIsl.p, size_t(nn) * sizeof(int), hipMemcpyDeviceToHost, st));
+			PXH_CHECK(hipStreamSynchronize(st));
+			for (int i = 0; i < nn; i++)
+				isl[i] = isl[i] >= 0 && isl[i] < int(newBody.size()) && newBody[isl[i]] >= 0 ? newBody[isl[i]] : i;
+			PXH_CHECK(hipMemcpy(s->slIsl.p, isl.data(), size_t(nn) * sizeof(int), hipMemcpyHostToDevice));
+			s->slJCountDirty = true;
+		}
 		PXH_CHECK(s->scene.ensure(size_t(nn)));
 		PXH_CHECK(hipMemcpyAsync(s->scene.p, hScene.data(), hScene.size() * sizeof(int), hipMemcpyHostToDevice, st));
 	}
@@ -1090,6 +1180,45 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 		if (int r = uploadJoints(s))
 			return r;
 	}
+	return PXH_OK;
+}
+
+int32_t pxh_scene_get_sleep_state(pxh_scene s, float* wake_counter, int32_t* asleep, int32_t n)
+{
+	if (!s || n < 0 || n > s->n)
+		return fail(PXH_ERR_INVALID, "pxh_scene_get_sleep_state: bad arguments");
+	if (!s->desc.enable_sleeping)
+		return fail(PXH_ERR_STATE, "pxh_scene_get_sleep_state: sleeping is off for this scene");
+	if (n == 0)
+		return PXH_OK;
+	hipStream_t st = s->stream;
+	PXH_CHECK(s->slWcOut.ensure(size_t(n)));
+	PXH_CHECK(s->slAsleepOut.ensure(size_t(n)));
+	pxh::launch_sleep_read(n, s->sleepState(), s->slWcOut.p, s->slAsleepOut.p, st);
+	if (wake_counter)
+		PXH_CHECK(hipMemcpyAsync(wake_counter, s->slWcOut.p, size_t(n) * sizeof(float), hipMemcpyDeviceToHost, st));
+	if (asleep)
+		PXH_CHECK(hipMemcpyAsync(asleep, s->slAsleepOut.p, size_t(n) * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+	PXH_CHECK(hipStreamSynchronize(st));
+	return PXH_OK;
+}
+
+int32_t pxh_scene_wake_bodies(pxh_scene s, const int32_t* bodies, int32_t k)
+{
+	if (!s || k < 0 || (k > 0 && !bodies))
+		return fail(PXH_ERR_INVALID, "pxh_scene_wake_bodies: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_wake_bodies: not allowed while simulating");
+	for (int i = 0; i < k; i++)
+		if (bodies[i] < 0 || bodies[i] >= s->n)
+			return fail(PXH_ERR_INVALID, "pxh_scene_wake_bodies: body index out of range");
+	if (!s->desc.enable_sleeping || k == 0)
+		return PXH_OK;   // bodies that never sleep are awake
+	hipStream_t st = s->stream;
+	PXH_CHECK(s->slList.ensure(size_t(k)));
+	PXH_CHECK(hipMemcpyAsync(s->slList.p, bodies, size_t(k) * sizeof(int32_t), hipMemcpyHostToDevice, st));
+	pxh::launch_sleep_wake_bodies(k, s->slList.p, s->sleepState(), st);
+	PXH_CHECK(hipStreamSynchronize(st));
 	return PXH_OK;
 }
 
@@ -1193,6 +1322,28 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const int Q = s->Q;
 	s->statPlanePairs = Q;
 
+	// sleeping: joints per body when bodies or joints changed; per-step buffers
+	const bool sleeping = d.enable_sleeping != 0;
+	if (sleeping)
+	{
+		if (s->slJCountDirty)
+		{
+			std::vector<int> jc(size_t(n), 0);
+			for (size_t j = 0; j < s->hJType.size(); j++)
+			{
+				if (s->hJA[j] >= 0) jc[s->hJA[j]]++;
+				if (s->hJB[j] >= 0) jc[s->hJB[j]]++;
+			}
+			PXH_CHECK(s->slJCount.ensure(size_t(n)));
+			PXH_CHECK(hipStreamSynchronize(st));
+			PXH_CHECK(hipMemcpy(s->slJCount.p, jc.data(), size_t(n) * sizeof(int), hipMemcpyHostToDevice));
+			s->slJCountDirty = false;
+		}
+		PXH_CHECK(s->slCounted.ensure(size_t(n)));
+		PXH_CHECK(s->slNotReady.ensure(size_t(n)));
+		PXH_CHECK(s->slParent.ensure(size_t(n)));
+	}
+
 	// 2. narrowphase into per-slot contact sets (slots: body-plane pairs, then
 	//    body pairs up to the count on the device) and 3. the constraint list
 	//    (touching slots in slot order). One host round trip reads the pair
@@ -1232,6 +1383,27 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		}
 		s->profEnd(pk);
 		pxh::launch_joint_filter(cap, Q, dP, s->pairs.p, s->jKeys.p, s->nJKeys, s->ctN.p, st);
+		if (sleeping)
+		{
+			// the sleep stages act once on the whole pair set: make sure it fit
+			const uint32_t* srcP[1] = {dP};
+			r = readCounts(st, s->hostCounts, srcP, 1);
+			if (r)
+				return r;
+			if (int(s->hostCounts[0]) > cap)
+			{
+				if (attempt > 0)
+					return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: pair buffer overflow after growth");
+				s->pairCap = size_t(s->hostCounts[0]) + size_t(s->hostCounts[0]) / 4;
+				continue;
+			}
+			const pxh::SleepState ss = s->sleepState();
+			pxh::launch_sleep_count(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->qPlane.p, s->planes.p, s->bmin.p,
+			                        s->bmax.p, co, s->jKeys.p, s->nJKeys, ss, st);
+			pxh::launch_sleep_wake(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->ctN.p, ss, st);
+			pxh::launch_sleep_islands(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->ctN.p, int(s->hJType.size()),
+			                          s->jInfo.p, s->lin.p, s->ang.p, ss, st);
+		}
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
 		pxh::launch_touch_flags(S, Q, dP, s->ctN.p, s->flags.p, st);
@@ -1486,7 +1658,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	                         d.angular_damping * dt, d.max_linear_velocity_sq, d.max_angular_velocity_sq, pgs, sb, dt,
 	                         s->extPending ? s->extLin.p : nullptr, s->extPending ? s->extAng.p : nullptr,
 	                         d.enable_body_accelerations ? s->prevLin.p : nullptr,
-	                         d.enable_body_accelerations ? s->prevAng.p : nullptr, st);
+	                         d.enable_body_accelerations ? s->prevAng.p : nullptr, sleeping ? s->slFlags.p : nullptr, st);
 	s->extPending = false;
 	s->lastDt = dt;
 
@@ -1505,7 +1677,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	const int nJ = int(s->hJType.size());
 	if (nJ > 0)
 		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->pos.p, s->rot.p, sb, invStepDt, 1.0f / dt,
-		                       0.5f * biasCoefficient, d.tolerance_length, s->jHead.p, s->jRows.p, st);
+		                       0.5f * biasCoefficient, d.tolerance_length, s->jHead.p, s->jRows.p,
+		                       sleeping ? s->slFlags.p : nullptr, st);
 	// four step counters, then four per-block partial sums of the preparation
 	PXH_CHECK(s->counters.ensure(4 + 4 * ((Cc + 255) / 256)));
 	// friction patches and colours of this step's rows go to the [cur] buffers
@@ -1587,7 +1760,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		const float elapsed = stepDt * float(std::min(it, P_));
 		for (int c = 0; c + 1 < int(s->jColorStart.size()); c++)
 			pxh::launch_joint_solve(s->jColorStart[c], s->jColorStart[c + 1], s->jInfo.p, s->jHead.p, s->jRows.p, sb,
-			                        elapsed, it == P_ - 1, st);
+			                        elapsed, it == P_ - 1, sleeping ? s->slFlags.p : nullptr, st);
 		for (int c = 0; c < nColors; c++)
 		{
 			if (colorStart[c + 1] == colorStart[c])
@@ -1614,10 +1787,11 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	s->patchCur = cur ^ 1;
 
 	// 8. write back poses and velocities
+	//    (with the sleep check of every awake body)
 	if (pgs)
-		pxh::launch_pgs_finalize(n, dt, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
+		pxh::launch_pgs_finalize(n, dt, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, sb, s->sleepState(), st);
 	else
-		pxh::launch_finalize(n, s->pos.p, s->rot.p, s->lin.p, s->ang.p, sb, st);
+		pxh::launch_finalize(n, dt, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, sb, s->sleepState(), st);
 	PXH_CHECK(hipGetLastError());
 	s->simulating = true;
 	return PXH_OK;
@@ -1824,7 +1998,7 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene s, const void* data, const ui
 		PXH_CHECK(hipMemsetAsync(s->extAng.p, 0, sizeof(float4) * s->extAng.cap, s->stream));
 	}
 	pxh::launch_set_data(n, index, kind, s->pos.p, s->rot.p, s->lin.p, s->ang.p, s->invI.p, s->extLin.p, s->extAng.p,
-	                     static_cast<const float*>(data), s->stream);
+	                     static_cast<const float*>(data), s->sleepState(), s->stream);
 	if (kind >= PXH_WRITE_FORCE)
 		s->extPending = true;
 	PXH_CHECK(hipGetLastError());

@@ -45,7 +45,8 @@ class PxoDesc(ctypes.Structure):
                 ("angular_damping", ctypes.c_float), ("max_lin_vel_sq", ctypes.c_float),
                 ("max_ang_vel_sq", ctypes.c_float), ("bounce_threshold", ctypes.c_float),
                 ("friction_offset_threshold", ctypes.c_float), ("correlation_distance", ctypes.c_float),
-                ("tolerance_length", ctypes.c_float), ("order_salt", ctypes.c_uint)]
+                ("tolerance_length", ctypes.c_float), ("order_salt", ctypes.c_uint),
+                ("enable_sleeping", ctypes.c_int), ("sleep_threshold", ctypes.c_float)]
 
 
 _pxo = None
@@ -252,12 +253,14 @@ class OracleWorld:
     """The plain-C oracle's world (TGS or PGS)."""
 
     def __init__(self, scene, pos_iters=4, vel_iters=1, gravity=(0.0, -9.81, 0.0), friction=0.5, solver="tgs",
-                 order_salt=0):
+                 order_salt=0, sleep_threshold=0.0):
         self.lib = pxo()
         self.desc = default_desc(pos_iters, vel_iters, gravity)
         self.desc.static_friction = self.desc.dynamic_friction = friction
         self.desc.solver = 1 if solver == "tgs" else 0
         self.desc.order_salt = order_salt
+        self.desc.enable_sleeping = int(sleep_threshold > 0.0)
+        self.desc.sleep_threshold = sleep_threshold
         self.h = VP(self.lib.pxo_world_create(ctypes.byref(self.desc)))
         self.n = scene.n
         dims = f32(scene.dims).copy()
@@ -298,9 +301,21 @@ class OracleWorld:
         self.lib.pxo_world_get_bodies(self.h, fp(pose), fp(lv), fp(av))
         return pose, lv, av
 
+    def sleep_state(self):
+        """(wake counter, asleep) per body"""
+        wc = np.zeros(self.n, np.float32)
+        asleep = np.zeros(self.n, np.int32)
+        self.lib.pxo_world_get_sleep(self.h, fp(wc), ip(asleep))
+        return wc, asleep.astype(bool)
+
+    def wake(self, bodies):
+        b = i32(bodies)
+        self.lib.pxo_world_wake(self.h, int(b.size), ip(b))
+
     def set_bodies(self, pose, lv, av):
-        pose, lv, av = f32(pose), f32(lv), f32(av)
-        self.lib.pxo_world_set_bodies(self.h, fp(pose), fp(lv), fp(av))
+        """pose / linear / angular velocity arrays (any may be None: kept)"""
+        arrs = [f32(a) if a is not None else None for a in (pose, lv, av)]
+        self.lib.pxo_world_set_bodies(self.h, *[fp(a) if a is not None else None for a in arrs])
 
     def set_forces(self, force=None, torque=None):
         """addForce / addTorque (eFORCE) for the next step, (n,3) arrays."""
@@ -396,7 +411,7 @@ class RefScene:
     """The reference PhysX CPU PxScene (TGS or PGS) on the same arrays."""
 
     def __init__(self, scene, solver="tgs", pos_iters=4, vel_iters=1, threads=8, gravity=(0.0, -9.81, 0.0),
-                 pabp=False, friction=0.5, report_contacts=False):
+                 pabp=False, friction=0.5, report_contacts=False, sleep_threshold=0.0):
         self.lib = ref()
         g = f32(gravity)
         self.h = VP(self.lib.pxref_scene_create2(fp(g), 1 if solver == "tgs" else 0, int(pabp), int(threads),
@@ -429,6 +444,16 @@ class RefScene:
             self._keep += arrs
             assert self.lib.pxref_scene_add_joints(self.h, int(arrs[0].shape[0]), ip(arrs[0]), ip(arrs[1]),
                                                    ip(arrs[2]), fp(arrs[3]), fp(arrs[4])) == arrs[0].shape[0]
+        if sleep_threshold:
+            assert self.lib.pxref_scene_set_sleep_threshold(self.h, ctypes.c_float(sleep_threshold)) == 0
+
+    def sleep_state(self):
+        """(wake counter, asleep) per body: PxRigidDynamic::getWakeCounter,
+        isSleeping"""
+        wc = np.zeros(self.n, np.float32)
+        asleep = np.zeros(self.n, np.int32)
+        self.lib.pxref_scene_sleep_state(self.h, fp(wc), ip(asleep))
+        return wc, asleep.astype(bool)
 
     def add_forces(self, force=None, torque=None):
         """PxRigidDynamic::addForce / addTorque (eFORCE) on every body, (n,3) arrays."""
