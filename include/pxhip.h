@@ -147,6 +147,24 @@ int32_t pxh_scene_get_rigid_dynamic_data(pxh_scene scene, void* data, const uint
 int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, const uint32_t* index, int32_t kind,
                                          int32_t n, void* stream);
 
+/* Static triangle mesh (PxPhysics::createTriangleMesh + a static actor with
+ * PxTriangleMeshGeometry, identity scale; physx/include/geometry/
+ * PxTriangleMeshGeometry.h) in world `world`, at most one per world. Host
+ * arrays: verts[nv*3] (mesh frame), tris[nt*3] vertex indices wound
+ * counter-clockwise about the outward normal (the mesh is one-sided:
+ * bodies whose centre is behind a triangle do not touch it), pose[7] (p xyz,
+ * q xyzw). Degenerate triangles are dropped. Bodies meet the mesh through a
+ * bounding-volume tree on the device (the reference's midphase, BV4 in
+ * geomutils/src/mesh/GuBV4*.cpp) and a contact generation restated after
+ * the reference's PCM mesh contacts (geomutils/src/pcm/
+ * GuPCMContactSphereMesh.cpp, GuPCMTriangleContactGen.cpp): up to 4 contact
+ * patches per body, one constraint each; in contact data and constraint
+ * tables the static side of patch k reads -1 - (plane count + k). A step
+ * returns PXH_ERR_CAPACITY when more than 128 triangles meet one body's
+ * bounds. Returns the mesh index. */
+int32_t pxh_scene_add_triangle_mesh(pxh_scene scene, int32_t nv, const float* verts, int32_t nt, const int32_t* tris,
+                                    const float* pose, int32_t world);
+
 /* Sleeping (enable_sleeping): bodies and islands fall asleep and wake as in
  * the reference's CPU pipeline. Each body keeps a wake counter (0.4 s when
  * added or woken) that counts down while the body's mean motion over the

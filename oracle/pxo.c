@@ -8,6 +8,7 @@
  */
 #include "pxo.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -2406,6 +2407,12 @@ typedef struct {
 	jrow r[6];
 } pxo_joint;
 
+/* a static triangle mesh instance: its world and triangle range; a triangle
+ * in world space with its unit normal (counter-clockwise winding), bounds and
+ * vertex ids */
+typedef struct tmesh_s { int scene, first, count; v3 blo, bhi; } tmesh_t;
+typedef struct tri_s { v3 v[3]; v3 n; v3 lo, hi; int vi[3]; } tri_t;
+
 typedef struct {
 	pxo_desc d;
 	int n, cap;
@@ -2446,6 +2453,11 @@ typedef struct {
 	v3 *accL, *accA;
 	unsigned char *asleep, *ready, *wakeIsl;
 	int *isl, *jcount;
+	/* static triangle meshes, at most one per world: world-space triangles */
+	struct tmesh_s* tm;
+	int ntm;
+	struct tri_s* tri;
+	int ntri, ntmverts;
 } pxo_world;
 
 void pxo_default_desc(pxo_desc* d)
@@ -2489,6 +2501,7 @@ void pxo_world_release(void* wp)
 	free(w->agg); free(w->aggSelf);
 	free(w->wc); free(w->accL); free(w->accA); free(w->asleep); free(w->ready); free(w->wakeIsl); free(w->isl);
 	free(w->jcount);
+	free(w->tm); free(w->tri);
 	free(w);
 }
 
@@ -2778,7 +2791,9 @@ static void push_con(pxo_world* w, int a, int b, const cset* c)
 	k->b = b;
 	k->c = *c;
 	const uint32_t la = (uint32_t)w->local[a];
-	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b] : (0x80000000u | (uint32_t)w->plocal[-1 - b]);
+	/* statics: a plane's world-local index, or 0x400000 + patch for a mesh patch */
+	const uint32_t lb = b >= 0 ? (uint32_t)w->local[b]
+	                           : (0x80000000u | (-1 - b < w->m ? (uint32_t)w->plocal[-1 - b] : 0x400000u + (uint32_t)(-1 - b - w->m)));
 	k->key = con_key(la, lb) ^ ((uint64_t)w->d.order_salt * 0x9E3779B97F4A7C15ull);
 	/* world-relative pair identity: world 16 | local a 24 | local b or 0x800000 | plane 24 */
 	k->pkey = ((uint64_t)((uint32_t)w->scene[a] & 0xFFFFu) << 48) | ((uint64_t)(la & 0xFFFFFFu) << 24) |
@@ -3767,6 +3782,597 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 	return first;
 }
 
+/* ---- static triangle meshes ----
+ * Contacts of a body against a static triangle mesh, restated after the
+ * reference's PCM mesh contact generation (geomutils/src/pcm/
+ * GuPCMContactSphereMesh.cpp:160-501, GuPCMTriangleContactGen.cpp,
+ * GuPCMContactConvexCommon.h:131-230, GuPersistentContactManifold.cpp:
+ * 1315-1420, 1678-2160). Triangles whose bounds meet the body's bounds
+ * inflated by the contact distance, in triangle order; a triangle whose plane
+ * has the body's centre behind it is culled (one-sided meshes). Per triangle:
+ * a sphere takes the closest point (face contacts at once; edge and vertex
+ * contacts deferred, by distance, and dropped when a face or a closer contact
+ * already holds that edge or vertex); a capsule the part of its segment over
+ * the triangle (else the closest edge point, deferred like a sphere's); a box
+ * or hull its face most opposed to the triangle normal clipped to the
+ * triangle's prism, and its deepest vertex when over the triangle. Contacts
+ * group into patches by normal (within 0.996, about 5 degrees, the reference's
+ * acceptance); a patch holds TM_PC contacts before it is reduced to 4 (the
+ * deepest, the farthest from it, then the extremes across their line: the
+ * first four picks of SinglePersistentContactManifold::reduceContacts); the
+ * deepest TM_MAXP patches become one constraint each (a sphere keeps one
+ * contact per patch, GU_SPHERE_MANIFOLD_CACHE_SIZE). */
+#define TM_MAX_TRIS 128
+#define TM_MAXP 4
+#define TM_PC 8
+#define TM_NPATCH 6
+
+typedef struct { v3 p; float s; } tmc_t;
+typedef struct { v3 n; int count; float maxPen; tmc_t c[TM_PC + 1]; } tpatch_t;
+typedef struct { tpatch_t p[TM_NPATCH]; int np; } tpatches_t;
+
+/* reduction of a patch's contacts to 4 */
+static int tm_reduce4(tmc_t* c, int n, v3 nrm)
+{
+	if (n <= 4)
+		return n;
+	int used[TM_PC + 1];
+	for (int i = 0; i < n; i++)
+		used[i] = 0;
+	int i0 = 0;
+	for (int i = 1; i < n; i++)
+		if (c[i0].s > c[i].s)
+			i0 = i;
+	used[i0] = 1;
+	int i1 = -1;
+	float best = -1.0f;
+	for (int i = 0; i < n; i++)
+	{
+		if (used[i]) continue;
+		const v3 d = sub(c[i].p, c[i0].p);
+		const float dd = dot(d, d);
+		if (dd > best) { best = dd; i1 = i; }
+	}
+	used[i1] = 1;
+	const v3 ax = cross(sub(c[i1].p, c[i0].p), nrm);
+	int i2 = -1, i3 = -1;
+	float mx = -FLT_MAX, mn = FLT_MAX;
+	for (int i = 0; i < n; i++)
+	{
+		if (used[i]) continue;
+		const float d = dot(sub(c[i].p, c[i0].p), ax);
+		if (d > mx) { mx = d; i2 = i; }
+		if (mn > d) { mn = d; i3 = i; }
+	}
+	used[i2] = 1;
+	if (i3 == i2 || mx * mn > 0.0f)
+	{
+		/* both on one side of the line (or one candidate): the next largest */
+		mx = -FLT_MAX;
+		i3 = -1;
+		for (int i = 0; i < n; i++)
+		{
+			if (used[i]) continue;
+			const float d = dot(sub(c[i].p, c[i0].p), ax);
+			if (d > mx) { mx = d; i3 = i; }
+		}
+	}
+	const tmc_t a = c[i0], b = c[i1], e = c[i2], f = c[i3];
+	c[0] = a; c[1] = b; c[2] = e; c[3] = f;
+	return 4;
+}
+
+/* adds a contact (point, separation) with normal n to its patch */
+static void tm_add(tpatches_t* P, v3 n, v3 p, float s)
+{
+	for (int k = 0; k < P->np; k++)
+	{
+		tpatch_t* q = &P->p[k];
+		if (!(dot(q->n, n) > 0.996f))
+			continue;
+		for (int j = 0; j < q->count; j++)
+		{
+			const v3 d = sub(q->c[j].p, p);
+			if (1e-10f > dot(d, d))
+			{
+				if (q->c[j].s > s)
+				{
+					q->c[j].p = p;
+					q->c[j].s = s;
+				}
+				if (q->maxPen > s)
+					q->maxPen = s;
+				return;
+			}
+		}
+		q->c[q->count].p = p;
+		q->c[q->count].s = s;
+		q->count++;
+		if (q->maxPen > s)
+			q->maxPen = s;
+		if (q->count > TM_PC)
+			q->count = tm_reduce4(q->c, q->count, q->n);
+		return;
+	}
+	int k = P->np;
+	if (k == TM_NPATCH)
+	{
+		/* no room: a deeper contact replaces the shallowest patch */
+		k = 0;
+		for (int j = 1; j < TM_NPATCH; j++)
+			if (P->p[j].maxPen > P->p[k].maxPen)
+				k = j;
+		if (!(P->p[k].maxPen > s))
+			return;
+	}
+	else
+		P->np++;
+	P->p[k].n = n;
+	P->p[k].count = 1;
+	P->p[k].maxPen = s;
+	P->p[k].c[0].p = p;
+	P->p[k].c[0].s = s;
+}
+
+/* pcmDistancePointTriangleSquared (GuPCMContactSphereMesh.cpp:160-288):
+ * closest point of triangle abc to p and its feature (0 face, 1-3 vertex a,
+ * b, c, 4 edge ab, 5 edge bc, 6 edge ca) */
+enum { TF_FACE = 0, TF_V0 = 1, TF_V1 = 2, TF_V2 = 3, TF_E01 = 4, TF_E12 = 5, TF_E20 = 6 };
+static float tm_point_tri(v3 p, v3 a, v3 b, v3 c, v3* cp, int* fc)
+{
+	const v3 ab = sub(b, a), ac = sub(c, a), bc = sub(c, b);
+	const v3 ap = sub(p, a), bp = sub(p, b), cq = sub(p, c);
+	const float d1 = dot(ab, ap), d2 = dot(ac, ap), d3 = dot(ab, bp), d4 = dot(ac, bp), d5 = dot(ab, cq),
+	            d6 = dot(ac, cq);
+	const float unom = d4 - d3, udenom = d5 - d6;
+	const v3 n = cross(ab, ac);
+	if (0.0f > d1 && 0.0f > d2) { *fc = TF_V0; *cp = a; return dot(ap, ap); }
+	if (d3 >= 0.0f && d3 >= d4) { *fc = TF_V1; *cp = b; return dot(bp, bp); }
+	if (d6 >= 0.0f && d6 >= d5) { *fc = TF_V2; *cp = c; return dot(cq, cq); }
+	const float vc = dot(n, cross(ap, bp));
+	if (0.0f > vc && d1 >= 0.0f && 0.0f > d3)
+	{
+		*fc = TF_E01;
+		const v3 q = add(scl(ab, d1 / (d1 - d3)), a);
+		*cp = q;
+		const v3 vv = sub(p, q);
+		return dot(vv, vv);
+	}
+	const float va = dot(n, cross(bp, cq));
+	if (0.0f > va && d4 >= d3 && d5 >= d6)
+	{
+		*fc = TF_E12;
+		const v3 q = add(scl(bc, unom / (unom + udenom)), b);
+		*cp = q;
+		const v3 vv = sub(p, q);
+		return dot(vv, vv);
+	}
+	const float vb = dot(n, cross(cq, ap));
+	if (0.0f > vb && d2 >= 0.0f && 0.0f > d6)
+	{
+		*fc = TF_E20;
+		const v3 q = add(scl(ac, d2 / (d2 - d6)), a);
+		*cp = q;
+		const v3 vv = sub(p, q);
+		return dot(vv, vv);
+	}
+	*fc = TF_FACE;
+	const float t = dot(n, sub(a, p)) / dot(n, n);
+	const v3 vv = scl(n, t);
+	*cp = add(p, vv);
+	return dot(vv, vv);
+}
+
+/* edge and vertex caches of the deferred contacts (CachedEdge / CachedVertex,
+ * geomutils/src/mesh/GuTriangleCache.h:38-90): add returns 1 when new */
+typedef struct { uint64_t e[3 * TM_MAX_TRIS]; int ne; int v[3 * TM_MAX_TRIS]; int nv; } tm_cache_t;
+static int tm_cache_edge(tm_cache_t* c, int a, int b)
+{
+	const uint64_t k = ((uint64_t)(uint32_t)(a < b ? a : b) << 32) | (uint32_t)(a < b ? b : a);
+	for (int i = 0; i < c->ne; i++)
+		if (c->e[i] == k) return 0;
+	c->e[c->ne++] = k;
+	return 1;
+}
+static int tm_cache_vertex(tm_cache_t* c, int a)
+{
+	for (int i = 0; i < c->nv; i++)
+		if (c->v[i] == a) return 0;
+	c->v[c->nv++] = a;
+	return 1;
+}
+
+typedef struct { v3 p, n; float sq; int fc, t; } tm_def_t;
+
+/* deferred edge and vertex contacts by distance, each unless its feature is
+ * already held (PCMSphereVsMeshContactGeneration::generateLastContacts,
+ * GuPCMContactSphereMesh.cpp:400-501); r: the sphere or capsule radius */
+static void tm_flush_deferred(const pxo_world* w, tm_def_t* d, int nd, tm_cache_t* cache, float r, tpatches_t* P)
+{
+	for (int i = 1; i < nd; i++)   /* stable insertion sort by squared distance */
+	{
+		const tm_def_t x = d[i];
+		int j = i - 1;
+		while (j >= 0 && d[j].sq > x.sq)
+		{
+			d[j + 1] = d[j];
+			j--;
+		}
+		d[j + 1] = x;
+	}
+	for (int i = 0; i < nd; i++)
+	{
+		const tri_t* t = &w->tri[d[i].t];
+		const int e01 = tm_cache_edge(cache, t->vi[0], t->vi[1]);
+		const int e12 = tm_cache_edge(cache, t->vi[1], t->vi[2]);
+		const int e20 = tm_cache_edge(cache, t->vi[2], t->vi[0]);
+		const int v0 = tm_cache_vertex(cache, t->vi[0]);
+		const int v1 = tm_cache_vertex(cache, t->vi[1]);
+		const int v2 = tm_cache_vertex(cache, t->vi[2]);
+		const int fc = d[i].fc;
+		const int need = fc == TF_V0 ? v0 : fc == TF_V1 ? v1 : fc == TF_V2 ? v2 : fc == TF_E01 ? e01 : fc == TF_E12 ? e12 : e20;
+		if (need)
+		{
+			const float dist = sqrtf(d[i].sq);
+			tm_add(P, d[i].n, sub(d[i].p, scl(d[i].n, r)), dist - r);
+		}
+	}
+}
+
+static void tm_cache_tri(tm_cache_t* cache, const tri_t* t)
+{
+	tm_cache_edge(cache, t->vi[0], t->vi[1]);
+	tm_cache_edge(cache, t->vi[1], t->vi[2]);
+	tm_cache_edge(cache, t->vi[2], t->vi[0]);
+	tm_cache_vertex(cache, t->vi[0]);
+	tm_cache_vertex(cache, t->vi[1]);
+	tm_cache_vertex(cache, t->vi[2]);
+}
+
+/* one-sided: the body's centre must not be behind the triangle's plane */
+static int tm_culled(const tri_t* t, v3 c) { return 0.0f > dot(c, t->n) - dot(t->v[0], t->n); }
+
+static void tm_sphere(const pxo_world* w, const int* idx, int nc, v3 c, float r, float cd, tpatches_t* P)
+{
+	const float infl = r + cd;
+	const float inflSq = infl * infl;
+	static tm_cache_t cache;
+	static tm_def_t def[TM_MAX_TRIS];
+	cache.ne = cache.nv = 0;
+	int nd = 0;
+	for (int k = 0; k < nc; k++)
+	{
+		const tri_t* t = &w->tri[idx[k]];
+		if (tm_culled(t, c))
+			continue;
+		v3 cp;
+		int fc;
+		const float sq = tm_point_tri(c, t->v[0], t->v[1], t->v[2], &cp, &fc);
+		if (!(inflSq > sq))
+			continue;
+		if (fc == TF_FACE)
+		{
+			tm_cache_tri(&cache, t);
+			const float dist = sqrtf(sq);
+			tm_add(P, t->n, sub(c, scl(t->n, r)), dist - r);
+		}
+		else
+		{
+			def[nd].p = c;
+			def[nd].n = normalize(sub(c, cp));
+			def[nd].sq = sq;
+			def[nd].fc = fc;
+			def[nd].t = idx[k];
+			nd++;
+		}
+	}
+	/* deferred points are reported on the sphere: centre - n r */
+	tm_flush_deferred(w, def, nd, &cache, r, P);
+}
+
+/* inward normals of the triangle's edge planes (the prism over it) */
+static void tm_prism(const tri_t* t, v3* m)
+{
+	for (int k = 0; k < 3; k++)
+		m[k] = cross(t->n, sub(t->v[(k + 1) % 3], t->v[k]));
+}
+
+static void tm_capsule(const pxo_world* w, const int* idx, int nc, v3 p0, v3 p1, v3 c, float r, float cd,
+                       tpatches_t* P)
+{
+	const float infl = r + cd;
+	static tm_cache_t cache;
+	static tm_def_t def[TM_MAX_TRIS];
+	cache.ne = cache.nv = 0;
+	int nd = 0;
+	for (int k = 0; k < nc; k++)
+	{
+		const tri_t* t = &w->tri[idx[k]];
+		if (tm_culled(t, c))
+			continue;
+		v3 m[3];
+		tm_prism(t, m);
+		/* the part of the segment over the triangle */
+		float t0 = 0.0f, t1 = 1.0f;
+		int empty = 0;
+		for (int e = 0; e < 3 && !empty; e++)
+		{
+			const float a = dot(m[e], sub(p0, t->v[e])), b = dot(m[e], sub(p1, t->v[e]));
+			if (0.0f > a && 0.0f > b)
+				empty = 1;
+			else if (0.0f > a)
+			{
+				const float u = a / (a - b);
+				if (u > t0) t0 = u;
+			}
+			else if (0.0f > b)
+			{
+				const float u = a / (a - b);
+				if (t1 > u) t1 = u;
+			}
+		}
+		if (!empty && t1 >= t0)
+		{
+			const v3 d = sub(p1, p0);
+			const v3 q[2] = {add(p0, scl(d, t0)), add(p0, scl(d, t1))};
+			const int nq = t1 > t0 ? 2 : 1;
+			int any = 0;
+			for (int j = 0; j < nq; j++)
+			{
+				const float s = dot(sub(q[j], t->v[0]), t->n);
+				if (infl > s)
+				{
+					tm_add(P, t->n, sub(q[j], scl(t->n, r)), s - r);
+					any = 1;
+				}
+			}
+			if (any)
+				tm_cache_tri(&cache, t);
+			continue;
+		}
+		/* beside the triangle: the closest point of its boundary */
+		float best = FLT_MAX;
+		v3 bs = p0, bt = p0;
+		int bf = TF_E01;
+		for (int e = 0; e < 3; e++)
+		{
+			v3 cs, ct;
+			const float sq = seg_closest2(p0, p1, t->v[e], t->v[(e + 1) % 3], &cs, &ct);
+			if (best > sq)
+			{
+				best = sq;
+				bs = cs;
+				bt = ct;
+				bf = e == 0 ? TF_E01 : e == 1 ? TF_E12 : TF_E20;
+			}
+		}
+		if (!(infl * infl > best) || !(best > 0.0f))
+			continue;
+		const v3 nn = normalize(sub(bs, bt));
+		if (!(dot(nn, t->n) > 0.0f))
+			continue;
+		def[nd].p = bs;
+		def[nd].n = nn;
+		def[nd].sq = best;
+		def[nd].fc = bf;
+		def[nd].t = idx[k];
+		nd++;
+	}
+	tm_flush_deferred(w, def, nd, &cache, r, P);
+}
+
+/* box or hull: the face most opposed to the triangle normal, clipped to the
+ * triangle's prism (Sutherland-Hodgman), and the deepest vertex over it */
+static void tm_poly(const pxo_world* w, const int* idx, int nc, const polyd* H, xf pose, float cd, tpatches_t* P)
+{
+	const m33 R = qmat(pose.q);
+	for (int k = 0; k < nc; k++)
+	{
+		const tri_t* t = &w->tri[idx[k]];
+		if (tm_culled(t, pose.p))
+			continue;
+		int f = 0;
+		float best = FLT_MAX;
+		for (int j = 0; j < H->np; j++)
+		{
+			const float d = dot(mmul(R, H->pn[j]), t->n);
+			if (best > d) { best = d; f = j; }
+		}
+		v3 buf0[64 + 3], buf1[64 + 3];
+		int n0 = H->pc[f] > 64 ? 64 : H->pc[f];
+		for (int j = 0; j < n0; j++)
+			buf0[j] = add(mmul(R, H->v[H->vr[H->pf[f] + j]]), pose.p);
+		v3 m[3];
+		tm_prism(t, m);
+		v3* in = buf0;
+		v3* out = buf1;
+		for (int e = 0; e < 3 && n0 > 0; e++)
+		{
+			int n1 = 0;
+			for (int j = 0; j < n0; j++)
+			{
+				const v3 a = in[j], b = in[(j + 1) % n0];
+				const float da = dot(m[e], sub(a, t->v[e])), db = dot(m[e], sub(b, t->v[e]));
+				if (da >= 0.0f)
+					out[n1++] = a;
+				if ((da >= 0.0f) != (db >= 0.0f))
+					out[n1++] = add(a, scl(sub(b, a), da / (da - db)));
+			}
+			v3* tmp = in;
+			in = out;
+			out = tmp;
+			n0 = n1;
+		}
+		for (int j = 0; j < n0; j++)
+		{
+			const float s = dot(sub(in[j], t->v[0]), t->n);
+			if (cd > s)
+				tm_add(P, t->n, sub(in[j], scl(t->n, s)), s);
+		}
+		/* the deepest vertex, when over the triangle */
+		int vbest = 0;
+		float dbest = FLT_MAX;
+		for (int j = 0; j < H->nv; j++)
+		{
+			const float d = dot(mmul(R, H->v[j]), t->n);
+			if (dbest > d) { dbest = d; vbest = j; }
+		}
+		const v3 sv = add(mmul(R, H->v[vbest]), pose.p);
+		if (dot(m[0], sub(sv, t->v[0])) >= 0.0f && dot(m[1], sub(sv, t->v[1])) >= 0.0f &&
+		    dot(m[2], sub(sv, t->v[2])) >= 0.0f)
+		{
+			const float s = dot(sub(sv, t->v[0]), t->n);
+			if (cd > s)
+				tm_add(P, t->n, sub(sv, scl(t->n, s)), s);
+		}
+	}
+}
+
+/* a body's contact patches against its world's mesh: constraints with the
+ * static index -1 - (plane count + patch); returns -1 when more than
+ * TM_MAX_TRIS triangles meet the body's bounds */
+static int tm_body(pxo_world* w, int i, const tmesh_t* M, float cd)
+{
+	v3 mn, mx;
+	shape_bounds(w->type[i], w->dims + 4 * i, w->pose[i], &mn, &mx);
+	mn = sub(mn, V(cd, cd, cd));
+	mx = add(mx, V(cd, cd, cd));
+	int idx[TM_MAX_TRIS];
+	int nc = 0;
+	for (int k = M->first; k < M->first + M->count; k++)
+	{
+		const tri_t* t = &w->tri[k];
+		if (t->lo.x > mx.x || mn.x > t->hi.x || t->lo.y > mx.y || mn.y > t->hi.y || t->lo.z > mx.z || mn.z > t->hi.z)
+			continue;
+		if (nc == TM_MAX_TRIS)
+			return -1;
+		idx[nc++] = k;
+	}
+	if (nc == 0)
+		return 0;
+	tpatches_t P;
+	P.np = 0;
+	const int type = w->type[i];
+	if (type == PXO_SPHERE)
+		tm_sphere(w, idx, nc, w->pose[i].p, w->dims[4 * i], cd, &P);
+	else if (type == PXO_CAPSULE)
+	{
+		v3 s, e;
+		capsule_segment(w->pose[i], w->dims[4 * i + 1], &s, &e);
+		tm_capsule(w, idx, nc, s, e, w->pose[i].p, w->dims[4 * i], cd, &P);
+	}
+	else if (type == PXO_CONVEX)
+		tm_poly(w, idx, nc, hull_of(w->dims[4 * i]), w->pose[i], cd, &P);
+	else
+	{
+		boxpoly b;
+		box_poly(V(w->dims[4 * i], w->dims[4 * i + 1], w->dims[4 * i + 2]), &b);
+		tm_poly(w, idx, nc, &b.d, w->pose[i], cd, &P);
+	}
+	/* deepest patches first (stable) */
+	int order[TM_NPATCH];
+	for (int k = 0; k < P.np; k++)
+	{
+		int j = k - 1;
+		while (j >= 0 && P.p[order[j]].maxPen > P.p[k].maxPen)
+		{
+			order[j + 1] = order[j];
+			j--;
+		}
+		order[j + 1] = k;
+	}
+	const int keep = P.np < TM_MAXP ? P.np : TM_MAXP;
+	for (int k = 0; k < keep; k++)
+	{
+		tpatch_t* q = &P.p[order[k]];
+		int cnt = tm_reduce4(q->c, q->count, q->n);
+		if (type == PXO_SPHERE)
+		{
+			int d = 0;
+			for (int j = 1; j < cnt; j++)
+				if (q->c[d].s > q->c[j].s) d = j;
+			q->c[0] = q->c[d];
+			cnt = 1;
+		}
+		cset c;
+		c.n = q->n;
+		c.count = cnt;
+		for (int j = 0; j < cnt; j++)
+		{
+			c.p[j] = q->c[j].p;
+			c.sep[j] = q->c[j].s;
+		}
+		push_con(w, i, -1 - (w->m + k), &c);
+	}
+	return keep;
+}
+
+int pxo_world_add_trimesh(void* wp, int nv, const float* verts, int nt, const int32_t* tris, const float* pose,
+                          int32_t scene)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int k = 0; k < w->ntm; k++)
+		if (w->tm[k].scene == scene)
+			return -1;   /* one mesh per world */
+	for (int k = 0; k < 3 * nt; k++)
+		if (tris[k] < 0 || tris[k] >= nv)
+			return -2;
+	const xf t = load_pose(pose);
+	v3* P = (v3*)malloc(sizeof(v3) * (size_t)(nv ? nv : 1));
+	for (int k = 0; k < nv; k++)
+		P[k] = add(qrot(t.q, V(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2])), t.p);
+	w->tri = (tri_t*)realloc(w->tri, sizeof(tri_t) * (size_t)(w->ntri + nt + 1));
+	const int first = w->ntri;
+	for (int k = 0; k < nt; k++)
+	{
+		tri_t* r = &w->tri[w->ntri];
+		for (int j = 0; j < 3; j++)
+		{
+			r->v[j] = P[tris[3 * k + j]];
+			r->vi[j] = w->ntmverts + tris[3 * k + j];
+		}
+		const v3 n = cross(sub(r->v[1], r->v[0]), sub(r->v[2], r->v[0]));
+		if (!(dot(n, n) > 0.0f))
+			continue;   /* degenerate: not stored */
+		r->n = normalize(n);
+		r->lo = r->v[0];
+		r->hi = r->v[0];
+		for (int j = 1; j < 3; j++)
+		{
+			r->lo = V(fminf(r->lo.x, r->v[j].x), fminf(r->lo.y, r->v[j].y), fminf(r->lo.z, r->v[j].z));
+			r->hi = V(fmaxf(r->hi.x, r->v[j].x), fmaxf(r->hi.y, r->v[j].y), fmaxf(r->hi.z, r->v[j].z));
+		}
+		w->ntri++;
+	}
+	free(P);
+	w->ntmverts += nv;
+	w->tm = (tmesh_t*)realloc(w->tm, sizeof(tmesh_t) * (size_t)(w->ntm + 1));
+	tmesh_t* M = &w->tm[w->ntm];
+	M->scene = scene;
+	M->first = first;
+	M->count = w->ntri - first;
+	/* broadphase bounds: the local box about the pose (Gu::computeBounds of a
+	 * triangle mesh shape, geomutils/src/GuBounds.cpp:354-395) */
+	{
+		v3 lmin = V(3.4e38f, 3.4e38f, 3.4e38f), lmax = V(-3.4e38f, -3.4e38f, -3.4e38f);
+		for (int k = 0; k < nv; k++)
+		{
+			lmin = V(fminf(lmin.x, verts[3 * k]), fminf(lmin.y, verts[3 * k + 1]), fminf(lmin.z, verts[3 * k + 2]));
+			lmax = V(fmaxf(lmax.x, verts[3 * k]), fmaxf(lmax.y, verts[3 * k + 1]), fmaxf(lmax.z, verts[3 * k + 2]));
+		}
+		const v3 c = V(0.5f * (lmin.x + lmax.x), 0.5f * (lmin.y + lmax.y), 0.5f * (lmin.z + lmax.z));
+		const v3 e = V(0.5f * (lmax.x - lmin.x), 0.5f * (lmax.y - lmin.y), 0.5f * (lmax.z - lmin.z));
+		const v3 wc = add(qrot(t.q, c), t.p);
+		const m33 R = qmat(t.q);
+		const v3 we = V(fabsf(R.c0.x) * e.x + fabsf(R.c1.x) * e.y + fabsf(R.c2.x) * e.z,
+		                fabsf(R.c0.y) * e.x + fabsf(R.c1.y) * e.y + fabsf(R.c2.y) * e.z,
+		                fabsf(R.c0.z) * e.x + fabsf(R.c1.z) * e.y + fabsf(R.c2.z) * e.z);
+		M->blo = sub(wc, we);
+		M->bhi = add(wc, we);
+	}
+	return w->ntm++;
+}
+
 /* ---- sleeping ----
  * Islands, wake-up and deactivation follow the reference CPU pipeline's
  * observable order (simulationcontroller/src/ScPipeline.cpp:1472-1776,
@@ -4000,6 +4606,19 @@ int pxo_world_step(void* wp, float dt)
 			for (int q = 0; q < w->m; q++)
 				if (w->pscene[q] == w->scene[i] && plane_counted(w, i, q, co))
 					counted[i]++;
+			/* a mesh whose broadphase bounds overlap the body's */
+			for (int k = 0; k < w->ntm; k++)
+			{
+				if (w->tm[k].scene != w->scene[i])
+					continue;
+				v3 mn, mx;
+				shape_bounds(w->type[i], w->dims + 4 * i, w->pose[i], &mn, &mx);
+				const v3 a = V(mn.x - co, mn.y - co, mn.z - co), b = V(mx.x + co, mx.y + co, mx.z + co);
+				const v3 lo = w->tm[k].blo, hi = w->tm[k].bhi;
+				if (!(lo.x - co > b.x || a.x > hi.x + co || lo.y - co > b.y || a.y > hi.y + co || lo.z - co > b.z ||
+				      a.z > hi.z + co))
+					counted[i]++;
+			}
 		}
 	}
 
@@ -4039,6 +4658,14 @@ int pxo_world_step(void* wp, float dt)
 			if (k > 0)
 				push_con(w, i, -1 - q, &c);
 		}
+	/* body-mesh contacts (one mesh per world) */
+	for (int k = 0; k < w->ntm; k++)
+		for (int i = 0; i < n; i++)
+			if (w->scene[i] == w->tm[k].scene && tm_body(w, i, &w->tm[k], cd) < 0)
+			{
+				free(counted);
+				return -6;   /* more than TM_MAX_TRIS triangles under one body */
+			}
 	if (d->enable_sleeping)
 		sleep_islands(w);
 	w->stats[2] = w->ncon;

@@ -88,6 +88,14 @@ int   pxo_world_step(void* w, float dt);
 void  pxo_world_get_bodies(void* w, float* pose, float* lin_vel, float* ang_vel);
 void  pxo_world_set_bodies(void* w, const float* pose, const float* lin_vel, const float* ang_vel);
 void  pxo_world_set_forces(void* w, const float* force, const float* torque);
+/* Static triangle mesh (PxTriangleMeshGeometry, identity scale, on a static
+ * actor at pose p xyz q xyzw) in a world: nv vertices (verts[nv*3]), nt
+ * triangles (tris[nt*3], counter-clockwise about the outward normal; the mesh
+ * is one-sided). At most one per world. Returns the mesh index, -1 when the
+ * world has one, -2 for an index out of range. A step returns -6 when more
+ * than 128 triangles meet one body's bounds. */
+int   pxo_world_add_trimesh(void* w, int nv, const float* verts, int nt, const int32_t* tris, const float* pose,
+                            int32_t scene);
 /* Sleep state per body (PxRigidDynamic::getWakeCounter, isSleeping). */
 void  pxo_world_get_sleep(void* w, float* wake_counter, int32_t* asleep);
 /* PxRigidDynamic::wakeUp on k bodies: wake counter 0.4 s, the rest of each

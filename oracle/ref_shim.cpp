@@ -43,9 +43,11 @@ bool ensureSdk()
 
 // convex meshes cooked by pxref_convex_create, named by index
 std::vector<PxConvexMesh*> gConvex;
+std::vector<PxTriangleMesh*> gTriMesh;
 
 // geometry codes shared with the product C-ABI: 0 sphere, 1 plane, 2 capsule,
-// 3 box, 5 convex mesh (g[0] = index of a mesh from pxref_convex_create)
+// 3 box, 5 convex mesh (g[0] = index of a mesh from pxref_convex_create),
+// 8 triangle mesh (g[0] = index of a mesh from pxref_trimesh_create)
 PxGeometryHolder makeGeom(int type, const float* g)
 {
 	switch (type)
@@ -54,6 +56,7 @@ PxGeometryHolder makeGeom(int type, const float* g)
 	case 1: return PxGeometryHolder(PxPlaneGeometry());
 	case 2: return PxGeometryHolder(PxCapsuleGeometry(g[0], g[1]));
 	case 5: return PxGeometryHolder(PxConvexMeshGeometry(gConvex[size_t(g[0])]));
+	case 8: return PxGeometryHolder(PxTriangleMeshGeometry(gTriMesh[size_t(g[0])]));
 	default: return PxGeometryHolder(PxBoxGeometry(g[0], g[1], g[2]));
 	}
 }
@@ -253,6 +256,45 @@ int pxref_convex_data(int id, float* verts, float* planes, int* polys, int* vref
 	counts[0] = int(nv);
 	counts[1] = int(np);
 	counts[2] = nr;
+	return 0;
+}
+
+// Cooks a triangle mesh (PxCreateTriangleMesh, default cooking parameters:
+// BV4 midphase, active-edge data) from nv vertices and nt index triples;
+// returns its mesh index (geometry code 8 with dims[0] = index), or -1.
+int pxref_trimesh_create(const float* verts, int nv, const int* tris, int nt)
+{
+	if (!ensureSdk())
+		return -1;
+	PxTriangleMeshDesc desc;
+	desc.points.count = PxU32(nv);
+	desc.points.stride = sizeof(PxVec3);
+	desc.points.data = verts;
+	desc.triangles.count = PxU32(nt);
+	desc.triangles.stride = 3 * sizeof(PxU32);
+	desc.triangles.data = tris;
+	PxCookingParams params{PxTolerancesScale()};
+	PxTriangleMesh* m = PxCreateTriangleMesh(params, desc, gPhysics->getPhysicsInsertionCallback());
+	if (!m)
+		return -1;
+	gTriMesh.push_back(m);
+	return int(gTriMesh.size()) - 1;
+}
+
+// Adds a static actor with one triangle mesh shape (identity scale) at pose;
+// the shape's contact offset as given, rest offset 0.
+int pxref_scene_add_trimesh(void* h, int mesh, const float* pose, float contactOffset)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	if (mesh < 0 || size_t(mesh) >= gTriMesh.size())
+		return -1;
+	PxRigidStatic* s = gPhysics->createRigidStatic(makePose(pose));
+	PxShape* shape = PxRigidActorExt::createExclusiveShape(*s, PxTriangleMeshGeometry(gTriMesh[size_t(mesh)]),
+	                                                       *rs->material);
+	shape->setContactOffset(contactOffset);
+	shape->setRestOffset(0.0f);
+	s->userData = reinterpret_cast<void*>(intptr_t(-(++rs->nPlanes)));
+	rs->scene->addActor(*s);
 	return 0;
 }
 

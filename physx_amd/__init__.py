@@ -64,7 +64,7 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
             "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate",
-            "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies")
+            "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies", "pxh_scene_add_triangle_mesh")
 
 _lib = None
 
@@ -164,10 +164,21 @@ class Scene:
         self.n += scene.n
         for members, self_coll in (getattr(scene, "aggregates", None) or []):
             self.add_aggregate(np.asarray(members) + base, self_coll)
+        for m in (getattr(scene, "meshes", None) or []):
+            self.add_triangle_mesh(m["verts"], m["tris"], m["pose"], int(m["world"]))
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
                             np.where(j["body1"] >= 0, j["body1"] + base, -1), j["frame0"], j["frame1"])
+
+    def add_triangle_mesh(self, verts, tris, pose=(0, 0, 0, 0, 0, 0, 1), world: int = 0) -> int:
+        """pxh_scene_add_triangle_mesh: a static triangle mesh (verts (nv,3),
+        tris (nt,3) counter-clockwise about the outward normal) at pose in a
+        world; returns the mesh index."""
+        v, t, po = _f32(verts).reshape(-1, 3), _i32(tris).reshape(-1, 3), _f32(pose).reshape(7)
+        return _check(self.lib.pxh_scene_add_triangle_mesh(self.h, int(v.shape[0]), v.ctypes.data_as(_F),
+                                                           int(t.shape[0]), t.ctypes.data_as(_I32),
+                                                           po.ctypes.data_as(_F), int(world)))
 
     def add_aggregate(self, bodies, self_collisions: bool = False) -> int:
         """pxh_scene_add_aggregate (PxAggregate with the bodies): returns its id."""

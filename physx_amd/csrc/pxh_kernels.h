@@ -87,6 +87,24 @@ struct PatchIO
 	int4* conRec;
 };
 
+// Static triangle meshes: triangles in world space (vertices with the vertex
+// id in w, unit normal, bounds) and each mesh's bounding-volume tree
+constexpr int MESH_MAX_TRIS = 128;   // candidate triangles per body
+constexpr int MESH_MAXP = 4;         // contact patches (constraints) per body and mesh
+struct MeshSet
+{
+	const float4* v0;
+	const float4* v1;
+	const float4* v2;
+	const float4* n;
+	const float4* lo;
+	const float4* hi;
+	const float4* nodeLo;     // tree nodes, depth-first
+	const float4* nodeHi;
+	const int4* nodeInfo;     // first leaf entry, leaf count (0: inner node), next node after the subtree
+	const int* leafTri;       // triangles of the leaves
+};
+
 // Sleeping, one entry per body: the sleep filter of PxsRigidBody
 // (lowlevel/software/include/PxsRigidBody.h) and the island manager's node
 // state. flags == nullptr: sleeping is off.
@@ -154,13 +172,14 @@ void launch_copy_contacts(int C, int maxPairs, const int* order, const int* conA
                           const float4* ctN, const float4* ctP, const float* nF, float4* out, uint32_t* nbPairs,
                           hipStream_t st);
 // slots [0, Q) are body-plane pairs, body pairs follow up to the device count *dP
-void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st);
+void launch_touch_flags(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, uint32_t* flags,
+                        hipStream_t st);
 // constraint list; conPKey: world-relative pair identity (world tag 16 | local a
 // 24 | local b or 0x800000 | plane 24)
-void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
-                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
-                      const int* worldTag, int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey,
-                      int* conColor, hipStream_t st);
+void launch_con_build(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
+                      const uint2* pairs, const int* qBody, const int* qPlane, const int* meshBody, int nPlanes,
+                      const int* local, const int* planeLocal, const int* world, const int* worldTag, int* conA,
+                      int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor, hipStream_t st);
 // out[i] = in[idx[i]], elements of 4 or 16 bytes
 void launch_gather(int n, const int* idx, const void* in, void* out, int bytes, hipStream_t st);
 // colouring over list[i] (identity when null) for i below *mDev (or m when
@@ -276,17 +295,26 @@ void launch_joint_solve(int begin, int end, const int4* info, const float4* head
 // planes whose bounds it overlaps, joints)
 void launch_sleep_count(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
                         const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
-                        const uint64_t* jointKeys, int nKeys, SleepState ss, hipStream_t st);
+                        const uint64_t* jointKeys, int nKeys, int nMB, const int* meshBody, const int* world,
+                        const float4* meshLo, const float4* meshHi, SleepState ss, hipStream_t st);
 // touching pairs of an awake and a sleeping body wake the sleeping body's
 // island, as do islands a wake-up named; then the pairs of sleeping bodies
 // lose their contacts
-void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
-                       SleepState ss, hipStream_t st);
+void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, int nMB,
+                       const int* meshBody, float4* ctN, SleepState ss, hipStream_t st);
 // islands of the awake bodies over touching pairs and joints; islands whose
 // bodies were all ready for sleep go to sleep (velocities zeroed) and their
 // pairs lose their contacts
-void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
-                          int J, const int4* jInfo, float4* lin, float4* ang, SleepState ss, hipStream_t st);
+void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, int nMB,
+                          const int* meshBody, float4* ctN, int J, const int4* jInfo, float4* lin, float4* ang,
+                          SleepState ss, hipStream_t st);
+// body-mesh contacts of the nMB bodies of meshBody (worlds with a mesh;
+// worldMesh[w] = first tree node, node count, or -1) into slots
+// slotBase + k * MESH_MAXP + patch; *overflow set when a body meets more than
+// MESH_MAX_TRIS triangles
+void launch_np_mesh(int nMB, const int* meshBody, const int* world, const int4* worldMesh, const MeshSet& M,
+                    const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd,
+                    int slotBase, float4* ctN, float4* ctP, uint32_t* overflow, hipStream_t st);
 // PxRigidDynamic::wakeUp of k bodies (device indices)
 void launch_sleep_wake_bodies(int k, const int* bodies, SleepState ss, hipStream_t st);
 // sleep state readback: wake counter and asleep flag per body

@@ -17,6 +17,7 @@
 #include "pxh_np.h"
 #include "pxh_convex.h"
 #include "pxh_joint.h"
+#include "pxh_mesh.h"
 
 #include <algorithm>
 
@@ -211,14 +212,11 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t* kin, 
 // bounds and broadphase
 // ---------------------------------------------------------------------------
 
-// Gu::computeBounds (geomutils/src/GuBounds.cpp:354-395) plus the broadphase's
-// contact-distance inflation (lowlevelaabb/src/BpBroadPhaseABP.cpp:1187-1192)
-__global__ void k_bounds(int n, const float4* pos, const float4* rot, const float4* geom, HullSet hs, float co,
-                         float4* bmin, float4* bmax)
+// a body's shape bounds, centre - w to centre + w (Gu::computeBounds,
+// geomutils/src/GuBounds.cpp:354-395)
+PXH_D void shapeBounds(int i, const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, f3& mn,
+                       f3& mx)
 {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i >= n)
-		return;
 	const float4 g = geom[i];
 	f3 p = mk(pos[i]);
 	f3 w;
@@ -246,8 +244,21 @@ __global__ void k_bounds(int n, const float4* pos, const float4* rot, const floa
 		w = mk(fabsf(c0.x) + fabsf(c1.x) + fabsf(c2.x), fabsf(c0.y) + fabsf(c1.y) + fabsf(c2.y),
 		       fabsf(c0.z) + fabsf(c1.z) + fabsf(c2.z));
 	}
-	const f3 mn = p - w, mx = p + w;
-	bmin[i] = make_float4(mn.x - co, mn.y - co, mn.z - co, g.w);
+	mn = p - w;
+	mx = p + w;
+}
+
+// shape bounds plus the broadphase's contact-distance inflation
+// (lowlevelaabb/src/BpBroadPhaseABP.cpp:1187-1192)
+__global__ void k_bounds(int n, const float4* pos, const float4* rot, const float4* geom, HullSet hs, float co,
+                         float4* bmin, float4* bmax)
+{
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n)
+		return;
+	f3 mn, mx;
+	shapeBounds(i, pos, rot, geom, hs, mn, mx);
+	bmin[i] = make_float4(mn.x - co, mn.y - co, mn.z - co, geom[i].w);
 	bmax[i] = make_float4(mx.x + co, mx.y + co, mx.z + co, 0.0f);
 }
 
@@ -853,11 +864,18 @@ __global__ void k_copy_contacts(int C, int maxPairs, const int* order, const int
 }
 
 // slots: [0, Q) body-plane pairs, then body pairs up to the device count *dP
-__global__ void k_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags)
+// slots: body-plane [0, Q), body pairs [Q, pairEnd) up to the count on the
+// device, body-mesh patches [pairEnd, S)
+PXH_D bool liveSlot(int s, int Q, int pairEnd, const uint32_t* dP)
+{
+	return s < Q || s >= pairEnd || uint32_t(s - Q) < *dP;
+}
+
+__global__ void k_touch_flags(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, uint32_t* flags)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
 	if (s < S)
-		flags[s] = (s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
+		flags[s] = liveSlot(s, Q, pairEnd, dP) && __float_as_int(ctN[s].w) > 0 ? 1u : 0u;
 }
 
 // Pair identity across steps, world-relative so it survives changes to other
@@ -871,18 +889,27 @@ PXH_D uint64_t worldPairKey(int w, uint32_t la, uint32_t lb)
 
 // constraint list: touching pair slots in slot order, with the scene-local
 // ordering key used by the colouring
-__global__ void k_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
-                            const uint2* pairs, const int* qBody, const int* qPlane, const int* local,
-                            const int* planeLocal, const int* world, const int* worldTag, int* conA, int* conB,
-                            int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor)
+__global__ void k_con_build(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
+                            const uint2* pairs, const int* qBody, const int* qPlane, const int* meshBody, int nPlanes,
+                            const int* local, const int* planeLocal, const int* world, const int* worldTag, int* conA,
+                            int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= S || !((s < Q || uint32_t(s - Q) < *dP) && __float_as_int(ctN[s].w) > 0))
+	if (s >= S || !(liveSlot(s, Q, pairEnd, dP) && __float_as_int(ctN[s].w) > 0))
 		return;
 	const int c = int(scanned[s]);
 	int a, b;
 	uint32_t la, lb;
-	if (s >= Q)
+	if (s >= pairEnd)
+	{
+		// a mesh patch: static index plane count + patch, local id 0x400000 + patch
+		const int k = s - pairEnd, p = k % MESH_MAXP;
+		a = meshBody[k / MESH_MAXP];
+		b = -1 - (nPlanes + p);
+		la = uint32_t(local[a]);
+		lb = 0x80000000u | (0x400000u + uint32_t(p));
+	}
+	else if (s >= Q)
 	{
 		const uint2 ab = pairs[s - Q];
 		a = int(ab.x); b = int(ab.y);
@@ -2173,6 +2200,121 @@ __global__ void __launch_bounds__(SOLVE_WORLD_THREADS) k_solve_worlds(
 // jointed bodies do not collide (PxConstraintFlag::eCOLLISION_ENABLED is off
 // by default, physx/include/PxConstraint.h): their pair slots lose their
 // contacts before the constraint list is built
+// Body-mesh narrowphase, one lane per body of a world with a mesh (k-th of
+// meshBody): candidate triangles through the mesh's tree (depth-first nodes,
+// each with the index after its subtree), taken in triangle order, then the
+// patches of pxh_mesh.h into slots slotBase + k * MESH_MAXP + patch (deepest
+// first; empty slots count 0). More than MESH_MAX_TRIS candidates set
+// *overflow and leave the body without mesh contacts.
+__global__ void __launch_bounds__(64) k_np_mesh(int nMB, const int* meshBody, const int* world, const int4* worldMesh,
+                                                MeshSet M, const float4* pos, const float4* rot, const float4* geom,
+                                                HullSet hs, float cd, int slotBase, float4* ctN, float4* ctP,
+                                                uint32_t* overflow)
+{
+	const int k = blockIdx.x * blockDim.x + threadIdx.x;
+	if (k >= nMB)
+		return;
+	const int i = meshBody[k];
+	const int base = slotBase + k * MESH_MAXP;
+	for (int p = 0; p < MESH_MAXP; p++)
+		ctN[base + p].w = __int_as_float(0);
+	const int4 wm = worldMesh[world[i]];
+	if (wm.x < 0)
+		return;
+	f3 mn, mx;
+	shapeBounds(i, pos, rot, geom, hs, mn, mx);
+	mn = mn - mk(cd, cd, cd);
+	mx = mx + mk(cd, cd, cd);
+	int idx[MESH_MAX_TRIS];
+	int nc = 0;
+	for (int nd = wm.x; nd < wm.x + wm.y;)
+	{
+		const float4 lo = M.nodeLo[nd], hi = M.nodeHi[nd];
+		const int4 info = M.nodeInfo[nd];
+		if (lo.x > mx.x || mn.x > hi.x || lo.y > mx.y || mn.y > hi.y || lo.z > mx.z || mn.z > hi.z)
+		{
+			nd = info.z;
+			continue;
+		}
+		for (int j = 0; j < info.y; j++)
+		{
+			const int t = M.leafTri[info.x + j];
+			const float4 tl = M.lo[t], th = M.hi[t];
+			if (tl.x > mx.x || mn.x > th.x || tl.y > mx.y || mn.y > th.y || tl.z > mx.z || mn.z > th.z)
+				continue;
+			if (nc == MESH_MAX_TRIS)
+			{
+				atomicOr(overflow, 1u);
+				return;
+			}
+			idx[nc++] = t;
+		}
+		nd++;
+	}
+	if (nc == 0)
+		return;
+	for (int a = 1; a < nc; a++)   // triangle order
+	{
+		const int x = idx[a];
+		int b = a - 1;
+		while (b >= 0 && idx[b] > x)
+		{
+			idx[b + 1] = idx[b];
+			b--;
+		}
+		idx[b + 1] = x;
+	}
+	MPatches P;
+	P.np = 0;
+	const float4 g = geom[i];
+	const int type = int(g.w);
+	xf x;
+	x.p = mk(pos[i]);
+	x.q = mkq(rot[i]);
+	if (type == GEOM_SPHERE || type == GEOM_CAPSULE)
+	{
+		MeshCache C;
+		MeshDef def[MESH_MAX_TRIS];
+		if (type == GEOM_SPHERE)
+			meshSphere(M, idx, nc, x.p, g.x, cd, C, def, P);
+		else
+		{
+			const f3 tmp = qrot(x.q, mk(1.0f, 0.0f, 0.0f)) * g.y;
+			meshCapsule(M, idx, nc, x.p + tmp, x.p - tmp, x.p, g.x, cd, C, def, P);
+		}
+	}
+	else
+		meshPoly(M, idx, nc, type == GEOM_CONVEXMESH ? hullPoly(hs, int(g.x)) : boxPoly(mk(g.x, g.y, g.z)), x, cd, P);
+	int order[MESH_NPATCH];
+	for (int a = 0; a < P.np; a++)
+	{
+		int b = a - 1;
+		while (b >= 0 && P.p[order[b]].maxPen > P.p[a].maxPen)
+		{
+			order[b + 1] = order[b];
+			b--;
+		}
+		order[b + 1] = a;
+	}
+	const int keep = P.np < MESH_MAXP ? P.np : MESH_MAXP;
+	for (int a = 0; a < keep; a++)
+	{
+		MPatch& q = P.p[order[a]];
+		int cnt = meshReduce4(q.c, q.count, q.n);
+		if (type == GEOM_SPHERE)
+		{
+			int d = 0;
+			for (int j = 1; j < cnt; j++)
+				if (q.c[d].s > q.c[j].s) d = j;
+			q.c[0] = q.c[d];
+			cnt = 1;
+		}
+		ctN[base + a] = f4(q.n, __int_as_float(cnt));
+		for (int j = 0; j < cnt; j++)
+			ctP[4 * int64_t(base + a) + j] = f4(q.c[j].p, q.c[j].s);
+	}
+}
+
 // whether the bodies of a pair are joined: its (lower, higher) key is among
 // the sorted joint keys
 PXH_D bool pairJointed(uint2 ab, const uint64_t* keys, int nKeys)
@@ -2225,9 +2367,24 @@ __global__ void k_sleep_count_init(int n, SleepState ss)
 // geomutils/src/GuBounds.cpp:210-250), inflated by the plane's contact offset.
 __global__ void k_sleep_count(int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
                               const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
-                              const uint64_t* keys, int nKeys, SleepState ss)
+                              const uint64_t* keys, int nKeys, int nMB, const int* meshBody, const int* world,
+                              const float4* meshLo, const float4* meshHi, SleepState ss)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= Q + cap)
+	{
+		// a mesh whose broadphase bounds (its world bounds inflated by its
+		// contact offset) overlap the body's
+		const int k = s - Q - cap;
+		if (k >= nMB)
+			return;
+		const int i = meshBody[k];
+		const float4 lo = meshLo[world[i]], hi = meshHi[world[i]], a = bmin[i], b = bmax[i];
+		if (!(lo.x - co > b.x || a.x > hi.x + co || lo.y - co > b.y || a.y > hi.y + co || lo.z - co > b.z ||
+		      a.z > hi.z + co))
+			atomicAdd(&ss.counted[i], 1);
+		return;
+	}
 	if (s < Q)
 	{
 		const int i = qBody[s];
@@ -2285,10 +2442,17 @@ __global__ void k_sleep_wake_apply(int n, SleepState ss)
 }
 
 // pairs of sleeping bodies lose their contacts (inactive contact managers)
-__global__ void k_sleep_drop(int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
-                             const uint32_t* flags)
+__global__ void k_sleep_drop(int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, int nMB,
+                             const int* meshBody, float4* ctN, const uint32_t* flags)
 {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= Q + cap)
+	{
+		const int k = s - Q - cap;
+		if (k < nMB * MESH_MAXP && asleepBody(flags, meshBody[k / MESH_MAXP]))
+			ctN[s].w = __int_as_float(0);
+		return;
+	}
 	if (s < Q)
 	{
 		if (asleepBody(flags, qBody[s]))
@@ -2913,26 +3077,37 @@ void launch_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs,
 }
 void launch_sleep_count(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody,
                         const int* qPlane, const float4* planes, const float4* bmin, const float4* bmax, float co,
-                        const uint64_t* jointKeys, int nKeys, SleepState ss, hipStream_t st)
+                        const uint64_t* jointKeys, int nKeys, int nMB, const int* meshBody, const int* world,
+                        const float4* meshLo, const float4* meshHi, SleepState ss, hipStream_t st)
 {
 	PXH_LAUNCH(k_sleep_count_init, n, n, ss);
-	PXH_LAUNCH(k_sleep_count, Q + cap, Q, cap, dP, pairs, qBody, qPlane, planes, bmin, bmax, co, jointKeys, nKeys, ss);
+	PXH_LAUNCH(k_sleep_count, Q + cap + nMB, Q, cap, dP, pairs, qBody, qPlane, planes, bmin, bmax, co, jointKeys,
+	           nKeys, nMB, meshBody, world, meshLo, meshHi, ss);
 }
-void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
-                       SleepState ss, hipStream_t st)
+void launch_sleep_wake(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, int nMB,
+                       const int* meshBody, float4* ctN, SleepState ss, hipStream_t st)
 {
 	PXH_LAUNCH(k_sleep_wake_mark, cap, Q, cap, dP, pairs, ctN, ss);
 	PXH_LAUNCH(k_sleep_wake_apply, n, n, ss);
-	PXH_LAUNCH(k_sleep_drop, Q + cap, Q, cap, dP, pairs, qBody, ctN, ss.flags);
+	PXH_LAUNCH(k_sleep_drop, Q + cap + nMB * MESH_MAXP, Q, cap, dP, pairs, qBody, nMB, meshBody, ctN, ss.flags);
 }
-void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, float4* ctN,
-                          int J, const int4* jInfo, float4* lin, float4* ang, SleepState ss, hipStream_t st)
+void launch_sleep_islands(int n, int Q, int cap, const uint32_t* dP, const uint2* pairs, const int* qBody, int nMB,
+                          const int* meshBody, float4* ctN, int J, const int4* jInfo, float4* lin, float4* ang,
+                          SleepState ss, hipStream_t st)
 {
 	PXH_LAUNCH(k_sleep_isl_init, n, n, ss);
 	PXH_LAUNCH(k_sleep_union, cap + J, Q, cap, dP, pairs, ctN, J, jInfo, ss);
 	PXH_LAUNCH(k_sleep_label, n, n, ss);
 	PXH_LAUNCH(k_sleep_deactivate, n, n, ss, lin, ang);
-	PXH_LAUNCH(k_sleep_drop, Q + cap, Q, cap, dP, pairs, qBody, ctN, ss.flags);
+	PXH_LAUNCH(k_sleep_drop, Q + cap + nMB * MESH_MAXP, Q, cap, dP, pairs, qBody, nMB, meshBody, ctN, ss.flags);
+}
+void launch_np_mesh(int nMB, const int* meshBody, const int* world, const int4* worldMesh, const MeshSet& M,
+                    const float4* pos, const float4* rot, const float4* geom, const HullSet& hs, float cd,
+                    int slotBase, float4* ctN, float4* ctP, uint32_t* overflow, hipStream_t st)
+{
+	if (nMB > 0)
+		k_np_mesh<<<nblk(nMB, 64), 64, 0, st>>>(nMB, meshBody, world, worldMesh, M, pos, rot, geom, hs, cd, slotBase,
+		                                        ctN, ctP, overflow);
 }
 void launch_sleep_wake_bodies(int k, const int* bodies, SleepState ss, hipStream_t st)
 {
@@ -2957,17 +3132,18 @@ void launch_joint_solve(int begin, int end, const int4* info, const float4* head
 	k_joint_solve<<<nblk(end - begin, 64), 64, 0, st>>>(begin, end, info, head, rows, sb.vel, sb.dq, sb.i0, sb.i1,
 	                                                     elapsed, conclude ? 1 : 0, sleepFlags);
 }
-void launch_touch_flags(int S, int Q, const uint32_t* dP, const float4* ctN, uint32_t* flags, hipStream_t st)
+void launch_touch_flags(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, uint32_t* flags,
+                        hipStream_t st)
 {
-	PXH_LAUNCH(k_touch_flags, S, S, Q, dP, ctN, flags);
+	PXH_LAUNCH(k_touch_flags, S, S, Q, pairEnd, dP, ctN, flags);
 }
-void launch_con_build(int S, int Q, const uint32_t* dP, const float4* ctN, const uint32_t* scanned, const uint2* pairs,
-                      const int* qBody, const int* qPlane, const int* local, const int* planeLocal, const int* world,
-                      const int* worldTag, int* conA, int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey,
-                      int* conColor, hipStream_t st)
+void launch_con_build(int S, int Q, int pairEnd, const uint32_t* dP, const float4* ctN, const uint32_t* scanned,
+                      const uint2* pairs, const int* qBody, const int* qPlane, const int* meshBody, int nPlanes,
+                      const int* local, const int* planeLocal, const int* world, const int* worldTag, int* conA,
+                      int* conB, int* conSlot, uint64_t* conKey, uint64_t* conPKey, int* conColor, hipStream_t st)
 {
-	PXH_LAUNCH(k_con_build, S, S, Q, dP, ctN, scanned, pairs, qBody, qPlane, local, planeLocal, world, worldTag, conA,
-	           conB, conSlot, conKey, conPKey, conColor);
+	PXH_LAUNCH(k_con_build, S, S, Q, pairEnd, dP, ctN, scanned, pairs, qBody, qPlane, meshBody, nPlanes, local,
+	           planeLocal, world, worldTag, conA, conB, conSlot, conKey, conPKey, conColor);
 }
 // out[i] = in[idx[i]] for elements of `bytes` (4 or 16) bytes
 __global__ void k_gather4(int n, const int* idx, const uint32_t* in, uint32_t* out)

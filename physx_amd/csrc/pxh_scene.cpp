@@ -255,6 +255,31 @@ struct pxh_scene_t
 	DBuf<int4> jInfo;
 	DBuf<float4> jFrame, jHead, jRows;
 	DBuf<uint64_t> jKeys;
+	// static triangle meshes, at most one per world: host copies of each mesh's
+	// world-space triangles (w of the vertices: vertex id), its broadphase
+	// bounds; on the device the triangles and trees of every mesh, per world
+	// the mesh's tree range (-1: none) and bounds, and the bodies of the
+	// worlds with a mesh
+	struct MeshInst
+	{
+		int world = 0;
+		std::vector<float4> v0, v1, v2, n, lo, hi;
+		float4 blo{}, bhi{};
+	};
+	std::vector<MeshInst> meshes;
+	bool meshDirty = false;
+	int nMeshBodies = 0;
+	DBuf<float4> mV0, mV1, mV2, mN, mLo, mHi, mNodeLo, mNodeHi, meshLo, meshHi;
+	DBuf<int4> mNodeInfo, worldMesh;
+	DBuf<int> mLeafTri, meshBody;
+	DBuf<uint32_t> meshOverflow;
+	pxh::MeshSet meshSet() const
+	{
+		pxh::MeshSet m;
+		m.v0 = mV0.p; m.v1 = mV1.p; m.v2 = mV2.p; m.n = mN.p; m.lo = mLo.p; m.hi = mHi.p;
+		m.nodeLo = mNodeLo.p; m.nodeHi = mNodeHi.p; m.nodeInfo = mNodeInfo.p; m.leafTri = mLeafTri.p;
+		return m;
+	}
 	// sleeping (desc.enable_sleeping): per body the sleep flags, wake counter,
 	// accumulated motion velocities and island label; per island label the
 	// pending wake-up (set between steps by wake-ups); per step the counted
@@ -424,6 +449,114 @@ int uploadVec(DBuf<T>& b, const std::vector<T>& v)
 	return PXH_OK;
 }
 
+// Bounding-volume tree over triangles items[b, e) of one mesh (their bounds
+// lo/hi): split at the median centroid along the widest axis of the
+// centroids (ties by triangle index), leaves of at most 4; nodes depth-first,
+// info = (first leaf entry, leaf count or 0, index after the subtree).
+void buildMeshTree(const std::vector<float4>& lo, const std::vector<float4>& hi, std::vector<int>& items, int b, int e,
+                   std::vector<float4>& nlo, std::vector<float4>& nhi, std::vector<int4>& info,
+                   std::vector<int>& leafTri)
+{
+	const int me = int(info.size());
+	float4 l = make_float4(3.4e38f, 3.4e38f, 3.4e38f, 0.0f), h = make_float4(-3.4e38f, -3.4e38f, -3.4e38f, 0.0f);
+	float4 cl = l, ch = h;
+	for (int k = b; k < e; k++)
+	{
+		const float4 a = lo[items[k]], c = hi[items[k]];
+		l.x = std::min(l.x, a.x); l.y = std::min(l.y, a.y); l.z = std::min(l.z, a.z);
+		h.x = std::max(h.x, c.x); h.y = std::max(h.y, c.y); h.z = std::max(h.z, c.z);
+		const float mx = 0.5f * (a.x + c.x), my = 0.5f * (a.y + c.y), mz = 0.5f * (a.z + c.z);
+		cl.x = std::min(cl.x, mx); cl.y = std::min(cl.y, my); cl.z = std::min(cl.z, mz);
+		ch.x = std::max(ch.x, mx); ch.y = std::max(ch.y, my); ch.z = std::max(ch.z, mz);
+	}
+	nlo.push_back(l);
+	nhi.push_back(h);
+	info.push_back(make_int4(0, 0, 0, 0));
+	if (e - b <= 4)
+	{
+		info[me] = make_int4(int(leafTri.size()), e - b, me + 1, 0);
+		for (int k = b; k < e; k++)
+			leafTri.push_back(items[k]);
+		return;
+	}
+	const float ex = ch.x - cl.x, ey = ch.y - cl.y, ez = ch.z - cl.z;
+	const int axis = (ex >= ey && ex >= ez) ? 0 : (ey >= ez ? 1 : 2);
+	auto key = [&](int t) {
+		const float4 a = lo[t], c = hi[t];
+		return axis == 0 ? a.x + c.x : (axis == 1 ? a.y + c.y : a.z + c.z);
+	};
+	std::sort(items.begin() + b, items.begin() + e, [&](int x, int y) {
+		const float kx = key(x), ky = key(y);
+		return kx < ky || (kx == ky && x < y);
+	});
+	const int mid = (b + e) / 2;
+	buildMeshTree(lo, hi, items, b, mid, nlo, nhi, info, leafTri);
+	buildMeshTree(lo, hi, items, mid, e, nlo, nhi, info, leafTri);
+	info[me] = make_int4(0, 0, int(info.size()), 0);
+}
+
+// every mesh's triangles and tree to the device, world ranges and bounds
+int uploadMeshes(pxh_scene_t* s)
+{
+	std::vector<float4> v0, v1, v2, n, lo, hi, nlo, nhi;
+	std::vector<int4> info;
+	std::vector<int> leaf;
+	const int nW = worldCount(s);
+	std::vector<int4> wm(size_t(std::max(nW, 1)), make_int4(-1, 0, 0, 0));
+	std::vector<float4> wlo(size_t(std::max(nW, 1))), whi(size_t(std::max(nW, 1)));
+	for (const auto& m : s->meshes)
+	{
+		const int tb = int(v0.size()), nb = int(info.size());
+		v0.insert(v0.end(), m.v0.begin(), m.v0.end());
+		v1.insert(v1.end(), m.v1.begin(), m.v1.end());
+		v2.insert(v2.end(), m.v2.begin(), m.v2.end());
+		n.insert(n.end(), m.n.begin(), m.n.end());
+		lo.insert(lo.end(), m.lo.begin(), m.lo.end());
+		hi.insert(hi.end(), m.hi.begin(), m.hi.end());
+		std::vector<int> items;
+		for (int k = 0; k < int(m.v0.size()); k++)
+			items.push_back(k);
+		std::vector<float4> tl, th;
+		std::vector<int4> ti;
+		std::vector<int> tlf;
+		if (!items.empty())
+			buildMeshTree(m.lo, m.hi, items, 0, int(items.size()), tl, th, ti, tlf);
+		const int lb = int(leaf.size());
+		for (auto& x : ti)
+		{
+			if (x.y > 0)
+				x.x += lb;
+			x.z += nb;
+			info.push_back(x);
+		}
+		nlo.insert(nlo.end(), tl.begin(), tl.end());
+		nhi.insert(nhi.end(), th.begin(), th.end());
+		for (int t : tlf)
+			leaf.push_back(tb + t);
+		if (m.world < nW)
+		{
+			wm[m.world] = make_int4(nb, int(ti.size()), 0, 0);
+			wlo[m.world] = m.blo;
+			whi[m.world] = m.bhi;
+		}
+	}
+	if (int r = uploadVec(s->mV0, v0)) return r;
+	if (int r = uploadVec(s->mV1, v1)) return r;
+	if (int r = uploadVec(s->mV2, v2)) return r;
+	if (int r = uploadVec(s->mN, n)) return r;
+	if (int r = uploadVec(s->mLo, lo)) return r;
+	if (int r = uploadVec(s->mHi, hi)) return r;
+	if (int r = uploadVec(s->mNodeLo, nlo)) return r;
+	if (int r = uploadVec(s->mNodeHi, nhi)) return r;
+	if (int r = uploadVec(s->mNodeInfo, info)) return r;
+	if (int r = uploadVec(s->mLeafTri, leaf)) return r;
+	if (int r = uploadVec(s->worldMesh, wm)) return r;
+	if (int r = uploadVec(s->meshLo, wlo)) return r;
+	if (int r = uploadVec(s->meshHi, whi)) return r;
+	s->meshDirty = false;
+	return PXH_OK;
+}
+
 int rebuild_topology(pxh_scene_t* s)
 {
 	// tags of new worlds, in creation order
@@ -444,6 +577,21 @@ int rebuild_topology(pxh_scene_t* s)
 				qb.push_back(i);
 				qp.push_back(p);
 			}
+	}
+	// bodies of the worlds with a mesh
+	{
+		std::vector<char> has(size_t(std::max(worldCount(s), 1)), 0);
+		for (const auto& m : s->meshes)
+			if (m.world < int(has.size()))
+				has[m.world] = 1;
+		std::vector<int> mb;
+		for (int i = 0; i < s->n; i++)
+			if (has[s->hScene[i]])
+				mb.push_back(i);
+		s->nMeshBodies = int(mb.size());
+		if (int r = uploadVec(s->meshBody, mb))
+			return r;
+		s->meshDirty = true;
 	}
 	s->Q = int(qb.size());
 	PXH_CHECK(s->qBody.ensure(std::max<size_t>(qb.size(), 1)));
@@ -1146,6 +1294,22 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 	s->worldPlanes.swap(worldPlanes);
 	s->hWorldTag.swap(tags);
 	s->topologyDirty = true;
+	// meshes of the kept worlds, renumbered
+	{
+		int dropped = 0;
+		for (int w = 0; w < nW; w++)
+			dropped += drop[w];
+		std::vector<pxh_scene_t::MeshInst> kept;
+		for (auto& m : s->meshes)
+		{
+			if (m.world < nW && drop[m.world])
+				continue;
+			m.world = m.world < nW ? newIdx[m.world] : m.world - dropped;
+			kept.push_back(std::move(m));
+		}
+		s->meshes.swap(kept);
+		s->meshDirty = true;
+	}
 	// aggregate membership of the kept bodies
 	if (!s->hAggSelf.empty())
 	{
@@ -1181,6 +1345,96 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 			return r;
 	}
 	return PXH_OK;
+}
+
+int32_t pxh_scene_add_triangle_mesh(pxh_scene s, int32_t nv, const float* verts, int32_t nt, const int32_t* tris,
+                                    const float* pose, int32_t world)
+{
+	if (!s || nv <= 0 || nt <= 0 || !verts || !tris || !pose || world < 0)
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_triangle_mesh: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_add_triangle_mesh: not allowed while simulating");
+	for (const auto& m : s->meshes)
+		if (m.world == world)
+			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_add_triangle_mesh: one mesh per world");
+	for (int k = 0; k < 3 * nt; k++)
+		if (tris[k] < 0 || tris[k] >= nv)
+			return fail(PXH_ERR_INVALID, "pxh_scene_add_triangle_mesh: vertex index out of range");
+	// PxQuat::rotate and the quaternion's matrix, in the oracle's arithmetic
+	const float qx = pose[3], qy = pose[4], qz = pose[5], qw = pose[6];
+	auto rot = [&](float x, float y, float z, float* o) {
+		const float vx = 2.0f * x, vy = 2.0f * y, vz = 2.0f * z;
+		const float w2 = qw * qw - 0.5f;
+		const float dot2 = qx * vx + qy * vy + qz * vz;
+		o[0] = vx * w2 + (qy * vz - qz * vy) * qw + qx * dot2;
+		o[1] = vy * w2 + (qz * vx - qx * vz) * qw + qy * dot2;
+		o[2] = vz * w2 + (qx * vy - qy * vx) * qw + qz * dot2;
+	};
+	std::vector<float> P(size_t(3) * nv);
+	float lmin[3] = {3.4e38f, 3.4e38f, 3.4e38f}, lmax[3] = {-3.4e38f, -3.4e38f, -3.4e38f};
+	for (int k = 0; k < nv; k++)
+	{
+		float r[3];
+		rot(verts[3 * k], verts[3 * k + 1], verts[3 * k + 2], r);
+		for (int c = 0; c < 3; c++)
+		{
+			P[3 * k + c] = r[c] + pose[c];
+			lmin[c] = std::min(lmin[c], verts[3 * k + c]);
+			lmax[c] = std::max(lmax[c], verts[3 * k + c]);
+		}
+	}
+	pxh_scene_t::MeshInst m;
+	m.world = world;
+	for (int k = 0; k < nt; k++)
+	{
+		const int a = tris[3 * k], b = tris[3 * k + 1], c = tris[3 * k + 2];
+		const float* A = &P[3 * a];
+		const float* B = &P[3 * b];
+		const float* C = &P[3 * c];
+		const float e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+		const float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+		const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+		if (!(nn > 0.0f))
+			continue;   // degenerate triangle: not stored
+		const float inv = 1.0f / std::sqrt(nn);
+		auto vtx = [&](const float* v, int id) {
+			float id_f;
+			std::memcpy(&id_f, &id, sizeof(float));
+			return make_float4(v[0], v[1], v[2], id_f);
+		};
+		m.v0.push_back(vtx(A, a));
+		m.v1.push_back(vtx(B, b));
+		m.v2.push_back(vtx(C, c));
+		m.n.push_back(make_float4(n[0] * inv, n[1] * inv, n[2] * inv, 0.0f));
+		m.lo.push_back(make_float4(std::fmin(std::fmin(A[0], B[0]), C[0]), std::fmin(std::fmin(A[1], B[1]), C[1]),
+		                           std::fmin(std::fmin(A[2], B[2]), C[2]), 0.0f));
+		m.hi.push_back(make_float4(std::fmax(std::fmax(A[0], B[0]), C[0]), std::fmax(std::fmax(A[1], B[1]), C[1]),
+		                           std::fmax(std::fmax(A[2], B[2]), C[2]), 0.0f));
+	}
+	if (m.v0.empty())
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_triangle_mesh: every triangle is degenerate");
+	// broadphase bounds: the local box about the pose (Gu::computeBounds of a
+	// triangle mesh shape, geomutils/src/GuBounds.cpp:354-395)
+	{
+		const float cx = 0.5f * (lmin[0] + lmax[0]), cy = 0.5f * (lmin[1] + lmax[1]), cz = 0.5f * (lmin[2] + lmax[2]);
+		const float ex = 0.5f * (lmax[0] - lmin[0]), ey = 0.5f * (lmax[1] - lmin[1]), ez = 0.5f * (lmax[2] - lmin[2]);
+		float c[3];
+		rot(cx, cy, cz, c);
+		const float x2 = qx + qx, y2 = qy + qy, z2 = qz + qz;
+		const float xx = x2 * qx, yy = y2 * qy, zz = z2 * qz, xy = x2 * qy, xz = x2 * qz, xw = x2 * qw;
+		const float yz = y2 * qz, yw = y2 * qw, zw = z2 * qw;
+		const float R[3][3] = {{1.0f - yy - zz, xy - zw, xz + yw}, {xy + zw, 1.0f - xx - zz, yz - xw},
+		                       {xz - yw, yz + xw, 1.0f - xx - yy}};   // rows
+		float w[3];
+		for (int r = 0; r < 3; r++)
+			w[r] = std::fabs(R[r][0]) * ex + std::fabs(R[r][1]) * ey + std::fabs(R[r][2]) * ez;
+		m.blo = make_float4(c[0] + pose[0] - w[0], c[1] + pose[1] - w[1], c[2] + pose[2] - w[2], 0.0f);
+		m.bhi = make_float4(c[0] + pose[0] + w[0], c[1] + pose[1] + w[1], c[2] + pose[2] + w[2], 0.0f);
+	}
+	s->meshes.push_back(std::move(m));
+	s->meshDirty = true;
+	s->topologyDirty = true;
+	return int32_t(s->meshes.size()) - 1;
 }
 
 int32_t pxh_scene_get_sleep_state(pxh_scene s, float* wake_counter, int32_t* asleep, int32_t n)
@@ -1250,6 +1504,12 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	if (s->topologyDirty)
 	{
 		int r = rebuild_topology(s);
+		if (r)
+			return r;
+	}
+	if (s->meshDirty)
+	{
+		int r = uploadMeshes(s);
 		if (r)
 			return r;
 	}
@@ -1353,7 +1613,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	for (int attempt = 0;; attempt++)
 	{
 		const int cap = int(s->pairCap);
-		const int S = Q + cap;
+		const int pairEnd = Q + cap, nMB = s->nMeshBodies;
+		const int S = pairEnd + nMB * pxh::MESH_MAXP;   // slots: planes, body pairs, mesh patches
 		if (attempt > 0)
 		{
 			r = bp_write_pairs(s, n, s->scene.p, bpGroup, s->rowOff.p, st, s->pairCap);
@@ -1382,6 +1643,13 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			                      d.tolerance_length, s->cvxWit.p, s->cvxList.p, s->ctN.p, s->ctP.p, st);
 		}
 		s->profEnd(pk);
+		if (nMB > 0)
+		{
+			PXH_CHECK(s->meshOverflow.ensure(1));
+			pxh::launch_fill_u32(s->meshOverflow.p, 1, 0u, st);
+			pxh::launch_np_mesh(nMB, s->meshBody.p, s->scene.p, s->worldMesh.p, s->meshSet(), s->pos.p, s->rot.p,
+			                    s->geom.p, s->hullSet(), cd, pairEnd, s->ctN.p, s->ctP.p, s->meshOverflow.p, st);
+		}
 		pxh::launch_joint_filter(cap, Q, dP, s->pairs.p, s->jKeys.p, s->nJKeys, s->ctN.p, st);
 		if (sleeping)
 		{
@@ -1399,14 +1667,15 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 			}
 			const pxh::SleepState ss = s->sleepState();
 			pxh::launch_sleep_count(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->qPlane.p, s->planes.p, s->bmin.p,
-			                        s->bmax.p, co, s->jKeys.p, s->nJKeys, ss, st);
-			pxh::launch_sleep_wake(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->ctN.p, ss, st);
-			pxh::launch_sleep_islands(n, Q, cap, dP, s->pairs.p, s->qBody.p, s->ctN.p, int(s->hJType.size()),
-			                          s->jInfo.p, s->lin.p, s->ang.p, ss, st);
+			                        s->bmax.p, co, s->jKeys.p, s->nJKeys, nMB, s->meshBody.p, s->scene.p, s->meshLo.p,
+			                        s->meshHi.p, ss, st);
+			pxh::launch_sleep_wake(n, Q, cap, dP, s->pairs.p, s->qBody.p, nMB, s->meshBody.p, s->ctN.p, ss, st);
+			pxh::launch_sleep_islands(n, Q, cap, dP, s->pairs.p, s->qBody.p, nMB, s->meshBody.p, s->ctN.p,
+			                          int(s->hJType.size()), s->jInfo.p, s->lin.p, s->ang.p, ss, st);
 		}
 		PXH_CHECK(s->flags.ensure(size_t(S) + 1));
 		PXH_CHECK(s->scanTmp.ensure(pxh::scan_tmp_size(int64_t(S) + 1) + 1024));
-		pxh::launch_touch_flags(S, Q, dP, s->ctN.p, s->flags.p, st);
+		pxh::launch_touch_flags(S, Q, pairEnd, dP, s->ctN.p, s->flags.p, st);
 		PXH_CHECK(hipMemsetAsync(s->flags.p + S, 0, sizeof(uint32_t), st));
 		pxh::launch_scan(s->flags.p, int64_t(S) + 1, s->scanTmp.p, st);
 		const size_t Sc = std::max(S, 1);
@@ -1416,15 +1685,17 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		PXH_CHECK(s->conColor.ensure(Sc));
 		PXH_CHECK(s->conKey.ensure(Sc));
 		PXH_CHECK(s->conPKey.ensure(Sc));
-		pxh::launch_con_build(S, Q, dP, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p, s->local.p,
-		                      s->planeLocal.p, s->scene.p, s->worldTag.p, s->conA.p, s->conB.p, s->conSlot.p,
-		                      s->conKey.p, s->conPKey.p, s->conColor.p, st);
-		const uint32_t* src[2] = {dP, s->flags.p + S};
-		r = readCounts(st, s->hostCounts, src, 2);
+		pxh::launch_con_build(S, Q, pairEnd, dP, s->ctN.p, s->flags.p, s->pairs.p, s->qBody.p, s->qPlane.p,
+		                      s->meshBody.p, s->m, s->local.p, s->planeLocal.p, s->scene.p, s->worldTag.p, s->conA.p,
+		                      s->conB.p, s->conSlot.p, s->conKey.p, s->conPKey.p, s->conColor.p, st);
+		const uint32_t* src[3] = {dP, s->flags.p + S, nMB > 0 ? s->meshOverflow.p : dP};
+		r = readCounts(st, s->hostCounts, src, 3);
 		if (r)
 			return r;
 		P = int(s->hostCounts[0]);
 		C = int(s->hostCounts[1]);
+		if (nMB > 0 && s->hostCounts[2])
+			return fail(PXH_ERR_CAPACITY, "pxh_scene_simulate: more than 128 mesh triangles meet one body's bounds");
 		if (P <= cap)
 			break;
 		if (attempt > 0)

@@ -64,6 +64,10 @@ class Scene:
     joints: dict | None = None
     # aggregates: list of (member body indices, self_collisions)
     aggregates: list | None = None
+    # static triangle meshes, at most one per world: list of dicts with
+    # "verts" (nv,3) f32, "tris" (nt,3) int32 (counter-clockwise about the
+    # outward normal), "pose" (7,) f32 (p xyz, q xyzw), "world" int
+    meshes: list | None = None
 
     @property
     def n(self) -> int:
@@ -92,11 +96,14 @@ class Scene:
                       "body0": np.where(j0[keep] >= 0, new[np.maximum(j0[keep], 0)], -1).astype(np.int32),
                       "body1": np.where(j1[keep] >= 0, new[np.maximum(j1[keep], 0)], -1).astype(np.int32),
                       "frame0": self.joints["frame0"][keep].copy(), "frame1": self.joints["frame1"][keep].copy()}
+        meshes = None
+        if self.meshes:
+            meshes = [dict(m, world=int(m["world"]) - w0) for m in self.meshes if w0 <= int(m["world"]) < w1]
         return Scene(self.shape_type[b].copy(), self.dims[b].copy(), self.pose[b].copy(),
                      self.lin_vel[b].copy(), self.ang_vel[b].copy(), self.mass[b].copy(),
                      self.inertia[b].copy(), (self.scene_id[b] - w0).astype(np.int32),
                      self.planes[p].copy(), (self.plane_scene[p] - w0).astype(np.int32), w1 - w0,
-                     name, dict(self.params), self.hull_points, self.hull_counts, joints)
+                     name, dict(self.params), self.hull_points, self.hull_counts, joints, None, meshes)
 
 
 def mass_props(shape_type: np.ndarray, dims: np.ndarray, density: float):
@@ -505,4 +512,45 @@ def aggregate_pile(n_groups=64, per_group=4, seed=7) -> Scene:
                  "aggregate_pile", {"n_groups": n_groups, "per_group": per_group, "seed": seed})
     sc.aggregates = [(np.arange(g * per_group, (g + 1) * per_group, dtype=np.int32), g % 2 == 1)
                      for g in range(n_groups)]
+    return sc
+
+
+def terrain_mesh(cells=16, size=16.0, amp=0.4, seed=1):
+    """A terrain as a triangle mesh: a (cells+1)^2 grid of heights over a
+    size x size square centred at the origin, y = amp * (0.6 sin(a x + p)
+    cos(b z + q) + 0.4 sin(c (x + z) + r)) with seeded wave numbers and
+    phases, two triangles per cell wound counter-clockwise about +y.
+    Returns (verts (nv,3) f32, tris (nt,3) int32)."""
+    k = np.arange(6)
+    a, b, c = (uniform(seed, k[i:i + 1], 0, 0.4, 1.2)[0] for i in range(3))
+    p, q, r = (uniform(seed, k[i:i + 1], 1, 0.0, 2 * math.pi)[0] for i in range(3, 6))
+    xs = np.linspace(-size / 2, size / 2, cells + 1)
+    X, Z = np.meshgrid(xs, xs, indexing="ij")
+    Y = amp * (0.6 * np.sin(a * X + p) * np.cos(b * Z + q) + 0.4 * np.sin(c * (X + Z) + r))
+    verts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
+    i, j = np.meshgrid(np.arange(cells), np.arange(cells), indexing="ij")
+    v00 = (i * (cells + 1) + j).ravel()
+    v01, v10 = v00 + 1, v00 + cells + 1
+    v11 = v10 + 1
+    tris = np.concatenate([np.stack([v00, v01, v10], 1), np.stack([v01, v11, v10], 1)]).astype(np.int32)
+    return verts, tris
+
+
+def terrain_drop(n_scenes=1, per_side=6, layers=1, cells=16, size=16.0, amp=0.4, seed=9) -> Scene:
+    """mixed_pile's spheres, capsules, boxes and hulls (per_side^2 x layers per
+    world) dropped onto a terrain mesh instead of a plane: one terrain_mesh per
+    world (the same mesh, identity pose), bodies raised by amp + 0.1 m."""
+    base = mixed_pile(n_bodies=per_side * per_side * layers, per_side=per_side, seed=seed)
+    verts, tris = terrain_mesh(cells, size, amp, seed)
+    n1 = base.n
+    rep = lambda a: np.concatenate([a] * n_scenes)  # noqa: E731
+    pose = rep(base.pose).copy()
+    pose[:, 1] += amp + 0.1
+    sc = Scene(rep(base.shape_type), rep(base.dims), pose, rep(base.lin_vel), rep(base.ang_vel), rep(base.mass),
+               rep(base.inertia), np.repeat(np.arange(n_scenes, dtype=np.int32), n1), np.zeros((0, 4), np.float32),
+               np.zeros(0, np.int32), n_scenes, "terrain_drop",
+               dict(n_scenes=n_scenes, per_side=per_side, layers=layers, cells=cells, size=size, amp=amp, seed=seed),
+               base.hull_points, base.hull_counts)
+    ident = np.array([0, 0, 0, 0, 0, 0, 1], np.float32)
+    sc.meshes = [{"verts": verts, "tris": tris, "pose": ident, "world": w} for w in range(n_scenes)]
     return sc

@@ -213,6 +213,23 @@ def pxo_add_hull(h):
     return pxo().pxo_hull_add(int(v.shape[0]), _vp(v), int(pl.shape[0]), _vp(pl), _vp(po), _vp(vr), _vp(ie))
 
 
+_TRIMESH_IDS = {}
+
+
+def ref_trimesh(verts, tris):
+    """Cook a triangle mesh with the reference cooker (cached by content);
+    returns its mesh index (geometry code 8, dims[0])."""
+    v, t = f32(verts).reshape(-1, 3), i32(tris).reshape(-1, 3)
+    key = (v.tobytes(), t.tobytes())
+    if key not in _TRIMESH_IDS:
+        lib = ref()
+        lib.pxref_trimesh_create.restype = ctypes.c_int
+        mid = lib.pxref_trimesh_create(_vp(v), int(v.shape[0]), _vp(t), int(t.shape[0]))
+        assert mid >= 0
+        _TRIMESH_IDS[key] = mid
+    return _TRIMESH_IDS[key]
+
+
 def ref_cook(points):
     """Cook a hull with the reference cooker; returns (mesh index, hull dict)."""
     lib = ref()
@@ -281,6 +298,11 @@ class OracleWorld:
         for members, self_coll in (getattr(scene, "aggregates", None) or []):
             m = i32(members)
             assert self.lib.pxo_world_add_aggregate(self.h, int(m.size), ip(m), int(bool(self_coll))) >= 0
+        for m in (getattr(scene, "meshes", None) or []):
+            v, t, po = f32(m["verts"]), i32(m["tris"]), f32(m["pose"])
+            self._keep += [v, t, po]
+            assert self.lib.pxo_world_add_trimesh(self.h, int(v.shape[0]), fp(v), int(t.shape[0]), ip(t), fp(po),
+                                                  int(m["world"])) >= 0
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
@@ -438,6 +460,11 @@ class RefScene:
         for members, self_coll in (getattr(scene, "aggregates", None) or []):
             m = i32(members)
             assert self.lib.pxref_scene_add_aggregate(self.h, int(m.size), ip(m), int(bool(self_coll))) == 0
+        for m in (getattr(scene, "meshes", None) or []):
+            mid = ref_trimesh(m["verts"], m["tris"])
+            po = f32(m["pose"])
+            self._keep.append(po)
+            assert self.lib.pxref_scene_add_trimesh(self.h, mid, fp(po), ctypes.c_float(0.02)) == 0
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]
