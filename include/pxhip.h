@@ -165,6 +165,19 @@ int32_t pxh_scene_set_rigid_dynamic_data(pxh_scene scene, const void* data, cons
 int32_t pxh_scene_add_triangle_mesh(pxh_scene scene, int32_t nv, const float* verts, int32_t nt, const int32_t* tris,
                                     const float* pose, int32_t world);
 
+/* Static heightfield (PxCreateHeightField + a static actor with
+ * PxHeightFieldGeometry(hf, no flags, height_scale, row_scale, column_scale);
+ * physx/include/geometry/PxHeightFieldGeometry.h, PxHeightFieldDesc.h) in
+ * world `world`, added as the triangle mesh of its cells: rows x cols int16
+ * samples (host, row-major), tess[rows*cols] (host, may be NULL) each
+ * sample's tessellation flag (the cell's diagonal runs from its zeroth vertex,
+ * PxHeightFieldSample::tessFlag), vertex (row * row_scale, height *
+ * height_scale, col * column_scale) in the heightfield's frame at pose[7].
+ * Counts as the world's one mesh. Returns the mesh index. */
+int32_t pxh_scene_add_heightfield(pxh_scene scene, int32_t rows, int32_t cols, const int16_t* heights,
+                                  const uint8_t* tess, float height_scale, float row_scale, float column_scale,
+                                  const float* pose, int32_t world);
+
 /* Sleeping (enable_sleeping): bodies and islands fall asleep and wake as in
  * the reference's CPU pipeline. Each body keeps a wake counter (0.4 s when
  * added or woken) that counts down while the body's mean motion over the

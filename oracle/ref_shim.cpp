@@ -44,10 +44,13 @@ bool ensureSdk()
 // convex meshes cooked by pxref_convex_create, named by index
 std::vector<PxConvexMesh*> gConvex;
 std::vector<PxTriangleMesh*> gTriMesh;
+std::vector<PxHeightField*> gHeightField;
 
 // geometry codes shared with the product C-ABI: 0 sphere, 1 plane, 2 capsule,
 // 3 box, 5 convex mesh (g[0] = index of a mesh from pxref_convex_create),
-// 8 triangle mesh (g[0] = index of a mesh from pxref_trimesh_create)
+// 8 triangle mesh (g[0] = index of a mesh from pxref_trimesh_create),
+// 9 heightfield (g[0] = index from pxref_heightfield_create, g[1] height
+// scale, g[2] row scale, g[3] column scale)
 PxGeometryHolder makeGeom(int type, const float* g)
 {
 	switch (type)
@@ -57,6 +60,7 @@ PxGeometryHolder makeGeom(int type, const float* g)
 	case 2: return PxGeometryHolder(PxCapsuleGeometry(g[0], g[1]));
 	case 5: return PxGeometryHolder(PxConvexMeshGeometry(gConvex[size_t(g[0])]));
 	case 8: return PxGeometryHolder(PxTriangleMeshGeometry(gTriMesh[size_t(g[0])]));
+	case 9: return PxGeometryHolder(PxHeightFieldGeometry(gHeightField[size_t(g[0])], PxMeshGeometryFlags(), g[1], g[2], g[3]));
 	default: return PxGeometryHolder(PxBoxGeometry(g[0], g[1], g[2]));
 	}
 }
@@ -279,6 +283,51 @@ int pxref_trimesh_create(const float* verts, int nv, const int* tris, int nt)
 		return -1;
 	gTriMesh.push_back(m);
 	return int(gTriMesh.size()) - 1;
+}
+
+// Creates a heightfield of rows x cols samples (PxCreateHeightField):
+// heights[rows*cols] row-major, tess[rows*cols] (may be null) the sample's
+// tessellation flag (the cell's diagonal from its zeroth vertex); returns its
+// index (geometry code 9), or -1.
+int pxref_heightfield_create(int rows, int cols, const int16_t* heights, const uint8_t* tess)
+{
+	if (!ensureSdk())
+		return -1;
+	std::vector<PxHeightFieldSample> samples(size_t(rows) * cols);
+	for (size_t i = 0; i < samples.size(); i++)
+	{
+		samples[i].height = heights[i];
+		samples[i].materialIndex0 = 0;
+		samples[i].materialIndex1 = 0;
+		if (tess && tess[i])
+			samples[i].setTessFlag();
+	}
+	PxHeightFieldDesc desc;
+	desc.nbRows = PxU32(rows);
+	desc.nbColumns = PxU32(cols);
+	desc.samples.data = samples.data();
+	desc.samples.stride = sizeof(PxHeightFieldSample);
+	PxHeightField* hf = PxCreateHeightField(desc, gPhysics->getPhysicsInsertionCallback());
+	if (!hf)
+		return -1;
+	gHeightField.push_back(hf);
+	return int(gHeightField.size()) - 1;
+}
+
+// Adds a static actor with one heightfield shape (geom as for code 9) at pose.
+int pxref_scene_add_heightfield(void* h, const float* geom, const float* pose, float contactOffset)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	if (geom[0] < 0.0f || size_t(geom[0]) >= gHeightField.size())
+		return -1;
+	PxGeometryHolder g = makeGeom(9, geom);
+	PxRigidStatic* s = gPhysics->createRigidStatic(makePose(pose));
+	PxShape* shape = PxRigidActorExt::createExclusiveShape(*s, g.any(), *rs->material);
+	shape->setContactOffset(contactOffset);
+	shape->setRestOffset(0.0f);
+	s->userData = reinterpret_cast<void*>(intptr_t(-(++rs->nPlanes)));
+	rs->scene->addActor(*s);
+	return 0;
 }
 
 // Adds a static actor with one triangle mesh shape (identity scale) at pose;

@@ -64,7 +64,8 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
             "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate",
-            "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies", "pxh_scene_add_triangle_mesh")
+            "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies", "pxh_scene_add_triangle_mesh",
+            "pxh_scene_add_heightfield")
 
 _lib = None
 
@@ -165,7 +166,12 @@ class Scene:
         for members, self_coll in (getattr(scene, "aggregates", None) or []):
             self.add_aggregate(np.asarray(members) + base, self_coll)
         for m in (getattr(scene, "meshes", None) or []):
-            self.add_triangle_mesh(m["verts"], m["tris"], m["pose"], int(m["world"]))
+            if "heightfield" in m:
+                hf = m["heightfield"]
+                self.add_heightfield(hf["heights"], hf.get("tess"), hf["height_scale"], hf["row_scale"],
+                                     hf["column_scale"], m["pose"], int(m["world"]))
+            else:
+                self.add_triangle_mesh(m["verts"], m["tris"], m["pose"], int(m["world"]))
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
@@ -179,6 +185,19 @@ class Scene:
         return _check(self.lib.pxh_scene_add_triangle_mesh(self.h, int(v.shape[0]), v.ctypes.data_as(_F),
                                                            int(t.shape[0]), t.ctypes.data_as(_I32),
                                                            po.ctypes.data_as(_F), int(world)))
+
+    def add_heightfield(self, heights, tess=None, height_scale=1.0, row_scale=1.0, column_scale=1.0,
+                        pose=(0, 0, 0, 0, 0, 0, 1), world: int = 0) -> int:
+        """pxh_scene_add_heightfield: heights (rows, cols) int16 samples, tess
+        (rows, cols) tessellation flags or None; returns the mesh index."""
+        h = np.ascontiguousarray(heights, dtype=np.int16)
+        rows, cols = h.shape
+        t = np.ascontiguousarray(tess, dtype=np.uint8) if tess is not None else None
+        po = _f32(pose).reshape(7)
+        return _check(self.lib.pxh_scene_add_heightfield(
+            self.h, int(rows), int(cols), h.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)),
+            t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)) if t is not None else None, ctypes.c_float(height_scale),
+            ctypes.c_float(row_scale), ctypes.c_float(column_scale), po.ctypes.data_as(_F), int(world)))
 
     def add_aggregate(self, bodies, self_collisions: bool = False) -> int:
         """pxh_scene_add_aggregate (PxAggregate with the bodies): returns its id."""

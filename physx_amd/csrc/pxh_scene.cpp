@@ -1437,6 +1437,46 @@ int32_t pxh_scene_add_triangle_mesh(pxh_scene s, int32_t nv, const float* verts,
 	return int32_t(s->meshes.size()) - 1;
 }
 
+int32_t pxh_scene_add_heightfield(pxh_scene s, int32_t rows, int32_t cols, const int16_t* heights,
+                                  const uint8_t* tess, float height_scale, float row_scale, float column_scale,
+                                  const float* pose, int32_t world)
+{
+	if (!s || rows < 2 || cols < 2 || !heights || !pose || !(row_scale > 0.0f) || !(column_scale > 0.0f) ||
+	    !(height_scale > 0.0f))
+		return fail(PXH_ERR_INVALID, "pxh_scene_add_heightfield: bad arguments");
+	// the heightfield's vertices and triangles (Gu::HeightField::
+	// getTriangleVertexIndices, geomutils/src/hf/GuHeightField.h:708-760):
+	// vertex row * cols + col at (row * row_scale, height * height_scale,
+	// col * column_scale); cell (row, col) splits along the diagonal from its
+	// zeroth vertex when that sample's tessellation flag is set
+	std::vector<float> v(size_t(3) * rows * cols);
+	for (int r = 0; r < rows; r++)
+		for (int c = 0; c < cols; c++)
+		{
+			float* o = &v[3 * (size_t(r) * cols + c)];
+			o[0] = float(r) * row_scale;
+			o[1] = float(heights[size_t(r) * cols + c]) * height_scale;
+			o[2] = float(c) * column_scale;
+		}
+	std::vector<int32_t> t;
+	for (int r = 0; r + 1 < rows; r++)
+		for (int c = 0; c + 1 < cols; c++)
+		{
+			const int cell = r * cols + c;
+			if (tess && tess[cell])
+			{
+				const int a[6] = {cell + cols, cell, cell + cols + 1, cell + 1, cell + cols + 1, cell};
+				t.insert(t.end(), a, a + 6);
+			}
+			else
+			{
+				const int a[6] = {cell, cell + 1, cell + cols, cell + cols + 1, cell + cols, cell + 1};
+				t.insert(t.end(), a, a + 6);
+			}
+		}
+	return pxh_scene_add_triangle_mesh(s, rows * cols, v.data(), int32_t(t.size() / 3), t.data(), pose, world);
+}
+
 int32_t pxh_scene_get_sleep_state(pxh_scene s, float* wake_counter, int32_t* asleep, int32_t n)
 {
 	if (!s || n < 0 || n > s->n)

@@ -554,3 +554,54 @@ def terrain_drop(n_scenes=1, per_side=6, layers=1, cells=16, size=16.0, amp=0.4,
     ident = np.array([0, 0, 0, 0, 0, 0, 1], np.float32)
     sc.meshes = [{"verts": verts, "tris": tris, "pose": ident, "world": w} for w in range(n_scenes)]
     return sc
+
+
+def heightfield_mesh(heights, tess=None, height_scale=1.0, row_scale=1.0, column_scale=1.0):
+    """The triangle mesh of a heightfield's cells, as pxh_scene_add_heightfield
+    builds it (Gu::HeightField::getTriangleVertexIndices): vertex row * cols +
+    col at (row * row_scale, height * height_scale, col * column_scale); a
+    cell's diagonal runs from its zeroth vertex when that sample's
+    tessellation flag is set. Returns (verts (nv,3) f32, tris (nt,3) int32)."""
+    h = np.asarray(heights, np.int16)
+    rows, cols = h.shape
+    r, c = np.meshgrid(np.arange(rows), np.arange(cols), indexing="ij")
+    verts = np.stack([r.astype(np.float32) * np.float32(row_scale), h.astype(np.float32) * np.float32(height_scale),
+                      c.astype(np.float32) * np.float32(column_scale)], -1).reshape(-1, 3).astype(np.float32)
+    tris = []
+    for rr in range(rows - 1):
+        for cc in range(cols - 1):
+            cell = rr * cols + cc
+            if tess is not None and tess[rr][cc]:
+                tris += [[cell + cols, cell, cell + cols + 1], [cell + 1, cell + cols + 1, cell]]
+            else:
+                tris += [[cell, cell + 1, cell + cols], [cell + cols + 1, cell + cols, cell + 1]]
+    return verts, np.array(tris, np.int32)
+
+
+def terrain_heights(rows=17, cols=17, amp=0.4, seed=9, height_scale=0.01):
+    """int16 samples of terrain_mesh's height function over a rows x cols grid
+    at 1 m spacing (heights in units of height_scale) and a seeded
+    checkerboard of tessellation flags"""
+    k = np.arange(6)
+    a, b, c = (uniform(seed, k[i:i + 1], 0, 0.4, 1.2)[0] for i in range(3))
+    p, q, r = (uniform(seed, k[i:i + 1], 1, 0.0, 2 * math.pi)[0] for i in range(3, 6))
+    X, Z = np.meshgrid(np.arange(rows, dtype=np.float64), np.arange(cols, dtype=np.float64), indexing="ij")
+    Y = amp * (0.6 * np.sin(a * X + p) * np.cos(b * Z + q) + 0.4 * np.sin(c * (X + Z) + r))
+    heights = np.round(Y / height_scale).astype(np.int16)
+    tess = ((np.arange(rows)[:, None] + np.arange(cols)[None, :] + seed) % 2).astype(np.uint8)
+    return heights, tess
+
+
+def heightfield_drop(n_scenes=1, per_side=6, layers=1, rows=17, cols=17, amp=0.4, seed=9) -> Scene:
+    """terrain_drop over a heightfield instead: terrain_heights samples at
+    0.01 m, 1 m rows and columns, the field centred on the origin (pose
+    (-(rows-1)/2, 0, -(cols-1)/2)); the scene's meshes carry both the
+    heightfield and its triangles"""
+    sc = terrain_drop(n_scenes, per_side, layers, 16, 16.0, amp, seed)
+    heights, tess = terrain_heights(rows, cols, amp, seed)
+    hf = {"heights": heights, "tess": tess, "height_scale": 0.01, "row_scale": 1.0, "column_scale": 1.0}
+    verts, tris = heightfield_mesh(heights, tess, 0.01, 1.0, 1.0)
+    pose = np.array([-(rows - 1) / 2, 0, -(cols - 1) / 2, 0, 0, 0, 1], np.float32)
+    sc.meshes = [{"verts": verts, "tris": tris, "pose": pose, "world": w, "heightfield": hf} for w in range(n_scenes)]
+    sc.name = "heightfield_drop"
+    return sc

@@ -230,6 +230,26 @@ def ref_trimesh(verts, tris):
     return _TRIMESH_IDS[key]
 
 
+_HF_IDS = {}
+
+
+def ref_heightfield_geom(hf):
+    """A heightfield (dict: heights (rows, cols) int16, tess or None,
+    height_scale, row_scale, column_scale) created in the reference SDK
+    (cached by content); returns its geometry row (index, height scale, row
+    scale, column scale) for code 9."""
+    h = np.ascontiguousarray(hf["heights"], np.int16)
+    t = np.ascontiguousarray(hf["tess"], np.uint8) if hf.get("tess") is not None else None
+    key = (h.tobytes(), None if t is None else t.tobytes(), h.shape)
+    if key not in _HF_IDS:
+        lib = ref()
+        lib.pxref_heightfield_create.restype = ctypes.c_int
+        hid = lib.pxref_heightfield_create(int(h.shape[0]), int(h.shape[1]), _vp(h), _vp(t) if t is not None else None)
+        assert hid >= 0
+        _HF_IDS[key] = hid
+    return np.array([_HF_IDS[key], hf["height_scale"], hf["row_scale"], hf["column_scale"]], np.float32)
+
+
 def ref_cook(points):
     """Cook a hull with the reference cooker; returns (mesh index, hull dict)."""
     lib = ref()
@@ -461,10 +481,15 @@ class RefScene:
             m = i32(members)
             assert self.lib.pxref_scene_add_aggregate(self.h, int(m.size), ip(m), int(bool(self_coll))) == 0
         for m in (getattr(scene, "meshes", None) or []):
-            mid = ref_trimesh(m["verts"], m["tris"])
             po = f32(m["pose"])
             self._keep.append(po)
-            assert self.lib.pxref_scene_add_trimesh(self.h, mid, fp(po), ctypes.c_float(0.02)) == 0
+            if "heightfield" in m:
+                g = ref_heightfield_geom(m["heightfield"])
+                self._keep.append(g)
+                assert self.lib.pxref_scene_add_heightfield(self.h, fp(g), fp(po), ctypes.c_float(0.02)) == 0
+            else:
+                mid = ref_trimesh(m["verts"], m["tris"])
+                assert self.lib.pxref_scene_add_trimesh(self.h, mid, fp(po), ctypes.c_float(0.02)) == 0
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
             arrs = [i32(j["type"]), i32(j["body0"]), i32(j["body1"]), f32(j["frame0"]), f32(j["frame1"])]

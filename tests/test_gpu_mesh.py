@@ -153,3 +153,29 @@ def test_too_many_triangles_under_a_body(cuda):
     g = P.Scene(sc)
     with pytest.raises(P.PxhError, match="128 mesh triangles"):
         g.simulate(DT)
+
+
+def test_heightfield_equals_its_triangle_mesh_and_oracle(cuda):
+    """A heightfield added through pxh_scene_add_heightfield steps bit for bit
+    like the same cells added as a triangle mesh (the library builds the
+    triangles itself), and one step of its bodies matches the oracle's
+    (constraint count, velocities within 1e-5)."""
+    sc = S.heightfield_drop(n_scenes=2, per_side=6, seed=9)
+    as_mesh = S.Scene(**{**sc.__dict__})
+    as_mesh.meshes = [{k: v for k, v in m.items() if k != "heightfield"} for m in sc.meshes]
+    a = P.Scene(sc)
+    b = P.Scene(as_mesh)
+    for _ in range(40):
+        a.step(DT)
+        b.step(DT)
+    for x, y in zip(a.bodies(), b.bodies()):
+        assert np.array_equal(x, y)
+    pose, lv, av = a.bodies()
+    st = S.Scene(**{**sc.__dict__})
+    st.pose, st.lin_vel, st.ang_vel = pose, lv, av
+    g = P.Scene(st)
+    o = L.OracleWorld(st)
+    g.step(DT)
+    o.step(DT)
+    assert g.stats()["constraints"] == o.stats()["constraints"] > 30
+    _close(g.bodies(), o.bodies(), 1e-5)
