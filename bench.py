@@ -232,6 +232,7 @@ def run_batched(args, P, scenes):
            "plain_sample": f"{len(worlds)} worlds (every {stride}th, {nb} bodies), {args.plain_steps} steps each "
                            f"after {args.plain_warmup} warm-up steps, one scene per world stepped in turn",
            "speedup_vs_plain": round(batched / plain_rate, 2),
+           "slowest_steps_ms": [[int(i) + 1, round(float(ts[i]), 3)] for i in np.argsort(ts)[::-1][:12]],
            "solve_shape": {"layout": ["auto", "colour launches", "per-world persistent"][tuning[0]],
                            "workgroups": tuning[1], "lds_bodies": tuning[2], "tuned_at_step": args.autotune_at,
                            "autotune_log_ms_per_step": [[a, b, c, round(m, 4)] for a, b, c, m in tuning_log]}}

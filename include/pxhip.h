@@ -274,9 +274,11 @@ int32_t pxh_scene_copy_contact_data(pxh_scene scene, void* data, uint32_t* nb_pa
  * solve keeps in LDS (at most 1000, 64 B each; larger worlds work in HBM);
  * tuning[3] autotuning: when positive, the next steps try every candidate
  * shape of the solve (the colour launches; where every world fits one
- * workgroup, the per-world solve at 1, 2, 4 and 8 workgroups per compute unit
- * with records in LDS or HBM) for tuning[3] steps each, timing the solve
- * with events, then keep the fastest (tuning[3] reads 0 again once done).
+ * workgroup, the per-world solve at 8, 4, 2 and 1 workgroups per compute unit
+ * with records in LDS or HBM) for tuning[3] steps each (one step when that
+ * step takes more than 1.25 times the best mean so far), timing the solve
+ * with events, then keep the fastest mean (tuning[3] reads 0 again once
+ * done).
  * The getter returns the count written; the log of the last autotuning
  * (each candidate's shape tuning[0..2] and its mean solve milliseconds per
  * step) comes from pxh_scene_get_tuning_log, which returns the candidates'

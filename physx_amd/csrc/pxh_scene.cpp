@@ -64,7 +64,9 @@ struct DBuf
 	{
 		if (n <= cap && p)
 			return hipSuccess;
-		size_t nc = std::max<size_t>(n + n / 4, 64);
+		// doubling: a growing scene reallocates (and drains the device) at
+		// most once per doubling of its counts
+		size_t nc = std::max<size_t>(2 * n, 64);
 		T* q = nullptr;
 		hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), nc * sizeof(T));
 		if (e != hipSuccess)
@@ -84,7 +86,9 @@ struct DBuf
 	{
 		if (n <= cap && p)
 			return hipSuccess;
-		size_t nc = std::max<size_t>(n + n / 4, 64);
+		// doubling: a growing scene reallocates (and drains the device) at
+		// most once per doubling of its counts
+		size_t nc = std::max<size_t>(2 * n, 64);
 		T* q = nullptr;
 		hipError_t e = hipMalloc(reinterpret_cast<void**>(&q), nc * sizeof(T));
 		if (e != hipSuccess)
@@ -1565,14 +1569,15 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 		if (s->tuneCands.empty())
 		{
 			// the colour launches, and where every world fits one workgroup the
-			// per-world solve at 1, 2, 4 and 8 workgroups per compute unit with
-			// velocity records in LDS or in HBM
+			// per-world solve at 8, 4, 2 and 1 workgroups per compute unit with
+			// velocity records in LDS or in HBM (the shapes that are usually
+			// fastest first, so that slow ones are dropped after one step)
 			int dev = 0, cus = 256;
 			(void)hipGetDevice(&dev);
 			(void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
 			s->tuneCands.push_back(pxh_scene_t::TuneCand{{PXH_SOLVE_LAYOUT_COLORS, 0, s->tuning[2]}});
 			if (d.solver_type == PXH_SOLVER_TGS && s->maxWorldBodies <= PXH_WORLD_SOLVE_MAX_BODIES)
-				for (int per : {1, 2, 4, 8})
+				for (int per : {8, 4, 2, 1})
 					for (int lds : {std::min(1000, std::max(s->maxWorldBodies, 1)), 0})
 						s->tuneCands.push_back(pxh_scene_t::TuneCand{{PXH_SOLVE_LAYOUT_WORLDS, per * cus, lds}});
 			s->tuneIdx = 0;
@@ -2132,11 +2137,19 @@ int32_t pxh_scene_fetch_results(pxh_scene s, int32_t block)
 		PXH_CHECK(hipEventElapsedTime(&ms, s->tuneEv[0], s->tuneEv[1]));
 		pxh_scene_t::TuneCand& c = s->tuneCands[s->tuneIdx];
 		c.ms += ms;
-		if (++c.steps >= s->tuning[3] && ++s->tuneIdx == s->tuneCands.size())
+		++c.steps;
+		// a candidate whose first step is more than 1.25 times the best mean so
+		// far is not tried again (each tried step is a real, timed step)
+		double bestMean = 0.0;
+		for (size_t k = 0; k < s->tuneIdx; k++)
+			if (bestMean == 0.0 || s->tuneCands[k].ms / s->tuneCands[k].steps < bestMean)
+				bestMean = s->tuneCands[k].ms / s->tuneCands[k].steps;
+		const bool slow = c.steps == 1 && bestMean > 0.0 && ms > 1.25 * bestMean;
+		if ((c.steps >= s->tuning[3] || slow) && ++s->tuneIdx == s->tuneCands.size())
 		{
 			size_t best = 0;
 			for (size_t k = 1; k < s->tuneCands.size(); k++)
-				if (s->tuneCands[k].ms < s->tuneCands[best].ms)
+				if (s->tuneCands[k].ms / s->tuneCands[k].steps < s->tuneCands[best].ms / s->tuneCands[best].steps)
 					best = k;
 			for (int i = 0; i < 3; i++)
 				s->tuning[i] = s->tuneCands[best].t[i];

@@ -238,7 +238,8 @@ def test_world_solve_single_large_world_and_free_fall(cuda):
 
 def test_autotuned_solve_keeps_every_bit(cuda):
     """Autotuning the solve's launch shape (each of the 9 candidates for 2
-    steps, then the fastest) changes no bit of the simulation: 30 steps of
+    steps, or for one when that step is 25% slower than the best mean so far,
+    then the fastest mean) changes no bit of the simulation: 30 steps of
     48 worlds equal the same steps with fixed colour launches, and the log
     holds every candidate with its measured time."""
     sc = scenes.batched_piles(n_scenes=48, nmin=8, nmax=256, seed=15)
