@@ -1524,10 +1524,6 @@ PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, cons
 	}
 	for (int f = lane; f < B.np; f += 64)
 		L.pb[f] = f4(B.pn(f), B.pd(f));
-	for (int e = lane; e < A.ne; e += 64)
-		L.ea[e] = A.edge(e);
-	for (int e = lane; e < B.ne; e += 64)
-		L.eb[e] = B.edge(e);
 	waveSync();
 	float best;
 	uint32_t bi;
@@ -1540,25 +1536,38 @@ PXH_D void convexSeparationWave(const Poly& A, const xf& x0, const Poly& B, cons
 		waveSync();
 		return;
 	}
-	// the polytopes' polygon and vertex-ref arrays in LDS too (polygon tests,
-	// witness polygons, clipping), read through the same accessors; pairs
-	// rejected by a face axis above never need them. A's vertices and planes
-	// in its own frame stay in global memory (cached): keeping them out of
-	// LDS lets four waves share a SIMD.
+	// The edges in LDS for both searches, and for separated polytopes (the
+	// polygon tests of the closest-feature search) the polygon and vertex-ref
+	// arrays too, read through the same accessors; pairs rejected by a face
+	// axis above need neither, overlapping ones no polygons. A's vertices and
+	// planes in its own frame stay in global memory (cached): keeping them
+	// out of LDS lets four waves share a SIMD.
+	for (int e = lane; e < A.ne; e += 64)
+		L.ea[e] = A.edge(e);
+	for (int e = lane; e < B.ne; e += 64)
+		L.eb[e] = B.edge(e);
 	Poly Al = A, Bl = B;
 	if (!A.box)
 	{
-		for (int f = lane; f < A.np; f += 64)
-			L.polyA[f] = A.poly[f];
-		for (int k = lane; k < 2 * A.ne; k += 64)
-			L.vrA[k] = A.vr[k];
-		Al.poly = L.polyA; Al.vr = L.vrA; Al.ed = L.ea;
+		if (sepd)
+		{
+			for (int f = lane; f < A.np; f += 64)
+				L.polyA[f] = A.poly[f];
+			for (int k = lane; k < 2 * A.ne; k += 64)
+				L.vrA[k] = A.vr[k];
+			Al.poly = L.polyA; Al.vr = L.vrA;
+		}
+		Al.ed = L.ea;
 	}
-	for (int f = lane; f < B.np; f += 64)
-		L.polyB[f] = B.poly[f];
-	for (int k = lane; k < 2 * B.ne; k += 64)
-		L.vrB[k] = B.vr[k];
-	Bl.v = L.vb; Bl.pl = L.pb; Bl.poly = L.polyB; Bl.vr = L.vrB; Bl.ed = L.eb;
+	if (sepd)
+	{
+		for (int f = lane; f < B.np; f += 64)
+			L.polyB[f] = B.poly[f];
+		for (int k = lane; k < 2 * B.ne; k += 64)
+			L.vrB[k] = B.vr[k];
+		Bl.poly = L.polyB; Bl.vr = L.vrB;
+	}
+	Bl.v = L.vb; Bl.pl = L.pb; Bl.ed = L.eb;
 	waveSync();
 	float dist = 0.0f;
 	f3 n = mk(1.0f, 0.0f, 0.0f), cA = mk(0.0f, 0.0f, 0.0f), cB = cA;
