@@ -119,7 +119,8 @@ int32_t pxh_scene_fetch_results(pxh_scene scene, int32_t block);
  * is the same). Returns the aggregate's id. */
 int32_t pxh_scene_add_aggregate(pxh_scene scene, int32_t k, const int32_t* bodies, int32_t self_collisions);
 
-/* Joints without limits, drives or break forces: PxFixedJointCreate,
+/* Joints (no break forces; limits and drives through
+ * pxh_scene_set_joint_params): PxFixedJointCreate,
  * PxSphericalJointCreate, PxRevoluteJointCreate (free twist about the joint
  * frame's x axis), PxPrismaticJointCreate (free translation along x)
  * (physx/include/extensions/PxFixedJoint.h, PxSphericalJoint.h,
@@ -135,6 +136,22 @@ int32_t pxh_scene_add_aggregate(pxh_scene scene, int32_t k, const int32_t* bodie
 enum { PXH_JOINT_FIXED = 0, PXH_JOINT_SPHERICAL = 1, PXH_JOINT_REVOLUTE = 2, PXH_JOINT_PRISMATIC = 3 };
 int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, const int32_t* body0,
                              const int32_t* body1, const float* frame0, const float* frame1);
+
+/* Limits and drives of revolute and prismatic joints (joint indices as
+ * pxh_scene_add_joints returned them), params[8*i]: flags (1 limit, 2 drive,
+ * 4 drive freespin), lower, upper (radians about / metres along the joint
+ * frame's x axis; lower >= upper locks the axis), restitution, bounce
+ * threshold, drive velocity, drive force limit (an impulse per step; may be
+ * +inf), drive gear ratio. Replaces PxRevoluteJoint::setLimit
+ * (PxJointAngularLimitPair, hard), setRevoluteJointFlag(eLIMIT_ENABLED /
+ * eDRIVE_ENABLED / eDRIVE_FREESPIN), setDriveVelocity, setDriveForceLimit,
+ * setDriveGearRatio (physx/include/extensions/PxRevoluteJoint.h:109-230)
+ * and PxPrismaticJoint::setLimit (PxJointLinearLimitPair, hard),
+ * setPrismaticJointFlag(eLIMIT_ENABLED) (PxPrismaticJoint.h:97-140). Soft
+ * limits (stiffness, damping) are not modelled. Rows: RevoluteJointSolverPrep
+ * (physxextensions/src/ExtRevoluteJoint.cpp:195-258), PrismaticJointSolverPrep
+ * (ExtPrismaticJoint.cpp:119-153). */
+int32_t pxh_scene_set_joint_params(pxh_scene scene, int32_t k, const int32_t* joints, const float* params);
 
 /* PxDirectGPUAPI::getRigidDynamicData / setRigidDynamicData
  * (physx/include/PxDirectGPUAPI.h:311,325; NpDirectGPUAPI.cpp:63-92).

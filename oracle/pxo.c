@@ -2395,16 +2395,27 @@ typedef struct {
 
 enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
 
+/* limit and drive parameters of revolute and prismatic joints: flags JP_*,
+ * limit lower / upper (angles about or distances along the joint x axis),
+ * the limit's restitution and bounce threshold, drive velocity, force limit
+ * and gear ratio (PxRevoluteJoint::setLimit / setRevoluteJointFlag /
+ * setDriveVelocity / setDriveForceLimit / setDriveGearRatio,
+ * PxPrismaticJoint::setLimit / setPrismaticJointFlag) */
+enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
+#define JOINT_MAX_ROWS 8
+
 typedef struct {
 	int type, a, b, color;
 	xf f0, f1;
+	int pflags;
+	float lo, hi, rest, bthr, dvel, dfl, gear;
 	/* prepared each step */
 	int count;
 	float invMass0, invMass1;
 	v3 rA, rB;
 	v3 o0[3], o1[3];     /* angOrthoAxis0 / angOrthoAxis1 */
 	float orecip[3], oerr[3];
-	jrow r[6];
+	jrow r[JOINT_MAX_ROWS];
 } pxo_joint;
 
 /* a static triangle mesh instance: its world and triangle range; a triangle
@@ -3317,7 +3328,8 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
 
 /* ---------------------------------------------------------------------------
  * Joints: 1D constraint rows of fixed, spherical, revolute and prismatic
- * joints without limits or drives, TGS.
+ * joints, with the hard limits and the velocity drive of revolute and
+ * prismatic joints, TGS.
  * ------------------------------------------------------------------------- */
 
 /* Px1DConstraint (physx/include/PxConstraintDesc.h:143-230), the fields the
@@ -3326,11 +3338,14 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
  * unbounded. */
 typedef struct {
 	v3 lin0, ang0, lin1, ang1;
-	float geomErr, velTarget, minImp, maxImp;
-	int hint, angular;
+	float geomErr, velTarget, minImp, maxImp, rest, bthr;
+	int hint, angular, flags;   /* flags RF_* (Px1DConstraintFlag) */
 } row1d;
 
-enum { HINT_ROT_EQ = 1024, HINT_EQ = 2048 };   /* PxConstraintSolveHint */
+/* PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110) */
+enum { HINT_NONE = 0, HINT_ROT_EQ = 1024, HINT_ROT_INEQ = 1025, HINT_EQ = 2048, HINT_INEQ = 2049 };
+/* Px1DConstraintFlag: eKEEPBIAS, eRESTITUTION */
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2 };
 
 /* Jacobian of the relative rotation qa* qb against (omega_b - omega_a)
  * (computeJacobianAxes, physxextensions/src/ExtConstraintHelper.h:77-102) */
@@ -3369,12 +3384,49 @@ static void joint_axes(int type, int* lin, int* ang)
 	}
 }
 
+/* a hard limit row (ConstraintHelper::addLimit, ExtConstraintHelper.h:
+ * 316-393): an inequality keeping its bias while the limit is not reached,
+ * impulses at least zero, bouncing when restitution is set */
+static void limit_row(row1d* c, const pxo_joint* j)
+{
+	c->hint = c->angular ? HINT_ROT_INEQ : HINT_INEQ;
+	c->rest = j->rest;
+	c->bthr = j->bthr;
+	if (c->geomErr > 0.0f)
+		c->flags |= RF_KEEP_BIAS;
+	if (j->rest > 0.0f)
+		c->flags |= RF_RESTITUTION;
+	c->minImp = 0.0f;
+}
+
+/* the twist angle of frame B about frame A's x axis (computePhi,
+ * ExtRevoluteJoint.cpp:154-179) */
+static float twist_angle(qt qa, qt qb)
+{
+	const qt r = qmul(qconj(qa), qb);
+	float x = r.x, y = 0.0f, z = 0.0f, ww = r.w;
+	const float mag = sqrtf(x * x + y * y + z * z + ww * ww);
+	if (mag != 0.0f)
+	{
+		const float im = 1.0f / mag;
+		x *= im; ww *= im;
+	}
+	float angle = acosf(fminf(fmaxf(ww, -1.0f), 1.0f)) * 2.0f;
+	if (x < 0.0f)
+		angle = -angle;
+	return angle;
+}
+
 /* The joint's solverPrep with ConstraintHelper::prepareLockedAxes
  * (ExtConstraintHelper.h:150-208, 243-297): joint frames in the world
  * (computeJointFrames, :61-73), the neighbourhood operator (:44-48), rows
  * for the locked linear axes of frame A through the anchor, rows for the
- * locked rotation axes. Returns the row count; *ra / *rb the anchor offsets
- * (cA2wOut - bA2w.p, cB2wOut - bB2w.p). */
+ * locked rotation axes; then for a revolute joint its drive row and limit
+ * pair (RevoluteJointSolverPrep, ExtRevoluteJoint.cpp:195-258: a locked
+ * limit, lower >= upper, locks the twist instead), for a prismatic joint its
+ * limit pair along x (PrismaticJointSolverPrep, ExtPrismaticJoint.cpp:
+ * 119-153), limit rows through the unshifted anchor. Returns the row count;
+ * *ra / *rb the anchor offsets (cA2wOut - bA2w.p, cB2wOut - bB2w.p). */
 static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, v3* rbOut)
 {
 	xf cA, cB;
@@ -3386,21 +3438,29 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 	{
 		cB.q.x = -cB.q.x; cB.q.y = -cB.q.y; cB.q.z = -cB.q.z; cB.q.w = -cB.q.w;
 	}
-	v3 ra = sub(cB.p, bA.p);
+	const v3 ra0 = sub(cB.p, bA.p);
+	v3 ra = ra0;
 	const v3 rb = sub(cB.p, bB.p);
 	int lin, ang;
 	joint_axes(j->type, &lin, &ang);
+	const int limitOn = (j->type == PXO_JOINT_REVOLUTE || j->type == PXO_JOINT_PRISMATIC) && (j->pflags & JP_LIMIT);
+	const int limitLocked = limitOn && j->lo >= j->hi;
+	if (limitLocked)
+	{
+		if (j->type == PXO_JOINT_REVOLUTE) ang = 7;
+		else lin = 7;
+	}
 	const v3 cB2cAp = qrotinv(cA.q, sub(cB.p, cA.p));
 	int k = 0;
-	for (int i = 0; i < 6; i++)
+	for (int i = 0; i < JOINT_MAX_ROWS; i++)
 	{
 		memset(&rows[i], 0, sizeof(row1d));
 		rows[i].minImp = -3.402823466e+38f;
 		rows[i].maxImp = 3.402823466e+38f;
 	}
+	const m33 axes = qmat(cA.q);
 	if (lin)
 	{
-		const m33 axes = qmat(cA.q);
 		v3 ev = V(0, 0, 0);
 		if (lin & 1) ev = sub(ev, scl(axes.c0, cB2cAp.x));
 		if (lin & 2) ev = sub(ev, scl(axes.c1, cB2cAp.y));
@@ -3436,24 +3496,73 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 				c->angular = 1;
 			}
 	}
+	const v3 axis = axes.c0;
+	if (j->type == PXO_JOINT_REVOLUTE && !limitLocked)
+	{
+		if (j->pflags & JP_DRIVE)
+		{
+			row1d* c = &rows[k++];
+			c->hint = HINT_NONE;
+			c->ang0 = neg(axis);
+			c->ang1 = scl(neg(axis), j->gear);
+			c->velTarget = j->dvel;
+			c->minImp = -j->dfl;
+			c->maxImp = j->dfl;
+			c->angular = 1;
+			if (j->pflags & JP_FREESPIN)
+			{
+				if (j->dvel > 0.0f) c->minImp = 0.0f;
+				if (j->dvel < 0.0f) c->maxImp = 0.0f;
+			}
+		}
+		if (limitOn)
+		{
+			const float phi = twist_angle(cA.q, cB.q);
+			row1d* c = &rows[k++];
+			c->ang0 = neg(axis); c->ang1 = neg(axis); c->angular = 1;
+			c->geomErr = -(j->lo - phi);
+			limit_row(c, j);
+			c = &rows[k++];
+			c->ang0 = axis; c->ang1 = axis; c->angular = 1;
+			c->geomErr = j->hi - phi;
+			limit_row(c, j);
+		}
+	}
+	if (j->type == PXO_JOINT_PRISMATIC && limitOn && !limitLocked)
+	{
+		const float ordinate = cB2cAp.x;
+		row1d* c = &rows[k++];
+		c->lin0 = axis; c->ang0 = cross(ra0, axis); c->lin1 = axis; c->ang1 = cross(rb, axis);
+		c->geomErr = j->hi - ordinate;
+		limit_row(c, j);
+		const v3 na = neg(axis);
+		c = &rows[k++];
+		c->lin0 = na; c->ang0 = cross(ra0, na); c->lin1 = na; c->ang1 = cross(rb, na);
+		c->geomErr = -j->lo - -ordinate;
+		limit_row(c, j);
+	}
 	*raOut = ra;
 	*rbOut = rb;
 	return k;
 }
 
 /* orthogonalize (lowleveldynamics/src/DyConstraintSetup.cpp:313-470):
- * Gram-Schmidt in the inverse-mass metric over one group of rows (all
- * equality rows here), geometric error and target velocity carried along */
-static void orthogonalize_rows(row1d** row, v3* as0, v3* as1, int count, float im0, float im1)
+ * Gram-Schmidt in the inverse-mass metric over one group of rows, each row
+ * made orthogonal to the group's first eq rows (its equality rows; the
+ * inequality rows after them are not made orthogonal to each other),
+ * geometric error and target velocity carried along */
+static void orthogonalize_rows(row1d** row, v3* as0, v3* as1, int count, int eq, float im0, float im1)
 {
-	v3 lin0m[6], ang0m[6], lin1m[6], ang1m[6], lin0[6], ang0[6], lin1[6], ang1[6];
-	float g[6], t[6];
+	v3 lin0m[JOINT_MAX_ROWS], ang0m[JOINT_MAX_ROWS], lin1m[JOINT_MAX_ROWS], ang1m[JOINT_MAX_ROWS];
+	v3 lin0[JOINT_MAX_ROWS], ang0[JOINT_MAX_ROWS], lin1[JOINT_MAX_ROWS], ang1[JOINT_MAX_ROWS];
+	float g[JOINT_MAX_ROWS], t[JOINT_MAX_ROWS];
 	for (int i = 0; i < count; i++)
 	{
 		v3 l0 = row[i]->lin0, a0 = row[i]->ang0, l1 = row[i]->lin1, a1 = row[i]->ang1;
 		float gi = row[i]->geomErr, ti = row[i]->velTarget;
 		v3 s0v = as0[i], s1v = as1[i];
-		for (int k = 0; k < i; k++)
+		const int elim = i < eq ? i : eq;
+		for (int k = 0; k < elim; k++)
 		{
 			const v3 sa = add(mulv(l1, lin1m[k]), mulv(l0, lin0m[k]));
 			const v3 sb = add(mulv(s1v, ang1m[k]), mulv(s0v, ang0m[k]));
@@ -3492,16 +3601,19 @@ static void orthogonalize_rows(row1d** row, v3* as0, v3* as1, int count, float i
 /* One joint's rows for this step: the solverPrep rows, then
  * setupSolverConstraintStep (lowleveldynamics/src/DyTGSContactPrep.cpp:
  * 1939-2164) with preprocessRows (DyConstraintSetup.cpp:492-597): angular
- * equality rows take the rotational hint, rows sorted by hint, each group
- * orthogonalized; per row the unit response, the hard-constraint solver
- * constants (compute1dConstraintSolverConstantsTGS,
- * lowleveldynamics/shared/DyCpuGpu1dConstraint.h:418-486, erp = half the
- * bias coefficient), the maximum bias (computeMaxBiasVelocityTGS, :332-363)
- * and the orthogonal-basis terms of the rotational rows. */
+ * rows take the rotational hints, rows sorted by hint, the equality and
+ * inequality groups (linear, rotational) orthogonalized; per row the unit
+ * response, the solver constants (compute1dConstraintSolverConstantsTGS,
+ * lowleveldynamics/shared/DyCpuGpu1dConstraint.h:418-486: hard rows with
+ * erp = half the bias coefficient, bouncing limit rows with their bounce
+ * velocity; computeBounceVelocity :141-159 from the pair's velocity along
+ * the row at the start of the step), the maximum bias
+ * (computeMaxBiasVelocityTGS, :332-363), the flags (raiseInternalFlagsTGS,
+ * :607-623) and the orthogonal-basis terms of the rotational equality rows. */
 static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody* sb_, float invStepDt, float invDt,
                        float erp, float lengthScale)
 {
-	row1d rows[6];
+	row1d rows[JOINT_MAX_ROWS];
 	v3 ra, rb;
 	const int n = joint_rows(j, bA, bB, rows, &ra, &rb);
 	j->count = n;
@@ -3516,9 +3628,12 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 	}
 	for (int i = 0; i < n; i++)
 		if (rows[i].angular)
-			rows[i].hint = HINT_ROT_EQ;
+		{
+			if (rows[i].hint == HINT_EQ) rows[i].hint = HINT_ROT_EQ;
+			else if (rows[i].hint == HINT_INEQ) rows[i].hint = HINT_ROT_INEQ;
+		}
 	/* insertion sort by hint (stable) */
-	row1d* sorted[6];
+	row1d* sorted[JOINT_MAX_ROWS];
 	for (int i = 0; i < n; i++)
 	{
 		int k = i;
@@ -3526,7 +3641,7 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 			sorted[k] = sorted[k - 1];
 		sorted[k] = &rows[i];
 	}
-	v3 as0[6], as1[6];
+	v3 as0[JOINT_MAX_ROWS], as1[JOINT_MAX_ROWS];
 	for (int i = 0; i < n; i++)
 	{
 		as0[i] = mmul(sa->sqrtInvI, sorted[i]->ang0);
@@ -3534,10 +3649,17 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 	}
 	for (int i = 0; i < n;)
 	{
-		const int start = i++;
-		while (i < n && sorted[i]->hint == sorted[start]->hint)
+		const int major = sorted[i]->hint >> 8, start = i++;
+		while (i < n && (sorted[i]->hint >> 8) == major)
 			i++;
-		orthogonalize_rows(sorted + start, as0 + start, as1 + start, i - start, sa->invMass, sb_->invMass);
+		if (major == 4 || major == 8)
+		{
+			int eq = start;
+			while (eq < i && (sorted[eq]->hint & 255) == 0)
+				eq++;
+			orthogonalize_rows(sorted + start, as0 + start, as1 + start, i - start, eq - start, sa->invMass,
+			                   sb_->invMass);
+		}
 	}
 	int ortho = 0;
 	for (int i = 0; i < n; i++)
@@ -3551,6 +3673,11 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 		const float linSum = len2(s->lin0) * sa->invMass + len2(s->lin1) * sb_->invMass;
 		const float resp0 = len2(as0[i]), resp1 = len2(as1[i]);
 		const float unit = resp0 + resp1 + linSum;
+		/* projectVelocity of both bodies on the row as prepared (before the
+		 * linear rows' angular terms are cleared) */
+		const float vel0 = dot(sa->origLin, s->lin0) + dot(sa->origAng, s->ang0);
+		const float vel1 = dot(sb_->origLin, s->lin1) + dot(sb_->origAng, s->ang1);
+		const float jointSpeed = vel0 - vel1;
 		if (!c->angular)
 		{
 			s->ang0 = V(0, 0, 0);
@@ -3558,11 +3685,28 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 			s->flags &= ~JF_ANGULAR_ERROR;
 		}
 		const float recip = unit <= 0.0f ? 0.0f : 1.0f / unit;
-		s->maxBias = c->angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+		float bounce = jointSpeed * -c->rest;
+		if (!((c->flags & RF_RESTITUTION) && -jointSpeed > c->bthr && bounce * c->geomErr <= 0.0f))
+			bounce = 0.0f;
 		s->velMul = -1.0f;
-		s->biasScale = -invStepDt * erp;
-		s->error = c->geomErr * s->biasScale;
-		s->velTarget = c->velTarget - s->velMul * 0.0f;
+		if (bounce != 0.0f)
+		{
+			s->maxBias = 0.0f;
+			s->biasScale = 0.0f;
+			s->error = 0.0f;
+			s->velTarget = bounce - s->velMul * 0.0f;
+		}
+		else
+		{
+			s->maxBias = c->angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+			s->biasScale = -invStepDt * erp;
+			s->error = c->geomErr * s->biasScale;
+			s->velTarget = c->velTarget - s->velMul * 0.0f;
+		}
+		if (c->flags & RF_KEEP_BIAS)
+			s->flags |= JF_KEEP_BIAS;
+		if (c->hint & 1)
+			s->flags |= JF_INEQUALITY;
 		if (c->hint == HINT_ROT_EQ)
 		{
 			s->flags |= JF_ROT_EQ;
@@ -3572,7 +3716,7 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 			j->oerr[ortho] = c->geomErr;
 			ortho++;
 		}
-		else
+		else if (c->hint & HINT_EQ)
 			s->flags |= JF_ORTHO_TARGET;
 	}
 }
@@ -3655,6 +3799,35 @@ static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int c
 	b1->lin = l1; b1->ang = a1;
 }
 
+/* limit and drive parameters of k joints (params[8 * i]: flags JP_* as a
+ * float, lower, upper, restitution, bounce threshold, drive velocity, drive
+ * force limit, drive gear ratio). Only revolute and prismatic joints take
+ * them, prismatic joints no drive. Returns 0, or -1 on a bad argument. */
+int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const float* params)
+{
+	pxo_world* w = (pxo_world*)wp;
+	for (int i = 0; i < k; i++)
+	{
+		if (joints[i] < 0 || joints[i] >= w->nj)
+			return -1;
+		const int t = w->joint[joints[i]].type;
+		const int f = (int)params[8 * i];
+		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC)
+			return -1;
+		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE)
+			return -1;
+	}
+	for (int i = 0; i < k; i++)
+	{
+		pxo_joint* j = &w->joint[joints[i]];
+		const float* q = params + 8 * i;
+		j->pflags = (int)q[0];
+		j->lo = q[1]; j->hi = q[2]; j->rest = q[3]; j->bthr = q[4];
+		j->dvel = q[5]; j->dfl = q[6]; j->gear = q[7];
+	}
+	return 0;
+}
+
 /* PxAggregate (physx/include/PxAggregate.h): the members of an aggregate
  * without self-collisions never pair with each other (the reference's
  * broadphase sees the aggregate's bounds and its own self-collision pass is
@@ -3709,6 +3882,7 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 	{
 		pxo_joint* j = &w->joint[w->nj++];
 		memset(j, 0, sizeof(*j));
+		j->gear = 1.0f;
 		j->type = type[i];
 		j->a = body0[i];
 		j->b = body1[i];

@@ -144,6 +144,7 @@ struct RefScene
 	std::vector<PxRigidDynamic*> bodies;
 	int nPlanes = 0;
 	int joints = 0;
+	std::vector<PxJoint*> jointList;
 	ContactCollector* contacts = nullptr;
 };
 
@@ -560,9 +561,55 @@ int pxref_scene_add_joints(void* h, int k, const int* type, const int* body0, co
 		}
 		if (!j)
 			return -1;
+		rs->jointList.push_back(j);
 		rs->joints++;
 	}
 	return rs->joints;
+}
+
+// Limit and drive parameters of k joints (indices in creation order):
+// params[8*i] = flags (1 limit, 2 drive, 4 drive freespin), lower, upper,
+// restitution, bounce threshold, drive velocity, drive force limit, drive
+// gear ratio. Revolute: PxRevoluteJoint::setLimit(PxJointAngularLimitPair),
+// setRevoluteJointFlag(eLIMIT_ENABLED / eDRIVE_ENABLED / eDRIVE_FREESPIN),
+// setDriveVelocity, setDriveForceLimit, setDriveGearRatio; prismatic:
+// PxPrismaticJoint::setLimit(PxJointLinearLimitPair), eLIMIT_ENABLED.
+// Returns 0 or -1.
+int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float* params)
+{
+	RefScene* rs = static_cast<RefScene*>(h);
+	for (int i = 0; i < k; i++)
+	{
+		if (joints[i] < 0 || size_t(joints[i]) >= rs->jointList.size())
+			return -1;
+		PxJoint* j = rs->jointList[size_t(joints[i])];
+		const float* q = params + 8 * i;
+		const int f = int(q[0]);
+		if (PxRevoluteJoint* r = j->is<PxRevoluteJoint>())
+		{
+			PxJointAngularLimitPair lim(q[1], q[2]);
+			lim.restitution = q[3];
+			lim.bounceThreshold = q[4];
+			r->setLimit(lim);
+			r->setRevoluteJointFlag(PxRevoluteJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
+			r->setDriveVelocity(q[5], false);
+			r->setDriveForceLimit(q[6]);
+			r->setDriveGearRatio(q[7]);
+			r->setRevoluteJointFlag(PxRevoluteJointFlag::eDRIVE_ENABLED, (f & 2) != 0);
+			r->setRevoluteJointFlag(PxRevoluteJointFlag::eDRIVE_FREESPIN, (f & 4) != 0);
+		}
+		else if (PxPrismaticJoint* p = j->is<PxPrismaticJoint>())
+		{
+			PxJointLinearLimitPair lim(gPhysics->getTolerancesScale(), q[1], q[2]);
+			lim.restitution = q[3];
+			lim.bounceThreshold = q[4];
+			p->setLimit(lim);
+			p->setPrismaticJointFlag(PxPrismaticJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
+		}
+		else if (f)
+			return -1;
+	}
+	return 0;
 }
 
 // pose: n x 7, linVel/angVel: n x 3 (any may be null)

@@ -63,7 +63,7 @@ EXPORTED = ("pxh_get_version", "pxh_last_error", "pxh_default_scene_desc", "pxh_
             "pxh_scene_add_convex_meshes", "pxh_scene_copy_contact_data", "pxh_scene_set_tuning",
             "pxh_scene_get_tuning", "pxh_scene_get_tuning_log", "pxh_bp_update_handles", "pxh_scene_get_touch_events",
             "pxh_scene_add_cooked_convex", "pxh_bp_update_noalloc",
-            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_add_aggregate",
+            "pxh_scene_set_profiling_stride", "pxh_scene_add_joints", "pxh_scene_set_joint_params", "pxh_scene_add_aggregate",
             "pxh_scene_get_sleep_state", "pxh_scene_wake_bodies", "pxh_scene_add_triangle_mesh",
             "pxh_scene_add_heightfield")
 
@@ -174,8 +174,10 @@ class Scene:
                 self.add_triangle_mesh(m["verts"], m["tris"], m["pose"], int(m["world"]))
         j = getattr(scene, "joints", None)
         if j is not None and len(j["type"]):
-            self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
-                            np.where(j["body1"] >= 0, j["body1"] + base, -1), j["frame0"], j["frame1"])
+            first = self.add_joints(j["type"], np.where(j["body0"] >= 0, j["body0"] + base, -1),
+                                    np.where(j["body1"] >= 0, j["body1"] + base, -1), j["frame0"], j["frame1"])
+            if j.get("params") is not None:
+                self.set_joint_params(first + np.arange(len(j["type"])), j["params"])
 
     def add_triangle_mesh(self, verts, tris, pose=(0, 0, 0, 0, 0, 0, 1), world: int = 0) -> int:
         """pxh_scene_add_triangle_mesh: a static triangle mesh (verts (nv,3),
@@ -215,6 +217,15 @@ class Scene:
         return _check(self.lib.pxh_scene_add_joints(self.h, int(t.size), t.ctypes.data_as(_I32),
                                                     b0.ctypes.data_as(_I32), b1.ctypes.data_as(_I32),
                                                     f0.ctypes.data_as(_F), f1.ctypes.data_as(_F)))
+
+    def set_joint_params(self, joints, params):
+        """pxh_scene_set_joint_params: params (k, 8) per joint: flags (1 limit,
+        2 drive, 4 freespin), lower, upper, restitution, bounce threshold,
+        drive velocity, drive force limit, gear ratio."""
+        jj = _i32(joints)
+        p = _f32(params).reshape(-1, 8)
+        _check(self.lib.pxh_scene_set_joint_params(self.h, int(jj.size), jj.ctypes.data_as(_I32),
+                                                   p.ctypes.data_as(_F)))
 
     def add_cooked_convex(self, verts, planes, polys, indices, com) -> int:
         """pxh_scene_add_cooked_convex: a hull as the reference cooker wrote it

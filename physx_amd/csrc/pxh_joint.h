@@ -1,7 +1,11 @@
 // Joints on the device: the 1D constraint rows of fixed, spherical, revolute
-// and prismatic joints without limits or drives, for the TGS solver. One lane
-// per joint: the joint's solverPrep with ConstraintHelper::prepareLockedAxes
-// (physxextensions/src/ExtConstraintHelper.h:150-208, 243-297), then
+// and prismatic joints, with the hard limits and the velocity drive of
+// revolute and prismatic joints, for the TGS solver. One lane per joint: the
+// joint's solverPrep with ConstraintHelper::prepareLockedAxes
+// (physxextensions/src/ExtConstraintHelper.h:150-208, 243-297), the drive
+// row and limit pair (RevoluteJointSolverPrep, ExtRevoluteJoint.cpp:195-258;
+// PrismaticJointSolverPrep, ExtPrismaticJoint.cpp:119-153; addLimit,
+// ExtConstraintHelper.h:316-393), then
 // setupSolverConstraintStep with preprocessRows / orthogonalize
 // (lowleveldynamics/src/DyTGSContactPrep.cpp:1939-2164,
 // DyConstraintSetup.cpp:313-597); each iteration solve1DStep
@@ -13,10 +17,13 @@
 // Layout per joint j (in colour order):
 //   info[j]        int4  kind, body a, body b (-1: the world), colour
 //   frame[4j..+3]  float4 frame0 p, frame0 q, frame1 p, frame1 q
-//   head[8j..+7]   float4 (rA, invMass0), (rB, invMass1),
+//   param[2j..+1]  float4 (flags JP_*, lower, upper, restitution),
+//                         (bounce threshold, drive velocity, drive force
+//                         limit, drive gear ratio)
+//   head[9j..+8]   float4 (rA, invMass0), (rB, invMass1),
 //                         (angOrthoAxis0[k], recipResponse[k]) k = 0..2,
-//                         (angOrthoAxis1[k], error[k]) k = 0..2
-//   rows[36j + 6k ..+5] float4 per row k: (lin0, error), (lin1, biasScale),
+//                         (angOrthoAxis1[k], error[k]) k = 0..2, (row count)
+//   rows[48j + 6k ..+5] float4 per row k: (lin0, error), (lin1, biasScale),
 //                         (ang0, velMultiplier), (ang1, velTarget),
 //                         (minImpulse, maxImpulse, appliedForce, maxBias),
 //                         (flags as bits, 0, 0, 0)
@@ -27,7 +34,13 @@ namespace pxh {
 
 enum { JOINT_FIXED = 0, JOINT_SPHERICAL = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
 enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
-enum { JHINT_ROT_EQ = 1024, JHINT_EQ = 2048 };   // PxConstraintSolveHint
+// PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110)
+enum { JHINT_NONE = 0, JHINT_ROT_EQ = 1024, JHINT_ROT_INEQ = 1025, JHINT_EQ = 2048, JHINT_INEQ = 2049 };
+// Px1DConstraintFlag eKEEPBIAS, eRESTITUTION
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2 };
+// joint parameter flags: limit enabled, drive enabled, drive freespin
+enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
+constexpr int JOINT_MAX_ROWS = 8;
 
 // locked linear / angular axes (bit masks x y z) of each kind
 PXH_D void jointAxes(int kind, int& lin, int& ang)
@@ -36,19 +49,47 @@ PXH_D void jointAxes(int kind, int& lin, int& ang)
 	ang = kind == JOINT_SPHERICAL ? 0 : (kind == JOINT_REVOLUTE ? 6 : 7);
 }
 
-PXH_D int jointRowCount(int kind)
-{
-	int lin, ang;
-	jointAxes(kind, lin, ang);
-	return __popc(lin) + __popc(ang);
-}
-
 struct Row1D
 {
 	f3 lin0, ang0, lin1, ang1;
-	float geomErr, velTarget, minImp, maxImp;
-	int hint, angular;
+	float geomErr, velTarget, minImp, maxImp, rest, bthr;
+	int hint, angular, flags;
 };
+
+// a hard limit row (addLimit): an inequality keeping its bias while the limit
+// is not reached, impulses at least zero, bouncing when restitution is set
+PXH_D void limitRow(Row1D& c, float rest, float bthr)
+{
+	c.hint = c.angular ? JHINT_ROT_INEQ : JHINT_INEQ;
+	c.rest = rest;
+	c.bthr = bthr;
+	if (c.geomErr > 0.0f)
+		c.flags |= RF_KEEP_BIAS;
+	if (rest > 0.0f)
+		c.flags |= RF_RESTITUTION;
+	c.minImp = 0.0f;
+}
+
+// the twist angle of frame B about frame A's x axis (computePhi,
+// ExtRevoluteJoint.cpp:154-179)
+PXH_D float twistAngle(qt qa, qt qb)
+{
+	qt ca; ca.x = -qa.x; ca.y = -qa.y; ca.z = -qa.z; ca.w = qa.w;
+	const qt r = qmul(ca, qb);
+	float x = r.x, ww = r.w;
+	const float y = 0.0f, z = 0.0f;
+	const float mag = sqrtf(x * x + y * y + z * z + ww * ww);
+	if (mag != 0.0f)
+	{
+		const float im = 1.0f / mag;
+		x *= im;
+		ww *= im;
+	}
+	float angle = acosf(fminf(fmaxf(ww, -1.0f), 1.0f)) * 2.0f;
+	if (x < 0.0f)
+		angle = -angle;
+	return angle;
+}
 
 // computeJacobianAxes (ExtConstraintHelper.h:77-102)
 PXH_D void jacobianAxes(f3 row[3], qt qa, qt qb)
@@ -70,9 +111,11 @@ PXH_D void jacobianAxes(f3 row[3], qt qa, qt qb)
 	}
 }
 
-// the joint's rows (solverPrep); returns the count, ra / rb the anchor offsets
-PXH_D int jointRows(int kind, const xf& bA, const xf& bB, const xf& f0, const xf& f1, Row1D* rows, f3& raOut,
-                    f3& rbOut)
+// the joint's rows (solverPrep); returns the count, ra / rb the anchor
+// offsets. p0 / p1: the joint's parameters (flags, lower, upper,
+// restitution), (bounce threshold, drive velocity, drive force limit, gear).
+PXH_D int jointRows(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, const xf& f0, const xf& f1,
+                    Row1D* rows, f3& raOut, f3& rbOut)
 {
 	xf cA, cB;
 	cA.q = qmul(bA.q, f0.q);
@@ -83,24 +126,37 @@ PXH_D int jointRows(int kind, const xf& bA, const xf& bB, const xf& f0, const xf
 	{
 		cB.q.x = -cB.q.x; cB.q.y = -cB.q.y; cB.q.z = -cB.q.z; cB.q.w = -cB.q.w;
 	}
-	f3 ra = cB.p - bA.p;
+	const f3 ra0 = cB.p - bA.p;
+	f3 ra = ra0;
 	const f3 rb = cB.p - bB.p;
 	int lin, ang;
 	jointAxes(kind, lin, ang);
+	const int pflags = int(p0.x);
+	const float lo = p0.y, hi = p0.z;
+	const bool limitOn = (kind == JOINT_REVOLUTE || kind == JOINT_PRISMATIC) && (pflags & JP_LIMIT);
+	const bool limitLocked = limitOn && lo >= hi;
+	if (limitLocked)
+	{
+		if (kind == JOINT_REVOLUTE)
+			ang = 7;
+		else
+			lin = 7;
+	}
 	const f3 cB2cAp = qrotinv(cA.q, cB.p - cA.p);
-	for (int i = 0; i < 6; i++)
+	for (int i = 0; i < JOINT_MAX_ROWS; i++)
 	{
 		rows[i].lin0 = rows[i].ang0 = rows[i].lin1 = rows[i].ang1 = mk(0.0f, 0.0f, 0.0f);
-		rows[i].geomErr = rows[i].velTarget = 0.0f;
+		rows[i].geomErr = rows[i].velTarget = rows[i].rest = rows[i].bthr = 0.0f;
 		rows[i].minImp = -3.402823466e+38f;
 		rows[i].maxImp = 3.402823466e+38f;
 		rows[i].hint = 0;
 		rows[i].angular = 0;
+		rows[i].flags = 0;
 	}
 	int k = 0;
+	const m33 axes = qmat(cA.q);
 	if (lin)
 	{
-		const m33 axes = qmat(cA.q);
 		f3 ev = mk(0.0f, 0.0f, 0.0f);
 		if (lin & 1) ev = ev - axes.c0 * cB2cAp.x;
 		if (lin & 2) ev = ev - axes.c1 * cB2cAp.y;
@@ -136,23 +192,71 @@ PXH_D int jointRows(int kind, const xf& bA, const xf& bB, const xf& f0, const xf
 				c.angular = 1;
 			}
 	}
+	const f3 axis = axes.c0;
+	const f3 naxis = -axis;
+	if (kind == JOINT_REVOLUTE && !limitLocked)
+	{
+		if (pflags & JP_DRIVE)
+		{
+			Row1D& c = rows[k++];
+			c.hint = JHINT_NONE;
+			c.ang0 = naxis;
+			c.ang1 = naxis * p1.w;
+			c.velTarget = p1.y;
+			c.minImp = -p1.z;
+			c.maxImp = p1.z;
+			c.angular = 1;
+			if (pflags & JP_FREESPIN)
+			{
+				if (p1.y > 0.0f) c.minImp = 0.0f;
+				if (p1.y < 0.0f) c.maxImp = 0.0f;
+			}
+		}
+		if (limitOn)
+		{
+			const float phi = twistAngle(cA.q, cB.q);
+			Row1D& c0 = rows[k++];
+			c0.ang0 = naxis; c0.ang1 = naxis; c0.angular = 1;
+			c0.geomErr = -(lo - phi);
+			limitRow(c0, p0.w, p1.x);
+			Row1D& c1 = rows[k++];
+			c1.ang0 = axis; c1.ang1 = axis; c1.angular = 1;
+			c1.geomErr = hi - phi;
+			limitRow(c1, p0.w, p1.x);
+		}
+	}
+	if (kind == JOINT_PRISMATIC && limitOn && !limitLocked)
+	{
+		const float ordinate = cB2cAp.x;
+		Row1D& c0 = rows[k++];
+		c0.lin0 = axis; c0.ang0 = cross(ra0, axis); c0.lin1 = axis; c0.ang1 = cross(rb, axis);
+		c0.geomErr = hi - ordinate;
+		limitRow(c0, p0.w, p1.x);
+		Row1D& c1 = rows[k++];
+		c1.lin0 = naxis; c1.ang0 = cross(ra0, naxis); c1.lin1 = naxis; c1.ang1 = cross(rb, naxis);
+		c1.geomErr = -lo - -ordinate;
+		limitRow(c1, p0.w, p1.x);
+	}
 	raOut = ra;
 	rbOut = rb;
 	return k;
 }
 
-// orthogonalize over one group of equality rows (Gram-Schmidt in the
-// inverse-mass metric; angular inverse-mass scales are 1)
-PXH_D void orthogonalizeRows(Row1D* const* row, f3* as0, f3* as1, int count, float im0, float im1)
+// orthogonalize over one group of rows (Gram-Schmidt in the inverse-mass
+// metric; angular inverse-mass scales are 1), each row against the group's
+// first eq (equality) rows
+PXH_D void orthogonalizeRows(Row1D* const* row, f3* as0, f3* as1, int count, int eq, float im0, float im1)
 {
-	f3 lin0m[6], ang0m[6], lin1m[6], ang1m[6], lin0[6], ang0[6], lin1[6], ang1[6];
-	float g[6], t[6];
+	f3 lin0m[JOINT_MAX_ROWS], ang0m[JOINT_MAX_ROWS], lin1m[JOINT_MAX_ROWS], ang1m[JOINT_MAX_ROWS];
+	f3 lin0[JOINT_MAX_ROWS], ang0[JOINT_MAX_ROWS], lin1[JOINT_MAX_ROWS], ang1[JOINT_MAX_ROWS];
+	float g[JOINT_MAX_ROWS], t[JOINT_MAX_ROWS];
 	for (int i = 0; i < count; i++)
 	{
 		f3 l0 = row[i]->lin0, a0 = row[i]->ang0, l1 = row[i]->lin1, a1 = row[i]->ang1;
 		float gi = row[i]->geomErr, ti = row[i]->velTarget;
 		f3 s0v = as0[i], s1v = as1[i];
-		for (int k = 0; k < i; k++)
+		const int elim = i < eq ? i : eq;
+		for (int k = 0; k < elim; k++)
 		{
 			const f3 sa = mulv(l1, lin1m[k]) + mulv(l0, lin0m[k]);
 			const f3 sb = mulv(s1v, ang1m[k]) + mulv(s0v, ang0m[k]);
@@ -189,18 +293,24 @@ PXH_D void orthogonalizeRows(Row1D* const* row, f3* as0, f3* as1, int count, flo
 
 // prepare joint j's header and rows for this step. sqrtInvI0 / sqrtInvI1:
 // world-space square-root inverse inertia of the two bodies (zero for the
-// world); invMass0 / invMass1 likewise.
-PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const xf& f1, const m33& sI0, const m33& sI1,
-                     float invMass0, float invMass1, float invStepDt, float invDt, float erp, float lengthScale,
-                     float4* head, float4* rowsOut)
+// world); invMass0 / invMass1 likewise; v0 / w0 and v1 / w1 their velocities
+// at the start of the solve (zero for the world).
+PXH_D void prepJoint(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, const xf& f0, const xf& f1,
+                     const m33& sI0, const m33& sI1, float invMass0, float invMass1, f3 v0, f3 w0, f3 v1, f3 w1,
+                     float invStepDt, float invDt, float erp, float lengthScale, float4* head, float4* rowsOut)
 {
-	Row1D rows[6];
+	Row1D rows[JOINT_MAX_ROWS];
 	f3 ra, rb;
-	const int n = jointRows(kind, bA, bB, f0, f1, rows, ra, rb);
+	const int n = jointRows(kind, p0, p1, bA, bB, f0, f1, rows, ra, rb);
 	for (int i = 0; i < n; i++)
 		if (rows[i].angular)
-			rows[i].hint = JHINT_ROT_EQ;
-	Row1D* sorted[6];
+		{
+			if (rows[i].hint == JHINT_EQ)
+				rows[i].hint = JHINT_ROT_EQ;
+			else if (rows[i].hint == JHINT_INEQ)
+				rows[i].hint = JHINT_ROT_INEQ;
+		}
+	Row1D* sorted[JOINT_MAX_ROWS];
 	for (int i = 0; i < n; i++)
 	{
 		int k = i;
@@ -208,7 +318,7 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 			sorted[k] = sorted[k - 1];
 		sorted[k] = &rows[i];
 	}
-	f3 as0[6], as1[6];
+	f3 as0[JOINT_MAX_ROWS], as1[JOINT_MAX_ROWS];
 	for (int i = 0; i < n; i++)
 	{
 		as0[i] = mmul(sI0, sorted[i]->ang0);
@@ -216,10 +326,16 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 	}
 	for (int i = 0; i < n;)
 	{
-		const int start = i++;
-		while (i < n && sorted[i]->hint == sorted[start]->hint)
+		const int major = sorted[i]->hint >> 8, start = i++;
+		while (i < n && (sorted[i]->hint >> 8) == major)
 			i++;
-		orthogonalizeRows(sorted + start, as0 + start, as1 + start, i - start, invMass0, invMass1);
+		if (major == 4 || major == 8)
+		{
+			int eq = start;
+			while (eq < i && (sorted[eq]->hint & 255) == 0)
+				eq++;
+			orthogonalizeRows(sorted + start, as0 + start, as1 + start, i - start, eq - start, invMass0, invMass1);
+		}
 	}
 	f3 o0[3], o1[3];
 	float orecip[3], oerr[3];
@@ -236,6 +352,10 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 		const float linSum = dot(c.lin0, c.lin0) * invMass0 + dot(c.lin1, c.lin1) * invMass1;
 		const float resp0 = dot(as0[i], as0[i]), resp1 = dot(as1[i], as1[i]);
 		const float unit = resp0 + resp1 + linSum;
+		// projectVelocity of both bodies on the row as prepared
+		const float vel0 = dot(v0, c.lin0) + dot(w0, c.ang0);
+		const float vel1 = dot(v1, c.lin1) + dot(w1, c.ang1);
+		const float jointSpeed = vel0 - vel1;
 		f3 ang0 = c.ang0, ang1 = c.ang1;
 		if (!c.angular)
 		{
@@ -243,11 +363,29 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 			flags &= ~JF_ANGULAR_ERROR;
 		}
 		const float recip = unit <= 0.0f ? 0.0f : 1.0f / unit;
-		const float maxBias = c.angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+		float bounce = jointSpeed * -c.rest;
+		if (!((c.flags & RF_RESTITUTION) && -jointSpeed > c.bthr && bounce * c.geomErr <= 0.0f))
+			bounce = 0.0f;
 		const float velMul = -1.0f;
-		const float biasScale = -invStepDt * erp;
-		const float error = c.geomErr * biasScale;
-		const float velTarget = c.velTarget - velMul * 0.0f;
+		float maxBias, biasScale, error, velTarget;
+		if (bounce != 0.0f)
+		{
+			maxBias = 0.0f;
+			biasScale = 0.0f;
+			error = 0.0f;
+			velTarget = bounce - velMul * 0.0f;
+		}
+		else
+		{
+			maxBias = c.angular ? invDt * 0.75f : invDt * 15.0f * lengthScale;
+			biasScale = -invStepDt * erp;
+			error = c.geomErr * biasScale;
+			velTarget = c.velTarget - velMul * 0.0f;
+		}
+		if (c.flags & RF_KEEP_BIAS)
+			flags |= JF_KEEP_BIAS;
+		if (c.hint & 1)
+			flags |= JF_INEQUALITY;
 		if (c.hint == JHINT_ROT_EQ)
 		{
 			flags |= JF_ROT_EQ;
@@ -257,7 +395,7 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 			oerr[ortho] = c.geomErr;
 			ortho++;
 		}
-		else
+		else if (c.hint & JHINT_EQ)
 			flags |= JF_ORTHO_TARGET;
 		float4* r = rowsOut + 6 * i;
 		r[0] = f4(c.lin0, error);
@@ -274,6 +412,7 @@ PXH_D void prepJoint(int kind, const xf& bA, const xf& bB, const xf& f0, const x
 		head[2 + k] = f4(o0[k], orecip[k]);
 		head[5 + k] = f4(o1[k], oerr[k]);
 	}
+	head[8] = make_float4(__int_as_float(n), 0.0f, 0.0f, 0.0f);
 }
 
 // One pass over joint rows. v0 / v1: the bodies' velocity records (linear

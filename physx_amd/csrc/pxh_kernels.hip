@@ -2626,9 +2626,10 @@ PXH_D bool jointAsleep(int4 ji, const uint32_t* sleepFlags)
 	       (ji.z < 0 || (sleepFlags[ji.z] & SLEEP_ASLEEP));
 }
 
-__global__ void k_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
-                             const float4* vel, const float4* svI0, const float4* svI1, float invStepDt, float invDt,
-                             float erp, float lengthScale, float4* head, float4* rows, const uint32_t* sleepFlags)
+__global__ void k_joint_prep(int J, const int4* info, const float4* frame, const float4* param, const float4* pos,
+                             const float4* rot, const float4* vel, const float4* rec, const float4* svI0,
+                             const float4* svI1, float invStepDt, float invDt, float erp, float lengthScale,
+                             float4* head, float4* rows, const uint32_t* sleepFlags)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= J)
@@ -2640,8 +2641,16 @@ __global__ void k_joint_prep(int J, const int4* info, const float4* frame, const
 	const m33 sI0 = bodySqrtInvI(ji.y, svI0, svI1), sI1 = bodySqrtInvI(ji.z, svI0, svI1);
 	const float im0 = ji.y >= 0 ? vel[4 * int64_t(ji.y)].w : 0.0f;
 	const float im1 = ji.z >= 0 ? vel[4 * int64_t(ji.z)].w : 0.0f;
-	prepJoint(ji.x, bA, bB, jointFrame(frame + 4 * int64_t(j)), jointFrame(frame + 4 * int64_t(j) + 2), sI0, sI1, im0,
-	          im1, invStepDt, invDt, erp, lengthScale, head + JOINT_HEAD * int64_t(j), rows + JOINT_ROWS * int64_t(j));
+	// the bodies' velocities at the start of the solve (world frame)
+	const f3 zero = mk(0.0f, 0.0f, 0.0f);
+	const f3 v0 = ji.y >= 0 ? mk(rec[PREP_REC * int64_t(ji.y) + 4]) : zero;
+	const f3 w0 = ji.y >= 0 ? mk(rec[PREP_REC * int64_t(ji.y) + 5]) : zero;
+	const f3 v1 = ji.z >= 0 ? mk(rec[PREP_REC * int64_t(ji.z) + 4]) : zero;
+	const f3 w1 = ji.z >= 0 ? mk(rec[PREP_REC * int64_t(ji.z) + 5]) : zero;
+	prepJoint(ji.x, param[JOINT_PARAM * int64_t(j)], param[JOINT_PARAM * int64_t(j) + 1], bA, bB,
+	          jointFrame(frame + 4 * int64_t(j)), jointFrame(frame + 4 * int64_t(j) + 2), sI0, sI1, im0, im1, v0, w0,
+	          v1, w1, invStepDt, invDt, erp, lengthScale, head + JOINT_HEAD * int64_t(j),
+	          rows + JOINT_ROWS * int64_t(j));
 }
 
 __global__ void k_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, float4* vel,
@@ -2675,7 +2684,8 @@ __global__ void k_joint_solve(int begin, int end, const int4* info, const float4
 		q1 = mkq(svDQ[b]);
 	}
 	const m33 sI0 = bodySqrtInvI(a, svI0, svI1), sI1 = bodySqrtInvI(b, svI0, svI1);
-	solveJoint(jointRowCount(ji.x), head + JOINT_HEAD * int64_t(j), rows + JOINT_ROWS * int64_t(j), l0, a0, dl0, da0,
+	const float4* h = head + JOINT_HEAD * int64_t(j);
+	solveJoint(__float_as_int(h[8].x), h, rows + JOINT_ROWS * int64_t(j), l0, a0, dl0, da0,
 	           q0, sI0, l1, a1, dl1, da1, q1, sI1, elapsed, conclude != 0);
 	if (a >= 0)
 	{
@@ -3117,12 +3127,12 @@ void launch_sleep_read(int n, SleepState ss, float* wc, int* asleep, hipStream_t
 {
 	PXH_LAUNCH(k_sleep_read, n, n, ss, wc, asleep);
 }
-void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* pos, const float4* rot,
-                       SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale, float4* head,
-                       float4* rows, const uint32_t* sleepFlags, hipStream_t st)
+void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* param, const float4* pos,
+                       const float4* rot, SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale,
+                       float4* head, float4* rows, const uint32_t* sleepFlags, hipStream_t st)
 {
-	PXH_LAUNCH(k_joint_prep, J, J, info, frame, pos, rot, sb.vel, sb.i0, sb.i1, invStepDt, invDt, erp, lengthScale, head,
-	           rows, sleepFlags);
+	PXH_LAUNCH(k_joint_prep, J, J, info, frame, param, pos, rot, sb.vel, sb.rec, sb.i0, sb.i1, invStepDt, invDt, erp,
+	           lengthScale, head, rows, sleepFlags);
 }
 void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
                         float elapsed, bool conclude, const uint32_t* sleepFlags, hipStream_t st)

@@ -254,10 +254,11 @@ struct pxh_scene_t
 	// the sorted body-pair keys that keep jointed bodies from colliding
 	std::vector<int> hJType, hJA, hJB;
 	std::vector<float> hJF0, hJF1;
+	std::vector<float> hJP;   // 8 limit / drive parameters per joint
 	std::vector<int> jColorStart{0};
 	int nJKeys = 0;
 	DBuf<int4> jInfo;
-	DBuf<float4> jFrame, jHead, jRows;
+	DBuf<float4> jFrame, jParam, jHead, jRows;
 	DBuf<uint64_t> jKeys;
 	// static triangle meshes, at most one per world: host copies of each mesh's
 	// world-space triangles (w of the vertices: vertex id), its broadphase
@@ -409,7 +410,7 @@ struct pxh_scene_t
 		f(prevRow); f(conRec); f(htKey); f(htVal); f(bodyNext); f(colCounts); f(gatherTmp); f(gatherIdx);
 		f(worldOrder); f(segStart); f(solveQueue); f(extLin); f(extAng); f(prevLin); f(prevAng);
 		f(evFound); f(evFoundScan); f(evMatched); f(evLost); f(evLostScan); f(evFoundKeys); f(evLostKeys);
-		f(jInfo); f(jFrame); f(jHead); f(jRows); f(jKeys); f(bpGroup);
+		f(jInfo); f(jFrame); f(jParam); f(jHead); f(jRows); f(jKeys); f(bpGroup);
 	}
 	// device bytes the scene holds (PxSimulationStatistics::gpuMemHeap)
 	int64_t deviceBytes()
@@ -1041,6 +1042,7 @@ int uploadJoints(pxh_scene_t* s)
 		s->jColorStart[c + 1] += s->jColorStart[c];
 	std::vector<int4> info(size_t(std::max(J, 1)));
 	std::vector<float4> frame(size_t(4) * std::max(J, 1));
+	std::vector<float4> param(size_t(pxh::JOINT_PARAM) * std::max(J, 1), make_float4(0.0f, 0.0f, 0.0f, 0.0f));
 	std::vector<uint64_t> keys;
 	for (int k = 0; k < J; k++)
 	{
@@ -1052,6 +1054,9 @@ int uploadJoints(pxh_scene_t* s)
 		frame[4 * k + 1] = make_float4(f0[3], f0[4], f0[5], f0[6]);
 		frame[4 * k + 2] = make_float4(f1[0], f1[1], f1[2], 0.0f);
 		frame[4 * k + 3] = make_float4(f1[3], f1[4], f1[5], f1[6]);
+		const float* jp = &s->hJP[8 * size_t(i)];
+		param[2 * k] = make_float4(jp[0], jp[1], jp[2], jp[3]);
+		param[2 * k + 1] = make_float4(jp[4], jp[5], jp[6], jp[7]);
 		if (s->hJA[i] >= 0 && s->hJB[i] >= 0)
 		{
 			const uint32_t lo = uint32_t(std::min(s->hJA[i], s->hJB[i])), hi = uint32_t(std::max(s->hJA[i], s->hJB[i]));
@@ -1065,11 +1070,13 @@ int uploadJoints(pxh_scene_t* s)
 	PXH_CHECK(hipStreamSynchronize(s->stream));
 	PXH_CHECK(s->jInfo.ensure(info.size()));
 	PXH_CHECK(s->jFrame.ensure(frame.size()));
+	PXH_CHECK(s->jParam.ensure(param.size()));
 	PXH_CHECK(s->jHead.ensure(size_t(pxh::JOINT_HEAD) * std::max(J, 1)));
 	PXH_CHECK(s->jRows.ensure(size_t(pxh::JOINT_ROWS) * std::max(J, 1)));
 	PXH_CHECK(s->jKeys.ensure(std::max<size_t>(keys.size(), 1)));
 	PXH_CHECK(hipMemcpy(s->jInfo.p, info.data(), info.size() * sizeof(int4), hipMemcpyHostToDevice));
 	PXH_CHECK(hipMemcpy(s->jFrame.p, frame.data(), frame.size() * sizeof(float4), hipMemcpyHostToDevice));
+	PXH_CHECK(hipMemcpy(s->jParam.p, param.data(), param.size() * sizeof(float4), hipMemcpyHostToDevice));
 	if (!keys.empty())
 		PXH_CHECK(hipMemcpy(s->jKeys.p, keys.data(), keys.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
 	return PXH_OK;
@@ -1106,15 +1113,50 @@ int32_t pxh_scene_add_joints(pxh_scene s, int32_t k, const int32_t* type, const 
 		s->hJB.push_back(body1[i]);
 		s->hJF0.insert(s->hJF0.end(), frame0 + 7 * i, frame0 + 7 * i + 7);
 		s->hJF1.insert(s->hJF1.end(), frame1 + 7 * i, frame1 + 7 * i + 7);
+		// no limit or drive; gear ratio 1
+		const float none[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
+		s->hJP.insert(s->hJP.end(), none, none + 8);
 	}
 	if (int r = uploadJoints(s))
 	{
 		s->hJType.resize(first); s->hJA.resize(first); s->hJB.resize(first);
-		s->hJF0.resize(7 * size_t(first)); s->hJF1.resize(7 * size_t(first));
+		s->hJF0.resize(7 * size_t(first)); s->hJF1.resize(7 * size_t(first)); s->hJP.resize(8 * size_t(first));
 		(void)uploadJoints(s);
 		return r;
 	}
 	return first;
+}
+
+int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints, const float* params)
+{
+	if (!s || k < 0 || (k > 0 && (!joints || !params)))
+		return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: bad arguments");
+	if (s->simulating)
+		return fail(PXH_ERR_STATE, "pxh_scene_set_joint_params: not allowed while simulating");
+	const int J = int(s->hJType.size());
+	for (int i = 0; i < k; i++)
+	{
+		if (joints[i] < 0 || joints[i] >= J)
+			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: joint index out of range");
+		const float* q = params + 8 * size_t(i);
+		for (int c = 0; c < 8; c++)
+			if (!std::isfinite(q[c]) && !(c == 6 && q[c] > 0.0f))
+				return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: parameter is not finite");
+		const int f = int(q[0]);
+		if (float(f) != q[0] || f < 0 || f > 7)
+			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: unknown flags");
+		const int t = s->hJType[joints[i]];
+		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC)
+			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute and prismatic joints take limits");
+		if ((f & 6) && t != PXH_JOINT_REVOLUTE)
+			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute joints take a drive");
+		if (q[3] < 0.0f || q[3] > 1.0f || q[4] < 0.0f || q[6] < 0.0f)
+			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: restitution outside [0, 1] or a negative "
+			                             "bounce threshold or force limit");
+	}
+	for (int i = 0; i < k; i++)
+		std::copy(params + 8 * size_t(i), params + 8 * size_t(i) + 8, s->hJP.begin() + 8 * size_t(joints[i]));
+	return uploadJoints(s);
 }
 
 int32_t pxh_scene_add_convex_meshes(pxh_scene s, int32_t k, const int32_t* counts, const float* points,
@@ -1332,7 +1374,7 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 			newBody[keep[i]] = i;
 		auto remap = [&](int b) { return b < 0 ? -1 : (b < int(newBody.size()) ? newBody[b] : -1); };
 		std::vector<int> t, a, b;
-		std::vector<float> f0, f1;
+		std::vector<float> f0, f1, jp;
 		for (size_t i = 0; i < s->hJType.size(); i++)
 		{
 			const int na = remap(s->hJA[i]), nb = remap(s->hJB[i]);
@@ -1343,8 +1385,9 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 			b.push_back(nb);
 			f0.insert(f0.end(), s->hJF0.begin() + 7 * i, s->hJF0.begin() + 7 * i + 7);
 			f1.insert(f1.end(), s->hJF1.begin() + 7 * i, s->hJF1.begin() + 7 * i + 7);
+			jp.insert(jp.end(), s->hJP.begin() + 8 * i, s->hJP.begin() + 8 * i + 8);
 		}
-		s->hJType.swap(t); s->hJA.swap(a); s->hJB.swap(b); s->hJF0.swap(f0); s->hJF1.swap(f1);
+		s->hJType.swap(t); s->hJA.swap(a); s->hJB.swap(b); s->hJF0.swap(f0); s->hJF1.swap(f1); s->hJP.swap(jp);
 		if (int r = uploadJoints(s))
 			return r;
 	}
@@ -1992,7 +2035,7 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// 1180-1256; erp = half the bias coefficient, DyTGSContactPrep.cpp:2018)
 	const int nJ = int(s->hJType.size());
 	if (nJ > 0)
-		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->pos.p, s->rot.p, sb, invStepDt, 1.0f / dt,
+		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->jParam.p, s->pos.p, s->rot.p, sb, invStepDt, 1.0f / dt,
 		                       0.5f * biasCoefficient, d.tolerance_length, s->jHead.p, s->jRows.p,
 		                       sleeping ? s->slFlags.p : nullptr, st);
 	// four step counters, then four per-block partial sums of the preparation

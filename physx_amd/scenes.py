@@ -96,6 +96,8 @@ class Scene:
                       "body0": np.where(j0[keep] >= 0, new[np.maximum(j0[keep], 0)], -1).astype(np.int32),
                       "body1": np.where(j1[keep] >= 0, new[np.maximum(j1[keep], 0)], -1).astype(np.int32),
                       "frame0": self.joints["frame0"][keep].copy(), "frame1": self.joints["frame1"][keep].copy()}
+            if self.joints.get("params") is not None:
+                joints["params"] = self.joints["params"][keep].copy()
         meshes = None
         if self.meshes:
             meshes = [dict(m, world=int(m["world"]) - w0) for m in self.meshes if w0 <= int(m["world"]) < w1]
@@ -420,6 +422,52 @@ def joint_singles(n_groups=64, seed=5) -> Scene:
     sc.ang_vel = np.array(av, np.float32)
     sc.joints = {"type": np.array(jt, np.int32), "body0": np.array(jb0, np.int32), "body1": np.array(jb1, np.int32),
                  "frame0": np.array(jf0, np.float32), "frame1": np.array(jf1, np.float32)}
+    return sc
+
+
+def limited_joints(n_groups=48, seed=7) -> Scene:
+    """joint_singles' independent groups with revolute (even joints) and
+    prismatic (odd) joints carrying hard limits and, on revolute joints,
+    velocity drives: revolute limits [-U(0.05,0.5), U(0.05,0.5)] rad, every
+    third of them driven at U(-3,3) rad/s with a force limit of U(0.2,3) or
+    unbounded (every fourth drive freespin, gear ratio 1.5 between two
+    bodies), prismatic limits [-U(0.01,0.1), U(0.01,0.1)] m, the limits of
+    the joints to the world as body 1 with restitution 0.5 (bounce threshold
+    0.1), every eighth joint with its limit locked (lower = upper = 0). The random starting
+    velocities drive the joints into their limits within a few steps."""
+    sc = joint_singles(n_groups, seed)
+    k = len(sc.joints["type"])
+    ids = np.arange(k)
+    kind = np.where(ids % 2 == 0, JOINT_REVOLUTE, JOINT_PRISMATIC).astype(np.int32)
+    params = np.zeros((k, 8), np.float32)
+    params[:, 7] = 1.0
+    rev = kind == JOINT_REVOLUTE
+    lo = np.where(rev, -uniform(seed, ids, 70, 0.05, 0.5), -uniform(seed, ids, 70, 0.01, 0.1))
+    hi = np.where(rev, uniform(seed, ids, 71, 0.05, 0.5), uniform(seed, ids, 71, 0.01, 0.1))
+    locked = ids % 8 == 6
+    lo[locked] = 0.0
+    hi[locked] = 0.0
+    flags = np.ones(k, np.int32)
+    drive = rev & (ids % 3 == 0) & ~locked
+    flags[drive] |= 2
+    flags[drive & (ids % 4 == 0)] |= 4
+    params[:, 0] = flags
+    params[:, 1] = lo
+    params[:, 2] = hi
+    # restitution on joints to the world as body 1 (the reference's 4-wide
+    # joint preparation projects body 1's angular velocity differently from
+    # its one-joint preparation when it computes a bounce)
+    bouncy = (ids % 4 == 2) & (sc.joints["body1"] < 0)
+    params[bouncy, 3] = 0.5
+    params[bouncy, 4] = 0.1
+    params[:, 5] = np.where(drive, uniform(seed, ids, 72, -3.0, 3.0), 0.0)
+    params[:, 6] = np.where(drive & (ids % 5 == 0), np.inf, uniform(seed, ids, 73, 0.2, 3.0))
+    both = (sc.joints["body0"] >= 0) & (sc.joints["body1"] >= 0)
+    params[drive & both, 7] = 1.5
+    sc.joints["type"] = kind
+    sc.joints["params"] = params
+    sc.name = "limited_joints"
+    sc.params = {"n_groups": n_groups, "seed": seed}
     return sc
 
 

@@ -330,6 +330,10 @@ class OracleWorld:
             r = self.lib.pxo_world_add_joints(self.h, int(arrs[0].shape[0]), ip(arrs[0]), ip(arrs[1]), ip(arrs[2]),
                                               fp(arrs[3]), fp(arrs[4]))
             assert r >= 0, "oracle rejected the joints"
+            if j.get("params") is not None:
+                jj, pp = i32(np.arange(len(j["type"]))), f32(j["params"])
+                self._keep += [jj, pp]
+                assert self.lib.pxo_world_set_joint_params(self.h, int(jj.size), ip(jj), fp(pp)) == 0
 
     def step(self, dt, n=1):
         for _ in range(n):
@@ -496,6 +500,10 @@ class RefScene:
             self._keep += arrs
             assert self.lib.pxref_scene_add_joints(self.h, int(arrs[0].shape[0]), ip(arrs[0]), ip(arrs[1]),
                                                    ip(arrs[2]), fp(arrs[3]), fp(arrs[4])) == arrs[0].shape[0]
+            if j.get("params") is not None:
+                jj, pp = i32(np.arange(len(j["type"]))), f32(j["params"])
+                self._keep += [jj, pp]
+                assert self.lib.pxref_scene_set_joint_params(self.h, int(jj.size), ip(jj), fp(pp)) == 0
         if sleep_threshold:
             assert self.lib.pxref_scene_set_sleep_threshold(self.h, ctypes.c_float(sleep_threshold)) == 0
 
