@@ -138,17 +138,17 @@ int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, co
                              const int32_t* body1, const float* frame0, const float* frame1);
 
 /* Limits and drives of revolute and prismatic joints (joint indices as
- * pxh_scene_add_joints returned them), params[8*i]: flags (1 limit, 2 drive,
+ * pxh_scene_add_joints returned them), params[10*i]: flags (1 limit, 2 drive,
  * 4 drive freespin), lower, upper (radians about / metres along the joint
  * frame's x axis; lower >= upper locks the axis), restitution, bounce
  * threshold, drive velocity, drive force limit (an impulse per step; may be
- * +inf), drive gear ratio. Replaces PxRevoluteJoint::setLimit
- * (PxJointAngularLimitPair, hard), setRevoluteJointFlag(eLIMIT_ENABLED /
+ * +inf), drive gear ratio, limit stiffness, limit damping (either positive:
+ * a soft limit, a spring acting only beyond the stops). Replaces
+ * PxRevoluteJoint::setLimit (PxJointAngularLimitPair), setRevoluteJointFlag(eLIMIT_ENABLED /
  * eDRIVE_ENABLED / eDRIVE_FREESPIN), setDriveVelocity, setDriveForceLimit,
  * setDriveGearRatio (physx/include/extensions/PxRevoluteJoint.h:109-230)
- * and PxPrismaticJoint::setLimit (PxJointLinearLimitPair, hard),
- * setPrismaticJointFlag(eLIMIT_ENABLED) (PxPrismaticJoint.h:97-140). Soft
- * limits (stiffness, damping) are not modelled. Rows: RevoluteJointSolverPrep
+ * and PxPrismaticJoint::setLimit (PxJointLinearLimitPair),
+ * setPrismaticJointFlag(eLIMIT_ENABLED) (PxPrismaticJoint.h:97-140). Rows: RevoluteJointSolverPrep
  * (physxextensions/src/ExtRevoluteJoint.cpp:195-258), PrismaticJointSolverPrep
  * (ExtPrismaticJoint.cpp:119-153). */
 int32_t pxh_scene_set_joint_params(pxh_scene scene, int32_t k, const int32_t* joints, const float* params);

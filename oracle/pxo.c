@@ -2393,12 +2393,13 @@ typedef struct {
 	int flags;           /* JF_* */
 } jrow;
 
-enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
+enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16, JF_SPRING = 32 };
 
 /* limit and drive parameters of revolute and prismatic joints: flags JP_*,
  * limit lower / upper (angles about or distances along the joint x axis),
  * the limit's restitution and bounce threshold, drive velocity, force limit
- * and gear ratio (PxRevoluteJoint::setLimit / setRevoluteJointFlag /
+ * and gear ratio, the limit's spring stiffness and damping (a soft limit
+ * when either is positive) (PxRevoluteJoint::setLimit / setRevoluteJointFlag /
  * setDriveVelocity / setDriveForceLimit / setDriveGearRatio,
  * PxPrismaticJoint::setLimit / setPrismaticJointFlag) */
 enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
@@ -2408,7 +2409,7 @@ typedef struct {
 	int type, a, b, color;
 	xf f0, f1;
 	int pflags;
-	float lo, hi, rest, bthr, dvel, dfl, gear;
+	float lo, hi, rest, bthr, dvel, dfl, gear, stiff, damp;
 	/* prepared each step */
 	int count;
 	float invMass0, invMass1;
@@ -3338,14 +3339,14 @@ static void add_pair(pxo_world* w, int i, int j, float cd, float tol)
  * unbounded. */
 typedef struct {
 	v3 lin0, ang0, lin1, ang1;
-	float geomErr, velTarget, minImp, maxImp, rest, bthr;
+	float geomErr, velTarget, minImp, maxImp, rest, bthr, stiff, damp;
 	int hint, angular, flags;   /* flags RF_* (Px1DConstraintFlag) */
 } row1d;
 
 /* PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110) */
 enum { HINT_NONE = 0, HINT_ROT_EQ = 1024, HINT_ROT_INEQ = 1025, HINT_EQ = 2048, HINT_INEQ = 2049 };
-/* Px1DConstraintFlag: eKEEPBIAS, eRESTITUTION */
-enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2 };
+/* Px1DConstraintFlag: eKEEPBIAS, eRESTITUTION, eSPRING */
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4 };
 
 /* Jacobian of the relative rotation qa* qb against (omega_b - omega_a)
  * (computeJacobianAxes, physxextensions/src/ExtConstraintHelper.h:77-102) */
@@ -3384,11 +3385,23 @@ static void joint_axes(int type, int* lin, int* ang)
 	}
 }
 
-/* a hard limit row (ConstraintHelper::addLimit, ExtConstraintHelper.h:
- * 316-393): an inequality keeping its bias while the limit is not reached,
- * impulses at least zero, bouncing when restitution is set */
+/* soft limits: a spring with the joint's stiffness or damping */
+static int soft_limit(const pxo_joint* j) { return j->damp > 0.0f || j->stiff > 0.0f; }
+
+/* a limit row (ConstraintHelper::addLimit, ExtConstraintHelper.h:316-393):
+ * soft, a spring row pushing back (no solve hint); hard, an inequality
+ * keeping its bias while the limit is not reached, bouncing when
+ * restitution is set; impulses at least zero */
 static void limit_row(row1d* c, const pxo_joint* j)
 {
+	if (soft_limit(j))
+	{
+		c->flags |= RF_SPRING;
+		c->stiff = j->stiff;
+		c->damp = j->damp;
+		c->minImp = 0.0f;
+		return;
+	}
 	c->hint = c->angular ? HINT_ROT_INEQ : HINT_INEQ;
 	c->rest = j->rest;
 	c->bthr = j->bthr;
@@ -3517,29 +3530,47 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 		}
 		if (limitOn)
 		{
+			/* anglePair (ExtConstraintHelper.h:220-231): a soft limit only
+			 * beyond its stop */
 			const float phi = twist_angle(cA.q, cB.q);
-			row1d* c = &rows[k++];
-			c->ang0 = neg(axis); c->ang1 = neg(axis); c->angular = 1;
-			c->geomErr = -(j->lo - phi);
-			limit_row(c, j);
-			c = &rows[k++];
-			c->ang0 = axis; c->ang1 = axis; c->angular = 1;
-			c->geomErr = j->hi - phi;
-			limit_row(c, j);
+			const int soft = soft_limit(j);
+			if (!soft || phi < j->lo)
+			{
+				row1d* c = &rows[k++];
+				c->ang0 = neg(axis); c->ang1 = neg(axis); c->angular = 1;
+				c->geomErr = -(j->lo - phi);
+				limit_row(c, j);
+			}
+			if (!soft || phi > j->hi)
+			{
+				row1d* c = &rows[k++];
+				c->ang0 = axis; c->ang1 = axis; c->angular = 1;
+				c->geomErr = j->hi - phi;
+				limit_row(c, j);
+			}
 		}
 	}
 	if (j->type == PXO_JOINT_PRISMATIC && limitOn && !limitLocked)
 	{
+		/* linearLimit (ExtConstraintHelper.h:203-207): a soft limit only
+		 * beyond its stop */
 		const float ordinate = cB2cAp.x;
-		row1d* c = &rows[k++];
-		c->lin0 = axis; c->ang0 = cross(ra0, axis); c->lin1 = axis; c->ang1 = cross(rb, axis);
-		c->geomErr = j->hi - ordinate;
-		limit_row(c, j);
+		const int soft = soft_limit(j);
+		if (!soft || ordinate > j->hi)
+		{
+			row1d* c = &rows[k++];
+			c->lin0 = axis; c->ang0 = cross(ra0, axis); c->lin1 = axis; c->ang1 = cross(rb, axis);
+			c->geomErr = j->hi - ordinate;
+			limit_row(c, j);
+		}
 		const v3 na = neg(axis);
-		c = &rows[k++];
-		c->lin0 = na; c->ang0 = cross(ra0, na); c->lin1 = na; c->ang1 = cross(rb, na);
-		c->geomErr = -j->lo - -ordinate;
-		limit_row(c, j);
+		if (!soft || -ordinate > -j->lo)
+		{
+			row1d* c = &rows[k++];
+			c->lin0 = na; c->ang0 = cross(ra0, na); c->lin1 = na; c->ang1 = cross(rb, na);
+			c->geomErr = -j->lo - -ordinate;
+			limit_row(c, j);
+		}
 	}
 	*raOut = ra;
 	*rbOut = rb;
@@ -3610,8 +3641,8 @@ static void orthogonalize_rows(row1d** row, v3* as0, v3* as1, int count, int eq,
  * the row at the start of the step), the maximum bias
  * (computeMaxBiasVelocityTGS, :332-363), the flags (raiseInternalFlagsTGS,
  * :607-623) and the orthogonal-basis terms of the rotational equality rows. */
-static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody* sb_, float invStepDt, float invDt,
-                       float erp, float lengthScale)
+static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody* sb_, float stepDt, float invStepDt,
+                       float invDt, float erp, float lengthScale)
 {
 	row1d rows[JOINT_MAX_ROWS];
 	v3 ra, rb;
@@ -3689,7 +3720,20 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 		if (!((c->flags & RF_RESTITUTION) && -jointSpeed > c->bthr && bounce * c->geomErr <= 0.0f))
 			bounce = 0.0f;
 		s->velMul = -1.0f;
-		if (bounce != 0.0f)
+		if (c->flags & RF_SPRING)
+		{
+			/* a force spring: implicit in the step */
+			const float a = stepDt * (stepDt * c->stiff + c->damp);
+			const float b = stepDt * (c->damp * c->velTarget);
+			const float x = 1.0f / (1.0f + a * unit);
+			s->maxBias = 3.402823466e+38f;
+			s->biasScale = -x * c->stiff * stepDt;
+			s->error = s->biasScale * c->geomErr;
+			s->velMul = -x * a;
+			s->velTarget = x * b - s->velMul * 0.0f;
+			s->flags |= JF_SPRING;
+		}
+		else if (bounce != 0.0f)
 		{
 			s->maxBias = 0.0f;
 			s->biasScale = 0.0f;
@@ -3726,7 +3770,10 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
  * linear rows' angular terms recomputed from them, the rotational basis
  * projected out of the linear rows, the error resolved so far from the
  * bodies' accumulated motion, the response recomputed, and the clamped
- * impulse applied. conclude: conclude1DStep (:2750-2780) after the row is
+ * impulse applied; spring rows (soft limits) take their velocity multiplier
+ * and constant without the response (:2640-2660) and resolve no target over
+ * the elapsed time. conclude: conclude1DStep (:2750-2780; a spring then gives
+ * no impulse in the velocity iterations) after the row is
  * solved in the last position iteration. */
 static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int conclude)
 {
@@ -3767,16 +3814,18 @@ static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int c
 		const float angScale = (c->flags & JF_ANGULAR_ERROR) ? 1.0f : 0.0f;
 		const float dAng = angScale * (dot(raXnI, b0->dang) - dot(rbXnI, b1->dang));
 		const float dLin = dot(c->lin0, raMotion) - dot(c->lin1, rbMotion);
-		const float errorChange = (dLin + dAng) - c->velTarget * elapsed;
+		const int spring = (c->flags & JF_SPRING) != 0;
+		const float motion = dLin + dAng;
+		const float errorChange = spring ? motion : motion - c->velTarget * elapsed;
 		const v3 r0 = mulv(raXnI, raXnI), r1 = mulv(rbXnI, rbXnI);
 		const float resp0 = im0 * dot(c->lin0, c->lin0) + (r0.x + r0.y + r0.z);
 		const float resp1 = im1 * dot(c->lin1, c->lin1) - -(r1.x + r1.y + r1.z);
 		const float response = resp0 + resp1;
 		const float recip = response > 0.0f ? 1.0f / response : 0.0f;
-		const float vMul = recip * c->velMul;
+		const float vMul = spring ? c->velMul : recip * c->velMul;
 		const float ub = errorChange * c->biasScale + error;
 		const float bias = fminf(fmaxf(ub, minBias), c->maxBias);
-		const float constant = recip * (bias + c->velTarget);
+		const float constant = spring ? bias + c->velTarget : recip * (bias + c->velTarget);
 		const v3 v0 = add(mulv(l0, c->lin0), mulv(a0, raXnI));
 		const v3 v1 = add(mulv(l1, c->lin1), mulv(a1, rbXnI));
 		const v3 dv = sub(v0, v1);
@@ -3794,15 +3843,24 @@ static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int c
 			c->biasScale = 0.0f;
 			c->error = 0.0f;
 		}
+		if (conclude && spring)
+		{
+			/* no spring impulse in the velocity iterations */
+			c->velMul = 0.0f;
+			c->biasScale = 0.0f;
+			c->error = 0.0f;
+			c->velTarget = 0.0f;
+		}
 	}
 	b0->lin = l0; b0->ang = a0;
 	b1->lin = l1; b1->ang = a1;
 }
 
-/* limit and drive parameters of k joints (params[8 * i]: flags JP_* as a
+/* limit and drive parameters of k joints (params[10 * i]: flags JP_* as a
  * float, lower, upper, restitution, bounce threshold, drive velocity, drive
- * force limit, drive gear ratio). Only revolute and prismatic joints take
- * them, prismatic joints no drive. Returns 0, or -1 on a bad argument. */
+ * force limit, drive gear ratio, limit stiffness, limit damping). Only
+ * revolute and prismatic joints take them, prismatic joints no drive.
+ * Returns 0, or -1 on a bad argument. */
 int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const float* params)
 {
 	pxo_world* w = (pxo_world*)wp;
@@ -3811,7 +3869,7 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 		if (joints[i] < 0 || joints[i] >= w->nj)
 			return -1;
 		const int t = w->joint[joints[i]].type;
-		const int f = (int)params[8 * i];
+		const int f = (int)params[10 * i];
 		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC)
 			return -1;
 		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE)
@@ -3820,10 +3878,10 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 	for (int i = 0; i < k; i++)
 	{
 		pxo_joint* j = &w->joint[joints[i]];
-		const float* q = params + 8 * i;
+		const float* q = params + 10 * i;
 		j->pflags = (int)q[0];
 		j->lo = q[1]; j->hi = q[2]; j->rest = q[3]; j->bthr = q[4];
-		j->dvel = q[5]; j->dfl = q[6]; j->gear = q[7];
+		j->dvel = q[5]; j->dfl = q[6]; j->gear = q[7]; j->stiff = q[8]; j->damp = q[9];
 	}
 	return 0;
 }
@@ -5010,7 +5068,8 @@ int pxo_world_step(void* wp, float dt)
 			if (joint_asleep(w, j))
 				continue;
 			prep_joint(j, j->a >= 0 ? w->pose[j->a] : id, j->b >= 0 ? w->pose[j->b] : id,
-			           j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, invStepDt, 1.0f / dt,
+			           j->a >= 0 ? &sb[j->a] : &worldBody, j->b >= 0 ? &sb[j->b] : &worldBody, stepDt, invStepDt,
+			           1.0f / dt,
 			           0.5f * biasCoefficient, d->tolerance_length);
 		}
 	}

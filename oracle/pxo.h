@@ -109,9 +109,10 @@ void  pxo_world_wake(void* w, int k, const int32_t* bodies);
 enum { PXO_JOINT_FIXED = 0, PXO_JOINT_SPHERICAL = 1, PXO_JOINT_REVOLUTE = 2, PXO_JOINT_PRISMATIC = 3 };
 int   pxo_world_add_joints(void* w, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
                            const float* frame0, const float* frame1);
-/* limit / drive parameters of revolute and prismatic joints: params[8*i] =
+/* limit / drive parameters of revolute and prismatic joints: params[10*i] =
  * flags (1 limit, 2 drive, 4 drive freespin), lower, upper, restitution,
- * bounce threshold, drive velocity, drive force limit, drive gear ratio */
+ * bounce threshold, drive velocity, drive force limit, drive gear ratio,
+ * limit stiffness, limit damping */
 int   pxo_world_set_joint_params(void* w, int k, const int32_t* joints, const float* params);
 /* Aggregate of k bodies (PxAggregate): without self-collisions its members
  * never pair with each other. Returns the aggregate id or -1. */

@@ -568,9 +568,10 @@ int pxref_scene_add_joints(void* h, int k, const int* type, const int* body0, co
 }
 
 // Limit and drive parameters of k joints (indices in creation order):
-// params[8*i] = flags (1 limit, 2 drive, 4 drive freespin), lower, upper,
+// params[10*i] = flags (1 limit, 2 drive, 4 drive freespin), lower, upper,
 // restitution, bounce threshold, drive velocity, drive force limit, drive
-// gear ratio. Revolute: PxRevoluteJoint::setLimit(PxJointAngularLimitPair),
+// gear ratio, limit stiffness, limit damping (a soft limit when either is
+// positive). Revolute: PxRevoluteJoint::setLimit(PxJointAngularLimitPair),
 // setRevoluteJointFlag(eLIMIT_ENABLED / eDRIVE_ENABLED / eDRIVE_FREESPIN),
 // setDriveVelocity, setDriveForceLimit, setDriveGearRatio; prismatic:
 // PxPrismaticJoint::setLimit(PxJointLinearLimitPair), eLIMIT_ENABLED.
@@ -583,13 +584,15 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 		if (joints[i] < 0 || size_t(joints[i]) >= rs->jointList.size())
 			return -1;
 		PxJoint* j = rs->jointList[size_t(joints[i])];
-		const float* q = params + 8 * i;
+		const float* q = params + 10 * i;
 		const int f = int(q[0]);
 		if (PxRevoluteJoint* r = j->is<PxRevoluteJoint>())
 		{
 			PxJointAngularLimitPair lim(q[1], q[2]);
 			lim.restitution = q[3];
 			lim.bounceThreshold = q[4];
+			lim.stiffness = q[8];
+			lim.damping = q[9];
 			r->setLimit(lim);
 			r->setRevoluteJointFlag(PxRevoluteJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
 			r->setDriveVelocity(q[5], false);
@@ -603,6 +606,8 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 			PxJointLinearLimitPair lim(gPhysics->getTolerancesScale(), q[1], q[2]);
 			lim.restitution = q[3];
 			lim.bounceThreshold = q[4];
+			lim.stiffness = q[8];
+			lim.damping = q[9];
 			p->setLimit(lim);
 			p->setPrismaticJointFlag(PxPrismaticJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
 		}

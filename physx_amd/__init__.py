@@ -219,11 +219,12 @@ class Scene:
                                                     f0.ctypes.data_as(_F), f1.ctypes.data_as(_F)))
 
     def set_joint_params(self, joints, params):
-        """pxh_scene_set_joint_params: params (k, 8) per joint: flags (1 limit,
-        2 drive, 4 freespin), lower, upper, restitution, bounce threshold,
-        drive velocity, drive force limit, gear ratio."""
+        """pxh_scene_set_joint_params: params (k, 10) per joint: flags (1
+        limit, 2 drive, 4 freespin), lower, upper, restitution, bounce
+        threshold, drive velocity, drive force limit, gear ratio, limit
+        stiffness, limit damping."""
         jj = _i32(joints)
-        p = _f32(params).reshape(-1, 8)
+        p = _f32(params).reshape(-1, 10)
         _check(self.lib.pxh_scene_set_joint_params(self.h, int(jj.size), jj.ctypes.data_as(_I32),
                                                    p.ctypes.data_as(_F)))
 

@@ -10,16 +10,18 @@
 // (lowleveldynamics/src/DyTGSContactPrep.cpp:1939-2164,
 // DyConstraintSetup.cpp:313-597); each iteration solve1DStep
 // (DyTGSContactPrep.cpp:2418-2747) and, after the last position iteration,
-// conclude1DStep (:2750-2780). The GPU reference's counterparts are
+// conclude1DStep (:2750-2780: springs give no impulse in the velocity
+// iterations). The GPU reference's counterparts are
 // constraintBlockPrePrep.cu:1168,1219 and solve1DBlockTGS
 // (gpusolver/src/CUDA/solverBlockTGS.cuh:744).
 //
 // Layout per joint j (in colour order):
 //   info[j]        int4  kind, body a, body b (-1: the world), colour
 //   frame[4j..+3]  float4 frame0 p, frame0 q, frame1 p, frame1 q
-//   param[2j..+1]  float4 (flags JP_*, lower, upper, restitution),
+//   param[3j..+2]  float4 (flags JP_*, lower, upper, restitution),
 //                         (bounce threshold, drive velocity, drive force
-//                         limit, drive gear ratio)
+//                         limit, drive gear ratio), (limit stiffness, limit
+//                         damping, 0, 0)
 //   head[9j..+8]   float4 (rA, invMass0), (rB, invMass1),
 //                         (angOrthoAxis0[k], recipResponse[k]) k = 0..2,
 //                         (angOrthoAxis1[k], error[k]) k = 0..2, (row count)
@@ -33,11 +35,11 @@
 namespace pxh {
 
 enum { JOINT_FIXED = 0, JOINT_SPHERICAL = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
-enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16 };
+enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16, JF_SPRING = 32 };
 // PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110)
 enum { JHINT_NONE = 0, JHINT_ROT_EQ = 1024, JHINT_ROT_INEQ = 1025, JHINT_EQ = 2048, JHINT_INEQ = 2049 };
-// Px1DConstraintFlag eKEEPBIAS, eRESTITUTION
-enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2 };
+// Px1DConstraintFlag eKEEPBIAS, eRESTITUTION, eSPRING
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4 };
 // joint parameter flags: limit enabled, drive enabled, drive freespin
 enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
 constexpr int JOINT_MAX_ROWS = 8;
@@ -52,14 +54,24 @@ PXH_D void jointAxes(int kind, int& lin, int& ang)
 struct Row1D
 {
 	f3 lin0, ang0, lin1, ang1;
-	float geomErr, velTarget, minImp, maxImp, rest, bthr;
+	float geomErr, velTarget, minImp, maxImp, rest, bthr, stiff, damp;
 	int hint, angular, flags;
 };
 
-// a hard limit row (addLimit): an inequality keeping its bias while the limit
-// is not reached, impulses at least zero, bouncing when restitution is set
-PXH_D void limitRow(Row1D& c, float rest, float bthr)
+// a limit row (addLimit): soft (stiffness or damping positive), a spring row
+// pushing back (no solve hint); hard, an inequality keeping its bias while
+// the limit is not reached, bouncing when restitution is set; impulses at
+// least zero. p2: (stiffness, damping, -, -).
+PXH_D void limitRow(Row1D& c, float rest, float bthr, float4 p2)
 {
+	if (p2.x > 0.0f || p2.y > 0.0f)
+	{
+		c.flags |= RF_SPRING;
+		c.stiff = p2.x;
+		c.damp = p2.y;
+		c.minImp = 0.0f;
+		return;
+	}
 	c.hint = c.angular ? JHINT_ROT_INEQ : JHINT_INEQ;
 	c.rest = rest;
 	c.bthr = bthr;
@@ -113,9 +125,10 @@ PXH_D void jacobianAxes(f3 row[3], qt qa, qt qb)
 
 // the joint's rows (solverPrep); returns the count, ra / rb the anchor
 // offsets. p0 / p1: the joint's parameters (flags, lower, upper,
-// restitution), (bounce threshold, drive velocity, drive force limit, gear).
-PXH_D int jointRows(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, const xf& f0, const xf& f1,
-                    Row1D* rows, f3& raOut, f3& rbOut)
+// restitution), (bounce threshold, drive velocity, drive force limit, gear),
+// p2 (limit stiffness, limit damping, -, -).
+PXH_D int jointRows(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, const xf& bB, const xf& f0,
+                    const xf& f1, Row1D* rows, f3& raOut, f3& rbOut)
 {
 	xf cA, cB;
 	cA.q = qmul(bA.q, f0.q);
@@ -146,7 +159,7 @@ PXH_D int jointRows(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, 
 	for (int i = 0; i < JOINT_MAX_ROWS; i++)
 	{
 		rows[i].lin0 = rows[i].ang0 = rows[i].lin1 = rows[i].ang1 = mk(0.0f, 0.0f, 0.0f);
-		rows[i].geomErr = rows[i].velTarget = rows[i].rest = rows[i].bthr = 0.0f;
+		rows[i].geomErr = rows[i].velTarget = rows[i].rest = rows[i].bthr = rows[i].stiff = rows[i].damp = 0.0f;
 		rows[i].minImp = -3.402823466e+38f;
 		rows[i].maxImp = 3.402823466e+38f;
 		rows[i].hint = 0;
@@ -214,28 +227,44 @@ PXH_D int jointRows(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, 
 		}
 		if (limitOn)
 		{
+			// anglePair: a soft limit only beyond its stop
 			const float phi = twistAngle(cA.q, cB.q);
-			Row1D& c0 = rows[k++];
-			c0.ang0 = naxis; c0.ang1 = naxis; c0.angular = 1;
-			c0.geomErr = -(lo - phi);
-			limitRow(c0, p0.w, p1.x);
-			Row1D& c1 = rows[k++];
-			c1.ang0 = axis; c1.ang1 = axis; c1.angular = 1;
-			c1.geomErr = hi - phi;
-			limitRow(c1, p0.w, p1.x);
+			const bool soft = p2.x > 0.0f || p2.y > 0.0f;
+			if (!soft || phi < lo)
+			{
+				Row1D& c0 = rows[k++];
+				c0.ang0 = naxis; c0.ang1 = naxis; c0.angular = 1;
+				c0.geomErr = -(lo - phi);
+				limitRow(c0, p0.w, p1.x, p2);
+			}
+			if (!soft || phi > hi)
+			{
+				Row1D& c1 = rows[k++];
+				c1.ang0 = axis; c1.ang1 = axis; c1.angular = 1;
+				c1.geomErr = hi - phi;
+				limitRow(c1, p0.w, p1.x, p2);
+			}
 		}
 	}
 	if (kind == JOINT_PRISMATIC && limitOn && !limitLocked)
 	{
+		// linearLimit: a soft limit only beyond its stop
 		const float ordinate = cB2cAp.x;
-		Row1D& c0 = rows[k++];
-		c0.lin0 = axis; c0.ang0 = cross(ra0, axis); c0.lin1 = axis; c0.ang1 = cross(rb, axis);
-		c0.geomErr = hi - ordinate;
-		limitRow(c0, p0.w, p1.x);
-		Row1D& c1 = rows[k++];
-		c1.lin0 = naxis; c1.ang0 = cross(ra0, naxis); c1.lin1 = naxis; c1.ang1 = cross(rb, naxis);
-		c1.geomErr = -lo - -ordinate;
-		limitRow(c1, p0.w, p1.x);
+		const bool soft = p2.x > 0.0f || p2.y > 0.0f;
+		if (!soft || ordinate > hi)
+		{
+			Row1D& c0 = rows[k++];
+			c0.lin0 = axis; c0.ang0 = cross(ra0, axis); c0.lin1 = axis; c0.ang1 = cross(rb, axis);
+			c0.geomErr = hi - ordinate;
+			limitRow(c0, p0.w, p1.x, p2);
+		}
+		if (!soft || -ordinate > -lo)
+		{
+			Row1D& c1 = rows[k++];
+			c1.lin0 = naxis; c1.ang0 = cross(ra0, naxis); c1.lin1 = naxis; c1.ang1 = cross(rb, naxis);
+			c1.geomErr = -lo - -ordinate;
+			limitRow(c1, p0.w, p1.x, p2);
+		}
 	}
 	raOut = ra;
 	rbOut = rb;
@@ -295,13 +324,14 @@ PXH_D void orthogonalizeRows(Row1D* const* row, f3* as0, f3* as1, int count, int
 // world-space square-root inverse inertia of the two bodies (zero for the
 // world); invMass0 / invMass1 likewise; v0 / w0 and v1 / w1 their velocities
 // at the start of the solve (zero for the world).
-PXH_D void prepJoint(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB, const xf& f0, const xf& f1,
-                     const m33& sI0, const m33& sI1, float invMass0, float invMass1, f3 v0, f3 w0, f3 v1, f3 w1,
-                     float invStepDt, float invDt, float erp, float lengthScale, float4* head, float4* rowsOut)
+PXH_D void prepJoint(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, const xf& bB, const xf& f0,
+                     const xf& f1, const m33& sI0, const m33& sI1, float invMass0, float invMass1, f3 v0, f3 w0, f3 v1,
+                     f3 w1, float stepDt, float invStepDt, float invDt, float erp, float lengthScale, float4* head,
+                     float4* rowsOut)
 {
 	Row1D rows[JOINT_MAX_ROWS];
 	f3 ra, rb;
-	const int n = jointRows(kind, p0, p1, bA, bB, f0, f1, rows, ra, rb);
+	const int n = jointRows(kind, p0, p1, p2, bA, bB, f0, f1, rows, ra, rb);
 	for (int i = 0; i < n; i++)
 		if (rows[i].angular)
 		{
@@ -366,9 +396,22 @@ PXH_D void prepJoint(int kind, float4 p0, float4 p1, const xf& bA, const xf& bB,
 		float bounce = jointSpeed * -c.rest;
 		if (!((c.flags & RF_RESTITUTION) && -jointSpeed > c.bthr && bounce * c.geomErr <= 0.0f))
 			bounce = 0.0f;
-		const float velMul = -1.0f;
+		float velMul = -1.0f;
 		float maxBias, biasScale, error, velTarget;
-		if (bounce != 0.0f)
+		if (c.flags & RF_SPRING)
+		{
+			// a force spring: implicit in the step
+			const float a = stepDt * (stepDt * c.stiff + c.damp);
+			const float b = stepDt * (c.damp * c.velTarget);
+			const float x = 1.0f / (1.0f + a * unit);
+			maxBias = 3.402823466e+38f;
+			biasScale = -x * c.stiff * stepDt;
+			error = biasScale * c.geomErr;
+			velMul = -x * a;
+			velTarget = x * b - velMul * 0.0f;
+			flags |= JF_SPRING;
+		}
+		else if (bounce != 0.0f)
 		{
 			maxBias = 0.0f;
 			biasScale = 0.0f;
@@ -470,16 +513,18 @@ PXH_D void solveJoint(int n, const float4* head, float4* rows, f3& l0, f3& a0, f
 		const float angScale = (flags & JF_ANGULAR_ERROR) ? 1.0f : 0.0f;
 		const float dAng = angScale * (dot(raXnI, dang0) - dot(rbXnI, dang1));
 		const float dLin = dot(lin0, raMotion) - dot(lin1, rbMotion);
-		const float errorChange = (dLin + dAng) - velTarget * elapsed;
+		const bool spring = (flags & JF_SPRING) != 0;
+		const float motion = dLin + dAng;
+		const float errorChange = spring ? motion : motion - velTarget * elapsed;
 		const f3 q0 = mulv(raXnI, raXnI), q1 = mulv(rbXnI, rbXnI);
 		const float resp0 = im0 * dot(lin0, lin0) + (q0.x + q0.y + q0.z);
 		const float resp1 = im1 * dot(lin1, lin1) - -(q1.x + q1.y + q1.z);
 		const float response = resp0 + resp1;
 		const float recip = response > 0.0f ? 1.0f / response : 0.0f;
-		const float vMul = recip * velMul;
+		const float vMul = spring ? velMul : recip * velMul;
 		const float ub = errorChange * biasScale + error;
 		const float bias = fminf(fmaxf(ub, minBias), maxBias);
-		const float constant = recip * (bias + velTarget);
+		const float constant = spring ? bias + velTarget : recip * (bias + velTarget);
 		const f3 v0 = mulv(l0, lin0) + mulv(a0, raXnI);
 		const f3 v1 = mulv(l1, lin1) + mulv(a1, rbXnI);
 		const f3 dv = v0 - v1;
@@ -496,6 +541,14 @@ PXH_D void solveJoint(int n, const float4* head, float4* rows, f3& l0, f3& a0, f
 		{
 			r[0].w = 0.0f;
 			r[1].w = 0.0f;
+		}
+		if (conclude && spring)
+		{
+			// no spring impulse in the velocity iterations
+			r[0].w = 0.0f;
+			r[1].w = 0.0f;
+			r[2].w = 0.0f;
+			r[3].w = 0.0f;
 		}
 	}
 }

@@ -274,7 +274,7 @@ void launch_bp_inflate(int n, const float* bounds, const float* dist, const uint
                        float4* bmax, int* scene, int* group, int* bodyStart, hipStream_t st);
 
 // joints (pxh_joint.h): float4 per joint of the prepared header and rows
-constexpr int JOINT_HEAD = 9, JOINT_ROWS = 48, JOINT_PARAM = 2;
+constexpr int JOINT_HEAD = 9, JOINT_ROWS = 48, JOINT_PARAM = 3;
 // body pairs of slots [Q, Q + min(*dP, cap))
 // whose (lower, higher) body key is in the sorted jointKeys[nKeys] lose their
 // contacts (jointed bodies do not collide)
@@ -283,8 +283,8 @@ void launch_joint_filter(int cap, int Q, const uint32_t* dP, const uint2* pairs,
 // rows of J joints from the poses at the step's start and the solver bodies
 // (joints whose bodies all sleep are skipped; sleepFlags may be null)
 void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* param, const float4* pos,
-                       const float4* rot, SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale,
-                       float4* head, float4* rows, const uint32_t* sleepFlags, hipStream_t st);
+                       const float4* rot, SolverBodies sb, float stepDt, float invStepDt, float invDt, float erp,
+                       float lengthScale, float4* head, float4* rows, const uint32_t* sleepFlags, hipStream_t st);
 // one pass over joints [begin, end) (one colour)
 void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
                         float elapsed, bool conclude, const uint32_t* sleepFlags, hipStream_t st);

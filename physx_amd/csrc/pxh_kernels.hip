@@ -2628,8 +2628,8 @@ PXH_D bool jointAsleep(int4 ji, const uint32_t* sleepFlags)
 
 __global__ void k_joint_prep(int J, const int4* info, const float4* frame, const float4* param, const float4* pos,
                              const float4* rot, const float4* vel, const float4* rec, const float4* svI0,
-                             const float4* svI1, float invStepDt, float invDt, float erp, float lengthScale,
-                             float4* head, float4* rows, const uint32_t* sleepFlags)
+                             const float4* svI1, float stepDt, float invStepDt, float invDt, float erp,
+                             float lengthScale, float4* head, float4* rows, const uint32_t* sleepFlags)
 {
 	const int j = blockIdx.x * blockDim.x + threadIdx.x;
 	if (j >= J)
@@ -2647,10 +2647,10 @@ __global__ void k_joint_prep(int J, const int4* info, const float4* frame, const
 	const f3 w0 = ji.y >= 0 ? mk(rec[PREP_REC * int64_t(ji.y) + 5]) : zero;
 	const f3 v1 = ji.z >= 0 ? mk(rec[PREP_REC * int64_t(ji.z) + 4]) : zero;
 	const f3 w1 = ji.z >= 0 ? mk(rec[PREP_REC * int64_t(ji.z) + 5]) : zero;
-	prepJoint(ji.x, param[JOINT_PARAM * int64_t(j)], param[JOINT_PARAM * int64_t(j) + 1], bA, bB,
-	          jointFrame(frame + 4 * int64_t(j)), jointFrame(frame + 4 * int64_t(j) + 2), sI0, sI1, im0, im1, v0, w0,
-	          v1, w1, invStepDt, invDt, erp, lengthScale, head + JOINT_HEAD * int64_t(j),
-	          rows + JOINT_ROWS * int64_t(j));
+	const float4* jp = param + JOINT_PARAM * int64_t(j);
+	prepJoint(ji.x, jp[0], jp[1], jp[2], bA, bB, jointFrame(frame + 4 * int64_t(j)),
+	          jointFrame(frame + 4 * int64_t(j) + 2), sI0, sI1, im0, im1, v0, w0, v1, w1, stepDt, invStepDt, invDt,
+	          erp, lengthScale, head + JOINT_HEAD * int64_t(j), rows + JOINT_ROWS * int64_t(j));
 }
 
 __global__ void k_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, float4* vel,
@@ -3128,11 +3128,11 @@ void launch_sleep_read(int n, SleepState ss, float* wc, int* asleep, hipStream_t
 	PXH_LAUNCH(k_sleep_read, n, n, ss, wc, asleep);
 }
 void launch_joint_prep(int J, const int4* info, const float4* frame, const float4* param, const float4* pos,
-                       const float4* rot, SolverBodies sb, float invStepDt, float invDt, float erp, float lengthScale,
-                       float4* head, float4* rows, const uint32_t* sleepFlags, hipStream_t st)
+                       const float4* rot, SolverBodies sb, float stepDt, float invStepDt, float invDt, float erp,
+                       float lengthScale, float4* head, float4* rows, const uint32_t* sleepFlags, hipStream_t st)
 {
-	PXH_LAUNCH(k_joint_prep, J, J, info, frame, param, pos, rot, sb.vel, sb.rec, sb.i0, sb.i1, invStepDt, invDt, erp,
-	           lengthScale, head, rows, sleepFlags);
+	PXH_LAUNCH(k_joint_prep, J, J, info, frame, param, pos, rot, sb.vel, sb.rec, sb.i0, sb.i1, stepDt, invStepDt,
+	           invDt, erp, lengthScale, head, rows, sleepFlags);
 }
 void launch_joint_solve(int begin, int end, const int4* info, const float4* head, float4* rows, SolverBodies sb,
                         float elapsed, bool conclude, const uint32_t* sleepFlags, hipStream_t st)

@@ -254,7 +254,7 @@ struct pxh_scene_t
 	// the sorted body-pair keys that keep jointed bodies from colliding
 	std::vector<int> hJType, hJA, hJB;
 	std::vector<float> hJF0, hJF1;
-	std::vector<float> hJP;   // 8 limit / drive parameters per joint
+	std::vector<float> hJP;   // 10 limit / drive parameters per joint
 	std::vector<int> jColorStart{0};
 	int nJKeys = 0;
 	DBuf<int4> jInfo;
@@ -1054,9 +1054,10 @@ int uploadJoints(pxh_scene_t* s)
 		frame[4 * k + 1] = make_float4(f0[3], f0[4], f0[5], f0[6]);
 		frame[4 * k + 2] = make_float4(f1[0], f1[1], f1[2], 0.0f);
 		frame[4 * k + 3] = make_float4(f1[3], f1[4], f1[5], f1[6]);
-		const float* jp = &s->hJP[8 * size_t(i)];
-		param[2 * k] = make_float4(jp[0], jp[1], jp[2], jp[3]);
-		param[2 * k + 1] = make_float4(jp[4], jp[5], jp[6], jp[7]);
+		const float* jp = &s->hJP[10 * size_t(i)];
+		param[3 * k] = make_float4(jp[0], jp[1], jp[2], jp[3]);
+		param[3 * k + 1] = make_float4(jp[4], jp[5], jp[6], jp[7]);
+		param[3 * k + 2] = make_float4(jp[8], jp[9], 0.0f, 0.0f);
 		if (s->hJA[i] >= 0 && s->hJB[i] >= 0)
 		{
 			const uint32_t lo = uint32_t(std::min(s->hJA[i], s->hJB[i])), hi = uint32_t(std::max(s->hJA[i], s->hJB[i]));
@@ -1114,13 +1115,13 @@ int32_t pxh_scene_add_joints(pxh_scene s, int32_t k, const int32_t* type, const 
 		s->hJF0.insert(s->hJF0.end(), frame0 + 7 * i, frame0 + 7 * i + 7);
 		s->hJF1.insert(s->hJF1.end(), frame1 + 7 * i, frame1 + 7 * i + 7);
 		// no limit or drive; gear ratio 1
-		const float none[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f};
-		s->hJP.insert(s->hJP.end(), none, none + 8);
+		const float none[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f};
+		s->hJP.insert(s->hJP.end(), none, none + 10);
 	}
 	if (int r = uploadJoints(s))
 	{
 		s->hJType.resize(first); s->hJA.resize(first); s->hJB.resize(first);
-		s->hJF0.resize(7 * size_t(first)); s->hJF1.resize(7 * size_t(first)); s->hJP.resize(8 * size_t(first));
+		s->hJF0.resize(7 * size_t(first)); s->hJF1.resize(7 * size_t(first)); s->hJP.resize(10 * size_t(first));
 		(void)uploadJoints(s);
 		return r;
 	}
@@ -1138,8 +1139,8 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 	{
 		if (joints[i] < 0 || joints[i] >= J)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: joint index out of range");
-		const float* q = params + 8 * size_t(i);
-		for (int c = 0; c < 8; c++)
+		const float* q = params + 10 * size_t(i);
+		for (int c = 0; c < 10; c++)
 			if (!std::isfinite(q[c]) && !(c == 6 && q[c] > 0.0f))
 				return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: parameter is not finite");
 		const int f = int(q[0]);
@@ -1150,12 +1151,12 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute and prismatic joints take limits");
 		if ((f & 6) && t != PXH_JOINT_REVOLUTE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute joints take a drive");
-		if (q[3] < 0.0f || q[3] > 1.0f || q[4] < 0.0f || q[6] < 0.0f)
+		if (q[3] < 0.0f || q[3] > 1.0f || q[4] < 0.0f || q[6] < 0.0f || q[8] < 0.0f || q[9] < 0.0f)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: restitution outside [0, 1] or a negative "
-			                             "bounce threshold or force limit");
+			                             "bounce threshold, force limit, stiffness or damping");
 	}
 	for (int i = 0; i < k; i++)
-		std::copy(params + 8 * size_t(i), params + 8 * size_t(i) + 8, s->hJP.begin() + 8 * size_t(joints[i]));
+		std::copy(params + 10 * size_t(i), params + 10 * size_t(i) + 10, s->hJP.begin() + 10 * size_t(joints[i]));
 	return uploadJoints(s);
 }
 
@@ -1385,7 +1386,7 @@ int32_t pxh_scene_remove_worlds(pxh_scene s, const int32_t* worlds, int32_t k)
 			b.push_back(nb);
 			f0.insert(f0.end(), s->hJF0.begin() + 7 * i, s->hJF0.begin() + 7 * i + 7);
 			f1.insert(f1.end(), s->hJF1.begin() + 7 * i, s->hJF1.begin() + 7 * i + 7);
-			jp.insert(jp.end(), s->hJP.begin() + 8 * i, s->hJP.begin() + 8 * i + 8);
+			jp.insert(jp.end(), s->hJP.begin() + 10 * i, s->hJP.begin() + 10 * i + 10);
 		}
 		s->hJType.swap(t); s->hJA.swap(a); s->hJB.swap(b); s->hJF0.swap(f0); s->hJF1.swap(f1); s->hJP.swap(jp);
 		if (int r = uploadJoints(s))
@@ -2035,7 +2036,8 @@ int32_t pxh_scene_simulate(pxh_scene s, float dt)
 	// 1180-1256; erp = half the bias coefficient, DyTGSContactPrep.cpp:2018)
 	const int nJ = int(s->hJType.size());
 	if (nJ > 0)
-		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->jParam.p, s->pos.p, s->rot.p, sb, invStepDt, 1.0f / dt,
+		pxh::launch_joint_prep(nJ, s->jInfo.p, s->jFrame.p, s->jParam.p, s->pos.p, s->rot.p, sb, stepDt, invStepDt,
+		                       1.0f / dt,
 		                       0.5f * biasCoefficient, d.tolerance_length, s->jHead.p, s->jRows.p,
 		                       sleeping ? s->slFlags.p : nullptr, st);
 	// four step counters, then four per-block partial sums of the preparation

@@ -433,13 +433,15 @@ def limited_joints(n_groups=48, seed=7) -> Scene:
     unbounded (every fourth drive freespin, gear ratio 1.5 between two
     bodies), prismatic limits [-U(0.01,0.1), U(0.01,0.1)] m, the limits of
     the joints to the world as body 1 with restitution 0.5 (bounce threshold
-    0.1), every eighth joint with its limit locked (lower = upper = 0). The random starting
+    0.1), every eighth joint with its limit locked (lower = upper = 0), every
+    fifth otherwise soft (stiffness 20 N m/rad, damping 0.5 revolute; 200 N/m,
+    5 prismatic). The random starting
     velocities drive the joints into their limits within a few steps."""
     sc = joint_singles(n_groups, seed)
     k = len(sc.joints["type"])
     ids = np.arange(k)
     kind = np.where(ids % 2 == 0, JOINT_REVOLUTE, JOINT_PRISMATIC).astype(np.int32)
-    params = np.zeros((k, 8), np.float32)
+    params = np.zeros((k, 10), np.float32)
     params[:, 7] = 1.0
     rev = kind == JOINT_REVOLUTE
     lo = np.where(rev, -uniform(seed, ids, 70, 0.05, 0.5), -uniform(seed, ids, 70, 0.01, 0.1))
@@ -464,6 +466,10 @@ def limited_joints(n_groups=48, seed=7) -> Scene:
     params[:, 6] = np.where(drive & (ids % 5 == 0), np.inf, uniform(seed, ids, 73, 0.2, 3.0))
     both = (sc.joints["body0"] >= 0) & (sc.joints["body1"] >= 0)
     params[drive & both, 7] = 1.5
+    # soft limits (springs beyond the stops) on every fifth joint
+    soft = (ids % 5 == 4) & ~locked & ~bouncy
+    params[soft, 8] = np.where(rev[soft], 20.0, 200.0)
+    params[soft, 9] = np.where(rev[soft], 0.5, 5.0)
     sc.joints["type"] = kind
     sc.joints["params"] = params
     sc.name = "limited_joints"

@@ -51,7 +51,7 @@ def _twist(pose_q, f0_q, f1_q):
 @pytest.mark.parametrize("seed", [7, 8])
 def test_limits_and_drives_vs_live_reference(seed):
     """48 independent revolute and prismatic joints with hard limits (some
-    locked, some with restitution 0.5), velocity drives (force-limited,
+    locked, some with restitution 0.5) or soft ones, velocity drives (force-limited,
     unbounded, freespin, gear ratio 1.5): along 60 steps of the reference's
     trajectory the oracle's step from the reference's state matches the
     reference's next state, velocities within 3e-4 of max(|v|, g dt) (and of
@@ -90,7 +90,7 @@ def test_revolute_limit_holds():
     f0 = np.concatenate([[-0.5, 0.0, 0.0], f1[3:]])                   # anchor 0.5 m to the body's -x
     pose = [0.5, 2.0, 0.0, 0, 0, 0, 1]
     for lim_on, lo, hi in ((1, -0.3, 0.2), (0, -0.3, 0.2)):
-        sc = _one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], pose, f0, f1, [lim_on, lo, hi, 0, 0, 0, 0, 1])
+        sc = _one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], pose, f0, f1, [lim_on, lo, hi, 0, 0, 0, 0, 1, 0, 0])
         o = L.OracleWorld(sc)
         angles = []
         for _ in range(240):
@@ -108,11 +108,11 @@ def _drive_cases():
     f = np.concatenate([[0, 1, 0], _q([0, 1, 0], 0.0)])
     body = [0, 1, 0, 0, 0, 0, 1]
     return [_one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
-                       [2, 0, 0, 0, 0, 2.0, np.inf, 1]),
+                       [2, 0, 0, 0, 0, 2.0, np.inf, 1, 0, 0]),
             _one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
-                       [2, 0, 0, 0, 0, 2.0, 0.001, 1]),
+                       [2, 0, 0, 0, 0, 2.0, 0.001, 1, 0, 0]),
             _one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
-                       [6, 0, 0, 0, 0, -2.0, np.inf, 1], av=(3.0, 0, 0))]
+                       [6, 0, 0, 0, 0, -2.0, np.inf, 1, 0, 0], av=(3.0, 0, 0))]
 
 
 def test_drive_reaches_its_velocity():
@@ -149,18 +149,18 @@ def test_prismatic_limit_and_locked_limits():
     up = _q([0, 0, 1], np.pi / 2)       # frame x axis = world y
     f = np.concatenate([[0, 1, 0], up])
     sc = _one_joint(S.JOINT_PRISMATIC, S.BOX, [0.1, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1], np.concatenate([[0, 0, 0], up]),
-                    f, [1, -0.2, 0.3, 0, 0, 0, 0, 1])
+                    f, [1, -0.2, 0.3, 0, 0, 0, 0, 1, 0, 0])
     o = L.OracleWorld(sc)
     o.step(DT, 120)
     assert abs(o.bodies()[0][0, 1] - 0.7) <= 1e-3, o.bodies()[0][0]
     sc = _one_joint(S.JOINT_PRISMATIC, S.BOX, [0.1, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1], np.concatenate([[0, 0, 0], up]),
-                    f, [1, 0.0, 0.0, 0, 0, 0, 0, 1])
+                    f, [1, 0.0, 0.0, 0, 0, 0, 0, 1, 0, 0])
     o = L.OracleWorld(sc)
     o.step(DT, 120)
     assert abs(o.bodies()[0][0, 1] - 1.0) <= 1e-3
     fx = np.concatenate([[0, 1, 0], _q([0, 1, 0], 0.0)])
     sc = _one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, 1], fx,
-                    [1, 0.1, 0.1, 0, 0, 0, 0, 1], av=(3.0, 0, 0))
+                    [1, 0.1, 0.1, 0, 0, 0, 0, 1, 0, 0], av=(3.0, 0, 0))
     o = L.OracleWorld(sc, gravity=(0.0, 0.0, 0.0))
     o.step(DT, 60)
     assert abs(_twist(o.bodies()[0][0, 3:], [0, 0, 0, 1], fx[3:])) <= 2e-3
@@ -169,9 +169,9 @@ def test_prismatic_limit_and_locked_limits():
 @need_ref
 def test_locked_and_prismatic_cases_vs_reference():
     """The slider (here with restitution 0.3 at its stops and a bounce
-    threshold of 0.5 m/s), the locked revolute joint and the three drives
-    above also match the reference SDK step by step (positions within 1e-4 m
-    over 120 steps, with gravity). A resting limit approaches its stop at
+    threshold of 0.5 m/s), the locked revolute joint, the three drives above
+    and the hinged box on a soft limit also match the reference SDK step by
+    step (positions within 1e-4 m over 120 steps, with gravity). A resting limit approaches its stop at
     g dt = 0.16 m/s each step and sits on it to rounding, so with a bounce
     threshold below that speed whether a step bounces turns on the sign of a
     rounding-sized error: the reference's and the oracle's resting bounces
@@ -179,10 +179,16 @@ def test_locked_and_prismatic_cases_vs_reference():
     up = _q([0, 0, 1], np.pi / 2)
     f = np.concatenate([[0, 1, 0], up])
     cases = [_one_joint(S.JOINT_PRISMATIC, S.BOX, [0.1, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1],
-                        np.concatenate([[0, 0, 0], up]), f, [1, -0.2, 0.3, 0.3, 0.5, 0, 0, 1])]
+                        np.concatenate([[0, 0, 0], up]), f, [1, -0.2, 0.3, 0.3, 0.5, 0, 0, 1, 0, 0])]
     fx = np.concatenate([[0, 1, 0], _q([0, 1, 0], 0.0)])
     cases.append(_one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1], [0, 0, 0, 0, 0, 0, 1],
-                            fx, [1, 0.1, 0.1, 0, 0, 0, 0, 1], av=(3.0, 1.0, 0)))
+                            fx, [1, 0.1, 0.1, 0, 0, 0, 0, 1, 0, 0], av=(3.0, 1.0, 0)))
+    # the hinged box of test_revolute_limit_holds on a soft limit (a spring
+    # of 20 N m/rad, damping 0.5, beyond the stops)
+    f1 = np.concatenate([[0.0, 2.0, 0.0], _q([0, 1, 0], -np.pi / 2)])
+    f0 = np.concatenate([[-0.5, 0.0, 0.0], f1[3:]])
+    cases.append(_one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], [0.5, 2.0, 0.0, 0, 0, 0, 1], f0, f1,
+                            [1, -0.3, 0.2, 0, 0, 0, 0, 1, 20.0, 0.5]))
     for sc in cases + _drive_cases():
         o = L.OracleWorld(sc)
         r = L.RefScene(sc, "tgs", 4, 1, threads=1)

@@ -58,10 +58,10 @@ def test_single_joint_cases_track_oracle(cuda):
     f0 = np.concatenate([[-0.5, 0.0, 0.0], f1[3:]])
     up = _q([0, 0, 1], np.pi / 2)
     cases = [(_one_joint(S.JOINT_REVOLUTE, S.BOX, [0.3, 0.1, 0.1, 0], [0.5, 2.0, 0.0, 0, 0, 0, 1], f0, f1,
-                         [1, -0.3, 0.2, 0, 0, 0, 0, 1]), (0.0, -9.81, 0.0)),
+                         [1, -0.3, 0.2, 0, 0, 0, 0, 1, 0, 0]), (0.0, -9.81, 0.0)),
              (_one_joint(S.JOINT_PRISMATIC, S.BOX, [0.1, 0.1, 0.1, 0], [0, 1, 0, 0, 0, 0, 1],
                          np.concatenate([[0, 0, 0], up]), np.concatenate([[0, 1, 0], up]),
-                         [1, -0.2, 0.3, 0, 0, 0, 0, 1]), (0.0, -9.81, 0.0))]
+                         [1, -0.2, 0.3, 0, 0, 0, 0, 1, 0, 0]), (0.0, -9.81, 0.0))]
     cases += [(c, (0.0, 0.0, 0.0)) for c in _drive_cases()]
     for k, (sc, grav) in enumerate(cases):
         _track(sc, 120, 1e-5, grav)
@@ -74,12 +74,12 @@ def test_joint_params_argument_checks(cuda):
     fixed = int(np.where(kinds == S.JOINT_FIXED)[0][0])
     prism = int(np.where(kinds == S.JOINT_PRISMATIC)[0][0])
     with pytest.raises(P.PxhError, match="only revolute and prismatic"):
-        g.set_joint_params([fixed], [[1, -0.1, 0.1, 0, 0, 0, 0, 1]])
+        g.set_joint_params([fixed], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="only revolute joints take a drive"):
-        g.set_joint_params([prism], [[2, 0, 0, 0, 0, 1, 1, 1]])
+        g.set_joint_params([prism], [[2, 0, 0, 0, 0, 1, 1, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="out of range"):
-        g.set_joint_params([len(kinds)], [[1, -0.1, 0.1, 0, 0, 0, 0, 1]])
+        g.set_joint_params([len(kinds)], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="unknown flags"):
-        g.set_joint_params([prism], [[9, -0.1, 0.1, 0, 0, 0, 0, 1]])
-    g.set_joint_params([prism], [[1, -0.1, 0.1, 0, 0, 0, 0, 1]])
+        g.set_joint_params([prism], [[9, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
+    g.set_joint_params([prism], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.step(DT)
