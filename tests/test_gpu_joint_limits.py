@@ -83,3 +83,50 @@ def test_joint_params_argument_checks(cuda):
         g.set_joint_params([prism], [[9, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.set_joint_params([prism], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.step(DT)
+
+
+def _limited_chains():
+    """jointed_chains with limits on every revolute (±0.3 rad, soft on every
+    third) and prismatic (±0.05 m) joint and a drive on every fourth revolute"""
+    sc = S.jointed_chains(n_scenes=8, links=12, drops=6, seed=3)
+    k = len(sc.joints["type"])
+    p = np.zeros((k, 10), np.float32)
+    p[:, 7] = 1.0
+    rev = sc.joints["type"] == S.JOINT_REVOLUTE
+    pri = sc.joints["type"] == S.JOINT_PRISMATIC
+    ids = np.arange(k)
+    p[rev, 0] = 1
+    p[rev, 1], p[rev, 2] = -0.3, 0.3
+    p[rev & (ids % 3 == 0), 8] = 50.0
+    p[rev & (ids % 3 == 0), 9] = 1.0
+    p[rev & (ids % 4 == 1), 0] = 3
+    p[rev & (ids % 4 == 1), 5] = 1.5
+    p[rev & (ids % 4 == 1), 6] = 2.0
+    p[pri, 0] = 1
+    p[pri, 1], p[pri, 2] = -0.05, 0.05
+    sc.joints["params"] = p
+    return sc
+
+
+def test_limited_world_alone_in_batch_and_after_removal(cuda):
+    """Jointed chains with limits (hard and soft) and drives: a world's bits
+    stepped alone equal its slice of a batch of eight, also after another
+    world is removed (joint parameters renumbered with the joints)."""
+    sc = _limited_chains()
+    assert (sc.joints["params"][:, 0] > 0).any()
+    alone = P.Scene(sc.subset(5), position_iterations=4, velocity_iterations=1)
+    batch = P.Scene(sc, position_iterations=4, velocity_iterations=1)
+    batch.set_tuning(layout=P.PXH_SOLVE_LAYOUT_COLORS)
+    for _ in range(20):
+        alone.step(DT)
+        batch.step(DT)
+    f, c = batch.world_range(5)
+    for x, y in zip(alone.bodies(), batch.bodies()):
+        assert np.array_equal(x, y[f:f + c])
+    batch.remove_worlds([2])
+    f2, c2 = batch.world_range(4)
+    for _ in range(20):
+        alone.step(DT)
+        batch.step(DT)
+    for x, y in zip(alone.bodies(), batch.bodies()):
+        assert np.array_equal(x, y[f2:f2 + c2])
