@@ -137,13 +137,17 @@ enum { PXH_JOINT_FIXED = 0, PXH_JOINT_SPHERICAL = 1, PXH_JOINT_REVOLUTE = 2, PXH
 int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, const int32_t* body0,
                              const int32_t* body1, const float* frame0, const float* frame1);
 
-/* Limits and drives of revolute and prismatic joints (joint indices as
+/* Limits and drives of revolute, prismatic and spherical joints (joint indices as
  * pxh_scene_add_joints returned them), params[10*i]: flags (1 limit, 2 drive,
  * 4 drive freespin), lower, upper (radians about / metres along the joint
  * frame's x axis; lower >= upper locks the axis), restitution, bounce
  * threshold, drive velocity, drive force limit (an impulse per step; may be
  * +inf), drive gear ratio, limit stiffness, limit damping (either positive:
- * a soft limit, a spring acting only beyond the stops). Replaces
+ * a soft limit, a spring acting only beyond the stops); for a spherical
+ * joint lower / upper are the cone's y and z angles
+ * (PxSphericalJoint::setLimitCone(PxJointLimitCone), eLIMIT_ENABLED,
+ * PxSphericalJoint.h:97-120; SphericalJointSolverPrep,
+ * ExtSphericalJoint.cpp:157-190). Replaces
  * PxRevoluteJoint::setLimit (PxJointAngularLimitPair), setRevoluteJointFlag(eLIMIT_ENABLED /
  * eDRIVE_ENABLED / eDRIVE_FREESPIN), setDriveVelocity, setDriveForceLimit,
  * setDriveGearRatio (physx/include/extensions/PxRevoluteJoint.h:109-230)

@@ -3430,6 +3430,68 @@ static float twist_angle(qt qa, qt qb)
 	return angle;
 }
 
+/* the closest point of the ellipse with radii (ry, rz) in the y-z plane to
+ * point (PxEllipseClamp, physx/include/foundation/PxMathUtils.h:204-262:
+ * Lagrange multiplier, Newton steps from Eberly's initial estimate) */
+static v3 ellipse_clamp(v3 point, float ry, float rz)
+{
+	const float qy = fabsf(point.y), qz = fabsf(point.z);
+	const float tinyEps = 1e-6f;
+	if (ry >= rz)
+	{
+		if (qz < tinyEps)
+			return V(0.0f, point.y > 0.0f ? ry : -ry, 0.0f);
+	}
+	else if (qy < tinyEps)
+		return V(0.0f, 0.0f, point.z > 0.0f ? rz : -rz);
+	const float e2y = ry * ry, e2z = rz * rz, eqy = ry * qy, eqz = rz * qz;
+	float t = fmaxf(eqy - e2y, eqz - e2z);
+	float dy = 0.0f, dz = 0.0f;
+	for (int i = 0; i < 20; i++)
+	{
+		dy = 1.0f / (t + e2y);
+		dz = 1.0f / (t + e2z);
+		const float d2y = eqy * dy, d2z = eqz * dz;
+		const float fy = d2y * d2y, fz = d2z * d2z;
+		const float f = fy + fz - 1.0f;
+		if (f < 1e-4f)
+			return V(0.0f, e2y * point.y * dy, e2z * point.z * dz);
+		const float df = (0.0f * 0.0f + fy * dy + fz * dz) * -2.0f;
+		t = t - f / df;
+	}
+	const float ry_ = e2y * point.y * dy, rz_ = e2z * point.z * dz;
+	const float a = ry_ / ry, b = rz_ / rz;
+	const float s = 1.0f / sqrtf(a * a + b * b);
+	return V(0.0f * s, ry_ * s, rz_ * s);
+}
+
+/* the cone limit's axis and error (ConeLimitHelperTanLess::getLimit with
+ * computeAxisAndError, common/src/CmConeLimitHelper.h:50-69, 163-178):
+ * swing.x = 0; the axis turns the twist axis outward from the limit, the
+ * error is positive inside the cone */
+static float cone_limit(qt swing, float ymax, float zmax, v3* axis)
+{
+	const float x2 = swing.x * 2.0f, w2 = swing.w * 2.0f;
+	const v3 twistAxis = V((swing.w * w2) - 1.0f + swing.x * x2, (swing.z * w2) + swing.y * x2,
+	                       (-swing.y * w2) + swing.z * x2);
+	const v3 swingAngle = V(0.0f, 4.0f * atan2f(swing.y, 1.0f + swing.w), 4.0f * atan2f(swing.z, 1.0f + swing.w));
+	const v3 p = ellipse_clamp(swingAngle, ymax, zmax);
+	const v3 normal = V(0.0f, p.y / (ymax * ymax), p.z / (zmax * zmax));
+	const v3 r = V(0.0f, tanf(p.y / 4.0f), tanf(p.z / 4.0f)), d = V(0.0f, normal.y, normal.z);
+	const v3 px = V(1.0f, 0.0f, 0.0f);
+	const float r2 = dot(r, r), a = 1.0f - r2, b = 1.0f / (1.0f + r2), b2 = b * b;
+	const float v1 = 2.0f * a * b2;
+	const v3 v2 = V(a, 2.0f * r.z, -2.0f * r.y);
+	const v3 coneLine = sub(scl(v2, v1), px);
+	const float rd = dot(r, d);
+	const float dv1 = -4.0f * rd * (3.0f - r2) * b2 * b;
+	const v3 dv2 = V(-2.0f * rd, 2.0f * d.z, -2.0f * d.y);
+	const v3 coneNormal = add(scl(dv2, v1), scl(v2, dv1));
+	const float inv = 1.0f / sqrtf(dot(coneNormal, coneNormal));
+	*axis = scl(cross(coneLine, coneNormal), inv);
+	return dot(cross(coneLine, *axis), twistAxis);
+}
+
 /* The joint's solverPrep with ConstraintHelper::prepareLockedAxes
  * (ExtConstraintHelper.h:150-208, 243-297): joint frames in the world
  * (computeJointFrames, :61-73), the neighbourhood operator (:44-48), rows
@@ -3470,6 +3532,27 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 		memset(&rows[i], 0, sizeof(row1d));
 		rows[i].minImp = -3.402823466e+38f;
 		rows[i].maxImp = 3.402823466e+38f;
+	}
+	if (j->type == PXO_JOINT_SPHERICAL && (j->pflags & JP_LIMIT))
+	{
+		/* the cone limit row comes first (SphericalJointSolverPrep,
+		 * ExtSphericalJoint.cpp:157-190): swing of B's frame in A's
+		 * (PxSeparateSwingTwist, PxMathUtils.h:272-276), its limit axis in
+		 * the world */
+		const qt q = qmul(qconj(cA.q), cB.q);
+		qt tw = {0.0f, 0.0f, 0.0f, 1.0f};
+		if (q.x != 0.0f)
+		{
+			const float sc = 1.0f / sqrtf(q.x * q.x + 0.0f * 0.0f + 0.0f * 0.0f + q.w * q.w);
+			tw.x = q.x * sc; tw.y = 0.0f * sc; tw.z = 0.0f * sc; tw.w = q.w * sc;
+		}
+		const qt swing = qmul(q, qconj(tw));
+		v3 ax;
+		const float err = cone_limit(swing, j->lo, j->hi, &ax);
+		row1d* c = &rows[k++];
+		c->ang0 = qrot(cA.q, ax); c->ang1 = c->ang0; c->angular = 1;
+		c->geomErr = err;
+		limit_row(c, j);
 	}
 	const m33 axes = qmat(cA.q);
 	if (lin)
@@ -3870,7 +3953,7 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 			return -1;
 		const int t = w->joint[joints[i]].type;
 		const int f = (int)params[10 * i];
-		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC)
+		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC && t != PXO_JOINT_SPHERICAL)
 			return -1;
 		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE)
 			return -1;

@@ -574,7 +574,9 @@ int pxref_scene_add_joints(void* h, int k, const int* type, const int* body0, co
 // positive). Revolute: PxRevoluteJoint::setLimit(PxJointAngularLimitPair),
 // setRevoluteJointFlag(eLIMIT_ENABLED / eDRIVE_ENABLED / eDRIVE_FREESPIN),
 // setDriveVelocity, setDriveForceLimit, setDriveGearRatio; prismatic:
-// PxPrismaticJoint::setLimit(PxJointLinearLimitPair), eLIMIT_ENABLED.
+// PxPrismaticJoint::setLimit(PxJointLinearLimitPair), eLIMIT_ENABLED;
+// spherical: PxSphericalJoint::setLimitCone(PxJointLimitCone(lower, upper)),
+// eLIMIT_ENABLED.
 // Returns 0 or -1.
 int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float* params)
 {
@@ -610,6 +612,17 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 			lim.damping = q[9];
 			p->setLimit(lim);
 			p->setPrismaticJointFlag(PxPrismaticJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
+		}
+		else if (PxSphericalJoint* sj = j->is<PxSphericalJoint>())
+		{
+			// lower / upper: the cone's y and z angles
+			PxJointLimitCone lim(q[1], q[2]);
+			lim.restitution = q[3];
+			lim.bounceThreshold = q[4];
+			lim.stiffness = q[8];
+			lim.damping = q[9];
+			sj->setLimitCone(lim);
+			sj->setSphericalJointFlag(PxSphericalJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
 		}
 		else if (f)
 			return -1;

@@ -123,6 +123,65 @@ PXH_D void jacobianAxes(f3 row[3], qt qa, qt qb)
 	}
 }
 
+// the closest point of the ellipse with radii (ry, rz) in the y-z plane to
+// point (PxEllipseClamp, physx/include/foundation/PxMathUtils.h:204-262)
+PXH_D f3 ellipseClamp(f3 point, float ry, float rz)
+{
+	const float qy = fabsf(point.y), qz = fabsf(point.z);
+	const float tinyEps = 1e-6f;
+	if (ry >= rz)
+	{
+		if (qz < tinyEps)
+			return mk(0.0f, point.y > 0.0f ? ry : -ry, 0.0f);
+	}
+	else if (qy < tinyEps)
+		return mk(0.0f, 0.0f, point.z > 0.0f ? rz : -rz);
+	const float e2y = ry * ry, e2z = rz * rz, eqy = ry * qy, eqz = rz * qz;
+	float t = fmaxf(eqy - e2y, eqz - e2z);
+	float dy = 0.0f, dz = 0.0f;
+	for (int i = 0; i < 20; i++)
+	{
+		dy = 1.0f / (t + e2y);
+		dz = 1.0f / (t + e2z);
+		const float d2y = eqy * dy, d2z = eqz * dz;
+		const float fy = d2y * d2y, fz = d2z * d2z;
+		const float f = fy + fz - 1.0f;
+		if (f < 1e-4f)
+			return mk(0.0f, e2y * point.y * dy, e2z * point.z * dz);
+		const float df = (0.0f * 0.0f + fy * dy + fz * dz) * -2.0f;
+		t = t - f / df;
+	}
+	const float ry_ = e2y * point.y * dy, rz_ = e2z * point.z * dz;
+	const float a = ry_ / ry, b = rz_ / rz;
+	const float sc = 1.0f / sqrtf(a * a + b * b);
+	return mk(0.0f * sc, ry_ * sc, rz_ * sc);
+}
+
+// the cone limit's axis and error (ConeLimitHelperTanLess::getLimit with
+// computeAxisAndError, common/src/CmConeLimitHelper.h:50-69, 163-178)
+PXH_D float coneLimit(qt swing, float ymax, float zmax, f3& axis)
+{
+	const float x2 = swing.x * 2.0f, w2 = swing.w * 2.0f;
+	const f3 twistAxis = mk((swing.w * w2) - 1.0f + swing.x * x2, (swing.z * w2) + swing.y * x2,
+	                        (-swing.y * w2) + swing.z * x2);
+	const f3 swingAngle = mk(0.0f, 4.0f * atan2f(swing.y, 1.0f + swing.w), 4.0f * atan2f(swing.z, 1.0f + swing.w));
+	const f3 p = ellipseClamp(swingAngle, ymax, zmax);
+	const f3 normal = mk(0.0f, p.y / (ymax * ymax), p.z / (zmax * zmax));
+	const f3 r = mk(0.0f, tanf(p.y / 4.0f), tanf(p.z / 4.0f)), d = mk(0.0f, normal.y, normal.z);
+	const f3 px = mk(1.0f, 0.0f, 0.0f);
+	const float r2 = dot(r, r), a = 1.0f - r2, b = 1.0f / (1.0f + r2), b2 = b * b;
+	const float v1 = 2.0f * a * b2;
+	const f3 v2 = mk(a, 2.0f * r.z, -2.0f * r.y);
+	const f3 coneLine = v2 * v1 - px;
+	const float rd = dot(r, d);
+	const float dv1 = -4.0f * rd * (3.0f - r2) * b2 * b;
+	const f3 dv2 = mk(-2.0f * rd, 2.0f * d.z, -2.0f * d.y);
+	const f3 coneNormal = dv2 * v1 + v2 * dv1;
+	const float inv = 1.0f / sqrtf(dot(coneNormal, coneNormal));
+	axis = cross(coneLine, coneNormal) * inv;
+	return dot(cross(coneLine, axis), twistAxis);
+}
+
 // the joint's rows (solverPrep); returns the count, ra / rb the anchor
 // offsets. p0 / p1: the joint's parameters (flags, lower, upper,
 // restitution), (bounce threshold, drive velocity, drive force limit, gear),
@@ -167,6 +226,30 @@ PXH_D int jointRows(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, con
 		rows[i].flags = 0;
 	}
 	int k = 0;
+	if (kind == JOINT_SPHERICAL && (pflags & JP_LIMIT))
+	{
+		// the cone limit row comes first (SphericalJointSolverPrep,
+		// ExtSphericalJoint.cpp:157-190): the swing of B's frame in A's
+		// (PxSeparateSwingTwist, PxMathUtils.h:272-276), its axis in the world
+		qt ca; ca.x = -cA.q.x; ca.y = -cA.q.y; ca.z = -cA.q.z; ca.w = cA.q.w;
+		const qt q = qmul(ca, cB.q);
+		qt tw; tw.x = 0.0f; tw.y = 0.0f; tw.z = 0.0f; tw.w = 1.0f;
+		if (q.x != 0.0f)
+		{
+			const float sc = 1.0f / sqrtf(q.x * q.x + 0.0f * 0.0f + 0.0f * 0.0f + q.w * q.w);
+			tw.x = q.x * sc; tw.y = 0.0f * sc; tw.z = 0.0f * sc; tw.w = q.w * sc;
+		}
+		qt tc; tc.x = -tw.x; tc.y = -tw.y; tc.z = -tw.z; tc.w = tw.w;
+		const qt swing = qmul(q, tc);
+		f3 ax;
+		const float err = coneLimit(swing, lo, hi, ax);
+		Row1D& c = rows[k++];
+		c.ang0 = qrot(cA.q, ax);
+		c.ang1 = c.ang0;
+		c.angular = 1;
+		c.geomErr = err;
+		limitRow(c, p0.w, p1.x, p2);
+	}
 	const m33 axes = qmat(cA.q);
 	if (lin)
 	{

@@ -1147,8 +1147,9 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 		if (float(f) != q[0] || f < 0 || f > 7)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: unknown flags");
 		const int t = s->hJType[joints[i]];
-		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC)
-			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute and prismatic joints take limits");
+		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL)
+			return fail(PXH_ERR_UNSUPPORTED,
+			            "pxh_scene_set_joint_params: only revolute, prismatic and spherical joints take limits");
 		if ((f & 6) && t != PXH_JOINT_REVOLUTE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute joints take a drive");
 		if (q[3] < 0.0f || q[3] > 1.0f || q[4] < 0.0f || q[6] < 0.0f || q[8] < 0.0f || q[9] < 0.0f)

@@ -477,6 +477,28 @@ def limited_joints(n_groups=48, seed=7) -> Scene:
     return sc
 
 
+def cone_limited_joints(n_groups=48, seed=7) -> Scene:
+    """joint_singles' groups with spherical joints whose cone limits have
+    y / z angles U(0.1, 0.8) rad (every fifth soft: stiffness 30 N m/rad,
+    damping 0.5); the random starting velocities swing them into the cones'
+    rims within a few steps."""
+    sc = joint_singles(n_groups, seed)
+    k = len(sc.joints["type"])
+    ids = np.arange(k)
+    p = np.zeros((k, 10), np.float32)
+    p[:, 0] = 1
+    p[:, 7] = 1.0
+    p[:, 1] = uniform(seed, ids, 80, 0.1, 0.8)
+    p[:, 2] = uniform(seed, ids, 81, 0.1, 0.8)
+    p[ids % 5 == 4, 8] = 30.0
+    p[ids % 5 == 4, 9] = 0.5
+    sc.joints["type"] = np.full(k, JOINT_SPHERICAL, np.int32)
+    sc.joints["params"] = p
+    sc.name = "cone_limited_joints"
+    sc.params = {"n_groups": n_groups, "seed": seed}
+    return sc
+
+
 def _qmul(a, b):
     ax, ay, az, aw = a
     bx, by, bz, bw = b

@@ -3,7 +3,8 @@ limit_row, twist_angle, prep_joint) before they check the GPU: against the
 live reference PhysX 5.6.1 CPU SDK (oracle/_ref: PxRevoluteJoint::setLimit
 with PxJointAngularLimitPair, eLIMIT_ENABLED / eDRIVE_ENABLED /
 eDRIVE_FREESPIN, setDriveVelocity / setDriveForceLimit / setDriveGearRatio;
-PxPrismaticJoint::setLimit with PxJointLinearLimitPair) one step at a time
+PxPrismaticJoint::setLimit with PxJointLinearLimitPair; PxSphericalJoint::
+setLimitCone with PxJointLimitCone) one step at a time
 along the reference's own trajectory, and against what the constraints fix
 in closed form."""
 import numpy as np
@@ -196,3 +197,57 @@ def test_locked_and_prismatic_cases_vs_reference():
             o.step(DT)
             r.step(DT)
             assert np.abs(o.bodies()[0][:, :3] - r.bodies()[0][:, :3]).max() <= 1e-4, s
+
+
+@need_ref
+@pytest.mark.parametrize("seed", [7, 8])
+def test_cone_limits_vs_live_reference(seed):
+    """48 spherical joints with cone limits (elliptic, y / z angles 0.1-0.8
+    rad; every fifth soft): along 60 steps of the reference's trajectory the
+    oracle's step from the reference's state matches the reference's next
+    state within 1e-4 of max(|v|, g dt) (measured 4e-5: the arc tangents and
+    tangents of the swing come from the same libm); without the limits the
+    bodies end far apart."""
+    sc = S.cone_limited_joints(48, seed)
+    r = L.RefScene(sc, "tgs", 4, 1, threads=1)
+    o = L.OracleWorld(sc, pos_iters=4, vel_iters=1)
+    for s in range(60):
+        o.set_bodies(*r.bodies())
+        o.step(DT)
+        r.step(DT)
+        op, ol, oa = o.bodies()
+        rp, rl, ra = r.bodies()
+        lt = 1e-4 * np.maximum(np.abs(rl).max(1), 9.81 * DT)
+        at = 1e-4 * np.maximum(np.abs(ra).max(1), 1.0)
+        assert (np.abs(ol - rl).max(1) <= lt).all() and (np.abs(oa - ra).max(1) <= at).all(), s
+    free = S.cone_limited_joints(48, seed)
+    free.joints.pop("params")
+    rf = L.RefScene(free, "tgs", 4, 1, threads=1)
+    rf.step(DT, 60)
+    assert np.abs(rf.bodies()[0][:, :3] - r.bodies()[0][:, :3]).max() > 0.5
+
+
+def test_cone_limit_holds():
+    """Closed form: a box hanging from a spherical joint 0.5 m above its
+    centre, its cone (0.3, 0.5 rad about the joint's y and z axes, the cone
+    around the joint's x axis pointing down) pushed sideways at (3, 0, 2)
+    m/s: the angle between the box's hanging axis and the cone axis stays
+    within the larger cone angle plus the impact overshoot (0.08 rad) over
+    200 steps; without the limit it swings beyond."""
+    down = _q([0, 0, 1], -np.pi / 2)          # joint x axis = world -y
+    f1 = np.concatenate([[0.0, 2.5, 0.0], down])
+    f0 = np.concatenate([[0.0, 0.5, 0.0], down])
+    for on in (1, 0):
+        sc = _one_joint(S.JOINT_SPHERICAL, S.BOX, [0.1, 0.3, 0.1, 0], [0, 2.0, 0, 0, 0, 0, 1], f0, f1,
+                        [on, 0.3, 0.5, 0, 0, 0, 0, 1, 0, 0], lv=(3.0, 0.0, 2.0))
+        o = L.OracleWorld(sc)
+        worst = 0.0
+        for _ in range(200):
+            o.step(DT)
+            pose = o.bodies()[0][0]
+            hang = S._qrot(pose[3:].astype(np.float64), np.array([0.0, -1.0, 0.0]))
+            worst = max(worst, float(np.arccos(np.clip(-hang[1], -1, 1))))
+        if on:
+            assert worst <= 0.5 + 0.08, worst
+        else:
+            assert worst > 0.7, worst

@@ -40,6 +40,14 @@ def _track(sc, steps, rel, gravity=(0.0, -9.81, 0.0)):
 
 
 @pytest.mark.parametrize("seed", [7, 8])
+def test_cone_limited_joints_track_oracle(cuda, seed):
+    """48 spherical joints with elliptic cone limits (hard and soft): 60
+    steps, each within 3e-5 of the oracle's step from the same state (the
+    swing's arc tangents and tangents are the device's)."""
+    _track(S.cone_limited_joints(48, seed), 60, 3e-5)
+
+
+@pytest.mark.parametrize("seed", [7, 8])
 def test_limited_joints_track_oracle(cuda, seed):
     """48 revolute and prismatic joints with hard limits (some locked, some
     with restitution), force-limited / unbounded / freespin drives and gear
@@ -73,7 +81,7 @@ def test_joint_params_argument_checks(cuda):
     kinds = sc.joints["type"]
     fixed = int(np.where(kinds == S.JOINT_FIXED)[0][0])
     prism = int(np.where(kinds == S.JOINT_PRISMATIC)[0][0])
-    with pytest.raises(P.PxhError, match="only revolute and prismatic"):
+    with pytest.raises(P.PxhError, match="joints take limits"):
         g.set_joint_params([fixed], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="only revolute joints take a drive"):
         g.set_joint_params([prism], [[2, 0, 0, 0, 0, 1, 1, 1, 0, 0]])
