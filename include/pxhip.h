@@ -124,7 +124,8 @@ int32_t pxh_scene_add_aggregate(pxh_scene scene, int32_t k, const int32_t* bodie
  * PxSphericalJointCreate, PxRevoluteJointCreate (free twist about the joint
  * frame's x axis), PxPrismaticJointCreate (free translation along x)
  * (physx/include/extensions/PxFixedJoint.h, PxSphericalJoint.h,
- * PxRevoluteJoint.h, PxPrismaticJoint.h), solved as 1D constraint rows
+ * PxRevoluteJoint.h, PxPrismaticJoint.h), PxDistanceJointCreate
+ * (PxDistanceJoint.h), solved as 1D constraint rows
  * (the GPU reference's constraintBlockPrePrep.cu:1168,1219 and
  * solve1DBlockTGS, gpusolver/src/CUDA/solverBlockTGS.cuh:744). Host arrays:
  * type[k], body0[k] / body1[k] (body indices, -1 = the world frame, never
@@ -133,7 +134,8 @@ int32_t pxh_scene_add_aggregate(pxh_scene scene, int32_t k, const int32_t* bodie
  * Jointed bodies do not collide (PxConstraintFlag::eCOLLISION_ENABLED off,
  * the joints' default). TGS only (a PGS step with joints returns
  * PXH_ERR_UNSUPPORTED). Returns the index of the first joint. */
-enum { PXH_JOINT_FIXED = 0, PXH_JOINT_SPHERICAL = 1, PXH_JOINT_REVOLUTE = 2, PXH_JOINT_PRISMATIC = 3 };
+enum { PXH_JOINT_FIXED = 0, PXH_JOINT_SPHERICAL = 1, PXH_JOINT_REVOLUTE = 2, PXH_JOINT_PRISMATIC = 3,
+       PXH_JOINT_DISTANCE = 4 };
 int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, const int32_t* body0,
                              const int32_t* body1, const float* frame0, const float* frame1);
 
@@ -147,7 +149,13 @@ int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, co
  * joint lower / upper are the cone's y and z angles
  * (PxSphericalJoint::setLimitCone(PxJointLimitCone), eLIMIT_ENABLED,
  * PxSphericalJoint.h:97-120; SphericalJointSolverPrep,
- * ExtSphericalJoint.cpp:157-190). Replaces
+ * ExtSphericalJoint.cpp:157-190); for a distance joint flags are 1 max
+ * distance, 2 min distance, 4 spring, lower / upper the min / max distance
+ * and the restitution slot the tolerance (PxDistanceJoint::setMinDistance /
+ * setMaxDistance / setTolerance / setStiffness / setDamping /
+ * setDistanceJointFlag, PxDistanceJoint.h:80-170; DistanceJointSolverPrep,
+ * ExtDistanceJoint.cpp:179-271; a new distance joint keeps a max distance of
+ * 0 with tolerance 0.025, PxDistanceJointCreate's defaults). Replaces
  * PxRevoluteJoint::setLimit (PxJointAngularLimitPair), setRevoluteJointFlag(eLIMIT_ENABLED /
  * eDRIVE_ENABLED / eDRIVE_FREESPIN), setDriveVelocity, setDriveForceLimit,
  * setDriveGearRatio (physx/include/extensions/PxRevoluteJoint.h:109-230)

@@ -3381,6 +3381,7 @@ static void joint_axes(int type, int* lin, int* ang)
 	case PXO_JOINT_FIXED: *lin = 7; *ang = 7; break;
 	case PXO_JOINT_SPHERICAL: *lin = 7; *ang = 0; break;
 	case PXO_JOINT_REVOLUTE: *lin = 7; *ang = 6; break;
+	case PXO_JOINT_DISTANCE: *lin = 0; *ang = 0; break;
 	default: *lin = 6; *ang = 7; break;
 	}
 }
@@ -3532,6 +3533,64 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 		memset(&rows[i], 0, sizeof(row1d));
 		rows[i].minImp = -3.402823466e+38f;
 		rows[i].maxImp = 3.402823466e+38f;
+	}
+	if (j->type == PXO_JOINT_DISTANCE)
+	{
+		/* DistanceJointSolverPrep (ExtDistanceJoint.cpp:179-271): rows along
+		 * the anchors' direction through frame B's anchor; flags 1 max
+		 * distance, 2 min distance, 4 spring; lower / upper the min / max
+		 * distance, the restitution slot the tolerance */
+		const int enforceMax = (j->pflags & 1) && j->hi >= 0.0f;
+		const int enforceMin = (j->pflags & 2) && j->lo >= 0.0f;
+		if (enforceMax || enforceMin)
+		{
+			v3 dir = sub(cA.p, cB.p);
+			const float dist = sqrtf(dot(dir, dir));
+			if (dist > 0.0f)
+				dir = scl(dir, 1.0f / dist);
+			if (dist < 1.192092896e-07f)
+				dir = V(1.0f, 0.0f, 0.0f);
+			const float tol = j->rest;
+			row1d* c = &rows[k];
+			const int both = enforceMin && enforceMax;
+			const int nrows = both && j->lo != j->hi ? 2 : 1;
+			for (int r = 0; r < nrows; r++)
+			{
+				row1d* e = &c[r];
+				e->lin0 = dir; e->ang0 = cross(ra0, dir);
+				e->lin1 = dir; e->ang1 = cross(rb, dir);
+				if (j->pflags & 4)
+				{
+					e->flags |= RF_SPRING;
+					e->stiff = j->stiff;
+					e->damp = j->damp;
+				}
+			}
+			if (both && j->lo == j->hi)
+			{
+				const float error = dist - j->hi;
+				c->geomErr = error > tol ? error - tol : (error < -tol ? error + tol : 0.0f);
+			}
+			else
+			{
+				if (enforceMin)
+				{
+					c->geomErr = dist - j->lo + tol;
+					c->minImp = 0.0f;
+					if (dist >= j->lo)
+						c->flags |= RF_KEEP_BIAS;
+					c++;
+				}
+				if (enforceMax)
+				{
+					c->geomErr = dist - j->hi - tol;
+					c->maxImp = 0.0f;
+					if (dist <= j->hi)
+						c->flags |= RF_KEEP_BIAS;
+				}
+			}
+			k += nrows;
+		}
 	}
 	if (j->type == PXO_JOINT_SPHERICAL && (j->pflags & JP_LIMIT))
 	{
@@ -3953,9 +4012,10 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 			return -1;
 		const int t = w->joint[joints[i]].type;
 		const int f = (int)params[10 * i];
-		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC && t != PXO_JOINT_SPHERICAL)
+		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC && t != PXO_JOINT_SPHERICAL &&
+		    t != PXO_JOINT_DISTANCE)
 			return -1;
-		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE)
+		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_DISTANCE)
 			return -1;
 	}
 	for (int i = 0; i < k; i++)
@@ -4008,7 +4068,7 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 		return w->nj;
 	for (int i = 0; i < k; i++)
 	{
-		if (type[i] < 0 || type[i] > PXO_JOINT_PRISMATIC || body0[i] >= w->n || body1[i] >= w->n ||
+		if (type[i] < 0 || type[i] > PXO_JOINT_DISTANCE || body0[i] >= w->n || body1[i] >= w->n ||
 		    (body0[i] < 0 && body1[i] < 0) || body0[i] == body1[i] || body0[i] < -1 || body1[i] < -1)
 			return -1;
 	}
@@ -4025,6 +4085,13 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 		memset(j, 0, sizeof(*j));
 		j->gear = 1.0f;
 		j->type = type[i];
+		if (j->type == PXO_JOINT_DISTANCE)
+		{
+			/* PxDistanceJointCreate's defaults (ExtDistanceJoint.cpp:37-47):
+			 * max distance 0 enforced, tolerance 0.025 */
+			j->pflags = 1;
+			j->rest = 0.025f;
+		}
 		j->a = body0[i];
 		j->b = body1[i];
 		j->f0.p = V(frame0[7 * i], frame0[7 * i + 1], frame0[7 * i + 2]);

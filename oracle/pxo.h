@@ -106,7 +106,8 @@ void  pxo_world_wake(void* w, int k, const int32_t* bodies);
  * or -1 for the world frame; frame0/frame1 k x 7 local frames (p xyz, q xyzw)
  * in each body's frame (the world's: world coordinates). Jointed bodies do
  * not collide. TGS only. Returns the first joint's index or -1. */
-enum { PXO_JOINT_FIXED = 0, PXO_JOINT_SPHERICAL = 1, PXO_JOINT_REVOLUTE = 2, PXO_JOINT_PRISMATIC = 3 };
+enum { PXO_JOINT_FIXED = 0, PXO_JOINT_SPHERICAL = 1, PXO_JOINT_REVOLUTE = 2, PXO_JOINT_PRISMATIC = 3,
+       PXO_JOINT_DISTANCE = 4 };
 int   pxo_world_add_joints(void* w, int k, const int32_t* type, const int32_t* body0, const int32_t* body1,
                            const float* frame0, const float* frame1);
 /* limit / drive parameters of revolute and prismatic joints: params[10*i] =

@@ -536,8 +536,9 @@ int pxref_scene_add_aggregate(void* h, int k, const int* bodies, int selfCollisi
 }
 
 // Adds k joints without limits or drives: type 0 fixed, 1 spherical,
-// 2 revolute, 3 prismatic (PxFixedJointCreate, PxSphericalJointCreate,
-// PxRevoluteJointCreate, PxPrismaticJointCreate); body0/body1: indices of
+// 2 revolute, 3 prismatic, 4 distance (PxFixedJointCreate,
+// PxSphericalJointCreate, PxRevoluteJointCreate, PxPrismaticJointCreate,
+// PxDistanceJointCreate with its defaults); body0/body1: indices of
 // bodies added by pxref_scene_add_bodies or -1 for the world; frame0/frame1:
 // k x 7 local frames (px py pz qx qy qz qw). Collision between the jointed
 // bodies stays off (the joints' default). Returns the joint count or -1.
@@ -557,6 +558,7 @@ int pxref_scene_add_joints(void* h, int k, const int* type, const int* body0, co
 		case 1: j = PxSphericalJointCreate(*gPhysics, a0, f0, a1, f1); break;
 		case 2: j = PxRevoluteJointCreate(*gPhysics, a0, f0, a1, f1); break;
 		case 3: j = PxPrismaticJointCreate(*gPhysics, a0, f0, a1, f1); break;
+		case 4: j = PxDistanceJointCreate(*gPhysics, a0, f0, a1, f1); break;
 		default: return -1;
 		}
 		if (!j)
@@ -623,6 +625,19 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 			lim.damping = q[9];
 			sj->setLimitCone(lim);
 			sj->setSphericalJointFlag(PxSphericalJointFlag::eLIMIT_ENABLED, (f & 1) != 0);
+		}
+		else if (PxDistanceJoint* dj = j->is<PxDistanceJoint>())
+		{
+			// flags 1 max, 2 min distance, 4 spring; lower / upper the min /
+			// max distance; the restitution slot the tolerance
+			dj->setMinDistance(q[1]);
+			dj->setMaxDistance(q[2]);
+			dj->setTolerance(q[3]);
+			dj->setStiffness(q[8]);
+			dj->setDamping(q[9]);
+			dj->setDistanceJointFlag(PxDistanceJointFlag::eMAX_DISTANCE_ENABLED, (f & 1) != 0);
+			dj->setDistanceJointFlag(PxDistanceJointFlag::eMIN_DISTANCE_ENABLED, (f & 2) != 0);
+			dj->setDistanceJointFlag(PxDistanceJointFlag::eSPRING_ENABLED, (f & 4) != 0);
 		}
 		else if (f)
 			return -1;

@@ -34,7 +34,7 @@
 
 namespace pxh {
 
-enum { JOINT_FIXED = 0, JOINT_SPHERICAL = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3 };
+enum { JOINT_FIXED = 0, JOINT_SPHERICAL = 1, JOINT_REVOLUTE = 2, JOINT_PRISMATIC = 3, JOINT_DISTANCE = 4 };
 enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY = 8, JF_KEEP_BIAS = 16, JF_SPRING = 32 };
 // PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110)
 enum { JHINT_NONE = 0, JHINT_ROT_EQ = 1024, JHINT_ROT_INEQ = 1025, JHINT_EQ = 2048, JHINT_INEQ = 2049 };
@@ -47,8 +47,8 @@ constexpr int JOINT_MAX_ROWS = 8;
 // locked linear / angular axes (bit masks x y z) of each kind
 PXH_D void jointAxes(int kind, int& lin, int& ang)
 {
-	lin = kind == JOINT_PRISMATIC ? 6 : 7;
-	ang = kind == JOINT_SPHERICAL ? 0 : (kind == JOINT_REVOLUTE ? 6 : 7);
+	lin = kind == JOINT_DISTANCE ? 0 : (kind == JOINT_PRISMATIC ? 6 : 7);
+	ang = kind == JOINT_SPHERICAL || kind == JOINT_DISTANCE ? 0 : (kind == JOINT_REVOLUTE ? 6 : 7);
 }
 
 struct Row1D
@@ -226,6 +226,64 @@ PXH_D int jointRows(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, con
 		rows[i].flags = 0;
 	}
 	int k = 0;
+	if (kind == JOINT_DISTANCE)
+	{
+		// DistanceJointSolverPrep (ExtDistanceJoint.cpp:179-271): rows along
+		// the anchors' direction through frame B's anchor; flags 1 max
+		// distance, 2 min distance, 4 spring; lower / upper the min / max
+		// distance, the restitution slot the tolerance
+		const bool enforceMax = (pflags & 1) && hi >= 0.0f;
+		const bool enforceMin = (pflags & 2) && lo >= 0.0f;
+		if (enforceMax || enforceMin)
+		{
+			f3 dir = cA.p - cB.p;
+			const float dist = sqrtf(dot(dir, dir));
+			if (dist > 0.0f)
+				dir = dir * (1.0f / dist);
+			if (dist < 1.192092896e-07f)
+				dir = mk(1.0f, 0.0f, 0.0f);
+			const float tol = p0.w;
+			const bool both = enforceMin && enforceMax;
+			const int nrows = both && lo != hi ? 2 : 1;
+			for (int r = 0; r < nrows; r++)
+			{
+				Row1D& e = rows[k + r];
+				e.lin0 = dir; e.ang0 = cross(ra0, dir);
+				e.lin1 = dir; e.ang1 = cross(rb, dir);
+				if (pflags & 4)
+				{
+					e.flags |= RF_SPRING;
+					e.stiff = p2.x;
+					e.damp = p2.y;
+				}
+			}
+			if (both && lo == hi)
+			{
+				const float error = dist - hi;
+				rows[k].geomErr = error > tol ? error - tol : (error < -tol ? error + tol : 0.0f);
+			}
+			else
+			{
+				int c = k;
+				if (enforceMin)
+				{
+					rows[c].geomErr = dist - lo + tol;
+					rows[c].minImp = 0.0f;
+					if (dist >= lo)
+						rows[c].flags |= RF_KEEP_BIAS;
+					c++;
+				}
+				if (enforceMax)
+				{
+					rows[c].geomErr = dist - hi - tol;
+					rows[c].maxImp = 0.0f;
+					if (dist <= hi)
+						rows[c].flags |= RF_KEEP_BIAS;
+				}
+			}
+			k += nrows;
+		}
+	}
 	if (kind == JOINT_SPHERICAL && (pflags & JP_LIMIT))
 	{
 		// the cone limit row comes first (SphericalJointSolverPrep,

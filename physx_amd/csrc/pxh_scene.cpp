@@ -1094,7 +1094,7 @@ int32_t pxh_scene_add_joints(pxh_scene s, int32_t k, const int32_t* type, const 
 		return fail(PXH_ERR_STATE, "pxh_scene_add_joints: not allowed while simulating");
 	for (int i = 0; i < k; i++)
 	{
-		if (type[i] < PXH_JOINT_FIXED || type[i] > PXH_JOINT_PRISMATIC)
+		if (type[i] < PXH_JOINT_FIXED || type[i] > PXH_JOINT_DISTANCE)
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: unknown joint type");
 		if (body0[i] < -1 || body1[i] < -1 || body0[i] >= s->n || body1[i] >= s->n)
 			return fail(PXH_ERR_INVALID, "pxh_scene_add_joints: body index out of range");
@@ -1114,9 +1114,13 @@ int32_t pxh_scene_add_joints(pxh_scene s, int32_t k, const int32_t* type, const 
 		s->hJB.push_back(body1[i]);
 		s->hJF0.insert(s->hJF0.end(), frame0 + 7 * i, frame0 + 7 * i + 7);
 		s->hJF1.insert(s->hJF1.end(), frame1 + 7 * i, frame1 + 7 * i + 7);
-		// no limit or drive; gear ratio 1
+		// no limit or drive; gear ratio 1. A distance joint starts with
+		// PxDistanceJointCreate's defaults: max distance 0 enforced,
+		// tolerance 0.025 (ExtDistanceJoint.cpp:37-47)
 		const float none[10] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f};
-		s->hJP.insert(s->hJP.end(), none, none + 10);
+		const float dist[10] = {1.0f, 0.0f, 0.0f, 0.025f, 0.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f};
+		const float* d0 = type[i] == PXH_JOINT_DISTANCE ? dist : none;
+		s->hJP.insert(s->hJP.end(), d0, d0 + 10);
 	}
 	if (int r = uploadJoints(s))
 	{
@@ -1147,12 +1151,13 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 		if (float(f) != q[0] || f < 0 || f > 7)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: unknown flags");
 		const int t = s->hJType[joints[i]];
-		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL)
-			return fail(PXH_ERR_UNSUPPORTED,
-			            "pxh_scene_set_joint_params: only revolute, prismatic and spherical joints take limits");
-		if ((f & 6) && t != PXH_JOINT_REVOLUTE)
+		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL &&
+		    t != PXH_JOINT_DISTANCE)
+			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: fixed joints take no limits");
+		if ((f & 6) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_DISTANCE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute joints take a drive");
-		if (q[3] < 0.0f || q[3] > 1.0f || q[4] < 0.0f || q[6] < 0.0f || q[8] < 0.0f || q[9] < 0.0f)
+		if (q[3] < 0.0f || (q[3] > 1.0f && t != PXH_JOINT_DISTANCE) || q[4] < 0.0f || q[6] < 0.0f || q[8] < 0.0f ||
+		    q[9] < 0.0f)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: restitution outside [0, 1] or a negative "
 			                             "bounce threshold, force limit, stiffness or damping");
 	}

@@ -17,7 +17,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 SPHERE, PLANE, CAPSULE, BOX, CONVEX = 0, 1, 2, 3, 5
-JOINT_FIXED, JOINT_SPHERICAL, JOINT_REVOLUTE, JOINT_PRISMATIC = 0, 1, 2, 3
+JOINT_FIXED, JOINT_SPHERICAL, JOINT_REVOLUTE, JOINT_PRISMATIC, JOINT_DISTANCE = 0, 1, 2, 3, 4
 
 _GOLDEN = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -495,6 +495,36 @@ def cone_limited_joints(n_groups=48, seed=7) -> Scene:
     sc.joints["type"] = np.full(k, JOINT_SPHERICAL, np.int32)
     sc.joints["params"] = p
     sc.name = "cone_limited_joints"
+    sc.params = {"n_groups": n_groups, "seed": seed}
+    return sc
+
+
+def distance_joints(n_groups=48, seed=7, equal=True) -> Scene:
+    """joint_singles' groups with distance joints: the first frames' anchors
+    kept at most U(0.05, 0.3) m apart (every third also at least that less
+    0.05 m; with equal, every seventh exactly that distance; every fifth a
+    spring of 100 N/m, damping 2), tolerance 0.01 m."""
+    sc = joint_singles(n_groups, seed)
+    k = len(sc.joints["type"])
+    ids = np.arange(k)
+    p = np.zeros((k, 10), np.float32)
+    p[:, 7] = 1.0
+    mx = uniform(seed, ids, 82, 0.05, 0.3)
+    p[:, 0] = 1
+    p[:, 2] = mx
+    p[ids % 3 == 0, 0] = 3
+    p[ids % 3 == 0, 1] = mx[ids % 3 == 0] - 0.05
+    eq = (ids % 7 == 2) & equal
+    p[eq, 0] = 3
+    p[eq, 1] = mx[eq]
+    p[:, 3] = 0.01
+    spring = ids % 5 == 4
+    p[spring, 0] = p[spring, 0] + 4
+    p[spring, 8] = 100.0
+    p[spring, 9] = 2.0
+    sc.joints["type"] = np.full(k, JOINT_DISTANCE, np.int32)
+    sc.joints["params"] = p
+    sc.name = "distance_joints"
     sc.params = {"n_groups": n_groups, "seed": seed}
     return sc
 

@@ -40,6 +40,13 @@ def _track(sc, steps, rel, gravity=(0.0, -9.81, 0.0)):
 
 
 @pytest.mark.parametrize("seed", [7, 8])
+def test_distance_joints_track_oracle(cuda, seed):
+    """48 distance joints (max, min and max, exact distance, springs): 60
+    steps, each within 3e-5 of the oracle's step from the same state."""
+    _track(S.distance_joints(48, seed), 60, 3e-5)
+
+
+@pytest.mark.parametrize("seed", [7, 8])
 def test_cone_limited_joints_track_oracle(cuda, seed):
     """48 spherical joints with elliptic cone limits (hard and soft): 60
     steps, each within 3e-5 of the oracle's step from the same state (the
@@ -81,7 +88,7 @@ def test_joint_params_argument_checks(cuda):
     kinds = sc.joints["type"]
     fixed = int(np.where(kinds == S.JOINT_FIXED)[0][0])
     prism = int(np.where(kinds == S.JOINT_PRISMATIC)[0][0])
-    with pytest.raises(P.PxhError, match="joints take limits"):
+    with pytest.raises(P.PxhError, match="fixed joints take no limits"):
         g.set_joint_params([fixed], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="only revolute joints take a drive"):
         g.set_joint_params([prism], [[2, 0, 0, 0, 0, 1, 1, 1, 0, 0]])
