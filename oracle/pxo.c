@@ -2402,7 +2402,7 @@ enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY =
  * when either is positive) (PxRevoluteJoint::setLimit / setRevoluteJointFlag /
  * setDriveVelocity / setDriveForceLimit / setDriveGearRatio,
  * PxPrismaticJoint::setLimit / setPrismaticJointFlag) */
-enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
+enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4, JP_COLLIDE = 8 };
 #define JOINT_MAX_ROWS 8
 
 typedef struct {
@@ -3998,6 +3998,8 @@ static void solve_joint(pxo_joint* j, sbody* b0, sbody* b1, float elapsed, int c
 	b1->lin = l1; b1->ang = a1;
 }
 
+static void joint_keys(pxo_world* w);
+
 /* limit and drive parameters of k joints (params[10 * i]: flags JP_* as a
  * float, lower, upper, restitution, bounce threshold, drive velocity, drive
  * force limit, drive gear ratio, limit stiffness, limit damping). Only
@@ -4012,7 +4014,7 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 			return -1;
 		const int t = w->joint[joints[i]].type;
 		const int f = (int)params[10 * i];
-		if (f && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC && t != PXO_JOINT_SPHERICAL &&
+		if ((f & 7) && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_PRISMATIC && t != PXO_JOINT_SPHERICAL &&
 		    t != PXO_JOINT_DISTANCE)
 			return -1;
 		if ((f & (JP_DRIVE | JP_FREESPIN)) && t != PXO_JOINT_REVOLUTE && t != PXO_JOINT_DISTANCE)
@@ -4026,6 +4028,7 @@ int pxo_world_set_joint_params(void* wp, int k, const int32_t* joints, const flo
 		j->lo = q[1]; j->hi = q[2]; j->rest = q[3]; j->bthr = q[4];
 		j->dvel = q[5]; j->dfl = q[6]; j->gear = q[7]; j->stiff = q[8]; j->damp = q[9];
 	}
+	joint_keys(w);
 	return 0;
 }
 
@@ -4148,11 +4151,20 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 	}
 	free(used);
 	free(next);
+	joint_keys(w);
+	return first;
+}
+
+/* the sorted body-pair keys of the joints whose bodies do not collide
+ * (PxConstraintFlag::eCOLLISION_ENABLED off, the default; flag 8 of the
+ * joint parameters turns it on) */
+static void joint_keys(pxo_world* w)
+{
 	int nk = 0;
 	for (int i = 0; i < w->nj; i++)
 	{
 		const pxo_joint* j = &w->joint[i];
-		if (j->a < 0 || j->b < 0)
+		if (j->a < 0 || j->b < 0 || (j->pflags & JP_COLLIDE))
 			continue;
 		const uint32_t lo = (uint32_t)(j->a < j->b ? j->a : j->b), hi = (uint32_t)(j->a < j->b ? j->b : j->a);
 		w->jkeys[nk++] = ((uint64_t)lo << 32) | hi;
@@ -4161,7 +4173,6 @@ int pxo_world_add_joints(void* wp, int k, const int32_t* type, const int32_t* bo
 	/* jointed() searches nj keys: pad with keys no pair has */
 	for (int i = nk; i < w->nj; i++)
 		w->jkeys[i] = ~0ull;
-	return first;
 }
 
 /* ---- static triangle meshes ----

@@ -1058,7 +1058,9 @@ int uploadJoints(pxh_scene_t* s)
 		param[3 * k] = make_float4(jp[0], jp[1], jp[2], jp[3]);
 		param[3 * k + 1] = make_float4(jp[4], jp[5], jp[6], jp[7]);
 		param[3 * k + 2] = make_float4(jp[8], jp[9], 0.0f, 0.0f);
-		if (s->hJA[i] >= 0 && s->hJB[i] >= 0)
+		// jointed bodies do not collide unless the joint enables it
+		// (PxConstraintFlag::eCOLLISION_ENABLED: parameter flag 8)
+		if (s->hJA[i] >= 0 && s->hJB[i] >= 0 && !(int(jp[0]) & 8))
 		{
 			const uint32_t lo = uint32_t(std::min(s->hJA[i], s->hJB[i])), hi = uint32_t(std::max(s->hJA[i], s->hJB[i]));
 			keys.push_back((uint64_t(lo) << 32) | hi);
@@ -1148,10 +1150,10 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 			if (!std::isfinite(q[c]) && !(c == 6 && q[c] > 0.0f))
 				return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: parameter is not finite");
 		const int f = int(q[0]);
-		if (float(f) != q[0] || f < 0 || f > 7)
+		if (float(f) != q[0] || f < 0 || f > 15)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: unknown flags");
 		const int t = s->hJType[joints[i]];
-		if (f && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL &&
+		if ((f & 7) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL &&
 		    t != PXH_JOINT_DISTANCE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: fixed joints take no limits");
 		if ((f & 6) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_DISTANCE)

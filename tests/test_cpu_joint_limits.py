@@ -304,3 +304,34 @@ def test_distance_joint_keeps_its_range():
         assert max(d) <= 0.5 + 0.01 + 0.02, max(d)
         if flags == 3:
             assert min(d) >= 0.4 - 0.01 - 0.02, min(d)
+
+
+def _overlapping_pair(flags):
+    st = np.array([S.BOX, S.BOX], np.int32)
+    dims = np.array([[0.3, 0.3, 0.3, 0], [0.3, 0.3, 0.3, 0]], np.float32)
+    pose = np.array([[0, 5, 0, 0, 0, 0, 1], [0.5, 5, 0, 0, 0, 0, 1]], np.float32)
+    m, inn = S.mass_props(st, dims, 1000.0)
+    j = {"type": np.array([S.JOINT_SPHERICAL], np.int32), "body0": np.array([0], np.int32),
+         "body1": np.array([1], np.int32), "frame0": np.array([[0.25, 0, 0, 0, 0, 0, 1]], np.float32),
+         "frame1": np.array([[-0.25, 0, 0, 0, 0, 0, 1]], np.float32),
+         "params": np.array([[flags, 0, 0, 0, 0, 0, 0, 1, 0, 0]], np.float32)}
+    return S.Scene(st, dims, pose, np.zeros((2, 3), np.float32), np.zeros((2, 3), np.float32), m, inn,
+                   np.zeros(2, np.int32), np.zeros((0, 4), np.float32), np.zeros(0, np.int32), 1, "pair", {},
+                   None, None, j)
+
+
+def test_jointed_bodies_collide_when_enabled():
+    """Two overlapping boxes joined by a spherical joint: no contact by
+    default, one touching pair once the joint enables collision (parameter
+    flag 8, PxConstraintFlag::eCOLLISION_ENABLED); the reference SDK agrees
+    on both and its velocities after a step match within 1e-4."""
+    for flags, touching in ((0, 0), (8, 1)):
+        sc = _overlapping_pair(flags)
+        o = L.OracleWorld(sc)
+        o.step(DT)
+        assert o.stats()["constraints"] == touching
+        if L.have_ref():
+            r = L.RefScene(sc, "tgs", 4, 1, threads=1, report_contacts=True)
+            r.step(DT)
+            assert (len(r.contacts()) > 0) == bool(touching)
+            assert np.abs(o.bodies()[1] - r.bodies()[1]).max() <= 1e-4 * max(np.abs(r.bodies()[1]).max(), 9.81 * DT)

@@ -95,7 +95,7 @@ def test_joint_params_argument_checks(cuda):
     with pytest.raises(P.PxhError, match="out of range"):
         g.set_joint_params([len(kinds)], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="unknown flags"):
-        g.set_joint_params([prism], [[9, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
+        g.set_joint_params([prism], [[17, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.set_joint_params([prism], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.step(DT)
 
@@ -145,3 +145,18 @@ def test_limited_world_alone_in_batch_and_after_removal(cuda):
         batch.step(DT)
     for x, y in zip(alone.bodies(), batch.bodies()):
         assert np.array_equal(x, y[f2:f2 + c2])
+
+
+def test_jointed_bodies_collide_when_enabled(cuda):
+    """The overlapping jointed boxes: no constraint by default, one with
+    collision enabled (parameter flag 8), the velocities within 1e-5 of the
+    oracle's either way."""
+    from test_cpu_joint_limits import _overlapping_pair
+    for flags, touching in ((0, 0), (8, 1)):
+        sc = _overlapping_pair(flags)
+        g = P.Scene(sc)
+        o = L.OracleWorld(sc)
+        g.step(DT)
+        o.step(DT)
+        assert g.stats()["constraints"] == touching == o.stats()["constraints"]
+        _close(g.bodies(), o.bodies(), 1e-5, flags)
