@@ -142,7 +142,8 @@ int32_t pxh_scene_add_joints(pxh_scene scene, int32_t k, const int32_t* type, co
 /* Limits and drives of revolute, prismatic and spherical joints (joint indices as
  * pxh_scene_add_joints returned them), params[10*i]: flags (1 limit, 2 drive,
  * 4 drive freespin, 8 the jointed bodies collide: PxConstraintFlag::
- * eCOLLISION_ENABLED, any joint type), lower, upper (radians about / metres along the joint
+ * eCOLLISION_ENABLED, any joint type; 16 the drive force limit is a force,
+ * PxConstraintFlag::eDRIVE_LIMITS_ARE_FORCES), lower, upper (radians about / metres along the joint
  * frame's x axis; lower >= upper locks the axis), restitution, bounce
  * threshold, drive velocity, drive force limit (an impulse per step; may be
  * +inf), drive gear ratio, limit stiffness, limit damping (either positive:

@@ -2402,7 +2402,7 @@ enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY =
  * when either is positive) (PxRevoluteJoint::setLimit / setRevoluteJointFlag /
  * setDriveVelocity / setDriveForceLimit / setDriveGearRatio,
  * PxPrismaticJoint::setLimit / setPrismaticJointFlag) */
-enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4, JP_COLLIDE = 8 };
+enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4, JP_COLLIDE = 8, JP_FORCES = 16 };
 #define JOINT_MAX_ROWS 8
 
 typedef struct {
@@ -3345,8 +3345,8 @@ typedef struct {
 
 /* PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110) */
 enum { HINT_NONE = 0, HINT_ROT_EQ = 1024, HINT_ROT_INEQ = 1025, HINT_EQ = 2048, HINT_INEQ = 2049 };
-/* Px1DConstraintFlag: eKEEPBIAS, eRESTITUTION, eSPRING */
-enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4 };
+/* Px1DConstraintFlag: eKEEPBIAS, eRESTITUTION, eSPRING, eHAS_DRIVE_LIMIT */
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4, RF_DRIVE_LIMIT = 8 };
 
 /* Jacobian of the relative rotation qa* qb against (omega_b - omega_a)
  * (computeJacobianAxes, physxextensions/src/ExtConstraintHelper.h:77-102) */
@@ -3664,6 +3664,7 @@ static int joint_rows(const pxo_joint* j, xf bA, xf bB, row1d* rows, v3* raOut, 
 			c->minImp = -j->dfl;
 			c->maxImp = j->dfl;
 			c->angular = 1;
+			c->flags |= RF_DRIVE_LIMIT;
 			if (j->pflags & JP_FREESPIN)
 			{
 				if (j->dvel > 0.0f) c->minImp = 0.0f;
@@ -3840,7 +3841,11 @@ static void prep_joint(pxo_joint* j, xf bA, xf bB, const sbody* sa, const sbody*
 		const row1d* c = sorted[i];
 		jrow* s = &j->r[i];
 		s->lin0 = c->lin0; s->lin1 = c->lin1; s->ang0 = c->ang0; s->ang1 = c->ang1;
-		s->minImp = c->minImp; s->maxImp = c->maxImp;
+		/* computeMinMaxImpulseOrForceAsImpulse (DyCpuGpu1dConstraint.h:71-79):
+		 * a drive's limits are forces with PxConstraintFlag::
+		 * eDRIVE_LIMITS_ARE_FORCES (parameter flag 16) */
+		const float driveScale = ((c->flags & RF_DRIVE_LIMIT) && (j->pflags & JP_FORCES)) ? 1.0f / invDt : 1.0f;
+		s->minImp = c->minImp * driveScale; s->maxImp = c->maxImp * driveScale;
 		s->applied = 0.0f;
 		s->flags = JF_ANGULAR_ERROR;
 		const float linSum = len2(s->lin0) * sa->invMass + len2(s->lin1) * sb_->invMass;

@@ -591,6 +591,7 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 		const float* q = params + 10 * i;
 		const int f = int(q[0]);
 		j->setConstraintFlag(PxConstraintFlag::eCOLLISION_ENABLED, (f & 8) != 0);
+		j->setConstraintFlag(PxConstraintFlag::eDRIVE_LIMITS_ARE_FORCES, (f & 16) != 0);
 		if (PxRevoluteJoint* r = j->is<PxRevoluteJoint>())
 		{
 			PxJointAngularLimitPair lim(q[1], q[2]);
@@ -640,7 +641,7 @@ int pxref_scene_set_joint_params(void* h, int k, const int* joints, const float*
 			dj->setDistanceJointFlag(PxDistanceJointFlag::eMIN_DISTANCE_ENABLED, (f & 2) != 0);
 			dj->setDistanceJointFlag(PxDistanceJointFlag::eSPRING_ENABLED, (f & 4) != 0);
 		}
-		else if (f & 7)
+		else if (f & 23)
 			return -1;
 	}
 	return 0;

@@ -39,9 +39,9 @@ enum { JF_ROT_EQ = 1, JF_ORTHO_TARGET = 2, JF_ANGULAR_ERROR = 4, JF_INEQUALITY =
 // PxConstraintSolveHint (physx/include/PxConstraintDesc.h:79-110)
 enum { JHINT_NONE = 0, JHINT_ROT_EQ = 1024, JHINT_ROT_INEQ = 1025, JHINT_EQ = 2048, JHINT_INEQ = 2049 };
 // Px1DConstraintFlag eKEEPBIAS, eRESTITUTION, eSPRING
-enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4 };
+enum { RF_KEEP_BIAS = 1, RF_RESTITUTION = 2, RF_SPRING = 4, RF_DRIVE_LIMIT = 8 };
 // joint parameter flags: limit enabled, drive enabled, drive freespin
-enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4 };
+enum { JP_LIMIT = 1, JP_DRIVE = 2, JP_FREESPIN = 4, JP_COLLIDE = 8, JP_FORCES = 16 };
 constexpr int JOINT_MAX_ROWS = 8;
 
 // locked linear / angular axes (bit masks x y z) of each kind
@@ -360,6 +360,7 @@ PXH_D int jointRows(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, con
 			c.minImp = -p1.z;
 			c.maxImp = p1.z;
 			c.angular = 1;
+			c.flags |= RF_DRIVE_LIMIT;
 			if (pflags & JP_FREESPIN)
 			{
 				if (p1.y > 0.0f) c.minImp = 0.0f;
@@ -586,7 +587,9 @@ PXH_D void prepJoint(int kind, float4 p0, float4 p1, float4 p2, const xf& bA, co
 		r[1] = f4(c.lin1, biasScale);
 		r[2] = f4(ang0, velMul);
 		r[3] = f4(ang1, velTarget);
-		r[4] = make_float4(c.minImp, c.maxImp, 0.0f, maxBias);
+		// a drive's limits are forces with eDRIVE_LIMITS_ARE_FORCES (flag 16)
+		const float driveScale = ((c.flags & RF_DRIVE_LIMIT) && (int(p0.x) & JP_FORCES)) ? 1.0f / invDt : 1.0f;
+		r[4] = make_float4(c.minImp * driveScale, c.maxImp * driveScale, 0.0f, maxBias);
 		r[5] = make_float4(__int_as_float(flags), 0.0f, 0.0f, 0.0f);
 	}
 	head[0] = f4(ra, invMass0);

@@ -1150,13 +1150,13 @@ int32_t pxh_scene_set_joint_params(pxh_scene s, int32_t k, const int32_t* joints
 			if (!std::isfinite(q[c]) && !(c == 6 && q[c] > 0.0f))
 				return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: parameter is not finite");
 		const int f = int(q[0]);
-		if (float(f) != q[0] || f < 0 || f > 15)
+		if (float(f) != q[0] || f < 0 || f > 31)
 			return fail(PXH_ERR_INVALID, "pxh_scene_set_joint_params: unknown flags");
 		const int t = s->hJType[joints[i]];
 		if ((f & 7) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_PRISMATIC && t != PXH_JOINT_SPHERICAL &&
 		    t != PXH_JOINT_DISTANCE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: fixed joints take no limits");
-		if ((f & 6) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_DISTANCE)
+		if ((f & 22) && t != PXH_JOINT_REVOLUTE && t != PXH_JOINT_DISTANCE)
 			return fail(PXH_ERR_UNSUPPORTED, "pxh_scene_set_joint_params: only revolute joints take a drive");
 		if (q[3] < 0.0f || (q[3] > 1.0f && t != PXH_JOINT_DISTANCE) || q[4] < 0.0f || q[6] < 0.0f || q[8] < 0.0f ||
 		    q[9] < 0.0f)

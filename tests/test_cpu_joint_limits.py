@@ -113,7 +113,10 @@ def _drive_cases():
             _one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
                        [2, 0, 0, 0, 0, 2.0, 0.001, 1, 0, 0]),
             _one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
-                       [6, 0, 0, 0, 0, -2.0, np.inf, 1, 0, 0], av=(3.0, 0, 0))]
+                       [6, 0, 0, 0, 0, -2.0, np.inf, 1, 0, 0], av=(3.0, 0, 0)),
+            # the force limit as a force (eDRIVE_LIMITS_ARE_FORCES): 0.06 N m
+            _one_joint(S.JOINT_REVOLUTE, S.SPHERE, [0.2, 0, 0, 0], body, [0, 0, 0, 0, 0, 0, 1], f,
+                       [18, 0, 0, 0, 0, 2.0, 0.06, 1, 0, 0])]
 
 
 def test_drive_reaches_its_velocity():
@@ -124,8 +127,9 @@ def test_drive_reaches_its_velocity():
     other components stay 0; with a force limit F (an impulse per step) one
     step changes its angular momentum about the hinge by F (1e-4 relative);
     freespin lets a sphere already turning faster than the drive asks keep
-    its speed (bar the default angular damping)."""
-    a, b, c = _drive_cases()
+    its speed (bar the default angular damping); with the limit a force
+    (eDRIVE_LIMITS_ARE_FORCES) one step changes the momentum by limit x dt."""
+    a, b, c, d = _drive_cases()
     o = L.OracleWorld(a, gravity=(0.0, 0.0, 0.0))
     o.step(DT)
     w = o.bodies()[2][0]
@@ -138,6 +142,11 @@ def test_drive_reaches_its_velocity():
     o.step(DT)
     # only the body's default angular damping (0.05 / s) slows it
     assert 2.99 <= o.bodies()[2][0][0] <= 3.0, o.bodies()[2][0]
+    # a force limit of 0.06 N m acts as the impulse 0.06 dt = 0.001 per step
+    o = L.OracleWorld(d, gravity=(0.0, 0.0, 0.0))
+    o.step(DT)
+    w = o.bodies()[2][0]
+    assert abs(-w[0] * float(d.inertia[0][0]) - 0.06 * DT) <= 1e-7, (w, d.inertia[0])
 
 
 def test_prismatic_limit_and_locked_limits():

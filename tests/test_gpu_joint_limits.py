@@ -95,7 +95,7 @@ def test_joint_params_argument_checks(cuda):
     with pytest.raises(P.PxhError, match="out of range"):
         g.set_joint_params([len(kinds)], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     with pytest.raises(P.PxhError, match="unknown flags"):
-        g.set_joint_params([prism], [[17, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
+        g.set_joint_params([prism], [[33, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.set_joint_params([prism], [[1, -0.1, 0.1, 0, 0, 0, 0, 1, 0, 0]])
     g.step(DT)
 
